@@ -1,0 +1,163 @@
+"""ctypes mirror of include/cppwrf_amd.h and the loader of the native library.
+
+The library is built in-tree (cppwrf_amd/libcppwrf_amd.so, see cppwrf_amd/build.py).  Loading never
+falls back to anything else: a missing or stale library is an ImportError naming the build command.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcppwrf_amd.so")
+
+MI_OK = 0
+MI_EINVAL = -1
+MI_EHIP = -2
+MI_ENOMEM = -3
+MI_ENONFINITE = -4
+MI_ENODEV = -5
+
+MI_BC_PERIODIC, MI_BC_SYMMETRIC, MI_BC_SPECIFIED, MI_BC_OPEN = 0, 1, 2, 3
+MI_FP64, MI_FP32_BULK = 0, 1
+
+# field ids
+MI_U, MI_V, MI_W, MI_T, MI_PH, MI_MU, MI_P, MI_AL = range(8)
+MI_WW, MI_RU_M, MI_RV_M, MI_WW_M, MI_ALT, MI_PHP, MI_MUTS = range(8, 15)
+MI_RU_TEND, MI_RV_TEND, MI_RW_TEND, MI_T_TEND, MI_PH_TEND = range(15, 20)
+MI_MUB, MI_PB, MI_ALB, MI_PHB, MI_T_INIT = range(32, 37)
+MI_MSFTX, MI_MSFTY, MI_MSFUX, MI_MSFUY, MI_MSFVX, MI_MSFVY, MI_MSFVX_INV = range(40, 47)
+
+MI_CASE_REST_STATE, MI_CASE_WARM_BUBBLE, MI_CASE_IGW, MI_CASE_BUBBLE3D = range(4)
+
+# staggering of each field id: (x extra, y extra, z extra, rank)
+STAGGER = {
+    MI_U: "U", MI_V: "V", MI_W: "W", MI_T: "M", MI_PH: "W", MI_MU: "M2", MI_P: "M", MI_AL: "M",
+    MI_WW: "W", MI_RU_M: "U", MI_RV_M: "V", MI_WW_M: "W", MI_ALT: "M", MI_PHP: "M", MI_MUTS: "M2",
+    MI_RU_TEND: "U", MI_RV_TEND: "V", MI_RW_TEND: "W", MI_T_TEND: "M", MI_PH_TEND: "W",
+    MI_MUB: "M2", MI_PB: "M", MI_ALB: "M", MI_PHB: "W", MI_T_INIT: "M",
+}
+
+FIELD_NAMES = {
+    MI_U: "u", MI_V: "v", MI_W: "w", MI_T: "t", MI_PH: "ph", MI_MU: "mu", MI_P: "p", MI_AL: "al",
+    MI_WW: "ww", MI_RU_M: "ru_m", MI_RV_M: "rv_m", MI_WW_M: "ww_m", MI_ALT: "alt", MI_PHP: "php",
+    MI_MUTS: "muts", MI_RU_TEND: "ru_tend", MI_RV_TEND: "rv_tend", MI_RW_TEND: "rw_tend",
+    MI_T_TEND: "t_tend", MI_PH_TEND: "ph_tend",
+}
+
+
+def field_shape(fid: int, nx: int, ny: int, nz: int) -> tuple[int, int, int]:
+    """(nzs, nys, nxs) of a field's logical points, the dense array layout of the C ABI."""
+    st = STAGGER[fid]
+    if st == "M2":
+        return (1, ny, nx)
+    nxs = nx + (1 if st == "U" else 0)
+    nys = ny + (1 if st == "V" and ny > 1 else 0)
+    nzs = nz + (1 if st == "W" else 0)
+    return (nzs, nys, nxs)
+
+
+class MiGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("bc_x", C.c_int32),
+                ("bc_y", C.c_int32), ("dx", C.c_double), ("dy", C.c_double), ("dt", C.c_double)]
+
+
+class MiParams(C.Structure):
+    _fields_ = [("h_adv_order", C.c_int32), ("v_adv_order", C.c_int32), ("top_lid", C.c_int32),
+                ("w_damping", C.c_int32), ("w_crit_cfl", C.c_double), ("epssm", C.c_double),
+                ("smdiv", C.c_double), ("emdiv", C.c_double), ("damp_opt", C.c_int32),
+                ("hypsometric_opt", C.c_int32), ("zdamp", C.c_double), ("dampcoef", C.c_double),
+                ("time_step_sound", C.c_int32), ("precision", C.c_int32)]
+
+
+_PD = C.POINTER(C.c_double)
+
+
+class MiLevels(C.Structure):
+    _fields_ = [(n, _PD) for n in ("dnw", "rdnw", "dn", "rdn", "fnm", "fnp", "c1h", "c2h", "c3h", "c4h",
+                                   "c1f", "c2f", "c3f", "c4f")] + \
+               [(n, C.c_double) for n in ("cf1", "cf2", "cf3", "cfn", "cfn1", "rdx", "rdy", "p_top")]
+
+
+class MiCaseSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("grid", MiGrid), ("ztop", C.c_double), ("seed", C.c_uint64),
+                ("noise_amp", C.c_double), ("bubble_dtheta", C.c_double), ("bubble_radius", C.c_double),
+                ("bubble_zc", C.c_double), ("u0", C.c_double)]
+
+
+def default_params(**kw) -> MiParams:
+    """The reference's defaults (src/cases/model.cpp:256-281)."""
+    p = MiParams(h_adv_order=5, v_adv_order=3, top_lid=0, w_damping=0, w_crit_cfl=1.0, epssm=0.1, smdiv=0.1,
+                 emdiv=0.01, damp_opt=0, hypsometric_opt=1, zdamp=5000.0, dampcoef=0.2, time_step_sound=0,
+                 precision=MI_FP64)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _proto(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_VP = C.c_void_p
+_I = C.c_int
+_I64 = C.c_int64
+
+# name -> (restype, argtypes), every function include/cppwrf_amd.h declares
+SIGNATURES = {
+    "mi_last_error": (C.c_char_p, []),
+    "mi_abi_version": (_I, []),
+    "mi_auto_sound_steps": (_I, [C.c_double, C.c_double, C.c_double]),
+    "mi_ctx_create": (_I, [_I, _VP, C.POINTER(_VP)]),
+    "mi_ctx_destroy": (_I, [_VP]),
+    "mi_domain_create": (_I, [_VP, C.POINTER(MiGrid), C.POINTER(MiParams), C.POINTER(MiLevels), C.POINTER(_VP)]),
+    "mi_domain_destroy": (_I, [_VP]),
+    "mi_domain_n_sound": (_I, [_VP]),
+    "mi_domain_device_bytes": (_I64, [_VP]),
+    "mi_domain_set_field": (_I, [_VP, _I, _PD, _I64]),
+    "mi_domain_get_field": (_I, [_VP, _I, _PD, _I64]),
+    "mi_step": (_I, [_VP, _I]),
+    "mi_domain_sync": (_I, [_VP]),
+    "mi_step_begin": (_I, [_VP]),
+    "mi_rk_prep": (_I, [_VP, _I]),
+    "mi_rk_tendency": (_I, [_VP, _I]),
+    "mi_small_steps": (_I, [_VP, _I]),
+    "mi_calc_p_rho_phi": (_I, [_VP, _I]),
+    "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
+    "mi_case_create": (_I, [C.POINTER(MiCaseSpec), C.POINTER(_VP)]),
+    "mi_case_destroy": (_I, [_VP]),
+    "mi_case_levels": (_I, [_VP, C.POINTER(MiLevels)]),
+    "mi_case_get_field": (_I, [_VP, _I, _PD, _I64]),
+    "mi_case_load": (_I, [_VP, _VP]),
+}
+
+_lib = None
+
+
+def lib():
+    """The native library (loaded once).  Raises ImportError if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            _proto(L, name, res, args)
+        if L.mi_abi_version() != 1:
+            raise ImportError("libcppwrf_amd.so ABI version mismatch: rebuild it")
+        _lib = L
+    return _lib
+
+
+class MiError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def check(rc: int) -> None:
+    if rc != MI_OK:
+        raise MiError(rc, lib().mi_last_error().decode())

@@ -1,0 +1,192 @@
+/* cppwrf_amd: the RK3 split-explicit dynamics step of WRF-ARW on AMD MI355X
+ * (gfx950), behind a C ABI.
+ *
+ * The reference has no C ABI: its operator interface is the C++ src/dyn
+ * surface, dyn::RK3 (reference src/dyn/rk3.hpp:34-78) with
+ *   RK3(const Grid&, const LargeStepParams&, const AcousticParams&,
+ *       int time_step_sound, const ScalarParams&)
+ *   void step(State&, Tendencies&, const BaseState&)
+ * and the containers Grid (src/core/grid.hpp:21-55), State / Tendencies /
+ * BaseState (src/dyn/state.hpp:30-119).  This header is the replacement
+ * boundary: a domain handle owns the device-resident State and BaseState of
+ * one forecast, the caller hands the initial fields over as plain host arrays
+ * and steps it.  The C++ facade cppwrf_amd/csrc/dyn_facade.hpp gives the
+ * reference's class names and methods on top of these functions.
+ *
+ * Conventions
+ *  - Every function returns MI_OK (0) or a negative MI_E* code; no exception
+ *    crosses the ABI.  mi_last_error() gives the message of the calling
+ *    thread's last failure.
+ *  - Arrays are double, dense, logical points only, i fastest, then j, then k:
+ *    element (i, j, k) of a field with nxs x nys x nzs points is
+ *    a[(k * nys + j) * nxs + i].  Staggering as the reference
+ *    (src/core/grid.hpp:3-7): U has nx+1 points in x, V ny+1 in y (ny == 1:
+ *    one), W nz+1 levels.
+ *  - The caller owns every pointer it passes; the library copies what it
+ *    needs before the call returns.
+ *  - One domain must not be used by two host threads at once.  Different
+ *    domains may be stepped from different threads.
+ */
+#ifndef CPPWRF_AMD_H
+#define CPPWRF_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_ABI_VERSION 1
+
+/* status codes */
+#define MI_OK 0
+#define MI_EINVAL (-1)     /* bad argument (reference: std::runtime_error for bad parameters, rk3.cpp:50) */
+#define MI_EHIP (-2)       /* HIP runtime error (reference: CUDA errors, backend.cpp:23-25) */
+#define MI_ENOMEM (-3)     /* device allocation failed (reference: backend.cpp:312-331) */
+#define MI_ENONFINITE (-4) /* a non-finite value was found by mi_check_finite */
+#define MI_ENODEV (-5)     /* no GPU, or the library was built without device code */
+
+/* lateral boundary conditions, in the order of the reference's enum BC (src/core/grid.hpp:19) */
+enum { MI_BC_PERIODIC = 0, MI_BC_SYMMETRIC = 1, MI_BC_SPECIFIED = 2, MI_BC_OPEN = 3 };
+
+/* arithmetic of a domain: plain FP64 (bit-identical to the reference CPU FP64 build) or
+ * FP32 storage and arithmetic for the bulk with FP64 kept in the column solves */
+enum { MI_FP64 = 0, MI_FP32_BULK = 1 };
+
+/* reference Grid (src/core/grid.hpp:21-31) */
+typedef struct mi_grid {
+  int32_t nx, ny, nz; /* mass points; ny == 1 is an x-z slice (the j stride is 0) */
+  int32_t bc_x, bc_y; /* MI_BC_* */
+  double dx, dy;      /* m */
+  double dt;          /* large step, s */
+} mi_grid;
+
+/* reference LargeStepParams (src/dyn/large_step.hpp:14-22) and AcousticParams (src/dyn/rk3.hpp:20-27) */
+typedef struct mi_params {
+  int32_t h_adv_order;     /* 5 (upwind) or 6 (centred) */
+  int32_t v_adv_order;     /* 3 or 4 */
+  int32_t top_lid;         /* 0: constant-pressure top (WRF default) */
+  int32_t w_damping;       /* 1: damp w where the vertical Courant number exceeds 1 */
+  double w_crit_cfl;
+  double epssm, smdiv, emdiv;
+  int32_t damp_opt;        /* 0 or 3 (implicit Rayleigh damping of w in the top zdamp metres) */
+  int32_t hypsometric_opt; /* 1 */
+  double zdamp, dampcoef;
+  int32_t time_step_sound; /* acoustic steps per large step (even); 0: the reference's automatic rule */
+  int32_t precision;       /* MI_FP64 or MI_FP32_BULK */
+} mi_params;
+
+/* reference BaseState vertical coefficients (src/dyn/state.hpp:109-116, set by BaseState::set_levels,
+ * src/dyn/state.cpp:86-116).  Arrays of nz (mass levels) or nz+1 (w levels) values. */
+typedef struct mi_levels {
+  const double *dnw, *rdnw;              /* nz */
+  const double *dn, *rdn, *fnm, *fnp;    /* nz+1 (index 0 and nz unused) */
+  const double *c1h, *c2h, *c3h, *c4h;   /* nz */
+  const double *c1f, *c2f, *c3f, *c4f;   /* nz+1 */
+  double cf1, cf2, cf3, cfn, cfn1;
+  double rdx, rdy; /* 1/dx and 1/dy (0 for an x-z slice, as the reference BaseState) */
+  double p_top;
+} mi_levels;
+
+/* field identifiers for mi_domain_set_field / mi_domain_get_field */
+enum {
+  /* state carried from step to step (reference State::*_2, p, al) */
+  MI_U = 0, MI_V = 1, MI_W = 2, MI_T = 3, MI_PH = 4, MI_MU = 5, MI_P = 6, MI_AL = 7,
+  /* diagnostics of the last stage (read only) */
+  MI_WW = 8, MI_RU_M = 9, MI_RV_M = 10, MI_WW_M = 11, MI_ALT = 12, MI_PHP = 13, MI_MUTS = 14,
+  MI_RU_TEND = 15, MI_RV_TEND = 16, MI_RW_TEND = 17, MI_T_TEND = 18, MI_PH_TEND = 19,
+  /* base state (write once, before the first step).  3-D fields take nzs values for a horizontally
+   * uniform profile or nxs*nys*nzs values; 2-D fields take 1 value or nx*ny values. */
+  MI_MUB = 32, MI_PB = 33, MI_ALB = 34, MI_PHB = 35, MI_T_INIT = 36,
+  MI_MSFTX = 40, MI_MSFTY = 41, MI_MSFUX = 42, MI_MSFUY = 43, MI_MSFVX = 44, MI_MSFVY = 45,
+  MI_MSFVX_INV = 46
+};
+
+typedef struct mi_ctx mi_ctx;       /* a GPU and the HIP stream its domains launch on */
+typedef struct mi_domain mi_domain; /* one forecast: device-resident State, BaseState, work fields */
+
+/* message of the calling thread's last failure ("" if none) */
+const char* mi_last_error(void);
+int mi_abi_version(void);
+
+/* reference RK3::auto_sound_steps (src/dyn/rk3.cpp:69-72): max(2*(int(300 dt/min(dx,dy) - 0.01)+1), 4) */
+int mi_auto_sound_steps(double dt, double dx, double dy);
+
+/* A context on HIP device `device`.  stream: a hipStream_t to launch on, or NULL for a stream the
+ * context creates (reference: the thread-local launch stream, src/core/backend.cpp:73-81). */
+int mi_ctx_create(int device, void* stream, mi_ctx** out);
+int mi_ctx_destroy(mi_ctx* ctx);
+
+/* reference RK3::RK3 with State/Tendencies/BaseState allocation (src/dyn/rk3.cpp:48-56,
+ * src/dyn/state.cpp:10-23): allocates every device field of the domain, zero-filled */
+int mi_domain_create(mi_ctx* ctx, const mi_grid* g, const mi_params* p, const mi_levels* lv, mi_domain** out);
+int mi_domain_destroy(mi_domain* d);
+/* acoustic steps per large step in use (reference RK3::n_sound, rk3.hpp:44) */
+int mi_domain_n_sound(const mi_domain* d);
+/* device bytes the domain holds */
+int64_t mi_domain_device_bytes(const mi_domain* d);
+
+/* Copy a host field into the domain (reference Field::fill_host, src/core/field.hpp:113-119, followed by
+ * the halo fill of src/core/halo.cpp).  n = number of values (see the field ids above). */
+int mi_domain_set_field(mi_domain* d, int field, const double* host, int64_t n);
+/* Copy a field's logical points to the host (stream-ordered: waits for the domain's queued work). */
+int mi_domain_get_field(mi_domain* d, int field, double* host, int64_t n);
+
+/* reference RK3::step (src/dyn/rk3.cpp:86-170): advance nsteps large steps.  Asynchronous on the
+ * context's stream; mi_domain_sync waits. */
+int mi_step(mi_domain* d, int nsteps);
+int mi_domain_sync(mi_domain* d);
+
+/* The phases of one step, for tests (same launches as mi_step, in its order):
+ *   mi_step_begin      the _2 -> _1 copies (rk3.cpp:94-98)
+ *   mi_rk_prep         LargeStep::prep (large_step.cpp:697-733)
+ *   mi_rk_tendency     LargeStep::tendencies (large_step.cpp:798-861)
+ *   mi_small_steps     SmallStep::run_stage (small_step.cpp:490-607)
+ *   mi_calc_p_rho_phi  SmallStep::calc_p_rho_phi (small_step.cpp:609-617)
+ * rk = 1, 2, 3. */
+int mi_step_begin(mi_domain* d);
+int mi_rk_prep(mi_domain* d, int rk);
+int mi_rk_tendency(mi_domain* d, int rk);
+int mi_small_steps(mi_domain* d, int rk);
+int mi_calc_p_rho_phi(mi_domain* d, int rk);
+
+/* MI_ENONFINITE if any point of u, v, w, t, ph, mu is NaN or infinite (the reference's runner check,
+ * src/cases/domain_runner.hpp:70-71); *bad_field receives the first such field id (or -1). */
+int mi_check_finite(mi_domain* d, int* bad_field);
+
+/* ---------------------------------------------------------------- synthetic initial states
+ * Host-only builders of the idealized states the tests and benchmarks use: the reference's case
+ * recipes (src/cases/cases.cpp:17-287: sounding, eta levels, hydrostatic base state) and the 3-D
+ * warm bubble of this library's workloads.  No GPU is touched. */
+enum {
+  MI_CASE_REST_STATE = 0,  /* reference configs/rest_state.ini recipe */
+  MI_CASE_WARM_BUBBLE = 1, /* reference warm_bubble (x-z Bryan & Fritsch 2002) */
+  MI_CASE_IGW = 2,         /* reference igw (Skamarock & Klemp 1994, f = 0) */
+  MI_CASE_BUBBLE3D = 3     /* 3-D dry bubble in an N = 0.01 atmosphere at rest, seeded noise */
+};
+typedef struct mi_case_spec {
+  int32_t kind; /* MI_CASE_* */
+  mi_grid grid;
+  double ztop;          /* m */
+  uint64_t seed;        /* noise seed (bubble3d) */
+  double noise_amp;     /* K, half-width of the uniform theta noise (bubble3d) */
+  double bubble_dtheta; /* K (bubble3d: +2) */
+  double bubble_radius; /* m (bubble3d: 4000) */
+  double bubble_zc;     /* m (bubble3d: 2000) */
+  double u0;            /* uniform wind (rest_state's case.u0) */
+} mi_case_spec;
+typedef struct mi_case mi_case;
+int mi_case_create(const mi_case_spec* spec, mi_case** out);
+int mi_case_destroy(mi_case* c);
+/* the level coefficients; the pointers stay valid until mi_case_destroy */
+int mi_case_levels(const mi_case* c, mi_levels* out);
+/* the initial value of a state or base field (same ids and sizes as mi_domain_get_field; base
+ * 3-D fields as full nxs*nys*nzs arrays) */
+int mi_case_get_field(const mi_case* c, int field, double* host, int64_t n);
+/* load every initial field of a case into a domain of the same grid */
+int mi_case_load(const mi_case* c, mi_domain* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPPWRF_AMD_H */

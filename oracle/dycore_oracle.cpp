@@ -1,0 +1,925 @@
+// CPU oracle of the RK3 split-explicit dynamics step -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+//
+// Each function below restates one kernel of the reference CPU FP64 build and cites it.  The
+// expressions keep the reference's operand order token for token: the FP64 mode of the HIP library
+// must agree with this file bit for bit, and both must agree with the reference.  Compiled with
+// -ffp-contract=off (no fused multiply-add), as the reference (CMakeLists.txt:33-37).
+//
+// Scope: dry dynamics on flat or uniform-map grids with periodic or symmetric lateral boundaries,
+// h_adv_order 5/6, v_adv_order 3/4, damp_opt 0/3, w_damping 0/1, top_lid 0/1, hypsometric_opt 1.
+#include "oracle.h"
+
+#include <emmintrin.h>
+#include <omp.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+extern "C" __m128d _ZGVbN2vv_pow(__m128d, __m128d);
+
+namespace {
+
+thread_local std::string g_err;
+
+constexpr double kG = 9.81;                   // constants.hpp:5
+constexpr double kRd = 287.0;                 // constants.hpp:6
+constexpr double kCp = 7.0 * kRd / 2.0;       // constants.hpp:7
+constexpr double kCpovcv = kCp / (kCp - kRd); // constants.hpp:11
+constexpr double kP1000 = 100000.0;           // constants.hpp:12
+constexpr double kT0 = 300.0;                 // constants.hpp:13
+constexpr double H = 0.5, Q = 0.25;
+constexpr double kReradius = 1. / 6370.0e03;  // large_step.cpp:29
+
+enum class St { M, U, V, W };
+
+// A field with the reference's allocation semantics: halo 3 in x, 3 in y (0 for an x-z slice, whose
+// j stride is 0), staggering as src/core/grid.hpp:3-7 (field.hpp:44-70, halo.cpp:12-48).
+struct F {
+  int nxs = 0, nys = 0, nzs = 0, hy = 0;
+  long sj = 0, sk = 0, off = 0;
+  bool face_x = false, face_y = false;
+  std::vector<double> a;
+  void init(int nx, int ny, int nz, St s, bool rank3) {
+    const bool two_d = ny == 1;
+    nxs = nx + (s == St::U ? 1 : 0);
+    nys = ny + ((s == St::V && !two_d) ? 1 : 0);
+    nzs = rank3 ? nz + (s == St::W ? 1 : 0) : 1;
+    hy = two_d ? 0 : 3;
+    face_x = s == St::U;
+    face_y = s == St::V && !two_d;
+    const long pitch = nxs + 6;
+    sj = two_d ? 0 : pitch;
+    const long plane = pitch * (nys + 2 * hy);
+    sk = rank3 ? plane : 0;
+    off = 3 + hy * sj;
+    a.assign(size_t(plane) * size_t(nzs), 0.0);
+  }
+  double& operator()(int i, int j, int k) { return a[size_t(off + i + j * sj + k * sk)]; }
+  double operator()(int i, int j, int k) const { return a[size_t(off + i + j * sj + k * sk)]; }
+  double& operator()(int i, int j) { return a[size_t(off + i + j * sj)]; }
+  double operator()(int i, int j) const { return a[size_t(off + i + j * sj)]; }
+};
+
+enum class Par { Even, Odd, OddFace };
+
+struct Grid {
+  int nx, ny, nz, bcx, bcy;
+  double dx, dy, dt;
+  bool two_d() const { return ny == 1; }
+  int nyv() const { return two_d() ? ny : ny + 1; }
+  int u_i0() const { return bcx != MI_BC_PERIODIC ? 1 : 0; }  // grid.hpp:41
+  int v_j0() const { return (bcy != MI_BC_PERIODIC && !two_d()) ? 1 : 0; }  // grid.hpp:43
+};
+
+// One halo line (halo.cpp:52-84): slot a of the direction with n mass points.
+inline void halo_point(double* base, long sn, int a, int n, bool face, bool periodic, bool odd, bool odd_face) {
+  const int nl = face ? 4 : 3;
+  const int i = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
+  int src;
+  if (periodic) src = ((i % n) + n) % n;
+  else if (face) src = i <= 0 ? -i : 2 * n - i;
+  else src = i < 0 ? -1 - i : 2 * n - 1 - i;
+  double v = base[src * sn];
+  if (!periodic && src == i) {
+    if (odd_face) base[i * sn] = -v;
+    return;
+  }
+  if (!periodic && odd) v = -v;
+  base[i * sn] = v;
+}
+
+// fill_halo (halo.cpp:114-131): x over the interior rows, then y over the full x extent.
+void fill(F& f, const Grid& g, Par px, Par py) {
+  const int na = f.face_x ? 8 : 6;
+  for (int k = 0; k < f.nzs; ++k)
+    for (int j = 0; j < f.nys; ++j)
+      for (int a = 0; a < na; ++a)
+        halo_point(&f(0, j, k), 1, a, g.nx, f.face_x, g.bcx == MI_BC_PERIODIC, px != Par::Even, px == Par::OddFace);
+  if (g.two_d()) return;
+  const int nb = f.face_y ? 8 : 6;
+  for (int k = 0; k < f.nzs; ++k)
+    for (int i = -3; i < f.nxs + 3; ++i)
+      for (int a = 0; a < nb; ++a)
+        halo_point(&f(i, 0, k), f.sj, a, g.ny, f.face_y, g.bcy == MI_BC_PERIODIC, py != Par::Even, py == Par::OddFace);
+}
+// default parities (halo.cpp:224-228): U odd in x, V odd in y, everything else even
+void fill(F& f, const Grid& g) { fill(f, g, f.face_x ? Par::Odd : Par::Even, f.face_y ? Par::Odd : Par::Even); }
+
+template <class K>
+void pfor(int i0, int i1, int j0, int j1, int k0, int k1, K k) {  // backend.hpp:509-519
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int kk = k0; kk < k1; ++kk)
+    for (int j = j0; j < j1; ++j)
+      for (int i = i0; i < i1; ++i) k(i, j, kk);
+}
+template <class K>
+void cfor(int nx, int ny, K k) {  // backend.hpp:520-528
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) k(i, j);
+}
+
+inline double sgn1(double a) { return a >= 0.0 ? 1.0 : -1.0; }  // kernels_common.hpp:11
+// kernels_common.hpp:13-17
+inline double flux5(double qm3, double qm2, double qm1, double q0, double qp1, double qp2, double ua, double up) {
+  return (37.0 * (q0 + qm1) - 8.0 * (qp1 + qm2) + (qp2 + qm3)) / 60.0 -
+         up * sgn1(ua) * ((qp2 - qm3) - 5.0 * (qp1 - qm2) + 10.0 * (q0 - qm1)) / 60.0;
+}
+// kernels_common.hpp:19-22
+inline double flux3(double qm2, double qm1, double q0, double qp1, double ua, double up) {
+  return (7.0 * (q0 + qm1) - (qp1 + qm2)) / 12.0 + up * sgn1(ua) * ((qp1 - qm2) - 3.0 * (q0 - qm1)) / 12.0;
+}
+
+struct Lev {
+  std::vector<double> dnw, rdnw, dn, rdn, fnm, fnp, c1h, c2h, c3h, c4h, c1f, c2f, c3f, c4f;
+  double cf1, cf2, cf3, cfn, cfn1, rdx, rdy, p_top;
+};
+
+struct Dom {
+  Grid g;
+  mi_params p;
+  Lev c;
+  int n_sound = 0;
+  bool libmvec = false;
+  // State (state.hpp:30-78)
+  F u1, u2, v1, v2, w1, w2, t1, t2, ph1, ph2, mu1, mu2, pp, al, alt, php, ww;
+  F mut, muu, muv, muts, muus, muvs, ru, rv, rw, ru_m, rv_m, ww_m;
+  // Tendencies (state.hpp:80-90)
+  F ru_tend, rv_tend, rw_tend, t_tend, ph_tend, mu_tend;
+  // SmallStep work fields (small_step.hpp:43-49)
+  F u_save, v_save, w_save, t_save, ph_save, ww1, c2a, pm1, t_2save, ca, calpha, cgamma, mu_save, mudf, muave, w_rhs,
+      t2a, dvdxi;
+  // BaseState (state.hpp:92-119), stored in full with halos
+  F mub, pb, alb, t_init, phb, msftx, msfty, msfux, msfuy, msfvx, msfvy, msfvx_inv;
+
+  explicit Dom(const mi_grid& gg, const mi_params& pp_, const mi_levels& lv) {
+    g = Grid{gg.nx, gg.ny, gg.nz, gg.bc_x, gg.bc_y, gg.dx, gg.dy, gg.dt};
+    p = pp_;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    auto v1d = [](const double* a, int n) { return std::vector<double>(a, a + n); };
+    c.dnw = v1d(lv.dnw, nz); c.rdnw = v1d(lv.rdnw, nz);
+    c.dn = v1d(lv.dn, nz + 1); c.rdn = v1d(lv.rdn, nz + 1); c.fnm = v1d(lv.fnm, nz + 1); c.fnp = v1d(lv.fnp, nz + 1);
+    c.c1h = v1d(lv.c1h, nz); c.c2h = v1d(lv.c2h, nz); c.c3h = v1d(lv.c3h, nz); c.c4h = v1d(lv.c4h, nz);
+    c.c1f = v1d(lv.c1f, nz + 1); c.c2f = v1d(lv.c2f, nz + 1); c.c3f = v1d(lv.c3f, nz + 1); c.c4f = v1d(lv.c4f, nz + 1);
+    c.cf1 = lv.cf1; c.cf2 = lv.cf2; c.cf3 = lv.cf3; c.cfn = lv.cfn; c.cfn1 = lv.cfn1;
+    c.rdx = lv.rdx; c.rdy = lv.rdy; c.p_top = lv.p_top;
+    n_sound = p.time_step_sound > 0 ? p.time_step_sound : mi_auto_sound_steps_(g.dt, g.dx, g.dy);
+    auto f3 = [&](F& f, St s) { f.init(nx, ny, nz, s, true); };
+    auto f2 = [&](F& f, St s) { f.init(nx, ny, nz, s, false); };
+    for (F* f : {&u1, &u2, &ru, &ru_m, &ru_tend, &u_save}) f3(*f, St::U);
+    for (F* f : {&v1, &v2, &rv, &rv_m, &rv_tend, &v_save}) f3(*f, St::V);
+    for (F* f : {&w1, &w2, &ph1, &ph2, &ww, &rw, &ww_m, &rw_tend, &ph_tend, &w_save, &ph_save, &ww1, &ca, &calpha,
+                 &cgamma, &w_rhs, &phb})
+      f3(*f, St::W);
+    for (F* f : {&t1, &t2, &pp, &al, &alt, &php, &t_tend, &t_save, &c2a, &pm1, &t_2save, &t2a, &dvdxi, &pb, &alb, &t_init})
+      f3(*f, St::M);
+    for (F* f : {&mu1, &mu2, &mut, &muts, &mu_tend, &mu_save, &mudf, &muave, &mub, &msftx, &msfty}) f2(*f, St::M);
+    for (F* f : {&muu, &muus, &msfux, &msfuy}) f2(*f, St::U);
+    for (F* f : {&muv, &muvs, &msfvx, &msfvy, &msfvx_inv}) f2(*f, St::V);
+    for (F* f : {&msftx, &msfty, &msfux, &msfuy, &msfvx, &msfvy, &msfvx_inv}) std::fill(f->a.begin(), f->a.end(), 1.0);
+  }
+  static int mi_auto_sound_steps_(double dt, double dx, double dy) {
+    const double spacing = dx < dy ? dx : dy;
+    const int n = 2 * (int(300.0 * dt / spacing - 0.01) + 1);
+    return n > 4 ? n : 4;
+  }
+  int nyv() const { return g.nyv(); }
+
+  // ------------------------------------------------------------------ LargeStep::prep (large_step.cpp:697-733)
+  void rk_prep() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz, nyv_ = nyv();
+    for (F* f : {&u2, &v2, &w2, &t2, &ph2, &pp, &al, &mu2}) fill(*f, g);
+    // KMu (large_step.cpp:49-66), non-specified edges
+    pfor(0, nx + 1, 0, nyv_, 0, 1, [&](int i, int j, int) {
+      if (i < nx && j < ny) mut(i, j) = mu2(i, j) + mub(i, j);
+      if (j < ny) muu(i, j) = H * (mu2(i, j) + mu2(i - 1, j) + mub(i, j) + mub(i - 1, j));
+      if (i < nx && j < nyv_) muv(i, j) = H * (mu2(i, j) + mu2(i, j - 1) + mub(i, j) + mub(i, j - 1));
+    });
+    fill(mut, g);
+    // KPrep3 (large_step.cpp:68-81)
+    pfor(0, nx + 1, 0, nyv_, 0, nz + 1, [&](int i, int j, int k) {
+      if (j < ny && k < nz) ru(i, j, k) = u2(i, j, k) * (c.c1h[k] * muu(i, j) + c.c2h[k]) / msfuy(i, j);
+      if (i < nx && j < nyv_ && k < nz) rv(i, j, k) = v2(i, j, k) * (c.c1h[k] * muv(i, j) + c.c2h[k]) * msfvx_inv(i, j);
+      if (i < nx && j < ny) rw(i, j, k) = w2(i, j, k) * (c.c1f[k] * mut(i, j) + c.c2f[k]) / msfty(i, j);
+      if (i < nx && j < ny && k < nz) {
+        alt(i, j, k) = al(i, j, k) + alb(i, j, k);
+        php(i, j, k) = H * (phb(i, j, k) + phb(i, j, k + 1) + ph2(i, j, k) + ph2(i, j, k + 1));
+      }
+    });
+    for (F* f : {&ru, &rv, &rw, &alt, &php}) fill(*f, g);
+    fill(muu, g, Par::OddFace, Par::Even);
+    fill(muv, g, Par::Even, Par::Odd);
+    // KWW calc_ww_cp (large_step.cpp:83-108)
+    auto mu_u = [&](int i, int j) { return H * (mu2(i, j) + mub(i, j) + mu2(i - 1, j) + mub(i - 1, j)); };
+    auto mu_v = [&](int i, int j) { return H * (mu2(i, j) + mub(i, j) + mu2(i, j - 1) + mub(i, j - 1)); };
+    auto divv = [&](int i, int j, int k) {
+      const double mu1_ = mu_u(i + 1, j), mu0 = mu_u(i, j), mv1 = mu_v(i, j + 1), mv0 = mu_v(i, j);
+      return msftx(i, j) * c.dnw[k] *
+             (c.rdx * ((c.c1h[k] * mu1_ + c.c2h[k]) * u2(i + 1, j, k) / msfuy(i + 1, j) -
+                       (c.c1h[k] * mu0 + c.c2h[k]) * u2(i, j, k) / msfuy(i, j)) +
+              c.rdy * ((c.c1h[k] * mv1 + c.c2h[k]) * v2(i, j + 1, k) * msfvx_inv(i, j + 1) -
+                       (c.c1h[k] * mv0 + c.c2h[k]) * v2(i, j, k) * msfvx_inv(i, j)));
+    };
+    cfor(nx, ny, [&](int i, int j) {
+      double dmdt = 0;
+      for (int k = 0; k < nz; ++k) dmdt += divv(i, j, k);
+      double w = 0;
+      ww(i, j, 0) = 0;
+      for (int k = 1; k < nz; ++k) {
+        w = w - c.dnw[k - 1] * c.c1h[k - 1] * dmdt - divv(i, j, k - 1);
+        ww(i, j, k) = w;
+      }
+      ww(i, j, nz) = 0;
+    });
+    fill(ww, g);
+  }
+
+  // ------------------------------------------------------------------ LargeStep::tendencies (large_step.cpp:798-861)
+  void rk_tendency() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool do_y = !g.two_d();
+    const bool top_lid = p.top_lid != 0;
+    const double hup = p.h_adv_order == 5 ? 1.0 : 0.0, vup = p.v_adv_order == 3 ? 1.0 : 0.0;
+    const double rdx = c.rdx, rdy = c.rdy;
+    const double wd_dt = p.w_damping == 1 ? g.dt : 0.0, w_crit = p.w_crit_cfl;
+
+    // KTendU (large_step.cpp:152-264), interior faces, no Coriolis, unmapped
+    {
+      auto fx = [&](int i, int j, int k) {
+        double vel = H * (ru(i, j, k) + ru(i - 1, j, k));
+        return vel * flux5(u2(i - 3, j, k), u2(i - 2, j, k), u2(i - 1, j, k), u2(i, j, k), u2(i + 1, j, k), u2(i + 2, j, k),
+                           vel, hup);
+      };
+      auto fy = [&](int i, int j, int k) {
+        double vel = H * (rv(i, j, k) + rv(i - 1, j, k));
+        return vel * flux5(u2(i, j - 3, k), u2(i, j - 2, k), u2(i, j - 1, k), u2(i, j, k), u2(i, j + 1, k), u2(i, j + 2, k),
+                           vel, hup);
+      };
+      auto fz = [&](int i, int j, int k) {
+        if (k == 0 || k == nz) return 0.0;
+        double vel = H * (ww(i - 1, j, k) + ww(i, j, k));
+        if (k == 1 || k == nz - 1) return vel * (c.fnm[k] * u2(i, j, k) + c.fnp[k] * u2(i, j, k - 1));
+        return vel * flux3(u2(i, j, k - 2), u2(i, j, k - 1), u2(i, j, k), u2(i, j, k + 1), -vel, vup);
+      };
+      auto pu = [&](int i, int j, int k) { return pp(i - 1, j, k) + pp(i, j, k); };
+      auto dpn = [&](int i, int j, int k) {
+        if (k == 0) return H * (c.cf1 * pu(i, j, 0) + c.cf2 * pu(i, j, 1) + c.cf3 * pu(i, j, 2));
+        if (k == nz) return top_lid ? H * (c.cfn * pu(i, j, nz - 1) + c.cfn1 * pu(i, j, nz - 2)) : 0.0;
+        return H * (c.fnm[k] * pu(i, j, k) + c.fnp[k] * pu(i, j, k - 1));
+      };
+      pfor(g.u_i0(), nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+        double t = -(msfux(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+        if (do_y) t -= (msfux(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        t -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+        const double rm = msfux(i, j) / msfuy(i, j);
+        double dpx = rm * H * rdx * (c.c1h[k] * muu(i, j) + c.c2h[k]) *
+                     ((ph2(i, j, k + 1) - ph2(i - 1, j, k + 1) + ph2(i, j, k) - ph2(i - 1, j, k)) +
+                      (alt(i, j, k) + alt(i - 1, j, k)) * (pp(i, j, k) - pp(i - 1, j, k)) +
+                      (al(i, j, k) + al(i - 1, j, k)) * (pb(i, j, k) - pb(i - 1, j, k)));
+        dpx += rm * rdx * (php(i, j, k) - php(i - 1, j, k)) *
+               (c.rdnw[k] * (dpn(i, j, k + 1) - dpn(i, j, k)) - H * (c.c1h[k] * mu2(i - 1, j) + c.c1h[k] * mu2(i, j)));
+        t -= 1.0 * dpx;  // mc.cqu = 1 (dry)
+        const double rw4 = Q * (rw(i - 1, j, k + 1) + rw(i - 1, j, k) + rw(i, j, k + 1) + rw(i, j, k));
+        t = t - u2(i, j, k) * kReradius * rw4;
+        ru_tend(i, j, k) = t;
+      });
+    }
+    // KTendV (large_step.cpp:267-376), 3-D only
+    if (do_y) {
+      auto fx = [&](int i, int j, int k) {
+        double vel = H * (ru(i, j, k) + ru(i, j - 1, k));
+        return vel * flux5(v2(i - 3, j, k), v2(i - 2, j, k), v2(i - 1, j, k), v2(i, j, k), v2(i + 1, j, k), v2(i + 2, j, k),
+                           vel, hup);
+      };
+      auto fy = [&](int i, int j, int k) {
+        double vel = H * (rv(i, j, k) + rv(i, j - 1, k));
+        return vel * flux5(v2(i, j - 3, k), v2(i, j - 2, k), v2(i, j - 1, k), v2(i, j, k), v2(i, j + 1, k), v2(i, j + 2, k),
+                           vel, hup);
+      };
+      auto fz = [&](int i, int j, int k) {
+        if (k == 0 || k == nz) return 0.0;
+        double vel = H * (ww(i, j - 1, k) + ww(i, j, k));
+        if (k == 1 || k == nz - 1) return vel * (c.fnm[k] * v2(i, j, k) + c.fnp[k] * v2(i, j, k - 1));
+        return vel * flux3(v2(i, j, k - 2), v2(i, j, k - 1), v2(i, j, k), v2(i, j, k + 1), -vel, vup);
+      };
+      auto pv = [&](int i, int j, int k) { return pp(i, j - 1, k) + pp(i, j, k); };
+      auto dpn = [&](int i, int j, int k) {
+        if (k == 0) return H * (c.cf1 * pv(i, j, 0) + c.cf2 * pv(i, j, 1) + c.cf3 * pv(i, j, 2));
+        if (k == nz) return top_lid ? H * (c.cfn * pv(i, j, nz - 1) + c.cfn1 * pv(i, j, nz - 2)) : 0.0;
+        return H * (c.fnm[k] * pv(i, j, k) + c.fnp[k] * pv(i, j, k - 1));
+      };
+      pfor(0, nx, g.v_j0(), ny, 0, nz, [&](int i, int j, int k) {
+        const double rm = msfvy(i, j) / msfvx(i, j);
+        double t = -(msfvy(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+        t -= (msfvy(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        t -= rm * c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+        double dpy = rm * H * rdy * (c.c1h[k] * muv(i, j) + c.c2h[k]) *
+                     ((ph2(i, j, k + 1) - ph2(i, j - 1, k + 1) + ph2(i, j, k) - ph2(i, j - 1, k)) +
+                      (alt(i, j, k) + alt(i, j - 1, k)) * (pp(i, j, k) - pp(i, j - 1, k)) +
+                      (al(i, j, k) + al(i, j - 1, k)) * (pb(i, j, k) - pb(i, j - 1, k)));
+        dpy += rm * rdy * (php(i, j, k) - php(i, j - 1, k)) *
+               (c.rdnw[k] * (dpn(i, j, k + 1) - dpn(i, j, k)) - H * (c.c1h[k] * mu2(i, j - 1) + c.c1h[k] * mu2(i, j)));
+        t -= 1.0 * dpy;  // mc.cqv = 1 (dry)
+        const double rw4 = Q * (rw(i, j - 1, k + 1) + rw(i, j - 1, k) + rw(i, j, k + 1) + rw(i, j, k));
+        t = t - rm * v2(i, j, k) * kReradius * rw4;
+        rv_tend(i, j, k) = t;
+      });
+    }
+    // KTendW (large_step.cpp:379-499), levels 1..nz
+    {
+      auto rux = [&](int i, int j, int k) {
+        if (k < nz) return c.fnm[k] * ru(i, j, k) + c.fnp[k] * ru(i, j, k - 1);
+        return (2.0 - c.fnm[nz - 1]) * ru(i, j, nz - 1) - c.fnp[nz - 1] * ru(i, j, nz - 2);
+      };
+      auto rvy = [&](int i, int j, int k) {
+        if (k < nz) return c.fnm[k] * rv(i, j, k) + c.fnp[k] * rv(i, j, k - 1);
+        return (2.0 - c.fnm[nz - 1]) * rv(i, j, nz - 1) - c.fnp[nz - 1] * rv(i, j, nz - 2);
+      };
+      auto fx = [&](int i, int j, int k) {
+        double vel = rux(i, j, k);
+        return vel * flux5(w2(i - 3, j, k), w2(i - 2, j, k), w2(i - 1, j, k), w2(i, j, k), w2(i + 1, j, k), w2(i + 2, j, k),
+                           vel, hup);
+      };
+      auto fy = [&](int i, int j, int k) {
+        double vel = rvy(i, j, k);
+        return vel * flux5(w2(i, j - 3, k), w2(i, j - 2, k), w2(i, j - 1, k), w2(i, j, k), w2(i, j + 1, k), w2(i, j + 2, k),
+                           vel, hup);
+      };
+      auto fz = [&](int i, int j, int m) {
+        if (m == 0 || m == nz - 1) return 0.25 * (ww(i, j, m + 1) + ww(i, j, m)) * (w2(i, j, m + 1) + w2(i, j, m));
+        double vel = H * (ww(i, j, m + 1) + ww(i, j, m));
+        return vel * flux3(w2(i, j, m - 1), w2(i, j, m), w2(i, j, m + 1), w2(i, j, m + 2), -vel, vup);
+      };
+      pfor(0, nx, 0, ny, 1, nz + 1, [&](int i, int j, int k) {
+        double t = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+        if (do_y) t -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        const double rmsft = 1.0 / msfty(i, j);
+        if (k < nz) {
+          t -= c.rdn[k] * (fz(i, j, k) - fz(i, j, k - 1));
+          const double cqw = 0.0, cq1 = 1.0 / (1.0 + cqw), cq2 = cqw * cq1;
+          t += rmsft * kG * (cq1 * c.rdn[k] * (pp(i, j, k) - pp(i, j, k - 1)) - c.c1f[k] * mu2(i, j) -
+                             cq2 * (c.c1f[k] * mub(i, j) + c.c2f[k]));
+          if (wd_dt != 0.0) {
+            const double muf = c.c1f[k] * mut(i, j) + c.c2f[k];
+            double cfl = ww(i, j, k) / muf * c.rdnw[k] * wd_dt;
+            cfl = cfl < 0.0 ? -cfl : cfl;
+            if (cfl > 1.0) t = t - (w2(i, j, k) < 0.0 ? -1.0 : 1.0) * 0.3 * (cfl - w_crit) * muf;
+          }
+          const double ruw =
+              H * (c.fnm[k] * (ru(i, j, k) + ru(i + 1, j, k)) + c.fnp[k] * (ru(i, j, k - 1) + ru(i + 1, j, k - 1)));
+          const double rvw =
+              H * (c.fnm[k] * (rv(i, j, k) + rv(i, j + 1, k)) + c.fnp[k] * (rv(i, j, k - 1) + rv(i, j + 1, k - 1)));
+          const double rmt = msftx(i, j) / msfty(i, j);
+          const double uw =
+              H * (c.fnm[k] * (u2(i, j, k) + u2(i + 1, j, k)) + c.fnp[k] * (u2(i, j, k - 1) + u2(i + 1, j, k - 1)));
+          const double vw =
+              H * (c.fnm[k] * (v2(i, j, k) + v2(i, j + 1, k)) + c.fnp[k] * (v2(i, j, k - 1) + v2(i, j + 1, k - 1)));
+          t = t + kReradius * (ruw * uw + rmt * rvw * vw);
+        } else {
+          t += 2.0 * c.rdn[nz - 1] * fz(i, j, nz - 1);
+          const double cqw = 0.0, cq1 = 1.0 / (1.0 + cqw), cq2 = cqw * cq1;
+          t += rmsft * kG * (cq1 * 2.0 * c.rdnw[nz - 1] * (-pp(i, j, nz - 1)) - c.c1f[nz] * mu2(i, j) -
+                             cq2 * (c.c1f[nz] * mub(i, j) + c.c2f[nz]));
+        }
+        rw_tend(i, j, k) = t;
+      });
+    }
+    // KTendT (large_step.cpp:502-565)
+    {
+      auto fx = [&](int i, int j, int k) {
+        double vel = ru(i, j, k);
+        return vel * flux5(t2(i - 3, j, k), t2(i - 2, j, k), t2(i - 1, j, k), t2(i, j, k), t2(i + 1, j, k), t2(i + 2, j, k),
+                           vel, hup);
+      };
+      auto fy = [&](int i, int j, int k) {
+        double vel = rv(i, j, k);
+        return vel * flux5(t2(i, j - 3, k), t2(i, j - 2, k), t2(i, j - 1, k), t2(i, j, k), t2(i, j + 1, k), t2(i, j + 2, k),
+                           vel, hup);
+      };
+      auto fz = [&](int i, int j, int k) {
+        if (k == 0 || k == nz) return 0.0;
+        double vel = ww(i, j, k);
+        if (k == 1 || k == nz - 1) return vel * (c.fnm[k] * t2(i, j, k) + c.fnp[k] * t2(i, j, k - 1));
+        return vel * flux3(t2(i, j, k - 2), t2(i, j, k - 1), t2(i, j, k), t2(i, j, k + 1), -vel, vup);
+      };
+      pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+        double r = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+        if (do_y) r -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        r -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+        t_tend(i, j, k) = r;
+      });
+    }
+    // KTendPh rhs_ph (large_step.cpp:570-668), interior (no specified/open edges)
+    {
+      auto wdwn = [&](int i, int j, int m) {
+        return H * (ww(i, j, m + 1) + ww(i, j, m)) * c.rdnw[m] *
+               (ph2(i, j, m + 1) - ph2(i, j, m) + phb(i, j, m + 1) - phb(i, j, m));
+      };
+      auto grad6x = [&](int i, int j, int k) {
+        return 45.0 * (ph2(i + 1, j, k) - ph2(i - 1, j, k)) - 9.0 * (ph2(i + 2, j, k) - ph2(i - 2, j, k)) +
+               (ph2(i + 3, j, k) - ph2(i - 3, j, k)) + 45.0 * (phb(i + 1, j, k) - phb(i - 1, j, k)) -
+               9.0 * (phb(i + 2, j, k) - phb(i - 2, j, k)) + (phb(i + 3, j, k) - phb(i - 3, j, k));
+      };
+      auto grad6y = [&](int i, int j, int k) {
+        return 45.0 * (ph2(i, j + 1, k) - ph2(i, j - 1, k)) - 9.0 * (ph2(i, j + 2, k) - ph2(i, j - 2, k)) +
+               (ph2(i, j + 3, k) - ph2(i, j - 3, k)) + 45.0 * (phb(i, j + 1, k) - phb(i, j - 1, k)) -
+               9.0 * (phb(i, j + 2, k) - phb(i, j - 2, k)) + (phb(i, j + 3, k) - phb(i, j - 3, k));
+      };
+      pfor(0, nx, 0, ny, 1, nz + 1, [&](int i, int j, int k) {
+        double t = 0;
+        if (k < nz) t = t - (c.fnm[k] * wdwn(i, j, k) + c.fnp[k] * wdwn(i, j, k - 1));
+        t = t + (c.c1f[k] * mut(i, j) + c.c2f[k]) * kG * w2(i, j, k) / msfty(i, j);
+        const double r60 = 1.0 / 60.0;
+        if (do_y) {
+          const double mvp = c.c1f[k] * muv(i, j + 1) + c.c2f[k], mvm = c.c1f[k] * muv(i, j) + c.c2f[k];
+          const double vp =
+              k < nz ? v2(i, j + 1, k) + v2(i, j + 1, k - 1) : c.cfn * v2(i, j + 1, nz - 1) + c.cfn1 * v2(i, j + 1, nz - 2);
+          const double vm = k < nz ? v2(i, j, k) + v2(i, j, k - 1) : c.cfn * v2(i, j, nz - 1) + c.cfn1 * v2(i, j, nz - 2);
+          const double cy = (k < nz ? 0.25 : H) * rdy / msfty(i, j);
+          t = t - cy * ((mvp * vp * msfvy(i, j + 1) + mvm * vm * msfvy(i, j)) * r60 * grad6y(i, j, k));
+        }
+        const double mup = c.c1f[k] * muu(i + 1, j) + c.c2f[k], mum = c.c1f[k] * muu(i, j) + c.c2f[k];
+        const double up =
+            k < nz ? u2(i + 1, j, k) + u2(i + 1, j, k - 1) : c.cfn * u2(i + 1, j, nz - 1) + c.cfn1 * u2(i + 1, j, nz - 2);
+        const double um = k < nz ? u2(i, j, k) + u2(i, j, k - 1) : c.cfn * u2(i, j, nz - 1) + c.cfn1 * u2(i, j, nz - 2);
+        const double cx = (k < nz ? 0.25 : H) * rdx / msfty(i, j);
+        t = t - cx * ((mup * up * msfux(i + 1, j) + mum * um * msfux(i, j)) * r60 * grad6x(i, j, k));
+        ph_tend(i, j, k) = t;
+      });
+    }
+  }
+
+  // ------------------------------------------------------------------ SmallStep::run_stage (small_step.cpp:490-607)
+  // p_rho_point (small_step.cpp:97-114)
+  void p_rho_point(int i, int j, int k, double smdiv, bool damp) {
+    const double m = c.c1h[k] * muts(i, j) + c.c2h[k];
+    const double a = -((1. / m) * (alt(i, j, k) * (c.c1h[k] * mu2(i, j)) + c.rdnw[k] * (ph2(i, j, k + 1) - ph2(i, j, k))));
+    al(i, j, k) = a;
+    double p_ = c2a(i, j, k) *
+                (alt(i, j, k) * (t2(i, j, k) - c.c1h[k] * mu2(i, j) * t_save(i, j, k)) / (m * (kT0 + t_save(i, j, k))) - a);
+    if (damp) {
+      const double ptmp = p_;
+      p_ = p_ + smdiv * (p_ - pm1(i, j, k));
+      pm1(i, j, k) = ptmp;
+    } else {
+      pm1(i, j, k) = p_;
+    }
+    pp(i, j, k) = p_;
+  }
+
+  void small_steps(int rk) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz, nyv_ = nyv();
+    const bool do_y = !g.two_d();
+    const double dt = g.dt, dts_full = dt / n_sound;
+    const double dt_rk = rk == 1 ? dt / 3.0 : (rk == 2 ? 0.5 * dt : dt);
+    const double dts = rk == 1 ? dt_rk : dts_full;
+    const int ns = rk == 1 ? 1 : (rk == 2 ? n_sound / 2 : n_sound);
+    const double epssm = p.epssm;
+    if (rk == 1) std::fill(mudf.a.begin(), mudf.a.end(), 0.0);
+    // KPrep small_step_prep (small_step.cpp:39-93)
+    const bool first = rk == 1;
+    pfor(0, nx + 1, 0, nyv_, 0, nz + 1, [&](int i, int j, int k) {
+      const double mts = mub(i, j) + mu1(i, j);
+      const double mus_u = first ? muu(i, j) : H * (mts + mub(i - 1, j) + mu1(i - 1, j));
+      const double mus_v = first ? muv(i, j) : H * (mts + mub(i, j - 1) + mu1(i, j - 1));
+      if (k == 0) {
+        if (i < nx && j < ny) { muts(i, j) = mts; mu_save(i, j) = mu2(i, j); mu2(i, j) = mu1(i, j) - mu2(i, j); }
+        if (j < ny) muus(i, j) = mus_u;
+        if (i < nx && j < nyv_) muvs(i, j) = mus_v;
+      }
+      if (j < ny && k < nz) {
+        const double ui = u2(i, j, k);
+        u_save(i, j, k) = ui;
+        u2(i, j, k) = ((c.c1h[k] * mus_u + c.c2h[k]) * u1(i, j, k) - (c.c1h[k] * muu(i, j) + c.c2h[k]) * ui) / msfuy(i, j);
+      }
+      if (i < nx && j < nyv_ && k < nz) {
+        const double vi = v2(i, j, k);
+        v_save(i, j, k) = vi;
+        v2(i, j, k) =
+            ((c.c1h[k] * mus_v + c.c2h[k]) * v1(i, j, k) - (c.c1h[k] * muv(i, j) + c.c2h[k]) * vi) * msfvx_inv(i, j);
+      }
+      if (i < nx && j < ny) {
+        if (k < nz) {
+          const double ti = t2(i, j, k);
+          t_save(i, j, k) = ti;
+          t2(i, j, k) = (c.c1h[k] * mts + c.c2h[k]) * t1(i, j, k) - (c.c1h[k] * mut(i, j) + c.c2h[k]) * ti;
+          c2a(i, j, k) = kCpovcv * (pb(i, j, k) + pp(i, j, k)) / alt(i, j, k);
+        }
+        const double wi = w2(i, j, k), phi = ph2(i, j, k);
+        w_save(i, j, k) = wi;
+        w2(i, j, k) = ((c.c1f[k] * mts + c.c2f[k]) * w1(i, j, k) - (c.c1f[k] * mut(i, j) + c.c2f[k]) * wi) / msfty(i, j);
+        ph_save(i, j, k) = phi;
+        ph2(i, j, k) = ph1(i, j, k) - phi;
+        ww1(i, j, k) = (k == 0 || k == nz) ? 0.0 : ww(i, j, k);
+      }
+    });
+    for (F* f : {&u2, &v2, &ph2, &t_save, &u_save, &v_save}) fill(*f, g);
+    for (F* f : {&mu2, &muts}) fill(*f, g);
+    // KPRho calc_p_rho (small_step.cpp:116-121)
+    pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) { p_rho_point(i, j, k, 0.0, false); });
+    for (F* f : {&pp, &al}) fill(*f, g);
+    // KCoefW calc_coef_w (small_step.cpp:124-148), dry (cq1 = 1)
+    const double cof_h = 0.5 * dts * kG * (1.0 + epssm);
+    const double cof = cof_h * cof_h;
+    const double lid = p.top_lid ? 0.0 : 1.0;
+    cfor(nx, ny, [&](int i, int j) {
+      auto mh = [&](int k, double m) { return c.c1h[k] * m + c.c2h[k]; };
+      auto mf = [&](int k, double m) { return c.c1f[k] * m + c.c2f[k]; };
+      const double m = mut(i, j);
+      ca(i, j, 0) = 0; calpha(i, j, 0) = 0; cgamma(i, j, 0) = 0;
+      ca(i, j, 1) = 0;
+      ca(i, j, nz) = -2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(i, j, nz - 1) * lid / (mh(nz - 1, m) * mf(nz - 1, m));
+      for (int k = 2; k < nz; ++k)
+        ca(i, j, k) = -1.0 * cof * c.rdn[k] * c.rdnw[k - 1] * c2a(i, j, k - 1) / (mh(k - 1, m) * mf(k - 1, m));
+      for (int k = 1; k < nz; ++k) {
+        const double cq1 = 1.0;
+        const double b = 1.0 + cq1 * cof * c.rdn[k] *
+                                   (c.rdnw[k] * c2a(i, j, k) / (mh(k, m) * mf(k, m)) +
+                                    c.rdnw[k - 1] * c2a(i, j, k - 1) / (mh(k - 1, m) * mf(k, m)));
+        const double cc = -cq1 * cof * c.rdn[k] * c.rdnw[k] * c2a(i, j, k) / (mh(k, m) * mf(k + 1, m));
+        calpha(i, j, k) = 1.0 / (b - ca(i, j, k) * cgamma(i, j, k - 1));
+        cgamma(i, j, k) = cc * calpha(i, j, k);
+      }
+      const double b = 1.0 + 2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(i, j, nz - 1) / (mh(nz - 1, m) * mf(nz, m));
+      calpha(i, j, nz) = 1.0 / (b - ca(i, j, nz) * cgamma(i, j, nz - 1));
+      cgamma(i, j, nz) = 0;
+    });
+
+    const double emdx = p.emdiv * (1. / c.rdx), emdy = do_y ? p.emdiv * (1. / c.rdy) : 0.0;
+    const int ui0 = g.u_i0(), ui1 = nx, vj0 = g.v_j0(), vj1 = ny;
+    for (int it = 1; it <= ns; ++it) {
+      // KUV advance_uv (small_step.cpp:151-190)
+      auto dpn = [&](int di, int dj, int i, int j, int k) {
+        auto ps = [&](int kk) { return pp(i, j, kk) + pp(i - di, j - dj, kk); };
+        if (k == 0) return H * (c.cf1 * ps(0) + c.cf2 * ps(1) + c.cf3 * ps(2));
+        if (k == nz) return p.top_lid ? H * (c.cf1 * ps(nz - 1) + c.cf2 * ps(nz - 2) + c.cf3 * ps(nz - 3)) : 0.0;
+        return H * (c.fnm[k] * ps(k) + c.fnp[k] * ps(k - 1));
+      };
+      auto dp = [&](int di, int dj, int i, int j, int k, double mface, double rd, double rm) {
+        double d = rm * H * rd * (c.c1h[k] * mface + c.c2h[k]) *
+                   (((ph2(i, j, k + 1) - ph2(i - di, j - dj, k + 1)) + (ph2(i, j, k) - ph2(i - di, j - dj, k))) +
+                    (alt(i, j, k) + alt(i - di, j - dj, k)) * (pp(i, j, k) - pp(i - di, j - dj, k)) +
+                    (al(i, j, k) + al(i - di, j - dj, k)) * (pb(i, j, k) - pb(i - di, j - dj, k)));
+        d += rm * rd * (php(i, j, k) - php(i - di, j - dj, k)) *
+             (c.rdnw[k] * (dpn(di, dj, i, j, k + 1) - dpn(di, dj, i, j, k)) -
+              H * (c.c1h[k] * mu2(i - di, j - dj) + c.c1h[k] * mu2(i, j)));
+        return d;
+      };
+      pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+        if (i >= ui0 && i < ui1) {
+          const double mudf_x = -emdx * (mudf(i, j) - mudf(i - 1, j)) / msfuy(i, j);
+          u2(i, j, k) = u2(i, j, k) + dts * ru_tend(i, j, k) -
+                        dts * 1.0 * dp(1, 0, i, j, k, muu(i, j), c.rdx, msfux(i, j) / msfuy(i, j)) + c.c1h[k] * mudf_x;
+        }
+        if (do_y && j >= vj0 && j < vj1) {
+          const double mudf_y = -emdy * (mudf(i, j) - mudf(i, j - 1)) * msfvx_inv(i, j);
+          v2(i, j, k) = v2(i, j, k) + dts * rv_tend(i, j, k) -
+                        dts * 1.0 * dp(0, 1, i, j, k, muv(i, j), c.rdy, msfvy(i, j) / msfvx(i, j)) + c.c1h[k] * mudf_y;
+        }
+      });
+      if (do_y) { fill(u2, g); fill(v2, g); }
+      else fill(u2, g);
+      // KMuTDiv (small_step.cpp:194-207)
+      pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+        const double fx = (u2(i + 1, j, k) + (c.c1h[k] * muu(i + 1, j) + c.c2h[k]) * u_save(i + 1, j, k) / msfuy(i + 1, j)) -
+                          (u2(i, j, k) + (c.c1h[k] * muu(i, j) + c.c2h[k]) * u_save(i, j, k) / msfuy(i, j));
+        double d = c.rdx * fx;
+        if (do_y)
+          d += c.rdy * ((v2(i, j + 1, k) + (c.c1h[k] * muv(i, j + 1) + c.c2h[k]) * v_save(i, j + 1, k) * msfvx_inv(i, j + 1)) -
+                        (v2(i, j, k) + (c.c1h[k] * muv(i, j) + c.c2h[k]) * v_save(i, j, k) * msfvx_inv(i, j)));
+        dvdxi(i, j, k) = msftx(i, j) * msfty(i, j) * d;
+      });
+      // KMuTCol (small_step.cpp:209-248)
+      const bool firstit = it == 1, last = it == ns;
+      const double rn = double(ns);
+      cfor(nx, ny, [&](int i, int j) {
+        double dmdt = 0;
+        for (int k = 0; k < nz; ++k) dmdt += c.dnw[k] * dvdxi(i, j, k);
+        const double mtend = mu_tend(i, j);
+        const double mu_old = mu2(i, j);
+        const double mu_new = mu_old + dts * (dmdt + mtend);
+        mu2(i, j) = mu_new;
+        mudf(i, j) = dmdt + mtend;
+        muts(i, j) = mut(i, j) + mu_new;
+        muave(i, j) = H * ((1.0 + epssm) * mu_new + (1.0 - epssm) * mu_old);
+        auto acc_ww = [&](int k, double wpp) {
+          const double sw = (firstit ? 0.0 : ww_m(i, j, k)) + wpp;
+          ww_m(i, j, k) = last ? sw / rn + ww1(i, j, k) : sw;
+        };
+        double ww_full = 0;
+        ww(i, j, 0) = 0;
+        acc_ww(0, 0.0);
+        for (int k = 0; k < nz; ++k) {
+          double wpp_hi = 0;
+          if (k + 1 < nz) {
+            ww_full = ww_full - c.dnw[k] * (c.c1h[k] * dmdt + dvdxi(i, j, k) + c.c1h[k] * mtend) / msfty(i, j);
+            wpp_hi = ww_full - ww1(i, j, k + 1);
+            ww(i, j, k + 1) = wpp_hi;
+          } else {
+            ww(i, j, nz) = 0;
+          }
+          acc_ww(k + 1, wpp_hi);
+        }
+      });
+      // KMuTPoint (small_step.cpp:250-276)
+      pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+        auto wdtn = [&](int kk) {
+          if (kk == 0 || kk == nz) return 0.0;
+          return ww(i, j, kk) * (c.fnm[kk] * t_save(i, j, kk) + c.fnp[kk] * t_save(i, j, kk - 1));
+        };
+        const double su = (firstit ? 0.0 : ru_m(i, j, k)) + u2(i, j, k);
+        ru_m(i, j, k) = last ? su / rn + (c.c1h[k] * muu(i, j) + c.c2h[k]) * u_save(i, j, k) / msfuy(i, j) : su;
+        if (do_y) {
+          const double sv = (firstit ? 0.0 : rv_m(i, j, k)) + v2(i, j, k);
+          rv_m(i, j, k) = last ? sv / rn + (c.c1h[k] * muv(i, j) + c.c2h[k]) * v_save(i, j, k) * msfvx_inv(i, j) : sv;
+        }
+        const double tk = t2(i, j, k);
+        t_2save(i, j, k) = tk;
+        double hflux = H * c.rdx *
+                       (u2(i + 1, j, k) * (t_save(i + 1, j, k) + t_save(i, j, k)) -
+                        u2(i, j, k) * (t_save(i, j, k) + t_save(i - 1, j, k)));
+        if (do_y)
+          hflux += H * c.rdy *
+                   (v2(i, j + 1, k) * (t_save(i, j + 1, k) + t_save(i, j, k)) -
+                    v2(i, j, k) * (t_save(i, j, k) + t_save(i, j - 1, k)));
+        const double tn = tk + msfty(i, j) * dts * t_tend(i, j, k);
+        t2(i, j, k) = tn - dts * msfty(i, j) * (msftx(i, j) * hflux + c.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
+      });
+      for (F* f : {&mu2, &mudf, &muts}) fill(*f, g);
+      // advance_w: KWRhs, KWExpl, KWSolve, KWPRho (small_step.cpp:292-416)
+      auto mh = [&](int k, double m) { return c.c1h[k] * m + c.c2h[k]; };
+      auto mf = [&](int k, double m) { return c.c1f[k] * m + c.c2f[k]; };
+      pfor(0, nx, 0, ny, 0, nz + 1, [&](int i, int j, int k) {
+        // KWCommon::rhs (small_step.cpp:312-319)
+        double R;
+        if (k == 0) R = 0;
+        else if (p.top_lid && k == nz) R = 0;
+        else {
+          auto wdwn = [&](int m) {
+            return H * (ww(i, j, m) + ww(i, j, m - 1)) * c.rdnw[m - 1] *
+                   (ph_save(i, j, m) - ph_save(i, j, m - 1) + phb(i, j, m) - phb(i, j, m - 1));
+          };
+          double r = dts * (ph_tend(i, j, k) + H * kG * (1.0 - epssm) * w2(i, j, k));
+          if (k < nz) r -= dts * (c.fnm[k] * wdwn(k + 1) + c.fnp[k] * wdwn(k));
+          R = ph2(i, j, k) + msfty(i, j) * r / mf(k, mut(i, j));
+        }
+        w_rhs(i, j, k) = R;
+        // KWCommon::t2ave (small_step.cpp:302-305)
+        if (k < nz) {
+          const double tav = H * ((1.0 + epssm) * t2(i, j, k) + (1.0 - epssm) * t_2save(i, j, k));
+          t2a(i, j, k) = (tav + c.c1h[k] * muave(i, j) * kT0) / (mh(k, muts(i, j)) * (kT0 + t_save(i, j, k)));
+        }
+      });
+      pfor(0, nx, 0, ny, 0, nz + 1, [&](int i, int j, int k) {  // KWExpl (small_step.cpp:331-378), flat terrain
+        if (k == 0) { w2(i, j, 0) = 0; return; }
+        const double m = mut(i, j), mav = muave(i, j);
+        const double ope = 1.0 + epssm, ome = 1.0 - epssm;
+        const double msft_inv = 1.0 / msfty(i, j);
+        const double r_k = w_rhs(i, j, k), r_km1 = w_rhs(i, j, k - 1), ph_k = ph2(i, j, k), ph_km1 = ph2(i, j, k - 1);
+        if (k < nz) {
+          const double r_kp1 = w_rhs(i, j, k + 1), ph_kp1 = ph2(i, j, k + 1);
+          w2(i, j, k) = w2(i, j, k) + dts * rw_tend(i, j, k) +
+                        msft_inv * 1.0 *
+                            (H * dts * kG * c.rdn[k] *
+                             (c2a(i, j, k) * c.rdnw[k] / mh(k, m) * (ope * (r_kp1 - r_k) + ome * (ph_kp1 - ph_k)) -
+                              c2a(i, j, k - 1) * c.rdnw[k - 1] / mh(k - 1, m) * (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)))) +
+                        dts * kG * msft_inv *
+                            (c.rdn[k] * (c2a(i, j, k) * alt(i, j, k) * t2a(i, j, k) - c2a(i, j, k - 1) * alt(i, j, k - 1) * t2a(i, j, k - 1)) -
+                             c.c1f[k] * mav);
+        } else {
+          const int km = nz - 1;
+          double wn = w2(i, j, k) + dts * rw_tend(i, j, k) +
+                      msft_inv * (-H * dts * kG / mh(km, m) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * c2a(i, j, km) *
+                                      (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)) -
+                                  dts * kG * (2.0 * c.rdnw[km] * c2a(i, j, km) * alt(i, j, km) * t2a(i, j, km) + c.c1f[k] * mav));
+          if (p.top_lid) wn = 0;
+          w2(i, j, k) = wn;
+        }
+      });
+      const double dampmag = p.damp_opt == 3 ? dts * p.dampcoef : 0.0, hdepth = p.zdamp;
+      cfor(nx, ny, [&](int i, int j) {  // KWSolve (small_step.cpp:381-407)
+        const double m = mut(i, j), mts = muts(i, j), ope = 1.0 + epssm;
+        for (int k = 1; k <= nz; ++k) w2(i, j, k) = (w2(i, j, k) - ca(i, j, k) * w2(i, j, k - 1)) * calpha(i, j, k);
+        for (int k = nz - 1; k >= 1; --k) w2(i, j, k) = w2(i, j, k) - cgamma(i, j, k) * w2(i, j, k + 1);
+        if (dampmag != 0.0) {
+          const double pi = 4.0 * std::atan(1.0);
+          const double htop = (ph_save(i, j, nz) + phb(i, j, nz)) / kG, hbot = htop - hdepth;
+          for (int k = nz; k >= 1; --k) {
+            const double hk = (ph_save(i, j, k) + phb(i, j, k)) / kG;
+            double dampwt = 0;
+            if (hk >= hbot) {
+              const double sn = std::sin(H * pi * (hk - hbot) / hdepth);
+              dampwt = dampmag * sn * sn;
+            }
+            w2(i, j, k) = (w2(i, j, k) - dampwt * mf(k, m) * w_save(i, j, k)) / (1.0 + dampwt);
+          }
+        }
+        for (int k = nz; k >= 1; --k)
+          ph2(i, j, k) = w_rhs(i, j, k) + msfty(i, j) * H * dts * kG * ope * w2(i, j, k) / mf(k, mts);
+      });
+      pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) { p_rho_point(i, j, k, p.smdiv, true); });  // KWPRho
+      for (F* f : {&ph2, &pp, &al}) fill(*f, g);
+    }
+    for (F* f : {&muts, &ru_m, &rv_m, &ww_m}) fill(*f, g);
+    // KFinish small_step_finish (small_step.cpp:419-446), dry
+    pfor(0, nx + 1, 0, nyv_, 0, nz + 1, [&](int i, int j, int k) {
+      if (j < ny && k < nz) {
+        const double mus_u = H * (muts(i, j) + muts(i - 1, j));
+        u2(i, j, k) = (msfuy(i, j) * u2(i, j, k) + u_save(i, j, k) * (c.c1h[k] * muu(i, j) + c.c2h[k])) / (c.c1h[k] * mus_u + c.c2h[k]);
+      }
+      if (i < nx && j < nyv_ && k < nz) {
+        const double mus_v = H * (muts(i, j) + muts(i, j - 1));
+        v2(i, j, k) = (msfvx(i, j) * v2(i, j, k) + v_save(i, j, k) * (c.c1h[k] * muv(i, j) + c.c2h[k])) / (c.c1h[k] * mus_v + c.c2h[k]);
+      }
+      if (i < nx && j < ny) {
+        const double m = mut(i, j), mts = muts(i, j);
+        w2(i, j, k) = (msfty(i, j) * w2(i, j, k) + w_save(i, j, k) * (c.c1f[k] * m + c.c2f[k])) / (c.c1f[k] * mts + c.c2f[k]);
+        ph2(i, j, k) = ph2(i, j, k) + ph_save(i, j, k);
+        ww(i, j, k) = ww(i, j, k) + ww1(i, j, k);
+        if (k < nz)
+          t2(i, j, k) = (t2(i, j, k) - 0.0 + t_save(i, j, k) * (c.c1h[k] * m + c.c2h[k])) / (c.c1h[k] * mts + c.c2h[k]);
+      }
+    });
+    // KFinishMu (small_step.cpp:448-454)
+    pfor(0, nx + 1, 0, nyv_, 0, 1, [&](int i, int j, int) {
+      if (i < nx && j < ny) mu2(i, j) = mu2(i, j) + mu_save(i, j);
+      if (j < ny) muus(i, j) = H * (muts(i, j) + muts(i - 1, j));
+      if (i < nx && j < nyv_) muvs(i, j) = H * (muts(i, j) + muts(i, j - 1));
+    });
+    for (F* f : {&u2, &v2, &w2, &t2, &ph2, &ww, &mu2}) fill(*f, g);
+  }
+
+  // ------------------------------------------------------------------ calc_p_rho_phi (small_step.cpp:456-477,609-617)
+  void calc_p_rho_phi() {
+    const int nx = g.nx;
+    pfor(0, nx, 0, g.ny, 0, g.nz, [&](int i, int j, int k) {
+      const double a = -1.0 / (c.c1h[k] * muts(i, j) + c.c2h[k]) *
+                       (alb(i, j, k) * (c.c1h[k] * mu2(i, j)) + c.rdnw[k] * (ph2(i, j, k + 1) - ph2(i, j, k)));
+      al(i, j, k) = a;
+      const double base = kRd * (kT0 + t2(i, j, k)) / (kP1000 * (a + alb(i, j, k)));
+      double pw;
+      // vm::pow lanes (backend.hpp:144,169-175): libmvec for pairs of the nx-trip loop, scalar for an odd tail
+      if (libmvec && nx > 1 && !((nx & 1) && i == nx - 1))
+        pw = _mm_cvtsd_f64(_ZGVbN2vv_pow(_mm_set1_pd(base), _mm_set1_pd(kCpovcv)));
+      else
+        pw = std::pow(base, kCpovcv);
+      pp(i, j, k) = kP1000 * pw - pb(i, j, k);
+    });
+    fill(pp, g);
+    fill(al, g);
+  }
+
+  // RK3::step's copies (rk3.cpp:94-98)
+  void step_begin() {
+    u1.a = u2.a; v1.a = v2.a; w1.a = w2.a; t1.a = t2.a; ph1.a = ph2.a; mu1.a = mu2.a;
+    fill(mu1, g);
+  }
+  void step() {  // RK3::step (rk3.cpp:86-170), dry, no physics / boundaries / diffusion
+    step_begin();
+    for (int rk = 1; rk <= 3; ++rk) {
+      rk_prep();
+      rk_tendency();
+      small_steps(rk);
+      calc_p_rho_phi();
+    }
+  }
+
+  F* field(int id) {
+    switch (id) {
+      case MI_U: return &u2; case MI_V: return &v2; case MI_W: return &w2; case MI_T: return &t2;
+      case MI_PH: return &ph2; case MI_MU: return &mu2; case MI_P: return &pp; case MI_AL: return &al;
+      case MI_WW: return &ww; case MI_RU_M: return &ru_m; case MI_RV_M: return &rv_m; case MI_WW_M: return &ww_m;
+      case MI_ALT: return &alt; case MI_PHP: return &php; case MI_MUTS: return &muts;
+      case MI_RU_TEND: return &ru_tend; case MI_RV_TEND: return &rv_tend; case MI_RW_TEND: return &rw_tend;
+      case MI_T_TEND: return &t_tend; case MI_PH_TEND: return &ph_tend;
+      case MI_MUB: return &mub; case MI_PB: return &pb; case MI_ALB: return &alb; case MI_PHB: return &phb;
+      case MI_T_INIT: return &t_init; case MI_MSFTX: return &msftx; case MI_MSFTY: return &msfty;
+      case MI_MSFUX: return &msfux; case MI_MSFUY: return &msfuy; case MI_MSFVX: return &msfvx;
+      case MI_MSFVY: return &msfvy; case MI_MSFVX_INV: return &msfvx_inv;
+      default: return nullptr;
+    }
+  }
+};
+
+bool is_base(int id) { return id >= MI_MUB; }
+
+int fail(const std::string& m, int code = MI_EINVAL) {
+  g_err = m;
+  return code;
+}
+
+}  // namespace
+
+struct orc_domain {
+  std::unique_ptr<Dom> d;
+};
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+int mi_auto_sound_steps_oracle(double dt, double dx, double dy) { return Dom::mi_auto_sound_steps_(dt, dx, dy); }
+
+int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv, orc_domain** out) {
+  try {
+    if (!g || !p || !lv || !out) return fail("null argument");
+    if (g->nx < 1 || g->ny < 1 || g->nz < 3) return fail("bad grid size");
+    if (g->bc_x > MI_BC_SYMMETRIC || g->bc_y > MI_BC_SYMMETRIC) return fail("oracle: periodic or symmetric only");
+    if (p->h_adv_order != 5 && p->h_adv_order != 6) return fail("h_adv_order must be 5 or 6");
+    if (p->v_adv_order != 3 && p->v_adv_order != 4) return fail("v_adv_order must be 3 or 4");
+    if (p->hypsometric_opt != 1) return fail("oracle: hypsometric_opt 1 only");
+    if (p->damp_opt != 0 && p->damp_opt != 3) return fail("oracle: damp_opt 0 or 3 only");
+    auto* o = new orc_domain;
+    o->d = std::make_unique<Dom>(*g, *p, *lv);
+    if (o->d->n_sound < 2 || o->d->n_sound % 2) {
+      delete o;
+      return fail("time_step_sound must be even and >= 2");
+    }
+    *out = o;
+    return MI_OK;
+  } catch (const std::exception& e) {
+    return fail(e.what(), MI_ENOMEM);
+  }
+}
+
+int orc_domain_destroy(orc_domain* d) {
+  delete d;
+  return MI_OK;
+}
+int orc_domain_n_sound(const orc_domain* d) { return d->d->n_sound; }
+int orc_set_libmvec(orc_domain* d, int on) {
+  d->d->libmvec = on != 0;
+  return MI_OK;
+}
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return MI_OK;
+}
+
+int orc_set_field(orc_domain* od, int id, const double* h, int64_t n) {
+  Dom& d = *od->d;
+  F* f = d.field(id);
+  if (!f) return fail("unknown field id " + std::to_string(id));
+  const int64_t full = int64_t(f->nxs) * f->nys * f->nzs;
+  if (is_base(id)) {
+    // base fields: a profile (nzs values; 2-D: one) repeated over the allocation halos included, or
+    // the logical points with the halo filled by the field's boundary rule
+    if (n == f->nzs) {
+      for (int k = 0; k < f->nzs; ++k)
+        for (int j = -f->hy; j < f->nys + f->hy; ++j)
+          for (int i = -3; i < f->nxs + 3; ++i) (*f)(i, j, k) = h[k];
+      return MI_OK;
+    }
+    if (n != full) return fail("base field size mismatch");
+  } else if (n != full) {
+    return fail("field size mismatch");
+  }
+  for (int k = 0; k < f->nzs; ++k)
+    for (int j = 0; j < f->nys; ++j)
+      for (int i = 0; i < f->nxs; ++i) (*f)(i, j, k) = h[(size_t(k) * f->nys + j) * f->nxs + i];
+  fill(*f, d.g);
+  return MI_OK;
+}
+
+int orc_get_field(orc_domain* od, int id, double* h, int64_t n) {
+  Dom& d = *od->d;
+  F* f = d.field(id);
+  if (!f) return fail("unknown field id " + std::to_string(id));
+  if (n != int64_t(f->nxs) * f->nys * f->nzs) return fail("field size mismatch");
+  for (int k = 0; k < f->nzs; ++k)
+    for (int j = 0; j < f->nys; ++j)
+      for (int i = 0; i < f->nxs; ++i) h[(size_t(k) * f->nys + j) * f->nxs + i] = (*f)(i, j, k);
+  return MI_OK;
+}
+
+int orc_step(orc_domain* d, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) d->d->step();
+  return MI_OK;
+}
+int orc_step_begin(orc_domain* d) {
+  d->d->step_begin();
+  return MI_OK;
+}
+int orc_rk_prep(orc_domain* d, int) {
+  d->d->rk_prep();
+  return MI_OK;
+}
+int orc_rk_tendency(orc_domain* d, int) {
+  d->d->rk_tendency();
+  return MI_OK;
+}
+int orc_small_steps(orc_domain* d, int rk) {
+  if (rk < 1 || rk > 3) return fail("rk must be 1, 2 or 3");
+  d->d->small_steps(rk);
+  return MI_OK;
+}
+int orc_calc_p_rho_phi(orc_domain* d, int) {
+  d->d->calc_p_rho_phi();
+  return MI_OK;
+}
+
+}  // extern "C"
