@@ -1,0 +1,70 @@
+// Device-side building blocks of the dynamics kernels: field views, the domain geometry, the level
+// coefficients and the advection flux operators.
+//
+// Layout in HBM.  Every 3-D field of a domain has the same strides: a row of P doubles (P a multiple
+// of 32, i.e. 256 bytes, with point i = 0 at element 8 so a wavefront's 64 consecutive i start on a
+// 64-byte boundary), ny+1+2*hy rows per level and nz+1 levels.  U, V, W and mass fields differ only in
+// how many of those points they use (staggering as the reference, src/core/grid.hpp:3-7), so one
+// index i + j*sj + k*sk serves every field and kernels need a single set of strides.  An x-z slice
+// (ny == 1) has sj == 0: j +- 1 reads the same row, as in the reference.  2-D fields use one level of
+// the same plane.  Horizontally uniform base-state fields are 1-D profiles read through BField
+// (strides 0 in i and j).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/cppwrf_amd.h"
+#include <stdint.h>
+
+namespace mi_dev {
+
+constexpr double kG = 9.81;
+constexpr double kRd = 287.0;
+constexpr double kCp = 7.0 * kRd / 2.0;
+constexpr double kCpovcv = kCp / (kCp - kRd);
+constexpr double kP1000 = 100000.0;
+constexpr double kT0 = 300.0;
+constexpr double H = 0.5, Q = 0.25;
+constexpr double kReradius = 1. / 6370.0e03;
+constexpr int kHalo = 3;
+constexpr int kLpad = 8;
+
+struct Geo {
+  int nx, ny, nz;
+  int nyv;        // V points in y: ny + 1, or 1 for an x-z slice
+  int64_t sj, sk; // strides of j and k (sj == 0 for an x-z slice)
+  int hy;         // halo rows in y (0 for an x-z slice)
+  int bcx, bcy;   // MI_BC_*
+  bool two_d;
+  __host__ __device__ int64_t at(int i, int j, int k) const { return int64_t(i) + int64_t(j) * sj + int64_t(k) * sk; }
+  __host__ __device__ int64_t at2(int i, int j) const { return int64_t(i) + int64_t(j) * sj; }
+};
+
+// base-state field: full (si = 1, sj, sk of the domain) or a horizontally uniform profile (si = sj = 0)
+struct BField {
+  const double* p;
+  int64_t si, sj, sk;
+  __device__ double operator()(int i, int j, int k) const { return p[int64_t(i) * si + int64_t(j) * sj + int64_t(k) * sk]; }
+};
+
+// vertical coefficients (device copies of the level arrays)
+struct Lv {
+  const double *dnw, *rdnw, *dn, *rdn, *fnm, *fnp, *c1h, *c2h, *c3h, *c4h, *c1f, *c2f, *c3f, *c4f;
+  double cf1, cf2, cf3, cfn, cfn1, rdx, rdy, p_top;
+};
+
+__device__ __forceinline__ double sgn1(double a) { return a >= 0.0 ? 1.0 : -1.0; }
+
+// 5th-order upwind flux of the reference (src/dyn/kernels_common.hpp:13-17); up = 0 gives the 6th-order
+// centred flux
+__device__ __forceinline__ double flux5(double qm3, double qm2, double qm1, double q0, double qp1, double qp2, double ua,
+                                        double up) {
+  return (37.0 * (q0 + qm1) - 8.0 * (qp1 + qm2) + (qp2 + qm3)) / 60.0 -
+         up * sgn1(ua) * ((qp2 - qm3) - 5.0 * (qp1 - qm2) + 10.0 * (q0 - qm1)) / 60.0;
+}
+// 3rd-order upwind flux (kernels_common.hpp:19-22); up = 0 gives the 4th-order centred flux
+__device__ __forceinline__ double flux3(double qm2, double qm1, double q0, double qp1, double ua, double up) {
+  return (7.0 * (q0 + qm1) - (qp1 + qm2)) / 12.0 + up * sgn1(ua) * ((qp1 - qm2) - 3.0 * (q0 - qm1)) / 12.0;
+}
+
+}  // namespace mi_dev

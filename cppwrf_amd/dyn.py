@@ -1,0 +1,199 @@
+"""Python view of the library, named after the reference's src/dyn surface.
+
+    ctx = Context(device=0)
+    case = Case.bubble3d(64, 64, 40, dx=1000.0, ztop=20000.0, dt=6.0, seed=1)
+    rk = RK3(ctx, case.grid, case.params(time_step_sound=6, damp_opt=3), case)   # dyn::RK3 + State
+    rk.step(10)                                                                  # RK3::step x 10
+    u = rk.field(MI_U)
+
+The arrays are numpy float64 in the C ABI's dense layout: shape (nzs, nys, nxs), i fastest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi as A
+from ._abi import check, field_shape, lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class Grid:
+    """reference Grid (src/core/grid.hpp:21-31)"""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dt: float
+    bc_x: int = A.MI_BC_PERIODIC
+    bc_y: int = A.MI_BC_PERIODIC
+
+    def c(self) -> A.MiGrid:
+        return A.MiGrid(nx=self.nx, ny=self.ny, nz=self.nz, bc_x=self.bc_x, bc_y=self.bc_y, dx=self.dx, dy=self.dy,
+                        dt=self.dt)
+
+    @property
+    def points(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+class Case:
+    """A synthetic initial state built by the library's host generator (mi_case_*)."""
+
+    STATE = (A.MI_U, A.MI_V, A.MI_W, A.MI_T, A.MI_PH, A.MI_MU, A.MI_P, A.MI_AL)
+
+    def __init__(self, kind: int, grid: Grid, ztop: float, seed: int = 0, noise_amp: float = 0.0,
+                 bubble_dtheta: float = 2.0, bubble_radius: float = 4000.0, bubble_zc: float = 2000.0, u0: float = 0.0):
+        self.grid = grid
+        self.spec = A.MiCaseSpec(kind=kind, grid=grid.c(), ztop=ztop, seed=seed, noise_amp=noise_amp,
+                                 bubble_dtheta=bubble_dtheta, bubble_radius=bubble_radius, bubble_zc=bubble_zc, u0=u0)
+        h = C.c_void_p()
+        check(lib().mi_case_create(C.byref(self.spec), C.byref(h)))
+        self._h = h
+        self.levels = A.MiLevels()
+        check(lib().mi_case_levels(self._h, C.byref(self.levels)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mi_case_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def field(self, fid: int) -> np.ndarray:
+        a = np.zeros(field_shape(fid, self.grid.nx, self.grid.ny, self.grid.nz))
+        check(lib().mi_case_get_field(self._h, fid, _ptr(a), a.size))
+        return a
+
+    def profile(self, fid: int) -> np.ndarray:
+        """a base-state field's vertical profile (the generator's base state is horizontally uniform)"""
+        return np.ascontiguousarray(self.field(fid)[:, 0, 0])
+
+    def params(self, **kw) -> A.MiParams:
+        return A.default_params(**kw)
+
+    # the workloads of this library and the reference's idealized cases
+    @classmethod
+    def bubble3d(cls, nx, ny, nz, dx, ztop, dt, seed=1, noise_amp=0.05, dy=None, bc=A.MI_BC_PERIODIC, **kw):
+        g = Grid(nx, ny, nz, dx, dy if dy is not None else dx, dt, bc, bc)
+        return cls(A.MI_CASE_BUBBLE3D, g, ztop, seed=seed, noise_amp=noise_amp, **kw)
+
+    @classmethod
+    def igw(cls):
+        return cls(A.MI_CASE_IGW, Grid(300, 1, 10, 1000.0, 1000.0, 6.0), 10000.0)
+
+    @classmethod
+    def warm_bubble(cls):
+        return cls(A.MI_CASE_WARM_BUBBLE, Grid(200, 1, 100, 100.0, 100.0, 1.0, A.MI_BC_SYMMETRIC), 10000.0)
+
+    @classmethod
+    def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0):
+        return cls(A.MI_CASE_REST_STATE, Grid(nx, ny, nz, 2000.0, 2000.0, 10.0), 20000.0, u0=u0)
+
+
+class Context:
+    """A GPU and the stream its domains launch on (mi_ctx)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        h = C.c_void_p()
+        check(lib().mi_ctx_create(device, C.c_void_p(stream) if stream else None, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mi_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class RK3:
+    """reference dyn::RK3 together with the State / BaseState it steps (src/dyn/rk3.hpp:34-78)."""
+
+    def __init__(self, ctx: Context, grid: Grid, params: A.MiParams, case: Case | None = None,
+                 levels: A.MiLevels | None = None):
+        if levels is None:
+            if case is None:
+                raise ValueError("RK3 needs the level coefficients: pass a Case or levels")
+            levels = case.levels
+        self.ctx = ctx
+        self.grid = grid
+        self.params = params
+        h = C.c_void_p()
+        check(lib().mi_domain_create(ctx.handle, C.byref(grid.c()), C.byref(params), C.byref(levels), C.byref(h)))
+        self._h = h
+        if case is not None:
+            check(lib().mi_case_load(case.handle, self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mi_domain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def n_sound(self) -> int:
+        return lib().mi_domain_n_sound(self._h)
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().mi_domain_device_bytes(self._h)
+
+    def step(self, n: int = 1) -> None:
+        check(lib().mi_step(self._h, n))
+
+    def sync(self) -> None:
+        check(lib().mi_domain_sync(self._h))
+
+    def field(self, fid: int) -> np.ndarray:
+        a = np.zeros(field_shape(fid, self.grid.nx, self.grid.ny, self.grid.nz))
+        check(lib().mi_domain_get_field(self._h, fid, _ptr(a), a.size))
+        return a
+
+    def set_field(self, fid: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(lib().mi_domain_set_field(self._h, fid, _ptr(a), a.size))
+
+    def check_finite(self) -> int:
+        bad = C.c_int(-1)
+        rc = lib().mi_check_finite(self._h, C.byref(bad))
+        if rc not in (A.MI_OK, A.MI_ENONFINITE):
+            check(rc)
+        return bad.value
+
+    # the phases of a step, for tests
+    def step_begin(self):
+        check(lib().mi_step_begin(self._h))
+
+    def rk_prep(self, rk):
+        check(lib().mi_rk_prep(self._h, rk))
+
+    def rk_tendency(self, rk):
+        check(lib().mi_rk_tendency(self._h, rk))
+
+    def small_steps(self, rk):
+        check(lib().mi_small_steps(self._h, rk))
+
+    def calc_p_rho_phi(self, rk):
+        check(lib().mi_calc_p_rho_phi(self._h, rk))

@@ -1,0 +1,68 @@
+"""The C ABI: the library loads, exports every function include/cppwrf_amd.h declares, and its host-only
+entry points behave (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cppwrf_amd import _abi as A
+from cppwrf_amd.dyn import Case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "cppwrf_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mi_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_exports_every_declared_function():
+    L = A.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), f"libcppwrf_amd.so does not export {n}"
+    assert set(names) == set(A.SIGNATURES), "the ctypes table and the header differ"
+
+
+def test_abi_version():
+    assert A.lib().mi_abi_version() == 1
+
+
+@pytest.mark.parametrize("dt,dx,dy,want", [(6.0, 1000.0, 1000.0, 4), (1.0, 100.0, 100.0, 6), (10.0, 2000.0, 2000.0, 4),
+                                           (90.0, 15000.0, 15000.0, 4), (18.0, 3000.0, 3000.0, 4), (60.0, 3000.0, 3000.0, 12)])
+def test_auto_sound_steps(dt, dx, dy, want):
+    # reference RK3::auto_sound_steps (src/dyn/rk3.cpp:69-72): max(2*(int(300 dt/min(dx,dy) - 0.01)+1), 4)
+    assert A.lib().mi_auto_sound_steps(dt, dx, dy) == want
+
+
+def test_case_levels_are_consistent():
+    c = Case.bubble3d(16, 12, 20, dx=1000.0, ztop=10000.0, dt=6.0)
+    lv = c.levels
+    nz = 20
+    dnw = np.ctypeslib.as_array(lv.dnw, shape=(nz,))
+    rdnw = np.ctypeslib.as_array(lv.rdnw, shape=(nz,))
+    assert np.all(dnw < 0) and np.allclose(dnw * rdnw, 1.0, rtol=1e-15)
+    assert abs(dnw.sum() + 1.0) < 1e-14  # eta goes from 1 to 0
+    phb = c.profile(A.MI_PHB)
+    assert phb[0] == 0.0 and np.all(np.diff(phb) > 0)
+    assert abs(phb[-1] / 9.81 - 10000.0) < 5.0  # model top at ztop
+
+
+def test_case_noise_is_seeded():
+    a = Case.bubble3d(8, 8, 6, dx=1000.0, ztop=6000.0, dt=6.0, seed=5).field(A.MI_T)
+    b = Case.bubble3d(8, 8, 6, dx=1000.0, ztop=6000.0, dt=6.0, seed=5).field(A.MI_T)
+    c = Case.bubble3d(8, 8, 6, dx=1000.0, ztop=6000.0, dt=6.0, seed=6).field(A.MI_T)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+def test_errors_are_reported():
+    L = A.lib()
+    h = C.c_void_p()
+    bad = A.MiCaseSpec(kind=99, grid=A.MiGrid(nx=4, ny=4, nz=4, dx=1.0, dy=1.0, dt=1.0), ztop=1000.0)
+    assert L.mi_case_create(C.byref(bad), C.byref(h)) == A.MI_EINVAL
+    assert b"unknown kind" in L.mi_last_error()
+    assert L.mi_case_create(None, C.byref(h)) == A.MI_EINVAL
