@@ -127,6 +127,11 @@ SIGNATURES = {
     "mi_small_steps": (_I, [_VP, _I]),
     "mi_calc_p_rho_phi": (_I, [_VP, _I]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
+    "mi_domain_autotune": (_I, [_VP, _I]),
+    "mi_domain_get_tuning": (_I, [_VP, C.POINTER(_I), _I]),
+    "mi_domain_set_tuning": (_I, [_VP, C.POINTER(_I), _I]),
+    "mi_profile_kernel": (_I, [_VP, C.c_char_p]),
+    "mi_profile_read": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(_I), C.POINTER(C.c_double)]),
     "mi_case_create": (_I, [C.POINTER(MiCaseSpec), C.POINTER(_VP)]),
     "mi_case_destroy": (_I, [_VP]),
     "mi_case_levels": (_I, [_VP, C.POINTER(MiLevels)]),

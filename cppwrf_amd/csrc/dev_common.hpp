@@ -29,22 +29,27 @@ constexpr double kReradius = 1. / 6370.0e03;
 constexpr int kHalo = 3;
 constexpr int kLpad = 8;
 
+// Element offsets are 32-bit (every allocation holds fewer than 2^31 doubles; the domain checks it):
+// half the address registers of 64-bit index arithmetic, which matters for the occupancy of the
+// column kernels.
+using idx_t = int;
+
 struct Geo {
   int nx, ny, nz;
-  int nyv;        // V points in y: ny + 1, or 1 for an x-z slice
-  int64_t sj, sk; // strides of j and k (sj == 0 for an x-z slice)
-  int hy;         // halo rows in y (0 for an x-z slice)
-  int bcx, bcy;   // MI_BC_*
+  int nyv;       // V points in y: ny + 1, or 1 for an x-z slice
+  idx_t sj, sk;  // strides of j and k (sj == 0 for an x-z slice)
+  int hy;        // halo rows in y (0 for an x-z slice)
+  int bcx, bcy;  // MI_BC_*
   bool two_d;
-  __host__ __device__ int64_t at(int i, int j, int k) const { return int64_t(i) + int64_t(j) * sj + int64_t(k) * sk; }
-  __host__ __device__ int64_t at2(int i, int j) const { return int64_t(i) + int64_t(j) * sj; }
+  __host__ __device__ idx_t at(int i, int j, int k) const { return i + j * sj + k * sk; }
+  __host__ __device__ idx_t at2(int i, int j) const { return i + j * sj; }
 };
 
 // base-state field: full (si = 1, sj, sk of the domain) or a horizontally uniform profile (si = sj = 0)
 struct BField {
   const double* p;
-  int64_t si, sj, sk;
-  __device__ double operator()(int i, int j, int k) const { return p[int64_t(i) * si + int64_t(j) * sj + int64_t(k) * sk]; }
+  idx_t si, sj, sk;
+  __device__ double operator()(int i, int j, int k) const { return p[i * si + j * sj + k * sk]; }
 };
 
 // vertical coefficients (device copies of the level arrays)

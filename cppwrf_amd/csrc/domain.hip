@@ -65,8 +65,31 @@ struct mi_domain {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
+  std::string prof_name;
+  mi_fp64::KernelTimer timer{};
+  int colcfg[mi_fp64::kNumColKernels] = {0, 0, 0, 0};  // block shape of each column kernel
+  std::vector<double*> backup;                          // the carried state, for the autotuner
+
+  // launches of fn, with the kernel timer active when a kernel is selected
+  template <class F>
+  void timed(F&& fn) {
+    if (prof_name.empty()) {
+      fn();
+      return;
+    }
+    timer.name = prof_name.c_str();
+    mi_fp64::g_timer = &timer;
+    try {
+      fn();
+    } catch (...) {
+      mi_fp64::g_timer = nullptr;
+      throw;
+    }
+    mi_fp64::g_timer = nullptr;
+  }
 
   ~mi_domain() {
+    for (hipEvent_t e : timer.ev) (void)hipEventDestroy(e);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void* p : allocs) (void)hipFree(p);
@@ -124,29 +147,26 @@ struct mi_domain {
   }
 
   // ------------------------------------------------------------------ the phases of a step
-  void step_begin() {
-    mi_fp64::launch_copy_step(a, n3, n2, ctx->stream);
-    halo({{a.f.mu1, -1, -1}});
-  }
+  // RK3::step's _2 -> _1 copies (rk3.cpp:94-98).  The copies include the halos, which are fresh, so
+  // the reference's refresh of mu_1's halo rewrites the values already there and is left out.
+  void step_begin() { mi_fp64::launch_copy_step(a, n3, n2, ctx->stream); }
 
-  // LargeStep::prep (large_step.cpp:697-733)
+  // LargeStep::prep (large_step.cpp:697-733).  Its first halo refresh (u, v, w, t, ph, p, al, mu of the
+  // stage) is left out: every path that changes those fields (the end of a stage, mi_domain_set_field)
+  // refreshes their halos itself, so at this point it would rewrite identical values.
   void rk_prep() {
     auto& f = a.f;
-    halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.w2, -1, -1}, {f.t2, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1},
-          {f.mu2, -1, -1}});
-    mi_fp64::launch_rk_prep_mu(a, ctx->stream);
-    halo({{f.mut, -1, -1}});
-    mi_fp64::launch_rk_prep_couple(a, ctx->stream);
-    halo({{f.ru, -1, -1}, {f.rv, -1, -1}, {f.rw, -1, -1}, {f.alt, -1, -1}, {f.php, -1, -1}, {f.muu, 2, 0}, {f.muv, 0, 1}});
-    mi_fp64::launch_rk_prep_ww(a, ctx->stream);
-    halo({{f.ww, -1, -1}});
+    mi_fp64::launch_large_prep(a, colcfg[mi_fp64::kColLargePrep], ctx->stream);
+    halo({{f.mut, -1, -1}, {f.ru, -1, -1}, {f.rv, -1, -1}, {f.rw, -1, -1}, {f.alt, -1, -1}, {f.php, -1, -1},
+          {f.muu, 2, 0}, {f.muv, 0, 1}, {f.ww, -1, -1}});
   }
 
   // LargeStep::tendencies (large_step.cpp:798-861)
   void rk_tendency() { mi_fp64::launch_tendencies(a, ctx->stream); }
 
-  // SmallStep::run_stage (small_step.cpp:490-607)
-  void small_steps(int rk) {
+  // SmallStep::run_stage (small_step.cpp:490-607); with prhophi, calc_p_rho_phi (small_step.cpp:609-617)
+  // is fused into the finishing kernel
+  void small_steps(int rk, bool prhophi) {
     auto& f = a.f;
     const double dt = grid.dt, dts_full = dt / n_sound;
     const double dt_rk = rk == 1 ? dt / 3.0 : (rk == 2 ? 0.5 * dt : dt);
@@ -155,27 +175,30 @@ struct mi_domain {
     hipStream_t st = ctx->stream;
     if (rk == 1) mi_fp64::launch_fill(f.mudf - off, n2, 0.0, st);
     mi_fp64::launch_ss_prep(a, rk == 1, st);
-    halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.ph2, -1, -1}, {f.t_save, -1, -1}, {f.u_save, -1, -1}, {f.v_save, -1, -1},
-          {f.mu2, -1, -1}, {f.muts, -1, -1}});
-    mi_fp64::launch_p_rho(a, false, st);
-    halo({{f.p, -1, -1}, {f.al, -1, -1}});
     const double cof_h = 0.5 * dts * kG * (1.0 + prm.epssm);
-    mi_fp64::launch_coef_w(a, cof_h * cof_h, st);
+    mi_fp64::launch_ss_column(a, colcfg[mi_fp64::kColSsColumn], cof_h * cof_h, st);
+    halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.ph2, -1, -1}, {f.t_save, -1, -1}, {f.u_save, -1, -1}, {f.v_save, -1, -1},
+          {f.mu2, -1, -1}, {f.muts, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
     for (int it = 1; it <= ns; ++it) {
       mi_fp64::launch_uv(a, dts, st);
       if (grid.ny > 1) halo({{f.u2, -1, -1}, {f.v2, -1, -1}});
       else halo({{f.u2, -1, -1}});
-      mi_fp64::launch_mu_t(a, dts, it, ns, st);
-      halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}});
-      mi_fp64::launch_w(a, dts, st);
-      halo({{f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
+      mi_fp64::launch_acoustic(a, colcfg[mi_fp64::kColAcoustic], dts, it, ns, st);
+      if (it < ns)
+        halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
+      else  // the reference's refresh of muts, ru_m, rv_m, ww_m after the loop joins the last one
+        halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1},
+              {f.ru_m, -1, -1}, {f.rv_m, -1, -1}, {f.ww_m, -1, -1}});
     }
-    halo({{f.muts, -1, -1}, {f.ru_m, -1, -1}, {f.rv_m, -1, -1}, {f.ww_m, -1, -1}});
-    mi_fp64::launch_finish(a, st);
-    halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.w2, -1, -1}, {f.t2, -1, -1}, {f.ph2, -1, -1}, {f.ww, -1, -1}, {f.mu2, -1, -1}});
+    mi_fp64::launch_finish(a, colcfg[mi_fp64::kColFinish], prhophi, st);
+    if (prhophi)
+      halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.w2, -1, -1}, {f.t2, -1, -1}, {f.ph2, -1, -1}, {f.ww, -1, -1}, {f.mu2, -1, -1},
+            {f.p, -1, -1}, {f.al, -1, -1}});
+    else
+      halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.w2, -1, -1}, {f.t2, -1, -1}, {f.ph2, -1, -1}, {f.ww, -1, -1}, {f.mu2, -1, -1}});
   }
 
-  // SmallStep::calc_p_rho_phi (small_step.cpp:609-617)
+  // SmallStep::calc_p_rho_phi (small_step.cpp:609-617), for the phase-by-phase entry points
   void calc_p_rho_phi() {
     mi_fp64::launch_p_rho_phi(a, ctx->stream);
     halo({{a.f.p, -1, -1}, {a.f.al, -1, -1}});
@@ -186,15 +209,72 @@ struct mi_domain {
     for (int rk = 1; rk <= 3; ++rk) {
       rk_prep();
       rk_tendency();
-      small_steps(rk);
-      calc_p_rho_phi();
+      small_steps(rk, true);
+    }
+  }
+
+  // the fields a step reads from the previous one (everything else is recomputed within a step)
+  std::vector<double*> carried() { return {a.f.u2, a.f.v2, a.f.w2, a.f.t2, a.f.ph2, a.f.mu2, a.f.p, a.f.al}; }
+  void save_or_restore(bool save) {
+    std::vector<double*> c = carried();
+    if (backup.empty())
+      for (double* p : c) backup.push_back(alloc(p == a.f.mu2 ? n2 : n3));
+    for (size_t q = 0; q < c.size(); ++q) {
+      const int64_t n = c[q] == a.f.mu2 ? n2 : n3;
+      double* live = c[q] - off;
+      hip_check(hipMemcpyAsync(save ? backup[q] : live, save ? live : backup[q], size_t(n) * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
+    }
+  }
+
+  // Time every block shape of each column kernel on this domain and keep the fastest; the state is
+  // restored afterwards, so the forecast continues exactly as if no step had been taken.
+  void autotune(int trials) {
+    if (trials < 1) trials = 1;
+    const std::string keep = prof_name;
+    save_or_restore(true);
+    for (int s = 0; s < 1; ++s) timed([&] { step_launches(); });  // warm-up
+    for (int kern = 0; kern < mi_fp64::kNumColKernels; ++kern) {
+      double best_t = 1e300;
+      int best = colcfg[kern];
+      for (int cfg = 0; cfg < mi_fp64::kNumColCfgs; ++cfg) {
+        if (!mi_fp64::col_cfg_fits(kern, cfg, grid.nz)) continue;
+        colcfg[kern] = cfg;
+        prof_name = mi_fp64::kColKernelNames[kern];
+        timer.used = 0;
+        timer.bytes = 0;
+        for (int s = 0; s < trials; ++s) timed([&] { step_launches(); });
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        double tot = 0;
+        for (size_t q = 0; q + 1 < timer.used; q += 2) {
+          float ms = 0;
+          hip_check(hipEventElapsedTime(&ms, timer.ev[q], timer.ev[q + 1]), "hipEventElapsedTime");
+          tot += ms;
+        }
+        if (tot < best_t) {
+          best_t = tot;
+          best = cfg;
+        }
+      }
+      colcfg[kern] = best;
+    }
+    prof_name = keep;
+    timer.used = 0;
+    timer.bytes = 0;
+    save_or_restore(false);
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (exec) {  // the graph holds the old shapes
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
     }
   }
 
   void step(int nsteps) {
     if (nsteps <= 0) return;
-    if (!use_graph) {
-      for (int s = 0; s < nsteps; ++s) step_launches();
+    if (!use_graph || !prof_name.empty()) {
+      for (int s = 0; s < nsteps; ++s) timed([&] { step_launches(); });
       hip_check(hipGetLastError(), "kernel launch");
       return;
     }
@@ -230,6 +310,7 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   d.prm = p;
   d.n_sound = p.time_step_sound > 0 ? p.time_step_sound : mi_auto_sound_steps(g.dt, g.dx, g.dy);
   if (d.n_sound < 2 || d.n_sound % 2) throw Error(MI_EINVAL, "time_step_sound must be even and >= 2");
+  if (!mi_fp64::col_cfg_fits(mi_fp64::kColAcoustic, mi_fp64::kNumColCfgs - 1, g.nz)) throw Error(MI_EINVAL, "nz must be at most 255");
   if (const char* e = std::getenv("MI_GRAPHS")) d.use_graph = std::strcmp(e, "0") != 0;
 
   const bool two_d = g.ny == 1;
@@ -277,9 +358,8 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   add3(f.t_tend, St::M, MI_T_TEND); add3(f.ph_tend, St::W, MI_PH_TEND); add2(f.mu_tend, St::M2, -1);
   add3(f.u_save, St::U, -1); add3(f.v_save, St::V, -1); add3(f.w_save, St::W, -1); add3(f.t_save, St::M, -1);
   add3(f.ph_save, St::W, -1); add3(f.ww1, St::W, -1); add3(f.c2a, St::M, -1); add3(f.pm1, St::M, -1);
-  add3(f.t_2save, St::M, -1); add3(f.ca, St::W, -1); add3(f.calpha, St::W, -1); add3(f.cgamma, St::W, -1);
+  add3(f.ca, St::W, -1); add3(f.calpha, St::W, -1); add3(f.cgamma, St::W, -1);
   add2(f.mu_save, St::M2, -1); add2(f.mudf, St::M2, -1); add2(f.muave, St::M2, -1);
-  add3(f.w_rhs, St::W, -1); add3(f.t2a, St::M, -1); add3(f.dvdxi, St::M, -1);
   add2(f.mub, St::M2, MI_MUB);
   add2(f.msftx, St::M2, MI_MSFTX); add2(f.msfty, St::M2, MI_MSFTY);
   add2(f.msfux, St::U2, MI_MSFUX); add2(f.msfuy, St::U2, MI_MSFUY);
@@ -317,6 +397,8 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   for (int q = 0; q < 14; ++q) *dst[q] = lvb + size_t(q) * (nz + 1);
   c.cf1 = lv.cf1; c.cf2 = lv.cf2; c.cf3 = lv.cf3; c.cfn = lv.cfn; c.cfn1 = lv.cfn1;
   c.rdx = lv.rdx; c.rdy = lv.rdy; c.p_top = lv.p_top;
+  for (int kern = 0; kern < mi_fp64::kNumColKernels; ++kern) d.colcfg[kern] = mi_fp64::col_cfg_default(kern, g.nx, g.ny, g.nz);
+  if (d.n3 >= (int64_t(1) << 31)) throw Error(MI_EINVAL, "grid too large: a field must hold fewer than 2^31 points");
   // run parameters
   mi_fp64::Params& q = d.a.q;
   q.hup = p.h_adv_order == 5 ? 1.0 : 0.0;
@@ -502,7 +584,7 @@ int mi_rk_tendency(mi_domain* d, int rk) {
 int mi_small_steps(mi_domain* d, int rk) {
   return mi_detail::guard([&] {
     if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
-    d->small_steps(rk);
+    d->small_steps(rk, false);
     hip_check(hipGetLastError(), "kernel launch");
   });
 }
@@ -511,6 +593,63 @@ int mi_calc_p_rho_phi(mi_domain* d, int rk) {
     if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
     d->calc_p_rho_phi();
     hip_check(hipGetLastError(), "kernel launch");
+  });
+}
+
+int mi_domain_autotune(mi_domain* d, int trials) {
+  return mi_detail::guard([&] {
+    if (!d) throw Error(MI_EINVAL, "null argument");
+    hip_check(hipSetDevice(d->ctx->device), "hipSetDevice");
+    d->autotune(trials);
+  });
+}
+
+int mi_domain_set_tuning(mi_domain* d, const int* shapes, int n) {
+  return mi_detail::guard([&] {
+    if (!d || !shapes) throw Error(MI_EINVAL, "null argument");
+    for (int q = 0; q < n && q < mi_fp64::kNumColKernels; ++q)
+      if (!mi_fp64::col_cfg_fits(q, shapes[q], d->grid.nz)) throw Error(MI_EINVAL, "block shape does not fit this nz");
+    for (int q = 0; q < n && q < mi_fp64::kNumColKernels; ++q) d->colcfg[q] = shapes[q];
+    if (d->exec) {
+      (void)hipGraphExecDestroy(d->exec);
+      (void)hipGraphDestroy(d->graph);
+      d->exec = nullptr;
+      d->graph = nullptr;
+    }
+  });
+}
+
+int mi_domain_get_tuning(const mi_domain* d, int* shapes, int n) {
+  return mi_detail::guard([&] {
+    if (!d || !shapes) throw Error(MI_EINVAL, "null argument");
+    for (int q = 0; q < n && q < mi_fp64::kNumColKernels; ++q) shapes[q] = d->colcfg[q];
+  });
+}
+
+int mi_profile_kernel(mi_domain* d, const char* name) {
+  return mi_detail::guard([&] {
+    if (!d) throw Error(MI_EINVAL, "null argument");
+    d->prof_name = name ? name : "";
+    d->timer.used = 0;
+    d->timer.bytes = 0;
+  });
+}
+
+int mi_profile_read(mi_domain* d, double* total_ms, int* launches, double* alg_bytes) {
+  return mi_detail::guard([&] {
+    if (!d) throw Error(MI_EINVAL, "null argument");
+    hip_check(hipStreamSynchronize(d->ctx->stream), "hipStreamSynchronize");
+    double tot = 0;
+    for (size_t q = 0; q + 1 < d->timer.used; q += 2) {
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, d->timer.ev[q], d->timer.ev[q + 1]), "hipEventElapsedTime");
+      tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = int(d->timer.used / 2);
+    if (alg_bytes) *alg_bytes = d->timer.bytes;
+    d->timer.used = 0;
+    d->timer.bytes = 0;
   });
 }
 

@@ -9,9 +9,14 @@
 // 512-byte coalesced row segment per load); column kernels (vertical sums, the tridiagonal solve) run
 // one thread per column and march in k, every load still i-contiguous across the wavefront.
 #include "kernels_fp64.hpp"
+
+#include <cstring>
+
 #include "libm_port.hpp"
 
 namespace mi_fp64 {
+
+thread_local KernelTimer* g_timer = nullptr;
 
 using namespace mi_dev;
 
@@ -38,34 +43,86 @@ inline dim3 grid_of(const Range& r, int by) {
   const int j = int(blockIdx.y);                   \
   if (i >= (nx) || j >= (ny)) return;
 
+// HIP-event timing of the launches of one kernel (KernelTimer): events bracket each launch of the
+// selected kernel on the stream it runs on, with the algorithmic bytes of that launch
+bool timing(const char* name) { return g_timer && std::strcmp(name, g_timer->name) == 0; }
+void time_begin(hipStream_t st) {
+  KernelTimer& t = *g_timer;
+  if (t.used + 2 > t.ev.size()) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    t.ev.push_back(a);
+    t.ev.push_back(b);
+  }
+  (void)hipEventRecord(t.ev[t.used], st);
+}
+void time_end(hipStream_t st, double bytes) {
+  KernelTimer& t = *g_timer;
+  (void)hipEventRecord(t.ev[t.used + 1], st);
+  t.used += 2;
+  t.bytes += bytes;
+}
+
+// algorithmic bytes of one point of a kernel: distinct 3-D arrays it reads and writes, 8 bytes each
+// (pointwise-recomputable inputs and 1-D / 2-D arrays are not counted)
+double point_bytes(const char* name) {
+  struct E { const char* n; int arrays; };
+  static const E table[] = {
+      {"large_prep", 11},  // u v w al ph -> ru rv rw alt php ww
+      {"tend_u", 11},      // u ru rv ww p ph alt al php rw -> ru_tend
+      {"tend_v", 11},      // v ru rv ww p ph alt al php rw -> rv_tend
+      {"tend_w", 8},       // w ru rv ww p u v -> rw_tend
+      {"tend_t", 5},       // t ru rv ww -> t_tend
+      {"tend_ph", 6},      // ph ww w u v -> ph_tend
+      {"ss_prep", 25},     // u v w t ph (_1 and _2) p alt ww -> u v w t ph, their 5 saves, c2a ww1
+      {"ss_column", 11},   // ph alt t t_save c2a -> al p pm1 a alpha gamma
+      {"uv", 11},          // u v ru_tend rv_tend p al alt ph php -> u v
+      // u v u_save v_save t_save t_tend t ww1 ww_m ru_m rv_m ph_tend w ph_save ph rw_tend c2a alt a alpha gamma
+      // w_save pm1 -> ww ww_m ru_m rv_m t w ph al p pm1
+      {"acoustic", 34},
+      {"finish", 20},      // u v w t ph ww and the six saves -> u v w t ph ww, al p
+      {"p_rho_phi", 4},    // ph t -> al p
+  };
+  for (const E& e : table)
+    if (std::strcmp(e.n, name) == 0) return 8.0 * e.arrays;
+  return 0.0;
+}
+
 template <class K, class... T>
-void launch3(K kern, const Range& r, hipStream_t st, T... args) {
+void launch3(const char* name, K kern, const Range& r, hipStream_t st, T... args) {
   if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) return;
   const int by = (r.j1 - r.j0) == 1 ? 1 : 4;
+  const bool t = timing(name);
+  if (t) time_begin(st);
   hipLaunchKernelGGL(kern, grid_of(r, by), dim3(kBx, by, 1), 0, st, r, args...);
+  if (t) time_end(st, point_bytes(name) * double(r.i1 - r.i0) * double(r.j1 - r.j0) * double(r.k1 - r.k0));
 }
 template <class K, class... T>
-void launch_cols(K kern, int nx, int ny, hipStream_t st, T... args) {
+void launch_cols(const char* name, K kern, int nx, int ny, int nz, hipStream_t st, T... args) {
+  const bool t = timing(name);
+  if (t) time_begin(st);
   hipLaunchKernelGGL(kern, dim3(unsigned((nx + kBx - 1) / kBx), unsigned(ny), 1), dim3(kBx, 1, 1), 0, st, args...);
+  if (t) time_end(st, point_bytes(name) * double(nx) * double(ny) * double(nz));
 }
 
 // ----------------------------------------------------------------------------- halo fill
 // One halo slot of one direction (reference semantics src/core/halo.cpp:52-84): slot a = 0..7 maps to
 // i = 0, -1, -2, -3, n, n+1, n+2, n+3 on face-staggered points and -1, -2, -3, n, n+1, n+2 otherwise.
-__device__ __forceinline__ void halo_slot(double* base, int64_t sn, int a, int n, bool face, bool periodic, int par) {
+__device__ __forceinline__ void halo_slot(double* base, idx_t sn, int a, int n, bool face, bool periodic, int par) {
   const int nl = face ? 4 : 3;
   const int i = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
   int src;
   if (periodic) src = ((i % n) + n) % n;
   else if (face) src = i <= 0 ? -i : 2 * n - i;
   else src = i < 0 ? -1 - i : 2 * n - 1 - i;
-  double v = base[int64_t(src) * sn];
+  double v = base[idx_t(src) * sn];
   if (!periodic && src == i) {
-    if (par == 2) base[int64_t(i) * sn] = -v;
+    if (par == 2) base[idx_t(i) * sn] = -v;
     return;
   }
   if (!periodic && par != 0) v = -v;
-  base[int64_t(i) * sn] = v;
+  base[idx_t(i) * sn] = v;
 }
 
 // x pass: thread (slot, row j, level k, field); rows of the field's interior
@@ -125,70 +182,95 @@ __global__ void k_fill(double* p, int64_t n, double v) {
   if (e < n) p[e] = v;
 }
 
+// ----------------------------------------------------------------------------- column-group kernels
+// A block holds CI consecutive columns (i) of one row j and kKT threads per column along k: the
+// pointwise parts run in parallel over (column, level), the recurrences of a column (sums in k order,
+// the tridiagonal sweeps) run on one thread per column out of LDS.  LDS arrays are interleaved by
+// column, a[k * CI + ci], so the CI threads of one level touch consecutive banks.
+// A block is CI columns x KT threads per column; a thread takes levels kt, kt + KT, ...  The shape is
+// chosen per kernel and grid from kColCfgs (the domain's default, or its autotuner: every shape gives
+// the same bits, the sums and sweeps of a column run on one thread in k order whatever the shape).
+
+struct ColPos {
+  int ci, kt, i, j;
+};
+template <int CI>
+__device__ __forceinline__ ColPos col_pos() {
+  ColPos c;
+  c.ci = int(threadIdx.x) % CI;
+  c.kt = int(threadIdx.x) / CI;
+  c.i = int(blockIdx.x) * CI + c.ci;
+  c.j = int(blockIdx.y);
+  return c;
+}
+#define MI_COLUMN_KERNEL template <int CI, int KT> __global__ void __launch_bounds__(CI * KT)
+#define LDS(arr, k, ci) (arr)[(k) * CI + (ci)]
+
 // ----------------------------------------------------------------------------- rk_step_prep
-// KMu: full column masses at the stage (reference src/dyn/large_step.cpp:49-66)
-__global__ void k_mu_uv(Range r, Args a) {
-  MI_POINT_INDEX(r)
-  (void)k;
-  const Geo& g = a.g;
-  const Fields& f = a.f;
-  const int nx = g.nx, ny = g.ny;
-  auto mu = [&](int ii, int jj) { return f.mu2[g.at2(ii, jj)]; };
-  auto mub = [&](int ii, int jj) { return f.mub[g.at2(ii, jj)]; };
-  if (i < nx && j < ny) f.mut[g.at2(i, j)] = mu(i, j) + mub(i, j);
-  if (j < ny) f.muu[g.at2(i, j)] = H * (mu(i, j) + mu(i - 1, j) + mub(i, j) + mub(i - 1, j));
-  if (i < nx && j < g.nyv) f.muv[g.at2(i, j)] = H * (mu(i, j) + mu(i, j - 1) + mub(i, j) + mub(i, j - 1));
-}
-
-// KPrep3: couple_momentum, calc_alt, calc_php (large_step.cpp:68-81)
-__global__ void k_couple(Range r, Args a) {
-  MI_POINT_INDEX(r)
+// One kernel over the (nx+1) x nyv columns of the U/V/W points (reference LargeStep::prep,
+// large_step.cpp:697-733, without its halo exchanges, none of which these kernels read):
+//   KMu   the full column masses (large_step.cpp:49-66), written by the k = 0 threads and recomputed
+//         by the others with the same expression (identical values)
+//   KPrep3 couple_momentum, calc_alt, calc_php (large_step.cpp:68-81)
+//   KWW   calc_ww_cp (large_step.cpp:83-108): divergence in parallel, the sums in k order from LDS
+MI_COLUMN_KERNEL k_large_prep(Args a) {
+  constexpr int kKT = KT;
+  extern __shared__ double lds[];
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  if (j < ny && k < nz) f.ru[x] = f.u2[x] * (c.c1h[k] * f.muu[x2] + c.c2h[k]) / f.msfuy[x2];
-  if (i < nx && j < g.nyv && k < nz) f.rv[x] = f.v2[x] * (c.c1h[k] * f.muv[x2] + c.c2h[k]) * f.msfvx_inv[x2];
-  if (i < nx && j < ny) f.rw[x] = f.w2[x] * (c.c1f[k] * f.mut[x2] + c.c2f[k]) / f.msfty[x2];
-  if (i < nx && j < ny && k < nz) {
-    f.alt[x] = f.al[x] + f.alb(i, j, k);
-    f.php[x] = H * (f.phb(i, j, k) + f.phb(i, j, k + 1) + f.ph2[x] + f.ph2[x + g.sk]);
+  const ColPos cp = col_pos<CI>();
+  const int i = cp.i, j = cp.j, nx = g.nx, ny = g.ny, nz = g.nz;
+  const bool on = i <= nx;
+  const bool mass = i < nx && j < ny;
+  double* dv = lds;  // nz values per column
+  if (on) {
+    const idx_t x2 = g.at2(i, j);
+    auto mu = [&](int ii, int jj) { return f.mu2[g.at2(ii, jj)]; };
+    auto mub = [&](int ii, int jj) { return f.mub[g.at2(ii, jj)]; };
+    const double mut = mass ? mu(i, j) + mub(i, j) : 0.0;
+    const double muu = j < ny ? H * (mu(i, j) + mu(i - 1, j) + mub(i, j) + mub(i - 1, j)) : 0.0;
+    const double muv = (i < nx && j < g.nyv) ? H * (mu(i, j) + mu(i, j - 1) + mub(i, j) + mub(i, j - 1)) : 0.0;
+    if (cp.kt == 0) {
+      if (mass) f.mut[x2] = mut;
+      if (j < ny) f.muu[x2] = muu;
+      if (i < nx && j < g.nyv) f.muv[x2] = muv;
+    }
+    // calc_ww_cp's own column masses (summed in another order than calc_mu_uv)
+    const double mu1 = mass ? H * (mu(i + 1, j) + mub(i + 1, j) + mu(i, j) + mub(i, j)) : 0.0;
+    const double mu0 = mass ? H * (mu(i, j) + mub(i, j) + mu(i - 1, j) + mub(i - 1, j)) : 0.0;
+    const double mv1 = mass ? H * (mu(i, j + 1) + mub(i, j + 1) + mu(i, j) + mub(i, j)) : 0.0;
+    const double mv0 = mass ? H * (mu(i, j) + mub(i, j) + mu(i, j - 1) + mub(i, j - 1)) : 0.0;
+    for (int k = cp.kt; k <= nz; k += kKT) {
+      const idx_t x = g.at(i, j, k);
+      if (j < ny && k < nz) f.ru[x] = f.u2[x] * (c.c1h[k] * muu + c.c2h[k]) / f.msfuy[x2];
+      if (i < nx && j < g.nyv && k < nz) f.rv[x] = f.v2[x] * (c.c1h[k] * muv + c.c2h[k]) * f.msfvx_inv[x2];
+      if (mass) {
+        f.rw[x] = f.w2[x] * (c.c1f[k] * mut + c.c2f[k]) / f.msfty[x2];
+        if (k < nz) {
+          f.alt[x] = f.al[x] + f.alb(i, j, k);
+          f.php[x] = H * (f.phb(i, j, k) + f.phb(i, j, k + 1) + f.ph2[x] + f.ph2[x + g.sk]);
+          LDS(dv, k, cp.ci) =
+              f.msftx[x2] * c.dnw[k] *
+              (c.rdx * ((c.c1h[k] * mu1 + c.c2h[k]) * f.u2[x + 1] / f.msfuy[x2 + 1] - (c.c1h[k] * mu0 + c.c2h[k]) * f.u2[x] / f.msfuy[x2]) +
+               c.rdy * ((c.c1h[k] * mv1 + c.c2h[k]) * f.v2[x + g.sj] * f.msfvx_inv[x2 + g.sj] -
+                        (c.c1h[k] * mv0 + c.c2h[k]) * f.v2[x] * f.msfvx_inv[x2]));
+        }
+      }
+    }
   }
-}
-
-// KWW calc_ww_cp (large_step.cpp:83-108): column sum of the mass divergence in k order, then the
-// running eta-velocity integral
-__global__ void k_ww(Args a) {
-  const Geo& g = a.g;
-  MI_COLUMN_INDEX(g.nx, g.ny)
-  const Fields& f = a.f;
-  const Lv& c = a.c;
-  const int nz = g.nz;
-  auto mu_u = [&](int ii, int jj) {
-    return H * (f.mu2[g.at2(ii, jj)] + f.mub[g.at2(ii, jj)] + f.mu2[g.at2(ii - 1, jj)] + f.mub[g.at2(ii - 1, jj)]);
-  };
-  auto mu_v = [&](int ii, int jj) {
-    return H * (f.mu2[g.at2(ii, jj)] + f.mub[g.at2(ii, jj)] + f.mu2[g.at2(ii, jj - 1)] + f.mub[g.at2(ii, jj - 1)]);
-  };
-  const double mu1 = mu_u(i + 1, j), mu0 = mu_u(i, j), mv1 = mu_v(i, j + 1), mv0 = mu_v(i, j);
-  const double msftx = f.msftx[g.at2(i, j)], msfuy1 = f.msfuy[g.at2(i + 1, j)], msfuy0 = f.msfuy[g.at2(i, j)];
-  const double mvi1 = f.msfvx_inv[g.at2(i, j + 1)], mvi0 = f.msfvx_inv[g.at2(i, j)];
-  auto divv = [&](int kk) {
-    const int64_t x = g.at(i, j, kk);
-    return msftx * c.dnw[kk] *
-           (c.rdx * ((c.c1h[kk] * mu1 + c.c2h[kk]) * f.u2[x + 1] / msfuy1 - (c.c1h[kk] * mu0 + c.c2h[kk]) * f.u2[x] / msfuy0) +
-            c.rdy * ((c.c1h[kk] * mv1 + c.c2h[kk]) * f.v2[x + g.sj] * mvi1 - (c.c1h[kk] * mv0 + c.c2h[kk]) * f.v2[x] * mvi0));
-  };
-  double dmdt = 0;
-  for (int kk = 0; kk < nz; ++kk) dmdt += divv(kk);
-  double w = 0;
-  f.ww[g.at(i, j, 0)] = 0;
-  for (int kk = 1; kk < nz; ++kk) {
-    w = w - c.dnw[kk - 1] * c.c1h[kk - 1] * dmdt - divv(kk - 1);
-    f.ww[g.at(i, j, kk)] = w;
+  __syncthreads();
+  if (on && mass && cp.kt == 0) {
+    double dmdt = 0;
+    for (int k = 0; k < nz; ++k) dmdt += LDS(dv, k, cp.ci);
+    double w = 0;
+    f.ww[g.at(i, j, 0)] = 0;
+    for (int k = 1; k < nz; ++k) {
+      w = w - c.dnw[k - 1] * c.c1h[k - 1] * dmdt - LDS(dv, k - 1, cp.ci);
+      f.ww[g.at(i, j, k)] = w;
+    }
+    f.ww[g.at(i, j, nz)] = 0;
   }
-  f.ww[g.at(i, j, nz)] = 0;
 }
 
 // ----------------------------------------------------------------------------- rk_tendency
@@ -199,26 +281,26 @@ __global__ void k_tend_u(Range r, Args a) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const int nz = g.nz;
-  const int64_t sj = g.sj, sk = g.sk;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const idx_t sj = g.sj, sk = g.sk;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double* u = f.u2;
   const double hup = a.q.hup, vup = a.q.vup;
-  auto fx = [&](int64_t y) {  // face between y-1 and y
+  auto fx = [&](idx_t y) {  // face between y-1 and y
     const double vel = H * (f.ru[y] + f.ru[y - 1]);
     return vel * flux5(u[y - 3], u[y - 2], u[y - 1], u[y], u[y + 1], u[y + 2], vel, hup);
   };
-  auto fy = [&](int64_t y) {
+  auto fy = [&](idx_t y) {
     const double vel = H * (f.rv[y] + f.rv[y - 1]);
     return vel * flux5(u[y - 3 * sj], u[y - 2 * sj], u[y - sj], u[y], u[y + sj], u[y + 2 * sj], vel, hup);
   };
   auto fz = [&](int kk) {
     if (kk == 0 || kk == nz) return 0.0;
-    const int64_t y = g.at(i, j, kk);
+    const idx_t y = g.at(i, j, kk);
     const double vel = H * (f.ww[y - 1] + f.ww[y]);
     if (kk == 1 || kk == nz - 1) return vel * (c.fnm[kk] * u[y] + c.fnp[kk] * u[y - sk]);
     return vel * flux3(u[y - 2 * sk], u[y - sk], u[y], u[y + sk], -vel, vup);
   };
-  auto pu = [&](int kk) { const int64_t y = g.at(i, j, kk); return f.p[y - 1] + f.p[y]; };
+  auto pu = [&](int kk) { const idx_t y = g.at(i, j, kk); return f.p[y - 1] + f.p[y]; };
   auto dpn = [&](int kk) {
     if (kk == 0) return H * (c.cf1 * pu(0) + c.cf2 * pu(1) + c.cf3 * pu(2));
     if (kk == nz) return a.q.top_lid ? H * (c.cfn * pu(nz - 1) + c.cfn1 * pu(nz - 2)) : 0.0;
@@ -247,26 +329,26 @@ __global__ void k_tend_v(Range r, Args a) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const int nz = g.nz;
-  const int64_t sj = g.sj, sk = g.sk;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const idx_t sj = g.sj, sk = g.sk;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double* v = f.v2;
   const double hup = a.q.hup, vup = a.q.vup;
-  auto fx = [&](int64_t y) {
+  auto fx = [&](idx_t y) {
     const double vel = H * (f.ru[y] + f.ru[y - sj]);
     return vel * flux5(v[y - 3], v[y - 2], v[y - 1], v[y], v[y + 1], v[y + 2], vel, hup);
   };
-  auto fy = [&](int64_t y) {
+  auto fy = [&](idx_t y) {
     const double vel = H * (f.rv[y] + f.rv[y - sj]);
     return vel * flux5(v[y - 3 * sj], v[y - 2 * sj], v[y - sj], v[y], v[y + sj], v[y + 2 * sj], vel, hup);
   };
   auto fz = [&](int kk) {
     if (kk == 0 || kk == nz) return 0.0;
-    const int64_t y = g.at(i, j, kk);
+    const idx_t y = g.at(i, j, kk);
     const double vel = H * (f.ww[y - sj] + f.ww[y]);
     if (kk == 1 || kk == nz - 1) return vel * (c.fnm[kk] * v[y] + c.fnp[kk] * v[y - sk]);
     return vel * flux3(v[y - 2 * sk], v[y - sk], v[y], v[y + sk], -vel, vup);
   };
-  auto pv = [&](int kk) { const int64_t y = g.at(i, j, kk); return f.p[y - sj] + f.p[y]; };
+  auto pv = [&](int kk) { const idx_t y = g.at(i, j, kk); return f.p[y - sj] + f.p[y]; };
   auto dpn = [&](int kk) {
     if (kk == 0) return H * (c.cf1 * pv(0) + c.cf2 * pv(1) + c.cf3 * pv(2));
     if (kk == nz) return a.q.top_lid ? H * (c.cfn * pv(nz - 1) + c.cfn1 * pv(nz - 2)) : 0.0;
@@ -295,28 +377,28 @@ __global__ void k_tend_w(Range r, Args a) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const int nz = g.nz;
-  const int64_t sj = g.sj, sk = g.sk;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const idx_t sj = g.sj, sk = g.sk;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double* w = f.w2;
   const double hup = a.q.hup, vup = a.q.vup;
-  auto rux = [&](int64_t y) {  // ru at w level k
+  auto rux = [&](idx_t y) {  // ru at w level k
     if (k < nz) return c.fnm[k] * f.ru[y] + c.fnp[k] * f.ru[y - sk];
     return (2.0 - c.fnm[nz - 1]) * f.ru[y - sk] - c.fnp[nz - 1] * f.ru[y - 2 * sk];
   };
-  auto rvy = [&](int64_t y) {
+  auto rvy = [&](idx_t y) {
     if (k < nz) return c.fnm[k] * f.rv[y] + c.fnp[k] * f.rv[y - sk];
     return (2.0 - c.fnm[nz - 1]) * f.rv[y - sk] - c.fnp[nz - 1] * f.rv[y - 2 * sk];
   };
-  auto fx = [&](int64_t y) {
+  auto fx = [&](idx_t y) {
     const double vel = rux(y);
     return vel * flux5(w[y - 3], w[y - 2], w[y - 1], w[y], w[y + 1], w[y + 2], vel, hup);
   };
-  auto fy = [&](int64_t y) {
+  auto fy = [&](idx_t y) {
     const double vel = rvy(y);
     return vel * flux5(w[y - 3 * sj], w[y - 2 * sj], w[y - sj], w[y], w[y + sj], w[y + 2 * sj], vel, hup);
   };
   auto fz = [&](int m) {  // vertical flux at mass level m
-    const int64_t y = g.at(i, j, m);
+    const idx_t y = g.at(i, j, m);
     if (m == 0 || m == nz - 1) return 0.25 * (f.ww[y + sk] + f.ww[y]) * (w[y + sk] + w[y]);
     const double vel = H * (f.ww[y + sk] + f.ww[y]);
     return vel * flux3(w[y - sk], w[y], w[y + sk], w[y + 2 * sk], -vel, vup);
@@ -357,21 +439,21 @@ __global__ void k_tend_t(Range r, Args a) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const int nz = g.nz;
-  const int64_t sj = g.sj, sk = g.sk;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const idx_t sj = g.sj, sk = g.sk;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double* t2 = f.t2;
   const double hup = a.q.hup, vup = a.q.vup;
-  auto fx = [&](int64_t y) {
+  auto fx = [&](idx_t y) {
     const double vel = f.ru[y];
     return vel * flux5(t2[y - 3], t2[y - 2], t2[y - 1], t2[y], t2[y + 1], t2[y + 2], vel, hup);
   };
-  auto fy = [&](int64_t y) {
+  auto fy = [&](idx_t y) {
     const double vel = f.rv[y];
     return vel * flux5(t2[y - 3 * sj], t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], t2[y + 2 * sj], vel, hup);
   };
   auto fz = [&](int kk) {
     if (kk == 0 || kk == nz) return 0.0;
-    const int64_t y = g.at(i, j, kk);
+    const idx_t y = g.at(i, j, kk);
     const double vel = f.ww[y];
     if (kk == 1 || kk == nz - 1) return vel * (c.fnm[kk] * t2[y] + c.fnp[kk] * t2[y - sk]);
     return vel * flux3(t2[y - 2 * sk], t2[y - sk], t2[y], t2[y + sk], -vel, vup);
@@ -390,13 +472,13 @@ __global__ void k_tend_ph(Range r, Args a) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const int nz = g.nz;
-  const int64_t sj = g.sj, sk = g.sk;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const idx_t sj = g.sj, sk = g.sk;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double* ph = f.ph2;
   const double* u = f.u2;
   const double* v = f.v2;
   auto wdwn = [&](int m) {
-    const int64_t y = g.at(i, j, m);
+    const idx_t y = g.at(i, j, m);
     return H * (f.ww[y + sk] + f.ww[y]) * c.rdnw[m] * (ph[y + sk] - ph[y] + f.phb(i, j, m + 1) - f.phb(i, j, m));
   };
   auto grad6x = [&]() {
@@ -414,7 +496,7 @@ __global__ void k_tend_ph(Range r, Args a) {
   if (k < nz) t = t - (c.fnm[k] * wdwn(k) + c.fnp[k] * wdwn(k - 1));
   t = t + (c.c1f[k] * f.mut[x2] + c.c2f[k]) * kG * f.w2[x] / msfty;
   const double r60 = 1.0 / 60.0;
-  const int64_t xt = g.at(i, j, nz - 1);  // top extrapolation reads levels nz-1, nz-2
+  const idx_t xt = g.at(i, j, nz - 1);  // top extrapolation reads levels nz-1, nz-2
   if (!g.two_d) {
     const double mvp = c.c1f[k] * f.muv[x2 + sj] + c.c2f[k], mvm = c.c1f[k] * f.muv[x2] + c.c2f[k];
     const double vp = k < nz ? v[x + sj] + v[x + sj - sk] : c.cfn * v[xt + sj] + c.cfn1 * v[xt + sj - sk];
@@ -438,7 +520,7 @@ __global__ void k_ss_prep(Range r, Args a, int first) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double mts = f.mub[x2] + f.mu1[x2];
   const double mus_u = first ? f.muu[x2] : H * (mts + f.mub[x2 - 1] + f.mu1[x2 - 1]);
   const double mus_v = first ? f.muv[x2] : H * (mts + f.mub[x2 - g.sj] + f.mu1[x2 - g.sj]);
@@ -478,17 +560,18 @@ __global__ void k_ss_prep(Range r, Args a, int first) {
   }
 }
 
-// p_rho_point (small_step.cpp:97-114): KPRho (damp = 0) and KWPRho (damp = 1, smdiv damping)
-__global__ void k_p_rho(Range r, Args a, int damp) {
-  MI_POINT_INDEX(r)
+// p_rho_point (small_step.cpp:97-114): al'' and p'' of one point from phi'' at levels k and k+1;
+// damp: the smdiv divergence damping of the substeps (KWPRho), else the stage start (KPRho)
+__device__ __forceinline__ void p_rho_at(const Args& a, int i, int j, int k, double ph_k, double ph_kp1, double mu2,
+                                         double muts, double t2, bool damp) {
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double m = c.c1h[k] * f.muts[x2] + c.c2h[k];
-  const double al = -((1. / m) * (f.alt[x] * (c.c1h[k] * f.mu2[x2]) + c.rdnw[k] * (f.ph2[x + g.sk] - f.ph2[x])));
+  const idx_t x = g.at(i, j, k);
+  const double m = c.c1h[k] * muts + c.c2h[k];
+  const double al = -((1. / m) * (f.alt[x] * (c.c1h[k] * mu2) + c.rdnw[k] * (ph_kp1 - ph_k)));
   f.al[x] = al;
-  double pp = f.c2a[x] * (f.alt[x] * (f.t2[x] - c.c1h[k] * f.mu2[x2] * f.t_save[x]) / (m * (kT0 + f.t_save[x])) - al);
+  double pp = f.c2a[x] * (f.alt[x] * (t2 - c.c1h[k] * mu2 * f.t_save[x]) / (m * (kT0 + f.t_save[x])) - al);
   if (damp) {
     const double ptmp = pp;
     pp = pp + a.q.smdiv * (pp - f.pm1[x]);
@@ -499,39 +582,61 @@ __global__ void k_p_rho(Range r, Args a, int damp) {
   f.p[x] = pp;
 }
 
-// KCoefW calc_coef_w (small_step.cpp:124-148), dry
-__global__ void k_coef_w(Args a, double cof) {
+// KPRho (small_step.cpp:116-121) and KCoefW calc_coef_w (small_step.cpp:124-148), dry, one column
+// group: p'', al'' and the b, c, a coefficients in parallel, then the alpha / gamma recurrence
+MI_COLUMN_KERNEL k_ss_column(Args a, double cof) {
+  constexpr int kKT = KT;
+  extern __shared__ double lds[];
   const Geo& g = a.g;
-  MI_COLUMN_INDEX(g.nx, g.ny)
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const int nz = g.nz;
-  const int64_t x0 = g.at(i, j, 0), sk = g.sk;
-  const double m = f.mut[g.at2(i, j)];
-  const double lid = a.q.top_lid ? 0.0 : 1.0;
-  auto mh = [&](int kk) { return c.c1h[kk] * m + c.c2h[kk]; };
-  auto mf = [&](int kk) { return c.c1f[kk] * m + c.c2f[kk]; };
-  auto c2a = [&](int kk) { return f.c2a[x0 + kk * sk]; };
-  f.ca[x0] = 0;
-  f.calpha[x0] = 0;
-  f.cgamma[x0] = 0;
-  f.ca[x0 + sk] = 0;
-  f.ca[x0 + nz * sk] = -2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(nz - 1) * lid / (mh(nz - 1) * mf(nz - 1));
-  for (int kk = 2; kk < nz; ++kk)
-    f.ca[x0 + kk * sk] = -1.0 * cof * c.rdn[kk] * c.rdnw[kk - 1] * c2a(kk - 1) / (mh(kk - 1) * mf(kk - 1));
-  double gam = 0.0;  // gamma(k-1)
-  for (int kk = 1; kk < nz; ++kk) {
-    const double cq1 = 1.0;
-    const double b = 1.0 + cq1 * cof * c.rdn[kk] * (c.rdnw[kk] * c2a(kk) / (mh(kk) * mf(kk)) + c.rdnw[kk - 1] * c2a(kk - 1) / (mh(kk - 1) * mf(kk)));
-    const double cc = -cq1 * cof * c.rdn[kk] * c.rdnw[kk] * c2a(kk) / (mh(kk) * mf(kk + 1));
-    const double al = 1.0 / (b - f.ca[x0 + kk * sk] * gam);
-    f.calpha[x0 + kk * sk] = al;
-    gam = cc * al;
-    f.cgamma[x0 + kk * sk] = gam;
+  const ColPos cp = col_pos<CI>();
+  const int i = cp.i, j = cp.j, nz = g.nz;
+  const bool on = i < g.nx;
+  double* B = lds;                  // b(k), k = 1..nz
+  double* CC = B + (nz + 1) * CI;  // c(k), k = 1..nz-1
+  double* AA = CC + (nz + 1) * CI; // a(k)
+  if (on) {
+    const idx_t x2 = g.at2(i, j);
+    const double m = f.mut[x2];
+    const double mu2 = f.mu2[x2], muts = f.muts[x2];
+    const double lid = a.q.top_lid ? 0.0 : 1.0;
+    auto mh = [&](int kk) { return c.c1h[kk] * m + c.c2h[kk]; };
+    auto mf = [&](int kk) { return c.c1f[kk] * m + c.c2f[kk]; };
+    auto c2a = [&](int kk) { return f.c2a[g.at(i, j, kk)]; };
+    for (int k = cp.kt; k <= nz; k += kKT) {
+      const idx_t x = g.at(i, j, k);
+      if (k < nz) p_rho_at(a, i, j, k, f.ph2[x], f.ph2[x + g.sk], mu2, muts, f.t2[x], false);
+      double ak;
+      if (k <= 1) ak = 0;
+      else if (k == nz) ak = -2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(nz - 1) * lid / (mh(nz - 1) * mf(nz - 1));
+      else ak = -1.0 * cof * c.rdn[k] * c.rdnw[k - 1] * c2a(k - 1) / (mh(k - 1) * mf(k - 1));
+      f.ca[x] = ak;
+      LDS(AA, k, cp.ci) = ak;
+      if (k >= 1 && k < nz) {
+        const double cq1 = 1.0;
+        LDS(B, k, cp.ci) = 1.0 + cq1 * cof * c.rdn[k] * (c.rdnw[k] * c2a(k) / (mh(k) * mf(k)) + c.rdnw[k - 1] * c2a(k - 1) / (mh(k - 1) * mf(k)));
+        LDS(CC, k, cp.ci) = -cq1 * cof * c.rdn[k] * c.rdnw[k] * c2a(k) / (mh(k) * mf(k + 1));
+      } else if (k == nz) {
+        LDS(B, k, cp.ci) = 1.0 + 2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(nz - 1) / (mh(nz - 1) * mf(nz));
+      }
+    }
   }
-  const double b = 1.0 + 2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(nz - 1) / (mh(nz - 1) * mf(nz));
-  f.calpha[x0 + nz * sk] = 1.0 / (b - f.ca[x0 + nz * sk] * gam);
-  f.cgamma[x0 + nz * sk] = 0;
+  __syncthreads();
+  if (on && cp.kt == 0) {
+    const idx_t x0 = g.at(i, j, 0), sk = g.sk;
+    f.calpha[x0] = 0;
+    f.cgamma[x0] = 0;
+    double gam = 0.0;
+    for (int k = 1; k < nz; ++k) {
+      const double al = 1.0 / (LDS(B, k, cp.ci) - LDS(AA, k, cp.ci) * gam);
+      f.calpha[x0 + k * sk] = al;
+      gam = LDS(CC, k, cp.ci) * al;
+      f.cgamma[x0 + k * sk] = gam;
+    }
+    f.calpha[x0 + nz * sk] = 1.0 / (LDS(B, nz, cp.ci) - LDS(AA, nz, cp.ci) * gam);
+    f.cgamma[x0 + nz * sk] = 0;
+  }
 }
 
 // ----------------------------------------------------------------------------- advance_uv
@@ -542,14 +647,14 @@ __global__ void k_uv(Range r, Args a, double dts) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const int nz = g.nz;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j), sk = g.sk;
-  auto dpn = [&](int64_t d, int kk) {
-    auto ps = [&](int m) { const int64_t y = g.at(i, j, m); return f.p[y] + f.p[y - d]; };
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), sk = g.sk;
+  auto dpn = [&](idx_t d, int kk) {
+    auto ps = [&](int m) { const idx_t y = g.at(i, j, m); return f.p[y] + f.p[y - d]; };
     if (kk == 0) return H * (c.cf1 * ps(0) + c.cf2 * ps(1) + c.cf3 * ps(2));
     if (kk == nz) return a.q.top_lid ? H * (c.cf1 * ps(nz - 1) + c.cf2 * ps(nz - 2) + c.cf3 * ps(nz - 3)) : 0.0;
     return H * (c.fnm[kk] * ps(kk) + c.fnp[kk] * ps(kk - 1));
   };
-  auto dp = [&](int64_t d, int di, int dj, double mface, double rd, double rm) {
+  auto dp = [&](idx_t d, int di, int dj, double mface, double rd, double rm) {
     double dd = rm * H * rd * (c.c1h[k] * mface + c.c2h[k]) *
                 (((f.ph2[x + sk] - f.ph2[x + sk - d]) + (f.ph2[x] - f.ph2[x - d])) + (f.alt[x] + f.alt[x - d]) * (f.p[x] - f.p[x - d]) +
                  (f.al[x] + f.al[x - d]) * (f.pb(i, j, k) - f.pb(i - di, j - dj, k)));
@@ -564,253 +669,307 @@ __global__ void k_uv(Range r, Args a, double dts) {
   }
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
   if (!g.two_d && j >= vj0 && j < g.ny) {
-    const int64_t sj = g.sj;
+    const idx_t sj = g.sj;
     const double mudf_y = -a.q.emdy * (f.mudf[x2] - f.mudf[x2 - sj]) * f.msfvx_inv[x2];
     f.v2[x] = f.v2[x] + dts * f.rv_tend[x] - dts * 1.0 * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) + c.c1h[k] * mudf_y;
   }
 }
 
-// ----------------------------------------------------------------------------- advance_mu_t + sumflux
-// KMuTDiv (small_step.cpp:194-207)
-__global__ void k_mut_div(Range r, Args a) {
-  MI_POINT_INDEX(r)
+// ----------------------------------------------------------------------------- one acoustic substep's column work
+// advance_mu_t + sumflux and advance_w + calc_p_rho of one substep in one kernel (reference
+// SmallStep::run_stage, small_step.cpp:548-593).  Between KMuTPoint and KWRhs the reference refreshes
+// the halos of mu, mudf and muts; none of the advance_w kernels reads a halo point (each reads its own
+// column only), so the refresh moves after this kernel.  Phases, separated by barriers:
+//   1 KMuTDiv (small_step.cpp:194-207): dvdxi of every level (parallel)
+//   2 KMuTCol (small_step.cpp:209-248): dmdt in k order, mu, mudf, muts, muave, omega'' (one thread)
+//   3 KMuTCol's ww_m sums and KMuTPoint (small_step.cpp:250-276): ru_m, rv_m, theta'' (parallel)
+//   4 KWRhs (small_step.cpp:312-328): explicit phi'' (R) and t_2ave (parallel)
+//   5 KWExpl (small_step.cpp:331-378): explicit w'' (parallel)
+//   6 KWSolve's tridiagonal sweeps (small_step.cpp:386-387, one thread)
+//   7 KWSolve's damp_opt 3 damping and phi'' (small_step.cpp:388-406, parallel: each level depends on
+//     its own values only)
+//   8 KWPRho (small_step.cpp:409-414): p'', al'' with divergence damping (parallel)
+MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
+  constexpr int kKT = KT;
+  extern __shared__ double lds[];
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j), sj = g.sj;
-  const double fx = (f.u2[x + 1] + (c.c1h[k] * f.muu[x2 + 1] + c.c2h[k]) * f.u_save[x + 1] / f.msfuy[x2 + 1]) -
-                    (f.u2[x] + (c.c1h[k] * f.muu[x2] + c.c2h[k]) * f.u_save[x] / f.msfuy[x2]);
-  double d = c.rdx * fx;
-  if (!g.two_d)
-    d += c.rdy * ((f.v2[x + sj] + (c.c1h[k] * f.muv[x2 + sj] + c.c2h[k]) * f.v_save[x + sj] * f.msfvx_inv[x2 + sj]) -
-                  (f.v2[x] + (c.c1h[k] * f.muv[x2] + c.c2h[k]) * f.v_save[x] * f.msfvx_inv[x2]));
-  f.dvdxi[x] = f.msftx[x2] * f.msfty[x2] * d;
-}
-
-// KMuTCol (small_step.cpp:209-248): mu, mudf, muts, muave; omega'' and its running sum ww_m
-__global__ void k_mut_col(Args a, double dts, int it, int ns) {
-  const Geo& g = a.g;
-  MI_COLUMN_INDEX(g.nx, g.ny)
-  const Fields& f = a.f;
-  const Lv& c = a.c;
-  const int nz = g.nz;
-  const int64_t x0 = g.at(i, j, 0), x2 = g.at2(i, j), sk = g.sk;
+  const ColPos cp = col_pos<CI>();
+  const int i = cp.i, j = cp.j, nz = g.nz, ci = cp.ci;
+  const bool on = i < g.nx;
+  const int L = (nz + 1) * CI;
+  double* DV = lds;       // dvdxi(k)
+  double* WW = DV + L;    // omega''(k)
+  double* R = WW + L;     // explicit phi''(k)
+  double* T2A = R + L;    // t_2ave(k)
+  double* W = T2A + L;    // w''(k)
+  double* AA = W + L;     // calc_coef_w a(k), alpha(k), gamma(k)
+  double* AL = AA + L;
+  double* GA = AL + L;
+  double* PH = GA + L;    // phi'' after the solve
+  double* S2 = PH + L;    // per column: mu'', muts, muave after advance_mu_t
   const bool first = it == 1, last = it == ns;
   const double rn = double(ns);
-  double dmdt = 0;
-  for (int kk = 0; kk < nz; ++kk) dmdt += c.dnw[kk] * f.dvdxi[x0 + kk * sk];
-  const double mtend = f.mu_tend[x2];
-  const double mu_old = f.mu2[x2];
-  const double mu_new = mu_old + dts * (dmdt + mtend);
-  f.mu2[x2] = mu_new;
-  f.mudf[x2] = dmdt + mtend;
-  f.muts[x2] = f.mut[x2] + mu_new;
-  f.muave[x2] = H * ((1.0 + a.q.epssm) * mu_new + (1.0 - a.q.epssm) * mu_old);
-  const double msfty = f.msfty[x2];
-  auto acc_ww = [&](int kk, double wpp) {
-    const int64_t y = x0 + kk * sk;
-    const double sw = (first ? 0.0 : f.ww_m[y]) + wpp;
-    f.ww_m[y] = last ? sw / rn + f.ww1[y] : sw;
-  };
-  double ww_full = 0;
-  f.ww[x0] = 0;
-  acc_ww(0, 0.0);
-  for (int kk = 0; kk < nz; ++kk) {
-    double wpp_hi = 0;
-    if (kk + 1 < nz) {
-      ww_full = ww_full - c.dnw[kk] * (c.c1h[kk] * dmdt + f.dvdxi[x0 + kk * sk] + c.c1h[kk] * mtend) / msfty;
-      wpp_hi = ww_full - f.ww1[x0 + (kk + 1) * sk];
-      f.ww[x0 + (kk + 1) * sk] = wpp_hi;
-    } else {
-      f.ww[x0 + nz * sk] = 0;
-    }
-    acc_ww(kk + 1, wpp_hi);
-  }
-}
-
-// KMuTPoint (small_step.cpp:250-276): ru_m / rv_m sums and the theta'' update
-__global__ void k_mut_point(Range r, Args a, double dts, int it, int ns) {
-  MI_POINT_INDEX(r)
-  const Geo& g = a.g;
-  const Fields& f = a.f;
-  const Lv& c = a.c;
-  const int nz = g.nz;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j), sj = g.sj, sk = g.sk;
-  const bool first = it == 1, last = it == ns;
-  const double rn = double(ns);
-  auto wdtn = [&](int kk) {
-    if (kk == 0 || kk == nz) return 0.0;
-    const int64_t y = g.at(i, j, kk);
-    return f.ww[y] * (c.fnm[kk] * f.t_save[y] + c.fnp[kk] * f.t_save[y - sk]);
-  };
-  const double su = (first ? 0.0 : f.ru_m[x]) + f.u2[x];
-  f.ru_m[x] = last ? su / rn + (c.c1h[k] * f.muu[x2] + c.c2h[k]) * f.u_save[x] / f.msfuy[x2] : su;
-  if (!g.two_d) {
-    const double sv = (first ? 0.0 : f.rv_m[x]) + f.v2[x];
-    f.rv_m[x] = last ? sv / rn + (c.c1h[k] * f.muv[x2] + c.c2h[k]) * f.v_save[x] * f.msfvx_inv[x2] : sv;
-  }
-  const double tk = f.t2[x];
-  f.t_2save[x] = tk;
-  const double* ts = f.t_save;
-  double hflux = H * c.rdx * (f.u2[x + 1] * (ts[x + 1] + ts[x]) - f.u2[x] * (ts[x] + ts[x - 1]));
-  if (!g.two_d) hflux += H * c.rdy * (f.v2[x + sj] * (ts[x + sj] + ts[x]) - f.v2[x] * (ts[x] + ts[x - sj]));
-  const double msfty = f.msfty[x2];
-  const double tn = tk + msfty * dts * f.t_tend[x];
-  f.t2[x] = tn - dts * msfty * (f.msftx[x2] * hflux + c.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
-}
-
-// ----------------------------------------------------------------------------- advance_w
-// KWRhs (small_step.cpp:312-328): explicit phi'' (rhs) and t_2ave
-__global__ void k_w_rhs(Range r, Args a, double dts) {
-  MI_POINT_INDEX(r)
-  const Geo& g = a.g;
-  const Fields& f = a.f;
-  const Lv& c = a.c;
-  const int nz = g.nz;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j), sk = g.sk;
   const double epssm = a.q.epssm;
-  double R;
-  if (k == 0) {
-    R = 0;
-  } else if (a.q.top_lid && k == nz) {
-    R = 0;
-  } else {
-    auto wdwn = [&](int m) {
-      const int64_t y = g.at(i, j, m);
-      return H * (f.ww[y] + f.ww[y - sk]) * c.rdnw[m - 1] * (f.ph_save[y] - f.ph_save[y - sk] + f.phb(i, j, m) - f.phb(i, j, m - 1));
-    };
-    double rr = dts * (f.ph_tend[x] + H * kG * (1.0 - epssm) * f.w2[x]);
-    if (k < nz) rr -= dts * (c.fnm[k] * wdwn(k + 1) + c.fnp[k] * wdwn(k));
-    R = f.ph2[x] + f.msfty[x2] * rr / (c.c1f[k] * f.mut[x2] + c.c2f[k]);
-  }
-  f.w_rhs[x] = R;
-  if (k < nz) {
-    const double tav = H * ((1.0 + epssm) * f.t2[x] + (1.0 - epssm) * f.t_2save[x]);
-    f.t2a[x] = (tav + c.c1h[k] * f.muave[x2] * kT0) / ((c.c1h[k] * f.muts[x2] + c.c2h[k]) * (kT0 + f.t_save[x]));
-  }
-}
-
-// KWExpl (small_step.cpp:331-378): explicit w'', flat lower boundary
-__global__ void k_w_expl(Range r, Args a, double dts) {
-  MI_POINT_INDEX(r)
-  const Geo& g = a.g;
-  const Fields& f = a.f;
-  const Lv& c = a.c;
-  const int nz = g.nz;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j), sk = g.sk;
-  if (k == 0) {
-    f.w2[x] = 0;
-    return;
-  }
-  const double m = f.mut[x2], mav = f.muave[x2];
-  const double ope = 1.0 + a.q.epssm, ome = 1.0 - a.q.epssm;
-  const double msft_inv = 1.0 / f.msfty[x2];
-  auto mh = [&](int kk) { return c.c1h[kk] * m + c.c2h[kk]; };
-  const double* R = f.w_rhs;
-  const double* ph = f.ph2;
-  const double r_k = R[x], r_km1 = R[x - sk], ph_k = ph[x], ph_km1 = ph[x - sk];
-  if (k < nz) {
-    const double r_kp1 = R[x + sk], ph_kp1 = ph[x + sk];
-    f.w2[x] = f.w2[x] + dts * f.rw_tend[x] +
-              msft_inv * 1.0 *
-                  (H * dts * kG * c.rdn[k] *
-                   (f.c2a[x] * c.rdnw[k] / mh(k) * (ope * (r_kp1 - r_k) + ome * (ph_kp1 - ph_k)) -
-                    f.c2a[x - sk] * c.rdnw[k - 1] / mh(k - 1) * (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)))) +
-              dts * kG * msft_inv *
-                  (c.rdn[k] * (f.c2a[x] * f.alt[x] * f.t2a[x] - f.c2a[x - sk] * f.alt[x - sk] * f.t2a[x - sk]) - c.c1f[k] * mav);
-  } else {
-    const int km = nz - 1;
-    const int64_t xm = x - sk;
-    double wn = f.w2[x] + dts * f.rw_tend[x] +
-                msft_inv * (-H * dts * kG / mh(km) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * f.c2a[xm] * (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)) -
-                            dts * kG * (2.0 * c.rdnw[km] * f.c2a[xm] * f.alt[xm] * f.t2a[xm] + c.c1f[k] * mav));
-    if (a.q.top_lid) wn = 0;
-    f.w2[x] = wn;
-  }
-}
-
-// KWSolve (small_step.cpp:381-407): tridiagonal forward/back substitution, damp_opt = 3 Rayleigh
-// damping of w near the top, phi'' from the implicit w
-__global__ void k_w_solve(Args a, double dts) {
-  const Geo& g = a.g;
-  MI_COLUMN_INDEX(g.nx, g.ny)
-  const Fields& f = a.f;
-  const Lv& c = a.c;
-  const int nz = g.nz;
-  const int64_t x0 = g.at(i, j, 0), x2 = g.at2(i, j), sk = g.sk;
-  const double m = f.mut[x2], mts = f.muts[x2], ope = 1.0 + a.q.epssm;
-  double* w = f.w2 + x0;
-  double wprev = w[0];
-  for (int kk = 1; kk <= nz; ++kk) {
-    const int64_t y = kk * sk;
-    wprev = (w[y] - f.ca[x0 + y] * wprev) * f.calpha[x0 + y];
-    w[y] = wprev;
-  }
-  double wnext = w[nz * sk];
-  for (int kk = nz - 1; kk >= 1; --kk) {
-    const int64_t y = kk * sk;
-    wnext = w[y] - f.cgamma[x0 + y] * wnext;
-    w[y] = wnext;
-  }
-  const double dampmag = a.q.damp_opt == 3 ? dts * a.q.dampcoef : 0.0;
-  if (dampmag != 0.0) {
-    const double pi = 3.141592653589793;  // 4 atan(1), as the compiler folds it
-    const double hdepth = a.q.zdamp;
-    const double htop = (f.ph_save[x0 + nz * sk] + f.phb(i, j, nz)) / kG, hbot = htop - hdepth;
-    for (int kk = nz; kk >= 1; --kk) {
-      const int64_t y = kk * sk;
-      const double hk = (f.ph_save[x0 + y] + f.phb(i, j, kk)) / kG;
-      double dampwt = 0;
-      if (hk >= hbot) {
-        const double sn = mi_libm::sin(H * pi * (hk - hbot) / hdepth);
-        dampwt = dampmag * sn * sn;
-      }
-      w[y] = (w[y] - dampwt * (c.c1f[kk] * m + c.c2f[kk]) * f.w_save[x0 + y]) / (1.0 + dampwt);
+  const idx_t x2 = g.at2(i, j), sk = g.sk, sj = g.sj;
+  // 1: dvdxi
+  if (on) {
+    for (int k = cp.kt; k <= nz; k += kKT) {
+      const idx_t x = g.at(i, j, k);
+      LDS(AA, k, ci) = f.ca[x];
+      LDS(AL, k, ci) = f.calpha[x];
+      LDS(GA, k, ci) = f.cgamma[x];
+      LDS(PH, k, ci) = f.ww1[x];  // ww1 for phase 2 (PH is free until phase 7)
+      if (k == nz) continue;
+      const double fx = (f.u2[x + 1] + (c.c1h[k] * f.muu[x2 + 1] + c.c2h[k]) * f.u_save[x + 1] / f.msfuy[x2 + 1]) -
+                        (f.u2[x] + (c.c1h[k] * f.muu[x2] + c.c2h[k]) * f.u_save[x] / f.msfuy[x2]);
+      double d = c.rdx * fx;
+      if (!g.two_d)
+        d += c.rdy * ((f.v2[x + sj] + (c.c1h[k] * f.muv[x2 + sj] + c.c2h[k]) * f.v_save[x + sj] * f.msfvx_inv[x2 + sj]) -
+                      (f.v2[x] + (c.c1h[k] * f.muv[x2] + c.c2h[k]) * f.v_save[x] * f.msfvx_inv[x2]));
+      LDS(DV, k, ci) = f.msftx[x2] * f.msfty[x2] * d;
     }
   }
-  const double msfty = f.msfty[x2];
-  for (int kk = nz; kk >= 1; --kk) {
-    const int64_t y = kk * sk;
-    f.ph2[x0 + y] = f.w_rhs[x0 + y] + msfty * H * dts * kG * ope * w[y] / (c.c1f[kk] * mts + c.c2f[kk]);
+  __syncthreads();
+  // 2: the column mass and omega''
+  if (on && cp.kt == 0) {
+    double dmdt = 0;
+    for (int k = 0; k < nz; ++k) dmdt += c.dnw[k] * LDS(DV, k, ci);
+    const double mtend = f.mu_tend[x2];
+    const double mu_old = f.mu2[x2];
+    const double mu_new = mu_old + dts * (dmdt + mtend);
+    const double muts = f.mut[x2] + mu_new;
+    const double muave = H * ((1.0 + epssm) * mu_new + (1.0 - epssm) * mu_old);
+    f.mu2[x2] = mu_new;
+    f.mudf[x2] = dmdt + mtend;
+    f.muts[x2] = muts;
+    f.muave[x2] = muave;
+    S2[ci] = mu_new;
+    S2[CI + ci] = muts;
+    S2[2 * CI + ci] = muave;
+    const double msfty = f.msfty[x2];
+    double ww_full = 0;
+    LDS(WW, 0, ci) = 0;
+    for (int k = 0; k < nz; ++k) {
+      double wpp_hi = 0;
+      if (k + 1 < nz) {
+        ww_full = ww_full - c.dnw[k] * (c.c1h[k] * dmdt + LDS(DV, k, ci) + c.c1h[k] * mtend) / msfty;
+        wpp_hi = ww_full - LDS(PH, k + 1, ci);
+      }
+      LDS(WW, k + 1, ci) = wpp_hi;
+    }
+  }
+  __syncthreads();
+  // 3: ww_m sums, ru_m / rv_m sums, theta''
+  if (on) {
+    const double msfty = f.msfty[x2], msftx = f.msftx[x2];
+    for (int k = cp.kt; k <= nz; k += kKT) {
+      const idx_t x = g.at(i, j, k);
+      {
+        const double wk = LDS(WW, k, ci);
+        f.ww[x] = wk;
+        const double sw = (first ? 0.0 : f.ww_m[x]) + wk;
+        f.ww_m[x] = last ? sw / rn + LDS(PH, k, ci) : sw;
+      }
+      if (k == nz) continue;
+      auto wdtn = [&](int kk) {
+        if (kk == 0 || kk == nz) return 0.0;
+        const idx_t y = g.at(i, j, kk);
+        return LDS(WW, kk, ci) * (c.fnm[kk] * f.t_save[y] + c.fnp[kk] * f.t_save[y - sk]);
+      };
+      const double su = (first ? 0.0 : f.ru_m[x]) + f.u2[x];
+      f.ru_m[x] = last ? su / rn + (c.c1h[k] * f.muu[x2] + c.c2h[k]) * f.u_save[x] / f.msfuy[x2] : su;
+      if (!g.two_d) {
+        const double sv = (first ? 0.0 : f.rv_m[x]) + f.v2[x];
+        f.rv_m[x] = last ? sv / rn + (c.c1h[k] * f.muv[x2] + c.c2h[k]) * f.v_save[x] * f.msfvx_inv[x2] : sv;
+      }
+      const double tk = f.t2[x];
+      LDS(T2A, k, ci) = tk;  // t_2save, read back by this thread in phase 4
+      const double* ts = f.t_save;
+      double hflux = H * c.rdx * (f.u2[x + 1] * (ts[x + 1] + ts[x]) - f.u2[x] * (ts[x] + ts[x - 1]));
+      if (!g.two_d) hflux += H * c.rdy * (f.v2[x + sj] * (ts[x + sj] + ts[x]) - f.v2[x] * (ts[x] + ts[x - sj]));
+      const double tn = tk + msfty * dts * f.t_tend[x];
+      f.t2[x] = tn - dts * msfty * (msftx * hflux + c.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
+    }
+  }
+  // 4: explicit phi'' and t_2ave (this thread's levels: t'' and t_2save are its own writes above)
+  const double mu_new = S2[ci], muts = S2[CI + ci], muave = S2[2 * CI + ci];
+  if (on) {
+    const double m = f.mut[x2], msfty = f.msfty[x2];
+    for (int k = cp.kt; k <= nz; k += kKT) {
+      const idx_t x = g.at(i, j, k);
+      double Rk;
+      if (k == 0) {
+        Rk = 0;
+      } else if (a.q.top_lid && k == nz) {
+        Rk = 0;
+      } else {
+        auto wdwn = [&](int mm) {
+          const idx_t y = g.at(i, j, mm);
+          return H * (LDS(WW, mm, ci) + LDS(WW, mm - 1, ci)) * c.rdnw[mm - 1] *
+                 (f.ph_save[y] - f.ph_save[y - sk] + f.phb(i, j, mm) - f.phb(i, j, mm - 1));
+        };
+        double rr = dts * (f.ph_tend[x] + H * kG * (1.0 - epssm) * f.w2[x]);
+        if (k < nz) rr -= dts * (c.fnm[k] * wdwn(k + 1) + c.fnp[k] * wdwn(k));
+        Rk = f.ph2[x] + msfty * rr / (c.c1f[k] * m + c.c2f[k]);
+      }
+      LDS(R, k, ci) = Rk;
+      if (k < nz) {
+        const double tav = H * ((1.0 + epssm) * f.t2[x] + (1.0 - epssm) * LDS(T2A, k, ci));
+        LDS(T2A, k, ci) = (tav + c.c1h[k] * muave * kT0) / ((c.c1h[k] * muts + c.c2h[k]) * (kT0 + f.t_save[x]));
+      }
+    }
+  }
+  __syncthreads();
+  // 5: explicit w'' (flat lower boundary: w''(0) = 0)
+  if (on) {
+    const double m = f.mut[x2], mav = muave;
+    const double ope = 1.0 + epssm, ome = 1.0 - epssm;
+    const double msft_inv = 1.0 / f.msfty[x2];
+    auto mh = [&](int kk) { return c.c1h[kk] * m + c.c2h[kk]; };
+    for (int k = cp.kt; k <= nz; k += kKT) {
+      if (k == 0) {
+        LDS(W, 0, ci) = 0;
+        continue;
+      }
+      const idx_t x = g.at(i, j, k);
+      const double* ph = f.ph2;
+      const double r_k = LDS(R, k, ci), r_km1 = LDS(R, k - 1, ci), ph_k = ph[x], ph_km1 = ph[x - sk];
+      double wk;
+      if (k < nz) {
+        const double r_kp1 = LDS(R, k + 1, ci), ph_kp1 = ph[x + sk];
+        wk = f.w2[x] + dts * f.rw_tend[x] +
+             msft_inv * 1.0 *
+                 (H * dts * kG * c.rdn[k] *
+                  (f.c2a[x] * c.rdnw[k] / mh(k) * (ope * (r_kp1 - r_k) + ome * (ph_kp1 - ph_k)) -
+                   f.c2a[x - sk] * c.rdnw[k - 1] / mh(k - 1) * (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)))) +
+             dts * kG * msft_inv *
+                 (c.rdn[k] * (f.c2a[x] * f.alt[x] * LDS(T2A, k, ci) - f.c2a[x - sk] * f.alt[x - sk] * LDS(T2A, k - 1, ci)) -
+                  c.c1f[k] * mav);
+      } else {
+        const int km = nz - 1;
+        const idx_t xm = x - sk;
+        wk = f.w2[x] + dts * f.rw_tend[x] +
+             msft_inv * (-H * dts * kG / mh(km) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * f.c2a[xm] * (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)) -
+                         dts * kG * (2.0 * c.rdnw[km] * f.c2a[xm] * f.alt[xm] * LDS(T2A, km, ci) + c.c1f[k] * mav));
+        if (a.q.top_lid) wk = 0;
+      }
+      LDS(W, k, ci) = wk;
+    }
+  }
+  __syncthreads();
+  // 6: tridiagonal sweeps
+  if (on && cp.kt == 0) {
+    double wprev = LDS(W, 0, ci);
+    for (int k = 1; k <= nz; ++k) {
+      wprev = (LDS(W, k, ci) - LDS(AA, k, ci) * wprev) * LDS(AL, k, ci);
+      LDS(W, k, ci) = wprev;
+    }
+    double wnext = LDS(W, nz, ci);
+    for (int k = nz - 1; k >= 1; --k) {
+      wnext = LDS(W, k, ci) - LDS(GA, k, ci) * wnext;
+      LDS(W, k, ci) = wnext;
+    }
+  }
+  __syncthreads();
+  // 7: damping near the top, phi'' from the implicit w''
+  if (on) {
+    const double m = f.mut[x2], ope = 1.0 + epssm, msfty = f.msfty[x2];
+    const double dampmag = a.q.damp_opt == 3 ? dts * a.q.dampcoef : 0.0;
+    const double hdepth = a.q.zdamp;
+    const double htop = (f.ph_save[g.at(i, j, nz)] + f.phb(i, j, nz)) / kG, hbot = htop - hdepth;
+    for (int k = cp.kt; k <= nz; k += kKT) {
+      const idx_t x = g.at(i, j, k);
+      double wk = LDS(W, k, ci);
+      if (k == 0) {
+        f.w2[x] = wk;
+        LDS(PH, 0, ci) = f.ph2[x];
+        continue;
+      }
+      if (dampmag != 0.0) {
+        const double pi = 3.141592653589793;  // 4 atan(1), folded
+        const double hk = (f.ph_save[x] + f.phb(i, j, k)) / kG;
+        double dampwt = 0;
+        if (hk >= hbot) {
+          const double sn = mi_libm::sin(H * pi * (hk - hbot) / hdepth);
+          dampwt = dampmag * sn * sn;
+        }
+        wk = (wk - dampwt * (c.c1f[k] * m + c.c2f[k]) * f.w_save[x]) / (1.0 + dampwt);
+      }
+      f.w2[x] = wk;
+      const double phk = LDS(R, k, ci) + msfty * H * dts * kG * ope * wk / (c.c1f[k] * muts + c.c2f[k]);
+      f.ph2[x] = phk;
+      LDS(PH, k, ci) = phk;
+    }
+  }
+  __syncthreads();
+  // 8: p'', al'' with divergence damping
+  if (on) {
+    for (int k = cp.kt; k < nz; k += kKT)
+      p_rho_at(a, i, j, k, LDS(PH, k, ci), LDS(PH, k + 1, ci), mu_new, muts, f.t2[g.at(i, j, k)], true);
   }
 }
 
 // ----------------------------------------------------------------------------- small_step_finish
-// KFinish (small_step.cpp:419-446), dry
-__global__ void k_finish(Range r, Args a) {
-  MI_POINT_INDEX(r)
+// KFinish + KFinishMu (small_step.cpp:419-454) over the (nx+1) x nyv columns and, with prhophi set,
+// calc_p_rho_phi (KPRhoPhi, small_step.cpp:456-477) of the mass columns from the recoupled state: the
+// reference refreshes halos in between, which KPRhoPhi does not read.
+MI_COLUMN_KERNEL k_finish(Args a, int prhophi) {
+  constexpr int kKT = KT;
+  extern __shared__ double lds[];
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  if (j < ny && k < nz) {
-    const double mus_u = H * (f.muts[x2] + f.muts[x2 - 1]);
-    f.u2[x] = (f.msfuy[x2] * f.u2[x] + f.u_save[x] * (c.c1h[k] * f.muu[x2] + c.c2h[k])) / (c.c1h[k] * mus_u + c.c2h[k]);
+  const ColPos cp = col_pos<CI>();
+  const int i = cp.i, j = cp.j, nx = g.nx, ny = g.ny, nz = g.nz, ci = cp.ci;
+  const bool on = i <= nx;
+  const bool mass = i < nx && j < ny;
+  double* PH = lds;
+  double* MU = PH + (nz + 1) * CI;
+  const idx_t x2 = g.at2(i, j);
+  if (on) {
+    const double muts = f.muts[x2];
+    const double mus_u = j < ny ? H * (muts + f.muts[x2 - 1]) : 0.0;
+    const double mus_v = (i < nx && j < g.nyv) ? H * (muts + f.muts[x2 - g.sj]) : 0.0;
+    const double m = mass ? f.mut[x2] : 0.0;
+    for (int k = cp.kt; k <= nz; k += kKT) {
+      const idx_t x = g.at(i, j, k);
+      if (j < ny && k < nz)
+        f.u2[x] = (f.msfuy[x2] * f.u2[x] + f.u_save[x] * (c.c1h[k] * f.muu[x2] + c.c2h[k])) / (c.c1h[k] * mus_u + c.c2h[k]);
+      if (i < nx && j < g.nyv && k < nz)
+        f.v2[x] = (f.msfvx[x2] * f.v2[x] + f.v_save[x] * (c.c1h[k] * f.muv[x2] + c.c2h[k])) / (c.c1h[k] * mus_v + c.c2h[k]);
+      if (mass) {
+        f.w2[x] = (f.msfty[x2] * f.w2[x] + f.w_save[x] * (c.c1f[k] * m + c.c2f[k])) / (c.c1f[k] * muts + c.c2f[k]);
+        const double phk = f.ph2[x] + f.ph_save[x];
+        f.ph2[x] = phk;
+        LDS(PH, k, ci) = phk;
+        f.ww[x] = f.ww[x] + f.ww1[x];
+        if (k < nz) f.t2[x] = (f.t2[x] - 0.0 + f.t_save[x] * (c.c1h[k] * m + c.c2h[k])) / (c.c1h[k] * muts + c.c2h[k]);
+      }
+    }
+    if (cp.kt == 0) {
+      if (mass) {
+        const double mu = f.mu2[x2] + f.mu_save[x2];
+        f.mu2[x2] = mu;
+        MU[ci] = mu;
+      }
+      if (j < ny) f.muus[x2] = mus_u;
+      if (i < nx && j < g.nyv) f.muvs[x2] = mus_v;
+    }
   }
-  if (i < nx && j < g.nyv && k < nz) {
-    const double mus_v = H * (f.muts[x2] + f.muts[x2 - g.sj]);
-    f.v2[x] = (f.msfvx[x2] * f.v2[x] + f.v_save[x] * (c.c1h[k] * f.muv[x2] + c.c2h[k])) / (c.c1h[k] * mus_v + c.c2h[k]);
+  if (!prhophi) return;
+  __syncthreads();
+  if (on && mass) {
+    const double mu2 = MU[ci], muts = f.muts[x2];
+    for (int k = cp.kt; k < nz; k += kKT) {
+      const idx_t x = g.at(i, j, k);
+      const double al = -1.0 / (c.c1h[k] * muts + c.c2h[k]) * (f.alb(i, j, k) * (c.c1h[k] * mu2) + c.rdnw[k] * (LDS(PH, k + 1, ci) - LDS(PH, k, ci)));
+      f.al[x] = al;
+      const double base = kRd * (kT0 + f.t2[x]) / (kP1000 * (al + f.alb(i, j, k)));
+      f.p[x] = kP1000 * mi_libm::pow(base, kCpovcv) - f.pb(i, j, k);
+    }
   }
-  if (i < nx && j < ny) {
-    const double m = f.mut[x2], mts = f.muts[x2];
-    f.w2[x] = (f.msfty[x2] * f.w2[x] + f.w_save[x] * (c.c1f[k] * m + c.c2f[k])) / (c.c1f[k] * mts + c.c2f[k]);
-    f.ph2[x] = f.ph2[x] + f.ph_save[x];
-    f.ww[x] = f.ww[x] + f.ww1[x];
-    if (k < nz) f.t2[x] = (f.t2[x] - 0.0 + f.t_save[x] * (c.c1h[k] * m + c.c2h[k])) / (c.c1h[k] * mts + c.c2h[k]);
-  }
-}
-
-// KFinishMu (small_step.cpp:448-454)
-__global__ void k_finish_mu(Range r, Args a) {
-  MI_POINT_INDEX(r)
-  (void)k;
-  const Geo& g = a.g;
-  const Fields& f = a.f;
-  const int64_t x2 = g.at2(i, j);
-  if (i < g.nx && j < g.ny) f.mu2[x2] = f.mu2[x2] + f.mu_save[x2];
-  if (j < g.ny) f.muus[x2] = H * (f.muts[x2] + f.muts[x2 - 1]);
-  if (i < g.nx && j < g.nyv) f.muvs[x2] = H * (f.muts[x2] + f.muts[x2 - g.sj]);
 }
 
 // ----------------------------------------------------------------------------- calc_p_rho_phi
@@ -820,7 +979,7 @@ __global__ void k_p_rho_phi(Range r, Args a) {
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const int64_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double al = -1.0 / (c.c1h[k] * f.muts[x2] + c.c2h[k]) * (f.alb(i, j, k) * (c.c1h[k] * f.mu2[x2]) + c.rdnw[k] * (f.ph2[x + g.sk] - f.ph2[x]));
   f.al[x] = al;
   const double base = kRd * (kT0 + f.t2[x]) / (kP1000 * (al + f.alb(i, j, k)));
@@ -832,7 +991,7 @@ __global__ void k_finite(Range r, Args a, int* flags) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
-  const int64_t x = g.at(i, j, k);
+  const idx_t x = g.at(i, j, k);
   auto bad = [](double v) { return !(v - v == 0.0); };
   if (i < g.nx + 1 && j < g.ny && k < g.nz && bad(f.u2[x])) flags[MI_U] = 1;
   if (i < g.nx && j < g.nyv && k < g.nz && bad(f.v2[x])) flags[MI_V] = 1;
@@ -874,65 +1033,100 @@ void launch_fill(double* p, int64_t n, double v, hipStream_t st) {
   hipLaunchKernelGGL(k_fill, dim3(unsigned((n + 255) / 256), 1, 1), dim3(256, 1, 1), 0, st, p, n, v);
 }
 
-void launch_rk_prep_mu(const Args& a, hipStream_t st) {
-  const Geo& g = a.g;
-  launch3(k_mu_uv, Range{0, g.nx + 1, 0, g.nyv, 0, 1}, st, a);
+constexpr size_t kLdsMax = 65536;
+size_t col_lds_bytes(int ci, int arrays, int nz, int extra) {
+  return (size_t(arrays) * size_t(nz + 1) + size_t(extra)) * size_t(ci) * sizeof(double);
 }
-void launch_rk_prep_couple(const Args& a, hipStream_t st) {
-  const Geo& g = a.g;
-  launch3(k_couple, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a);
+
+template <template <int, int> class Launch, class... T>
+void dispatch_cfg(int cfg, T&&... args) {
+  switch (cfg) {
+    case 0: Launch<16, 16>::go(args...); break;
+    case 1: Launch<16, 32>::go(args...); break;
+    case 2: Launch<32, 16>::go(args...); break;
+    case 3: Launch<8, 32>::go(args...); break;
+    case 4: Launch<8, 64>::go(args...); break;
+    case 5: Launch<4, 128>::go(args...); break;
+    default: Launch<2, 256>::go(args...); break;
+  }
 }
-void launch_rk_prep_ww(const Args& a, hipStream_t st) { launch_cols(k_ww, a.g.nx, a.g.ny, st, a); }
+
+// one launch of a column kernel with block shape (CI, KT), timed when selected
+#define MI_COL_LAUNCHER(Name, kern)                                                                              \
+  template <int CI, int KT>                                                                                    \
+  struct Name {                                                                                                \
+    template <class... T>                                                                                      \
+    static void go(const char* name, int arrays, int extra, int ncx, int ncy, hipStream_t st, const Args& a,    \
+                   T... args) {                                                                                \
+      const size_t lds = col_lds_bytes(CI, arrays, a.g.nz, extra);                                            \
+      const bool t = timing(name);                                                                           \
+      if (t) time_begin(st);                                                                                 \
+      hipLaunchKernelGGL((kern<CI, KT>), dim3(unsigned((ncx + CI - 1) / CI), unsigned(ncy), 1), dim3(CI * KT, 1, 1), \
+                         lds, st, a, args...);                                                               \
+      if (t) time_end(st, point_bytes(name) * double(a.g.nx) * double(a.g.ny) * double(a.g.nz));              \
+    }                                                                                                          \
+  };
+MI_COL_LAUNCHER(LargePrepL, k_large_prep)
+MI_COL_LAUNCHER(SsColumnL, k_ss_column)
+MI_COL_LAUNCHER(AcousticL, k_acoustic)
+MI_COL_LAUNCHER(FinishL, k_finish)
+
+// LDS arrays (per level, extra per column) of each column kernel, in ColKernel order
+constexpr int kColArrays[kNumColKernels][2] = {{1, 0}, {3, 0}, {9, 3}, {1, 1}};
+
+bool col_cfg_fits(int kernel, int cfg, int nz) {
+  if (cfg < 0 || cfg >= kNumColCfgs) return false;
+  return col_lds_bytes(kColCfgs[cfg][0], kColArrays[kernel][0], nz, kColArrays[kernel][1]) <= kLdsMax;
+}
+
+int col_cfg_default(int kernel, int nx, int ny, int nz) {
+  // small domains: one level per thread (latency); larger: 16 columns (full 128-byte rows)
+  const bool small = int64_t(nx) * ny <= 16384;
+  const int pref[] = {small ? (nz + 1 <= 64 ? 4 : (nz + 1 <= 128 ? 5 : 6)) : 0, 3, 5, 6};
+  for (int c : pref)
+    if (col_cfg_fits(kernel, c, nz)) return c;
+  return 6;
+}
+
+void launch_large_prep(const Args& a, int cfg, hipStream_t st) {
+  dispatch_cfg<LargePrepL>(cfg, "large_prep", 1, 0, a.g.nx + 1, a.g.nyv, st, a);
+}
 
 void launch_tendencies(const Args& a, hipStream_t st) {
   const Geo& g = a.g;
   const int ui0 = g.bcx != MI_BC_PERIODIC ? 1 : 0;
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
-  launch3(k_tend_u, Range{ui0, g.nx, 0, g.ny, 0, g.nz}, st, a);
-  if (!g.two_d) launch3(k_tend_v, Range{0, g.nx, vj0, g.ny, 0, g.nz}, st, a);
-  launch3(k_tend_w, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
-  launch3(k_tend_t, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
-  launch3(k_tend_ph, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
+  launch3("tend_u", k_tend_u, Range{ui0, g.nx, 0, g.ny, 0, g.nz}, st, a);
+  if (!g.two_d) launch3("tend_v", k_tend_v, Range{0, g.nx, vj0, g.ny, 0, g.nz}, st, a);
+  launch3("tend_w", k_tend_w, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
+  launch3("tend_t", k_tend_t, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
+  launch3("tend_ph", k_tend_ph, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
 }
 
 void launch_ss_prep(const Args& a, bool first, hipStream_t st) {
   const Geo& g = a.g;
-  launch3(k_ss_prep, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, first ? 1 : 0);
+  launch3("ss_prep", k_ss_prep, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, first ? 1 : 0);
 }
-void launch_p_rho(const Args& a, bool damp, hipStream_t st) {
-  const Geo& g = a.g;
-  launch3(k_p_rho, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, damp ? 1 : 0);
+void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st) {
+  dispatch_cfg<SsColumnL>(cfg, "ss_column", 3, 0, a.g.nx, a.g.ny, st, a, cof);
 }
-void launch_coef_w(const Args& a, double cof, hipStream_t st) { launch_cols(k_coef_w, a.g.nx, a.g.ny, st, a, cof); }
 void launch_uv(const Args& a, double dts, hipStream_t st) {
   const Geo& g = a.g;
-  launch3(k_uv, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, dts);
+  launch3("uv", k_uv, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, dts);
 }
-void launch_mu_t(const Args& a, double dts, int it, int ns, hipStream_t st) {
-  const Geo& g = a.g;
-  launch3(k_mut_div, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
-  launch_cols(k_mut_col, g.nx, g.ny, st, a, dts, it, ns);
-  launch3(k_mut_point, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, dts, it, ns);
+void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
+  dispatch_cfg<AcousticL>(cfg, "acoustic", 9, 3, a.g.nx, a.g.ny, st, a, dts, it, ns);
 }
-void launch_w(const Args& a, double dts, hipStream_t st) {
-  const Geo& g = a.g;
-  launch3(k_w_rhs, Range{0, g.nx, 0, g.ny, 0, g.nz + 1}, st, a, dts);
-  launch3(k_w_expl, Range{0, g.nx, 0, g.ny, 0, g.nz + 1}, st, a, dts);
-  launch_cols(k_w_solve, g.nx, g.ny, st, a, dts);
-  launch3(k_p_rho, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, 1);
-}
-void launch_finish(const Args& a, hipStream_t st) {
-  const Geo& g = a.g;
-  launch3(k_finish, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a);
-  launch3(k_finish_mu, Range{0, g.nx + 1, 0, g.nyv, 0, 1}, st, a);
+void launch_finish(const Args& a, int cfg, bool prhophi, hipStream_t st) {
+  dispatch_cfg<FinishL>(cfg, "finish", 1, 1, a.g.nx + 1, a.g.nyv, st, a, prhophi ? 1 : 0);
 }
 void launch_p_rho_phi(const Args& a, hipStream_t st) {
   const Geo& g = a.g;
-  launch3(k_p_rho_phi, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
+  launch3("p_rho_phi", k_p_rho_phi, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
 }
 void launch_finite(const Args& a, int* flags, hipStream_t st) {
   const Geo& g = a.g;
-  launch3(k_finite, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, flags);
+  launch3("finite", k_finite, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, flags);
 }
 
 }  // namespace mi_fp64

@@ -1,6 +1,8 @@
 // FP64 dynamics kernels: declarations and the argument block shared by all of them.
 #pragma once
 
+#include <vector>
+
 #include "dev_common.hpp"
 
 namespace mi_fp64 {
@@ -20,8 +22,8 @@ struct Fields {
   // tendencies (src/dyn/state.hpp:80-90)
   double *ru_tend, *rv_tend, *rw_tend, *t_tend, *ph_tend, *mu_tend;
   // acoustic-step work fields (src/dyn/small_step.hpp:43-49)
-  double *u_save, *v_save, *w_save, *t_save, *ph_save, *ww1, *c2a, *pm1, *t_2save, *ca, *calpha, *cgamma;
-  double *mu_save, *mudf, *muave, *w_rhs, *t2a, *dvdxi;
+  double *u_save, *v_save, *w_save, *t_save, *ph_save, *ww1, *c2a, *pm1, *ca, *calpha, *cgamma;
+  double *mu_save, *mudf, *muave;
   // base state: 2-D fields stored in full with halos, 3-D through BField
   double *mub, *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy, *msfvx_inv;
   BField pb, alb, phb;
@@ -52,20 +54,34 @@ struct HaloSet {
   unsigned char face_x[kHaloMax], face_y[kHaloMax], px[kHaloMax], py[kHaloMax];
 };
 
+// per-kernel event timing: while g_timer points to a timer, every launch of the kernel named
+// g_timer->name is bracketed by a pair of events on its stream
+struct KernelTimer {
+  const char* name;
+  std::vector<hipEvent_t> ev;  // start, end, start, end, ...
+  size_t used = 0;
+  double bytes = 0;            // algorithmic bytes of the timed launches
+};
+extern thread_local KernelTimer* g_timer;
+
 void launch_halo(const Args& a, const HaloSet& s, hipStream_t st);
 void launch_copy_step(const Args& a, int64_t n_alloc3, int64_t n_alloc2, hipStream_t st);
 void launch_fill(double* p, int64_t n, double v, hipStream_t st);
-void launch_rk_prep_mu(const Args& a, hipStream_t st);
-void launch_rk_prep_couple(const Args& a, hipStream_t st);
-void launch_rk_prep_ww(const Args& a, hipStream_t st);
+// column kernels and their block shapes (columns x threads per column)
+enum ColKernel { kColLargePrep = 0, kColSsColumn = 1, kColAcoustic = 2, kColFinish = 3, kNumColKernels = 4 };
+constexpr int kNumColCfgs = 7;
+constexpr int kColCfgs[kNumColCfgs][2] = {{16, 16}, {16, 32}, {32, 16}, {8, 32}, {8, 64}, {4, 128}, {2, 256}};
+constexpr const char* kColKernelNames[kNumColKernels] = {"large_prep", "ss_column", "acoustic", "finish"};
+bool col_cfg_fits(int kernel, int cfg, int nz);
+int col_cfg_default(int kernel, int nx, int ny, int nz);
+
+void launch_large_prep(const Args& a, int cfg, hipStream_t st);
 void launch_tendencies(const Args& a, hipStream_t st);
 void launch_ss_prep(const Args& a, bool first, hipStream_t st);
-void launch_p_rho(const Args& a, bool damp, hipStream_t st);
-void launch_coef_w(const Args& a, double cof, hipStream_t st);
+void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st);
 void launch_uv(const Args& a, double dts, hipStream_t st);
-void launch_mu_t(const Args& a, double dts, int it, int ns, hipStream_t st);
-void launch_w(const Args& a, double dts, hipStream_t st);
-void launch_finish(const Args& a, hipStream_t st);
+void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st);
+void launch_finish(const Args& a, int cfg, bool prhophi, hipStream_t st);
 void launch_p_rho_phi(const Args& a, hipStream_t st);
 void launch_finite(const Args& a, int* flags, hipStream_t st);
 

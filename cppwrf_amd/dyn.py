@@ -175,6 +175,22 @@ class RK3:
         a = np.ascontiguousarray(a, dtype=np.float64)
         check(lib().mi_domain_set_field(self._h, fid, _ptr(a), a.size))
 
+    def autotune(self, trials: int = 1) -> list[int]:
+        """time every block shape of the column kernels on this domain, keep the fastest (state restored)"""
+        check(lib().mi_domain_autotune(self._h, trials))
+        return self.tuning
+
+    @property
+    def tuning(self) -> list[int]:
+        a = (C.c_int * 4)()
+        check(lib().mi_domain_get_tuning(self._h, a, 4))
+        return list(a)
+
+    @tuning.setter
+    def tuning(self, shapes) -> None:
+        a = (C.c_int * 4)(*shapes)
+        check(lib().mi_domain_set_tuning(self._h, a, 4))
+
     def check_finite(self) -> int:
         bad = C.c_int(-1)
         rc = lib().mi_check_finite(self._h, C.byref(bad))

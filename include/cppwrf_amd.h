@@ -154,6 +154,25 @@ int mi_calc_p_rho_phi(mi_domain* d, int rk);
  * src/cases/domain_runner.hpp:70-71); *bad_field receives the first such field id (or -1). */
 int mi_check_finite(mi_domain* d, int* bad_field);
 
+/* Autotuner (no counterpart in the reference): times each block shape (columns per block x threads per
+ * column) of the column kernels on this domain for `trials` steps and keeps the fastest.  The carried
+ * state is saved first and restored after, so the forecast's trajectory is unchanged, and every shape
+ * gives bit-identical results.  mi_domain_get_tuning returns the chosen shape index of each column kernel
+ * (prep, acoustic prologue, acoustic substep, finish) into shapes[0..n). */
+int mi_domain_autotune(mi_domain* d, int trials);
+int mi_domain_get_tuning(const mi_domain* d, int* shapes, int n);
+/* sets the shapes explicitly (index 0..6 of the library's table; MI_EINVAL if one does not fit nz) */
+int mi_domain_set_tuning(mi_domain* d, const int* shapes, int n);
+
+/* Per-kernel timing, for measurement (no counterpart in the reference; its CPPWRF_KTIME timer,
+ * src/core/backend.hpp:419-425, synchronises around every launch instead).  While a kernel name is
+ * selected, mi_step launches without a graph and brackets every launch of that kernel with a pair of
+ * HIP events on the domain's stream.  mi_profile_read waits for the stream, returns the summed event
+ * time, the number of timed launches and their algorithmic bytes (distinct 3-D arrays read and written
+ * x 8 bytes x points), and clears the counters.  name NULL or "" stops the timing. */
+int mi_profile_kernel(mi_domain* d, const char* name);
+int mi_profile_read(mi_domain* d, double* total_ms, int* launches, double* alg_bytes);
+
 /* ---------------------------------------------------------------- synthetic initial states
  * Host-only builders of the idealized states the tests and benchmarks use: the reference's case
  * recipes (src/cases/cases.cpp:17-287: sounding, eta levels, hydrostatic base state) and the 3-D

@@ -101,3 +101,33 @@ def test_bubble64_100_steps_bitwise(ctx):
     assert gpu.check_finite() == -1
     w = gpu.field(A.MI_W)
     assert np.abs(w).max() > 1.0  # the bubble rises: the comparison is not of a state at rest
+
+
+@pytest.mark.parametrize("shape", range(7))
+def test_every_block_shape_bitwise(ctx, shape):
+    """The column kernels give the same bits for every block shape the autotuner may choose."""
+    case = Case.bubble3d(24, 20, 16, dx=1000.0, ztop=8000.0, dt=6.0, seed=7, bubble_zc=3000.0)
+    prm = case.params(time_step_sound=6, damp_opt=3, zdamp=3000.0)
+    print(f"[parity] block shape {shape}", flush=True)
+    gpu = RK3(ctx, case.grid, prm, case)
+    gpu.tuning = [shape] * 4
+    orc = OracleDomain(case, prm)
+    gpu.step(3)
+    orc.step(3)
+    _compare(gpu, orc, STATE, f"shape {shape}")
+
+
+def test_autotune_keeps_the_trajectory(ctx):
+    """Autotuning steps the domain to time the shapes, then restores its state: a tuned run equals the oracle."""
+    case = Case.bubble3d(32, 24, 20, dx=1000.0, ztop=10000.0, dt=6.0, seed=5, bubble_zc=3000.0)
+    prm = case.params(time_step_sound=6, damp_opt=3, zdamp=4000.0)
+    print("[parity] autotune", flush=True)
+    gpu = RK3(ctx, case.grid, prm, case)
+    orc = OracleDomain(case, prm)
+    gpu.step(2)
+    orc.step(2)
+    shapes = gpu.autotune(1)
+    assert all(0 <= s < 7 for s in shapes)
+    gpu.step(4)
+    orc.step(4)
+    _compare(gpu, orc, STATE, f"after autotune {shapes}")
