@@ -24,6 +24,27 @@ namespace {
 
 constexpr int kBx = 64;
 
+// MI355X hands consecutive workgroups to its 8 XCDs in turn, and each XCD has its own L2.  Blocks
+// that share cache lines (the halves of a 128-byte row, the halo rows of a stencil, neighbouring
+// levels) should run on the same XCD, so every kernel renumbers its workgroups: the n-th workgroup of
+// XCD x gets the n-th block of the x-th contiguous eighth of the grid.
+constexpr unsigned kXcds = 8;
+struct Blk {
+  int x, y, z;
+};
+__device__ __forceinline__ Blk xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const unsigned nb = gx * gy * gz;
+  const unsigned p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned xcd = p % kXcds, q = nb / kXcds, r = nb % kXcds;
+  const unsigned l = xcd * q + (xcd < r ? xcd : r) + p / kXcds;  // a bijection of [0, nb)
+  Blk b;
+  b.x = int(l % gx);
+  b.y = int((l / gx) % gy);
+  b.z = int(l / (gx * gy));
+  return b;
+}
+
 struct Range {
   int i0, i1, j0, j1, k0, k1;
 };
@@ -33,15 +54,11 @@ inline dim3 grid_of(const Range& r, int by) {
 }
 
 #define MI_POINT_INDEX(r)                          \
-  const int i = (r).i0 + int(blockIdx.x) * kBx + int(threadIdx.x); \
-  const int j = (r).j0 + int(blockIdx.y) * int(blockDim.y) + int(threadIdx.y); \
-  const int k = (r).k0 + int(blockIdx.z);          \
+  const Blk blk_ = xcd_block();                    \
+  const int i = (r).i0 + blk_.x * kBx + int(threadIdx.x); \
+  const int j = (r).j0 + blk_.y * int(blockDim.y) + int(threadIdx.y); \
+  const int k = (r).k0 + blk_.z;                   \
   if (i >= (r).i1 || j >= (r).j1) return;
-
-#define MI_COLUMN_INDEX(nx, ny)                    \
-  const int i = int(blockIdx.x) * kBx + int(threadIdx.x); \
-  const int j = int(blockIdx.y);                   \
-  if (i >= (nx) || j >= (ny)) return;
 
 // HIP-event timing of the launches of one kernel (KernelTimer): events bracket each launch of the
 // selected kernel on the stream it runs on, with the algorithmic bytes of that launch
@@ -196,13 +213,18 @@ struct ColPos {
 };
 template <int CI>
 __device__ __forceinline__ ColPos col_pos() {
+  const Blk b = xcd_block();
   ColPos c;
   c.ci = int(threadIdx.x) % CI;
   c.kt = int(threadIdx.x) / CI;
-  c.i = int(blockIdx.x) * CI + c.ci;
-  c.j = int(blockIdx.y);
+  c.i = b.x * CI + c.ci;
+  c.j = b.y;
   return c;
 }
+// The k-order recurrences of a column run on one thread; their operands come from LDS in blocks of
+// kChain levels loaded together, so the chain waits on LDS once per block rather than once per level.
+constexpr int kChain = 8;
+
 #define MI_COLUMN_KERNEL template <int CI, int KT> __global__ void __launch_bounds__(CI * KT)
 #define LDS(arr, k, ci) (arr)[(k) * CI + (ci)]
 
@@ -262,12 +284,31 @@ MI_COLUMN_KERNEL k_large_prep(Args a) {
   __syncthreads();
   if (on && mass && cp.kt == 0) {
     double dmdt = 0;
-    for (int k = 0; k < nz; ++k) dmdt += LDS(dv, k, cp.ci);
+    for (int k0 = 0; k0 < nz; k0 += kChain) {
+      double v[kChain];
+#pragma unroll
+      for (int q = 0; q < kChain; ++q) v[q] = k0 + q < nz ? LDS(dv, k0 + q, cp.ci) : 0.0;
+#pragma unroll
+      for (int q = 0; q < kChain; ++q)
+        if (k0 + q < nz) dmdt += v[q];
+    }
     double w = 0;
     f.ww[g.at(i, j, 0)] = 0;
-    for (int k = 1; k < nz; ++k) {
-      w = w - c.dnw[k - 1] * c.c1h[k - 1] * dmdt - LDS(dv, k - 1, cp.ci);
-      f.ww[g.at(i, j, k)] = w;
+    for (int k0 = 1; k0 < nz; k0 += kChain) {
+      double dvv[kChain], pv[kChain];
+#pragma unroll
+      for (int q = 0; q < kChain; ++q)
+        if (k0 + q < nz) {
+          const int kk = k0 + q;
+          dvv[q] = LDS(dv, kk - 1, cp.ci);
+          pv[q] = c.dnw[kk - 1] * c.c1h[kk - 1] * dmdt;
+        }
+#pragma unroll
+      for (int q = 0; q < kChain; ++q)
+        if (k0 + q < nz) {
+          w = w - pv[q] - dvv[q];
+          f.ww[g.at(i, j, k0 + q)] = w;
+        }
     }
     f.ww[g.at(i, j, nz)] = 0;
   }
@@ -628,11 +669,23 @@ MI_COLUMN_KERNEL k_ss_column(Args a, double cof) {
     f.calpha[x0] = 0;
     f.cgamma[x0] = 0;
     double gam = 0.0;
-    for (int k = 1; k < nz; ++k) {
-      const double al = 1.0 / (LDS(B, k, cp.ci) - LDS(AA, k, cp.ci) * gam);
-      f.calpha[x0 + k * sk] = al;
-      gam = LDS(CC, k, cp.ci) * al;
-      f.cgamma[x0 + k * sk] = gam;
+    for (int k0 = 1; k0 < nz; k0 += kChain) {
+      double bv[kChain], av[kChain], cv[kChain];
+#pragma unroll
+      for (int q = 0; q < kChain; ++q)
+        if (k0 + q < nz) {
+          bv[q] = LDS(B, k0 + q, cp.ci);
+          av[q] = LDS(AA, k0 + q, cp.ci);
+          cv[q] = LDS(CC, k0 + q, cp.ci);
+        }
+#pragma unroll
+      for (int q = 0; q < kChain; ++q)
+        if (k0 + q < nz) {
+          const double al = 1.0 / (bv[q] - av[q] * gam);
+          f.calpha[x0 + (k0 + q) * sk] = al;
+          gam = cv[q] * al;
+          f.cgamma[x0 + (k0 + q) * sk] = gam;
+        }
     }
     f.calpha[x0 + nz * sk] = 1.0 / (LDS(B, nz, cp.ci) - LDS(AA, nz, cp.ci) * gam);
     f.cgamma[x0 + nz * sk] = 0;
@@ -679,235 +732,301 @@ __global__ void k_uv(Range r, Args a, double dts) {
 // advance_mu_t + sumflux and advance_w + calc_p_rho of one substep in one kernel (reference
 // SmallStep::run_stage, small_step.cpp:548-593).  Between KMuTPoint and KWRhs the reference refreshes
 // the halos of mu, mudf and muts; none of the advance_w kernels reads a halo point (each reads its own
-// column only), so the refresh moves after this kernel.  Phases, separated by barriers:
-//   1 KMuTDiv (small_step.cpp:194-207): dvdxi of every level (parallel)
-//   2 KMuTCol (small_step.cpp:209-248): dmdt in k order, mu, mudf, muts, muave, omega'' (one thread)
-//   3 KMuTCol's ww_m sums and KMuTPoint (small_step.cpp:250-276): ru_m, rv_m, theta'' (parallel)
-//   4 KWRhs (small_step.cpp:312-328): explicit phi'' (R) and t_2ave (parallel)
-//   5 KWExpl (small_step.cpp:331-378): explicit w'' (parallel)
-//   6 KWSolve's tridiagonal sweeps (small_step.cpp:386-387, one thread)
-//   7 KWSolve's damp_opt 3 damping and phi'' (small_step.cpp:388-406, parallel: each level depends on
-//     its own values only)
-//   8 KWPRho (small_step.cpp:409-414): p'', al'' with divergence damping (parallel)
+// column only), so the refresh moves after this kernel.
+//
+// One thread per level (KT >= nz + 1).  Every global load a thread needs is issued in one burst at the
+// start, so a block waits on memory once; the phases after it exchange values through LDS only.  The
+// recurrences of the column keep their order and operands: the per-level terms they add are computed in
+// parallel first (the same products and quotients the reference forms inside its loops), then one thread
+// per column chains the additions.
+//   1  KMuTDiv dvdxi (small_step.cpp:194-207) and the dnw*dvdxi terms of dmdt
+//   2  KMuTCol: dmdt, mu, mudf, muts, muave (small_step.cpp:218-230), one thread
+//   3  KMuTCol's omega'' terms dnw*(c1h*dmdt + dvdxi + c1h*mu_tend)/msfty, parallel
+//   4  the running omega'' integral, one thread
+//   5  omega'' = integral - ww1; KMuTCol's ww_m sums; KMuTPoint (small_step.cpp:250-276): ru_m, rv_m,
+//      theta''; KWRhs (small_step.cpp:312-328): explicit phi'' and t_2ave
+//   6  KWExpl (small_step.cpp:331-378): explicit w''
+//   7  KWSolve's tridiagonal sweeps (small_step.cpp:386-387), one thread
+//   8  KWSolve's damp_opt 3 damping and phi'' (small_step.cpp:388-406), parallel over levels
+//   9  KWPRho (small_step.cpp:409-414)
 MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
-  constexpr int kKT = KT;
   extern __shared__ double lds[];
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
   const ColPos cp = col_pos<CI>();
-  const int i = cp.i, j = cp.j, nz = g.nz, ci = cp.ci;
-  const bool on = i < g.nx;
+  const int i = cp.i, j = cp.j, nz = g.nz, ci = cp.ci, k = cp.kt;
+  const bool on = i < g.nx && k <= nz;
   const int L = (nz + 1) * CI;
-  double* DV = lds;       // dvdxi(k)
-  double* WW = DV + L;    // omega''(k)
-  double* R = WW + L;     // explicit phi''(k)
-  double* T2A = R + L;    // t_2ave(k)
-  double* W = T2A + L;    // w''(k)
-  double* AA = W + L;     // calc_coef_w a(k), alpha(k), gamma(k)
+  double* DV = lds;     // dnw*dvdxi(k), then dvdxi(k)
+  double* TK = DV + L;  // the omega'' terms, then the integral
+  double* WW = TK + L;  // omega''(k)
+  double* R = WW + L;   // explicit phi''(k)
+  double* T2A = R + L;  // t_2ave(k)
+  double* W = T2A + L;  // w''(k)
+  double* AA = W + L;   // calc_coef_w a(k), alpha(k), gamma(k)
   double* AL = AA + L;
   double* GA = AL + L;
-  double* PH = GA + L;    // phi'' after the solve
-  double* S2 = PH + L;    // per column: mu'', muts, muave after advance_mu_t
+  double* PH = GA + L;  // phi'' after the solve
+  double* S2 = PH + L;  // per column: dmdt, mu'' new, muts, muave
   const bool first = it == 1, last = it == ns;
   const double rn = double(ns);
   const double epssm = a.q.epssm;
-  const idx_t x2 = g.at2(i, j), sk = g.sk, sj = g.sj;
-  // 1: dvdxi
+  const idx_t sk = g.sk, sj = g.sj;
+  const idx_t x2 = g.at2(i, j), x = g.at(i, j, k);
+  const bool mk = k < nz, ge1 = k >= 1;  // a mass level; above the surface
+
+  // ---------------------------------------------------------------- the load burst
+  double u0 = 0, u1 = 0, us0 = 0, us1 = 0, v0 = 0, v1 = 0, vs0 = 0, vs1 = 0;
+  double ww1 = 0, wwm = 0, ca = 0, cal = 0, cga = 0, rum = 0, rvm = 0;
+  double t2 = 0, ttend = 0, ts = 0, tsxp = 0, tsxm = 0, tsyp = 0, tsym = 0, tskm = 0, tskp = 0;
+  double phtend = 0, w2 = 0, phs = 0, phs_m = 0, phs_p = 0, ph = 0, ph_m = 0, ph_p = 0, rwt = 0;
+  double c2a = 0, c2a_m = 0, alt = 0, alt_m = 0, wsv = 0, pm1 = 0;
+  double msfty = 1, msftx = 1, mut = 0, muu0 = 0, muu1 = 0, muv0 = 0, muv1 = 0, msfuy0 = 1, msfuy1 = 1, mvi0 = 1, mvi1 = 1;
+  double mu_old = 0, mtend = 0, ph_top = 0;
   if (on) {
-    for (int k = cp.kt; k <= nz; k += kKT) {
-      const idx_t x = g.at(i, j, k);
-      LDS(AA, k, ci) = f.ca[x];
-      LDS(AL, k, ci) = f.calpha[x];
-      LDS(GA, k, ci) = f.cgamma[x];
-      LDS(PH, k, ci) = f.ww1[x];  // ww1 for phase 2 (PH is free until phase 7)
-      if (k == nz) continue;
-      const double fx = (f.u2[x + 1] + (c.c1h[k] * f.muu[x2 + 1] + c.c2h[k]) * f.u_save[x + 1] / f.msfuy[x2 + 1]) -
-                        (f.u2[x] + (c.c1h[k] * f.muu[x2] + c.c2h[k]) * f.u_save[x] / f.msfuy[x2]);
+    msfty = f.msfty[x2];
+    msftx = f.msftx[x2];
+    mut = f.mut[x2];
+    muu0 = f.muu[x2];
+    muu1 = f.muu[x2 + 1];
+    msfuy0 = f.msfuy[x2];
+    msfuy1 = f.msfuy[x2 + 1];
+    if (!g.two_d) {
+      muv0 = f.muv[x2];
+      muv1 = f.muv[x2 + sj];
+      mvi0 = f.msfvx_inv[x2];
+      mvi1 = f.msfvx_inv[x2 + sj];
+    }
+    if (k == 0) {
+      mu_old = f.mu2[x2];
+      mtend = f.mu_tend[x2];
+    }
+    ww1 = f.ww1[x];
+    if (!first) wwm = f.ww_m[x];
+    ca = f.ca[x];
+    cal = f.calpha[x];
+    cga = f.cgamma[x];
+    phtend = f.ph_tend[x];
+    w2 = f.w2[x];
+    phs = f.ph_save[x];
+    ph = f.ph2[x];
+    ph_top = f.ph_save[g.at(i, j, nz)];
+    if (ge1) {
+      phs_m = f.ph_save[x - sk];
+      ph_m = f.ph2[x - sk];
+      rwt = f.rw_tend[x];
+      c2a_m = f.c2a[x - sk];
+      alt_m = f.alt[x - sk];
+      if (a.q.damp_opt == 3) wsv = f.w_save[x];
+    }
+    if (mk) {
+      u0 = f.u2[x];
+      u1 = f.u2[x + 1];
+      us0 = f.u_save[x];
+      us1 = f.u_save[x + 1];
+      if (!g.two_d) {
+        v0 = f.v2[x];
+        v1 = f.v2[x + sj];
+        vs0 = f.v_save[x];
+        vs1 = f.v_save[x + sj];
+      }
+      if (!first) {
+        rum = f.ru_m[x];
+        if (!g.two_d) rvm = f.rv_m[x];
+      }
+      t2 = f.t2[x];
+      ttend = f.t_tend[x];
+      ts = f.t_save[x];
+      tsxp = f.t_save[x + 1];
+      tsxm = f.t_save[x - 1];
+      tsyp = f.t_save[x + sj];
+      tsym = f.t_save[x - sj];
+      if (ge1) tskm = f.t_save[x - sk];
+      if (k + 1 < nz) tskp = f.t_save[x + sk];
+      phs_p = f.ph_save[x + sk];
+      ph_p = f.ph2[x + sk];
+      c2a = f.c2a[x];
+      alt = f.alt[x];
+      pm1 = f.pm1[x];
+    }
+  }
+
+  // ---------------------------------------------------------------- 1: dvdxi
+  double dv = 0;
+  if (on) {
+    LDS(AA, k, ci) = ca;
+    LDS(AL, k, ci) = cal;
+    LDS(GA, k, ci) = cga;
+    if (mk) {
+      const double fx = (u1 + (c.c1h[k] * muu1 + c.c2h[k]) * us1 / msfuy1) - (u0 + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0);
       double d = c.rdx * fx;
-      if (!g.two_d)
-        d += c.rdy * ((f.v2[x + sj] + (c.c1h[k] * f.muv[x2 + sj] + c.c2h[k]) * f.v_save[x + sj] * f.msfvx_inv[x2 + sj]) -
-                      (f.v2[x] + (c.c1h[k] * f.muv[x2] + c.c2h[k]) * f.v_save[x] * f.msfvx_inv[x2]));
-      LDS(DV, k, ci) = f.msftx[x2] * f.msfty[x2] * d;
+      if (!g.two_d) d += c.rdy * ((v1 + (c.c1h[k] * muv1 + c.c2h[k]) * vs1 * mvi1) - (v0 + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0));
+      dv = msftx * msfty * d;
+      LDS(DV, k, ci) = c.dnw[k] * dv;
     }
   }
   __syncthreads();
-  // 2: the column mass and omega''
-  if (on && cp.kt == 0) {
+  // ---------------------------------------------------------------- 2: the column mass
+  if (on && k == 0) {
     double dmdt = 0;
-    for (int k = 0; k < nz; ++k) dmdt += c.dnw[k] * LDS(DV, k, ci);
-    const double mtend = f.mu_tend[x2];
-    const double mu_old = f.mu2[x2];
+    for (int kk = 0; kk < nz; ++kk) dmdt += LDS(DV, kk, ci);
     const double mu_new = mu_old + dts * (dmdt + mtend);
-    const double muts = f.mut[x2] + mu_new;
+    const double muts = mut + mu_new;
     const double muave = H * ((1.0 + epssm) * mu_new + (1.0 - epssm) * mu_old);
     f.mu2[x2] = mu_new;
     f.mudf[x2] = dmdt + mtend;
     f.muts[x2] = muts;
     f.muave[x2] = muave;
-    S2[ci] = mu_new;
-    S2[CI + ci] = muts;
-    S2[2 * CI + ci] = muave;
-    const double msfty = f.msfty[x2];
+    S2[ci] = dmdt;
+    S2[CI + ci] = mu_new;
+    S2[2 * CI + ci] = muts;
+    S2[3 * CI + ci] = muave;
+    S2[4 * CI + ci] = mtend;
+  }
+  __syncthreads();
+  const double dmdt = S2[ci], mu_new = S2[CI + ci], muts = S2[2 * CI + ci], muave = S2[3 * CI + ci];
+  // ---------------------------------------------------------------- 3: the omega'' terms
+  if (on && k + 1 < nz) {
+    const double mt = S2[4 * CI + ci];
+    LDS(TK, k, ci) = c.dnw[k] * (c.c1h[k] * dmdt + dv + c.c1h[k] * mt) / msfty;
+  }
+  __syncthreads();
+  // ---------------------------------------------------------------- 4: the running integral
+  if (on && k == 0) {
     double ww_full = 0;
-    LDS(WW, 0, ci) = 0;
-    for (int k = 0; k < nz; ++k) {
-      double wpp_hi = 0;
-      if (k + 1 < nz) {
-        ww_full = ww_full - c.dnw[k] * (c.c1h[k] * dmdt + LDS(DV, k, ci) + c.c1h[k] * mtend) / msfty;
-        wpp_hi = ww_full - LDS(PH, k + 1, ci);
-      }
-      LDS(WW, k + 1, ci) = wpp_hi;
+    for (int kk = 0; kk + 1 < nz; ++kk) {
+      ww_full = ww_full - LDS(TK, kk, ci);
+      LDS(TK, kk, ci) = ww_full;  // integral up to level kk + 1
     }
   }
   __syncthreads();
-  // 3: ww_m sums, ru_m / rv_m sums, theta''
+  // ---------------------------------------------------------------- 5: omega'', theta'', explicit phi''
+  double wwk = 0;
   if (on) {
-    const double msfty = f.msfty[x2], msftx = f.msftx[x2];
-    for (int k = cp.kt; k <= nz; k += kKT) {
-      const idx_t x = g.at(i, j, k);
-      {
-        const double wk = LDS(WW, k, ci);
-        f.ww[x] = wk;
-        const double sw = (first ? 0.0 : f.ww_m[x]) + wk;
-        f.ww_m[x] = last ? sw / rn + LDS(PH, k, ci) : sw;
-      }
-      if (k == nz) continue;
-      auto wdtn = [&](int kk) {
-        if (kk == 0 || kk == nz) return 0.0;
-        const idx_t y = g.at(i, j, kk);
-        return LDS(WW, kk, ci) * (c.fnm[kk] * f.t_save[y] + c.fnp[kk] * f.t_save[y - sk]);
-      };
-      const double su = (first ? 0.0 : f.ru_m[x]) + f.u2[x];
-      f.ru_m[x] = last ? su / rn + (c.c1h[k] * f.muu[x2] + c.c2h[k]) * f.u_save[x] / f.msfuy[x2] : su;
+    wwk = (k >= 1 && k < nz) ? LDS(TK, k - 1, ci) - ww1 : 0.0;
+    LDS(WW, k, ci) = wwk;
+  }
+  __syncthreads();
+  double t2new = 0;
+  if (on) {
+    f.ww[x] = wwk;
+    {
+      const double sw = (first ? 0.0 : wwm) + wwk;
+      f.ww_m[x] = last ? sw / rn + ww1 : sw;
+    }
+    if (mk) {
+      // wdtn(k) and wdtn(k+1): omega'' times theta at the w level (0 at the bottom and the top)
+      const double wdtn_k = (k == 0) ? 0.0 : wwk * (c.fnm[k] * ts + c.fnp[k] * tskm);
+      const double wdtn_kp = (k + 1 == nz) ? 0.0 : LDS(WW, k + 1, ci) * (c.fnm[k + 1] * tskp + c.fnp[k + 1] * ts);
+      const double su = (first ? 0.0 : rum) + u0;
+      f.ru_m[x] = last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su;
       if (!g.two_d) {
-        const double sv = (first ? 0.0 : f.rv_m[x]) + f.v2[x];
-        f.rv_m[x] = last ? sv / rn + (c.c1h[k] * f.muv[x2] + c.c2h[k]) * f.v_save[x] * f.msfvx_inv[x2] : sv;
+        const double sv = (first ? 0.0 : rvm) + v0;
+        f.rv_m[x] = last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv;
       }
-      const double tk = f.t2[x];
-      LDS(T2A, k, ci) = tk;  // t_2save, read back by this thread in phase 4
-      const double* ts = f.t_save;
-      double hflux = H * c.rdx * (f.u2[x + 1] * (ts[x + 1] + ts[x]) - f.u2[x] * (ts[x] + ts[x - 1]));
-      if (!g.two_d) hflux += H * c.rdy * (f.v2[x + sj] * (ts[x + sj] + ts[x]) - f.v2[x] * (ts[x] + ts[x - sj]));
-      const double tn = tk + msfty * dts * f.t_tend[x];
-      f.t2[x] = tn - dts * msfty * (msftx * hflux + c.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
+      double hflux = H * c.rdx * (u1 * (tsxp + ts) - u0 * (ts + tsxm));
+      if (!g.two_d) hflux += H * c.rdy * (v1 * (tsyp + ts) - v0 * (ts + tsym));
+      const double tn = t2 + msfty * dts * ttend;
+      t2new = tn - dts * msfty * (msftx * hflux + c.rdnw[k] * (wdtn_kp - wdtn_k));
+      f.t2[x] = t2new;
+      const double tav = H * ((1.0 + epssm) * t2new + (1.0 - epssm) * t2);
+      LDS(T2A, k, ci) = (tav + c.c1h[k] * muave * kT0) / ((c.c1h[k] * muts + c.c2h[k]) * (kT0 + ts));
     }
-  }
-  // 4: explicit phi'' and t_2ave (this thread's levels: t'' and t_2save are its own writes above)
-  const double mu_new = S2[ci], muts = S2[CI + ci], muave = S2[2 * CI + ci];
-  if (on) {
-    const double m = f.mut[x2], msfty = f.msfty[x2];
-    for (int k = cp.kt; k <= nz; k += kKT) {
-      const idx_t x = g.at(i, j, k);
-      double Rk;
-      if (k == 0) {
-        Rk = 0;
-      } else if (a.q.top_lid && k == nz) {
-        Rk = 0;
-      } else {
-        auto wdwn = [&](int mm) {
-          const idx_t y = g.at(i, j, mm);
-          return H * (LDS(WW, mm, ci) + LDS(WW, mm - 1, ci)) * c.rdnw[mm - 1] *
-                 (f.ph_save[y] - f.ph_save[y - sk] + f.phb(i, j, mm) - f.phb(i, j, mm - 1));
-        };
-        double rr = dts * (f.ph_tend[x] + H * kG * (1.0 - epssm) * f.w2[x]);
-        if (k < nz) rr -= dts * (c.fnm[k] * wdwn(k + 1) + c.fnp[k] * wdwn(k));
-        Rk = f.ph2[x] + msfty * rr / (c.c1f[k] * m + c.c2f[k]);
+    double Rk;
+    if (k == 0 || (a.q.top_lid && k == nz)) {
+      Rk = 0;
+    } else {
+      // wdwn(m): omega'' at the mass level times the layer's geopotential difference
+      const double wdwn_k = H * (wwk + LDS(WW, k - 1, ci)) * c.rdnw[k - 1] * (phs - phs_m + f.phb(i, j, k) - f.phb(i, j, k - 1));
+      double rr = dts * (phtend + H * kG * (1.0 - epssm) * w2);
+      if (mk) {
+        const double wdwn_kp = H * (LDS(WW, k + 1, ci) + wwk) * c.rdnw[k] * (phs_p - phs + f.phb(i, j, k + 1) - f.phb(i, j, k));
+        rr -= dts * (c.fnm[k] * wdwn_kp + c.fnp[k] * wdwn_k);
       }
-      LDS(R, k, ci) = Rk;
-      if (k < nz) {
-        const double tav = H * ((1.0 + epssm) * f.t2[x] + (1.0 - epssm) * LDS(T2A, k, ci));
-        LDS(T2A, k, ci) = (tav + c.c1h[k] * muave * kT0) / ((c.c1h[k] * muts + c.c2h[k]) * (kT0 + f.t_save[x]));
-      }
+      Rk = ph + msfty * rr / (c.c1f[k] * mut + c.c2f[k]);
     }
+    LDS(R, k, ci) = Rk;
   }
   __syncthreads();
-  // 5: explicit w'' (flat lower boundary: w''(0) = 0)
+  // ---------------------------------------------------------------- 6: explicit w''
   if (on) {
-    const double m = f.mut[x2], mav = muave;
+    const double m = mut, mav = muave;
     const double ope = 1.0 + epssm, ome = 1.0 - epssm;
-    const double msft_inv = 1.0 / f.msfty[x2];
-    auto mh = [&](int kk) { return c.c1h[kk] * m + c.c2h[kk]; };
-    for (int k = cp.kt; k <= nz; k += kKT) {
-      if (k == 0) {
-        LDS(W, 0, ci) = 0;
-        continue;
-      }
-      const idx_t x = g.at(i, j, k);
-      const double* ph = f.ph2;
-      const double r_k = LDS(R, k, ci), r_km1 = LDS(R, k - 1, ci), ph_k = ph[x], ph_km1 = ph[x - sk];
-      double wk;
-      if (k < nz) {
-        const double r_kp1 = LDS(R, k + 1, ci), ph_kp1 = ph[x + sk];
-        wk = f.w2[x] + dts * f.rw_tend[x] +
+    const double msft_inv = 1.0 / msfty;
+    double wk;
+    if (k == 0) {
+      wk = 0;
+    } else {
+      const double r_k = LDS(R, k, ci), r_km1 = LDS(R, k - 1, ci);
+      if (mk) {
+        const double r_kp1 = LDS(R, k + 1, ci);
+        wk = w2 + dts * rwt +
              msft_inv * 1.0 *
                  (H * dts * kG * c.rdn[k] *
-                  (f.c2a[x] * c.rdnw[k] / mh(k) * (ope * (r_kp1 - r_k) + ome * (ph_kp1 - ph_k)) -
-                   f.c2a[x - sk] * c.rdnw[k - 1] / mh(k - 1) * (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)))) +
-             dts * kG * msft_inv *
-                 (c.rdn[k] * (f.c2a[x] * f.alt[x] * LDS(T2A, k, ci) - f.c2a[x - sk] * f.alt[x - sk] * LDS(T2A, k - 1, ci)) -
-                  c.c1f[k] * mav);
+                  (c2a * c.rdnw[k] / (c.c1h[k] * m + c.c2h[k]) * (ope * (r_kp1 - r_k) + ome * (ph_p - ph)) -
+                   c2a_m * c.rdnw[k - 1] / (c.c1h[k - 1] * m + c.c2h[k - 1]) * (ope * (r_k - r_km1) + ome * (ph - ph_m)))) +
+             dts * kG * msft_inv * (c.rdn[k] * (c2a * alt * LDS(T2A, k, ci) - c2a_m * alt_m * LDS(T2A, k - 1, ci)) - c.c1f[k] * mav);
       } else {
         const int km = nz - 1;
-        const idx_t xm = x - sk;
-        wk = f.w2[x] + dts * f.rw_tend[x] +
-             msft_inv * (-H * dts * kG / mh(km) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * f.c2a[xm] * (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)) -
-                         dts * kG * (2.0 * c.rdnw[km] * f.c2a[xm] * f.alt[xm] * LDS(T2A, km, ci) + c.c1f[k] * mav));
+        wk = w2 + dts * rwt +
+             msft_inv * (-H * dts * kG / (c.c1h[km] * m + c.c2h[km]) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * c2a_m * (ope * (r_k - r_km1) + ome * (ph - ph_m)) -
+                         dts * kG * (2.0 * c.rdnw[km] * c2a_m * alt_m * LDS(T2A, km, ci) + c.c1f[k] * mav));
         if (a.q.top_lid) wk = 0;
       }
-      LDS(W, k, ci) = wk;
     }
+    LDS(W, k, ci) = wk;
   }
   __syncthreads();
-  // 6: tridiagonal sweeps
-  if (on && cp.kt == 0) {
+  // ---------------------------------------------------------------- 7: tridiagonal sweeps
+  if (on && k == 0) {
     double wprev = LDS(W, 0, ci);
-    for (int k = 1; k <= nz; ++k) {
-      wprev = (LDS(W, k, ci) - LDS(AA, k, ci) * wprev) * LDS(AL, k, ci);
-      LDS(W, k, ci) = wprev;
+    for (int kk = 1; kk <= nz; ++kk) {
+      wprev = (LDS(W, kk, ci) - LDS(AA, kk, ci) * wprev) * LDS(AL, kk, ci);
+      LDS(W, kk, ci) = wprev;
     }
     double wnext = LDS(W, nz, ci);
-    for (int k = nz - 1; k >= 1; --k) {
-      wnext = LDS(W, k, ci) - LDS(GA, k, ci) * wnext;
-      LDS(W, k, ci) = wnext;
+    for (int kk = nz - 1; kk >= 1; --kk) {
+      wnext = LDS(W, kk, ci) - LDS(GA, kk, ci) * wnext;
+      LDS(W, kk, ci) = wnext;
     }
   }
   __syncthreads();
-  // 7: damping near the top, phi'' from the implicit w''
+  // ---------------------------------------------------------------- 8: damping near the top, phi''
   if (on) {
-    const double m = f.mut[x2], ope = 1.0 + epssm, msfty = f.msfty[x2];
-    const double dampmag = a.q.damp_opt == 3 ? dts * a.q.dampcoef : 0.0;
-    const double hdepth = a.q.zdamp;
-    const double htop = (f.ph_save[g.at(i, j, nz)] + f.phb(i, j, nz)) / kG, hbot = htop - hdepth;
-    for (int k = cp.kt; k <= nz; k += kKT) {
-      const idx_t x = g.at(i, j, k);
-      double wk = LDS(W, k, ci);
-      if (k == 0) {
-        f.w2[x] = wk;
-        LDS(PH, 0, ci) = f.ph2[x];
-        continue;
-      }
+    double wk = LDS(W, k, ci);
+    double phk = ph;  // the surface phi'' is not solved for
+    if (k >= 1) {
+      const double dampmag = a.q.damp_opt == 3 ? dts * a.q.dampcoef : 0.0;
       if (dampmag != 0.0) {
+        const double hdepth = a.q.zdamp;
         const double pi = 3.141592653589793;  // 4 atan(1), folded
-        const double hk = (f.ph_save[x] + f.phb(i, j, k)) / kG;
+        const double htop = (ph_top + f.phb(i, j, nz)) / kG, hbot = htop - hdepth;
+        const double hk = (phs + f.phb(i, j, k)) / kG;
         double dampwt = 0;
         if (hk >= hbot) {
           const double sn = mi_libm::sin(H * pi * (hk - hbot) / hdepth);
           dampwt = dampmag * sn * sn;
         }
-        wk = (wk - dampwt * (c.c1f[k] * m + c.c2f[k]) * f.w_save[x]) / (1.0 + dampwt);
+        wk = (wk - dampwt * (c.c1f[k] * mut + c.c2f[k]) * wsv) / (1.0 + dampwt);
       }
-      f.w2[x] = wk;
-      const double phk = LDS(R, k, ci) + msfty * H * dts * kG * ope * wk / (c.c1f[k] * muts + c.c2f[k]);
+      phk = LDS(R, k, ci) + msfty * H * dts * kG * (1.0 + epssm) * wk / (c.c1f[k] * muts + c.c2f[k]);
       f.ph2[x] = phk;
-      LDS(PH, k, ci) = phk;
     }
+    f.w2[x] = wk;
+    LDS(PH, k, ci) = phk;
   }
   __syncthreads();
-  // 8: p'', al'' with divergence damping
-  if (on) {
-    for (int k = cp.kt; k < nz; k += kKT)
-      p_rho_at(a, i, j, k, LDS(PH, k, ci), LDS(PH, k + 1, ci), mu_new, muts, f.t2[g.at(i, j, k)], true);
+  // ---------------------------------------------------------------- 9: p'', al'' with divergence damping
+  if (on && mk) {
+    const double m = c.c1h[k] * muts + c.c2h[k];
+    const double al = -((1. / m) * (alt * (c.c1h[k] * mu_new) + c.rdnw[k] * (LDS(PH, k + 1, ci) - LDS(PH, k, ci))));
+    f.al[x] = al;
+    double pp = c2a * (alt * (t2new - c.c1h[k] * mu_new * ts) / (m * (kT0 + ts)) - al);
+    const double ptmp = pp;
+    pp = pp + a.q.smdiv * (pp - pm1);
+    f.pm1[x] = ptmp;
+    f.p[x] = pp;
   }
 }
 
@@ -1072,15 +1191,22 @@ MI_COL_LAUNCHER(AcousticL, k_acoustic)
 MI_COL_LAUNCHER(FinishL, k_finish)
 
 // LDS arrays (per level, extra per column) of each column kernel, in ColKernel order
-constexpr int kColArrays[kNumColKernels][2] = {{1, 0}, {3, 0}, {9, 3}, {1, 1}};
+constexpr int kColArrays[kNumColKernels][2] = {{1, 0}, {3, 0}, {10, 5}, {1, 1}};
 
 bool col_cfg_fits(int kernel, int cfg, int nz) {
   if (cfg < 0 || cfg >= kNumColCfgs) return false;
+  if (kernel == kColAcoustic && kColCfgs[cfg][1] < nz + 1) return false;  // one level per thread
   return col_lds_bytes(kColCfgs[cfg][0], kColArrays[kernel][0], nz, kColArrays[kernel][1]) <= kLdsMax;
 }
 
 int col_cfg_default(int kernel, int nx, int ny, int nz) {
-  // small domains: one level per thread (latency); larger: 16 columns (full 128-byte rows)
+  // the substep kernel: the narrowest one-level-per-thread shape; the others: one level per thread on
+  // small domains (latency), 16 columns (full 128-byte rows) on larger ones
+  if (kernel == kColAcoustic) {
+    for (int c : {4, 5, 6})
+      if (col_cfg_fits(kernel, c, nz)) return c;
+    return 6;
+  }
   const bool small = int64_t(nx) * ny <= 16384;
   const int pref[] = {small ? (nz + 1 <= 64 ? 4 : (nz + 1 <= 128 ? 5 : 6)) : 0, 3, 5, 6};
   for (int c : pref)
@@ -1115,7 +1241,7 @@ void launch_uv(const Args& a, double dts, hipStream_t st) {
   launch3("uv", k_uv, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, dts);
 }
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
-  dispatch_cfg<AcousticL>(cfg, "acoustic", 9, 3, a.g.nx, a.g.ny, st, a, dts, it, ns);
+  dispatch_cfg<AcousticL>(cfg, "acoustic", 10, 5, a.g.nx, a.g.ny, st, a, dts, it, ns);
 }
 void launch_finish(const Args& a, int cfg, bool prhophi, hipStream_t st) {
   dispatch_cfg<FinishL>(cfg, "finish", 1, 1, a.g.nx + 1, a.g.nyv, st, a, prhophi ? 1 : 0);

@@ -110,7 +110,10 @@ def test_every_block_shape_bitwise(ctx, shape):
     prm = case.params(time_step_sound=6, damp_opt=3, zdamp=3000.0)
     print(f"[parity] block shape {shape}", flush=True)
     gpu = RK3(ctx, case.grid, prm, case)
-    gpu.tuning = [shape] * 4
+    # the substep kernel needs a thread per level (16 + 1 here): shapes with fewer keep its default
+    acoustic = shape if shape not in (0, 2) else gpu.tuning[2]
+    gpu.tuning = [shape, shape, acoustic, shape]
+    assert gpu.tuning == [shape, shape, acoustic, shape]
     orc = OracleDomain(case, prm)
     gpu.step(3)
     orc.step(3)
