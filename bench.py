@@ -9,9 +9,13 @@ built on the host.  A step is one RK3 large step (three stages, ten acoustic sub
 is K steps with the state already resident in HBM, bracketed by device synchronisations.
 
 Prints one JSON line on stdout with the throughput in grid-point-steps/s (nx*ny*nz*steps/s), the
-roofline of the dominant kernel (HIP events around its launches during the timed steps; algorithmic
-bytes = distinct 3-D arrays it reads and writes x 8 B x points) and the CPU baseline (the C++/OpenMP
-oracle on this host's cores, a bounded sample of the same workload).
+roofline of the dominant kernel and the CPU baseline (the C++/OpenMP oracle on this host's cores, a
+bounded sample of the same workload).
+
+The timed region replays the step's HIP graph K times.  The dominant kernel's launch duration comes from
+HIP events recorded on the domain's stream around each of its launches during K further steps run
+launch by launch (events cannot be read back from inside a replayed graph); its algorithmic bytes are
+the distinct arrays it reads and writes x 8 B x the points of one launch.
 """
 from __future__ import annotations
 
@@ -78,6 +82,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default block shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each forecast domain runs on a "
@@ -90,6 +95,8 @@ def main():
     g = case.grid
     ctx = Context(0)
     rk = RK3(ctx, g, prm, case)
+    if not args.no_autotune:
+        log("autotuned block shapes:", rk.autotune(1))  # steps the domain, then restores its state
     rk.step(args.warmup)
     rk.sync()
     # the dominant kernel: event time per step of each candidate over two untimed steps
@@ -100,25 +107,21 @@ def main():
     dominant = max(per_kernel, key=per_kernel.get)
     log("kernel ms/step:", json.dumps({k: round(v, 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1])}))
 
-    # timed region: K steps, the dominant kernel's launches bracketed by events on the domain's stream
-    import ctypes as C
-    from cppwrf_amd._abi import check, lib
-    check(lib().mi_profile_kernel(rk.handle, dominant.encode()))
+    # timed region: K graph-replayed steps, bracketed by device synchronisations
     rk.sync()
     t0 = time.perf_counter()
     rk.step(args.steps)
     rk.sync()
     t1 = time.perf_counter()
-    ms, nl, b = C.c_double(), C.c_int(), C.c_double()
-    check(lib().mi_profile_read(rk.handle, C.byref(ms), C.byref(nl), C.byref(b)))
-    check(lib().mi_profile_kernel(rk.handle, None))
+    # the dominant kernel over K more steps, its launches bracketed by events on the domain's stream
+    ev_ms, nl, b = profile(rk, dominant, args.steps)
     bad = rk.check_finite()
     if bad != -1:
         raise SystemExit(f"non-finite values in field {bad}")
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = g.points * args.steps / (t1 - t0)
-    avg_launch_s = ms.value / 1e3 / nl.value
-    achieved = b.value / nl.value / avg_launch_s / 1e9
+    avg_launch_s = ev_ms / 1e3 / nl
+    achieved = b / nl / avg_launch_s / 1e9
 
     out = {
         "metric": METRIC,
@@ -133,14 +136,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "dry warm bubble 64x64x40 (configs[0] recipe), plain FP64, bit-identical to the "
+        "config": {"workload": "dry warm bubble 64x64x40, 1 km, 20 km top, plain FP64, bit-identical to the "
                                "reference CPU FP64 build with scalar glibc pow",
                    "nx": g.nx, "ny": g.ny, "nz": g.nz, "dx_m": g.dx, "dt_s": g.dt, "time_step_sound": rk.n_sound,
                    "damp_opt": 3, "h_adv_order": 5, "v_adv_order": 3, "precision": "fp64"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": b.value / nl.value, "avg_launch_us": avg_launch_s * 1e6,
-                     "launches": nl.value},
+                     "alg_bytes_per_launch": b / nl, "avg_launch_us": avg_launch_s * 1e6,
+                     "launches": nl},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, prm)

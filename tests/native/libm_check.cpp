@@ -1,0 +1,43 @@
+// Compares the glibc pow/sin ports of cppwrf_amd/csrc/libm_port.hpp (host build of the same source the
+// device compiles) with the host glibc on seeded samples over the ranges the dynamics path uses:
+// pow(x, cp/cv) for calc_p_rho_phi, pow(x, y) on a wide range, sin on [0, 2.4262] for the Rayleigh layer.
+// Exit status 0 when every result is bit-identical.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "../../cppwrf_amd/csrc/libm_port.hpp"
+
+static uint64_t state = 0x5eed0123456789ULL;
+static uint64_t splitmix64() {
+  uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+static double u01() { return double(splitmix64() >> 11) * 0x1p-53; }
+static bool same(double a, double b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+
+int main(int argc, char** argv) {
+  const long n = argc > 1 ? std::atol(argv[1]) : 1000000;
+  const double cpovcv = 1004.5 / (1004.5 - 287.0);
+  long bad[3] = {0, 0, 0};
+  for (long t = 0; t < n; ++t) {
+    double x = 0.2 + 3.0 * u01();
+    double a = std::pow(x, cpovcv), b = mi_libm::pow(x, cpovcv);
+    if (!same(a, b) && bad[0]++ < 3) std::printf("pow(%a, cp/cv): glibc %a port %a\n", x, a, b);
+    x = 0.01 + 10.0 * u01();
+    const double y = -5.0 + 10.0 * u01();
+    a = std::pow(x, y);
+    b = mi_libm::pow(x, y);
+    if (!same(a, b) && bad[1]++ < 3) std::printf("pow(%a, %a): glibc %a port %a\n", x, y, a, b);
+    const double z = 2.4262 * u01();
+    a = std::sin(z);
+    b = mi_libm::sin(z);
+    if (!same(a, b) && bad[2]++ < 3) std::printf("sin(%a): glibc %a port %a\n", z, a, b);
+  }
+  std::printf("samples %ld, mismatches: pow(x, cp/cv) %ld, pow(x, y) %ld, sin %ld\n", n, bad[0], bad[1], bad[2]);
+  return bad[0] + bad[1] + bad[2] != 0;
+}
