@@ -27,6 +27,10 @@ MI_WW, MI_RU_M, MI_RV_M, MI_WW_M, MI_ALT, MI_PHP, MI_MUTS = range(8, 15)
 MI_RU_TEND, MI_RV_TEND, MI_RW_TEND, MI_T_TEND, MI_PH_TEND = range(15, 20)
 MI_MUB, MI_PB, MI_ALB, MI_PHB, MI_T_INIT = range(32, 37)
 MI_MSFTX, MI_MSFTY, MI_MSFUX, MI_MSFUY, MI_MSFVX, MI_MSFVY, MI_MSFVX_INV = range(40, 47)
+MI_CQU, MI_CQV, MI_CQW, MI_H_DIABATIC = range(20, 24)
+MI_Q, MI_Q_TENDF = 64, 96
+MI_MAX_MOIST, MI_MAX_SCALARS = 8, 16
+ABI_VERSION = 2
 
 MI_CASE_REST_STATE, MI_CASE_WARM_BUBBLE, MI_CASE_IGW, MI_CASE_BUBBLE3D = range(4)
 
@@ -36,14 +40,24 @@ STAGGER = {
     MI_WW: "W", MI_RU_M: "U", MI_RV_M: "V", MI_WW_M: "W", MI_ALT: "M", MI_PHP: "M", MI_MUTS: "M2",
     MI_RU_TEND: "U", MI_RV_TEND: "V", MI_RW_TEND: "W", MI_T_TEND: "M", MI_PH_TEND: "W",
     MI_MUB: "M2", MI_PB: "M", MI_ALB: "M", MI_PHB: "W", MI_T_INIT: "M",
+    MI_CQU: "U", MI_CQV: "V", MI_CQW: "W", MI_H_DIABATIC: "M",
+    MI_MSFTX: "M2", MI_MSFTY: "M2", MI_MSFUX: "U2", MI_MSFUY: "U2", MI_MSFVX: "V2", MI_MSFVY: "V2",
+    MI_MSFVX_INV: "V2",
 }
+for _n in range(MI_MAX_SCALARS):
+    STAGGER[MI_Q + _n] = "M"
+    STAGGER[MI_Q_TENDF + _n] = "M"
 
 FIELD_NAMES = {
     MI_U: "u", MI_V: "v", MI_W: "w", MI_T: "t", MI_PH: "ph", MI_MU: "mu", MI_P: "p", MI_AL: "al",
     MI_WW: "ww", MI_RU_M: "ru_m", MI_RV_M: "rv_m", MI_WW_M: "ww_m", MI_ALT: "alt", MI_PHP: "php",
     MI_MUTS: "muts", MI_RU_TEND: "ru_tend", MI_RV_TEND: "rv_tend", MI_RW_TEND: "rw_tend",
-    MI_T_TEND: "t_tend", MI_PH_TEND: "ph_tend",
+    MI_T_TEND: "t_tend", MI_PH_TEND: "ph_tend", MI_CQU: "cqu", MI_CQV: "cqv", MI_CQW: "cqw",
+    MI_H_DIABATIC: "h_diabatic",
 }
+for _n in range(MI_MAX_SCALARS):
+    FIELD_NAMES[MI_Q + _n] = f"q{_n}"
+    FIELD_NAMES[MI_Q_TENDF + _n] = f"q{_n}_tendf"
 
 
 def field_shape(fid: int, nx: int, ny: int, nz: int) -> tuple[int, int, int]:
@@ -51,6 +65,10 @@ def field_shape(fid: int, nx: int, ny: int, nz: int) -> tuple[int, int, int]:
     st = STAGGER[fid]
     if st == "M2":
         return (1, ny, nx)
+    if st == "U2":
+        return (1, ny, nx + 1)
+    if st == "V2":
+        return (1, ny + (1 if ny > 1 else 0), nx)
     nxs = nx + (1 if st == "U" else 0)
     nys = ny + (1 if st == "V" and ny > 1 else 0)
     nzs = nz + (1 if st == "W" else 0)
@@ -67,7 +85,8 @@ class MiParams(C.Structure):
                 ("w_damping", C.c_int32), ("w_crit_cfl", C.c_double), ("epssm", C.c_double),
                 ("smdiv", C.c_double), ("emdiv", C.c_double), ("damp_opt", C.c_int32),
                 ("hypsometric_opt", C.c_int32), ("zdamp", C.c_double), ("dampcoef", C.c_double),
-                ("time_step_sound", C.c_int32), ("precision", C.c_int32)]
+                ("time_step_sound", C.c_int32), ("precision", C.c_int32), ("n_moist", C.c_int32),
+                ("n_scalars", C.c_int32), ("moist_adv_opt", C.c_int32), ("scalar_adv_opt", C.c_int32)]
 
 
 _PD = C.POINTER(C.c_double)
@@ -89,7 +108,7 @@ def default_params(**kw) -> MiParams:
     """The reference's defaults (src/cases/model.cpp:256-281)."""
     p = MiParams(h_adv_order=5, v_adv_order=3, top_lid=0, w_damping=0, w_crit_cfl=1.0, epssm=0.1, smdiv=0.1,
                  emdiv=0.01, damp_opt=0, hypsometric_opt=1, zdamp=5000.0, dampcoef=0.2, time_step_sound=0,
-                 precision=MI_FP64)
+                 precision=MI_FP64, n_moist=0, n_scalars=0, moist_adv_opt=1, scalar_adv_opt=1)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -125,6 +144,7 @@ SIGNATURES = {
     "mi_rk_prep": (_I, [_VP, _I]),
     "mi_rk_tendency": (_I, [_VP, _I]),
     "mi_small_steps": (_I, [_VP, _I]),
+    "mi_scalar_transport": (_I, [_VP, _I]),
     "mi_calc_p_rho_phi": (_I, [_VP, _I]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
     "mi_domain_autotune": (_I, [_VP, _I]),
@@ -151,7 +171,7 @@ def lib():
         L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
         for name, (res, args) in SIGNATURES.items():
             _proto(L, name, res, args)
-        if L.mi_abi_version() != 1:
+        if L.mi_abi_version() != ABI_VERSION:
             raise ImportError("libcppwrf_amd.so ABI version mismatch: rebuild it")
         _lib = L
     return _lib

@@ -72,4 +72,22 @@ __device__ __forceinline__ double flux3(double qm2, double qm1, double q0, doubl
   return (7.0 * (q0 + qm1) - (qp1 + qm2)) / 12.0 + up * sgn1(ua) * ((qp1 - qm2) - 3.0 * (q0 - qm1)) / 12.0;
 }
 
+// advect_scalar_pd's own statement functions (kernels_common.hpp:23-33): folded constants, so their
+// roundoff differs from flux5 / flux3
+__device__ __forceinline__ double flux5_pd(double qm3, double qm2, double qm1, double q0, double qp1, double qp2,
+                                           double ua, double up) {
+  const double f6 = (37. / 60.) * (q0 + qm1) - (2. / 15.) * (qp1 + qm2) + (1. / 60.) * (qp2 + qm3);
+  return f6 - up * sgn1(ua) * (1. / 60.) * ((qp2 - qm3) - 5. * (qp1 - qm2) + 10. * (q0 - qm1));
+}
+__device__ __forceinline__ double flux3_pd(double qm2, double qm1, double q0, double qp1, double ua, double up) {
+  const double f4 = (7. / 12.) * (q0 + qm1) - (1. / 12.) * (qp1 + qm2);
+  return f4 + up * sgn1(ua) * (1. / 12.) * ((qp1 - qm2) - 3. * (q0 - qm1));
+}
+// first-order upwind flux with Courant number cr (kernels_common.hpp:83-88)
+__device__ __forceinline__ double flux_upwind(double qm1, double q0, double cr) {
+  const double a = cr < 0.0 ? -cr : cr;
+  const double lo = cr + a, hi = cr - a;
+  return 0.5 * (lo < 1.0 ? lo : 1.0) * qm1 + 0.5 * (hi > -1.0 ? hi : -1.0) * q0;
+}
+
 }  // namespace mi_dev

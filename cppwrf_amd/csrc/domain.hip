@@ -62,8 +62,12 @@ struct mi_domain {
   std::vector<FieldDesc> fields;
   double *pb_buf = nullptr, *alb_buf = nullptr, *phb_buf = nullptr, *t_init_buf = nullptr;
   int* flags = nullptr;  // device finiteness flags
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // the scalars and their buffers; qpar flips with every step (three stage swaps)
+  mi_fp64::Scalars sc{};
+  int n_sc = 0, qpar = 0;
+  // one graph per scalar-buffer parity: the step's launches name the buffers (none without scalars)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   bool use_graph = true;
   std::string prof_name;
   mi_fp64::KernelTimer timer{};
@@ -90,9 +94,16 @@ struct mi_domain {
 
   ~mi_domain() {
     for (hipEvent_t e : timer.ev) (void)hipEventDestroy(e);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graphs();
     for (void* p : allocs) (void)hipFree(p);
+  }
+  void drop_graphs() {
+    for (int q = 0; q < 2; ++q) {
+      if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
+      if (graph[q]) (void)hipGraphDestroy(graph[q]);
+      exec[q] = nullptr;
+      graph[q] = nullptr;
+    }
   }
 
   double* alloc(int64_t n) {
@@ -149,16 +160,20 @@ struct mi_domain {
   // ------------------------------------------------------------------ the phases of a step
   // RK3::step's _2 -> _1 copies (rk3.cpp:94-98).  The copies include the halos, which are fresh, so
   // the reference's refresh of mu_1's halo rewrites the values already there and is left out.
-  void step_begin() { mi_fp64::launch_copy_step(a, n3, n2, ctx->stream); }
+  void step_begin() {
+    mi_fp64::launch_copy_step(a, n3, n2, ctx->stream);
+    mi_fp64::launch_copy_scalars(a, sc, n_sc, n3, ctx->stream);  // q_1 = q_2 (rk3.cpp:99)
+  }
 
   // LargeStep::prep (large_step.cpp:697-733).  Its first halo refresh (u, v, w, t, ph, p, al, mu of the
   // stage) is left out: every path that changes those fields (the end of a stage, mi_domain_set_field)
-  // refreshes their halos itself, so at this point it would rewrite identical values.
+  // refreshes their halos itself, so at this point it would rewrite identical values.  The refresh of
+  // its outputs (mut, ru, rv, rw, alt, php, muu, muv, ww; large_step.cpp:718-720) is written by the
+  // kernel itself: each output point stores its halo images.  The same holds for every refresh of the
+  // small steps below.
   void rk_prep() {
-    auto& f = a.f;
     mi_fp64::launch_large_prep(a, colcfg[mi_fp64::kColLargePrep], ctx->stream);
-    halo({{f.mut, -1, -1}, {f.ru, -1, -1}, {f.rv, -1, -1}, {f.rw, -1, -1}, {f.alt, -1, -1}, {f.php, -1, -1},
-          {f.muu, 2, 0}, {f.muv, 0, 1}, {f.ww, -1, -1}});
+    if (a.q.moist) mi_fp64::launch_cq(a, sc, ctx->stream);  // calc_cq (large_step.cpp:724-733)
   }
 
   // LargeStep::tendencies (large_step.cpp:798-861)
@@ -176,33 +191,40 @@ struct mi_domain {
     if (rk == 1) mi_fp64::launch_fill(f.mudf - off, n2, 0.0, st);
     mi_fp64::launch_ss_prep(a, rk == 1, st);
     const double cof_h = 0.5 * dts * kG * (1.0 + prm.epssm);
+    // halos of u'', v'', ph'', t_save, u_save, v_save, mu'', muts, p, al (small_step.cpp:526-528): images
+    // written by the two kernels above
     mi_fp64::launch_ss_column(a, colcfg[mi_fp64::kColSsColumn], cof_h * cof_h, st);
-    halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.ph2, -1, -1}, {f.t_save, -1, -1}, {f.u_save, -1, -1}, {f.v_save, -1, -1},
-          {f.mu2, -1, -1}, {f.muts, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
     for (int it = 1; it <= ns; ++it) {
-      mi_fp64::launch_uv(a, dts, st);
-      if (grid.ny > 1) halo({{f.u2, -1, -1}, {f.v2, -1, -1}});
-      else halo({{f.u2, -1, -1}});
+      mi_fp64::launch_uv(a, dts, st);  // with the halos of u'', v''
+      // with the halos of mu'', mudf, muts, ph'', p, al, and of ru_m, rv_m, ww_m (their refresh after
+      // the loop; the sums' halo images are rewritten at every substep, the last one's stay)
       mi_fp64::launch_acoustic(a, colcfg[mi_fp64::kColAcoustic], dts, it, ns, st);
-      if (it < ns)
-        halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
-      else  // the reference's refresh of muts, ru_m, rv_m, ww_m after the loop joins the last one
-        halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1},
-              {f.ru_m, -1, -1}, {f.rv_m, -1, -1}, {f.ww_m, -1, -1}});
     }
-    mi_fp64::launch_finish(a, colcfg[mi_fp64::kColFinish], prhophi, st);
-    if (prhophi)
-      halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.w2, -1, -1}, {f.t2, -1, -1}, {f.ph2, -1, -1}, {f.ww, -1, -1}, {f.mu2, -1, -1},
-            {f.p, -1, -1}, {f.al, -1, -1}});
-    else
-      halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.w2, -1, -1}, {f.t2, -1, -1}, {f.ph2, -1, -1}, {f.ww, -1, -1}, {f.mu2, -1, -1}});
+    // with the halos of u, v, w, t, ph, ww, mu (and p, al); the microphysics heating leaves theta_m at
+    // the last stage with moisture (small_step.cpp:602)
+    const double hd_dt = (rk == 3 && a.q.moist) ? dts * double(ns) : 0.0;
+    mi_fp64::launch_finish(a, colcfg[mi_fp64::kColFinish], prhophi, hd_dt, st);
+  }
+
+  // ScalarTransport::run_stage (scalars.cpp:436-461): the plain species, then the positive-definite ones
+  // (last stage, adv_opt 1); each species' new value becomes its q_2 (the buffers swap), halos included
+  void scalar_transport(int rk) {
+    if (n_sc == 0) return;
+    const double dt_rk = rk == 1 ? grid.dt / 3.0 : (rk == 2 ? 0.5 * grid.dt : grid.dt);
+    int plain[MI_MAX_SCALARS], pd[MI_MAX_SCALARS], np = 0, nd = 0;
+    for (int n = 0; n < n_sc; ++n) {
+      const bool is_pd = rk == 3 && (n < prm.n_moist ? prm.moist_adv_opt : prm.scalar_adv_opt) == 1;
+      if (is_pd) pd[nd++] = n;
+      else plain[np++] = n;
+    }
+    mi_fp64::launch_scalars(a, sc, plain, np, false, dt_rk, ctx->stream);
+    mi_fp64::launch_scalars(a, sc, pd, nd, true, dt_rk, ctx->stream);
+    for (int n = 0; n < n_sc; ++n) std::swap(sc.q2[n], sc.qn[n]);
+    qpar ^= 1;
   }
 
   // SmallStep::calc_p_rho_phi (small_step.cpp:609-617), for the phase-by-phase entry points
-  void calc_p_rho_phi() {
-    mi_fp64::launch_p_rho_phi(a, ctx->stream);
-    halo({{a.f.p, -1, -1}, {a.f.al, -1, -1}});
-  }
+  void calc_p_rho_phi() { mi_fp64::launch_p_rho_phi(a, ctx->stream); }  // with the halos of p, al
 
   void step_launches() {
     step_begin();
@@ -210,12 +232,26 @@ struct mi_domain {
       rk_prep();
       rk_tendency();
       small_steps(rk, true);
+      scalar_transport(rk);
     }
   }
 
   // the fields a step reads from the previous one (everything else is recomputed within a step)
-  std::vector<double*> carried() { return {a.f.u2, a.f.v2, a.f.w2, a.f.t2, a.f.ph2, a.f.mu2, a.f.p, a.f.al}; }
+  std::vector<double*> carried() {
+    std::vector<double*> c = {a.f.u2, a.f.v2, a.f.w2, a.f.t2, a.f.ph2, a.f.mu2, a.f.p, a.f.al};
+    for (int n = 0; n < n_sc; ++n) c.push_back(sc.q2[n]);
+    return c;
+  }
+  mi_fp64::Scalars sc_saved{};
+  int qpar_saved = 0;
   void save_or_restore(bool save) {
+    if (save) {
+      sc_saved = sc;
+      qpar_saved = qpar;
+    } else {  // the buffers as they were named at the save
+      sc = sc_saved;
+      qpar = qpar_saved;
+    }
     std::vector<double*> c = carried();
     if (backup.empty())
       for (double* p : c) backup.push_back(alloc(p == a.f.mu2 ? n2 : n3));
@@ -263,12 +299,7 @@ struct mi_domain {
     timer.bytes = 0;
     save_or_restore(false);
     hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    if (exec) {  // the graph holds the old shapes
-      (void)hipGraphExecDestroy(exec);
-      (void)hipGraphDestroy(graph);
-      exec = nullptr;
-      graph = nullptr;
-    }
+    drop_graphs();  // the graphs hold the old shapes
   }
 
   void step(int nsteps) {
@@ -278,15 +309,28 @@ struct mi_domain {
       hip_check(hipGetLastError(), "kernel launch");
       return;
     }
-    if (!exec) {
-      hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-      step_launches();
-      hipGraph_t gr = nullptr;
-      hip_check(hipStreamEndCapture(ctx->stream, &gr), "hipStreamEndCapture");
-      graph = gr;
-      hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    for (int s = 0; s < nsteps; ++s) {
+      const int par = qpar;
+      if (!exec[par]) {
+        // capturing runs the host side of step_launches (the buffer swaps) without running the step, so
+        // the host state is put back before the launch below performs the step
+        const mi_fp64::Scalars keep = sc;
+        hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        step_launches();
+        hipGraph_t gr = nullptr;
+        hip_check(hipStreamEndCapture(ctx->stream, &gr), "hipStreamEndCapture");
+        graph[par] = gr;
+        hip_check(hipGraphInstantiate(&exec[par], gr, nullptr, nullptr, 0), "hipGraphInstantiate");
+        sc = keep;
+        qpar = par;
+      }
+      hip_check(hipGraphLaunch(exec[par], ctx->stream), "hipGraphLaunch");
+      // the host side of the step: three stage swaps of every species' buffers
+      for (int r = 0; r < (n_sc > 0 ? 3 : 0); ++r) {
+        for (int n = 0; n < n_sc; ++n) std::swap(sc.q2[n], sc.qn[n]);
+        qpar ^= 1;
+      }
     }
-    for (int s = 0; s < nsteps; ++s) hip_check(hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
   }
 };
 
@@ -298,6 +342,9 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   if (g.bc_y != MI_BC_PERIODIC && g.bc_y != MI_BC_SYMMETRIC) throw Error(MI_EINVAL, "bc_y: periodic or symmetric");
   if (g.bc_x == MI_BC_PERIODIC && g.nx < 3) throw Error(MI_EINVAL, "periodic x needs nx >= 3");
   if (g.ny > 1 && g.bc_y == MI_BC_PERIODIC && g.ny < 3) throw Error(MI_EINVAL, "periodic y needs ny >= 3");
+  // a symmetric fill with fewer than 4 points mirrors halo slots into one another (order-dependent)
+  if (g.bc_x == MI_BC_SYMMETRIC && g.nx < 4) throw Error(MI_EINVAL, "symmetric x needs nx >= 4");
+  if (g.ny > 1 && g.bc_y == MI_BC_SYMMETRIC && g.ny < 4) throw Error(MI_EINVAL, "symmetric y needs ny >= 4");
   if (!(g.dx > 0) || !(g.dt > 0) || (g.ny > 1 && !(g.dy > 0))) throw Error(MI_EINVAL, "grid: dx, dy, dt must be > 0");
   if (p.h_adv_order != 5 && p.h_adv_order != 6) throw Error(MI_EINVAL, "h_adv_order must be 5 or 6");
   if (p.v_adv_order != 3 && p.v_adv_order != 4) throw Error(MI_EINVAL, "v_adv_order must be 3 or 4");
@@ -305,6 +352,10 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   if (p.hypsometric_opt != 1) throw Error(MI_EINVAL, "hypsometric_opt must be 1");
   if (p.precision != MI_FP64) throw Error(MI_EINVAL, "precision: only MI_FP64 is available");
   if (p.damp_opt == 3 && !(p.zdamp > 0)) throw Error(MI_EINVAL, "zdamp must be > 0");
+  if (p.n_moist < 0 || p.n_moist > MI_MAX_MOIST) throw Error(MI_EINVAL, "n_moist must be in 0..8");
+  if (p.n_scalars < p.n_moist || p.n_scalars > MI_MAX_SCALARS) throw Error(MI_EINVAL, "n_scalars must be in n_moist..16");
+  if ((p.moist_adv_opt != 0 && p.moist_adv_opt != 1) || (p.scalar_adv_opt != 0 && p.scalar_adv_opt != 1))
+    throw Error(MI_EINVAL, "moist_adv_opt and scalar_adv_opt must be 0 or 1");
   d.ctx = ctx;
   d.grid = g;
   d.prm = p;
@@ -366,6 +417,20 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   add2(f.msfvx, St::V2, MI_MSFVX); add2(f.msfvy, St::V2, MI_MSFVY); add2(f.msfvx_inv, St::V2, MI_MSFVX_INV);
   for (double* m : {f.msftx, f.msfty, f.msfux, f.msfuy, f.msfvx, f.msfvy, f.msfvx_inv})
     mi_fp64::launch_fill(m - d.off, d.n2, 1.0, ctx->stream);
+  // moisture and scalars
+  if (p.n_moist > 0) {
+    add3(f.cqu, St::U, MI_CQU); add3(f.cqv, St::V, MI_CQV); add3(f.cqw, St::W, MI_CQW);
+    add3(f.hdia, St::M, MI_H_DIABATIC);
+  }
+  d.n_sc = p.n_scalars;
+  d.sc.n_moist = p.n_moist;
+  for (int n = 0; n < d.n_sc; ++n) {
+    add3(d.sc.q1[n], St::M, -1);
+    add3(d.sc.q2[n], St::M, MI_Q + n);
+    add3(d.sc.qn[n], St::M, -1);
+    add3(d.sc.qtf[n], St::M, MI_Q_TENDF + n);
+    add3(d.sc.scale[n], St::M, -1);
+  }
   // base 3-D fields start as profiles (uniform base state)
   const int nz = g.nz;
   d.pb_buf = d.alloc(nz + 1);
@@ -413,6 +478,7 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   q.zdamp = p.zdamp;
   q.top_lid = p.top_lid;
   q.damp_opt = p.damp_opt;
+  q.moist = p.n_moist > 0 ? 1 : 0;
 }
 
 FieldDesc* find(mi_domain* d, int id) {
@@ -502,7 +568,7 @@ int mi_domain_set_field(mi_domain* d, int id, const double* host, int64_t n) {
     }
     FieldDesc* fd = find(d, id);
     if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " cannot be set");
-    const bool base2 = id >= MI_MUB;
+    const bool base2 = id >= MI_MUB && id <= MI_MSFVX_INV;
     const int nxs = d->nxs(fd->st), nys = d->nys(fd->st), nzs = d->nzs(fd->st);
     const int64_t alloc_n = (fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2) ? d->n2 : d->n3;
     double* base = *fd->slot - d->off;
@@ -588,6 +654,13 @@ int mi_small_steps(mi_domain* d, int rk) {
     hip_check(hipGetLastError(), "kernel launch");
   });
 }
+int mi_scalar_transport(mi_domain* d, int rk) {
+  return mi_detail::guard([&] {
+    if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
+    d->scalar_transport(rk);
+    hip_check(hipGetLastError(), "kernel launch");
+  });
+}
 int mi_calc_p_rho_phi(mi_domain* d, int rk) {
   return mi_detail::guard([&] {
     if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
@@ -610,12 +683,7 @@ int mi_domain_set_tuning(mi_domain* d, const int* shapes, int n) {
     for (int q = 0; q < n && q < mi_fp64::kNumColKernels; ++q)
       if (!mi_fp64::col_cfg_fits(q, shapes[q], d->grid.nz)) throw Error(MI_EINVAL, "block shape does not fit this nz");
     for (int q = 0; q < n && q < mi_fp64::kNumColKernels; ++q) d->colcfg[q] = shapes[q];
-    if (d->exec) {
-      (void)hipGraphExecDestroy(d->exec);
-      (void)hipGraphDestroy(d->graph);
-      d->exec = nullptr;
-      d->graph = nullptr;
-    }
+    d->drop_graphs();
   });
 }
 

@@ -107,13 +107,17 @@ double point_bytes(const char* name) {
 }
 
 template <class K, class... T>
-void launch3(const char* name, K kern, const Range& r, hipStream_t st, T... args) {
+void launch3a(const char* name, double arrays, K kern, const Range& r, hipStream_t st, T... args) {
   if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) return;
   const int by = (r.j1 - r.j0) == 1 ? 1 : 4;
   const bool t = timing(name);
   if (t) time_begin(st);
   hipLaunchKernelGGL(kern, grid_of(r, by), dim3(kBx, by, 1), 0, st, r, args...);
-  if (t) time_end(st, point_bytes(name) * double(r.i1 - r.i0) * double(r.j1 - r.j0) * double(r.k1 - r.k0));
+  if (t) time_end(st, 8.0 * arrays * double(r.i1 - r.i0) * double(r.j1 - r.j0) * double(r.k1 - r.k0));
+}
+template <class K, class... T>
+void launch3(const char* name, K kern, const Range& r, hipStream_t st, T... args) {
+  launch3a(name, point_bytes(name) / 8.0, kern, r, st, args...);
 }
 template <class K, class... T>
 void launch_cols(const char* name, K kern, int nx, int ny, int nz, hipStream_t st, T... args) {
@@ -165,6 +169,52 @@ __global__ void k_halo_y(Geo g, HaloSet s) {
   if (a >= (face ? 8 : 6)) return;
   double* base = s.p[f] + g.at(i, 0, k);
   halo_slot(base, g.sj, a, g.ny, face, g.bcy == MI_BC_PERIODIC, s.py[f]);
+}
+
+// ----------------------------------------------------------------------------- halo images
+// A kernel whose outputs the reference refreshes with fill_halo right after it (src/core/halo.cpp:52-84:
+// an x pass over the interior rows, then a y pass over whole rows) writes that refresh itself: with each
+// interior point it stores every halo slot whose source is that point, with the sign the fill gives it.
+// Copies and sign changes are exact, so the halo holds the same bits as after the fill, and the fill's
+// launches disappear.  Only points within 4 of an edge have images; the others store once.
+// Halo kinds: mass points (even), u faces (odd in x), v faces (odd in y), muu (odd in x, wall face
+// included) and muv (odd in y), as the reference's HaloField parities.
+enum HaloKind : int { kHMass = 0, kHU = 1, kHV = 2, kHMuu = 3, kHMuv = 4 };
+
+// emit(index, negate) for the point itself and every slot of one direction that copies index i
+template <class F>
+__device__ __forceinline__ void halo_images(int i, int n, bool face, bool periodic, int par, F&& emit) {
+  // the point itself, unless it is a periodic face slot (index n takes index 0's value); a symmetric
+  // wall face of an OddFace field changes sign in place
+  if (!(periodic && face && i == n)) emit(i, !periodic && face && par == 2 && (i == 0 || i == n));
+  if (i >= 4 && i + 4 < n) return;
+  const int nl = face ? 4 : 3, ns = face ? 8 : 6;
+#pragma unroll 1
+  for (int a = 0; a < ns; ++a) {
+    const int sl = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
+    if (sl == i) continue;
+    int src;
+    if (periodic) src = ((sl % n) + n) % n;
+    else if (face) src = sl <= 0 ? -sl : 2 * n - sl;
+    else src = sl < 0 ? -1 - sl : 2 * n - 1 - sl;
+    if (src == i) emit(sl, !periodic && par != 0);
+  }
+}
+
+// p[at(i, j) + kofs] = v and its halo images
+template <int KIND>
+__device__ __forceinline__ void put(double* p, const Geo& g, int i, int j, idx_t kofs, double v) {
+  constexpr bool fx = KIND == kHU || KIND == kHMuu, fy = KIND == kHV || KIND == kHMuv;
+  constexpr int px = KIND == kHU ? 1 : (KIND == kHMuu ? 2 : 0), py = fy ? 1 : 0;
+  const bool perx = g.bcx == MI_BC_PERIODIC, pery = g.bcy == MI_BC_PERIODIC;
+  double* q = p + kofs;
+  halo_images(i, g.nx, fx, perx, px, [&](int ii, bool nx_) {
+    if (g.two_d) {
+      q[ii] = nx_ ? -v : v;
+      return;
+    }
+    halo_images(j, g.ny, fy, pery, py, [&](int jj, bool ny_) { q[ii + jj * g.sj] = (nx_ != ny_) ? -v : v; });
+  });
 }
 
 // ----------------------------------------------------------------------------- copies
@@ -254,9 +304,9 @@ MI_COLUMN_KERNEL k_large_prep(Args a) {
     const double muu = j < ny ? H * (mu(i, j) + mu(i - 1, j) + mub(i, j) + mub(i - 1, j)) : 0.0;
     const double muv = (i < nx && j < g.nyv) ? H * (mu(i, j) + mu(i, j - 1) + mub(i, j) + mub(i, j - 1)) : 0.0;
     if (cp.kt == 0) {
-      if (mass) f.mut[x2] = mut;
-      if (j < ny) f.muu[x2] = muu;
-      if (i < nx && j < g.nyv) f.muv[x2] = muv;
+      if (mass) put<kHMass>(f.mut, g, i, j, 0, mut);
+      if (j < ny) put<kHMuu>(f.muu, g, i, j, 0, muu);
+      if (i < nx && j < g.nyv) put<kHMuv>(f.muv, g, i, j, 0, muv);
     }
     // calc_ww_cp's own column masses (summed in another order than calc_mu_uv)
     const double mu1 = mass ? H * (mu(i + 1, j) + mub(i + 1, j) + mu(i, j) + mub(i, j)) : 0.0;
@@ -265,13 +315,13 @@ MI_COLUMN_KERNEL k_large_prep(Args a) {
     const double mv0 = mass ? H * (mu(i, j) + mub(i, j) + mu(i, j - 1) + mub(i, j - 1)) : 0.0;
     for (int k = cp.kt; k <= nz; k += kKT) {
       const idx_t x = g.at(i, j, k);
-      if (j < ny && k < nz) f.ru[x] = f.u2[x] * (c.c1h[k] * muu + c.c2h[k]) / f.msfuy[x2];
-      if (i < nx && j < g.nyv && k < nz) f.rv[x] = f.v2[x] * (c.c1h[k] * muv + c.c2h[k]) * f.msfvx_inv[x2];
+      if (j < ny && k < nz) put<kHU>(f.ru, g, i, j, k * g.sk, f.u2[x] * (c.c1h[k] * muu + c.c2h[k]) / f.msfuy[x2]);
+      if (i < nx && j < g.nyv && k < nz) put<kHV>(f.rv, g, i, j, k * g.sk, f.v2[x] * (c.c1h[k] * muv + c.c2h[k]) * f.msfvx_inv[x2]);
       if (mass) {
-        f.rw[x] = f.w2[x] * (c.c1f[k] * mut + c.c2f[k]) / f.msfty[x2];
+        put<kHMass>(f.rw, g, i, j, k * g.sk, f.w2[x] * (c.c1f[k] * mut + c.c2f[k]) / f.msfty[x2]);
         if (k < nz) {
-          f.alt[x] = f.al[x] + f.alb(i, j, k);
-          f.php[x] = H * (f.phb(i, j, k) + f.phb(i, j, k + 1) + f.ph2[x] + f.ph2[x + g.sk]);
+          put<kHMass>(f.alt, g, i, j, k * g.sk, f.al[x] + f.alb(i, j, k));
+          put<kHMass>(f.php, g, i, j, k * g.sk, H * (f.phb(i, j, k) + f.phb(i, j, k + 1) + f.ph2[x] + f.ph2[x + g.sk]));
           LDS(dv, k, cp.ci) =
               f.msftx[x2] * c.dnw[k] *
               (c.rdx * ((c.c1h[k] * mu1 + c.c2h[k]) * f.u2[x + 1] / f.msfuy[x2 + 1] - (c.c1h[k] * mu0 + c.c2h[k]) * f.u2[x] / f.msfuy[x2]) +
@@ -293,7 +343,7 @@ MI_COLUMN_KERNEL k_large_prep(Args a) {
         if (k0 + q < nz) dmdt += v[q];
     }
     double w = 0;
-    f.ww[g.at(i, j, 0)] = 0;
+    put<kHMass>(f.ww, g, i, j, 0, 0.0);
     for (int k0 = 1; k0 < nz; k0 += kChain) {
       double dvv[kChain], pv[kChain];
 #pragma unroll
@@ -307,10 +357,10 @@ MI_COLUMN_KERNEL k_large_prep(Args a) {
       for (int q = 0; q < kChain; ++q)
         if (k0 + q < nz) {
           w = w - pv[q] - dvv[q];
-          f.ww[g.at(i, j, k0 + q)] = w;
+          put<kHMass>(f.ww, g, i, j, (k0 + q) * g.sk, w);
         }
     }
-    f.ww[g.at(i, j, nz)] = 0;
+    put<kHMass>(f.ww, g, i, j, nz * g.sk, 0.0);
   }
 }
 
@@ -357,7 +407,7 @@ __global__ void k_tend_u(Range r, Args a) {
                 (f.al[x] + f.al[x - 1]) * (f.pb(i, j, k) - f.pb(i - 1, j, k)));
   dpx += rm * c.rdx * (f.php[x] - f.php[x - 1]) *
          (c.rdnw[k] * (dpn(k + 1) - dpn(k)) - H * (c.c1h[k] * f.mu2[x2 - 1] + c.c1h[k] * f.mu2[x2]));
-  t -= 1.0 * dpx;
+  t -= (a.q.moist ? f.cqu[x] : 1.0) * dpx;  // kern::MoistCoef: 1 when dry
   const double rw4 = Q * (f.rw[x - 1 + sk] + f.rw[x - 1] + f.rw[x + sk] + f.rw[x]);
   t = t - u[x] * kReradius * rw4;
   f.ru_tend[x] = t;
@@ -405,7 +455,7 @@ __global__ void k_tend_v(Range r, Args a) {
                 (f.al[x] + f.al[x - sj]) * (f.pb(i, j, k) - f.pb(i, j - 1, k)));
   dpy += rm * c.rdy * (f.php[x] - f.php[x - sj]) *
          (c.rdnw[k] * (dpn(k + 1) - dpn(k)) - H * (c.c1h[k] * f.mu2[x2 - sj] + c.c1h[k] * f.mu2[x2]));
-  t -= 1.0 * dpy;
+  t -= (a.q.moist ? f.cqv[x] : 1.0) * dpy;
   const double rw4 = Q * (f.rw[x - sj + sk] + f.rw[x - sj] + f.rw[x + sk] + f.rw[x]);
   t = t - rm * v[x] * kReradius * rw4;
   f.rv_tend[x] = t;
@@ -450,7 +500,7 @@ __global__ void k_tend_w(Range r, Args a) {
   const double rmsft = 1.0 / msfty;
   if (k < nz) {
     t -= c.rdn[k] * (fz(k) - fz(k - 1));
-    const double cqw = 0.0, cq1 = 1.0 / (1.0 + cqw), cq2 = cqw * cq1;
+    const double cqw = a.q.moist ? f.cqw[x] : 0.0, cq1 = 1.0 / (1.0 + cqw), cq2 = cqw * cq1;
     t += rmsft * kG * (cq1 * c.rdn[k] * (f.p[x] - f.p[x - sk]) - c.c1f[k] * f.mu2[x2] - cq2 * (c.c1f[k] * f.mub[x2] + c.c2f[k]));
     if (a.q.wd_dt != 0.0) {
       const double muf = c.c1f[k] * f.mut[x2] + c.c2f[k];
@@ -466,7 +516,7 @@ __global__ void k_tend_w(Range r, Args a) {
     t = t + kReradius * (ruw * uw + rmt * rvw * vw);
   } else {
     t += 2.0 * c.rdn[nz - 1] * fz(nz - 1);
-    const double cqw = 0.0, cq1 = 1.0 / (1.0 + cqw), cq2 = cqw * cq1;
+    const double cqw = a.q.moist ? f.cqw[x - sk] : 0.0, cq1 = 1.0 / (1.0 + cqw), cq2 = cqw * cq1;
     t += rmsft * kG *
          (cq1 * 2.0 * c.rdnw[nz - 1] * (-f.p[x - sk]) - c.c1f[nz] * f.mu2[x2] - cq2 * (c.c1f[nz] * f.mub[x2] + c.c2f[nz]));
   }
@@ -503,6 +553,8 @@ __global__ void k_tend_t(Range r, Args a) {
   double rr = -(msftx * c.rdx) * (fx(x + 1) - fx(x));
   if (!g.two_d) rr -= (msftx * c.rdy) * (fy(x + sj) - fy(x));
   rr -= c.rdnw[k] * (fz(k + 1) - fz(k));
+  // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
+  if (a.q.moist) rr += (c.c1h[k] * f.mut[x2] + c.c2h[k]) * f.hdia[x] / f.msfty[x2];
   f.t_tend[x] = rr;
 }
 
@@ -567,28 +619,28 @@ __global__ void k_ss_prep(Range r, Args a, int first) {
   const double mus_v = first ? f.muv[x2] : H * (mts + f.mub[x2 - g.sj] + f.mu1[x2 - g.sj]);
   if (k == 0) {
     if (i < nx && j < ny) {
-      f.muts[x2] = mts;
+      put<kHMass>(f.muts, g, i, j, 0, mts);
       const double m2 = f.mu2[x2];
       f.mu_save[x2] = m2;
-      f.mu2[x2] = f.mu1[x2] - m2;
+      put<kHMass>(f.mu2, g, i, j, 0, f.mu1[x2] - m2);
     }
     if (j < ny) f.muus[x2] = mus_u;
     if (i < nx && j < g.nyv) f.muvs[x2] = mus_v;
   }
   if (j < ny && k < nz) {
     const double ui = f.u2[x];
-    f.u_save[x] = ui;
-    f.u2[x] = ((c.c1h[k] * mus_u + c.c2h[k]) * f.u1[x] - (c.c1h[k] * f.muu[x2] + c.c2h[k]) * ui) / f.msfuy[x2];
+    put<kHU>(f.u_save, g, i, j, k * g.sk, ui);
+    put<kHU>(f.u2, g, i, j, k * g.sk, ((c.c1h[k] * mus_u + c.c2h[k]) * f.u1[x] - (c.c1h[k] * f.muu[x2] + c.c2h[k]) * ui) / f.msfuy[x2]);
   }
   if (i < nx && j < g.nyv && k < nz) {
     const double vi = f.v2[x];
-    f.v_save[x] = vi;
-    f.v2[x] = ((c.c1h[k] * mus_v + c.c2h[k]) * f.v1[x] - (c.c1h[k] * f.muv[x2] + c.c2h[k]) * vi) * f.msfvx_inv[x2];
+    put<kHV>(f.v_save, g, i, j, k * g.sk, vi);
+    put<kHV>(f.v2, g, i, j, k * g.sk, ((c.c1h[k] * mus_v + c.c2h[k]) * f.v1[x] - (c.c1h[k] * f.muv[x2] + c.c2h[k]) * vi) * f.msfvx_inv[x2]);
   }
   if (i < nx && j < ny) {
     if (k < nz) {
       const double ti = f.t2[x];
-      f.t_save[x] = ti;
+      put<kHMass>(f.t_save, g, i, j, k * g.sk, ti);
       f.t2[x] = (c.c1h[k] * mts + c.c2h[k]) * f.t1[x] - (c.c1h[k] * f.mut[x2] + c.c2h[k]) * ti;
       f.c2a[x] = kCpovcv * (f.pb(i, j, k) + f.p[x]) / f.alt[x];
     }
@@ -596,7 +648,7 @@ __global__ void k_ss_prep(Range r, Args a, int first) {
     f.w_save[x] = wi;
     f.w2[x] = ((c.c1f[k] * mts + c.c2f[k]) * f.w1[x] - (c.c1f[k] * f.mut[x2] + c.c2f[k]) * wi) / f.msfty[x2];
     f.ph_save[x] = phi;
-    f.ph2[x] = f.ph1[x] - phi;
+    put<kHMass>(f.ph2, g, i, j, k * g.sk, f.ph1[x] - phi);
     f.ww1[x] = (k == 0 || k == nz) ? 0.0 : f.ww[x];
   }
 }
@@ -611,7 +663,7 @@ __device__ __forceinline__ void p_rho_at(const Args& a, int i, int j, int k, dou
   const idx_t x = g.at(i, j, k);
   const double m = c.c1h[k] * muts + c.c2h[k];
   const double al = -((1. / m) * (f.alt[x] * (c.c1h[k] * mu2) + c.rdnw[k] * (ph_kp1 - ph_k)));
-  f.al[x] = al;
+  put<kHMass>(f.al, g, i, j, k * g.sk, al);
   double pp = f.c2a[x] * (f.alt[x] * (t2 - c.c1h[k] * mu2 * f.t_save[x]) / (m * (kT0 + f.t_save[x])) - al);
   if (damp) {
     const double ptmp = pp;
@@ -620,7 +672,7 @@ __device__ __forceinline__ void p_rho_at(const Args& a, int i, int j, int k, dou
   } else {
     f.pm1[x] = pp;
   }
-  f.p[x] = pp;
+  put<kHMass>(f.p, g, i, j, k * g.sk, pp);
 }
 
 // KPRho (small_step.cpp:116-121) and KCoefW calc_coef_w (small_step.cpp:124-148), dry, one column
@@ -648,14 +700,15 @@ MI_COLUMN_KERNEL k_ss_column(Args a, double cof) {
     for (int k = cp.kt; k <= nz; k += kKT) {
       const idx_t x = g.at(i, j, k);
       if (k < nz) p_rho_at(a, i, j, k, f.ph2[x], f.ph2[x + g.sk], mu2, muts, f.t2[x], false);
+      // kern::MoistCoef::cq1 = 1 / (1 + cqw), 1 when dry
+      const double cq1 = (a.q.moist && k >= 1 && k < nz) ? 1.0 / (1.0 + f.cqw[x]) : 1.0;
       double ak;
       if (k <= 1) ak = 0;
       else if (k == nz) ak = -2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(nz - 1) * lid / (mh(nz - 1) * mf(nz - 1));
-      else ak = -1.0 * cof * c.rdn[k] * c.rdnw[k - 1] * c2a(k - 1) / (mh(k - 1) * mf(k - 1));
+      else ak = -cq1 * cof * c.rdn[k] * c.rdnw[k - 1] * c2a(k - 1) / (mh(k - 1) * mf(k - 1));
       f.ca[x] = ak;
       LDS(AA, k, cp.ci) = ak;
       if (k >= 1 && k < nz) {
-        const double cq1 = 1.0;
         LDS(B, k, cp.ci) = 1.0 + cq1 * cof * c.rdn[k] * (c.rdnw[k] * c2a(k) / (mh(k) * mf(k)) + c.rdnw[k - 1] * c2a(k - 1) / (mh(k - 1) * mf(k)));
         LDS(CC, k, cp.ci) = -cq1 * cof * c.rdn[k] * c.rdnw[k] * c2a(k) / (mh(k) * mf(k + 1));
       } else if (k == nz) {
@@ -718,13 +771,13 @@ __global__ void k_uv(Range r, Args a, double dts) {
   const int ui0 = g.bcx != MI_BC_PERIODIC ? 1 : 0;
   if (i >= ui0 && i < g.nx) {
     const double mudf_x = -a.q.emdx * (f.mudf[x2] - f.mudf[x2 - 1]) / f.msfuy[x2];
-    f.u2[x] = f.u2[x] + dts * f.ru_tend[x] - dts * 1.0 * dp(1, 1, 0, f.muu[x2], c.rdx, f.msfux[x2] / f.msfuy[x2]) + c.c1h[k] * mudf_x;
+    put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x] - dts * (a.q.moist ? f.cqu[x] : 1.0) * dp(1, 1, 0, f.muu[x2], c.rdx, f.msfux[x2] / f.msfuy[x2]) + c.c1h[k] * mudf_x);
   }
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
   if (!g.two_d && j >= vj0 && j < g.ny) {
     const idx_t sj = g.sj;
     const double mudf_y = -a.q.emdy * (f.mudf[x2] - f.mudf[x2 - sj]) * f.msfvx_inv[x2];
-    f.v2[x] = f.v2[x] + dts * f.rv_tend[x] - dts * 1.0 * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) + c.c1h[k] * mudf_y;
+    put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x] - dts * (a.q.moist ? f.cqv[x] : 1.0) * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) + c.c1h[k] * mudf_y);
   }
 }
 
@@ -874,9 +927,9 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     const double mu_new = mu_old + dts * (dmdt + mtend);
     const double muts = mut + mu_new;
     const double muave = H * ((1.0 + epssm) * mu_new + (1.0 - epssm) * mu_old);
-    f.mu2[x2] = mu_new;
-    f.mudf[x2] = dmdt + mtend;
-    f.muts[x2] = muts;
+    put<kHMass>(f.mu2, g, i, j, 0, mu_new);
+    put<kHMass>(f.mudf, g, i, j, 0, dmdt + mtend);
+    put<kHMass>(f.muts, g, i, j, 0, muts);
     f.muave[x2] = muave;
     S2[ci] = dmdt;
     S2[CI + ci] = mu_new;
@@ -913,17 +966,17 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     f.ww[x] = wwk;
     {
       const double sw = (first ? 0.0 : wwm) + wwk;
-      f.ww_m[x] = last ? sw / rn + ww1 : sw;
+      put<kHMass>(f.ww_m, g, i, j, k * sk, last ? sw / rn + ww1 : sw);
     }
     if (mk) {
       // wdtn(k) and wdtn(k+1): omega'' times theta at the w level (0 at the bottom and the top)
       const double wdtn_k = (k == 0) ? 0.0 : wwk * (c.fnm[k] * ts + c.fnp[k] * tskm);
       const double wdtn_kp = (k + 1 == nz) ? 0.0 : LDS(WW, k + 1, ci) * (c.fnm[k + 1] * tskp + c.fnp[k + 1] * ts);
       const double su = (first ? 0.0 : rum) + u0;
-      f.ru_m[x] = last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su;
+      put<kHU>(f.ru_m, g, i, j, k * sk, last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su);
       if (!g.two_d) {
         const double sv = (first ? 0.0 : rvm) + v0;
-        f.rv_m[x] = last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv;
+        put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
       }
       double hflux = H * c.rdx * (u1 * (tsxp + ts) - u0 * (ts + tsxm));
       if (!g.two_d) hflux += H * c.rdy * (v1 * (tsyp + ts) - v0 * (ts + tsym));
@@ -962,7 +1015,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       if (mk) {
         const double r_kp1 = LDS(R, k + 1, ci);
         wk = w2 + dts * rwt +
-             msft_inv * 1.0 *
+             msft_inv * (a.q.moist ? 1.0 / (1.0 + f.cqw[x]) : 1.0) *
                  (H * dts * kG * c.rdn[k] *
                   (c2a * c.rdnw[k] / (c.c1h[k] * m + c.c2h[k]) * (ope * (r_kp1 - r_k) + ome * (ph_p - ph)) -
                    c2a_m * c.rdnw[k - 1] / (c.c1h[k - 1] * m + c.c2h[k - 1]) * (ope * (r_k - r_km1) + ome * (ph - ph_m)))) +
@@ -1011,7 +1064,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
         wk = (wk - dampwt * (c.c1f[k] * mut + c.c2f[k]) * wsv) / (1.0 + dampwt);
       }
       phk = LDS(R, k, ci) + msfty * H * dts * kG * (1.0 + epssm) * wk / (c.c1f[k] * muts + c.c2f[k]);
-      f.ph2[x] = phk;
+      put<kHMass>(f.ph2, g, i, j, k * sk, phk);
     }
     f.w2[x] = wk;
     LDS(PH, k, ci) = phk;
@@ -1021,12 +1074,12 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   if (on && mk) {
     const double m = c.c1h[k] * muts + c.c2h[k];
     const double al = -((1. / m) * (alt * (c.c1h[k] * mu_new) + c.rdnw[k] * (LDS(PH, k + 1, ci) - LDS(PH, k, ci))));
-    f.al[x] = al;
+    put<kHMass>(f.al, g, i, j, k * sk, al);
     double pp = c2a * (alt * (t2new - c.c1h[k] * mu_new * ts) / (m * (kT0 + ts)) - al);
     const double ptmp = pp;
     pp = pp + a.q.smdiv * (pp - pm1);
     f.pm1[x] = ptmp;
-    f.p[x] = pp;
+    put<kHMass>(f.p, g, i, j, k * sk, pp);
   }
 }
 
@@ -1034,7 +1087,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
 // KFinish + KFinishMu (small_step.cpp:419-454) over the (nx+1) x nyv columns and, with prhophi set,
 // calc_p_rho_phi (KPRhoPhi, small_step.cpp:456-477) of the mass columns from the recoupled state: the
 // reference refreshes halos in between, which KPRhoPhi does not read.
-MI_COLUMN_KERNEL k_finish(Args a, int prhophi) {
+MI_COLUMN_KERNEL k_finish(Args a, int prhophi, double hd_dt) {
   constexpr int kKT = KT;
   extern __shared__ double lds[];
   const Geo& g = a.g;
@@ -1055,22 +1108,22 @@ MI_COLUMN_KERNEL k_finish(Args a, int prhophi) {
     for (int k = cp.kt; k <= nz; k += kKT) {
       const idx_t x = g.at(i, j, k);
       if (j < ny && k < nz)
-        f.u2[x] = (f.msfuy[x2] * f.u2[x] + f.u_save[x] * (c.c1h[k] * f.muu[x2] + c.c2h[k])) / (c.c1h[k] * mus_u + c.c2h[k]);
+        put<kHU>(f.u2, g, i, j, k * g.sk, (f.msfuy[x2] * f.u2[x] + f.u_save[x] * (c.c1h[k] * f.muu[x2] + c.c2h[k])) / (c.c1h[k] * mus_u + c.c2h[k]));
       if (i < nx && j < g.nyv && k < nz)
-        f.v2[x] = (f.msfvx[x2] * f.v2[x] + f.v_save[x] * (c.c1h[k] * f.muv[x2] + c.c2h[k])) / (c.c1h[k] * mus_v + c.c2h[k]);
+        put<kHV>(f.v2, g, i, j, k * g.sk, (f.msfvx[x2] * f.v2[x] + f.v_save[x] * (c.c1h[k] * f.muv[x2] + c.c2h[k])) / (c.c1h[k] * mus_v + c.c2h[k]));
       if (mass) {
-        f.w2[x] = (f.msfty[x2] * f.w2[x] + f.w_save[x] * (c.c1f[k] * m + c.c2f[k])) / (c.c1f[k] * muts + c.c2f[k]);
+        put<kHMass>(f.w2, g, i, j, k * g.sk, (f.msfty[x2] * f.w2[x] + f.w_save[x] * (c.c1f[k] * m + c.c2f[k])) / (c.c1f[k] * muts + c.c2f[k]));
         const double phk = f.ph2[x] + f.ph_save[x];
-        f.ph2[x] = phk;
+        put<kHMass>(f.ph2, g, i, j, k * g.sk, phk);
         LDS(PH, k, ci) = phk;
-        f.ww[x] = f.ww[x] + f.ww1[x];
-        if (k < nz) f.t2[x] = (f.t2[x] - 0.0 + f.t_save[x] * (c.c1h[k] * m + c.c2h[k])) / (c.c1h[k] * muts + c.c2h[k]);
+        put<kHMass>(f.ww, g, i, j, k * g.sk, f.ww[x] + f.ww1[x]);
+        if (k < nz) put<kHMass>(f.t2, g, i, j, k * g.sk, (f.t2[x] - (hd_dt != 0.0 ? hd_dt * (c.c1h[k] * m + c.c2h[k]) * f.hdia[x] : 0.0) + f.t_save[x] * (c.c1h[k] * m + c.c2h[k])) / (c.c1h[k] * muts + c.c2h[k]));
       }
     }
     if (cp.kt == 0) {
       if (mass) {
         const double mu = f.mu2[x2] + f.mu_save[x2];
-        f.mu2[x2] = mu;
+        put<kHMass>(f.mu2, g, i, j, 0, mu);
         MU[ci] = mu;
       }
       if (j < ny) f.muus[x2] = mus_u;
@@ -1084,9 +1137,9 @@ MI_COLUMN_KERNEL k_finish(Args a, int prhophi) {
     for (int k = cp.kt; k < nz; k += kKT) {
       const idx_t x = g.at(i, j, k);
       const double al = -1.0 / (c.c1h[k] * muts + c.c2h[k]) * (f.alb(i, j, k) * (c.c1h[k] * mu2) + c.rdnw[k] * (LDS(PH, k + 1, ci) - LDS(PH, k, ci)));
-      f.al[x] = al;
+      put<kHMass>(f.al, g, i, j, k * g.sk, al);
       const double base = kRd * (kT0 + f.t2[x]) / (kP1000 * (al + f.alb(i, j, k)));
-      f.p[x] = kP1000 * mi_libm::pow(base, kCpovcv) - f.pb(i, j, k);
+      put<kHMass>(f.p, g, i, j, k * g.sk, kP1000 * mi_libm::pow(base, kCpovcv) - f.pb(i, j, k));
     }
   }
 }
@@ -1100,9 +1153,239 @@ __global__ void k_p_rho_phi(Range r, Args a) {
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double al = -1.0 / (c.c1h[k] * f.muts[x2] + c.c2h[k]) * (f.alb(i, j, k) * (c.c1h[k] * f.mu2[x2]) + c.rdnw[k] * (f.ph2[x + g.sk] - f.ph2[x]));
-  f.al[x] = al;
+  put<kHMass>(f.al, g, i, j, k * g.sk, al);
   const double base = kRd * (kT0 + f.t2[x]) / (kP1000 * (al + f.alb(i, j, k)));
-  f.p[x] = kP1000 * mi_libm::pow(base, kCpovcv) - f.pb(i, j, k);
+  put<kHMass>(f.p, g, i, j, k * g.sk, kP1000 * mi_libm::pow(base, kCpovcv) - f.pb(i, j, k));
+}
+
+// ----------------------------------------------------------------------------- calc_cq
+// KCq (reference src/dyn/large_step.cpp:127-148): per species, the two neighbours, WRF's order
+__global__ void k_cq(Range r, Args a, Scalars s) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const idx_t x = g.at(i, j, k);
+  if (j < g.ny) {
+    double t = 0;
+    for (int q = 0; q < s.n_moist; ++q) t = t + s.q2[q][x] + s.q2[q][x - 1];
+    f.cqu[x] = 1.0 / (1.0 + 0.5 * t);
+  }
+  if (i < g.nx && j < g.nyv) {
+    double t = 0;
+    for (int q = 0; q < s.n_moist; ++q) t = t + s.q2[q][x] + s.q2[q][x - g.sj];
+    f.cqv[x] = 1.0 / (1.0 + 0.5 * t);
+  }
+  if (i < g.nx && j < g.ny && k >= 1) {
+    double t = 0;
+    for (int q = 0; q < s.n_moist; ++q) t = t + s.q2[q][x] + s.q2[q][x - g.sk];
+    f.cqw[x] = 0.5 * t;
+  }
+}
+
+// ----------------------------------------------------------------------------- scalar transport
+// The species a launch advances (ScalarTransport::update's batch, scalars.cpp:311-434)
+struct SpList {
+  int n;
+  int sp[MI_MAX_SCALARS];
+};
+
+// Face fluxes of one species at the faces of cell (i, j, k) (reference Fluxes, src/dyn/scalars.cpp:42-140;
+// periodic or symmetric edges): q = q_2 (stage value), qo = q_1, with the acoustic-step mass fluxes
+// ru_m, rv_m, ww_m and the stage-end column mass muts.  Each face is recomputed by the cells on both
+// sides with the same expression, which gives the reference's stored face values bit for bit.
+struct ScFaces {
+  const Args& a;
+  const double* q;
+  const double* qo;
+  double dt;
+  bool pd;
+  __device__ double f5(double m3, double m2, double m1, double q0, double p1, double p2, double v) const {
+    return pd ? flux5_pd(m3, m2, m1, q0, p1, p2, v, a.q.hup) : flux5(m3, m2, m1, q0, p1, p2, v, a.q.hup);
+  }
+  __device__ double f3(double m2, double m1, double q0, double p1, double v) const {
+    return pd ? flux3_pd(m2, m1, q0, p1, v, a.q.vup) : flux3(m2, m1, q0, p1, v, a.q.vup);
+  }
+  // x face at U point y (west face of the cell at y)
+  __device__ double fx(idx_t y) const {
+    const double vel = a.f.ru_m[y];
+    return vel * f5(q[y - 3], q[y - 2], q[y - 1], q[y], q[y + 1], q[y + 2], vel);
+  }
+  __device__ double fy(idx_t y) const {
+    const idx_t sj = a.g.sj;
+    const double vel = a.f.rv_m[y];
+    return vel * f5(q[y - 3 * sj], q[y - 2 * sj], q[y - sj], q[y], q[y + sj], q[y + 2 * sj], vel);
+  }
+  // w level kk of the column whose level-0 index is y0
+  __device__ double fz(idx_t y0, int kk) const {
+    const int nz = a.g.nz;
+    if (kk == 0 || kk == nz) return 0;
+    const idx_t sk = a.g.sk, y = y0 + kk * sk;
+    const double vel = a.f.ww_m[y];
+    if (kk == 1 || kk == nz - 1) return vel * (a.c.fnm[kk] * q[y] + a.c.fnp[kk] * q[y - sk]);
+    return vel * f3(q[y - 2 * sk], q[y - sk], q[y], q[y + sk], -vel);
+  }
+  __device__ double mh(idx_t y2, int k) const { return a.c.c1h[k] * a.f.muts[y2] + a.c.c2h[k]; }
+  __device__ double fxl(idx_t y, idx_t y2, int k) const {
+    const Fields& f = a.f;
+    const double dx = 2.0 / (f.msfty[y2] + f.msfty[y2 - 1]) / a.c.rdx, mu = H * (mh(y2, k) + mh(y2 - 1, k)), vel = f.ru_m[y];
+    const double cr = vel * dt / dx / mu;
+    return mu * (dx / dt) * flux_upwind(qo[y - 1], qo[y], cr);
+  }
+  __device__ double fyl(idx_t y, idx_t y2, int k) const {
+    const Fields& f = a.f;
+    const idx_t sj = a.g.sj;
+    const double dy = 2.0 / (f.msftx[y2] + f.msftx[y2 - sj]) / a.c.rdy, mu = H * (mh(y2, k) + mh(y2 - sj, k)),
+                 vel = f.rv_m[y];
+    const double cr = vel * dt / dy / mu;
+    return mu * (dy / dt) * flux_upwind(qo[y - sj], qo[y], cr);
+  }
+  __device__ double fzl(idx_t y0, idx_t y2, int kk) const {
+    if (kk == 0 || kk == a.g.nz) return 0;
+    const idx_t sk = a.g.sk, y = y0 + kk * sk;
+    const double dz = 2.0 / (a.c.rdnw[kk] + a.c.rdnw[kk - 1]), mu = mh(y2, kk), vel = a.f.ww_m[y];
+    const double cr = vel * dt / dz / mu;
+    return mu * (dz / dt) * flux_upwind(qo[y - sk], qo[y], cr);
+  }
+};
+
+__device__ __forceinline__ double m_full(const Lv& c, int k, double mub, double mu) { return c.c1h[k] * (mub + mu) + c.c2h[k]; }
+
+// rk_update_scalar (KUpdate, scalars.cpp:145-158): advect_scalar's tendency and the mass-weighted update,
+// every species of the list; the new values go to the species' second buffer with their halo images
+__global__ void k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), x0 = g.at(i, j, 0);
+  const double msftx = f.msftx[x2], msfty = f.msfty[x2];
+  const double m_old = m_full(c, k, f.mub[x2], f.mu1[x2]), m_new = m_full(c, k, f.mub[x2], f.mu2[x2]);
+#pragma unroll 1
+  for (int m = 0; m < l.n; ++m) {
+    const int n = l.sp[m];
+    const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, false};
+    double adv = -(msftx * c.rdx) * (fl.fx(x + 1) - fl.fx(x));
+    if (!g.two_d) adv -= (msftx * c.rdy) * (fl.fy(x + g.sj) - fl.fy(x));
+    adv = adv - c.rdnw[k] * (fl.fz(x0, k + 1) - fl.fz(x0, k));
+    double tend = adv * msfty;
+    tend += s.qtf[n][x];
+    put<kHMass>(s.qn[n], g, i, j, k * g.sk, (m_old * s.q1[n][x] + dt_rk * tend) / m_new);
+  }
+}
+
+// rk_update_scalar_pd, first step (KPreUpdatePD, scalars.cpp:174-181): the held tendency folded into
+// q_1, with its halo images (the reference refreshes q_1's halo next)
+__global__ void k_sc_pd_pre(Range r, Args a, Scalars s, SpList l, double dt_rk) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const double m = m_full(a.c, k, f.mub[x2], f.mu1[x2]);
+#pragma unroll 1
+  for (int q = 0; q < l.n; ++q) {
+    const int n = l.sp[q];
+    put<kHMass>(s.q1[n], g, i, j, k * g.sk, (m * s.q1[n][x] + dt_rk * s.qtf[n][x]) / m);
+  }
+}
+
+// advect_scalar_pd's high-minus-low and low-order fluxes of the six faces of a cell (KFacePDX/Y/Z,
+// scalars.cpp:186-212)
+struct PdFaces {
+  double dhx0, lox0, dhx1, lox1, dhy0, loy0, dhy1, loy1, dhz0, loz0, dhz1, loz1;
+};
+__device__ __forceinline__ PdFaces pd_faces(const ScFaces& fl, const Geo& g, int i, int j, int k) {
+  PdFaces p;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), x0 = g.at(i, j, 0);
+  p.lox0 = fl.fxl(x, x2, k);
+  p.dhx0 = fl.fx(x) - p.lox0;
+  p.lox1 = fl.fxl(x + 1, x2 + 1, k);
+  p.dhx1 = fl.fx(x + 1) - p.lox1;
+  if (!g.two_d) {
+    p.loy0 = fl.fyl(x, x2, k);
+    p.dhy0 = fl.fy(x) - p.loy0;
+    p.loy1 = fl.fyl(x + g.sj, x2 + g.sj, k);
+    p.dhy1 = fl.fy(x + g.sj) - p.loy1;
+  } else {
+    p.loy0 = p.dhy0 = p.loy1 = p.dhy1 = 0;
+  }
+  p.loz0 = fl.fzl(x0, x2, k);
+  p.dhz0 = fl.fz(x0, k) - p.loz0;
+  p.loz1 = fl.fzl(x0, x2, k + 1);
+  p.dhz1 = fl.fz(x0, k + 1) - p.loz1;
+  return p;
+}
+
+// advect_scalar_pd's limiter (KScalePD, scalars.cpp:220-248): the scale of each cell's outgoing
+// high-minus-low fluxes, with its halo images
+__global__ void k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const double msftx = f.msftx[x2], msfty = f.msfty[x2];
+  const double mxy = msftx * msfty;
+#pragma unroll 1
+  for (int q = 0; q < l.n; ++q) {
+    const int n = l.sp[q];
+    const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
+    const PdFaces p = pd_faces(fl, g, i, j, k);
+    double hlow = c.rdx * (p.lox1 - p.lox0);
+    if (!g.two_d) hlow += c.rdy * (p.loy1 - p.loy0);
+    const double ph_low = ((c.c1h[k] * f.mub[x2] + c.c2h[k]) + c.c1h[k] * f.mu1[x2]) * s.q1[n][x] -
+                          dt_rk * (mxy * hlow + msfty * c.rdnw[k] * (p.loz1 - p.loz0));
+    double hout = c.rdx * ((p.dhx1 > 0 ? p.dhx1 : 0.0) - (p.dhx0 < 0 ? p.dhx0 : 0.0));
+    if (!g.two_d) hout += c.rdy * ((p.dhy1 > 0 ? p.dhy1 : 0.0) - (p.dhy0 < 0 ? p.dhy0 : 0.0));
+    const double out = mxy * hout + msfty * c.rdnw[k] * ((p.dhz1 < 0 ? p.dhz1 : 0.0) - (p.dhz0 > 0 ? p.dhz0 : 0.0));
+    const double flux_out = dt_rk * out;
+    double sc = 1;
+    if (flux_out > ph_low) {
+      sc = ph_low / (flux_out + 1e-20);
+      if (sc < 0) sc = 0;
+    }
+    put<kHMass>(s.scale[n], g, i, j, k * g.sk, sc);
+  }
+}
+
+// advect_scalar_pd's limited divergence and rk_update_scalar (KUpdatePD, scalars.cpp:252-291)
+__global__ void k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nz = g.nz;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), sj = g.sj, sk = g.sk;
+  const double msftx = f.msftx[x2], msfty = f.msfty[x2];
+  const double m_old = m_full(c, k, f.mub[x2], f.mu1[x2]), m_new = m_full(c, k, f.mub[x2], f.mu2[x2]);
+#pragma unroll 1
+  for (int q = 0; q < l.n; ++q) {
+    const int n = l.sp[q];
+    const double* sc = s.scale[n];
+    const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
+    const PdFaces p = pd_faces(fl, g, i, j, k);
+    // the limited flux of a face: scaled by the scale of the cell it leaves
+    auto lim = [](double d, double s_minus, double s_plus) { return d * (d > 0 ? s_minus : (d < 0 ? s_plus : 1.0)); };
+    const double sc0 = sc[x];
+    const double dx0 = lim(p.dhx0, sc[x - 1], sc0), dx1 = lim(p.dhx1, sc0, sc[x + 1]);
+    // eta decreases upward: a positive w-level flux leaves the cell above it
+    const double dz0 = k == 0 ? 0.0 : lim(p.dhz0, sc0, sc[x - sk]);
+    const double dz1 = k + 1 == nz ? 0.0 : lim(p.dhz1, sc[x + sk], sc0);
+    double tend = -c.rdnw[k] * (dz1 - dz0 + p.loz1 - p.loz0);
+    tend -= msftx * (c.rdx * (dx1 - dx0 + p.lox1 - p.lox0));
+    if (!g.two_d) {
+      const double dy0 = lim(p.dhy0, sc[x - sj], sc0), dy1 = lim(p.dhy1, sc0, sc[x + sj]);
+      tend -= msftx * (c.rdy * (dy1 - dy0 + p.loy1 - p.loy0));
+    }
+    put<kHMass>(s.qn[n], g, i, j, k * sk, (m_old * s.q1[n][x] + dt_rk * (tend * msfty)) / m_new);
+  }
+}
+
+// RK3::step's q_1 = q_2 copies (rk3.cpp:99), whole allocations, one species per grid row
+__global__ void k_copy_scalars(Scalars s, int64_t n3, int64_t off) {
+  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= n3) return;
+  const int n = int(blockIdx.y);
+  (s.q1[n] - off)[e] = (s.q2[n] - off)[e];
 }
 
 // finiteness of the prognostic fields (flags[fld] = 1 when a NaN or an infinity is found)
@@ -1243,8 +1526,34 @@ void launch_uv(const Args& a, double dts, hipStream_t st) {
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
   dispatch_cfg<AcousticL>(cfg, "acoustic", 10, 5, a.g.nx, a.g.ny, st, a, dts, it, ns);
 }
-void launch_finish(const Args& a, int cfg, bool prhophi, hipStream_t st) {
-  dispatch_cfg<FinishL>(cfg, "finish", 1, 1, a.g.nx + 1, a.g.nyv, st, a, prhophi ? 1 : 0);
+void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st) {
+  dispatch_cfg<FinishL>(cfg, "finish", 1, 1, a.g.nx + 1, a.g.nyv, st, a, prhophi ? 1 : 0, hd_dt);
+}
+void launch_cq(const Args& a, const Scalars& s, hipStream_t st) {
+  const Geo& g = a.g;
+  // the moist species -> cqu, cqv, cqw
+  launch3a("cq", s.n_moist + 3, k_cq, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz}, st, a, s);
+}
+void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, bool pd, double dt_rk, hipStream_t st) {
+  if (n <= 0) return;
+  SpList l{};
+  l.n = n;
+  for (int q = 0; q < n; ++q) l.sp[q] = sp[q];
+  const Geo& g = a.g;
+  const Range r{0, g.nx, 0, g.ny, 0, g.nz};
+  // algorithmic arrays: the mass fluxes ru_m, rv_m, ww_m once, then per species
+  if (!pd) {
+    launch3a("sc_update", 3 + 4 * n, k_sc_update, r, st, a, s, l, dt_rk);  // q_2 q_1 q_tendf -> q_new
+    return;
+  }
+  launch3a("sc_pd_pre", 3 * n, k_sc_pd_pre, r, st, a, s, l, dt_rk);            // q_1 q_tendf -> q_1
+  launch3a("sc_pd_scale", 3 + 3 * n, k_sc_pd_scale, r, st, a, s, l, dt_rk);    // q_2 q_1 -> scale
+  launch3a("sc_pd_update", 3 + 4 * n, k_sc_pd_update, r, st, a, s, l, dt_rk); // q_2 q_1 scale -> q_new
+}
+void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hipStream_t st) {
+  if (n <= 0) return;
+  const int64_t off = int64_t(kLpad) + int64_t(a.g.hy) * a.g.sj;
+  hipLaunchKernelGGL(k_copy_scalars, dim3(unsigned((n3 + 255) / 256), unsigned(n), 1), dim3(256, 1, 1), 0, st, s, n3, off);
 }
 void launch_p_rho_phi(const Args& a, hipStream_t st) {
   const Geo& g = a.g;

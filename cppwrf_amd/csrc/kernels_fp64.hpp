@@ -27,6 +27,19 @@ struct Fields {
   // base state: 2-D fields stored in full with halos, 3-D through BField
   double *mub, *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy, *msfvx_inv;
   BField pb, alb, phb;
+  // moisture (src/dyn/state.hpp:72-74): calc_cq coefficients and the microphysics heating
+  double *cqu, *cqv, *cqw, *hdia;
+};
+
+// The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
+// transport.  q2[n] / qn[n] are the two buffers of species n; the host swaps them after every stage.
+struct Scalars {
+  double* q1[MI_MAX_SCALARS];
+  double* q2[MI_MAX_SCALARS];
+  double* qtf[MI_MAX_SCALARS];
+  double* qn[MI_MAX_SCALARS];
+  double* scale[MI_MAX_SCALARS];  // advect_scalar_pd's outflow scale of each species
+  int n_moist;
 };
 
 struct Params {
@@ -35,6 +48,7 @@ struct Params {
   double wd_dt, w_crit;  // w_damp: the large step (0: off) and the critical Courant number
   double dampcoef, zdamp;
   int top_lid, damp_opt;
+  int moist;  // moist species present: calc_cq coefficients and the heating terms are used
 };
 
 struct Args {
@@ -81,7 +95,12 @@ void launch_ss_prep(const Args& a, bool first, hipStream_t st);
 void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st);
 void launch_uv(const Args& a, double dts, hipStream_t st);
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st);
-void launch_finish(const Args& a, int cfg, bool prhophi, hipStream_t st);
+void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st);
+// calc_cq (KCq) of the stage state
+void launch_cq(const Args& a, const Scalars& s, hipStream_t st);
+// the transport of species sp[0..n): rk_update_scalar (pd = 0) or rk_update_scalar_pd (pd = 1)
+void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, bool pd, double dt_rk, hipStream_t st);
+void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hipStream_t st);
 void launch_p_rho_phi(const Args& a, hipStream_t st);
 void launch_finite(const Args& a, int* flags, hipStream_t st);
 

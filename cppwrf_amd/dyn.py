@@ -211,5 +211,8 @@ class RK3:
     def small_steps(self, rk):
         check(lib().mi_small_steps(self._h, rk))
 
+    def scalar_transport(self, rk):
+        check(lib().mi_scalar_transport(self._h, rk))
+
     def calc_p_rho_phi(self, rk):
         check(lib().mi_calc_p_rho_phi(self._h, rk))

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 1
+#define MI_ABI_VERSION 2
 
 /* status codes */
 #define MI_OK 0
@@ -74,7 +74,17 @@ typedef struct mi_params {
   double zdamp, dampcoef;
   int32_t time_step_sound; /* acoustic steps per large step (even); 0: the reference's automatic rule */
   int32_t precision;       /* MI_FP64 or MI_FP32_BULK */
+  /* advected scalars (reference State::q_1 / q_2 and ScalarParams, src/dyn/state.hpp:53-60,
+   * src/dyn/scalars.hpp:14-19): the first n_moist are moist species (qv first), which couple to the
+   * dynamics through calc_cq and theta_m; the rest are passive tracers */
+  int32_t n_moist;         /* 0..MI_MAX_MOIST */
+  int32_t n_scalars;       /* n_moist..MI_MAX_SCALARS */
+  int32_t moist_adv_opt;   /* 0: advect_scalar; 1: positive-definite advect_scalar_pd at the last stage */
+  int32_t scalar_adv_opt;  /* the same for the passive tracers */
 } mi_params;
+
+#define MI_MAX_MOIST 8    /* reference KCq::kMax (src/dyn/large_step.cpp:128) */
+#define MI_MAX_SCALARS 16
 
 /* reference BaseState vertical coefficients (src/dyn/state.hpp:109-116, set by BaseState::set_levels,
  * src/dyn/state.cpp:86-116).  Arrays of nz (mass levels) or nz+1 (w levels) values. */
@@ -95,11 +105,19 @@ enum {
   /* diagnostics of the last stage (read only) */
   MI_WW = 8, MI_RU_M = 9, MI_RV_M = 10, MI_WW_M = 11, MI_ALT = 12, MI_PHP = 13, MI_MUTS = 14,
   MI_RU_TEND = 15, MI_RV_TEND = 16, MI_RW_TEND = 17, MI_T_TEND = 18, MI_PH_TEND = 19,
+  /* calc_cq coefficients of the last stage (read only; with moisture): U, V and W points */
+  MI_CQU = 20, MI_CQV = 21, MI_CQW = 22,
+  /* microphysics heating of the last step, theta_m K/s (input; zero unless set; used with moisture:
+   * rk_addtend_dry and small_step_finish, src/dyn/large_step.cpp:557, small_step.cpp:437-440) */
+  MI_H_DIABATIC = 23,
   /* base state (write once, before the first step).  3-D fields take nzs values for a horizontally
    * uniform profile or nxs*nys*nzs values; 2-D fields take 1 value or nx*ny values. */
   MI_MUB = 32, MI_PB = 33, MI_ALB = 34, MI_PHB = 35, MI_T_INIT = 36,
   MI_MSFTX = 40, MI_MSFTY = 41, MI_MSFUX = 42, MI_MSFUY = 43, MI_MSFVX = 44, MI_MSFVY = 45,
-  MI_MSFVX_INV = 46
+  MI_MSFVX_INV = 46,
+  /* scalar n (0 <= n < n_scalars) at mass points: MI_Q + n is its state (q_2), MI_Q_TENDF + n its held
+   * tendency (input, mass-coupled, held over the step as the reference's q_tendf) */
+  MI_Q = 64, MI_Q_TENDF = 96
 };
 
 typedef struct mi_ctx mi_ctx;       /* a GPU and the HIP stream its domains launch on */
@@ -142,12 +160,14 @@ int mi_domain_sync(mi_domain* d);
  *   mi_rk_prep         LargeStep::prep (large_step.cpp:697-733)
  *   mi_rk_tendency     LargeStep::tendencies (large_step.cpp:798-861)
  *   mi_small_steps     SmallStep::run_stage (small_step.cpp:490-607)
+ *   mi_scalar_transport ScalarTransport::run_stage (scalars.cpp:436-461)
  *   mi_calc_p_rho_phi  SmallStep::calc_p_rho_phi (small_step.cpp:609-617)
  * rk = 1, 2, 3. */
 int mi_step_begin(mi_domain* d);
 int mi_rk_prep(mi_domain* d, int rk);
 int mi_rk_tendency(mi_domain* d, int rk);
 int mi_small_steps(mi_domain* d, int rk);
+int mi_scalar_transport(mi_domain* d, int rk);
 int mi_calc_p_rho_phi(mi_domain* d, int rk);
 
 /* MI_ENONFINITE if any point of u, v, w, t, ph, mu is NaN or infinite (the reference's runner check,

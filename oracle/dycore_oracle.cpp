@@ -5,8 +5,10 @@
 // must agree with this file bit for bit, and both must agree with the reference.  Compiled with
 // -ffp-contract=off (no fused multiply-add), as the reference (CMakeLists.txt:33-37).
 //
-// Scope: dry dynamics on flat or uniform-map grids with periodic or symmetric lateral boundaries,
-// h_adv_order 5/6, v_adv_order 3/4, damp_opt 0/3, w_damping 0/1, top_lid 0/1, hypsometric_opt 1.
+// Scope: dynamics on flat grids with map factors and periodic or symmetric lateral boundaries,
+// h_adv_order 5/6, v_adv_order 3/4, damp_opt 0/3, w_damping 0/1, top_lid 0/1, hypsometric_opt 1; moist
+// species coupled through calc_cq and theta_m, and the transport of moist species and passive tracers
+// (advect_scalar, advect_scalar_pd) with the mass fluxes of the acoustic steps.
 #include "oracle.h"
 
 #include <emmintrin.h>
@@ -133,6 +135,21 @@ inline double flux5(double qm3, double qm2, double qm1, double q0, double qp1, d
 inline double flux3(double qm2, double qm1, double q0, double qp1, double ua, double up) {
   return (7.0 * (q0 + qm1) - (qp1 + qm2)) / 12.0 + up * sgn1(ua) * ((qp1 - qm2) - 3.0 * (q0 - qm1)) / 12.0;
 }
+// advect_scalar_pd's statement functions with folded constants (kernels_common.hpp:23-33)
+inline double flux5_pd(double qm3, double qm2, double qm1, double q0, double qp1, double qp2, double ua, double up) {
+  const double f6 = (37. / 60.) * (q0 + qm1) - (2. / 15.) * (qp1 + qm2) + (1. / 60.) * (qp2 + qm3);
+  return f6 - up * sgn1(ua) * (1. / 60.) * ((qp2 - qm3) - 5. * (qp1 - qm2) + 10. * (q0 - qm1));
+}
+inline double flux3_pd(double qm2, double qm1, double q0, double qp1, double ua, double up) {
+  const double f4 = (7. / 12.) * (q0 + qm1) - (1. / 12.) * (qp1 + qm2);
+  return f4 + up * sgn1(ua) * (1. / 12.) * ((qp1 - qm2) - 3. * (q0 - qm1));
+}
+// first-order upwind flux with Courant number cr (kernels_common.hpp:83-88)
+inline double flux_upwind(double qm1, double q0, double cr) {
+  const double a = cr < 0.0 ? -cr : cr;
+  const double lo = cr + a, hi = cr - a;
+  return 0.5 * (lo < 1.0 ? lo : 1.0) * qm1 + 0.5 * (hi > -1.0 ? hi : -1.0) * q0;
+}
 
 struct Lev {
   std::vector<double> dnw, rdnw, dn, rdn, fnm, fnp, c1h, c2h, c3h, c4h, c1f, c2f, c3f, c4f;
@@ -155,6 +172,12 @@ struct Dom {
       t2a, dvdxi;
   // BaseState (state.hpp:92-119), stored in full with halos
   F mub, pb, alb, t_init, phb, msftx, msfty, msfux, msfuy, msfvx, msfvy, msfvx_inv;
+  // scalars (state.hpp:53-76): q_1, q_2, q_tendf, calc_cq coefficients, h_diabatic
+  int n_moist = 0, n_sc = 0;
+  std::vector<F> q1, q2, qtf, qnew;
+  F cqu, cqv, cqw, hdia;
+  // advect_scalar_pd work (scalars.cpp:295-298)
+  F sc_scale, dhx, lox, dhy, loy, dhz, loz;
 
   explicit Dom(const mi_grid& gg, const mi_params& pp_, const mi_levels& lv) {
     g = Grid{gg.nx, gg.ny, gg.nz, gg.bc_x, gg.bc_y, gg.dx, gg.dy, gg.dt};
@@ -181,7 +204,21 @@ struct Dom {
     for (F* f : {&muu, &muus, &msfux, &msfuy}) f2(*f, St::U);
     for (F* f : {&muv, &muvs, &msfvx, &msfvy, &msfvx_inv}) f2(*f, St::V);
     for (F* f : {&msftx, &msfty, &msfux, &msfuy, &msfvx, &msfvy, &msfvx_inv}) std::fill(f->a.begin(), f->a.end(), 1.0);
+    n_moist = p.n_moist;
+    n_sc = p.n_scalars;
+    for (auto* v : {&q1, &q2, &qtf, &qnew}) {
+      v->resize(size_t(n_sc));
+      for (F& f : *v) f3(f, St::M);
+    }
+    f3(cqu, St::U); f3(cqv, St::V); f3(cqw, St::W); f3(hdia, St::M);
+    f3(sc_scale, St::M); f3(dhx, St::U); f3(lox, St::U); f3(dhy, St::V); f3(loy, St::V); f3(dhz, St::W); f3(loz, St::W);
   }
+  bool moist() const { return n_moist > 0; }
+  // kern::MoistCoef (kernels_common.hpp:93-100): 1, 1, 0 when dry
+  double mc_cqu(int i, int j, int k) const { return moist() ? cqu(i, j, k) : 1.0; }
+  double mc_cqv(int i, int j, int k) const { return moist() ? cqv(i, j, k) : 1.0; }
+  double mc_cqw(int i, int j, int k) const { return moist() ? cqw(i, j, k) : 0.0; }
+  double mc_cq1(int i, int j, int k) const { return 1.0 / (1.0 + mc_cqw(i, j, k)); }
   static int mi_auto_sound_steps_(double dt, double dx, double dy) {
     const double spacing = dx < dy ? dx : dy;
     const int n = 2 * (int(300.0 * dt / spacing - 0.01) + 1);
@@ -236,6 +273,25 @@ struct Dom {
       ww(i, j, nz) = 0;
     });
     fill(ww, g);
+    if (moist()) {  // KCq calc_cq (large_step.cpp:127-148, 724-733): per species, both neighbours
+      pfor(0, nx + 1, 0, nyv_, 0, nz, [&](int i, int j, int k) {
+        if (j < ny) {
+          double t = 0;
+          for (int s = 0; s < n_moist; ++s) t = t + q2[s](i, j, k) + q2[s](i - 1, j, k);
+          cqu(i, j, k) = 1.0 / (1.0 + 0.5 * t);
+        }
+        if (i < nx && j < nyv_) {
+          double t = 0;
+          for (int s = 0; s < n_moist; ++s) t = t + q2[s](i, j, k) + q2[s](i, j - 1, k);
+          cqv(i, j, k) = 1.0 / (1.0 + 0.5 * t);
+        }
+        if (i < nx && j < ny && k >= 1) {
+          double t = 0;
+          for (int s = 0; s < n_moist; ++s) t = t + q2[s](i, j, k) + q2[s](i, j, k - 1);
+          cqw(i, j, k) = 0.5 * t;
+        }
+      });
+    }
   }
 
   // ------------------------------------------------------------------ LargeStep::tendencies (large_step.cpp:798-861)
@@ -282,7 +338,7 @@ struct Dom {
                       (al(i, j, k) + al(i - 1, j, k)) * (pb(i, j, k) - pb(i - 1, j, k)));
         dpx += rm * rdx * (php(i, j, k) - php(i - 1, j, k)) *
                (c.rdnw[k] * (dpn(i, j, k + 1) - dpn(i, j, k)) - H * (c.c1h[k] * mu2(i - 1, j) + c.c1h[k] * mu2(i, j)));
-        t -= 1.0 * dpx;  // mc.cqu = 1 (dry)
+        t -= mc_cqu(i, j, k) * dpx;
         const double rw4 = Q * (rw(i - 1, j, k + 1) + rw(i - 1, j, k) + rw(i, j, k + 1) + rw(i, j, k));
         t = t - u2(i, j, k) * kReradius * rw4;
         ru_tend(i, j, k) = t;
@@ -323,7 +379,7 @@ struct Dom {
                       (al(i, j, k) + al(i, j - 1, k)) * (pb(i, j, k) - pb(i, j - 1, k)));
         dpy += rm * rdy * (php(i, j, k) - php(i, j - 1, k)) *
                (c.rdnw[k] * (dpn(i, j, k + 1) - dpn(i, j, k)) - H * (c.c1h[k] * mu2(i, j - 1) + c.c1h[k] * mu2(i, j)));
-        t -= 1.0 * dpy;  // mc.cqv = 1 (dry)
+        t -= mc_cqv(i, j, k) * dpy;
         const double rw4 = Q * (rw(i, j - 1, k + 1) + rw(i, j - 1, k) + rw(i, j, k + 1) + rw(i, j, k));
         t = t - rm * v2(i, j, k) * kReradius * rw4;
         rv_tend(i, j, k) = t;
@@ -360,7 +416,7 @@ struct Dom {
         const double rmsft = 1.0 / msfty(i, j);
         if (k < nz) {
           t -= c.rdn[k] * (fz(i, j, k) - fz(i, j, k - 1));
-          const double cqw = 0.0, cq1 = 1.0 / (1.0 + cqw), cq2 = cqw * cq1;
+          const double cqw_ = mc_cqw(i, j, k), cq1 = 1.0 / (1.0 + cqw_), cq2 = cqw_ * cq1;
           t += rmsft * kG * (cq1 * c.rdn[k] * (pp(i, j, k) - pp(i, j, k - 1)) - c.c1f[k] * mu2(i, j) -
                              cq2 * (c.c1f[k] * mub(i, j) + c.c2f[k]));
           if (wd_dt != 0.0) {
@@ -381,7 +437,7 @@ struct Dom {
           t = t + kReradius * (ruw * uw + rmt * rvw * vw);
         } else {
           t += 2.0 * c.rdn[nz - 1] * fz(i, j, nz - 1);
-          const double cqw = 0.0, cq1 = 1.0 / (1.0 + cqw), cq2 = cqw * cq1;
+          const double cqw_ = mc_cqw(i, j, nz - 1), cq1 = 1.0 / (1.0 + cqw_), cq2 = cqw_ * cq1;
           t += rmsft * kG * (cq1 * 2.0 * c.rdnw[nz - 1] * (-pp(i, j, nz - 1)) - c.c1f[nz] * mu2(i, j) -
                              cq2 * (c.c1f[nz] * mub(i, j) + c.c2f[nz]));
         }
@@ -410,6 +466,8 @@ struct Dom {
         double r = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
         if (do_y) r -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
         r -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+        // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
+        if (moist()) r += (c.c1h[k] * mut(i, j) + c.c2h[k]) * hdia(i, j, k) / msfty(i, j);
         t_tend(i, j, k) = r;
       });
     }
@@ -533,10 +591,10 @@ struct Dom {
       ca(i, j, 0) = 0; calpha(i, j, 0) = 0; cgamma(i, j, 0) = 0;
       ca(i, j, 1) = 0;
       ca(i, j, nz) = -2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(i, j, nz - 1) * lid / (mh(nz - 1, m) * mf(nz - 1, m));
-      for (int k = 2; k < nz; ++k)
-        ca(i, j, k) = -1.0 * cof * c.rdn[k] * c.rdnw[k - 1] * c2a(i, j, k - 1) / (mh(k - 1, m) * mf(k - 1, m));
+      for (int k = 2; k < nz; ++k)  // c1f at the level below, as the reference (small_step.cpp:132-133)
+        ca(i, j, k) = -mc_cq1(i, j, k) * cof * c.rdn[k] * c.rdnw[k - 1] * c2a(i, j, k - 1) / (mh(k - 1, m) * mf(k - 1, m));
       for (int k = 1; k < nz; ++k) {
-        const double cq1 = 1.0;
+        const double cq1 = mc_cq1(i, j, k);
         const double b = 1.0 + cq1 * cof * c.rdn[k] *
                                    (c.rdnw[k] * c2a(i, j, k) / (mh(k, m) * mf(k, m)) +
                                     c.rdnw[k - 1] * c2a(i, j, k - 1) / (mh(k - 1, m) * mf(k, m)));
@@ -573,12 +631,12 @@ struct Dom {
         if (i >= ui0 && i < ui1) {
           const double mudf_x = -emdx * (mudf(i, j) - mudf(i - 1, j)) / msfuy(i, j);
           u2(i, j, k) = u2(i, j, k) + dts * ru_tend(i, j, k) -
-                        dts * 1.0 * dp(1, 0, i, j, k, muu(i, j), c.rdx, msfux(i, j) / msfuy(i, j)) + c.c1h[k] * mudf_x;
+                        dts * mc_cqu(i, j, k) * dp(1, 0, i, j, k, muu(i, j), c.rdx, msfux(i, j) / msfuy(i, j)) + c.c1h[k] * mudf_x;
         }
         if (do_y && j >= vj0 && j < vj1) {
           const double mudf_y = -emdy * (mudf(i, j) - mudf(i, j - 1)) * msfvx_inv(i, j);
           v2(i, j, k) = v2(i, j, k) + dts * rv_tend(i, j, k) -
-                        dts * 1.0 * dp(0, 1, i, j, k, muv(i, j), c.rdy, msfvy(i, j) / msfvx(i, j)) + c.c1h[k] * mudf_y;
+                        dts * mc_cqv(i, j, k) * dp(0, 1, i, j, k, muv(i, j), c.rdy, msfvy(i, j) / msfvx(i, j)) + c.c1h[k] * mudf_y;
         }
       });
       if (do_y) { fill(u2, g); fill(v2, g); }
@@ -683,7 +741,7 @@ struct Dom {
         if (k < nz) {
           const double r_kp1 = w_rhs(i, j, k + 1), ph_kp1 = ph2(i, j, k + 1);
           w2(i, j, k) = w2(i, j, k) + dts * rw_tend(i, j, k) +
-                        msft_inv * 1.0 *
+                        msft_inv * mc_cq1(i, j, k) *
                             (H * dts * kG * c.rdn[k] *
                              (c2a(i, j, k) * c.rdnw[k] / mh(k, m) * (ope * (r_kp1 - r_k) + ome * (ph_kp1 - ph_k)) -
                               c2a(i, j, k - 1) * c.rdnw[k - 1] / mh(k - 1, m) * (ope * (r_k - r_km1) + ome * (ph_k - ph_km1)))) +
@@ -725,7 +783,9 @@ struct Dom {
       for (F* f : {&ph2, &pp, &al}) fill(*f, g);
     }
     for (F* f : {&muts, &ru_m, &rv_m, &ww_m}) fill(*f, g);
-    // KFinish small_step_finish (small_step.cpp:419-446), dry
+    // KFinish small_step_finish (small_step.cpp:419-446); hd_dt removes the microphysics heating at
+    // the last stage with moisture (small_step.cpp:602)
+    const double hd_dt = (rk == 3 && moist()) ? dts * double(ns) : 0.0;
     pfor(0, nx + 1, 0, nyv_, 0, nz + 1, [&](int i, int j, int k) {
       if (j < ny && k < nz) {
         const double mus_u = H * (muts(i, j) + muts(i - 1, j));
@@ -741,7 +801,9 @@ struct Dom {
         ph2(i, j, k) = ph2(i, j, k) + ph_save(i, j, k);
         ww(i, j, k) = ww(i, j, k) + ww1(i, j, k);
         if (k < nz)
-          t2(i, j, k) = (t2(i, j, k) - 0.0 + t_save(i, j, k) * (c.c1h[k] * m + c.c2h[k])) / (c.c1h[k] * mts + c.c2h[k]);
+          t2(i, j, k) = (t2(i, j, k) - (hd_dt != 0.0 ? hd_dt * (c.c1h[k] * m + c.c2h[k]) * hdia(i, j, k) : 0.0) +
+                         t_save(i, j, k) * (c.c1h[k] * m + c.c2h[k])) /
+                        (c.c1h[k] * mts + c.c2h[k]);
       }
     });
     // KFinishMu (small_step.cpp:448-454)
@@ -751,6 +813,166 @@ struct Dom {
       if (i < nx && j < nyv_) muvs(i, j) = H * (muts(i, j) + muts(i, j - 1));
     });
     for (F* f : {&u2, &v2, &w2, &t2, &ph2, &ww, &mu2}) fill(*f, g);
+  }
+
+
+  // ------------------------------------------------------------------ ScalarTransport (scalars.cpp:295-461)
+  // Face fluxes of one species (Fluxes, scalars.cpp:42-140), periodic / symmetric edges: q = q_2 (stage),
+  // qo = q_1, the acoustic-step mass fluxes ru_m, rv_m, ww_m and the stage-end column mass muts.
+  struct Fl {
+    const Dom& d;
+    const F &q, &qo;
+    double dt, hup, vup;
+    bool pd;
+    double f5(double a, double b, double cc, double e, double f, double h, double v) const {
+      return pd ? flux5_pd(a, b, cc, e, f, h, v, hup) : flux5(a, b, cc, e, f, h, v, hup);
+    }
+    double f3(double a, double b, double cc, double e, double v, double up) const {
+      return pd ? flux3_pd(a, b, cc, e, v, up) : flux3(a, b, cc, e, v, up);
+    }
+    double fx(int i, int j, int k) const {
+      const double vel = d.ru_m(i, j, k);
+      return vel * f5(q(i - 3, j, k), q(i - 2, j, k), q(i - 1, j, k), q(i, j, k), q(i + 1, j, k), q(i + 2, j, k), vel);
+    }
+    double fy(int i, int j, int k) const {
+      const double vel = d.rv_m(i, j, k);
+      return vel * f5(q(i, j - 3, k), q(i, j - 2, k), q(i, j - 1, k), q(i, j, k), q(i, j + 1, k), q(i, j + 2, k), vel);
+    }
+    double fz(int i, int j, int k) const {
+      const int nz = d.g.nz;
+      if (k == 0 || k == nz) return 0;
+      const double vel = d.ww_m(i, j, k);
+      if (k == 1 || k == nz - 1) return vel * (d.c.fnm[k] * q(i, j, k) + d.c.fnp[k] * q(i, j, k - 1));
+      return vel * f3(q(i, j, k - 2), q(i, j, k - 1), q(i, j, k), q(i, j, k + 1), -vel, vup);
+    }
+    double mh(int i, int j, int k) const { return d.c.c1h[k] * d.muts(i, j) + d.c.c2h[k]; }
+    double fxl(int i, int j, int k) const {
+      const double dx = 2.0 / (d.msfty(i, j) + d.msfty(i - 1, j)) / d.c.rdx, mu = H * (mh(i, j, k) + mh(i - 1, j, k)),
+                   vel = d.ru_m(i, j, k);
+      const double cr = vel * dt / dx / mu;
+      return mu * (dx / dt) * flux_upwind(qo(i - 1, j, k), qo(i, j, k), cr);
+    }
+    double fyl(int i, int j, int k) const {
+      const double dy = 2.0 / (d.msftx(i, j) + d.msftx(i, j - 1)) / d.c.rdy, mu = H * (mh(i, j, k) + mh(i, j - 1, k)),
+                   vel = d.rv_m(i, j, k);
+      const double cr = vel * dt / dy / mu;
+      return mu * (dy / dt) * flux_upwind(qo(i, j - 1, k), qo(i, j, k), cr);
+    }
+    double fzl(int i, int j, int k) const {
+      if (k == 0 || k == d.g.nz) return 0;
+      const double dz = 2.0 / (d.c.rdnw[k] + d.c.rdnw[k - 1]), mu = mh(i, j, k), vel = d.ww_m(i, j, k);
+      const double cr = vel * dt / dz / mu;
+      return mu * (dz / dt) * flux_upwind(qo(i, j, k - 1), qo(i, j, k), cr);
+    }
+    double adv(int i, int j, int k) const {  // advect_scalar, horizontal terms times msftx
+      double r = -(d.msftx(i, j) * d.c.rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+      if (!d.g.two_d()) r -= (d.msftx(i, j) * d.c.rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+      return r - d.c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+    }
+  };
+  double m_full(int k, double mub_, double mu_) const { return c.c1h[k] * (mub_ + mu_) + c.c2h[k]; }  // scalars.cpp:142
+
+  // rk_update_scalar (KUpdate, scalars.cpp:145-158) of species n into qnew[n]
+  void update_plain(int n, double dt_rk) {
+    const double hup = p.h_adv_order == 5 ? 1.0 : 0.0, vup = p.v_adv_order == 3 ? 1.0 : 0.0;
+    const Fl f{*this, q2[n], q1[n], dt_rk, hup, vup, false};
+    pfor(0, g.nx, 0, g.ny, 0, g.nz, [&](int i, int j, int k) {
+      double tend = f.adv(i, j, k) * msfty(i, j);
+      tend += qtf[n](i, j, k);
+      qnew[n](i, j, k) = (m_full(k, mub(i, j), mu1(i, j)) * q1[n](i, j, k) + dt_rk * tend) / m_full(k, mub(i, j), mu2(i, j));
+    });
+  }
+  // rk_update_scalar_pd (KPreUpdatePD, KFacePD*, KScalePD, KUpdatePD; scalars.cpp:174-291, 380-434)
+  void update_pd(int n, double dt_rk) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool do_y = !g.two_d();
+    const double hup = p.h_adv_order == 5 ? 1.0 : 0.0, vup = p.v_adv_order == 3 ? 1.0 : 0.0;
+    F& qo = q1[n];
+    pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+      const double m = m_full(k, mub(i, j), mu1(i, j));
+      qo(i, j, k) = (m * qo(i, j, k) + dt_rk * qtf[n](i, j, k)) / m;
+    });
+    fill(qo, g);
+    const Fl f{*this, q2[n], qo, dt_rk, hup, vup, true};
+    pfor(0, nx + 1, 0, ny, 0, nz, [&](int i, int j, int k) {
+      const double l = f.fxl(i, j, k);
+      dhx(i, j, k) = f.fx(i, j, k) - l;
+      lox(i, j, k) = l;
+    });
+    if (do_y)
+      pfor(0, nx, 0, ny + 1, 0, nz, [&](int i, int j, int k) {
+        const double l = f.fyl(i, j, k);
+        dhy(i, j, k) = f.fy(i, j, k) - l;
+        loy(i, j, k) = l;
+      });
+    pfor(0, nx, 0, ny, 0, nz + 1, [&](int i, int j, int k) {
+      const double l = f.fzl(i, j, k);
+      dhz(i, j, k) = f.fz(i, j, k) - l;
+      loz(i, j, k) = l;
+    });
+    pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {  // KScalePD
+      const double mxy = msftx(i, j) * msfty(i, j);
+      double hlow = c.rdx * (lox(i + 1, j, k) - lox(i, j, k));
+      if (do_y) hlow += c.rdy * (loy(i, j + 1, k) - loy(i, j, k));
+      const double ph_low = ((c.c1h[k] * mub(i, j) + c.c2h[k]) + c.c1h[k] * mu1(i, j)) * qo(i, j, k) -
+                            dt_rk * (mxy * hlow + msfty(i, j) * c.rdnw[k] * (loz(i, j, k + 1) - loz(i, j, k)));
+      const double fxp = dhx(i + 1, j, k), fxm = dhx(i, j, k);
+      double hout = c.rdx * ((fxp > 0 ? fxp : 0.0) - (fxm < 0 ? fxm : 0.0));
+      if (do_y) {
+        const double fyp = dhy(i, j + 1, k), fym = dhy(i, j, k);
+        hout += c.rdy * ((fyp > 0 ? fyp : 0.0) - (fym < 0 ? fym : 0.0));
+      }
+      const double fzp = dhz(i, j, k + 1), fzm = dhz(i, j, k);
+      const double out = mxy * hout + msfty(i, j) * c.rdnw[k] * ((fzp < 0 ? fzp : 0.0) - (fzm > 0 ? fzm : 0.0));
+      const double flux_out = dt_rk * out;
+      double sc = 1;
+      if (flux_out > ph_low) {
+        sc = ph_low / (flux_out + 1e-20);
+        if (sc < 0) sc = 0;
+      }
+      sc_scale(i, j, k) = sc;
+    });
+    fill(sc_scale, g);
+    auto ddx = [&](int i, int j, int k) {
+      const double dd = dhx(i, j, k);
+      return dd * (dd > 0 ? sc_scale(i - 1, j, k) : (dd < 0 ? sc_scale(i, j, k) : 1.0));
+    };
+    auto ddy = [&](int i, int j, int k) {
+      const double dd = dhy(i, j, k);
+      return dd * (dd > 0 ? sc_scale(i, j - 1, k) : (dd < 0 ? sc_scale(i, j, k) : 1.0));
+    };
+    auto ddz = [&](int i, int j, int k) {
+      if (k == 0 || k == nz) return 0.0;
+      const double dd = dhz(i, j, k);
+      return dd * (dd > 0 ? sc_scale(i, j, k) : (dd < 0 ? sc_scale(i, j, k - 1) : 1.0));
+    };
+    pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {  // KUpdatePD
+      double tend = -c.rdnw[k] * (ddz(i, j, k + 1) - ddz(i, j, k) + loz(i, j, k + 1) - loz(i, j, k));
+      tend -= msftx(i, j) * (c.rdx * (ddx(i + 1, j, k) - ddx(i, j, k) + lox(i + 1, j, k) - lox(i, j, k)));
+      if (do_y) tend -= msftx(i, j) * (c.rdy * (ddy(i, j + 1, k) - ddy(i, j, k) + loy(i, j + 1, k) - loy(i, j, k)));
+      qnew[n](i, j, k) = (m_full(k, mub(i, j), mu1(i, j)) * qo(i, j, k) + dt_rk * (tend * msfty(i, j))) /
+                         m_full(k, mub(i, j), mu2(i, j));
+    });
+  }
+  // ScalarTransport::run_stage (scalars.cpp:436-461): plain species first, then the PD ones; each new
+  // value replaces q_2 (swap), then the halos of all scalars
+  void scalars(int rk) {
+    if (n_sc == 0) return;
+    const double dt_rk = rk == 1 ? g.dt / 3.0 : (rk == 2 ? 0.5 * g.dt : g.dt);
+    std::vector<int> pd_n, plain_n;
+    for (int n = 0; n < n_sc; ++n) {
+      const bool pd = rk == 3 && (n < n_moist ? p.moist_adv_opt : p.scalar_adv_opt) == 1;
+      (pd ? pd_n : plain_n).push_back(n);
+    }
+    for (int n : plain_n) {
+      update_plain(n, dt_rk);
+      std::swap(q2[n].a, qnew[n].a);
+    }
+    for (int n : pd_n) {
+      update_pd(n, dt_rk);
+      std::swap(q2[n].a, qnew[n].a);
+    }
+    for (int n = 0; n < n_sc; ++n) fill(q2[n], g);
   }
 
   // ------------------------------------------------------------------ calc_p_rho_phi (small_step.cpp:456-477,609-617)
@@ -777,13 +999,15 @@ struct Dom {
   void step_begin() {
     u1.a = u2.a; v1.a = v2.a; w1.a = w2.a; t1.a = t2.a; ph1.a = ph2.a; mu1.a = mu2.a;
     fill(mu1, g);
+    for (int n = 0; n < n_sc; ++n) q1[n].a = q2[n].a;  // rk3.cpp:99
   }
-  void step() {  // RK3::step (rk3.cpp:86-170), dry, no physics / boundaries / diffusion
+  void step() {  // RK3::step (rk3.cpp:86-170), no physics / boundaries / diffusion
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
       rk_prep();
       rk_tendency();
       small_steps(rk);
+      scalars(rk);
       calc_p_rho_phi();
     }
   }
@@ -800,12 +1024,16 @@ struct Dom {
       case MI_T_INIT: return &t_init; case MI_MSFTX: return &msftx; case MI_MSFTY: return &msfty;
       case MI_MSFUX: return &msfux; case MI_MSFUY: return &msfuy; case MI_MSFVX: return &msfvx;
       case MI_MSFVY: return &msfvy; case MI_MSFVX_INV: return &msfvx_inv;
-      default: return nullptr;
+      case MI_CQU: return &cqu; case MI_CQV: return &cqv; case MI_CQW: return &cqw; case MI_H_DIABATIC: return &hdia;
+      default:
+        if (id >= MI_Q && id < MI_Q + n_sc) return &q2[size_t(id - MI_Q)];
+        if (id >= MI_Q_TENDF && id < MI_Q_TENDF + n_sc) return &qtf[size_t(id - MI_Q_TENDF)];
+        return nullptr;
     }
   }
 };
 
-bool is_base(int id) { return id >= MI_MUB; }
+bool is_base(int id) { return id >= MI_MUB && id <= MI_MSFVX_INV; }
 
 int fail(const std::string& m, int code = MI_EINVAL) {
   g_err = m;
@@ -833,6 +1061,8 @@ int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv,
     if (p->v_adv_order != 3 && p->v_adv_order != 4) return fail("v_adv_order must be 3 or 4");
     if (p->hypsometric_opt != 1) return fail("oracle: hypsometric_opt 1 only");
     if (p->damp_opt != 0 && p->damp_opt != 3) return fail("oracle: damp_opt 0 or 3 only");
+    if (p->n_moist < 0 || p->n_moist > MI_MAX_MOIST || p->n_scalars < p->n_moist || p->n_scalars > MI_MAX_SCALARS)
+      return fail("bad n_moist / n_scalars");
     auto* o = new orc_domain;
     o->d = std::make_unique<Dom>(*g, *p, *lv);
     if (o->d->n_sound < 2 || o->d->n_sound % 2) {
@@ -915,6 +1145,11 @@ int orc_rk_tendency(orc_domain* d, int) {
 int orc_small_steps(orc_domain* d, int rk) {
   if (rk < 1 || rk > 3) return fail("rk must be 1, 2 or 3");
   d->d->small_steps(rk);
+  return MI_OK;
+}
+int orc_scalar_transport(orc_domain* d, int rk) {
+  if (rk < 1 || rk > 3) return fail("rk must be 1, 2 or 3");
+  d->d->scalars(rk);
   return MI_OK;
 }
 int orc_calc_p_rho_phi(orc_domain* d, int) {

@@ -38,6 +38,7 @@ def olib():
             "orc_rk_prep": (I, [VP, I]),
             "orc_rk_tendency": (I, [VP, I]),
             "orc_small_steps": (I, [VP, I]),
+            "orc_scalar_transport": (I, [VP, I]),
             "orc_calc_p_rho_phi": (I, [VP, I]),
         }
         for n, (r, a) in sig.items():
@@ -108,6 +109,9 @@ class OracleDomain:
 
     def small_steps(self, rk):
         _ok(olib().orc_small_steps(self._h, rk))
+
+    def scalar_transport(self, rk):
+        _ok(olib().orc_scalar_transport(self._h, rk))
 
     def calc_p_rho_phi(self, rk):
         _ok(olib().orc_calc_p_rho_phi(self._h, rk))

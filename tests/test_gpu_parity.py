@@ -16,6 +16,7 @@ STATE = (A.MI_U, A.MI_V, A.MI_W, A.MI_T, A.MI_PH, A.MI_MU, A.MI_P, A.MI_AL)
 PREP = (A.MI_WW, A.MI_ALT, A.MI_PHP)
 TEND = (A.MI_RU_TEND, A.MI_RV_TEND, A.MI_RW_TEND, A.MI_T_TEND, A.MI_PH_TEND)
 STAGE = STATE + (A.MI_RU_M, A.MI_RV_M, A.MI_WW_M, A.MI_MUTS, A.MI_WW)
+CQ = (A.MI_CQU, A.MI_CQV, A.MI_CQW)
 
 
 def _cases():
@@ -134,3 +135,79 @@ def test_autotune_keeps_the_trajectory(ctx):
     gpu.step(4)
     orc.step(4)
     _compare(gpu, orc, STATE, f"after autotune {shapes}")
+
+
+def _moist(name):
+    """A moist case: two moist species (PD at the last stage) and a passive tracer (plain advection), a
+    held tendency on each and a microphysics heating field, so every moist term of the step is non-zero."""
+    if name == "moist3d":
+        case = Case.bubble3d(20, 16, 14, dx=1000.0, ztop=8000.0, dt=6.0, seed=9, bubble_zc=3000.0)
+        kw = dict(time_step_sound=6, damp_opt=3, zdamp=3000.0)
+    else:  # x-z slice with symmetric walls
+        case = Case.bubble3d(40, 1, 20, dx=500.0, ztop=8000.0, dt=3.0, seed=2, bubble_zc=2500.0, bc=A.MI_BC_SYMMETRIC)
+        kw = dict(time_step_sound=6)
+    prm = case.params(n_moist=2, n_scalars=3, moist_adv_opt=1, scalar_adv_opt=0, **kw)
+    g = case.grid
+    rng = np.random.default_rng(11)
+    shp = A.field_shape(A.MI_Q, g.nx, g.ny, g.nz)
+    prof = np.exp(-np.arange(g.nz) / 5.0)[:, None, None]
+    fields = {
+        A.MI_Q + 0: 0.012 * prof * (1 + 0.2 * rng.random(shp)),
+        A.MI_Q + 1: 1e-3 * (rng.random(shp) > 0.8),
+        A.MI_Q + 2: rng.random(shp),
+        A.MI_Q_TENDF + 0: 1e-4 * rng.standard_normal(shp),
+        A.MI_Q_TENDF + 1: 1e-5 * rng.standard_normal(shp),
+        A.MI_Q_TENDF + 2: 1e-3 * rng.standard_normal(shp),
+        A.MI_H_DIABATIC: 1e-3 * rng.standard_normal(shp),
+    }
+    return case, prm, fields
+
+
+def _moist_pair(ctx, name):
+    case, prm, fields = _moist(name)
+    gpu = RK3(ctx, case.grid, prm, case)
+    orc = OracleDomain(case, prm)
+    for fid, a in fields.items():
+        gpu.set_field(fid, a)
+        orc.set_field(fid, a)
+    return gpu, orc
+
+
+@pytest.mark.parametrize("name", ["moist3d", "moist_xz"])
+def test_moist_phases_bitwise(ctx, name):
+    """calc_cq, the moist terms of the tendencies and acoustic steps, and the scalar transport (plain and
+    positive-definite), phase by phase over two steps."""
+    print(f"[parity] moist phases {name}", flush=True)
+    gpu, orc = _moist_pair(ctx, name)
+    QS = tuple(A.MI_Q + n for n in range(3))
+    for step in range(2):
+        gpu.step_begin()
+        orc.step_begin()
+        for rk in (1, 2, 3):
+            gpu.rk_prep(rk)
+            orc.rk_prep(rk)
+            _compare(gpu, orc, PREP + CQ, f"step {step} rk {rk} prep")
+            gpu.rk_tendency(rk)
+            orc.rk_tendency(rk)
+            _compare(gpu, orc, TEND, f"step {step} rk {rk} tendencies")
+            gpu.small_steps(rk)
+            orc.small_steps(rk)
+            _compare(gpu, orc, STAGE, f"step {step} rk {rk} acoustic steps")
+            gpu.scalar_transport(rk)
+            orc.scalar_transport(rk)
+            _compare(gpu, orc, QS, f"step {step} rk {rk} scalar transport")
+            gpu.calc_p_rho_phi(rk)
+            orc.calc_p_rho_phi(rk)
+            _compare(gpu, orc, (A.MI_P, A.MI_AL), f"step {step} rk {rk} calc_p_rho_phi")
+
+
+def test_moist_graph_steps_bitwise(ctx):
+    """The replayed step graphs (one per scalar-buffer parity) over 11 steps, moist, with the oracle."""
+    print("[parity] moist graph steps", flush=True)
+    gpu, orc = _moist_pair(ctx, "moist3d")
+    for chunk in (1, 3, 7):
+        gpu.step(chunk)
+        orc.step(chunk)
+        _compare(gpu, orc, STATE + tuple(A.MI_Q + n for n in range(3)), f"moist after a chunk of {chunk} steps")
+    q = gpu.field(A.MI_Q + 1)
+    assert q.min() >= -1e-12 * 1e-3  # the PD species stays non-negative
