@@ -176,8 +176,12 @@ struct mi_domain {
     if (a.q.moist) mi_fp64::launch_cq(a, sc, ctx->stream);  // calc_cq (large_step.cpp:724-733)
   }
 
-  // LargeStep::tendencies (large_step.cpp:798-861)
-  void rk_tendency() { mi_fp64::launch_tendencies(a, ctx->stream); }
+  // LargeStep::tendencies (large_step.cpp:798-861); at the first stage, with a diffusion driver,
+  // zero_tendf and the sixth-order diffusion into the held tendencies first
+  void rk_tendency(int rk) {
+    if (rk == 1 && a.q.use_tendf) mi_fp64::launch_sixth_dyn(a, ctx->stream);
+    mi_fp64::launch_tendencies(a, ctx->stream);
+  }
 
   // SmallStep::run_stage (small_step.cpp:490-607); with prhophi, calc_p_rho_phi (small_step.cpp:609-617)
   // is fused into the finishing kernel
@@ -217,6 +221,8 @@ struct mi_domain {
       if (is_pd) pd[nd++] = n;
       else plain[np++] = n;
     }
+    // rk_scalar_tend at the first stage: the step's tendency of every scalar, sixth-order diffusion included
+    if (rk == 1 && a.q.use_tendf) mi_fp64::launch_sixth_scalars(a, sc, n_sc, dt_rk, ctx->stream);
     mi_fp64::launch_scalars(a, sc, plain, np, false, dt_rk, ctx->stream);
     mi_fp64::launch_scalars(a, sc, pd, nd, true, dt_rk, ctx->stream);
     for (int n = 0; n < n_sc; ++n) std::swap(sc.q2[n], sc.qn[n]);
@@ -230,7 +236,7 @@ struct mi_domain {
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
       rk_prep();
-      rk_tendency();
+      rk_tendency(rk);
       small_steps(rk, true);
       scalar_transport(rk);
     }
@@ -356,6 +362,9 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   if (p.n_scalars < p.n_moist || p.n_scalars > MI_MAX_SCALARS) throw Error(MI_EINVAL, "n_scalars must be in n_moist..16");
   if ((p.moist_adv_opt != 0 && p.moist_adv_opt != 1) || (p.scalar_adv_opt != 0 && p.scalar_adv_opt != 1))
     throw Error(MI_EINVAL, "moist_adv_opt and scalar_adv_opt must be 0 or 1");
+  if ((p.mapped != 0 && p.mapped != 1) || (p.coriolis != 0 && p.coriolis != 1))
+    throw Error(MI_EINVAL, "mapped and coriolis must be 0 or 1");
+  if (p.diff_6th_opt < 0 || p.diff_6th_opt > 2) throw Error(MI_EINVAL, "diff_6th_opt must be 0, 1 or 2");
   d.ctx = ctx;
   d.grid = g;
   d.prm = p;
@@ -417,6 +426,12 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   add2(f.msfvx, St::V2, MI_MSFVX); add2(f.msfvy, St::V2, MI_MSFVY); add2(f.msfvx_inv, St::V2, MI_MSFVX_INV);
   for (double* m : {f.msftx, f.msfty, f.msfux, f.msfuy, f.msfvx, f.msfvy, f.msfvx_inv})
     mi_fp64::launch_fill(m - d.off, d.n2, 1.0, ctx->stream);
+  // Coriolis and grid rotation: f = e = sina = 0, cosa = 1 unless set
+  add2(f.fcor, St::M2, MI_F); add2(f.ecor, St::M2, MI_E); add2(f.sina, St::M2, MI_SINA); add2(f.cosa, St::M2, MI_COSA);
+  mi_fp64::launch_fill(f.cosa - d.off, d.n2, 1.0, ctx->stream);
+  if (p.diff_6th_opt != 0) {  // the held tendencies of a diffusion driver
+    add3(f.ru_tendf, St::U, -1); add3(f.rv_tendf, St::V, -1); add3(f.rw_tendf, St::W, -1); add3(f.t_tendf, St::M, -1);
+  }
   // moisture and scalars
   if (p.n_moist > 0) {
     add3(f.cqu, St::U, MI_CQU); add3(f.cqv, St::V, MI_CQV); add3(f.cqw, St::W, MI_CQW);
@@ -430,6 +445,7 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
     add3(d.sc.qn[n], St::M, -1);
     add3(d.sc.qtf[n], St::M, MI_Q_TENDF + n);
     add3(d.sc.scale[n], St::M, -1);
+    if (p.diff_6th_opt != 0) add3(d.sc.qtw[n], St::M, -1);
   }
   // base 3-D fields start as profiles (uniform base state)
   const int nz = g.nz;
@@ -479,6 +495,12 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   q.top_lid = p.top_lid;
   q.damp_opt = p.damp_opt;
   q.moist = p.n_moist > 0 ? 1 : 0;
+  q.mapped = p.mapped;
+  q.cor = (p.mapped != 0 || p.coriolis != 0) ? 1 : 0;
+  q.use_tendf = p.diff_6th_opt != 0 ? 1 : 0;
+  q.diff_6th_opt = p.diff_6th_opt;
+  q.diff_6th_factor = p.diff_6th_factor;
+  q.dt = g.dt;
 }
 
 FieldDesc* find(mi_domain* d, int id) {
@@ -568,7 +590,7 @@ int mi_domain_set_field(mi_domain* d, int id, const double* host, int64_t n) {
     }
     FieldDesc* fd = find(d, id);
     if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " cannot be set");
-    const bool base2 = id >= MI_MUB && id <= MI_MSFVX_INV;
+    const bool base2 = id >= MI_MUB && id <= MI_COSA;
     const int nxs = d->nxs(fd->st), nys = d->nys(fd->st), nzs = d->nzs(fd->st);
     const int64_t alloc_n = (fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2) ? d->n2 : d->n3;
     double* base = *fd->slot - d->off;
@@ -643,7 +665,7 @@ int mi_rk_prep(mi_domain* d, int rk) {
 int mi_rk_tendency(mi_domain* d, int rk) {
   return mi_detail::guard([&] {
     if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
-    d->rk_tendency();
+    d->rk_tendency(rk);
     hip_check(hipGetLastError(), "kernel launch");
   });
 }

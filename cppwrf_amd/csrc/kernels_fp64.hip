@@ -366,6 +366,14 @@ MI_COLUMN_KERNEL k_large_prep(Args a) {
 
 // ----------------------------------------------------------------------------- rk_tendency
 // KTendU (large_step.cpp:152-264): advect_u + horizontal pressure gradient + curvature, interior faces
+// MapC::vxgm (large_step.cpp:121-124): the map-factor curvature term at the mass point of index y (2-D y2)
+__device__ __forceinline__ double vxgm(const Args& a, idx_t y, idx_t y2) {
+  const Fields& f = a.f;
+  const idx_t sj = a.g.sj;
+  return H * (f.u2[y] + f.u2[y + 1]) * (f.msfvx[y2 + sj] - f.msfvx[y2]) * a.c.rdy -
+         H * (f.v2[y] + f.v2[y + sj]) * (f.msfuy[y2 + 1] - f.msfuy[y2]) * a.c.rdx;
+}
+
 __global__ void k_tend_u(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -408,8 +416,17 @@ __global__ void k_tend_u(Range r, Args a) {
   dpx += rm * c.rdx * (f.php[x] - f.php[x - 1]) *
          (c.rdnw[k] * (dpn(k + 1) - dpn(k)) - H * (c.c1h[k] * f.mu2[x2 - 1] + c.c1h[k] * f.mu2[x2]));
   t -= (a.q.moist ? f.cqu[x] : 1.0) * dpx;  // kern::MoistCoef: 1 when dry
+  // Coriolis, curvature (large_step.cpp:248-259)
+  const double rv4 = Q * (f.rv[x - 1 + sj] + f.rv[x + sj] + f.rv[x - 1] + f.rv[x]);
   const double rw4 = Q * (f.rw[x - 1 + sk] + f.rw[x - 1] + f.rw[x + sk] + f.rw[x]);
-  t = t - u[x] * kReradius * rw4;
+  if (a.q.cor)
+    t = t + rm * H * (f.fcor[x2] + f.fcor[x2 - 1]) * rv4 -
+        H * (f.ecor[x2] + f.ecor[x2 - 1]) * H * (f.cosa[x2] + f.cosa[x2 - 1]) * rw4;
+  if (a.q.mapped)
+    t = t + H * (vxgm(a, x, x2) + vxgm(a, x - 1, x2 - 1)) * rv4 - u[x] * kReradius * rw4;
+  else
+    t = t - u[x] * kReradius * rw4;
+  if (a.q.use_tendf) t += f.ru_tendf[x] / msfuy;  // rk_addtend_dry
   f.ru_tend[x] = t;
 }
 
@@ -456,8 +473,17 @@ __global__ void k_tend_v(Range r, Args a) {
   dpy += rm * c.rdy * (f.php[x] - f.php[x - sj]) *
          (c.rdnw[k] * (dpn(k + 1) - dpn(k)) - H * (c.c1h[k] * f.mu2[x2 - sj] + c.c1h[k] * f.mu2[x2]));
   t -= (a.q.moist ? f.cqv[x] : 1.0) * dpy;
+  // Coriolis, curvature (large_step.cpp:360-371)
+  const double ru4 = Q * (f.ru[x] + f.ru[x + 1] + f.ru[x - sj] + f.ru[x + 1 - sj]);
   const double rw4 = Q * (f.rw[x - sj + sk] + f.rw[x - sj] + f.rw[x + sk] + f.rw[x]);
-  t = t - rm * v[x] * kReradius * rw4;
+  if (a.q.cor)
+    t = t - rm * H * (f.fcor[x2] + f.fcor[x2 - sj]) * ru4 +
+        rm * H * (f.ecor[x2] + f.ecor[x2 - sj]) * H * (f.sina[x2] + f.sina[x2 - sj]) * rw4;
+  if (a.q.mapped)
+    t = t - H * (vxgm(a, x, x2) + vxgm(a, x - sj, x2 - sj)) * ru4 - rm * v[x] * kReradius * rw4;
+  else
+    t = t - rm * v[x] * kReradius * rw4;
+  if (a.q.use_tendf) t += f.rv_tendf[x] * f.msfvx_inv[x2];  // rk_addtend_dry
   f.rv_tend[x] = t;
 }
 
@@ -511,6 +537,7 @@ __global__ void k_tend_w(Range r, Args a) {
     const double ruw = H * (c.fnm[k] * (f.ru[x] + f.ru[x + 1]) + c.fnp[k] * (f.ru[x - sk] + f.ru[x - sk + 1]));
     const double rvw = H * (c.fnm[k] * (f.rv[x] + f.rv[x + sj]) + c.fnp[k] * (f.rv[x - sk] + f.rv[x - sk + sj]));
     const double rmt = msftx / msfty;
+    if (a.q.cor) t = t + f.ecor[x2] * (f.cosa[x2] * ruw - rmt * f.sina[x2] * rvw);
     const double uw = H * (c.fnm[k] * (f.u2[x] + f.u2[x + 1]) + c.fnp[k] * (f.u2[x - sk] + f.u2[x - sk + 1]));
     const double vw = H * (c.fnm[k] * (f.v2[x] + f.v2[x + sj]) + c.fnp[k] * (f.v2[x - sk] + f.v2[x - sk + sj]));
     t = t + kReradius * (ruw * uw + rmt * rvw * vw);
@@ -520,6 +547,7 @@ __global__ void k_tend_w(Range r, Args a) {
     t += rmsft * kG *
          (cq1 * 2.0 * c.rdnw[nz - 1] * (-f.p[x - sk]) - c.c1f[nz] * f.mu2[x2] - cq2 * (c.c1f[nz] * f.mub[x2] + c.c2f[nz]));
   }
+  if (a.q.use_tendf) t += f.rw_tendf[x] / msfty;  // rk_addtend_dry
   f.rw_tend[x] = t;
 }
 
@@ -553,6 +581,7 @@ __global__ void k_tend_t(Range r, Args a) {
   double rr = -(msftx * c.rdx) * (fx(x + 1) - fx(x));
   if (!g.two_d) rr -= (msftx * c.rdy) * (fy(x + sj) - fy(x));
   rr -= c.rdnw[k] * (fz(k + 1) - fz(k));
+  if (a.q.use_tendf) rr += f.t_tendf[x] / f.msfty[x2];  // rk_addtend_dry
   // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
   if (a.q.moist) rr += (c.c1h[k] * f.mut[x2] + c.c2h[k]) * f.hdia[x] / f.msfty[x2];
   f.t_tend[x] = rr;
@@ -1158,6 +1187,86 @@ __global__ void k_p_rho_phi(Range r, Args a) {
   put<kHMass>(f.p, g, i, j, k * g.sk, kP1000 * mi_libm::pow(base, kCpovcv) - f.pb(i, j, k));
 }
 
+// ----------------------------------------------------------------------------- sixth-order diffusion
+// sixth_order_diffusion (reference src/dyn/diffusion/sixth_order.hpp:13-210; WRF
+// module_big_step_utilities_em.F): the flux differences of one direction, then the flux-limited variant
+__device__ __forceinline__ double d6(double m3, double m2, double m1, double q0, double p1, double p2) {
+  return 10.0 * (q0 - m1) - 5.0 * (p1 - m2) + (p2 - m3);
+}
+// tendency + tend_x + tend_y of one field at index y (2-D index y2): kind 'u', 'v' or mass ('m' and 'w');
+// c1, c2 the level's mass coefficients; mu the column mass (mut for the dynamics, muts for the scalars)
+template <char KIND>
+__device__ __forceinline__ double sixth_at(double tend, const Args& a, const double* q, const double* mu, idx_t y, idx_t y2,
+                                           double c1, double c2, double coef) {
+  const Fields& f = a.f;
+  const idx_t sj = a.g.sj;
+  const int opt = a.q.diff_6th_opt;
+  auto lim = [&](double dfl, double diff) { return (opt == 2 && dfl * diff <= 0.0) ? 0.0 : dfl; };
+  const double dxp0 = lim(d6(q[y - 3], q[y - 2], q[y - 1], q[y], q[y + 1], q[y + 2]), q[y] - q[y - 1]);
+  const double dxp1 = lim(d6(q[y - 2], q[y - 1], q[y], q[y + 1], q[y + 2], q[y + 3]), q[y + 1] - q[y]);
+  const double dyp0 = lim(d6(q[y - 3 * sj], q[y - 2 * sj], q[y - sj], q[y], q[y + sj], q[y + 2 * sj]), q[y] - q[y - sj]);
+  const double dyp1 = lim(d6(q[y - 2 * sj], q[y - sj], q[y], q[y + sj], q[y + 2 * sj], q[y + 3 * sj]), q[y + sj] - q[y]);
+  auto mf = [&](int di, int dj) { return c1 * mu[y2 + di + dj * sj] + c2; };
+  double mx0, mx1, my0, my1, sx, sy;
+  if (KIND == 'u') {
+    mx0 = mf(-1, 0);
+    mx1 = mf(0, 0);
+    my0 = 0.25 * (mf(-1, -1) + mf(0, -1) + mf(-1, 0) + mf(0, 0));
+    my1 = 0.25 * (mf(-1, 0) + mf(0, 0) + mf(-1, 1) + mf(0, 1));
+    sx = f.msfux[y2];
+    sy = f.msfuy[y2];
+  } else if (KIND == 'v') {
+    mx0 = 0.25 * (mf(-1, -1) + mf(0, -1) + mf(-1, 0) + mf(0, 0));
+    mx1 = 0.25 * (mf(0, -1) + mf(1, -1) + mf(0, 0) + mf(1, 0));
+    my0 = mf(0, -1);
+    my1 = mf(0, 0);
+    sx = f.msfvx[y2];
+    sy = f.msfvy[y2];
+  } else {
+    mx0 = 0.5 * (mf(-1, 0) + mf(0, 0));
+    mx1 = 0.5 * (mf(0, 0) + mf(1, 0));
+    my0 = 0.5 * (mf(0, -1) + mf(0, 0));
+    my1 = 0.5 * (mf(0, 0) + mf(0, 1));
+    sx = f.msftx[y2];
+    sy = f.msfty[y2];
+  }
+  const double tx = coef * sx * (mx1 * dxp1 - mx0 * dxp0);
+  const double ty = coef * sy * (my1 * dyp1 - my0 * dyp0);
+  return tend + tx + ty;
+}
+
+// zero_tendf + DiffusionDriver::dynamics (diffusion_driver.cpp:266-281): the held tendencies of u, v, w,
+// theta over the (nx+1) x nyv x (nz+1) points, 0 where sixth_order_diffusion does not reach (w levels 0, nz)
+__global__ void k_sixth_dyn(Range r, Args a, double coef) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  // tendency(i,j,k) = tendency + tend_x + tend_y with the tendency just zeroed
+  if (j < ny && k < nz) f.ru_tendf[x] = sixth_at<'u'>(0.0, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef);
+  if (!g.two_d && i < nx && j < g.nyv && k < nz)
+    f.rv_tendf[x] = sixth_at<'v'>(0.0, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef);
+  if (i < nx && j < ny) {
+    f.rw_tendf[x] = (k >= 1 && k < nz) ? sixth_at<'m'>(0.0, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], coef) : 0.0;
+    if (k < nz) f.t_tendf[x] = sixth_at<'m'>(0.0, a, f.t2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef);
+  }
+}
+
+// zero_tendf, the caller's tendency, then DiffusionDriver::scalar (diffusion_driver.cpp:297-327; mut there
+// is muts, dt the first stage's)
+__global__ void k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Lv& c = a.c;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+#pragma unroll 1
+  for (int n = 0; n < n_sc; ++n) {
+    s.qtw[n][x] = sixth_at<'m'>(0.0 + s.qtf[n][x], a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], coef);
+  }
+}
+
 // ----------------------------------------------------------------------------- calc_cq
 // KCq (reference src/dyn/large_step.cpp:127-148): per species, the two neighbours, WRF's order
 __global__ void k_cq(Range r, Args a, Scalars s) {
@@ -1268,7 +1377,7 @@ __global__ void k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
     if (!g.two_d) adv -= (msftx * c.rdy) * (fl.fy(x + g.sj) - fl.fy(x));
     adv = adv - c.rdnw[k] * (fl.fz(x0, k + 1) - fl.fz(x0, k));
     double tend = adv * msfty;
-    tend += s.qtf[n][x];
+    tend += (a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x];
     put<kHMass>(s.qn[n], g, i, j, k * g.sk, (m_old * s.q1[n][x] + dt_rk * tend) / m_new);
   }
 }
@@ -1284,7 +1393,7 @@ __global__ void k_sc_pd_pre(Range r, Args a, Scalars s, SpList l, double dt_rk) 
 #pragma unroll 1
   for (int q = 0; q < l.n; ++q) {
     const int n = l.sp[q];
-    put<kHMass>(s.q1[n], g, i, j, k * g.sk, (m * s.q1[n][x] + dt_rk * s.qtf[n][x]) / m);
+    put<kHMass>(s.q1[n], g, i, j, k * g.sk, (m * s.q1[n][x] + dt_rk * (a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x]) / m);
   }
 }
 
@@ -1528,6 +1637,17 @@ void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStre
 }
 void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st) {
   dispatch_cfg<FinishL>(cfg, "finish", 1, 1, a.g.nx + 1, a.g.nyv, st, a, prhophi ? 1 : 0, hd_dt);
+}
+void launch_sixth_dyn(const Args& a, hipStream_t st) {
+  const Geo& g = a.g;
+  const double coef = a.q.diff_6th_factor * 0.015625 / (2.0 * a.q.dt);
+  launch3a("sixth", 8, k_sixth_dyn, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, coef);  // u v w t -> 4 tendf
+}
+void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, hipStream_t st) {
+  if (n <= 0) return;
+  const Geo& g = a.g;
+  const double coef = a.q.diff_6th_factor * 0.015625 / (2.0 * dt_rk);
+  launch3a("sixth_sc", 3 * n, k_sixth_sc, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, s, n, coef);  // q, tendf -> tendf
 }
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st) {
   const Geo& g = a.g;

@@ -29,6 +29,9 @@ struct Fields {
   BField pb, alb, phb;
   // moisture (src/dyn/state.hpp:72-74): calc_cq coefficients and the microphysics heating
   double *cqu, *cqv, *cqw, *hdia;
+  // Coriolis and grid rotation (state.hpp:105-106); held tendencies of the first stage (state.hpp:85-88)
+  double *fcor, *ecor, *sina, *cosa;
+  double *ru_tendf, *rv_tendf, *rw_tendf, *t_tendf;
 };
 
 // The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
@@ -39,6 +42,7 @@ struct Scalars {
   double* qtf[MI_MAX_SCALARS];
   double* qn[MI_MAX_SCALARS];
   double* scale[MI_MAX_SCALARS];  // advect_scalar_pd's outflow scale of each species
+  double* qtw[MI_MAX_SCALARS];    // the step's tendency: the caller's plus the sixth-order diffusion
   int n_moist;
 };
 
@@ -49,6 +53,11 @@ struct Params {
   double dampcoef, zdamp;
   int top_lid, damp_opt;
   int moist;  // moist species present: calc_cq coefficients and the heating terms are used
+  int mapped, cor;     // MapC::mapped (curvature with vxgm), MapC::cor (Coriolis terms)
+  int use_tendf;       // held tendencies added (a diffusion driver is attached)
+  int diff_6th_opt;    // 1 linear, 2 monotonic
+  double diff_6th_factor;
+  double dt;           // the large step
 };
 
 struct Args {
@@ -96,6 +105,10 @@ void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st);
 void launch_uv(const Args& a, double dts, hipStream_t st);
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st);
 void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st);
+// sixth-order diffusion of u, v, w, theta into the held tendencies (first stage)
+void launch_sixth_dyn(const Args& a, hipStream_t st);
+// the same for the scalars, into their step tendencies (first stage, after the acoustic steps)
+void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, hipStream_t st);
 // calc_cq (KCq) of the stage state
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st);
 // the transport of species sp[0..n): rk_update_scalar (pd = 0) or rk_update_scalar_pd (pd = 1)

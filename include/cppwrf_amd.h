@@ -81,6 +81,17 @@ typedef struct mi_params {
   int32_t n_scalars;       /* n_moist..MI_MAX_SCALARS */
   int32_t moist_adv_opt;   /* 0: advect_scalar; 1: positive-definite advect_scalar_pd at the last stage */
   int32_t scalar_adv_opt;  /* the same for the passive tracers */
+  /* map projection and Coriolis (reference BaseState::mapped / coriolis, src/dyn/state.hpp:93-107):
+   * mapped = 1 adds the map-factor curvature terms (vxgm) to the u and v tendencies; coriolis = 1 the
+   * Coriolis terms with the fields MI_F, MI_E, MI_SINA, MI_COSA (both as the reference's MapC::cor) */
+  int32_t mapped;
+  int32_t coriolis;
+  /* sixth-order horizontal hyperdiffusion (reference sixth_order_diffusion, src/dyn/diffusion/sixth_order.hpp):
+   * 0 off, 1 linear, 2 monotonic flux-limited; factor = the non-dimensional diff_6th_factor.  Applied at the
+   * first RK stage into the held tendencies of u, v, w, theta and every scalar */
+  int32_t diff_6th_opt;
+  int32_t reserved0;       /* 0 */
+  double diff_6th_factor;
 } mi_params;
 
 #define MI_MAX_MOIST 8    /* reference KCq::kMax (src/dyn/large_step.cpp:128) */
@@ -115,6 +126,8 @@ enum {
   MI_MUB = 32, MI_PB = 33, MI_ALB = 34, MI_PHB = 35, MI_T_INIT = 36,
   MI_MSFTX = 40, MI_MSFTY = 41, MI_MSFUX = 42, MI_MSFUY = 43, MI_MSFVX = 44, MI_MSFVY = 45,
   MI_MSFVX_INV = 46,
+  /* Coriolis parameters 2 Omega sin(lat), 2 Omega cos(lat) and the grid rotation (2-D mass points) */
+  MI_F = 47, MI_E = 48, MI_SINA = 49, MI_COSA = 50,
   /* scalar n (0 <= n < n_scalars) at mass points: MI_Q + n is its state (q_2), MI_Q_TENDF + n its held
    * tendency (input, mass-coupled, held over the step as the reference's q_tendf) */
   MI_Q = 64, MI_Q_TENDF = 96

@@ -178,6 +178,11 @@ struct Dom {
   F cqu, cqv, cqw, hdia;
   // advect_scalar_pd work (scalars.cpp:295-298)
   F sc_scale, dhx, lox, dhy, loy, dhz, loz;
+  // Coriolis and grid rotation (state.hpp:105-107); held tendencies of the first stage (state.hpp:85-88)
+  F fcor, ecor, sina, cosa;
+  F ru_tendf, rv_tendf, rw_tendf, t_tendf;
+  std::vector<F> qtw;  // q_tendf of the step: the caller's tendency plus the sixth-order diffusion
+  bool mapped = false, cor = false, use_tendf = false;
 
   explicit Dom(const mi_grid& gg, const mi_params& pp_, const mi_levels& lv) {
     g = Grid{gg.nx, gg.ny, gg.nz, gg.bc_x, gg.bc_y, gg.dx, gg.dy, gg.dt};
@@ -212,6 +217,71 @@ struct Dom {
     }
     f3(cqu, St::U); f3(cqv, St::V); f3(cqw, St::W); f3(hdia, St::M);
     f3(sc_scale, St::M); f3(dhx, St::U); f3(lox, St::U); f3(dhy, St::V); f3(loy, St::V); f3(dhz, St::W); f3(loz, St::W);
+    for (F* f : {&fcor, &ecor, &sina, &cosa}) f2(*f, St::M);
+    std::fill(cosa.a.begin(), cosa.a.end(), 1.0);
+    f3(ru_tendf, St::U); f3(rv_tendf, St::V); f3(rw_tendf, St::W); f3(t_tendf, St::M);
+    qtw.resize(size_t(n_sc));
+    for (F& f : qtw) f3(f, St::M);
+    mapped = p.mapped != 0;
+    cor = mapped || p.coriolis != 0;  // MapC::from(b, b.mapped || b.coriolis) (large_step.cpp:810)
+    use_tendf = p.diff_6th_opt != 0;   // a DiffusionDriver attached (large_step.cpp:828)
+  }
+  // MapC::vxgm (large_step.cpp:121-124): curvature at mass point (i, j, k) from the uncoupled u, v
+  double vxgm(int i, int j, int k) const {
+    return H * (u2(i, j, k) + u2(i + 1, j, k)) * (msfvx(i, j + 1) - msfvx(i, j)) * c.rdy -
+           H * (v2(i, j, k) + v2(i, j + 1, k)) * (msfuy(i + 1, j) - msfuy(i, j)) * c.rdx;
+  }
+
+  // ------------------------------------------------------------------ sixth_order_diffusion
+  // (diffusion/sixth_order.hpp:13-210), periodic / symmetric loop limits: u over i = 0..nx, v over
+  // j = 0..ny, w levels 1..nz-1, mass points otherwise; coef = factor 0.015625 / (2 dt)
+  static double d6(double m3, double m2, double m1, double q0, double p1, double p2) {
+    return 10.0 * (q0 - m1) - 5.0 * (p1 - m2) + (p2 - m3);
+  }
+  template <class Fld, class Mu>
+  void sixth(char name, const Fld& fld, F& tend, const Mu& mut_, double dt, const std::vector<double>& c1,
+             const std::vector<double>& c2) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz, opt = p.diff_6th_opt;
+    const double coef = p.diff_6th_factor * 0.015625 / (2.0 * dt);
+    auto lim = [&](double dfl, double diff) { return (opt == 2 && dfl * diff <= 0.0) ? 0.0 : dfl; };
+    int i1 = nx, j1 = ny, k0 = 0;
+    if (name == 'u') i1 = nx + 1;
+    if (name == 'v') j1 = ny + 1;
+    if (name == 'w') k0 = 1;
+    pfor(0, i1, 0, j1, k0, nz, [&](int i, int j, int k) {
+      auto q = [&](int a, int b) { return fld(i + a, j + b, k); };
+      const double dxp0 = lim(d6(q(-3, 0), q(-2, 0), q(-1, 0), q(0, 0), q(1, 0), q(2, 0)), q(0, 0) - q(-1, 0));
+      const double dxp1 = lim(d6(q(-2, 0), q(-1, 0), q(0, 0), q(1, 0), q(2, 0), q(3, 0)), q(1, 0) - q(0, 0));
+      const double dyp0 = lim(d6(q(0, -3), q(0, -2), q(0, -1), q(0, 0), q(0, 1), q(0, 2)), q(0, 0) - q(0, -1));
+      const double dyp1 = lim(d6(q(0, -2), q(0, -1), q(0, 0), q(0, 1), q(0, 2), q(0, 3)), q(0, 1) - q(0, 0));
+      auto mf = [&](int a, int b) { return c1[k] * mut_(i + a, j + b) + c2[k]; };
+      double mx0, mx1, my0, my1, sx, sy;
+      if (name == 'u') {
+        mx0 = mf(-1, 0);
+        mx1 = mf(0, 0);
+        my0 = 0.25 * (mf(-1, -1) + mf(0, -1) + mf(-1, 0) + mf(0, 0));
+        my1 = 0.25 * (mf(-1, 0) + mf(0, 0) + mf(-1, 1) + mf(0, 1));
+        sx = msfux(i, j);
+        sy = msfuy(i, j);
+      } else if (name == 'v') {
+        mx0 = 0.25 * (mf(-1, -1) + mf(0, -1) + mf(-1, 0) + mf(0, 0));
+        mx1 = 0.25 * (mf(0, -1) + mf(1, -1) + mf(0, 0) + mf(1, 0));
+        my0 = mf(0, -1);
+        my1 = mf(0, 0);
+        sx = msfvx(i, j);
+        sy = msfvy(i, j);
+      } else {
+        mx0 = 0.5 * (mf(-1, 0) + mf(0, 0));
+        mx1 = 0.5 * (mf(0, 0) + mf(1, 0));
+        my0 = 0.5 * (mf(0, -1) + mf(0, 0));
+        my1 = 0.5 * (mf(0, 0) + mf(0, 1));
+        sx = msftx(i, j);
+        sy = msfty(i, j);
+      }
+      const double tx = coef * sx * (mx1 * dxp1 - mx0 * dxp0);
+      const double ty = coef * sy * (my1 * dyp1 - my0 * dyp0);
+      tend(i, j, k) = tend(i, j, k) + tx + ty;
+    });
   }
   bool moist() const { return n_moist > 0; }
   // kern::MoistCoef (kernels_common.hpp:93-100): 1, 1, 0 when dry
@@ -295,8 +365,20 @@ struct Dom {
   }
 
   // ------------------------------------------------------------------ LargeStep::tendencies (large_step.cpp:798-861)
-  void rk_tendency() {
+  void rk_tendency(int rk) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (rk == 1 && use_tendf) {  // zero_tendf, then DiffusionDriver::dynamics (diffusion_driver.cpp:266-281)
+      for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf}) std::fill(f->a.begin(), f->a.end(), 0.0);
+      auto fu = [&](int i, int j, int k) { return u2(i, j, k); };
+      auto fv = [&](int i, int j, int k) { return v2(i, j, k); };
+      auto fw = [&](int i, int j, int k) { return w2(i, j, k); };
+      auto ft = [&](int i, int j, int k) { return t2(i, j, k); };
+      auto mu_t = [&](int i, int j) { return mut(i, j); };
+      sixth('u', fu, ru_tendf, mu_t, g.dt, c.c1h, c.c2h);
+      if (!g.two_d()) sixth('v', fv, rv_tendf, mu_t, g.dt, c.c1h, c.c2h);
+      sixth('w', fw, rw_tendf, mu_t, g.dt, c.c1f, c.c2f);
+      sixth('m', ft, t_tendf, mu_t, g.dt, c.c1h, c.c2h);
+    }
     const bool do_y = !g.two_d();
     const bool top_lid = p.top_lid != 0;
     const double hup = p.h_adv_order == 5 ? 1.0 : 0.0, vup = p.v_adv_order == 3 ? 1.0 : 0.0;
@@ -339,8 +421,16 @@ struct Dom {
         dpx += rm * rdx * (php(i, j, k) - php(i - 1, j, k)) *
                (c.rdnw[k] * (dpn(i, j, k + 1) - dpn(i, j, k)) - H * (c.c1h[k] * mu2(i - 1, j) + c.c1h[k] * mu2(i, j)));
         t -= mc_cqu(i, j, k) * dpx;
+        const double rv4 = Q * (rv(i - 1, j + 1, k) + rv(i, j + 1, k) + rv(i - 1, j, k) + rv(i, j, k));
         const double rw4 = Q * (rw(i - 1, j, k + 1) + rw(i - 1, j, k) + rw(i, j, k + 1) + rw(i, j, k));
-        t = t - u2(i, j, k) * kReradius * rw4;
+        if (cor)
+          t = t + rm * H * (fcor(i, j) + fcor(i - 1, j)) * rv4 -
+              H * (ecor(i, j) + ecor(i - 1, j)) * H * (cosa(i, j) + cosa(i - 1, j)) * rw4;
+        if (mapped)
+          t = t + H * (vxgm(i, j, k) + vxgm(i - 1, j, k)) * rv4 - u2(i, j, k) * kReradius * rw4;
+        else
+          t = t - u2(i, j, k) * kReradius * rw4;
+        if (use_tendf) t += ru_tendf(i, j, k) / msfuy(i, j);
         ru_tend(i, j, k) = t;
       });
     }
@@ -380,8 +470,16 @@ struct Dom {
         dpy += rm * rdy * (php(i, j, k) - php(i, j - 1, k)) *
                (c.rdnw[k] * (dpn(i, j, k + 1) - dpn(i, j, k)) - H * (c.c1h[k] * mu2(i, j - 1) + c.c1h[k] * mu2(i, j)));
         t -= mc_cqv(i, j, k) * dpy;
+        const double ru4 = Q * (ru(i, j, k) + ru(i + 1, j, k) + ru(i, j - 1, k) + ru(i + 1, j - 1, k));
         const double rw4 = Q * (rw(i, j - 1, k + 1) + rw(i, j - 1, k) + rw(i, j, k + 1) + rw(i, j, k));
-        t = t - rm * v2(i, j, k) * kReradius * rw4;
+        if (cor)
+          t = t - rm * H * (fcor(i, j) + fcor(i, j - 1)) * ru4 +
+              rm * H * (ecor(i, j) + ecor(i, j - 1)) * H * (sina(i, j) + sina(i, j - 1)) * rw4;
+        if (mapped)
+          t = t - H * (vxgm(i, j, k) + vxgm(i, j - 1, k)) * ru4 - rm * v2(i, j, k) * kReradius * rw4;
+        else
+          t = t - rm * v2(i, j, k) * kReradius * rw4;
+        if (use_tendf) t += rv_tendf(i, j, k) * msfvx_inv(i, j);
         rv_tend(i, j, k) = t;
       });
     }
@@ -430,6 +528,7 @@ struct Dom {
           const double rvw =
               H * (c.fnm[k] * (rv(i, j, k) + rv(i, j + 1, k)) + c.fnp[k] * (rv(i, j, k - 1) + rv(i, j + 1, k - 1)));
           const double rmt = msftx(i, j) / msfty(i, j);
+          if (cor) t = t + ecor(i, j) * (cosa(i, j) * ruw - rmt * sina(i, j) * rvw);
           const double uw =
               H * (c.fnm[k] * (u2(i, j, k) + u2(i + 1, j, k)) + c.fnp[k] * (u2(i, j, k - 1) + u2(i + 1, j, k - 1)));
           const double vw =
@@ -441,6 +540,7 @@ struct Dom {
           t += rmsft * kG * (cq1 * 2.0 * c.rdnw[nz - 1] * (-pp(i, j, nz - 1)) - c.c1f[nz] * mu2(i, j) -
                              cq2 * (c.c1f[nz] * mub(i, j) + c.c2f[nz]));
         }
+        if (use_tendf) t += rw_tendf(i, j, k) / msfty(i, j);
         rw_tend(i, j, k) = t;
       });
     }
@@ -466,6 +566,7 @@ struct Dom {
         double r = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
         if (do_y) r -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
         r -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+        if (use_tendf) r += t_tendf(i, j, k) / msfty(i, j);
         // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
         if (moist()) r += (c.c1h[k] * mut(i, j) + c.c2h[k]) * hdia(i, j, k) / msfty(i, j);
         t_tend(i, j, k) = r;
@@ -878,7 +979,7 @@ struct Dom {
     const Fl f{*this, q2[n], q1[n], dt_rk, hup, vup, false};
     pfor(0, g.nx, 0, g.ny, 0, g.nz, [&](int i, int j, int k) {
       double tend = f.adv(i, j, k) * msfty(i, j);
-      tend += qtf[n](i, j, k);
+      tend += (use_tendf ? qtw : qtf)[n](i, j, k);
       qnew[n](i, j, k) = (m_full(k, mub(i, j), mu1(i, j)) * q1[n](i, j, k) + dt_rk * tend) / m_full(k, mub(i, j), mu2(i, j));
     });
   }
@@ -890,7 +991,7 @@ struct Dom {
     F& qo = q1[n];
     pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
       const double m = m_full(k, mub(i, j), mu1(i, j));
-      qo(i, j, k) = (m * qo(i, j, k) + dt_rk * qtf[n](i, j, k)) / m;
+      qo(i, j, k) = (m * qo(i, j, k) + dt_rk * (use_tendf ? qtw : qtf)[n](i, j, k)) / m;
     });
     fill(qo, g);
     const Fl f{*this, q2[n], qo, dt_rk, hup, vup, true};
@@ -959,6 +1060,16 @@ struct Dom {
   void scalars(int rk) {
     if (n_sc == 0) return;
     const double dt_rk = rk == 1 ? g.dt / 3.0 : (rk == 2 ? 0.5 * g.dt : g.dt);
+    if (rk == 1 && use_tendf) {  // zero_tendf + the caller's tendency, then DiffusionDriver::scalar (mut = muts)
+      for (int n = 0; n < n_sc; ++n) {
+        F& w = qtw[size_t(n)];
+        pfor(0, g.nx, 0, g.ny, 0, g.nz, [&](int i, int j, int k) { w(i, j, k) = 0.0 + qtf[size_t(n)](i, j, k); });
+        const F& qq = q2[size_t(n)];
+        auto fq = [&](int i, int j, int k) { return qq(i, j, k); };
+        auto mu_s = [&](int i, int j) { return muts(i, j); };
+        sixth('m', fq, w, mu_s, dt_rk, c.c1h, c.c2h);
+      }
+    }
     std::vector<int> pd_n, plain_n;
     for (int n = 0; n < n_sc; ++n) {
       const bool pd = rk == 3 && (n < n_moist ? p.moist_adv_opt : p.scalar_adv_opt) == 1;
@@ -1005,7 +1116,7 @@ struct Dom {
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
       rk_prep();
-      rk_tendency();
+      rk_tendency(rk);
       small_steps(rk);
       scalars(rk);
       calc_p_rho_phi();
@@ -1024,6 +1135,7 @@ struct Dom {
       case MI_T_INIT: return &t_init; case MI_MSFTX: return &msftx; case MI_MSFTY: return &msfty;
       case MI_MSFUX: return &msfux; case MI_MSFUY: return &msfuy; case MI_MSFVX: return &msfvx;
       case MI_MSFVY: return &msfvy; case MI_MSFVX_INV: return &msfvx_inv;
+      case MI_F: return &fcor; case MI_E: return &ecor; case MI_SINA: return &sina; case MI_COSA: return &cosa;
       case MI_CQU: return &cqu; case MI_CQV: return &cqv; case MI_CQW: return &cqw; case MI_H_DIABATIC: return &hdia;
       default:
         if (id >= MI_Q && id < MI_Q + n_sc) return &q2[size_t(id - MI_Q)];
@@ -1033,7 +1145,7 @@ struct Dom {
   }
 };
 
-bool is_base(int id) { return id >= MI_MUB && id <= MI_MSFVX_INV; }
+bool is_base(int id) { return id >= MI_MUB && id <= MI_COSA; }
 
 int fail(const std::string& m, int code = MI_EINVAL) {
   g_err = m;
@@ -1063,6 +1175,7 @@ int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv,
     if (p->damp_opt != 0 && p->damp_opt != 3) return fail("oracle: damp_opt 0 or 3 only");
     if (p->n_moist < 0 || p->n_moist > MI_MAX_MOIST || p->n_scalars < p->n_moist || p->n_scalars > MI_MAX_SCALARS)
       return fail("bad n_moist / n_scalars");
+    if (p->diff_6th_opt < 0 || p->diff_6th_opt > 2) return fail("diff_6th_opt must be 0, 1 or 2");
     auto* o = new orc_domain;
     o->d = std::make_unique<Dom>(*g, *p, *lv);
     if (o->d->n_sound < 2 || o->d->n_sound % 2) {
@@ -1138,8 +1251,9 @@ int orc_rk_prep(orc_domain* d, int) {
   d->d->rk_prep();
   return MI_OK;
 }
-int orc_rk_tendency(orc_domain* d, int) {
-  d->d->rk_tendency();
+int orc_rk_tendency(orc_domain* d, int rk) {
+  if (rk < 1 || rk > 3) return fail("rk must be 1, 2 or 3");
+  d->d->rk_tendency(rk);
   return MI_OK;
 }
 int orc_small_steps(orc_domain* d, int rk) {

@@ -211,3 +211,64 @@ def test_moist_graph_steps_bitwise(ctx):
         _compare(gpu, orc, STATE + tuple(A.MI_Q + n for n in range(3)), f"moist after a chunk of {chunk} steps")
     q = gpu.field(A.MI_Q + 1)
     assert q.min() >= -1e-12 * 1e-3  # the PD species stays non-negative
+
+
+def _mapped_case():
+    """Map factors varying in [0.95, 1.05] (a Lambert-like pattern at every staggering), Coriolis and grid
+    rotation fields, sixth-order diffusion (monotonic) and moisture: every optional term of the step."""
+    case = Case.bubble3d(20, 16, 14, dx=1000.0, ztop=8000.0, dt=6.0, seed=21, bubble_zc=3000.0)
+    g = case.grid
+    prm = case.params(time_step_sound=6, mapped=1, coriolis=1, diff_6th_opt=2, diff_6th_factor=0.12,
+                      n_moist=1, n_scalars=2, damp_opt=3, zdamp=3000.0)
+
+    def m(xf, yf):
+        return 1.0 + 0.05 * np.sin(np.pi * xf / g.nx) * np.cos(np.pi * yf / g.ny)
+
+    ii, jj = np.meshgrid(np.arange(g.nx + 1, dtype=float), np.arange(g.ny + 1, dtype=float))
+    mass = m(ii[:g.ny, :g.nx] + 0.5, jj[:g.ny, :g.nx] + 0.5)[None]
+    uf = m(ii[:g.ny, :], jj[:g.ny, :] + 0.5)[None]
+    vf = m(ii[:, :g.nx] + 0.5, jj[:, :g.nx])[None]
+    rng = np.random.default_rng(5)
+    shp = A.field_shape(A.MI_Q, g.nx, g.ny, g.nz)
+    lat = np.deg2rad(30 + 5 * (jj[:g.ny, :g.nx] / g.ny))[None]
+    fields = {
+        A.MI_MSFTX: mass, A.MI_MSFTY: 1.01 * mass, A.MI_MSFUX: uf, A.MI_MSFUY: 1.01 * uf, A.MI_MSFVX: vf,
+        A.MI_MSFVY: 1.01 * vf, A.MI_MSFVX_INV: 1.0 / vf,
+        A.MI_F: 2 * 7.292e-5 * np.sin(lat), A.MI_E: 2 * 7.292e-5 * np.cos(lat),
+        A.MI_SINA: 0.1 * np.ones_like(mass), A.MI_COSA: np.sqrt(0.99) * np.ones_like(mass),
+        A.MI_Q + 0: 0.01 * np.exp(-np.arange(g.nz) / 5.0)[:, None, None] * (1 + 0.1 * rng.random(shp)),
+        A.MI_Q + 1: (rng.random(shp) > 0.6).astype(float),
+        A.MI_Q_TENDF + 0: 1e-5 * rng.standard_normal(shp),
+    }
+    return case, prm, fields
+
+
+def test_mapped_coriolis_diffusion_phases_bitwise(ctx):
+    print("[parity] mapped + coriolis + sixth-order diffusion + moisture phases", flush=True)
+    case, prm, fields = _mapped_case()
+    gpu = RK3(ctx, case.grid, prm, case)
+    orc = OracleDomain(case, prm)
+    for fid, a in fields.items():
+        gpu.set_field(fid, a)
+        orc.set_field(fid, a)
+    QS = (A.MI_Q, A.MI_Q + 1)
+    for step in range(2):
+        gpu.step_begin()
+        orc.step_begin()
+        for rk in (1, 2, 3):
+            gpu.rk_prep(rk)
+            orc.rk_prep(rk)
+            gpu.rk_tendency(rk)
+            orc.rk_tendency(rk)
+            _compare(gpu, orc, TEND, f"step {step} rk {rk} tendencies")
+            gpu.small_steps(rk)
+            orc.small_steps(rk)
+            _compare(gpu, orc, STAGE, f"step {step} rk {rk} acoustic steps")
+            gpu.scalar_transport(rk)
+            orc.scalar_transport(rk)
+            _compare(gpu, orc, QS, f"step {step} rk {rk} scalar transport")
+            gpu.calc_p_rho_phi(rk)
+            orc.calc_p_rho_phi(rk)
+    gpu.step(5)
+    orc.step(5)
+    _compare(gpu, orc, STATE + QS, "after 5 graph steps")

@@ -1,0 +1,69 @@
+"""The oracle's sixth-order diffusion and Coriolis terms, pinned by closed forms.
+
+- Sixth-order diffusion of a Fourier mode: the flux difference 10(q0-q-1) - 5(q1-q-2) + (q2-q-3) of
+  exp(i theta j) is 32 i sin^5(theta/2) at the face, so the tendency is -64 coef mu sin^6(theta/2) q with
+  coef = factor / 64 / (2 dt_rk).  A scalar wave in a fluid at rest, held over the three RK stages with
+  dt_rk = dt/3 (scalar diffusion uses the first stage's length), is therefore multiplied by
+  1 - 1.5 factor sin^6(theta/2) in one step.  The monotonic variant keeps every flux of the 2-dx wave (each
+  is aligned with its gradient), so it damps that wave the same way; on a top hat it makes no new extrema.
+- Coriolis on an f-plane: a uniform flow u0 turns inertially, v = -u0 sin(f t), u = u0 cos(f t).
+"""
+import numpy as np
+import pytest
+
+from cppwrf_amd import _abi as A
+from cppwrf_amd.dyn import Case
+from oracle_lib import OracleDomain, olib
+
+
+@pytest.fixture(scope="module", autouse=True)
+def threads():
+    olib().orc_set_threads(8)
+
+
+@pytest.mark.parametrize("opt,wavelength", [(1, 2), (1, 4), (1, 8), (2, 2)])
+def test_sixth_order_damps_a_scalar_wave_by_the_closed_form(opt, wavelength):
+    factor = 0.12
+    case = Case.rest_state(nx=16, ny=8, nz=10)
+    g = case.grid
+    o = OracleDomain(case, case.params(time_step_sound=4, n_scalars=1, scalar_adv_opt=0, diff_6th_opt=opt,
+                                       diff_6th_factor=factor))
+    theta = 2 * np.pi / wavelength
+    q0 = 1.0 + 0.01 * np.cos(theta * np.arange(g.nx))[None, None, :] * np.ones(A.field_shape(A.MI_Q, g.nx, g.ny, g.nz))
+    o.set_field(A.MI_Q, q0)
+    o.step(1)
+    q = o.field(A.MI_Q)
+    want = 1.0 + 0.01 * (1 - 1.5 * factor * np.sin(theta / 2) ** 6) * (q0 - 1.0) / 0.01
+    assert np.abs(q - want).max() < 1e-10
+
+
+def test_monotonic_sixth_order_makes_no_new_extrema():
+    """diff_6th_opt 2 zeroes every flux that runs up its gradient, so each face flux is down-gradient: a
+    top hat is smoothed without new extrema, while the linear form over- and undershoots."""
+    case = Case.rest_state(nx=16, ny=8, nz=10)
+    g = case.grid
+    hat = (np.abs(np.arange(g.nx) - 7.5) < 3).astype(float)
+    q0 = 1.0 + 0.5 * hat[None, None, :] * np.ones(A.field_shape(A.MI_Q, g.nx, g.ny, g.nz))
+    out = {}
+    for opt in (1, 2):
+        o = OracleDomain(case, case.params(time_step_sound=4, n_scalars=1, scalar_adv_opt=0, diff_6th_opt=opt))
+        o.set_field(A.MI_Q, q0)
+        o.step(1)
+        out[opt] = o.field(A.MI_Q)
+    assert out[1].max() > 1.5 + 1e-4 and out[1].min() < 1.0 - 1e-4
+    assert out[2].max() <= 1.5 + 1e-13 and out[2].min() >= 1.0 - 1e-13
+    assert np.abs(out[2] - q0).max() > 1e-3  # it does diffuse
+
+
+def test_inertial_oscillation():
+    f = 1e-4
+    case = Case.rest_state(nx=12, ny=10, nz=10, u0=5.0)
+    g = case.grid
+    o = OracleDomain(case, case.params(time_step_sound=4, coriolis=1))
+    o.set_field(A.MI_F, np.full((1, g.ny, g.nx), f))
+    nsteps = 20
+    o.step(nsteps)
+    t = nsteps * g.dt
+    u, v = o.field(A.MI_U), o.field(A.MI_V)
+    assert np.abs(v[:, 1:-1, :] + 5.0 * np.sin(f * t)).max() < 2e-4 * 5.0 * f * t
+    assert np.abs(u - 5.0 * np.cos(f * t)).max() < 1e-5
