@@ -28,7 +28,7 @@ MI_RU_TEND, MI_RV_TEND, MI_RW_TEND, MI_T_TEND, MI_PH_TEND = range(15, 20)
 MI_MUB, MI_PB, MI_ALB, MI_PHB, MI_T_INIT = range(32, 37)
 MI_MSFTX, MI_MSFTY, MI_MSFUX, MI_MSFUY, MI_MSFVX, MI_MSFVY, MI_MSFVX_INV = range(40, 47)
 MI_CQU, MI_CQV, MI_CQW, MI_H_DIABATIC = range(20, 24)
-MI_F, MI_E, MI_SINA, MI_COSA = range(47, 51)
+MI_F, MI_E, MI_SINA, MI_COSA, MI_HT = range(47, 52)
 MI_Q, MI_Q_TENDF = 64, 96
 MI_MAX_MOIST, MI_MAX_SCALARS = 8, 16
 ABI_VERSION = 2
@@ -43,7 +43,7 @@ STAGGER = {
     MI_MUB: "M2", MI_PB: "M", MI_ALB: "M", MI_PHB: "W", MI_T_INIT: "M",
     MI_CQU: "U", MI_CQV: "V", MI_CQW: "W", MI_H_DIABATIC: "M",
     MI_MSFTX: "M2", MI_MSFTY: "M2", MI_MSFUX: "U2", MI_MSFUY: "U2", MI_MSFVX: "V2", MI_MSFVY: "V2",
-    MI_MSFVX_INV: "V2", MI_F: "M2", MI_E: "M2", MI_SINA: "M2", MI_COSA: "M2",
+    MI_MSFVX_INV: "V2", MI_F: "M2", MI_E: "M2", MI_SINA: "M2", MI_COSA: "M2", MI_HT: "M2",
 }
 for _n in range(MI_MAX_SCALARS):
     STAGGER[MI_Q + _n] = "M"
@@ -88,7 +88,7 @@ class MiParams(C.Structure):
                 ("hypsometric_opt", C.c_int32), ("zdamp", C.c_double), ("dampcoef", C.c_double),
                 ("time_step_sound", C.c_int32), ("precision", C.c_int32), ("n_moist", C.c_int32),
                 ("n_scalars", C.c_int32), ("moist_adv_opt", C.c_int32), ("scalar_adv_opt", C.c_int32),
-                ("mapped", C.c_int32), ("coriolis", C.c_int32), ("diff_6th_opt", C.c_int32), ("reserved0", C.c_int32),
+                ("mapped", C.c_int32), ("coriolis", C.c_int32), ("diff_6th_opt", C.c_int32), ("terrain", C.c_int32),
                 ("diff_6th_factor", C.c_double)]
 
 
@@ -112,7 +112,7 @@ def default_params(**kw) -> MiParams:
     p = MiParams(h_adv_order=5, v_adv_order=3, top_lid=0, w_damping=0, w_crit_cfl=1.0, epssm=0.1, smdiv=0.1,
                  emdiv=0.01, damp_opt=0, hypsometric_opt=1, zdamp=5000.0, dampcoef=0.2, time_step_sound=0,
                  precision=MI_FP64, n_moist=0, n_scalars=0, moist_adv_opt=1, scalar_adv_opt=1, mapped=0,
-                 coriolis=0, diff_6th_opt=0, reserved0=0, diff_6th_factor=0.12)
+                 coriolis=0, diff_6th_opt=0, terrain=0, diff_6th_factor=0.12)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

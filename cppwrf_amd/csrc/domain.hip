@@ -7,6 +7,7 @@
 // every field's x pass still precedes its y pass, as the reference's batched fill (halo.cpp:133-190).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -240,6 +241,7 @@ struct mi_domain {
       small_steps(rk, true);
       scalar_transport(rk);
     }
+    if (a.q.terrain) mi_fp64::launch_set_w_surface(a, ctx->stream);  // rk3.cpp:163
   }
 
   // the fields a step reads from the previous one (everything else is recomputed within a step)
@@ -365,6 +367,7 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   if ((p.mapped != 0 && p.mapped != 1) || (p.coriolis != 0 && p.coriolis != 1))
     throw Error(MI_EINVAL, "mapped and coriolis must be 0 or 1");
   if (p.diff_6th_opt < 0 || p.diff_6th_opt > 2) throw Error(MI_EINVAL, "diff_6th_opt must be 0, 1 or 2");
+  if (p.terrain != 0 && p.terrain != 1) throw Error(MI_EINVAL, "terrain must be 0 or 1");
   d.ctx = ctx;
   d.grid = g;
   d.prm = p;
@@ -447,15 +450,24 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
     add3(d.sc.scale[n], St::M, -1);
     if (p.diff_6th_opt != 0) add3(d.sc.qtw[n], St::M, -1);
   }
-  // base 3-D fields start as profiles (uniform base state)
+  // base 3-D fields: profiles (flat terrain, horizontally uniform base state) or full arrays (terrain)
   const int nz = g.nz;
-  d.pb_buf = d.alloc(nz + 1);
-  d.alb_buf = d.alloc(nz + 1);
-  d.phb_buf = d.alloc(nz + 1);
-  d.t_init_buf = d.alloc(nz + 1);
-  f.pb = BField{d.pb_buf, 0, 0, 1};
-  f.alb = BField{d.alb_buf, 0, 0, 1};
-  f.phb = BField{d.phb_buf, 0, 0, 1};
+  if (p.terrain) {
+    add3(d.pb_buf, St::M, MI_PB); add3(d.alb_buf, St::M, MI_ALB); add3(d.phb_buf, St::W, MI_PHB);
+    add3(d.t_init_buf, St::M, MI_T_INIT);
+    add2(f.ht, St::M2, MI_HT);
+    f.pb = BField{d.pb_buf, 1, G.sj, G.sk};
+    f.alb = BField{d.alb_buf, 1, G.sj, G.sk};
+    f.phb = BField{d.phb_buf, 1, G.sj, G.sk};
+  } else {
+    d.pb_buf = d.alloc(nz + 1);
+    d.alb_buf = d.alloc(nz + 1);
+    d.phb_buf = d.alloc(nz + 1);
+    d.t_init_buf = d.alloc(nz + 1);
+    f.pb = BField{d.pb_buf, 0, 0, 1};
+    f.alb = BField{d.alb_buf, 0, 0, 1};
+    f.phb = BField{d.phb_buf, 0, 0, 1};
+  }
   {
     void* fl = nullptr;
     hip_check(hipMalloc(&fl, 64 * sizeof(int)), "hipMalloc");
@@ -501,6 +513,7 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   q.diff_6th_opt = p.diff_6th_opt;
   q.diff_6th_factor = p.diff_6th_factor;
   q.dt = g.dt;
+  q.terrain = p.terrain;
 }
 
 FieldDesc* find(mi_domain* d, int id) {
@@ -579,8 +592,19 @@ int mi_domain_set_field(mi_domain* d, int id, const double* host, int64_t n) {
     if (!d || !host) throw Error(MI_EINVAL, "null argument");
     hipStream_t st = d->ctx->stream;
     const int nz = d->grid.nz;
+    const bool base3 = id == MI_PB || id == MI_ALB || id == MI_PHB || id == MI_T_INIT;
+    // a base 3-D profile on a domain with terrain: repeated over every column, halos included
+    if (base3 && d->prm.terrain && n == nz + (id == MI_PHB ? 1 : 0)) {
+      FieldDesc* fd = find(d, id);
+      std::vector<double> h(static_cast<size_t>(d->n3), 0.0);
+      for (int64_t e = 0; e < d->n3; ++e) h[size_t(e)] = host[std::min<int64_t>(e / d->a.g.sk, n - 1)];
+      hip_check(hipMemcpyAsync(*fd->slot - d->off, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st),
+                "hipMemcpyAsync");
+      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+      return;
+    }
     // base 3-D profiles
-    if (id == MI_PB || id == MI_ALB || id == MI_PHB || id == MI_T_INIT) {
+    if (base3 && !d->prm.terrain) {
       const int nzs = nz + (id == MI_PHB ? 1 : 0);
       if (n != nzs) throw Error(MI_EINVAL, "base 3-D fields take a profile of nz (phb: nz+1) values");
       double* dst = id == MI_PB ? d->pb_buf : id == MI_ALB ? d->alb_buf : id == MI_PHB ? d->phb_buf : d->t_init_buf;
@@ -590,7 +614,7 @@ int mi_domain_set_field(mi_domain* d, int id, const double* host, int64_t n) {
     }
     FieldDesc* fd = find(d, id);
     if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " cannot be set");
-    const bool base2 = id >= MI_MUB && id <= MI_COSA;
+    const bool base2 = id == MI_MUB || (id >= MI_MSFTX && id <= MI_HT);
     const int nxs = d->nxs(fd->st), nys = d->nys(fd->st), nzs = d->nzs(fd->st);
     const int64_t alloc_n = (fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2) ? d->n2 : d->n3;
     double* base = *fd->slot - d->off;

@@ -1038,7 +1038,22 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     const double msft_inv = 1.0 / msfty;
     double wk;
     if (k == 0) {
+      // the surface w'': 0 on flat ground; with terrain, from the small-step u'', v'' and the slope of ht
+      // (small_step.cpp:339-353)
       wk = 0;
+      if (a.q.terrain) {
+        const double* ht = f.ht;
+        const double* us = f.u2;
+        const double* vs = f.v2;
+        wk = msftx * H * c.rdx *
+             ((ht[x2 + 1] - ht[x2]) * (c.cf1 * us[x + 1] + c.cf2 * us[x + 1 + sk] + c.cf3 * us[x + 1 + 2 * sk]) +
+              (ht[x2] - ht[x2 - 1]) * (c.cf1 * us[x] + c.cf2 * us[x + sk] + c.cf3 * us[x + 2 * sk]));
+        if (!g.two_d)
+          wk = msfty * H * c.rdy *
+                   ((ht[x2 + sj] - ht[x2]) * (c.cf1 * vs[x + sj] + c.cf2 * vs[x + sj + sk] + c.cf3 * vs[x + sj + 2 * sk]) +
+                    (ht[x2] - ht[x2 - sj]) * (c.cf1 * vs[x] + c.cf2 * vs[x + sk] + c.cf3 * vs[x + 2 * sk])) +
+               wk;
+      }
     } else {
       const double r_k = LDS(R, k, ci), r_km1 = LDS(R, k - 1, ci);
       if (mk) {
@@ -1265,6 +1280,27 @@ __global__ void k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
   for (int n = 0; n < n_sc; ++n) {
     s.qtw[n][x] = sixth_at<'m'>(0.0 + s.qtf[n][x], a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], coef);
   }
+}
+
+// ----------------------------------------------------------------------------- set_w_surface
+// KSetWSurface (rk3.cpp:173-190), fill = false: the kinematic surface w from the new u, v; with the
+// halo images of w (the reference refreshes w's halo next)
+__global__ void k_set_w_surface(Range r, Args a) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const idx_t x = g.at(i, j, 0), x2 = g.at2(i, j), sj = g.sj, sk = g.sk;
+  const double *ht = f.ht, *u = f.u2, *v = f.v2;
+  (void)k;  // one level
+  const double ws =
+      f.msfty[x2] * 0.5 * c.rdy *
+          ((ht[x2 + sj] - ht[x2]) * (c.cf1 * v[x + sj] + c.cf2 * v[x + sj + sk] + c.cf3 * v[x + sj + 2 * sk]) +
+           (ht[x2] - ht[x2 - sj]) * (c.cf1 * v[x] + c.cf2 * v[x + sk] + c.cf3 * v[x + 2 * sk])) +
+      f.msftx[x2] * 0.5 * c.rdx *
+          ((ht[x2 + 1] - ht[x2]) * (c.cf1 * u[x + 1] + c.cf2 * u[x + 1 + sk] + c.cf3 * u[x + 1 + 2 * sk]) +
+           (ht[x2] - ht[x2 - 1]) * (c.cf1 * u[x] + c.cf2 * u[x + sk] + c.cf3 * u[x + 2 * sk]));
+  put<kHMass>(f.w2, g, i, j, 0, ws);
 }
 
 // ----------------------------------------------------------------------------- calc_cq
@@ -1648,6 +1684,10 @@ void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, 
   const Geo& g = a.g;
   const double coef = a.q.diff_6th_factor * 0.015625 / (2.0 * dt_rk);
   launch3a("sixth_sc", 3 * n, k_sixth_sc, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, s, n, coef);  // q, tendf -> tendf
+}
+void launch_set_w_surface(const Args& a, hipStream_t st) {
+  const Geo& g = a.g;
+  launch3a("set_w_surface", 7, k_set_w_surface, Range{0, g.nx, 0, g.ny, 0, 1}, st, a);  // u, v (3 levels each) -> w (1)
 }
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st) {
   const Geo& g = a.g;

@@ -30,7 +30,7 @@ struct Fields {
   // moisture (src/dyn/state.hpp:72-74): calc_cq coefficients and the microphysics heating
   double *cqu, *cqv, *cqw, *hdia;
   // Coriolis and grid rotation (state.hpp:105-106); held tendencies of the first stage (state.hpp:85-88)
-  double *fcor, *ecor, *sina, *cosa;
+  double *fcor, *ecor, *sina, *cosa, *ht;
   double *ru_tendf, *rv_tendf, *rw_tendf, *t_tendf;
 };
 
@@ -58,6 +58,7 @@ struct Params {
   int diff_6th_opt;    // 1 linear, 2 monotonic
   double diff_6th_factor;
   double dt;           // the large step
+  int terrain;         // the surface w follows the terrain (ht)
 };
 
 struct Args {
@@ -109,6 +110,8 @@ void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream
 void launch_sixth_dyn(const Args& a, hipStream_t st);
 // the same for the scalars, into their step tendencies (first stage, after the acoustic steps)
 void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, hipStream_t st);
+// set_w_surface (fill = false) after a step, with terrain
+void launch_set_w_surface(const Args& a, hipStream_t st);
 // calc_cq (KCq) of the stage state
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st);
 // the transport of species sp[0..n): rk_update_scalar (pd = 0) or rk_update_scalar_pd (pd = 1)

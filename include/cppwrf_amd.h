@@ -90,7 +90,10 @@ typedef struct mi_params {
    * 0 off, 1 linear, 2 monotonic flux-limited; factor = the non-dimensional diff_6th_factor.  Applied at the
    * first RK stage into the held tendencies of u, v, w, theta and every scalar */
   int32_t diff_6th_opt;
-  int32_t reserved0;       /* 0 */
+  /* 1: terrain (reference BaseState not flat): the base state pb, alb, phb, t_init is given in full
+   * (nxs*nys*nzs values), the terrain height MI_HT sets the lower boundary w (advance_w's surface w''
+   * and set_w_surface at the end of each step, src/dyn/small_step.cpp:339-353, rk3.cpp:173-200) */
+  int32_t terrain;
   double diff_6th_factor;
 } mi_params;
 
@@ -128,6 +131,7 @@ enum {
   MI_MSFVX_INV = 46,
   /* Coriolis parameters 2 Omega sin(lat), 2 Omega cos(lat) and the grid rotation (2-D mass points) */
   MI_F = 47, MI_E = 48, MI_SINA = 49, MI_COSA = 50,
+  MI_HT = 51, /* terrain height, m (2-D mass points; with terrain) */
   /* scalar n (0 <= n < n_scalars) at mass points: MI_Q + n is its state (q_2), MI_Q_TENDF + n its held
    * tendency (input, mass-coupled, held over the step as the reference's q_tendf) */
   MI_Q = 64, MI_Q_TENDF = 96

@@ -179,10 +179,10 @@ struct Dom {
   // advect_scalar_pd work (scalars.cpp:295-298)
   F sc_scale, dhx, lox, dhy, loy, dhz, loz;
   // Coriolis and grid rotation (state.hpp:105-107); held tendencies of the first stage (state.hpp:85-88)
-  F fcor, ecor, sina, cosa;
+  F fcor, ecor, sina, cosa, ht;
   F ru_tendf, rv_tendf, rw_tendf, t_tendf;
   std::vector<F> qtw;  // q_tendf of the step: the caller's tendency plus the sixth-order diffusion
-  bool mapped = false, cor = false, use_tendf = false;
+  bool mapped = false, cor = false, use_tendf = false, terrain = false;
 
   explicit Dom(const mi_grid& gg, const mi_params& pp_, const mi_levels& lv) {
     g = Grid{gg.nx, gg.ny, gg.nz, gg.bc_x, gg.bc_y, gg.dx, gg.dy, gg.dt};
@@ -217,7 +217,7 @@ struct Dom {
     }
     f3(cqu, St::U); f3(cqv, St::V); f3(cqw, St::W); f3(hdia, St::M);
     f3(sc_scale, St::M); f3(dhx, St::U); f3(lox, St::U); f3(dhy, St::V); f3(loy, St::V); f3(dhz, St::W); f3(loz, St::W);
-    for (F* f : {&fcor, &ecor, &sina, &cosa}) f2(*f, St::M);
+    for (F* f : {&fcor, &ecor, &sina, &cosa, &ht}) f2(*f, St::M);
     std::fill(cosa.a.begin(), cosa.a.end(), 1.0);
     f3(ru_tendf, St::U); f3(rv_tendf, St::V); f3(rw_tendf, St::W); f3(t_tendf, St::M);
     qtw.resize(size_t(n_sc));
@@ -225,6 +225,7 @@ struct Dom {
     mapped = p.mapped != 0;
     cor = mapped || p.coriolis != 0;  // MapC::from(b, b.mapped || b.coriolis) (large_step.cpp:810)
     use_tendf = p.diff_6th_opt != 0;   // a DiffusionDriver attached (large_step.cpp:828)
+    terrain = p.terrain != 0;          // !b.ht.uniform() (small_step.cpp:583, rk3.cpp:163)
   }
   // MapC::vxgm (large_step.cpp:121-124): curvature at mass point (i, j, k) from the uncoupled u, v
   double vxgm(int i, int j, int k) const {
@@ -833,8 +834,22 @@ struct Dom {
           t2a(i, j, k) = (tav + c.c1h[k] * muave(i, j) * kT0) / (mh(k, muts(i, j)) * (kT0 + t_save(i, j, k)));
         }
       });
-      pfor(0, nx, 0, ny, 0, nz + 1, [&](int i, int j, int k) {  // KWExpl (small_step.cpp:331-378), flat terrain
-        if (k == 0) { w2(i, j, 0) = 0; return; }
+      pfor(0, nx, 0, ny, 0, nz + 1, [&](int i, int j, int k) {  // KWExpl (small_step.cpp:331-378)
+        if (k == 0) {  // the surface w'' from the small-step u'', v'' and the terrain slope (small_step.cpp:339-353)
+          double ws = 0;
+          if (terrain) {
+            ws = msftx(i, j) * H * c.rdx *
+                 ((ht(i + 1, j) - ht(i, j)) * (c.cf1 * u2(i + 1, j, 0) + c.cf2 * u2(i + 1, j, 1) + c.cf3 * u2(i + 1, j, 2)) +
+                  (ht(i, j) - ht(i - 1, j)) * (c.cf1 * u2(i, j, 0) + c.cf2 * u2(i, j, 1) + c.cf3 * u2(i, j, 2)));
+            if (do_y)
+              ws = msfty(i, j) * H * c.rdy *
+                       ((ht(i, j + 1) - ht(i, j)) * (c.cf1 * v2(i, j + 1, 0) + c.cf2 * v2(i, j + 1, 1) + c.cf3 * v2(i, j + 1, 2)) +
+                        (ht(i, j) - ht(i, j - 1)) * (c.cf1 * v2(i, j, 0) + c.cf2 * v2(i, j, 1) + c.cf3 * v2(i, j, 2))) +
+                   ws;
+          }
+          w2(i, j, 0) = ws;
+          return;
+        }
         const double m = mut(i, j), mav = muave(i, j);
         const double ope = 1.0 + epssm, ome = 1.0 - epssm;
         const double msft_inv = 1.0 / msfty(i, j);
@@ -1112,7 +1127,21 @@ struct Dom {
     fill(mu1, g);
     for (int n = 0; n < n_sc; ++n) q1[n].a = q2[n].a;  // rk3.cpp:99
   }
-  void step() {  // RK3::step (rk3.cpp:86-170), no physics / boundaries / diffusion
+  // set_w_surface (rk3.cpp:173-200, fill = false): the kinematic surface w from the new u, v
+  void set_w_surface() {
+    cfor(g.nx, g.ny, [&](int i, int j) {
+      const double ws =
+          msfty(i, j) * 0.5 * c.rdy *
+              ((ht(i, j + 1) - ht(i, j)) * (c.cf1 * v2(i, j + 1, 0) + c.cf2 * v2(i, j + 1, 1) + c.cf3 * v2(i, j + 1, 2)) +
+               (ht(i, j) - ht(i, j - 1)) * (c.cf1 * v2(i, j, 0) + c.cf2 * v2(i, j, 1) + c.cf3 * v2(i, j, 2))) +
+          msftx(i, j) * 0.5 * c.rdx *
+              ((ht(i + 1, j) - ht(i, j)) * (c.cf1 * u2(i + 1, j, 0) + c.cf2 * u2(i + 1, j, 1) + c.cf3 * u2(i + 1, j, 2)) +
+               (ht(i, j) - ht(i - 1, j)) * (c.cf1 * u2(i, j, 0) + c.cf2 * u2(i, j, 1) + c.cf3 * u2(i, j, 2)));
+      w2(i, j, 0) = ws;
+    });
+    fill(w2, g);
+  }
+  void step() {  // RK3::step (rk3.cpp:86-170), no physics / boundaries
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
       rk_prep();
@@ -1121,6 +1150,7 @@ struct Dom {
       scalars(rk);
       calc_p_rho_phi();
     }
+    if (terrain) set_w_surface();  // rk3.cpp:163
   }
 
   F* field(int id) {
@@ -1135,6 +1165,7 @@ struct Dom {
       case MI_T_INIT: return &t_init; case MI_MSFTX: return &msftx; case MI_MSFTY: return &msfty;
       case MI_MSFUX: return &msfux; case MI_MSFUY: return &msfuy; case MI_MSFVX: return &msfvx;
       case MI_MSFVY: return &msfvy; case MI_MSFVX_INV: return &msfvx_inv;
+      case MI_HT: return &ht;
       case MI_F: return &fcor; case MI_E: return &ecor; case MI_SINA: return &sina; case MI_COSA: return &cosa;
       case MI_CQU: return &cqu; case MI_CQV: return &cqv; case MI_CQW: return &cqw; case MI_H_DIABATIC: return &hdia;
       default:
@@ -1145,7 +1176,7 @@ struct Dom {
   }
 };
 
-bool is_base(int id) { return id >= MI_MUB && id <= MI_COSA; }
+bool is_base(int id) { return id >= MI_MUB && id <= MI_HT; }
 
 int fail(const std::string& m, int code = MI_EINVAL) {
   g_err = m;
