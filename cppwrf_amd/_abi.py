@@ -33,7 +33,7 @@ MI_Q, MI_Q_TENDF = 64, 96
 MI_MAX_MOIST, MI_MAX_SCALARS = 8, 16
 ABI_VERSION = 2
 
-MI_CASE_REST_STATE, MI_CASE_WARM_BUBBLE, MI_CASE_IGW, MI_CASE_BUBBLE3D = range(4)
+MI_CASE_REST_STATE, MI_CASE_WARM_BUBBLE, MI_CASE_IGW, MI_CASE_BUBBLE3D, MI_CASE_HILLS = range(5)
 
 # staggering of each field id: (x extra, y extra, z extra, rank)
 STAGGER = {
@@ -104,7 +104,10 @@ class MiLevels(C.Structure):
 class MiCaseSpec(C.Structure):
     _fields_ = [("kind", C.c_int32), ("grid", MiGrid), ("ztop", C.c_double), ("seed", C.c_uint64),
                 ("noise_amp", C.c_double), ("bubble_dtheta", C.c_double), ("bubble_radius", C.c_double),
-                ("bubble_zc", C.c_double), ("u0", C.c_double)]
+                ("bubble_zc", C.c_double), ("u0", C.c_double), ("n_hills", C.c_int32), ("moist", C.c_int32),
+                ("hill_height", C.c_double), ("hill_wmin", C.c_double), ("hill_wmax", C.c_double),
+                ("jet_min", C.c_double), ("jet_max", C.c_double), ("map_amp", C.c_double), ("etac", C.c_double),
+                ("p_top", C.c_double)]
 
 
 def default_params(**kw) -> MiParams:

@@ -76,16 +76,62 @@ uint64_t splitmix64(uint64_t& s) {
 
 struct mi_case {
   mi_grid g{};
+  int kind = 0;
   int nz = 0;
   std::vector<double> dnw, rdnw, dn, rdn, fnm, fnp, c1h, c2h, c3h, c4h, c1f, c2f, c3f, c4f, znw, znu;
   double cf1 = 0, cf2 = 0, cf3 = 0, cfn = 0, cfn1 = 0, rdx = 0, rdy = 0, p_top = 0;
   double mub = 0;
   std::vector<double> pb, alb, phb, t_init;             // profiles
   std::vector<double> u, v, w, t, ph, mu, p, al;        // dense state
+  // hills: the base state in full, terrain, map factors, Coriolis, moisture (dense arrays)
+  bool full = false;
+  std::vector<double> mub2, pb3, alb3, phb3, t_init3, ht, fcor, ecor, qv;
+  std::vector<double> msftx, msfty, msfux, msfuy, msfvx, msfvy, msfvx_inv;
 };
 
 namespace {
 
+
+// BaseState::set_levels, hybrid_opt = 0 (state.cpp:86-116), from the eta levels e
+void set_levels(mi_case& c, const std::vector<double>& e, const mi_grid& G) {
+  const int nz = G.nz, ny = G.ny;
+  c.znw = e;
+  c.znu.assign(size_t(nz), 0.0);
+  c.dnw.assign(size_t(nz), 0.0);
+  c.rdnw.assign(size_t(nz), 0.0);
+  c.dn.assign(size_t(nz) + 1, 0.0);
+  c.rdn.assign(size_t(nz) + 1, 0.0);
+  c.fnm.assign(size_t(nz) + 1, 0.0);
+  c.fnp.assign(size_t(nz) + 1, 0.0);
+  for (int k = 0; k < nz; ++k) {
+    c.dnw[k] = c.znw[k + 1] - c.znw[k];
+    c.rdnw[k] = 1.0 / c.dnw[k];
+    c.znu[k] = 0.5 * (c.znw[k + 1] + c.znw[k]);
+  }
+  for (int k = 1; k < nz; ++k) {
+    c.dn[k] = 0.5 * (c.dnw[k] + c.dnw[k - 1]);
+    c.rdn[k] = 1.0 / c.dn[k];
+    c.fnp[k] = 0.5 * c.dnw[k] / c.dn[k];
+    c.fnm[k] = 0.5 * c.dnw[k - 1] / c.dn[k];
+  }
+  const double cof1 = (2. * c.dn[1] + c.dn[2]) / (c.dn[1] + c.dn[2]) * c.dnw[0] / c.dn[1];
+  const double cof2 = c.dn[1] / (c.dn[1] + c.dn[2]) * c.dnw[0] / c.dn[2];
+  c.cf1 = c.fnp[1] + cof1;
+  c.cf2 = c.fnm[1] - cof1 - cof2;
+  c.cf3 = cof2;
+  c.cfn = (0.5 * c.dnw[nz - 1] + c.dn[nz - 1]) / c.dn[nz - 1];
+  c.cfn1 = -0.5 * c.dnw[nz - 1] / c.dn[nz - 1];
+  c.c1h.assign(size_t(nz), 1.0);
+  c.c2h.assign(size_t(nz), 0.0);
+  c.c3h = c.znu;
+  c.c4h.assign(size_t(nz), 0.0);
+  c.c1f.assign(size_t(nz) + 1, 1.0);
+  c.c2f.assign(size_t(nz) + 1, 0.0);
+  c.c3f = c.znw;
+  c.c4f.assign(size_t(nz) + 1, 0.0);
+  c.rdx = 1.0 / G.dx;
+  c.rdy = ny == 1 ? 0.0 : 1.0 / G.dy;
+}
 
 void build(mi_case& c, const mi_case_spec& sp) {
   const mi_grid& G = sp.grid;
@@ -171,43 +217,7 @@ void build(mi_case& c, const mi_case_spec& sp) {
     e[0] = 1.0;
     e[size_t(nz)] = 0.0;
   }
-  // BaseState::set_levels, hybrid_opt = 0 (state.cpp:86-116)
-  c.znw = e;
-  c.znu.assign(size_t(nz), 0.0);
-  c.dnw.assign(size_t(nz), 0.0);
-  c.rdnw.assign(size_t(nz), 0.0);
-  c.dn.assign(size_t(nz) + 1, 0.0);
-  c.rdn.assign(size_t(nz) + 1, 0.0);
-  c.fnm.assign(size_t(nz) + 1, 0.0);
-  c.fnp.assign(size_t(nz) + 1, 0.0);
-  for (int k = 0; k < nz; ++k) {
-    c.dnw[k] = c.znw[k + 1] - c.znw[k];
-    c.rdnw[k] = 1.0 / c.dnw[k];
-    c.znu[k] = 0.5 * (c.znw[k + 1] + c.znw[k]);
-  }
-  for (int k = 1; k < nz; ++k) {
-    c.dn[k] = 0.5 * (c.dnw[k] + c.dnw[k - 1]);
-    c.rdn[k] = 1.0 / c.dn[k];
-    c.fnp[k] = 0.5 * c.dnw[k] / c.dn[k];
-    c.fnm[k] = 0.5 * c.dnw[k - 1] / c.dn[k];
-  }
-  const double cof1 = (2. * c.dn[1] + c.dn[2]) / (c.dn[1] + c.dn[2]) * c.dnw[0] / c.dn[1];
-  const double cof2 = c.dn[1] / (c.dn[1] + c.dn[2]) * c.dnw[0] / c.dn[2];
-  c.cf1 = c.fnp[1] + cof1;
-  c.cf2 = c.fnm[1] - cof1 - cof2;
-  c.cf3 = cof2;
-  c.cfn = (0.5 * c.dnw[nz - 1] + c.dn[nz - 1]) / c.dn[nz - 1];
-  c.cfn1 = -0.5 * c.dnw[nz - 1] / c.dn[nz - 1];
-  c.c1h.assign(size_t(nz), 1.0);
-  c.c2h.assign(size_t(nz), 0.0);
-  c.c3h = c.znu;
-  c.c4h.assign(size_t(nz), 0.0);
-  c.c1f.assign(size_t(nz) + 1, 1.0);
-  c.c2f.assign(size_t(nz) + 1, 0.0);
-  c.c3f = c.znw;
-  c.c4f.assign(size_t(nz) + 1, 0.0);
-  c.rdx = 1.0 / G.dx;
-  c.rdy = ny == 1 ? 0.0 : 1.0 / G.dy;
+  set_levels(c, e, G);
   // dry hydrostatic base state and perturbation state (cases.cpp:224-287)
   const double mub = p_surf - c.p_top;
   c.mub = mub;
@@ -264,6 +274,217 @@ void build(mi_case& c, const mi_case_spec& sp) {
       }
 }
 
+
+// US standard atmosphere (1976) temperature, K, to 47 km
+double us_std_t(double z) {
+  if (z < 11000.0) return 288.15 - 0.0065 * z;
+  if (z < 20000.0) return 216.65;
+  if (z < 32000.0) return 216.65 + 0.001 * (z - 20000.0);
+  return 228.65 + 0.0028 * (z - 32000.0);
+}
+// its pressure from the layer-wise barometric formula (g0 = 9.80665, R = 287.053), Pa
+double us_std_p(double z) {
+  const double g0 = 9.80665, R = 287.053;
+  struct L { double z0, t0, lapse; };
+  const L ls[] = {{0.0, 288.15, -0.0065}, {11000.0, 216.65, 0.0}, {20000.0, 216.65, 0.001}, {32000.0, 228.65, 0.0028}};
+  double p = 101325.0;
+  for (int n = 0; n < 4; ++n) {
+    const double z1 = n < 3 ? ls[n + 1].z0 : 1e9, top = std::min(z, z1), dz = top - ls[n].z0;
+    if (dz <= 0) break;
+    if (ls[n].lapse == 0.0) p *= std::exp(-g0 * dz / (R * ls[n].t0));
+    else p *= std::pow((ls[n].t0 + ls[n].lapse * dz) / ls[n].t0, -g0 / (R * ls[n].lapse));
+    if (z <= z1) break;
+  }
+  return p;
+}
+
+// A real-data-shaped domain: seeded Gaussian hills (summed over periodic images), the US standard
+// atmosphere, a hybrid coordinate (WRF hybrid_opt 2 with eta_c), seeded u and v jets with +-1 m/s
+// noise, map factors 1 +- map_amp, Coriolis from 30 N to 50 N, optional qv from a relative humidity of
+// 80 % at the ground falling to 10 % at 10 km.  The base state is dry and hydrostatic column by column
+// (each column's surface pressure is the sounding's at its terrain height), as the reference's cases
+// build it (src/cases/cases.cpp:224-287) with a horizontally varying mub; theta of the initial state is
+// theta_m with moisture, its al and ph the hydrostatic ones at the base pressure.
+void build_hills(mi_case& c, const mi_case_spec& sp) {
+  const mi_grid& G = sp.grid;
+  const int nx = G.nx, ny = G.ny, nz = G.nz;
+  const bool two_d = ny == 1;
+  c.full = true;
+  uint64_t seed = sp.seed;
+  auto u01 = [&]() { return double(splitmix64(seed) >> 11) * 0x1p-53; };
+  // sounding: theta of the US standard atmosphere, integrated hydrostatically as the other cases
+  auto theta_std = [](double z) { return us_std_t(z) * std::pow(kP1000 / us_std_p(z), kRcp); };
+  const double p_surf = 101325.0;
+  const double hmax = sp.hill_height;
+  Sounding snd(theta_std, p_surf, 40000.0);
+  if (!(sp.p_top > 0) || sp.p_top >= p_surf) throw std::invalid_argument("case: p_top must be in (0, p_surf)");
+  c.p_top = sp.p_top;
+  const double ztop = snd.z_of_p(sp.p_top);
+  // eta levels equally spaced in height over flat ground
+  std::vector<double> e(size_t(nz) + 1);
+  for (int k = 0; k <= nz; ++k) e[size_t(k)] = (snd.p(ztop * k / nz) - c.p_top) / (p_surf - c.p_top);
+  e[0] = 1.0;
+  e[size_t(nz)] = 0.0;
+  set_levels(c, e, G);
+  // hybrid coefficients (WRF hybrid_opt 2): B(eta) a cubic from 0 at eta_c to 1 at the ground
+  const double ec = sp.etac, pd = kP1000 - c.p_top;
+  auto Bf = [&](double eta) {
+    if (ec <= 0.0) return eta;
+    if (eta < ec) return 0.0;
+    const double b1 = 2. * ec * ec * (1. - ec), b2 = -ec * (4. - 3. * ec - ec * ec * ec), b3 = 2. * (1. - ec * ec * ec),
+                 b4 = -(1. - ec * ec), b5 = (1. - ec) * (1. - ec) * (1. - ec) * (1. - ec);
+    return (b1 + b2 * eta + b3 * eta * eta + b4 * eta * eta * eta) / b5;
+  };
+  for (int k = 0; k <= nz; ++k) {
+    c.c3f[k] = Bf(c.znw[k]);
+    c.c4f[k] = (c.znw[k] - c.c3f[k]) * pd;
+  }
+  for (int k = 0; k < nz; ++k) {
+    c.c3h[k] = 0.5 * (c.c3f[k + 1] + c.c3f[k]);
+    c.c4h[k] = (c.znu[k] - c.c3h[k]) * pd;
+    c.c1h[k] = (c.c3f[k + 1] - c.c3f[k]) / (c.znw[k + 1] - c.znw[k]);
+    c.c2h[k] = (1.0 - c.c1h[k]) * pd;
+  }
+  c.c1f[0] = 1.0;
+  for (int k = 1; k < nz; ++k) c.c1f[k] = (c.c3h[k] - c.c3h[k - 1]) / (c.znu[k] - c.znu[k - 1]);
+  c.c1f[nz] = 0.0;
+  for (int k = 0; k <= nz; ++k) c.c2f[k] = (1.0 - c.c1f[k]) * pd;
+  // terrain: Gaussian hills, periodic images
+  const double Lx = nx * G.dx, Ly = ny * G.dy;
+  struct Hill { double x, y, h, w; };
+  std::vector<Hill> hills;
+  for (int n = 0; n < sp.n_hills; ++n) {
+    Hill h;
+    h.x = u01() * Lx;
+    h.y = two_d ? 0.0 : u01() * Ly;
+    h.h = u01() * hmax;
+    h.w = sp.hill_wmin + u01() * (sp.hill_wmax - sp.hill_wmin);
+    hills.push_back(h);
+  }
+  const size_t n2 = size_t(nx) * ny;
+  c.ht.assign(n2, 0.0);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double x = (i + 0.5) * G.dx, y = (j + 0.5) * G.dy;
+      double h = 0;
+      for (const Hill& hl : hills)
+        for (int a = -1; a <= 1; ++a)
+          for (int b = (two_d ? 0 : -1); b <= (two_d ? 0 : 1); ++b) {
+            const double dx = (x - hl.x - a * Lx) / hl.w, dy = two_d ? 0.0 : (y - hl.y - b * Ly) / hl.w;
+            h += hl.h * std::exp(-(dx * dx + dy * dy));
+          }
+      c.ht[size_t(j) * nx + i] = h;
+    }
+  // map factors at each staggering, Coriolis
+  const int nxu = nx + 1, nyv = two_d ? 1 : ny + 1;
+  auto mf = [&](double xi, double yj) {
+    return 1.0 + sp.map_amp * std::sin(kPi * xi / nx) * (two_d ? 1.0 : std::cos(kPi * yj / ny));
+  };
+  c.msftx.assign(n2, 0.0);
+  c.msfux.assign(size_t(nxu) * ny, 0.0);
+  c.msfvx.assign(size_t(nx) * nyv, 0.0);
+  c.fcor.assign(n2, 0.0);
+  c.ecor.assign(n2, 0.0);
+  const double omega2 = 2.0 * 7.2921e-5;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      c.msftx[size_t(j) * nx + i] = mf(i + 0.5, j + 0.5);
+      const double lat = (30.0 + 20.0 * (j + 0.5) / ny) * kPi / 180.0;
+      c.fcor[size_t(j) * nx + i] = omega2 * std::sin(lat);
+      c.ecor[size_t(j) * nx + i] = omega2 * std::cos(lat);
+    }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nxu; ++i) c.msfux[size_t(j) * nxu + i] = mf(i, j + 0.5);
+  for (int j = 0; j < nyv; ++j)
+    for (int i = 0; i < nx; ++i) c.msfvx[size_t(j) * nx + i] = mf(i + 0.5, j);
+  c.msfty = c.msftx;
+  c.msfuy = c.msfux;
+  c.msfvy = c.msfvx;
+  c.msfvx_inv = c.msfvx;
+  for (double& m : c.msfvx_inv) m = 1.0 / m;
+  // base state, column by column
+  const size_t nm = n2 * size_t(nz), nw = n2 * size_t(nz + 1);
+  c.mub2.assign(n2, 0.0);
+  c.pb3.assign(nm, 0.0);
+  c.alb3.assign(nm, 0.0);
+  c.t_init3.assign(nm, 0.0);
+  c.phb3.assign(nw, 0.0);
+  c.t.assign(nm, 0.0);
+  c.al.assign(nm, 0.0);
+  c.p.assign(nm, 0.0);
+  c.ph.assign(nw, 0.0);
+  c.w.assign(nw, 0.0);
+  c.mu.assign(n2, 0.0);
+  if (sp.moist) c.qv.assign(nm, 0.0);
+  std::vector<double> zm(nm);
+  auto I = [&](int i, int j, int k) { return (size_t(k) * ny + j) * nx + i; };
+  const double rvrd = 461.6 / kRd;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t q = size_t(j) * nx + i;
+      const double mub = snd.p(c.ht[q]) - c.p_top;
+      c.mub2[q] = mub;
+      double phb = kG * c.ht[q], ph = 0.0;
+      c.phb3[I(i, j, 0)] = phb;
+      for (int k = 0; k < nz; ++k) {
+        const double pb = c.c3h[k] * mub + c.c4h[k] + c.p_top;
+        const double z = snd.z_of_p(pb);
+        const double tb = snd.theta(z);
+        const double alb = (kRd / kP1000) * tb * std::pow(pb / kP1000, kCvpm);
+        c.pb3[I(i, j, k)] = pb;
+        c.alb3[I(i, j, k)] = alb;
+        c.t_init3[I(i, j, k)] = tb - kT0;
+        double th = tb;
+        if (sp.moist) {
+          const double tk = tb * std::pow(pb / kP1000, kRcp);
+          const double rh = z < 10000.0 ? 0.8 - 0.7 * z / 10000.0 : 0.1;
+          const double es = 611.2 * std::exp(17.67 * (tk - 273.15) / (tk - 29.65));  // Bolton (1980)
+          const double qv = rh * 0.622 * es / (pb - es);
+          c.qv[I(i, j, k)] = qv;
+          th = tb * (1.0 + rvrd * qv);  // theta_m
+        }
+        c.t[I(i, j, k)] = th - kT0;
+        const double alt = (kRd / kP1000) * th * std::pow(pb / kP1000, kCvpm);
+        c.al[I(i, j, k)] = alt - alb;
+        phb -= c.dnw[k] * (c.c1h[k] * mub + c.c2h[k]) * alb;
+        ph -= c.dnw[k] * ((c.c1h[k] * mub + c.c2h[k]) * c.al[I(i, j, k)]);
+        c.phb3[I(i, j, k + 1)] = phb;
+        c.ph[I(i, j, k + 1)] = ph;
+        zm[I(i, j, k)] = z;
+      }
+    }
+  // jets: u(z), v(z) Gaussian in height around 10 km, maxima seeded, plus +-1 m/s noise
+  const double uj = sp.jet_min + u01() * (sp.jet_max - sp.jet_min), vj = sp.jet_min + u01() * (sp.jet_max - sp.jet_min);
+  auto jet = [](double amp, double z) { const double s = (z - 10000.0) / 4000.0; return amp * std::exp(-s * s); };
+  c.u.assign(size_t(nxu) * ny * nz, 0.0);
+  c.v.assign(size_t(nx) * nyv * nz, 0.0);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nxu; ++i) {
+        const double z = zm[I(i < nx ? i : 0, j, k)];
+        double uu = jet(uj, z) + (2.0 * u01() - 1.0);
+        if (G.bc_x == MI_BC_SYMMETRIC && (i == 0 || i == nx)) uu = 0.0;
+        c.u[(size_t(k) * ny + j) * nxu + i] = uu;
+      }
+  if (!two_d)
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < nyv; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const double z = zm[I(i, j < ny ? j : 0, k)];
+          double vv = jet(vj, z) + (2.0 * u01() - 1.0);
+          if (G.bc_y == MI_BC_SYMMETRIC && (j == 0 || j == ny)) vv = 0.0;
+          c.v[(size_t(k) * nyv + j) * nx + i] = vv;
+        }
+  // periodic u at i = nx, v at j = ny: the values of i = 0, j = 0
+  if (G.bc_x == MI_BC_PERIODIC)
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j) c.u[(size_t(k) * ny + j) * nxu + nx] = c.u[(size_t(k) * ny + j) * nxu + 0];
+  if (!two_d && G.bc_y == MI_BC_PERIODIC)
+    for (int k = 0; k < nz; ++k)
+      for (int i = 0; i < nx; ++i) c.v[(size_t(k) * nyv + ny) * nx + i] = c.v[(size_t(k) * nyv + 0) * nx + i];
+  (void)ztop;
+}
+
 }  // namespace
 
 extern "C" {
@@ -272,7 +493,16 @@ int mi_case_create(const mi_case_spec* spec, mi_case** out) {
   return mi_detail::guard([&] {
     if (!spec || !out) throw std::invalid_argument("null argument");
     auto c = std::make_unique<mi_case>();
-    build(*c, *spec);
+    c->kind = spec->kind;
+    if (spec->kind == MI_CASE_HILLS) {
+      const mi_grid& G = spec->grid;
+      if (G.nx < 1 || G.ny < 1 || G.nz < 3) throw std::invalid_argument("case: bad grid size");
+      c->g = G;
+      c->nz = G.nz;
+      build_hills(*c, *spec);
+    } else {
+      build(*c, *spec);
+    }
     *out = c.release();
   });
 }
@@ -310,6 +540,30 @@ int mi_case_get_field(const mi_case* c, int id, double* h, int64_t n) {
       for (size_t k = 0; k < prof.size(); ++k)
         for (int64_t q = 0; q < plane; ++q) h[int64_t(k) * plane + q] = prof[k];
     };
+    if (c->full) {  // hills: the base state in full and the fields of a real-data-shaped domain
+      switch (id) {
+        case MI_MUB: put(c->mub2); return;
+        case MI_PB: put(c->pb3); return;
+        case MI_ALB: put(c->alb3); return;
+        case MI_PHB: put(c->phb3); return;
+        case MI_T_INIT: put(c->t_init3); return;
+        case MI_HT: put(c->ht); return;
+        case MI_MSFTX: put(c->msftx); return;
+        case MI_MSFTY: put(c->msfty); return;
+        case MI_MSFUX: put(c->msfux); return;
+        case MI_MSFUY: put(c->msfuy); return;
+        case MI_MSFVX: put(c->msfvx); return;
+        case MI_MSFVY: put(c->msfvy); return;
+        case MI_MSFVX_INV: put(c->msfvx_inv); return;
+        case MI_F: put(c->fcor); return;
+        case MI_E: put(c->ecor); return;
+        case MI_Q:
+          if (c->qv.empty()) throw std::invalid_argument("case: dry, no qv");
+          put(c->qv);
+          return;
+        default: break;
+      }
+    }
     switch (id) {
       case MI_U: put(c->u); break;
       case MI_V: put(c->v); break;
@@ -336,11 +590,31 @@ int mi_case_load(const mi_case* c, mi_domain* d) {
     auto chk = [](int rc) {
       if (rc != MI_OK) throw mi_detail::Error(rc, mi_last_error());
     };
-    chk(mi_domain_set_field(d, MI_MUB, &c->mub, 1));
-    chk(mi_domain_set_field(d, MI_PB, c->pb.data(), int64_t(c->pb.size())));
-    chk(mi_domain_set_field(d, MI_ALB, c->alb.data(), int64_t(c->alb.size())));
-    chk(mi_domain_set_field(d, MI_PHB, c->phb.data(), int64_t(c->phb.size())));
-    chk(mi_domain_set_field(d, MI_T_INIT, c->t_init.data(), int64_t(c->t_init.size())));
+    auto set = [&](int id, const std::vector<double>& v) { chk(mi_domain_set_field(d, id, v.data(), int64_t(v.size()))); };
+    if (c->full) {
+      set(MI_MUB, c->mub2);
+      set(MI_PB, c->pb3);
+      set(MI_ALB, c->alb3);
+      set(MI_PHB, c->phb3);
+      set(MI_T_INIT, c->t_init3);
+      set(MI_HT, c->ht);
+      set(MI_MSFTX, c->msftx);
+      set(MI_MSFTY, c->msfty);
+      set(MI_MSFUX, c->msfux);
+      set(MI_MSFUY, c->msfuy);
+      set(MI_MSFVX, c->msfvx);
+      set(MI_MSFVY, c->msfvy);
+      set(MI_MSFVX_INV, c->msfvx_inv);
+      set(MI_F, c->fcor);
+      set(MI_E, c->ecor);
+      if (!c->qv.empty()) set(MI_Q, c->qv);
+    } else {
+      chk(mi_domain_set_field(d, MI_MUB, &c->mub, 1));
+      chk(mi_domain_set_field(d, MI_PB, c->pb.data(), int64_t(c->pb.size())));
+      chk(mi_domain_set_field(d, MI_ALB, c->alb.data(), int64_t(c->alb.size())));
+      chk(mi_domain_set_field(d, MI_PHB, c->phb.data(), int64_t(c->phb.size())));
+      chk(mi_domain_set_field(d, MI_T_INIT, c->t_init.data(), int64_t(c->t_init.size())));
+    }
     chk(mi_domain_set_field(d, MI_U, c->u.data(), int64_t(c->u.size())));
     chk(mi_domain_set_field(d, MI_V, c->v.data(), int64_t(c->v.size())));
     chk(mi_domain_set_field(d, MI_W, c->w.data(), int64_t(c->w.size())));

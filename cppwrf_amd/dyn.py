@@ -50,11 +50,18 @@ class Case:
 
     STATE = (A.MI_U, A.MI_V, A.MI_W, A.MI_T, A.MI_PH, A.MI_MU, A.MI_P, A.MI_AL)
 
+    # the fields a real-data-shaped case carries beyond the state and the base profiles
+    HILLS_EXTRA = (A.MI_HT, A.MI_MSFTX, A.MI_MSFTY, A.MI_MSFUX, A.MI_MSFUY, A.MI_MSFVX, A.MI_MSFVY, A.MI_MSFVX_INV,
+                   A.MI_F, A.MI_E)
+
     def __init__(self, kind: int, grid: Grid, ztop: float, seed: int = 0, noise_amp: float = 0.0,
-                 bubble_dtheta: float = 2.0, bubble_radius: float = 4000.0, bubble_zc: float = 2000.0, u0: float = 0.0):
+                 bubble_dtheta: float = 2.0, bubble_radius: float = 4000.0, bubble_zc: float = 2000.0, u0: float = 0.0,
+                 **hills):
         self.grid = grid
+        self.kind = kind
         self.spec = A.MiCaseSpec(kind=kind, grid=grid.c(), ztop=ztop, seed=seed, noise_amp=noise_amp,
-                                 bubble_dtheta=bubble_dtheta, bubble_radius=bubble_radius, bubble_zc=bubble_zc, u0=u0)
+                                 bubble_dtheta=bubble_dtheta, bubble_radius=bubble_radius, bubble_zc=bubble_zc, u0=u0,
+                                 **hills)
         h = C.c_void_p()
         check(lib().mi_case_create(C.byref(self.spec), C.byref(h)))
         self._h = h
@@ -79,7 +86,23 @@ class Case:
         """a base-state field's vertical profile (the generator's base state is horizontally uniform)"""
         return np.ascontiguousarray(self.field(fid)[:, 0, 0])
 
+    @property
+    def full_base(self) -> bool:
+        """the base state varies horizontally (terrain): base fields are full arrays"""
+        return self.kind == A.MI_CASE_HILLS
+
+    @property
+    def moist(self) -> bool:
+        return self.kind == A.MI_CASE_HILLS and bool(self.spec.moist)
+
     def params(self, **kw) -> A.MiParams:
+        if self.kind == A.MI_CASE_HILLS:  # the flags a real-data-shaped domain needs
+            kw.setdefault("terrain", 1)
+            kw.setdefault("mapped", 1)
+            kw.setdefault("coriolis", 1)
+            if self.spec.moist:
+                kw.setdefault("n_moist", 1)
+                kw.setdefault("n_scalars", max(1, kw.get("n_scalars", 1)))
         return A.default_params(**kw)
 
     # the workloads of this library and the reference's idealized cases
@@ -99,6 +122,16 @@ class Case:
     @classmethod
     def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0):
         return cls(A.MI_CASE_REST_STATE, Grid(nx, ny, nz, 2000.0, 2000.0, 10.0), 20000.0, u0=u0)
+
+    @classmethod
+    def hills(cls, nx, ny, nz, dx, dt, seed=1, n_hills=12, hill_height=2500.0, hill_wmin=30000.0, hill_wmax=150000.0,
+              jet_min=5.0, jet_max=40.0, map_amp=0.05, etac=0.2, p_top=5000.0, moist=True, bc=A.MI_BC_PERIODIC):
+        """real-data-shaped domain: seeded hills, US standard atmosphere, jets, map factors, Coriolis, hybrid
+        coordinate with a p_top top, qv from a relative humidity of 80 % falling to 10 % at 10 km"""
+        g = Grid(nx, ny, nz, dx, dx, dt, bc, bc)
+        return cls(A.MI_CASE_HILLS, g, 0.0, seed=seed, n_hills=n_hills, moist=1 if moist else 0, hill_height=hill_height,
+                   hill_wmin=hill_wmin, hill_wmax=hill_wmax, jet_min=jet_min, jet_max=jet_max, map_amp=map_amp,
+                   etac=etac, p_top=p_top)
 
 
 class Context:

@@ -218,7 +218,10 @@ enum {
   MI_CASE_REST_STATE = 0,  /* reference configs/rest_state.ini recipe */
   MI_CASE_WARM_BUBBLE = 1, /* reference warm_bubble (x-z Bryan & Fritsch 2002) */
   MI_CASE_IGW = 2,         /* reference igw (Skamarock & Klemp 1994, f = 0) */
-  MI_CASE_BUBBLE3D = 3     /* 3-D dry bubble in an N = 0.01 atmosphere at rest, seeded noise */
+  MI_CASE_BUBBLE3D = 3,    /* 3-D dry bubble in an N = 0.01 atmosphere at rest, seeded noise */
+  MI_CASE_HILLS = 4        /* real-data-shaped: seeded hills, US standard atmosphere, jets, map factors,
+                              Coriolis, hybrid coordinate, optional qv; needs terrain, mapped, coriolis
+                              (and n_moist >= 1 when moist) in mi_params */
 };
 typedef struct mi_case_spec {
   int32_t kind; /* MI_CASE_* */
@@ -230,6 +233,15 @@ typedef struct mi_case_spec {
   double bubble_radius; /* m (bubble3d: 4000) */
   double bubble_zc;     /* m (bubble3d: 2000) */
   double u0;            /* uniform wind (rest_state's case.u0) */
+  /* hills: a real-data-shaped domain (terrain, US standard atmosphere, jets, map factors, Coriolis) */
+  int32_t n_hills;      /* seeded Gaussian hills (periodic images) */
+  int32_t moist;        /* 1: qv from RH 80 % at the ground to 10 % at 10 km (scalar 0) */
+  double hill_height;   /* m, heights uniform in [0, hill_height] */
+  double hill_wmin, hill_wmax; /* m, e-folding widths uniform in [hill_wmin, hill_wmax] */
+  double jet_min, jet_max;     /* m/s, u and v jet maxima uniform in [jet_min, jet_max], plus +-1 m/s noise */
+  double map_amp;       /* map factors 1 +- map_amp (0.05) */
+  double etac;          /* hybrid coordinate: B(eta) = 0 above eta = etac (WRF hybrid_opt 2); 0: sigma */
+  double p_top;         /* Pa (hills; the model top is where the sounding reaches p_top) */
 } mi_case_spec;
 typedef struct mi_case mi_case;
 int mi_case_create(const mi_case_spec* spec, mi_case** out);

@@ -69,11 +69,19 @@ class OracleDomain:
         for fid in case.STATE:
             a = case.field(fid)
             _ok(olib().orc_set_field(h, fid, _ptr(a), a.size))
-        for fid in (A.MI_PB, A.MI_ALB, A.MI_PHB, A.MI_T_INIT):
-            prof = case.profile(fid)
-            _ok(olib().orc_set_field(h, fid, _ptr(prof), prof.size))
-        mub = case.profile(A.MI_MUB)
-        _ok(olib().orc_set_field(h, A.MI_MUB, _ptr(mub), 1))
+        if getattr(case, "full_base", False):  # terrain: base fields in full, and the real-data fields
+            for fid in (A.MI_MUB, A.MI_PB, A.MI_ALB, A.MI_PHB, A.MI_T_INIT) + case.HILLS_EXTRA:
+                a = case.field(fid)
+                _ok(olib().orc_set_field(h, fid, _ptr(a), a.size))
+            if case.moist:
+                a = case.field(A.MI_Q)
+                _ok(olib().orc_set_field(h, A.MI_Q, _ptr(a), a.size))
+        else:
+            for fid in (A.MI_PB, A.MI_ALB, A.MI_PHB, A.MI_T_INIT):
+                prof = case.profile(fid)
+                _ok(olib().orc_set_field(h, fid, _ptr(prof), prof.size))
+            mub = case.profile(A.MI_MUB)
+            _ok(olib().orc_set_field(h, A.MI_MUB, _ptr(mub), 1))
         _ok(olib().orc_set_libmvec(h, 1 if libmvec else 0))
         self.nx, self.ny, self.nz = g.nx, g.ny, g.nz
 
