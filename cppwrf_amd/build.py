@@ -19,7 +19,10 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcppwrf_amd.so")
 ARCH = os.environ.get("MI_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels_fp64.hip", "domain.hip"]
+# (source, object, defines): the kernels and the domain are compiled once per arithmetic
+PRECISIONS = [("fp64", ["-DMI_REAL=double", "-DMI_NS=mi_fp64"]), ("fp32", ["-DMI_REAL=float", "-DMI_NS=mi_fp32"])]
+HIP_UNITS = [(src, f"{src}.{tag}.o", defs) for src in ("kernels_fp64.hip", "domain.hip") for tag, defs in PRECISIONS] + \
+            [("capi.hip", "capi.hip.o", [])]
 HOST_SOURCES = ["cases.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -43,15 +46,14 @@ def _run(cmd: list[str]) -> None:
 def build_library(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     jobs = []
-    for s in HIP_SOURCES:
-        obj = os.path.join(BUILD, s + ".o")
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, s)])
+    for src, obj, defs in HIP_UNITS:
+        jobs.append([HIPCC, *HIP_FLAGS, *defs, "-c", "-o", os.path.join(BUILD, obj), os.path.join(CSRC, src)])
     for s in HOST_SOURCES:
         obj = os.path.join(BUILD, s + ".o")
         jobs.append(["g++", *COMMON, "-c", "-o", obj, os.path.join(CSRC, s)])
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         list(ex.map(_run, jobs))
-    objs = [os.path.join(BUILD, s + ".o") for s in HIP_SOURCES + HOST_SOURCES]
+    objs = [os.path.join(BUILD, obj) for _, obj, _ in HIP_UNITS] + [os.path.join(BUILD, s + ".o") for s in HOST_SOURCES]
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -1,0 +1,173 @@
+// The extern "C" boundary of include/cppwrf_amd.h: argument checks, error capture (no exception crosses
+// the ABI) and dispatch to the domain implementation of the requested arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+
+#include "../../include/cppwrf_amd.h"
+#include "domain_api.hpp"
+#include "errors.hpp"
+
+namespace {
+
+using mi_detail::Error;
+
+thread_local std::string g_last_error;
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(e == hipErrorOutOfMemory ? MI_ENOMEM : MI_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+mi_domain* checked(mi_domain* d) {
+  if (!d) throw Error(MI_EINVAL, "null argument");
+  hip_check(hipSetDevice(d->context()->device), "hipSetDevice");
+  return d;
+}
+
+int run_phase(mi_domain* d, MiPhase which, int rk) {
+  return mi_detail::guard([&] {
+    if (which != kPhaseStepBegin && (rk < 1 || rk > 3)) throw Error(MI_EINVAL, "rk must be 1, 2 or 3");
+    checked(d)->phase(which, rk);
+  });
+}
+
+}  // namespace
+
+namespace mi_detail {
+void set_last_error(const std::string& m) { g_last_error = m; }
+}  // namespace mi_detail
+
+extern "C" {
+
+const char* mi_last_error(void) { return g_last_error.c_str(); }
+int mi_abi_version(void) { return MI_ABI_VERSION; }
+
+// reference RK3::auto_sound_steps (src/dyn/rk3.cpp:69-72)
+int mi_auto_sound_steps(double dt, double dx, double dy) {
+  const double spacing = dx < dy ? dx : dy;
+  const int n = 2 * (int(300.0 * dt / spacing - 0.01) + 1);
+  return n > 4 ? n : 4;
+}
+
+int mi_ctx_create(int device, void* stream, mi_ctx** out) {
+  return mi_detail::guard([&] {
+    if (!out) throw Error(MI_EINVAL, "null argument");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(MI_ENODEV, "no HIP device");
+    if (device < 0 || device >= n) throw Error(MI_EINVAL, "device index out of range");
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    auto c = std::make_unique<mi_ctx>();
+    c->device = device;
+    if (stream) {
+      c->stream = static_cast<hipStream_t>(stream);
+    } else {
+      hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+      c->own_stream = true;
+    }
+    *out = c.release();
+  });
+}
+
+int mi_ctx_destroy(mi_ctx* c) {
+  return mi_detail::guard([&] {
+    if (!c) return;
+    if (c->own_stream) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+  });
+}
+
+int mi_domain_create(mi_ctx* ctx, const mi_grid* g, const mi_params* p, const mi_levels* lv, mi_domain** out) {
+  return mi_detail::guard([&] {
+    if (!ctx || !g || !p || !lv || !out) throw Error(MI_EINVAL, "null argument");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (p->precision == MI_FP64) *out = mi_fp64::make_domain(ctx, *g, *p, *lv);
+    else if (p->precision == MI_FP32_BULK) *out = mi_fp32::make_domain(ctx, *g, *p, *lv);
+    else throw Error(MI_EINVAL, "precision must be MI_FP64 or MI_FP32_BULK");
+  });
+}
+
+int mi_domain_destroy(mi_domain* d) {
+  return mi_detail::guard([&] {
+    if (!d) return;
+    (void)hipStreamSynchronize(d->context()->stream);
+    delete d;
+  });
+}
+
+int mi_domain_n_sound(const mi_domain* d) { return d ? d->n_sound() : MI_EINVAL; }
+int64_t mi_domain_device_bytes(const mi_domain* d) { return d ? d->device_bytes() : MI_EINVAL; }
+
+int mi_domain_set_field(mi_domain* d, int id, const double* host, int64_t n) {
+  return mi_detail::guard([&] {
+    if (!host) throw Error(MI_EINVAL, "null argument");
+    checked(d)->set_field(id, host, n);
+  });
+}
+
+int mi_domain_get_field(mi_domain* d, int id, double* host, int64_t n) {
+  return mi_detail::guard([&] {
+    if (!host) throw Error(MI_EINVAL, "null argument");
+    checked(d)->get_field(id, host, n);
+  });
+}
+
+int mi_step(mi_domain* d, int nsteps) {
+  return mi_detail::guard([&] {
+    if (nsteps < 0) throw Error(MI_EINVAL, "nsteps must be >= 0");
+    checked(d)->step(nsteps);
+  });
+}
+
+int mi_domain_sync(mi_domain* d) {
+  return mi_detail::guard([&] { hip_check(hipStreamSynchronize(checked(d)->context()->stream), "hipStreamSynchronize"); });
+}
+
+int mi_step_begin(mi_domain* d) { return run_phase(d, kPhaseStepBegin, 0); }
+int mi_rk_prep(mi_domain* d, int rk) { return run_phase(d, kPhaseRkPrep, rk); }
+int mi_rk_tendency(mi_domain* d, int rk) { return run_phase(d, kPhaseRkTendency, rk); }
+int mi_small_steps(mi_domain* d, int rk) { return run_phase(d, kPhaseSmallSteps, rk); }
+int mi_scalar_transport(mi_domain* d, int rk) { return run_phase(d, kPhaseScalars, rk); }
+int mi_calc_p_rho_phi(mi_domain* d, int rk) { return run_phase(d, kPhaseCalcPRhoPhi, rk); }
+
+int mi_domain_autotune(mi_domain* d, int trials) {
+  return mi_detail::guard([&] { checked(d)->autotune(trials); });
+}
+
+int mi_domain_set_tuning(mi_domain* d, const int* shapes, int n) {
+  return mi_detail::guard([&] {
+    if (!shapes) throw Error(MI_EINVAL, "null argument");
+    checked(d)->set_tuning(shapes, n);
+  });
+}
+
+int mi_domain_get_tuning(const mi_domain* d, int* shapes, int n) {
+  return mi_detail::guard([&] {
+    if (!d || !shapes) throw Error(MI_EINVAL, "null argument");
+    d->get_tuning(shapes, n);
+  });
+}
+
+int mi_profile_kernel(mi_domain* d, const char* name) {
+  return mi_detail::guard([&] { checked(d)->profile_kernel(name); });
+}
+
+int mi_profile_read(mi_domain* d, double* total_ms, int* launches, double* alg_bytes) {
+  return mi_detail::guard([&] { checked(d)->profile_read(total_ms, launches, alg_bytes); });
+}
+
+int mi_check_finite(mi_domain* d, int* bad_field) {
+  int bad = -1;
+  const int rc = mi_detail::guard([&] { bad = checked(d)->check_finite(bad_field); });
+  if (rc != MI_OK) return rc;
+  if (bad >= 0) {
+    mi_detail::set_last_error("non-finite value in field " + std::to_string(bad));
+    return MI_ENONFINITE;
+  }
+  return MI_OK;
+}
+
+}  // extern "C"

@@ -12,18 +12,19 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cppwrf_amd.h"
+#include "domain_api.hpp"
 #include "errors.hpp"
 #include "kernels_fp64.hpp"
 
+namespace MI_NS {
 namespace {
 
 using namespace mi_dev;
 using mi_detail::Error;
-
-thread_local std::string g_last_error;
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw Error(e == hipErrorOutOfMemory ? MI_ENOMEM : MI_EHIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -32,29 +33,20 @@ void hip_check(hipError_t e, const char* what) {
 enum class St : char { M, U, V, W, M2, U2, V2 };
 
 struct FieldDesc {
-  double** slot;  // where the kernels' Fields keeps its pointer
+  void** slot;  // where the kernels' Fields keeps its pointer
   St st;
-  int id;         // MI_* id or -1
+  int id;       // MI_* id or -1
+  int esize;    // bytes per element: 8 (FP64) or 4 (FP32 storage)
 };
 
 }  // namespace
 
-namespace mi_detail {
-void set_last_error(const std::string& m) { g_last_error = m; }
-}  // namespace mi_detail
-
-struct mi_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-};
-
-struct mi_domain {
+struct Domain final : mi_domain {
   mi_ctx* ctx = nullptr;
   mi_grid grid{};
   mi_params prm{};
-  int n_sound = 0;
-  mi_fp64::Args a{};
+  int n_sound_ = 0;
+  Args a{};
   int64_t n3 = 0, n2 = 0;  // doubles per 3-D / 2-D allocation
   int64_t off = 0;         // offset of point (0,0,0) in an allocation
   int nxa = 0, nya = 0;
@@ -64,16 +56,16 @@ struct mi_domain {
   double *pb_buf = nullptr, *alb_buf = nullptr, *phb_buf = nullptr, *t_init_buf = nullptr;
   int* flags = nullptr;  // device finiteness flags
   // the scalars and their buffers; qpar flips with every step (three stage swaps)
-  mi_fp64::Scalars sc{};
+  Scalars sc{};
   int n_sc = 0, qpar = 0;
   // one graph per scalar-buffer parity: the step's launches name the buffers (none without scalars)
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   bool use_graph = true;
   std::string prof_name;
-  mi_fp64::KernelTimer timer{};
-  int colcfg[mi_fp64::kNumColKernels] = {0, 0, 0, 0};  // block shape of each column kernel
-  std::vector<double*> backup;                          // the carried state, for the autotuner
+  KernelTimer timer{};
+  int colcfg[kNumColKernels] = {0, 0, 0, 0};  // block shape of each column kernel
+  std::vector<char*> backup;                            // the carried state, for the autotuner
 
   // launches of fn, with the kernel timer active when a kernel is selected
   template <class F>
@@ -83,17 +75,17 @@ struct mi_domain {
       return;
     }
     timer.name = prof_name.c_str();
-    mi_fp64::g_timer = &timer;
+    g_timer = &timer;
     try {
       fn();
     } catch (...) {
-      mi_fp64::g_timer = nullptr;
+      g_timer = nullptr;
       throw;
     }
-    mi_fp64::g_timer = nullptr;
+    g_timer = nullptr;
   }
 
-  ~mi_domain() {
+  ~Domain() override {
     for (hipEvent_t e : timer.ev) (void)hipEventDestroy(e);
     drop_graphs();
     for (void* p : allocs) (void)hipFree(p);
@@ -107,13 +99,14 @@ struct mi_domain {
     }
   }
 
-  double* alloc(int64_t n) {
+  template <class T = double>
+  T* alloc(int64_t n) {
     void* p = nullptr;
-    hip_check(hipMalloc(&p, size_t(n) * sizeof(double)), "hipMalloc");
+    hip_check(hipMalloc(&p, size_t(n) * sizeof(T)), "hipMalloc");
     allocs.push_back(p);
-    bytes += n * int64_t(sizeof(double));
-    hip_check(hipMemsetAsync(p, 0, size_t(n) * sizeof(double), ctx->stream), "hipMemsetAsync");
-    return static_cast<double*>(p);
+    bytes += n * int64_t(sizeof(T));
+    hip_check(hipMemsetAsync(p, 0, size_t(n) * sizeof(T), ctx->stream), "hipMemsetAsync");
+    return static_cast<T*>(p);
   }
 
   // ------------------------------------------------------------------ geometry of a staggering
@@ -126,7 +119,7 @@ struct mi_domain {
   bool face_x(St s) const { return s == St::U || s == St::U2; }
   bool face_y(St s) const { return (s == St::V || s == St::V2) && grid.ny > 1; }
 
-  const FieldDesc* desc_of(const double* p) const {
+  const FieldDesc* desc_of(const void* p) const {
     for (const FieldDesc& f : fields)
       if (*f.slot == p) return &f;
     return nullptr;
@@ -134,18 +127,19 @@ struct mi_domain {
 
   // ------------------------------------------------------------------ halo batches
   struct HaloReq {
-    double* p;
+    void* p;
     int px, py;  // -1: default parity of the staggering
   };
   void halo(std::initializer_list<HaloReq> reqs) {
-    mi_fp64::HaloSet s{};
+    HaloSet s{};
     s.nkmax = 1;
     for (const HaloReq& r : reqs) {
       const FieldDesc* d = desc_of(r.p);
       if (!d) throw Error(MI_EINVAL, "internal: halo of an unknown field");
-      if (s.n == mi_fp64::kHaloMax) throw Error(MI_EINVAL, "internal: too many fields in one halo batch");
+      if (s.n == kHaloMax) throw Error(MI_EINVAL, "internal: too many fields in one halo batch");
       const int n = s.n++;
       s.p[n] = r.p;
+      s.f32[n] = d->esize == 4 ? 1 : 0;
       s.nxs[n] = nxs(d->st);
       s.nys[n] = nys(d->st);
       s.nzs[n] = nzs(d->st);
@@ -155,15 +149,15 @@ struct mi_domain {
       s.py[n] = (unsigned char)(r.py >= 0 ? r.py : (face_y(d->st) ? 1 : 0));
       if (s.nzs[n] > s.nkmax) s.nkmax = s.nzs[n];
     }
-    mi_fp64::launch_halo(a, s, ctx->stream);
+    launch_halo(a, s, ctx->stream);
   }
 
   // ------------------------------------------------------------------ the phases of a step
   // RK3::step's _2 -> _1 copies (rk3.cpp:94-98).  The copies include the halos, which are fresh, so
   // the reference's refresh of mu_1's halo rewrites the values already there and is left out.
   void step_begin() {
-    mi_fp64::launch_copy_step(a, n3, n2, ctx->stream);
-    mi_fp64::launch_copy_scalars(a, sc, n_sc, n3, ctx->stream);  // q_1 = q_2 (rk3.cpp:99)
+    launch_copy_step(a, n3, n2, ctx->stream);
+    launch_copy_scalars(a, sc, n_sc, n3, ctx->stream);  // q_1 = q_2 (rk3.cpp:99)
   }
 
   // LargeStep::prep (large_step.cpp:697-733).  Its first halo refresh (u, v, w, t, ph, p, al, mu of the
@@ -173,42 +167,42 @@ struct mi_domain {
   // kernel itself: each output point stores its halo images.  The same holds for every refresh of the
   // small steps below.
   void rk_prep() {
-    mi_fp64::launch_large_prep(a, colcfg[mi_fp64::kColLargePrep], ctx->stream);
-    if (a.q.moist) mi_fp64::launch_cq(a, sc, ctx->stream);  // calc_cq (large_step.cpp:724-733)
+    launch_large_prep(a, colcfg[kColLargePrep], ctx->stream);
+    if (a.q.moist) launch_cq(a, sc, ctx->stream);  // calc_cq (large_step.cpp:724-733)
   }
 
   // LargeStep::tendencies (large_step.cpp:798-861); at the first stage, with a diffusion driver,
   // zero_tendf and the sixth-order diffusion into the held tendencies first
   void rk_tendency(int rk) {
-    if (rk == 1 && a.q.use_tendf) mi_fp64::launch_sixth_dyn(a, ctx->stream);
-    mi_fp64::launch_tendencies(a, ctx->stream);
+    if (rk == 1 && a.q.use_tendf) launch_sixth_dyn(a, ctx->stream);
+    launch_tendencies(a, ctx->stream);
   }
 
   // SmallStep::run_stage (small_step.cpp:490-607); with prhophi, calc_p_rho_phi (small_step.cpp:609-617)
   // is fused into the finishing kernel
   void small_steps(int rk, bool prhophi) {
     auto& f = a.f;
-    const double dt = grid.dt, dts_full = dt / n_sound;
+    const double dt = grid.dt, dts_full = dt / n_sound_;
     const double dt_rk = rk == 1 ? dt / 3.0 : (rk == 2 ? 0.5 * dt : dt);
     const double dts = rk == 1 ? dt_rk : dts_full;
-    const int ns = rk == 1 ? 1 : (rk == 2 ? n_sound / 2 : n_sound);
+    const int ns = rk == 1 ? 1 : (rk == 2 ? n_sound_ / 2 : n_sound_);
     hipStream_t st = ctx->stream;
-    if (rk == 1) mi_fp64::launch_fill(f.mudf - off, n2, 0.0, st);
-    mi_fp64::launch_ss_prep(a, rk == 1, st);
+    if (rk == 1) launch_fill(f.mudf - off, n2, 0.0, st);
+    launch_ss_prep(a, rk == 1, st);
     const double cof_h = 0.5 * dts * kG * (1.0 + prm.epssm);
     // halos of u'', v'', ph'', t_save, u_save, v_save, mu'', muts, p, al (small_step.cpp:526-528): images
     // written by the two kernels above
-    mi_fp64::launch_ss_column(a, colcfg[mi_fp64::kColSsColumn], cof_h * cof_h, st);
+    launch_ss_column(a, colcfg[kColSsColumn], cof_h * cof_h, st);
     for (int it = 1; it <= ns; ++it) {
-      mi_fp64::launch_uv(a, dts, st);  // with the halos of u'', v''
+      launch_uv(a, dts, st);  // with the halos of u'', v''
       // with the halos of mu'', mudf, muts, ph'', p, al, and of ru_m, rv_m, ww_m (their refresh after
       // the loop; the sums' halo images are rewritten at every substep, the last one's stay)
-      mi_fp64::launch_acoustic(a, colcfg[mi_fp64::kColAcoustic], dts, it, ns, st);
+      launch_acoustic(a, colcfg[kColAcoustic], dts, it, ns, st);
     }
     // with the halos of u, v, w, t, ph, ww, mu (and p, al); the microphysics heating leaves theta_m at
     // the last stage with moisture (small_step.cpp:602)
     const double hd_dt = (rk == 3 && a.q.moist) ? dts * double(ns) : 0.0;
-    mi_fp64::launch_finish(a, colcfg[mi_fp64::kColFinish], prhophi, hd_dt, st);
+    launch_finish(a, colcfg[kColFinish], prhophi, hd_dt, st);
   }
 
   // ScalarTransport::run_stage (scalars.cpp:436-461): the plain species, then the positive-definite ones
@@ -223,15 +217,15 @@ struct mi_domain {
       else plain[np++] = n;
     }
     // rk_scalar_tend at the first stage: the step's tendency of every scalar, sixth-order diffusion included
-    if (rk == 1 && a.q.use_tendf) mi_fp64::launch_sixth_scalars(a, sc, n_sc, dt_rk, ctx->stream);
-    mi_fp64::launch_scalars(a, sc, plain, np, false, dt_rk, ctx->stream);
-    mi_fp64::launch_scalars(a, sc, pd, nd, true, dt_rk, ctx->stream);
+    if (rk == 1 && a.q.use_tendf) launch_sixth_scalars(a, sc, n_sc, dt_rk, ctx->stream);
+    launch_scalars(a, sc, plain, np, false, dt_rk, ctx->stream);
+    launch_scalars(a, sc, pd, nd, true, dt_rk, ctx->stream);
     for (int n = 0; n < n_sc; ++n) std::swap(sc.q2[n], sc.qn[n]);
     qpar ^= 1;
   }
 
   // SmallStep::calc_p_rho_phi (small_step.cpp:609-617), for the phase-by-phase entry points
-  void calc_p_rho_phi() { mi_fp64::launch_p_rho_phi(a, ctx->stream); }  // with the halos of p, al
+  void calc_p_rho_phi() { launch_p_rho_phi(a, ctx->stream); }  // with the halos of p, al
 
   void step_launches() {
     step_begin();
@@ -241,16 +235,22 @@ struct mi_domain {
       small_steps(rk, true);
       scalar_transport(rk);
     }
-    if (a.q.terrain) mi_fp64::launch_set_w_surface(a, ctx->stream);  // rk3.cpp:163
+    if (a.q.terrain) launch_set_w_surface(a, ctx->stream);  // rk3.cpp:163
   }
 
   // the fields a step reads from the previous one (everything else is recomputed within a step)
-  std::vector<double*> carried() {
-    std::vector<double*> c = {a.f.u2, a.f.v2, a.f.w2, a.f.t2, a.f.ph2, a.f.mu2, a.f.p, a.f.al};
-    for (int n = 0; n < n_sc; ++n) c.push_back(sc.q2[n]);
+  struct Carried {
+    void* p;        // point (0,0,0)
+    int64_t bytes;  // the whole allocation
+  };
+  std::vector<Carried> carried() {
+    const int64_t b3 = n3 * int64_t(sizeof(real)), b2 = n2 * int64_t(sizeof(double));
+    std::vector<Carried> c = {{a.f.u2, b3}, {a.f.v2, b3}, {a.f.w2, b3}, {a.f.t2, b3}, {a.f.ph2, b3},
+                              {a.f.mu2, b2}, {a.f.p, b3}, {a.f.al, b3}};
+    for (int n = 0; n < n_sc; ++n) c.push_back({sc.q2[n], b3});
     return c;
   }
-  mi_fp64::Scalars sc_saved{};
+  Scalars sc_saved{};
   int qpar_saved = 0;
   void save_or_restore(bool save) {
     if (save) {
@@ -260,31 +260,31 @@ struct mi_domain {
       sc = sc_saved;
       qpar = qpar_saved;
     }
-    std::vector<double*> c = carried();
+    std::vector<Carried> c = carried();
     if (backup.empty())
-      for (double* p : c) backup.push_back(alloc(p == a.f.mu2 ? n2 : n3));
+      for (const Carried& x : c) backup.push_back(alloc<char>(x.bytes));
     for (size_t q = 0; q < c.size(); ++q) {
-      const int64_t n = c[q] == a.f.mu2 ? n2 : n3;
-      double* live = c[q] - off;
-      hip_check(hipMemcpyAsync(save ? backup[q] : live, save ? live : backup[q], size_t(n) * sizeof(double),
+      const int64_t elt = c[q].p == static_cast<void*>(a.f.mu2) ? int64_t(sizeof(double)) : int64_t(sizeof(real));
+      char* live = static_cast<char*>(c[q].p) - off * elt;
+      hip_check(hipMemcpyAsync(save ? backup[q] : live, save ? live : backup[q], size_t(c[q].bytes),
                                hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
     }
   }
 
   // Time every block shape of each column kernel on this domain and keep the fastest; the state is
   // restored afterwards, so the forecast continues exactly as if no step had been taken.
-  void autotune(int trials) {
+  void autotune(int trials) override {
     if (trials < 1) trials = 1;
     const std::string keep = prof_name;
     save_or_restore(true);
     for (int s = 0; s < 1; ++s) timed([&] { step_launches(); });  // warm-up
-    for (int kern = 0; kern < mi_fp64::kNumColKernels; ++kern) {
+    for (int kern = 0; kern < kNumColKernels; ++kern) {
       double best_t = 1e300;
       int best = colcfg[kern];
-      for (int cfg = 0; cfg < mi_fp64::kNumColCfgs; ++cfg) {
-        if (!mi_fp64::col_cfg_fits(kern, cfg, grid.nz)) continue;
+      for (int cfg = 0; cfg < kNumColCfgs; ++cfg) {
+        if (!col_cfg_fits(kern, cfg, grid.nz)) continue;
         colcfg[kern] = cfg;
-        prof_name = mi_fp64::kColKernelNames[kern];
+        prof_name = kColKernelNames[kern];
         timer.used = 0;
         timer.bytes = 0;
         for (int s = 0; s < trials; ++s) timed([&] { step_launches(); });
@@ -310,7 +310,7 @@ struct mi_domain {
     drop_graphs();  // the graphs hold the old shapes
   }
 
-  void step(int nsteps) {
+  void step(int nsteps) override {
     if (nsteps <= 0) return;
     if (!use_graph || !prof_name.empty()) {
       for (int s = 0; s < nsteps; ++s) timed([&] { step_launches(); });
@@ -322,7 +322,7 @@ struct mi_domain {
       if (!exec[par]) {
         // capturing runs the host side of step_launches (the buffer swaps) without running the step, so
         // the host state is put back before the launch below performs the step
-        const mi_fp64::Scalars keep = sc;
+        const Scalars keep = sc;
         hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         step_launches();
         hipGraph_t gr = nullptr;
@@ -340,11 +340,24 @@ struct mi_domain {
       }
     }
   }
+
+  // ------------------------------------------------------------------ the interface behind the C ABI
+  mi_ctx* context() const override { return ctx; }
+  int n_sound() const override { return n_sound_; }
+  int64_t device_bytes() const override { return bytes; }
+  void set_field(int id, const double* host, int64_t n) override;
+  void get_field(int id, double* host, int64_t n) override;
+  void phase(MiPhase which, int rk) override;
+  void set_tuning(const int* shapes, int n) override;
+  void get_tuning(int* shapes, int n) const override;
+  void profile_kernel(const char* name) override;
+  void profile_read(double* total_ms, int* launches, double* alg_bytes) override;
+  int check_finite(int* bad_field) override;
 };
 
 namespace {
 
-void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv) {
+void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv) {
   if (g.nx < 1 || g.ny < 1 || g.nz < 3) throw Error(MI_EINVAL, "grid: need nx >= 1, ny >= 1, nz >= 3");
   if (g.bc_x != MI_BC_PERIODIC && g.bc_x != MI_BC_SYMMETRIC) throw Error(MI_EINVAL, "bc_x: periodic or symmetric");
   if (g.bc_y != MI_BC_PERIODIC && g.bc_y != MI_BC_SYMMETRIC) throw Error(MI_EINVAL, "bc_y: periodic or symmetric");
@@ -358,7 +371,7 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   if (p.v_adv_order != 3 && p.v_adv_order != 4) throw Error(MI_EINVAL, "v_adv_order must be 3 or 4");
   if (p.damp_opt != 0 && p.damp_opt != 3) throw Error(MI_EINVAL, "damp_opt must be 0 or 3");
   if (p.hypsometric_opt != 1) throw Error(MI_EINVAL, "hypsometric_opt must be 1");
-  if (p.precision != MI_FP64) throw Error(MI_EINVAL, "precision: only MI_FP64 is available");
+  if (p.precision != MI_FP64 && p.precision != MI_FP32_BULK) throw Error(MI_EINVAL, "precision: MI_FP64 or MI_FP32_BULK");
   if (p.damp_opt == 3 && !(p.zdamp > 0)) throw Error(MI_EINVAL, "zdamp must be > 0");
   if (p.n_moist < 0 || p.n_moist > MI_MAX_MOIST) throw Error(MI_EINVAL, "n_moist must be in 0..8");
   if (p.n_scalars < p.n_moist || p.n_scalars > MI_MAX_SCALARS) throw Error(MI_EINVAL, "n_scalars must be in n_moist..16");
@@ -371,9 +384,9 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   d.ctx = ctx;
   d.grid = g;
   d.prm = p;
-  d.n_sound = p.time_step_sound > 0 ? p.time_step_sound : mi_auto_sound_steps(g.dt, g.dx, g.dy);
-  if (d.n_sound < 2 || d.n_sound % 2) throw Error(MI_EINVAL, "time_step_sound must be even and >= 2");
-  if (!mi_fp64::col_cfg_fits(mi_fp64::kColAcoustic, mi_fp64::kNumColCfgs - 1, g.nz)) throw Error(MI_EINVAL, "nz must be at most 255");
+  d.n_sound_ = p.time_step_sound > 0 ? p.time_step_sound : mi_auto_sound_steps(g.dt, g.dx, g.dy);
+  if (d.n_sound_ < 2 || d.n_sound_ % 2) throw Error(MI_EINVAL, "time_step_sound must be even and >= 2");
+  if (!col_cfg_fits(kColAcoustic, kNumColCfgs - 1, g.nz)) throw Error(MI_EINVAL, "nz must be at most 255");
   if (const char* e = std::getenv("MI_GRAPHS")) d.use_graph = std::strcmp(e, "0") != 0;
 
   const bool two_d = g.ny == 1;
@@ -397,13 +410,15 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   d.n3 = G.sk * int64_t(g.nz + 1);
 
   auto& f = d.a.f;
-  auto add3 = [&](double*& slot, St s, int id) {
-    slot = d.alloc(d.n3) + d.off;
-    d.fields.push_back(FieldDesc{&slot, s, id});
+  // 3-D fields of the storage type of their Fields member (real, or double for the FP64 islands)
+  auto add3 = [&](auto*& slot, St s, int id) {
+    using T = std::remove_reference_t<decltype(*slot)>;
+    slot = d.template alloc<T>(d.n3) + d.off;
+    d.fields.push_back(FieldDesc{reinterpret_cast<void**>(&slot), s, id, int(sizeof(T))});
   };
   auto add2 = [&](double*& slot, St s, int id) {
     slot = d.alloc(d.n2) + d.off;
-    d.fields.push_back(FieldDesc{&slot, s, id});
+    d.fields.push_back(FieldDesc{reinterpret_cast<void**>(&slot), s, id, int(sizeof(double))});
   };
   add3(f.u1, St::U, -1); add3(f.u2, St::U, MI_U);
   add3(f.v1, St::V, -1); add3(f.v2, St::V, MI_V);
@@ -428,10 +443,10 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   add2(f.msfux, St::U2, MI_MSFUX); add2(f.msfuy, St::U2, MI_MSFUY);
   add2(f.msfvx, St::V2, MI_MSFVX); add2(f.msfvy, St::V2, MI_MSFVY); add2(f.msfvx_inv, St::V2, MI_MSFVX_INV);
   for (double* m : {f.msftx, f.msfty, f.msfux, f.msfuy, f.msfvx, f.msfvy, f.msfvx_inv})
-    mi_fp64::launch_fill(m - d.off, d.n2, 1.0, ctx->stream);
+    launch_fill(m - d.off, d.n2, 1.0, ctx->stream);
   // Coriolis and grid rotation: f = e = sina = 0, cosa = 1 unless set
   add2(f.fcor, St::M2, MI_F); add2(f.ecor, St::M2, MI_E); add2(f.sina, St::M2, MI_SINA); add2(f.cosa, St::M2, MI_COSA);
-  mi_fp64::launch_fill(f.cosa - d.off, d.n2, 1.0, ctx->stream);
+  launch_fill(f.cosa - d.off, d.n2, 1.0, ctx->stream);
   if (p.diff_6th_opt != 0) {  // the held tendencies of a diffusion driver
     add3(f.ru_tendf, St::U, -1); add3(f.rv_tendf, St::V, -1); add3(f.rw_tendf, St::W, -1); add3(f.t_tendf, St::M, -1);
   }
@@ -490,10 +505,10 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   for (int q = 0; q < 14; ++q) *dst[q] = lvb + size_t(q) * (nz + 1);
   c.cf1 = lv.cf1; c.cf2 = lv.cf2; c.cf3 = lv.cf3; c.cfn = lv.cfn; c.cfn1 = lv.cfn1;
   c.rdx = lv.rdx; c.rdy = lv.rdy; c.p_top = lv.p_top;
-  for (int kern = 0; kern < mi_fp64::kNumColKernels; ++kern) d.colcfg[kern] = mi_fp64::col_cfg_default(kern, g.nx, g.ny, g.nz);
+  for (int kern = 0; kern < kNumColKernels; ++kern) d.colcfg[kern] = col_cfg_default(kern, g.nx, g.ny, g.nz);
   if (d.n3 >= (int64_t(1) << 31)) throw Error(MI_EINVAL, "grid too large: a field must hold fewer than 2^31 points");
   // run parameters
-  mi_fp64::Params& q = d.a.q;
+  Params& q = d.a.q;
   q.hup = p.h_adv_order == 5 ? 1.0 : 0.0;
   q.vup = p.v_adv_order == 3 ? 1.0 : 0.0;
   q.epssm = p.epssm;
@@ -516,7 +531,7 @@ void build_domain(mi_domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& 
   q.terrain = p.terrain;
 }
 
-FieldDesc* find(mi_domain* d, int id) {
+FieldDesc* find(Domain* d, int id) {
   for (FieldDesc& f : d->fields)
     if (f.id == id) return &f;
   return nullptr;
@@ -524,269 +539,154 @@ FieldDesc* find(mi_domain* d, int id) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-const char* mi_last_error(void) { return g_last_error.c_str(); }
-int mi_abi_version(void) { return MI_ABI_VERSION; }
-
-int mi_auto_sound_steps(double dt, double dx, double dy) {
-  const double spacing = dx < dy ? dx : dy;
-  const int n = 2 * (int(300.0 * dt / spacing - 0.01) + 1);
-  return n > 4 ? n : 4;
+// host <-> device copies of one field's allocation, converting between the ABI's double and the storage type
+void copy_in(void* dev, const std::vector<double>& h, int esize, hipStream_t st) {
+  if (esize == 8) {
+    hip_check(hipMemcpyAsync(dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  } else {
+    std::vector<float> f(h.begin(), h.end());
+    hip_check(hipMemcpyAsync(dev, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");  // f goes out of scope
+  }
 }
-
-int mi_ctx_create(int device, void* stream, mi_ctx** out) {
-  return mi_detail::guard([&] {
-    if (!out) throw Error(MI_EINVAL, "null argument");
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(MI_ENODEV, "no HIP device");
-    if (device < 0 || device >= n) throw Error(MI_EINVAL, "device index out of range");
-    hip_check(hipSetDevice(device), "hipSetDevice");
-    auto c = std::make_unique<mi_ctx>();
-    c->device = device;
-    if (stream) {
-      c->stream = static_cast<hipStream_t>(stream);
-    } else {
-      hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-      c->own_stream = true;
-    }
-    *out = c.release();
-  });
-}
-
-int mi_ctx_destroy(mi_ctx* c) {
-  return mi_detail::guard([&] {
-    if (!c) return;
-    if (c->own_stream) {
-      (void)hipStreamSynchronize(c->stream);
-      (void)hipStreamDestroy(c->stream);
-    }
-    delete c;
-  });
-}
-
-int mi_domain_create(mi_ctx* ctx, const mi_grid* g, const mi_params* p, const mi_levels* lv, mi_domain** out) {
-  return mi_detail::guard([&] {
-    if (!ctx || !g || !p || !lv || !out) throw Error(MI_EINVAL, "null argument");
-    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-    auto d = std::make_unique<mi_domain>();
-    build_domain(*d, ctx, *g, *p, *lv);
-    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    *out = d.release();
-  });
-}
-
-int mi_domain_destroy(mi_domain* d) {
-  return mi_detail::guard([&] {
-    if (!d) return;
-    (void)hipStreamSynchronize(d->ctx->stream);
-    delete d;
-  });
-}
-
-int mi_domain_n_sound(const mi_domain* d) { return d ? d->n_sound : MI_EINVAL; }
-int64_t mi_domain_device_bytes(const mi_domain* d) { return d ? d->bytes : MI_EINVAL; }
-
-int mi_domain_set_field(mi_domain* d, int id, const double* host, int64_t n) {
-  return mi_detail::guard([&] {
-    if (!d || !host) throw Error(MI_EINVAL, "null argument");
-    hipStream_t st = d->ctx->stream;
-    const int nz = d->grid.nz;
-    const bool base3 = id == MI_PB || id == MI_ALB || id == MI_PHB || id == MI_T_INIT;
-    // a base 3-D profile on a domain with terrain: repeated over every column, halos included
-    if (base3 && d->prm.terrain && n == nz + (id == MI_PHB ? 1 : 0)) {
-      FieldDesc* fd = find(d, id);
-      std::vector<double> h(static_cast<size_t>(d->n3), 0.0);
-      for (int64_t e = 0; e < d->n3; ++e) h[size_t(e)] = host[std::min<int64_t>(e / d->a.g.sk, n - 1)];
-      hip_check(hipMemcpyAsync(*fd->slot - d->off, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st),
-                "hipMemcpyAsync");
-      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-      return;
-    }
-    // base 3-D profiles
-    if (base3 && !d->prm.terrain) {
-      const int nzs = nz + (id == MI_PHB ? 1 : 0);
-      if (n != nzs) throw Error(MI_EINVAL, "base 3-D fields take a profile of nz (phb: nz+1) values");
-      double* dst = id == MI_PB ? d->pb_buf : id == MI_ALB ? d->alb_buf : id == MI_PHB ? d->phb_buf : d->t_init_buf;
-      hip_check(hipMemcpyAsync(dst, host, size_t(n) * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-      return;
-    }
-    FieldDesc* fd = find(d, id);
-    if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " cannot be set");
-    const bool base2 = id == MI_MUB || (id >= MI_MSFTX && id <= MI_HT);
-    const int nxs = d->nxs(fd->st), nys = d->nys(fd->st), nzs = d->nzs(fd->st);
-    const int64_t alloc_n = (fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2) ? d->n2 : d->n3;
-    double* base = *fd->slot - d->off;
-    if (base2 && n == 1) {  // horizontally uniform value, halos included
-      mi_fp64::launch_fill(base, alloc_n, host[0], st);
-      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-      return;
-    }
-    if (n != int64_t(nxs) * nys * nzs) throw Error(MI_EINVAL, "field size mismatch");
-    std::vector<double> h(static_cast<size_t>(alloc_n), 0.0);
-    hip_check(hipMemcpyAsync(h.data(), base, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+void copy_out(std::vector<double>& h, const void* dev, int esize, hipStream_t st) {
+  if (esize == 8) {
+    hip_check(hipMemcpyAsync(h.data(), dev, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    const Geo& G = d->a.g;
-    for (int k = 0; k < nzs; ++k)
-      for (int j = 0; j < nys; ++j)
-        for (int i = 0; i < nxs; ++i) h[size_t(d->off + G.at(i, j, k))] = host[(int64_t(k) * nys + j) * nxs + i];
-    hip_check(hipMemcpyAsync(base, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    d->halo({{*fd->slot, -1, -1}});
+  } else {
+    std::vector<float> f(h.size());
+    hip_check(hipMemcpyAsync(f.data(), dev, f.size() * sizeof(float), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-  });
+    for (size_t q = 0; q < f.size(); ++q) h[q] = f[q];
+  }
 }
 
-int mi_domain_get_field(mi_domain* d, int id, double* host, int64_t n) {
-  return mi_detail::guard([&] {
-    if (!d || !host) throw Error(MI_EINVAL, "null argument");
-    FieldDesc* fd = find(d, id);
-    if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " cannot be read");
-    const int nxs = d->nxs(fd->st), nys = d->nys(fd->st), nzs = d->nzs(fd->st);
-    if (n != int64_t(nxs) * nys * nzs) throw Error(MI_EINVAL, "field size mismatch");
-    const int64_t alloc_n = (fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2) ? d->n2 : d->n3;
-    std::vector<double> h(static_cast<size_t>(alloc_n));
-    hipStream_t st = d->ctx->stream;
-    hip_check(hipMemcpyAsync(h.data(), *fd->slot - d->off, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+}  // namespace
+
+void Domain::set_field(int id, const double* host, int64_t n) {
+  hipStream_t st = ctx->stream;
+  const int nz = grid.nz;
+  const bool base3 = id == MI_PB || id == MI_ALB || id == MI_PHB || id == MI_T_INIT;
+  // a base 3-D profile on a domain with terrain: repeated over every column, halos included
+  if (base3 && prm.terrain && n == nz + (id == MI_PHB ? 1 : 0)) {
+    FieldDesc* fd = find(this, id);
+    std::vector<double> h(static_cast<size_t>(n3), 0.0);
+    for (int64_t e = 0; e < n3; ++e) h[size_t(e)] = host[std::min<int64_t>(e / a.g.sk, n - 1)];
+    copy_in(static_cast<char*>(*fd->slot) - off * fd->esize, h, fd->esize, st);
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    const Geo& G = d->a.g;
-    for (int k = 0; k < nzs; ++k)
-      for (int j = 0; j < nys; ++j)
-        for (int i = 0; i < nxs; ++i) host[(int64_t(k) * nys + j) * nxs + i] = h[size_t(d->off + G.at(i, j, k))];
-  });
-}
-
-int mi_step(mi_domain* d, int nsteps) {
-  return mi_detail::guard([&] {
-    if (!d) throw Error(MI_EINVAL, "null argument");
-    if (nsteps < 0) throw Error(MI_EINVAL, "nsteps must be >= 0");
-    hip_check(hipSetDevice(d->ctx->device), "hipSetDevice");
-    d->step(nsteps);
-  });
-}
-
-int mi_domain_sync(mi_domain* d) {
-  return mi_detail::guard([&] {
-    if (!d) throw Error(MI_EINVAL, "null argument");
-    hip_check(hipStreamSynchronize(d->ctx->stream), "hipStreamSynchronize");
-  });
-}
-
-int mi_step_begin(mi_domain* d) {
-  return mi_detail::guard([&] {
-    if (!d) throw Error(MI_EINVAL, "null argument");
-    d->step_begin();
-    hip_check(hipGetLastError(), "kernel launch");
-  });
-}
-int mi_rk_prep(mi_domain* d, int rk) {
-  return mi_detail::guard([&] {
-    if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
-    d->rk_prep();
-    hip_check(hipGetLastError(), "kernel launch");
-  });
-}
-int mi_rk_tendency(mi_domain* d, int rk) {
-  return mi_detail::guard([&] {
-    if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
-    d->rk_tendency(rk);
-    hip_check(hipGetLastError(), "kernel launch");
-  });
-}
-int mi_small_steps(mi_domain* d, int rk) {
-  return mi_detail::guard([&] {
-    if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
-    d->small_steps(rk, false);
-    hip_check(hipGetLastError(), "kernel launch");
-  });
-}
-int mi_scalar_transport(mi_domain* d, int rk) {
-  return mi_detail::guard([&] {
-    if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
-    d->scalar_transport(rk);
-    hip_check(hipGetLastError(), "kernel launch");
-  });
-}
-int mi_calc_p_rho_phi(mi_domain* d, int rk) {
-  return mi_detail::guard([&] {
-    if (!d || rk < 1 || rk > 3) throw Error(MI_EINVAL, "bad argument");
-    d->calc_p_rho_phi();
-    hip_check(hipGetLastError(), "kernel launch");
-  });
-}
-
-int mi_domain_autotune(mi_domain* d, int trials) {
-  return mi_detail::guard([&] {
-    if (!d) throw Error(MI_EINVAL, "null argument");
-    hip_check(hipSetDevice(d->ctx->device), "hipSetDevice");
-    d->autotune(trials);
-  });
-}
-
-int mi_domain_set_tuning(mi_domain* d, const int* shapes, int n) {
-  return mi_detail::guard([&] {
-    if (!d || !shapes) throw Error(MI_EINVAL, "null argument");
-    for (int q = 0; q < n && q < mi_fp64::kNumColKernels; ++q)
-      if (!mi_fp64::col_cfg_fits(q, shapes[q], d->grid.nz)) throw Error(MI_EINVAL, "block shape does not fit this nz");
-    for (int q = 0; q < n && q < mi_fp64::kNumColKernels; ++q) d->colcfg[q] = shapes[q];
-    d->drop_graphs();
-  });
-}
-
-int mi_domain_get_tuning(const mi_domain* d, int* shapes, int n) {
-  return mi_detail::guard([&] {
-    if (!d || !shapes) throw Error(MI_EINVAL, "null argument");
-    for (int q = 0; q < n && q < mi_fp64::kNumColKernels; ++q) shapes[q] = d->colcfg[q];
-  });
-}
-
-int mi_profile_kernel(mi_domain* d, const char* name) {
-  return mi_detail::guard([&] {
-    if (!d) throw Error(MI_EINVAL, "null argument");
-    d->prof_name = name ? name : "";
-    d->timer.used = 0;
-    d->timer.bytes = 0;
-  });
-}
-
-int mi_profile_read(mi_domain* d, double* total_ms, int* launches, double* alg_bytes) {
-  return mi_detail::guard([&] {
-    if (!d) throw Error(MI_EINVAL, "null argument");
-    hip_check(hipStreamSynchronize(d->ctx->stream), "hipStreamSynchronize");
-    double tot = 0;
-    for (size_t q = 0; q + 1 < d->timer.used; q += 2) {
-      float ms = 0;
-      hip_check(hipEventElapsedTime(&ms, d->timer.ev[q], d->timer.ev[q + 1]), "hipEventElapsedTime");
-      tot += ms;
-    }
-    if (total_ms) *total_ms = tot;
-    if (launches) *launches = int(d->timer.used / 2);
-    if (alg_bytes) *alg_bytes = d->timer.bytes;
-    d->timer.used = 0;
-    d->timer.bytes = 0;
-  });
-}
-
-int mi_check_finite(mi_domain* d, int* bad_field) {
-  int rc = MI_OK;
-  const int g = mi_detail::guard([&] {
-    if (!d) throw Error(MI_EINVAL, "null argument");
-    hipStream_t st = d->ctx->stream;
-    hip_check(hipMemsetAsync(d->flags, 0, 64 * sizeof(int), st), "hipMemsetAsync");
-    mi_fp64::launch_finite(d->a, d->flags, st);
-    int h[64];
-    hip_check(hipMemcpyAsync(h, d->flags, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    return;
+  }
+  // base 3-D profiles
+  if (base3 && !prm.terrain) {
+    const int nzs = nz + (id == MI_PHB ? 1 : 0);
+    if (n != nzs) throw Error(MI_EINVAL, "base 3-D fields take a profile of nz (phb: nz+1) values");
+    double* dst = id == MI_PB ? pb_buf : id == MI_ALB ? alb_buf : id == MI_PHB ? phb_buf : t_init_buf;
+    hip_check(hipMemcpyAsync(dst, host, size_t(n) * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    int bad = -1;
-    for (int q = 0; q < 64 && bad < 0; ++q)
-      if (h[q]) bad = q;
-    if (bad_field) *bad_field = bad;
-    if (bad >= 0) {
-      mi_detail::set_last_error("non-finite value in field " + std::to_string(bad));
-      rc = MI_ENONFINITE;
-    }
-  });
-  return g != MI_OK ? g : rc;
+    return;
+  }
+  FieldDesc* fd = find(this, id);
+  if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " cannot be set");
+  const bool base2 = id == MI_MUB || (id >= MI_MSFTX && id <= MI_HT);
+  const int fnx = nxs(fd->st), fny = nys(fd->st), fnz = nzs(fd->st);
+  const int64_t alloc_n = (fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2) ? n2 : n3;
+  void* base = static_cast<char*>(*fd->slot) - off * fd->esize;
+  if (base2 && n == 1) {  // horizontally uniform value, halos included
+    launch_fill(static_cast<double*>(base), alloc_n, host[0], st);
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return;
+  }
+  if (n != int64_t(fnx) * fny * fnz) throw Error(MI_EINVAL, "field size mismatch");
+  std::vector<double> h(static_cast<size_t>(alloc_n), 0.0);
+  copy_out(h, base, fd->esize, st);
+  const Geo& G = a.g;
+  for (int k = 0; k < fnz; ++k)
+    for (int j = 0; j < fny; ++j)
+      for (int i = 0; i < fnx; ++i) h[size_t(off + G.at(i, j, k))] = host[(int64_t(k) * fny + j) * fnx + i];
+  copy_in(base, h, fd->esize, st);
+  halo({{*fd->slot, -1, -1}});
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
-}  // extern "C"
+void Domain::get_field(int id, double* host, int64_t n) {
+  FieldDesc* fd = find(this, id);
+  if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " cannot be read");
+  const int fnx = nxs(fd->st), fny = nys(fd->st), fnz = nzs(fd->st);
+  if (n != int64_t(fnx) * fny * fnz) throw Error(MI_EINVAL, "field size mismatch");
+  const int64_t alloc_n = (fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2) ? n2 : n3;
+  std::vector<double> h(static_cast<size_t>(alloc_n));
+  copy_out(h, static_cast<char*>(*fd->slot) - off * fd->esize, fd->esize, ctx->stream);
+  const Geo& G = a.g;
+  for (int k = 0; k < fnz; ++k)
+    for (int j = 0; j < fny; ++j)
+      for (int i = 0; i < fnx; ++i) host[(int64_t(k) * fny + j) * fnx + i] = h[size_t(off + G.at(i, j, k))];
+}
+
+void Domain::phase(MiPhase which, int rk) {
+  switch (which) {
+    case kPhaseStepBegin: step_begin(); break;
+    case kPhaseRkPrep: rk_prep(); break;
+    case kPhaseRkTendency: rk_tendency(rk); break;
+    case kPhaseSmallSteps: small_steps(rk, false); break;
+    case kPhaseScalars: scalar_transport(rk); break;
+    case kPhaseCalcPRhoPhi: calc_p_rho_phi(); break;
+  }
+  hip_check(hipGetLastError(), "kernel launch");
+}
+
+void Domain::set_tuning(const int* shapes, int n) {
+  for (int q = 0; q < n && q < kNumColKernels; ++q)
+    if (!col_cfg_fits(q, shapes[q], grid.nz)) throw Error(MI_EINVAL, "block shape does not fit this nz");
+  for (int q = 0; q < n && q < kNumColKernels; ++q) colcfg[q] = shapes[q];
+  drop_graphs();
+}
+
+void Domain::get_tuning(int* shapes, int n) const {
+  for (int q = 0; q < n && q < kNumColKernels; ++q) shapes[q] = colcfg[q];
+}
+
+void Domain::profile_kernel(const char* name) {
+  prof_name = name ? name : "";
+  timer.used = 0;
+  timer.bytes = 0;
+}
+
+void Domain::profile_read(double* total_ms, int* launches, double* alg_bytes) {
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  double tot = 0;
+  for (size_t q = 0; q + 1 < timer.used; q += 2) {
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, timer.ev[q], timer.ev[q + 1]), "hipEventElapsedTime");
+    tot += ms;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = int(timer.used / 2);
+  if (alg_bytes) *alg_bytes = timer.bytes;
+  timer.used = 0;
+  timer.bytes = 0;
+}
+
+int Domain::check_finite(int* bad_field) {
+  hipStream_t st = ctx->stream;
+  hip_check(hipMemsetAsync(flags, 0, 64 * sizeof(int), st), "hipMemsetAsync");
+  launch_finite(a, flags, st);
+  int h[64];
+  hip_check(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  int bad = -1;
+  for (int q = 0; q < 64 && bad < 0; ++q)
+    if (h[q]) bad = q;
+  if (bad_field) *bad_field = bad;
+  return bad;
+}
+
+mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv) {
+  auto d = std::make_unique<Domain>();
+  build_domain(*d, ctx, g, p, lv);
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return d.release();
+}
+
+}  // namespace MI_NS

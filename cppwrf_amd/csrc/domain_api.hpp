@@ -1,0 +1,43 @@
+// The precision-independent interface of a domain behind the C ABI.  One implementation per arithmetic
+// (domain.hip compiled for FP64 and for FP32-bulk storage); the extern "C" functions of capi.hip reach
+// them through this interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/cppwrf_amd.h"
+
+struct mi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+};
+
+// phases of RK3::step for the phase-by-phase entry points
+enum MiPhase { kPhaseStepBegin, kPhaseRkPrep, kPhaseRkTendency, kPhaseSmallSteps, kPhaseScalars, kPhaseCalcPRhoPhi };
+
+struct mi_domain {
+  virtual ~mi_domain() = default;
+  virtual mi_ctx* context() const = 0;
+  virtual int n_sound() const = 0;
+  virtual int64_t device_bytes() const = 0;
+  virtual void set_field(int id, const double* host, int64_t n) = 0;
+  virtual void get_field(int id, double* host, int64_t n) = 0;
+  virtual void step(int nsteps) = 0;
+  virtual void phase(MiPhase which, int rk) = 0;
+  virtual void autotune(int trials) = 0;
+  virtual void set_tuning(const int* shapes, int n) = 0;
+  virtual void get_tuning(int* shapes, int n) const = 0;
+  virtual void profile_kernel(const char* name) = 0;
+  virtual void profile_read(double* total_ms, int* launches, double* alg_bytes) = 0;
+  virtual int check_finite(int* bad_field) = 0;  // -1 or the first field id with a non-finite value
+};
+
+namespace mi_fp64 {
+mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv);
+}
+namespace mi_fp32 {
+mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv);
+}

@@ -14,7 +14,7 @@
 
 #include "libm_port.hpp"
 
-namespace mi_fp64 {
+namespace MI_NS {
 
 thread_local KernelTimer* g_timer = nullptr;
 
@@ -130,14 +130,15 @@ void launch_cols(const char* name, K kern, int nx, int ny, int nz, hipStream_t s
 // ----------------------------------------------------------------------------- halo fill
 // One halo slot of one direction (reference semantics src/core/halo.cpp:52-84): slot a = 0..7 maps to
 // i = 0, -1, -2, -3, n, n+1, n+2, n+3 on face-staggered points and -1, -2, -3, n, n+1, n+2 otherwise.
-__device__ __forceinline__ void halo_slot(double* base, idx_t sn, int a, int n, bool face, bool periodic, int par) {
+template <class T>
+__device__ __forceinline__ void halo_slot(T* base, idx_t sn, int a, int n, bool face, bool periodic, int par) {
   const int nl = face ? 4 : 3;
   const int i = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
   int src;
   if (periodic) src = ((i % n) + n) % n;
   else if (face) src = i <= 0 ? -i : 2 * n - i;
   else src = i < 0 ? -1 - i : 2 * n - 1 - i;
-  double v = base[idx_t(src) * sn];
+  T v = base[idx_t(src) * sn];
   if (!periodic && src == i) {
     if (par == 2) base[idx_t(i) * sn] = -v;
     return;
@@ -155,8 +156,9 @@ __global__ void k_halo_x(Geo g, HaloSet s) {
   if (f >= s.n || k >= s.nzs[f] || j >= s.nys[f]) return;
   const bool face = s.face_x[f] != 0;
   if (a >= (face ? 8 : 6)) return;
-  double* base = s.p[f] + g.at(0, j, k);
-  halo_slot(base, 1, a, g.nx, face, g.bcx == MI_BC_PERIODIC, s.px[f]);
+  const bool per = g.bcx == MI_BC_PERIODIC;
+  if (s.f32[f]) halo_slot(static_cast<float*>(s.p[f]) + g.at(0, j, k), 1, a, g.nx, face, per, s.px[f]);
+  else halo_slot(static_cast<double*>(s.p[f]) + g.at(0, j, k), 1, a, g.nx, face, per, s.px[f]);
 }
 // y pass: thread (x index over the whole row incl. halos, slot, level, field)
 __global__ void k_halo_y(Geo g, HaloSet s) {
@@ -167,8 +169,9 @@ __global__ void k_halo_y(Geo g, HaloSet s) {
   if (f >= s.n || k >= s.nzs[f] || i >= s.nxs[f] + kHalo) return;
   const bool face = s.face_y[f] != 0;
   if (a >= (face ? 8 : 6)) return;
-  double* base = s.p[f] + g.at(i, 0, k);
-  halo_slot(base, g.sj, a, g.ny, face, g.bcy == MI_BC_PERIODIC, s.py[f]);
+  const bool per = g.bcy == MI_BC_PERIODIC;
+  if (s.f32[f]) halo_slot(static_cast<float*>(s.p[f]) + g.at(i, 0, k), g.sj, a, g.ny, face, per, s.py[f]);
+  else halo_slot(static_cast<double*>(s.p[f]) + g.at(i, 0, k), g.sj, a, g.ny, face, per, s.py[f]);
 }
 
 // ----------------------------------------------------------------------------- halo images
@@ -202,18 +205,18 @@ __device__ __forceinline__ void halo_images(int i, int n, bool face, bool period
 }
 
 // p[at(i, j) + kofs] = v and its halo images
-template <int KIND>
-__device__ __forceinline__ void put(double* p, const Geo& g, int i, int j, idx_t kofs, double v) {
+template <int KIND, class T>
+__device__ __forceinline__ void put(T* p, const Geo& g, int i, int j, idx_t kofs, double v) {
   constexpr bool fx = KIND == kHU || KIND == kHMuu, fy = KIND == kHV || KIND == kHMuv;
   constexpr int px = KIND == kHU ? 1 : (KIND == kHMuu ? 2 : 0), py = fy ? 1 : 0;
   const bool perx = g.bcx == MI_BC_PERIODIC, pery = g.bcy == MI_BC_PERIODIC;
-  double* q = p + kofs;
+  T* q = p + kofs;
   halo_images(i, g.nx, fx, perx, px, [&](int ii, bool nx_) {
     if (g.two_d) {
-      q[ii] = nx_ ? -v : v;
+      q[ii] = T(nx_ ? -v : v);
       return;
     }
-    halo_images(j, g.ny, fy, pery, py, [&](int jj, bool ny_) { q[ii + jj * g.sj] = (nx_ != ny_) ? -v : v; });
+    halo_images(j, g.ny, fy, pery, py, [&](int jj, bool ny_) { q[ii + jj * g.sj] = T((nx_ != ny_) ? -v : v); });
   });
 }
 
@@ -228,7 +231,7 @@ __global__ void k_copy_step(Fields f, const double* base3, const double* base2, 
   const int which = int(blockIdx.y);
   if (which < 5) {
     if (e >= n3) return;
-    double *dst, *src;
+    real *dst, *src;
     int64_t off;
     switch (which) {
       case 0: dst = f.u1; src = f.u2; off = off3_u; break;
@@ -382,7 +385,7 @@ __global__ void k_tend_u(Range r, Args a) {
   const int nz = g.nz;
   const idx_t sj = g.sj, sk = g.sk;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double* u = f.u2;
+  const auto* u = f.u2;
   const double hup = a.q.hup, vup = a.q.vup;
   auto fx = [&](idx_t y) {  // face between y-1 and y
     const double vel = H * (f.ru[y] + f.ru[y - 1]);
@@ -439,7 +442,7 @@ __global__ void k_tend_v(Range r, Args a) {
   const int nz = g.nz;
   const idx_t sj = g.sj, sk = g.sk;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double* v = f.v2;
+  const auto* v = f.v2;
   const double hup = a.q.hup, vup = a.q.vup;
   auto fx = [&](idx_t y) {
     const double vel = H * (f.ru[y] + f.ru[y - sj]);
@@ -496,7 +499,7 @@ __global__ void k_tend_w(Range r, Args a) {
   const int nz = g.nz;
   const idx_t sj = g.sj, sk = g.sk;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double* w = f.w2;
+  const auto* w = f.w2;
   const double hup = a.q.hup, vup = a.q.vup;
   auto rux = [&](idx_t y) {  // ru at w level k
     if (k < nz) return c.fnm[k] * f.ru[y] + c.fnp[k] * f.ru[y - sk];
@@ -560,7 +563,7 @@ __global__ void k_tend_t(Range r, Args a) {
   const int nz = g.nz;
   const idx_t sj = g.sj, sk = g.sk;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double* t2 = f.t2;
+  const auto* t2 = f.t2;
   const double hup = a.q.hup, vup = a.q.vup;
   auto fx = [&](idx_t y) {
     const double vel = f.ru[y];
@@ -596,9 +599,9 @@ __global__ void k_tend_ph(Range r, Args a) {
   const int nz = g.nz;
   const idx_t sj = g.sj, sk = g.sk;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double* ph = f.ph2;
-  const double* u = f.u2;
-  const double* v = f.v2;
+  const auto* ph = f.ph2;
+  const auto* u = f.u2;
+  const auto* v = f.v2;
   auto wdwn = [&](int m) {
     const idx_t y = g.at(i, j, m);
     return H * (f.ww[y + sk] + f.ww[y]) * c.rdnw[m] * (ph[y + sk] - ph[y] + f.phb(i, j, m + 1) - f.phb(i, j, m));
@@ -1042,9 +1045,9 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       // (small_step.cpp:339-353)
       wk = 0;
       if (a.q.terrain) {
-        const double* ht = f.ht;
-        const double* us = f.u2;
-        const double* vs = f.v2;
+        const auto* ht = f.ht;
+        const auto* us = f.u2;
+        const auto* vs = f.v2;
         wk = msftx * H * c.rdx *
              ((ht[x2 + 1] - ht[x2]) * (c.cf1 * us[x + 1] + c.cf2 * us[x + 1 + sk] + c.cf3 * us[x + 1 + 2 * sk]) +
               (ht[x2] - ht[x2 - 1]) * (c.cf1 * us[x] + c.cf2 * us[x + sk] + c.cf3 * us[x + 2 * sk]));
@@ -1211,7 +1214,7 @@ __device__ __forceinline__ double d6(double m3, double m2, double m1, double q0,
 // tendency + tend_x + tend_y of one field at index y (2-D index y2): kind 'u', 'v' or mass ('m' and 'w');
 // c1, c2 the level's mass coefficients; mu the column mass (mut for the dynamics, muts for the scalars)
 template <char KIND>
-__device__ __forceinline__ double sixth_at(double tend, const Args& a, const double* q, const double* mu, idx_t y, idx_t y2,
+__device__ __forceinline__ double sixth_at(double tend, const Args& a, const real* q, const double* mu, idx_t y, idx_t y2,
                                            double c1, double c2, double coef) {
   const Fields& f = a.f;
   const idx_t sj = a.g.sj;
@@ -1291,7 +1294,8 @@ __global__ void k_set_w_surface(Range r, Args a) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, 0), x2 = g.at2(i, j), sj = g.sj, sk = g.sk;
-  const double *ht = f.ht, *u = f.u2, *v = f.v2;
+  const double* ht = f.ht;
+  const real *u = f.u2, *v = f.v2;
   (void)k;  // one level
   const double ws =
       f.msfty[x2] * 0.5 * c.rdy *
@@ -1340,8 +1344,8 @@ struct SpList {
 // sides with the same expression, which gives the reference's stored face values bit for bit.
 struct ScFaces {
   const Args& a;
-  const double* q;
-  const double* qo;
+  const real* q;
+  const real* qo;
   double dt;
   bool pd;
   __device__ double f5(double m3, double m2, double m1, double q0, double p1, double p2, double v) const {
@@ -1505,7 +1509,7 @@ __global__ void k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
 #pragma unroll 1
   for (int q = 0; q < l.n; ++q) {
     const int n = l.sp[q];
-    const double* sc = s.scale[n];
+    const real* sc = s.scale[n];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
     const PdFaces p = pd_faces(fl, g, i, j, k);
     // the limited flux of a face: scaled by the scale of the cell it leaves
@@ -1724,4 +1728,4 @@ void launch_finite(const Args& a, int* flags, hipStream_t st) {
   launch3("finite", k_finite, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, flags);
 }
 
-}  // namespace mi_fp64
+}  // namespace MI_NS

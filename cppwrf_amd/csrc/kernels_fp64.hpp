@@ -1,11 +1,24 @@
-// FP64 dynamics kernels: declarations and the argument block shared by all of them.
+// The dynamics kernels: declarations and the argument block shared by all of them.
+//
+// The source is compiled once per arithmetic: MI_REAL = double in namespace mi_fp64 (plain FP64, bit-
+// identical to the reference CPU FP64 build) and MI_REAL = float in namespace mi_fp32 (FP32-bulk: the
+// 3-D prognostic, tendency and work fields stored in FP32).  Whatever the storage, the base state, the
+// 2-D column masses, php, the tridiagonal coefficients and every LDS column buffer (the mass-divergence
+// sums, the omega integral, the Thomas solve of advance_w, p'' and al'' of calc_p_rho) stay FP64.
 #pragma once
 
 #include <vector>
 
 #include "dev_common.hpp"
 
-namespace mi_fp64 {
+#ifndef MI_REAL
+#define MI_REAL double
+#define MI_NS mi_fp64
+#endif
+
+namespace MI_NS {
+
+using real = MI_REAL;  // storage of the 3-D bulk fields
 
 using mi_dev::BField;
 using mi_dev::Geo;
@@ -15,34 +28,38 @@ using mi_dev::Lv;
 // Passed by value to every kernel: kernels read the members they use from the kernel-argument segment.
 struct Fields {
   // state (reference State, src/dyn/state.hpp:30-78)
-  double *u1, *u2, *v1, *v2, *w1, *w2, *t1, *t2, *ph1, *ph2, *mu1, *mu2;
-  double *p, *al, *alt, *php, *ww;
+  real *u1, *u2, *v1, *v2, *w1, *w2, *t1, *t2, *ph1, *ph2;
+  double *mu1, *mu2;
+  real *p, *al, *alt, *ww;
+  double* php;  // the full geopotential: its horizontal differences need FP64
   double *mut, *muu, *muv, *muts, *muus, *muvs;
-  double *ru, *rv, *rw, *ru_m, *rv_m, *ww_m;
+  real *ru, *rv, *rw, *ru_m, *rv_m, *ww_m;
   // tendencies (src/dyn/state.hpp:80-90)
-  double *ru_tend, *rv_tend, *rw_tend, *t_tend, *ph_tend, *mu_tend;
+  real *ru_tend, *rv_tend, *rw_tend, *t_tend, *ph_tend;
+  double* mu_tend;
   // acoustic-step work fields (src/dyn/small_step.hpp:43-49)
-  double *u_save, *v_save, *w_save, *t_save, *ph_save, *ww1, *c2a, *pm1, *ca, *calpha, *cgamma;
+  real *u_save, *v_save, *w_save, *t_save, *ph_save, *ww1, *c2a, *pm1;
+  double *ca, *calpha, *cgamma;  // the tridiagonal coefficients of advance_w
   double *mu_save, *mudf, *muave;
   // base state: 2-D fields stored in full with halos, 3-D through BField
   double *mub, *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy, *msfvx_inv;
   BField pb, alb, phb;
   // moisture (src/dyn/state.hpp:72-74): calc_cq coefficients and the microphysics heating
-  double *cqu, *cqv, *cqw, *hdia;
+  real *cqu, *cqv, *cqw, *hdia;
   // Coriolis and grid rotation (state.hpp:105-106); held tendencies of the first stage (state.hpp:85-88)
   double *fcor, *ecor, *sina, *cosa, *ht;
-  double *ru_tendf, *rv_tendf, *rw_tendf, *t_tendf;
+  real *ru_tendf, *rv_tendf, *rw_tendf, *t_tendf;
 };
 
 // The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
 // transport.  q2[n] / qn[n] are the two buffers of species n; the host swaps them after every stage.
 struct Scalars {
-  double* q1[MI_MAX_SCALARS];
-  double* q2[MI_MAX_SCALARS];
-  double* qtf[MI_MAX_SCALARS];
-  double* qn[MI_MAX_SCALARS];
-  double* scale[MI_MAX_SCALARS];  // advect_scalar_pd's outflow scale of each species
-  double* qtw[MI_MAX_SCALARS];    // the step's tendency: the caller's plus the sixth-order diffusion
+  real* q1[MI_MAX_SCALARS];
+  real* q2[MI_MAX_SCALARS];
+  real* qtf[MI_MAX_SCALARS];
+  real* qn[MI_MAX_SCALARS];
+  real* scale[MI_MAX_SCALARS];  // advect_scalar_pd's outflow scale of each species
+  real* qtw[MI_MAX_SCALARS];    // the step's tendency: the caller's plus the sixth-order diffusion
   int n_moist;
 };
 
@@ -73,7 +90,8 @@ constexpr int kHaloMax = 16;
 struct HaloSet {
   int n;
   int nkmax;
-  double* p[kHaloMax];
+  void* p[kHaloMax];
+  unsigned char f32[kHaloMax];  // element type: 1 float, 0 double
   int nxs[kHaloMax], nys[kHaloMax], nzs[kHaloMax];
   unsigned char face_x[kHaloMax], face_y[kHaloMax], px[kHaloMax], py[kHaloMax];
 };
@@ -120,4 +138,4 @@ void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hip
 void launch_p_rho_phi(const Args& a, hipStream_t st);
 void launch_finite(const Args& a, int* flags, hipStream_t st);
 
-}  // namespace mi_fp64
+}  // namespace MI_NS

@@ -49,8 +49,10 @@ extern "C" {
 /* lateral boundary conditions, in the order of the reference's enum BC (src/core/grid.hpp:19) */
 enum { MI_BC_PERIODIC = 0, MI_BC_SYMMETRIC = 1, MI_BC_SPECIFIED = 2, MI_BC_OPEN = 3 };
 
-/* arithmetic of a domain: plain FP64 (bit-identical to the reference CPU FP64 build) or
- * FP32 storage and arithmetic for the bulk with FP64 kept in the column solves */
+/* arithmetic of a domain: plain FP64 (bit-identical to the reference CPU FP64 build) or FP32-bulk: the
+ * 3-D prognostic, tendency, scalar and work fields stored in FP32 (half the HBM traffic), with FP64 kept
+ * for the base state, the 2-D column masses, the full geopotential php, the advance_w tridiagonal solve,
+ * the column mass-divergence sums and calc_p_rho; fields cross the ABI as double either way */
 enum { MI_FP64 = 0, MI_FP32_BULK = 1 };
 
 /* reference Grid (src/core/grid.hpp:21-31) */
