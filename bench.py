@@ -1,29 +1,38 @@
 #!/usr/bin/env python3
-"""Throughput of the RK3 split-explicit dynamics step on one MI355X.
+"""Throughput of the RK3 split-explicit dynamics step on one MI355X, FP32-bulk against plain FP64.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--workload hills|bubble64]
 
-Workload: the 64x64x40 dry warm bubble (1 km grid, 20 km top, periodic, damp_opt 3 Rayleigh layer of
-5 km, dt 6 s with 6 acoustic steps, 5th/3rd-order advection), plain FP64, synthetic seeded initial state
-built on the host.  A step is one RK3 large step (three stages, ten acoustic substeps); the timed region
-is K steps with the state already resident in HBM, bracketed by device synchronisations.
+Workload "hills" (default): the real-data-shaped 119x89x44 domain at 15 km (seeded hills up to 2.5 km high
+and 30-150 km wide, US standard atmosphere, seeded u/v jets of 5-40 m/s with +-1 m/s noise, map factors
+in [0.95, 1.05], Coriolis, a hybrid coordinate with a 50 hPa top, six moist species advected with the
+positive-definite limiter, qv from 80 % relative humidity falling to 10 % at 10 km), dt 90 s with 4
+acoustic steps, Rayleigh damping of w in the top 5 km.  Its lateral boundaries are periodic.
+Workload "bubble64": the 64x64x40 dry warm bubble at 1 km (dt 6 s, 6 acoustic steps, damp_opt 3).
 
-Prints one JSON line on stdout with the throughput in grid-point-steps/s (nx*ny*nz*steps/s), the
-roofline of the dominant kernel and the CPU baseline (the C++/OpenMP oracle on this host's cores, a
-bounded sample of the same workload).
+A step is one RK3 large step (three stages and their acoustic substeps, scalar transport included).  The
+timed region replays the step's HIP graph K times with the state resident in HBM, bracketed by device
+synchronisations; it is measured for the FP64 domain and for the FP32-bulk domain of the same initial
+state.  `value` is the FP32-bulk throughput in grid-point-steps/s (nx*ny*nz*steps/s); the FP64 figure,
+the speed-up and the relative L2 / max error of every variable of the FP32-bulk state against the FP64
+state after the same steps are printed beside it.
 
-The timed region replays the step's HIP graph K times.  The dominant kernel's launch duration comes from
-HIP events recorded on the domain's stream around each of its launches during K further steps run
-launch by launch (events cannot be read back from inside a replayed graph); its algorithmic bytes are
-the distinct arrays it reads and writes x 8 B x the points of one launch.
+The roofline is that of the dominant kernel of the FP32-bulk step: HIP events recorded on the domain's
+stream around each of its launches during K further steps run launch by launch (events cannot be read
+back from inside a replayed graph); algorithmic bytes = distinct 3-D arrays it reads and writes x their
+storage size x the points of one launch.  The CPU baseline is the C++/OpenMP oracle (the FP64 restatement
+of the reference) on this host's cores, a bounded sample of the same workload.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
@@ -32,23 +41,35 @@ sys.path.insert(0, os.path.join(HERE, "tests"))
 METRIC = "grid-point-steps/s FP32-batched vs FP64 plain at stated L2; p50/p99 forecast latency"
 HBM_PEAK_GBS = 8000.0
 # kernels whose event timing is sampled to find the dominant one
-CANDIDATES = ["large_prep", "tend_u", "tend_v", "tend_w", "tend_t", "tend_ph", "ss_prep", "ss_column", "uv", "acoustic",
-              "finish"]
+CANDIDATES = ["large_prep", "cq", "tend_u", "tend_v", "tend_w", "tend_t", "tend_ph", "ss_prep", "ss_column", "uv",
+              "acoustic", "finish", "sc_update", "sc_pd_pre", "sc_pd_scale", "sc_pd_update", "sixth", "sixth_sc",
+              "set_w_surface"]
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(nx=64, ny=64, nz=40):
+def workload(name):
     from cppwrf_amd.dyn import Case
-    case = Case.bubble3d(nx, ny, nz, dx=1000.0, ztop=20000.0, dt=6.0, seed=1, noise_amp=0.05)
-    prm = case.params(time_step_sound=6, damp_opt=3, zdamp=5000.0, dampcoef=0.2)
-    return case, prm
+    if name == "bubble64":
+        case = Case.bubble3d(64, 64, 40, dx=1000.0, ztop=20000.0, dt=6.0, seed=1, noise_amp=0.05)
+        kw = dict(time_step_sound=6, damp_opt=3, zdamp=5000.0, dampcoef=0.2)
+        desc = {"workload": "dry warm bubble 64x64x40, 1 km, 20 km top, periodic, damp_opt 3, dt 6 s",
+                "nx": 64, "ny": 64, "nz": 40, "dx_m": 1000.0, "dt_s": 6.0, "time_step_sound": 6}
+    else:
+        case = Case.hills(119, 89, 44, dx=15000.0, dt=90.0, seed=1)
+        kw = dict(time_step_sound=4, damp_opt=3, zdamp=5000.0, dampcoef=0.2, n_moist=6, n_scalars=6,
+                  moist_adv_opt=1)
+        desc = {"workload": "real-data-shaped 119x89x44 at 15 km: 12 seeded hills, US standard atmosphere, jets, "
+                            "map factors, Coriolis, hybrid coordinate (50 hPa top), 6 moist species with the PD "
+                            "limiter, dt 90 s, 4 acoustic steps; periodic lateral boundaries",
+                "nx": 119, "ny": 89, "nz": 44, "dx_m": 15000.0, "dt_s": 90.0, "time_step_sound": 4,
+                "moist_species": 6, "lateral_bc": "periodic"}
+    return case, kw, desc
 
 
 def profile(rk, name, steps):
-    import ctypes as C
     from cppwrf_amd._abi import check, lib
     check(lib().mi_profile_kernel(rk.handle, name.encode()))
     rk.step(steps)
@@ -58,7 +79,28 @@ def profile(rk, name, steps):
     return ms.value, n.value, b.value
 
 
-def cpu_baseline(case, prm, budget_s=12.0):
+def timed_steps(rk, steps):
+    rk.sync()
+    t0 = time.perf_counter()
+    rk.step(steps)
+    rk.sync()
+    return time.perf_counter() - t0
+
+
+def error_report(d32, d64, n_moist):
+    from cppwrf_amd import _abi as A
+    ids = {"U": A.MI_U, "V": A.MI_V, "W": A.MI_W, "T": A.MI_T, "PH": A.MI_PH, "MU": A.MI_MU}
+    if n_moist > 0:
+        ids["QVAPOR"] = A.MI_Q
+    out = {}
+    for name, fid in ids.items():
+        a, b = d32.field(fid), d64.field(fid)
+        nb = float(np.linalg.norm(b))
+        out[name] = {"rel_l2": float(np.linalg.norm(a - b)) / nb if nb > 0 else 0.0, "max_abs": float(np.abs(a - b).max())}
+    return out
+
+
+def cpu_baseline(case, prm, budget_s=15.0):
     """The oracle (C++/OpenMP, bit-identical to the reference CPU FP64 build) on this host's cores."""
     from oracle_lib import OracleDomain, olib
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -67,13 +109,14 @@ def cpu_baseline(case, prm, budget_s=12.0):
     t = time.perf_counter()
     orc.step(1)
     one = time.perf_counter() - t
-    n = max(1, min(100, int(budget_s / max(one, 1e-3))))
+    n = max(1, min(300, int(budget_s / max(one, 1e-3))))
     t = time.perf_counter()
     orc.step(n)
     dt = time.perf_counter() - t
     g = case.grid
     return {"value": g.nx * g.ny * g.nz * n / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle, {n} steps of the same 64x64x40 workload after 1 untimed step, {threads} OpenMP threads"}
+            "sample": f"oracle (FP64), {n} steps of the same {g.nx}x{g.ny}x{g.nz} workload after 1 untimed step, "
+                      f"{threads} OpenMP threads"}
 
 
 def main():
@@ -81,6 +124,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=["hills", "bubble64"], default="hills")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default block shapes")
     args = ap.parse_args()
@@ -89,64 +133,71 @@ def main():
               f"single GPU; independent domains shard across GPUs with no communication)", file=sys.stderr)
         sys.exit(2)
 
+    from cppwrf_amd import _abi as A
     from cppwrf_amd.dyn import Context, RK3
 
-    case, prm = workload()
+    case, kw, desc = workload(args.workload)
     g = case.grid
     ctx = Context(0)
-    rk = RK3(ctx, g, prm, case)
-    if not args.no_autotune:
-        log("autotuned block shapes:", rk.autotune(1))  # steps the domain, then restores its state
-    rk.step(args.warmup)
-    rk.sync()
-    # the dominant kernel: event time per step of each candidate over two untimed steps
+    doms = {}
+    for tag, prec in (("fp64", A.MI_FP64), ("fp32", A.MI_FP32_BULK)):
+        rk = RK3(ctx, g, case.params(precision=prec, **kw), case)
+        if not args.no_autotune:
+            log(f"{tag}: autotuned block shapes", rk.autotune(1))  # steps the domain, then restores its state
+        doms[tag] = rk
+    # the same steps on both: warm-up, then the timed region
+    secs = {}
+    for tag, rk in doms.items():
+        rk.step(args.warmup)
+        secs[tag] = timed_steps(rk, args.steps)
+        log(f"{tag}: {secs[tag] * 1e3 / args.steps:.3f} ms/step")
+    for tag, rk in doms.items():
+        bad = rk.check_finite()
+        if bad != -1:
+            raise SystemExit(f"{tag}: non-finite values in field {bad}")
+    err = error_report(doms["fp32"], doms["fp64"], kw.get("n_moist", 0))
+    pts = g.nx * g.ny * g.nz
+    v64 = pts * args.steps / secs["fp64"]
+    v32 = pts * args.steps / secs["fp32"]
+
+    # the dominant kernel of the FP32-bulk step: event time per step of each candidate over two steps
+    rk = doms["fp32"]
     per_kernel = {}
     for name in CANDIDATES:
         ms, n, _ = profile(rk, name, 2)
-        per_kernel[name] = ms / 2
+        if n:
+            per_kernel[name] = ms / 2
     dominant = max(per_kernel, key=per_kernel.get)
-    log("kernel ms/step:", json.dumps({k: round(v, 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1])}))
-
-    # timed region: K graph-replayed steps, bracketed by device synchronisations
-    rk.sync()
-    t0 = time.perf_counter()
-    rk.step(args.steps)
-    rk.sync()
-    t1 = time.perf_counter()
-    # the dominant kernel over K more steps, its launches bracketed by events on the domain's stream
+    log("fp32 kernel ms/step:", json.dumps({k: round(v, 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1])}))
     ev_ms, nl, b = profile(rk, dominant, args.steps)
-    bad = rk.check_finite()
-    if bad != -1:
-        raise SystemExit(f"non-finite values in field {bad}")
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = g.points * args.steps / (t1 - t0)
     avg_launch_s = ev_ms / 1e3 / nl
     achieved = b / nl / avg_launch_s / 1e9
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": v32,
         "unit": "grid-point-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": secs["fp32"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32-bulk (f64 islands)",
         "data": "synthetic",
-        "config": {"workload": "dry warm bubble 64x64x40, 1 km, 20 km top, plain FP64, bit-identical to the "
-                               "reference CPU FP64 build with scalar glibc pow",
-                   "nx": g.nx, "ny": g.ny, "nz": g.nz, "dx_m": g.dx, "dt_s": g.dt, "time_step_sound": rk.n_sound,
-                   "damp_opt": 3, "h_adv_order": 5, "v_adv_order": 3, "precision": "fp64"},
+        "config": dict(desc, precision="fp32-bulk"),
+        "fp64": {"value": v64, "ms_per_step": secs["fp64"] * 1e3 / args.steps, "dtype": "f64",
+                 "note": "plain FP64, bit-identical to the reference CPU FP64 build"},
+        "speedup_fp32_vs_fp64": v32 / v64,
+        "fp32_error_vs_fp64": {"after_steps": args.warmup + args.steps,
+                               "simulated_s": (args.warmup + args.steps) * g.dt, "per_variable": err},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": b / nl, "avg_launch_us": avg_launch_s * 1e6,
-                     "launches": nl},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": b / nl,
+                     "avg_launch_us": avg_launch_s * 1e6, "launches": nl, "precision": "fp32-bulk"},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, prm)
+        out["cpu_baseline"] = cpu_baseline(case, case.params(**kw))
     print(json.dumps(out), flush=True)
 
 

@@ -153,6 +153,9 @@ SIGNATURES = {
     "mi_small_steps": (_I, [_VP, _I]),
     "mi_scalar_transport": (_I, [_VP, _I]),
     "mi_calc_p_rho_phi": (_I, [_VP, _I]),
+    "mi_batch_create": (_I, [_VP, C.POINTER(_VP), _I, C.POINTER(_VP)]),
+    "mi_batch_destroy": (_I, [_VP]),
+    "mi_batch_step": (_I, [_VP, _I]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
     "mi_domain_autotune": (_I, [_VP, _I]),
     "mi_domain_get_tuning": (_I, [_VP, C.POINTER(_I), _I]),

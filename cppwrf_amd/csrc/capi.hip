@@ -4,6 +4,7 @@
 
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/cppwrf_amd.h"
 #include "domain_api.hpp"
@@ -33,6 +34,63 @@ int run_phase(mi_domain* d, MiPhase which, int rk) {
 }
 
 }  // namespace
+
+// A batch: one graph per combination of the members' scalar-buffer parities (the captured launches name
+// the buffers), each member's step a branch forked from and joined back to the context's stream.
+struct mi_batch {
+  mi_ctx* ctx = nullptr;
+  std::vector<mi_domain*> m;
+  std::vector<hipStream_t> st;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  struct Graph {
+    std::vector<int> key;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+  };
+  std::vector<Graph> graphs;
+
+  ~mi_batch() {
+    for (Graph& gr : graphs) {
+      if (gr.x) (void)hipGraphExecDestroy(gr.x);
+      if (gr.g) (void)hipGraphDestroy(gr.g);
+    }
+    for (hipEvent_t e : join) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (hipStream_t s : st) (void)hipStreamDestroy(s);
+  }
+  std::vector<int> key() const {
+    std::vector<int> k;
+    for (mi_domain* d : m) k.push_back(d->buffer_parity());
+    return k;
+  }
+  hipGraphExec_t exec_for(const std::vector<int>& k) {
+    for (Graph& gr : graphs)
+      if (gr.key == k) return gr.x;
+    hipStream_t s0 = ctx->stream;
+    hip_check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      hip_check(hipEventRecord(fork, s0), "hipEventRecord");
+      for (size_t q = 0; q < m.size(); ++q) {
+        hip_check(hipStreamWaitEvent(st[q], fork, 0), "hipStreamWaitEvent");
+        m[q]->capture_step(st[q]);
+        hip_check(hipEventRecord(join[q], st[q]), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(s0, join[q], 0), "hipStreamWaitEvent");
+      }
+    } catch (...) {
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(s0, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
+    }
+    Graph gr;
+    gr.key = k;
+    hip_check(hipStreamEndCapture(s0, &gr.g), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    graphs.push_back(gr);
+    return gr.x;
+  }
+};
 
 namespace mi_detail {
 void set_last_error(const std::string& m) { g_last_error = m; }
@@ -157,6 +215,61 @@ int mi_profile_kernel(mi_domain* d, const char* name) {
 
 int mi_profile_read(mi_domain* d, double* total_ms, int* launches, double* alg_bytes) {
   return mi_detail::guard([&] { checked(d)->profile_read(total_ms, launches, alg_bytes); });
+}
+
+int mi_batch_create(mi_ctx* ctx, mi_domain* const* members, int n, mi_batch** out) {
+  return mi_detail::guard([&] {
+    if (!ctx || !members || !out || n < 1) throw Error(MI_EINVAL, "bad argument");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto b = std::make_unique<mi_batch>();
+    b->ctx = ctx;
+    for (int q = 0; q < n; ++q) {
+      if (!members[q]) throw Error(MI_EINVAL, "null member");
+      if (members[q]->context()->device != ctx->device) throw Error(MI_EINVAL, "members must be on the batch's device");
+      for (int r = 0; r < q; ++r)
+        if (members[r] == members[q]) throw Error(MI_EINVAL, "a domain appears twice in a batch");
+      b->m.push_back(members[q]);
+      hipStream_t s = nullptr;
+      hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");  // default priority
+      b->st.push_back(s);
+      hipEvent_t e = nullptr;
+      hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      b->join.push_back(e);
+    }
+    hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+    *out = b.release();
+  });
+}
+
+int mi_batch_destroy(mi_batch* b) {
+  return mi_detail::guard([&] {
+    if (!b) return;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+  });
+}
+
+int mi_batch_step(mi_batch* b, int nsteps) {
+  return mi_detail::guard([&] {
+    if (!b || nsteps < 0) throw Error(MI_EINVAL, "bad argument");
+    hip_check(hipSetDevice(b->ctx->device), "hipSetDevice");
+    // the members' own streams may still hold work: the batch's stream waits for it
+    for (mi_domain* d : b->m)
+      if (d->context()->stream != b->ctx->stream) {
+        hip_check(hipEventRecord(b->fork, d->context()->stream), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(b->ctx->stream, b->fork, 0), "hipStreamWaitEvent");
+      }
+    for (int s = 0; s < nsteps; ++s) {
+      hipGraphExec_t x = b->exec_for(b->key());
+      hip_check(hipGraphLaunch(x, b->ctx->stream), "hipGraphLaunch");
+      for (mi_domain* d : b->m) d->advance_host();
+    }
+    // and each member's stream waits for the batch, so work queued on a member afterwards follows it
+    hip_check(hipEventRecord(b->fork, b->ctx->stream), "hipEventRecord");
+    for (mi_domain* d : b->m)
+      if (d->context()->stream != b->ctx->stream)
+        hip_check(hipStreamWaitEvent(d->context()->stream, b->fork, 0), "hipStreamWaitEvent");
+  });
 }
 
 int mi_check_finite(mi_domain* d, int* bad_field) {

@@ -64,6 +64,8 @@ struct Domain final : mi_domain {
   bool use_graph = true;
   std::string prof_name;
   KernelTimer timer{};
+  hipStream_t launch_stream = nullptr;  // while a batch captures this domain's step: its branch's stream
+  hipStream_t stream() const { return launch_stream ? launch_stream : ctx->stream; }
   int colcfg[kNumColKernels] = {0, 0, 0, 0};  // block shape of each column kernel
   std::vector<char*> backup;                            // the carried state, for the autotuner
 
@@ -149,15 +151,15 @@ struct Domain final : mi_domain {
       s.py[n] = (unsigned char)(r.py >= 0 ? r.py : (face_y(d->st) ? 1 : 0));
       if (s.nzs[n] > s.nkmax) s.nkmax = s.nzs[n];
     }
-    launch_halo(a, s, ctx->stream);
+    launch_halo(a, s, stream());
   }
 
   // ------------------------------------------------------------------ the phases of a step
   // RK3::step's _2 -> _1 copies (rk3.cpp:94-98).  The copies include the halos, which are fresh, so
   // the reference's refresh of mu_1's halo rewrites the values already there and is left out.
   void step_begin() {
-    launch_copy_step(a, n3, n2, ctx->stream);
-    launch_copy_scalars(a, sc, n_sc, n3, ctx->stream);  // q_1 = q_2 (rk3.cpp:99)
+    launch_copy_step(a, n3, n2, stream());
+    launch_copy_scalars(a, sc, n_sc, n3, stream());  // q_1 = q_2 (rk3.cpp:99)
   }
 
   // LargeStep::prep (large_step.cpp:697-733).  Its first halo refresh (u, v, w, t, ph, p, al, mu of the
@@ -167,15 +169,15 @@ struct Domain final : mi_domain {
   // kernel itself: each output point stores its halo images.  The same holds for every refresh of the
   // small steps below.
   void rk_prep() {
-    launch_large_prep(a, colcfg[kColLargePrep], ctx->stream);
-    if (a.q.moist) launch_cq(a, sc, ctx->stream);  // calc_cq (large_step.cpp:724-733)
+    launch_large_prep(a, colcfg[kColLargePrep], stream());
+    if (a.q.moist) launch_cq(a, sc, stream());  // calc_cq (large_step.cpp:724-733)
   }
 
   // LargeStep::tendencies (large_step.cpp:798-861); at the first stage, with a diffusion driver,
   // zero_tendf and the sixth-order diffusion into the held tendencies first
   void rk_tendency(int rk) {
-    if (rk == 1 && a.q.use_tendf) launch_sixth_dyn(a, ctx->stream);
-    launch_tendencies(a, ctx->stream);
+    if (rk == 1 && a.q.use_tendf) launch_sixth_dyn(a, stream());
+    launch_tendencies(a, stream());
   }
 
   // SmallStep::run_stage (small_step.cpp:490-607); with prhophi, calc_p_rho_phi (small_step.cpp:609-617)
@@ -186,7 +188,7 @@ struct Domain final : mi_domain {
     const double dt_rk = rk == 1 ? dt / 3.0 : (rk == 2 ? 0.5 * dt : dt);
     const double dts = rk == 1 ? dt_rk : dts_full;
     const int ns = rk == 1 ? 1 : (rk == 2 ? n_sound_ / 2 : n_sound_);
-    hipStream_t st = ctx->stream;
+    hipStream_t st = stream();
     if (rk == 1) launch_fill(f.mudf - off, n2, 0.0, st);
     launch_ss_prep(a, rk == 1, st);
     const double cof_h = 0.5 * dts * kG * (1.0 + prm.epssm);
@@ -217,15 +219,15 @@ struct Domain final : mi_domain {
       else plain[np++] = n;
     }
     // rk_scalar_tend at the first stage: the step's tendency of every scalar, sixth-order diffusion included
-    if (rk == 1 && a.q.use_tendf) launch_sixth_scalars(a, sc, n_sc, dt_rk, ctx->stream);
-    launch_scalars(a, sc, plain, np, false, dt_rk, ctx->stream);
-    launch_scalars(a, sc, pd, nd, true, dt_rk, ctx->stream);
+    if (rk == 1 && a.q.use_tendf) launch_sixth_scalars(a, sc, n_sc, dt_rk, stream());
+    launch_scalars(a, sc, plain, np, false, dt_rk, stream());
+    launch_scalars(a, sc, pd, nd, true, dt_rk, stream());
     for (int n = 0; n < n_sc; ++n) std::swap(sc.q2[n], sc.qn[n]);
     qpar ^= 1;
   }
 
   // SmallStep::calc_p_rho_phi (small_step.cpp:609-617), for the phase-by-phase entry points
-  void calc_p_rho_phi() { launch_p_rho_phi(a, ctx->stream); }  // with the halos of p, al
+  void calc_p_rho_phi() { launch_p_rho_phi(a, stream()); }  // with the halos of p, al
 
   void step_launches() {
     step_begin();
@@ -235,7 +237,7 @@ struct Domain final : mi_domain {
       small_steps(rk, true);
       scalar_transport(rk);
     }
-    if (a.q.terrain) launch_set_w_surface(a, ctx->stream);  // rk3.cpp:163
+    if (a.q.terrain) launch_set_w_surface(a, stream());  // rk3.cpp:163
   }
 
   // the fields a step reads from the previous one (everything else is recomputed within a step)
@@ -353,6 +355,31 @@ struct Domain final : mi_domain {
   void profile_kernel(const char* name) override;
   void profile_read(double* total_ms, int* launches, double* alg_bytes) override;
   int check_finite(int* bad_field) override;
+  // a batch: this domain's step launched on stream s (under capture), the host state left as it was
+  void capture_step(hipStream_t s) override {
+    const Scalars keep = sc;
+    const int kq = qpar;
+    launch_stream = s;
+    try {
+      step_launches();
+    } catch (...) {
+      launch_stream = nullptr;
+      sc = keep;
+      qpar = kq;
+      throw;
+    }
+    launch_stream = nullptr;
+    sc = keep;
+    qpar = kq;
+  }
+  // the host side of one step run elsewhere (a batch graph): three stage swaps of the scalar buffers
+  void advance_host() override {
+    for (int r = 0; r < (n_sc > 0 ? 3 : 0); ++r) {
+      for (int n = 0; n < n_sc; ++n) std::swap(sc.q2[n], sc.qn[n]);
+      qpar ^= 1;
+    }
+  }
+  int buffer_parity() const override { return qpar; }
 };
 
 namespace {

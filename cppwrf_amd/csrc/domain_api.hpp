@@ -33,6 +33,11 @@ struct mi_domain {
   virtual void profile_kernel(const char* name) = 0;
   virtual void profile_read(double* total_ms, int* launches, double* alg_bytes) = 0;
   virtual int check_finite(int* bad_field) = 0;  // -1 or the first field id with a non-finite value
+  // batches: launch one step onto stream s (the caller captures it), the host bookkeeping of a step that
+  // ran elsewhere, and the parity of the scalar buffers a captured step names
+  virtual void capture_step(hipStream_t s) = 0;
+  virtual void advance_host() = 0;
+  virtual int buffer_parity() const = 0;
 };
 
 namespace mi_fp64 {

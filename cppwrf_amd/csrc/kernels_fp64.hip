@@ -81,7 +81,8 @@ void time_end(hipStream_t st, double bytes) {
   t.bytes += bytes;
 }
 
-// algorithmic bytes of one point of a kernel: distinct 3-D arrays it reads and writes, 8 bytes each
+// algorithmic bytes of one point of a kernel: distinct 3-D arrays it reads and writes, each of the
+// storage type (8 bytes in FP64, 4 in FP32-bulk)
 // (pointwise-recomputable inputs and 1-D / 2-D arrays are not counted)
 double point_bytes(const char* name) {
   struct E { const char* n; int arrays; };
@@ -102,7 +103,7 @@ double point_bytes(const char* name) {
       {"p_rho_phi", 4},    // ph t -> al p
   };
   for (const E& e : table)
-    if (std::strcmp(e.n, name) == 0) return 8.0 * e.arrays;
+    if (std::strcmp(e.n, name) == 0) return double(sizeof(real)) * e.arrays;  // bulk arrays in the storage type
   return 0.0;
 }
 
@@ -113,11 +114,11 @@ void launch3a(const char* name, double arrays, K kern, const Range& r, hipStream
   const bool t = timing(name);
   if (t) time_begin(st);
   hipLaunchKernelGGL(kern, grid_of(r, by), dim3(kBx, by, 1), 0, st, r, args...);
-  if (t) time_end(st, 8.0 * arrays * double(r.i1 - r.i0) * double(r.j1 - r.j0) * double(r.k1 - r.k0));
+  if (t) time_end(st, double(sizeof(real)) * arrays * double(r.i1 - r.i0) * double(r.j1 - r.j0) * double(r.k1 - r.k0));
 }
 template <class K, class... T>
 void launch3(const char* name, K kern, const Range& r, hipStream_t st, T... args) {
-  launch3a(name, point_bytes(name) / 8.0, kern, r, st, args...);
+  launch3a(name, point_bytes(name) / double(sizeof(real)), kern, r, st, args...);
 }
 template <class K, class... T>
 void launch_cols(const char* name, K kern, int nx, int ny, int nz, hipStream_t st, T... args) {

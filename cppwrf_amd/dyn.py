@@ -155,6 +155,33 @@ class Context:
         self.close()
 
 
+class Batch:
+    """Domains of one context stepped together, one HIP graph per step with a branch per member (mi_batch)."""
+
+    def __init__(self, ctx: Context, members):
+        self.ctx = ctx
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.handle.value for m in self.members])
+        h = C.c_void_p()
+        check(lib().mi_batch_create(ctx.handle, arr, len(self.members), C.byref(h)))
+        self._h = h
+
+    def step(self, n: int = 1) -> None:
+        check(lib().mi_batch_step(self._h, n))
+
+    def sync(self) -> None:
+        for m in self.members:
+            m.sync()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 class RK3:
     """reference dyn::RK3 together with the State / BaseState it steps (src/dyn/rk3.hpp:34-78)."""
 

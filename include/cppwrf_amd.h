@@ -189,6 +189,16 @@ int mi_small_steps(mi_domain* d, int rk);
 int mi_scalar_transport(mi_domain* d, int rk);
 int mi_calc_p_rho_phi(mi_domain* d, int rk);
 
+/* Batches (ensembles, serving): the steps of several domains of one context advance together.  Each
+ * step of the batch is one HIP graph holding every member's step as an independent branch, so the
+ * members' kernels overlap on the device (reference: the tiles of main.cpp:30-42 stepped one thread per
+ * patch; here one graph launch for all).  Members keep their own state and may differ in size and
+ * options; a member stepped on its own in between is picked up (the batch re-captures). */
+typedef struct mi_batch mi_batch;
+int mi_batch_create(mi_ctx* ctx, mi_domain* const* members, int n, mi_batch** out);
+int mi_batch_destroy(mi_batch* b);
+int mi_batch_step(mi_batch* b, int nsteps);
+
 /* MI_ENONFINITE if any point of u, v, w, t, ph, mu is NaN or infinite (the reference's runner check,
  * src/cases/domain_runner.hpp:70-71); *bad_field receives the first such field id (or -1). */
 int mi_check_finite(mi_domain* d, int* bad_field);

@@ -1,0 +1,51 @@
+"""Batches: domains of different sizes and options stepped together in one graph per step (mi_batch) give,
+member by member, the bits each gives stepped alone; and a member stepped alone in between is picked up."""
+import numpy as np
+import pytest
+
+from cppwrf_amd import _abi as A
+from cppwrf_amd.dyn import Batch, Case, Context, RK3
+
+pytestmark = pytest.mark.gpu
+
+STATE = (A.MI_U, A.MI_V, A.MI_W, A.MI_T, A.MI_PH, A.MI_MU, A.MI_P, A.MI_AL)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return Context(0)
+
+
+def members(ctx):
+    out = []
+    specs = [
+        (Case.bubble3d(24, 20, 16, dx=1000.0, ztop=8000.0, dt=6.0, seed=1, bubble_zc=3000.0), dict(time_step_sound=6)),
+        (Case.bubble3d(40, 28, 20, dx=2000.0, ztop=10000.0, dt=10.0, seed=2, bubble_zc=3000.0),
+         dict(time_step_sound=6, damp_opt=3, zdamp=4000.0)),
+        (Case.hills(30, 24, 16, dx=15000.0, dt=90.0, seed=3), dict(time_step_sound=4, n_scalars=2)),
+        (Case.bubble3d(24, 20, 16, dx=1000.0, ztop=8000.0, dt=6.0, seed=4, bubble_zc=3000.0),
+         dict(time_step_sound=6, precision=A.MI_FP32_BULK, n_moist=1, n_scalars=1)),
+    ]
+    for case, kw in specs:
+        out.append((case, case.params(**kw)))
+    return out
+
+
+def test_batch_equals_alone(ctx):
+    print("[batch] 4 members, alone vs batched", flush=True)
+    ms = members(ctx)
+    alone = [RK3(ctx, c.grid, p, c) for c, p in ms]
+    mixed = [RK3(ctx, c.grid, p, c) for c, p in ms]
+    for d in alone:
+        d.step(7)
+    b = Batch(ctx, mixed)
+    b.step(3)
+    mixed[1].step(1)  # one member on its own in between: its scalar-buffer parity no longer matches
+    b.step(3)
+    for q in (0, 2, 3):  # every member at 7 steps
+        mixed[q].step(1)
+    for q, (da, dm) in enumerate(zip(alone, mixed)):
+        fids = STATE + ((A.MI_Q,) if ms[q][1].n_scalars else ())
+        for fid in fids:
+            a, m = da.field(fid), dm.field(fid)
+            assert np.array_equal(a.view(np.uint64), m.view(np.uint64)), (q, A.FIELD_NAMES[fid])
