@@ -209,8 +209,6 @@ def test_moist_graph_steps_bitwise(ctx):
         gpu.step(chunk)
         orc.step(chunk)
         _compare(gpu, orc, STATE + tuple(A.MI_Q + n for n in range(3)), f"moist after a chunk of {chunk} steps")
-    q = gpu.field(A.MI_Q + 1)
-    assert q.min() >= -1e-12 * 1e-3  # the PD species stays non-negative
 
 
 def _mapped_case():
