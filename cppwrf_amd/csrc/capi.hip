@@ -96,6 +96,15 @@ namespace mi_detail {
 void set_last_error(const std::string& m) { g_last_error = m; }
 }  // namespace mi_detail
 
+void mi_ctx_release(mi_ctx* c) {
+  if (c->refs.fetch_sub(1) != 1) return;
+  if (c->own_stream) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+}
+
 extern "C" {
 
 const char* mi_last_error(void) { return g_last_error.c_str(); }
@@ -129,12 +138,7 @@ int mi_ctx_create(int device, void* stream, mi_ctx** out) {
 
 int mi_ctx_destroy(mi_ctx* c) {
   return mi_detail::guard([&] {
-    if (!c) return;
-    if (c->own_stream) {
-      (void)hipStreamSynchronize(c->stream);
-      (void)hipStreamDestroy(c->stream);
-    }
-    delete c;
+    if (c) mi_ctx_release(c);
   });
 }
 
@@ -145,14 +149,17 @@ int mi_domain_create(mi_ctx* ctx, const mi_grid* g, const mi_params* p, const mi
     if (p->precision == MI_FP64) *out = mi_fp64::make_domain(ctx, *g, *p, *lv);
     else if (p->precision == MI_FP32_BULK) *out = mi_fp32::make_domain(ctx, *g, *p, *lv);
     else throw Error(MI_EINVAL, "precision must be MI_FP64 or MI_FP32_BULK");
+    ctx->refs.fetch_add(1);
   });
 }
 
 int mi_domain_destroy(mi_domain* d) {
   return mi_detail::guard([&] {
     if (!d) return;
-    (void)hipStreamSynchronize(d->context()->stream);
+    mi_ctx* c = d->context();
+    (void)hipStreamSynchronize(c->stream);
     delete d;
+    mi_ctx_release(c);
   });
 }
 
@@ -237,6 +244,7 @@ int mi_batch_create(mi_ctx* ctx, mi_domain* const* members, int n, mi_batch** ou
       b->join.push_back(e);
     }
     hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+    ctx->refs.fetch_add(1);
     *out = b.release();
   });
 }
@@ -244,8 +252,10 @@ int mi_batch_create(mi_ctx* ctx, mi_domain* const* members, int n, mi_batch** ou
 int mi_batch_destroy(mi_batch* b) {
   return mi_detail::guard([&] {
     if (!b) return;
-    (void)hipStreamSynchronize(b->ctx->stream);
+    mi_ctx* c = b->ctx;
+    (void)hipStreamSynchronize(c->stream);
     delete b;
+    mi_ctx_release(c);
   });
 }
 

@@ -5,15 +5,20 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 
 #include "../../include/cppwrf_amd.h"
 
+// A context lives until it is destroyed and no domain or batch made on it remains (each holds a
+// reference), so the order in which a caller releases objects cannot free a stream still in use.
 struct mi_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  std::atomic<int> refs{1};  // the caller's, plus one per live domain or batch
 };
+void mi_ctx_release(mi_ctx* c);  // drop one reference; the last one frees the context
 
 // phases of RK3::step for the phase-by-phase entry points
 enum MiPhase { kPhaseStepBegin, kPhaseRkPrep, kPhaseRkTendency, kPhaseSmallSteps, kPhaseScalars, kPhaseCalcPRhoPhi };

@@ -1163,7 +1163,7 @@ MI_COLUMN_KERNEL k_finish(Args a, int prhophi, double hd_dt) {
         put<kHMass>(f.w2, g, i, j, k * g.sk, (f.msfty[x2] * f.w2[x] + f.w_save[x] * (c.c1f[k] * m + c.c2f[k])) / (c.c1f[k] * muts + c.c2f[k]));
         const double phk = f.ph2[x] + f.ph_save[x];
         put<kHMass>(f.ph2, g, i, j, k * g.sk, phk);
-        LDS(PH, k, ci) = phk;
+        LDS(PH, k, ci) = double(real(phk));  // the value as stored (the same as calc_p_rho_phi reads alone)
         put<kHMass>(f.ww, g, i, j, k * g.sk, f.ww[x] + f.ww1[x]);
         if (k < nz) put<kHMass>(f.t2, g, i, j, k * g.sk, (f.t2[x] - (hd_dt != 0.0 ? hd_dt * (c.c1h[k] * m + c.c2h[k]) * f.hdia[x] : 0.0) + f.t_save[x] * (c.c1h[k] * m + c.c2h[k])) / (c.c1h[k] * muts + c.c2h[k]));
       }
