@@ -298,7 +298,7 @@ double us_std_p(double z) {
   return p;
 }
 
-// A real-data-shaped domain: seeded Gaussian hills (summed over periodic images), the US standard
+// A real-data-shaped domain: seeded Gaussian hills (summed over periodic images, capped smoothly), the US standard
 // atmosphere, a hybrid coordinate (WRF hybrid_opt 2 with eta_c), seeded u and v jets with +-1 m/s
 // noise, map factors 1 +- map_amp, Coriolis from 30 N to 50 N, optional qv from a relative humidity of
 // 80 % at the ground falling to 10 % at 10 km.  The base state is dry and hydrostatic column by column
@@ -373,7 +373,10 @@ void build_hills(mi_case& c, const mi_case_spec& sp) {
             const double dx = (x - hl.x - a * Lx) / hl.w, dy = two_d ? 0.0 : (y - hl.y - b * Ly) / hl.w;
             h += hl.h * std::exp(-(dx * dx + dy * dy));
           }
-      c.ht[size_t(j) * nx + i] = h;
+      // overlapping hills (and their periodic images on small domains) are capped smoothly at 1.2 times the
+      // tallest single hill, as real terrain stays within a few km
+      const double cap = 1.2 * hmax;
+      c.ht[size_t(j) * nx + i] = cap > 0 ? cap * std::tanh(h / cap) : 0.0;
     }
   // map factors at each staggering, Coriolis
   const int nxu = nx + 1, nyv = two_d ? 1 : ny + 1;
