@@ -1200,7 +1200,7 @@ __global__ void k_p_rho_phi(Range r, Args a) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double al = -1.0 / (c.c1h[k] * f.muts[x2] + c.c2h[k]) * (f.alb(i, j, k) * (c.c1h[k] * f.mu2[x2]) + c.rdnw[k] * (f.ph2[x + g.sk] - f.ph2[x]));
+  const double al = -1.0 / (c.c1h[k] * f.muts[x2] + c.c2h[k]) * (f.alb(i, j, k) * (c.c1h[k] * f.mu2[x2]) + c.rdnw[k] * (double(f.ph2[x + g.sk]) - double(f.ph2[x])));
   put<kHMass>(f.al, g, i, j, k * g.sk, al);
   const double base = kRd * (kT0 + f.t2[x]) / (kP1000 * (al + f.alb(i, j, k)));
   put<kHMass>(f.p, g, i, j, k * g.sk, kP1000 * mi_libm::pow(base, kCpovcv) - f.pb(i, j, k));
