@@ -156,6 +156,7 @@ SIGNATURES = {
     "mi_batch_create": (_I, [_VP, C.POINTER(_VP), _I, C.POINTER(_VP)]),
     "mi_batch_destroy": (_I, [_VP]),
     "mi_batch_step": (_I, [_VP, _I]),
+    "mi_cfl_max": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
     "mi_domain_autotune": (_I, [_VP, _I]),
     "mi_domain_get_tuning": (_I, [_VP, C.POINTER(_I), _I]),

@@ -282,6 +282,10 @@ int mi_batch_step(mi_batch* b, int nsteps) {
   });
 }
 
+int mi_cfl_max(mi_domain* d, double* vert, double* horiz) {
+  return mi_detail::guard([&] { checked(d)->cfl_max(vert, horiz); });
+}
+
 int mi_check_finite(mi_domain* d, int* bad_field) {
   int bad = -1;
   const int rc = mi_detail::guard([&] { bad = checked(d)->check_finite(bad_field); });

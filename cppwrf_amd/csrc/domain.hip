@@ -355,6 +355,7 @@ struct Domain final : mi_domain {
   void profile_kernel(const char* name) override;
   void profile_read(double* total_ms, int* launches, double* alg_bytes) override;
   int check_finite(int* bad_field) override;
+  void cfl_max(double* vert, double* horiz) override;
   // a batch: this domain's step launched on stream s (under capture), the host state left as it was
   void capture_step(hipStream_t s) override {
     const Scalars keep = sc;
@@ -707,6 +708,20 @@ int Domain::check_finite(int* bad_field) {
     if (h[q]) bad = q;
   if (bad_field) *bad_field = bad;
   return bad;
+}
+
+void Domain::cfl_max(double* vert, double* horiz) {
+  hipStream_t st = ctx->stream;
+  unsigned long long* out = reinterpret_cast<unsigned long long*>(flags + 32);  // two 8-byte slots of the flag buffer
+  hip_check(hipMemsetAsync(out, 0, 2 * sizeof(unsigned long long), st), "hipMemsetAsync");
+  if (grid.nz >= 2) launch_cfl(a, out, st);
+  unsigned long long h[2];
+  hip_check(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  double v[2];
+  std::memcpy(v, h, sizeof(v));
+  if (vert) *vert = v[0];
+  if (horiz) *horiz = v[1];
 }
 
 mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv) {

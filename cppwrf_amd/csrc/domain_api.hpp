@@ -38,6 +38,7 @@ struct mi_domain {
   virtual void profile_kernel(const char* name) = 0;
   virtual void profile_read(double* total_ms, int* launches, double* alg_bytes) = 0;
   virtual int check_finite(int* bad_field) = 0;  // -1 or the first field id with a non-finite value
+  virtual void cfl_max(double* vert, double* horiz) = 0;
   // batches: launch one step onto stream s (the caller captures it), the host bookkeeping of a step that
   // ran elsewhere, and the parity of the scalar buffers a captured step names
   virtual void capture_step(hipStream_t s) = 0;

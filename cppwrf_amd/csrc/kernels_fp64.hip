@@ -1538,6 +1538,26 @@ __global__ void k_copy_scalars(Scalars s, int64_t n3, int64_t off) {
   (s.q1[n] - off)[e] = (s.q2[n] - off)[e];
 }
 
+// LargeStep::cfl_max's pointwise Courant numbers (FVertCfl, FHorizCfl; large_step.cpp:672-687) over levels
+// 1..nz-1, reduced with an atomic max of the bit patterns: for non-negative doubles the integer order is the
+// numeric order, so the maximum is exact and independent of the order of the points; NaN counts as 0
+__global__ void k_cfl(Range r, Args a, unsigned long long* out) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const double dt = a.q.dt;
+  double v = f.ww[x] / (c.c1f[k] * f.mut[x2] + c.c2f[k]) * c.rdnw[k] * dt;
+  v = v < 0.0 ? -v : v;
+  double ha = f.u2[x] * c.rdx * f.msfux[x2] * dt, hb = f.v2[x] * c.rdy * f.msfvy[x2] * dt;
+  ha = ha < 0.0 ? -ha : ha;
+  hb = hb < 0.0 ? -hb : hb;
+  const double h = hb > ha ? hb : ha;
+  if (v > 0.0) atomicMax(&out[0], static_cast<unsigned long long>(__double_as_longlong(v)));
+  if (h > 0.0) atomicMax(&out[1], static_cast<unsigned long long>(__double_as_longlong(h)));
+}
+
 // finiteness of the prognostic fields (flags[fld] = 1 when a NaN or an infinity is found)
 __global__ void k_finite(Range r, Args a, int* flags) {
   MI_POINT_INDEX(r)
@@ -1693,6 +1713,10 @@ void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, 
 void launch_set_w_surface(const Args& a, hipStream_t st) {
   const Geo& g = a.g;
   launch3a("set_w_surface", 7, k_set_w_surface, Range{0, g.nx, 0, g.ny, 0, 1}, st, a);  // u, v (3 levels each) -> w (1)
+}
+void launch_cfl(const Args& a, unsigned long long* out, hipStream_t st) {
+  const Geo& g = a.g;
+  launch3a("cfl", 4, k_cfl, Range{0, g.nx, 0, g.ny, 1, g.nz}, st, a, out);  // ww u v (mut 2-D) read
 }
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st) {
   const Geo& g = a.g;

@@ -130,6 +130,8 @@ void launch_sixth_dyn(const Args& a, hipStream_t st);
 void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, hipStream_t st);
 // set_w_surface (fill = false) after a step, with terrain
 void launch_set_w_surface(const Args& a, hipStream_t st);
+// w_damp's Courant-number maxima into out[0] (vertical), out[1] (horizontal) as non-negative bit patterns
+void launch_cfl(const Args& a, unsigned long long* out, hipStream_t st);
 // calc_cq (KCq) of the stage state
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st);
 // the transport of species sp[0..n): rk_update_scalar (pd = 0) or rk_update_scalar_pd (pd = 1)

@@ -251,6 +251,12 @@ class RK3:
         a = (C.c_int * 4)(*shapes)
         check(lib().mi_domain_set_tuning(self._h, a, 4))
 
+    def cfl_max(self) -> tuple[float, float]:
+        """w_damp's vertical and horizontal Courant-number maxima of the current fields (LargeStep::cfl_max)"""
+        v, h = C.c_double(), C.c_double()
+        check(lib().mi_cfl_max(self._h, C.byref(v), C.byref(h)))
+        return v.value, h.value
+
     def check_finite(self) -> int:
         bad = C.c_int(-1)
         rc = lib().mi_check_finite(self._h, C.byref(bad))

@@ -189,6 +189,12 @@ int mi_small_steps(mi_domain* d, int rk);
 int mi_scalar_transport(mi_domain* d, int rk);
 int mi_calc_p_rho_phi(mi_domain* d, int rk);
 
+/* The domain maxima of w_damp's vertical and horizontal Courant numbers from the current fields
+ * (reference LargeStep::cfl_max, src/dyn/large_step.cpp:862-876: |ww / (c1f mut + c2f) rdnw dt| and
+ * max(|u rdx msfux dt|, |v rdy msfvy dt|) over levels 1..nz-1; a NaN never wins).  Called between
+ * mi_rk_tendency and mi_small_steps it gives the reference's on_cfl values of that stage. */
+int mi_cfl_max(mi_domain* d, double* vert, double* horiz);
+
 /* Batches (ensembles, serving): the steps of several domains of one context advance together.  Each
  * step of the batch is one HIP graph holding every member's step as an independent branch, so the
  * members' kernels overlap on the device (reference: the tiles of main.cpp:30-42 stepped one thread per

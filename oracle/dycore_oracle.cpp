@@ -1121,6 +1121,26 @@ struct Dom {
     fill(al, g);
   }
 
+  // LargeStep::cfl_max (large_step.cpp:862-876, FVertCfl / FHorizCfl :672-687)
+  void cfl_max(double& vert, double& horiz) {
+    double v = 0, h = 0;
+    const double dt = g.dt;
+    for (int k = 1; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+          double x = ww(i, j, k) / (c.c1f[k] * mut(i, j) + c.c2f[k]) * c.rdnw[k] * dt;
+          x = x < 0.0 ? -x : x;
+          if (x > v) v = x;  // a NaN never takes the maximum
+          double a = u2(i, j, k) * c.rdx * msfux(i, j) * dt, b = v2(i, j, k) * c.rdy * msfvy(i, j) * dt;
+          a = a < 0.0 ? -a : a;
+          b = b < 0.0 ? -b : b;
+          const double m = b > a ? b : a;
+          if (m > h) h = m;
+        }
+    vert = v;
+    horiz = h;
+  }
+
   // RK3::step's copies (rk3.cpp:94-98)
   void step_begin() {
     u1.a = u2.a; v1.a = v2.a; w1.a = w2.a; t1.a = t2.a; ph1.a = ph2.a; mu1.a = mu2.a;
@@ -1295,6 +1315,10 @@ int orc_small_steps(orc_domain* d, int rk) {
 int orc_scalar_transport(orc_domain* d, int rk) {
   if (rk < 1 || rk > 3) return fail("rk must be 1, 2 or 3");
   d->d->scalars(rk);
+  return MI_OK;
+}
+int orc_cfl_max(orc_domain* d, double* vert, double* horiz) {
+  d->d->cfl_max(*vert, *horiz);
   return MI_OK;
 }
 int orc_calc_p_rho_phi(orc_domain* d, int) {

@@ -37,6 +37,7 @@ int orc_rk_prep(orc_domain* d, int rk);
 int orc_rk_tendency(orc_domain* d, int rk);
 int orc_small_steps(orc_domain* d, int rk);
 int orc_scalar_transport(orc_domain* d, int rk); /* ScalarTransport::run_stage */
+int orc_cfl_max(orc_domain* d, double* vert, double* horiz); /* LargeStep::cfl_max */
 int orc_calc_p_rho_phi(orc_domain* d, int rk);
 
 #ifdef __cplusplus

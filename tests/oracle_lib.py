@@ -39,6 +39,7 @@ def olib():
             "orc_rk_tendency": (I, [VP, I]),
             "orc_small_steps": (I, [VP, I]),
             "orc_scalar_transport": (I, [VP, I]),
+            "orc_cfl_max": (I, [VP, PD, PD]),
             "orc_calc_p_rho_phi": (I, [VP, I]),
         }
         for n, (r, a) in sig.items():
@@ -117,6 +118,11 @@ class OracleDomain:
 
     def small_steps(self, rk):
         _ok(olib().orc_small_steps(self._h, rk))
+
+    def cfl_max(self):
+        v, h = C.c_double(), C.c_double()
+        _ok(olib().orc_cfl_max(self._h, C.byref(v), C.byref(h)))
+        return v.value, h.value
 
     def scalar_transport(self, rk):
         _ok(olib().orc_scalar_transport(self._h, rk))

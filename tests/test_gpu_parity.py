@@ -303,3 +303,21 @@ def test_hills_bitwise(ctx):
         orc2.step(chunk)
         _compare(gpu2, orc2, STATE + QS, f"hills after a chunk of {chunk} steps")
     assert np.abs(gpu2.field(A.MI_W)[0]).max() > 0  # the surface w follows the terrain
+
+
+def test_cfl_max_equals_oracle(ctx):
+    """mi_cfl_max between the tendencies and the acoustic steps (the reference's on_cfl point) equals the
+    oracle's maxima exactly, on the hills domain."""
+    case = Case.hills(30, 24, 20, dx=15000.0, dt=90.0, seed=6)
+    prm = case.params(time_step_sound=4)
+    gpu = RK3(ctx, case.grid, prm, case)
+    orc = OracleDomain(case, prm)
+    gpu.step(2)
+    orc.step(2)
+    gpu.step_begin()
+    orc.step_begin()
+    for d in (gpu, orc):
+        d.rk_prep(1)
+        d.rk_tendency(1)
+    g, o = gpu.cfl_max(), orc.cfl_max()
+    assert g == o and g[1] > 0 and g[0] > 0, (g, o)

@@ -67,3 +67,14 @@ def test_inertial_oscillation():
     u, v = o.field(A.MI_U), o.field(A.MI_V)
     assert np.abs(v[:, 1:-1, :] + 5.0 * np.sin(f * t)).max() < 2e-4 * 5.0 * f * t
     assert np.abs(u - 5.0 * np.cos(f * t)).max() < 1e-5
+
+
+def test_cfl_max_uniform_flow():
+    """LargeStep::cfl_max on a uniform 5 m/s flow at rest otherwise: horizontal 5 dt/dx, vertical ~0."""
+    case = Case.rest_state(nx=12, ny=10, nz=10, u0=5.0)
+    g = case.grid
+    o = OracleDomain(case, case.params(time_step_sound=4))
+    o.step(2)
+    vert, horiz = o.cfl_max()
+    assert abs(horiz - 5.0 * g.dt / g.dx) < 1e-12
+    assert vert < 1e-12
