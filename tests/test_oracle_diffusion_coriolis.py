@@ -76,5 +76,5 @@ def test_cfl_max_uniform_flow():
     o = OracleDomain(case, case.params(time_step_sound=4))
     o.step(2)
     vert, horiz = o.cfl_max()
-    assert abs(horiz - 5.0 * g.dt / g.dx) < 1e-12
+    assert abs(horiz - 5.0 * g.dt / g.dx) < 1e-9 * 5.0 * g.dt / g.dx  # u stays 5 to roundoff
     assert vert < 1e-12
