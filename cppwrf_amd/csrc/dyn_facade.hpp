@@ -15,6 +15,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/cppwrf_amd.h"
@@ -51,26 +52,53 @@ class Context {
   mi_ctx* h_ = nullptr;
 };
 
+// reference ScalarParams (src/dyn/scalars.hpp:14-19); the advection orders are the dynamics' own
+struct ScalarParams {
+  int moist_adv_opt = 1, scalar_adv_opt = 1;
+};
+// What the reference's BaseState and drivers carry beside the parameter structs: the number of moist
+// species and tracers in State::q_2 (state.hpp:53-60), BaseState::mapped / coriolis (state.hpp:93-107),
+// a terrain-following base state, the sixth-order diffusion of DiffusionDriver (diffusion/sixth_order.hpp)
+// and the storage precision of the bulk 3-D fields.
+struct DomainOptions {
+  int n_moist = 0, n_scalars = 0;
+  bool mapped = false, coriolis = false, terrain = false;
+  int diff_6th_opt = 0;
+  double diff_6th_factor = 0.12;
+  int precision = MI_FP64;
+};
+
+// number of values of a field id in the dense layout of the C ABI
+inline size_t field_size(int field, const mi_grid& g) {
+  const size_t nx = size_t(g.nx), ny = size_t(g.ny), nz = size_t(g.nz), vy = g.ny > 1 ? 1 : 0;
+  switch (field) {
+    case MI_U: case MI_RU_M: case MI_RU_TEND: case MI_CQU: return (nx + 1) * ny * nz;
+    case MI_V: case MI_RV_M: case MI_RV_TEND: case MI_CQV: return nx * (ny + vy) * nz;
+    case MI_W: case MI_PH: case MI_WW: case MI_WW_M: case MI_RW_TEND: case MI_PH_TEND: case MI_CQW: case MI_PHB:
+      return nx * ny * (nz + 1);
+    case MI_MU: case MI_MUTS: case MI_MUB: case MI_MSFTX: case MI_MSFTY: case MI_F: case MI_E: case MI_SINA:
+    case MI_COSA: case MI_HT:
+      return nx * ny;
+    case MI_MSFUX: case MI_MSFUY: return (nx + 1) * ny;
+    case MI_MSFVX: case MI_MSFVY: case MI_MSFVX_INV: return nx * (ny + vy);
+    default: return nx * ny * nz;  // mass points: t, p, al, alt, php, tendencies, h_diabatic, scalars
+  }
+}
+
 // The device-resident State + BaseState of the domain (reference src/dyn/state.hpp:30-119)
 class StateView {
  public:
   explicit StateView(mi_domain* d, const mi_grid& g) : d_(d), g_(g) {}
-  size_t size(int field) const {
-    const bool u = field == MI_U || field == MI_RU_M || field == MI_RU_TEND;
-    const bool v = field == MI_V || field == MI_RV_M || field == MI_RV_TEND;
-    const bool w = field == MI_W || field == MI_PH || field == MI_WW || field == MI_WW_M || field == MI_RW_TEND ||
-                   field == MI_PH_TEND || field == MI_PHB;
-    const bool two = field == MI_MU || field == MI_MUTS || field == MI_MUB;
-    const size_t nxs = size_t(g_.nx + (u ? 1 : 0)), nys = size_t(g_.ny + (v && g_.ny > 1 ? 1 : 0));
-    const size_t nzs = two ? 1 : size_t(g_.nz + (w ? 1 : 0));
-    return nxs * nys * nzs;
-  }
+  size_t size(int field) const { return field_size(field, g_); }
   void set(int field, const std::vector<double>& h) { check(mi_domain_set_field(d_, field, h.data(), int64_t(h.size()))); }
   std::vector<double> get(int field) const {
     std::vector<double> h(size(field));
     check(mi_domain_get_field(d_, field, h.data(), int64_t(h.size())));
     return h;
   }
+  // reference State::q_2 of species n (n_moist moist species first, then the tracers)
+  void set_q(int n, const std::vector<double>& h) { set(MI_Q + n, h); }
+  std::vector<double> q(int n) const { return get(MI_Q + n); }
 
  private:
   mi_domain* d_;
@@ -81,7 +109,7 @@ class StateView {
 class RK3 {
  public:
   RK3(const Context& ctx, const mi_grid& g, const LargeStepParams& lp, const AcousticParams& ap, int time_step_sound,
-      const mi_levels& levels, int precision = MI_FP64)
+      const mi_levels& levels, const ScalarParams& sp = {}, const DomainOptions& opt = {})
       : g_(g) {
     if (lp.top_lid != ap.top_lid) throw std::runtime_error("top_lid must agree between LargeStepParams and AcousticParams");
     mi_params p{};
@@ -98,7 +126,16 @@ class RK3 {
     p.zdamp = ap.zdamp;
     p.dampcoef = ap.dampcoef;
     p.time_step_sound = time_step_sound;
-    p.precision = precision;
+    p.precision = opt.precision;
+    p.n_moist = opt.n_moist;
+    p.n_scalars = opt.n_scalars;
+    p.moist_adv_opt = sp.moist_adv_opt;
+    p.scalar_adv_opt = sp.scalar_adv_opt;
+    p.mapped = opt.mapped ? 1 : 0;
+    p.coriolis = opt.coriolis ? 1 : 0;
+    p.diff_6th_opt = opt.diff_6th_opt;
+    p.terrain = opt.terrain ? 1 : 0;
+    p.diff_6th_factor = opt.diff_6th_factor;
     check(mi_domain_create(ctx.get(), &g, &p, &levels, &d_));
   }
   ~RK3() { mi_domain_destroy(d_); }
@@ -113,10 +150,41 @@ class RK3 {
   void sync() { check(mi_domain_sync(d_)); }
   StateView state() { return StateView(d_, g_); }
   mi_domain* handle() const { return d_; }
+  // w_damp's vertical and horizontal Courant maxima of the current fields (the values the reference
+  // passes to RK3::on_cfl, rk3.hpp:46-47)
+  std::pair<double, double> cfl_max() {
+    double v = 0, h = 0;
+    check(mi_cfl_max(d_, &v, &h));
+    return {v, h};
+  }
+  // the reference runner's finiteness check (src/cases/domain_runner.hpp:70-71); -1 when all are finite
+  int first_nonfinite() {
+    int bad = -1;
+    const int rc = mi_check_finite(d_, &bad);
+    if (rc != MI_ENONFINITE) check(rc);
+    return bad;
+  }
 
  private:
   mi_grid g_;
   mi_domain* d_ = nullptr;
+};
+
+// Several domains stepped together, one HIP graph per step (the reference's tiles of main.cpp:30-42)
+class Batch {
+ public:
+  Batch(const Context& ctx, const std::vector<RK3*>& members) {
+    std::vector<mi_domain*> h;
+    for (RK3* m : members) h.push_back(m->handle());
+    check(mi_batch_create(ctx.get(), h.data(), int(h.size()), &b_));
+  }
+  ~Batch() { mi_batch_destroy(b_); }
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+  void step(int nsteps = 1) { check(mi_batch_step(b_, nsteps)); }
+
+ private:
+  mi_batch* b_ = nullptr;
 };
 
 }  // namespace cppwrf_amd::dyn

@@ -13,6 +13,14 @@ int main() {
   if (dyn::RK3::auto_sound_steps(6.0, 1000.0, 1000.0) != 4) return 2;
   if (dyn::RK3::auto_sound_steps(90.0, 15000.0, 15000.0) != 4) return 3;
   if (dyn::RK3::auto_sound_steps(60.0, 3000.0, 3000.0) != 12) return 4;
+  // dense ABI sizes of the staggered fields (x-y-z and an x-z slice, whose v has no extra row)
+  const mi_grid g3{16, 12, 10, MI_BC_PERIODIC, MI_BC_PERIODIC, 1000.0, 1000.0, 6.0};
+  const mi_grid g2{16, 1, 10, MI_BC_PERIODIC, MI_BC_PERIODIC, 1000.0, 1000.0, 6.0};
+  if (dyn::field_size(MI_U, g3) != 17 * 12 * 10 || dyn::field_size(MI_V, g3) != 16 * 13 * 10) return 7;
+  if (dyn::field_size(MI_PH, g3) != 16 * 12 * 11 || dyn::field_size(MI_MU, g3) != 16 * 12) return 8;
+  if (dyn::field_size(MI_MSFUX, g3) != 17 * 12 || dyn::field_size(MI_MSFVY, g3) != 16 * 13) return 9;
+  if (dyn::field_size(MI_Q + 3, g3) != 16 * 12 * 10 || dyn::field_size(MI_CQW, g3) != 16 * 12 * 11) return 10;
+  if (dyn::field_size(MI_V, g2) != 16 * 10 || dyn::field_size(MI_MSFVX, g2) != 16) return 11;
   try {
     dyn::Context ctx(0);
     // a GPU is present: a domain with an unsupported advection order must be refused
