@@ -35,6 +35,21 @@ ABI_VERSION = 2
 
 MI_CASE_REST_STATE, MI_CASE_WARM_BUBBLE, MI_CASE_IGW, MI_CASE_BUBBLE3D, MI_CASE_HILLS = range(5)
 
+# specified lateral boundaries
+MI_BDY_U, MI_BDY_V, MI_BDY_T, MI_BDY_PH, MI_BDY_MU, MI_BDY_Q = 0, 1, 2, 3, 4, 16
+MI_BDY_XS, MI_BDY_XE, MI_BDY_YS, MI_BDY_YE = range(4)
+MI_SPEC_BDY_WIDTH = 5
+
+
+def bdy_shape(var: int, side: int, nx: int, ny: int, nz: int) -> tuple[int, int, int]:
+    """(b, k, pos) shape of one side's boundary slab of a variable (pos fastest in memory)."""
+    if side in (MI_BDY_XS, MI_BDY_XE):
+        npos = ny + (1 if var == MI_BDY_V else 0)
+    else:
+        npos = nx + (1 if var == MI_BDY_U else 0)
+    nk = {MI_BDY_PH: nz + 1, MI_BDY_MU: 1}.get(var, nz)
+    return (MI_SPEC_BDY_WIDTH, nk, npos)
+
 # staggering of each field id: (x extra, y extra, z extra, rank)
 STAGGER = {
     MI_U: "U", MI_V: "V", MI_W: "W", MI_T: "M", MI_PH: "W", MI_MU: "M2", MI_P: "M", MI_AL: "M",
@@ -153,6 +168,10 @@ SIGNATURES = {
     "mi_small_steps": (_I, [_VP, _I]),
     "mi_scalar_transport": (_I, [_VP, _I]),
     "mi_calc_p_rho_phi": (_I, [_VP, _I]),
+    "mi_step_end": (_I, [_VP]),
+    "mi_bdy_set": (_I, [_VP, _I, _I, _PD, _PD, _I64]),
+    "mi_bdy_set_time": (_I, [_VP, C.c_double]),
+    "mi_bdy_get_time": (_I, [_VP, C.POINTER(C.c_double)]),
     "mi_batch_create": (_I, [_VP, C.POINTER(_VP), _I, C.POINTER(_VP)]),
     "mi_batch_destroy": (_I, [_VP]),
     "mi_batch_step": (_I, [_VP, _I]),

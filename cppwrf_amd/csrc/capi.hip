@@ -28,7 +28,8 @@ mi_domain* checked(mi_domain* d) {
 
 int run_phase(mi_domain* d, MiPhase which, int rk) {
   return mi_detail::guard([&] {
-    if (which != kPhaseStepBegin && (rk < 1 || rk > 3)) throw Error(MI_EINVAL, "rk must be 1, 2 or 3");
+    if (which != kPhaseStepBegin && which != kPhaseStepEnd && (rk < 1 || rk > 3))
+      throw Error(MI_EINVAL, "rk must be 1, 2 or 3");
     checked(d)->phase(which, rk);
   });
 }
@@ -197,6 +198,23 @@ int mi_rk_tendency(mi_domain* d, int rk) { return run_phase(d, kPhaseRkTendency,
 int mi_small_steps(mi_domain* d, int rk) { return run_phase(d, kPhaseSmallSteps, rk); }
 int mi_scalar_transport(mi_domain* d, int rk) { return run_phase(d, kPhaseScalars, rk); }
 int mi_calc_p_rho_phi(mi_domain* d, int rk) { return run_phase(d, kPhaseCalcPRhoPhi, rk); }
+int mi_step_end(mi_domain* d) { return run_phase(d, kPhaseStepEnd, 0); }
+
+int mi_bdy_set(mi_domain* d, int var, int side, const double* values, const double* tends, int64_t n) {
+  return mi_detail::guard([&] {
+    if (!values || !tends) throw Error(MI_EINVAL, "null boundary data");
+    checked(d)->bdy_set(var, side, values, tends, n);
+  });
+}
+int mi_bdy_set_time(mi_domain* d, double dtbc) {
+  return mi_detail::guard([&] { checked(d)->bdy_set_time(dtbc); });
+}
+int mi_bdy_get_time(mi_domain* d, double* dtbc) {
+  return mi_detail::guard([&] {
+    if (!dtbc) throw Error(MI_EINVAL, "null argument");
+    *dtbc = checked(d)->bdy_get_time();
+  });
+}
 
 int mi_domain_autotune(mi_domain* d, int trials) {
   return mi_detail::guard([&] { checked(d)->autotune(trials); });

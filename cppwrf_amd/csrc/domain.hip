@@ -67,6 +67,12 @@ struct Domain final : mi_domain {
   hipStream_t launch_stream = nullptr;  // while a batch captures this domain's step: its branch's stream
   hipStream_t stream() const { return launch_stream ? launch_stream : ctx->stream; }
   int colcfg[kNumColKernels] = {0, 0, 0, 0};  // block shape of each column kernel
+  // specified boundaries: the data buffer of each variable (U V T PH MU, then the scalars) and which
+  // scalars have data (the others take flow_dep_bdy)
+  bool spec = false;
+  double* bdy_buf[5 + MI_MAX_SCALARS] = {};
+  int64_t bdy_len[5 + MI_MAX_SCALARS] = {};
+  bool q_has_bdy[MI_MAX_SCALARS] = {};
   std::vector<char*> backup;                            // the carried state, for the autotuner
 
   // launches of fn, with the kernel timer active when a kernel is selected
@@ -92,7 +98,9 @@ struct Domain final : mi_domain {
     drop_graphs();
     for (void* p : allocs) (void)hipFree(p);
   }
+  int launch_version = 0;  // bumped whenever the step's launches change (shapes, boundary data present)
   void drop_graphs() {
+    ++launch_version;
     for (int q = 0; q < 2; ++q) {
       if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
       if (graph[q]) (void)hipGraphDestroy(graph[q]);
@@ -157,9 +165,26 @@ struct Domain final : mi_domain {
   // ------------------------------------------------------------------ the phases of a step
   // RK3::step's _2 -> _1 copies (rk3.cpp:94-98).  The copies include the halos, which are fresh, so
   // the reference's refresh of mu_1's halo rewrites the values already there and is left out.
+  //
+  // With specified boundaries the step's clock advances first (rk3.cpp:93), and mu_1's halo is refreshed:
+  // mu_2's is not fresh after spec_bdy_final (large_step.cpp:700-706), so the copy brings a stale one.
   void step_begin() {
+    if (spec) launch_bdy_time(a, stream());
     launch_copy_step(a, n3, n2, stream());
     launch_copy_scalars(a, sc, n_sc, n3, stream());  // q_1 = q_2 (rk3.cpp:99)
+    if (spec) halo({{a.f.mu1, -1, -1}});
+  }
+  // the boundary data of the dry variables (all) or of the scalars with data
+  BdySet bdy_dry() const {
+    BdySet s{};
+    for (int v = 0; v < 5; ++v) s.v[s.n++] = BdyVar{v, 0, bdy_buf[v]};
+    return s;
+  }
+  BdySet bdy_scalars() const {
+    BdySet s{};
+    for (int n = 0; n < n_sc; ++n)
+      if (q_has_bdy[n]) s.v[s.n++] = BdyVar{kBdyQ, n, bdy_buf[5 + n]};
+    return s;
   }
 
   // LargeStep::prep (large_step.cpp:697-733).  Its first halo refresh (u, v, w, t, ph, p, al, mu of the
@@ -168,16 +193,27 @@ struct Domain final : mi_domain {
   // its outputs (mut, ru, rv, rw, alt, php, muu, muv, ww; large_step.cpp:718-720) is written by the
   // kernel itself: each output point stores its halo images.  The same holds for every refresh of the
   // small steps below.
-  void rk_prep() {
+  //
+  // With specified boundaries the first stage does refresh u, v, w, t, ph, p, al (not mu): spec_bdy_final
+  // changed the spec zone after the last refresh.
+  void rk_prep(int rk) {
+    if (spec && rk == 1) halo({{a.f.u2, -1, -1}, {a.f.v2, -1, -1}, {a.f.w2, -1, -1}, {a.f.t2, -1, -1},
+                               {a.f.ph2, -1, -1}, {a.f.p, -1, -1}, {a.f.al, -1, -1}});
     launch_large_prep(a, colcfg[kColLargePrep], stream());
     if (a.q.moist) launch_cq(a, sc, stream());  // calc_cq (large_step.cpp:724-733)
   }
 
   // LargeStep::tendencies (large_step.cpp:798-861); at the first stage, with a diffusion driver,
   // zero_tendf and the sixth-order diffusion into the held tendencies first
+  //
+  // With specified boundaries: mu_tend zeroed every stage (large_step.cpp:815-818), the relaxation of rows
+  // 1..3 into the held tendencies at the first stage (:825) and the spec zone's tendencies last (:859).
   void rk_tendency(int rk) {
+    if (spec) launch_fill(a.f.mu_tend - off, n2, 0.0, stream());
     if (rk == 1 && a.q.use_tendf) launch_sixth_dyn(a, stream());
+    if (rk == 1 && spec) launch_bdy(a, sc, bdy_dry(), kBdyRelax, stream());
     launch_tendencies(a, stream());
+    if (spec) launch_bdy(a, sc, bdy_dry(), kBdySpec, stream());
   }
 
   // SmallStep::run_stage (small_step.cpp:490-607); with prhophi, calc_p_rho_phi (small_step.cpp:609-617)
@@ -201,6 +237,7 @@ struct Domain final : mi_domain {
       // the loop; the sums' halo images are rewritten at every substep, the last one's stay)
       launch_acoustic(a, colcfg[kColAcoustic], dts, it, ns, st);
     }
+    if (spec) launch_bdy_zero_grad(a, st);  // zero_grad_bdy of the spec zone's w (lateral_bc.cpp:374-387)
     // with the halos of u, v, w, t, ph, ww, mu (and p, al); the microphysics heating leaves theta_m at
     // the last stage with moisture (small_step.cpp:602)
     const double hd_dt = (rk == 3 && a.q.moist) ? dts * double(ns) : 0.0;
@@ -218,26 +255,50 @@ struct Domain final : mi_domain {
       if (is_pd) pd[nd++] = n;
       else plain[np++] = n;
     }
-    // rk_scalar_tend at the first stage: the step's tendency of every scalar, sixth-order diffusion included
+    // rk_scalar_tend at the first stage: the step's tendency of every scalar, sixth-order diffusion and the
+    // boundary relaxation included (scalars.cpp:446-455); the spec zone's tendency every stage
     if (rk == 1 && a.q.use_tendf) launch_sixth_scalars(a, sc, n_sc, dt_rk, stream());
+    if (rk == 1 && spec) launch_bdy(a, sc, bdy_scalars(), kBdyRelax, stream());
+    if (spec) launch_bdy(a, sc, bdy_scalars(), kBdySpec, stream());
     launch_scalars(a, sc, plain, np, false, dt_rk, stream());
     launch_scalars(a, sc, pd, nd, true, dt_rk, stream());
     for (int n = 0; n < n_sc; ++n) std::swap(sc.q2[n], sc.qn[n]);
     qpar ^= 1;
+    if (spec) {  // flow_dep_bdy of the species without data (scalars.cpp:420-429)
+      int fd[MI_MAX_SCALARS], nf = 0;
+      for (int n = 0; n < n_sc; ++n)
+        if (!q_has_bdy[n]) fd[nf++] = n;
+      launch_bdy_flow_dep(a, sc, fd, nf, stream());
+    }
   }
 
   // SmallStep::calc_p_rho_phi (small_step.cpp:609-617), for the phase-by-phase entry points
   void calc_p_rho_phi() { launch_p_rho_phi(a, stream()); }  // with the halos of p, al
 
+  // the end of RK3::step (rk3.cpp:159-167): spec_bdy_final, then the surface w of terrain, which refreshes
+  // the u, v halos first (set_w_surface, rk3.cpp:194; those refreshes change nothing without a
+  // LateralBC).  The reference also reruns calc_p_rho_phi on the interior after spec_bdy_final: its
+  // inputs there are unchanged and the scalar pow gives back the same bits, so it is left out.
+  void step_end() {
+    if (spec) {
+      launch_bdy(a, sc, bdy_dry(), kBdyFinal, stream());
+      launch_bdy(a, sc, bdy_scalars(), kBdyFinal, stream());
+    }
+    if (a.q.terrain) {
+      if (spec) halo({{a.f.u2, -1, -1}, {a.f.v2, -1, -1}});
+      launch_set_w_surface(a, stream());
+    }
+  }
+
   void step_launches() {
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
-      rk_prep();
+      rk_prep(rk);
       rk_tendency(rk);
       small_steps(rk, true);
       scalar_transport(rk);
     }
-    if (a.q.terrain) launch_set_w_surface(a, stream());  // rk3.cpp:163
+    step_end();
   }
 
   // the fields a step reads from the previous one (everything else is recomputed within a step)
@@ -278,6 +339,7 @@ struct Domain final : mi_domain {
   void autotune(int trials) override {
     if (trials < 1) trials = 1;
     const std::string keep = prof_name;
+    const double dtbc = spec ? bdy_get_time() : 0.0;
     save_or_restore(true);
     for (int s = 0; s < 1; ++s) timed([&] { step_launches(); });  // warm-up
     for (int kern = 0; kern < kNumColKernels; ++kern) {
@@ -309,6 +371,7 @@ struct Domain final : mi_domain {
     timer.bytes = 0;
     save_or_restore(false);
     hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (spec) bdy_set_time(dtbc);
     drop_graphs();  // the graphs hold the old shapes
   }
 
@@ -356,6 +419,19 @@ struct Domain final : mi_domain {
   void profile_read(double* total_ms, int* launches, double* alg_bytes) override;
   int check_finite(int* bad_field) override;
   void cfl_max(double* vert, double* horiz) override;
+  void bdy_set(int var, int side, const double* values, const double* tends, int64_t n) override;
+  void bdy_set_time(double dtbc) override {
+    if (!spec) throw Error(MI_EINVAL, "the domain has no specified boundaries");
+    hip_check(hipMemcpyAsync(a.f.bdy_time, &dtbc, sizeof(double), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  }
+  double bdy_get_time() override {
+    if (!spec) throw Error(MI_EINVAL, "the domain has no specified boundaries");
+    double t = 0;
+    hip_check(hipMemcpyAsync(&t, a.f.bdy_time, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return t;
+  }
   // a batch: this domain's step launched on stream s (under capture), the host state left as it was
   void capture_step(hipStream_t s) override {
     const Scalars keep = sc;
@@ -380,15 +456,21 @@ struct Domain final : mi_domain {
       qpar ^= 1;
     }
   }
-  int buffer_parity() const override { return qpar; }
+  // the key of a batch's captured step: the scalar-buffer parity and the version of the launch schedule
+  int buffer_parity() const override { return qpar + 2 * launch_version; }
 };
 
 namespace {
 
 void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv) {
   if (g.nx < 1 || g.ny < 1 || g.nz < 3) throw Error(MI_EINVAL, "grid: need nx >= 1, ny >= 1, nz >= 3");
-  if (g.bc_x != MI_BC_PERIODIC && g.bc_x != MI_BC_SYMMETRIC) throw Error(MI_EINVAL, "bc_x: periodic or symmetric");
-  if (g.bc_y != MI_BC_PERIODIC && g.bc_y != MI_BC_SYMMETRIC) throw Error(MI_EINVAL, "bc_y: periodic or symmetric");
+  if (g.bc_x < MI_BC_PERIODIC || g.bc_x > MI_BC_SPECIFIED) throw Error(MI_EINVAL, "bc_x: periodic, symmetric or specified");
+  if (g.bc_y < MI_BC_PERIODIC || g.bc_y > MI_BC_SPECIFIED) throw Error(MI_EINVAL, "bc_y: periodic, symmetric or specified");
+  if ((g.bc_x == MI_BC_SPECIFIED) != (g.bc_y == MI_BC_SPECIFIED))
+    throw Error(MI_EINVAL, "specified boundaries on both x and y or on neither");
+  // the relaxation zone of 5 rows on each side, and the 3-row margin of the sixth-order diffusion
+  if (g.bc_x == MI_BC_SPECIFIED && (g.ny == 1 || g.nx < 10 || g.ny < 10))
+    throw Error(MI_EINVAL, "specified boundaries need a 3-D grid of at least 10 x 10 columns");
   if (g.bc_x == MI_BC_PERIODIC && g.nx < 3) throw Error(MI_EINVAL, "periodic x needs nx >= 3");
   if (g.ny > 1 && g.bc_y == MI_BC_PERIODIC && g.ny < 3) throw Error(MI_EINVAL, "periodic y needs ny >= 3");
   // a symmetric fill with fewer than 4 points mirrors halo slots into one another (order-dependent)
@@ -475,8 +557,23 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   // Coriolis and grid rotation: f = e = sina = 0, cosa = 1 unless set
   add2(f.fcor, St::M2, MI_F); add2(f.ecor, St::M2, MI_E); add2(f.sina, St::M2, MI_SINA); add2(f.cosa, St::M2, MI_COSA);
   launch_fill(f.cosa - d.off, d.n2, 1.0, ctx->stream);
-  if (p.diff_6th_opt != 0) {  // the held tendencies of a diffusion driver
+  d.spec = g.bc_x == MI_BC_SPECIFIED;
+  const bool held = p.diff_6th_opt != 0 || d.spec;  // a diffusion driver or a LateralBC holds tendencies
+  if (held) {
     add3(f.ru_tendf, St::U, -1); add3(f.rv_tendf, St::V, -1); add3(f.rw_tendf, St::W, -1); add3(f.t_tendf, St::M, -1);
+  }
+  if (d.spec) {
+    add3(f.ph_tendf, St::W, -1);
+    f.bdy_time = d.alloc(1);
+    // each variable's eight slabs (lateral_bc.cpp:219-233): values and tendencies of xs, xe, ys, ye
+    const int64_t nz = g.nz, w = MI_SPEC_BDY_WIDTH;
+    for (int v = 0; v < 5 + p.n_scalars; ++v) {
+      const int kind = v < 5 ? v : 5;
+      const int64_t nk = kind == kBdyPH ? nz + 1 : (kind == kBdyMU ? 1 : nz);
+      const int64_t sx = (g.ny + (kind == kBdyV ? 1 : 0)) * nk * w, sy = (g.nx + (kind == kBdyU ? 1 : 0)) * nk * w;
+      d.bdy_len[v] = 2 * (2 * sx + 2 * sy);
+      d.bdy_buf[v] = d.alloc(d.bdy_len[v]);
+    }
   }
   // moisture and scalars
   if (p.n_moist > 0) {
@@ -491,7 +588,7 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     add3(d.sc.qn[n], St::M, -1);
     add3(d.sc.qtf[n], St::M, MI_Q_TENDF + n);
     add3(d.sc.scale[n], St::M, -1);
-    if (p.diff_6th_opt != 0) add3(d.sc.qtw[n], St::M, -1);
+    if (held) add3(d.sc.qtw[n], St::M, -1);
   }
   // base 3-D fields: profiles (flat terrain, horizontally uniform base state) or full arrays (terrain)
   const int nz = g.nz;
@@ -552,7 +649,14 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   q.moist = p.n_moist > 0 ? 1 : 0;
   q.mapped = p.mapped;
   q.cor = (p.mapped != 0 || p.coriolis != 0) ? 1 : 0;
-  q.use_tendf = p.diff_6th_opt != 0 ? 1 : 0;
+  q.use_tendf = held ? 1 : 0;
+  q.spec = d.spec ? 1 : 0;
+  if (d.spec)  // lbc_fcx_gcx (lateral_bc.cpp:198-211), Fortran loop = b + 1 over spec_zone + 1 .. spec_zone + relax_zone
+    for (int loop = 2; loop <= 5; ++loop) {
+      const double num = double(1 + 4 - loop), den = double(4 - 1);
+      q.fcx[loop - 1] = 0.1 / g.dt * num / den;
+      q.gcx[loop - 1] = 1.0 / g.dt / 50.0 * num / den;
+    }
   q.diff_6th_opt = p.diff_6th_opt;
   q.diff_6th_factor = p.diff_6th_factor;
   q.dt = g.dt;
@@ -655,11 +759,12 @@ void Domain::get_field(int id, double* host, int64_t n) {
 void Domain::phase(MiPhase which, int rk) {
   switch (which) {
     case kPhaseStepBegin: step_begin(); break;
-    case kPhaseRkPrep: rk_prep(); break;
+    case kPhaseRkPrep: rk_prep(rk); break;
     case kPhaseRkTendency: rk_tendency(rk); break;
     case kPhaseSmallSteps: small_steps(rk, false); break;
     case kPhaseScalars: scalar_transport(rk); break;
     case kPhaseCalcPRhoPhi: calc_p_rho_phi(); break;
+    case kPhaseStepEnd: step_end(); break;
   }
   hip_check(hipGetLastError(), "kernel launch");
 }
@@ -722,6 +827,31 @@ void Domain::cfl_max(double* vert, double* horiz) {
   std::memcpy(v, h, sizeof(v));
   if (vert) *vert = v[0];
   if (horiz) *horiz = v[1];
+}
+
+void Domain::bdy_set(int var, int side, const double* values, const double* tends, int64_t n) {
+  if (!spec) throw Error(MI_EINVAL, "the domain has no specified boundaries");
+  if (side < MI_BDY_XS || side > MI_BDY_YE) throw Error(MI_EINVAL, "boundary side must be MI_BDY_XS .. MI_BDY_YE");
+  int v = -1;
+  if (var >= MI_BDY_U && var <= MI_BDY_MU) v = var;
+  else if (var >= MI_BDY_Q && var < MI_BDY_Q + n_sc) v = 5 + (var - MI_BDY_Q);
+  if (v < 0) throw Error(MI_EINVAL, "boundary variable must be MI_BDY_U .. MI_BDY_MU or MI_BDY_Q + a scalar");
+  const int kind = v < 5 ? v : 5;
+  const int64_t nz = grid.nz, w = MI_SPEC_BDY_WIDTH;
+  const int64_t nk = kind == kBdyPH ? nz + 1 : (kind == kBdyMU ? 1 : nz);
+  const int64_t sx = (grid.ny + (kind == kBdyV ? 1 : 0)) * nk * w, sy = (grid.nx + (kind == kBdyU ? 1 : 0)) * nk * w;
+  const int64_t len = side < 2 ? sx : sy;
+  if (n != len) throw Error(MI_EINVAL, "boundary slab size mismatch: expected " + std::to_string(len));
+  const int64_t o = side < 2 ? side * sx : 2 * sx + (side - 2) * sy, half = 2 * sx + 2 * sy;
+  hipStream_t st = ctx->stream;
+  hip_check(hipMemcpyAsync(bdy_buf[v] + o, values, size_t(n) * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(bdy_buf[v] + half + o, tends, size_t(n) * sizeof(double), hipMemcpyHostToDevice, st),
+            "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (v >= 5 && !q_has_bdy[v - 5]) {
+    q_has_bdy[v - 5] = true;  // the step's launches change (relax and spec instead of flow_dep)
+    drop_graphs();
+  }
 }
 
 mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv) {

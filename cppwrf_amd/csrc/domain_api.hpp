@@ -21,7 +21,9 @@ struct mi_ctx {
 void mi_ctx_release(mi_ctx* c);  // drop one reference; the last one frees the context
 
 // phases of RK3::step for the phase-by-phase entry points
-enum MiPhase { kPhaseStepBegin, kPhaseRkPrep, kPhaseRkTendency, kPhaseSmallSteps, kPhaseScalars, kPhaseCalcPRhoPhi };
+enum MiPhase {
+  kPhaseStepBegin, kPhaseRkPrep, kPhaseRkTendency, kPhaseSmallSteps, kPhaseScalars, kPhaseCalcPRhoPhi, kPhaseStepEnd
+};
 
 struct mi_domain {
   virtual ~mi_domain() = default;
@@ -39,6 +41,10 @@ struct mi_domain {
   virtual void profile_read(double* total_ms, int* launches, double* alg_bytes) = 0;
   virtual int check_finite(int* bad_field) = 0;  // -1 or the first field id with a non-finite value
   virtual void cfl_max(double* vert, double* horiz) = 0;
+  // specified lateral boundaries (LateralBC::set / set_scalar, dtbc)
+  virtual void bdy_set(int var, int side, const double* values, const double* tends, int64_t n) = 0;
+  virtual void bdy_set_time(double dtbc) = 0;
+  virtual double bdy_get_time() = 0;
   // batches: launch one step onto stream s (the caller captures it), the host bookkeeping of a step that
   // ran elsewhere, and the parity of the scalar buffers a captured step names
   virtual void capture_step(hipStream_t s) = 0;

@@ -131,10 +131,18 @@ void launch_cols(const char* name, K kern, int nx, int ny, int nz, hipStream_t s
 // ----------------------------------------------------------------------------- halo fill
 // One halo slot of one direction (reference semantics src/core/halo.cpp:52-84): slot a = 0..7 maps to
 // i = 0, -1, -2, -3, n, n+1, n+2, n+3 on face-staggered points and -1, -2, -3, n, n+1, n+2 otherwise.
+// bc = MI_BC_SPECIFIED: the outermost row copied outward (halo.cpp:67-69, WRF set_physical_bc*).
 template <class T>
-__device__ __forceinline__ void halo_slot(T* base, idx_t sn, int a, int n, bool face, bool periodic, int par) {
+__device__ __forceinline__ void halo_slot(T* base, idx_t sn, int a, int n, bool face, int bc, int par) {
   const int nl = face ? 4 : 3;
   const int i = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
+  if (bc == MI_BC_SPECIFIED) {
+    const int last = face ? n : n - 1;
+    const int src = i < 0 ? 0 : (i > last ? last : i);
+    if (src != i) base[idx_t(i) * sn] = base[idx_t(src) * sn];
+    return;
+  }
+  const bool periodic = bc == MI_BC_PERIODIC;
   int src;
   if (periodic) src = ((i % n) + n) % n;
   else if (face) src = i <= 0 ? -i : 2 * n - i;
@@ -157,9 +165,8 @@ __global__ void k_halo_x(Geo g, HaloSet s) {
   if (f >= s.n || k >= s.nzs[f] || j >= s.nys[f]) return;
   const bool face = s.face_x[f] != 0;
   if (a >= (face ? 8 : 6)) return;
-  const bool per = g.bcx == MI_BC_PERIODIC;
-  if (s.f32[f]) halo_slot(static_cast<float*>(s.p[f]) + g.at(0, j, k), 1, a, g.nx, face, per, s.px[f]);
-  else halo_slot(static_cast<double*>(s.p[f]) + g.at(0, j, k), 1, a, g.nx, face, per, s.px[f]);
+  if (s.f32[f]) halo_slot(static_cast<float*>(s.p[f]) + g.at(0, j, k), 1, a, g.nx, face, g.bcx, s.px[f]);
+  else halo_slot(static_cast<double*>(s.p[f]) + g.at(0, j, k), 1, a, g.nx, face, g.bcx, s.px[f]);
 }
 // y pass: thread (x index over the whole row incl. halos, slot, level, field)
 __global__ void k_halo_y(Geo g, HaloSet s) {
@@ -170,9 +177,8 @@ __global__ void k_halo_y(Geo g, HaloSet s) {
   if (f >= s.n || k >= s.nzs[f] || i >= s.nxs[f] + kHalo) return;
   const bool face = s.face_y[f] != 0;
   if (a >= (face ? 8 : 6)) return;
-  const bool per = g.bcy == MI_BC_PERIODIC;
-  if (s.f32[f]) halo_slot(static_cast<float*>(s.p[f]) + g.at(i, 0, k), g.sj, a, g.ny, face, per, s.py[f]);
-  else halo_slot(static_cast<double*>(s.p[f]) + g.at(i, 0, k), g.sj, a, g.ny, face, per, s.py[f]);
+  if (s.f32[f]) halo_slot(static_cast<float*>(s.p[f]) + g.at(i, 0, k), g.sj, a, g.ny, face, g.bcy, s.py[f]);
+  else halo_slot(static_cast<double*>(s.p[f]) + g.at(i, 0, k), g.sj, a, g.ny, face, g.bcy, s.py[f]);
 }
 
 // ----------------------------------------------------------------------------- halo images
@@ -187,21 +193,24 @@ enum HaloKind : int { kHMass = 0, kHU = 1, kHV = 2, kHMuu = 3, kHMuv = 4 };
 
 // emit(index, negate) for the point itself and every slot of one direction that copies index i
 template <class F>
-__device__ __forceinline__ void halo_images(int i, int n, bool face, bool periodic, int par, F&& emit) {
+__device__ __forceinline__ void halo_images(int i, int n, bool face, int bc, int par, F&& emit) {
+  const bool periodic = bc == MI_BC_PERIODIC, flat = bc == MI_BC_SPECIFIED;
+  const bool mirror = !periodic && !flat;  // symmetric: odd fields change sign
   // the point itself, unless it is a periodic face slot (index n takes index 0's value); a symmetric
   // wall face of an OddFace field changes sign in place
-  if (!(periodic && face && i == n)) emit(i, !periodic && face && par == 2 && (i == 0 || i == n));
+  if (!(periodic && face && i == n)) emit(i, mirror && face && par == 2 && (i == 0 || i == n));
   if (i >= 4 && i + 4 < n) return;
-  const int nl = face ? 4 : 3, ns = face ? 8 : 6;
+  const int nl = face ? 4 : 3, ns = face ? 8 : 6, last = face ? n : n - 1;
 #pragma unroll 1
   for (int a = 0; a < ns; ++a) {
     const int sl = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
     if (sl == i) continue;
     int src;
-    if (periodic) src = ((sl % n) + n) % n;
+    if (flat) src = sl < 0 ? 0 : (sl > last ? last : sl);
+    else if (periodic) src = ((sl % n) + n) % n;
     else if (face) src = sl <= 0 ? -sl : 2 * n - sl;
     else src = sl < 0 ? -1 - sl : 2 * n - 1 - sl;
-    if (src == i) emit(sl, !periodic && par != 0);
+    if (src == i) emit(sl, mirror && par != 0);
   }
 }
 
@@ -210,14 +219,13 @@ template <int KIND, class T>
 __device__ __forceinline__ void put(T* p, const Geo& g, int i, int j, idx_t kofs, double v) {
   constexpr bool fx = KIND == kHU || KIND == kHMuu, fy = KIND == kHV || KIND == kHMuv;
   constexpr int px = KIND == kHU ? 1 : (KIND == kHMuu ? 2 : 0), py = fy ? 1 : 0;
-  const bool perx = g.bcx == MI_BC_PERIODIC, pery = g.bcy == MI_BC_PERIODIC;
   T* q = p + kofs;
-  halo_images(i, g.nx, fx, perx, px, [&](int ii, bool nx_) {
+  halo_images(i, g.nx, fx, g.bcx, px, [&](int ii, bool nx_) {
     if (g.two_d) {
       q[ii] = T(nx_ ? -v : v);
       return;
     }
-    halo_images(j, g.ny, fy, pery, py, [&](int jj, bool ny_) { q[ii + jj * g.sj] = T((nx_ != ny_) ? -v : v); });
+    halo_images(j, g.ny, fy, g.bcy, py, [&](int jj, bool ny_) { q[ii + jj * g.sj] = T((nx_ != ny_) ? -v : v); });
   });
 }
 
@@ -305,8 +313,19 @@ MI_COLUMN_KERNEL k_large_prep(Args a) {
     auto mu = [&](int ii, int jj) { return f.mu2[g.at2(ii, jj)]; };
     auto mub = [&](int ii, int jj) { return f.mub[g.at2(ii, jj)]; };
     const double mut = mass ? mu(i, j) + mub(i, j) : 0.0;
-    const double muu = j < ny ? H * (mu(i, j) + mu(i - 1, j) + mub(i, j) + mub(i - 1, j)) : 0.0;
-    const double muv = (i < nx && j < g.nyv) ? H * (mu(i, j) + mu(i, j - 1) + mub(i, j) + mub(i, j - 1)) : 0.0;
+    // a specified edge face takes its column's mass twice (large_step.cpp:55-63)
+    const bool sp = g.bcx == MI_BC_SPECIFIED;
+    double muu = 0.0, muv = 0.0;
+    if (j < ny) {
+      if (sp && i == 0) muu = H * (mu(0, j) + mu(0, j) + mub(0, j) + mub(0, j));
+      else if (sp && i == nx) muu = H * (mu(nx - 1, j) + mu(nx - 1, j) + mub(nx - 1, j) + mub(nx - 1, j));
+      else muu = H * (mu(i, j) + mu(i - 1, j) + mub(i, j) + mub(i - 1, j));
+    }
+    if (i < nx && j < g.nyv) {
+      if (sp && j == 0) muv = H * (mu(i, 0) + mu(i, 0) + mub(i, 0) + mub(i, 0));
+      else if (sp && j == ny) muv = H * (mu(i, ny - 1) + mu(i, ny - 1) + mub(i, ny - 1) + mub(i, ny - 1));
+      else muv = H * (mu(i, j) + mu(i, j - 1) + mub(i, j) + mub(i, j - 1));
+    }
     if (cp.kt == 0) {
       if (mass) put<kHMass>(f.mut, g, i, j, 0, mut);
       if (j < ny) put<kHMuu>(f.muu, g, i, j, 0, muu);
@@ -378,6 +397,22 @@ __device__ __forceinline__ double vxgm(const Args& a, idx_t y, idx_t y2) {
          H * (f.v2[y] + f.v2[y + sj]) * (f.msfuy[y2 + 1] - f.msfuy[y2]) * a.c.rdx;
 }
 
+// kern::ord_face / ord_mass (kernels_common.hpp:45-58): the horizontal advection order next to a
+// specified edge for a flux between mass points f-1 and f (ord_face) or at mass point f-1 of a field
+// staggered in that direction (ord_mass); 5 stands for the domain's own order
+__device__ __forceinline__ int ord_face(int f, int n, bool on) {
+  if (!on) return 5;
+  if (f <= 1 || f >= n - 1) return 2;
+  if (f == 2 || f == n - 2) return 3;
+  return 5;
+}
+__device__ __forceinline__ int ord_mass(int f, int n, bool on) {
+  if (!on) return 5;
+  if (f <= 1 || f >= n) return 2;
+  if (f == 2 || f == n - 1) return 3;
+  return 5;
+}
+
 __global__ void k_tend_u(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -388,12 +423,26 @@ __global__ void k_tend_u(Range r, Args a) {
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const auto* u = f.u2;
   const double hup = a.q.hup, vup = a.q.vup;
-  auto fx = [&](idx_t y) {  // face between y-1 and y
+  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  auto fx = [&](idx_t y, int fi) {  // the flux at mass point fi - 1, between the u points y-1 and y
+    const int o = ord_mass(fi, g.nx, sp);
+    if (o == 2) {  // the upstream normal velocity at the boundary face (large_step.cpp:160-167)
+      if (fi <= 1) {
+        const double ub = u[y] < 0.0 ? double(u[y]) : double(u[y - 1]);
+        return Q * (f.ru[y] + f.ru[y - 1]) * (u[y] + ub);
+      }
+      const double ub = u[y - 1] > 0.0 ? double(u[y - 1]) : double(u[y]);
+      return Q * (f.ru[y] + f.ru[y - 1]) * (u[y - 1] + ub);
+    }
     const double vel = H * (f.ru[y] + f.ru[y - 1]);
+    if (o == 3) return vel * flux3(u[y - 2], u[y - 1], u[y], u[y + 1], vel, hup);
     return vel * flux5(u[y - 3], u[y - 2], u[y - 1], u[y], u[y + 1], u[y + 2], vel, hup);
   };
-  auto fy = [&](idx_t y) {
+  auto fy = [&](idx_t y, int fj) {
+    const int o = ord_face(fj, g.ny, sp);
+    if (o == 2) return Q * (f.rv[y] + f.rv[y - 1]) * (u[y] + u[y - sj]);
     const double vel = H * (f.rv[y] + f.rv[y - 1]);
+    if (o == 3) return vel * flux3(u[y - 2 * sj], u[y - sj], u[y], u[y + sj], vel, hup);
     return vel * flux5(u[y - 3 * sj], u[y - 2 * sj], u[y - sj], u[y], u[y + sj], u[y + 2 * sj], vel, hup);
   };
   auto fz = [&](int kk) {
@@ -410,8 +459,8 @@ __global__ void k_tend_u(Range r, Args a) {
     return H * (c.fnm[kk] * pu(kk) + c.fnp[kk] * pu(kk - 1));
   };
   const double msfux = f.msfux[x2], msfuy = f.msfuy[x2];
-  double t = -(msfux * c.rdx) * (fx(x + 1) - fx(x));
-  if (!g.two_d) t -= (msfux * c.rdy) * (fy(x + sj) - fy(x));
+  double t = -(msfux * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+  if (!g.two_d) t -= (msfux * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
   t -= c.rdnw[k] * (fz(k + 1) - fz(k));
   const double rm = msfux / msfuy;
   double dpx = rm * H * c.rdx * (c.c1h[k] * f.muu[x2] + c.c2h[k]) *
@@ -445,12 +494,26 @@ __global__ void k_tend_v(Range r, Args a) {
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const auto* v = f.v2;
   const double hup = a.q.hup, vup = a.q.vup;
-  auto fx = [&](idx_t y) {
+  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  auto fx = [&](idx_t y, int fi) {
+    const int o = ord_face(fi, g.nx, sp);
+    if (o == 2) return Q * (f.ru[y] + f.ru[y - sj]) * (v[y] + v[y - 1]);
     const double vel = H * (f.ru[y] + f.ru[y - sj]);
+    if (o == 3) return vel * flux3(v[y - 2], v[y - 1], v[y], v[y + 1], vel, hup);
     return vel * flux5(v[y - 3], v[y - 2], v[y - 1], v[y], v[y + 1], v[y + 2], vel, hup);
   };
-  auto fy = [&](idx_t y) {
+  auto fy = [&](idx_t y, int fj) {  // the flux at mass row fj - 1
+    const int o = ord_mass(fj, g.ny, sp);
+    if (o == 2) {  // the upstream normal velocity at the boundary face (large_step.cpp:282-289)
+      if (fj <= 1) {
+        const double vb = v[y] < 0.0 ? double(v[y]) : double(v[y - sj]);
+        return Q * (f.rv[y] + f.rv[y - sj]) * (v[y] + vb);
+      }
+      const double vb = v[y - sj] > 0.0 ? double(v[y - sj]) : double(v[y]);
+      return Q * (f.rv[y] + f.rv[y - sj]) * (vb + v[y - sj]);
+    }
     const double vel = H * (f.rv[y] + f.rv[y - sj]);
+    if (o == 3) return vel * flux3(v[y - 2 * sj], v[y - sj], v[y], v[y + sj], vel, hup);
     return vel * flux5(v[y - 3 * sj], v[y - 2 * sj], v[y - sj], v[y], v[y + sj], v[y + 2 * sj], vel, hup);
   };
   auto fz = [&](int kk) {
@@ -468,8 +531,8 @@ __global__ void k_tend_v(Range r, Args a) {
   };
   const double msfvy = f.msfvy[x2], msfvx = f.msfvx[x2];
   const double rm = msfvy / msfvx;
-  double t = -(msfvy * c.rdx) * (fx(x + 1) - fx(x));
-  t -= (msfvy * c.rdy) * (fy(x + sj) - fy(x));
+  double t = -(msfvy * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+  t -= (msfvy * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
   t -= rm * c.rdnw[k] * (fz(k + 1) - fz(k));
   double dpy = rm * H * c.rdy * (c.c1h[k] * f.muv[x2] + c.c2h[k]) *
                ((f.ph2[x + sk] - f.ph2[x + sk - sj] + f.ph2[x] - f.ph2[x - sj]) + (f.alt[x] + f.alt[x - sj]) * (f.p[x] - f.p[x - sj]) +
@@ -510,12 +573,19 @@ __global__ void k_tend_w(Range r, Args a) {
     if (k < nz) return c.fnm[k] * f.rv[y] + c.fnp[k] * f.rv[y - sk];
     return (2.0 - c.fnm[nz - 1]) * f.rv[y - sk] - c.fnp[nz - 1] * f.rv[y - 2 * sk];
   };
-  auto fx = [&](idx_t y) {
+  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  auto fx = [&](idx_t y, int fi) {
     const double vel = rux(y);
+    const int o = ord_face(fi, g.nx, sp);
+    if (o == 2) return H * vel * (w[y] + w[y - 1]);
+    if (o == 3) return vel * flux3(w[y - 2], w[y - 1], w[y], w[y + 1], vel, hup);
     return vel * flux5(w[y - 3], w[y - 2], w[y - 1], w[y], w[y + 1], w[y + 2], vel, hup);
   };
-  auto fy = [&](idx_t y) {
+  auto fy = [&](idx_t y, int fj) {
     const double vel = rvy(y);
+    const int o = ord_face(fj, g.ny, sp);
+    if (o == 2) return H * vel * (w[y] + w[y - sj]);
+    if (o == 3) return vel * flux3(w[y - 2 * sj], w[y - sj], w[y], w[y + sj], vel, hup);
     return vel * flux5(w[y - 3 * sj], w[y - 2 * sj], w[y - sj], w[y], w[y + sj], w[y + 2 * sj], vel, hup);
   };
   auto fz = [&](int m) {  // vertical flux at mass level m
@@ -525,8 +595,8 @@ __global__ void k_tend_w(Range r, Args a) {
     return vel * flux3(w[y - sk], w[y], w[y + sk], w[y + 2 * sk], -vel, vup);
   };
   const double msftx = f.msftx[x2], msfty = f.msfty[x2];
-  double t = -(msftx * c.rdx) * (fx(x + 1) - fx(x));
-  if (!g.two_d) t -= (msftx * c.rdy) * (fy(x + sj) - fy(x));
+  double t = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+  if (!g.two_d) t -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
   const double rmsft = 1.0 / msfty;
   if (k < nz) {
     t -= c.rdn[k] * (fz(k) - fz(k - 1));
@@ -566,12 +636,19 @@ __global__ void k_tend_t(Range r, Args a) {
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const auto* t2 = f.t2;
   const double hup = a.q.hup, vup = a.q.vup;
-  auto fx = [&](idx_t y) {
+  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  auto fx = [&](idx_t y, int fi) {
     const double vel = f.ru[y];
+    const int o = ord_face(fi, g.nx, sp);
+    if (o == 2) return H * vel * (t2[y] + t2[y - 1]);
+    if (o == 3) return vel * flux3(t2[y - 2], t2[y - 1], t2[y], t2[y + 1], vel, hup);
     return vel * flux5(t2[y - 3], t2[y - 2], t2[y - 1], t2[y], t2[y + 1], t2[y + 2], vel, hup);
   };
-  auto fy = [&](idx_t y) {
+  auto fy = [&](idx_t y, int fj) {
     const double vel = f.rv[y];
+    const int o = ord_face(fj, g.ny, sp);
+    if (o == 2) return H * vel * (t2[y] + t2[y - sj]);
+    if (o == 3) return vel * flux3(t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], vel, hup);
     return vel * flux5(t2[y - 3 * sj], t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], t2[y + 2 * sj], vel, hup);
   };
   auto fz = [&](int kk) {
@@ -582,8 +659,8 @@ __global__ void k_tend_t(Range r, Args a) {
     return vel * flux3(t2[y - 2 * sk], t2[y - sk], t2[y], t2[y + sk], -vel, vup);
   };
   const double msftx = f.msftx[x2];
-  double rr = -(msftx * c.rdx) * (fx(x + 1) - fx(x));
-  if (!g.two_d) rr -= (msftx * c.rdy) * (fy(x + sj) - fy(x));
+  double rr = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+  if (!g.two_d) rr -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
   rr -= c.rdnw[k] * (fz(k + 1) - fz(k));
   if (a.q.use_tendf) rr += f.t_tendf[x] / f.msfty[x2];  // rk_addtend_dry
   // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
@@ -617,24 +694,58 @@ __global__ void k_tend_ph(Range r, Args a) {
            45.0 * (f.phb(i, j + 1, k) - f.phb(i, j - 1, k)) - 9.0 * (f.phb(i, j + 2, k) - f.phb(i, j - 2, k)) +
            (f.phb(i, j + 3, k) - f.phb(i, j - 3, k));
   };
+  auto grad4x = [&]() {
+    return 8.0 * (ph[x + 1] - ph[x - 1]) - (ph[x + 2] - ph[x - 2]) + 8.0 * (f.phb(i + 1, j, k) - f.phb(i - 1, j, k)) -
+           (f.phb(i + 2, j, k) - f.phb(i - 2, j, k));
+  };
+  auto grad4y = [&]() {
+    return 8.0 * (ph[x + sj] - ph[x - sj]) - (ph[x + 2 * sj] - ph[x - 2 * sj]) +
+           8.0 * (f.phb(i, j + 1, k) - f.phb(i, j - 1, k)) - (f.phb(i, j + 2, k) - f.phb(i, j - 2, k));
+  };
+  // one-sided differences (dfwd) of ph + phb at index y (base point (bi, bj))
+  auto dfwdx = [&](idx_t y, int bi) { return f.phb(bi + 1, j, k) - f.phb(bi, j, k) + ph[y + 1] - ph[y]; };
+  auto dfwdy = [&](idx_t y, int bj) { return f.phb(i, bj + 1, k) - f.phb(i, bj, k) + ph[y + sj] - ph[y]; };
+  // rhs_ph next to specified edges (large_step.cpp:596-605): none on the outer row, 2nd order on the next,
+  // then 4th order in y (none in x), 6th from the fourth row in
+  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  auto horder = [&](int ii, int n, bool is_x) {
+    if (!sp) return 6;
+    if (ii <= 0 || ii >= n - 1) return 0;
+    if (ii == 1 || ii == n - 2) return 2;
+    if (ii == 2 || ii == n - 3) return is_x ? 0 : 4;
+    return 6;
+  };
   const double msfty = f.msfty[x2];
   double t = 0;
   if (k < nz) t = t - (c.fnm[k] * wdwn(k) + c.fnp[k] * wdwn(k - 1));
   t = t + (c.c1f[k] * f.mut[x2] + c.c2f[k]) * kG * f.w2[x] / msfty;
-  const double r60 = 1.0 / 60.0;
+  const double r60 = 1.0 / 60.0, r12 = 1.0 / 12.0;
   const idx_t xt = g.at(i, j, nz - 1);  // top extrapolation reads levels nz-1, nz-2
   if (!g.two_d) {
     const double mvp = c.c1f[k] * f.muv[x2 + sj] + c.c2f[k], mvm = c.c1f[k] * f.muv[x2] + c.c2f[k];
     const double vp = k < nz ? v[x + sj] + v[x + sj - sk] : c.cfn * v[xt + sj] + c.cfn1 * v[xt + sj - sk];
     const double vm = k < nz ? v[x] + v[x - sk] : c.cfn * v[xt] + c.cfn1 * v[xt - sk];
     const double cy = (k < nz ? 0.25 : H) * c.rdy / msfty;
-    t = t - cy * ((mvp * vp * f.msfvy[x2 + sj] + mvm * vm * f.msfvy[x2]) * r60 * grad6y());
+    const int o = horder(j, g.ny, false);
+    if (o == 6)
+      t = t - cy * ((mvp * vp * f.msfvy[x2 + sj] + mvm * vm * f.msfvy[x2]) * r60 * grad6y());
+    else if (o == 4)
+      t = t - cy * ((mvp * vp * f.msfvy[x2 + sj] + mvm * vm * f.msfvy[x2]) * r12 * grad4y());
+    else if (o == 2)
+      t = t - cy * (mvp * vp * f.msfvy[x2 + sj] * dfwdy(x, j) + mvm * vm * f.msfvy[x2] * dfwdy(x - sj, j - 1));
   }
   const double mup = c.c1f[k] * f.muu[x2 + 1] + c.c2f[k], mum = c.c1f[k] * f.muu[x2] + c.c2f[k];
   const double up = k < nz ? u[x + 1] + u[x + 1 - sk] : c.cfn * u[xt + 1] + c.cfn1 * u[xt + 1 - sk];
   const double um = k < nz ? u[x] + u[x - sk] : c.cfn * u[xt] + c.cfn1 * u[xt - sk];
   const double cx = (k < nz ? 0.25 : H) * c.rdx / msfty;
-  t = t - cx * ((mup * up * f.msfux[x2 + 1] + mum * um * f.msfux[x2]) * r60 * grad6x());
+  const int o = horder(i, g.nx, true);
+  if (o == 6)
+    t = t - cx * ((mup * up * f.msfux[x2 + 1] + mum * um * f.msfux[x2]) * r60 * grad6x());
+  else if (o == 2)
+    t = t - cx * (mup * up * f.msfux[x2 + 1] * dfwdx(x, i) + mum * um * f.msfux[x2] * dfwdx(x - 1, i - 1));
+  else if (o == 4)
+    t = t - cx * ((mup * up * f.msfux[x2 + 1] + mum * um * f.msfux[x2]) * r12 * grad4x());
+  if (sp) t = t + f.ph_tendf[x] / msfty;  // rk_addtend_dry with a LateralBC (large_step.cpp:663)
   f.ph_tend[x] = t;
 }
 
@@ -801,13 +912,24 @@ __global__ void k_uv(Range r, Args a, double dts) {
           (c.rdnw[k] * (dpn(d, k + 1) - dpn(d, k)) - H * (c.c1h[k] * f.mu2[x2 - d] + c.c1h[k] * f.mu2[x2]));
     return dd;
   };
+  // Edge (small_step.cpp:39-45): with specified boundaries the spec zone's faces are not advanced here but by
+  // spec_bdyupdate, u += dts ru_tend (LateralBC::update_uv, lateral_bc.cpp:356-364), fused in; the launch then
+  // covers the east face i = nx and the north face j = ny too
+  const int nx = g.nx, ny = g.ny;
+  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  const bool uface = sp && (i <= 0 || i >= nx || j == 0 || j == ny - 1);
+  const bool vface = sp && (i == 0 || i == nx - 1 || j <= 0 || j >= ny);
   const int ui0 = g.bcx != MI_BC_PERIODIC ? 1 : 0;
-  if (i >= ui0 && i < g.nx) {
+  if (uface) {
+    if (j < ny) put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x]);
+  } else if (i >= ui0 && i < nx && j < ny) {
     const double mudf_x = -a.q.emdx * (f.mudf[x2] - f.mudf[x2 - 1]) / f.msfuy[x2];
     put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x] - dts * (a.q.moist ? f.cqu[x] : 1.0) * dp(1, 1, 0, f.muu[x2], c.rdx, f.msfux[x2] / f.msfuy[x2]) + c.c1h[k] * mudf_x);
   }
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
-  if (!g.two_d && j >= vj0 && j < g.ny) {
+  if (vface) {
+    if (i < nx) put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x]);
+  } else if (!g.two_d && j >= vj0 && j < ny && i < nx) {
     const idx_t sj = g.sj;
     const double mudf_y = -a.q.emdy * (f.mudf[x2] - f.mudf[x2 - sj]) * f.msfvx_inv[x2];
     put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x] - dts * (a.q.moist ? f.cqv[x] : 1.0) * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) + c.c1h[k] * mudf_y);
@@ -835,6 +957,12 @@ __global__ void k_uv(Range r, Args a, double dts) {
 //   7  KWSolve's tridiagonal sweeps (small_step.cpp:386-387), one thread
 //   8  KWSolve's damp_opt 3 damping and phi'' (small_step.cpp:388-406), parallel over levels
 //   9  KWPRho (small_step.cpp:409-414)
+// With specified boundaries a spec-zone column (Edge::mass, small_step.cpp:39-45) takes LateralBC's
+// updates instead: mu, muts and theta'' += dts * tendency (update_mu_t, lateral_bc.cpp:366-372), omega''
+// left as it is but still summed into ww_m (small_step.cpp:214-220), no advance_w, and phi'' by
+// spec_bdyupdate_ph (lateral_bc.cpp:133-144); its w is the zero-gradient copy of the next row in,
+// written once after the last substep (k_bdy_zero_grad).  The east and north domain faces of ru_m, rv_m
+// are accumulated by the last column and row (KSumEdge, small_step.cpp:279-290).
 MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   extern __shared__ double lds[];
   const Geo& g = a.g;
@@ -861,6 +989,9 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   const idx_t sk = g.sk, sj = g.sj;
   const idx_t x2 = g.at2(i, j), x = g.at(i, j, k);
   const bool mk = k < nz, ge1 = k >= 1;  // a mass level; above the surface
+  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  const bool ring = sp && (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1);
+  const bool east = sp && i == g.nx - 1, north = sp && j == g.ny - 1;
 
   // ---------------------------------------------------------------- the load burst
   double u0 = 0, u1 = 0, us0 = 0, us1 = 0, v0 = 0, v1 = 0, vs0 = 0, vs1 = 0;
@@ -869,7 +1000,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   double phtend = 0, w2 = 0, phs = 0, phs_m = 0, phs_p = 0, ph = 0, ph_m = 0, ph_p = 0, rwt = 0;
   double c2a = 0, c2a_m = 0, alt = 0, alt_m = 0, wsv = 0, pm1 = 0;
   double msfty = 1, msftx = 1, mut = 0, muu0 = 0, muu1 = 0, muv0 = 0, muv1 = 0, msfuy0 = 1, msfuy1 = 1, mvi0 = 1, mvi1 = 1;
-  double mu_old = 0, mtend = 0, ph_top = 0;
+  double mu_old = 0, mtend = 0, ph_top = 0, muts_old = 0, wwcur = 0, rum1 = 0, rvm1 = 0;
   if (on) {
     msfty = f.msfty[x2];
     msftx = f.msftx[x2];
@@ -887,7 +1018,9 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     if (k == 0) {
       mu_old = f.mu2[x2];
       mtend = f.mu_tend[x2];
+      if (ring) muts_old = f.muts[x2];
     }
+    if (ring) wwcur = f.ww[x];
     ww1 = f.ww1[x];
     if (!first) wwm = f.ww_m[x];
     ca = f.ca[x];
@@ -920,6 +1053,8 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       if (!first) {
         rum = f.ru_m[x];
         if (!g.two_d) rvm = f.rv_m[x];
+        if (east) rum1 = f.ru_m[x + 1];
+        if (north) rvm1 = f.rv_m[x + sj];
       }
       t2 = f.t2[x];
       ttend = f.t_tend[x];
@@ -954,7 +1089,17 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 2: the column mass
-  if (on && k == 0) {
+  if (on && k == 0 && ring) {  // the spec zone: mu and muts += dts * mu_tend
+    const double mu_new = mu_old + dts * mtend;
+    const double muts = muts_old + dts * mtend;
+    put<kHMass>(f.mu2, g, i, j, 0, mu_new);
+    put<kHMass>(f.muts, g, i, j, 0, muts);
+    S2[ci] = 0.0;
+    S2[CI + ci] = mu_new;
+    S2[2 * CI + ci] = muts;
+    S2[3 * CI + ci] = 0.0;
+    S2[4 * CI + ci] = mtend;
+  } else if (on && k == 0) {
     double dmdt = 0;
     for (int kk = 0; kk < nz; ++kk) dmdt += LDS(DV, kk, ci);
     const double mu_new = mu_old + dts * (dmdt + mtend);
@@ -973,13 +1118,13 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   __syncthreads();
   const double dmdt = S2[ci], mu_new = S2[CI + ci], muts = S2[2 * CI + ci], muave = S2[3 * CI + ci];
   // ---------------------------------------------------------------- 3: the omega'' terms
-  if (on && k + 1 < nz) {
+  if (on && !ring && k + 1 < nz) {
     const double mt = S2[4 * CI + ci];
     LDS(TK, k, ci) = c.dnw[k] * (c.c1h[k] * dmdt + dv + c.c1h[k] * mt) / msfty;
   }
   __syncthreads();
   // ---------------------------------------------------------------- 4: the running integral
-  if (on && k == 0) {
+  if (on && !ring && k == 0) {
     double ww_full = 0;
     for (int kk = 0; kk + 1 < nz; ++kk) {
       ww_full = ww_full - LDS(TK, kk, ci);
@@ -990,12 +1135,19 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   // ---------------------------------------------------------------- 5: omega'', theta'', explicit phi''
   double wwk = 0;
   if (on) {
-    wwk = (k >= 1 && k < nz) ? LDS(TK, k - 1, ci) - ww1 : 0.0;
+    wwk = (!ring && k >= 1 && k < nz) ? LDS(TK, k - 1, ci) - ww1 : 0.0;
     LDS(WW, k, ci) = wwk;
   }
   __syncthreads();
   double t2new = 0;
-  if (on) {
+  if (on && ring) {  // the spec zone: sumflux of ww as it stands, theta'' += dts * t_tend
+    const double sw = (first ? 0.0 : wwm) + wwcur;
+    put<kHMass>(f.ww_m, g, i, j, k * sk, last ? sw / rn + ww1 : sw);
+    if (mk) {
+      t2new = t2 + dts * ttend;
+      f.t2[x] = t2new;
+    }
+  } else if (on) {
     f.ww[x] = wwk;
     {
       const double sw = (first ? 0.0 : wwm) + wwk;
@@ -1005,12 +1157,6 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       // wdtn(k) and wdtn(k+1): omega'' times theta at the w level (0 at the bottom and the top)
       const double wdtn_k = (k == 0) ? 0.0 : wwk * (c.fnm[k] * ts + c.fnp[k] * tskm);
       const double wdtn_kp = (k + 1 == nz) ? 0.0 : LDS(WW, k + 1, ci) * (c.fnm[k + 1] * tskp + c.fnp[k + 1] * ts);
-      const double su = (first ? 0.0 : rum) + u0;
-      put<kHU>(f.ru_m, g, i, j, k * sk, last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su);
-      if (!g.two_d) {
-        const double sv = (first ? 0.0 : rvm) + v0;
-        put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
-      }
       double hflux = H * c.rdx * (u1 * (tsxp + ts) - u0 * (ts + tsxm));
       if (!g.two_d) hflux += H * c.rdy * (v1 * (tsyp + ts) - v0 * (ts + tsym));
       const double tn = t2 + msfty * dts * ttend;
@@ -1034,9 +1180,25 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     }
     LDS(R, k, ci) = Rk;
   }
+  if (on && mk) {  // sumflux of ru, rv (KMuTPoint), and of the east and north domain faces (KSumEdge)
+    const double su = (first ? 0.0 : rum) + u0;
+    put<kHU>(f.ru_m, g, i, j, k * sk, last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su);
+    if (!g.two_d) {
+      const double sv = (first ? 0.0 : rvm) + v0;
+      put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
+    }
+    if (east) {
+      const double se = (first ? 0.0 : rum1) + u1;
+      put<kHU>(f.ru_m, g, i + 1, j, k * sk, last ? se / rn + (c.c1h[k] * muu1 + c.c2h[k]) * us1 / msfuy1 : se);
+    }
+    if (north) {
+      const double sn = (first ? 0.0 : rvm1) + v1;
+      put<kHV>(f.rv_m, g, i, j + 1, k * sk, last ? sn / rn + (c.c1h[k] * muv1 + c.c2h[k]) * vs1 * mvi1 : sn);
+    }
+  }
   __syncthreads();
   // ---------------------------------------------------------------- 6: explicit w''
-  if (on) {
+  if (on && !ring) {
     const double m = mut, mav = muave;
     const double ope = 1.0 + epssm, ome = 1.0 - epssm;
     const double msft_inv = 1.0 / msfty;
@@ -1080,7 +1242,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 7: tridiagonal sweeps
-  if (on && k == 0) {
+  if (on && !ring && k == 0) {
     double wprev = LDS(W, 0, ci);
     for (int kk = 1; kk <= nz; ++kk) {
       wprev = (LDS(W, kk, ci) - LDS(AA, kk, ci) * wprev) * LDS(AL, kk, ci);
@@ -1094,7 +1256,14 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 8: damping near the top, phi''
-  if (on) {
+  if (on && ring) {  // spec_bdyupdate_ph, every level
+    const double mts = S2[2 * CI + ci], mt = S2[4 * CI + ci];
+    const double mu_o = mts - dts * mt;
+    const double m_old = c.c1f[k] * mu_o + c.c2f[k], m_new = c.c1f[k] * mts + c.c2f[k];
+    const double phk = ph * m_old / m_new + dts * phtend / m_new + phs * (m_old / m_new - 1.0);
+    put<kHMass>(f.ph2, g, i, j, k * sk, phk);
+    LDS(PH, k, ci) = phk;
+  } else if (on) {
     double wk = LDS(W, k, ci);
     double phk = ph;  // the surface phi'' is not solved for
     if (k >= 1) {
@@ -1254,22 +1423,35 @@ __device__ __forceinline__ double sixth_at(double tend, const Args& a, const rea
   return tend + tx + ty;
 }
 
+// where sixth_order_diffusion reaches (sixth_order.hpp:174-205): everywhere on periodic and symmetric
+// sides, three rows in from specified ones ('u' and 'v' two on their own staggered side)
+__device__ __forceinline__ bool sixth_in(const Geo& g, int diff, char kind, int i, int j) {
+  if (!diff) return false;
+  if (g.bcx != MI_BC_SPECIFIED) return true;
+  const int ie = kind == 'u' ? g.nx - 2 : g.nx - 3, je = kind == 'v' ? g.ny - 2 : g.ny - 3;
+  return i >= 3 && i < ie && j >= 3 && j < je;
+}
+
 // zero_tendf + DiffusionDriver::dynamics (diffusion_driver.cpp:266-281): the held tendencies of u, v, w,
-// theta over the (nx+1) x nyv x (nz+1) points, 0 where sixth_order_diffusion does not reach (w levels 0, nz)
+// theta (and ph with specified boundaries) over the (nx+1) x nyv x (nz+1) points, 0 where
+// sixth_order_diffusion does not reach (w levels 0, nz; the specified edges; everywhere without it)
 __global__ void k_sixth_dyn(Range r, Args a, double coef) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, diff = a.q.diff_6th_opt != 0;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   // tendency(i,j,k) = tendency + tend_x + tend_y with the tendency just zeroed
-  if (j < ny && k < nz) f.ru_tendf[x] = sixth_at<'u'>(0.0, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef);
+  if (j < ny && k < nz)
+    f.ru_tendf[x] = sixth_in(g, diff, 'u', i, j) ? sixth_at<'u'>(0.0, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : 0.0;
   if (!g.two_d && i < nx && j < g.nyv && k < nz)
-    f.rv_tendf[x] = sixth_at<'v'>(0.0, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef);
+    f.rv_tendf[x] = sixth_in(g, diff, 'v', i, j) ? sixth_at<'v'>(0.0, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : 0.0;
   if (i < nx && j < ny) {
-    f.rw_tendf[x] = (k >= 1 && k < nz) ? sixth_at<'m'>(0.0, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], coef) : 0.0;
-    if (k < nz) f.t_tendf[x] = sixth_at<'m'>(0.0, a, f.t2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef);
+    const bool in = sixth_in(g, diff, 'm', i, j);
+    f.rw_tendf[x] = (in && k >= 1 && k < nz) ? sixth_at<'m'>(0.0, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], coef) : 0.0;
+    if (k < nz) f.t_tendf[x] = in ? sixth_at<'m'>(0.0, a, f.t2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : 0.0;
+    if (a.q.spec) f.ph_tendf[x] = 0.0;
   }
 }
 
@@ -1280,9 +1462,11 @@ __global__ void k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
   const Geo& g = a.g;
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  const bool in = sixth_in(g, a.q.diff_6th_opt != 0, 'm', i, j);
 #pragma unroll 1
   for (int n = 0; n < n_sc; ++n) {
-    s.qtw[n][x] = sixth_at<'m'>(0.0 + s.qtf[n][x], a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], coef);
+    const double t0 = 0.0 + s.qtf[n][x];
+    s.qtw[n][x] = in ? sixth_at<'m'>(t0, a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
   }
 }
 
@@ -1352,18 +1536,29 @@ struct ScFaces {
   __device__ double f5(double m3, double m2, double m1, double q0, double p1, double p2, double v) const {
     return pd ? flux5_pd(m3, m2, m1, q0, p1, p2, v, a.q.hup) : flux5(m3, m2, m1, q0, p1, p2, v, a.q.hup);
   }
-  __device__ double f3(double m2, double m1, double q0, double p1, double v) const {
-    return pd ? flux3_pd(m2, m1, q0, p1, v, a.q.vup) : flux3(m2, m1, q0, p1, v, a.q.vup);
+  __device__ double f3(double m2, double m1, double q0, double p1, double v, double up) const {
+    return pd ? flux3_pd(m2, m1, q0, p1, v, up) : flux3(m2, m1, q0, p1, v, up);
   }
-  // x face at U point y (west face of the cell at y)
-  __device__ double fx(idx_t y) const {
+  __device__ bool sp() const { return a.g.bcx == MI_BC_SPECIFIED; }
+  // x face at U point y (west face of cell fi), the order degraded next to a specified edge
+  __device__ double fx(idx_t y, int fi) const {
     const double vel = a.f.ru_m[y];
+    const int o = ord_face(fi, a.g.nx, sp());
+    if (o == 2) return H * vel * (q[y] + q[y - 1]);
+    if (o == 3) return vel * f3(q[y - 2], q[y - 1], q[y], q[y + 1], vel, a.q.hup);
     return vel * f5(q[y - 3], q[y - 2], q[y - 1], q[y], q[y + 1], q[y + 2], vel);
   }
-  __device__ double fy(idx_t y) const {
+  __device__ double fy(idx_t y, int fj) const {
     const idx_t sj = a.g.sj;
     const double vel = a.f.rv_m[y];
+    const int o = ord_face(fj, a.g.ny, sp());
+    if (o == 2) return H * vel * (q[y] + q[y - sj]);
+    if (o == 3) return vel * f3(q[y - 2 * sj], q[y - sj], q[y], q[y + sj], vel, a.q.hup);
     return vel * f5(q[y - 3 * sj], q[y - 2 * sj], q[y - sj], q[y], q[y + sj], q[y + 2 * sj], vel);
+  }
+  // the spec zone (scalars.cpp:100-107): not advected; the limiter leaves the outermost row alone
+  __device__ bool spec_cell(int i, int j) const {
+    return sp() && (i == 0 || i == a.g.nx - 1 || j == 0 || j == a.g.ny - 1);
   }
   // w level kk of the column whose level-0 index is y0
   __device__ double fz(idx_t y0, int kk) const {
@@ -1372,13 +1567,14 @@ struct ScFaces {
     const idx_t sk = a.g.sk, y = y0 + kk * sk;
     const double vel = a.f.ww_m[y];
     if (kk == 1 || kk == nz - 1) return vel * (a.c.fnm[kk] * q[y] + a.c.fnp[kk] * q[y - sk]);
-    return vel * f3(q[y - 2 * sk], q[y - sk], q[y], q[y + sk], -vel);
+    return vel * f3(q[y - 2 * sk], q[y - sk], q[y], q[y + sk], -vel, a.q.vup);
   }
   __device__ double mh(idx_t y2, int k) const { return a.c.c1h[k] * a.f.muts[y2] + a.c.c2h[k]; }
-  __device__ double fxl(idx_t y, idx_t y2, int k) const {
+  __device__ double fxl(idx_t y, idx_t y2, int k, int fi) const {
     const Fields& f = a.f;
     const double dx = 2.0 / (f.msfty[y2] + f.msfty[y2 - 1]) / a.c.rdx, mu = H * (mh(y2, k) + mh(y2 - 1, k)), vel = f.ru_m[y];
-    const double cr = vel * dt / dx / mu;
+    // advect_scalar_pd, degrade_xs, i == ids+1 only: vel = ru/mu, cr = vel*dt/dx (scalars.cpp:88-89)
+    const double cr = (sp() && fi == 1) ? (vel / mu) * dt / dx : vel * dt / dx / mu;
     return mu * (dx / dt) * flux_upwind(qo[y - 1], qo[y], cr);
   }
   __device__ double fyl(idx_t y, idx_t y2, int k) const {
@@ -1414,9 +1610,12 @@ __global__ void k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
   for (int m = 0; m < l.n; ++m) {
     const int n = l.sp[m];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, false};
-    double adv = -(msftx * c.rdx) * (fl.fx(x + 1) - fl.fx(x));
-    if (!g.two_d) adv -= (msftx * c.rdy) * (fl.fy(x + g.sj) - fl.fy(x));
-    adv = adv - c.rdnw[k] * (fl.fz(x0, k + 1) - fl.fz(x0, k));
+    double adv = 0;
+    if (!fl.spec_cell(i, j)) {
+      adv = -(msftx * c.rdx) * (fl.fx(x + 1, i + 1) - fl.fx(x, i));
+      if (!g.two_d) adv -= (msftx * c.rdy) * (fl.fy(x + g.sj, j + 1) - fl.fy(x, j));
+      adv = adv - c.rdnw[k] * (fl.fz(x0, k + 1) - fl.fz(x0, k));
+    }
     double tend = adv * msfty;
     tend += (a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x];
     put<kHMass>(s.qn[n], g, i, j, k * g.sk, (m_old * s.q1[n][x] + dt_rk * tend) / m_new);
@@ -1446,15 +1645,15 @@ struct PdFaces {
 __device__ __forceinline__ PdFaces pd_faces(const ScFaces& fl, const Geo& g, int i, int j, int k) {
   PdFaces p;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), x0 = g.at(i, j, 0);
-  p.lox0 = fl.fxl(x, x2, k);
-  p.dhx0 = fl.fx(x) - p.lox0;
-  p.lox1 = fl.fxl(x + 1, x2 + 1, k);
-  p.dhx1 = fl.fx(x + 1) - p.lox1;
+  p.lox0 = fl.fxl(x, x2, k, i);
+  p.dhx0 = fl.fx(x, i) - p.lox0;
+  p.lox1 = fl.fxl(x + 1, x2 + 1, k, i + 1);
+  p.dhx1 = fl.fx(x + 1, i + 1) - p.lox1;
   if (!g.two_d) {
     p.loy0 = fl.fyl(x, x2, k);
-    p.dhy0 = fl.fy(x) - p.loy0;
+    p.dhy0 = fl.fy(x, j) - p.loy0;
     p.loy1 = fl.fyl(x + g.sj, x2 + g.sj, k);
-    p.dhy1 = fl.fy(x + g.sj) - p.loy1;
+    p.dhy1 = fl.fy(x + g.sj, j + 1) - p.loy1;
   } else {
     p.loy0 = p.dhy0 = p.loy1 = p.dhy1 = 0;
   }
@@ -1479,6 +1678,10 @@ __global__ void k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk
   for (int q = 0; q < l.n; ++q) {
     const int n = l.sp[q];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
+    if (fl.spec_cell(i, j)) {  // WRF limits interior cells only
+      put<kHMass>(s.scale[n], g, i, j, k * g.sk, 1.0);
+      continue;
+    }
     const PdFaces p = pd_faces(fl, g, i, j, k);
     double hlow = c.rdx * (p.lox1 - p.lox0);
     if (!g.two_d) hlow += c.rdy * (p.loy1 - p.loy0);
@@ -1512,6 +1715,10 @@ __global__ void k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
     const int n = l.sp[q];
     const real* sc = s.scale[n];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
+    if (fl.spec_cell(i, j)) {  // the spec zone: only the held tendency, folded into q_1 before
+      put<kHMass>(s.qn[n], g, i, j, k * sk, m_old * s.q1[n][x] / m_new);
+      continue;
+    }
     const PdFaces p = pd_faces(fl, g, i, j, k);
     // the limited flux of a face: scaled by the scale of the cell it leaves
     auto lim = [](double d, double s_minus, double s_plus) { return d * (d > 0 ? s_minus : (d < 0 ? s_plus : 1.0)); };
@@ -1528,6 +1735,168 @@ __global__ void k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
     }
     put<kHMass>(s.qn[n], g, i, j, k * sk, (m_old * s.q1[n][x] + dt_rk * (tend * msfty)) / m_new);
   }
+}
+
+// ----------------------------------------------------------------------------- specified lateral boundaries
+// The ring of LateralBC (reference src/dyn/lateral_bc.cpp:34-58): side 0..3 = S, N, W, E; the S and N rows
+// b cover i = b .. ibe-b, the W and E columns j = b+1 .. jbe-b-1 (the corners belong to the rows); ibe, jbe
+// the last index of the staggering.  Slabs are stored xs, xe, ys, ye: S, N, W, E -> ys, ye, xs, xe.
+__device__ __forceinline__ bool ring_map(int pos, int side, int b, int ibe, int jbe, int& i, int& j) {
+  switch (side) {
+    case 0: j = b; i = pos; return pos >= b && pos <= ibe - b;
+    case 1: j = jbe - b; i = pos; return pos >= b && pos <= ibe - b;
+    case 2: i = b; j = pos; return pos >= b + 1 && pos <= jbe - b - 1;
+    default: i = ibe - b; j = pos; return pos >= b + 1 && pos <= jbe - b - 1;
+  }
+}
+__device__ __forceinline__ int slab_of_side(int side) { return side < 2 ? side + 2 : side - 2; }
+__device__ __forceinline__ int bdy_nk(int kind, const Geo& g) { return kind == kBdyPH ? g.nz + 1 : (kind == kBdyMU ? 1 : g.nz); }
+__device__ __forceinline__ int bdy_npos(int kind, int sl, const Geo& g) {
+  return sl < 2 ? g.ny + (kind == kBdyV ? 1 : 0) : g.nx + (kind == kBdyU ? 1 : 0);
+}
+// offset of slab sl (0..3 values, 4..7 tendencies) in a variable's buffer
+__device__ __forceinline__ int64_t bdy_slab(int kind, int sl, const Geo& g) {
+  const int64_t nk = bdy_nk(kind, g), w = 5;
+  const int64_t sx = int64_t(bdy_npos(kind, 0, g)) * nk * w, sy = int64_t(bdy_npos(kind, 2, g)) * nk * w;
+  const int h = sl / 4, s4 = sl % 4;
+  return int64_t(h) * (2 * sx + 2 * sy) + (s4 < 2 ? s4 * sx : 2 * sx + (s4 - 2) * sy);
+}
+
+// One launch over the rows of every variable in bs: thread (pos, k), block z = (variable, side, row).
+//   kBdySpec   spec_bdytend (KSpecTend, lateral_bc.cpp:67-79): the spec zone's tendency = the data's
+//   kBdyRelax  relax_bdytend (KRelax, lateral_bc.cpp:82-121) of rows 1..3 into the held tendencies
+//   kBdyFinal  spec_bdy_final (KFinal, lateral_bc.cpp:180-194): the spec zone's values at t + dt, without
+//              halo images (the reference refreshes no halo after it)
+__global__ void k_bdy(Args a, Scalars s, BdySet bs, int mode, int b0, int nb) {
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int pos = int(blockIdx.x) * 64 + int(threadIdx.x), k = int(blockIdx.y), z = int(blockIdx.z);
+  const int vi = z / (4 * nb), side = (z / nb) % 4, b = b0 + z % nb;
+  if (vi >= bs.n) return;
+  const BdyVar V = bs.v[vi];
+  const int kind = V.kind, nk = bdy_nk(kind, g);
+  if (k >= nk) return;
+  const int ibe = kind == kBdyU ? g.nx : g.nx - 1, jbe = kind == kBdyV ? g.ny : g.ny - 1;
+  int i, j;
+  if (!ring_map(pos, side, b, ibe, jbe, i, j)) return;
+  const int sl = slab_of_side(side), npos = bdy_npos(kind, sl, g);
+  const double* vals = V.base + bdy_slab(kind, sl, g);
+  const double* tens = V.base + bdy_slab(kind, sl + 4, g);
+  auto at = [&](int p, int bb) { return int64_t(p) + int64_t(k) * npos + int64_t(bb) * npos * nk; };
+  const double dtbc = *f.bdy_time;
+  auto bval = [&](int p, int bb) { return vals[at(p, bb)] + dtbc * tens[at(p, bb)]; };
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  if (mode == kBdySpec) {
+    const double t = tens[at(pos, b)];
+    switch (kind) {
+      case kBdyU: f.ru_tend[x] = real(t); break;
+      case kBdyV: f.rv_tend[x] = real(t); break;
+      case kBdyT: f.t_tend[x] = real(t); break;
+      case kBdyPH: f.ph_tend[x] = real(t); break;
+      case kBdyMU: f.mu_tend[x2] = t; break;
+      default: s.qtw[V.q][x] = real(t); break;
+    }
+    return;
+  }
+  if (mode == kBdyFinal) {
+    const double bv = bval(pos, b);
+    switch (kind) {
+      case kBdyU: f.u2[x] = real(f.msfuy[x2] * bv / (c.c1h[k] * f.muus[x2] + c.c2h[k])); break;
+      case kBdyV: f.v2[x] = real(f.msfvx[x2] * bv / (c.c1h[k] * f.muvs[x2] + c.c2h[k])); break;
+      case kBdyT: f.t2[x] = real(bv / (c.c1h[k] * f.muts[x2] + c.c2h[k])); break;
+      case kBdyPH: f.ph2[x] = real(bv / (c.c1f[k] * f.muts[x2] + c.c2f[k])); break;
+      case kBdyMU: f.mu2[x2] = bv; break;
+      default: s.q2[V.q][x] = real(bv / (c.c1h[k] * f.muts[x2] + c.c2h[k])); break;
+    }
+    return;
+  }
+  // the model field as the data holds it, at (ii, jj)
+  auto F = [&](int ii, int jj) -> double {
+    const idx_t y = g.at(ii, jj, k), y2 = g.at2(ii, jj);
+    switch (kind) {
+      case kBdyU: return f.ru[y];
+      case kBdyV: return f.rv[y];
+      case kBdyT: return f.t2[y] * (c.c1h[k] * f.mut[y2] + c.c2h[k]);
+      case kBdyPH: return f.ph2[y] * (c.c1f[k] * f.mut[y2] + c.c2f[k]);
+      case kBdyMU: return f.mu2[y2];
+      default: return s.q2[V.q][y] * (c.c1h[k] * f.mut[y2] + c.c2h[k]);
+    }
+  };
+  double fls0, fls1, fls2, fls3, fls4;
+  fls0 = bval(pos, b) - F(i, j);
+  if (side <= 1) {
+    const int im1 = i - 1 > 0 ? i - 1 : 0, ip1 = i + 1 < ibe ? i + 1 : ibe;
+    const int jin = side == 0 ? j + 1 : j - 1, jout = side == 0 ? j - 1 : j + 1;
+    fls1 = bval(im1, b) - F(im1, j);
+    fls2 = bval(ip1, b) - F(ip1, j);
+    fls3 = bval(i, b - 1) - F(i, jout);
+    fls4 = bval(i, b + 1) - F(i, jin);
+  } else {
+    const int iin = side == 2 ? i + 1 : i - 1, iout = side == 2 ? i - 1 : i + 1;
+    fls1 = bval(j - 1, b) - F(i, j - 1);
+    fls2 = bval(j + 1, b) - F(i, j + 1);
+    fls3 = bval(j, b - 1) - F(iout, j);
+    fls4 = bval(j, b + 1) - F(iin, j);
+  }
+  const double fc = a.q.fcx[b], gc = a.q.gcx[b];
+  const double sm = fls1 + fls2 + fls3 + fls4 - 4.0 * fls0;
+  const double val = fc * fls0 - gc * sm;
+  switch (kind) {  // *_save -> *_tendf (u, v times msfuy, msfvx); scalars straight into the tendency
+    case kBdyU: f.ru_tendf[x] = real(f.ru_tendf[x] + val * f.msfuy[x2]); break;
+    case kBdyV: f.rv_tendf[x] = real(f.rv_tendf[x] + val * f.msfvx[x2]); break;
+    case kBdyT: f.t_tendf[x] = real(f.t_tendf[x] + val); break;
+    case kBdyPH: f.ph_tendf[x] = real(f.ph_tendf[x] + val); break;
+    case kBdyMU: f.mu_tend[x2] = f.mu_tend[x2] + val; break;
+    default: s.qtw[V.q][x] = real(s.qtw[V.q][x] + fc * fls0 - gc * sm); break;
+  }
+}
+
+// zero_grad_bdy on w (KZeroGrad, lateral_bc.cpp:146-161): the spec zone's w from the first row in, every
+// level.  The reference runs it after every substep's advance_w; nothing reads the spec zone's w in between
+// (advance_w skips those columns), so the copy after the last substep leaves the same values.
+__global__ void k_bdy_zero_grad(Args a) {
+  const Geo& g = a.g;
+  const int pos = int(blockIdx.x) * 64 + int(threadIdx.x), k = int(blockIdx.y), side = int(blockIdx.z);
+  const int ibe = g.nx - 1, jbe = g.ny - 1;
+  int i, j;
+  if (k > g.nz || !ring_map(pos, side, 0, ibe, jbe, i, j)) return;
+  const int ic = i < 1 ? 1 : (i > ibe - 1 ? ibe - 1 : i), jc = j < 1 ? 1 : (j > jbe - 1 ? jbe - 1 : j);
+  real* w = a.f.w2;
+  switch (side) {
+    case 0: w[g.at(i, j, k)] = w[g.at(ic, 1, k)]; break;
+    case 1: w[g.at(i, j, k)] = w[g.at(ic, jbe - 1, k)]; break;
+    case 2: w[g.at(i, j, k)] = w[g.at(1, jc, k)]; break;
+    default: w[g.at(i, j, k)] = w[g.at(ibe - 1, jc, k)]; break;
+  }
+}
+
+// flow_dep_bdy (KFlowDep, lateral_bc.cpp:163-178) of the species without boundary data: the spec zone takes
+// the next row in at outflow and 0 at inflow (by the sign of ru_m, rv_m); with the halo images the
+// reference's refresh of q_2 writes next
+__global__ void k_bdy_flow_dep(Args a, Scalars s, SpList l) {
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const int pos = int(blockIdx.x) * 64 + int(threadIdx.x), k = int(blockIdx.y);
+  const int q = int(blockIdx.z) / 4, side = int(blockIdx.z) % 4;
+  const int ibe = g.nx - 1, jbe = g.ny - 1;
+  int i, j;
+  if (q >= l.n || k >= g.nz || !ring_map(pos, side, 0, ibe, jbe, i, j)) return;
+  real* fq = s.q2[l.sp[q]];
+  const int ic = i < 1 ? 1 : (i > ibe - 1 ? ibe - 1 : i), jc = j < 1 ? 1 : (j > jbe - 1 ? jbe - 1 : j);
+  double v;
+  switch (side) {
+    case 0: v = f.rv_m[g.at(i, j, k)] < 0.0 ? double(fq[g.at(ic, 1, k)]) : 0.0; break;
+    case 1: v = f.rv_m[g.at(i, j + 1, k)] > 0.0 ? double(fq[g.at(ic, jbe - 1, k)]) : 0.0; break;
+    case 2: v = f.ru_m[g.at(i, j, k)] < 0.0 ? double(fq[g.at(1, jc, k)]) : 0.0; break;
+    default: v = f.ru_m[g.at(i + 1, j, k)] > 0.0 ? double(fq[g.at(ibe - 1, jc, k)]) : 0.0; break;
+  }
+  put<kHMass>(fq, g, i, j, k * g.sk, v);
+}
+
+// LateralBC::advance_time (rk3.cpp:93): dtbc += dt at the start of each step, on the device
+__global__ void k_bdy_time(double* t, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *t = *t + dt;
 }
 
 // RK3::step's q_1 = q_2 copies (rk3.cpp:99), whole allocations, one species per grid row
@@ -1691,7 +2060,8 @@ void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st) {
 }
 void launch_uv(const Args& a, double dts, hipStream_t st) {
   const Geo& g = a.g;
-  launch3("uv", k_uv, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, dts);
+  const int e = g.bcx == MI_BC_SPECIFIED ? 1 : 0;  // the east and north faces of the spec zone
+  launch3("uv", k_uv, Range{0, g.nx + e, 0, g.ny + e, 0, g.nz}, st, a, dts);
 }
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
   dispatch_cfg<AcousticL>(cfg, "acoustic", 10, 5, a.g.nx, a.g.ny, st, a, dts, it, ns);
@@ -1738,6 +2108,36 @@ void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, bool 
   launch3a("sc_pd_pre", 3 * n, k_sc_pd_pre, r, st, a, s, l, dt_rk);            // q_1 q_tendf -> q_1
   launch3a("sc_pd_scale", 3 + 3 * n, k_sc_pd_scale, r, st, a, s, l, dt_rk);    // q_2 q_1 -> scale
   launch3a("sc_pd_update", 3 + 4 * n, k_sc_pd_update, r, st, a, s, l, dt_rk); // q_2 q_1 scale -> q_new
+}
+void launch_bdy(const Args& a, const Scalars& s, const BdySet& bs, BdyMode mode, hipStream_t st) {
+  if (bs.n <= 0) return;
+  const Geo& g = a.g;
+  const int maxpos = std::max(g.nx, g.ny) + 1;
+  const int b0 = mode == kBdyRelax ? 1 : 0, nb = mode == kBdyRelax ? 3 : 1;
+  const char* name = mode == kBdySpec ? "bdy_spec" : (mode == kBdyRelax ? "bdy_relax" : "bdy_final");
+  const bool t = timing(name);
+  if (t) time_begin(st);
+  hipLaunchKernelGGL(k_bdy, dim3(unsigned((maxpos + 63) / 64), unsigned(g.nz + 1), unsigned(bs.n * 4 * nb)), dim3(64, 1, 1),
+                     0, st, a, s, bs, int(mode), b0, nb);
+  if (t) time_end(st, 0.0);
+}
+void launch_bdy_zero_grad(const Args& a, hipStream_t st) {
+  const Geo& g = a.g;
+  const int maxpos = std::max(g.nx, g.ny) + 1;
+  hipLaunchKernelGGL(k_bdy_zero_grad, dim3(unsigned((maxpos + 63) / 64), unsigned(g.nz + 1), 4), dim3(64, 1, 1), 0, st, a);
+}
+void launch_bdy_flow_dep(const Args& a, const Scalars& s, const int* sp, int n, hipStream_t st) {
+  if (n <= 0) return;
+  SpList l{};
+  l.n = n;
+  for (int q = 0; q < n; ++q) l.sp[q] = sp[q];
+  const Geo& g = a.g;
+  const int maxpos = std::max(g.nx, g.ny) + 1;
+  hipLaunchKernelGGL(k_bdy_flow_dep, dim3(unsigned((maxpos + 63) / 64), unsigned(g.nz), unsigned(4 * n)), dim3(64, 1, 1), 0, st,
+                     a, s, l);
+}
+void launch_bdy_time(const Args& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_bdy_time, dim3(1, 1, 1), dim3(64, 1, 1), 0, st, a.f.bdy_time, a.q.dt);
 }
 void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hipStream_t st) {
   if (n <= 0) return;

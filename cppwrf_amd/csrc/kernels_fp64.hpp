@@ -49,6 +49,10 @@ struct Fields {
   // Coriolis and grid rotation (state.hpp:105-106); held tendencies of the first stage (state.hpp:85-88)
   double *fcor, *ecor, *sina, *cosa, *ht;
   real *ru_tendf, *rv_tendf, *rw_tendf, *t_tendf;
+  // specified boundaries: the held tendency of ph (reference Tendencies::ph_tendf) and the seconds since
+  // the start of the boundary interval (LateralBC::dtbc), kept on the device so a replayed step advances it
+  real* ph_tendf;
+  double* bdy_time;
 };
 
 // The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
@@ -59,9 +63,27 @@ struct Scalars {
   real* qtf[MI_MAX_SCALARS];
   real* qn[MI_MAX_SCALARS];
   real* scale[MI_MAX_SCALARS];  // advect_scalar_pd's outflow scale of each species
-  real* qtw[MI_MAX_SCALARS];    // the step's tendency: the caller's plus the sixth-order diffusion
+  real* qtw[MI_MAX_SCALARS];    // the step's tendency: the caller's plus diffusion and boundary relaxation
   int n_moist;
 };
+
+// Specified lateral boundaries: the boundary data of one variable, one device buffer of eight slabs (values
+// of the sides xs, xe, ys, ye, then their tendencies), each (pos, k, b) with pos fastest (reference
+// LateralBC, src/dyn/lateral_bc.hpp:17-19, 77-84)
+enum BdyKind : int { kBdyU = 0, kBdyV = 1, kBdyT = 2, kBdyPH = 3, kBdyMU = 4, kBdyQ = 5 };
+struct BdyVar {
+  int kind;  // BdyKind
+  int q;     // the scalar (kBdyQ)
+  const double* base;
+};
+constexpr int kBdyMax = 5 + MI_MAX_SCALARS;
+struct BdySet {
+  int n;
+  BdyVar v[kBdyMax];
+};
+// what a ring launch does: the spec zone's tendency from the data (spec_bdytend), the relaxation of rows
+// 1..3 (relax_bdytend), the spec zone's values at the end of a step (spec_bdy_final)
+enum BdyMode : int { kBdySpec = 0, kBdyRelax = 1, kBdyFinal = 2 };
 
 struct Params {
   double hup, vup;   // 1 for 5th / 3rd-order upwind, 0 for 6th / 4th-order centred
@@ -76,6 +98,8 @@ struct Params {
   double diff_6th_factor;
   double dt;           // the large step
   int terrain;         // the surface w follows the terrain (ht)
+  int spec;            // specified lateral boundaries (both directions)
+  double fcx[5], gcx[5];  // their relaxation weights (lbc_fcx_gcx)
 };
 
 struct Args {
@@ -126,6 +150,11 @@ void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStre
 void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st);
 // sixth-order diffusion of u, v, w, theta into the held tendencies (first stage)
 void launch_sixth_dyn(const Args& a, hipStream_t st);
+// specified boundaries (reference LateralBC): ring launches over the variables of bs
+void launch_bdy(const Args& a, const Scalars& s, const BdySet& bs, BdyMode mode, hipStream_t st);
+void launch_bdy_zero_grad(const Args& a, hipStream_t st);
+void launch_bdy_flow_dep(const Args& a, const Scalars& s, const int* sp, int n, hipStream_t st);
+void launch_bdy_time(const Args& a, hipStream_t st);
 // the same for the scalars, into their step tendencies (first stage, after the acoustic steps)
 void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, hipStream_t st);
 // set_w_surface (fill = false) after a step, with terrain

@@ -120,8 +120,8 @@ class Case:
         return cls(A.MI_CASE_WARM_BUBBLE, Grid(200, 1, 100, 100.0, 100.0, 1.0, A.MI_BC_SYMMETRIC), 10000.0)
 
     @classmethod
-    def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0):
-        return cls(A.MI_CASE_REST_STATE, Grid(nx, ny, nz, 2000.0, 2000.0, 10.0), 20000.0, u0=u0)
+    def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0, bc=A.MI_BC_PERIODIC):
+        return cls(A.MI_CASE_REST_STATE, Grid(nx, ny, nz, 2000.0, 2000.0, 10.0, bc, bc), 20000.0, u0=u0)
 
     @classmethod
     def hills(cls, nx, ny, nz, dx, dt, seed=1, n_hills=12, hill_height=2500.0, hill_wmin=30000.0, hill_wmax=150000.0,
@@ -282,3 +282,29 @@ class RK3:
 
     def calc_p_rho_phi(self, rk):
         check(lib().mi_calc_p_rho_phi(self._h, rk))
+
+    def step_end(self):
+        check(lib().mi_step_end(self._h))
+
+    # specified lateral boundaries (reference LateralBC::set / set_scalar / dtbc)
+    def bdy_set(self, var: int, side: int, values: np.ndarray, tends: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        t = np.ascontiguousarray(tends, dtype=np.float64)
+        if v.size != t.size:
+            raise ValueError("values and tendencies differ in size")
+        check(lib().mi_bdy_set(self._h, var, side, _ptr(v), _ptr(t), v.size))
+
+    def load_boundary(self, bdy: dict) -> None:
+        """every slab of a {(var, side): (values, tendencies)} dict (cppwrf_amd.boundary)"""
+        for (var, side), (v, t) in bdy.items():
+            self.bdy_set(var, side, v, t)
+
+    @property
+    def bdy_time(self) -> float:
+        d = C.c_double()
+        check(lib().mi_bdy_get_time(self._h, C.byref(d)))
+        return d.value
+
+    @bdy_time.setter
+    def bdy_time(self, v: float) -> None:
+        check(lib().mi_bdy_set_time(self._h, float(v)))

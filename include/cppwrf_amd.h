@@ -175,12 +175,13 @@ int mi_step(mi_domain* d, int nsteps);
 int mi_domain_sync(mi_domain* d);
 
 /* The phases of one step, for tests (same launches as mi_step, in its order):
- *   mi_step_begin      the _2 -> _1 copies (rk3.cpp:94-98)
+ *   mi_step_begin      the boundary clock (rk3.cpp:93) and the _2 -> _1 copies (rk3.cpp:94-98)
  *   mi_rk_prep         LargeStep::prep (large_step.cpp:697-733)
  *   mi_rk_tendency     LargeStep::tendencies (large_step.cpp:798-861)
  *   mi_small_steps     SmallStep::run_stage (small_step.cpp:490-607)
  *   mi_scalar_transport ScalarTransport::run_stage (scalars.cpp:436-461)
  *   mi_calc_p_rho_phi  SmallStep::calc_p_rho_phi (small_step.cpp:609-617)
+ *   mi_step_end        spec_bdy_final and the surface w of terrain (rk3.cpp:159-167)
  * rk = 1, 2, 3. */
 int mi_step_begin(mi_domain* d);
 int mi_rk_prep(mi_domain* d, int rk);
@@ -188,6 +189,31 @@ int mi_rk_tendency(mi_domain* d, int rk);
 int mi_small_steps(mi_domain* d, int rk);
 int mi_scalar_transport(mi_domain* d, int rk);
 int mi_calc_p_rho_phi(mi_domain* d, int rk);
+int mi_step_end(mi_domain* d);
+
+/* ---------------------------------------------------------------- specified lateral boundaries
+ * Real-data boundaries, bc_x = bc_y = MI_BC_SPECIFIED (reference LateralBC, src/dyn/lateral_bc.hpp:28-91,
+ * lateral_bc.cpp:67-457; WRF spec_bdy_width 5 = spec_zone 1 + relax_zone 4).  The outermost row of every
+ * side (the spec zone) takes its tendency and, at the end of each step, its value from boundary data; the
+ * next three rows are relaxed toward the data (fcx/gcx ramp of lbc_fcx_gcx, lateral_bc.cpp:198-211).
+ *
+ * Boundary data of one variable and side: values and tendencies for the current boundary interval, each
+ * npos * nk * 5 doubles in (pos, k, b) order, pos fastest: pos = j on the x sides MI_BDY_XS/XE (ny values,
+ * ny + 1 for V), pos = i on the y sides MI_BDY_YS/YE (nx, nx + 1 for U); k over the variable's levels (nz,
+ * nz + 1 for PH, 1 for MU); b = 0 the outermost row.  The values are coupled as a WRF boundary file holds
+ * them: u (c1h muu + c2h) / msfuy, v (c1h muv + c2h) / msfvx, t (c1h mut + c2h), ph (c1f mut + c2f),
+ * q (c1h mut + c2h), and mu' itself.  At a step the boundary value is values + dtbc * tendencies, dtbc
+ * the seconds since the start of the interval, which each step advances by dt before anything else;
+ * set it back with mi_bdy_set_time when the next interval's data is loaded.  Variables without data hold
+ * zeros; scalars without data take flow-dependent values in the spec zone (flow_dep_bdy). */
+enum { MI_BDY_U = 0, MI_BDY_V = 1, MI_BDY_T = 2, MI_BDY_PH = 3, MI_BDY_MU = 4, MI_BDY_Q = 16 };
+enum { MI_BDY_XS = 0, MI_BDY_XE = 1, MI_BDY_YS = 2, MI_BDY_YE = 3 };
+#define MI_SPEC_BDY_WIDTH 5
+/* reference LateralBC::set / set_scalar (lateral_bc.cpp:240-275); var MI_BDY_Q + n for scalar n */
+int mi_bdy_set(mi_domain* d, int var, int side, const double* values, const double* tends, int64_t n);
+/* reference LateralBC::dtbc (lateral_bc.hpp:58-59) */
+int mi_bdy_set_time(mi_domain* d, double dtbc);
+int mi_bdy_get_time(mi_domain* d, double* dtbc);
 
 /* The domain maxima of w_damp's vertical and horizontal Courant numbers from the current fields
  * (reference LargeStep::cfl_max, src/dyn/large_step.cpp:862-876: |ww / (c1f mut + c2f) rdnw dt| and

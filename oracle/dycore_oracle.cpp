@@ -78,10 +78,17 @@ struct Grid {
 };
 
 // One halo line (halo.cpp:52-84): slot a of the direction with n mass points.
-inline void halo_point(double* base, long sn, int a, int n, bool face, bool periodic, bool odd, bool odd_face) {
+inline void halo_point(double* base, long sn, int a, int n, bool face, bool periodic, bool odd, bool odd_face,
+                       bool flat) {
   const int nl = face ? 4 : 3;
   const int i = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
   int src;
+  if (flat) {  // specified: the outermost row copied outward (halo.cpp:67-69)
+    const int last = face ? n : n - 1;
+    src = i < 0 ? 0 : (i > last ? last : i);
+    base[i * sn] = base[src * sn];
+    return;
+  }
   if (periodic) src = ((i % n) + n) % n;
   else if (face) src = i <= 0 ? -i : 2 * n - i;
   else src = i < 0 ? -1 - i : 2 * n - 1 - i;
@@ -100,13 +107,15 @@ void fill(F& f, const Grid& g, Par px, Par py) {
   for (int k = 0; k < f.nzs; ++k)
     for (int j = 0; j < f.nys; ++j)
       for (int a = 0; a < na; ++a)
-        halo_point(&f(0, j, k), 1, a, g.nx, f.face_x, g.bcx == MI_BC_PERIODIC, px != Par::Even, px == Par::OddFace);
+        halo_point(&f(0, j, k), 1, a, g.nx, f.face_x, g.bcx == MI_BC_PERIODIC, px != Par::Even, px == Par::OddFace,
+                   g.bcx == MI_BC_SPECIFIED);
   if (g.two_d()) return;
   const int nb = f.face_y ? 8 : 6;
   for (int k = 0; k < f.nzs; ++k)
     for (int i = -3; i < f.nxs + 3; ++i)
       for (int a = 0; a < nb; ++a)
-        halo_point(&f(i, 0, k), f.sj, a, g.ny, f.face_y, g.bcy == MI_BC_PERIODIC, py != Par::Even, py == Par::OddFace);
+        halo_point(&f(i, 0, k), f.sj, a, g.ny, f.face_y, g.bcy == MI_BC_PERIODIC, py != Par::Even, py == Par::OddFace,
+                   g.bcy == MI_BC_SPECIFIED);
 }
 // default parities (halo.cpp:224-228): U odd in x, V odd in y, everything else even
 void fill(F& f, const Grid& g) { fill(f, g, f.face_x ? Par::Odd : Par::Even, f.face_y ? Par::Odd : Par::Even); }
@@ -151,6 +160,22 @@ inline double flux_upwind(double qm1, double q0, double cr) {
   return 0.5 * (lo < 1.0 ? lo : 1.0) * qm1 + 0.5 * (hi > -1.0 ? hi : -1.0) * q0;
 }
 
+// kern::ord_face / ord_mass (kernels_common.hpp:45-58): the horizontal advection order next to a
+// specified edge, for a flux between mass points f-1 and f (ord_face) or at mass point f-1 of a field
+// staggered in that direction (ord_mass); 5 stands for the domain's own order
+inline int ord_face(int f, int n, bool on) {
+  if (!on) return 5;
+  if (f <= 1 || f >= n - 1) return 2;
+  if (f == 2 || f == n - 2) return 3;
+  return 5;
+}
+inline int ord_mass(int f, int n, bool on) {
+  if (!on) return 5;
+  if (f <= 1 || f >= n) return 2;
+  if (f == 2 || f == n - 1) return 3;
+  return 5;
+}
+
 struct Lev {
   std::vector<double> dnw, rdnw, dn, rdn, fnm, fnp, c1h, c2h, c3h, c4h, c1f, c2f, c3f, c4f;
   double cf1, cf2, cf3, cfn, cfn1, rdx, rdy, p_top;
@@ -180,9 +205,21 @@ struct Dom {
   F sc_scale, dhx, lox, dhy, loy, dhz, loz;
   // Coriolis and grid rotation (state.hpp:105-107); held tendencies of the first stage (state.hpp:85-88)
   F fcor, ecor, sina, cosa, ht;
-  F ru_tendf, rv_tendf, rw_tendf, t_tendf;
+  F ru_tendf, rv_tendf, rw_tendf, t_tendf, ph_tendf;
   std::vector<F> qtw;  // q_tendf of the step: the caller's tendency plus the sixth-order diffusion
   bool mapped = false, cor = false, use_tendf = false, terrain = false;
+  // specified lateral boundaries (LateralBC, lateral_bc.hpp:28-91): spec_zone 1, relax_zone 4
+  static constexpr int kBw = 5, kSpecZone = 1, kRelaxZone = 4;
+  bool spec = false;
+  double fcx[kBw] = {}, gcx[kBw] = {}, dtbc = 0;
+  // one variable's boundary data: values and tendencies of the sides xs, xe, ys, ye, (pos, k, b) pos fastest
+  struct Bdy {
+    bool have = false;
+    int nk = 0, npos[4] = {};
+    std::vector<double> v[4], t[4];
+  };
+  Bdy bdy[5];             // MI_BDY_U, V, T, PH, MU
+  std::vector<Bdy> qbdy;  // scalars
 
   explicit Dom(const mi_grid& gg, const mi_params& pp_, const mi_levels& lv) {
     g = Grid{gg.nx, gg.ny, gg.nz, gg.bc_x, gg.bc_y, gg.dx, gg.dy, gg.dt};
@@ -219,14 +256,173 @@ struct Dom {
     f3(sc_scale, St::M); f3(dhx, St::U); f3(lox, St::U); f3(dhy, St::V); f3(loy, St::V); f3(dhz, St::W); f3(loz, St::W);
     for (F* f : {&fcor, &ecor, &sina, &cosa, &ht}) f2(*f, St::M);
     std::fill(cosa.a.begin(), cosa.a.end(), 1.0);
-    f3(ru_tendf, St::U); f3(rv_tendf, St::V); f3(rw_tendf, St::W); f3(t_tendf, St::M);
+    f3(ru_tendf, St::U); f3(rv_tendf, St::V); f3(rw_tendf, St::W); f3(t_tendf, St::M); f3(ph_tendf, St::W);
     qtw.resize(size_t(n_sc));
     for (F& f : qtw) f3(f, St::M);
     mapped = p.mapped != 0;
     cor = mapped || p.coriolis != 0;  // MapC::from(b, b.mapped || b.coriolis) (large_step.cpp:810)
-    use_tendf = p.diff_6th_opt != 0;   // a DiffusionDriver attached (large_step.cpp:828)
+    spec = g.bcx == MI_BC_SPECIFIED;  // with bcy the same (checked at creation)
+    use_tendf = p.diff_6th_opt != 0 || spec;  // a DiffusionDriver or a LateralBC attached (large_step.cpp:828)
     terrain = p.terrain != 0;          // !b.ht.uniform() (small_step.cpp:583, rk3.cpp:163)
+    if (spec) {
+      // lbc_fcx_gcx (lateral_bc.cpp:198-211), Fortran loop = b + 1 over spec_zone + 1 .. spec_zone + relax_zone
+      for (int loop = kSpecZone + 1; loop <= kSpecZone + kRelaxZone; ++loop) {
+        const double num = double(kSpecZone + kRelaxZone - loop), den = double(kRelaxZone - 1);
+        fcx[loop - 1] = 0.1 / g.dt * num / den;
+        gcx[loop - 1] = 1.0 / g.dt / 50.0 * num / den;
+      }
+      const int nkv[5] = {nz, nz, nz, nz + 1, 1};  // U V T PH MU (lateral_bc.cpp:219)
+      for (int v = 0; v < 5; ++v) init_bdy(bdy[v], nkv[v], v == 0, v == 1);
+      qbdy.resize(size_t(n_sc));
+      for (Bdy& b : qbdy) init_bdy(b, nz, false, false);
+    }
   }
+  void init_bdy(Bdy& b, int nk, bool u, bool v) {  // lateral_bc.cpp:225-233
+    b.nk = nk;
+    b.npos[0] = b.npos[1] = g.ny + (v ? 1 : 0);  // x sides: pos = j
+    b.npos[2] = b.npos[3] = g.nx + (u ? 1 : 0);  // y sides: pos = i
+    for (int s_ = 0; s_ < 4; ++s_) {
+      b.v[s_].assign(size_t(b.npos[s_]) * size_t(nk) * kBw, 0.0);
+      b.t[s_].assign(size_t(b.npos[s_]) * size_t(nk) * kBw, 0.0);
+    }
+  }
+  // Slab / SlabSet (lateral_bc.cpp:21-32): value + dtbc * tendency of side slab sl
+  size_t bidx(const Bdy& B, int sl, int pos, int k, int b) const {
+    return size_t(pos) + size_t(k) * size_t(B.npos[sl]) + size_t(b) * size_t(B.npos[sl]) * size_t(B.nk);
+  }
+  double bval(const Bdy& B, int sl, int pos, int k, int b) const {
+    const size_t o = bidx(B, sl, pos, k, b);
+    return B.v[sl][o] + dtbc * B.t[sl][o];
+  }
+  static int slab_of_side(int side) { return side < 2 ? side + 2 : side - 2; }  // S,N,W,E -> ys,ye,xs,xe
+  // Ring (lateral_bc.cpp:34-58): rows b0 .. b0+nb-1 of the sides S, N (i = b .. ibe-b) and W, E
+  // (j = b+1 .. jbe-b-1); ibe, jbe the last index of the staggering ('u': nx, 'v': ny in y)
+  template <class K>
+  void ring(char var, int b0, int nb, int nk, K kern) {
+    const int ibe = var == 'u' ? g.nx : g.nx - 1, jbe = var == 'v' ? g.ny : g.ny - 1;
+    for (int side = 0; side < 4; ++side)
+      for (int b = b0; b < b0 + nb; ++b)
+        for (int k = 0; k < nk; ++k) {
+          if (side < 2) {
+            for (int i = b; i <= ibe - b; ++i) kern(i, side, b, i, side == 0 ? b : jbe - b, k);
+          } else {
+            for (int j = b + 1; j <= jbe - b - 1; ++j) kern(j, side, b, side == 2 ? b : ibe - b, j, k);
+          }
+        }
+  }
+  // KRelax (lateral_bc.cpp:82-121): fls0 .. fls4 of row b (1..3) against the model field fm as the data
+  // holds it; out(i, j, k, b, fls0, fls1 + fls2 + fls3 + fls4 - 4 fls0)
+  template <class Fm, class Out>
+  void relax(char var, int nk, const Bdy& B, const Fm& fm, const Out& out) {
+    const int ibe = var == 'u' ? g.nx : g.nx - 1;
+    ring(var, kSpecZone, kRelaxZone - 1, nk, [&](int pos, int side, int b, int i, int j, int k) {
+      const int sl = slab_of_side(side);
+      auto bv = [&](int p_, int bb) { return bval(B, sl, p_, k, bb); };
+      double fls0, fls1, fls2, fls3, fls4;
+      fls0 = bv(pos, b) - fm(i, j, k);
+      if (side <= 1) {
+        const int im1 = i - 1 > 0 ? i - 1 : 0, ip1 = i + 1 < ibe ? i + 1 : ibe;
+        const int jin = side == 0 ? j + 1 : j - 1, jout = side == 0 ? j - 1 : j + 1;
+        fls1 = bv(im1, b) - fm(im1, j, k);
+        fls2 = bv(ip1, b) - fm(ip1, j, k);
+        fls3 = bv(i, b - 1) - fm(i, jout, k);
+        fls4 = bv(i, b + 1) - fm(i, jin, k);
+      } else {
+        const int iin = side == 2 ? i + 1 : i - 1, iout = side == 2 ? i - 1 : i + 1;
+        fls1 = bv(j - 1, b) - fm(i, j - 1, k);
+        fls2 = bv(j + 1, b) - fm(i, j + 1, k);
+        fls3 = bv(j, b - 1) - fm(iout, j, k);
+        fls4 = bv(j, b + 1) - fm(iin, j, k);
+      }
+      out(i, j, k, b, fls0, fls1 + fls2 + fls3 + fls4 - 4.0 * fls0);
+    });
+  }
+  // LateralBC::relax_dry (lateral_bc.cpp:292-338): into the held tendencies (u, v times msfuy, msfvx)
+  void relax_dry() {
+    const int nz = g.nz;
+    relax('u', nz, bdy[MI_BDY_U], [&](int i, int j, int k) { return ru(i, j, k); },
+          [&](int i, int j, int k, int b, double f0, double sm) { ru_tendf(i, j, k) += (fcx[b] * f0 - gcx[b] * sm) * msfuy(i, j); });
+    relax('v', nz, bdy[MI_BDY_V], [&](int i, int j, int k) { return rv(i, j, k); },
+          [&](int i, int j, int k, int b, double f0, double sm) { rv_tendf(i, j, k) += (fcx[b] * f0 - gcx[b] * sm) * msfvx(i, j); });
+    relax('h', nz + 1, bdy[MI_BDY_PH], [&](int i, int j, int k) { return ph2(i, j, k) * (c.c1f[k] * mut(i, j) + c.c2f[k]); },
+          [&](int i, int j, int k, int b, double f0, double sm) { ph_tendf(i, j, k) += fcx[b] * f0 - gcx[b] * sm; });
+    relax('t', nz, bdy[MI_BDY_T], [&](int i, int j, int k) { return t2(i, j, k) * (c.c1h[k] * mut(i, j) + c.c2h[k]); },
+          [&](int i, int j, int k, int b, double f0, double sm) { t_tendf(i, j, k) += fcx[b] * f0 - gcx[b] * sm; });
+    relax('m', 1, bdy[MI_BDY_MU], [&](int i, int j, int) { return mu2(i, j); },
+          [&](int i, int j, int, int b, double f0, double sm) { mu_tend(i, j) += fcx[b] * f0 - gcx[b] * sm; });
+  }
+  // KSpecTend (lateral_bc.cpp:67-79): the spec zone's tendency = the data's
+  void spec_tend(char var, int nk, const Bdy& B, F& out) {
+    ring(var, 0, kSpecZone, nk, [&](int pos, int side, int b, int i, int j, int k) {
+      const int sl = slab_of_side(side);
+      out(i, j, k) = B.t[sl][bidx(B, sl, pos, k, b)];
+    });
+  }
+  void spec_dry() {  // LateralBC::spec_dry (lateral_bc.cpp:340-354)
+    spec_tend('u', g.nz, bdy[MI_BDY_U], ru_tend);
+    spec_tend('v', g.nz, bdy[MI_BDY_V], rv_tend);
+    spec_tend('h', g.nz + 1, bdy[MI_BDY_PH], ph_tend);
+    spec_tend('t', g.nz, bdy[MI_BDY_T], t_tend);
+    spec_tend('m', 1, bdy[MI_BDY_MU], mu_tend);
+  }
+  // KUpdate (lateral_bc.cpp:123-131): field += dts * tend in the spec zone
+  void spec_update(char var, int nk, F& f, const F& t, double dts) {
+    ring(var, 0, kSpecZone, nk, [&](int, int, int, int i, int j, int k) { f(i, j, k) += dts * t(i, j, k); });
+  }
+  // LateralBC::update_ph_w (lateral_bc.cpp:374-387): KUpdatePh (:133-144) and zero_grad_bdy on w (:146-161)
+  void update_ph_w(double dts) {
+    const int ibe = g.nx - 1, jbe = g.ny - 1;
+    ring('h', 0, kSpecZone, g.nz + 1, [&](int, int side, int, int i, int j, int k) {
+      const double mu_old = muts(i, j) - dts * mu_tend(i, j);
+      const double m_old = c.c1f[k] * mu_old + c.c2f[k], m_new = c.c1f[k] * muts(i, j) + c.c2f[k];
+      ph2(i, j, k) = ph2(i, j, k) * m_old / m_new + dts * ph_tend(i, j, k) / m_new + ph_save(i, j, k) * (m_old / m_new - 1.0);
+      const int ic = i < 1 ? 1 : (i > ibe - 1 ? ibe - 1 : i);
+      const int jc = j < 1 ? 1 : (j > jbe - 1 ? jbe - 1 : j);
+      switch (side) {
+        case 0: w2(i, j, k) = w2(ic, 1, k); break;
+        case 1: w2(i, j, k) = w2(ic, jbe - 1, k); break;
+        case 2: w2(i, j, k) = w2(1, jc, k); break;
+        default: w2(i, j, k) = w2(ibe - 1, jc, k); break;
+      }
+    });
+  }
+  // KFlowDep flow_dep_bdy (lateral_bc.cpp:163-178): the spec zone of a scalar without data
+  void flow_dep(F& f) {
+    const int ibe = g.nx - 1, jbe = g.ny - 1;
+    ring('q', 0, kSpecZone, g.nz, [&](int, int side, int, int i, int j, int k) {
+      const int ic = i < 1 ? 1 : (i > ibe - 1 ? ibe - 1 : i);
+      const int jc = j < 1 ? 1 : (j > jbe - 1 ? jbe - 1 : j);
+      switch (side) {
+        case 0: f(i, j, k) = rv_m(i, j, k) < 0.0 ? f(ic, 1, k) : 0.0; break;
+        case 1: f(i, j, k) = rv_m(i, j + 1, k) > 0.0 ? f(ic, jbe - 1, k) : 0.0; break;
+        case 2: f(i, j, k) = ru_m(i, j, k) < 0.0 ? f(1, jc, k) : 0.0; break;
+        default: f(i, j, k) = ru_m(i + 1, j, k) > 0.0 ? f(ibe - 1, jc, k) : 0.0; break;
+      }
+    });
+  }
+  // KFinal spec_bdy_final (lateral_bc.cpp:180-194, 420-455): the exact boundary values at t + dt
+  template <class Cpl>
+  void final_one(char var, int nk, const Bdy& B, F& f, const Cpl& cpl) {
+    ring(var, 0, kSpecZone, nk, [&](int pos, int side, int b, int i, int j, int k) {
+      f(i, j, k) = cpl(i, j, k, bval(B, slab_of_side(side), pos, k, b));
+    });
+  }
+  void spec_final() {
+    const int nz = g.nz;
+    final_one('u', nz, bdy[MI_BDY_U], u2,
+              [&](int i, int j, int k, double bv) { return msfuy(i, j) * bv / (c.c1h[k] * muus(i, j) + c.c2h[k]); });
+    final_one('v', nz, bdy[MI_BDY_V], v2,
+              [&](int i, int j, int k, double bv) { return msfvx(i, j) * bv / (c.c1h[k] * muvs(i, j) + c.c2h[k]); });
+    final_one('t', nz, bdy[MI_BDY_T], t2, [&](int i, int j, int k, double bv) { return bv / (c.c1h[k] * muts(i, j) + c.c2h[k]); });
+    final_one('h', nz + 1, bdy[MI_BDY_PH], ph2,
+              [&](int i, int j, int k, double bv) { return bv / (c.c1f[k] * muts(i, j) + c.c2f[k]); });
+    final_one('m', 1, bdy[MI_BDY_MU], mu2, [&](int, int, int, double bv) { return bv; });
+    for (int n = 0; n < n_sc; ++n)
+      if (qbdy[size_t(n)].have)
+        final_one('t', nz, qbdy[size_t(n)], q2[size_t(n)],
+                  [&](int i, int j, int k, double bv) { return bv / (c.c1h[k] * muts(i, j) + c.c2h[k]); });
+  }
+
   // MapC::vxgm (large_step.cpp:121-124): curvature at mass point (i, j, k) from the uncoupled u, v
   double vxgm(int i, int j, int k) const {
     return H * (u2(i, j, k) + u2(i + 1, j, k)) * (msfvx(i, j + 1) - msfvx(i, j)) * c.rdy -
@@ -245,11 +441,16 @@ struct Dom {
     const int nx = g.nx, ny = g.ny, nz = g.nz, opt = p.diff_6th_opt;
     const double coef = p.diff_6th_factor * 0.015625 / (2.0 * dt);
     auto lim = [&](double dfl, double diff) { return (opt == 2 && dfl * diff <= 0.0) ? 0.0 : dfl; };
-    int i1 = nx, j1 = ny, k0 = 0;
+    int i0 = 0, j0 = 0, i1 = nx, j1 = ny, k0 = 0;
     if (name == 'u') i1 = nx + 1;
     if (name == 'v') j1 = ny + 1;
     if (name == 'w') k0 = 1;
-    pfor(0, i1, 0, j1, k0, nz, [&](int i, int j, int k) {
+    if (spec) {  // specified limits (sixth_order.hpp:176-205): three rows in from every side
+      i0 = 3; j0 = 3;
+      i1 = name == 'u' ? nx - 2 : nx - 3;
+      j1 = name == 'v' ? ny - 2 : ny - 3;
+    }
+    pfor(i0, i1, j0, j1, k0, nz, [&](int i, int j, int k) {
       auto q = [&](int a, int b) { return fld(i + a, j + b, k); };
       const double dxp0 = lim(d6(q(-3, 0), q(-2, 0), q(-1, 0), q(0, 0), q(1, 0), q(2, 0)), q(0, 0) - q(-1, 0));
       const double dxp1 = lim(d6(q(-2, 0), q(-1, 0), q(0, 0), q(1, 0), q(2, 0), q(3, 0)), q(1, 0) - q(0, 0));
@@ -300,12 +501,22 @@ struct Dom {
   // ------------------------------------------------------------------ LargeStep::prep (large_step.cpp:697-733)
   void rk_prep() {
     const int nx = g.nx, ny = g.ny, nz = g.nz, nyv_ = nyv();
-    for (F* f : {&u2, &v2, &w2, &t2, &ph2, &pp, &al, &mu2}) fill(*f, g);
-    // KMu (large_step.cpp:49-66), non-specified edges
+    // mu_2's halo is not refreshed with specified boundaries (large_step.cpp:700-706)
+    for (F* f : {&u2, &v2, &w2, &t2, &ph2, &pp, &al}) fill(*f, g);
+    if (!spec) fill(mu2, g);
+    // KMu (large_step.cpp:49-66): a specified edge face takes its column's mass twice
     pfor(0, nx + 1, 0, nyv_, 0, 1, [&](int i, int j, int) {
       if (i < nx && j < ny) mut(i, j) = mu2(i, j) + mub(i, j);
-      if (j < ny) muu(i, j) = H * (mu2(i, j) + mu2(i - 1, j) + mub(i, j) + mub(i - 1, j));
-      if (i < nx && j < nyv_) muv(i, j) = H * (mu2(i, j) + mu2(i, j - 1) + mub(i, j) + mub(i, j - 1));
+      if (j < ny) {
+        if (spec && i == 0) muu(i, j) = H * (mu2(0, j) + mu2(0, j) + mub(0, j) + mub(0, j));
+        else if (spec && i == nx) muu(i, j) = H * (mu2(nx - 1, j) + mu2(nx - 1, j) + mub(nx - 1, j) + mub(nx - 1, j));
+        else muu(i, j) = H * (mu2(i, j) + mu2(i - 1, j) + mub(i, j) + mub(i - 1, j));
+      }
+      if (i < nx && j < nyv_) {
+        if (spec && j == 0) muv(i, j) = H * (mu2(i, 0) + mu2(i, 0) + mub(i, 0) + mub(i, 0));
+        else if (spec && j == ny) muv(i, j) = H * (mu2(i, ny - 1) + mu2(i, ny - 1) + mub(i, ny - 1) + mub(i, ny - 1));
+        else muv(i, j) = H * (mu2(i, j) + mu2(i, j - 1) + mub(i, j) + mub(i, j - 1));
+      }
     });
     fill(mut, g);
     // KPrep3 (large_step.cpp:68-81)
@@ -368,33 +579,51 @@ struct Dom {
   // ------------------------------------------------------------------ LargeStep::tendencies (large_step.cpp:798-861)
   void rk_tendency(int rk) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (spec) std::fill(mu_tend.a.begin(), mu_tend.a.end(), 0.0);  // zero_tend of mu_tend, every stage (:815-818)
     if (rk == 1 && use_tendf) {  // zero_tendf, then DiffusionDriver::dynamics (diffusion_driver.cpp:266-281)
-      for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf}) std::fill(f->a.begin(), f->a.end(), 0.0);
+      for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf, &ph_tendf}) std::fill(f->a.begin(), f->a.end(), 0.0);
       auto fu = [&](int i, int j, int k) { return u2(i, j, k); };
       auto fv = [&](int i, int j, int k) { return v2(i, j, k); };
       auto fw = [&](int i, int j, int k) { return w2(i, j, k); };
       auto ft = [&](int i, int j, int k) { return t2(i, j, k); };
       auto mu_t = [&](int i, int j) { return mut(i, j); };
-      sixth('u', fu, ru_tendf, mu_t, g.dt, c.c1h, c.c2h);
-      if (!g.two_d()) sixth('v', fv, rv_tendf, mu_t, g.dt, c.c1h, c.c2h);
-      sixth('w', fw, rw_tendf, mu_t, g.dt, c.c1f, c.c2f);
-      sixth('m', ft, t_tendf, mu_t, g.dt, c.c1h, c.c2h);
+      if (p.diff_6th_opt != 0) {
+        sixth('u', fu, ru_tendf, mu_t, g.dt, c.c1h, c.c2h);
+        if (!g.two_d()) sixth('v', fv, rv_tendf, mu_t, g.dt, c.c1h, c.c2h);
+        sixth('w', fw, rw_tendf, mu_t, g.dt, c.c1f, c.c2f);
+        sixth('m', ft, t_tendf, mu_t, g.dt, c.c1h, c.c2h);
+      }
+      if (spec) relax_dry();  // relax_bdy_dry (large_step.cpp:825)
     }
     const bool do_y = !g.two_d();
     const bool top_lid = p.top_lid != 0;
     const double hup = p.h_adv_order == 5 ? 1.0 : 0.0, vup = p.v_adv_order == 3 ? 1.0 : 0.0;
     const double rdx = c.rdx, rdy = c.rdy;
     const double wd_dt = p.w_damping == 1 ? g.dt : 0.0, w_crit = p.w_crit_cfl;
+    const int nxo = nx, nyo = ny;  // kern::Spec: degraded orders next to specified edges
 
     // KTendU (large_step.cpp:152-264), interior faces, no Coriolis, unmapped
     {
       auto fx = [&](int i, int j, int k) {
+        const int o = ord_mass(i, nxo, spec);
+        if (o == 2) {  // the upstream normal velocity at the boundary face (large_step.cpp:160-167)
+          if (i <= 1) {
+            const double ub = u2(i, j, k) < 0.0 ? u2(i, j, k) : u2(i - 1, j, k);
+            return Q * (ru(i, j, k) + ru(i - 1, j, k)) * (u2(i, j, k) + ub);
+          }
+          const double ub = u2(i - 1, j, k) > 0.0 ? u2(i - 1, j, k) : u2(i, j, k);
+          return Q * (ru(i, j, k) + ru(i - 1, j, k)) * (u2(i - 1, j, k) + ub);
+        }
         double vel = H * (ru(i, j, k) + ru(i - 1, j, k));
+        if (o == 3) return vel * flux3(u2(i - 2, j, k), u2(i - 1, j, k), u2(i, j, k), u2(i + 1, j, k), vel, hup);
         return vel * flux5(u2(i - 3, j, k), u2(i - 2, j, k), u2(i - 1, j, k), u2(i, j, k), u2(i + 1, j, k), u2(i + 2, j, k),
                            vel, hup);
       };
       auto fy = [&](int i, int j, int k) {
+        const int o = ord_face(j, nyo, spec);
+        if (o == 2) return Q * (rv(i, j, k) + rv(i - 1, j, k)) * (u2(i, j, k) + u2(i, j - 1, k));
         double vel = H * (rv(i, j, k) + rv(i - 1, j, k));
+        if (o == 3) return vel * flux3(u2(i, j - 2, k), u2(i, j - 1, k), u2(i, j, k), u2(i, j + 1, k), vel, hup);
         return vel * flux5(u2(i, j - 3, k), u2(i, j - 2, k), u2(i, j - 1, k), u2(i, j, k), u2(i, j + 1, k), u2(i, j + 2, k),
                            vel, hup);
       };
@@ -438,12 +667,25 @@ struct Dom {
     // KTendV (large_step.cpp:267-376), 3-D only
     if (do_y) {
       auto fx = [&](int i, int j, int k) {
+        const int o = ord_face(i, nxo, spec);
+        if (o == 2) return Q * (ru(i, j, k) + ru(i, j - 1, k)) * (v2(i, j, k) + v2(i - 1, j, k));
         double vel = H * (ru(i, j, k) + ru(i, j - 1, k));
+        if (o == 3) return vel * flux3(v2(i - 2, j, k), v2(i - 1, j, k), v2(i, j, k), v2(i + 1, j, k), vel, hup);
         return vel * flux5(v2(i - 3, j, k), v2(i - 2, j, k), v2(i - 1, j, k), v2(i, j, k), v2(i + 1, j, k), v2(i + 2, j, k),
                            vel, hup);
       };
       auto fy = [&](int i, int j, int k) {
+        const int o = ord_mass(j, nyo, spec);
+        if (o == 2) {  // upstream normal velocity at the boundary face (large_step.cpp:282-289)
+          if (j <= 1) {
+            const double vb = v2(i, j, k) < 0.0 ? v2(i, j, k) : v2(i, j - 1, k);
+            return Q * (rv(i, j, k) + rv(i, j - 1, k)) * (v2(i, j, k) + vb);
+          }
+          const double vb = v2(i, j - 1, k) > 0.0 ? v2(i, j - 1, k) : v2(i, j, k);
+          return Q * (rv(i, j, k) + rv(i, j - 1, k)) * (vb + v2(i, j - 1, k));
+        }
         double vel = H * (rv(i, j, k) + rv(i, j - 1, k));
+        if (o == 3) return vel * flux3(v2(i, j - 2, k), v2(i, j - 1, k), v2(i, j, k), v2(i, j + 1, k), vel, hup);
         return vel * flux5(v2(i, j - 3, k), v2(i, j - 2, k), v2(i, j - 1, k), v2(i, j, k), v2(i, j + 1, k), v2(i, j + 2, k),
                            vel, hup);
       };
@@ -496,11 +738,17 @@ struct Dom {
       };
       auto fx = [&](int i, int j, int k) {
         double vel = rux(i, j, k);
+        const int o = ord_face(i, nxo, spec);
+        if (o == 2) return H * vel * (w2(i, j, k) + w2(i - 1, j, k));
+        if (o == 3) return vel * flux3(w2(i - 2, j, k), w2(i - 1, j, k), w2(i, j, k), w2(i + 1, j, k), vel, hup);
         return vel * flux5(w2(i - 3, j, k), w2(i - 2, j, k), w2(i - 1, j, k), w2(i, j, k), w2(i + 1, j, k), w2(i + 2, j, k),
                            vel, hup);
       };
       auto fy = [&](int i, int j, int k) {
         double vel = rvy(i, j, k);
+        const int o = ord_face(j, nyo, spec);
+        if (o == 2) return H * vel * (w2(i, j, k) + w2(i, j - 1, k));
+        if (o == 3) return vel * flux3(w2(i, j - 2, k), w2(i, j - 1, k), w2(i, j, k), w2(i, j + 1, k), vel, hup);
         return vel * flux5(w2(i, j - 3, k), w2(i, j - 2, k), w2(i, j - 1, k), w2(i, j, k), w2(i, j + 1, k), w2(i, j + 2, k),
                            vel, hup);
       };
@@ -549,11 +797,17 @@ struct Dom {
     {
       auto fx = [&](int i, int j, int k) {
         double vel = ru(i, j, k);
+        const int o = ord_face(i, nxo, spec);
+        if (o == 2) return H * vel * (t2(i, j, k) + t2(i - 1, j, k));
+        if (o == 3) return vel * flux3(t2(i - 2, j, k), t2(i - 1, j, k), t2(i, j, k), t2(i + 1, j, k), vel, hup);
         return vel * flux5(t2(i - 3, j, k), t2(i - 2, j, k), t2(i - 1, j, k), t2(i, j, k), t2(i + 1, j, k), t2(i + 2, j, k),
                            vel, hup);
       };
       auto fy = [&](int i, int j, int k) {
         double vel = rv(i, j, k);
+        const int o = ord_face(j, nyo, spec);
+        if (o == 2) return H * vel * (t2(i, j, k) + t2(i, j - 1, k));
+        if (o == 3) return vel * flux3(t2(i, j - 2, k), t2(i, j - 1, k), t2(i, j, k), t2(i, j + 1, k), vel, hup);
         return vel * flux5(t2(i, j - 3, k), t2(i, j - 2, k), t2(i, j - 1, k), t2(i, j, k), t2(i, j + 1, k), t2(i, j + 2, k),
                            vel, hup);
       };
@@ -589,28 +843,61 @@ struct Dom {
                (ph2(i, j + 3, k) - ph2(i, j - 3, k)) + 45.0 * (phb(i, j + 1, k) - phb(i, j - 1, k)) -
                9.0 * (phb(i, j + 2, k) - phb(i, j - 2, k)) + (phb(i, j + 3, k) - phb(i, j - 3, k));
       };
+      auto grad4x = [&](int i, int j, int k) {
+        return 8.0 * (ph2(i + 1, j, k) - ph2(i - 1, j, k)) - (ph2(i + 2, j, k) - ph2(i - 2, j, k)) +
+               8.0 * (phb(i + 1, j, k) - phb(i - 1, j, k)) - (phb(i + 2, j, k) - phb(i - 2, j, k));
+      };
+      auto grad4y = [&](int i, int j, int k) {
+        return 8.0 * (ph2(i, j + 1, k) - ph2(i, j - 1, k)) - (ph2(i, j + 2, k) - ph2(i, j - 2, k)) +
+               8.0 * (phb(i, j + 1, k) - phb(i, j - 1, k)) - (phb(i, j + 2, k) - phb(i, j - 2, k));
+      };
+      auto dfwdx = [&](int i, int j, int k) { return phb(i + 1, j, k) - phb(i, j, k) + ph2(i + 1, j, k) - ph2(i, j, k); };
+      auto dfwdy = [&](int i, int j, int k) { return phb(i, j + 1, k) - phb(i, j, k) + ph2(i, j + 1, k) - ph2(i, j, k); };
+      // rhs_ph next to specified edges (large_step.cpp:596-605): none on the outer row, 2nd order on the
+      // next, then 4th order in y (none in x), 6th from the fourth row in
+      auto horder = [&](int i, int n, bool is_x) {
+        if (!spec) return 6;
+        if (i <= 0 || i >= n - 1) return 0;
+        if (i == 1 || i == n - 2) return 2;
+        if (i == 2 || i == n - 3) return is_x ? 0 : 4;
+        return 6;
+      };
       pfor(0, nx, 0, ny, 1, nz + 1, [&](int i, int j, int k) {
         double t = 0;
         if (k < nz) t = t - (c.fnm[k] * wdwn(i, j, k) + c.fnp[k] * wdwn(i, j, k - 1));
         t = t + (c.c1f[k] * mut(i, j) + c.c2f[k]) * kG * w2(i, j, k) / msfty(i, j);
-        const double r60 = 1.0 / 60.0;
+        const double r60 = 1.0 / 60.0, r12 = 1.0 / 12.0;
         if (do_y) {
           const double mvp = c.c1f[k] * muv(i, j + 1) + c.c2f[k], mvm = c.c1f[k] * muv(i, j) + c.c2f[k];
           const double vp =
               k < nz ? v2(i, j + 1, k) + v2(i, j + 1, k - 1) : c.cfn * v2(i, j + 1, nz - 1) + c.cfn1 * v2(i, j + 1, nz - 2);
           const double vm = k < nz ? v2(i, j, k) + v2(i, j, k - 1) : c.cfn * v2(i, j, nz - 1) + c.cfn1 * v2(i, j, nz - 2);
           const double cy = (k < nz ? 0.25 : H) * rdy / msfty(i, j);
-          t = t - cy * ((mvp * vp * msfvy(i, j + 1) + mvm * vm * msfvy(i, j)) * r60 * grad6y(i, j, k));
+          const int o = horder(j, ny, false);
+          if (o == 6)
+            t = t - cy * ((mvp * vp * msfvy(i, j + 1) + mvm * vm * msfvy(i, j)) * r60 * grad6y(i, j, k));
+          else if (o == 4)
+            t = t - cy * ((mvp * vp * msfvy(i, j + 1) + mvm * vm * msfvy(i, j)) * r12 * grad4y(i, j, k));
+          else if (o == 2)
+            t = t - cy * (mvp * vp * msfvy(i, j + 1) * dfwdy(i, j, k) + mvm * vm * msfvy(i, j) * dfwdy(i, j - 1, k));
         }
         const double mup = c.c1f[k] * muu(i + 1, j) + c.c2f[k], mum = c.c1f[k] * muu(i, j) + c.c2f[k];
         const double up =
             k < nz ? u2(i + 1, j, k) + u2(i + 1, j, k - 1) : c.cfn * u2(i + 1, j, nz - 1) + c.cfn1 * u2(i + 1, j, nz - 2);
         const double um = k < nz ? u2(i, j, k) + u2(i, j, k - 1) : c.cfn * u2(i, j, nz - 1) + c.cfn1 * u2(i, j, nz - 2);
         const double cx = (k < nz ? 0.25 : H) * rdx / msfty(i, j);
-        t = t - cx * ((mup * up * msfux(i + 1, j) + mum * um * msfux(i, j)) * r60 * grad6x(i, j, k));
+        const int o = horder(i, nx, true);
+        if (o == 6)
+          t = t - cx * ((mup * up * msfux(i + 1, j) + mum * um * msfux(i, j)) * r60 * grad6x(i, j, k));
+        else if (o == 2)
+          t = t - cx * (mup * up * msfux(i + 1, j) * dfwdx(i, j, k) + mum * um * msfux(i, j) * dfwdx(i - 1, j, k));
+        else if (o == 4)
+          t = t - cx * ((mup * up * msfux(i + 1, j) + mum * um * msfux(i, j)) * r12 * grad4x(i, j, k));
+        if (spec) t = t + ph_tendf(i, j, k) / msfty(i, j);  // rk_addtend_dry with a LateralBC (large_step.cpp:663)
         ph_tend(i, j, k) = t;
       });
     }
+    if (spec) spec_dry();  // spec_bdy_dry (large_step.cpp:859)
   }
 
   // ------------------------------------------------------------------ SmallStep::run_stage (small_step.cpp:490-607)
@@ -729,18 +1016,25 @@ struct Dom {
               H * (c.c1h[k] * mu2(i - di, j - dj) + c.c1h[k] * mu2(i, j)));
         return d;
       };
+      // Edge (small_step.cpp:39-45): the spec zone's faces are advanced by spec_bdyupdate instead
+      auto uface = [&](int i, int j) { return spec && (i <= 0 || i >= nx || j == 0 || j == ny - 1); };
+      auto vface = [&](int i, int j) { return spec && (i == 0 || i == nx - 1 || j <= 0 || j >= ny); };
       pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
-        if (i >= ui0 && i < ui1) {
+        if (i >= ui0 && i < ui1 && !uface(i, j)) {
           const double mudf_x = -emdx * (mudf(i, j) - mudf(i - 1, j)) / msfuy(i, j);
           u2(i, j, k) = u2(i, j, k) + dts * ru_tend(i, j, k) -
                         dts * mc_cqu(i, j, k) * dp(1, 0, i, j, k, muu(i, j), c.rdx, msfux(i, j) / msfuy(i, j)) + c.c1h[k] * mudf_x;
         }
-        if (do_y && j >= vj0 && j < vj1) {
+        if (do_y && j >= vj0 && j < vj1 && !vface(i, j)) {
           const double mudf_y = -emdy * (mudf(i, j) - mudf(i, j - 1)) * msfvx_inv(i, j);
           v2(i, j, k) = v2(i, j, k) + dts * rv_tend(i, j, k) -
                         dts * mc_cqv(i, j, k) * dp(0, 1, i, j, k, muv(i, j), c.rdy, msfvy(i, j) / msfvx(i, j)) + c.c1h[k] * mudf_y;
         }
       });
+      if (spec) {  // LateralBC::update_uv (lateral_bc.cpp:356-364)
+        spec_update('u', nz, u2, ru_tend, dts);
+        spec_update('v', nz, v2, rv_tend, dts);
+      }
       if (do_y) { fill(u2, g); fill(v2, g); }
       else fill(u2, g);
       // KMuTDiv (small_step.cpp:194-207)
@@ -756,7 +1050,15 @@ struct Dom {
       // KMuTCol (small_step.cpp:209-248)
       const bool firstit = it == 1, last = it == ns;
       const double rn = double(ns);
+      auto ring0 = [&](int i, int j) { return spec && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1); };
       cfor(nx, ny, [&](int i, int j) {
+        if (ring0(i, j)) {  // spec zone: no advance_mu_t; sumflux still accumulates ww (small_step.cpp:214-220)
+          for (int k = 0; k <= nz; ++k) {
+            const double sw = (firstit ? 0.0 : ww_m(i, j, k)) + ww(i, j, k);
+            ww_m(i, j, k) = last ? sw / rn + ww1(i, j, k) : sw;
+          }
+          return;
+        }
         double dmdt = 0;
         for (int k = 0; k < nz; ++k) dmdt += c.dnw[k] * dvdxi(i, j, k);
         const double mtend = mu_tend(i, j);
@@ -799,6 +1101,7 @@ struct Dom {
         }
         const double tk = t2(i, j, k);
         t_2save(i, j, k) = tk;
+        if (ring0(i, j)) return;  // spec zone: theta'' by spec_bdyupdate
         double hflux = H * c.rdx *
                        (u2(i + 1, j, k) * (t_save(i + 1, j, k) + t_save(i, j, k)) -
                         u2(i, j, k) * (t_save(i, j, k) + t_save(i - 1, j, k)));
@@ -809,11 +1112,31 @@ struct Dom {
         const double tn = tk + msfty(i, j) * dts * t_tend(i, j, k);
         t2(i, j, k) = tn - dts * msfty(i, j) * (msftx(i, j) * hflux + c.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
       });
+      if (spec) {
+        // KSumEdge (small_step.cpp:279-290): the east and north domain faces are real faces
+        for (int k = 0; k < nz; ++k) {
+          for (int j = 0; j < ny; ++j) {
+            const double su = (firstit ? 0.0 : ru_m(nx, j, k)) + u2(nx, j, k);
+            const double cpl = (c.c1h[k] * muu(nx, j) + c.c2h[k]) * u_save(nx, j, k);
+            ru_m(nx, j, k) = last ? su / rn + cpl / msfuy(nx, j) : su;
+          }
+          for (int i = 0; i < nx; ++i) {
+            const double sv = (firstit ? 0.0 : rv_m(i, ny, k)) + v2(i, ny, k);
+            const double cpl = (c.c1h[k] * muv(i, ny) + c.c2h[k]) * v_save(i, ny, k);
+            rv_m(i, ny, k) = last ? sv / rn + cpl * msfvx_inv(i, ny) : sv;
+          }
+        }
+        // LateralBC::update_mu_t (lateral_bc.cpp:366-372)
+        spec_update('t', nz, t2, t_tend, dts);
+        spec_update('t', 1, mu2, mu_tend, dts);
+        spec_update('t', 1, muts, mu_tend, dts);
+      }
       for (F* f : {&mu2, &mudf, &muts}) fill(*f, g);
       // advance_w: KWRhs, KWExpl, KWSolve, KWPRho (small_step.cpp:292-416)
       auto mh = [&](int k, double m) { return c.c1h[k] * m + c.c2h[k]; };
       auto mf = [&](int k, double m) { return c.c1f[k] * m + c.c2f[k]; };
       pfor(0, nx, 0, ny, 0, nz + 1, [&](int i, int j, int k) {
+        if (ring0(i, j)) return;  // KWRhs: not in the spec zone
         // KWCommon::rhs (small_step.cpp:312-319)
         double R;
         if (k == 0) R = 0;
@@ -835,6 +1158,7 @@ struct Dom {
         }
       });
       pfor(0, nx, 0, ny, 0, nz + 1, [&](int i, int j, int k) {  // KWExpl (small_step.cpp:331-378)
+        if (ring0(i, j)) return;  // spec zone: zero_grad_bdy
         if (k == 0) {  // the surface w'' from the small-step u'', v'' and the terrain slope (small_step.cpp:339-353)
           double ws = 0;
           if (terrain) {
@@ -876,6 +1200,7 @@ struct Dom {
       });
       const double dampmag = p.damp_opt == 3 ? dts * p.dampcoef : 0.0, hdepth = p.zdamp;
       cfor(nx, ny, [&](int i, int j) {  // KWSolve (small_step.cpp:381-407)
+        if (ring0(i, j)) return;  // spec zone: spec_bdyupdate_ph
         const double m = mut(i, j), mts = muts(i, j), ope = 1.0 + epssm;
         for (int k = 1; k <= nz; ++k) w2(i, j, k) = (w2(i, j, k) - ca(i, j, k) * w2(i, j, k - 1)) * calpha(i, j, k);
         for (int k = nz - 1; k >= 1; --k) w2(i, j, k) = w2(i, j, k) - cgamma(i, j, k) * w2(i, j, k + 1);
@@ -895,6 +1220,7 @@ struct Dom {
         for (int k = nz; k >= 1; --k)
           ph2(i, j, k) = w_rhs(i, j, k) + msfty(i, j) * H * dts * kG * ope * w2(i, j, k) / mf(k, mts);
       });
+      if (spec) update_ph_w(dts);  // spec_bdyupdate_ph, zero_grad_bdy(w) (small_step.cpp:588)
       pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) { p_rho_point(i, j, k, p.smdiv, true); });  // KWPRho
       for (F* f : {&ph2, &pp, &al}) fill(*f, g);
     }
@@ -948,10 +1274,16 @@ struct Dom {
     }
     double fx(int i, int j, int k) const {
       const double vel = d.ru_m(i, j, k);
+      const int o = ord_face(i, d.g.nx, d.spec);
+      if (o == 2) return H * vel * (q(i, j, k) + q(i - 1, j, k));
+      if (o == 3) return vel * f3(q(i - 2, j, k), q(i - 1, j, k), q(i, j, k), q(i + 1, j, k), vel, hup);
       return vel * f5(q(i - 3, j, k), q(i - 2, j, k), q(i - 1, j, k), q(i, j, k), q(i + 1, j, k), q(i + 2, j, k), vel);
     }
     double fy(int i, int j, int k) const {
       const double vel = d.rv_m(i, j, k);
+      const int o = ord_face(j, d.g.ny, d.spec);
+      if (o == 2) return H * vel * (q(i, j, k) + q(i, j - 1, k));
+      if (o == 3) return vel * f3(q(i, j - 2, k), q(i, j - 1, k), q(i, j, k), q(i, j + 1, k), vel, hup);
       return vel * f5(q(i, j - 3, k), q(i, j - 2, k), q(i, j - 1, k), q(i, j, k), q(i, j + 1, k), q(i, j + 2, k), vel);
     }
     double fz(int i, int j, int k) const {
@@ -965,7 +1297,8 @@ struct Dom {
     double fxl(int i, int j, int k) const {
       const double dx = 2.0 / (d.msfty(i, j) + d.msfty(i - 1, j)) / d.c.rdx, mu = H * (mh(i, j, k) + mh(i - 1, j, k)),
                    vel = d.ru_m(i, j, k);
-      const double cr = vel * dt / dx / mu;
+      // advect_scalar_pd, degrade_xs, i == ids+1 only: vel = ru/mu, cr = vel*dt/dx (scalars.cpp:88-89)
+      const double cr = (d.spec && i == 1) ? (vel / mu) * dt / dx : vel * dt / dx / mu;
       return mu * (dx / dt) * flux_upwind(qo(i - 1, j, k), qo(i, j, k), cr);
     }
     double fyl(int i, int j, int k) const {
@@ -980,7 +1313,12 @@ struct Dom {
       const double cr = vel * dt / dz / mu;
       return mu * (dz / dt) * flux_upwind(qo(i, j, k - 1), qo(i, j, k), cr);
     }
+    // the spec zone (scalars.cpp:100-107): not advected; the limiter leaves the outermost row alone
+    bool spec_cell(int i, int j) const {
+      return d.spec && (i == 0 || i == d.g.nx - 1 || j == 0 || j == d.g.ny - 1);
+    }
     double adv(int i, int j, int k) const {  // advect_scalar, horizontal terms times msftx
+      if (spec_cell(i, j)) return 0;
       double r = -(d.msftx(i, j) * d.c.rdx) * (fx(i + 1, j, k) - fx(i, j, k));
       if (!d.g.two_d()) r -= (d.msftx(i, j) * d.c.rdy) * (fy(i, j + 1, k) - fy(i, j, k));
       return r - d.c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
@@ -1027,6 +1365,7 @@ struct Dom {
       loz(i, j, k) = l;
     });
     pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {  // KScalePD
+      if (f.spec_cell(i, j)) { sc_scale(i, j, k) = 1; return; }
       const double mxy = msftx(i, j) * msfty(i, j);
       double hlow = c.rdx * (lox(i + 1, j, k) - lox(i, j, k));
       if (do_y) hlow += c.rdy * (loy(i, j + 1, k) - loy(i, j, k));
@@ -1063,6 +1402,10 @@ struct Dom {
       return dd * (dd > 0 ? sc_scale(i, j, k) : (dd < 0 ? sc_scale(i, j, k - 1) : 1.0));
     };
     pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {  // KUpdatePD
+      if (f.spec_cell(i, j)) {
+        qnew[n](i, j, k) = m_full(k, mub(i, j), mu1(i, j)) * qo(i, j, k) / m_full(k, mub(i, j), mu2(i, j));
+        return;
+      }
       double tend = -c.rdnw[k] * (ddz(i, j, k + 1) - ddz(i, j, k) + loz(i, j, k + 1) - loz(i, j, k));
       tend -= msftx(i, j) * (c.rdx * (ddx(i + 1, j, k) - ddx(i, j, k) + lox(i + 1, j, k) - lox(i, j, k)));
       if (do_y) tend -= msftx(i, j) * (c.rdy * (ddy(i, j + 1, k) - ddy(i, j, k) + loy(i, j + 1, k) - loy(i, j, k)));
@@ -1082,36 +1425,48 @@ struct Dom {
         const F& qq = q2[size_t(n)];
         auto fq = [&](int i, int j, int k) { return qq(i, j, k); };
         auto mu_s = [&](int i, int j) { return muts(i, j); };
-        sixth('m', fq, w, mu_s, dt_rk, c.c1h, c.c2h);
+        if (p.diff_6th_opt != 0) sixth('m', fq, w, mu_s, dt_rk, c.c1h, c.c2h);
+        // relax_bdy_scalar (lateral_bc.cpp:389-404): accumulated straight into the tendency
+        if (spec && qbdy[size_t(n)].have)
+          relax('q', g.nz, qbdy[size_t(n)], [&](int i, int j, int k) { return qq(i, j, k) * (c.c1h[k] * mut(i, j) + c.c2h[k]); },
+                [&](int i, int j, int k, int b, double f0, double sm) { w(i, j, k) = w(i, j, k) + fcx[b] * f0 - gcx[b] * sm; });
       }
     }
+    if (spec)  // spec_bdy_scalar, every stage (lateral_bc.cpp:406-418)
+      for (int n = 0; n < n_sc; ++n)
+        if (qbdy[size_t(n)].have) spec_tend('q', g.nz, qbdy[size_t(n)], qtw[size_t(n)]);
     std::vector<int> pd_n, plain_n;
     for (int n = 0; n < n_sc; ++n) {
       const bool pd = rk == 3 && (n < n_moist ? p.moist_adv_opt : p.scalar_adv_opt) == 1;
       (pd ? pd_n : plain_n).push_back(n);
     }
+    // flow_dep_bdy of the species without boundary data after each update (scalars.cpp:420-429)
     for (int n : plain_n) {
       update_plain(n, dt_rk);
       std::swap(q2[n].a, qnew[n].a);
+      if (spec && !qbdy[size_t(n)].have) flow_dep(q2[n]);
     }
     for (int n : pd_n) {
       update_pd(n, dt_rk);
       std::swap(q2[n].a, qnew[n].a);
+      if (spec && !qbdy[size_t(n)].have) flow_dep(q2[n]);
     }
     for (int n = 0; n < n_sc; ++n) fill(q2[n], g);
   }
 
   // ------------------------------------------------------------------ calc_p_rho_phi (small_step.cpp:456-477,609-617)
-  void calc_p_rho_phi() {
-    const int nx = g.nx;
-    pfor(0, nx, 0, g.ny, 0, g.nz, [&](int i, int j, int k) {
+  void calc_p_rho_phi() { calc_p_rho_phi_range(0, g.nx, 0, g.ny); }
+  // KPRhoPhi over [i0, i1) x [j0, j1): the libm lanes count from i0 over i1 - i0 trips (small_step.cpp:619-633)
+  void calc_p_rho_phi_range(int i0, int i1, int j0, int j1) {
+    const int nx = i1 - i0;
+    pfor(i0, i1, j0, j1, 0, g.nz, [&](int i, int j, int k) {
       const double a = -1.0 / (c.c1h[k] * muts(i, j) + c.c2h[k]) *
                        (alb(i, j, k) * (c.c1h[k] * mu2(i, j)) + c.rdnw[k] * (ph2(i, j, k + 1) - ph2(i, j, k)));
       al(i, j, k) = a;
       const double base = kRd * (kT0 + t2(i, j, k)) / (kP1000 * (a + alb(i, j, k)));
       double pw;
       // vm::pow lanes (backend.hpp:144,169-175): libmvec for pairs of the nx-trip loop, scalar for an odd tail
-      if (libmvec && nx > 1 && !((nx & 1) && i == nx - 1))
+      if (libmvec && nx > 1 && !((nx & 1) && i - i0 == nx - 1))
         pw = _mm_cvtsd_f64(_ZGVbN2vv_pow(_mm_set1_pd(base), _mm_set1_pd(kCpovcv)));
       else
         pw = std::pow(base, kCpovcv);
@@ -1143,12 +1498,15 @@ struct Dom {
 
   // RK3::step's copies (rk3.cpp:94-98)
   void step_begin() {
+    if (spec) dtbc += g.dt;  // LateralBC::advance_time (rk3.cpp:93)
     u1.a = u2.a; v1.a = v2.a; w1.a = w2.a; t1.a = t2.a; ph1.a = ph2.a; mu1.a = mu2.a;
     fill(mu1, g);
     for (int n = 0; n < n_sc; ++n) q1[n].a = q2[n].a;  // rk3.cpp:99
   }
   // set_w_surface (rk3.cpp:173-200, fill = false): the kinematic surface w from the new u, v
   void set_w_surface() {
+    fill(u2, g);  // set_w_surface refreshes the u, v halos first (rk3.cpp:194)
+    fill(v2, g);
     cfor(g.nx, g.ny, [&](int i, int j) {
       const double ws =
           msfty(i, j) * 0.5 * c.rdy *
@@ -1161,7 +1519,21 @@ struct Dom {
     });
     fill(w2, g);
   }
-  void step() {  // RK3::step (rk3.cpp:86-170), no physics / boundaries
+  // the end of RK3::step (rk3.cpp:159-167): spec_bdy_final, calc_p_rho_phi on the interior, set_w_surface.
+  // With scalar pow the interior recomputation gives back the same bits (its inputs are unchanged
+  // there), so it runs only with the libmvec lanes, whose split it changes.
+  void step_end() {
+    if (spec) {
+      spec_final();
+      if (libmvec) {
+        calc_p_rho_phi_range(kSpecZone, g.nx - kSpecZone, kSpecZone, g.ny - kSpecZone);
+        fill(pp, g);
+        fill(al, g);
+      }
+    }
+    if (terrain) set_w_surface();
+  }
+  void step() {  // RK3::step (rk3.cpp:86-170), no physics
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
       rk_prep();
@@ -1170,7 +1542,7 @@ struct Dom {
       scalars(rk);
       calc_p_rho_phi();
     }
-    if (terrain) set_w_surface();  // rk3.cpp:163
+    step_end();
   }
 
   F* field(int id) {
@@ -1219,7 +1591,12 @@ int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv,
   try {
     if (!g || !p || !lv || !out) return fail("null argument");
     if (g->nx < 1 || g->ny < 1 || g->nz < 3) return fail("bad grid size");
-    if (g->bc_x > MI_BC_SYMMETRIC || g->bc_y > MI_BC_SYMMETRIC) return fail("oracle: periodic or symmetric only");
+    if (g->bc_x > MI_BC_SPECIFIED || g->bc_y > MI_BC_SPECIFIED || g->bc_x < 0 || g->bc_y < 0)
+      return fail("oracle: periodic, symmetric or specified boundaries");
+    if ((g->bc_x == MI_BC_SPECIFIED) != (g->bc_y == MI_BC_SPECIFIED))
+      return fail("specified boundaries on both x and y or on neither");
+    if (g->bc_x == MI_BC_SPECIFIED && (g->ny == 1 || g->nx < 10 || g->ny < 10))
+      return fail("specified boundaries need a 3-D grid of at least 10 x 10 columns");
     if (p->h_adv_order != 5 && p->h_adv_order != 6) return fail("h_adv_order must be 5 or 6");
     if (p->v_adv_order != 3 && p->v_adv_order != 4) return fail("v_adv_order must be 3 or 4");
     if (p->hypsometric_opt != 1) return fail("oracle: hypsometric_opt 1 only");
@@ -1323,6 +1700,33 @@ int orc_cfl_max(orc_domain* d, double* vert, double* horiz) {
 }
 int orc_calc_p_rho_phi(orc_domain* d, int) {
   d->d->calc_p_rho_phi();
+  return MI_OK;
+}
+int orc_step_end(orc_domain* d) {
+  d->d->step_end();
+  return MI_OK;
+}
+int orc_bdy_set(orc_domain* od, int var, int side, const double* values, const double* tends, int64_t n) {
+  Dom& d = *od->d;
+  if (!d.spec) return fail("the domain has no specified boundaries");
+  if (side < 0 || side > 3 || !values || !tends) return fail("bad side or null data");
+  Dom::Bdy* b = nullptr;
+  if (var >= MI_BDY_U && var <= MI_BDY_MU) b = &d.bdy[var];
+  else if (var >= MI_BDY_Q && var < MI_BDY_Q + d.n_sc) b = &d.qbdy[size_t(var - MI_BDY_Q)];
+  if (!b) return fail("bad boundary variable");
+  const size_t want = size_t(b->npos[side]) * size_t(b->nk) * Dom::kBw;
+  if (n < 0 || size_t(n) != want) return fail("wrong boundary slab size");
+  b->v[side].assign(values, values + n);
+  b->t[side].assign(tends, tends + n);
+  b->have = true;
+  return MI_OK;
+}
+int orc_bdy_set_time(orc_domain* od, double dtbc) {
+  od->d->dtbc = dtbc;
+  return MI_OK;
+}
+int orc_bdy_get_time(orc_domain* od, double* dtbc) {
+  *dtbc = od->d->dtbc;
   return MI_OK;
 }
 

@@ -39,6 +39,11 @@ int orc_small_steps(orc_domain* d, int rk);
 int orc_scalar_transport(orc_domain* d, int rk); /* ScalarTransport::run_stage */
 int orc_cfl_max(orc_domain* d, double* vert, double* horiz); /* LargeStep::cfl_max */
 int orc_calc_p_rho_phi(orc_domain* d, int rk);
+int orc_step_end(orc_domain* d); /* spec_bdy_final, set_w_surface */
+/* specified boundaries: as mi_bdy_set / mi_bdy_set_time / mi_bdy_get_time */
+int orc_bdy_set(orc_domain* d, int var, int side, const double* values, const double* tends, int64_t n);
+int orc_bdy_set_time(orc_domain* d, double dtbc);
+int orc_bdy_get_time(orc_domain* d, double* dtbc);
 
 #ifdef __cplusplus
 }

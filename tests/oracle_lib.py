@@ -41,6 +41,10 @@ def olib():
             "orc_scalar_transport": (I, [VP, I]),
             "orc_cfl_max": (I, [VP, PD, PD]),
             "orc_calc_p_rho_phi": (I, [VP, I]),
+            "orc_step_end": (I, [VP]),
+            "orc_bdy_set": (I, [VP, I, I, PD, PD, C.c_int64]),
+            "orc_bdy_set_time": (I, [VP, C.c_double]),
+            "orc_bdy_get_time": (I, [VP, PD]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -129,3 +133,26 @@ class OracleDomain:
 
     def calc_p_rho_phi(self, rk):
         _ok(olib().orc_calc_p_rho_phi(self._h, rk))
+
+    def step_end(self):
+        _ok(olib().orc_step_end(self._h))
+
+    def bdy_set(self, var, side, values, tends):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        t = np.ascontiguousarray(tends, dtype=np.float64)
+        _ok(olib().orc_bdy_set(self._h, var, side, _ptr(v), _ptr(t), v.size))
+
+    def load_boundary(self, bdy):
+        """every slab of a {(var, side): (values, tendencies)} dict"""
+        for (var, side), (v, t) in bdy.items():
+            self.bdy_set(var, side, v, t)
+
+    @property
+    def bdy_time(self):
+        d = C.c_double()
+        _ok(olib().orc_bdy_get_time(self._h, C.byref(d)))
+        return d.value
+
+    @bdy_time.setter
+    def bdy_time(self, v):
+        _ok(olib().orc_bdy_set_time(self._h, float(v)))
