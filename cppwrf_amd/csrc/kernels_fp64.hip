@@ -191,26 +191,47 @@ __global__ void k_halo_y(Geo g, HaloSet s) {
 // included) and muv (odd in y), as the reference's HaloField parities.
 enum HaloKind : int { kHMass = 0, kHU = 1, kHV = 2, kHMuu = 3, kHMuv = 4 };
 
-// emit(index, negate) for the point itself and every slot of one direction that copies index i
+// emit(index, negate) for the point itself and every slot of one direction that copies index i.  The
+// images are the solutions sl of src(sl) == i over the halo slots of the fill above, enumerated directly
+// (n >= 3 periodic, n >= 4 symmetric, as the domain requires, so each slot has one source):
+//   periodic   i + n for i <= 2 (faces: i <= 3, and i + 2n within the slots when n = 3) and i - n for
+//              i >= n - 3; a face's point n is itself the image of point 0
+//   symmetric  -1 - i for i <= 2 and 2n - 1 - i for i >= n - 3 (faces: -i for 1 <= i <= 3, 2n - i for
+//              n - 3 <= i <= n - 1), odd fields negated; an OddFace wall face negated in place
+//   specified  the outermost point copied to the three slots beyond it
 template <class F>
 __device__ __forceinline__ void halo_images(int i, int n, bool face, int bc, int par, F&& emit) {
-  const bool periodic = bc == MI_BC_PERIODIC, flat = bc == MI_BC_SPECIFIED;
-  const bool mirror = !periodic && !flat;  // symmetric: odd fields change sign
-  // the point itself, unless it is a periodic face slot (index n takes index 0's value); a symmetric
-  // wall face of an OddFace field changes sign in place
-  if (!(periodic && face && i == n)) emit(i, mirror && face && par == 2 && (i == 0 || i == n));
-  if (i >= 4 && i + 4 < n) return;
-  const int nl = face ? 4 : 3, ns = face ? 8 : 6, last = face ? n : n - 1;
-#pragma unroll 1
-  for (int a = 0; a < ns; ++a) {
-    const int sl = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
-    if (sl == i) continue;
-    int src;
-    if (flat) src = sl < 0 ? 0 : (sl > last ? last : sl);
-    else if (periodic) src = ((sl % n) + n) % n;
-    else if (face) src = sl <= 0 ? -sl : 2 * n - sl;
-    else src = sl < 0 ? -1 - sl : 2 * n - 1 - sl;
-    if (src == i) emit(sl, mirror && par != 0);
+  if (bc == MI_BC_PERIODIC) {
+    if (!(face && i == n)) emit(i, false);
+    if (i <= (face ? 3 : 2)) emit(i + n, false);
+    if (face && i + 2 * n <= n + 3) emit(i + 2 * n, false);  // n = 3: slot n + 3 wraps twice
+    if (i >= n - 3 && i < n) emit(i - n, false);
+    return;
+  }
+  if (bc == MI_BC_SPECIFIED) {
+    emit(i, false);
+    const int last = face ? n : n - 1;
+    if (i == 0) {
+      emit(-1, false);
+      emit(-2, false);
+      emit(-3, false);
+    }
+    if (i == last) {
+      emit(last + 1, false);
+      emit(last + 2, false);
+      emit(last + 3, false);
+    }
+    return;
+  }
+  const bool odd = par != 0;
+  if (face) {
+    emit(i, par == 2 && (i == 0 || i == n));
+    if (i >= 1 && i <= 3) emit(-i, odd);
+    if (i >= n - 3 && i <= n - 1) emit(2 * n - i, odd);
+  } else {
+    emit(i, false);
+    if (i <= 2) emit(-1 - i, odd);
+    if (i >= n - 3) emit(2 * n - 1 - i, odd);
   }
 }
 
