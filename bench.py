@@ -7,7 +7,10 @@ Workload "hills" (default): the real-data-shaped 119x89x44 domain at 15 km (seed
 and 30-150 km wide, US standard atmosphere, seeded u/v jets of 5-40 m/s with +-1 m/s noise, map factors
 in [0.95, 1.05], Coriolis, a hybrid coordinate with a 50 hPa top, six moist species advected with the
 positive-definite limiter, qv from 80 % relative humidity falling to 10 % at 10 km), dt 90 s with 4
-acoustic steps, Rayleigh damping of w in the top 5 km.  Its lateral boundaries are periodic.
+acoustic steps, Rayleigh damping of w in the top 5 km.  Its lateral boundaries are specified: a 5-point
+zone (1 spec + 4 relaxation rows) nudged toward boundary data that start at the initial state and drift
+with a seeded 6-hour oscillation of the winds, theta and qv (cppwrf_amd.boundary.drifting_boundary); the
+other moist species take flow-dependent boundary values.
 Workload "bubble64": the 64x64x40 dry warm bubble at 1 km (dt 6 s, 6 acoustic steps, damp_opt 3).
 
 A step is one RK3 large step (three stages and their acoustic substeps, scalar transport included).  The
@@ -51,22 +54,28 @@ def log(*a):
 
 
 def workload(name):
+    """(case, parameters, description, boundary data or None)"""
+    from cppwrf_amd import _abi as A
+    from cppwrf_amd.boundary import drifting_boundary
     from cppwrf_amd.dyn import Case
     if name == "bubble64":
         case = Case.bubble3d(64, 64, 40, dx=1000.0, ztop=20000.0, dt=6.0, seed=1, noise_amp=0.05)
         kw = dict(time_step_sound=6, damp_opt=3, zdamp=5000.0, dampcoef=0.2)
         desc = {"workload": "dry warm bubble 64x64x40, 1 km, 20 km top, periodic, damp_opt 3, dt 6 s",
                 "nx": 64, "ny": 64, "nz": 40, "dx_m": 1000.0, "dt_s": 6.0, "time_step_sound": 6}
+        bdy = None
     else:
-        case = Case.hills(119, 89, 44, dx=15000.0, dt=90.0, seed=1)
+        case = Case.hills(119, 89, 44, dx=15000.0, dt=90.0, seed=1, bc=A.MI_BC_SPECIFIED)
         kw = dict(time_step_sound=4, damp_opt=3, zdamp=5000.0, dampcoef=0.2, n_moist=6, n_scalars=6,
                   moist_adv_opt=1)
         desc = {"workload": "real-data-shaped 119x89x44 at 15 km: 12 seeded hills, US standard atmosphere, jets, "
                             "map factors, Coriolis, hybrid coordinate (50 hPa top), 6 moist species with the PD "
-                            "limiter, dt 90 s, 4 acoustic steps; periodic lateral boundaries",
+                            "limiter, dt 90 s, 4 acoustic steps; specified lateral boundaries (5-point relaxation "
+                            "zone, seeded slowly varying boundary data)",
                 "nx": 119, "ny": 89, "nz": 44, "dx_m": 15000.0, "dt_s": 90.0, "time_step_sound": 4,
-                "moist_species": 6, "lateral_bc": "periodic"}
-    return case, kw, desc
+                "moist_species": 6, "lateral_bc": "specified"}
+        bdy = drifting_boundary(case, seed=17)
+    return case, kw, desc, bdy
 
 
 def profile(rk, name, steps):
@@ -100,12 +109,89 @@ def error_report(d32, d64, n_moist):
     return out
 
 
-def cpu_baseline(case, prm, budget_s=15.0):
+def ensemble_bench(ctx, members=32, steps=10, seed=2024):
+    """The batched ensemble: `members` domains of uneven sizes (nx 60-220, ny 60-180, nz 35/44/50, dx 3-15 km),
+    each the 3-D warm bubble with its own seed plus +-0.5 K theta and +-1 m/s wind perturbations, dt = 6 s per
+    km of dx, 6 acoustic steps, damp_opt 3; all members advance in one HIP graph per step (mi_batch).  The same
+    ensemble is stepped in FP64 and in FP32-bulk; the FP32 error is the largest relative L2 over the members."""
+    from cppwrf_amd import _abi as A
+    from cppwrf_amd.dyn import Batch, Case, RK3
+    rng = np.random.default_rng(seed)
+    specs = []
+    for m in range(members):
+        nx, ny = int(rng.integers(60, 221)), int(rng.integers(60, 181))
+        nz, dx = int(rng.choice([35, 44, 50])), float(rng.uniform(3000.0, 15000.0))
+        specs.append((nx, ny, nz, dx, int(rng.integers(1 << 30))))
+    pts = sum(nx * ny * nz for nx, ny, nz, _, _ in specs)
+    out, states = {"members": members, "steps": steps, "grid_points": pts}, {}
+    for tag, prec in (("fp64", A.MI_FP64), ("fp32", A.MI_FP32_BULK)):
+        doms = []
+        for nx, ny, nz, dx, sd in specs:
+            case = Case.bubble3d(nx, ny, nz, dx=dx, ztop=20000.0, dt=6.0 * dx / 1000.0, seed=sd, noise_amp=0.05)
+            rk = RK3(ctx, case.grid, case.params(time_step_sound=6, damp_opt=3, zdamp=5000.0, precision=prec), case)
+            prng = np.random.default_rng(sd)
+            for fid, amp in ((A.MI_T, 0.5), (A.MI_U, 1.0), (A.MI_V, 1.0)):
+                f = case.field(fid)
+                rk.set_field(fid, f + prng.uniform(-amp, amp, size=f.shape))
+            doms.append(rk)
+        b = Batch(ctx, doms)
+        b.step(1)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        dt = time.perf_counter() - t0
+        out[tag] = {"value": pts * steps / dt, "unit": "grid-point-steps/s", "ms_per_step": dt * 1e3 / steps}
+        states[tag] = [{fid: d.field(fid) for fid in (A.MI_U, A.MI_V, A.MI_W, A.MI_T)} for d in doms]
+        b.close()
+        for d in doms:
+            d.close()
+    worst = {}
+    for name, fid in (("U", A.MI_U), ("V", A.MI_V), ("W", A.MI_W), ("T", A.MI_T)):
+        worst[name] = max(float(np.linalg.norm(a[fid] - b[fid]) / np.linalg.norm(b[fid]))
+                          for a, b in zip(states["fp32"], states["fp64"]))
+    out["speedup_fp32_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    out["fp32_max_rel_l2_over_members"] = worst
+    return out
+
+
+def serving_bench(ctx, seconds=20.0, rate=2.0, scale=0.5, seed=1):
+    """The serving trace (Poisson arrivals, domains 50-300 x 50-300 x 35/44/50 at 3-15 km, forecasts of 1-6 h,
+    the warm-bubble recipe per request) replayed in real time through the continuous-batching server in
+    FP32-bulk, horizontal sizes and forecast lengths scaled by `scale`; 5 % of the requests (at least one) are
+    rerun alone in FP64 for the error of the served FP32-bulk results."""
+    from cppwrf_amd import _abi as A
+    from cppwrf_amd.dyn import RK3
+    from cppwrf_amd.serving import poisson_trace, run_trace
+    reqs = poisson_trace(seconds, rate, seed, scale, A.MI_FP32_BULK)
+    keep = {r.rid for r in reqs[:: max(1, len(reqs) // max(1, len(reqs) // 20))]}
+    res = run_trace(reqs, 200 << 30, ctx=ctx, keep=keep)
+    errs = {}
+    for r in reqs:
+        if r.result is None:
+            continue
+        r.precision = A.MI_FP64
+        case = r.case()
+        d = RK3(ctx, case.grid, r.params(case), case)
+        d.step(r.steps)
+        for name, fid in (("U", A.MI_U), ("V", A.MI_V), ("W", A.MI_W), ("T", A.MI_T), ("MU", A.MI_MU)):
+            b = d.field(fid)
+            e = float(np.linalg.norm(r.result[fid] - b) / max(np.linalg.norm(b), 1e-300))
+            errs[name] = max(errs.get(name, 0.0), e)
+        d.close()
+    res.update({"trace_seconds": seconds, "rate_per_s": rate, "scale": scale, "precision": "fp32-bulk",
+                "fp64_reruns": len(keep), "fp32_max_rel_l2_vs_fp64_reruns": errs})
+    return res
+
+
+def cpu_baseline(case, prm, bdy, budget_s=15.0):
     """The oracle (C++/OpenMP, bit-identical to the reference CPU FP64 build) on this host's cores."""
     from oracle_lib import OracleDomain, olib
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     olib().orc_set_threads(threads)
     orc = OracleDomain(case, prm)
+    if bdy:
+        orc.load_boundary(bdy)
     t = time.perf_counter()
     orc.step(1)
     one = time.perf_counter() - t
@@ -127,6 +213,8 @@ def main():
     ap.add_argument("--workload", choices=["hills", "bubble64"], default="hills")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default block shapes")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-member batched ensemble")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving trace")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each forecast domain runs on a "
@@ -136,12 +224,14 @@ def main():
     from cppwrf_amd import _abi as A
     from cppwrf_amd.dyn import Context, RK3
 
-    case, kw, desc = workload(args.workload)
+    case, kw, desc, bdy = workload(args.workload)
     g = case.grid
     ctx = Context(0)
     doms = {}
     for tag, prec in (("fp64", A.MI_FP64), ("fp32", A.MI_FP32_BULK)):
         rk = RK3(ctx, g, case.params(precision=prec, **kw), case)
+        if bdy:
+            rk.load_boundary(bdy)
         if not args.no_autotune:
             log(f"{tag}: autotuned block shapes", rk.autotune(1))  # steps the domain, then restores its state
         doms[tag] = rk
@@ -196,8 +286,16 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": b / nl,
                      "avg_launch_us": avg_launch_s * 1e6, "launches": nl, "precision": "fp32-bulk"},
     }
+    for d in doms.values():
+        d.close()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(ctx)
+        log("ensemble:", json.dumps(out["ensemble"]))
+    if not args.no_serving:
+        out["serving"] = serving_bench(ctx)
+        log("serving:", json.dumps(out["serving"]))
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, case.params(**kw))
+        out["cpu_baseline"] = cpu_baseline(case, case.params(**kw), bdy)
     print(json.dumps(out), flush=True)
 
 

@@ -236,6 +236,7 @@ struct Domain final : mi_domain {
       // with the halos of mu'', mudf, muts, ph'', p, al, and of ru_m, rv_m, ww_m (their refresh after
       // the loop; the sums' halo images are rewritten at every substep, the last one's stay)
       launch_acoustic(a, colcfg[kColAcoustic], dts, it, ns, st);
+      if (spec) launch_acoustic_ring(a, dts, it, ns, st);  // the spec zone's columns
     }
     if (spec) launch_bdy_zero_grad(a, st);  // zero_grad_bdy of the spec zone's w (lateral_bc.cpp:374-387)
     // with the halos of u, v, w, t, ph, ww, mu (and p, al); the microphysics heating leaves theta_m at
@@ -496,7 +497,7 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   d.prm = p;
   d.n_sound_ = p.time_step_sound > 0 ? p.time_step_sound : mi_auto_sound_steps(g.dt, g.dx, g.dy);
   if (d.n_sound_ < 2 || d.n_sound_ % 2) throw Error(MI_EINVAL, "time_step_sound must be even and >= 2");
-  if (!col_cfg_fits(kColAcoustic, kNumColCfgs - 1, g.nz)) throw Error(MI_EINVAL, "nz must be at most 255");
+  if (!col_cfg_fits(kColAcoustic, kColCfgTallest, g.nz)) throw Error(MI_EINVAL, "nz must be at most 255");
   if (const char* e = std::getenv("MI_GRAPHS")) d.use_graph = std::strcmp(e, "0") != 0;
 
   const bool two_d = g.ny == 1;

@@ -53,6 +53,10 @@ inline dim3 grid_of(const Range& r, int by) {
   return dim3(unsigned((r.i1 - r.i0 + kBx - 1) / kBx), unsigned((r.j1 - r.j0 + by - 1) / by), unsigned(r.k1 - r.k0));
 }
 
+// point kernels run 64 x 4 threads per block (launch3a): the bound lets the heavier ones keep their
+// registers instead of spilling
+#define MI_POINT_KERNEL __global__ void __launch_bounds__(256)
+
 #define MI_POINT_INDEX(r)                          \
   const Blk blk_ = xcd_block();                    \
   const int i = (r).i0 + blk_.x * kBx + int(threadIdx.x); \
@@ -191,47 +195,35 @@ __global__ void k_halo_y(Geo g, HaloSet s) {
 // included) and muv (odd in y), as the reference's HaloField parities.
 enum HaloKind : int { kHMass = 0, kHU = 1, kHV = 2, kHMuu = 3, kHMuv = 4 };
 
-// emit(index, negate) for the point itself and every slot of one direction that copies index i.  The
-// images are the solutions sl of src(sl) == i over the halo slots of the fill above, enumerated directly
-// (n >= 3 periodic, n >= 4 symmetric, as the domain requires, so each slot has one source):
-//   periodic   i + n for i <= 2 (faces: i <= 3, and i + 2n within the slots when n = 3) and i - n for
-//              i >= n - 3; a face's point n is itself the image of point 0
-//   symmetric  -1 - i for i <= 2 and 2n - 1 - i for i >= n - 3 (faces: -i for 1 <= i <= 3, 2n - i for
-//              n - 3 <= i <= n - 1), odd fields negated; an OddFace wall face negated in place
-//   specified  the outermost point copied to the three slots beyond it
+// emit(index, negate) for the point itself and every slot of one direction that copies index i: the slots
+// whose source in the fill above is i.  Points 4 or more from both edges have none.  The slot loop stays a
+// loop (compact code: the kernels' register use does not grow with the number of stores), and the periodic
+// source is folded without integer division (slots lie in [-3, n + 3], n >= 3).
 template <class F>
 __device__ __forceinline__ void halo_images(int i, int n, bool face, int bc, int par, F&& emit) {
-  if (bc == MI_BC_PERIODIC) {
-    if (!(face && i == n)) emit(i, false);
-    if (i <= (face ? 3 : 2)) emit(i + n, false);
-    if (face && i + 2 * n <= n + 3) emit(i + 2 * n, false);  // n = 3: slot n + 3 wraps twice
-    if (i >= n - 3 && i < n) emit(i - n, false);
-    return;
-  }
-  if (bc == MI_BC_SPECIFIED) {
-    emit(i, false);
-    const int last = face ? n : n - 1;
-    if (i == 0) {
-      emit(-1, false);
-      emit(-2, false);
-      emit(-3, false);
+  const bool periodic = bc == MI_BC_PERIODIC, flat = bc == MI_BC_SPECIFIED;
+  const bool mirror = !periodic && !flat;  // symmetric: odd fields change sign
+  // the point itself, unless it is a periodic face slot (index n takes index 0's value); a symmetric
+  // wall face of an OddFace field changes sign in place
+  if (!(periodic && face && i == n)) emit(i, mirror && face && par == 2 && (i == 0 || i == n));
+  if (i >= 4 && i + 4 < n) return;
+  const int nl = face ? 4 : 3, ns = face ? 8 : 6, last = face ? n : n - 1;
+#pragma unroll 1
+  for (int a = 0; a < ns; ++a) {
+    const int sl = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
+    if (sl == i) continue;
+    int src;
+    if (flat) {
+      src = sl < 0 ? 0 : (sl > last ? last : sl);
+    } else if (periodic) {
+      src = sl < 0 ? sl + n : sl;
+      while (src >= n) src -= n;
+    } else if (face) {
+      src = sl <= 0 ? -sl : 2 * n - sl;
+    } else {
+      src = sl < 0 ? -1 - sl : 2 * n - 1 - sl;
     }
-    if (i == last) {
-      emit(last + 1, false);
-      emit(last + 2, false);
-      emit(last + 3, false);
-    }
-    return;
-  }
-  const bool odd = par != 0;
-  if (face) {
-    emit(i, par == 2 && (i == 0 || i == n));
-    if (i >= 1 && i <= 3) emit(-i, odd);
-    if (i >= n - 3 && i <= n - 1) emit(2 * n - i, odd);
-  } else {
-    emit(i, false);
-    if (i <= 2) emit(-1 - i, odd);
-    if (i >= n - 3) emit(2 * n - 1 - i, odd);
+    if (src == i) emit(sl, mirror && par != 0);
   }
 }
 
@@ -434,7 +426,7 @@ __device__ __forceinline__ int ord_mass(int f, int n, bool on) {
   return 5;
 }
 
-__global__ void k_tend_u(Range r, Args a) {
+MI_POINT_KERNEL k_tend_u(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -505,7 +497,7 @@ __global__ void k_tend_u(Range r, Args a) {
 }
 
 // KTendV (large_step.cpp:267-376)
-__global__ void k_tend_v(Range r, Args a) {
+MI_POINT_KERNEL k_tend_v(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -576,7 +568,7 @@ __global__ void k_tend_v(Range r, Args a) {
 }
 
 // KTendW (large_step.cpp:379-499), w levels 1..nz
-__global__ void k_tend_w(Range r, Args a) {
+MI_POINT_KERNEL k_tend_w(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -647,7 +639,7 @@ __global__ void k_tend_w(Range r, Args a) {
 }
 
 // KTendT (large_step.cpp:502-565)
-__global__ void k_tend_t(Range r, Args a) {
+MI_POINT_KERNEL k_tend_t(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -690,7 +682,7 @@ __global__ void k_tend_t(Range r, Args a) {
 }
 
 // KTendPh rhs_ph (large_step.cpp:570-668), w levels 1..nz, 6th-order horizontal gradient
-__global__ void k_tend_ph(Range r, Args a) {
+MI_POINT_KERNEL k_tend_ph(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -772,7 +764,7 @@ __global__ void k_tend_ph(Range r, Args a) {
 
 // ----------------------------------------------------------------------------- small_step_prep
 // KPrep (reference src/dyn/small_step.cpp:39-93)
-__global__ void k_ss_prep(Range r, Args a, int first) {
+MI_POINT_KERNEL k_ss_prep(Range r, Args a, int first) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -912,7 +904,7 @@ MI_COLUMN_KERNEL k_ss_column(Args a, double cof) {
 
 // ----------------------------------------------------------------------------- advance_uv
 // KUV (small_step.cpp:151-190)
-__global__ void k_uv(Range r, Args a, double dts) {
+MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -978,12 +970,8 @@ __global__ void k_uv(Range r, Args a, double dts) {
 //   7  KWSolve's tridiagonal sweeps (small_step.cpp:386-387), one thread
 //   8  KWSolve's damp_opt 3 damping and phi'' (small_step.cpp:388-406), parallel over levels
 //   9  KWPRho (small_step.cpp:409-414)
-// With specified boundaries a spec-zone column (Edge::mass, small_step.cpp:39-45) takes LateralBC's
-// updates instead: mu, muts and theta'' += dts * tendency (update_mu_t, lateral_bc.cpp:366-372), omega''
-// left as it is but still summed into ww_m (small_step.cpp:214-220), no advance_w, and phi'' by
-// spec_bdyupdate_ph (lateral_bc.cpp:133-144); its w is the zero-gradient copy of the next row in,
-// written once after the last substep (k_bdy_zero_grad).  The east and north domain faces of ru_m, rv_m
-// are accumulated by the last column and row (KSumEdge, small_step.cpp:279-290).
+// With specified boundaries the spec zone's columns (Edge::mass, small_step.cpp:39-45) are skipped here and
+// advanced by k_acoustic_ring.
 MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   extern __shared__ double lds[];
   const Geo& g = a.g;
@@ -991,7 +979,9 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   const Lv& c = a.c;
   const ColPos cp = col_pos<CI>();
   const int i = cp.i, j = cp.j, nz = g.nz, ci = cp.ci, k = cp.kt;
-  const bool on = i < g.nx && k <= nz;
+  // with specified boundaries the spec zone's columns are left to k_acoustic_ring
+  const bool ring = g.bcx == MI_BC_SPECIFIED && (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1);
+  const bool on = i < g.nx && k <= nz && !ring;
   const int L = (nz + 1) * CI;
   double* DV = lds;     // dnw*dvdxi(k), then dvdxi(k)
   double* TK = DV + L;  // the omega'' terms, then the integral
@@ -1010,9 +1000,6 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   const idx_t sk = g.sk, sj = g.sj;
   const idx_t x2 = g.at2(i, j), x = g.at(i, j, k);
   const bool mk = k < nz, ge1 = k >= 1;  // a mass level; above the surface
-  const bool sp = g.bcx == MI_BC_SPECIFIED;
-  const bool ring = sp && (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1);
-  const bool east = sp && i == g.nx - 1, north = sp && j == g.ny - 1;
 
   // ---------------------------------------------------------------- the load burst
   double u0 = 0, u1 = 0, us0 = 0, us1 = 0, v0 = 0, v1 = 0, vs0 = 0, vs1 = 0;
@@ -1021,7 +1008,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   double phtend = 0, w2 = 0, phs = 0, phs_m = 0, phs_p = 0, ph = 0, ph_m = 0, ph_p = 0, rwt = 0;
   double c2a = 0, c2a_m = 0, alt = 0, alt_m = 0, wsv = 0, pm1 = 0;
   double msfty = 1, msftx = 1, mut = 0, muu0 = 0, muu1 = 0, muv0 = 0, muv1 = 0, msfuy0 = 1, msfuy1 = 1, mvi0 = 1, mvi1 = 1;
-  double mu_old = 0, mtend = 0, ph_top = 0, muts_old = 0, wwcur = 0, rum1 = 0, rvm1 = 0;
+  double mu_old = 0, mtend = 0, ph_top = 0;
   if (on) {
     msfty = f.msfty[x2];
     msftx = f.msftx[x2];
@@ -1039,9 +1026,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     if (k == 0) {
       mu_old = f.mu2[x2];
       mtend = f.mu_tend[x2];
-      if (ring) muts_old = f.muts[x2];
     }
-    if (ring) wwcur = f.ww[x];
     ww1 = f.ww1[x];
     if (!first) wwm = f.ww_m[x];
     ca = f.ca[x];
@@ -1074,8 +1059,6 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       if (!first) {
         rum = f.ru_m[x];
         if (!g.two_d) rvm = f.rv_m[x];
-        if (east) rum1 = f.ru_m[x + 1];
-        if (north) rvm1 = f.rv_m[x + sj];
       }
       t2 = f.t2[x];
       ttend = f.t_tend[x];
@@ -1110,17 +1093,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 2: the column mass
-  if (on && k == 0 && ring) {  // the spec zone: mu and muts += dts * mu_tend
-    const double mu_new = mu_old + dts * mtend;
-    const double muts = muts_old + dts * mtend;
-    put<kHMass>(f.mu2, g, i, j, 0, mu_new);
-    put<kHMass>(f.muts, g, i, j, 0, muts);
-    S2[ci] = 0.0;
-    S2[CI + ci] = mu_new;
-    S2[2 * CI + ci] = muts;
-    S2[3 * CI + ci] = 0.0;
-    S2[4 * CI + ci] = mtend;
-  } else if (on && k == 0) {
+  if (on && k == 0) {
     double dmdt = 0;
     for (int kk = 0; kk < nz; ++kk) dmdt += LDS(DV, kk, ci);
     const double mu_new = mu_old + dts * (dmdt + mtend);
@@ -1139,13 +1112,13 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   __syncthreads();
   const double dmdt = S2[ci], mu_new = S2[CI + ci], muts = S2[2 * CI + ci], muave = S2[3 * CI + ci];
   // ---------------------------------------------------------------- 3: the omega'' terms
-  if (on && !ring && k + 1 < nz) {
+  if (on && k + 1 < nz) {
     const double mt = S2[4 * CI + ci];
     LDS(TK, k, ci) = c.dnw[k] * (c.c1h[k] * dmdt + dv + c.c1h[k] * mt) / msfty;
   }
   __syncthreads();
   // ---------------------------------------------------------------- 4: the running integral
-  if (on && !ring && k == 0) {
+  if (on && k == 0) {
     double ww_full = 0;
     for (int kk = 0; kk + 1 < nz; ++kk) {
       ww_full = ww_full - LDS(TK, kk, ci);
@@ -1156,19 +1129,12 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   // ---------------------------------------------------------------- 5: omega'', theta'', explicit phi''
   double wwk = 0;
   if (on) {
-    wwk = (!ring && k >= 1 && k < nz) ? LDS(TK, k - 1, ci) - ww1 : 0.0;
+    wwk = (k >= 1 && k < nz) ? LDS(TK, k - 1, ci) - ww1 : 0.0;
     LDS(WW, k, ci) = wwk;
   }
   __syncthreads();
   double t2new = 0;
-  if (on && ring) {  // the spec zone: sumflux of ww as it stands, theta'' += dts * t_tend
-    const double sw = (first ? 0.0 : wwm) + wwcur;
-    put<kHMass>(f.ww_m, g, i, j, k * sk, last ? sw / rn + ww1 : sw);
-    if (mk) {
-      t2new = t2 + dts * ttend;
-      f.t2[x] = t2new;
-    }
-  } else if (on) {
+  if (on) {
     f.ww[x] = wwk;
     {
       const double sw = (first ? 0.0 : wwm) + wwk;
@@ -1178,6 +1144,12 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       // wdtn(k) and wdtn(k+1): omega'' times theta at the w level (0 at the bottom and the top)
       const double wdtn_k = (k == 0) ? 0.0 : wwk * (c.fnm[k] * ts + c.fnp[k] * tskm);
       const double wdtn_kp = (k + 1 == nz) ? 0.0 : LDS(WW, k + 1, ci) * (c.fnm[k + 1] * tskp + c.fnp[k + 1] * ts);
+      const double su = (first ? 0.0 : rum) + u0;
+      put<kHU>(f.ru_m, g, i, j, k * sk, last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su);
+      if (!g.two_d) {
+        const double sv = (first ? 0.0 : rvm) + v0;
+        put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
+      }
       double hflux = H * c.rdx * (u1 * (tsxp + ts) - u0 * (ts + tsxm));
       if (!g.two_d) hflux += H * c.rdy * (v1 * (tsyp + ts) - v0 * (ts + tsym));
       const double tn = t2 + msfty * dts * ttend;
@@ -1201,25 +1173,9 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     }
     LDS(R, k, ci) = Rk;
   }
-  if (on && mk) {  // sumflux of ru, rv (KMuTPoint), and of the east and north domain faces (KSumEdge)
-    const double su = (first ? 0.0 : rum) + u0;
-    put<kHU>(f.ru_m, g, i, j, k * sk, last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su);
-    if (!g.two_d) {
-      const double sv = (first ? 0.0 : rvm) + v0;
-      put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
-    }
-    if (east) {
-      const double se = (first ? 0.0 : rum1) + u1;
-      put<kHU>(f.ru_m, g, i + 1, j, k * sk, last ? se / rn + (c.c1h[k] * muu1 + c.c2h[k]) * us1 / msfuy1 : se);
-    }
-    if (north) {
-      const double sn = (first ? 0.0 : rvm1) + v1;
-      put<kHV>(f.rv_m, g, i, j + 1, k * sk, last ? sn / rn + (c.c1h[k] * muv1 + c.c2h[k]) * vs1 * mvi1 : sn);
-    }
-  }
   __syncthreads();
   // ---------------------------------------------------------------- 6: explicit w''
-  if (on && !ring) {
+  if (on) {
     const double m = mut, mav = muave;
     const double ope = 1.0 + epssm, ome = 1.0 - epssm;
     const double msft_inv = 1.0 / msfty;
@@ -1263,7 +1219,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 7: tridiagonal sweeps
-  if (on && !ring && k == 0) {
+  if (on && k == 0) {
     double wprev = LDS(W, 0, ci);
     for (int kk = 1; kk <= nz; ++kk) {
       wprev = (LDS(W, kk, ci) - LDS(AA, kk, ci) * wprev) * LDS(AL, kk, ci);
@@ -1277,14 +1233,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 8: damping near the top, phi''
-  if (on && ring) {  // spec_bdyupdate_ph, every level
-    const double mts = S2[2 * CI + ci], mt = S2[4 * CI + ci];
-    const double mu_o = mts - dts * mt;
-    const double m_old = c.c1f[k] * mu_o + c.c2f[k], m_new = c.c1f[k] * mts + c.c2f[k];
-    const double phk = ph * m_old / m_new + dts * phtend / m_new + phs * (m_old / m_new - 1.0);
-    put<kHMass>(f.ph2, g, i, j, k * sk, phk);
-    LDS(PH, k, ci) = phk;
-  } else if (on) {
+  if (on) {
     double wk = LDS(W, k, ci);
     double phk = ph;  // the surface phi'' is not solved for
     if (k >= 1) {
@@ -1312,6 +1261,129 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   if (on && mk) {
     const double m = c.c1h[k] * muts + c.c2h[k];
     const double al = -((1. / m) * (alt * (c.c1h[k] * mu_new) + c.rdnw[k] * (LDS(PH, k + 1, ci) - LDS(PH, k, ci))));
+    put<kHMass>(f.al, g, i, j, k * sk, al);
+    double pp = c2a * (alt * (t2new - c.c1h[k] * mu_new * ts) / (m * (kT0 + ts)) - al);
+    const double ptmp = pp;
+    pp = pp + a.q.smdiv * (pp - pm1);
+    f.pm1[x] = ptmp;
+    put<kHMass>(f.p, g, i, j, k * sk, pp);
+  }
+}
+
+// ----------------------------------------------------------------------------- the spec zone's substep
+// One acoustic substep of the spec zone's columns with specified boundaries (a block per column, a thread per
+// level), in the reference's order (SmallStep::run_stage, small_step.cpp:548-593, with LateralBC's updates,
+// lateral_bc.cpp:356-387):
+//   KMuTCol's ring branch: omega'' left as it stands, summed into ww_m (small_step.cpp:214-220)
+//   KMuTPoint's ru_m, rv_m sums (theta'' not advanced there) and KSumEdge's east and north domain faces
+//   update_mu_t: theta'', mu'' and muts += dts * tendency
+//   spec_bdyupdate_ph (KUpdatePh): phi'' of every level from the stage masses; w waits for zero_grad_bdy
+//   KWPRho: p'', al'' with divergence damping
+// The interior columns do not read these columns within a substep (every acoustic kernel reads its own
+// column, and the spec zone's faces were advanced by k_uv before), so the two kernels are independent.
+template <int KT>
+__global__ void __launch_bounds__(KT) k_acoustic_ring(Args a, double dts, int it, int ns) {
+  __shared__ double PH[KT + 1];
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  int col = int(blockIdx.x), i, j;  // S row, N row, W column, E column
+  if (col < nx) {
+    i = col;
+    j = 0;
+  } else if ((col -= nx) < nx) {
+    i = col;
+    j = ny - 1;
+  } else if ((col -= nx) < ny - 2) {
+    i = 0;
+    j = 1 + col;
+  } else {
+    i = nx - 1;
+    j = 1 + col - (ny - 2);
+  }
+  const int k = int(threadIdx.x);
+  const bool on = k <= nz, mk = k < nz;
+  const bool first = it == 1, last = it == ns;
+  const double rn = double(ns);
+  const idx_t sk = g.sk, sj = g.sj, x2 = g.at2(i, j), x = g.at(i, j, on ? k : nz);
+  const double mu_old = f.mu2[x2], mtend = f.mu_tend[x2], muts_old = f.muts[x2];
+  double wwcur = 0, wwm = 0, ww1 = 0, ph = 0, phtend = 0, phs = 0;
+  double u0 = 0, us0 = 0, v0 = 0, vs0 = 0, rum = 0, rvm = 0, u1 = 0, us1 = 0, v1 = 0, vs1 = 0, rum1 = 0, rvm1 = 0;
+  double t2 = 0, ttend = 0, ts = 0, c2a = 0, alt = 0, pm1 = 0;
+  if (on) {
+    wwcur = f.ww[x];
+    if (!first) wwm = f.ww_m[x];
+    ww1 = f.ww1[x];
+    ph = f.ph2[x];
+    phtend = f.ph_tend[x];
+    phs = f.ph_save[x];
+    if (mk) {
+      u0 = f.u2[x];
+      us0 = f.u_save[x];
+      v0 = f.v2[x];
+      vs0 = f.v_save[x];
+      t2 = f.t2[x];
+      ttend = f.t_tend[x];
+      ts = f.t_save[x];
+      c2a = f.c2a[x];
+      alt = f.alt[x];
+      pm1 = f.pm1[x];
+      if (i == nx - 1) {
+        u1 = f.u2[x + 1];
+        us1 = f.u_save[x + 1];
+      }
+      if (j == ny - 1) {
+        v1 = f.v2[x + sj];
+        vs1 = f.v_save[x + sj];
+      }
+      if (!first) {
+        rum = f.ru_m[x];
+        rvm = f.rv_m[x];
+        if (i == nx - 1) rum1 = f.ru_m[x + 1];
+        if (j == ny - 1) rvm1 = f.rv_m[x + sj];
+      }
+    }
+  }
+  __syncthreads();  // every level has read mu'' and muts before they change
+  const double mu_new = mu_old + dts * mtend, muts = muts_old + dts * mtend;
+  if (k == 0) {
+    put<kHMass>(f.mu2, g, i, j, 0, mu_new);
+    put<kHMass>(f.muts, g, i, j, 0, muts);
+  }
+  double t2new = 0;
+  if (on) {
+    const double sw = (first ? 0.0 : wwm) + wwcur;
+    put<kHMass>(f.ww_m, g, i, j, k * sk, last ? sw / rn + ww1 : sw);
+    if (mk) {
+      const double muu0 = f.muu[x2], msfuy0 = f.msfuy[x2], muv0 = f.muv[x2], mvi0 = f.msfvx_inv[x2];
+      const double su = (first ? 0.0 : rum) + u0;
+      put<kHU>(f.ru_m, g, i, j, k * sk, last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su);
+      const double sv = (first ? 0.0 : rvm) + v0;
+      put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
+      if (i == nx - 1) {
+        const double se = (first ? 0.0 : rum1) + u1;
+        put<kHU>(f.ru_m, g, i + 1, j, k * sk,
+                 last ? se / rn + (c.c1h[k] * f.muu[x2 + 1] + c.c2h[k]) * us1 / f.msfuy[x2 + 1] : se);
+      }
+      if (j == ny - 1) {
+        const double sn = (first ? 0.0 : rvm1) + v1;
+        put<kHV>(f.rv_m, g, i, j + 1, k * sk,
+                 last ? sn / rn + (c.c1h[k] * f.muv[x2 + sj] + c.c2h[k]) * vs1 * f.msfvx_inv[x2 + sj] : sn);
+      }
+      t2new = t2 + dts * ttend;
+      f.t2[x] = t2new;
+    }
+    const double mu_o = muts - dts * mtend;
+    const double m_old = c.c1f[k] * mu_o + c.c2f[k], m_new = c.c1f[k] * muts + c.c2f[k];
+    const double phk = ph * m_old / m_new + dts * phtend / m_new + phs * (m_old / m_new - 1.0);
+    put<kHMass>(f.ph2, g, i, j, k * sk, phk);
+    PH[k] = phk;
+  }
+  __syncthreads();
+  if (mk) {
+    const double m = c.c1h[k] * muts + c.c2h[k];
+    const double al = -((1. / m) * (alt * (c.c1h[k] * mu_new) + c.rdnw[k] * (PH[k + 1] - PH[k])));
     put<kHMass>(f.al, g, i, j, k * sk, al);
     double pp = c2a * (alt * (t2new - c.c1h[k] * mu_new * ts) / (m * (kT0 + ts)) - al);
     const double ptmp = pp;
@@ -1384,7 +1456,7 @@ MI_COLUMN_KERNEL k_finish(Args a, int prhophi, double hd_dt) {
 
 // ----------------------------------------------------------------------------- calc_p_rho_phi
 // KPRhoPhi (small_step.cpp:456-477), hypsometric_opt = 1, glibc pow
-__global__ void k_p_rho_phi(Range r, Args a) {
+MI_POINT_KERNEL k_p_rho_phi(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1456,7 +1528,7 @@ __device__ __forceinline__ bool sixth_in(const Geo& g, int diff, char kind, int 
 // zero_tendf + DiffusionDriver::dynamics (diffusion_driver.cpp:266-281): the held tendencies of u, v, w,
 // theta (and ph with specified boundaries) over the (nx+1) x nyv x (nz+1) points, 0 where
 // sixth_order_diffusion does not reach (w levels 0, nz; the specified edges; everywhere without it)
-__global__ void k_sixth_dyn(Range r, Args a, double coef) {
+MI_POINT_KERNEL k_sixth_dyn(Range r, Args a, double coef) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1478,7 +1550,7 @@ __global__ void k_sixth_dyn(Range r, Args a, double coef) {
 
 // zero_tendf, the caller's tendency, then DiffusionDriver::scalar (diffusion_driver.cpp:297-327; mut there
 // is muts, dt the first stage's)
-__global__ void k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
+MI_POINT_KERNEL k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Lv& c = a.c;
@@ -1494,7 +1566,7 @@ __global__ void k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
 // ----------------------------------------------------------------------------- set_w_surface
 // KSetWSurface (rk3.cpp:173-190), fill = false: the kinematic surface w from the new u, v; with the
 // halo images of w (the reference refreshes w's halo next)
-__global__ void k_set_w_surface(Range r, Args a) {
+MI_POINT_KERNEL k_set_w_surface(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1515,7 +1587,7 @@ __global__ void k_set_w_surface(Range r, Args a) {
 
 // ----------------------------------------------------------------------------- calc_cq
 // KCq (reference src/dyn/large_step.cpp:127-148): per species, the two neighbours, WRF's order
-__global__ void k_cq(Range r, Args a, Scalars s) {
+MI_POINT_KERNEL k_cq(Range r, Args a, Scalars s) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1619,7 +1691,7 @@ __device__ __forceinline__ double m_full(const Lv& c, int k, double mub, double 
 
 // rk_update_scalar (KUpdate, scalars.cpp:145-158): advect_scalar's tendency and the mass-weighted update,
 // every species of the list; the new values go to the species' second buffer with their halo images
-__global__ void k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
+MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1645,7 +1717,7 @@ __global__ void k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
 
 // rk_update_scalar_pd, first step (KPreUpdatePD, scalars.cpp:174-181): the held tendency folded into
 // q_1, with its halo images (the reference refreshes q_1's halo next)
-__global__ void k_sc_pd_pre(Range r, Args a, Scalars s, SpList l, double dt_rk) {
+MI_POINT_KERNEL k_sc_pd_pre(Range r, Args a, Scalars s, SpList l, double dt_rk) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1687,7 +1759,7 @@ __device__ __forceinline__ PdFaces pd_faces(const ScFaces& fl, const Geo& g, int
 
 // advect_scalar_pd's limiter (KScalePD, scalars.cpp:220-248): the scale of each cell's outgoing
 // high-minus-low fluxes, with its halo images
-__global__ void k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk) {
+MI_POINT_KERNEL k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1722,7 +1794,7 @@ __global__ void k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk
 }
 
 // advect_scalar_pd's limited divergence and rk_update_scalar (KUpdatePD, scalars.cpp:252-291)
-__global__ void k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
+MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1931,7 +2003,7 @@ __global__ void k_copy_scalars(Scalars s, int64_t n3, int64_t off) {
 // LargeStep::cfl_max's pointwise Courant numbers (FVertCfl, FHorizCfl; large_step.cpp:672-687) over levels
 // 1..nz-1, reduced with an atomic max of the bit patterns: for non-negative doubles the integer order is the
 // numeric order, so the maximum is exact and independent of the order of the points; NaN counts as 0
-__global__ void k_cfl(Range r, Args a, unsigned long long* out) {
+MI_POINT_KERNEL k_cfl(Range r, Args a, unsigned long long* out) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1949,7 +2021,7 @@ __global__ void k_cfl(Range r, Args a, unsigned long long* out) {
 }
 
 // finiteness of the prognostic fields (flags[fld] = 1 when a NaN or an infinity is found)
-__global__ void k_finite(Range r, Args a, int* flags) {
+MI_POINT_KERNEL k_finite(Range r, Args a, int* flags) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -2009,7 +2081,12 @@ void dispatch_cfg(int cfg, T&&... args) {
     case 3: Launch<8, 32>::go(args...); break;
     case 4: Launch<8, 64>::go(args...); break;
     case 5: Launch<4, 128>::go(args...); break;
-    default: Launch<2, 256>::go(args...); break;
+    case 6: Launch<2, 256>::go(args...); break;
+    case 7: Launch<64, 4>::go(args...); break;
+    case 8: Launch<64, 8>::go(args...); break;
+    case 9: Launch<64, 16>::go(args...); break;
+    case 10: Launch<32, 8>::go(args...); break;
+    default: Launch<16, 64>::go(args...); break;
   }
 }
 
@@ -2140,6 +2217,16 @@ void launch_bdy(const Args& a, const Scalars& s, const BdySet& bs, BdyMode mode,
   if (t) time_begin(st);
   hipLaunchKernelGGL(k_bdy, dim3(unsigned((maxpos + 63) / 64), unsigned(g.nz + 1), unsigned(bs.n * 4 * nb)), dim3(64, 1, 1),
                      0, st, a, s, bs, int(mode), b0, nb);
+  if (t) time_end(st, 0.0);
+}
+void launch_acoustic_ring(const Args& a, double dts, int it, int ns, hipStream_t st) {
+  const Geo& g = a.g;
+  const unsigned ncol = unsigned(2 * g.nx + 2 * (g.ny - 2));
+  const bool t = timing("acoustic_ring");
+  if (t) time_begin(st);
+  if (g.nz + 1 <= 64) hipLaunchKernelGGL(k_acoustic_ring<64>, dim3(ncol), dim3(64), 0, st, a, dts, it, ns);
+  else if (g.nz + 1 <= 128) hipLaunchKernelGGL(k_acoustic_ring<128>, dim3(ncol), dim3(128), 0, st, a, dts, it, ns);
+  else hipLaunchKernelGGL(k_acoustic_ring<256>, dim3(ncol), dim3(256), 0, st, a, dts, it, ns);
   if (t) time_end(st, 0.0);
 }
 void launch_bdy_zero_grad(const Args& a, hipStream_t st) {

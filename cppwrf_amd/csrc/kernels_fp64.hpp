@@ -135,8 +135,11 @@ void launch_copy_step(const Args& a, int64_t n_alloc3, int64_t n_alloc2, hipStre
 void launch_fill(double* p, int64_t n, double v, hipStream_t st);
 // column kernels and their block shapes (columns x threads per column)
 enum ColKernel { kColLargePrep = 0, kColSsColumn = 1, kColAcoustic = 2, kColFinish = 3, kNumColKernels = 4 };
-constexpr int kNumColCfgs = 7;
-constexpr int kColCfgs[kNumColCfgs][2] = {{16, 16}, {16, 32}, {32, 16}, {8, 32}, {8, 64}, {4, 128}, {2, 256}};
+constexpr int kNumColCfgs = 12;
+// 64 or 32 columns per block: a wavefront reads 64 (32) consecutive points of one level (two levels)
+constexpr int kColCfgs[kNumColCfgs][2] = {{16, 16}, {16, 32}, {32, 16}, {8, 32}, {8, 64}, {4, 128}, {2, 256},
+                                          {64, 4}, {64, 8}, {64, 16}, {32, 8}, {16, 64}};
+constexpr int kColCfgTallest = 6;  // 2 x 256: the most levels per column (the substep kernel's nz limit)
 constexpr const char* kColKernelNames[kNumColKernels] = {"large_prep", "ss_column", "acoustic", "finish"};
 bool col_cfg_fits(int kernel, int cfg, int nz);
 int col_cfg_default(int kernel, int nx, int ny, int nz);
@@ -153,6 +156,7 @@ void launch_sixth_dyn(const Args& a, hipStream_t st);
 // specified boundaries (reference LateralBC): ring launches over the variables of bs
 void launch_bdy(const Args& a, const Scalars& s, const BdySet& bs, BdyMode mode, hipStream_t st);
 void launch_bdy_zero_grad(const Args& a, hipStream_t st);
+void launch_acoustic_ring(const Args& a, double dts, int it, int ns, hipStream_t st);
 void launch_bdy_flow_dep(const Args& a, const Scalars& s, const int* sp, int n, hipStream_t st);
 void launch_bdy_time(const Args& a, hipStream_t st);
 // the same for the scalars, into their step tendencies (first stage, after the acoustic steps)

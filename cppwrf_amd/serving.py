@@ -46,6 +46,7 @@ class Request:
     done: float = math.nan
     domain: RK3 | None = None
     taken: int = 0
+    result: dict | None = None
 
     @property
     def points(self) -> int:
@@ -147,10 +148,13 @@ class ForecastServer:
             self.stats.point_steps += r.points
         self.stats.ticks += 1
 
-    def release_finished(self) -> None:
-        """free the device state of retired forecasts (their results have been read)"""
+    def release_finished(self, keep=frozenset()) -> None:
+        """free the device state of retired forecasts; the final u, v, w, t, mu of those in `keep` (request ids)
+        are read back first into r.result"""
         for r in self.finished:
             if r.domain is not None:
+                if r.rid in keep:
+                    r.result = {fid: r.domain.field(fid) for fid in (A.MI_U, A.MI_V, A.MI_W, A.MI_T, A.MI_MU)}
                 r.domain.close()
                 r.domain = None
 
@@ -174,8 +178,9 @@ def poisson_trace(seconds: float, rate: float, seed: int, scale: float = 1.0, pr
     return out
 
 
-def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None) -> dict:
-    """replay the trace in real time (arrivals by the wall clock) until every request has finished"""
+def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None, keep=frozenset()) -> dict:
+    """replay the trace in real time (arrivals by the wall clock) until every request has finished; the final
+    states of the requests in `keep` stay in their Request.result"""
     ctx = ctx or Context(0)
     t0 = time.perf_counter()
     srv = ForecastServer(ctx, hbm_budget, clock=lambda: time.perf_counter() - t0)
@@ -188,8 +193,9 @@ def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None) 
             time.sleep(max(0.0, min(0.05, pending[0].arrival - now)))
             continue
         srv.tick(now)
-        srv.release_finished()
+        srv.release_finished(keep)
     srv.tick(time.perf_counter() - t0)  # the last boundary retires what the last step finished
+    srv.release_finished(keep)
     wall = time.perf_counter() - t0
     lat = np.array(srv.stats.latencies)
     return {
