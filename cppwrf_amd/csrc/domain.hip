@@ -140,13 +140,24 @@ struct Domain final : mi_domain {
     void* p;
     int px, py;  // -1: default parity of the staggering
   };
-  void halo(std::initializer_list<HaloReq> reqs) {
-    HaloSet s{};
-    s.nkmax = 1;
-    for (const HaloReq& r : reqs) {
+  // The refresh of the halos of a list of fields (fill_halos, src/core/halo.cpp:170-205): one launch per
+  // up to 16 fields; a symmetric OddFace field goes to a batch of its own, which takes the two-pass form.
+  void halo(std::initializer_list<HaloReq> reqs) { halo(reqs.begin(), int(reqs.size())); }
+  void halo(const HaloReq* reqs, int nreq) {
+    HaloSet set[2]{};
+    for (int q = 0; q < nreq; ++q) {
+      const HaloReq& r = reqs[q];
       const FieldDesc* d = desc_of(r.p);
       if (!d) throw Error(MI_EINVAL, "internal: halo of an unknown field");
-      if (s.n == kHaloMax) throw Error(MI_EINVAL, "internal: too many fields in one halo batch");
+      const unsigned char px = (unsigned char)(r.px >= 0 ? r.px : (face_x(d->st) ? 1 : 0));
+      const unsigned char py = (unsigned char)(r.py >= 0 ? r.py : (face_y(d->st) ? 1 : 0));
+      const bool odd = (grid.bc_x == MI_BC_SYMMETRIC && face_x(d->st) && px == 2) ||
+                       (grid.bc_y == MI_BC_SYMMETRIC && face_y(d->st) && py == 2);
+      HaloSet& s = set[odd ? 1 : 0];
+      if (s.n == kHaloMax) {
+        launch_halo(a, s, stream());
+        s = HaloSet{};
+      }
       const int n = s.n++;
       s.p[n] = r.p;
       s.f32[n] = d->esize == 4 ? 1 : 0;
@@ -155,11 +166,18 @@ struct Domain final : mi_domain {
       s.nzs[n] = nzs(d->st);
       s.face_x[n] = face_x(d->st);
       s.face_y[n] = face_y(d->st);
-      s.px[n] = (unsigned char)(r.px >= 0 ? r.px : (face_x(d->st) ? 1 : 0));
-      s.py[n] = (unsigned char)(r.py >= 0 ? r.py : (face_y(d->st) ? 1 : 0));
+      s.px[n] = px;
+      s.py[n] = py;
       if (s.nzs[n] > s.nkmax) s.nkmax = s.nzs[n];
     }
-    launch_halo(a, s, stream());
+    for (HaloSet& s : set)
+      if (s.n > 0) launch_halo(a, s, stream());
+  }
+  // the halos of one scalar buffer of the species sp[0..n)
+  void halo_scalars(real* const* bufs, const int* sp, int n) {
+    HaloReq r[MI_MAX_SCALARS];
+    for (int q = 0; q < n; ++q) r[q] = HaloReq{bufs[sp[q]], -1, -1};
+    halo(r, n);
   }
 
   // ------------------------------------------------------------------ the phases of a step
@@ -189,10 +207,10 @@ struct Domain final : mi_domain {
 
   // LargeStep::prep (large_step.cpp:697-733).  Its first halo refresh (u, v, w, t, ph, p, al, mu of the
   // stage) is left out: every path that changes those fields (the end of a stage, mi_domain_set_field)
-  // refreshes their halos itself, so at this point it would rewrite identical values.  The refresh of
-  // its outputs (mut, ru, rv, rw, alt, php, muu, muv, ww; large_step.cpp:718-720) is written by the
-  // kernel itself: each output point stores its halo images.  The same holds for every refresh of the
-  // small steps below.
+  // refreshes their halos itself, so at this point it would rewrite identical values.  The kernel fuses
+  // the reference's three launches, and its outputs' halos (mut, ru, rv, rw, alt, php, muu, muv, ww;
+  // large_step.cpp:711, 719-723) are refreshed in one batch after it: no output of the fused kernel
+  // is read at a halo point inside it.  The same holds for every fused kernel of the small steps below.
   //
   // With specified boundaries the first stage does refresh u, v, w, t, ph, p, al (not mu): spec_bdy_final
   // changed the spec zone after the last refresh.
@@ -200,6 +218,8 @@ struct Domain final : mi_domain {
     if (spec && rk == 1) halo({{a.f.u2, -1, -1}, {a.f.v2, -1, -1}, {a.f.w2, -1, -1}, {a.f.t2, -1, -1},
                                {a.f.ph2, -1, -1}, {a.f.p, -1, -1}, {a.f.al, -1, -1}});
     launch_large_prep(a, colcfg[kColLargePrep], stream());
+    halo({{a.f.mut, -1, -1}, {a.f.ru, -1, -1}, {a.f.rv, -1, -1}, {a.f.rw, -1, -1}, {a.f.alt, -1, -1},
+          {a.f.php, -1, -1}, {a.f.muu, 2, 0}, {a.f.muv, 0, 1}, {a.f.ww, -1, -1}});
     if (a.q.moist) launch_cq(a, sc, stream());  // calc_cq (large_step.cpp:724-733)
   }
 
@@ -228,21 +248,32 @@ struct Domain final : mi_domain {
     if (rk == 1) launch_fill(f.mudf - off, n2, 0.0, st);
     launch_ss_prep(a, rk == 1, st);
     const double cof_h = 0.5 * dts * kG * (1.0 + prm.epssm);
-    // halos of u'', v'', ph'', t_save, u_save, v_save, mu'', muts, p, al (small_step.cpp:526-528): images
-    // written by the two kernels above
     launch_ss_column(a, colcfg[kColSsColumn], cof_h * cof_h, st);
+    // the halos of u'', v'', ph'', t_save, u_save, v_save, mu'', muts, p, al (small_step.cpp:517-522), in
+    // one batch after the two kernels: the column kernel reads its inputs in its own column only
+    halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.ph2, -1, -1}, {f.t_save, -1, -1}, {f.u_save, -1, -1},
+          {f.v_save, -1, -1}, {f.mu2, -1, -1}, {f.muts, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
     for (int it = 1; it <= ns; ++it) {
-      launch_uv(a, dts, st);  // with the halos of u'', v''
-      // with the halos of mu'', mudf, muts, ph'', p, al, and of ru_m, rv_m, ww_m (their refresh after
-      // the loop; the sums' halo images are rewritten at every substep, the last one's stay)
+      launch_uv(a, dts, st);
+      if (!a.g.two_d) halo({{f.u2, -1, -1}, {f.v2, -1, -1}});  // small_step.cpp:540-541
+      else halo({{f.u2, -1, -1}});
       launch_acoustic(a, colcfg[kColAcoustic], dts, it, ns, st);
       if (spec) launch_acoustic_ring(a, dts, it, ns, st);  // the spec zone's columns
+      // the refreshes of mu'', mudf, muts and of ph'', p, al (small_step.cpp:572, 593) after the fused kernel
+      halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
     }
     if (spec) launch_bdy_zero_grad(a, st);  // zero_grad_bdy of the spec zone's w (lateral_bc.cpp:374-387)
-    // with the halos of u, v, w, t, ph, ww, mu (and p, al); the microphysics heating leaves theta_m at
-    // the last stage with moisture (small_step.cpp:602)
+    halo({{f.muts, -1, -1}, {f.ru_m, -1, -1}, {f.rv_m, -1, -1}, {f.ww_m, -1, -1}});  // small_step.cpp:597
+    // the microphysics heating leaves theta_m at the last stage with moisture (small_step.cpp:602)
     const double hd_dt = (rk == 3 && a.q.moist) ? dts * double(ns) : 0.0;
     launch_finish(a, colcfg[kColFinish], prhophi, hd_dt, st);
+    // small_step.cpp:606 (and 616: p, al of the fused calc_p_rho_phi)
+    if (prhophi)
+      halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.w2, -1, -1}, {f.t2, -1, -1}, {f.ph2, -1, -1}, {f.ww, -1, -1},
+            {f.mu2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
+    else
+      halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.w2, -1, -1}, {f.t2, -1, -1}, {f.ph2, -1, -1}, {f.ww, -1, -1},
+            {f.mu2, -1, -1}});
   }
 
   // ScalarTransport::run_stage (scalars.cpp:436-461): the plain species, then the positive-definite ones
@@ -261,20 +292,32 @@ struct Domain final : mi_domain {
     if (rk == 1 && a.q.use_tendf) launch_sixth_scalars(a, sc, n_sc, dt_rk, stream());
     if (rk == 1 && spec) launch_bdy(a, sc, bdy_scalars(), kBdyRelax, stream());
     if (spec) launch_bdy(a, sc, bdy_scalars(), kBdySpec, stream());
-    launch_scalars(a, sc, plain, np, false, dt_rk, stream());
-    launch_scalars(a, sc, pd, nd, true, dt_rk, stream());
+    launch_scalars(a, sc, plain, np, 0, dt_rk, stream());
+    if (nd > 0) {  // scalars.cpp:375-418, with the halo refreshes of q_1 and of the limiter's scale
+      launch_scalars(a, sc, pd, nd, 1, dt_rk, stream());
+      halo_scalars(sc.q1, pd, nd);
+      launch_scalars(a, sc, pd, nd, 2, dt_rk, stream());
+      halo_scalars(sc.scale, pd, nd);
+      launch_scalars(a, sc, pd, nd, 3, dt_rk, stream());
+    }
     for (int n = 0; n < n_sc; ++n) std::swap(sc.q2[n], sc.qn[n]);
     qpar ^= 1;
+    int all[MI_MAX_SCALARS];
+    for (int n = 0; n < n_sc; ++n) all[n] = n;
     if (spec) {  // flow_dep_bdy of the species without data (scalars.cpp:420-429)
       int fd[MI_MAX_SCALARS], nf = 0;
       for (int n = 0; n < n_sc; ++n)
         if (!q_has_bdy[n]) fd[nf++] = n;
       launch_bdy_flow_dep(a, sc, fd, nf, stream());
     }
+    halo_scalars(sc.q2, all, n_sc);  // scalars.cpp:458-460
   }
 
   // SmallStep::calc_p_rho_phi (small_step.cpp:609-617), for the phase-by-phase entry points
-  void calc_p_rho_phi() { launch_p_rho_phi(a, stream()); }  // with the halos of p, al
+  void calc_p_rho_phi() {
+    launch_p_rho_phi(a, stream());
+    halo({{a.f.p, -1, -1}, {a.f.al, -1, -1}});
+  }
 
   // the end of RK3::step (rk3.cpp:159-167): spec_bdy_final, then the surface w of terrain, which refreshes
   // the u, v halos first (set_w_surface, rk3.cpp:194; those refreshes change nothing without a
@@ -288,6 +331,7 @@ struct Domain final : mi_domain {
     if (a.q.terrain) {
       if (spec) halo({{a.f.u2, -1, -1}, {a.f.v2, -1, -1}});
       launch_set_w_surface(a, stream());
+      halo({{a.f.w2, -1, -1}});  // rk3.cpp:199
     }
   }
 

@@ -57,6 +57,16 @@ inline dim3 grid_of(const Range& r, int by) {
 // registers instead of spilling
 #define MI_POINT_KERNEL __global__ void __launch_bounds__(256)
 
+// the scalar kernels: block z = level + nz * (index in the species list), one species per thread
+#define MI_POINT_INDEX_SP(r)                                           \
+  const Blk blk_ = xcd_block();                                        \
+  const int nk_ = (r).k1 - (r).k0;                                     \
+  const int i = (r).i0 + blk_.x * kBx + int(threadIdx.x);              \
+  const int j = (r).j0 + blk_.y * int(blockDim.y) + int(threadIdx.y);  \
+  const int k = (r).k0 + blk_.z % nk_;                                 \
+  const int sq_ = blk_.z / nk_;                                        \
+  if (i >= (r).i1 || j >= (r).j1) return;
+
 #define MI_POINT_INDEX(r)                          \
   const Blk blk_ = xcd_block();                    \
   const int i = (r).i0 + blk_.x * kBx + int(threadIdx.x); \
@@ -118,6 +128,18 @@ void launch3a(const char* name, double arrays, K kern, const Range& r, hipStream
   const bool t = timing(name);
   if (t) time_begin(st);
   hipLaunchKernelGGL(kern, grid_of(r, by), dim3(kBx, by, 1), 0, st, r, args...);
+  if (t) time_end(st, double(sizeof(real)) * arrays * double(r.i1 - r.i0) * double(r.j1 - r.j0) * double(r.k1 - r.k0));
+}
+// a scalar kernel over the points of r for each of nsp species (grid z = levels x species)
+template <class K, class... T>
+void launch3s(const char* name, double arrays, K kern, const Range& r, int nsp, hipStream_t st, T... args) {
+  if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0 || nsp <= 0) return;
+  const int by = (r.j1 - r.j0) == 1 ? 1 : 4;
+  dim3 gr = grid_of(r, by);
+  gr.z *= unsigned(nsp);
+  const bool t = timing(name);
+  if (t) time_begin(st);
+  hipLaunchKernelGGL(kern, gr, dim3(kBx, by, 1), 0, st, r, args...);
   if (t) time_end(st, double(sizeof(real)) * arrays * double(r.i1 - r.i0) * double(r.j1 - r.j0) * double(r.k1 - r.k0));
 }
 template <class K, class... T>
@@ -185,61 +207,70 @@ __global__ void k_halo_y(Geo g, HaloSet s) {
   else halo_slot(static_cast<double*>(s.p[f]) + g.at(i, 0, k), g.sj, a, g.ny, face, g.bcy, s.py[f]);
 }
 
-// ----------------------------------------------------------------------------- halo images
-// A kernel whose outputs the reference refreshes with fill_halo right after it (src/core/halo.cpp:52-84:
-// an x pass over the interior rows, then a y pass over whole rows) writes that refresh itself: with each
-// interior point it stores every halo slot whose source is that point, with the sign the fill gives it.
-// Copies and sign changes are exact, so the halo holds the same bits as after the fill, and the fill's
-// launches disappear.  Only points within 4 of an edge have images; the others store once.
-// Halo kinds: mass points (even), u faces (odd in x), v faces (odd in y), muu (odd in x, wall face
-// included) and muv (odd in y), as the reference's HaloField parities.
-enum HaloKind : int { kHMass = 0, kHU = 1, kHV = 2, kHMuu = 3, kHMuv = 4 };
-
-// emit(index, negate) for the point itself and every slot of one direction that copies index i: the slots
-// whose source in the fill above is i.  Points 4 or more from both edges have none.  The slot loop stays a
-// loop (compact code: the kernels' register use does not grow with the number of stores), and the periodic
-// source is folded without integer division (slots lie in [-3, n + 3], n >= 3).
-template <class F>
-__device__ __forceinline__ void halo_images(int i, int n, bool face, int bc, int par, F&& emit) {
-  const bool periodic = bc == MI_BC_PERIODIC, flat = bc == MI_BC_SPECIFIED;
-  const bool mirror = !periodic && !flat;  // symmetric: odd fields change sign
-  // the point itself, unless it is a periodic face slot (index n takes index 0's value); a symmetric
-  // wall face of an OddFace field changes sign in place
-  if (!(periodic && face && i == n)) emit(i, mirror && face && par == 2 && (i == 0 || i == n));
-  if (i >= 4 && i + 4 < n) return;
-  const int nl = face ? 4 : 3, ns = face ? 8 : 6, last = face ? n : n - 1;
-#pragma unroll 1
-  for (int a = 0; a < ns; ++a) {
-    const int sl = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
-    if (sl == i) continue;
-    int src;
-    if (flat) {
-      src = sl < 0 ? 0 : (sl > last ? last : sl);
-    } else if (periodic) {
-      src = sl < 0 ? sl + n : sl;
-      while (src >= n) src -= n;
-    } else if (face) {
-      src = sl <= 0 ? -sl : 2 * n - sl;
-    } else {
-      src = sl < 0 ? -1 - sl : 2 * n - 1 - sl;
-    }
-    if (src == i) emit(sl, mirror && par != 0);
+// Every halo point of the fields of s in one pass (the reference's x pass then y pass, halo.cpp:114-131,
+// composed): a halo point (hi, hj) takes the value of the point its x slot maps to in the row its y slot maps
+// to, with the product of the two signs.  The x strips cover the interior rows that keep their values in y,
+// the y strips every x of the row, the x halo included, as the two passes (a periodic face row n, which takes
+// row 0, belongs to the y strips only, so no point is written twice with different values).  Not for a
+// symmetric OddFace field, whose wall face changes sign in place during the x pass (launch_halo keeps the
+// two passes for it).
+// src_of: the source index of slot position i of one direction and whether the copy negates; i itself when
+// the slot is a point of the domain that keeps its value
+__device__ __forceinline__ int src_of(int i, int n, bool face, int bc, int par, bool& neg) {
+  neg = false;
+  const int last = face ? n : n - 1;
+  if (bc == MI_BC_SPECIFIED) return i < 0 ? 0 : (i > last ? last : i);
+  if (bc == MI_BC_PERIODIC) return ((i % n) + n) % n;  // a face slot n takes point 0
+  if (i >= 0 && i <= last) return i;
+  neg = par != 0;
+  if (face) return i < 0 ? -i : 2 * n - i;
+  return i < 0 ? -1 - i : 2 * n - 1 - i;
+}
+template <class T>
+__device__ __forceinline__ void halo_point(T* base, const Geo& g, int hi, int hj, bool fx, bool fy, int px, int py) {
+  bool nx_, ny_;
+  const int sx = src_of(hi, g.nx, fx, g.bcx, px, nx_);
+  const int sy = g.two_d ? hj : src_of(hj, g.ny, fy, g.bcy, py, ny_);
+  if (g.two_d) ny_ = false;
+  if (sx == hi && sy == hj) return;
+  const T v = base[sx + sy * g.sj];
+  base[hi + hj * g.sj] = (nx_ != ny_) ? T(-v) : v;
+}
+__global__ void k_halo_xy(Geo g, HaloSet s) {
+  const int fk = int(blockIdx.y);
+  const int f = fk / s.nkmax, k = fk - f * s.nkmax;
+  if (f >= s.n || k >= s.nzs[f]) return;
+  const bool fx = s.face_x[f] != 0, fy = s.face_y[f] != 0;
+  const int nsx = fx ? 8 : 6, nsy = g.two_d ? 0 : (fy ? 8 : 6);
+  const int nys = s.nys[f], w = s.nxs[f] + 2 * kHalo;
+  const int e = int(blockIdx.x) * 256 + int(threadIdx.x);
+  int hi, hj;
+  if (e < nys * nsx) {  // x strips: slot a of interior row j
+    const int a = e % nsx;
+    hj = e / nsx;
+    const int nl = fx ? 4 : 3;
+    hi = a < nl ? (fx ? -a : -1 - a) : g.nx + (a - nl);
+    bool neg;
+    if (!g.two_d && src_of(hj, g.ny, fy, g.bcy, s.py[f], neg) != hj) return;
+  } else {  // y strips: slot b of column hi = -3 .. nxs + 2
+    const int e2 = e - nys * nsx;
+    if (e2 >= w * nsy) return;
+    hi = e2 % w - kHalo;
+    const int b = e2 / w, nl = fy ? 4 : 3;
+    hj = b < nl ? (fy ? -b : -1 - b) : g.ny + (b - nl);
   }
+  const idx_t ofs = idx_t(k) * g.sk;
+  if (s.f32[f]) halo_point(static_cast<float*>(s.p[f]) + ofs, g, hi, hj, fx, fy, s.px[f], s.py[f]);
+  else halo_point(static_cast<double*>(s.p[f]) + ofs, g, hi, hj, fx, fy, s.px[f], s.py[f]);
 }
 
-// p[at(i, j) + kofs] = v and its halo images
+// ----------------------------------------------------------------------------- stores
+// p[at(i, j) + kofs] = v.  The halo refreshes the reference makes after a kernel run as separate batched
+// launches (launch_halo); the kind names the field's halo parity there.
+enum HaloKind : int { kHMass = 0, kHU = 1, kHV = 2, kHMuu = 3, kHMuv = 4 };
 template <int KIND, class T>
 __device__ __forceinline__ void put(T* p, const Geo& g, int i, int j, idx_t kofs, double v) {
-  constexpr bool fx = KIND == kHU || KIND == kHMuu, fy = KIND == kHV || KIND == kHMuv;
-  constexpr int px = KIND == kHU ? 1 : (KIND == kHMuu ? 2 : 0), py = fy ? 1 : 0;
-  T* q = p + kofs;
-  halo_images(i, g.nx, fx, g.bcx, px, [&](int ii, bool nx_) {
-    if (g.two_d) {
-      q[ii] = T(nx_ ? -v : v);
-      return;
-    }
-    halo_images(j, g.ny, fy, g.bcy, py, [&](int jj, bool ny_) { q[ii + jj * g.sj] = T((nx_ != ny_) ? -v : v); });
-  });
+  p[kofs + i + j * g.sj] = T(v);
 }
 
 // ----------------------------------------------------------------------------- copies
@@ -1564,8 +1595,8 @@ MI_POINT_KERNEL k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
 }
 
 // ----------------------------------------------------------------------------- set_w_surface
-// KSetWSurface (rk3.cpp:173-190), fill = false: the kinematic surface w from the new u, v; with the
-// halo images of w (the reference refreshes w's halo next)
+// KSetWSurface (rk3.cpp:173-190), fill = false: the kinematic surface w from the new u, v (w's halo is
+// refreshed next, as rk3.cpp:199)
 MI_POINT_KERNEL k_set_w_surface(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -1690,18 +1721,17 @@ struct ScFaces {
 __device__ __forceinline__ double m_full(const Lv& c, int k, double mub, double mu) { return c.c1h[k] * (mub + mu) + c.c2h[k]; }
 
 // rk_update_scalar (KUpdate, scalars.cpp:145-158): advect_scalar's tendency and the mass-weighted update,
-// every species of the list; the new values go to the species' second buffer with their halo images
+// every species of the list; the new values go to the species' second buffer
 MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
-  MI_POINT_INDEX(r)
+  MI_POINT_INDEX_SP(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), x0 = g.at(i, j, 0);
   const double msftx = f.msftx[x2], msfty = f.msfty[x2];
   const double m_old = m_full(c, k, f.mub[x2], f.mu1[x2]), m_new = m_full(c, k, f.mub[x2], f.mu2[x2]);
-#pragma unroll 1
-  for (int m = 0; m < l.n; ++m) {
-    const int n = l.sp[m];
+  {
+    const int n = l.sp[sq_];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, false};
     double adv = 0;
     if (!fl.spec_cell(i, j)) {
@@ -1716,16 +1746,15 @@ MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
 }
 
 // rk_update_scalar_pd, first step (KPreUpdatePD, scalars.cpp:174-181): the held tendency folded into
-// q_1, with its halo images (the reference refreshes q_1's halo next)
+// q_1 (q_1's halo is refreshed next, as scalars.cpp:383)
 MI_POINT_KERNEL k_sc_pd_pre(Range r, Args a, Scalars s, SpList l, double dt_rk) {
-  MI_POINT_INDEX(r)
+  MI_POINT_INDEX_SP(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double m = m_full(a.c, k, f.mub[x2], f.mu1[x2]);
-#pragma unroll 1
-  for (int q = 0; q < l.n; ++q) {
-    const int n = l.sp[q];
+  {
+    const int n = l.sp[sq_];
     put<kHMass>(s.q1[n], g, i, j, k * g.sk, (m * s.q1[n][x] + dt_rk * (a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x]) / m);
   }
 }
@@ -1758,22 +1787,21 @@ __device__ __forceinline__ PdFaces pd_faces(const ScFaces& fl, const Geo& g, int
 }
 
 // advect_scalar_pd's limiter (KScalePD, scalars.cpp:220-248): the scale of each cell's outgoing
-// high-minus-low fluxes, with its halo images
+// high-minus-low fluxes (its halo is refreshed next, as scalars.cpp:411)
 MI_POINT_KERNEL k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk) {
-  MI_POINT_INDEX(r)
+  MI_POINT_INDEX_SP(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const double msftx = f.msftx[x2], msfty = f.msfty[x2];
   const double mxy = msftx * msfty;
-#pragma unroll 1
-  for (int q = 0; q < l.n; ++q) {
-    const int n = l.sp[q];
+  {
+    const int n = l.sp[sq_];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
     if (fl.spec_cell(i, j)) {  // WRF limits interior cells only
       put<kHMass>(s.scale[n], g, i, j, k * g.sk, 1.0);
-      continue;
+      return;
     }
     const PdFaces p = pd_faces(fl, g, i, j, k);
     double hlow = c.rdx * (p.lox1 - p.lox0);
@@ -1795,7 +1823,7 @@ MI_POINT_KERNEL k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk
 
 // advect_scalar_pd's limited divergence and rk_update_scalar (KUpdatePD, scalars.cpp:252-291)
 MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
-  MI_POINT_INDEX(r)
+  MI_POINT_INDEX_SP(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
@@ -1803,14 +1831,13 @@ MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), sj = g.sj, sk = g.sk;
   const double msftx = f.msftx[x2], msfty = f.msfty[x2];
   const double m_old = m_full(c, k, f.mub[x2], f.mu1[x2]), m_new = m_full(c, k, f.mub[x2], f.mu2[x2]);
-#pragma unroll 1
-  for (int q = 0; q < l.n; ++q) {
-    const int n = l.sp[q];
+  {
+    const int n = l.sp[sq_];
     const real* sc = s.scale[n];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
     if (fl.spec_cell(i, j)) {  // the spec zone: only the held tendency, folded into q_1 before
       put<kHMass>(s.qn[n], g, i, j, k * sk, m_old * s.q1[n][x] / m_new);
-      continue;
+      return;
     }
     const PdFaces p = pd_faces(fl, g, i, j, k);
     // the limited flux of a face: scaled by the scale of the cell it leaves
@@ -1858,8 +1885,8 @@ __device__ __forceinline__ int64_t bdy_slab(int kind, int sl, const Geo& g) {
 // One launch over the rows of every variable in bs: thread (pos, k), block z = (variable, side, row).
 //   kBdySpec   spec_bdytend (KSpecTend, lateral_bc.cpp:67-79): the spec zone's tendency = the data's
 //   kBdyRelax  relax_bdytend (KRelax, lateral_bc.cpp:82-121) of rows 1..3 into the held tendencies
-//   kBdyFinal  spec_bdy_final (KFinal, lateral_bc.cpp:180-194): the spec zone's values at t + dt, without
-//              halo images (the reference refreshes no halo after it)
+//   kBdyFinal  spec_bdy_final (KFinal, lateral_bc.cpp:180-194): the spec zone's values at t + dt (the
+//              reference refreshes no halo after it)
 __global__ void k_bdy(Args a, Scalars s, BdySet bs, int mode, int b0, int nb) {
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -1965,8 +1992,7 @@ __global__ void k_bdy_zero_grad(Args a) {
 }
 
 // flow_dep_bdy (KFlowDep, lateral_bc.cpp:163-178) of the species without boundary data: the spec zone takes
-// the next row in at outflow and 0 at inflow (by the sign of ru_m, rv_m); with the halo images the
-// reference's refresh of q_2 writes next
+// the next row in at outflow and 0 at inflow (by the sign of ru_m, rv_m); q_2's halo is refreshed next
 __global__ void k_bdy_flow_dep(Args a, Scalars s, SpList l) {
   const Geo& g = a.g;
   const Fields& f = a.f;
@@ -2044,16 +2070,28 @@ void launch_halo(const Args& a, const HaloSet& s, hipStream_t st) {
   if (s.n == 0) return;
   const Geo& g = a.g;
   int nymax = 0, nxmax = 0;
+  bool odd_face = false;
   for (int f = 0; f < s.n; ++f) {
     nymax = s.nys[f] > nymax ? s.nys[f] : nymax;
     nxmax = s.nxs[f] > nxmax ? s.nxs[f] : nxmax;
+    odd_face = odd_face || (g.bcx == MI_BC_SYMMETRIC && s.px[f] == 2 && s.face_x[f]) ||
+               (g.bcy == MI_BC_SYMMETRIC && s.py[f] == 2 && s.face_y[f]);
   }
   const unsigned nfk = unsigned(s.n * s.nkmax);
+  const bool t = timing("halo");
+  if (t) time_begin(st);
+  if (!odd_face) {  // one pass
+    const int pts = (nymax + 1) * 8 + (g.two_d ? 0 : (nxmax + 1 + 2 * kHalo) * 8);
+    hipLaunchKernelGGL(k_halo_xy, dim3(unsigned((pts + 255) / 256), nfk, 1), dim3(256, 1, 1), 0, st, g, s);
+    if (t) time_end(st, 0.0);
+    return;
+  }
   hipLaunchKernelGGL(k_halo_x, dim3(unsigned((nymax + 31) / 32), nfk, 1), dim3(256, 1, 1), 0, st, g, s);
   if (!g.two_d) {
     const int ni = nxmax + 2 * kHalo;
     hipLaunchKernelGGL(k_halo_y, dim3(unsigned((ni + 63) / 64), nfk, 1), dim3(64, 8, 1), 0, st, g, s);
   }
+  if (t) time_end(st, 0.0);
 }
 
 void launch_copy_step(const Args& a, int64_t n3, int64_t n2, hipStream_t st) {
@@ -2191,21 +2229,22 @@ void launch_cq(const Args& a, const Scalars& s, hipStream_t st) {
   // the moist species -> cqu, cqv, cqw
   launch3a("cq", s.n_moist + 3, k_cq, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz}, st, a, s);
 }
-void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, bool pd, double dt_rk, hipStream_t st) {
+// the kernels of the scalar transport of the species sp[0..n) (step: 0 rk_update_scalar; 1, 2, 3 the
+// positive-definite pre-update, limiter scale and update); algorithmic arrays: the mass fluxes ru_m, rv_m,
+// ww_m once, then per species
+void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, int step, double dt_rk, hipStream_t st) {
   if (n <= 0) return;
   SpList l{};
   l.n = n;
   for (int q = 0; q < n; ++q) l.sp[q] = sp[q];
   const Geo& g = a.g;
   const Range r{0, g.nx, 0, g.ny, 0, g.nz};
-  // algorithmic arrays: the mass fluxes ru_m, rv_m, ww_m once, then per species
-  if (!pd) {
-    launch3a("sc_update", 3 + 4 * n, k_sc_update, r, st, a, s, l, dt_rk);  // q_2 q_1 q_tendf -> q_new
-    return;
+  switch (step) {
+    case 0: launch3s("sc_update", 3 + 4 * n, k_sc_update, r, n, st, a, s, l, dt_rk); break;        // q_2 q_1 q_tendf -> q_new
+    case 1: launch3s("sc_pd_pre", 3 * n, k_sc_pd_pre, r, n, st, a, s, l, dt_rk); break;            // q_1 q_tendf -> q_1
+    case 2: launch3s("sc_pd_scale", 3 + 3 * n, k_sc_pd_scale, r, n, st, a, s, l, dt_rk); break;    // q_2 q_1 -> scale
+    default: launch3s("sc_pd_update", 3 + 4 * n, k_sc_pd_update, r, n, st, a, s, l, dt_rk); break; // q_2 q_1 scale -> q_new
   }
-  launch3a("sc_pd_pre", 3 * n, k_sc_pd_pre, r, st, a, s, l, dt_rk);            // q_1 q_tendf -> q_1
-  launch3a("sc_pd_scale", 3 + 3 * n, k_sc_pd_scale, r, st, a, s, l, dt_rk);    // q_2 q_1 -> scale
-  launch3a("sc_pd_update", 3 + 4 * n, k_sc_pd_update, r, st, a, s, l, dt_rk); // q_2 q_1 scale -> q_new
 }
 void launch_bdy(const Args& a, const Scalars& s, const BdySet& bs, BdyMode mode, hipStream_t st) {
   if (bs.n <= 0) return;

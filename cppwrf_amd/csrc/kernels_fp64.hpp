@@ -168,7 +168,7 @@ void launch_cfl(const Args& a, unsigned long long* out, hipStream_t st);
 // calc_cq (KCq) of the stage state
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st);
 // the transport of species sp[0..n): rk_update_scalar (pd = 0) or rk_update_scalar_pd (pd = 1)
-void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, bool pd, double dt_rk, hipStream_t st);
+void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, int step, double dt_rk, hipStream_t st);
 void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hipStream_t st);
 void launch_p_rho_phi(const Args& a, hipStream_t st);
 void launch_finite(const Args& a, int* flags, hipStream_t st);
