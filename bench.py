@@ -78,6 +78,24 @@ def workload(name):
     return case, kw, desc, bdy
 
 
+def measured_traffic(workload_name, kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (each counter scaled by its reading for a copy kernel of known size), or None.  The first line of a
+    kernel and precision is its most-dispatched block shape, the autotuned one."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic_hills119x89x44.md")
+    if workload_name != "hills" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        line = line.strip()
+        if not line.startswith("traffic:"):
+            continue
+        kv = dict(x.split("=", 1) for x in line.split()[1:])
+        if kv.get("kernel") == kernel and kv.get("precision") == precision:
+            return {"bytes": float(kv["bytes"]), "shape": kv.get("shape"),
+                    "source": "profiles/hbm_traffic_hills119x89x44.md"}
+    return None
+
+
 def profile(rk, name, steps):
     from cppwrf_amd._abi import check, lib
     check(lib().mi_profile_kernel(rk.handle, name.encode()))
@@ -262,6 +280,7 @@ def main():
     ev_ms, nl, b = profile(rk, dominant, args.steps)
     avg_launch_s = ev_ms / 1e3 / nl
     achieved = b / nl / avg_launch_s / 1e9
+    traffic = measured_traffic(args.workload, dominant, "fp32")
 
     out = {
         "metric": METRIC,
@@ -283,7 +302,8 @@ def main():
         "fp32_error_vs_fp64": {"after_steps": args.warmup + args.steps,
                                "simulated_s": (args.warmup + args.steps) * g.dt, "per_variable": err},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": b / nl,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic, "alg_bytes_per_launch": b / nl,
                      "avg_launch_us": avg_launch_s * 1e6, "launches": nl, "precision": "fp32-bulk"},
     }
     for d in doms.values():

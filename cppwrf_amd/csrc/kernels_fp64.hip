@@ -1765,25 +1765,23 @@ struct PdFaces {
   double dhx0, lox0, dhx1, lox1, dhy0, loy0, dhy1, loy1, dhz0, loz0, dhz1, loz1;
 };
 __device__ __forceinline__ PdFaces pd_faces(const ScFaces& fl, const Geo& g, int i, int j, int k) {
-  PdFaces p;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), x0 = g.at(i, j, 0);
-  p.lox0 = fl.fxl(x, x2, k, i);
-  p.dhx0 = fl.fx(x, i) - p.lox0;
-  p.lox1 = fl.fxl(x + 1, x2 + 1, k, i + 1);
-  p.dhx1 = fl.fx(x + 1, i + 1) - p.lox1;
+  const double lox0 = fl.fxl(x, x2, k, i);
+  const double dhx0 = fl.fx(x, i) - lox0;
+  const double lox1 = fl.fxl(x + 1, x2 + 1, k, i + 1);
+  const double dhx1 = fl.fx(x + 1, i + 1) - lox1;
+  double loy0 = 0, dhy0 = 0, loy1 = 0, dhy1 = 0;
   if (!g.two_d) {
-    p.loy0 = fl.fyl(x, x2, k);
-    p.dhy0 = fl.fy(x, j) - p.loy0;
-    p.loy1 = fl.fyl(x + g.sj, x2 + g.sj, k);
-    p.dhy1 = fl.fy(x + g.sj, j + 1) - p.loy1;
-  } else {
-    p.loy0 = p.dhy0 = p.loy1 = p.dhy1 = 0;
+    loy0 = fl.fyl(x, x2, k);
+    dhy0 = fl.fy(x, j) - loy0;
+    loy1 = fl.fyl(x + g.sj, x2 + g.sj, k);
+    dhy1 = fl.fy(x + g.sj, j + 1) - loy1;
   }
-  p.loz0 = fl.fzl(x0, x2, k);
-  p.dhz0 = fl.fz(x0, k) - p.loz0;
-  p.loz1 = fl.fzl(x0, x2, k + 1);
-  p.dhz1 = fl.fz(x0, k + 1) - p.loz1;
-  return p;
+  const double loz0 = fl.fzl(x0, x2, k);
+  const double dhz0 = fl.fz(x0, k) - loz0;
+  const double loz1 = fl.fzl(x0, x2, k + 1);
+  const double dhz1 = fl.fz(x0, k + 1) - loz1;
+  return PdFaces{dhx0, lox0, dhx1, lox1, dhy0, loy0, dhy1, loy1, dhz0, loz0, dhz1, loz1};
 }
 
 // advect_scalar_pd's limiter (KScalePD, scalars.cpp:220-248): the scale of each cell's outgoing
