@@ -264,6 +264,18 @@ def main():
         if bad != -1:
             raise SystemExit(f"{tag}: non-finite values in field {bad}")
     err = error_report(doms["fp32"], doms["fp64"], kw.get("n_moist", 0))
+    # the FP32-bulk error per forecast hour up to 6 h: both domains stepped on to each whole hour
+    done = args.warmup + args.steps
+    hourly = []
+    for hour in range(1, 7):
+        target = int(np.ceil(hour * 3600.0 / g.dt - 1e-9))
+        if target > done:
+            for rk in doms.values():
+                rk.step(target - done)
+            done = target
+        if target == done:
+            hourly.append({"hour": hour, "steps": done,
+                           "per_variable": error_report(doms["fp32"], doms["fp64"], kw.get("n_moist", 0))})
     pts = g.nx * g.ny * g.nz
     v64 = pts * args.steps / secs["fp64"]
     v32 = pts * args.steps / secs["fp32"]
@@ -300,7 +312,10 @@ def main():
                  "note": "plain FP64, bit-identical to the reference CPU FP64 build"},
         "speedup_fp32_vs_fp64": v32 / v64,
         "fp32_error_vs_fp64": {"after_steps": args.warmup + args.steps,
-                               "simulated_s": (args.warmup + args.steps) * g.dt, "per_variable": err},
+                               "simulated_s": (args.warmup + args.steps) * g.dt, "per_variable": err,
+                               "hourly": hourly,
+                               "max_rel_l2_6h": max(v["rel_l2"] for v in hourly[-1]["per_variable"].values())
+                               if hourly and hourly[-1]["hour"] == 6 else None},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic, "alg_bytes_per_launch": b / nl,

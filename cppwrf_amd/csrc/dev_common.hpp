@@ -58,36 +58,52 @@ struct Lv {
   double cf1, cf2, cf3, cfn, cfn1, rdx, rdy, p_top;
 };
 
-__device__ __forceinline__ double sgn1(double a) { return a >= 0.0 ? 1.0 : -1.0; }
+// The flux operators are templates on the arithmetic type T: double in the plain FP64 mode (the reference's
+// expressions, bit for bit) and float in the bulk of the FP32-bulk mode.  by60 / by12 are the operators'
+// divisions by 60 and 12 and qdiv a division of two values: IEEE divisions in FP64; in FP32 a multiply by
+// the rounded reciprocal of the constant and by the hardware reciprocal (1 ulp) of a value, which keeps the
+// division sequence out of the FP32-bulk kernels.
+__device__ __forceinline__ double by60(double x) { return x / 60.0; }
+__device__ __forceinline__ float by60(float x) { return x * (1.0f / 60.0f); }
+__device__ __forceinline__ double by12(double x) { return x / 12.0; }
+__device__ __forceinline__ float by12(float x) { return x * (1.0f / 12.0f); }
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+template <class T>
+__device__ __forceinline__ T sgn1(T a) { return a >= T(0) ? T(1) : T(-1); }
 
 // 5th-order upwind flux of the reference (src/dyn/kernels_common.hpp:13-17); up = 0 gives the 6th-order
 // centred flux
-__device__ __forceinline__ double flux5(double qm3, double qm2, double qm1, double q0, double qp1, double qp2, double ua,
-                                        double up) {
-  return (37.0 * (q0 + qm1) - 8.0 * (qp1 + qm2) + (qp2 + qm3)) / 60.0 -
-         up * sgn1(ua) * ((qp2 - qm3) - 5.0 * (qp1 - qm2) + 10.0 * (q0 - qm1)) / 60.0;
+template <class T>
+__device__ __forceinline__ T flux5(T qm3, T qm2, T qm1, T q0, T qp1, T qp2, T ua, T up) {
+  return by60(T(37) * (q0 + qm1) - T(8) * (qp1 + qm2) + (qp2 + qm3)) -
+         by60(up * sgn1(ua) * ((qp2 - qm3) - T(5) * (qp1 - qm2) + T(10) * (q0 - qm1)));
 }
 // 3rd-order upwind flux (kernels_common.hpp:19-22); up = 0 gives the 4th-order centred flux
-__device__ __forceinline__ double flux3(double qm2, double qm1, double q0, double qp1, double ua, double up) {
-  return (7.0 * (q0 + qm1) - (qp1 + qm2)) / 12.0 + up * sgn1(ua) * ((qp1 - qm2) - 3.0 * (q0 - qm1)) / 12.0;
+template <class T>
+__device__ __forceinline__ T flux3(T qm2, T qm1, T q0, T qp1, T ua, T up) {
+  return by12(T(7) * (q0 + qm1) - (qp1 + qm2)) + by12(up * sgn1(ua) * ((qp1 - qm2) - T(3) * (q0 - qm1)));
 }
 
 // advect_scalar_pd's own statement functions (kernels_common.hpp:23-33): folded constants, so their
 // roundoff differs from flux5 / flux3
-__device__ __forceinline__ double flux5_pd(double qm3, double qm2, double qm1, double q0, double qp1, double qp2,
-                                           double ua, double up) {
-  const double f6 = (37. / 60.) * (q0 + qm1) - (2. / 15.) * (qp1 + qm2) + (1. / 60.) * (qp2 + qm3);
-  return f6 - up * sgn1(ua) * (1. / 60.) * ((qp2 - qm3) - 5. * (qp1 - qm2) + 10. * (q0 - qm1));
+template <class T>
+__device__ __forceinline__ T flux5_pd(T qm3, T qm2, T qm1, T q0, T qp1, T qp2, T ua, T up) {
+  const T f6 = T(37. / 60.) * (q0 + qm1) - T(2. / 15.) * (qp1 + qm2) + T(1. / 60.) * (qp2 + qm3);
+  return f6 - up * sgn1(ua) * T(1. / 60.) * ((qp2 - qm3) - T(5.) * (qp1 - qm2) + T(10.) * (q0 - qm1));
 }
-__device__ __forceinline__ double flux3_pd(double qm2, double qm1, double q0, double qp1, double ua, double up) {
-  const double f4 = (7. / 12.) * (q0 + qm1) - (1. / 12.) * (qp1 + qm2);
-  return f4 + up * sgn1(ua) * (1. / 12.) * ((qp1 - qm2) - 3. * (q0 - qm1));
+template <class T>
+__device__ __forceinline__ T flux3_pd(T qm2, T qm1, T q0, T qp1, T ua, T up) {
+  const T f4 = T(7. / 12.) * (q0 + qm1) - T(1. / 12.) * (qp1 + qm2);
+  return f4 + up * sgn1(ua) * T(1. / 12.) * ((qp1 - qm2) - T(3.) * (q0 - qm1));
 }
 // first-order upwind flux with Courant number cr (kernels_common.hpp:83-88)
-__device__ __forceinline__ double flux_upwind(double qm1, double q0, double cr) {
-  const double a = cr < 0.0 ? -cr : cr;
-  const double lo = cr + a, hi = cr - a;
-  return 0.5 * (lo < 1.0 ? lo : 1.0) * qm1 + 0.5 * (hi > -1.0 ? hi : -1.0) * q0;
+template <class T>
+__device__ __forceinline__ T flux_upwind(T qm1, T q0, T cr) {
+  const T a = cr < T(0) ? -cr : cr;
+  const T lo = cr + a, hi = cr - a;
+  return T(0.5) * (lo < T(1) ? lo : T(1)) * qm1 + T(0.5) * (hi > T(-1) ? hi : T(-1)) * q0;
 }
 
 }  // namespace mi_dev

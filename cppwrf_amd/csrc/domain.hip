@@ -294,6 +294,7 @@ struct Domain final : mi_domain {
     if (spec) launch_bdy(a, sc, bdy_scalars(), kBdySpec, stream());
     launch_scalars(a, sc, plain, np, 0, dt_rk, stream());
     if (nd > 0) {  // scalars.cpp:375-418, with the halo refreshes of q_1 and of the limiter's scale
+      launch_pd_geom(a, dt_rk, stream());
       launch_scalars(a, sc, pd, nd, 1, dt_rk, stream());
       halo_scalars(sc.q1, pd, nd);
       launch_scalars(a, sc, pd, nd, 2, dt_rk, stream());
@@ -627,6 +628,10 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   }
   d.n_sc = p.n_scalars;
   d.sc.n_moist = p.n_moist;
+  if (p.n_scalars > 0) {  // the limiter's face geometry
+    add2(f.pd_dx, St::U2, -1); add2(f.pd_dxdt, St::U2, -1); add2(f.pd_dy, St::V2, -1); add2(f.pd_dydt, St::V2, -1);
+    f.pd_dz = d.alloc(2 * (int64_t(g.nz) + 1));
+  }
   for (int n = 0; n < d.n_sc; ++n) {
     add3(d.sc.q1[n], St::M, -1);
     add3(d.sc.q2[n], St::M, MI_Q + n);

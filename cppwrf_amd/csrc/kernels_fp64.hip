@@ -473,28 +473,28 @@ MI_POINT_KERNEL k_tend_u(Range r, Args a) {
     if (o == 2) {  // the upstream normal velocity at the boundary face (large_step.cpp:160-167)
       if (fi <= 1) {
         const double ub = u[y] < 0.0 ? double(u[y]) : double(u[y - 1]);
-        return Q * (f.ru[y] + f.ru[y - 1]) * (u[y] + ub);
+        return acc(Q * (f.ru[y] + f.ru[y - 1]) * (u[y] + ub));
       }
       const double ub = u[y - 1] > 0.0 ? double(u[y - 1]) : double(u[y]);
-      return Q * (f.ru[y] + f.ru[y - 1]) * (u[y - 1] + ub);
+      return acc(Q * (f.ru[y] + f.ru[y - 1]) * (u[y - 1] + ub));
     }
-    const double vel = H * (f.ru[y] + f.ru[y - 1]);
-    if (o == 3) return vel * flux3(u[y - 2], u[y - 1], u[y], u[y + 1], vel, hup);
-    return vel * flux5(u[y - 3], u[y - 2], u[y - 1], u[y], u[y + 1], u[y + 2], vel, hup);
+    const acc vel = acc(H * (f.ru[y] + f.ru[y - 1]));
+    if (o == 3) return acc(vel * flux3<acc>(u[y - 2], u[y - 1], u[y], u[y + 1], vel, acc(hup)));
+    return acc(vel * flux5<acc>(u[y - 3], u[y - 2], u[y - 1], u[y], u[y + 1], u[y + 2], vel, acc(hup)));
   };
   auto fy = [&](idx_t y, int fj) {
     const int o = ord_face(fj, g.ny, sp);
-    if (o == 2) return Q * (f.rv[y] + f.rv[y - 1]) * (u[y] + u[y - sj]);
-    const double vel = H * (f.rv[y] + f.rv[y - 1]);
-    if (o == 3) return vel * flux3(u[y - 2 * sj], u[y - sj], u[y], u[y + sj], vel, hup);
-    return vel * flux5(u[y - 3 * sj], u[y - 2 * sj], u[y - sj], u[y], u[y + sj], u[y + 2 * sj], vel, hup);
+    if (o == 2) return acc(Q * (f.rv[y] + f.rv[y - 1]) * (u[y] + u[y - sj]));
+    const acc vel = acc(H * (f.rv[y] + f.rv[y - 1]));
+    if (o == 3) return acc(vel * flux3<acc>(u[y - 2 * sj], u[y - sj], u[y], u[y + sj], vel, acc(hup)));
+    return acc(vel * flux5<acc>(u[y - 3 * sj], u[y - 2 * sj], u[y - sj], u[y], u[y + sj], u[y + 2 * sj], vel, acc(hup)));
   };
   auto fz = [&](int kk) {
-    if (kk == 0 || kk == nz) return 0.0;
+    if (kk == 0 || kk == nz) return acc(0.0);
     const idx_t y = g.at(i, j, kk);
-    const double vel = H * (f.ww[y - 1] + f.ww[y]);
-    if (kk == 1 || kk == nz - 1) return vel * (c.fnm[kk] * u[y] + c.fnp[kk] * u[y - sk]);
-    return vel * flux3(u[y - 2 * sk], u[y - sk], u[y], u[y + sk], -vel, vup);
+    const acc vel = acc(H * (f.ww[y - 1] + f.ww[y]));
+    if (kk == 1 || kk == nz - 1) return acc(vel * (c.fnm[kk] * u[y] + c.fnp[kk] * u[y - sk]));
+    return acc(vel * flux3<acc>(u[y - 2 * sk], u[y - sk], u[y], u[y + sk], -vel, acc(vup)));
   };
   auto pu = [&](int kk) { const idx_t y = g.at(i, j, kk); return f.p[y - 1] + f.p[y]; };
   auto dpn = [&](int kk) {
@@ -541,31 +541,31 @@ MI_POINT_KERNEL k_tend_v(Range r, Args a) {
   const bool sp = g.bcx == MI_BC_SPECIFIED;
   auto fx = [&](idx_t y, int fi) {
     const int o = ord_face(fi, g.nx, sp);
-    if (o == 2) return Q * (f.ru[y] + f.ru[y - sj]) * (v[y] + v[y - 1]);
-    const double vel = H * (f.ru[y] + f.ru[y - sj]);
-    if (o == 3) return vel * flux3(v[y - 2], v[y - 1], v[y], v[y + 1], vel, hup);
-    return vel * flux5(v[y - 3], v[y - 2], v[y - 1], v[y], v[y + 1], v[y + 2], vel, hup);
+    if (o == 2) return acc(Q * (f.ru[y] + f.ru[y - sj]) * (v[y] + v[y - 1]));
+    const acc vel = acc(H * (f.ru[y] + f.ru[y - sj]));
+    if (o == 3) return acc(vel * flux3<acc>(v[y - 2], v[y - 1], v[y], v[y + 1], vel, acc(hup)));
+    return acc(vel * flux5<acc>(v[y - 3], v[y - 2], v[y - 1], v[y], v[y + 1], v[y + 2], vel, acc(hup)));
   };
   auto fy = [&](idx_t y, int fj) {  // the flux at mass row fj - 1
     const int o = ord_mass(fj, g.ny, sp);
     if (o == 2) {  // the upstream normal velocity at the boundary face (large_step.cpp:282-289)
       if (fj <= 1) {
         const double vb = v[y] < 0.0 ? double(v[y]) : double(v[y - sj]);
-        return Q * (f.rv[y] + f.rv[y - sj]) * (v[y] + vb);
+        return acc(Q * (f.rv[y] + f.rv[y - sj]) * (v[y] + vb));
       }
       const double vb = v[y - sj] > 0.0 ? double(v[y - sj]) : double(v[y]);
-      return Q * (f.rv[y] + f.rv[y - sj]) * (vb + v[y - sj]);
+      return acc(Q * (f.rv[y] + f.rv[y - sj]) * (vb + v[y - sj]));
     }
-    const double vel = H * (f.rv[y] + f.rv[y - sj]);
-    if (o == 3) return vel * flux3(v[y - 2 * sj], v[y - sj], v[y], v[y + sj], vel, hup);
-    return vel * flux5(v[y - 3 * sj], v[y - 2 * sj], v[y - sj], v[y], v[y + sj], v[y + 2 * sj], vel, hup);
+    const acc vel = acc(H * (f.rv[y] + f.rv[y - sj]));
+    if (o == 3) return acc(vel * flux3<acc>(v[y - 2 * sj], v[y - sj], v[y], v[y + sj], vel, acc(hup)));
+    return acc(vel * flux5<acc>(v[y - 3 * sj], v[y - 2 * sj], v[y - sj], v[y], v[y + sj], v[y + 2 * sj], vel, acc(hup)));
   };
   auto fz = [&](int kk) {
-    if (kk == 0 || kk == nz) return 0.0;
+    if (kk == 0 || kk == nz) return acc(0.0);
     const idx_t y = g.at(i, j, kk);
-    const double vel = H * (f.ww[y - sj] + f.ww[y]);
-    if (kk == 1 || kk == nz - 1) return vel * (c.fnm[kk] * v[y] + c.fnp[kk] * v[y - sk]);
-    return vel * flux3(v[y - 2 * sk], v[y - sk], v[y], v[y + sk], -vel, vup);
+    const acc vel = acc(H * (f.ww[y - sj] + f.ww[y]));
+    if (kk == 1 || kk == nz - 1) return acc(vel * (c.fnm[kk] * v[y] + c.fnp[kk] * v[y - sk]));
+    return acc(vel * flux3<acc>(v[y - 2 * sk], v[y - sk], v[y], v[y + sk], -vel, acc(vup)));
   };
   auto pv = [&](int kk) { const idx_t y = g.at(i, j, kk); return f.p[y - sj] + f.p[y]; };
   auto dpn = [&](int kk) {
@@ -619,24 +619,24 @@ MI_POINT_KERNEL k_tend_w(Range r, Args a) {
   };
   const bool sp = g.bcx == MI_BC_SPECIFIED;
   auto fx = [&](idx_t y, int fi) {
-    const double vel = rux(y);
+    const acc vel = acc(rux(y));
     const int o = ord_face(fi, g.nx, sp);
-    if (o == 2) return H * vel * (w[y] + w[y - 1]);
-    if (o == 3) return vel * flux3(w[y - 2], w[y - 1], w[y], w[y + 1], vel, hup);
-    return vel * flux5(w[y - 3], w[y - 2], w[y - 1], w[y], w[y + 1], w[y + 2], vel, hup);
+    if (o == 2) return acc(H * vel * (w[y] + w[y - 1]));
+    if (o == 3) return acc(vel * flux3<acc>(w[y - 2], w[y - 1], w[y], w[y + 1], vel, acc(hup)));
+    return acc(vel * flux5<acc>(w[y - 3], w[y - 2], w[y - 1], w[y], w[y + 1], w[y + 2], vel, acc(hup)));
   };
   auto fy = [&](idx_t y, int fj) {
-    const double vel = rvy(y);
+    const acc vel = acc(rvy(y));
     const int o = ord_face(fj, g.ny, sp);
-    if (o == 2) return H * vel * (w[y] + w[y - sj]);
-    if (o == 3) return vel * flux3(w[y - 2 * sj], w[y - sj], w[y], w[y + sj], vel, hup);
-    return vel * flux5(w[y - 3 * sj], w[y - 2 * sj], w[y - sj], w[y], w[y + sj], w[y + 2 * sj], vel, hup);
+    if (o == 2) return acc(H * vel * (w[y] + w[y - sj]));
+    if (o == 3) return acc(vel * flux3<acc>(w[y - 2 * sj], w[y - sj], w[y], w[y + sj], vel, acc(hup)));
+    return acc(vel * flux5<acc>(w[y - 3 * sj], w[y - 2 * sj], w[y - sj], w[y], w[y + sj], w[y + 2 * sj], vel, acc(hup)));
   };
   auto fz = [&](int m) {  // vertical flux at mass level m
     const idx_t y = g.at(i, j, m);
-    if (m == 0 || m == nz - 1) return 0.25 * (f.ww[y + sk] + f.ww[y]) * (w[y + sk] + w[y]);
-    const double vel = H * (f.ww[y + sk] + f.ww[y]);
-    return vel * flux3(w[y - sk], w[y], w[y + sk], w[y + 2 * sk], -vel, vup);
+    if (m == 0 || m == nz - 1) return acc(0.25 * (f.ww[y + sk] + f.ww[y]) * (w[y + sk] + w[y]));
+    const acc vel = acc(H * (f.ww[y + sk] + f.ww[y]));
+    return acc(vel * flux3<acc>(w[y - sk], w[y], w[y + sk], w[y + 2 * sk], -vel, acc(vup)));
   };
   const double msftx = f.msftx[x2], msfty = f.msfty[x2];
   double t = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
@@ -682,25 +682,25 @@ MI_POINT_KERNEL k_tend_t(Range r, Args a) {
   const double hup = a.q.hup, vup = a.q.vup;
   const bool sp = g.bcx == MI_BC_SPECIFIED;
   auto fx = [&](idx_t y, int fi) {
-    const double vel = f.ru[y];
+    const acc vel = acc(f.ru[y]);
     const int o = ord_face(fi, g.nx, sp);
-    if (o == 2) return H * vel * (t2[y] + t2[y - 1]);
-    if (o == 3) return vel * flux3(t2[y - 2], t2[y - 1], t2[y], t2[y + 1], vel, hup);
-    return vel * flux5(t2[y - 3], t2[y - 2], t2[y - 1], t2[y], t2[y + 1], t2[y + 2], vel, hup);
+    if (o == 2) return acc(H * vel * (t2[y] + t2[y - 1]));
+    if (o == 3) return acc(vel * flux3<acc>(t2[y - 2], t2[y - 1], t2[y], t2[y + 1], vel, acc(hup)));
+    return acc(vel * flux5<acc>(t2[y - 3], t2[y - 2], t2[y - 1], t2[y], t2[y + 1], t2[y + 2], vel, acc(hup)));
   };
   auto fy = [&](idx_t y, int fj) {
-    const double vel = f.rv[y];
+    const acc vel = acc(f.rv[y]);
     const int o = ord_face(fj, g.ny, sp);
-    if (o == 2) return H * vel * (t2[y] + t2[y - sj]);
-    if (o == 3) return vel * flux3(t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], vel, hup);
-    return vel * flux5(t2[y - 3 * sj], t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], t2[y + 2 * sj], vel, hup);
+    if (o == 2) return acc(H * vel * (t2[y] + t2[y - sj]));
+    if (o == 3) return acc(vel * flux3<acc>(t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], vel, acc(hup)));
+    return acc(vel * flux5<acc>(t2[y - 3 * sj], t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], t2[y + 2 * sj], vel, acc(hup)));
   };
   auto fz = [&](int kk) {
-    if (kk == 0 || kk == nz) return 0.0;
+    if (kk == 0 || kk == nz) return acc(0.0);
     const idx_t y = g.at(i, j, kk);
-    const double vel = f.ww[y];
-    if (kk == 1 || kk == nz - 1) return vel * (c.fnm[kk] * t2[y] + c.fnp[kk] * t2[y - sk]);
-    return vel * flux3(t2[y - 2 * sk], t2[y - sk], t2[y], t2[y + sk], -vel, vup);
+    const acc vel = acc(f.ww[y]);
+    if (kk == 1 || kk == nz - 1) return acc(vel * (c.fnm[kk] * t2[y] + c.fnp[kk] * t2[y - sk]));
+    return acc(vel * flux3<acc>(t2[y - 2 * sk], t2[y - sk], t2[y], t2[y + sk], -vel, acc(vup)));
   };
   const double msftx = f.msftx[x2];
   double rr = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
@@ -1650,34 +1650,35 @@ struct SpList {
 // Face fluxes of one species at the faces of cell (i, j, k) (reference Fluxes, src/dyn/scalars.cpp:42-140;
 // periodic or symmetric edges): q = q_2 (stage value), qo = q_1, with the acoustic-step mass fluxes
 // ru_m, rv_m, ww_m and the stage-end column mass muts.  Each face is recomputed by the cells on both
-// sides with the same expression, which gives the reference's stored face values bit for bit.
+// sides with the same expression, which gives the reference's stored face values bit for bit.  The
+// arithmetic is `acc`: double in FP64 (the reference's expressions), float in FP32-bulk.
 struct ScFaces {
   const Args& a;
   const real* q;
   const real* qo;
-  double dt;
+  acc dt;
   bool pd;
-  __device__ double f5(double m3, double m2, double m1, double q0, double p1, double p2, double v) const {
-    return pd ? flux5_pd(m3, m2, m1, q0, p1, p2, v, a.q.hup) : flux5(m3, m2, m1, q0, p1, p2, v, a.q.hup);
+  __device__ acc f5(acc m3, acc m2, acc m1, acc q0, acc p1, acc p2, acc v) const {
+    return pd ? flux5_pd<acc>(m3, m2, m1, q0, p1, p2, v, acc(a.q.hup)) : flux5<acc>(m3, m2, m1, q0, p1, p2, v, acc(a.q.hup));
   }
-  __device__ double f3(double m2, double m1, double q0, double p1, double v, double up) const {
-    return pd ? flux3_pd(m2, m1, q0, p1, v, up) : flux3(m2, m1, q0, p1, v, up);
+  __device__ acc f3(acc m2, acc m1, acc q0, acc p1, acc v, acc up) const {
+    return pd ? flux3_pd<acc>(m2, m1, q0, p1, v, up) : flux3<acc>(m2, m1, q0, p1, v, up);
   }
   __device__ bool sp() const { return a.g.bcx == MI_BC_SPECIFIED; }
   // x face at U point y (west face of cell fi), the order degraded next to a specified edge
-  __device__ double fx(idx_t y, int fi) const {
-    const double vel = a.f.ru_m[y];
+  __device__ acc fx(idx_t y, int fi) const {
+    const acc vel = a.f.ru_m[y];
     const int o = ord_face(fi, a.g.nx, sp());
-    if (o == 2) return H * vel * (q[y] + q[y - 1]);
-    if (o == 3) return vel * f3(q[y - 2], q[y - 1], q[y], q[y + 1], vel, a.q.hup);
+    if (o == 2) return acc(H) * vel * (acc(q[y]) + acc(q[y - 1]));
+    if (o == 3) return vel * f3(q[y - 2], q[y - 1], q[y], q[y + 1], vel, acc(a.q.hup));
     return vel * f5(q[y - 3], q[y - 2], q[y - 1], q[y], q[y + 1], q[y + 2], vel);
   }
-  __device__ double fy(idx_t y, int fj) const {
+  __device__ acc fy(idx_t y, int fj) const {
     const idx_t sj = a.g.sj;
-    const double vel = a.f.rv_m[y];
+    const acc vel = a.f.rv_m[y];
     const int o = ord_face(fj, a.g.ny, sp());
-    if (o == 2) return H * vel * (q[y] + q[y - sj]);
-    if (o == 3) return vel * f3(q[y - 2 * sj], q[y - sj], q[y], q[y + sj], vel, a.q.hup);
+    if (o == 2) return acc(H) * vel * (acc(q[y]) + acc(q[y - sj]));
+    if (o == 3) return vel * f3(q[y - 2 * sj], q[y - sj], q[y], q[y + sj], vel, acc(a.q.hup));
     return vel * f5(q[y - 3 * sj], q[y - 2 * sj], q[y - sj], q[y], q[y + sj], q[y + 2 * sj], vel);
   }
   // the spec zone (scalars.cpp:100-107): not advected; the limiter leaves the outermost row alone
@@ -1685,40 +1686,69 @@ struct ScFaces {
     return sp() && (i == 0 || i == a.g.nx - 1 || j == 0 || j == a.g.ny - 1);
   }
   // w level kk of the column whose level-0 index is y0
-  __device__ double fz(idx_t y0, int kk) const {
+  __device__ acc fz(idx_t y0, int kk) const {
     const int nz = a.g.nz;
     if (kk == 0 || kk == nz) return 0;
     const idx_t sk = a.g.sk, y = y0 + kk * sk;
-    const double vel = a.f.ww_m[y];
-    if (kk == 1 || kk == nz - 1) return vel * (a.c.fnm[kk] * q[y] + a.c.fnp[kk] * q[y - sk]);
-    return vel * f3(q[y - 2 * sk], q[y - sk], q[y], q[y + sk], -vel, a.q.vup);
+    const acc vel = a.f.ww_m[y];
+    if (kk == 1 || kk == nz - 1) return vel * (acc(a.c.fnm[kk]) * acc(q[y]) + acc(a.c.fnp[kk]) * acc(q[y - sk]));
+    return vel * f3(q[y - 2 * sk], q[y - sk], q[y], q[y + sk], -vel, acc(a.q.vup));
   }
-  __device__ double mh(idx_t y2, int k) const { return a.c.c1h[k] * a.f.muts[y2] + a.c.c2h[k]; }
-  __device__ double fxl(idx_t y, idx_t y2, int k, int fi) const {
+  __device__ acc mh(idx_t y2, int k) const { return acc(a.c.c1h[k]) * acc(a.f.muts[y2]) + acc(a.c.c2h[k]); }
+  // the upwind fluxes of the limiter: the face width dx = 2 / (m + m) / rdx and dx / dt come precomputed
+  // (k_pd_geom: the same expressions, once per face column instead of once per level and species)
+  __device__ acc fxl(idx_t y, idx_t y2, int k, int fi) const {
     const Fields& f = a.f;
-    const double dx = 2.0 / (f.msfty[y2] + f.msfty[y2 - 1]) / a.c.rdx, mu = H * (mh(y2, k) + mh(y2 - 1, k)), vel = f.ru_m[y];
+    const acc dx = acc(f.pd_dx[y2]), mu = acc(H) * (mh(y2, k) + mh(y2 - 1, k)), vel = f.ru_m[y];
     // advect_scalar_pd, degrade_xs, i == ids+1 only: vel = ru/mu, cr = vel*dt/dx (scalars.cpp:88-89)
-    const double cr = (sp() && fi == 1) ? (vel / mu) * dt / dx : vel * dt / dx / mu;
-    return mu * (dx / dt) * flux_upwind(qo[y - 1], qo[y], cr);
+    const acc cr = (sp() && fi == 1) ? qdiv(qdiv(vel, mu) * dt, dx) : qdiv(qdiv(vel * dt, dx), mu);
+    return mu * acc(f.pd_dxdt[y2]) * flux_upwind<acc>(qo[y - 1], qo[y], cr);
   }
-  __device__ double fyl(idx_t y, idx_t y2, int k) const {
+  __device__ acc fyl(idx_t y, idx_t y2, int k) const {
     const Fields& f = a.f;
     const idx_t sj = a.g.sj;
-    const double dy = 2.0 / (f.msftx[y2] + f.msftx[y2 - sj]) / a.c.rdy, mu = H * (mh(y2, k) + mh(y2 - sj, k)),
-                 vel = f.rv_m[y];
-    const double cr = vel * dt / dy / mu;
-    return mu * (dy / dt) * flux_upwind(qo[y - sj], qo[y], cr);
+    const acc dy = acc(f.pd_dy[y2]), mu = acc(H) * (mh(y2, k) + mh(y2 - sj, k)), vel = f.rv_m[y];
+    const acc cr = qdiv(qdiv(vel * dt, dy), mu);
+    return mu * acc(f.pd_dydt[y2]) * flux_upwind<acc>(qo[y - sj], qo[y], cr);
   }
-  __device__ double fzl(idx_t y0, idx_t y2, int kk) const {
+  __device__ acc fzl(idx_t y0, idx_t y2, int kk) const {
     if (kk == 0 || kk == a.g.nz) return 0;
     const idx_t sk = a.g.sk, y = y0 + kk * sk;
-    const double dz = 2.0 / (a.c.rdnw[kk] + a.c.rdnw[kk - 1]), mu = mh(y2, kk), vel = a.f.ww_m[y];
-    const double cr = vel * dt / dz / mu;
-    return mu * (dz / dt) * flux_upwind(qo[y - sk], qo[y], cr);
+    const acc dz = acc(a.f.pd_dz[kk]), mu = mh(y2, kk), vel = a.f.ww_m[y];
+    const acc cr = qdiv(qdiv(vel * dt, dz), mu);
+    return mu * acc(a.f.pd_dz[a.g.nz + 1 + kk]) * flux_upwind<acc>(qo[y - sk], qo[y], cr);
   }
 };
 
-__device__ __forceinline__ double m_full(const Lv& c, int k, double mub, double mu) { return c.c1h[k] * (mub + mu) + c.c2h[k]; }
+// The face widths of the limiter's upwind fluxes (Fluxes::fxl/fyl/fzl, scalars.cpp:118-138) and their ratio
+// to the stage's dt: dx at the x face of (i, j), dy at the y face, dz at w level k (then dz / dt at nz+1+k)
+__global__ void k_pd_geom(Args a, double dt) {
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const int e = int(blockIdx.x) * 256 + int(threadIdx.x);
+  const int w = g.nx + 1, h = g.two_d ? 1 : g.ny + 1;
+  if (e < w * h) {
+    const int i = e % w, j = e / w;
+    const idx_t y2 = g.at2(i, j);
+    const double dx = 2.0 / (f.msfty[y2] + f.msfty[y2 - 1]) / a.c.rdx;
+    f.pd_dx[y2] = dx;
+    f.pd_dxdt[y2] = dx / dt;
+    const double dy = 2.0 / (f.msftx[y2] + f.msftx[y2 - g.sj]) / a.c.rdy;
+    f.pd_dy[y2] = dy;
+    f.pd_dydt[y2] = dy / dt;
+    return;
+  }
+  const int kk = e - w * h;
+  if (kk >= 1 && kk < g.nz) {
+    const double dz = 2.0 / (a.c.rdnw[kk] + a.c.rdnw[kk - 1]);
+    f.pd_dz[kk] = dz;
+    f.pd_dz[g.nz + 1 + kk] = dz / dt;
+  }
+}
+
+__device__ __forceinline__ acc m_full(const Lv& c, int k, double mub, double mu) {
+  return acc(c.c1h[k]) * (acc(mub) + acc(mu)) + acc(c.c2h[k]);
+}
 
 // rk_update_scalar (KUpdate, scalars.cpp:145-158): advect_scalar's tendency and the mass-weighted update,
 // every species of the list; the new values go to the species' second buffer
@@ -1728,20 +1758,20 @@ MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
   const Fields& f = a.f;
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), x0 = g.at(i, j, 0);
-  const double msftx = f.msftx[x2], msfty = f.msfty[x2];
-  const double m_old = m_full(c, k, f.mub[x2], f.mu1[x2]), m_new = m_full(c, k, f.mub[x2], f.mu2[x2]);
+  const acc msftx = acc(f.msftx[x2]), msfty = acc(f.msfty[x2]);
+  const acc m_old = m_full(c, k, f.mub[x2], f.mu1[x2]), m_new = m_full(c, k, f.mub[x2], f.mu2[x2]);
   {
     const int n = l.sp[sq_];
-    const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, false};
-    double adv = 0;
+    const ScFaces fl{a, s.q2[n], s.q1[n], acc(dt_rk), false};
+    acc adv = 0;
     if (!fl.spec_cell(i, j)) {
-      adv = -(msftx * c.rdx) * (fl.fx(x + 1, i + 1) - fl.fx(x, i));
-      if (!g.two_d) adv -= (msftx * c.rdy) * (fl.fy(x + g.sj, j + 1) - fl.fy(x, j));
-      adv = adv - c.rdnw[k] * (fl.fz(x0, k + 1) - fl.fz(x0, k));
+      adv = -(msftx * acc(c.rdx)) * (fl.fx(x + 1, i + 1) - fl.fx(x, i));
+      if (!g.two_d) adv -= (msftx * acc(c.rdy)) * (fl.fy(x + g.sj, j + 1) - fl.fy(x, j));
+      adv = adv - acc(c.rdnw[k]) * (fl.fz(x0, k + 1) - fl.fz(x0, k));
     }
-    double tend = adv * msfty;
-    tend += (a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x];
-    put<kHMass>(s.qn[n], g, i, j, k * g.sk, (m_old * s.q1[n][x] + dt_rk * tend) / m_new);
+    acc tend = adv * msfty;
+    tend += acc((a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x]);
+    put<kHMass>(s.qn[n], g, i, j, k * g.sk, qdiv(m_old * acc(s.q1[n][x]) + acc(dt_rk) * tend, m_new));
   }
 }
 
@@ -1752,35 +1782,36 @@ MI_POINT_KERNEL k_sc_pd_pre(Range r, Args a, Scalars s, SpList l, double dt_rk) 
   const Geo& g = a.g;
   const Fields& f = a.f;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double m = m_full(a.c, k, f.mub[x2], f.mu1[x2]);
+  const acc m = m_full(a.c, k, f.mub[x2], f.mu1[x2]);
   {
     const int n = l.sp[sq_];
-    put<kHMass>(s.q1[n], g, i, j, k * g.sk, (m * s.q1[n][x] + dt_rk * (a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x]) / m);
+    put<kHMass>(s.q1[n], g, i, j, k * g.sk,
+                qdiv(m * acc(s.q1[n][x]) + acc(dt_rk) * acc((a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x]), m));
   }
 }
 
 // advect_scalar_pd's high-minus-low and low-order fluxes of the six faces of a cell (KFacePDX/Y/Z,
 // scalars.cpp:186-212)
 struct PdFaces {
-  double dhx0, lox0, dhx1, lox1, dhy0, loy0, dhy1, loy1, dhz0, loz0, dhz1, loz1;
+  acc dhx0, lox0, dhx1, lox1, dhy0, loy0, dhy1, loy1, dhz0, loz0, dhz1, loz1;
 };
 __device__ __forceinline__ PdFaces pd_faces(const ScFaces& fl, const Geo& g, int i, int j, int k) {
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), x0 = g.at(i, j, 0);
-  const double lox0 = fl.fxl(x, x2, k, i);
-  const double dhx0 = fl.fx(x, i) - lox0;
-  const double lox1 = fl.fxl(x + 1, x2 + 1, k, i + 1);
-  const double dhx1 = fl.fx(x + 1, i + 1) - lox1;
-  double loy0 = 0, dhy0 = 0, loy1 = 0, dhy1 = 0;
+  const acc lox0 = fl.fxl(x, x2, k, i);
+  const acc dhx0 = fl.fx(x, i) - lox0;
+  const acc lox1 = fl.fxl(x + 1, x2 + 1, k, i + 1);
+  const acc dhx1 = fl.fx(x + 1, i + 1) - lox1;
+  acc loy0 = 0, dhy0 = 0, loy1 = 0, dhy1 = 0;
   if (!g.two_d) {
     loy0 = fl.fyl(x, x2, k);
     dhy0 = fl.fy(x, j) - loy0;
     loy1 = fl.fyl(x + g.sj, x2 + g.sj, k);
     dhy1 = fl.fy(x + g.sj, j + 1) - loy1;
   }
-  const double loz0 = fl.fzl(x0, x2, k);
-  const double dhz0 = fl.fz(x0, k) - loz0;
-  const double loz1 = fl.fzl(x0, x2, k + 1);
-  const double dhz1 = fl.fz(x0, k + 1) - loz1;
+  const acc loz0 = fl.fzl(x0, x2, k);
+  const acc dhz0 = fl.fz(x0, k) - loz0;
+  const acc loz1 = fl.fzl(x0, x2, k + 1);
+  const acc dhz1 = fl.fz(x0, k + 1) - loz1;
   return PdFaces{dhx0, lox0, dhx1, lox1, dhy0, loy0, dhy1, loy1, dhz0, loz0, dhz1, loz1};
 }
 
@@ -1792,28 +1823,29 @@ MI_POINT_KERNEL k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk
   const Fields& f = a.f;
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  const double msftx = f.msftx[x2], msfty = f.msfty[x2];
-  const double mxy = msftx * msfty;
+  const acc msftx = acc(f.msftx[x2]), msfty = acc(f.msfty[x2]);
+  const acc mxy = msftx * msfty;
+  const acc dt = acc(dt_rk), zero = 0;
   {
     const int n = l.sp[sq_];
-    const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
+    const ScFaces fl{a, s.q2[n], s.q1[n], dt, true};
     if (fl.spec_cell(i, j)) {  // WRF limits interior cells only
       put<kHMass>(s.scale[n], g, i, j, k * g.sk, 1.0);
       return;
     }
     const PdFaces p = pd_faces(fl, g, i, j, k);
-    double hlow = c.rdx * (p.lox1 - p.lox0);
-    if (!g.two_d) hlow += c.rdy * (p.loy1 - p.loy0);
-    const double ph_low = ((c.c1h[k] * f.mub[x2] + c.c2h[k]) + c.c1h[k] * f.mu1[x2]) * s.q1[n][x] -
-                          dt_rk * (mxy * hlow + msfty * c.rdnw[k] * (p.loz1 - p.loz0));
-    double hout = c.rdx * ((p.dhx1 > 0 ? p.dhx1 : 0.0) - (p.dhx0 < 0 ? p.dhx0 : 0.0));
-    if (!g.two_d) hout += c.rdy * ((p.dhy1 > 0 ? p.dhy1 : 0.0) - (p.dhy0 < 0 ? p.dhy0 : 0.0));
-    const double out = mxy * hout + msfty * c.rdnw[k] * ((p.dhz1 < 0 ? p.dhz1 : 0.0) - (p.dhz0 > 0 ? p.dhz0 : 0.0));
-    const double flux_out = dt_rk * out;
-    double sc = 1;
+    acc hlow = acc(c.rdx) * (p.lox1 - p.lox0);
+    if (!g.two_d) hlow += acc(c.rdy) * (p.loy1 - p.loy0);
+    const acc ph_low = ((acc(c.c1h[k]) * acc(f.mub[x2]) + acc(c.c2h[k])) + acc(c.c1h[k]) * acc(f.mu1[x2])) * acc(s.q1[n][x]) -
+                       dt * (mxy * hlow + msfty * acc(c.rdnw[k]) * (p.loz1 - p.loz0));
+    acc hout = acc(c.rdx) * ((p.dhx1 > zero ? p.dhx1 : zero) - (p.dhx0 < zero ? p.dhx0 : zero));
+    if (!g.two_d) hout += acc(c.rdy) * ((p.dhy1 > zero ? p.dhy1 : zero) - (p.dhy0 < zero ? p.dhy0 : zero));
+    const acc out = mxy * hout + msfty * acc(c.rdnw[k]) * ((p.dhz1 < zero ? p.dhz1 : zero) - (p.dhz0 > zero ? p.dhz0 : zero));
+    const acc flux_out = dt * out;
+    acc sc = 1;
     if (flux_out > ph_low) {
-      sc = ph_low / (flux_out + 1e-20);
-      if (sc < 0) sc = 0;
+      sc = qdiv(ph_low, flux_out + acc(1e-20));
+      if (sc < zero) sc = 0;
     }
     put<kHMass>(s.scale[n], g, i, j, k * g.sk, sc);
   }
@@ -1827,31 +1859,32 @@ MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
   const Lv& c = a.c;
   const int nz = g.nz;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), sj = g.sj, sk = g.sk;
-  const double msftx = f.msftx[x2], msfty = f.msfty[x2];
-  const double m_old = m_full(c, k, f.mub[x2], f.mu1[x2]), m_new = m_full(c, k, f.mub[x2], f.mu2[x2]);
+  const acc msftx = acc(f.msftx[x2]), msfty = acc(f.msfty[x2]);
+  const acc m_old = m_full(c, k, f.mub[x2], f.mu1[x2]), m_new = m_full(c, k, f.mub[x2], f.mu2[x2]);
+  const acc dt = acc(dt_rk);
   {
     const int n = l.sp[sq_];
     const real* sc = s.scale[n];
-    const ScFaces fl{a, s.q2[n], s.q1[n], dt_rk, true};
+    const ScFaces fl{a, s.q2[n], s.q1[n], dt, true};
     if (fl.spec_cell(i, j)) {  // the spec zone: only the held tendency, folded into q_1 before
-      put<kHMass>(s.qn[n], g, i, j, k * sk, m_old * s.q1[n][x] / m_new);
+      put<kHMass>(s.qn[n], g, i, j, k * sk, qdiv(m_old * acc(s.q1[n][x]), m_new));
       return;
     }
     const PdFaces p = pd_faces(fl, g, i, j, k);
     // the limited flux of a face: scaled by the scale of the cell it leaves
-    auto lim = [](double d, double s_minus, double s_plus) { return d * (d > 0 ? s_minus : (d < 0 ? s_plus : 1.0)); };
-    const double sc0 = sc[x];
-    const double dx0 = lim(p.dhx0, sc[x - 1], sc0), dx1 = lim(p.dhx1, sc0, sc[x + 1]);
+    auto lim = [](acc d, acc s_minus, acc s_plus) { return d * (d > acc(0) ? s_minus : (d < acc(0) ? s_plus : acc(1))); };
+    const acc sc0 = sc[x];
+    const acc dx0 = lim(p.dhx0, sc[x - 1], sc0), dx1 = lim(p.dhx1, sc0, sc[x + 1]);
     // eta decreases upward: a positive w-level flux leaves the cell above it
-    const double dz0 = k == 0 ? 0.0 : lim(p.dhz0, sc0, sc[x - sk]);
-    const double dz1 = k + 1 == nz ? 0.0 : lim(p.dhz1, sc[x + sk], sc0);
-    double tend = -c.rdnw[k] * (dz1 - dz0 + p.loz1 - p.loz0);
-    tend -= msftx * (c.rdx * (dx1 - dx0 + p.lox1 - p.lox0));
+    const acc dz0 = k == 0 ? acc(0) : lim(p.dhz0, sc0, sc[x - sk]);
+    const acc dz1 = k + 1 == nz ? acc(0) : lim(p.dhz1, sc[x + sk], sc0);
+    acc tend = -acc(c.rdnw[k]) * (dz1 - dz0 + p.loz1 - p.loz0);
+    tend -= msftx * (acc(c.rdx) * (dx1 - dx0 + p.lox1 - p.lox0));
     if (!g.two_d) {
-      const double dy0 = lim(p.dhy0, sc[x - sj], sc0), dy1 = lim(p.dhy1, sc0, sc[x + sj]);
-      tend -= msftx * (c.rdy * (dy1 - dy0 + p.loy1 - p.loy0));
+      const acc dy0 = lim(p.dhy0, sc[x - sj], sc0), dy1 = lim(p.dhy1, sc0, sc[x + sj]);
+      tend -= msftx * (acc(c.rdy) * (dy1 - dy0 + p.loy1 - p.loy0));
     }
-    put<kHMass>(s.qn[n], g, i, j, k * sk, (m_old * s.q1[n][x] + dt_rk * (tend * msfty)) / m_new);
+    put<kHMass>(s.qn[n], g, i, j, k * sk, qdiv(m_old * acc(s.q1[n][x]) + dt * (tend * msfty), m_new));
   }
 }
 
@@ -2230,6 +2263,12 @@ void launch_cq(const Args& a, const Scalars& s, hipStream_t st) {
 // the kernels of the scalar transport of the species sp[0..n) (step: 0 rk_update_scalar; 1, 2, 3 the
 // positive-definite pre-update, limiter scale and update); algorithmic arrays: the mass fluxes ru_m, rv_m,
 // ww_m once, then per species
+void launch_pd_geom(const Args& a, double dt, hipStream_t st) {
+  const Geo& g = a.g;
+  const int pts = (g.nx + 1) * (g.two_d ? 1 : g.ny + 1) + g.nz + 1;
+  hipLaunchKernelGGL(k_pd_geom, dim3(unsigned((pts + 255) / 256), 1, 1), dim3(256, 1, 1), 0, st, a, dt);
+}
+
 void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, int step, double dt_rk, hipStream_t st) {
   if (n <= 0) return;
   SpList l{};

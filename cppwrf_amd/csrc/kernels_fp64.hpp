@@ -19,6 +19,9 @@
 namespace MI_NS {
 
 using real = MI_REAL;  // storage of the 3-D bulk fields
+// arithmetic of the bulk kernels (advection fluxes, the positive-definite limiter, the scalar updates): double
+// in FP64, float in FP32-bulk; the column solves, column sums and p/al keep double in both
+using acc = MI_REAL;
 
 using mi_dev::BField;
 using mi_dev::Geo;
@@ -53,6 +56,8 @@ struct Fields {
   // the start of the boundary interval (LateralBC::dtbc), kept on the device so a replayed step advances it
   real* ph_tendf;
   double* bdy_time;
+  // the positive-definite limiter's face widths and their ratios to dt (2-D; pd_dz: dz, then dz / dt)
+  double *pd_dx, *pd_dxdt, *pd_dy, *pd_dydt, *pd_dz;
 };
 
 // The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
@@ -169,6 +174,7 @@ void launch_cfl(const Args& a, unsigned long long* out, hipStream_t st);
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st);
 // the transport of species sp[0..n): rk_update_scalar (pd = 0) or rk_update_scalar_pd (pd = 1)
 void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, int step, double dt_rk, hipStream_t st);
+void launch_pd_geom(const Args& a, double dt, hipStream_t st);
 void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hipStream_t st);
 void launch_p_rho_phi(const Args& a, hipStream_t st);
 void launch_finite(const Args& a, int* flags, hipStream_t st);
