@@ -557,13 +557,15 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   G.two_d = two_d;
   d.nxa = g.nx + 1;
   d.nya = G.nyv;
+  // i fastest, then k, then j (WRF's i-k-j memory order): the nz+1 rows of one column group are adjacent, so a
+  // column kernel's wavefront (a few columns at several levels) touches one page per array, not one per level
   const int64_t pitch = (int64_t(kLpad) + d.nxa + kHalo + 31) / 32 * 32;
   const int64_t rows = d.nya + 2 * G.hy;
-  G.sj = two_d ? 0 : pitch;
-  G.sk = pitch * rows;
+  G.sk = pitch;
+  G.sj = two_d ? 0 : pitch * int64_t(g.nz + 1);
   d.off = int64_t(kLpad) + int64_t(G.hy) * G.sj;
-  d.n2 = G.sk;
-  d.n3 = G.sk * int64_t(g.nz + 1);
+  d.n3 = pitch * int64_t(g.nz + 1) * rows;
+  d.n2 = d.n3;  // a 2-D field is indexed with the same j stride: it uses one row of each j's block
 
   auto& f = d.a.f;
   // 3-D fields of the storage type of their Fields member (real, or double for the FP64 islands)
@@ -755,7 +757,10 @@ void Domain::set_field(int id, const double* host, int64_t n) {
   if (base3 && prm.terrain && n == nz + (id == MI_PHB ? 1 : 0)) {
     FieldDesc* fd = find(this, id);
     std::vector<double> h(static_cast<size_t>(n3), 0.0);
-    for (int64_t e = 0; e < n3; ++e) h[size_t(e)] = host[std::min<int64_t>(e / a.g.sk, n - 1)];
+    for (int64_t e = 0; e < n3; ++e) {
+      const int64_t lev = a.g.sj ? (e / a.g.sk) % (nz + 1) : e / a.g.sk;  // row e / sk of j block e / sj
+      h[size_t(e)] = host[std::min<int64_t>(lev, n - 1)];
+    }
     copy_in(static_cast<char*>(*fd->slot) - off * fd->esize, h, fd->esize, st);
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     return;
