@@ -233,6 +233,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default block shapes")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-member batched ensemble")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving trace")
+    ap.add_argument("--error-hours", type=int, default=6,
+                    help="forecast hours over which the FP32-bulk error is reported hourly (0: only after the timed steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each forecast domain runs on a "
@@ -267,7 +269,7 @@ def main():
     # the FP32-bulk error per forecast hour up to 6 h: both domains stepped on to each whole hour
     done = args.warmup + args.steps
     hourly = []
-    for hour in range(1, 7):
+    for hour in range(1, args.error_hours + 1):
         target = int(np.ceil(hour * 3600.0 / g.dt - 1e-9))
         if target > done:
             for rk in doms.values():

@@ -1033,13 +1033,17 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   const bool mk = k < nz, ge1 = k >= 1;  // a mass level; above the surface
 
   // ---------------------------------------------------------------- the load burst
-  double u0 = 0, u1 = 0, us0 = 0, us1 = 0, v0 = 0, v1 = 0, vs0 = 0, vs1 = 0;
-  double ww1 = 0, wwm = 0, ca = 0, cal = 0, cga = 0, rum = 0, rvm = 0;
-  double t2 = 0, ttend = 0, ts = 0, tsxp = 0, tsxm = 0, tsyp = 0, tsym = 0, tskm = 0, tskp = 0;
-  double phtend = 0, w2 = 0, phs = 0, phs_m = 0, phs_p = 0, ph = 0, ph_m = 0, ph_p = 0, rwt = 0;
-  double c2a = 0, c2a_m = 0, alt = 0, alt_m = 0, wsv = 0, pm1 = 0;
+  // the values of the 3-D bulk fields stay in their storage type (half the registers in FP32-bulk); every
+  // expression below converts them to double before it combines two of them
+  real u0 = 0, u1 = 0, us0 = 0, us1 = 0, v0 = 0, v1 = 0, vs0 = 0, vs1 = 0;
+  real ww1 = 0, wwm = 0, rum = 0, rvm = 0;
+  double ca = 0, cal = 0, cga = 0;
+  real t2 = 0, ttend = 0, ts = 0, tsxp = 0, tsxm = 0, tsyp = 0, tsym = 0, tskm = 0, tskp = 0;
+  real phtend = 0, w2 = 0, phs = 0, phs_m = 0, phs_p = 0, ph = 0, ph_m = 0, ph_p = 0, rwt = 0;
+  real c2a = 0, c2a_m = 0, alt = 0, alt_m = 0, wsv = 0, pm1 = 0;
   double msfty = 1, msftx = 1, mut = 0, muu0 = 0, muu1 = 0, muv0 = 0, muv1 = 0, msfuy0 = 1, msfuy1 = 1, mvi0 = 1, mvi1 = 1;
-  double mu_old = 0, mtend = 0, ph_top = 0;
+  double mu_old = 0, mtend = 0;
+  real ph_top = 0;
   if (on) {
     msfty = f.msfty[x2];
     msftx = f.msftx[x2];
@@ -1181,8 +1185,8 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
         const double sv = (first ? 0.0 : rvm) + v0;
         put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
       }
-      double hflux = H * c.rdx * (u1 * (tsxp + ts) - u0 * (ts + tsxm));
-      if (!g.two_d) hflux += H * c.rdy * (v1 * (tsyp + ts) - v0 * (ts + tsym));
+      double hflux = H * c.rdx * (double(u1) * (double(tsxp) + ts) - double(u0) * (double(ts) + tsxm));
+      if (!g.two_d) hflux += H * c.rdy * (double(v1) * (double(tsyp) + ts) - double(v0) * (double(ts) + tsym));
       const double tn = t2 + msfty * dts * ttend;
       t2new = tn - dts * msfty * (msftx * hflux + c.rdnw[k] * (wdtn_kp - wdtn_k));
       f.t2[x] = t2new;
@@ -1194,10 +1198,10 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       Rk = 0;
     } else {
       // wdwn(m): omega'' at the mass level times the layer's geopotential difference
-      const double wdwn_k = H * (wwk + LDS(WW, k - 1, ci)) * c.rdnw[k - 1] * (phs - phs_m + f.phb(i, j, k) - f.phb(i, j, k - 1));
+      const double wdwn_k = H * (wwk + LDS(WW, k - 1, ci)) * c.rdnw[k - 1] * (double(phs) - phs_m + f.phb(i, j, k) - f.phb(i, j, k - 1));
       double rr = dts * (phtend + H * kG * (1.0 - epssm) * w2);
       if (mk) {
-        const double wdwn_kp = H * (LDS(WW, k + 1, ci) + wwk) * c.rdnw[k] * (phs_p - phs + f.phb(i, j, k + 1) - f.phb(i, j, k));
+        const double wdwn_kp = H * (LDS(WW, k + 1, ci) + wwk) * c.rdnw[k] * (double(phs_p) - phs + f.phb(i, j, k + 1) - f.phb(i, j, k));
         rr -= dts * (c.fnm[k] * wdwn_kp + c.fnp[k] * wdwn_k);
       }
       Rk = ph + msfty * rr / (c.c1f[k] * mut + c.c2f[k]);
@@ -1235,13 +1239,13 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
         wk = w2 + dts * rwt +
              msft_inv * (a.q.moist ? 1.0 / (1.0 + f.cqw[x]) : 1.0) *
                  (H * dts * kG * c.rdn[k] *
-                  (c2a * c.rdnw[k] / (c.c1h[k] * m + c.c2h[k]) * (ope * (r_kp1 - r_k) + ome * (ph_p - ph)) -
-                   c2a_m * c.rdnw[k - 1] / (c.c1h[k - 1] * m + c.c2h[k - 1]) * (ope * (r_k - r_km1) + ome * (ph - ph_m)))) +
-             dts * kG * msft_inv * (c.rdn[k] * (c2a * alt * LDS(T2A, k, ci) - c2a_m * alt_m * LDS(T2A, k - 1, ci)) - c.c1f[k] * mav);
+                  (c2a * c.rdnw[k] / (c.c1h[k] * m + c.c2h[k]) * (ope * (r_kp1 - r_k) + ome * (double(ph_p) - ph)) -
+                   c2a_m * c.rdnw[k - 1] / (c.c1h[k - 1] * m + c.c2h[k - 1]) * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)))) +
+             dts * kG * msft_inv * (c.rdn[k] * (double(c2a) * alt * LDS(T2A, k, ci) - double(c2a_m) * alt_m * LDS(T2A, k - 1, ci)) - c.c1f[k] * mav);
       } else {
         const int km = nz - 1;
         wk = w2 + dts * rwt +
-             msft_inv * (-H * dts * kG / (c.c1h[km] * m + c.c2h[km]) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * c2a_m * (ope * (r_k - r_km1) + ome * (ph - ph_m)) -
+             msft_inv * (-H * dts * kG / (c.c1h[km] * m + c.c2h[km]) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * c2a_m * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)) -
                          dts * kG * (2.0 * c.rdnw[km] * c2a_m * alt_m * LDS(T2A, km, ci) + c.c1f[k] * mav));
         if (a.q.top_lid) wk = 0;
       }

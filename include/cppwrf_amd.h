@@ -242,7 +242,9 @@ int mi_check_finite(mi_domain* d, int* bad_field);
  * (prep, acoustic prologue, acoustic substep, finish) into shapes[0..n). */
 int mi_domain_autotune(mi_domain* d, int trials);
 int mi_domain_get_tuning(const mi_domain* d, int* shapes, int n);
-/* sets the shapes explicitly (index 0..6 of the library's table; MI_EINVAL if one does not fit nz) */
+/* sets the shapes explicitly (index 0..11 of the library's table of columns x threads per column: 16x16,
+ * 16x32, 32x16, 8x32, 8x64, 4x128, 2x256, 64x4, 64x8, 64x16, 32x8, 16x64; the substep kernel needs at least
+ * nz+1 threads per column; MI_EINVAL if a shape does not fit nz) */
 int mi_domain_set_tuning(mi_domain* d, const int* shapes, int n);
 
 /* Per-kernel timing, for measurement (no counterpart in the reference; its CPPWRF_KTIME timer,

@@ -132,7 +132,7 @@ def test_autotune_keeps_the_trajectory(ctx):
     gpu.step(2)
     orc.step(2)
     shapes = gpu.autotune(1)
-    assert all(0 <= s < 7 for s in shapes)
+    assert all(0 <= s < 12 for s in shapes)
     gpu.step(4)
     orc.step(4)
     _compare(gpu, orc, STATE, f"after autotune {shapes}")
