@@ -146,12 +146,16 @@ def _proto(lib, name, res, args):
 _VP = C.c_void_p
 _I = C.c_int
 _I64 = C.c_int64
+# mi_cfl_fn / mi_stage_fn
+CFL_FN = C.CFUNCTYPE(None, _VP, C.c_double, C.c_double)
+STAGE_FN = C.CFUNCTYPE(None, _VP)
 
 # name -> (restype, argtypes), every function include/cppwrf_amd.h declares
 SIGNATURES = {
     "mi_last_error": (C.c_char_p, []),
     "mi_abi_version": (_I, []),
     "mi_auto_sound_steps": (_I, [C.c_double, C.c_double, C.c_double]),
+    "mi_adaptive_sound_steps": (_I, [C.c_double, C.c_double, C.c_double, C.c_double]),
     "mi_ctx_create": (_I, [_I, _VP, C.POINTER(_VP)]),
     "mi_ctx_destroy": (_I, [_VP]),
     "mi_domain_create": (_I, [_VP, C.POINTER(MiGrid), C.POINTER(MiParams), C.POINTER(MiLevels), C.POINTER(_VP)]),
@@ -176,6 +180,9 @@ SIGNATURES = {
     "mi_batch_destroy": (_I, [_VP]),
     "mi_batch_step": (_I, [_VP, _I]),
     "mi_cfl_max": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mi_domain_set_dt": (_I, [_VP, C.c_double, _I]),
+    "mi_domain_set_cfl_callback": (_I, [_VP, CFL_FN, _VP]),
+    "mi_domain_set_first_rk_prep_callback": (_I, [_VP, STAGE_FN, _VP]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
     "mi_domain_autotune": (_I, [_VP, _I]),
     "mi_domain_get_tuning": (_I, [_VP, C.POINTER(_I), _I]),

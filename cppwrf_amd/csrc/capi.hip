@@ -118,6 +118,13 @@ int mi_auto_sound_steps(double dt, double dx, double dy) {
   return n > 4 ? n : 4;
 }
 
+// reference RK3::adaptive_sound_steps (src/dyn/rk3.cpp:74-77), in double
+int mi_adaptive_sound_steps(double dt, double dx, double dy, double max_msft) {
+  const double spacing = dx < dy ? dx : dy;
+  const int n = 2 * (int(300.0 * dt / (spacing / max_msft) - 0.01) + 1);
+  return n > 4 ? n : 4;
+}
+
 int mi_ctx_create(int device, void* stream, mi_ctx** out) {
   return mi_detail::guard([&] {
     if (!out) throw Error(MI_EINVAL, "null argument");
@@ -287,6 +294,8 @@ int mi_batch_step(mi_batch* b, int nsteps) {
         hip_check(hipEventRecord(b->fork, d->context()->stream), "hipEventRecord");
         hip_check(hipStreamWaitEvent(b->ctx->stream, b->fork, 0), "hipStreamWaitEvent");
       }
+    for (mi_domain* d : b->m)
+      if (d->has_callbacks()) throw Error(MI_EINVAL, "a batch member has stage callbacks: step it alone");
     for (int s = 0; s < nsteps; ++s) {
       hipGraphExec_t x = b->exec_for(b->key());
       hip_check(hipGraphLaunch(x, b->ctx->stream), "hipGraphLaunch");
@@ -302,6 +311,18 @@ int mi_batch_step(mi_batch* b, int nsteps) {
 
 int mi_cfl_max(mi_domain* d, double* vert, double* horiz) {
   return mi_detail::guard([&] { checked(d)->cfl_max(vert, horiz); });
+}
+
+int mi_domain_set_dt(mi_domain* d, double dt, int n_sound) {
+  return mi_detail::guard([&] { checked(d)->set_dt(dt, n_sound); });
+}
+
+int mi_domain_set_cfl_callback(mi_domain* d, mi_cfl_fn fn, void* user) {
+  return mi_detail::guard([&] { checked(d)->set_cfl_callback(fn, user); });
+}
+
+int mi_domain_set_first_rk_prep_callback(mi_domain* d, mi_stage_fn fn, void* user) {
+  return mi_detail::guard([&] { checked(d)->set_prep_callback(fn, user); });
 }
 
 int mi_check_finite(mi_domain* d, int* bad_field) {

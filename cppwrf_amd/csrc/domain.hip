@@ -39,6 +39,19 @@ struct FieldDesc {
   int esize;    // bytes per element: 8 (FP64) or 4 (FP32 storage)
 };
 
+// the parameters that follow the large step: w_damp's dt, the sixth-order coefficient's dt and, with specified
+// boundaries, lbc_fcx_gcx (lateral_bc.cpp:198-211; Fortran loop = b + 1 over spec_zone + 1 .. spec_zone + relax_zone)
+void set_dt_params(Params& q, double dt, bool w_damping, bool spec) {
+  q.wd_dt = w_damping ? dt : 0.0;
+  q.dt = dt;
+  if (spec)
+    for (int loop = 2; loop <= 5; ++loop) {
+      const double num = double(1 + 4 - loop), den = double(4 - 1);
+      q.fcx[loop - 1] = 0.1 / dt * num / den;
+      q.gcx[loop - 1] = 1.0 / dt / 50.0 * num / den;
+    }
+}
+
 }  // namespace
 
 struct Domain final : mi_domain {
@@ -336,15 +349,48 @@ struct Domain final : mi_domain {
     }
   }
 
-  void step_launches() {
+  // RK3::step (rk3.cpp:86-170); with callbacks (host-driven steps only) the stage hooks of the reference:
+  // on_first_rk_prep after stage 1's prep (rk3.cpp:105), on_cfl after each stage's tendencies (rk3.cpp:115-119)
+  void step_launches(bool hooks = false) {
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
       rk_prep(rk);
+      if (hooks && rk == 1 && prep_fn) {
+        hip_check(hipStreamSynchronize(stream()), "hipStreamSynchronize");
+        prep_fn(prep_user);
+      }
       rk_tendency(rk);
+      if (hooks && cfl_fn) {
+        double vert = 0, horiz = 0;
+        cfl_max(&vert, &horiz);
+        cfl_fn(cfl_user, vert, horiz);
+      }
       small_steps(rk, true);
       scalar_transport(rk);
     }
     step_end();
+  }
+  mi_cfl_fn cfl_fn = nullptr;
+  void* cfl_user = nullptr;
+  mi_stage_fn prep_fn = nullptr;
+  void* prep_user = nullptr;
+  void set_cfl_callback(mi_cfl_fn fn, void* user) override {
+    cfl_fn = fn;
+    cfl_user = user;
+  }
+  void set_prep_callback(mi_stage_fn fn, void* user) override {
+    prep_fn = fn;
+    prep_user = user;
+  }
+  bool has_callbacks() const override { return cfl_fn != nullptr || prep_fn != nullptr; }
+  // RK3::set_dt with LateralBC::set_dt and the diffusion's dt (the adaptive time step)
+  void set_dt(double dt, int ns) override {
+    if (!(dt > 0)) throw Error(MI_EINVAL, "dt must be > 0");
+    if (ns < 2 || ns % 2) throw Error(MI_EINVAL, "time_step_sound must be even and >= 2");
+    grid.dt = dt;
+    n_sound_ = ns;
+    set_dt_params(a.q, dt, prm.w_damping == 1, spec);
+    drop_graphs();
   }
 
   // the fields a step reads from the previous one (everything else is recomputed within a step)
@@ -423,6 +469,11 @@ struct Domain final : mi_domain {
 
   void step(int nsteps) override {
     if (nsteps <= 0) return;
+    if (has_callbacks()) {
+      for (int s = 0; s < nsteps; ++s) timed([&] { step_launches(true); });
+      hip_check(hipGetLastError(), "kernel launch");
+      return;
+    }
     if (!use_graph || !prof_name.empty()) {
       for (int s = 0; s < nsteps; ++s) timed([&] { step_launches(); });
       hip_check(hipGetLastError(), "kernel launch");
@@ -692,7 +743,7 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   q.smdiv = p.smdiv;
   q.emdx = p.emdiv * (1. / lv.rdx);
   q.emdy = two_d ? 0.0 : p.emdiv * (1. / lv.rdy);
-  q.wd_dt = p.w_damping == 1 ? g.dt : 0.0;
+  q.wd_dt = 0.0;
   q.w_crit = p.w_crit_cfl;
   q.dampcoef = p.dampcoef;
   q.zdamp = p.zdamp;
@@ -703,15 +754,9 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   q.cor = (p.mapped != 0 || p.coriolis != 0) ? 1 : 0;
   q.use_tendf = held ? 1 : 0;
   q.spec = d.spec ? 1 : 0;
-  if (d.spec)  // lbc_fcx_gcx (lateral_bc.cpp:198-211), Fortran loop = b + 1 over spec_zone + 1 .. spec_zone + relax_zone
-    for (int loop = 2; loop <= 5; ++loop) {
-      const double num = double(1 + 4 - loop), den = double(4 - 1);
-      q.fcx[loop - 1] = 0.1 / g.dt * num / den;
-      q.gcx[loop - 1] = 1.0 / g.dt / 50.0 * num / den;
-    }
   q.diff_6th_opt = p.diff_6th_opt;
   q.diff_6th_factor = p.diff_6th_factor;
-  q.dt = g.dt;
+  set_dt_params(q, g.dt, p.w_damping == 1, d.spec);
   q.terrain = p.terrain;
 }
 

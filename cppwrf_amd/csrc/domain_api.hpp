@@ -41,6 +41,10 @@ struct mi_domain {
   virtual void profile_read(double* total_ms, int* launches, double* alg_bytes) = 0;
   virtual int check_finite(int* bad_field) = 0;  // -1 or the first field id with a non-finite value
   virtual void cfl_max(double* vert, double* horiz) = 0;
+  virtual void set_dt(double dt, int n_sound) = 0;
+  virtual void set_cfl_callback(mi_cfl_fn fn, void* user) = 0;
+  virtual void set_prep_callback(mi_stage_fn fn, void* user) = 0;
+  virtual bool has_callbacks() const = 0;
   // specified lateral boundaries (LateralBC::set / set_scalar, dtbc)
   virtual void bdy_set(int var, int side, const double* values, const double* tends, int64_t n) = 0;
   virtual void bdy_set_time(double dtbc) = 0;

@@ -13,6 +13,7 @@
 // bad parameters (src/dyn/rk3.cpp:50).
 #pragma once
 
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -144,9 +145,26 @@ class RK3 {
 
   // reference RK3::auto_sound_steps (src/dyn/rk3.cpp:69-72)
   static int auto_sound_steps(double dt, double dx, double dy) { return mi_auto_sound_steps(dt, dx, dy); }
+  // reference RK3::adaptive_sound_steps (src/dyn/rk3.cpp:74-77)
+  static int adaptive_sound_steps(double dt, double dx, double dy, double max_msft) {
+    return mi_adaptive_sound_steps(dt, dx, dy, max_msft);
+  }
   int n_sound() const { return mi_domain_n_sound(d_); }
+  // reference RK3::set_dt (src/dyn/rk3.cpp:79-84), with the boundary weights and the diffusion's dt
+  void set_dt(double dt, int n_sound) {
+    check(mi_domain_set_dt(d_, dt, n_sound));
+    g_.dt = dt;
+  }
+  // reference RK3::on_cfl and on_first_rk_prep (rk3.hpp:46-47, 62-64): while either is set, steps run launch by
+  // launch and call them at the reference's points
+  std::function<void(double, double)> on_cfl;
+  std::function<void()> on_first_rk_prep;
   // reference RK3::step (src/dyn/rk3.cpp:86-170); nsteps > 1 runs them back to back on the device
-  void step(int nsteps = 1) { check(mi_step(d_, nsteps)); }
+  void step(int nsteps = 1) {
+    check(mi_domain_set_cfl_callback(d_, on_cfl ? &RK3::cfl_thunk : nullptr, this));
+    check(mi_domain_set_first_rk_prep_callback(d_, on_first_rk_prep ? &RK3::prep_thunk : nullptr, this));
+    check(mi_step(d_, nsteps));
+  }
   void sync() { check(mi_domain_sync(d_)); }
   StateView state() { return StateView(d_, g_); }
   mi_domain* handle() const { return d_; }
@@ -166,6 +184,8 @@ class RK3 {
   }
 
  private:
+  static void cfl_thunk(void* self, double vert, double horiz) { static_cast<RK3*>(self)->on_cfl(vert, horiz); }
+  static void prep_thunk(void* self) { static_cast<RK3*>(self)->on_first_rk_prep(); }
   mi_grid g_;
   mi_domain* d_ = nullptr;
 };

@@ -11,6 +11,7 @@ The arrays are numpy float64 in the C ABI's dense layout: shape (nzs, nys, nxs),
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 from dataclasses import dataclass
 
 import numpy as np
@@ -250,6 +251,38 @@ class RK3:
     def tuning(self, shapes) -> None:
         a = (C.c_int * 4)(*shapes)
         check(lib().mi_domain_set_tuning(self._h, a, 4))
+
+    @staticmethod
+    def adaptive_sound_steps(dt: float, dx: float, dy: float, max_msft: float) -> int:
+        """RK3::adaptive_sound_steps: the acoustic step count of an adaptive time step (rk3.cpp:74-77)"""
+        return lib().mi_adaptive_sound_steps(dt, dx, dy, max_msft)
+
+    def set_dt(self, dt: float, n_sound: int) -> None:
+        """RK3::set_dt with the boundary relaxation weights and the diffusion's dt (the adaptive time step)"""
+        check(lib().mi_domain_set_dt(self._h, dt, n_sound))
+        self.grid = dataclasses.replace(self.grid, dt=dt)
+
+    @property
+    def on_cfl(self):
+        """RK3::on_cfl: f(vert, horiz) after each stage's tendencies (steps then run launch by launch)"""
+        return getattr(self, "_on_cfl", None)
+
+    @on_cfl.setter
+    def on_cfl(self, f) -> None:
+        self._on_cfl = f
+        self._cfl_thunk = A.CFL_FN(lambda _u, v, h: f(v, h)) if f is not None else A.CFL_FN()
+        check(lib().mi_domain_set_cfl_callback(self._h, self._cfl_thunk, None))
+
+    @property
+    def on_first_rk_prep(self):
+        """RK3::on_first_rk_prep: f() after stage 1's rk_step_prep"""
+        return getattr(self, "_on_prep", None)
+
+    @on_first_rk_prep.setter
+    def on_first_rk_prep(self, f) -> None:
+        self._on_prep = f
+        self._prep_thunk = A.STAGE_FN(lambda _u: f()) if f is not None else A.STAGE_FN()
+        check(lib().mi_domain_set_first_rk_prep_callback(self._h, self._prep_thunk, None))
 
     def cfl_max(self) -> tuple[float, float]:
         """w_damp's vertical and horizontal Courant-number maxima of the current fields (LargeStep::cfl_max)"""

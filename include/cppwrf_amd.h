@@ -148,6 +148,9 @@ int mi_abi_version(void);
 
 /* reference RK3::auto_sound_steps (src/dyn/rk3.cpp:69-72): max(2*(int(300 dt/min(dx,dy) - 0.01)+1), 4) */
 int mi_auto_sound_steps(double dt, double dx, double dy);
+/* RK3::adaptive_sound_steps (rk3.hpp:40-41, rk3.cpp:74-77): the acoustic step count of an adaptive time step,
+ * max(2*(int(300*dt/(min(dx,dy)/max_msft) - 0.01) + 1), 4), evaluated in double as the reference's FP64 build */
+int mi_adaptive_sound_steps(double dt, double dx, double dy, double max_msft);
 
 /* A context on HIP device `device`.  stream: a hipStream_t to launch on, or NULL for a stream the
  * context creates (reference: the thread-local launch stream, src/core/backend.cpp:73-81). */
@@ -220,6 +223,20 @@ int mi_bdy_get_time(mi_domain* d, double* dtbc);
  * max(|u rdx msfux dt|, |v rdy msfvy dt|) over levels 1..nz-1; a NaN never wins).  Called between
  * mi_rk_tendency and mi_small_steps it gives the reference's on_cfl values of that stage. */
 int mi_cfl_max(mi_domain* d, double* vert, double* horiz);
+
+/* Adaptive time step.  mi_domain_set_dt is RK3::set_dt (rk3.cpp:79-84) together with what an adaptive-step
+ * driver changes beside it (WRF adapt_timestep): the specified boundaries' relaxation weights
+ * (LateralBC::set_dt, lateral_bc.cpp:198-211) and the sixth-order diffusion's dt.  The next steps use the
+ * large step dt (> 0) and n_sound acoustic steps (even, >= 2); the boundary clock is kept. */
+int mi_domain_set_dt(mi_domain* d, double dt, int n_sound);
+/* RK3::on_cfl (rk3.hpp:46-47, rk3.cpp:115-119): called after each stage's rk_tendency with LargeStep::cfl_max's
+ * maxima (vertical, horizontal).  RK3::on_first_rk_prep (rk3.hpp:62-64, rk3.cpp:105): called in stage 1 right
+ * after rk_step_prep.  NULL removes a callback.  While either is set, mi_step runs its steps launch by launch (not
+ * as a replayed graph), synchronising the stream before each call; a batch refuses such a member. */
+typedef void (*mi_cfl_fn)(void* user, double vert, double horiz);
+typedef void (*mi_stage_fn)(void* user);
+int mi_domain_set_cfl_callback(mi_domain* d, mi_cfl_fn fn, void* user);
+int mi_domain_set_first_rk_prep_callback(mi_domain* d, mi_stage_fn fn, void* user);
 
 /* Batches (ensembles, serving): the steps of several domains of one context advance together.  Each
  * step of the batch is one HIP graph holding every member's step as an independent branch, so the

@@ -265,17 +265,28 @@ struct Dom {
     use_tendf = p.diff_6th_opt != 0 || spec;  // a DiffusionDriver or a LateralBC attached (large_step.cpp:828)
     terrain = p.terrain != 0;          // !b.ht.uniform() (small_step.cpp:583, rk3.cpp:163)
     if (spec) {
-      // lbc_fcx_gcx (lateral_bc.cpp:198-211), Fortran loop = b + 1 over spec_zone + 1 .. spec_zone + relax_zone
-      for (int loop = kSpecZone + 1; loop <= kSpecZone + kRelaxZone; ++loop) {
-        const double num = double(kSpecZone + kRelaxZone - loop), den = double(kRelaxZone - 1);
-        fcx[loop - 1] = 0.1 / g.dt * num / den;
-        gcx[loop - 1] = 1.0 / g.dt / 50.0 * num / den;
-      }
+      set_fcx();
       const int nkv[5] = {nz, nz, nz, nz + 1, 1};  // U V T PH MU (lateral_bc.cpp:219)
       for (int v = 0; v < 5; ++v) init_bdy(bdy[v], nkv[v], v == 0, v == 1);
       qbdy.resize(size_t(n_sc));
       for (Bdy& b : qbdy) init_bdy(b, nz, false, false);
     }
+  }
+  // lbc_fcx_gcx (LateralBC::set_dt, lateral_bc.cpp:198-211), Fortran loop = b + 1 over spec_zone + 1 ..
+  // spec_zone + relax_zone
+  void set_fcx() {
+    for (int loop = kSpecZone + 1; loop <= kSpecZone + kRelaxZone; ++loop) {
+      const double num = double(kSpecZone + kRelaxZone - loop), den = double(kRelaxZone - 1);
+      fcx[loop - 1] = 0.1 / g.dt * num / den;
+      gcx[loop - 1] = 1.0 / g.dt / 50.0 * num / den;
+    }
+  }
+  // RK3::set_dt (rk3.cpp:79-84) with LateralBC::set_dt and DiffusionDriver::set_dt (every user of the grid's dt
+  // reads it at its use)
+  void set_dt(double dt, int ns) {
+    g.dt = dt;
+    n_sound = ns;
+    if (spec) set_fcx();
   }
   void init_bdy(Bdy& b, int nk, bool u, bool v) {  // lateral_bc.cpp:225-233
     b.nk = nk;
@@ -1692,6 +1703,12 @@ int orc_small_steps(orc_domain* d, int rk) {
 int orc_scalar_transport(orc_domain* d, int rk) {
   if (rk < 1 || rk > 3) return fail("rk must be 1, 2 or 3");
   d->d->scalars(rk);
+  return MI_OK;
+}
+int orc_set_dt(orc_domain* d, double dt, int n_sound) {
+  if (!(dt > 0)) return fail("dt must be > 0");
+  if (n_sound < 2 || n_sound % 2) return fail("time_step_sound must be even and >= 2");
+  d->d->set_dt(dt, n_sound);
   return MI_OK;
 }
 int orc_cfl_max(orc_domain* d, double* vert, double* horiz) {

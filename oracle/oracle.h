@@ -38,6 +38,7 @@ int orc_rk_tendency(orc_domain* d, int rk);
 int orc_small_steps(orc_domain* d, int rk);
 int orc_scalar_transport(orc_domain* d, int rk); /* ScalarTransport::run_stage */
 int orc_cfl_max(orc_domain* d, double* vert, double* horiz); /* LargeStep::cfl_max */
+int orc_set_dt(orc_domain* d, double dt, int n_sound);        /* RK3::set_dt + LateralBC::set_dt (rk3.cpp:79-84) */
 int orc_calc_p_rho_phi(orc_domain* d, int rk);
 int orc_step_end(orc_domain* d); /* spec_bdy_final, set_w_surface */
 /* specified boundaries: as mi_bdy_set / mi_bdy_set_time / mi_bdy_get_time */

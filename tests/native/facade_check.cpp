@@ -13,6 +13,8 @@ int main() {
   if (dyn::RK3::auto_sound_steps(6.0, 1000.0, 1000.0) != 4) return 2;
   if (dyn::RK3::auto_sound_steps(90.0, 15000.0, 15000.0) != 4) return 3;
   if (dyn::RK3::auto_sound_steps(60.0, 3000.0, 3000.0) != 12) return 4;
+  // reference RK3::adaptive_sound_steps: the spacing divided by the largest map factor
+  if (dyn::RK3::adaptive_sound_steps(10.0, 1000.0, 1000.0, 1.2) != 8) return 12;
   // dense ABI sizes of the staggered fields (x-y-z and an x-z slice, whose v has no extra row)
   const mi_grid g3{16, 12, 10, MI_BC_PERIODIC, MI_BC_PERIODIC, 1000.0, 1000.0, 6.0};
   const mi_grid g2{16, 1, 10, MI_BC_PERIODIC, MI_BC_PERIODIC, 1000.0, 1000.0, 6.0};

@@ -40,6 +40,7 @@ def olib():
             "orc_small_steps": (I, [VP, I]),
             "orc_scalar_transport": (I, [VP, I]),
             "orc_cfl_max": (I, [VP, PD, PD]),
+            "orc_set_dt": (I, [VP, C.c_double, I]),
             "orc_calc_p_rho_phi": (I, [VP, I]),
             "orc_step_end": (I, [VP]),
             "orc_bdy_set": (I, [VP, I, I, PD, PD, C.c_int64]),
@@ -122,6 +123,9 @@ class OracleDomain:
 
     def small_steps(self, rk):
         _ok(olib().orc_small_steps(self._h, rk))
+
+    def set_dt(self, dt, n_sound):
+        _ok(olib().orc_set_dt(self._h, dt, n_sound))
 
     def cfl_max(self):
         v, h = C.c_double(), C.c_double()

@@ -39,6 +39,19 @@ def test_auto_sound_steps(dt, dx, dy, want):
     assert A.lib().mi_auto_sound_steps(dt, dx, dy) == want
 
 
+@pytest.mark.parametrize("dt,dx,dy,msft,want", [
+    (60.0, 15000.0, 15000.0, 1.0, 4),     # 300*60/15000 = 1.2 -> 2*(1+1) = 4
+    (60.0, 15000.0, 15000.0, 1.3, 4),     # spacing 11538: int(1.56 - 0.01) = 1 -> 4
+    (90.0, 15000.0, 12000.0, 1.05, 6),    # 300*90*1.05/12000 = 2.3625 -> 2*(2+1) = 6
+    (10.0, 1000.0, 1000.0, 1.0, 6),       # 3.0 - 0.01 -> 2 -> 6
+    (10.0, 1000.0, 1000.0, 1.2, 8),       # 3.6 -> 3 -> 8
+])
+def test_adaptive_sound_steps(dt, dx, dy, msft, want):
+    # reference RK3::adaptive_sound_steps (src/dyn/rk3.cpp:74-77): the spacing divided by the largest map factor
+    n = int(300.0 * dt / (min(dx, dy) / msft) - 0.01)
+    assert A.lib().mi_adaptive_sound_steps(dt, dx, dy, msft) == max(2 * (n + 1), 4) == want
+
+
 def test_case_levels_are_consistent():
     c = Case.bubble3d(16, 12, 20, dx=1000.0, ztop=10000.0, dt=6.0)
     lv = c.levels
