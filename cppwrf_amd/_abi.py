@@ -181,6 +181,9 @@ SIGNATURES = {
     "mi_batch_step": (_I, [_VP, _I]),
     "mi_cfl_max": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mi_domain_set_dt": (_I, [_VP, C.c_double, _I]),
+    "mi_domain_state_bytes": (_I64, [_VP]),
+    "mi_domain_save_state": (_I, [_VP, _VP, _I64]),
+    "mi_domain_load_state": (_I, [_VP, _VP, _I64]),
     "mi_domain_set_cfl_callback": (_I, [_VP, CFL_FN, _VP]),
     "mi_domain_set_first_rk_prep_callback": (_I, [_VP, STAGE_FN, _VP]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),

@@ -317,6 +317,20 @@ int mi_domain_set_dt(mi_domain* d, double dt, int n_sound) {
   return mi_detail::guard([&] { checked(d)->set_dt(dt, n_sound); });
 }
 
+int64_t mi_domain_state_bytes(mi_domain* d) {
+  int64_t b = MI_EINVAL;
+  const int rc = mi_detail::guard([&] { b = checked(d)->state_bytes(); });
+  return rc == MI_OK ? b : rc;
+}
+
+int mi_domain_save_state(mi_domain* d, void* host, int64_t bytes) {
+  return mi_detail::guard([&] { checked(d)->save_state(host, bytes); });
+}
+
+int mi_domain_load_state(mi_domain* d, const void* host, int64_t bytes) {
+  return mi_detail::guard([&] { checked(d)->load_state(host, bytes); });
+}
+
 int mi_domain_set_cfl_callback(mi_domain* d, mi_cfl_fn fn, void* user) {
   return mi_detail::guard([&] { checked(d)->set_cfl_callback(fn, user); });
 }

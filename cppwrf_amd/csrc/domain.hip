@@ -426,6 +426,76 @@ struct Domain final : mi_domain {
     }
   }
 
+  // ------------------------------------------------------------------ checkpoint and resume
+  // One host buffer: a header that names the domain's shape and storage, then every carried allocation (halos
+  // included) in carried() order, then the boundary clock.
+  struct StateHeader {
+    char magic[8];
+    int32_t version, nx, ny, nz, real_bytes, n_sc, n_fields, spec;
+    int64_t n3, n2, payload;
+  };
+  int64_t state_bytes() override {
+    int64_t b = int64_t(sizeof(StateHeader)) + int64_t(sizeof(double));
+    for (const Carried& c : carried()) b += c.bytes;
+    return b;
+  }
+  StateHeader state_header() {
+    StateHeader h{};
+    const char m[8] = {'M', 'I', 'S', 'T', 'A', 'T', 'E', '1'};
+    for (int q = 0; q < 8; ++q) h.magic[q] = m[q];
+    h.version = 1;
+    h.nx = grid.nx;
+    h.ny = grid.ny;
+    h.nz = grid.nz;
+    h.real_bytes = int32_t(sizeof(real));
+    h.n_sc = n_sc;
+    h.n_fields = int32_t(carried().size());
+    h.spec = spec ? 1 : 0;
+    h.n3 = n3;
+    h.n2 = n2;
+    h.payload = state_bytes();
+    return h;
+  }
+  char* carried_base(const Carried& c) {
+    const int64_t elt = c.p == static_cast<void*>(a.f.mu2) ? int64_t(sizeof(double)) : int64_t(sizeof(real));
+    return static_cast<char*>(c.p) - off * elt;
+  }
+  void save_state(void* host, int64_t bytes) override {
+    if (!host || bytes < state_bytes()) throw Error(MI_EINVAL, "state buffer too small: mi_domain_state_bytes gives the size");
+    const StateHeader h = state_header();
+    char* o = static_cast<char*>(host);
+    std::memcpy(o, &h, sizeof(h));
+    o += sizeof(h);
+    for (const Carried& c : carried()) {
+      hip_check(hipMemcpyAsync(o, carried_base(c), size_t(c.bytes), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+      o += c.bytes;
+    }
+    double t = spec ? bdy_get_time() : 0.0;
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    std::memcpy(o, &t, sizeof(t));
+  }
+  void load_state(const void* host, int64_t bytes) override {
+    if (!host || bytes < int64_t(sizeof(StateHeader))) throw Error(MI_EINVAL, "state buffer too small");
+    StateHeader h{};
+    std::memcpy(&h, host, sizeof(h));
+    const StateHeader mine = state_header();
+    if (std::memcmp(h.magic, mine.magic, 8) != 0 || h.version != mine.version)
+      throw Error(MI_EINVAL, "not a state buffer of this library");
+    if (h.nx != mine.nx || h.ny != mine.ny || h.nz != mine.nz || h.real_bytes != mine.real_bytes || h.n_sc != mine.n_sc ||
+        h.n_fields != mine.n_fields || h.spec != mine.spec || h.n3 != mine.n3 || h.n2 != mine.n2)
+      throw Error(MI_EINVAL, "the state buffer belongs to a domain of another shape, precision or scalar count");
+    if (bytes < h.payload) throw Error(MI_EINVAL, "state buffer truncated");
+    const char* o = static_cast<const char*>(host) + sizeof(h);
+    for (const Carried& c : carried()) {
+      hip_check(hipMemcpyAsync(carried_base(c), o, size_t(c.bytes), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+      o += c.bytes;
+    }
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    double t = 0;
+    std::memcpy(&t, o, sizeof(t));
+    if (spec) bdy_set_time(t);
+  }
+
   // Time every block shape of each column kernel on this domain and keep the fastest; the state is
   // restored afterwards, so the forecast continues exactly as if no step had been taken.
   void autotune(int trials) override {

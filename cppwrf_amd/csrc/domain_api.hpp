@@ -42,6 +42,9 @@ struct mi_domain {
   virtual int check_finite(int* bad_field) = 0;  // -1 or the first field id with a non-finite value
   virtual void cfl_max(double* vert, double* horiz) = 0;
   virtual void set_dt(double dt, int n_sound) = 0;
+  virtual int64_t state_bytes() = 0;
+  virtual void save_state(void* host, int64_t bytes) = 0;
+  virtual void load_state(const void* host, int64_t bytes) = 0;
   virtual void set_cfl_callback(mi_cfl_fn fn, void* user) = 0;
   virtual void set_prep_callback(mi_stage_fn fn, void* user) = 0;
   virtual bool has_callbacks() const = 0;

@@ -155,6 +155,15 @@ class RK3 {
     check(mi_domain_set_dt(d_, dt, n_sound));
     g_.dt = dt;
   }
+  // checkpoint and resume: the carried state as an opaque byte buffer (no counterpart in src/dyn)
+  std::vector<unsigned char> save_state() {
+    const int64_t n = mi_domain_state_bytes(d_);
+    if (n < 0) check(int(n));
+    std::vector<unsigned char> b(static_cast<size_t>(n));
+    check(mi_domain_save_state(d_, b.data(), n));
+    return b;
+  }
+  void load_state(const std::vector<unsigned char>& b) { check(mi_domain_load_state(d_, b.data(), int64_t(b.size()))); }
   // reference RK3::on_cfl and on_first_rk_prep (rk3.hpp:46-47, 62-64): while either is set, steps run launch by
   // launch and call them at the reference's points
   std::function<void(double, double)> on_cfl;

@@ -252,6 +252,20 @@ class RK3:
         a = (C.c_int * 4)(*shapes)
         check(lib().mi_domain_set_tuning(self._h, a, 4))
 
+    def save_state(self) -> np.ndarray:
+        """the carried state (halos included) and the boundary clock as an opaque byte buffer (checkpoint)"""
+        n = lib().mi_domain_state_bytes(self._h)
+        if n < 0:
+            check(int(n))
+        buf = np.zeros(int(n), dtype=np.uint8)
+        check(lib().mi_domain_save_state(self._h, buf.ctypes.data_as(C.c_void_p), buf.size))
+        return buf
+
+    def load_state(self, buf: np.ndarray) -> None:
+        """resume from a buffer of save_state (a domain of the same grid, parameters and inputs)"""
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        check(lib().mi_domain_load_state(self._h, buf.ctypes.data_as(C.c_void_p), buf.size))
+
     @staticmethod
     def adaptive_sound_steps(dt: float, dx: float, dy: float, max_msft: float) -> int:
         """RK3::adaptive_sound_steps: the acoustic step count of an adaptive time step (rk3.cpp:74-77)"""

@@ -233,6 +233,15 @@ int mi_domain_set_dt(mi_domain* d, double dt, int n_sound);
  * maxima (vertical, horizontal).  RK3::on_first_rk_prep (rk3.hpp:62-64, rk3.cpp:105): called in stage 1 right
  * after rk_step_prep.  NULL removes a callback.  While either is set, mi_step runs its steps launch by launch (not
  * as a replayed graph), synchronising the stream before each call; a batch refuses such a member. */
+/* Checkpoint and resume.  No counterpart in src/dyn: the reference restarts from NetCDF restart files
+ * (tests/robustness/run_test.sh:40-51); here the state a step carries to the next (u, v, w, theta, ph, mu, p, al
+ * and every scalar, halos included, and the boundary clock) goes to one opaque host buffer of
+ * mi_domain_state_bytes bytes.  A domain created with the same grid, parameters, base state and boundary data and
+ * loaded from the buffer continues bit for bit; a buffer of another shape, precision or scalar count is refused
+ * (MI_EINVAL).  mi_domain_state_bytes returns a negative status on error. */
+int64_t mi_domain_state_bytes(mi_domain* d);
+int mi_domain_save_state(mi_domain* d, void* host, int64_t bytes);
+int mi_domain_load_state(mi_domain* d, const void* host, int64_t bytes);
 typedef void (*mi_cfl_fn)(void* user, double vert, double horiz);
 typedef void (*mi_stage_fn)(void* user);
 int mi_domain_set_cfl_callback(mi_domain* d, mi_cfl_fn fn, void* user);
