@@ -108,8 +108,8 @@ class Case:
 
     # the workloads of this library and the reference's idealized cases
     @classmethod
-    def bubble3d(cls, nx, ny, nz, dx, ztop, dt, seed=1, noise_amp=0.05, dy=None, bc=A.MI_BC_PERIODIC, **kw):
-        g = Grid(nx, ny, nz, dx, dy if dy is not None else dx, dt, bc, bc)
+    def bubble3d(cls, nx, ny, nz, dx, ztop, dt, seed=1, noise_amp=0.05, dy=None, bc=A.MI_BC_PERIODIC, bc_y=None, **kw):
+        g = Grid(nx, ny, nz, dx, dy if dy is not None else dx, dt, bc, bc if bc_y is None else bc_y)
         return cls(A.MI_CASE_BUBBLE3D, g, ztop, seed=seed, noise_amp=noise_amp, **kw)
 
     @classmethod
@@ -121,15 +121,17 @@ class Case:
         return cls(A.MI_CASE_WARM_BUBBLE, Grid(200, 1, 100, 100.0, 100.0, 1.0, A.MI_BC_SYMMETRIC), 10000.0)
 
     @classmethod
-    def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0, bc=A.MI_BC_PERIODIC):
-        return cls(A.MI_CASE_REST_STATE, Grid(nx, ny, nz, 2000.0, 2000.0, 10.0, bc, bc), 20000.0, u0=u0)
+    def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0, bc=A.MI_BC_PERIODIC, bc_y=None):
+        return cls(A.MI_CASE_REST_STATE, Grid(nx, ny, nz, 2000.0, 2000.0, 10.0, bc, bc if bc_y is None else bc_y), 20000.0,
+                   u0=u0)
 
     @classmethod
     def hills(cls, nx, ny, nz, dx, dt, seed=1, n_hills=12, hill_height=2500.0, hill_wmin=30000.0, hill_wmax=150000.0,
-              jet_min=5.0, jet_max=40.0, map_amp=0.05, etac=0.2, p_top=5000.0, moist=True, bc=A.MI_BC_PERIODIC):
+              jet_min=5.0, jet_max=40.0, map_amp=0.05, etac=0.2, p_top=5000.0, moist=True, bc=A.MI_BC_PERIODIC,
+              bc_y=None):
         """real-data-shaped domain: seeded hills, US standard atmosphere, jets, map factors, Coriolis, hybrid
         coordinate with a p_top top, qv from a relative humidity of 80 % falling to 10 % at 10 km"""
-        g = Grid(nx, ny, nz, dx, dx, dt, bc, bc)
+        g = Grid(nx, ny, nz, dx, dx, dt, bc, bc if bc_y is None else bc_y)
         return cls(A.MI_CASE_HILLS, g, 0.0, seed=seed, n_hills=n_hills, moist=1 if moist else 0, hill_height=hill_height,
                    hill_wmin=hill_wmin, hill_wmax=hill_wmax, jet_min=jet_min, jet_max=jet_max, map_amp=map_amp,
                    etac=etac, p_top=p_top)

@@ -83,7 +83,7 @@ inline void halo_point(double* base, long sn, int a, int n, bool face, bool peri
   const int nl = face ? 4 : 3;
   const int i = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
   int src;
-  if (flat) {  // specified: the outermost row copied outward (halo.cpp:67-69)
+  if (flat) {  // specified or open: the outermost row copied outward (halo.cpp:67-69)
     const int last = face ? n : n - 1;
     src = i < 0 ? 0 : (i > last ? last : i);
     base[i * sn] = base[src * sn];
@@ -108,14 +108,14 @@ void fill(F& f, const Grid& g, Par px, Par py) {
     for (int j = 0; j < f.nys; ++j)
       for (int a = 0; a < na; ++a)
         halo_point(&f(0, j, k), 1, a, g.nx, f.face_x, g.bcx == MI_BC_PERIODIC, px != Par::Even, px == Par::OddFace,
-                   g.bcx == MI_BC_SPECIFIED);
+                   g.bcx == MI_BC_SPECIFIED || g.bcx == MI_BC_OPEN);
   if (g.two_d()) return;
   const int nb = f.face_y ? 8 : 6;
   for (int k = 0; k < f.nzs; ++k)
     for (int i = -3; i < f.nxs + 3; ++i)
       for (int a = 0; a < nb; ++a)
         halo_point(&f(i, 0, k), f.sj, a, g.ny, f.face_y, g.bcy == MI_BC_PERIODIC, py != Par::Even, py == Par::OddFace,
-                   g.bcy == MI_BC_SPECIFIED);
+                   g.bcy == MI_BC_SPECIFIED || g.bcy == MI_BC_OPEN);
 }
 // default parities (halo.cpp:224-228): U odd in x, V odd in y, everything else even
 void fill(F& f, const Grid& g) { fill(f, g, f.face_x ? Par::Odd : Par::Even, f.face_y ? Par::Odd : Par::Even); }
@@ -169,6 +169,32 @@ inline int ord_face(int f, int n, bool on) {
   if (f == 2 || f == n - 2) return 3;
   return 5;
 }
+// open boundaries (kernels_common.hpp:60-81): the outermost column (row) of a scalar takes, instead of the flux
+// divergence, rdx*(ub*(q_old(inner)-q_old(outer)) + q*(ru(b)-ru(a))) with the inflow mean normal mass flux ub
+template <class Q, class R>
+inline double open_x_scalar(const Q& q, const Q& qo, const R& ru, double rdx, int i, int j, int k, int nx) {
+  if (i == 0) {
+    double ub = 0.5 * (ru(0, j, k) + ru(1, j, k));
+    ub = ub < 0.0 ? ub : 0.0;
+    return rdx * (ub * (qo(1, j, k) - qo(0, j, k)) + q(0, j, k) * (ru(1, j, k) - ru(0, j, k)));
+  }
+  double ub = 0.5 * (ru(nx - 1, j, k) + ru(nx, j, k));
+  ub = ub > 0.0 ? ub : 0.0;
+  return rdx * (ub * (qo(nx - 1, j, k) - qo(nx - 2, j, k)) + q(nx - 1, j, k) * (ru(nx, j, k) - ru(nx - 1, j, k)));
+}
+template <class Q, class R>
+inline double open_y_scalar(const Q& q, const Q& qo, const R& rv, double rdy, int i, int j, int k, int ny) {
+  if (j == 0) {
+    double vb = 0.5 * (rv(i, 0, k) + rv(i, 1, k));
+    vb = vb < 0.0 ? vb : 0.0;
+    return rdy * (vb * (qo(i, 1, k) - qo(i, 0, k)) + q(i, 0, k) * (rv(i, 1, k) - rv(i, 0, k)));
+  }
+  double vb = 0.5 * (rv(i, ny - 1, k) + rv(i, ny, k));
+  vb = vb > 0.0 ? vb : 0.0;
+  return rdy * (vb * (qo(i, ny - 1, k) - qo(i, ny - 2, k)) + q(i, ny - 1, k) * (rv(i, ny, k) - rv(i, ny - 1, k)));
+}
+constexpr double kCb = 25.;  // phase speed of the open (radiative) boundaries (constants.hpp:19)
+
 inline int ord_mass(int f, int n, bool on) {
   if (!on) return 5;
   if (f <= 1 || f >= n) return 2;
@@ -211,6 +237,9 @@ struct Dom {
   // specified lateral boundaries (LateralBC, lateral_bc.hpp:28-91): spec_zone 1, relax_zone 4
   static constexpr int kBw = 5, kSpecZone = 1, kRelaxZone = 4;
   bool spec = false;
+  // open boundaries (WRF open_xs/xe, open_ys/ye; grid.hpp:17-19, kernels_common.hpp:33-44): ox, oy; flx, fly:
+  // the degraded advection orders of a flat (specified or open) direction (kern::Spec x, y)
+  bool ox = false, oy = false, flx = false, fly = false;
   double fcx[kBw] = {}, gcx[kBw] = {}, dtbc = 0;
   // one variable's boundary data: values and tendencies of the sides xs, xe, ys, ye, (pos, k, b) pos fastest
   struct Bdy {
@@ -262,6 +291,10 @@ struct Dom {
     mapped = p.mapped != 0;
     cor = mapped || p.coriolis != 0;  // MapC::from(b, b.mapped || b.coriolis) (large_step.cpp:810)
     spec = g.bcx == MI_BC_SPECIFIED;  // with bcy the same (checked at creation)
+    ox = g.bcx == MI_BC_OPEN;
+    oy = g.bcy == MI_BC_OPEN && !g.two_d();
+    flx = spec || ox;
+    fly = (spec || g.bcy == MI_BC_OPEN) && !g.two_d();
     use_tendf = p.diff_6th_opt != 0 || spec;  // a DiffusionDriver or a LateralBC attached (large_step.cpp:828)
     terrain = p.terrain != 0;          // !b.ht.uniform() (small_step.cpp:583, rk3.cpp:163)
     if (spec) {
@@ -460,6 +493,18 @@ struct Dom {
       i0 = 3; j0 = 3;
       i1 = name == 'u' ? nx - 2 : nx - 3;
       j1 = name == 'v' ? ny - 2 : ny - 3;
+    } else if (ox || oy) {  // open sides (sixth_order.hpp:179-204), 'v' with open_xs on all four sides
+      if (name == 'u') {
+        if (ox) { i0 = 3; i1 = nx - 2; }
+        if (oy) { j0 = 3; j1 = ny - 3; }
+      } else if (name == 'v') {
+        if (ox) { i0 = 3; i1 = nx - 3; j0 = 3; j1 = ny - 2; }
+        if (oy) { j0 = 3; j1 = ny - 2; }
+      } else {
+        if (ox) { i0 = 3; i1 = nx - 3; }
+        if (oy) { j0 = 3; j1 = ny - 3; }
+      }
+      if (g.bcx == MI_BC_PERIODIC) { i0 = 0; i1 = name == 'u' ? nx + 1 : nx; }
     }
     pfor(i0, i1, j0, j1, k0, nz, [&](int i, int j, int k) {
       auto q = [&](int a, int b) { return fld(i + a, j + b, k); };
@@ -515,17 +560,17 @@ struct Dom {
     // mu_2's halo is not refreshed with specified boundaries (large_step.cpp:700-706)
     for (F* f : {&u2, &v2, &w2, &t2, &ph2, &pp, &al}) fill(*f, g);
     if (!spec) fill(mu2, g);
-    // KMu (large_step.cpp:49-66): a specified edge face takes its column's mass twice
+    // KMu (large_step.cpp:49-66): a specified or open edge face takes its column's mass twice
     pfor(0, nx + 1, 0, nyv_, 0, 1, [&](int i, int j, int) {
       if (i < nx && j < ny) mut(i, j) = mu2(i, j) + mub(i, j);
       if (j < ny) {
-        if (spec && i == 0) muu(i, j) = H * (mu2(0, j) + mu2(0, j) + mub(0, j) + mub(0, j));
-        else if (spec && i == nx) muu(i, j) = H * (mu2(nx - 1, j) + mu2(nx - 1, j) + mub(nx - 1, j) + mub(nx - 1, j));
+        if (flx && i == 0) muu(i, j) = H * (mu2(0, j) + mu2(0, j) + mub(0, j) + mub(0, j));
+        else if (flx && i == nx) muu(i, j) = H * (mu2(nx - 1, j) + mu2(nx - 1, j) + mub(nx - 1, j) + mub(nx - 1, j));
         else muu(i, j) = H * (mu2(i, j) + mu2(i - 1, j) + mub(i, j) + mub(i - 1, j));
       }
       if (i < nx && j < nyv_) {
-        if (spec && j == 0) muv(i, j) = H * (mu2(i, 0) + mu2(i, 0) + mub(i, 0) + mub(i, 0));
-        else if (spec && j == ny) muv(i, j) = H * (mu2(i, ny - 1) + mu2(i, ny - 1) + mub(i, ny - 1) + mub(i, ny - 1));
+        if (fly && j == 0) muv(i, j) = H * (mu2(i, 0) + mu2(i, 0) + mub(i, 0) + mub(i, 0));
+        else if (fly && j == ny) muv(i, j) = H * (mu2(i, ny - 1) + mu2(i, ny - 1) + mub(i, ny - 1) + mub(i, ny - 1));
         else muv(i, j) = H * (mu2(i, j) + mu2(i, j - 1) + mub(i, j) + mub(i, j - 1));
       }
     });
@@ -616,13 +661,13 @@ struct Dom {
     // KTendU (large_step.cpp:152-264), interior faces, no Coriolis, unmapped
     {
       auto fx = [&](int i, int j, int k) {
-        const int o = ord_mass(i, nxo, spec);
-        if (o == 2) {  // the upstream normal velocity at the boundary face (large_step.cpp:160-167)
+        const int o = ord_mass(i, nxo, flx);
+        if (o == 2) {  // specified: the upstream normal velocity at the boundary face (large_step.cpp:160-167)
           if (i <= 1) {
-            const double ub = u2(i, j, k) < 0.0 ? u2(i, j, k) : u2(i - 1, j, k);
+            const double ub = (spec && u2(i, j, k) < 0.0) ? u2(i, j, k) : u2(i - 1, j, k);
             return Q * (ru(i, j, k) + ru(i - 1, j, k)) * (u2(i, j, k) + ub);
           }
-          const double ub = u2(i - 1, j, k) > 0.0 ? u2(i - 1, j, k) : u2(i, j, k);
+          const double ub = (spec && u2(i - 1, j, k) > 0.0) ? u2(i - 1, j, k) : u2(i, j, k);
           return Q * (ru(i, j, k) + ru(i - 1, j, k)) * (u2(i - 1, j, k) + ub);
         }
         double vel = H * (ru(i, j, k) + ru(i - 1, j, k));
@@ -631,7 +676,7 @@ struct Dom {
                            vel, hup);
       };
       auto fy = [&](int i, int j, int k) {
-        const int o = ord_face(j, nyo, spec);
+        const int o = ord_face(j, nyo, fly);
         if (o == 2) return Q * (rv(i, j, k) + rv(i - 1, j, k)) * (u2(i, j, k) + u2(i, j - 1, k));
         double vel = H * (rv(i, j, k) + rv(i - 1, j, k));
         if (o == 3) return vel * flux3(u2(i, j - 2, k), u2(i, j - 1, k), u2(i, j, k), u2(i, j + 1, k), vel, hup);
@@ -650,10 +695,50 @@ struct Dom {
         if (k == nz) return top_lid ? H * (c.cfn * pu(i, j, nz - 1) + c.cfn1 * pu(i, j, nz - 2)) : 0.0;
         return H * (c.fnm[k] * pu(i, j, k) + c.fnp[k] * pu(i, j, k - 1));
       };
-      pfor(g.u_i0(), nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+      // advect_u open_ys / open_ye (large_step.cpp:193-207): v-flux terms of the outermost rows
+      auto open_y_term = [&](int i, int j, int k) {
+        const int imin = 0, imax = nx - 1;
+        const int ip = i < imax ? i : imax, im = i - 1 > imin ? i - 1 : imin;
+        const double mrdy = msfux(i, j) * rdy;
+        if (j == 0) {
+          const double vw = H * (rv(ip, 0, k) + rv(im, 0, k));
+          const double vb = vw < 0.0 ? vw : 0.0;
+          const double dvm = rv(ip, 1, k) - rv(ip, 0, k), dvp = rv(im, 1, k) - rv(im, 0, k);
+          return mrdy * (vb * (u2(i, 1, k) - u2(i, 0, k)) + H * u2(i, 0, k) * (dvm + dvp));
+        }
+        const int je = ny;
+        const double vw = H * (rv(ip, je, k) + rv(im, je, k));
+        const double vb = vw > 0.0 ? vw : 0.0;
+        const double dvm = rv(ip, je, k) - rv(ip, je - 1, k), dvp = rv(im, je, k) - rv(im, je - 1, k);
+        return mrdy * (vb * (u2(i, je - 1, k) - u2(i, je - 2, k)) + H * u2(i, je - 1, k) * (dvm + dvp));
+      };
+      // open x: the boundary faces are advanced too (large_step.cpp:805)
+      pfor(ox ? 0 : g.u_i0(), ox ? nx + 1 : nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+        if (ox && (i == 0 || i == nx)) {
+          // open x boundary face (large_step.cpp:211-228): the radiation condition with phase speed cb and,
+          // unless open_ys/ye, the vertical advection
+          double t;
+          if (i == 0) {
+            double ub = ru(0, j, k) - kCb * (c.c1h[k] * mut(0, j) + c.c2h[k]);
+            ub = ub < 0.0 ? ub : 0.0;
+            t = 0.0 - rdx * ub * (u2(1, j, k) - u2(0, j, k));
+          } else {
+            double ub = ru(i, j, k) + kCb * (c.c1h[k] * mut(i - 1, j) + c.c2h[k]);
+            ub = ub > 0.0 ? ub : 0.0;
+            t = 0.0 - rdx * ub * (u2(i, j, k) - u2(i - 1, j, k));
+          }
+          if (!oy) t -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+          if (use_tendf) t += ru_tendf(i, j, k) / msfuy(i, j);
+          ru_tend(i, j, k) = t;
+          return;
+        }
         double t = -(msfux(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
-        if (do_y) t -= (msfux(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
-        t -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+        if (do_y) {
+          const bool orow = oy && (j == 0 || j == ny - 1);  // open_ys/ye: the outermost rows take the open terms
+          if (!orow) t -= (msfux(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+          else t -= open_y_term(i, j, k);
+        }
+        if (!(oy && flx && (i == 0 || i == nx))) t -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
         const double rm = msfux(i, j) / msfuy(i, j);
         double dpx = rm * H * rdx * (c.c1h[k] * muu(i, j) + c.c2h[k]) *
                      ((ph2(i, j, k + 1) - ph2(i - 1, j, k + 1) + ph2(i, j, k) - ph2(i - 1, j, k)) +
@@ -678,7 +763,7 @@ struct Dom {
     // KTendV (large_step.cpp:267-376), 3-D only
     if (do_y) {
       auto fx = [&](int i, int j, int k) {
-        const int o = ord_face(i, nxo, spec);
+        const int o = ord_face(i, nxo, flx);
         if (o == 2) return Q * (ru(i, j, k) + ru(i, j - 1, k)) * (v2(i, j, k) + v2(i - 1, j, k));
         double vel = H * (ru(i, j, k) + ru(i, j - 1, k));
         if (o == 3) return vel * flux3(v2(i - 2, j, k), v2(i - 1, j, k), v2(i, j, k), v2(i + 1, j, k), vel, hup);
@@ -686,13 +771,13 @@ struct Dom {
                            vel, hup);
       };
       auto fy = [&](int i, int j, int k) {
-        const int o = ord_mass(j, nyo, spec);
+        const int o = ord_mass(j, nyo, fly);
         if (o == 2) {  // upstream normal velocity at the boundary face (large_step.cpp:282-289)
           if (j <= 1) {
-            const double vb = v2(i, j, k) < 0.0 ? v2(i, j, k) : v2(i, j - 1, k);
+            const double vb = (spec && v2(i, j, k) < 0.0) ? v2(i, j, k) : v2(i, j - 1, k);
             return Q * (rv(i, j, k) + rv(i, j - 1, k)) * (v2(i, j, k) + vb);
           }
-          const double vb = v2(i, j - 1, k) > 0.0 ? v2(i, j - 1, k) : v2(i, j, k);
+          const double vb = (spec && v2(i, j - 1, k) > 0.0) ? v2(i, j - 1, k) : v2(i, j, k);
           return Q * (rv(i, j, k) + rv(i, j - 1, k)) * (vb + v2(i, j - 1, k));
         }
         double vel = H * (rv(i, j, k) + rv(i, j - 1, k));
@@ -712,10 +797,49 @@ struct Dom {
         if (k == nz) return top_lid ? H * (c.cfn * pv(i, j, nz - 1) + c.cfn1 * pv(i, j, nz - 2)) : 0.0;
         return H * (c.fnm[k] * pv(i, j, k) + c.fnp[k] * pv(i, j, k - 1));
       };
-      pfor(0, nx, g.v_j0(), ny, 0, nz, [&](int i, int j, int k) {
+      // advect_v open_xs / open_xe (large_step.cpp:307-323): u-flux terms of the outermost columns
+      auto open_x_term = [&](int i, int j, int k) {
+        const int jmin = 0, jmax = ny - 1;
+        const int jp = j < jmax ? j : jmax, jm = j - 1 > jmin ? j - 1 : jmin;
+        if (i == 0) {
+          const double mrdx = msfvy(0, j) * rdx;
+          const double uw = H * (ru(0, jp, k) + ru(0, jm, k));
+          const double ub = uw < 0.0 ? uw : 0.0;
+          const double dup = ru(1, jp, k) - ru(0, jp, k), dum = ru(1, jm, k) - ru(0, jm, k);
+          return mrdx * (ub * (v2(1, j, k) - v2(0, j, k)) + H * v2(0, j, k) * (dup + dum));
+        }
+        const int ie = nx;
+        const double mrdx = msfvy(ie - 1, j) * rdx;
+        const double uw = H * (ru(ie, jp, k) + ru(ie, jm, k));
+        const double ub = uw > 0.0 ? uw : 0.0;
+        const double dup = ru(ie, jp, k) - ru(ie - 1, jp, k), dum = ru(ie, jm, k) - ru(ie - 1, jm, k);
+        return mrdx * (ub * (v2(ie - 1, j, k) - v2(ie - 2, j, k)) + H * v2(ie - 1, j, k) * (dup + dum));
+      };
+      pfor(0, nx, oy ? 0 : g.v_j0(), oy ? ny + 1 : ny, 0, nz, [&](int i, int j, int k) {
+        if (oy && (j == 0 || j == ny)) {  // open y boundary face: radiation only (large_step.cpp:327-340)
+          double t;
+          if (j == 0) {
+            double vb = rv(i, 0, k) - kCb * (c.c1h[k] * mut(i, 0) + c.c2h[k]);
+            vb = vb < 0.0 ? vb : 0.0;
+            t = 0.0 - rdy * vb * (v2(i, 1, k) - v2(i, 0, k));
+          } else {
+            double vb = rv(i, j, k) + kCb * (c.c1h[k] * mut(i, j - 1) + c.c2h[k]);
+            vb = vb > 0.0 ? vb : 0.0;
+            t = 0.0 - rdy * vb * (v2(i, j, k) - v2(i, j - 1, k));
+          }
+          if (use_tendf) t += rv_tendf(i, j, k) * msfvx_inv(i, j);
+          rv_tend(i, j, k) = t;
+          return;
+        }
         const double rm = msfvy(i, j) / msfvx(i, j);
-        double t = -(msfvy(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
-        t -= (msfvy(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        double t;
+        if (ox && (i == 0 || i == nx - 1)) {  // open x: the y flux, then the open term instead of the x flux
+          t = -(msfvy(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+          t -= open_x_term(i, j, k);
+        } else {
+          t = -(msfvy(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+          t -= (msfvy(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        }
         t -= rm * c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
         double dpy = rm * H * rdy * (c.c1h[k] * muv(i, j) + c.c2h[k]) *
                      ((ph2(i, j, k + 1) - ph2(i, j - 1, k + 1) + ph2(i, j, k) - ph2(i, j - 1, k)) +
@@ -749,7 +873,7 @@ struct Dom {
       };
       auto fx = [&](int i, int j, int k) {
         double vel = rux(i, j, k);
-        const int o = ord_face(i, nxo, spec);
+        const int o = ord_face(i, nxo, flx);
         if (o == 2) return H * vel * (w2(i, j, k) + w2(i - 1, j, k));
         if (o == 3) return vel * flux3(w2(i - 2, j, k), w2(i - 1, j, k), w2(i, j, k), w2(i + 1, j, k), vel, hup);
         return vel * flux5(w2(i - 3, j, k), w2(i - 2, j, k), w2(i - 1, j, k), w2(i, j, k), w2(i + 1, j, k), w2(i + 2, j, k),
@@ -757,7 +881,7 @@ struct Dom {
       };
       auto fy = [&](int i, int j, int k) {
         double vel = rvy(i, j, k);
-        const int o = ord_face(j, nyo, spec);
+        const int o = ord_face(j, nyo, fly);
         if (o == 2) return H * vel * (w2(i, j, k) + w2(i, j - 1, k));
         if (o == 3) return vel * flux3(w2(i, j - 2, k), w2(i, j - 1, k), w2(i, j, k), w2(i, j + 1, k), vel, hup);
         return vel * flux5(w2(i, j - 3, k), w2(i, j - 2, k), w2(i, j - 1, k), w2(i, j, k), w2(i, j + 1, k), w2(i, j + 2, k),
@@ -768,9 +892,56 @@ struct Dom {
         double vel = H * (ww(i, j, m + 1) + ww(i, j, m));
         return vel * flux3(w2(i, j, m - 1), w2(i, j, m), w2(i, j, m + 1), w2(i, j, m + 2), -vel, vup);
       };
+      // advect_w open_xs/xe (open_ys/ye) terms of the outermost columns (rows), large_step.cpp:416-453
+      auto open_x_term = [&](int i, int j, int k) {
+        const int a = i == 0 ? 0 : nx - 1, b = i == 0 ? 1 : nx;
+        double uw, dru;
+        if (k < nz) {
+          uw = H * (c.fnm[k] * (ru(a, j, k) + ru(b, j, k)) + c.fnp[k] * (ru(a, j, k - 1) + ru(b, j, k - 1)));
+          dru = c.fnm[k] * (ru(b, j, k) - ru(a, j, k)) + c.fnp[k] * (ru(b, j, k - 1) - ru(a, j, k - 1));
+        } else {
+          const double fm = 2.0 - c.fnm[nz - 1], fp = c.fnp[nz - 1];
+          uw = H * (fm * (ru(a, j, nz - 1) + ru(b, j, nz - 1)) - fp * (ru(a, j, nz - 2) + ru(b, j, nz - 2)));
+          dru = fm * (ru(b, j, nz - 1) - ru(a, j, nz - 1)) - fp * (ru(b, j, nz - 2) - ru(a, j, nz - 2));
+        }
+        if (i == 0) {
+          const double ub = uw < 0.0 ? uw : 0.0;
+          return rdx * (ub * (w2(1, j, k) - w2(0, j, k)) + w2(0, j, k) * dru);
+        }
+        const double ub = uw > 0.0 ? uw : 0.0;
+        return rdx * (ub * (w2(i, j, k) - w2(i - 1, j, k)) + w2(i, j, k) * dru);
+      };
+      auto open_y_term = [&](int i, int j, int k) {
+        const int a = j == 0 ? 0 : ny - 1, b = j == 0 ? 1 : ny;
+        double vw, drv;
+        if (k < nz) {
+          vw = H * (c.fnm[k] * (rv(i, a, k) + rv(i, b, k)) + c.fnp[k] * (rv(i, a, k - 1) + rv(i, b, k - 1)));
+          drv = c.fnm[k] * (rv(i, b, k) - rv(i, a, k)) + c.fnp[k] * (rv(i, b, k - 1) - rv(i, a, k - 1));
+        } else {
+          const double fm = 2.0 - c.fnm[nz - 1], fp = c.fnp[nz - 1];
+          vw = H * (fm * (rv(i, a, nz - 1) + rv(i, b, nz - 1)) - fp * (rv(i, a, nz - 2) + rv(i, b, nz - 2)));
+          drv = fm * (rv(i, b, nz - 1) - rv(i, a, nz - 1)) - fp * (rv(i, b, nz - 2) - rv(i, a, nz - 2));
+        }
+        if (j == 0) {
+          const double vb = vw < 0.0 ? vw : 0.0;
+          return rdy * (vb * (w2(i, 1, k) - w2(i, 0, k)) + w2(i, 0, k) * drv);
+        }
+        const double vb = vw > 0.0 ? vw : 0.0;
+        return rdy * (vb * (w2(i, j, k) - w2(i, j - 1, k)) + w2(i, j, k) * drv);
+      };
       pfor(0, nx, 0, ny, 1, nz + 1, [&](int i, int j, int k) {
-        double t = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
-        if (do_y) t -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        double t;
+        const bool ocol = ox && (i == 0 || i == nx - 1), orow = oy && (j == 0 || j == ny - 1);
+        if (ocol || orow) {  // WRF order: y flux, x flux, open x terms, open y terms
+          t = 0.0;
+          if (do_y && !orow) t = t - (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+          if (!ocol) t = t - (msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+          if (ocol) t = t - open_x_term(i, j, k);
+          if (orow) t = t - open_y_term(i, j, k);
+        } else {
+          t = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+          if (do_y) t -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        }
         const double rmsft = 1.0 / msfty(i, j);
         if (k < nz) {
           t -= c.rdn[k] * (fz(i, j, k) - fz(i, j, k - 1));
@@ -808,7 +979,7 @@ struct Dom {
     {
       auto fx = [&](int i, int j, int k) {
         double vel = ru(i, j, k);
-        const int o = ord_face(i, nxo, spec);
+        const int o = ord_face(i, nxo, flx);
         if (o == 2) return H * vel * (t2(i, j, k) + t2(i - 1, j, k));
         if (o == 3) return vel * flux3(t2(i - 2, j, k), t2(i - 1, j, k), t2(i, j, k), t2(i + 1, j, k), vel, hup);
         return vel * flux5(t2(i - 3, j, k), t2(i - 2, j, k), t2(i - 1, j, k), t2(i, j, k), t2(i + 1, j, k), t2(i + 2, j, k),
@@ -816,7 +987,7 @@ struct Dom {
       };
       auto fy = [&](int i, int j, int k) {
         double vel = rv(i, j, k);
-        const int o = ord_face(j, nyo, spec);
+        const int o = ord_face(j, nyo, fly);
         if (o == 2) return H * vel * (t2(i, j, k) + t2(i, j - 1, k));
         if (o == 3) return vel * flux3(t2(i, j - 2, k), t2(i, j - 1, k), t2(i, j, k), t2(i, j + 1, k), vel, hup);
         return vel * flux5(t2(i, j - 3, k), t2(i, j - 2, k), t2(i, j - 1, k), t2(i, j, k), t2(i, j + 1, k), t2(i, j + 2, k),
@@ -829,8 +1000,18 @@ struct Dom {
         return vel * flux3(t2(i, j, k - 2), t2(i, j, k - 1), t2(i, j, k), t2(i, j, k + 1), -vel, vup);
       };
       pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
-        double r = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
-        if (do_y) r -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        double r;
+        const bool ocol = ox && (i == 0 || i == nx - 1), orow = oy && (j == 0 || j == ny - 1);
+        if (ocol || orow) {  // open boundary: WRF order y flux, x flux, open x terms, open y terms, vertical
+          r = 0.0;
+          if (do_y && !orow) r = r - (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+          if (!ocol) r = r - (msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+          if (ocol) r = r - open_x_scalar(t2, t2, ru, rdx, i, j, k, nx);
+          if (orow) r = r - open_y_scalar(t2, t2, rv, rdy, i, j, k, ny);
+        } else {
+          r = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+          if (do_y) r -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        }
         r -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
         if (use_tendf) r += t_tendf(i, j, k) / msfty(i, j);
         // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
@@ -866,11 +1047,11 @@ struct Dom {
       auto dfwdy = [&](int i, int j, int k) { return phb(i, j + 1, k) - phb(i, j, k) + ph2(i, j + 1, k) - ph2(i, j, k); };
       // rhs_ph next to specified edges (large_step.cpp:596-605): none on the outer row, 2nd order on the
       // next, then 4th order in y (none in x), 6th from the fourth row in
-      auto horder = [&](int i, int n, bool is_x) {
-        if (!spec) return 6;
+      auto horder = [&](int i, int n, bool on, bool is_x) {
+        if (!on) return 6;
         if (i <= 0 || i >= n - 1) return 0;
         if (i == 1 || i == n - 2) return 2;
-        if (i == 2 || i == n - 3) return is_x ? 0 : 4;
+        if (i == 2 || i == n - 3) return (is_x && !ox) ? 0 : 4;  // open x: 4th order (open_xs/xe only)
         return 6;
       };
       pfor(0, nx, 0, ny, 1, nz + 1, [&](int i, int j, int k) {
@@ -884,7 +1065,7 @@ struct Dom {
               k < nz ? v2(i, j + 1, k) + v2(i, j + 1, k - 1) : c.cfn * v2(i, j + 1, nz - 1) + c.cfn1 * v2(i, j + 1, nz - 2);
           const double vm = k < nz ? v2(i, j, k) + v2(i, j, k - 1) : c.cfn * v2(i, j, nz - 1) + c.cfn1 * v2(i, j, nz - 2);
           const double cy = (k < nz ? 0.25 : H) * rdy / msfty(i, j);
-          const int o = horder(j, ny, false);
+          const int o = horder(j, ny, fly, false);
           if (o == 6)
             t = t - cy * ((mvp * vp * msfvy(i, j + 1) + mvm * vm * msfvy(i, j)) * r60 * grad6y(i, j, k));
           else if (o == 4)
@@ -897,13 +1078,41 @@ struct Dom {
             k < nz ? u2(i + 1, j, k) + u2(i + 1, j, k - 1) : c.cfn * u2(i + 1, j, nz - 1) + c.cfn1 * u2(i + 1, j, nz - 2);
         const double um = k < nz ? u2(i, j, k) + u2(i, j, k - 1) : c.cfn * u2(i, j, nz - 1) + c.cfn1 * u2(i, j, nz - 2);
         const double cx = (k < nz ? 0.25 : H) * rdx / msfty(i, j);
-        const int o = horder(i, nx, true);
+        const int o = horder(i, nx, flx, true);
         if (o == 6)
           t = t - cx * ((mup * up * msfux(i + 1, j) + mum * um * msfux(i, j)) * r60 * grad6x(i, j, k));
         else if (o == 2)
           t = t - cx * (mup * up * msfux(i + 1, j) * dfwdx(i, j, k) + mum * um * msfux(i, j) * dfwdx(i - 1, j, k));
         else if (o == 4)
           t = t - cx * ((mup * up * msfux(i + 1, j) + mum * um * msfux(i, j)) * r12 * grad4x(i, j, k));
+        // open boundaries (large_step.cpp:636-661), south/north then west/east: inflow advection of ph by the
+        // normal wind at the w level
+        if (oy && (j == 0 || j == ny - 1)) {
+          const int kz = k < nz - 1 ? k : nz - 1;
+          const int a = j == 0 ? 0 : ny - 1, b = j == 0 ? 1 : ny;
+          const double vb = H * (c.fnm[kz] * (v2(i, b, kz) + v2(i, a, kz)) + c.fnp[kz] * (v2(i, b, kz - 1) + v2(i, a, kz - 1)));
+          const double muf = c.c1f[k] * mut(i, j) + c.c2f[k];
+          if (j == 0) {
+            const double vl = vb < 0.0 ? vb : 0.0;
+            t = t - rdy * muf * (vl * (ph2(i, 1, k) - ph2(i, 0, k)));
+          } else {
+            const double vr = vb > 0.0 ? vb : 0.0;
+            t = t - rdy * muf * (vr * (ph2(i, j, k) - ph2(i, j - 1, k)));
+          }
+        }
+        if (ox && (i == 0 || i == nx - 1) && !(i == 0 && k == nz)) {
+          const int kz = k < nz ? k : nz - 1;
+          const int a = i == 0 ? 0 : nx - 1, b = i == 0 ? 1 : nx;
+          const double ub = H * (c.fnm[kz] * (u2(b, j, kz) + u2(a, j, kz)) + c.fnp[kz] * (u2(b, j, kz - 1) + u2(a, j, kz - 1)));
+          const double muf = c.c1f[k] * mut(i, j) + c.c2f[k];
+          if (i == 0) {
+            const double ul = ub < 0.0 ? ub : 0.0;
+            t = t - (msftx(i, j) / msfty(i, j)) * rdx * muf * (ul * (ph2(1, j, k) - ph2(0, j, k)));
+          } else {
+            const double ur = ub > 0.0 ? ub : 0.0;
+            t = t - (msftx(i, j) / msfty(i, j)) * rdx * muf * (ur * (ph2(i, j, k) - ph2(i - 1, j, k)));
+          }
+        }
         if (spec) t = t + ph_tendf(i, j, k) / msfty(i, j);  // rk_addtend_dry with a LateralBC (large_step.cpp:663)
         ph_tend(i, j, k) = t;
       });
@@ -1030,13 +1239,18 @@ struct Dom {
       // Edge (small_step.cpp:39-45): the spec zone's faces are advanced by spec_bdyupdate instead
       auto uface = [&](int i, int j) { return spec && (i <= 0 || i >= nx || j == 0 || j == ny - 1); };
       auto vface = [&](int i, int j) { return spec && (i == 0 || i == nx - 1 || j <= 0 || j >= ny); };
-      pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
-        if (i >= ui0 && i < ui1 && !uface(i, j)) {
+      // open boundaries (small_step.cpp:175-184, 531): the boundary faces take the large-step tendency only
+      pfor(0, nx + (ox ? 1 : 0), 0, ny + (oy ? 1 : 0), 0, nz, [&](int i, int j, int k) {
+        if (ox && (i == 0 || i == nx) && j < ny) {
+          u2(i, j, k) = u2(i, j, k) + dts * ru_tend(i, j, k);
+        } else if (i >= ui0 && i < ui1 && !uface(i, j) && j < ny) {
           const double mudf_x = -emdx * (mudf(i, j) - mudf(i - 1, j)) / msfuy(i, j);
           u2(i, j, k) = u2(i, j, k) + dts * ru_tend(i, j, k) -
                         dts * mc_cqu(i, j, k) * dp(1, 0, i, j, k, muu(i, j), c.rdx, msfux(i, j) / msfuy(i, j)) + c.c1h[k] * mudf_x;
         }
-        if (do_y && j >= vj0 && j < vj1 && !vface(i, j)) {
+        if (do_y && oy && (j == 0 || j == ny) && i < nx) {
+          v2(i, j, k) = v2(i, j, k) + dts * rv_tend(i, j, k);
+        } else if (do_y && j >= vj0 && j < vj1 && !vface(i, j) && i < nx) {
           const double mudf_y = -emdy * (mudf(i, j) - mudf(i, j - 1)) * msfvx_inv(i, j);
           v2(i, j, k) = v2(i, j, k) + dts * rv_tend(i, j, k) -
                         dts * mc_cqv(i, j, k) * dp(0, 1, i, j, k, muv(i, j), c.rdy, msfvy(i, j) / msfvx(i, j)) + c.c1h[k] * mudf_y;
@@ -1123,20 +1337,24 @@ struct Dom {
         const double tn = tk + msfty(i, j) * dts * t_tend(i, j, k);
         t2(i, j, k) = tn - dts * msfty(i, j) * (msftx(i, j) * hflux + c.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
       });
-      if (spec) {
-        // KSumEdge (small_step.cpp:279-290): the east and north domain faces are real faces
+      if (spec || ox || oy) {
+        // KSumEdge (small_step.cpp:279-290, 560-569): the east and north domain faces are real faces
         for (int k = 0; k < nz; ++k) {
+          if (spec || ox)
           for (int j = 0; j < ny; ++j) {
             const double su = (firstit ? 0.0 : ru_m(nx, j, k)) + u2(nx, j, k);
             const double cpl = (c.c1h[k] * muu(nx, j) + c.c2h[k]) * u_save(nx, j, k);
             ru_m(nx, j, k) = last ? su / rn + cpl / msfuy(nx, j) : su;
           }
+          if (spec || oy)
           for (int i = 0; i < nx; ++i) {
             const double sv = (firstit ? 0.0 : rv_m(i, ny, k)) + v2(i, ny, k);
             const double cpl = (c.c1h[k] * muv(i, ny) + c.c2h[k]) * v_save(i, ny, k);
             rv_m(i, ny, k) = last ? sv / rn + cpl * msfvx_inv(i, ny) : sv;
           }
         }
+      }
+      if (spec) {
         // LateralBC::update_mu_t (lateral_bc.cpp:366-372)
         spec_update('t', nz, t2, t_tend, dts);
         spec_update('t', 1, mu2, mu_tend, dts);
@@ -1285,14 +1503,14 @@ struct Dom {
     }
     double fx(int i, int j, int k) const {
       const double vel = d.ru_m(i, j, k);
-      const int o = ord_face(i, d.g.nx, d.spec);
+      const int o = ord_face(i, d.g.nx, d.flx);
       if (o == 2) return H * vel * (q(i, j, k) + q(i - 1, j, k));
       if (o == 3) return vel * f3(q(i - 2, j, k), q(i - 1, j, k), q(i, j, k), q(i + 1, j, k), vel, hup);
       return vel * f5(q(i - 3, j, k), q(i - 2, j, k), q(i - 1, j, k), q(i, j, k), q(i + 1, j, k), q(i + 2, j, k), vel);
     }
     double fy(int i, int j, int k) const {
       const double vel = d.rv_m(i, j, k);
-      const int o = ord_face(j, d.g.ny, d.spec);
+      const int o = ord_face(j, d.g.ny, d.fly);
       if (o == 2) return H * vel * (q(i, j, k) + q(i, j - 1, k));
       if (o == 3) return vel * f3(q(i, j - 2, k), q(i, j - 1, k), q(i, j, k), q(i, j + 1, k), vel, hup);
       return vel * f5(q(i, j - 3, k), q(i, j - 2, k), q(i, j - 1, k), q(i, j, k), q(i, j + 1, k), q(i, j + 2, k), vel);
@@ -1309,7 +1527,7 @@ struct Dom {
       const double dx = 2.0 / (d.msfty(i, j) + d.msfty(i - 1, j)) / d.c.rdx, mu = H * (mh(i, j, k) + mh(i - 1, j, k)),
                    vel = d.ru_m(i, j, k);
       // advect_scalar_pd, degrade_xs, i == ids+1 only: vel = ru/mu, cr = vel*dt/dx (scalars.cpp:88-89)
-      const double cr = (d.spec && i == 1) ? (vel / mu) * dt / dx : vel * dt / dx / mu;
+      const double cr = (d.flx && i == 1) ? (vel / mu) * dt / dx : vel * dt / dx / mu;
       return mu * (dx / dt) * flux_upwind(qo(i - 1, j, k), qo(i, j, k), cr);
     }
     double fyl(int i, int j, int k) const {
@@ -1324,14 +1542,34 @@ struct Dom {
       const double cr = vel * dt / dz / mu;
       return mu * (dz / dt) * flux_upwind(qo(i, j, k - 1), qo(i, j, k), cr);
     }
-    // the spec zone (scalars.cpp:100-107): not advected; the limiter leaves the outermost row alone
+    // the spec zone (scalars.cpp:100-103): not advected
     bool spec_cell(int i, int j) const {
       return d.spec && (i == 0 || i == d.g.nx - 1 || j == 0 || j == d.g.ny - 1);
     }
-    double adv(int i, int j, int k) const {  // advect_scalar, horizontal terms times msftx
+    // the outermost row of a specified or open boundary: advect_scalar_pd does not limit it (scalars.cpp:104-107)
+    bool edge_cell(int i, int j) const {
+      return (d.flx && (i == 0 || i == d.g.nx - 1)) || (d.fly && (j == 0 || j == d.g.ny - 1));
+    }
+    bool open_col(int i) const { return d.ox && (i == 0 || i == d.g.nx - 1); }
+    bool open_row(int j) const { return d.oy && (j == 0 || j == d.g.ny - 1); }
+    // the open terms, x then y (scalars.cpp:110-116); fo: q itself (advect_scalar) or q_1 (advect_scalar_pd)
+    double open_terms(double r, int i, int j, int k, const F& fo) const {
+      if (open_col(i)) r = r - open_x_scalar(q, fo, d.ru_m, d.c.rdx, i, j, k, d.g.nx);
+      if (open_row(j)) r = r - open_y_scalar(q, fo, d.rv_m, d.c.rdy, i, j, k, d.g.ny);
+      return r;
+    }
+    double adv(int i, int j, int k) const {  // advect_scalar, horizontal terms times msftx (scalars.cpp:126-139)
       if (spec_cell(i, j)) return 0;
-      double r = -(d.msftx(i, j) * d.c.rdx) * (fx(i + 1, j, k) - fx(i, j, k));
-      if (!d.g.two_d()) r -= (d.msftx(i, j) * d.c.rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+      double r;
+      if (open_col(i) || open_row(j)) {  // WRF order: y flux, x flux, open x, open y, vertical
+        r = 0.0;
+        if (!d.g.two_d() && !open_row(j)) r = r - (d.msftx(i, j) * d.c.rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+        if (!open_col(i)) r = r - (d.msftx(i, j) * d.c.rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+        r = open_terms(r, i, j, k, q);
+      } else {
+        r = -(d.msftx(i, j) * d.c.rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+        if (!d.g.two_d()) r -= (d.msftx(i, j) * d.c.rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+      }
       return r - d.c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
     }
   };
@@ -1376,7 +1614,7 @@ struct Dom {
       loz(i, j, k) = l;
     });
     pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {  // KScalePD
-      if (f.spec_cell(i, j)) { sc_scale(i, j, k) = 1; return; }
+      if (f.edge_cell(i, j)) { sc_scale(i, j, k) = 1; return; }  // WRF limits interior cells only
       const double mxy = msftx(i, j) * msfty(i, j);
       double hlow = c.rdx * (lox(i + 1, j, k) - lox(i, j, k));
       if (do_y) hlow += c.rdy * (loy(i, j + 1, k) - loy(i, j, k));
@@ -1417,9 +1655,19 @@ struct Dom {
         qnew[n](i, j, k) = m_full(k, mub(i, j), mu1(i, j)) * qo(i, j, k) / m_full(k, mub(i, j), mu2(i, j));
         return;
       }
-      double tend = -c.rdnw[k] * (ddz(i, j, k + 1) - ddz(i, j, k) + loz(i, j, k + 1) - loz(i, j, k));
-      tend -= msftx(i, j) * (c.rdx * (ddx(i + 1, j, k) - ddx(i, j, k) + lox(i + 1, j, k) - lox(i, j, k)));
-      if (do_y) tend -= msftx(i, j) * (c.rdy * (ddy(i, j + 1, k) - ddy(i, j, k) + loy(i, j + 1, k) - loy(i, j, k)));
+      double tend;
+      if (f.open_col(i) || f.open_row(j)) {  // open: open terms (with q_1), vertical, x (not outer columns), y
+        tend = f.open_terms(0.0, i, j, k, qo);
+        tend = tend - c.rdnw[k] * (ddz(i, j, k + 1) - ddz(i, j, k) + loz(i, j, k + 1) - loz(i, j, k));
+        if (!f.open_col(i))
+          tend -= msftx(i, j) * (c.rdx * (ddx(i + 1, j, k) - ddx(i, j, k) + lox(i + 1, j, k) - lox(i, j, k)));
+        if (do_y && !f.open_row(j))
+          tend -= msftx(i, j) * (c.rdy * (ddy(i, j + 1, k) - ddy(i, j, k) + loy(i, j + 1, k) - loy(i, j, k)));
+      } else {
+        tend = -c.rdnw[k] * (ddz(i, j, k + 1) - ddz(i, j, k) + loz(i, j, k + 1) - loz(i, j, k));
+        tend -= msftx(i, j) * (c.rdx * (ddx(i + 1, j, k) - ddx(i, j, k) + lox(i + 1, j, k) - lox(i, j, k)));
+        if (do_y) tend -= msftx(i, j) * (c.rdy * (ddy(i, j + 1, k) - ddy(i, j, k) + loy(i, j + 1, k) - loy(i, j, k)));
+      }
       qnew[n](i, j, k) = (m_full(k, mub(i, j), mu1(i, j)) * qo(i, j, k) + dt_rk * (tend * msfty(i, j))) /
                          m_full(k, mub(i, j), mu2(i, j));
     });
@@ -1602,12 +1850,14 @@ int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv,
   try {
     if (!g || !p || !lv || !out) return fail("null argument");
     if (g->nx < 1 || g->ny < 1 || g->nz < 3) return fail("bad grid size");
-    if (g->bc_x > MI_BC_SPECIFIED || g->bc_y > MI_BC_SPECIFIED || g->bc_x < 0 || g->bc_y < 0)
-      return fail("oracle: periodic, symmetric or specified boundaries");
+    if (g->bc_x > MI_BC_OPEN || g->bc_y > MI_BC_OPEN || g->bc_x < 0 || g->bc_y < 0)
+      return fail("oracle: periodic, symmetric, specified or open boundaries");
     if ((g->bc_x == MI_BC_SPECIFIED) != (g->bc_y == MI_BC_SPECIFIED))
       return fail("specified boundaries on both x and y or on neither");
     if (g->bc_x == MI_BC_SPECIFIED && (g->ny == 1 || g->nx < 10 || g->ny < 10))
       return fail("specified boundaries need a 3-D grid of at least 10 x 10 columns");
+    if ((g->bc_x == MI_BC_OPEN && g->nx < 7) || (g->bc_y == MI_BC_OPEN && g->ny > 1 && g->ny < 7))
+      return fail("an open direction needs at least 7 columns");
     if (p->h_adv_order != 5 && p->h_adv_order != 6) return fail("h_adv_order must be 5 or 6");
     if (p->v_adv_order != 3 && p->v_adv_order != 4) return fail("v_adv_order must be 3 or 4");
     if (p->hypsometric_opt != 1) return fail("oracle: hypsometric_opt 1 only");
