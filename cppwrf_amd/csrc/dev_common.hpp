@@ -43,7 +43,17 @@ struct Geo {
   bool two_d;
   __host__ __device__ idx_t at(int i, int j, int k) const { return i + j * sj + k * sk; }
   __host__ __device__ idx_t at2(int i, int j) const { return i + j * sj; }
+  // the lateral boundary kinds (reference grid.hpp:45-53, kernels_common.hpp:33-44): specified (both
+  // directions), open (WRF open_xs/xe, open_ys/ye) and flat = specified or open (the outermost row copied
+  // outward, the advection orders degraded next to the edge)
+  __host__ __device__ bool spec() const { return bcx == MI_BC_SPECIFIED; }
+  __host__ __device__ bool ox() const { return bcx == MI_BC_OPEN; }
+  __host__ __device__ bool oy() const { return bcy == MI_BC_OPEN && !two_d; }
+  __host__ __device__ bool flx() const { return bcx == MI_BC_SPECIFIED || bcx == MI_BC_OPEN; }
+  __host__ __device__ bool fly() const { return (bcy == MI_BC_SPECIFIED || bcy == MI_BC_OPEN) && !two_d; }
 };
+__host__ __device__ inline bool flat_bc(int bc) { return bc == MI_BC_SPECIFIED || bc == MI_BC_OPEN; }
+constexpr double kCb = 25.;  // phase speed of the open (radiative) boundaries (reference constants.hpp:19)
 
 // base-state field: full (si = 1, sj, sk of the domain) or a horizontally uniform profile (si = sj = 0)
 struct BField {

@@ -631,13 +631,16 @@ namespace {
 
 void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv) {
   if (g.nx < 1 || g.ny < 1 || g.nz < 3) throw Error(MI_EINVAL, "grid: need nx >= 1, ny >= 1, nz >= 3");
-  if (g.bc_x < MI_BC_PERIODIC || g.bc_x > MI_BC_SPECIFIED) throw Error(MI_EINVAL, "bc_x: periodic, symmetric or specified");
-  if (g.bc_y < MI_BC_PERIODIC || g.bc_y > MI_BC_SPECIFIED) throw Error(MI_EINVAL, "bc_y: periodic, symmetric or specified");
+  if (g.bc_x < MI_BC_PERIODIC || g.bc_x > MI_BC_OPEN) throw Error(MI_EINVAL, "bc_x: periodic, symmetric, specified or open");
+  if (g.bc_y < MI_BC_PERIODIC || g.bc_y > MI_BC_OPEN) throw Error(MI_EINVAL, "bc_y: periodic, symmetric, specified or open");
   if ((g.bc_x == MI_BC_SPECIFIED) != (g.bc_y == MI_BC_SPECIFIED))
     throw Error(MI_EINVAL, "specified boundaries on both x and y or on neither");
   // the relaxation zone of 5 rows on each side, and the 3-row margin of the sixth-order diffusion
   if (g.bc_x == MI_BC_SPECIFIED && (g.ny == 1 || g.nx < 10 || g.ny < 10))
     throw Error(MI_EINVAL, "specified boundaries need a 3-D grid of at least 10 x 10 columns");
+  // an open side degrades the stencils over its 3 outer rows (the radiation face, 2nd, 4th order)
+  if ((g.bc_x == MI_BC_OPEN && g.nx < 7) || (g.ny > 1 && g.bc_y == MI_BC_OPEN && g.ny < 7))
+    throw Error(MI_EINVAL, "an open direction needs at least 7 columns");
   if (g.bc_x == MI_BC_PERIODIC && g.nx < 3) throw Error(MI_EINVAL, "periodic x needs nx >= 3");
   if (g.ny > 1 && g.bc_y == MI_BC_PERIODIC && g.ny < 3) throw Error(MI_EINVAL, "periodic y needs ny >= 3");
   // a symmetric fill with fewer than 4 points mirrors halo slots into one another (order-dependent)

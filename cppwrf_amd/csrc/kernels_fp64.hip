@@ -157,12 +157,12 @@ void launch_cols(const char* name, K kern, int nx, int ny, int nz, hipStream_t s
 // ----------------------------------------------------------------------------- halo fill
 // One halo slot of one direction (reference semantics src/core/halo.cpp:52-84): slot a = 0..7 maps to
 // i = 0, -1, -2, -3, n, n+1, n+2, n+3 on face-staggered points and -1, -2, -3, n, n+1, n+2 otherwise.
-// bc = MI_BC_SPECIFIED: the outermost row copied outward (halo.cpp:67-69, WRF set_physical_bc*).
+// bc = MI_BC_SPECIFIED or MI_BC_OPEN: the outermost row copied outward (halo.cpp:67-69, WRF set_physical_bc*).
 template <class T>
 __device__ __forceinline__ void halo_slot(T* base, idx_t sn, int a, int n, bool face, int bc, int par) {
   const int nl = face ? 4 : 3;
   const int i = a < nl ? (face ? -a : -1 - a) : n + (a - nl);
-  if (bc == MI_BC_SPECIFIED) {
+  if (flat_bc(bc)) {
     const int last = face ? n : n - 1;
     const int src = i < 0 ? 0 : (i > last ? last : i);
     if (src != i) base[idx_t(i) * sn] = base[idx_t(src) * sn];
@@ -219,7 +219,7 @@ __global__ void k_halo_y(Geo g, HaloSet s) {
 __device__ __forceinline__ int src_of(int i, int n, bool face, int bc, int par, bool& neg) {
   neg = false;
   const int last = face ? n : n - 1;
-  if (bc == MI_BC_SPECIFIED) return i < 0 ? 0 : (i > last ? last : i);
+  if (flat_bc(bc)) return i < 0 ? 0 : (i > last ? last : i);
   if (bc == MI_BC_PERIODIC) return ((i % n) + n) % n;  // a face slot n takes point 0
   if (i >= 0 && i <= last) return i;
   neg = par != 0;
@@ -357,17 +357,17 @@ MI_COLUMN_KERNEL k_large_prep(Args a) {
     auto mu = [&](int ii, int jj) { return f.mu2[g.at2(ii, jj)]; };
     auto mub = [&](int ii, int jj) { return f.mub[g.at2(ii, jj)]; };
     const double mut = mass ? mu(i, j) + mub(i, j) : 0.0;
-    // a specified edge face takes its column's mass twice (large_step.cpp:55-63)
-    const bool sp = g.bcx == MI_BC_SPECIFIED;
+    // a specified or open edge face takes its column's mass twice (large_step.cpp:55-63)
+    const bool spx = g.flx(), spy = g.fly();
     double muu = 0.0, muv = 0.0;
     if (j < ny) {
-      if (sp && i == 0) muu = H * (mu(0, j) + mu(0, j) + mub(0, j) + mub(0, j));
-      else if (sp && i == nx) muu = H * (mu(nx - 1, j) + mu(nx - 1, j) + mub(nx - 1, j) + mub(nx - 1, j));
+      if (spx && i == 0) muu = H * (mu(0, j) + mu(0, j) + mub(0, j) + mub(0, j));
+      else if (spx && i == nx) muu = H * (mu(nx - 1, j) + mu(nx - 1, j) + mub(nx - 1, j) + mub(nx - 1, j));
       else muu = H * (mu(i, j) + mu(i - 1, j) + mub(i, j) + mub(i - 1, j));
     }
     if (i < nx && j < g.nyv) {
-      if (sp && j == 0) muv = H * (mu(i, 0) + mu(i, 0) + mub(i, 0) + mub(i, 0));
-      else if (sp && j == ny) muv = H * (mu(i, ny - 1) + mu(i, ny - 1) + mub(i, ny - 1) + mub(i, ny - 1));
+      if (spy && j == 0) muv = H * (mu(i, 0) + mu(i, 0) + mub(i, 0) + mub(i, 0));
+      else if (spy && j == ny) muv = H * (mu(i, ny - 1) + mu(i, ny - 1) + mub(i, ny - 1) + mub(i, ny - 1));
       else muv = H * (mu(i, j) + mu(i, j - 1) + mub(i, j) + mub(i, j - 1));
     }
     if (cp.kt == 0) {
@@ -457,6 +457,40 @@ __device__ __forceinline__ int ord_mass(int f, int n, bool on) {
   return 5;
 }
 
+// Open boundaries, advect_scalar / advect_scalar_pd (kernels_common.hpp:60-81): the outermost column (row) takes,
+// instead of the flux divergence, rdx*(ub*(q_old(inner)-q_old(outer)) + q*(ru(b)-ru(a))) with the inflow mean
+// normal mass flux of the cell; returned without sign
+template <class T, class R>
+__device__ __forceinline__ double open_x_scalar(const Geo& g, const T* q, const T* qo, const R* ru, double rdx, int i,
+                                                int j, int k) {
+  auto Q = [&](const T* p, int ii) { return double(p[g.at(ii, j, k)]); };
+  auto RU = [&](int ii) { return double(ru[g.at(ii, j, k)]); };
+  const int nx = g.nx;
+  if (i == 0) {
+    double ub = 0.5 * (RU(0) + RU(1));
+    ub = ub < 0.0 ? ub : 0.0;
+    return rdx * (ub * (Q(qo, 1) - Q(qo, 0)) + Q(q, 0) * (RU(1) - RU(0)));
+  }
+  double ub = 0.5 * (RU(nx - 1) + RU(nx));
+  ub = ub > 0.0 ? ub : 0.0;
+  return rdx * (ub * (Q(qo, nx - 1) - Q(qo, nx - 2)) + Q(q, nx - 1) * (RU(nx) - RU(nx - 1)));
+}
+template <class T, class R>
+__device__ __forceinline__ double open_y_scalar(const Geo& g, const T* q, const T* qo, const R* rv, double rdy, int i,
+                                                int j, int k) {
+  auto Q = [&](const T* p, int jj) { return double(p[g.at(i, jj, k)]); };
+  auto RV = [&](int jj) { return double(rv[g.at(i, jj, k)]); };
+  const int ny = g.ny;
+  if (j == 0) {
+    double vb = 0.5 * (RV(0) + RV(1));
+    vb = vb < 0.0 ? vb : 0.0;
+    return rdy * (vb * (Q(qo, 1) - Q(qo, 0)) + Q(q, 0) * (RV(1) - RV(0)));
+  }
+  double vb = 0.5 * (RV(ny - 1) + RV(ny));
+  vb = vb > 0.0 ? vb : 0.0;
+  return rdy * (vb * (Q(qo, ny - 1) - Q(qo, ny - 2)) + Q(q, ny - 1) * (RV(ny) - RV(ny - 1)));
+}
+
 MI_POINT_KERNEL k_tend_u(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -467,15 +501,15 @@ MI_POINT_KERNEL k_tend_u(Range r, Args a) {
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const auto* u = f.u2;
   const double hup = a.q.hup, vup = a.q.vup;
-  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  const bool spx = g.flx(), spy = g.fly(), sx = g.spec();
   auto fx = [&](idx_t y, int fi) {  // the flux at mass point fi - 1, between the u points y-1 and y
-    const int o = ord_mass(fi, g.nx, sp);
-    if (o == 2) {  // the upstream normal velocity at the boundary face (large_step.cpp:160-167)
+    const int o = ord_mass(fi, g.nx, spx);
+    if (o == 2) {  // specified: the upstream normal velocity at the boundary face (large_step.cpp:160-167)
       if (fi <= 1) {
-        const double ub = u[y] < 0.0 ? double(u[y]) : double(u[y - 1]);
+        const double ub = (sx && u[y] < 0.0) ? double(u[y]) : double(u[y - 1]);
         return acc(Q * (f.ru[y] + f.ru[y - 1]) * (u[y] + ub));
       }
-      const double ub = u[y - 1] > 0.0 ? double(u[y - 1]) : double(u[y]);
+      const double ub = (sx && u[y - 1] > 0.0) ? double(u[y - 1]) : double(u[y]);
       return acc(Q * (f.ru[y] + f.ru[y - 1]) * (u[y - 1] + ub));
     }
     const acc vel = acc(H * (f.ru[y] + f.ru[y - 1]));
@@ -483,7 +517,7 @@ MI_POINT_KERNEL k_tend_u(Range r, Args a) {
     return acc(vel * flux5<acc>(u[y - 3], u[y - 2], u[y - 1], u[y], u[y + 1], u[y + 2], vel, acc(hup)));
   };
   auto fy = [&](idx_t y, int fj) {
-    const int o = ord_face(fj, g.ny, sp);
+    const int o = ord_face(fj, g.ny, spy);
     if (o == 2) return acc(Q * (f.rv[y] + f.rv[y - 1]) * (u[y] + u[y - sj]));
     const acc vel = acc(H * (f.rv[y] + f.rv[y - 1]));
     if (o == 3) return acc(vel * flux3<acc>(u[y - 2 * sj], u[y - sj], u[y], u[y + sj], vel, acc(hup)));
@@ -503,9 +537,51 @@ MI_POINT_KERNEL k_tend_u(Range r, Args a) {
     return H * (c.fnm[kk] * pu(kk) + c.fnp[kk] * pu(kk - 1));
   };
   const double msfux = f.msfux[x2], msfuy = f.msfuy[x2];
+  auto RU = [&](int ii, int jj) { return double(f.ru[g.at(ii, jj, k)]); };
+  auto RV = [&](int ii, int jj) { return double(f.rv[g.at(ii, jj, k)]); };
+  auto UU = [&](int ii, int jj) { return double(u[g.at(ii, jj, k)]); };
+  if (g.ox() && (i == 0 || i == g.nx)) {
+    // open x boundary face (large_step.cpp:211-228): the radiation condition with phase speed cb and, unless
+    // open_ys/ye, the vertical advection; no horizontal flux divergence, pressure gradient, Coriolis, curvature
+    double t;
+    if (i == 0) {
+      double ub = RU(0, j) - kCb * (c.c1h[k] * f.mut[g.at2(0, j)] + c.c2h[k]);
+      ub = ub < 0.0 ? ub : 0.0;
+      t = 0.0 - c.rdx * ub * (UU(1, j) - UU(0, j));
+    } else {
+      double ub = RU(i, j) + kCb * (c.c1h[k] * f.mut[g.at2(i - 1, j)] + c.c2h[k]);
+      ub = ub > 0.0 ? ub : 0.0;
+      t = 0.0 - c.rdx * ub * (UU(i, j) - UU(i - 1, j));
+    }
+    if (!g.oy()) t -= c.rdnw[k] * (fz(k + 1) - fz(k));
+    if (a.q.use_tendf) t += f.ru_tendf[x] / msfuy;  // rk_addtend_dry
+    f.ru_tend[x] = t;
+    return;
+  }
   double t = -(msfux * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
-  if (!g.two_d) t -= (msfux * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
-  t -= c.rdnw[k] * (fz(k + 1) - fz(k));
+  if (!g.two_d) {
+    if (!(g.oy() && (j == 0 || j == g.ny - 1))) {
+      t -= (msfux * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+    } else {  // advect_u open_ys / open_ye: v-flux terms of the outermost rows (large_step.cpp:193-207)
+      const int imax = g.nx - 1, ip = i < imax ? i : imax, im = i - 1 > 0 ? i - 1 : 0;
+      const double mrdy = msfux * c.rdy;
+      double ot;
+      if (j == 0) {
+        const double vw = H * (RV(ip, 0) + RV(im, 0));
+        const double vb = vw < 0.0 ? vw : 0.0;
+        const double dvm = RV(ip, 1) - RV(ip, 0), dvp = RV(im, 1) - RV(im, 0);
+        ot = mrdy * (vb * (UU(i, 1) - UU(i, 0)) + H * UU(i, 0) * (dvm + dvp));
+      } else {
+        const int je = g.ny;
+        const double vw = H * (RV(ip, je) + RV(im, je));
+        const double vb = vw > 0.0 ? vw : 0.0;
+        const double dvm = RV(ip, je) - RV(ip, je - 1), dvp = RV(im, je) - RV(im, je - 1);
+        ot = mrdy * (vb * (UU(i, je - 1) - UU(i, je - 2)) + H * UU(i, je - 1) * (dvm + dvp));
+      }
+      t -= ot;
+    }
+  }
+  if (!(g.oy() && spx && (i == 0 || i == g.nx))) t -= c.rdnw[k] * (fz(k + 1) - fz(k));
   const double rm = msfux / msfuy;
   double dpx = rm * H * c.rdx * (c.c1h[k] * f.muu[x2] + c.c2h[k]) *
                ((f.ph2[x + sk] - f.ph2[x + sk - 1] + f.ph2[x] - f.ph2[x - 1]) + (f.alt[x] + f.alt[x - 1]) * (f.p[x] - f.p[x - 1]) +
@@ -538,22 +614,22 @@ MI_POINT_KERNEL k_tend_v(Range r, Args a) {
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const auto* v = f.v2;
   const double hup = a.q.hup, vup = a.q.vup;
-  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  const bool spx = g.flx(), spy = g.fly(), sy = g.spec();
   auto fx = [&](idx_t y, int fi) {
-    const int o = ord_face(fi, g.nx, sp);
+    const int o = ord_face(fi, g.nx, spx);
     if (o == 2) return acc(Q * (f.ru[y] + f.ru[y - sj]) * (v[y] + v[y - 1]));
     const acc vel = acc(H * (f.ru[y] + f.ru[y - sj]));
     if (o == 3) return acc(vel * flux3<acc>(v[y - 2], v[y - 1], v[y], v[y + 1], vel, acc(hup)));
     return acc(vel * flux5<acc>(v[y - 3], v[y - 2], v[y - 1], v[y], v[y + 1], v[y + 2], vel, acc(hup)));
   };
   auto fy = [&](idx_t y, int fj) {  // the flux at mass row fj - 1
-    const int o = ord_mass(fj, g.ny, sp);
+    const int o = ord_mass(fj, g.ny, spy);
     if (o == 2) {  // the upstream normal velocity at the boundary face (large_step.cpp:282-289)
       if (fj <= 1) {
-        const double vb = v[y] < 0.0 ? double(v[y]) : double(v[y - sj]);
+        const double vb = (sy && v[y] < 0.0) ? double(v[y]) : double(v[y - sj]);
         return acc(Q * (f.rv[y] + f.rv[y - sj]) * (v[y] + vb));
       }
-      const double vb = v[y - sj] > 0.0 ? double(v[y - sj]) : double(v[y]);
+      const double vb = (sy && v[y - sj] > 0.0) ? double(v[y - sj]) : double(v[y]);
       return acc(Q * (f.rv[y] + f.rv[y - sj]) * (vb + v[y - sj]));
     }
     const acc vel = acc(H * (f.rv[y] + f.rv[y - sj]));
@@ -574,9 +650,50 @@ MI_POINT_KERNEL k_tend_v(Range r, Args a) {
     return H * (c.fnm[kk] * pv(kk) + c.fnp[kk] * pv(kk - 1));
   };
   const double msfvy = f.msfvy[x2], msfvx = f.msfvx[x2];
+  auto RU = [&](int ii, int jj) { return double(f.ru[g.at(ii, jj, k)]); };
+  auto RV = [&](int ii, int jj) { return double(f.rv[g.at(ii, jj, k)]); };
+  auto VV = [&](int ii, int jj) { return double(v[g.at(ii, jj, k)]); };
+  if (g.oy() && (j == 0 || j == g.ny)) {  // open y boundary face: radiation only (large_step.cpp:327-340)
+    double t;
+    if (j == 0) {
+      double vb = RV(i, 0) - kCb * (c.c1h[k] * f.mut[g.at2(i, 0)] + c.c2h[k]);
+      vb = vb < 0.0 ? vb : 0.0;
+      t = 0.0 - c.rdy * vb * (VV(i, 1) - VV(i, 0));
+    } else {
+      double vb = RV(i, j) + kCb * (c.c1h[k] * f.mut[g.at2(i, j - 1)] + c.c2h[k]);
+      vb = vb > 0.0 ? vb : 0.0;
+      t = 0.0 - c.rdy * vb * (VV(i, j) - VV(i, j - 1));
+    }
+    if (a.q.use_tendf) t += f.rv_tendf[x] * f.msfvx_inv[x2];  // rk_addtend_dry
+    f.rv_tend[x] = t;
+    return;
+  }
   const double rm = msfvy / msfvx;
-  double t = -(msfvy * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
-  t -= (msfvy * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+  double t;
+  if (g.ox() && (i == 0 || i == g.nx - 1)) {
+    // open x: the y flux, then advect_v's open_xs / open_xe term instead of the x flux (large_step.cpp:307-323)
+    t = -(msfvy * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+    const int jmax = g.ny - 1, jp = j < jmax ? j : jmax, jm = j - 1 > 0 ? j - 1 : 0;
+    double ot;
+    if (i == 0) {
+      const double mrdx = f.msfvy[g.at2(0, j)] * c.rdx;
+      const double uw = H * (RU(0, jp) + RU(0, jm));
+      const double ub = uw < 0.0 ? uw : 0.0;
+      const double dup = RU(1, jp) - RU(0, jp), dum = RU(1, jm) - RU(0, jm);
+      ot = mrdx * (ub * (VV(1, j) - VV(0, j)) + H * VV(0, j) * (dup + dum));
+    } else {
+      const int ie = g.nx;
+      const double mrdx = f.msfvy[g.at2(ie - 1, j)] * c.rdx;
+      const double uw = H * (RU(ie, jp) + RU(ie, jm));
+      const double ub = uw > 0.0 ? uw : 0.0;
+      const double dup = RU(ie, jp) - RU(ie - 1, jp), dum = RU(ie, jm) - RU(ie - 1, jm);
+      ot = mrdx * (ub * (VV(ie - 1, j) - VV(ie - 2, j)) + H * VV(ie - 1, j) * (dup + dum));
+    }
+    t -= ot;
+  } else {
+    t = -(msfvy * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+    t -= (msfvy * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+  }
   t -= rm * c.rdnw[k] * (fz(k + 1) - fz(k));
   double dpy = rm * H * c.rdy * (c.c1h[k] * f.muv[x2] + c.c2h[k]) *
                ((f.ph2[x + sk] - f.ph2[x + sk - sj] + f.ph2[x] - f.ph2[x - sj]) + (f.alt[x] + f.alt[x - sj]) * (f.p[x] - f.p[x - sj]) +
@@ -617,17 +734,17 @@ MI_POINT_KERNEL k_tend_w(Range r, Args a) {
     if (k < nz) return c.fnm[k] * f.rv[y] + c.fnp[k] * f.rv[y - sk];
     return (2.0 - c.fnm[nz - 1]) * f.rv[y - sk] - c.fnp[nz - 1] * f.rv[y - 2 * sk];
   };
-  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  const bool spx = g.flx(), spy = g.fly();
   auto fx = [&](idx_t y, int fi) {
     const acc vel = acc(rux(y));
-    const int o = ord_face(fi, g.nx, sp);
+    const int o = ord_face(fi, g.nx, spx);
     if (o == 2) return acc(H * vel * (w[y] + w[y - 1]));
     if (o == 3) return acc(vel * flux3<acc>(w[y - 2], w[y - 1], w[y], w[y + 1], vel, acc(hup)));
     return acc(vel * flux5<acc>(w[y - 3], w[y - 2], w[y - 1], w[y], w[y + 1], w[y + 2], vel, acc(hup)));
   };
   auto fy = [&](idx_t y, int fj) {
     const acc vel = acc(rvy(y));
-    const int o = ord_face(fj, g.ny, sp);
+    const int o = ord_face(fj, g.ny, spy);
     if (o == 2) return acc(H * vel * (w[y] + w[y - sj]));
     if (o == 3) return acc(vel * flux3<acc>(w[y - 2 * sj], w[y - sj], w[y], w[y + sj], vel, acc(hup)));
     return acc(vel * flux5<acc>(w[y - 3 * sj], w[y - 2 * sj], w[y - sj], w[y], w[y + sj], w[y + 2 * sj], vel, acc(hup)));
@@ -639,8 +756,63 @@ MI_POINT_KERNEL k_tend_w(Range r, Args a) {
     return acc(vel * flux3<acc>(w[y - sk], w[y], w[y + sk], w[y + 2 * sk], -vel, acc(vup)));
   };
   const double msftx = f.msftx[x2], msfty = f.msfty[x2];
-  double t = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
-  if (!g.two_d) t -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+  double t;
+  const bool ocol = g.ox() && (i == 0 || i == g.nx - 1), orow = g.oy() && (j == 0 || j == g.ny - 1);
+  if (ocol || orow) {
+    // advect_w's open terms (large_step.cpp:416-466), WRF order: y flux, x flux, open x terms, open y terms;
+    // ru (rv) at w level k from the two u (v) faces of the edge column (row), extrapolated at the lid
+    auto RU = [&](int ii, int kk) { return double(f.ru[g.at(ii, j, kk)]); };
+    auto RV = [&](int jj, int kk) { return double(f.rv[g.at(i, jj, kk)]); };
+    auto WW = [&](int ii, int jj) { return double(w[g.at(ii, jj, k)]); };
+    t = 0.0;
+    if (!g.two_d && !orow) t = t - (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+    if (!ocol) t = t - (msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+    if (ocol) {
+      const int ea = i == 0 ? 0 : g.nx - 1, eb = i == 0 ? 1 : g.nx;
+      double uw, dru;
+      if (k < nz) {
+        uw = H * (c.fnm[k] * (RU(ea, k) + RU(eb, k)) + c.fnp[k] * (RU(ea, k - 1) + RU(eb, k - 1)));
+        dru = c.fnm[k] * (RU(eb, k) - RU(ea, k)) + c.fnp[k] * (RU(eb, k - 1) - RU(ea, k - 1));
+      } else {
+        const double fm = 2.0 - c.fnm[nz - 1], fp = c.fnp[nz - 1];
+        uw = H * (fm * (RU(ea, nz - 1) + RU(eb, nz - 1)) - fp * (RU(ea, nz - 2) + RU(eb, nz - 2)));
+        dru = fm * (RU(eb, nz - 1) - RU(ea, nz - 1)) - fp * (RU(eb, nz - 2) - RU(ea, nz - 2));
+      }
+      double ot;
+      if (i == 0) {
+        const double ub = uw < 0.0 ? uw : 0.0;
+        ot = c.rdx * (ub * (WW(1, j) - WW(0, j)) + WW(0, j) * dru);
+      } else {
+        const double ub = uw > 0.0 ? uw : 0.0;
+        ot = c.rdx * (ub * (WW(i, j) - WW(i - 1, j)) + WW(i, j) * dru);
+      }
+      t = t - ot;
+    }
+    if (orow) {
+      const int ea = j == 0 ? 0 : g.ny - 1, eb = j == 0 ? 1 : g.ny;
+      double vw, drv;
+      if (k < nz) {
+        vw = H * (c.fnm[k] * (RV(ea, k) + RV(eb, k)) + c.fnp[k] * (RV(ea, k - 1) + RV(eb, k - 1)));
+        drv = c.fnm[k] * (RV(eb, k) - RV(ea, k)) + c.fnp[k] * (RV(eb, k - 1) - RV(ea, k - 1));
+      } else {
+        const double fm = 2.0 - c.fnm[nz - 1], fp = c.fnp[nz - 1];
+        vw = H * (fm * (RV(ea, nz - 1) + RV(eb, nz - 1)) - fp * (RV(ea, nz - 2) + RV(eb, nz - 2)));
+        drv = fm * (RV(eb, nz - 1) - RV(ea, nz - 1)) - fp * (RV(eb, nz - 2) - RV(ea, nz - 2));
+      }
+      double ot;
+      if (j == 0) {
+        const double vb = vw < 0.0 ? vw : 0.0;
+        ot = c.rdy * (vb * (WW(i, 1) - WW(i, 0)) + WW(i, 0) * drv);
+      } else {
+        const double vb = vw > 0.0 ? vw : 0.0;
+        ot = c.rdy * (vb * (WW(i, j) - WW(i, j - 1)) + WW(i, j) * drv);
+      }
+      t = t - ot;
+    }
+  } else {
+    t = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+    if (!g.two_d) t -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+  }
   const double rmsft = 1.0 / msfty;
   if (k < nz) {
     t -= c.rdn[k] * (fz(k) - fz(k - 1));
@@ -680,17 +852,17 @@ MI_POINT_KERNEL k_tend_t(Range r, Args a) {
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const auto* t2 = f.t2;
   const double hup = a.q.hup, vup = a.q.vup;
-  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  const bool spx = g.flx(), spy = g.fly();
   auto fx = [&](idx_t y, int fi) {
     const acc vel = acc(f.ru[y]);
-    const int o = ord_face(fi, g.nx, sp);
+    const int o = ord_face(fi, g.nx, spx);
     if (o == 2) return acc(H * vel * (t2[y] + t2[y - 1]));
     if (o == 3) return acc(vel * flux3<acc>(t2[y - 2], t2[y - 1], t2[y], t2[y + 1], vel, acc(hup)));
     return acc(vel * flux5<acc>(t2[y - 3], t2[y - 2], t2[y - 1], t2[y], t2[y + 1], t2[y + 2], vel, acc(hup)));
   };
   auto fy = [&](idx_t y, int fj) {
     const acc vel = acc(f.rv[y]);
-    const int o = ord_face(fj, g.ny, sp);
+    const int o = ord_face(fj, g.ny, spy);
     if (o == 2) return acc(H * vel * (t2[y] + t2[y - sj]));
     if (o == 3) return acc(vel * flux3<acc>(t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], vel, acc(hup)));
     return acc(vel * flux5<acc>(t2[y - 3 * sj], t2[y - 2 * sj], t2[y - sj], t2[y], t2[y + sj], t2[y + 2 * sj], vel, acc(hup)));
@@ -703,8 +875,18 @@ MI_POINT_KERNEL k_tend_t(Range r, Args a) {
     return acc(vel * flux3<acc>(t2[y - 2 * sk], t2[y - sk], t2[y], t2[y + sk], -vel, acc(vup)));
   };
   const double msftx = f.msftx[x2];
-  double rr = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
-  if (!g.two_d) rr -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+  double rr;
+  const bool ocol = g.ox() && (i == 0 || i == g.nx - 1), orow = g.oy() && (j == 0 || j == g.ny - 1);
+  if (ocol || orow) {  // open boundary: WRF order y flux, x flux, open x terms, open y terms (large_step.cpp:534-541)
+    rr = 0.0;
+    if (!g.two_d && !orow) rr = rr - (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+    if (!ocol) rr = rr - (msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+    if (ocol) rr = rr - open_x_scalar(a.g, t2, t2, f.ru, c.rdx, i, j, k);
+    if (orow) rr = rr - open_y_scalar(a.g, t2, t2, f.rv, c.rdy, i, j, k);
+  } else {
+    rr = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+    if (!g.two_d) rr -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+  }
   rr -= c.rdnw[k] * (fz(k + 1) - fz(k));
   if (a.q.use_tendf) rr += f.t_tendf[x] / f.msfty[x2];  // rk_addtend_dry
   // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
@@ -749,14 +931,14 @@ MI_POINT_KERNEL k_tend_ph(Range r, Args a) {
   // one-sided differences (dfwd) of ph + phb at index y (base point (bi, bj))
   auto dfwdx = [&](idx_t y, int bi) { return f.phb(bi + 1, j, k) - f.phb(bi, j, k) + ph[y + 1] - ph[y]; };
   auto dfwdy = [&](idx_t y, int bj) { return f.phb(i, bj + 1, k) - f.phb(i, bj, k) + ph[y + sj] - ph[y]; };
-  // rhs_ph next to specified edges (large_step.cpp:596-605): none on the outer row, 2nd order on the next,
-  // then 4th order in y (none in x), 6th from the fourth row in
-  const bool sp = g.bcx == MI_BC_SPECIFIED;
-  auto horder = [&](int ii, int n, bool is_x) {
-    if (!sp) return 6;
+  // rhs_ph next to specified or open edges (large_step.cpp:596-605): none on the outer row, 2nd order on the
+  // next, then 4th order in y (in x only with open_xs/xe), 6th from the fourth row in
+  const bool sp = g.spec(), ox = g.ox();
+  auto horder = [&](int ii, int n, bool on, bool is_x) {
+    if (!on) return 6;
     if (ii <= 0 || ii >= n - 1) return 0;
     if (ii == 1 || ii == n - 2) return 2;
-    if (ii == 2 || ii == n - 3) return is_x ? 0 : 4;
+    if (ii == 2 || ii == n - 3) return (is_x && !ox) ? 0 : 4;
     return 6;
   };
   const double msfty = f.msfty[x2];
@@ -770,7 +952,7 @@ MI_POINT_KERNEL k_tend_ph(Range r, Args a) {
     const double vp = k < nz ? v[x + sj] + v[x + sj - sk] : c.cfn * v[xt + sj] + c.cfn1 * v[xt + sj - sk];
     const double vm = k < nz ? v[x] + v[x - sk] : c.cfn * v[xt] + c.cfn1 * v[xt - sk];
     const double cy = (k < nz ? 0.25 : H) * c.rdy / msfty;
-    const int o = horder(j, g.ny, false);
+    const int o = horder(j, g.ny, g.fly(), false);
     if (o == 6)
       t = t - cy * ((mvp * vp * f.msfvy[x2 + sj] + mvm * vm * f.msfvy[x2]) * r60 * grad6y());
     else if (o == 4)
@@ -782,13 +964,43 @@ MI_POINT_KERNEL k_tend_ph(Range r, Args a) {
   const double up = k < nz ? u[x + 1] + u[x + 1 - sk] : c.cfn * u[xt + 1] + c.cfn1 * u[xt + 1 - sk];
   const double um = k < nz ? u[x] + u[x - sk] : c.cfn * u[xt] + c.cfn1 * u[xt - sk];
   const double cx = (k < nz ? 0.25 : H) * c.rdx / msfty;
-  const int o = horder(i, g.nx, true);
+  const int o = horder(i, g.nx, g.flx(), true);
   if (o == 6)
     t = t - cx * ((mup * up * f.msfux[x2 + 1] + mum * um * f.msfux[x2]) * r60 * grad6x());
   else if (o == 2)
     t = t - cx * (mup * up * f.msfux[x2 + 1] * dfwdx(x, i) + mum * um * f.msfux[x2] * dfwdx(x - 1, i - 1));
   else if (o == 4)
     t = t - cx * ((mup * up * f.msfux[x2 + 1] + mum * um * f.msfux[x2]) * r12 * grad4x());
+  // open boundaries (large_step.cpp:636-661), south/north then west/east: inflow advection of ph by the normal
+  // wind at the w level (the west lid level takes none: WRF's kz = kde there reads fnm/fnp = 0)
+  if (g.oy() && (j == 0 || j == g.ny - 1)) {
+    const int kz = k < nz - 1 ? k : nz - 1;
+    const int ea = j == 0 ? 0 : g.ny - 1, eb = j == 0 ? 1 : g.ny;
+    auto V = [&](int jj, int kk) { return double(v[g.at(i, jj, kk)]); };
+    const double vb = H * (c.fnm[kz] * (V(eb, kz) + V(ea, kz)) + c.fnp[kz] * (V(eb, kz - 1) + V(ea, kz - 1)));
+    const double muf = c.c1f[k] * f.mut[x2] + c.c2f[k];
+    if (j == 0) {
+      const double vl = vb < 0.0 ? vb : 0.0;
+      t = t - c.rdy * muf * (vl * (double(ph[g.at(i, 1, k)]) - ph[g.at(i, 0, k)]));
+    } else {
+      const double vr = vb > 0.0 ? vb : 0.0;
+      t = t - c.rdy * muf * (vr * (double(ph[x]) - ph[x - sj]));
+    }
+  }
+  if (ox && (i == 0 || i == g.nx - 1) && !(i == 0 && k == nz)) {
+    const int kz = k < nz ? k : nz - 1;
+    const int ea = i == 0 ? 0 : g.nx - 1, eb = i == 0 ? 1 : g.nx;
+    auto U = [&](int ii, int kk) { return double(u[g.at(ii, j, kk)]); };
+    const double ub = H * (c.fnm[kz] * (U(eb, kz) + U(ea, kz)) + c.fnp[kz] * (U(eb, kz - 1) + U(ea, kz - 1)));
+    const double muf = c.c1f[k] * f.mut[x2] + c.c2f[k];
+    if (i == 0) {
+      const double ul = ub < 0.0 ? ub : 0.0;
+      t = t - (f.msftx[x2] / msfty) * c.rdx * muf * (ul * (double(ph[g.at(1, j, k)]) - ph[g.at(0, j, k)]));
+    } else {
+      const double ur = ub > 0.0 ? ub : 0.0;
+      t = t - (f.msftx[x2] / msfty) * c.rdx * muf * (ur * (double(ph[x]) - ph[x - 1]));
+    }
+  }
   if (sp) t = t + f.ph_tendf[x] / msfty;  // rk_addtend_dry with a LateralBC (large_step.cpp:663)
   f.ph_tend[x] = t;
 }
@@ -960,20 +1172,21 @@ MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
   // spec_bdyupdate, u += dts ru_tend (LateralBC::update_uv, lateral_bc.cpp:356-364), fused in; the launch then
   // covers the east face i = nx and the north face j = ny too
   const int nx = g.nx, ny = g.ny;
-  const bool sp = g.bcx == MI_BC_SPECIFIED;
+  // Open boundaries (small_step.cpp:175-184): the boundary faces take the large-step tendency only
+  const bool sp = g.spec();
   const bool uface = sp && (i <= 0 || i >= nx || j == 0 || j == ny - 1);
   const bool vface = sp && (i == 0 || i == nx - 1 || j <= 0 || j >= ny);
   const int ui0 = g.bcx != MI_BC_PERIODIC ? 1 : 0;
-  if (uface) {
-    if (j < ny) put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x]);
-  } else if (i >= ui0 && i < nx && j < ny) {
+  if ((uface || (g.ox() && (i == 0 || i == nx))) && j < ny) {
+    put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x]);
+  } else if (!uface && i >= ui0 && i < nx && j < ny) {
     const double mudf_x = -a.q.emdx * (f.mudf[x2] - f.mudf[x2 - 1]) / f.msfuy[x2];
     put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x] - dts * (a.q.moist ? f.cqu[x] : 1.0) * dp(1, 1, 0, f.muu[x2], c.rdx, f.msfux[x2] / f.msfuy[x2]) + c.c1h[k] * mudf_x);
   }
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
-  if (vface) {
-    if (i < nx) put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x]);
-  } else if (!g.two_d && j >= vj0 && j < ny && i < nx) {
+  if ((vface || (g.oy() && (j == 0 || j == ny))) && i < nx) {
+    put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x]);
+  } else if (!vface && !g.two_d && j >= vj0 && j < ny && i < nx) {
     const idx_t sj = g.sj;
     const double mudf_y = -a.q.emdy * (f.mudf[x2] - f.mudf[x2 - sj]) * f.msfvx_inv[x2];
     put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x] - dts * (a.q.moist ? f.cqv[x] : 1.0) * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) + c.c1h[k] * mudf_y);
@@ -1184,6 +1397,16 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       if (!g.two_d) {
         const double sv = (first ? 0.0 : rvm) + v0;
         put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
+      }
+      // KSumEdge (small_step.cpp:279-290, 560-569): an open direction's east (north) domain face is a real face,
+      // summed by the column next to it (with specified boundaries k_acoustic_ring does it)
+      if (g.ox() && i == g.nx - 1) {
+        const double se = (first ? 0.0 : double(f.ru_m[x + 1])) + u1;
+        put<kHU>(f.ru_m, g, i + 1, j, k * sk, last ? se / rn + (c.c1h[k] * muu1 + c.c2h[k]) * us1 / msfuy1 : se);
+      }
+      if (g.oy() && j == g.ny - 1) {
+        const double sn = (first ? 0.0 : double(f.rv_m[x + sj])) + v1;
+        put<kHV>(f.rv_m, g, i, j + 1, k * sk, last ? sn / rn + (c.c1h[k] * muv1 + c.c2h[k]) * vs1 * mvi1 : sn);
       }
       double hflux = H * c.rdx * (double(u1) * (double(tsxp) + ts) - double(u0) * (double(ts) + tsxm));
       if (!g.two_d) hflux += H * c.rdy * (double(v1) * (double(tsyp) + ts) - double(v0) * (double(ts) + tsym));
@@ -1555,9 +1778,36 @@ __device__ __forceinline__ double sixth_at(double tend, const Args& a, const rea
 // sides, three rows in from specified ones ('u' and 'v' two on their own staggered side)
 __device__ __forceinline__ bool sixth_in(const Geo& g, int diff, char kind, int i, int j) {
   if (!diff) return false;
-  if (g.bcx != MI_BC_SPECIFIED) return true;
-  const int ie = kind == 'u' ? g.nx - 2 : g.nx - 3, je = kind == 'v' ? g.ny - 2 : g.ny - 3;
-  return i >= 3 && i < ie && j >= 3 && j < je;
+  if (g.spec()) {
+    const int ie = kind == 'u' ? g.nx - 2 : g.nx - 3, je = kind == 'v' ? g.ny - 2 : g.ny - 3;
+    return i >= 3 && i < ie && j >= 3 && j < je;
+  }
+  // open sides (sixth_order.hpp:179-204): 3 rows in, the end 3 (u, v along their stagger) or 4 rows in; 'v' with
+  // open_xs takes the specified limits on all four sides; a periodic x resets the x limits
+  const bool ox = g.ox(), oy = g.oy();
+  if (!ox && !oy) return true;
+  int is = 0, js = 0, ie, je;
+  if (kind == 'u') {
+    ie = g.nx + 1;
+    je = g.ny;
+    if (ox) { is = 3; ie = g.nx - 2; }
+    if (oy) { js = 3; je = g.ny - 3; }
+  } else if (kind == 'v') {
+    ie = g.nx;
+    je = g.ny + 1;
+    if (ox) { is = 3; ie = g.nx - 3; js = 3; je = g.ny - 2; }
+    if (oy) { js = 3; je = g.ny - 2; }
+  } else {
+    ie = g.nx;
+    je = g.ny;
+    if (ox) { is = 3; ie = g.nx - 3; }
+    if (oy) { js = 3; je = g.ny - 3; }
+  }
+  if (g.bcx == MI_BC_PERIODIC) {
+    is = 0;
+    ie = kind == 'u' ? g.nx + 1 : g.nx;
+  }
+  return i >= is && i < ie && j >= js && j < je;
 }
 
 // zero_tendf + DiffusionDriver::dynamics (diffusion_driver.cpp:266-281): the held tendencies of u, v, w,
@@ -1668,11 +1918,11 @@ struct ScFaces {
   __device__ acc f3(acc m2, acc m1, acc q0, acc p1, acc v, acc up) const {
     return pd ? flux3_pd<acc>(m2, m1, q0, p1, v, up) : flux3<acc>(m2, m1, q0, p1, v, up);
   }
-  __device__ bool sp() const { return a.g.bcx == MI_BC_SPECIFIED; }
+  __device__ bool sp() const { return a.g.spec(); }
   // x face at U point y (west face of cell fi), the order degraded next to a specified edge
   __device__ acc fx(idx_t y, int fi) const {
     const acc vel = a.f.ru_m[y];
-    const int o = ord_face(fi, a.g.nx, sp());
+    const int o = ord_face(fi, a.g.nx, a.g.flx());
     if (o == 2) return acc(H) * vel * (acc(q[y]) + acc(q[y - 1]));
     if (o == 3) return vel * f3(q[y - 2], q[y - 1], q[y], q[y + 1], vel, acc(a.q.hup));
     return vel * f5(q[y - 3], q[y - 2], q[y - 1], q[y], q[y + 1], q[y + 2], vel);
@@ -1680,14 +1930,26 @@ struct ScFaces {
   __device__ acc fy(idx_t y, int fj) const {
     const idx_t sj = a.g.sj;
     const acc vel = a.f.rv_m[y];
-    const int o = ord_face(fj, a.g.ny, sp());
+    const int o = ord_face(fj, a.g.ny, a.g.fly());
     if (o == 2) return acc(H) * vel * (acc(q[y]) + acc(q[y - sj]));
     if (o == 3) return vel * f3(q[y - 2 * sj], q[y - sj], q[y], q[y + sj], vel, acc(a.q.hup));
     return vel * f5(q[y - 3 * sj], q[y - 2 * sj], q[y - sj], q[y], q[y + sj], q[y + 2 * sj], vel);
   }
-  // the spec zone (scalars.cpp:100-107): not advected; the limiter leaves the outermost row alone
+  // the spec zone (scalars.cpp:100-103): not advected
   __device__ bool spec_cell(int i, int j) const {
     return sp() && (i == 0 || i == a.g.nx - 1 || j == 0 || j == a.g.ny - 1);
+  }
+  // the outermost row of a specified or open boundary: the limiter leaves it alone (scalars.cpp:104-107)
+  __device__ bool edge_cell(int i, int j) const {
+    return (a.g.flx() && (i == 0 || i == a.g.nx - 1)) || (a.g.fly() && (j == 0 || j == a.g.ny - 1));
+  }
+  __device__ bool open_col(int i) const { return a.g.ox() && (i == 0 || i == a.g.nx - 1); }
+  __device__ bool open_row(int j) const { return a.g.oy() && (j == 0 || j == a.g.ny - 1); }
+  // the open terms, x then y (scalars.cpp:110-116); fo: q itself (advect_scalar) or q_1 (advect_scalar_pd)
+  __device__ double open_terms(double r, int i, int j, int k, const real* fo) const {
+    if (open_col(i)) r = r - open_x_scalar(a.g, q, fo, a.f.ru_m, a.c.rdx, i, j, k);
+    if (open_row(j)) r = r - open_y_scalar(a.g, q, fo, a.f.rv_m, a.c.rdy, i, j, k);
+    return r;
   }
   // w level kk of the column whose level-0 index is y0
   __device__ acc fz(idx_t y0, int kk) const {
@@ -1705,7 +1967,7 @@ struct ScFaces {
     const Fields& f = a.f;
     const acc dx = acc(f.pd_dx[y2]), mu = acc(H) * (mh(y2, k) + mh(y2 - 1, k)), vel = f.ru_m[y];
     // advect_scalar_pd, degrade_xs, i == ids+1 only: vel = ru/mu, cr = vel*dt/dx (scalars.cpp:88-89)
-    const acc cr = (sp() && fi == 1) ? qdiv(qdiv(vel, mu) * dt, dx) : qdiv(qdiv(vel * dt, dx), mu);
+    const acc cr = (a.g.flx() && fi == 1) ? qdiv(qdiv(vel, mu) * dt, dx) : qdiv(qdiv(vel * dt, dx), mu);
     return mu * acc(f.pd_dxdt[y2]) * flux_upwind<acc>(qo[y - 1], qo[y], cr);
   }
   __device__ acc fyl(idx_t y, idx_t y2, int k) const {
@@ -1768,7 +2030,13 @@ MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
     const int n = l.sp[sq_];
     const ScFaces fl{a, s.q2[n], s.q1[n], acc(dt_rk), false};
     acc adv = 0;
-    if (!fl.spec_cell(i, j)) {
+    if (fl.open_col(i) || fl.open_row(j)) {  // WRF order: y flux, x flux, open x, open y, vertical (scalars.cpp:129-134)
+      acc r0 = 0;
+      if (!g.two_d && !fl.open_row(j)) r0 = r0 - (msftx * acc(c.rdy)) * (fl.fy(x + g.sj, j + 1) - fl.fy(x, j));
+      if (!fl.open_col(i)) r0 = r0 - (msftx * acc(c.rdx)) * (fl.fx(x + 1, i + 1) - fl.fx(x, i));
+      r0 = acc(fl.open_terms(double(r0), i, j, k, s.q2[n]));
+      adv = r0 - acc(c.rdnw[k]) * (fl.fz(x0, k + 1) - fl.fz(x0, k));
+    } else if (!fl.spec_cell(i, j)) {
       adv = -(msftx * acc(c.rdx)) * (fl.fx(x + 1, i + 1) - fl.fx(x, i));
       if (!g.two_d) adv -= (msftx * acc(c.rdy)) * (fl.fy(x + g.sj, j + 1) - fl.fy(x, j));
       adv = adv - acc(c.rdnw[k]) * (fl.fz(x0, k + 1) - fl.fz(x0, k));
@@ -1833,7 +2101,7 @@ MI_POINT_KERNEL k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk
   {
     const int n = l.sp[sq_];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt, true};
-    if (fl.spec_cell(i, j)) {  // WRF limits interior cells only
+    if (fl.edge_cell(i, j)) {  // WRF limits interior cells only
       put<kHMass>(s.scale[n], g, i, j, k * g.sk, 1.0);
       return;
     }
@@ -1882,11 +2150,22 @@ MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
     // eta decreases upward: a positive w-level flux leaves the cell above it
     const acc dz0 = k == 0 ? acc(0) : lim(p.dhz0, sc0, sc[x - sk]);
     const acc dz1 = k + 1 == nz ? acc(0) : lim(p.dhz1, sc[x + sk], sc0);
-    acc tend = -acc(c.rdnw[k]) * (dz1 - dz0 + p.loz1 - p.loz0);
-    tend -= msftx * (acc(c.rdx) * (dx1 - dx0 + p.lox1 - p.lox0));
-    if (!g.two_d) {
-      const acc dy0 = lim(p.dhy0, sc[x - sj], sc0), dy1 = lim(p.dhy1, sc0, sc[x + sj]);
-      tend -= msftx * (acc(c.rdy) * (dy1 - dy0 + p.loy1 - p.loy0));
+    acc tend;
+    if (fl.open_col(i) || fl.open_row(j)) {  // open: open terms (with q_1), vertical, x (not outer columns), y
+      tend = acc(fl.open_terms(0.0, i, j, k, s.q1[n]));
+      tend = tend - acc(c.rdnw[k]) * (dz1 - dz0 + p.loz1 - p.loz0);
+      if (!fl.open_col(i)) tend -= msftx * (acc(c.rdx) * (dx1 - dx0 + p.lox1 - p.lox0));
+      if (!g.two_d && !fl.open_row(j)) {
+        const acc dy0 = lim(p.dhy0, sc[x - sj], sc0), dy1 = lim(p.dhy1, sc0, sc[x + sj]);
+        tend -= msftx * (acc(c.rdy) * (dy1 - dy0 + p.loy1 - p.loy0));
+      }
+    } else {
+      tend = -acc(c.rdnw[k]) * (dz1 - dz0 + p.loz1 - p.loz0);
+      tend -= msftx * (acc(c.rdx) * (dx1 - dx0 + p.lox1 - p.lox0));
+      if (!g.two_d) {
+        const acc dy0 = lim(p.dhy0, sc[x - sj], sc0), dy1 = lim(p.dhy1, sc0, sc[x + sj]);
+        tend -= msftx * (acc(c.rdy) * (dy1 - dy0 + p.loy1 - p.loy0));
+      }
     }
     put<kHMass>(s.qn[n], g, i, j, k * sk, qdiv(m_old * acc(s.q1[n][x]) + dt * (tend * msfty), m_new));
   }
@@ -2215,8 +2494,10 @@ void launch_tendencies(const Args& a, hipStream_t st) {
   const Geo& g = a.g;
   const int ui0 = g.bcx != MI_BC_PERIODIC ? 1 : 0;
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
-  launch3("tend_u", k_tend_u, Range{ui0, g.nx, 0, g.ny, 0, g.nz}, st, a);
-  if (!g.two_d) launch3("tend_v", k_tend_v, Range{0, g.nx, vj0, g.ny, 0, g.nz}, st, a);
+  // open boundaries: the boundary faces are advanced too (large_step.cpp:804-806)
+  launch3("tend_u", k_tend_u, Range{g.ox() ? 0 : ui0, g.ox() ? g.nx + 1 : g.nx, 0, g.ny, 0, g.nz}, st, a);
+  if (!g.two_d)
+    launch3("tend_v", k_tend_v, Range{0, g.nx, g.oy() ? 0 : vj0, g.oy() ? g.ny + 1 : g.ny, 0, g.nz}, st, a);
   launch3("tend_w", k_tend_w, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
   launch3("tend_t", k_tend_t, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
   launch3("tend_ph", k_tend_ph, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
@@ -2231,8 +2512,9 @@ void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st) {
 }
 void launch_uv(const Args& a, double dts, hipStream_t st) {
   const Geo& g = a.g;
-  const int e = g.bcx == MI_BC_SPECIFIED ? 1 : 0;  // the east and north faces of the spec zone
-  launch3("uv", k_uv, Range{0, g.nx + e, 0, g.ny + e, 0, g.nz}, st, a, dts);
+  // the east (north) faces of the spec zone or of an open direction
+  const int ex = (g.spec() || g.ox()) ? 1 : 0, ey = (g.spec() || g.oy()) ? 1 : 0;
+  launch3("uv", k_uv, Range{0, g.nx + ex, 0, g.ny + ey, 0, g.nz}, st, a, dts);
 }
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
   dispatch_cfg<AcousticL>(cfg, "acoustic", 10, 5, a.g.nx, a.g.ny, st, a, dts, it, ns);

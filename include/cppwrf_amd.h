@@ -46,7 +46,9 @@ extern "C" {
 #define MI_ENONFINITE (-4) /* a non-finite value was found by mi_check_finite */
 #define MI_ENODEV (-5)     /* no GPU, or the library was built without device code */
 
-/* lateral boundary conditions, in the order of the reference's enum BC (src/core/grid.hpp:19) */
+/* lateral boundary conditions, in the order of the reference's enum BC (src/core/grid.hpp:19).  SPECIFIED on
+ * both directions or neither (a 3-D grid of at least 10 x 10 columns, data through mi_bdy_*); OPEN (WRF
+ * open_xs/xe, open_ys/ye: radiation faces and inflow terms) on either or both, at least 7 columns across */
 enum { MI_BC_PERIODIC = 0, MI_BC_SYMMETRIC = 1, MI_BC_SPECIFIED = 2, MI_BC_OPEN = 3 };
 
 /* arithmetic of a domain: plain FP64 (bit-identical to the reference CPU FP64 build) or FP32-bulk: the
