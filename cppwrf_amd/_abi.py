@@ -28,6 +28,7 @@ MI_RU_TEND, MI_RV_TEND, MI_RW_TEND, MI_T_TEND, MI_PH_TEND = range(15, 20)
 MI_MUB, MI_PB, MI_ALB, MI_PHB, MI_T_INIT = range(32, 37)
 MI_MSFTX, MI_MSFTY, MI_MSFUX, MI_MSFUY, MI_MSFVX, MI_MSFVY, MI_MSFVX_INV = range(40, 47)
 MI_CQU, MI_CQV, MI_CQW, MI_H_DIABATIC = range(20, 24)
+MI_RU_TENDF, MI_RV_TENDF, MI_RW_TENDF, MI_T_TENDF, MI_TH_FORCING, MI_QV_FORCING = range(24, 30)
 MI_F, MI_E, MI_SINA, MI_COSA, MI_HT = range(47, 52)
 MI_Q, MI_Q_TENDF = 64, 96
 MI_MAX_MOIST, MI_MAX_SCALARS = 8, 16
@@ -57,6 +58,7 @@ STAGGER = {
     MI_RU_TEND: "U", MI_RV_TEND: "V", MI_RW_TEND: "W", MI_T_TEND: "M", MI_PH_TEND: "W",
     MI_MUB: "M2", MI_PB: "M", MI_ALB: "M", MI_PHB: "W", MI_T_INIT: "M",
     MI_CQU: "U", MI_CQV: "V", MI_CQW: "W", MI_H_DIABATIC: "M",
+    MI_RU_TENDF: "U", MI_RV_TENDF: "V", MI_RW_TENDF: "W", MI_T_TENDF: "M", MI_TH_FORCING: "M", MI_QV_FORCING: "M",
     MI_MSFTX: "M2", MI_MSFTY: "M2", MI_MSFUX: "U2", MI_MSFUY: "U2", MI_MSFVX: "V2", MI_MSFVY: "V2",
     MI_MSFVX_INV: "V2", MI_F: "M2", MI_E: "M2", MI_SINA: "M2", MI_COSA: "M2", MI_HT: "M2",
 }
@@ -69,7 +71,8 @@ FIELD_NAMES = {
     MI_WW: "ww", MI_RU_M: "ru_m", MI_RV_M: "rv_m", MI_WW_M: "ww_m", MI_ALT: "alt", MI_PHP: "php",
     MI_MUTS: "muts", MI_RU_TEND: "ru_tend", MI_RV_TEND: "rv_tend", MI_RW_TEND: "rw_tend",
     MI_T_TEND: "t_tend", MI_PH_TEND: "ph_tend", MI_CQU: "cqu", MI_CQV: "cqv", MI_CQW: "cqw",
-    MI_H_DIABATIC: "h_diabatic",
+    MI_H_DIABATIC: "h_diabatic", MI_RU_TENDF: "ru_tendf", MI_RV_TENDF: "rv_tendf", MI_RW_TENDF: "rw_tendf",
+    MI_T_TENDF: "t_tendf", MI_TH_FORCING: "rthften", MI_QV_FORCING: "rqvften",
 }
 for _n in range(MI_MAX_SCALARS):
     FIELD_NAMES[MI_Q + _n] = f"q{_n}"
@@ -149,6 +152,15 @@ _I64 = C.c_int64
 # mi_cfl_fn / mi_stage_fn
 CFL_FN = C.CFUNCTYPE(None, _VP, C.c_double, C.c_double)
 STAGE_FN = C.CFUNCTYPE(None, _VP)
+# mi_phys_fn (user, domain)
+PHYS_FN = C.CFUNCTYPE(None, _VP, _VP)
+
+
+class MiColumnPhysics(C.Structure):
+    """mi_column_physics"""
+    _fields_ = [("user", _VP), ("part1", PHYS_FN), ("part2", PHYS_FN), ("after_rk", PHYS_FN),
+                ("after_all_rk_steps", PHYS_FN), ("th_forcing", C.c_int32), ("qv_forcing", C.c_int32)]
+
 
 # name -> (restype, argtypes), every function include/cppwrf_amd.h declares
 SIGNATURES = {
@@ -186,6 +198,7 @@ SIGNATURES = {
     "mi_domain_load_state": (_I, [_VP, _VP, _I64]),
     "mi_domain_set_cfl_callback": (_I, [_VP, CFL_FN, _VP]),
     "mi_domain_set_first_rk_prep_callback": (_I, [_VP, STAGE_FN, _VP]),
+    "mi_domain_set_column_physics": (_I, [_VP, C.POINTER(MiColumnPhysics)]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
     "mi_domain_autotune": (_I, [_VP, _I]),
     "mi_domain_get_tuning": (_I, [_VP, C.POINTER(_I), _I]),

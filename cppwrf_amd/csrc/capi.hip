@@ -339,6 +339,10 @@ int mi_domain_set_first_rk_prep_callback(mi_domain* d, mi_stage_fn fn, void* use
   return mi_detail::guard([&] { checked(d)->set_prep_callback(fn, user); });
 }
 
+int mi_domain_set_column_physics(mi_domain* d, const mi_column_physics* cp) {
+  return mi_detail::guard([&] { checked(d)->set_column_physics(cp); });
+}
+
 int mi_check_finite(mi_domain* d, int* bad_field) {
   int bad = -1;
   const int rc = mi_detail::guard([&] { bad = checked(d)->check_finite(bad_field); });

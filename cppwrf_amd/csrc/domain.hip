@@ -241,9 +241,19 @@ struct Domain final : mi_domain {
   //
   // With specified boundaries: mu_tend zeroed every stage (large_step.cpp:815-818), the relaxation of rows
   // 1..3 into the held tendencies at the first stage (:825) and the spec zone's tendencies last (:859).
-  void rk_tendency(int rk) {
+  //
+  // With a physics plugin (hooks on) the held tendencies were zeroed before part1; part2 sets the physics
+  // tendencies first (large_step.cpp:820-821) and the diffusion adds to them.
+  void rk_tendency(int rk, bool hooks = true) {
     if (spec) launch_fill(a.f.mu_tend - off, n2, 0.0, stream());
-    if (rk == 1 && a.q.use_tendf) launch_sixth_dyn(a, stream());
+    if (rk == 1 && a.q.use_tendf) {
+      if (phys_on(hooks)) {
+        call_phys(phys.part2);
+        launch_sixth_dyn(a, kSixthKeep, stream());
+      } else {
+        launch_sixth_dyn(a, kSixthZero, stream());
+      }
+    }
     if (rk == 1 && spec) launch_bdy(a, sc, bdy_dry(), kBdyRelax, stream());
     launch_tendencies(a, stream());
     if (spec) launch_bdy(a, sc, bdy_dry(), kBdySpec, stream());
@@ -337,7 +347,12 @@ struct Domain final : mi_domain {
   // the u, v halos first (set_w_surface, rk3.cpp:194; those refreshes change nothing without a
   // LateralBC).  The reference also reruns calc_p_rho_phi on the interior after spec_bdy_final: its
   // inputs there are unchanged and the scalar pow gives back the same bits, so it is left out.
-  void step_end() {
+  void step_end(bool hooks = true) {
+    if (phys_on(hooks)) call_phys(phys.after_rk);  // phy_prep_part2 (rk3.cpp:150)
+    step_end_dyn();
+    if (phys_on(hooks)) call_phys(phys.after_all_rk_steps);  // rk3.cpp:166
+  }
+  void step_end_dyn() {
     if (spec) {
       launch_bdy(a, sc, bdy_dry(), kBdyFinal, stream());
       launch_bdy(a, sc, bdy_scalars(), kBdyFinal, stream());
@@ -351,6 +366,8 @@ struct Domain final : mi_domain {
 
   // RK3::step (rk3.cpp:86-170); with callbacks (host-driven steps only) the stage hooks of the reference:
   // on_first_rk_prep after stage 1's prep (rk3.cpp:105), on_cfl after each stage's tendencies (rk3.cpp:115-119)
+  // A physics plugin's hooks run with the other hooks only: the autotuner's trial steps and graph captures
+  // (which a domain with a plugin never replays) leave them out.
   void step_launches(bool hooks = false) {
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
@@ -359,7 +376,8 @@ struct Domain final : mi_domain {
         hip_check(hipStreamSynchronize(stream()), "hipStreamSynchronize");
         prep_fn(prep_user);
       }
-      rk_tendency(rk);
+      if (rk == 1) physics_part1(hooks);
+      rk_tendency(rk, hooks);
       if (hooks && cfl_fn) {
         double vert = 0, horiz = 0;
         cfl_max(&vert, &horiz);
@@ -368,7 +386,60 @@ struct Domain final : mi_domain {
       small_steps(rk, true);
       scalar_transport(rk);
     }
-    step_end();
+    step_end(hooks);
+  }
+  // ------------------------------------------------------------------ column physics plugin
+  mi_column_physics phys{};
+  bool has_phys = false;
+  bool held_base = false;  // held tendencies without a plugin: a diffusion or specified boundaries
+  bool phys_on(bool hooks) const { return hooks && has_phys; }
+  void call_phys(mi_phys_fn fn) {
+    if (!fn) return;
+    hip_check(hipStreamSynchronize(stream()), "hipStreamSynchronize");
+    fn(phys.user, this);
+    hip_check(hipStreamSynchronize(stream()), "hipStreamSynchronize");
+  }
+  // first_rk_step_part1 (rk3.cpp:106-111): init_zero_tendency of every held tendency, scalars included, then part1
+  void physics_part1(bool hooks = true) {
+    if (!phys_on(hooks)) return;
+    launch_sixth_dyn(a, kSixthZeroOnly, stream());  // ru, rv, rw, t_tendf (and ph_tendf)
+    for (int n = 0; n < n_sc; ++n)
+      hip_check(hipMemsetAsync(sc.qtf[n] - off, 0, size_t(n3) * sizeof(real), stream()), "hipMemsetAsync");
+    call_phys(phys.part1);
+  }
+  void set_column_physics(const mi_column_physics* cp) override {
+    if (cp && cp->qv_forcing && prm.n_moist == 0) throw Error(MI_EINVAL, "qv_forcing needs moisture (n_moist > 0)");
+    if (cp && !a.f.ru_tendf) {  // the held tendencies, first needed now
+      add3(a.f.ru_tendf, St::U, MI_RU_TENDF);
+      add3(a.f.rv_tendf, St::V, MI_RV_TENDF);
+      add3(a.f.rw_tendf, St::W, MI_RW_TENDF);
+      add3(a.f.t_tendf, St::M, MI_T_TENDF);
+    }
+    // the scalars' step tendencies (the caller's plus diffusion and relaxation), read instead of MI_Q_TENDF
+    // once held tendencies are used
+    for (int n = 0; cp && n < n_sc; ++n)
+      if (!sc.qtw[n]) add3(sc.qtw[n], St::M, -1);
+    if (cp && cp->th_forcing && !thf_buf) thf_buf = add3(thf_buf, St::M, MI_TH_FORCING);
+    if (cp && cp->qv_forcing && !qvf_buf) qvf_buf = add3(qvf_buf, St::M, MI_QV_FORCING);
+    has_phys = cp != nullptr;
+    phys = cp ? *cp : mi_column_physics{};
+    a.q.use_tendf = (held_base || has_phys) ? 1 : 0;
+    if (a.q.use_tendf) {  // every buffer the kernels read or write with held tendencies must exist
+      bool ok = a.f.ru_tendf && a.f.rv_tendf && a.f.rw_tendf && a.f.t_tendf;
+      for (int n = 0; n < n_sc; ++n) ok = ok && sc.qtw[n];
+      if (!ok) throw Error(MI_EHIP, "held tendencies not allocated");
+    }
+    a.f.thf = (cp && cp->th_forcing) ? thf_buf : nullptr;
+    a.f.qvf = (cp && cp->qv_forcing) ? qvf_buf : nullptr;
+    drop_graphs();
+  }
+  real* thf_buf = nullptr;
+  real* qvf_buf = nullptr;
+  template <class T>
+  T* add3(T*& slot, St s, int id) {
+    slot = alloc<T>(n3) + off;
+    fields.push_back(FieldDesc{reinterpret_cast<void**>(&slot), s, id, int(sizeof(T))});
+    return slot;
   }
   mi_cfl_fn cfl_fn = nullptr;
   void* cfl_user = nullptr;
@@ -382,7 +453,7 @@ struct Domain final : mi_domain {
     prep_fn = fn;
     prep_user = user;
   }
-  bool has_callbacks() const override { return cfl_fn != nullptr || prep_fn != nullptr; }
+  bool has_callbacks() const override { return cfl_fn != nullptr || prep_fn != nullptr || has_phys; }
   // RK3::set_dt with LateralBC::set_dt and the diffusion's dt (the adaptive time step)
   void set_dt(double dt, int ns) override {
     if (!(dt > 0)) throw Error(MI_EINVAL, "dt must be > 0");
@@ -732,7 +803,8 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   d.spec = g.bc_x == MI_BC_SPECIFIED;
   const bool held = p.diff_6th_opt != 0 || d.spec;  // a diffusion driver or a LateralBC holds tendencies
   if (held) {
-    add3(f.ru_tendf, St::U, -1); add3(f.rv_tendf, St::V, -1); add3(f.rw_tendf, St::W, -1); add3(f.t_tendf, St::M, -1);
+    add3(f.ru_tendf, St::U, MI_RU_TENDF); add3(f.rv_tendf, St::V, MI_RV_TENDF); add3(f.rw_tendf, St::W, MI_RW_TENDF);
+    add3(f.t_tendf, St::M, MI_T_TENDF);
   }
   if (d.spec) {
     add3(f.ph_tendf, St::W, -1);
@@ -826,6 +898,7 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   q.mapped = p.mapped;
   q.cor = (p.mapped != 0 || p.coriolis != 0) ? 1 : 0;
   q.use_tendf = held ? 1 : 0;
+  d.held_base = held;
   q.spec = d.spec ? 1 : 0;
   q.diff_6th_opt = p.diff_6th_opt;
   q.diff_6th_factor = p.diff_6th_factor;
@@ -932,7 +1005,10 @@ void Domain::get_field(int id, double* host, int64_t n) {
 void Domain::phase(MiPhase which, int rk) {
   switch (which) {
     case kPhaseStepBegin: step_begin(); break;
-    case kPhaseRkPrep: rk_prep(rk); break;
+    case kPhaseRkPrep:
+      rk_prep(rk);
+      if (rk == 1) physics_part1();
+      break;
     case kPhaseRkTendency: rk_tendency(rk); break;
     case kPhaseSmallSteps: small_steps(rk, false); break;
     case kPhaseScalars: scalar_transport(rk); break;

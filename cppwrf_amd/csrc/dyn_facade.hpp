@@ -73,9 +73,10 @@ struct DomainOptions {
 inline size_t field_size(int field, const mi_grid& g) {
   const size_t nx = size_t(g.nx), ny = size_t(g.ny), nz = size_t(g.nz), vy = g.ny > 1 ? 1 : 0;
   switch (field) {
-    case MI_U: case MI_RU_M: case MI_RU_TEND: case MI_CQU: return (nx + 1) * ny * nz;
-    case MI_V: case MI_RV_M: case MI_RV_TEND: case MI_CQV: return nx * (ny + vy) * nz;
+    case MI_U: case MI_RU_M: case MI_RU_TEND: case MI_CQU: case MI_RU_TENDF: return (nx + 1) * ny * nz;
+    case MI_V: case MI_RV_M: case MI_RV_TEND: case MI_CQV: case MI_RV_TENDF: return nx * (ny + vy) * nz;
     case MI_W: case MI_PH: case MI_WW: case MI_WW_M: case MI_RW_TEND: case MI_PH_TEND: case MI_CQW: case MI_PHB:
+    case MI_RW_TENDF:
       return nx * ny * (nz + 1);
     case MI_MU: case MI_MUTS: case MI_MUB: case MI_MSFTX: case MI_MSFTY: case MI_F: case MI_E: case MI_SINA:
     case MI_COSA: case MI_HT:
@@ -104,6 +105,22 @@ class StateView {
  private:
   mi_domain* d_;
   mi_grid g_;
+};
+
+class RK3;
+// reference ColumnPhysics (src/dyn/column_physics.hpp:21-34): the column physics suite's hooks into the step.
+// The State and Tendencies are the domain's (rk.state()); part2 sets the physics tendencies into MI_RU_TENDF,
+// MI_RV_TENDF, MI_T_TENDF and MI_Q_TENDF + n.  th_forcing() / qv_forcing() = true keep the advective forcings
+// MI_TH_FORCING / MI_QV_FORCING (the reference returns the fields to fill).
+class ColumnPhysics {
+ public:
+  virtual ~ColumnPhysics() = default;
+  virtual void part1(RK3& rk) = 0;
+  virtual void part2(RK3& rk) = 0;
+  virtual void after_rk(RK3& rk) = 0;
+  virtual void after_all_rk_steps(RK3&) {}
+  virtual bool th_forcing() { return false; }
+  virtual bool qv_forcing() { return false; }
 };
 
 // reference dyn::RK3 (src/dyn/rk3.hpp:34-78)
@@ -174,6 +191,23 @@ class RK3 {
     check(mi_domain_set_first_rk_prep_callback(d_, on_first_rk_prep ? &RK3::prep_thunk : nullptr, this));
     check(mi_step(d_, nsteps));
   }
+  // reference RK3::set_column_physics: the plugin is not owned (rk3.hpp:53-62); nullptr detaches
+  void set_column_physics(ColumnPhysics* cp) {
+    cp_ = cp;
+    if (!cp) {
+      check(mi_domain_set_column_physics(d_, nullptr));
+      return;
+    }
+    mi_column_physics s{};
+    s.user = this;
+    s.part1 = [](void* u, mi_domain*) { auto* r = static_cast<RK3*>(u); r->cp_->part1(*r); };
+    s.part2 = [](void* u, mi_domain*) { auto* r = static_cast<RK3*>(u); r->cp_->part2(*r); };
+    s.after_rk = [](void* u, mi_domain*) { auto* r = static_cast<RK3*>(u); r->cp_->after_rk(*r); };
+    s.after_all_rk_steps = [](void* u, mi_domain*) { auto* r = static_cast<RK3*>(u); r->cp_->after_all_rk_steps(*r); };
+    s.th_forcing = cp->th_forcing() ? 1 : 0;
+    s.qv_forcing = cp->qv_forcing() ? 1 : 0;
+    check(mi_domain_set_column_physics(d_, &s));
+  }
   void sync() { check(mi_domain_sync(d_)); }
   StateView state() { return StateView(d_, g_); }
   mi_domain* handle() const { return d_; }
@@ -197,6 +231,7 @@ class RK3 {
   static void prep_thunk(void* self) { static_cast<RK3*>(self)->on_first_rk_prep(); }
   mi_grid g_;
   mi_domain* d_ = nullptr;
+  ColumnPhysics* cp_ = nullptr;
 };
 
 // Several domains stepped together, one HIP graph per step (the reference's tiles of main.cpp:30-42)

@@ -888,6 +888,18 @@ MI_POINT_KERNEL k_tend_t(Range r, Args a) {
     if (!g.two_d) rr -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
   }
   rr -= c.rdnw[k] * (fz(k + 1) - fz(k));
+  if (f.thf) {  // set_tend(RTHFTEN) (large_step.cpp:543-555): the advection times msfty; specified edges skip their flux
+    const bool scol = g.spec() && (i == 0 || i == g.nx - 1), srow = g.spec() && (j == 0 || j == g.ny - 1);
+    if (!scol && !srow) {
+      f.thf[x] = rr * f.msfty[x2];
+    } else {
+      double q = 0.0;
+      if (!scol) q = -(msftx * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
+      if (!g.two_d && !srow) q -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
+      q -= c.rdnw[k] * (fz(k + 1) - fz(k));
+      f.thf[x] = q * f.msfty[x2];
+    }
+  }
   if (a.q.use_tendf) rr += f.t_tendf[x] / f.msfty[x2];  // rk_addtend_dry
   // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
   if (a.q.moist) rr += (c.c1h[k] * f.mut[x2] + c.c2h[k]) * f.hdia[x] / f.msfty[x2];
@@ -1813,23 +1825,33 @@ __device__ __forceinline__ bool sixth_in(const Geo& g, int diff, char kind, int 
 // zero_tendf + DiffusionDriver::dynamics (diffusion_driver.cpp:266-281): the held tendencies of u, v, w,
 // theta (and ph with specified boundaries) over the (nx+1) x nyv x (nz+1) points, 0 where
 // sixth_order_diffusion does not reach (w levels 0, nz; the specified edges; everywhere without it)
-MI_POINT_KERNEL k_sixth_dyn(Range r, Args a, double coef) {
+MI_POINT_KERNEL k_sixth_dyn(Range r, Args a, double coef, int mode) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, diff = a.q.diff_6th_opt != 0;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, diff = mode != kSixthZeroOnly && a.q.diff_6th_opt != 0;
+  const bool keep = mode == kSixthKeep;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  // tendency(i,j,k) = tendency + tend_x + tend_y with the tendency just zeroed
-  if (j < ny && k < nz)
-    f.ru_tendf[x] = sixth_in(g, diff, 'u', i, j) ? sixth_at<'u'>(0.0, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : 0.0;
-  if (!g.two_d && i < nx && j < g.nyv && k < nz)
-    f.rv_tendf[x] = sixth_in(g, diff, 'v', i, j) ? sixth_at<'v'>(0.0, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : 0.0;
+  // tendency(i,j,k) = tendency + tend_x + tend_y with the tendency just zeroed, or holding 0 + the physics
+  // tendency of part2 (update_phy_ten adds into the zeroed field)
+  if (j < ny && k < nz) {
+    const double t0 = keep ? 0.0 + double(f.ru_tendf[x]) : 0.0;
+    f.ru_tendf[x] = sixth_in(g, diff, 'u', i, j) ? sixth_at<'u'>(t0, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
+  }
+  if (!g.two_d && i < nx && j < g.nyv && k < nz) {
+    const double t0 = keep ? 0.0 + double(f.rv_tendf[x]) : 0.0;
+    f.rv_tendf[x] = sixth_in(g, diff, 'v', i, j) ? sixth_at<'v'>(t0, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
+  }
   if (i < nx && j < ny) {
     const bool in = sixth_in(g, diff, 'm', i, j);
-    f.rw_tendf[x] = (in && k >= 1 && k < nz) ? sixth_at<'m'>(0.0, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], coef) : 0.0;
-    if (k < nz) f.t_tendf[x] = in ? sixth_at<'m'>(0.0, a, f.t2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : 0.0;
-    if (a.q.spec) f.ph_tendf[x] = 0.0;
+    const double w0 = keep ? 0.0 + double(f.rw_tendf[x]) : 0.0;
+    f.rw_tendf[x] = (in && k >= 1 && k < nz) ? sixth_at<'m'>(w0, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], coef) : w0;
+    if (k < nz) {
+      const double t0 = keep ? 0.0 + double(f.t_tendf[x]) : 0.0;
+      f.t_tendf[x] = in ? sixth_at<'m'>(t0, a, f.t2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
+    }
+    if (a.q.spec && !keep) f.ph_tendf[x] = 0.0;
   }
 }
 
@@ -2042,6 +2064,18 @@ MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
       adv = adv - acc(c.rdnw[k]) * (fl.fz(x0, k + 1) - fl.fz(x0, k));
     }
     acc tend = adv * msfty;
+    if (f.qvf && n == 0) {  // set_tend(RQVFTEN) of the vapour (scalars.cpp:147-150)
+      if (fl.spec_cell(i, j)) {  // adv_spec (scalars.cpp:119-125): no x flux in the outer columns, no y in the rows
+        const bool scol = i == 0 || i == g.nx - 1, srow = j == 0 || j == g.ny - 1;
+        acc rs = 0;
+        if (!scol) rs = -(msftx * acc(c.rdx)) * (fl.fx(x + 1, i + 1) - fl.fx(x, i));
+        if (!g.two_d && !srow) rs -= (msftx * acc(c.rdy)) * (fl.fy(x + g.sj, j + 1) - fl.fy(x, j));
+        rs = rs - acc(c.rdnw[k]) * (fl.fz(x0, k + 1) - fl.fz(x0, k));
+        f.qvf[x] = rs * msfty;
+      } else {
+        f.qvf[x] = tend;
+      }
+    }
     tend += acc((a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x]);
     put<kHMass>(s.qn[n], g, i, j, k * g.sk, qdiv(m_old * acc(s.q1[n][x]) + acc(dt_rk) * tend, m_new));
   }
@@ -2138,7 +2172,8 @@ MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
     const int n = l.sp[sq_];
     const real* sc = s.scale[n];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt, true};
-    if (fl.spec_cell(i, j)) {  // the spec zone: only the held tendency, folded into q_1 before
+    const bool ften = f.qvf && n == 0;
+    if (fl.spec_cell(i, j) && !ften) {  // the spec zone: only the held tendency, folded into q_1 before
       put<kHMass>(s.qn[n], g, i, j, k * sk, qdiv(m_old * acc(s.q1[n][x]), m_new));
       return;
     }
@@ -2150,6 +2185,19 @@ MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
     // eta decreases upward: a positive w-level flux leaves the cell above it
     const acc dz0 = k == 0 ? acc(0) : lim(p.dhz0, sc0, sc[x - sk]);
     const acc dz1 = k + 1 == nz ? acc(0) : lim(p.dhz1, sc[x + sk], sc0);
+    if (fl.spec_cell(i, j)) {  // RQVFTEN in the spec zone: WRF still forms the tendency, without the edge's own
+      // flux (scalars.cpp:262-269); the value is only the held tendency's
+      const bool scol = i == 0 || i == g.nx - 1, srow = j == 0 || j == g.ny - 1;
+      acc t = -acc(c.rdnw[k]) * (dz1 - dz0 + p.loz1 - p.loz0);
+      if (!scol) t -= msftx * (acc(c.rdx) * (dx1 - dx0 + p.lox1 - p.lox0));
+      if (!g.two_d && !srow) {
+        const acc dy0 = lim(p.dhy0, sc[x - sj], sc0), dy1 = lim(p.dhy1, sc0, sc[x + sj]);
+        t -= msftx * (acc(c.rdy) * (dy1 - dy0 + p.loy1 - p.loy0));
+      }
+      f.qvf[x] = t * msfty;
+      put<kHMass>(s.qn[n], g, i, j, k * sk, qdiv(m_old * acc(s.q1[n][x]), m_new));
+      return;
+    }
     acc tend;
     if (fl.open_col(i) || fl.open_row(j)) {  // open: open terms (with q_1), vertical, x (not outer columns), y
       tend = acc(fl.open_terms(0.0, i, j, k, s.q1[n]));
@@ -2167,6 +2215,7 @@ MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
         tend -= msftx * (acc(c.rdy) * (dy1 - dy0 + p.loy1 - p.loy0));
       }
     }
+    if (ften) f.qvf[x] = tend * msfty;
     put<kHMass>(s.qn[n], g, i, j, k * sk, qdiv(m_old * acc(s.q1[n][x]) + dt * (tend * msfty), m_new));
   }
 }
@@ -2522,10 +2571,10 @@ void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStre
 void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st) {
   dispatch_cfg<FinishL>(cfg, "finish", 1, 1, a.g.nx + 1, a.g.nyv, st, a, prhophi ? 1 : 0, hd_dt);
 }
-void launch_sixth_dyn(const Args& a, hipStream_t st) {
+void launch_sixth_dyn(const Args& a, int mode, hipStream_t st) {
   const Geo& g = a.g;
   const double coef = a.q.diff_6th_factor * 0.015625 / (2.0 * a.q.dt);
-  launch3a("sixth", 8, k_sixth_dyn, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, coef);  // u v w t -> 4 tendf
+  launch3a("sixth", 8, k_sixth_dyn, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, coef, mode);  // u v w t -> 4 tendf
 }
 void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, hipStream_t st) {
   if (n <= 0) return;

@@ -58,6 +58,8 @@ struct Fields {
   double* bdy_time;
   // the positive-definite limiter's face widths and their ratios to dt (2-D; pd_dz: dz, then dz / dt)
   double *pd_dx, *pd_dxdt, *pd_dy, *pd_dydt, *pd_dz;
+  // the advective forcings a column physics plugin asks for (WRF RTHFTEN, RQVFTEN); null when not wanted
+  real *thf, *qvf;
 };
 
 // The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
@@ -156,8 +158,10 @@ void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st);
 void launch_uv(const Args& a, double dts, hipStream_t st);
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st);
 void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st);
-// sixth-order diffusion of u, v, w, theta into the held tendencies (first stage)
-void launch_sixth_dyn(const Args& a, hipStream_t st);
+// the held tendencies of u, v, w, theta at the first stage: kSixthZero zeroes them and adds the sixth-order
+// diffusion; kSixthKeep adds it to what they hold (a physics plugin's part2); kSixthZeroOnly only zeroes
+enum SixthMode : int { kSixthZero = 0, kSixthKeep = 1, kSixthZeroOnly = 2 };
+void launch_sixth_dyn(const Args& a, int mode, hipStream_t st);
 // specified boundaries (reference LateralBC): ring launches over the variables of bs
 void launch_bdy(const Args& a, const Scalars& s, const BdySet& bs, BdyMode mode, hipStream_t st);
 void launch_bdy_zero_grad(const Args& a, hipStream_t st);

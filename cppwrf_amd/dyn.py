@@ -225,6 +225,43 @@ class RK3:
 
     def step(self, n: int = 1) -> None:
         check(lib().mi_step(self._h, n))
+        self._raise_hook_error()
+
+    def set_column_physics(self, cp) -> None:
+        """RK3::set_column_physics (reference src/dyn/column_physics.hpp:21-34): `cp` has the methods part1(rk3),
+        part2(rk3), after_rk(rk3), after_all_rk_steps(rk3) (any may be missing) and the flags th_forcing,
+        qv_forcing; None detaches.  The methods read and write fields with field / set_field, among them the held
+        tendencies MI_RU_TENDF, MI_RV_TENDF, MI_T_TENDF, MI_Q_TENDF + n in part2.  An exception raised in a method
+        is raised again when the step returns."""
+        self._phys = cp
+        if cp is None:
+            self._phys_struct = None
+            check(lib().mi_domain_set_column_physics(self._h, None))
+            return
+        self._hook_error = None
+
+        def thunk(name):
+            f = getattr(cp, name, None)
+            if f is None:
+                return A.PHYS_FN()
+
+            def call(_u, _d):
+                if self._hook_error is None:
+                    try:
+                        f(self)
+                    except BaseException as e:  # noqa: BLE001 - re-raised after the step
+                        self._hook_error = e
+            return A.PHYS_FN(call)
+        s = A.MiColumnPhysics(None, thunk("part1"), thunk("part2"), thunk("after_rk"), thunk("after_all_rk_steps"),
+                              1 if getattr(cp, "th_forcing", False) else 0, 1 if getattr(cp, "qv_forcing", False) else 0)
+        self._phys_struct = s
+        check(lib().mi_domain_set_column_physics(self._h, C.byref(s)))
+
+    def _raise_hook_error(self) -> None:
+        e = getattr(self, "_hook_error", None)
+        if e is not None:
+            self._hook_error = None
+            raise e
 
     def sync(self) -> None:
         check(lib().mi_domain_sync(self._h))
@@ -319,9 +356,11 @@ class RK3:
 
     def rk_prep(self, rk):
         check(lib().mi_rk_prep(self._h, rk))
+        self._raise_hook_error()
 
     def rk_tendency(self, rk):
         check(lib().mi_rk_tendency(self._h, rk))
+        self._raise_hook_error()
 
     def small_steps(self, rk):
         check(lib().mi_small_steps(self._h, rk))
@@ -334,6 +373,7 @@ class RK3:
 
     def step_end(self):
         check(lib().mi_step_end(self._h))
+        self._raise_hook_error()
 
     # specified lateral boundaries (reference LateralBC::set / set_scalar / dtbc)
     def bdy_set(self, var: int, side: int, values: np.ndarray, tends: np.ndarray) -> None:

@@ -128,6 +128,15 @@ enum {
   /* microphysics heating of the last step, theta_m K/s (input; zero unless set; used with moisture:
    * rk_addtend_dry and small_step_finish, src/dyn/large_step.cpp:557, small_step.cpp:437-440) */
   MI_H_DIABATIC = 23,
+  /* held tendencies of the first stage (reference Tendencies::ru_tendf, rv_tendf, rw_tendf, t_tendf, coupled;
+   * src/dyn/state.hpp:85-88): allocated when a diffusion, specified boundaries or a column physics plugin is
+   * attached.  At the first stage of every step they are zeroed, then take the plugin's part2 (set here),
+   * the sixth-order diffusion and the boundary relaxation, in that order (large_step.cpp:811-825) */
+  MI_RU_TENDF = 24, MI_RV_TENDF = 25, MI_RW_TENDF = 26, MI_T_TENDF = 27,
+  /* the advective forcings of a column physics plugin that asks for them (read only; WRF RTHFTEN, RQVFTEN:
+   * the advection tendency of theta and of the vapour times msfty, of the last stage;
+   * large_step.cpp:506, 543-555, scalars.cpp:147-150, 260-287) */
+  MI_TH_FORCING = 28, MI_QV_FORCING = 29,
   /* base state (write once, before the first step).  3-D fields take nzs values for a horizontally
    * uniform profile or nxs*nys*nzs values; 2-D fields take 1 value or nx*ny values. */
   MI_MUB = 32, MI_PB = 33, MI_ALB = 34, MI_PHB = 35, MI_T_INIT = 36,
@@ -248,6 +257,31 @@ typedef void (*mi_cfl_fn)(void* user, double vert, double horiz);
 typedef void (*mi_stage_fn)(void* user);
 int mi_domain_set_cfl_callback(mi_domain* d, mi_cfl_fn fn, void* user);
 int mi_domain_set_first_rk_prep_callback(mi_domain* d, mi_stage_fn fn, void* user);
+
+/* Column physics plugin (reference ColumnPhysics, src/dyn/column_physics.hpp:21-34, attached by
+ * RK3::set_column_physics): host functions called at WRF's points of the step (rk3.cpp:100-166,
+ * large_step.cpp:820-823), each with the stream synchronised, so that they may read and write fields with
+ * mi_domain_get_field / mi_domain_set_field:
+ *   part1              stage 1, after rk_step_prep and the first-stage hook, once the held tendencies
+ *                      (MI_*_TENDF and every MI_Q_TENDF) are zeroed: first_rk_step_part1
+ *   part2              stage 1, first thing of the tendencies: sets the physics tendencies into MI_RU_TENDF,
+ *                      MI_RV_TENDF, MI_T_TENDF, MI_Q_TENDF + n (update_phy_ten); the diffusion and the
+ *                      boundary relaxation are added after it
+ *   after_rk           after the third stage, before the specified boundaries' final values (phy_prep_part2)
+ *   after_all_rk_steps the end of the step (after_all_rk_steps)
+ * th_forcing / qv_forcing = 1 keep MI_TH_FORCING / MI_QV_FORCING (the latter needs n_moist > 0).  Any pointer
+ * may be NULL.  While a plugin is attached the held tendencies are used (as with a diffusion or specified
+ * boundaries), the caller's MI_Q_TENDF holds only for the step whose part2 sets it, and steps run launch by
+ * launch as with the callbacks above (mi_batch_step refuses the domain).  The per-phase entry points call
+ * part1 from mi_rk_prep(d, 1), part2 from mi_rk_tendency(d, 1) and the last two from mi_step_end.  cp == NULL
+ * detaches the plugin. */
+typedef void (*mi_phys_fn)(void* user, mi_domain* d);
+typedef struct mi_column_physics {
+  void* user;
+  mi_phys_fn part1, part2, after_rk, after_all_rk_steps;
+  int32_t th_forcing, qv_forcing;
+} mi_column_physics;
+int mi_domain_set_column_physics(mi_domain* d, const mi_column_physics* cp);
 
 /* Batches (ensembles, serving): the steps of several domains of one context advance together.  Each
  * step of the batch is one HIP graph holding every member's step as an independent branch, so the

@@ -234,6 +234,12 @@ struct Dom {
   F ru_tendf, rv_tendf, rw_tendf, t_tendf, ph_tendf;
   std::vector<F> qtw;  // q_tendf of the step: the caller's tendency plus the sixth-order diffusion
   bool mapped = false, cor = false, use_tendf = false, terrain = false;
+  // a column physics plugin (ColumnPhysics, column_physics.hpp:21-34; RK3::set_column_physics): its hooks and
+  // the advective forcings it asks for (RTHFTEN, RQVFTEN)
+  orc_column_physics phys{};
+  bool has_phys = false, held_base = false;
+  F thf, qvf;
+  bool want_thf = false, want_qvf = false;
   // specified lateral boundaries (LateralBC, lateral_bc.hpp:28-91): spec_zone 1, relax_zone 4
   static constexpr int kBw = 5, kSpecZone = 1, kRelaxZone = 4;
   bool spec = false;
@@ -296,6 +302,9 @@ struct Dom {
     flx = spec || ox;
     fly = (spec || g.bcy == MI_BC_OPEN) && !g.two_d();
     use_tendf = p.diff_6th_opt != 0 || spec;  // a DiffusionDriver or a LateralBC attached (large_step.cpp:828)
+    held_base = use_tendf;
+    f3(thf, St::M);
+    f3(qvf, St::M);
     terrain = p.terrain != 0;          // !b.ht.uniform() (small_step.cpp:583, rk3.cpp:163)
     if (spec) {
       set_fcx();
@@ -637,7 +646,13 @@ struct Dom {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (spec) std::fill(mu_tend.a.begin(), mu_tend.a.end(), 0.0);  // zero_tend of mu_tend, every stage (:815-818)
     if (rk == 1 && use_tendf) {  // zero_tendf, then DiffusionDriver::dynamics (diffusion_driver.cpp:266-281)
-      for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf, &ph_tendf}) std::fill(f->a.begin(), f->a.end(), 0.0);
+      if (!has_phys) {
+        for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf, &ph_tendf}) std::fill(f->a.begin(), f->a.end(), 0.0);
+      } else {  // zeroed before part1; part2 adds the physics tendencies (large_step.cpp:820-821)
+        if (phys.part2) phys.part2(phys.user);
+        for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf})
+          for (double& v : f->a) v = 0.0 + v;
+      }
       auto fu = [&](int i, int j, int k) { return u2(i, j, k); };
       auto fv = [&](int i, int j, int k) { return v2(i, j, k); };
       auto fw = [&](int i, int j, int k) { return w2(i, j, k); };
@@ -1013,6 +1028,18 @@ struct Dom {
           if (do_y) r -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
         }
         r -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+        if (want_thf) {  // set_tend(RTHFTEN): the advection times msfty; specified edges skip their edge flux
+          const bool scol = spec && (i == 0 || i == nx - 1), srow = spec && (j == 0 || j == ny - 1);
+          if (!scol && !srow) {
+            thf(i, j, k) = r * msfty(i, j);
+          } else {
+            double q = 0.0;
+            if (!scol) q = -(msftx(i, j) * rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+            if (do_y && !srow) q -= (msftx(i, j) * rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+            q -= c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+            thf(i, j, k) = q * msfty(i, j);
+          }
+        }
         if (use_tendf) r += t_tendf(i, j, k) / msfty(i, j);
         // rk_addtend_dry's coupled heating (large_step.cpp:557), with moisture
         if (moist()) r += (c.c1h[k] * mut(i, j) + c.c2h[k]) * hdia(i, j, k) / msfty(i, j);
@@ -1551,6 +1578,15 @@ struct Dom {
       return (d.flx && (i == 0 || i == d.g.nx - 1)) || (d.fly && (j == 0 || j == d.g.ny - 1));
     }
     bool open_col(int i) const { return d.ox && (i == 0 || i == d.g.nx - 1); }
+    // advect_scalar in a spec-zone cell, for RQVFTEN (scalars.cpp:119-125): x flux off in the outer columns, y
+    // off in the outer rows
+    double adv_spec(int i, int j, int k) const {
+      const bool scol = i == 0 || i == d.g.nx - 1, srow = j == 0 || j == d.g.ny - 1;
+      double r = 0.0;
+      if (!scol) r = -(d.msftx(i, j) * d.c.rdx) * (fx(i + 1, j, k) - fx(i, j, k));
+      if (!d.g.two_d() && !srow) r -= (d.msftx(i, j) * d.c.rdy) * (fy(i, j + 1, k) - fy(i, j, k));
+      return r - d.c.rdnw[k] * (fz(i, j, k + 1) - fz(i, j, k));
+    }
     bool open_row(int j) const { return d.oy && (j == 0 || j == d.g.ny - 1); }
     // the open terms, x then y (scalars.cpp:110-116); fo: q itself (advect_scalar) or q_1 (advect_scalar_pd)
     double open_terms(double r, int i, int j, int k, const F& fo) const {
@@ -1579,8 +1615,10 @@ struct Dom {
   void update_plain(int n, double dt_rk) {
     const double hup = p.h_adv_order == 5 ? 1.0 : 0.0, vup = p.v_adv_order == 3 ? 1.0 : 0.0;
     const Fl f{*this, q2[n], q1[n], dt_rk, hup, vup, false};
+    const bool ften = want_qvf && n == 0;
     pfor(0, g.nx, 0, g.ny, 0, g.nz, [&](int i, int j, int k) {
       double tend = f.adv(i, j, k) * msfty(i, j);
+      if (ften) qvf(i, j, k) = f.spec_cell(i, j) ? f.adv_spec(i, j, k) * msfty(i, j) : tend;  // scalars.cpp:150
       tend += (use_tendf ? qtw : qtf)[n](i, j, k);
       qnew[n](i, j, k) = (m_full(k, mub(i, j), mu1(i, j)) * q1[n](i, j, k) + dt_rk * tend) / m_full(k, mub(i, j), mu2(i, j));
     });
@@ -1650,8 +1688,17 @@ struct Dom {
       const double dd = dhz(i, j, k);
       return dd * (dd > 0 ? sc_scale(i, j, k) : (dd < 0 ? sc_scale(i, j, k - 1) : 1.0));
     };
+    const bool ften = want_qvf && n == 0;
     pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {  // KUpdatePD
       if (f.spec_cell(i, j)) {
+        if (ften) {  // WRF still forms the tendency here, without the edge's own flux (scalars.cpp:262-269)
+          const bool scol = i == 0 || i == nx - 1, srow = j == 0 || j == ny - 1;
+          double t = -c.rdnw[k] * (ddz(i, j, k + 1) - ddz(i, j, k) + loz(i, j, k + 1) - loz(i, j, k));
+          if (!scol) t -= msftx(i, j) * (c.rdx * (ddx(i + 1, j, k) - ddx(i, j, k) + lox(i + 1, j, k) - lox(i, j, k)));
+          if (do_y && !srow)
+            t -= msftx(i, j) * (c.rdy * (ddy(i, j + 1, k) - ddy(i, j, k) + loy(i, j + 1, k) - loy(i, j, k)));
+          qvf(i, j, k) = t * msfty(i, j);
+        }
         qnew[n](i, j, k) = m_full(k, mub(i, j), mu1(i, j)) * qo(i, j, k) / m_full(k, mub(i, j), mu2(i, j));
         return;
       }
@@ -1668,6 +1715,7 @@ struct Dom {
         tend -= msftx(i, j) * (c.rdx * (ddx(i + 1, j, k) - ddx(i, j, k) + lox(i + 1, j, k) - lox(i, j, k)));
         if (do_y) tend -= msftx(i, j) * (c.rdy * (ddy(i, j + 1, k) - ddy(i, j, k) + loy(i, j + 1, k) - loy(i, j, k)));
       }
+      if (ften) qvf(i, j, k) = tend * msfty(i, j);
       qnew[n](i, j, k) = (m_full(k, mub(i, j), mu1(i, j)) * qo(i, j, k) + dt_rk * (tend * msfty(i, j))) /
                          m_full(k, mub(i, j), mu2(i, j));
     });
@@ -1781,7 +1829,19 @@ struct Dom {
   // the end of RK3::step (rk3.cpp:159-167): spec_bdy_final, calc_p_rho_phi on the interior, set_w_surface.
   // With scalar pow the interior recomputation gives back the same bits (its inputs are unchanged
   // there), so it runs only with the libmvec lanes, whose split it changes.
+  // first_rk_step_part1 (rk3.cpp:106-111): init_zero_tendency of every held tendency, then part1
+  void physics_part1() {
+    if (!has_phys) return;
+    for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf, &ph_tendf}) std::fill(f->a.begin(), f->a.end(), 0.0);
+    for (F& f : qtf) std::fill(f.a.begin(), f.a.end(), 0.0);
+    if (phys.part1) phys.part1(phys.user);
+  }
   void step_end() {
+    if (has_phys && phys.after_rk) phys.after_rk(phys.user);  // phy_prep_part2 (rk3.cpp:150)
+    step_end_dyn();
+    if (has_phys && phys.after_all_rk_steps) phys.after_all_rk_steps(phys.user);  // rk3.cpp:166
+  }
+  void step_end_dyn() {
     if (spec) {
       spec_final();
       if (libmvec) {
@@ -1792,10 +1852,11 @@ struct Dom {
     }
     if (terrain) set_w_surface();
   }
-  void step() {  // RK3::step (rk3.cpp:86-170), no physics
+  void step() {  // RK3::step (rk3.cpp:86-170), no microphysics
     step_begin();
     for (int rk = 1; rk <= 3; ++rk) {
       rk_prep();
+      if (rk == 1) physics_part1();
       rk_tendency(rk);
       small_steps(rk);
       scalars(rk);
@@ -1819,6 +1880,12 @@ struct Dom {
       case MI_HT: return &ht;
       case MI_F: return &fcor; case MI_E: return &ecor; case MI_SINA: return &sina; case MI_COSA: return &cosa;
       case MI_CQU: return &cqu; case MI_CQV: return &cqv; case MI_CQW: return &cqw; case MI_H_DIABATIC: return &hdia;
+      case MI_RU_TENDF: return use_tendf ? &ru_tendf : nullptr;
+      case MI_RV_TENDF: return use_tendf ? &rv_tendf : nullptr;
+      case MI_RW_TENDF: return use_tendf ? &rw_tendf : nullptr;
+      case MI_T_TENDF: return use_tendf ? &t_tendf : nullptr;
+      case MI_TH_FORCING: return want_thf ? &thf : nullptr;
+      case MI_QV_FORCING: return want_qvf ? &qvf : nullptr;
       default:
         if (id >= MI_Q && id < MI_Q + n_sc) return &q2[size_t(id - MI_Q)];
         if (id >= MI_Q_TENDF && id < MI_Q_TENDF + n_sc) return &qtf[size_t(id - MI_Q_TENDF)];
@@ -1936,8 +2003,20 @@ int orc_step_begin(orc_domain* d) {
   d->d->step_begin();
   return MI_OK;
 }
-int orc_rk_prep(orc_domain* d, int) {
+int orc_rk_prep(orc_domain* d, int rk) {
+  if (rk < 1 || rk > 3) return fail("rk must be 1, 2 or 3");
   d->d->rk_prep();
+  if (rk == 1) d->d->physics_part1();
+  return MI_OK;
+}
+int orc_set_column_physics(orc_domain* od, const orc_column_physics* cp) {
+  Dom& d = *od->d;
+  if (cp && cp->qv_forcing && d.n_moist == 0) return fail("qv_forcing needs moisture (n_moist > 0)");
+  d.has_phys = cp != nullptr;
+  d.phys = cp ? *cp : orc_column_physics{};
+  d.use_tendf = d.held_base || d.has_phys;
+  d.want_thf = cp && cp->th_forcing;
+  d.want_qvf = cp && cp->qv_forcing;
   return MI_OK;
 }
 int orc_rk_tendency(orc_domain* d, int rk) {

@@ -40,7 +40,16 @@ int orc_scalar_transport(orc_domain* d, int rk); /* ScalarTransport::run_stage *
 int orc_cfl_max(orc_domain* d, double* vert, double* horiz); /* LargeStep::cfl_max */
 int orc_set_dt(orc_domain* d, double dt, int n_sound);        /* RK3::set_dt + LateralBC::set_dt (rk3.cpp:79-84) */
 int orc_calc_p_rho_phi(orc_domain* d, int rk);
-int orc_step_end(orc_domain* d); /* spec_bdy_final, set_w_surface */
+int orc_step_end(orc_domain* d); /* after_rk, spec_bdy_final, set_w_surface, after_all_rk_steps */
+/* a column physics plugin (reference ColumnPhysics, src/dyn/column_physics.hpp:21-34): the hooks of
+ * mi_domain_set_column_physics with the oracle's own user pointer; NULL detaches */
+typedef void (*orc_phys_fn)(void* user);
+typedef struct orc_column_physics {
+  void* user;
+  orc_phys_fn part1, part2, after_rk, after_all_rk_steps;
+  int32_t th_forcing, qv_forcing;
+} orc_column_physics;
+int orc_set_column_physics(orc_domain* d, const orc_column_physics* cp);
 /* specified boundaries: as mi_bdy_set / mi_bdy_set_time / mi_bdy_get_time */
 int orc_bdy_set(orc_domain* d, int var, int side, const double* values, const double* tends, int64_t n);
 int orc_bdy_set_time(orc_domain* d, double dtbc);

@@ -7,8 +7,23 @@
 
 #include "../../cppwrf_amd/csrc/dyn_facade.hpp"
 
+// a column physics plugin as a reference suite would derive it (compiled here; run by the GPU tests' mirror)
+struct Heating : cppwrf_amd::dyn::ColumnPhysics {
+  int calls = 0;
+  void part1(cppwrf_amd::dyn::RK3&) override { ++calls; }
+  void part2(cppwrf_amd::dyn::RK3& rk) override {
+    auto st = rk.state();
+    std::vector<double> t(st.size(MI_T_TENDF), 1e-3);
+    st.set(MI_T_TENDF, t);
+  }
+  void after_rk(cppwrf_amd::dyn::RK3&) override { ++calls; }
+  bool th_forcing() override { return true; }
+};
+
 int main() {
   namespace dyn = cppwrf_amd::dyn;
+  Heating heating;
+  if (heating.th_forcing() != true || heating.qv_forcing() != false) return 13;
   // reference RK3::auto_sound_steps: max(2*(int(300*dt/min(dx,dy) - 0.01) + 1), 4)
   if (dyn::RK3::auto_sound_steps(6.0, 1000.0, 1000.0) != 4) return 2;
   if (dyn::RK3::auto_sound_steps(90.0, 15000.0, 15000.0) != 4) return 3;
@@ -23,6 +38,9 @@ int main() {
   if (dyn::field_size(MI_MSFUX, g3) != 17 * 12 || dyn::field_size(MI_MSFVY, g3) != 16 * 13) return 9;
   if (dyn::field_size(MI_Q + 3, g3) != 16 * 12 * 10 || dyn::field_size(MI_CQW, g3) != 16 * 12 * 11) return 10;
   if (dyn::field_size(MI_V, g2) != 16 * 10 || dyn::field_size(MI_MSFVX, g2) != 16) return 11;
+  if (dyn::field_size(MI_RU_TENDF, g3) != 17 * 12 * 10 || dyn::field_size(MI_RW_TENDF, g3) != 16 * 12 * 11 ||
+      dyn::field_size(MI_TH_FORCING, g3) != 16 * 12 * 10)
+    return 14;
   try {
     dyn::Context ctx(0);
     // a GPU is present: a domain with an unsupported advection order must be refused

@@ -43,6 +43,7 @@ def olib():
             "orc_set_dt": (I, [VP, C.c_double, I]),
             "orc_calc_p_rho_phi": (I, [VP, I]),
             "orc_step_end": (I, [VP]),
+            "orc_set_column_physics": (I, [VP, C.POINTER(OrcColumnPhysics)]),
             "orc_bdy_set": (I, [VP, I, I, PD, PD, C.c_int64]),
             "orc_bdy_set_time": (I, [VP, C.c_double]),
             "orc_bdy_get_time": (I, [VP, PD]),
@@ -52,6 +53,14 @@ def olib():
             f.restype, f.argtypes = r, a
         _o = L
     return _o
+
+
+ORC_PHYS_FN = C.CFUNCTYPE(None, C.c_void_p)
+
+
+class OrcColumnPhysics(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("part1", ORC_PHYS_FN), ("part2", ORC_PHYS_FN), ("after_rk", ORC_PHYS_FN),
+                ("after_all_rk_steps", ORC_PHYS_FN), ("th_forcing", C.c_int32), ("qv_forcing", C.c_int32)]
 
 
 def _ok(rc):
@@ -140,6 +149,21 @@ class OracleDomain:
 
     def step_end(self):
         _ok(olib().orc_step_end(self._h))
+
+    def set_column_physics(self, cp):
+        """as RK3.set_column_physics: cp's methods get this domain"""
+        if cp is None:
+            self._phys = None
+            _ok(olib().orc_set_column_physics(self._h, None))
+            return
+
+        def thunk(name):
+            f = getattr(cp, name, None)
+            return ORC_PHYS_FN(lambda _u: f(self)) if f is not None else ORC_PHYS_FN()
+        self._phys = OrcColumnPhysics(None, thunk("part1"), thunk("part2"), thunk("after_rk"),
+                                      thunk("after_all_rk_steps"), 1 if getattr(cp, "th_forcing", False) else 0,
+                                      1 if getattr(cp, "qv_forcing", False) else 0)
+        _ok(olib().orc_set_column_physics(self._h, C.byref(self._phys)))
 
     def bdy_set(self, var, side, values, tends):
         v = np.ascontiguousarray(values, dtype=np.float64)
