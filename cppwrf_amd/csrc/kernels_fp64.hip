@@ -491,6 +491,7 @@ __device__ __forceinline__ double open_y_scalar(const Geo& g, const T* q, const 
   return rdy * (vb * (Q(qo, ny - 1) - Q(qo, ny - 2)) + Q(q, ny - 1) * (RV(ny) - RV(ny - 1)));
 }
 
+template <bool OPEN>
 MI_POINT_KERNEL k_tend_u(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -540,7 +541,7 @@ MI_POINT_KERNEL k_tend_u(Range r, Args a) {
   auto RU = [&](int ii, int jj) { return double(f.ru[g.at(ii, jj, k)]); };
   auto RV = [&](int ii, int jj) { return double(f.rv[g.at(ii, jj, k)]); };
   auto UU = [&](int ii, int jj) { return double(u[g.at(ii, jj, k)]); };
-  if (g.ox() && (i == 0 || i == g.nx)) {
+  if ((OPEN && g.ox()) && (i == 0 || i == g.nx)) {
     // open x boundary face (large_step.cpp:211-228): the radiation condition with phase speed cb and, unless
     // open_ys/ye, the vertical advection; no horizontal flux divergence, pressure gradient, Coriolis, curvature
     double t;
@@ -553,14 +554,14 @@ MI_POINT_KERNEL k_tend_u(Range r, Args a) {
       ub = ub > 0.0 ? ub : 0.0;
       t = 0.0 - c.rdx * ub * (UU(i, j) - UU(i - 1, j));
     }
-    if (!g.oy()) t -= c.rdnw[k] * (fz(k + 1) - fz(k));
+    if (!(OPEN && g.oy())) t -= c.rdnw[k] * (fz(k + 1) - fz(k));
     if (a.q.use_tendf) t += f.ru_tendf[x] / msfuy;  // rk_addtend_dry
     f.ru_tend[x] = t;
     return;
   }
   double t = -(msfux * c.rdx) * (fx(x + 1, i + 1) - fx(x, i));
   if (!g.two_d) {
-    if (!(g.oy() && (j == 0 || j == g.ny - 1))) {
+    if (!((OPEN && g.oy()) && (j == 0 || j == g.ny - 1))) {
       t -= (msfux * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
     } else {  // advect_u open_ys / open_ye: v-flux terms of the outermost rows (large_step.cpp:193-207)
       const int imax = g.nx - 1, ip = i < imax ? i : imax, im = i - 1 > 0 ? i - 1 : 0;
@@ -581,7 +582,7 @@ MI_POINT_KERNEL k_tend_u(Range r, Args a) {
       t -= ot;
     }
   }
-  if (!(g.oy() && spx && (i == 0 || i == g.nx))) t -= c.rdnw[k] * (fz(k + 1) - fz(k));
+  if (!((OPEN && g.oy()) && spx && (i == 0 || i == g.nx))) t -= c.rdnw[k] * (fz(k + 1) - fz(k));
   const double rm = msfux / msfuy;
   double dpx = rm * H * c.rdx * (c.c1h[k] * f.muu[x2] + c.c2h[k]) *
                ((f.ph2[x + sk] - f.ph2[x + sk - 1] + f.ph2[x] - f.ph2[x - 1]) + (f.alt[x] + f.alt[x - 1]) * (f.p[x] - f.p[x - 1]) +
@@ -604,6 +605,7 @@ MI_POINT_KERNEL k_tend_u(Range r, Args a) {
 }
 
 // KTendV (large_step.cpp:267-376)
+template <bool OPEN>
 MI_POINT_KERNEL k_tend_v(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -653,7 +655,7 @@ MI_POINT_KERNEL k_tend_v(Range r, Args a) {
   auto RU = [&](int ii, int jj) { return double(f.ru[g.at(ii, jj, k)]); };
   auto RV = [&](int ii, int jj) { return double(f.rv[g.at(ii, jj, k)]); };
   auto VV = [&](int ii, int jj) { return double(v[g.at(ii, jj, k)]); };
-  if (g.oy() && (j == 0 || j == g.ny)) {  // open y boundary face: radiation only (large_step.cpp:327-340)
+  if ((OPEN && g.oy()) && (j == 0 || j == g.ny)) {  // open y boundary face: radiation only (large_step.cpp:327-340)
     double t;
     if (j == 0) {
       double vb = RV(i, 0) - kCb * (c.c1h[k] * f.mut[g.at2(i, 0)] + c.c2h[k]);
@@ -670,7 +672,7 @@ MI_POINT_KERNEL k_tend_v(Range r, Args a) {
   }
   const double rm = msfvy / msfvx;
   double t;
-  if (g.ox() && (i == 0 || i == g.nx - 1)) {
+  if ((OPEN && g.ox()) && (i == 0 || i == g.nx - 1)) {
     // open x: the y flux, then advect_v's open_xs / open_xe term instead of the x flux (large_step.cpp:307-323)
     t = -(msfvy * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
     const int jmax = g.ny - 1, jp = j < jmax ? j : jmax, jm = j - 1 > 0 ? j - 1 : 0;
@@ -716,6 +718,7 @@ MI_POINT_KERNEL k_tend_v(Range r, Args a) {
 }
 
 // KTendW (large_step.cpp:379-499), w levels 1..nz
+template <bool OPEN>
 MI_POINT_KERNEL k_tend_w(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -757,7 +760,7 @@ MI_POINT_KERNEL k_tend_w(Range r, Args a) {
   };
   const double msftx = f.msftx[x2], msfty = f.msfty[x2];
   double t;
-  const bool ocol = g.ox() && (i == 0 || i == g.nx - 1), orow = g.oy() && (j == 0 || j == g.ny - 1);
+  const bool ocol = (OPEN && g.ox()) && (i == 0 || i == g.nx - 1), orow = (OPEN && g.oy()) && (j == 0 || j == g.ny - 1);
   if (ocol || orow) {
     // advect_w's open terms (large_step.cpp:416-466), WRF order: y flux, x flux, open x terms, open y terms;
     // ru (rv) at w level k from the two u (v) faces of the edge column (row), extrapolated at the lid
@@ -842,6 +845,7 @@ MI_POINT_KERNEL k_tend_w(Range r, Args a) {
 }
 
 // KTendT (large_step.cpp:502-565)
+template <bool OPEN, bool THF>
 MI_POINT_KERNEL k_tend_t(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -876,7 +880,7 @@ MI_POINT_KERNEL k_tend_t(Range r, Args a) {
   };
   const double msftx = f.msftx[x2];
   double rr;
-  const bool ocol = g.ox() && (i == 0 || i == g.nx - 1), orow = g.oy() && (j == 0 || j == g.ny - 1);
+  const bool ocol = (OPEN && g.ox()) && (i == 0 || i == g.nx - 1), orow = (OPEN && g.oy()) && (j == 0 || j == g.ny - 1);
   if (ocol || orow) {  // open boundary: WRF order y flux, x flux, open x terms, open y terms (large_step.cpp:534-541)
     rr = 0.0;
     if (!g.two_d && !orow) rr = rr - (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
@@ -888,7 +892,7 @@ MI_POINT_KERNEL k_tend_t(Range r, Args a) {
     if (!g.two_d) rr -= (msftx * c.rdy) * (fy(x + sj, j + 1) - fy(x, j));
   }
   rr -= c.rdnw[k] * (fz(k + 1) - fz(k));
-  if (f.thf) {  // set_tend(RTHFTEN) (large_step.cpp:543-555): the advection times msfty; specified edges skip their flux
+  if (THF && f.thf) {  // set_tend(RTHFTEN) (large_step.cpp:543-555): the advection times msfty; specified edges skip their flux
     const bool scol = g.spec() && (i == 0 || i == g.nx - 1), srow = g.spec() && (j == 0 || j == g.ny - 1);
     if (!scol && !srow) {
       f.thf[x] = rr * f.msfty[x2];
@@ -907,6 +911,7 @@ MI_POINT_KERNEL k_tend_t(Range r, Args a) {
 }
 
 // KTendPh rhs_ph (large_step.cpp:570-668), w levels 1..nz, 6th-order horizontal gradient
+template <bool OPEN>
 MI_POINT_KERNEL k_tend_ph(Range r, Args a) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -945,7 +950,7 @@ MI_POINT_KERNEL k_tend_ph(Range r, Args a) {
   auto dfwdy = [&](idx_t y, int bj) { return f.phb(i, bj + 1, k) - f.phb(i, bj, k) + ph[y + sj] - ph[y]; };
   // rhs_ph next to specified or open edges (large_step.cpp:596-605): none on the outer row, 2nd order on the
   // next, then 4th order in y (in x only with open_xs/xe), 6th from the fourth row in
-  const bool sp = g.spec(), ox = g.ox();
+  const bool sp = g.spec(), ox = (OPEN && g.ox());
   auto horder = [&](int ii, int n, bool on, bool is_x) {
     if (!on) return 6;
     if (ii <= 0 || ii >= n - 1) return 0;
@@ -985,7 +990,7 @@ MI_POINT_KERNEL k_tend_ph(Range r, Args a) {
     t = t - cx * ((mup * up * f.msfux[x2 + 1] + mum * um * f.msfux[x2]) * r12 * grad4x());
   // open boundaries (large_step.cpp:636-661), south/north then west/east: inflow advection of ph by the normal
   // wind at the w level (the west lid level takes none: WRF's kz = kde there reads fnm/fnp = 0)
-  if (g.oy() && (j == 0 || j == g.ny - 1)) {
+  if ((OPEN && g.oy()) && (j == 0 || j == g.ny - 1)) {
     const int kz = k < nz - 1 ? k : nz - 1;
     const int ea = j == 0 ? 0 : g.ny - 1, eb = j == 0 ? 1 : g.ny;
     auto V = [&](int jj, int kk) { return double(v[g.at(i, jj, kk)]); };
@@ -1159,6 +1164,7 @@ MI_COLUMN_KERNEL k_ss_column(Args a, double cof) {
 
 // ----------------------------------------------------------------------------- advance_uv
 // KUV (small_step.cpp:151-190)
+template <bool OPEN>
 MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -1189,14 +1195,14 @@ MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
   const bool uface = sp && (i <= 0 || i >= nx || j == 0 || j == ny - 1);
   const bool vface = sp && (i == 0 || i == nx - 1 || j <= 0 || j >= ny);
   const int ui0 = g.bcx != MI_BC_PERIODIC ? 1 : 0;
-  if ((uface || (g.ox() && (i == 0 || i == nx))) && j < ny) {
+  if ((uface || ((OPEN && g.ox()) && (i == 0 || i == nx))) && j < ny) {
     put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x]);
   } else if (!uface && i >= ui0 && i < nx && j < ny) {
     const double mudf_x = -a.q.emdx * (f.mudf[x2] - f.mudf[x2 - 1]) / f.msfuy[x2];
     put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x] - dts * (a.q.moist ? f.cqu[x] : 1.0) * dp(1, 1, 0, f.muu[x2], c.rdx, f.msfux[x2] / f.msfuy[x2]) + c.c1h[k] * mudf_x);
   }
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
-  if ((vface || (g.oy() && (j == 0 || j == ny))) && i < nx) {
+  if ((vface || ((OPEN && g.oy()) && (j == 0 || j == ny))) && i < nx) {
     put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x]);
   } else if (!vface && !g.two_d && j >= vj0 && j < ny && i < nx) {
     const idx_t sj = g.sj;
@@ -1355,6 +1361,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   // ---------------------------------------------------------------- 2: the column mass
   if (on && k == 0) {
     double dmdt = 0;
+#pragma unroll 8
     for (int kk = 0; kk < nz; ++kk) dmdt += LDS(DV, kk, ci);
     const double mu_new = mu_old + dts * (dmdt + mtend);
     const double muts = mut + mu_new;
@@ -1380,6 +1387,7 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   // ---------------------------------------------------------------- 4: the running integral
   if (on && k == 0) {
     double ww_full = 0;
+#pragma unroll 8
     for (int kk = 0; kk + 1 < nz; ++kk) {
       ww_full = ww_full - LDS(TK, kk, ci);
       LDS(TK, kk, ci) = ww_full;  // integral up to level kk + 1
@@ -1491,11 +1499,13 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   // ---------------------------------------------------------------- 7: tridiagonal sweeps
   if (on && k == 0) {
     double wprev = LDS(W, 0, ci);
+#pragma unroll 8
     for (int kk = 1; kk <= nz; ++kk) {
       wprev = (LDS(W, kk, ci) - LDS(AA, kk, ci) * wprev) * LDS(AL, kk, ci);
       LDS(W, kk, ci) = wprev;
     }
     double wnext = LDS(W, nz, ci);
+#pragma unroll 8
     for (int kk = nz - 1; kk >= 1; --kk) {
       wnext = LDS(W, kk, ci) - LDS(GA, kk, ci) * wnext;
       LDS(W, kk, ci) = wnext;
@@ -2544,12 +2554,16 @@ void launch_tendencies(const Args& a, hipStream_t st) {
   const int ui0 = g.bcx != MI_BC_PERIODIC ? 1 : 0;
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
   // open boundaries: the boundary faces are advanced too (large_step.cpp:804-806)
-  launch3("tend_u", k_tend_u, Range{g.ox() ? 0 : ui0, g.ox() ? g.nx + 1 : g.nx, 0, g.ny, 0, g.nz}, st, a);
+  // the open-boundary code and the RTHFTEN store compiled in only where used
+  const bool op = g.ox() || g.oy();
+  launch3("tend_u", op ? k_tend_u<true> : k_tend_u<false>, Range{g.ox() ? 0 : ui0, g.ox() ? g.nx + 1 : g.nx, 0, g.ny, 0, g.nz}, st, a);
   if (!g.two_d)
-    launch3("tend_v", k_tend_v, Range{0, g.nx, g.oy() ? 0 : vj0, g.oy() ? g.ny + 1 : g.ny, 0, g.nz}, st, a);
-  launch3("tend_w", k_tend_w, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
-  launch3("tend_t", k_tend_t, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
-  launch3("tend_ph", k_tend_ph, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
+    launch3("tend_v", op ? k_tend_v<true> : k_tend_v<false>, Range{0, g.nx, g.oy() ? 0 : vj0, g.oy() ? g.ny + 1 : g.ny, 0, g.nz}, st, a);
+  launch3("tend_w", op ? k_tend_w<true> : k_tend_w<false>, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
+  launch3("tend_t", op ? (a.f.thf ? k_tend_t<true, true> : k_tend_t<true, false>)
+                       : (a.f.thf ? k_tend_t<false, true> : k_tend_t<false, false>),
+          Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
+  launch3("tend_ph", op ? k_tend_ph<true> : k_tend_ph<false>, Range{0, g.nx, 0, g.ny, 1, g.nz + 1}, st, a);
 }
 
 void launch_ss_prep(const Args& a, bool first, hipStream_t st) {
@@ -2563,7 +2577,7 @@ void launch_uv(const Args& a, double dts, hipStream_t st) {
   const Geo& g = a.g;
   // the east (north) faces of the spec zone or of an open direction
   const int ex = (g.spec() || g.ox()) ? 1 : 0, ey = (g.spec() || g.oy()) ? 1 : 0;
-  launch3("uv", k_uv, Range{0, g.nx + ex, 0, g.ny + ey, 0, g.nz}, st, a, dts);
+  launch3("uv", (g.ox() || g.oy()) ? k_uv<true> : k_uv<false>, Range{0, g.nx + ex, 0, g.ny + ey, 0, g.nz}, st, a, dts);
 }
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
   dispatch_cfg<AcousticL>(cfg, "acoustic", 10, 5, a.g.nx, a.g.ny, st, a, dts, it, ns);
