@@ -152,6 +152,12 @@ _I64 = C.c_int64
 # mi_cfl_fn / mi_stage_fn
 CFL_FN = C.CFUNCTYPE(None, _VP, C.c_double, C.c_double)
 STAGE_FN = C.CFUNCTYPE(None, _VP)
+class MiFieldView(C.Structure):
+    """mi_field_view"""
+    _fields_ = [("data", _VP), ("elem_bytes", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("stride_k", C.c_int64), ("stride_j", C.c_int64)]
+
+
 # mi_phys_fn (user, domain)
 PHYS_FN = C.CFUNCTYPE(None, _VP, _VP)
 
@@ -199,6 +205,7 @@ SIGNATURES = {
     "mi_domain_set_cfl_callback": (_I, [_VP, CFL_FN, _VP]),
     "mi_domain_set_first_rk_prep_callback": (_I, [_VP, STAGE_FN, _VP]),
     "mi_domain_set_column_physics": (_I, [_VP, C.POINTER(MiColumnPhysics)]),
+    "mi_domain_field_view": (_I, [_VP, _I, C.POINTER(MiFieldView)]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
     "mi_domain_autotune": (_I, [_VP, _I]),
     "mi_domain_get_tuning": (_I, [_VP, C.POINTER(_I), _I]),

@@ -181,6 +181,13 @@ int mi_domain_set_field(mi_domain* d, int id, const double* host, int64_t n) {
   });
 }
 
+int mi_domain_field_view(mi_domain* d, int id, mi_field_view* out) {
+  return mi_detail::guard([&] {
+    if (!out) throw Error(MI_EINVAL, "null argument");
+    checked(d)->field_view(id, out);
+  });
+}
+
 int mi_domain_get_field(mi_domain* d, int id, double* host, int64_t n) {
   return mi_detail::guard([&] {
     if (!host) throw Error(MI_EINVAL, "null argument");

@@ -650,6 +650,7 @@ struct Domain final : mi_domain {
   int64_t device_bytes() const override { return bytes; }
   void set_field(int id, const double* host, int64_t n) override;
   void get_field(int id, double* host, int64_t n) override;
+  void field_view(int id, mi_field_view* out) override;
   void phase(MiPhase which, int rk) override;
   void set_tuning(const int* shapes, int n) override;
   void get_tuning(int* shapes, int n) const override;
@@ -986,6 +987,19 @@ void Domain::set_field(int id, const double* host, int64_t n) {
   copy_in(base, h, fd->esize, st);
   halo({{*fd->slot, -1, -1}});
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+void Domain::field_view(int id, mi_field_view* out) {
+  FieldDesc* fd = find(this, id);
+  if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " has no view");
+  const bool two = fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2;
+  out->data = *fd->slot;
+  out->elem_bytes = fd->esize;
+  out->nx = nxs(fd->st);
+  out->ny = nys(fd->st);
+  out->nz = two ? 1 : nzs(fd->st);
+  out->stride_k = two ? 0 : a.g.sk;
+  out->stride_j = a.g.sj;
 }
 
 void Domain::get_field(int id, double* host, int64_t n) {

@@ -32,6 +32,7 @@ struct mi_domain {
   virtual int64_t device_bytes() const = 0;
   virtual void set_field(int id, const double* host, int64_t n) = 0;
   virtual void get_field(int id, double* host, int64_t n) = 0;
+  virtual void field_view(int id, mi_field_view* out) = 0;
   virtual void step(int nsteps) = 0;
   virtual void phase(MiPhase which, int rk) = 0;
   virtual void autotune(int trials) = 0;

@@ -191,6 +191,12 @@ class RK3 {
     check(mi_domain_set_first_rk_prep_callback(d_, on_first_rk_prep ? &RK3::prep_thunk : nullptr, this));
     check(mi_step(d_, nsteps));
   }
+  // reference Field3::view (field.hpp:16-44): device pointer and strides of a field, for device-side plugins
+  mi_field_view view(int field) {
+    mi_field_view v{};
+    check(mi_domain_field_view(d_, field, &v));
+    return v;
+  }
   // reference RK3::set_column_physics: the plugin is not owned (rk3.hpp:53-62); nullptr detaches
   void set_column_physics(ColumnPhysics* cp) {
     cp_ = cp;

@@ -227,6 +227,12 @@ class RK3:
         check(lib().mi_step(self._h, n))
         self._raise_hook_error()
 
+    def field_view(self, fid: int) -> "A.MiFieldView":
+        """the device address, element size and strides of a field (reference Field3::view)"""
+        v = A.MiFieldView()
+        check(lib().mi_domain_field_view(self._h, fid, C.byref(v)))
+        return v
+
     def set_column_physics(self, cp) -> None:
         """RK3::set_column_physics (reference src/dyn/column_physics.hpp:21-34): `cp` has the methods part1(rk3),
         part2(rk3), after_rk(rk3), after_all_rk_steps(rk3) (any may be missing) and the flags th_forcing,

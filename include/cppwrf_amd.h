@@ -183,6 +183,22 @@ int mi_domain_set_field(mi_domain* d, int field, const double* host, int64_t n);
 /* Copy a field's logical points to the host (stream-ordered: waits for the domain's queued work). */
 int mi_domain_get_field(mi_domain* d, int field, double* host, int64_t n);
 
+/* Zero-copy device access for device-side plugins (reference Field3::view(), src/core/field.hpp:16-44: a
+ * pointer and int strides): the device address of point (0,0,0) of a field, the element size (8: double; 4:
+ * float, the FP32-bulk storage of the 3-D bulk fields) and the strides in elements: i contiguous, then k
+ * (stride_k), then j (stride_j; 0 for an x-z slice).  The 3 halo points on each horizontal side are
+ * addressable; a 2-D field has nz = 1 and stride_k = 0.  Work on the view goes on the context's stream (or
+ * synchronises with it).  A scalar's view (MI_Q + n) names the buffer of the moment: the two buffers of a
+ * scalar swap at every stage, so take it again after any step or stage.  Base fields stored as profiles (flat
+ * terrain) have no view (MI_EINVAL). */
+typedef struct mi_field_view {
+  void* data;
+  int32_t elem_bytes;
+  int32_t nx, ny, nz; /* logical points of the field's staggering */
+  int64_t stride_k, stride_j;
+} mi_field_view;
+int mi_domain_field_view(mi_domain* d, int field, mi_field_view* out);
+
 /* reference RK3::step (src/dyn/rk3.cpp:86-170): advance nsteps large steps.  Asynchronous on the
  * context's stream; mi_domain_sync waits. */
 int mi_step(mi_domain* d, int nsteps);
