@@ -1361,7 +1361,6 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   // ---------------------------------------------------------------- 2: the column mass
   if (on && k == 0) {
     double dmdt = 0;
-#pragma unroll 8
     for (int kk = 0; kk < nz; ++kk) dmdt += LDS(DV, kk, ci);
     const double mu_new = mu_old + dts * (dmdt + mtend);
     const double muts = mut + mu_new;
@@ -1387,7 +1386,6 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   // ---------------------------------------------------------------- 4: the running integral
   if (on && k == 0) {
     double ww_full = 0;
-#pragma unroll 8
     for (int kk = 0; kk + 1 < nz; ++kk) {
       ww_full = ww_full - LDS(TK, kk, ci);
       LDS(TK, kk, ci) = ww_full;  // integral up to level kk + 1
@@ -1417,16 +1415,6 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
       if (!g.two_d) {
         const double sv = (first ? 0.0 : rvm) + v0;
         put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
-      }
-      // KSumEdge (small_step.cpp:279-290, 560-569): an open direction's east (north) domain face is a real face,
-      // summed by the column next to it (with specified boundaries k_acoustic_ring does it)
-      if (g.ox() && i == g.nx - 1) {
-        const double se = (first ? 0.0 : double(f.ru_m[x + 1])) + u1;
-        put<kHU>(f.ru_m, g, i + 1, j, k * sk, last ? se / rn + (c.c1h[k] * muu1 + c.c2h[k]) * us1 / msfuy1 : se);
-      }
-      if (g.oy() && j == g.ny - 1) {
-        const double sn = (first ? 0.0 : double(f.rv_m[x + sj])) + v1;
-        put<kHV>(f.rv_m, g, i, j + 1, k * sk, last ? sn / rn + (c.c1h[k] * muv1 + c.c2h[k]) * vs1 * mvi1 : sn);
       }
       double hflux = H * c.rdx * (double(u1) * (double(tsxp) + ts) - double(u0) * (double(ts) + tsxm));
       if (!g.two_d) hflux += H * c.rdy * (double(v1) * (double(tsyp) + ts) - double(v0) * (double(ts) + tsym));
@@ -1499,13 +1487,11 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   // ---------------------------------------------------------------- 7: tridiagonal sweeps
   if (on && k == 0) {
     double wprev = LDS(W, 0, ci);
-#pragma unroll 8
     for (int kk = 1; kk <= nz; ++kk) {
       wprev = (LDS(W, kk, ci) - LDS(AA, kk, ci) * wprev) * LDS(AL, kk, ci);
       LDS(W, kk, ci) = wprev;
     }
     double wnext = LDS(W, nz, ci);
-#pragma unroll 8
     for (int kk = nz - 1; kk >= 1; --kk) {
       wnext = LDS(W, kk, ci) - LDS(GA, kk, ci) * wnext;
       LDS(W, kk, ci) = wnext;
@@ -1547,6 +1533,30 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     pp = pp + a.q.smdiv * (pp - pm1);
     f.pm1[x] = ptmp;
     put<kHMass>(f.p, g, i, j, k * sk, pp);
+  }
+}
+
+// KSumEdge (small_step.cpp:279-290, 560-569) of an open direction, after each substep's column kernel: the east
+// (north) domain face is a real face, its time average summed as the faces of the columns (with specified
+// boundaries k_acoustic_ring does it).  ydir 0: faces i = nx of the last column; 1: faces j = ny of the last row.
+MI_POINT_KERNEL k_open_sum(Range r, Args a, int it, int ns, int ydir) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const bool first = it == 1, last = it == ns;
+  const double rn = double(ns);
+  const idx_t sk = g.sk;
+  if (ydir == 0) {
+    const idx_t y = g.at(i + 1, j, k), y2 = g.at2(i + 1, j);
+    const real u1 = f.u2[y], us1 = f.u_save[y];
+    const double se = (first ? 0.0 : double(f.ru_m[y])) + u1;
+    put<kHU>(f.ru_m, g, i + 1, j, k * sk, last ? se / rn + (c.c1h[k] * f.muu[y2] + c.c2h[k]) * us1 / f.msfuy[y2] : se);
+  } else {
+    const idx_t y = g.at(i, j + 1, k), y2 = g.at2(i, j + 1);
+    const real v1 = f.v2[y], vs1 = f.v_save[y];
+    const double sn = (first ? 0.0 : double(f.rv_m[y])) + v1;
+    put<kHV>(f.rv_m, g, i, j + 1, k * sk, last ? sn / rn + (c.c1h[k] * f.muv[y2] + c.c2h[k]) * vs1 * f.msfvx_inv[y2] : sn);
   }
 }
 
@@ -2050,6 +2060,7 @@ __device__ __forceinline__ acc m_full(const Lv& c, int k, double mub, double mu)
 
 // rk_update_scalar (KUpdate, scalars.cpp:145-158): advect_scalar's tendency and the mass-weighted update,
 // every species of the list; the new values go to the species' second buffer
+template <bool OPEN, bool QVF>
 MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
   MI_POINT_INDEX_SP(r)
   const Geo& g = a.g;
@@ -2062,10 +2073,10 @@ MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
     const int n = l.sp[sq_];
     const ScFaces fl{a, s.q2[n], s.q1[n], acc(dt_rk), false};
     acc adv = 0;
-    if (fl.open_col(i) || fl.open_row(j)) {  // WRF order: y flux, x flux, open x, open y, vertical (scalars.cpp:129-134)
+    if ((OPEN && fl.open_col(i)) || (OPEN && fl.open_row(j))) {  // WRF order: y flux, x flux, open x, open y, vertical (scalars.cpp:129-134)
       acc r0 = 0;
-      if (!g.two_d && !fl.open_row(j)) r0 = r0 - (msftx * acc(c.rdy)) * (fl.fy(x + g.sj, j + 1) - fl.fy(x, j));
-      if (!fl.open_col(i)) r0 = r0 - (msftx * acc(c.rdx)) * (fl.fx(x + 1, i + 1) - fl.fx(x, i));
+      if (!g.two_d && !(OPEN && fl.open_row(j))) r0 = r0 - (msftx * acc(c.rdy)) * (fl.fy(x + g.sj, j + 1) - fl.fy(x, j));
+      if (!(OPEN && fl.open_col(i))) r0 = r0 - (msftx * acc(c.rdx)) * (fl.fx(x + 1, i + 1) - fl.fx(x, i));
       r0 = acc(fl.open_terms(double(r0), i, j, k, s.q2[n]));
       adv = r0 - acc(c.rdnw[k]) * (fl.fz(x0, k + 1) - fl.fz(x0, k));
     } else if (!fl.spec_cell(i, j)) {
@@ -2074,7 +2085,7 @@ MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
       adv = adv - acc(c.rdnw[k]) * (fl.fz(x0, k + 1) - fl.fz(x0, k));
     }
     acc tend = adv * msfty;
-    if (f.qvf && n == 0) {  // set_tend(RQVFTEN) of the vapour (scalars.cpp:147-150)
+    if (QVF && f.qvf && n == 0) {  // set_tend(RQVFTEN) of the vapour (scalars.cpp:147-150)
       if (fl.spec_cell(i, j)) {  // adv_spec (scalars.cpp:119-125): no x flux in the outer columns, no y in the rows
         const bool scol = i == 0 || i == g.nx - 1, srow = j == 0 || j == g.ny - 1;
         acc rs = 0;
@@ -2168,6 +2179,7 @@ MI_POINT_KERNEL k_sc_pd_scale(Range r, Args a, Scalars s, SpList l, double dt_rk
 }
 
 // advect_scalar_pd's limited divergence and rk_update_scalar (KUpdatePD, scalars.cpp:252-291)
+template <bool OPEN, bool QVF>
 MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_rk) {
   MI_POINT_INDEX_SP(r)
   const Geo& g = a.g;
@@ -2182,7 +2194,7 @@ MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
     const int n = l.sp[sq_];
     const real* sc = s.scale[n];
     const ScFaces fl{a, s.q2[n], s.q1[n], dt, true};
-    const bool ften = f.qvf && n == 0;
+    const bool ften = QVF && f.qvf && n == 0;
     if (fl.spec_cell(i, j) && !ften) {  // the spec zone: only the held tendency, folded into q_1 before
       put<kHMass>(s.qn[n], g, i, j, k * sk, qdiv(m_old * acc(s.q1[n][x]), m_new));
       return;
@@ -2209,11 +2221,11 @@ MI_POINT_KERNEL k_sc_pd_update(Range r, Args a, Scalars s, SpList l, double dt_r
       return;
     }
     acc tend;
-    if (fl.open_col(i) || fl.open_row(j)) {  // open: open terms (with q_1), vertical, x (not outer columns), y
+    if ((OPEN && fl.open_col(i)) || (OPEN && fl.open_row(j))) {  // open: open terms (with q_1), vertical, x (not outer columns), y
       tend = acc(fl.open_terms(0.0, i, j, k, s.q1[n]));
       tend = tend - acc(c.rdnw[k]) * (dz1 - dz0 + p.loz1 - p.loz0);
-      if (!fl.open_col(i)) tend -= msftx * (acc(c.rdx) * (dx1 - dx0 + p.lox1 - p.lox0));
-      if (!g.two_d && !fl.open_row(j)) {
+      if (!(OPEN && fl.open_col(i))) tend -= msftx * (acc(c.rdx) * (dx1 - dx0 + p.lox1 - p.lox0));
+      if (!g.two_d && !(OPEN && fl.open_row(j))) {
         const acc dy0 = lim(p.dhy0, sc[x - sj], sc0), dy1 = lim(p.dhy1, sc0, sc[x + sj]);
         tend -= msftx * (acc(c.rdy) * (dy1 - dy0 + p.loy1 - p.loy0));
       }
@@ -2581,6 +2593,9 @@ void launch_uv(const Args& a, double dts, hipStream_t st) {
 }
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
   dispatch_cfg<AcousticL>(cfg, "acoustic", 10, 5, a.g.nx, a.g.ny, st, a, dts, it, ns);
+  const Geo& g = a.g;
+  if (g.ox()) launch3("open_sum", k_open_sum, Range{g.nx - 1, g.nx, 0, g.ny, 0, g.nz}, st, a, it, ns, 0);
+  if (g.oy()) launch3("open_sum", k_open_sum, Range{0, g.nx, g.ny - 1, g.ny, 0, g.nz}, st, a, it, ns, 1);
 }
 void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st) {
   dispatch_cfg<FinishL>(cfg, "finish", 1, 1, a.g.nx + 1, a.g.nyv, st, a, prhophi ? 1 : 0, hd_dt);
@@ -2625,11 +2640,17 @@ void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, int s
   for (int q = 0; q < n; ++q) l.sp[q] = sp[q];
   const Geo& g = a.g;
   const Range r{0, g.nx, 0, g.ny, 0, g.nz};
+  // the open-boundary terms and the RQVFTEN store compiled in only where used
+  const bool op = g.ox() || g.oy(), qv = a.f.qvf != nullptr;
   switch (step) {
-    case 0: launch3s("sc_update", 3 + 4 * n, k_sc_update, r, n, st, a, s, l, dt_rk); break;        // q_2 q_1 q_tendf -> q_new
+    case 0: launch3s("sc_update", 3 + 4 * n, op ? (qv ? k_sc_update<true, true> : k_sc_update<true, false>)
+                                                  : (qv ? k_sc_update<false, true> : k_sc_update<false, false>),
+                     r, n, st, a, s, l, dt_rk); break;        // q_2 q_1 q_tendf -> q_new
     case 1: launch3s("sc_pd_pre", 3 * n, k_sc_pd_pre, r, n, st, a, s, l, dt_rk); break;            // q_1 q_tendf -> q_1
     case 2: launch3s("sc_pd_scale", 3 + 3 * n, k_sc_pd_scale, r, n, st, a, s, l, dt_rk); break;    // q_2 q_1 -> scale
-    default: launch3s("sc_pd_update", 3 + 4 * n, k_sc_pd_update, r, n, st, a, s, l, dt_rk); break; // q_2 q_1 scale -> q_new
+    default: launch3s("sc_pd_update", 3 + 4 * n, op ? (qv ? k_sc_pd_update<true, true> : k_sc_pd_update<true, false>)
+                                                        : (qv ? k_sc_pd_update<false, true> : k_sc_pd_update<false, false>),
+                      r, n, st, a, s, l, dt_rk); break; // q_2 q_1 scale -> q_new
   }
 }
 void launch_bdy(const Args& a, const Scalars& s, const BdySet& bs, BdyMode mode, hipStream_t st) {
