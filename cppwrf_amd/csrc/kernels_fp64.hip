@@ -1212,6 +1212,34 @@ MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
 }
 
 // ----------------------------------------------------------------------------- one acoustic substep's column work
+// FP32-bulk with 64 threads per column: the column recurrences run as wave-wide scans, one wavefront per column
+// (wave w of the block takes column w, lane k level k).  The sums then associate differently and the Thomas
+// sweeps become compositions of the affine maps y_k = p_k + q_k y_(k-1): the same solution in another rounding,
+// within FP32-bulk's error bar.  FP64 keeps the reference's serial order (bit-identical).
+constexpr bool kScanChains = sizeof(real) == 4;
+// inclusive prefix sum over the lanes of a wavefront
+__device__ __forceinline__ double wave_prefix_sum(double v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+// y_k = p_k + q_k y_(k-1) composed from lane 0 up (up = true) or y_k = p_k + q_k y_(k+1) from lane 63 down
+template <bool UP>
+__device__ __forceinline__ double wave_affine_scan(double p, double q, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double pp = UP ? __shfl_up(p, d, 64) : __shfl_down(p, d, 64);
+    const double qq = UP ? __shfl_up(q, d, 64) : __shfl_down(q, d, 64);
+    if (UP ? lane >= d : lane + d < 64) {
+      p = p + q * pp;
+      q = q * qq;
+    }
+  }
+  return p;
+}
 // advance_mu_t + sumflux and advance_w + calc_p_rho of one substep in one kernel (reference
 // SmallStep::run_stage, small_step.cpp:548-593).  Between KMuTPoint and KWRhs the reference refreshes
 // the halos of mu, mudf and muts; none of the advance_w kernels reads a halo point (each reads its own
@@ -1359,9 +1387,25 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 2: the column mass
+  // the scan layout: wave sw = column, lane sl = level; sw_on: that column is stepped here
+  const int sw = int(threadIdx.x) / 64, sl = int(threadIdx.x) % 64;
+  const int sw_i = i - ci + sw;
+  const bool sw_on = kScanChains && KT == 64 && sw_i < g.nx &&
+                     !(g.bcx == MI_BC_SPECIFIED && (sw_i == 0 || sw_i == g.nx - 1 || j == 0 || j == g.ny - 1));
+  if constexpr (kScanChains && KT == 64) {
+    double v = (sw_on && sl < nz) ? LDS(DV, sl, sw) : 0.0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (sw_on && sl == 0) S2[sw] = v;
+    __syncthreads();
+  }
   if (on && k == 0) {
     double dmdt = 0;
-    for (int kk = 0; kk < nz; ++kk) dmdt += LDS(DV, kk, ci);
+    if constexpr (kScanChains && KT == 64) {
+      dmdt = S2[ci];
+    } else {
+      for (int kk = 0; kk < nz; ++kk) dmdt += LDS(DV, kk, ci);
+    }
     const double mu_new = mu_old + dts * (dmdt + mtend);
     const double muts = mut + mu_new;
     const double muave = H * ((1.0 + epssm) * mu_new + (1.0 - epssm) * mu_old);
@@ -1384,7 +1428,11 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 4: the running integral
-  if (on && k == 0) {
+  if constexpr (kScanChains && KT == 64) {
+    const bool in = sw_on && sl + 1 < nz;
+    const double v = wave_prefix_sum(in ? LDS(TK, sl, sw) : 0.0, sl);
+    if (in) LDS(TK, sl, sw) = -v;  // integral up to level sl + 1
+  } else if (on && k == 0) {
     double ww_full = 0;
     for (int kk = 0; kk + 1 < nz; ++kk) {
       ww_full = ww_full - LDS(TK, kk, ci);
@@ -1485,7 +1533,26 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
   __syncthreads();
   // ---------------------------------------------------------------- 7: tridiagonal sweeps
-  if (on && k == 0) {
+  if constexpr (kScanChains && KT == 64) {
+    // forward: y_0 = w_0, y_k = (w_k - a_k y_(k-1)) alpha_k; back: w_nz = y_nz, w_k = y_k - gamma_k w_(k+1)
+    double p = 0.0, q = 1.0;
+    if (sw_on && sl == 0) {
+      p = LDS(W, 0, sw);
+      q = 0.0;
+    } else if (sw_on && sl <= nz) {
+      const double al = LDS(AL, sl, sw);
+      p = LDS(W, sl, sw) * al;
+      q = -(LDS(AA, sl, sw) * al);
+    }
+    const double y = wave_affine_scan<true>(p, q, sl);
+    double qb = 1.0, pb = 0.0;
+    if (sw_on && sl <= nz) {
+      pb = y;
+      qb = (sl >= 1 && sl < nz) ? -LDS(GA, sl, sw) : 0.0;
+    }
+    const double wsol = wave_affine_scan<false>(pb, qb, sl);
+    if (sw_on && sl <= nz) LDS(W, sl, sw) = wsol;
+  } else if (on && k == 0) {
     double wprev = LDS(W, 0, ci);
     for (int kk = 1; kk <= nz; ++kk) {
       wprev = (LDS(W, kk, ci) - LDS(AA, kk, ci) * wprev) * LDS(AL, kk, ci);
