@@ -276,14 +276,22 @@ struct Domain final : mi_domain {
     // one batch after the two kernels: the column kernel reads its inputs in its own column only
     halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.ph2, -1, -1}, {f.t_save, -1, -1}, {f.u_save, -1, -1},
           {f.v_save, -1, -1}, {f.mu2, -1, -1}, {f.muts, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
+    // With specified boundaries the substep refreshes below are left out: no kernel of the substeps reads
+    // those halo points (the spec zone's faces and columns are advanced from their own values, the interior's
+    // stencils reach the spec zone only), and the stage's last refreshes write them before any reader.
     for (int it = 1; it <= ns; ++it) {
       launch_uv(a, dts, st);
-      if (!a.g.two_d) halo({{f.u2, -1, -1}, {f.v2, -1, -1}});  // small_step.cpp:540-541
-      else halo({{f.u2, -1, -1}});
+      if (spec) {
+      } else if (!a.g.two_d) {
+        halo({{f.u2, -1, -1}, {f.v2, -1, -1}});  // small_step.cpp:540-541
+      } else {
+        halo({{f.u2, -1, -1}});
+      }
       launch_acoustic(a, colcfg[kColAcoustic], dts, it, ns, st);
       if (spec) launch_acoustic_ring(a, dts, it, ns, st);  // the spec zone's columns
       // the refreshes of mu'', mudf, muts and of ph'', p, al (small_step.cpp:572, 593) after the fused kernel
-      halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
+      if (!spec)
+        halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
     }
     if (spec) launch_bdy_zero_grad(a, st);  // zero_grad_bdy of the spec zone's w (lateral_bc.cpp:374-387)
     halo({{f.muts, -1, -1}, {f.ru_m, -1, -1}, {f.rv_m, -1, -1}, {f.ww_m, -1, -1}});  // small_step.cpp:597
