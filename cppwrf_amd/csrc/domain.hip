@@ -273,9 +273,11 @@ struct Domain final : mi_domain {
     const double cof_h = 0.5 * dts * kG * (1.0 + prm.epssm);
     launch_ss_column(a, colcfg[kColSsColumn], cof_h * cof_h, st);
     // the halos of u'', v'', ph'', t_save, u_save, v_save, mu'', muts, p, al (small_step.cpp:517-522), in
-    // one batch after the two kernels: the column kernel reads its inputs in its own column only
-    halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.ph2, -1, -1}, {f.t_save, -1, -1}, {f.u_save, -1, -1},
-          {f.v_save, -1, -1}, {f.mu2, -1, -1}, {f.muts, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
+    // one batch after the two kernels: the column kernel reads its inputs in its own column only.  With
+    // specified boundaries no substep kernel reads them (as the substep refreshes below)
+    if (!spec)
+      halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.ph2, -1, -1}, {f.t_save, -1, -1}, {f.u_save, -1, -1},
+            {f.v_save, -1, -1}, {f.mu2, -1, -1}, {f.muts, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
     // With specified boundaries the substep refreshes below are left out: no kernel of the substeps reads
     // those halo points (the spec zone's faces and columns are advanced from their own values, the interior's
     // stencils reach the spec zone only), and the stage's last refreshes write them before any reader.
