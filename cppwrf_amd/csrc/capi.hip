@@ -212,6 +212,9 @@ int mi_rk_tendency(mi_domain* d, int rk) { return run_phase(d, kPhaseRkTendency,
 int mi_small_steps(mi_domain* d, int rk) { return run_phase(d, kPhaseSmallSteps, rk); }
 int mi_scalar_transport(mi_domain* d, int rk) { return run_phase(d, kPhaseScalars, rk); }
 int mi_calc_p_rho_phi(mi_domain* d, int rk) { return run_phase(d, kPhaseCalcPRhoPhi, rk); }
+int mi_calc_p_rho_phi_interior(mi_domain* d, int spec_zone) {
+  return mi_detail::guard([&] { checked(d)->calc_p_rho_phi_interior(spec_zone); });
+}
 int mi_step_end(mi_domain* d) { return run_phase(d, kPhaseStepEnd, 0); }
 
 int mi_bdy_set(mi_domain* d, int var, int side, const double* values, const double* tends, int64_t n) {

@@ -347,6 +347,16 @@ struct Domain final : mi_domain {
     halo_scalars(sc.q2, all, n_sc);  // scalars.cpp:458-460
   }
 
+  // SmallStep::calc_p_rho_phi_interior (small_step.cpp:619-633): the columns spec_zone in from each non-periodic
+  // side; with the scalar pow every point gets the bits calc_p_rho_phi gave it (the reference's differences
+  // come from libmvec's lanes only)
+  void calc_p_rho_phi_interior(int sz) override {
+    if (sz < 0) throw Error(MI_EINVAL, "spec_zone must be >= 0");
+    const bool px = grid.bc_x == MI_BC_PERIODIC, py = grid.bc_y == MI_BC_PERIODIC || a.g.two_d;
+    launch_p_rho_phi_range(a, px ? 0 : sz, px ? grid.nx : grid.nx - sz, py ? 0 : sz, py ? grid.ny : grid.ny - sz, stream());
+    halo({{a.f.p, -1, -1}, {a.f.al, -1, -1}});
+    hip_check(hipGetLastError(), "kernel launch");
+  }
   // SmallStep::calc_p_rho_phi (small_step.cpp:609-617), for the phase-by-phase entry points
   void calc_p_rho_phi() {
     launch_p_rho_phi(a, stream());

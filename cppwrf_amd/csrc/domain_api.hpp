@@ -50,6 +50,7 @@ struct mi_domain {
   virtual void set_prep_callback(mi_stage_fn fn, void* user) = 0;
   virtual bool has_callbacks() const = 0;
   virtual void set_column_physics(const mi_column_physics* cp) = 0;
+  virtual void calc_p_rho_phi_interior(int spec_zone) = 0;
   // specified lateral boundaries (LateralBC::set / set_scalar, dtbc)
   virtual void bdy_set(int var, int side, const double* values, const double* tends, int64_t n) = 0;
   virtual void bdy_set_time(double dtbc) = 0;

@@ -2769,6 +2769,9 @@ void launch_p_rho_phi(const Args& a, hipStream_t st) {
   const Geo& g = a.g;
   launch3("p_rho_phi", k_p_rho_phi, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a);
 }
+void launch_p_rho_phi_range(const Args& a, int i0, int i1, int j0, int j1, hipStream_t st) {
+  if (i1 > i0 && j1 > j0) launch3("p_rho_phi", k_p_rho_phi, Range{i0, i1, j0, j1, 0, a.g.nz}, st, a);
+}
 void launch_finite(const Args& a, int* flags, hipStream_t st) {
   const Geo& g = a.g;
   launch3("finite", k_finite, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, flags);

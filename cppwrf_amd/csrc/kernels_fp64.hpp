@@ -181,6 +181,8 @@ void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, int s
 void launch_pd_geom(const Args& a, double dt, hipStream_t st);
 void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hipStream_t st);
 void launch_p_rho_phi(const Args& a, hipStream_t st);
+// the same on the columns i0 <= i < i1, j0 <= j < j1
+void launch_p_rho_phi_range(const Args& a, int i0, int i1, int j0, int j1, hipStream_t st);
 void launch_finite(const Args& a, int* flags, hipStream_t st);
 
 }  // namespace MI_NS

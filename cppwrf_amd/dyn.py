@@ -377,6 +377,10 @@ class RK3:
     def calc_p_rho_phi(self, rk):
         check(lib().mi_calc_p_rho_phi(self._h, rk))
 
+    def calc_p_rho_phi_interior(self, spec_zone: int) -> None:
+        """SmallStep::calc_p_rho_phi_interior: p, al again away from the spec zone"""
+        check(lib().mi_calc_p_rho_phi_interior(self._h, spec_zone))
+
     def step_end(self):
         check(lib().mi_step_end(self._h))
         self._raise_hook_error()

@@ -219,6 +219,9 @@ int mi_rk_tendency(mi_domain* d, int rk);
 int mi_small_steps(mi_domain* d, int rk);
 int mi_scalar_transport(mi_domain* d, int rk);
 int mi_calc_p_rho_phi(mi_domain* d, int rk);
+/* reference SmallStep::calc_p_rho_phi_interior (small_step.cpp:619-633): p and al again on the columns spec_zone
+ * in from each non-periodic side (solve_em after spec_bdy_final and after the microphysics; rk3.cpp:155-161) */
+int mi_calc_p_rho_phi_interior(mi_domain* d, int spec_zone);
 int mi_step_end(mi_domain* d);
 
 /* ---------------------------------------------------------------- specified lateral boundaries

@@ -2044,6 +2044,16 @@ int orc_cfl_max(orc_domain* d, double* vert, double* horiz) {
   d->d->cfl_max(*vert, *horiz);
   return MI_OK;
 }
+int orc_calc_p_rho_phi_interior(orc_domain* od, int sz) {  // SmallStep::calc_p_rho_phi_interior (small_step.cpp:619-633)
+  Dom& d = *od->d;
+  if (sz < 0) return fail("spec_zone must be >= 0");
+  const bool px = d.g.bcx == MI_BC_PERIODIC, py = d.g.bcy == MI_BC_PERIODIC || d.g.two_d();
+  const int i0 = px ? 0 : sz, i1 = px ? d.g.nx : d.g.nx - sz, j0 = py ? 0 : sz, j1 = py ? d.g.ny : d.g.ny - sz;
+  if (i1 > i0 && j1 > j0) d.calc_p_rho_phi_range(i0, i1, j0, j1);
+  fill(d.pp, d.g);
+  fill(d.al, d.g);
+  return MI_OK;
+}
 int orc_calc_p_rho_phi(orc_domain* d, int) {
   d->d->calc_p_rho_phi();
   return MI_OK;

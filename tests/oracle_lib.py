@@ -42,6 +42,7 @@ def olib():
             "orc_cfl_max": (I, [VP, PD, PD]),
             "orc_set_dt": (I, [VP, C.c_double, I]),
             "orc_calc_p_rho_phi": (I, [VP, I]),
+            "orc_calc_p_rho_phi_interior": (I, [VP, I]),
             "orc_step_end": (I, [VP]),
             "orc_set_column_physics": (I, [VP, C.POINTER(OrcColumnPhysics)]),
             "orc_bdy_set": (I, [VP, I, I, PD, PD, C.c_int64]),
@@ -146,6 +147,9 @@ class OracleDomain:
 
     def calc_p_rho_phi(self, rk):
         _ok(olib().orc_calc_p_rho_phi(self._h, rk))
+
+    def calc_p_rho_phi_interior(self, spec_zone):
+        _ok(olib().orc_calc_p_rho_phi_interior(self._h, spec_zone))
 
     def step_end(self):
         _ok(olib().orc_step_end(self._h))
