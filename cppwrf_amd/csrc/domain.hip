@@ -245,7 +245,7 @@ struct Domain final : mi_domain {
   // With a physics plugin (hooks on) the held tendencies were zeroed before part1; part2 sets the physics
   // tendencies first (large_step.cpp:820-821) and the diffusion adds to them.
   void rk_tendency(int rk, bool hooks = true) {
-    if (spec) launch_fill(a.f.mu_tend - off, n2, 0.0, stream());
+    if (spec) launch_fill2d(a, a.f.mu_tend - off, n2, 0.0, stream());
     if (rk == 1 && a.q.use_tendf) {
       if (phys_on(hooks)) {
         call_phys(phys.part2);
@@ -268,7 +268,7 @@ struct Domain final : mi_domain {
     const double dts = rk == 1 ? dt_rk : dts_full;
     const int ns = rk == 1 ? 1 : (rk == 2 ? n_sound_ / 2 : n_sound_);
     hipStream_t st = stream();
-    if (rk == 1) launch_fill(f.mudf - off, n2, 0.0, st);
+    if (rk == 1) launch_fill2d(a, f.mudf - off, n2, 0.0, st);
     launch_ss_prep(a, rk == 1, st);
     const double cof_h = 0.5 * dts * kG * (1.0 + prm.epssm);
     launch_ss_column(a, colcfg[kColSsColumn], cof_h * cof_h, st);

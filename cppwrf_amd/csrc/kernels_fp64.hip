@@ -304,6 +304,11 @@ __global__ void k_fill(double* p, int64_t n, double v) {
   const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (e < n) p[e] = v;
 }
+// a 2-D field's points only: the first pitch values of every row of sj (its allocation has a 3-D field's size)
+__global__ void k_fill2d(double* p, int64_t sj, int pitch, int64_t rows, double v) {
+  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (e < int64_t(pitch) * rows) p[(e / pitch) * sj + e % pitch] = v;
+}
 
 // ----------------------------------------------------------------------------- column-group kernels
 // A block holds CI consecutive columns (i) of one row j and kKT threads per column along k: the
@@ -2555,6 +2560,16 @@ void launch_copy_step(const Args& a, int64_t n3, int64_t n2, hipStream_t st) {
 
 void launch_fill(double* p, int64_t n, double v, hipStream_t st) {
   hipLaunchKernelGGL(k_fill, dim3(unsigned((n + 255) / 256), 1, 1), dim3(256, 1, 1), 0, st, p, n, v);
+}
+void launch_fill2d(const Args& a, double* p, int64_t n_alloc, double v, hipStream_t st) {
+  const Geo& g = a.g;
+  if (g.sj <= 0) {  // an x-z slice: one row
+    launch_fill(p, g.sk, v, st);
+    return;
+  }
+  const int64_t rows = n_alloc / g.sj, n = g.sk * rows;
+  hipLaunchKernelGGL(k_fill2d, dim3(unsigned((n + 255) / 256), 1, 1), dim3(256, 1, 1), 0, st, p, int64_t(g.sj),
+                     int(g.sk), rows, v);
 }
 
 constexpr size_t kLdsMax = 65536;

@@ -140,6 +140,8 @@ extern thread_local KernelTimer* g_timer;
 void launch_halo(const Args& a, const HaloSet& s, hipStream_t st);
 void launch_copy_step(const Args& a, int64_t n_alloc3, int64_t n_alloc2, hipStream_t st);
 void launch_fill(double* p, int64_t n, double v, hipStream_t st);
+// the same over a 2-D field's points (level 0 of every row) of an allocation of n_alloc values
+void launch_fill2d(const Args& a, double* p, int64_t n_alloc, double v, hipStream_t st);
 // column kernels and their block shapes (columns x threads per column)
 enum ColKernel { kColLargePrep = 0, kColSsColumn = 1, kColAcoustic = 2, kColFinish = 3, kNumColKernels = 4 };
 constexpr int kNumColCfgs = 12;
