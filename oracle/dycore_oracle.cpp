@@ -2044,6 +2044,19 @@ int orc_cfl_max(orc_domain* d, double* vert, double* horiz) {
   d->d->cfl_max(*vert, *horiz);
   return MI_OK;
 }
+// the flux operators on their own, for the closed-form checks: which = 0 flux5, 1 flux3, 2 flux5_pd, 3 flux3_pd
+// (q = qm3..qp2 or qm2..qp1), 4 flux_upwind (q = qm1, q0; ua is the Courant number)
+int orc_flux_eval(int which, const double* q, double ua, double up, double* out) {
+  if (!q || !out) return fail("null argument");
+  switch (which) {
+    case 0: *out = flux5(q[0], q[1], q[2], q[3], q[4], q[5], ua, up); return MI_OK;
+    case 1: *out = flux3(q[0], q[1], q[2], q[3], ua, up); return MI_OK;
+    case 2: *out = flux5_pd(q[0], q[1], q[2], q[3], q[4], q[5], ua, up); return MI_OK;
+    case 3: *out = flux3_pd(q[0], q[1], q[2], q[3], ua, up); return MI_OK;
+    case 4: *out = flux_upwind(q[0], q[1], ua); return MI_OK;
+    default: return fail("which must be 0..4");
+  }
+}
 int orc_calc_p_rho_phi_interior(orc_domain* od, int sz) {  // SmallStep::calc_p_rho_phi_interior (small_step.cpp:619-633)
   Dom& d = *od->d;
   if (sz < 0) return fail("spec_zone must be >= 0");

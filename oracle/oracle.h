@@ -41,6 +41,8 @@ int orc_cfl_max(orc_domain* d, double* vert, double* horiz); /* LargeStep::cfl_m
 int orc_set_dt(orc_domain* d, double dt, int n_sound);        /* RK3::set_dt + LateralBC::set_dt (rk3.cpp:79-84) */
 int orc_calc_p_rho_phi(orc_domain* d, int rk);
 int orc_calc_p_rho_phi_interior(orc_domain* d, int spec_zone);
+/* the advection flux operators alone (kernels_common.hpp:13-33, 83-88), for closed-form checks */
+int orc_flux_eval(int which, const double* q, double ua, double up, double* out);
 int orc_step_end(orc_domain* d); /* after_rk, spec_bdy_final, set_w_surface, after_all_rk_steps */
 /* a column physics plugin (reference ColumnPhysics, src/dyn/column_physics.hpp:21-34): the hooks of
  * mi_domain_set_column_physics with the oracle's own user pointer; NULL detaches */

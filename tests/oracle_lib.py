@@ -43,6 +43,7 @@ def olib():
             "orc_set_dt": (I, [VP, C.c_double, I]),
             "orc_calc_p_rho_phi": (I, [VP, I]),
             "orc_calc_p_rho_phi_interior": (I, [VP, I]),
+            "orc_flux_eval": (I, [I, PD, C.c_double, C.c_double, PD]),
             "orc_step_end": (I, [VP]),
             "orc_set_column_physics": (I, [VP, C.POINTER(OrcColumnPhysics)]),
             "orc_bdy_set": (I, [VP, I, I, PD, PD, C.c_int64]),
