@@ -1,0 +1,69 @@
+"""Options of the large step that no golden file of the reference exercises, checked on the oracle against what
+fixes them: top_lid holds w = 0 at the model top (the rigid lid of advance_w, small_step.cpp:375-378, 386); w_damp
+(large_step.cpp:475-480) acts only where the vertical Courant number exceeds 1 and there pulls w back; damp_opt 3
+(small_step.cpp:393-405) damps w in the top zdamp metres only in its first substep's column solve."""
+import numpy as np
+
+from cppwrf_amd import _abi as A
+from cppwrf_amd.dyn import Case
+from oracle_lib import OracleDomain
+
+STATE = (A.MI_U, A.MI_V, A.MI_W, A.MI_T, A.MI_PH, A.MI_MU, A.MI_P, A.MI_AL)
+
+
+def _bubble():
+    return Case.bubble3d(16, 14, 12, dx=1000.0, ztop=8000.0, dt=6.0, seed=3, bubble_zc=3000.0)
+
+
+def test_top_lid_keeps_w_zero_at_the_top():
+    case = _bubble()
+    lid = OracleDomain(case, case.params(time_step_sound=4, top_lid=1))
+    free = OracleDomain(case, case.params(time_step_sound=4, top_lid=0))
+    lid.step(5)
+    free.step(5)
+    assert np.all(lid.field(A.MI_W)[-1] == 0.0)
+    assert np.abs(free.field(A.MI_W)[-1]).max() > 0.0
+
+
+def _updraft(o, case, amp):
+    g = case.grid
+    z, y, x = np.meshgrid(np.arange(g.nz + 1), np.arange(g.ny), np.arange(g.nx), indexing="ij")
+    w = amp * np.sin(np.pi * z / g.nz) * np.exp(-((x - g.nx / 2) ** 2 + (y - g.ny / 2) ** 2) / 8.0)
+    o.set_field(A.MI_W, w)
+
+
+def _converge(o, case, amp):
+    """u = amp sin(4 pi x / L) in the lower half: a convergence line whose omega (calc_ww_cp) sets the vertical
+    Courant number w_damp tests"""
+    g = case.grid
+    z, y, x = np.meshgrid(np.arange(g.nz), np.arange(g.ny), np.arange(g.nx + 1), indexing="ij")
+    o.set_field(A.MI_U, amp * np.sin(4 * np.pi * x / g.nx) * (z < g.nz // 2))
+
+
+def test_w_damping_acts_only_beyond_courant_one():
+    # 10 km columns: the acoustic steps of dt 30 s stay stable horizontally
+    case = Case.bubble3d(16, 14, 12, dx=10000.0, ztop=8000.0, dt=30.0, seed=3, bubble_zc=3000.0)
+    runs = {}
+    for amp in (5.0, 250.0):  # vertical Courant numbers of about 0.03 and 1.7 (10 km columns, dt 30 s)
+        for wd in (0, 1):
+            o = OracleDomain(case, case.params(time_step_sound=8, w_damping=wd, w_crit_cfl=1.0))
+            _converge(o, case, amp)
+            o.step(1)
+            runs[amp, wd] = {fid: o.field(fid) for fid in STATE}
+    for fid in STATE:  # below the threshold: not a bit changes
+        assert np.array_equal(runs[5.0, 0][fid], runs[5.0, 1][fid]), A.FIELD_NAMES[fid]
+    w0, w1 = runs[250.0, 0][A.MI_W], runs[250.0, 1][A.MI_W]
+    assert not np.array_equal(w0, w1)
+    assert np.abs(w1).max() < np.abs(w0).max()  # the damping pulls the strongest updraft back
+
+
+def test_damp_opt3_damps_the_top_layer():
+    case = _bubble()
+    out = {}
+    for d in (0, 3):
+        o = OracleDomain(case, case.params(time_step_sound=4, damp_opt=d, zdamp=3000.0, dampcoef=0.2))
+        _updraft(o, case, 5.0)
+        o.step(1)
+        out[d] = o.field(A.MI_W)
+    top = slice(9, 12)  # w levels in the top 3 km of 8 km
+    assert np.sqrt((out[3][top] ** 2).mean()) < np.sqrt((out[0][top] ** 2).mean())
