@@ -191,6 +191,7 @@ SIGNATURES = {
     "mi_scalar_transport": (_I, [_VP, _I]),
     "mi_calc_p_rho_phi": (_I, [_VP, _I]),
     "mi_calc_p_rho_phi_interior": (_I, [_VP, _I]),
+    "mi_domain_set_diffusion": (_I, [_VP, _I, C.c_double]),
     "mi_step_end": (_I, [_VP]),
     "mi_bdy_set": (_I, [_VP, _I, _I, _PD, _PD, _I64]),
     "mi_bdy_set_time": (_I, [_VP, C.c_double]),

@@ -353,6 +353,10 @@ int mi_domain_set_column_physics(mi_domain* d, const mi_column_physics* cp) {
   return mi_detail::guard([&] { checked(d)->set_column_physics(cp); });
 }
 
+int mi_domain_set_diffusion(mi_domain* d, int opt, double factor) {
+  return mi_detail::guard([&] { checked(d)->set_diffusion(opt, factor); });
+}
+
 int mi_check_finite(mi_domain* d, int* bad_field) {
   int bad = -1;
   const int rc = mi_detail::guard([&] { bad = checked(d)->check_finite(bad_field); });

@@ -429,16 +429,7 @@ struct Domain final : mi_domain {
   }
   void set_column_physics(const mi_column_physics* cp) override {
     if (cp && cp->qv_forcing && prm.n_moist == 0) throw Error(MI_EINVAL, "qv_forcing needs moisture (n_moist > 0)");
-    if (cp && !a.f.ru_tendf) {  // the held tendencies, first needed now
-      add3(a.f.ru_tendf, St::U, MI_RU_TENDF);
-      add3(a.f.rv_tendf, St::V, MI_RV_TENDF);
-      add3(a.f.rw_tendf, St::W, MI_RW_TENDF);
-      add3(a.f.t_tendf, St::M, MI_T_TENDF);
-    }
-    // the scalars' step tendencies (the caller's plus diffusion and relaxation), read instead of MI_Q_TENDF
-    // once held tendencies are used
-    for (int n = 0; cp && n < n_sc; ++n)
-      if (!sc.qtw[n]) add3(sc.qtw[n], St::M, -1);
+    if (cp) ensure_held();
     if (cp && cp->th_forcing && !thf_buf) thf_buf = add3(thf_buf, St::M, MI_TH_FORCING);
     if (cp && cp->qv_forcing && !qvf_buf) qvf_buf = add3(qvf_buf, St::M, MI_QV_FORCING);
     has_phys = cp != nullptr;
@@ -455,6 +446,28 @@ struct Domain final : mi_domain {
   }
   real* thf_buf = nullptr;
   real* qvf_buf = nullptr;
+  // the held tendencies and the scalars' step tendencies, allocated once they are first needed
+  void ensure_held() {
+    if (!a.f.ru_tendf) {
+      add3(a.f.ru_tendf, St::U, MI_RU_TENDF);
+      add3(a.f.rv_tendf, St::V, MI_RV_TENDF);
+      add3(a.f.rw_tendf, St::W, MI_RW_TENDF);
+      add3(a.f.t_tendf, St::M, MI_T_TENDF);
+    }
+    for (int n = 0; n < n_sc; ++n)
+      if (!sc.qtw[n]) add3(sc.qtw[n], St::M, -1);
+  }
+  void set_diffusion(int opt, double factor) override {
+    if (opt < 0 || opt > 2) throw Error(MI_EINVAL, "diff_6th_opt must be 0, 1 or 2");
+    if (opt != 0) ensure_held();
+    prm.diff_6th_opt = opt;
+    prm.diff_6th_factor = factor;
+    a.q.diff_6th_opt = opt;
+    a.q.diff_6th_factor = factor;
+    held_base = opt != 0 || spec;
+    a.q.use_tendf = (held_base || has_phys) ? 1 : 0;
+    drop_graphs();
+  }
   template <class T>
   T* add3(T*& slot, St s, int id) {
     slot = alloc<T>(n3) + off;

@@ -51,6 +51,7 @@ struct mi_domain {
   virtual bool has_callbacks() const = 0;
   virtual void set_column_physics(const mi_column_physics* cp) = 0;
   virtual void calc_p_rho_phi_interior(int spec_zone) = 0;
+  virtual void set_diffusion(int opt, double factor) = 0;
   // specified lateral boundaries (LateralBC::set / set_scalar, dtbc)
   virtual void bdy_set(int var, int side, const double* values, const double* tends, int64_t n) = 0;
   virtual void bdy_set_time(double dtbc) = 0;

@@ -197,6 +197,10 @@ class RK3 {
     check(mi_domain_field_view(d_, field, &v));
     return v;
   }
+  // reference RK3::set_diffusion (rk3.hpp:53-62) with the sixth-order diffusion: opt 0 detaches
+  void set_diffusion(int diff_6th_opt, double diff_6th_factor = 0.12) {
+    check(mi_domain_set_diffusion(d_, diff_6th_opt, diff_6th_factor));
+  }
   // reference RK3::set_column_physics: the plugin is not owned (rk3.hpp:53-62); nullptr detaches
   void set_column_physics(ColumnPhysics* cp) {
     cp_ = cp;

@@ -377,6 +377,10 @@ class RK3:
     def calc_p_rho_phi(self, rk):
         check(lib().mi_calc_p_rho_phi(self._h, rk))
 
+    def set_diffusion(self, diff_6th_opt: int, diff_6th_factor: float = 0.12) -> None:
+        """RK3::set_diffusion with the sixth-order option: attach (1 linear, 2 monotonic), change or detach (0)"""
+        check(lib().mi_domain_set_diffusion(self._h, diff_6th_opt, diff_6th_factor))
+
     def calc_p_rho_phi_interior(self, spec_zone: int) -> None:
         """SmallStep::calc_p_rho_phi_interior: p, al again away from the spec zone"""
         check(lib().mi_calc_p_rho_phi_interior(self._h, spec_zone))

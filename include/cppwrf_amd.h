@@ -302,6 +302,12 @@ typedef struct mi_column_physics {
 } mi_column_physics;
 int mi_domain_set_column_physics(mi_domain* d, const mi_column_physics* cp);
 
+/* Attach, change or detach the sixth-order diffusion between steps (reference RK3::set_diffusion with a
+ * DiffusionDriver whose sixth-order option is on, rk3.hpp:53-62, diffusion_driver.cpp:266-327): opt 0 off, 1
+ * linear, 2 monotonic; factor the non-dimensional diff_6th_factor.  The same as creating the domain with these
+ * mi_params values, from the next step on. */
+int mi_domain_set_diffusion(mi_domain* d, int diff_6th_opt, double diff_6th_factor);
+
 /* Batches (ensembles, serving): the steps of several domains of one context advance together.  Each
  * step of the batch is one HIP graph holding every member's step as an independent branch, so the
  * members' kernels overlap on the device (reference: the tiles of main.cpp:30-42 stepped one thread per

@@ -2009,6 +2009,15 @@ int orc_rk_prep(orc_domain* d, int rk) {
   if (rk == 1) d->d->physics_part1();
   return MI_OK;
 }
+int orc_set_diffusion(orc_domain* od, int opt, double factor) {
+  Dom& d = *od->d;
+  if (opt < 0 || opt > 2) return fail("diff_6th_opt must be 0, 1 or 2");
+  d.p.diff_6th_opt = opt;
+  d.p.diff_6th_factor = factor;
+  d.held_base = opt != 0 || d.spec;
+  d.use_tendf = d.held_base || d.has_phys;
+  return MI_OK;
+}
 int orc_set_column_physics(orc_domain* od, const orc_column_physics* cp) {
   Dom& d = *od->d;
   if (cp && cp->qv_forcing && d.n_moist == 0) return fail("qv_forcing needs moisture (n_moist > 0)");

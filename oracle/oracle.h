@@ -53,6 +53,7 @@ typedef struct orc_column_physics {
   int32_t th_forcing, qv_forcing;
 } orc_column_physics;
 int orc_set_column_physics(orc_domain* d, const orc_column_physics* cp);
+int orc_set_diffusion(orc_domain* d, int diff_6th_opt, double diff_6th_factor);
 /* specified boundaries: as mi_bdy_set / mi_bdy_set_time / mi_bdy_get_time */
 int orc_bdy_set(orc_domain* d, int var, int side, const double* values, const double* tends, int64_t n);
 int orc_bdy_set_time(orc_domain* d, double dtbc);

@@ -44,6 +44,7 @@ def olib():
             "orc_calc_p_rho_phi": (I, [VP, I]),
             "orc_calc_p_rho_phi_interior": (I, [VP, I]),
             "orc_flux_eval": (I, [I, PD, C.c_double, C.c_double, PD]),
+            "orc_set_diffusion": (I, [VP, I, C.c_double]),
             "orc_step_end": (I, [VP]),
             "orc_set_column_physics": (I, [VP, C.POINTER(OrcColumnPhysics)]),
             "orc_bdy_set": (I, [VP, I, I, PD, PD, C.c_int64]),
@@ -148,6 +149,9 @@ class OracleDomain:
 
     def calc_p_rho_phi(self, rk):
         _ok(olib().orc_calc_p_rho_phi(self._h, rk))
+
+    def set_diffusion(self, opt, factor):
+        _ok(olib().orc_set_diffusion(self._h, opt, factor))
 
     def calc_p_rho_phi_interior(self, spec_zone):
         _ok(olib().orc_calc_p_rho_phi_interior(self._h, spec_zone))

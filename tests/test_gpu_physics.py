@@ -126,3 +126,27 @@ def test_physics_hook_error_is_raised(ctx):
     gpu.set_column_physics(Bad())
     with pytest.raises(ValueError, match="plugin failure"):
         gpu.step(1)
+
+
+def test_set_diffusion_between_steps(ctx):
+    """RK3::set_diffusion: sixth-order diffusion attached after 2 steps, changed after 4, detached after 6: the
+    oracle's trajectory bit for bit, the attached steps as a domain created with the option would take them"""
+    case = Case.hills(20, 18, 14, dx=15000.0, dt=90.0, seed=6)
+    prm = case.params(time_step_sound=4, damp_opt=3, n_moist=2, n_scalars=2)
+    gpu, orc = RK3(ctx, case.grid, prm, case), OracleDomain(case, prm)
+    print("[physics] set_diffusion", flush=True)
+    qs = tuple(A.MI_Q + n for n in range(prm.n_scalars))
+    for opt, factor in ((0, 0.12), (1, 0.12), (2, 0.25), (0, 0.12)):
+        gpu.set_diffusion(opt, factor)
+        orc.set_diffusion(opt, factor)
+        gpu.step(2)
+        orc.step(2)
+        _compare(gpu, orc, STATE + qs, f"diffusion {opt} {factor}")
+    # attached from the start: the same as a domain created with the option
+    fresh = RK3(ctx, case.grid, case.params(time_step_sound=4, damp_opt=3, n_moist=2, n_scalars=2, diff_6th_opt=2,
+                                           diff_6th_factor=0.25), case)
+    late = RK3(ctx, case.grid, prm, case)
+    late.set_diffusion(2, 0.25)
+    fresh.step(3)
+    late.step(3)
+    _compare(fresh, late, STATE + qs, "attached at creation or afterwards")
