@@ -10,7 +10,7 @@ import pytest
 
 from conftest import bitwise_equal, first_diff
 from cppwrf_amd import _abi as A
-from cppwrf_amd.dyn import Case, Context, RK3
+from cppwrf_amd.dyn import Batch, Case, Context, RK3
 from oracle_lib import OracleDomain
 
 pytestmark = pytest.mark.gpu
@@ -140,3 +140,16 @@ def test_open_fp32_close(ctx):
         l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         print(name, l2, flush=True)
         assert l2 < 1e-3, (name, l2)
+
+
+def test_open_domains_in_a_batch(ctx):
+    """Open-boundary domains of different sizes and directions stepped together in one graph per batch step
+    (mi_batch_step) end bit-identical to the oracle"""
+    print("[open] batch", flush=True)
+    pairs = [_pair(ctx, name) for name in ("x_bubble", "xy_hills_moist", "slice")]
+    b = Batch(ctx, [p[2] for p in pairs])
+    b.step(3)
+    b.sync()
+    for case, prm, gpu, orc in pairs:
+        orc.step(3)
+        _compare(gpu, orc, STATE + tuple(A.MI_Q + n for n in range(prm.n_scalars)), f"batch member {case.grid}")
