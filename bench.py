@@ -203,7 +203,7 @@ def serving_bench(ctx, seconds=20.0, rate=2.0, scale=0.5, seed=1):
 
 
 def cpu_baseline(case, prm, bdy, budget_s=15.0):
-    """The oracle (C++/OpenMP, bit-identical to the reference CPU FP64 build) on this host's cores."""
+    """The oracle (C++/OpenMP, bit-identical to the reference CPU FP64 build with scalar pow) on this host's cores."""
     from oracle_lib import OracleDomain, olib
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     olib().orc_set_threads(threads)
@@ -311,7 +311,7 @@ def main():
         "data": "synthetic",
         "config": dict(desc, precision="fp32-bulk"),
         "fp64": {"value": v64, "ms_per_step": secs["fp64"] * 1e3 / args.steps, "dtype": "f64",
-                 "note": "plain FP64, bit-identical to the reference CPU FP64 build"},
+                 "note": "plain FP64, bit-identical to the reference CPU FP64 build with CPPWRF_NO_LIBMVEC (scalar glibc pow in calc_p_rho_phi)"},
         "speedup_fp32_vs_fp64": v32 / v64,
         "fp32_error_vs_fp64": {"after_steps": args.warmup + args.steps,
                                "simulated_s": (args.warmup + args.steps) * g.dt, "per_variable": err,

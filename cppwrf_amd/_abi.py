@@ -199,6 +199,7 @@ SIGNATURES = {
     "mi_batch_create": (_I, [_VP, C.POINTER(_VP), _I, C.POINTER(_VP)]),
     "mi_batch_destroy": (_I, [_VP]),
     "mi_batch_step": (_I, [_VP, _I]),
+    "mi_batch_graph_count": (_I, [_VP]),
     "mi_cfl_max": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mi_domain_set_dt": (_I, [_VP, C.c_double, _I]),
     "mi_domain_state_bytes": (_I64, [_VP]),

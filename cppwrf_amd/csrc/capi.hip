@@ -52,10 +52,7 @@ struct mi_batch {
   std::vector<Graph> graphs;
 
   ~mi_batch() {
-    for (Graph& gr : graphs) {
-      if (gr.x) (void)hipGraphExecDestroy(gr.x);
-      if (gr.g) (void)hipGraphDestroy(gr.g);
-    }
+    for (Graph& gr : graphs) destroy(gr);
     for (hipEvent_t e : join) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
     for (hipStream_t s : st) (void)hipStreamDestroy(s);
@@ -65,9 +62,25 @@ struct mi_batch {
     for (mi_domain* d : m) k.push_back(d->buffer_parity());
     return k;
   }
+  static void destroy(Graph& gr) {
+    if (gr.x) (void)hipGraphExecDestroy(gr.x);
+    if (gr.g) (void)hipGraphDestroy(gr.g);
+    gr.x = nullptr;
+    gr.g = nullptr;
+  }
   hipGraphExec_t exec_for(const std::vector<int>& k) {
     for (Graph& gr : graphs)
       if (gr.key == k) return gr.x;
+    // a member's launch schedule changed (set_dt, set_tuning, ...): the graphs of its older schedules are
+    // never replayed again, so only those of the current schedules (one per buffer parity) are kept
+    std::vector<Graph> keep;
+    for (Graph& gr : graphs) {
+      bool current = gr.key.size() == k.size();
+      for (size_t q = 0; current && q < k.size(); ++q) current = (gr.key[q] >> 1) == (k[q] >> 1);
+      if (current) keep.push_back(gr);
+      else destroy(gr);
+    }
+    graphs.swap(keep);
     hipStream_t s0 = ctx->stream;
     hip_check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
@@ -318,6 +331,8 @@ int mi_batch_step(mi_batch* b, int nsteps) {
         hip_check(hipStreamWaitEvent(d->context()->stream, b->fork, 0), "hipStreamWaitEvent");
   });
 }
+
+int mi_batch_graph_count(const mi_batch* b) { return b ? int(b->graphs.size()) : MI_EINVAL; }
 
 int mi_cfl_max(mi_domain* d, double* vert, double* horiz) {
   return mi_detail::guard([&] { checked(d)->cfl_max(vert, horiz); });

@@ -433,6 +433,7 @@ struct Domain final : mi_domain {
     if (cp && cp->th_forcing && !thf_buf) thf_buf = add3(thf_buf, St::M, MI_TH_FORCING);
     if (cp && cp->qv_forcing && !qvf_buf) qvf_buf = add3(qvf_buf, St::M, MI_QV_FORCING);
     has_phys = cp != nullptr;
+    a.q.qtf_on = has_phys ? 1 : 0;
     phys = cp ? *cp : mi_column_physics{};
     a.q.use_tendf = (held_base || has_phys) ? 1 : 0;
     if (a.q.use_tendf) {  // every buffer the kernels read or write with held tendencies must exist
@@ -588,6 +589,7 @@ struct Domain final : mi_domain {
     if (h.nx != mine.nx || h.ny != mine.ny || h.nz != mine.nz || h.real_bytes != mine.real_bytes || h.n_sc != mine.n_sc ||
         h.n_fields != mine.n_fields || h.spec != mine.spec || h.n3 != mine.n3 || h.n2 != mine.n2)
       throw Error(MI_EINVAL, "the state buffer belongs to a domain of another shape, precision or scalar count");
+    if (h.payload != mine.payload) throw Error(MI_EINVAL, "the state buffer's header does not match this domain");
     if (bytes < h.payload) throw Error(MI_EINVAL, "state buffer truncated");
     const char* o = static_cast<const char*>(host) + sizeof(h);
     for (const Carried& c : carried()) {
@@ -660,13 +662,23 @@ struct Domain final : mi_domain {
         // the host state is put back before the launch below performs the step
         const Scalars keep = sc;
         hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        step_launches();
+        try {
+          step_launches();
+        } catch (...) {  // end the capture, so the stream takes launches again, and put the host state back
+          hipGraph_t junk = nullptr;
+          (void)hipStreamEndCapture(ctx->stream, &junk);
+          if (junk) (void)hipGraphDestroy(junk);
+          sc = keep;
+          qpar = par;
+          throw;
+        }
         hipGraph_t gr = nullptr;
-        hip_check(hipStreamEndCapture(ctx->stream, &gr), "hipStreamEndCapture");
-        graph[par] = gr;
-        hip_check(hipGraphInstantiate(&exec[par], gr, nullptr, nullptr, 0), "hipGraphInstantiate");
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
         sc = keep;
         qpar = par;
+        hip_check(ec, "hipStreamEndCapture");
+        graph[par] = gr;
+        hip_check(hipGraphInstantiate(&exec[par], gr, nullptr, nullptr, 0), "hipGraphInstantiate");
       }
       hip_check(hipGraphLaunch(exec[par], ctx->stream), "hipGraphLaunch");
       // the host side of the step: three stage swaps of every species' buffers

@@ -1947,8 +1947,9 @@ MI_POINT_KERNEL k_sixth_dyn(Range r, Args a, double coef, int mode) {
   }
 }
 
-// zero_tendf, the caller's tendency, then DiffusionDriver::scalar (diffusion_driver.cpp:297-327; mut there
-// is muts, dt the first stage's)
+// zero_tendf, a column physics plugin's tendency, then DiffusionDriver::scalar (diffusion_driver.cpp:297-327;
+// mut there is muts, dt the first stage's).  Without a plugin zero_tendf leaves q_tendf at 0 (large_step.cpp:814,
+// 888-893): what the caller holds in it is not read.
 MI_POINT_KERNEL k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
   MI_POINT_INDEX(r)
   const Geo& g = a.g;
@@ -1957,7 +1958,7 @@ MI_POINT_KERNEL k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
   const bool in = sixth_in(g, a.q.diff_6th_opt != 0, 'm', i, j);
 #pragma unroll 1
   for (int n = 0; n < n_sc; ++n) {
-    const double t0 = 0.0 + s.qtf[n][x];
+    const double t0 = 0.0 + (a.q.qtf_on ? double(s.qtf[n][x]) : 0.0);
     s.qtw[n][x] = in ? sixth_at<'m'>(t0, a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
   }
 }
@@ -2169,7 +2170,7 @@ MI_POINT_KERNEL k_sc_update(Range r, Args a, Scalars s, SpList l, double dt_rk) 
         f.qvf[x] = tend;
       }
     }
-    tend += acc((a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x]);
+    tend += acc((a.q.use_tendf ? s.qtw[n][x] : real(0)));
     put<kHMass>(s.qn[n], g, i, j, k * g.sk, qdiv(m_old * acc(s.q1[n][x]) + acc(dt_rk) * tend, m_new));
   }
 }
@@ -2185,7 +2186,7 @@ MI_POINT_KERNEL k_sc_pd_pre(Range r, Args a, Scalars s, SpList l, double dt_rk) 
   {
     const int n = l.sp[sq_];
     put<kHMass>(s.q1[n], g, i, j, k * g.sk,
-                qdiv(m * acc(s.q1[n][x]) + acc(dt_rk) * acc((a.q.use_tendf ? s.qtw[n] : s.qtf[n])[x]), m));
+                qdiv(m * acc(s.q1[n][x]) + acc(dt_rk) * acc((a.q.use_tendf ? s.qtw[n][x] : real(0))), m));
   }
 }
 

@@ -101,6 +101,7 @@ struct Params {
   int moist;  // moist species present: calc_cq coefficients and the heating terms are used
   int mapped, cor;     // MapC::mapped (curvature with vxgm), MapC::cor (Coriolis terms)
   int use_tendf;       // held tendencies added (a diffusion driver is attached)
+  int qtf_on;          // a column physics plugin sets the scalars' q_tendf (else zero_tendf leaves it 0)
   int diff_6th_opt;    // 1 linear, 2 monotonic
   double diff_6th_factor;
   double dt;           // the large step

@@ -176,6 +176,12 @@ class Batch:
         for m in self.members:
             m.sync()
 
+    @property
+    def graph_count(self) -> int:
+        n = lib().mi_batch_graph_count(self._h)
+        check(min(n, 0))
+        return n
+
     def close(self):
         if getattr(self, "_h", None):
             lib().mi_batch_destroy(self._h)

@@ -51,7 +51,8 @@ extern "C" {
  * open_xs/xe, open_ys/ye: radiation faces and inflow terms) on either or both, at least 7 columns across */
 enum { MI_BC_PERIODIC = 0, MI_BC_SYMMETRIC = 1, MI_BC_SPECIFIED = 2, MI_BC_OPEN = 3 };
 
-/* arithmetic of a domain: plain FP64 (bit-identical to the reference CPU FP64 build) or FP32-bulk: the
+/* arithmetic of a domain: plain FP64 (bit-identical to the reference CPU FP64 build whose calc_p_rho_phi takes
+ * glibc's scalar pow on every point, i.e. built with CPPWRF_NO_LIBMVEC) or FP32-bulk: the
  * 3-D prognostic, tendency, scalar and work fields stored in FP32 (half the HBM traffic), with FP64 kept
  * for the base state, the 2-D column masses, the full geopotential php, the advance_w tridiagonal solve,
  * the column mass-divergence sums and calc_p_rho; fields cross the ABI as double either way */
@@ -317,6 +318,9 @@ typedef struct mi_batch mi_batch;
 int mi_batch_create(mi_ctx* ctx, mi_domain* const* members, int n, mi_batch** out);
 int mi_batch_destroy(mi_batch* b);
 int mi_batch_step(mi_batch* b, int nsteps);
+/* The graphs a batch holds: at most one per buffer parity of the members' current launch schedules (a
+ * schedule change such as mi_domain_set_dt on a member frees the graphs of the old one). */
+int mi_batch_graph_count(const mi_batch* b);
 
 /* MI_ENONFINITE if any point of u, v, w, t, ph, mu is NaN or infinite (the reference's runner check,
  * src/cases/domain_runner.hpp:70-71); *bad_field receives the first such field id (or -1). */

@@ -645,6 +645,10 @@ struct Dom {
   void rk_tendency(int rk) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (spec) std::fill(mu_tend.a.begin(), mu_tend.a.end(), 0.0);  // zero_tend of mu_tend, every stage (:815-818)
+    // zero_tendf at the first stage without a plugin zeroes the scalars' q_tendf too (large_step.cpp:814,
+    // 888-893): a tendency the caller left there has no effect on the step
+    if (rk == 1 && !has_phys)
+      for (F& f : qtf) std::fill(f.a.begin(), f.a.end(), 0.0);
     if (rk == 1 && use_tendf) {  // zero_tendf, then DiffusionDriver::dynamics (diffusion_driver.cpp:266-281)
       if (!has_phys) {
         for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf, &ph_tendf}) std::fill(f->a.begin(), f->a.end(), 0.0);

@@ -49,3 +49,19 @@ def test_batch_equals_alone(ctx):
         for fid in fids:
             a, m = da.field(fid), dm.field(fid)
             assert np.array_equal(a.view(np.uint64), m.view(np.uint64)), (q, A.FIELD_NAMES[fid])
+
+
+def test_batch_graphs_stay_bounded_under_set_dt(ctx):
+    """an adaptive time step changes a member's schedule every step: the batch keeps the graphs of the current
+    schedules only (one per scalar-buffer parity), not one more per step"""
+    print("[batch] set_dt every step, graph count", flush=True)
+    ms = members(ctx)[2:4]
+    doms = [RK3(ctx, c.grid, p, c) for c, p in ms]
+    b = Batch(ctx, doms)
+    for s in range(8):
+        c, p = ms[0]
+        doms[0].set_dt(c.grid.dt * (1.0 - 0.01 * (s % 3)), 4)
+        b.step(1)
+        assert b.graph_count <= 2, b.graph_count
+    b.sync()
+    assert all(d.check_finite() == -1 for d in doms)

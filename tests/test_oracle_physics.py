@@ -3,7 +3,8 @@ rk3.cpp:100-166 and large_step.cpp:811-823) and the advective forcings RTHFTEN /
 scalars.cpp:147-150, 260-287), checked against what fixes them without their formulas:
  - the hooks run once per step in the reference's order, part1 sees the held tendencies zeroed;
  - a plugin that sets nothing leaves the trajectory bit-identical (the zeroing and the diffusion's order);
- - part2 setting a scalar's tendency every step is bit-identical to holding the same tendency without a plugin;
+ - without a plugin a tendency the caller holds in q_tendf is zeroed at the first stage (no effect), and part2's
+   enters the first stage's transport with the weight dt_rk / m_new;
  - RTHFTEN is the last stage's theta tendency times msfty when nothing else enters it;
  - RQVFTEN closes the vapour budget of the last stage bit for bit, for advect_scalar and advect_scalar_pd."""
 import numpy as np
@@ -67,25 +68,58 @@ def test_empty_plugin_changes_nothing(diff, zeros):
         assert np.array_equal(a.field(fid), b.field(fid)), A.FIELD_NAMES[fid]
 
 
-def test_part2_scalar_tendency_equals_a_held_one():
-    """zero_tendf, then part2's tendency, then the diffusion: the same bits as the caller's held MI_Q_TENDF"""
-    case, prm = _hills(diff_6th_opt=1)
+def _qt(case, prm):
     g = case.grid
     z, y, x = np.meshgrid(np.arange(g.nz), np.arange(g.ny), np.arange(g.nx), indexing="ij")
-    qt = [1e-4 * np.cos(0.3 * x + 0.2 * y + n) * np.exp(-0.2 * z) for n in range(prm.n_scalars)]
+    return [1e-4 * np.cos(0.3 * x + 0.2 * y + n) * np.exp(-0.2 * z) for n in range(prm.n_scalars)]
+
+
+@pytest.mark.parametrize("diff", [0, 1])
+def test_held_scalar_tendency_without_plugin_is_zeroed(diff):
+    """zero_tendf at the first stage zeroes q_tendf when no plugin is attached (large_step.cpp:814, 888-893): a
+    tendency the caller left in MI_Q_TENDF changes no bit of the step"""
+    case, prm = _hills(diff_6th_opt=diff)
+    qt = _qt(case, prm)
     a, b = OracleDomain(case, prm), OracleDomain(case, prm)
     for n in range(prm.n_scalars):
         a.set_field(A.MI_Q_TENDF + n, qt[n])
+    a.step(3)
+    b.step(3)
+    for fid in STATE + tuple(A.MI_Q + n for n in range(prm.n_scalars)):
+        assert np.array_equal(a.field(fid), b.field(fid)), A.FIELD_NAMES[fid]
+    for n in range(prm.n_scalars):
+        assert not np.any(a.field(A.MI_Q_TENDF + n)), "q_tendf left non-zero"
+
+
+def test_part2_scalar_tendency_enters_the_first_stage():
+    """part2's q_tendf (after zero_tendf, before the diffusion) enters rk_update_scalar with the weight
+    dt_rk / m_new of the first stage (scalars.cpp:145-158): two runs that differ only in it differ after the
+    first stage's transport by dt_rk * q_tendf / (c1h (mub + mu) + c2h), to rounding"""
+    case, prm = _hills(diff_6th_opt=1)
+    qt = _qt(case, prm)
+    a, b = OracleDomain(case, prm), OracleDomain(case, prm)
 
     class SetQ:
         def part2(self, d):
             for n in range(prm.n_scalars):
                 d.set_field(A.MI_Q_TENDF + n, qt[n])
     b.set_column_physics(SetQ())
-    a.step(3)
-    b.step(3)
-    for fid in STATE + tuple(A.MI_Q + n for n in range(prm.n_scalars)):
-        assert np.array_equal(a.field(fid), b.field(fid)), A.FIELD_NAMES[fid]
+    for o in (a, b):
+        o.step_begin()
+        o.rk_prep(1)
+        o.rk_tendency(1)
+        o.small_steps(1)
+        o.scalar_transport(1)
+    lv = case.levels
+    nz = case.grid.nz
+    c1h = np.array([lv.c1h[k] for k in range(nz)])[:, None, None]
+    c2h = np.array([lv.c2h[k] for k in range(nz)])[:, None, None]
+    m_new = c1h * (a.field(A.MI_MUB)[None] + a.field(A.MI_MU)[None]) + c2h
+    dt_rk = case.grid.dt / 3.0
+    for n in range(prm.n_scalars):
+        want = dt_rk * qt[n] / m_new
+        got = b.field(A.MI_Q + n) - a.field(A.MI_Q + n)
+        assert np.max(np.abs(got - want)) < 1e-9 * np.max(np.abs(want)) + 1e-18, n
 
 
 def test_rthften_is_the_theta_advection():
