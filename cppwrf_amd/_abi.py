@@ -125,7 +125,7 @@ class MiCaseSpec(C.Structure):
                 ("bubble_zc", C.c_double), ("u0", C.c_double), ("n_hills", C.c_int32), ("moist", C.c_int32),
                 ("hill_height", C.c_double), ("hill_wmin", C.c_double), ("hill_wmax", C.c_double),
                 ("jet_min", C.c_double), ("jet_max", C.c_double), ("map_amp", C.c_double), ("etac", C.c_double),
-                ("p_top", C.c_double)]
+                ("p_top", C.c_double), ("qv0", C.c_double), ("qv_hscale", C.c_double)]
 
 
 def default_params(**kw) -> MiParams:

@@ -26,43 +26,77 @@ constexpr double kG = 9.81, kRd = 287.0, kCp = 7.0 * kRd / 2.0, kCv = kCp - kRd;
 constexpr double kCvpm = -kCv / kCp, kRcp = kRd / kCp, kP1000 = 100000.0, kT0 = 300.0;
 constexpr double kPi = 3.141592653589793;  // std::numbers::pi
 
-// Exner function of height for a given theta(z), hydrostatic, on a table of step dz.
+constexpr double kRvovrd = 461.6 / kRd;  // R_v / R_d (reference constants.hpp)
+
+// Exner function of height for a given theta(z), hydrostatic, on a table of step dz (reference Sounding,
+// src/cases/cases.cpp:17-87).  With a vapour profile qv(z) the integration uses the density potential
+// temperature theta (1 + R_v/R_d qv) / (1 + qv), and the vapour load (the weight of the vapour above z,
+// trapezoid rule from the top) gives the dry pressure pd = p - load.
 class Sounding {
  public:
-  Sounding(std::function<double(double)> th, double p_surf, double zmax, double dz = 0.5) : th_(std::move(th)), dz_(dz) {
+  Sounding(std::function<double(double)> th, double p_surf, double zmax, double dz = 0.5,
+           std::function<double(double)> qv = nullptr)
+      : th_(std::move(th)), qv_(std::move(qv)), dz_(dz) {
     const int n = int(std::ceil(zmax / dz)) + 2;
     pi_.resize(size_t(n));
     pi_[0] = std::pow(p_surf / kP1000, kRcp);
     for (int m = 1; m < n; ++m) {
       const double z0 = (m - 1) * dz, z1 = m * dz;
-      const double f = (1.0 / th_(z0) + 4.0 / th_(0.5 * (z0 + z1)) + 1.0 / th_(z1)) / 6.0;
+      const double f = (1.0 / theta_rho(z0) + 4.0 / theta_rho(0.5 * (z0 + z1)) + 1.0 / theta_rho(z1)) / 6.0;
       pi_[size_t(m)] = pi_[size_t(m - 1)] - kG / kCp * f * dz;
+    }
+    if (qv_) {
+      auto rq = [&](int m) {  // vapour density times g: rho_d qv with rho_d = p / (R_d theta_m pi)
+        const double z = m * dz, pi = pi_[size_t(m)], pp = kP1000 * std::pow(pi, 1.0 / kRcp);
+        return pp / (kRd * th_(z) * (1.0 + kRvovrd * qv_(z)) * pi) * qv_(z);
+      };
+      w_.assign(size_t(n), 0.0);
+      for (int m = n - 2; m >= 0; --m) w_[size_t(m)] = w_[size_t(m + 1)] + 0.5 * kG * dz * (rq(m) + rq(m + 1));
     }
   }
   double theta(double z) const { return th_(z); }
+  double qv(double z) const { return qv_ ? qv_(z) : 0.0; }
+  double theta_rho(double z) const {
+    if (!qv_) return th_(z);
+    const double q = qv_(z);
+    return th_(z) * (1.0 + kRvovrd * q) / (1.0 + q);
+  }
   double p(double z) const {
     const double s = z / dz_;
     const int m = int(std::floor(s));
     if (m < 0 || m + 1 >= int(pi_.size())) throw std::runtime_error("sounding: height out of range");
     const double z0 = m * dz_, h = z - z0;
-    const double f = (1.0 / th_(z0) + 4.0 / th_(z0 + 0.5 * h) + 1.0 / th_(z)) / 6.0;
+    const double f = (1.0 / theta_rho(z0) + 4.0 / theta_rho(z0 + 0.5 * h) + 1.0 / theta_rho(z)) / 6.0;
     const double pi = pi_[size_t(m)] - kG / kCp * f * h;
     return kP1000 * std::pow(pi, 1.0 / kRcp);
   }
-  double z_of_p(double pp) const {
+  double vapour_load(double z) const {
+    if (!qv_) return 0.0;
+    const double s = z / dz_;
+    const int m = int(std::floor(s));
+    if (m < 0 || m + 1 >= int(w_.size())) throw std::runtime_error("sounding: height out of range");
+    const double a = s - m;
+    return (1.0 - a) * w_[size_t(m)] + a * w_[size_t(m + 1)];
+  }
+  double pd(double z) const { return p(z) - vapour_load(z); }
+  double z_of_p(double pp) const { return bisect(pp, false); }
+  double z_of_pd(double pp) const { return bisect(pp, qv_ != nullptr); }
+
+ private:
+  // the height where p (dry: pd) falls to pp, by bisection to 1e-9 m
+  double bisect(double pp, bool dry) const {
+    auto f = [&](double z) { return dry ? pd(z) : p(z); };
     double lo = 0, hi = double(pi_.size() - 2) * dz_;
-    if (pp > p(lo) || pp < p(hi)) throw std::runtime_error("sounding: pressure out of range");
+    if (pp > f(lo) || pp < f(hi)) throw std::runtime_error("sounding: pressure out of range");
     for (int it = 0; it < 200 && hi - lo > 1e-9; ++it) {
       const double mid = 0.5 * (lo + hi);
-      (p(mid) > pp ? lo : hi) = mid;
+      (f(mid) > pp ? lo : hi) = mid;
     }
     return 0.5 * (lo + hi);
   }
-
- private:
-  std::function<double(double)> th_;
+  std::function<double(double)> th_, qv_;
   double dz_;
-  std::vector<double> pi_;
+  std::vector<double> pi_, w_;
 };
 
 uint64_t splitmix64(uint64_t& s) {
@@ -272,6 +306,59 @@ void build(mi_case& c, const mi_case_spec& sp) {
         if (G.bc_x == MI_BC_SYMMETRIC && (i == 0 || i == nx)) uu = 0.0;
         c.u[(size_t(k) * ny + j) * nxu + i] = uu;
       }
+  if (sp.qv0 > 0) {
+    // the full moist state (reference initialize_moist, cases.cpp:290-352; WRF module_initialize_ideal with
+    // use_theta_m = 1): qv(z) = qv0 exp(-z / qv_hscale) from a moist sounding, mu' from the vapour load, p' in
+    // hydrostatic balance with the moisture loading of pg_buoy_w from the top down, theta_m, al and ph
+    const double hq = sp.qv_hscale > 0 ? sp.qv_hscale : 2500.0, qv0 = sp.qv0;
+    const Sounding msnd(theta_env, p_surf, sp.ztop * 1.2 + 1000.0, 0.5, [=](double z) { return qv0 * std::exp(-z / hq); });
+    const double pd_surf = p_surf - msnd.vapour_load(0.0);
+    const double mu1 = pd_surf - c.p_top - mub;
+    std::vector<double> qv(static_cast<size_t>(nz)), thd(static_cast<size_t>(nz));
+    for (int k = 0; k < nz; ++k) {
+      const double p_level = c.c3h[k] * (pd_surf - c.p_top) + c.c4h[k] + c.p_top;
+      const double z = msnd.z_of_pd(p_level);
+      qv[k] = msnd.qv(z);
+      thd[k] = msnd.theta(z);
+    }
+    c.qv.assign(nm, 0.0);
+    std::vector<double> th(static_cast<size_t>(nz));
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const double x = (i + 0.5) * G.dx, y = (j + 0.5) * G.dy;
+        for (int k = 0; k < nz; ++k) th[k] = thd[k] + pert(i, j, k, x, y, zm[k]);
+        auto eos = [&](int k, double pp) {
+          const double qvf = 1.0 + kRvovrd * qv[k];
+          const double alt = (kRd / kP1000) * th[k] * qvf * std::pow((pp + c.pb[k]) / kP1000, kCvpm);
+          c.al[I(i, j, k)] = alt - c.alb[k];
+          c.p[I(i, j, k)] = pp;
+        };
+        int k = nz - 1;
+        double qvf1 = 0.5 * (qv[k] + qv[k]);
+        double qvf2 = 1.0 / (1.0 + qvf1);
+        qvf1 = qvf1 * qvf2;
+        double pp = -0.5 * ((c.c1f[nz] * mu1) + qvf1 * (c.c1f[nz] * mub + c.c2f[nz])) / c.rdnw[k] / qvf2;
+        eos(k, pp);
+        for (k = nz - 2; k >= 0; --k) {
+          qvf1 = 0.5 * (qv[k] + qv[k + 1]);
+          qvf2 = 1.0 / (1.0 + qvf1);
+          qvf1 = qvf1 * qvf2;
+          pp = pp - ((c.c1f[k + 1] * mu1) + qvf1 * (c.c1f[k + 1] * mub + c.c2f[k + 1])) / qvf2 / c.rdn[k + 1];
+          eos(k, pp);
+        }
+        double ph = 0.0;
+        c.ph[(size_t(0) * ny + j) * nx + i] = 0.0;
+        for (k = 0; k < nz; ++k) {
+          ph = ph - c.dnw[k] * (((c.c1h[k] * mub + c.c2h[k]) + c.c1h[k] * mu1) * c.al[I(i, j, k)] + c.c1h[k] * mu1 * c.alb[k]);
+          c.ph[(size_t(k + 1) * ny + j) * nx + i] = ph;
+        }
+        for (k = 0; k < nz; ++k) {
+          c.t[I(i, j, k)] = th[k] * (1.0 + kRvovrd * qv[k]) - kT0;
+          c.qv[I(i, j, k)] = qv[k];
+        }
+        c.mu[size_t(j) * nx + i] = mu1;
+      }
+  }
 }
 
 
@@ -581,6 +668,10 @@ int mi_case_get_field(const mi_case* c, int id, double* h, int64_t n) {
       case MI_ALB: put_profile(c->alb); break;
       case MI_PHB: put_profile(c->phb); break;
       case MI_T_INIT: put_profile(c->t_init); break;
+      case MI_Q:
+        if (c->qv.empty()) throw std::invalid_argument("case: dry, no qv");
+        put(c->qv);
+        break;
       default: throw std::invalid_argument("case: no field " + std::to_string(id));
     }
     (void)nz;
@@ -617,6 +708,7 @@ int mi_case_load(const mi_case* c, mi_domain* d) {
       chk(mi_domain_set_field(d, MI_ALB, c->alb.data(), int64_t(c->alb.size())));
       chk(mi_domain_set_field(d, MI_PHB, c->phb.data(), int64_t(c->phb.size())));
       chk(mi_domain_set_field(d, MI_T_INIT, c->t_init.data(), int64_t(c->t_init.size())));
+      if (!c->qv.empty()) set(MI_Q, c->qv);
     }
     chk(mi_domain_set_field(d, MI_U, c->u.data(), int64_t(c->u.size())));
     chk(mi_domain_set_field(d, MI_V, c->v.data(), int64_t(c->v.size())));

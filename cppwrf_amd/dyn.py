@@ -94,7 +94,7 @@ class Case:
 
     @property
     def moist(self) -> bool:
-        return self.kind == A.MI_CASE_HILLS and bool(self.spec.moist)
+        return (self.kind == A.MI_CASE_HILLS and bool(self.spec.moist)) or self.spec.qv0 > 0
 
     def params(self, **kw) -> A.MiParams:
         if self.kind == A.MI_CASE_HILLS:  # the flags a real-data-shaped domain needs
@@ -104,6 +104,9 @@ class Case:
             if self.spec.moist:
                 kw.setdefault("n_moist", 1)
                 kw.setdefault("n_scalars", max(1, kw.get("n_scalars", 1)))
+        elif self.moist:
+            kw.setdefault("n_moist", 1)
+            kw.setdefault("n_scalars", max(1, kw.get("n_scalars", 1)))
         return A.default_params(**kw)
 
     # the workloads of this library and the reference's idealized cases
@@ -121,9 +124,9 @@ class Case:
         return cls(A.MI_CASE_WARM_BUBBLE, Grid(200, 1, 100, 100.0, 100.0, 1.0, A.MI_BC_SYMMETRIC), 10000.0)
 
     @classmethod
-    def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0, bc=A.MI_BC_PERIODIC, bc_y=None):
+    def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0, bc=A.MI_BC_PERIODIC, bc_y=None, qv0=0.0, qv_hscale=0.0):
         return cls(A.MI_CASE_REST_STATE, Grid(nx, ny, nz, 2000.0, 2000.0, 10.0, bc, bc if bc_y is None else bc_y), 20000.0,
-                   u0=u0)
+                   u0=u0, qv0=qv0, qv_hscale=qv_hscale)
 
     @classmethod
     def hills(cls, nx, ny, nz, dx, dt, seed=1, n_hills=12, hill_height=2500.0, hill_wmin=30000.0, hill_wmax=150000.0,

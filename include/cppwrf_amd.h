@@ -379,6 +379,10 @@ typedef struct mi_case_spec {
   double map_amp;       /* map factors 1 +- map_amp (0.05) */
   double etac;          /* hybrid coordinate: B(eta) = 0 above eta = etac (WRF hybrid_opt 2); 0: sigma */
   double p_top;         /* Pa (hills; the model top is where the sounding reaches p_top) */
+  /* idealized kinds: a moist full state with qv(z) = qv0 exp(-z / qv_hscale) (reference case.qv0 and
+   * case.qv_hscale, cases.cpp:184-188; 0: dry), in hydrostatic balance with its vapour load (scalar 0) */
+  double qv0;
+  double qv_hscale;     /* m (0: 2500) */
 } mi_case_spec;
 typedef struct mi_case mi_case;
 int mi_case_create(const mi_case_spec* spec, mi_case** out);

@@ -100,6 +100,9 @@ class OracleDomain:
                 _ok(olib().orc_set_field(h, fid, _ptr(prof), prof.size))
             mub = case.profile(A.MI_MUB)
             _ok(olib().orc_set_field(h, A.MI_MUB, _ptr(mub), 1))
+            if case.moist:
+                a = case.field(A.MI_Q)
+                _ok(olib().orc_set_field(h, A.MI_Q, _ptr(a), a.size))
         _ok(olib().orc_set_libmvec(h, 1 if libmvec else 0))
         self.nx, self.ny, self.nz = g.nx, g.ny, g.nz
 
