@@ -1,30 +1,28 @@
 #!/usr/bin/env python3
-"""Throughput of the RK3 split-explicit dynamics step on one MI355X, FP32-bulk against plain FP64.
+"""Throughput of the RK3 split-explicit dynamics step on one MI355X, FP32-bulk batched against plain FP64.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--workload hills|bubble64]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 
-Workload "hills" (default): the real-data-shaped 119x89x44 domain at 15 km (seeded hills up to 2.5 km high
-and 30-150 km wide, US standard atmosphere, seeded u/v jets of 5-40 m/s with +-1 m/s noise, map factors
-in [0.95, 1.05], Coriolis, a hybrid coordinate with a 50 hPa top, six moist species advected with the
-positive-definite limiter, qv from 80 % relative humidity falling to 10 % at 10 km), dt 90 s with 4
-acoustic steps, Rayleigh damping of w in the top 5 km.  Its lateral boundaries are specified: a 5-point
-zone (1 spec + 4 relaxation rows) nudged toward boundary data that start at the initial state and drift
-with a seeded 6-hour oscillation of the winds, theta and qv (cppwrf_amd.boundary.drifting_boundary); the
-other moist species take flow-dependent boundary values.
-Workload "bubble64": the 64x64x40 dry warm bubble at 1 km (dt 6 s, 6 acoustic steps, damp_opt 3).
+`value` (the headline) is the batched ensemble in FP32-bulk: 32 members of uneven sizes (nx 60-220, ny 60-180,
+nz 35/44/50, dx 3-15 km; the 3-D warm-bubble recipe with per-member seeds and perturbations, dt 6 s per km of dx,
+6 acoustic steps, damp_opt 3) advanced together, one HIP graph per batch step, in grid-point-steps/s
+(sum of nx*ny*nz over the members x steps / s).  The same ensemble in plain FP64 is timed beside it, with the
+speed-up and the largest per-member relative L2 error of FP32-bulk against FP64 after the same steps.
 
-A step is one RK3 large step (three stages and their acoustic substeps, scalar transport included).  The
-timed region replays the step's HIP graph K times with the state resident in HBM, bracketed by device
-synchronisations; it is measured for the FP64 domain and for the FP32-bulk domain of the same initial
-state.  `value` is the FP32-bulk throughput in grid-point-steps/s (nx*ny*nz*steps/s); the FP64 figure,
-the speed-up and the relative L2 / max error of every variable of the FP32-bulk state against the FP64
-state after the same steps are printed beside it.
+Beside it (cppwrf_amd/workloads.py):
+ - "c3": the 425x300x50 real-data-shaped domain at 3 km (terrain, hybrid coordinate, map factors, six moist
+   species with the positive-definite limiter, sixth-order diffusion, specified boundaries), FP32-bulk and FP64;
+   the roofline of the FP32-bulk step's dominant kernel is measured here (its working set is far beyond the
+   256 MB Infinity Cache): HIP events on the domain's stream around each of its launches during K further steps
+   run launch by launch (events cannot be read back from inside a replayed graph), algorithmic bytes = distinct
+   3-D arrays it reads and writes x their storage size x the points of one launch;
+ - "c2": the 119x89x44 domain at 15 km, FP32-bulk and FP64, with the FP32-bulk error per forecast hour to 6 h;
+ - "serving": a short Poisson trace through the continuous-batching server;
+ - "cpu_baseline": the C++/OpenMP oracle (FP64, bit-identical to the reference CPU build with scalar pow) on this
+   host's cores, a bounded sample of the 119x89x44 workload.
 
-The roofline is that of the dominant kernel of the FP32-bulk step: HIP events recorded on the domain's
-stream around each of its launches during K further steps run launch by launch (events cannot be read
-back from inside a replayed graph); algorithmic bytes = distinct 3-D arrays it reads and writes x their
-storage size x the points of one launch.  The CPU baseline is the C++/OpenMP oracle (the FP64 restatement
-of the reference) on this host's cores, a bounded sample of the same workload.
+A step is one RK3 large step (three stages, their acoustic substeps, scalar transport).  Timed regions replay
+HIP graphs with the state resident in HBM, bracketed by device synchronisations.
 """
 from __future__ import annotations
 
@@ -53,37 +51,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(name):
-    """(case, parameters, description, boundary data or None)"""
-    from cppwrf_amd import _abi as A
-    from cppwrf_amd.boundary import drifting_boundary
-    from cppwrf_amd.dyn import Case
-    if name == "bubble64":
-        case = Case.bubble3d(64, 64, 40, dx=1000.0, ztop=20000.0, dt=6.0, seed=1, noise_amp=0.05)
-        kw = dict(time_step_sound=6, damp_opt=3, zdamp=5000.0, dampcoef=0.2)
-        desc = {"workload": "dry warm bubble 64x64x40, 1 km, 20 km top, periodic, damp_opt 3, dt 6 s",
-                "nx": 64, "ny": 64, "nz": 40, "dx_m": 1000.0, "dt_s": 6.0, "time_step_sound": 6}
-        bdy = None
-    else:
-        case = Case.hills(119, 89, 44, dx=15000.0, dt=90.0, seed=1, bc=A.MI_BC_SPECIFIED)
-        kw = dict(time_step_sound=4, damp_opt=3, zdamp=5000.0, dampcoef=0.2, n_moist=6, n_scalars=6,
-                  moist_adv_opt=1)
-        desc = {"workload": "real-data-shaped 119x89x44 at 15 km: 12 seeded hills, US standard atmosphere, jets, "
-                            "map factors, Coriolis, hybrid coordinate (50 hPa top), 6 moist species with the PD "
-                            "limiter, dt 90 s, 4 acoustic steps; specified lateral boundaries (5-point relaxation "
-                            "zone, seeded slowly varying boundary data)",
-                "nx": 119, "ny": 89, "nz": 44, "dx_m": 15000.0, "dt_s": 90.0, "time_step_sound": 4,
-                "moist_species": 6, "lateral_bc": "specified"}
-        bdy = drifting_boundary(case, seed=17)
-    return case, kw, desc, bdy
-
-
-def measured_traffic(workload_name, kernel, precision):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
-    workload (each counter scaled by its reading for a copy kernel of known size), or None.  The first line of a
-    kernel and precision is its most-dispatched block shape, the autotuned one."""
-    path = os.path.join(HERE, "profiles", "hbm_traffic_hills119x89x44.md")
-    if workload_name != "hills" or not os.path.exists(path):
+def measured_traffic(profile_name, kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of a workload
+    (each counter scaled by its reading for a copy kernel of known size), or None.  The first line of a kernel and
+    precision is its most-dispatched block shape, the autotuned one."""
+    path = os.path.join(HERE, "profiles", profile_name)
+    if not os.path.exists(path):
         return None
     for line in open(path):
         line = line.strip()
@@ -91,8 +64,7 @@ def measured_traffic(workload_name, kernel, precision):
             continue
         kv = dict(x.split("=", 1) for x in line.split()[1:])
         if kv.get("kernel") == kernel and kv.get("precision") == precision:
-            return {"bytes": float(kv["bytes"]), "shape": kv.get("shape"),
-                    "source": "profiles/hbm_traffic_hills119x89x44.md"}
+            return {"bytes": float(kv["bytes"]), "shape": kv.get("shape"), "source": "profiles/" + profile_name}
     return None
 
 
@@ -114,70 +86,110 @@ def timed_steps(rk, steps):
     return time.perf_counter() - t0
 
 
-def error_report(d32, d64, n_moist):
+def ensemble_bench(ctx, steps, warmup, autotune):
+    """the batched ensemble in FP64 and FP32-bulk: timing and the per-member FP32-bulk error"""
     from cppwrf_amd import _abi as A
-    ids = {"U": A.MI_U, "V": A.MI_V, "W": A.MI_W, "T": A.MI_T, "PH": A.MI_PH, "MU": A.MI_MU}
-    if n_moist > 0:
-        ids["QVAPOR"] = A.MI_Q
-    out = {}
-    for name, fid in ids.items():
-        a, b = d32.field(fid), d64.field(fid)
-        nb = float(np.linalg.norm(b))
-        out[name] = {"rel_l2": float(np.linalg.norm(a - b)) / nb if nb > 0 else 0.0, "max_abs": float(np.abs(a - b).max())}
-    return out
-
-
-def ensemble_bench(ctx, members=32, steps=10, seed=2024):
-    """The batched ensemble: `members` domains of uneven sizes (nx 60-220, ny 60-180, nz 35/44/50, dx 3-15 km),
-    each the 3-D warm bubble with its own seed plus +-0.5 K theta and +-1 m/s wind perturbations, dt = 6 s per
-    km of dx, 6 acoustic steps, damp_opt 3; all members advance in one HIP graph per step (mi_batch).  The same
-    ensemble is stepped in FP64 and in FP32-bulk; the FP32 error is the largest relative L2 over the members."""
-    from cppwrf_amd import _abi as A
-    from cppwrf_amd.dyn import Batch, Case, RK3
-    rng = np.random.default_rng(seed)
-    specs = []
-    for m in range(members):
-        nx, ny = int(rng.integers(60, 221)), int(rng.integers(60, 181))
-        nz, dx = int(rng.choice([35, 44, 50])), float(rng.uniform(3000.0, 15000.0))
-        specs.append((nx, ny, nz, dx, int(rng.integers(1 << 30))))
+    from cppwrf_amd import workloads as W
+    from cppwrf_amd.dyn import Batch
+    specs = W.ensemble_specs()
     pts = sum(nx * ny * nz for nx, ny, nz, _, _ in specs)
-    out, states = {"members": members, "steps": steps, "grid_points": pts}, {}
+    out, doms = {"members": len(specs), "grid_points": pts}, {}
     for tag, prec in (("fp64", A.MI_FP64), ("fp32", A.MI_FP32_BULK)):
-        doms = []
-        for nx, ny, nz, dx, sd in specs:
-            case = Case.bubble3d(nx, ny, nz, dx=dx, ztop=20000.0, dt=6.0 * dx / 1000.0, seed=sd, noise_amp=0.05)
-            rk = RK3(ctx, case.grid, case.params(time_step_sound=6, damp_opt=3, zdamp=5000.0, precision=prec), case)
-            prng = np.random.default_rng(sd)
-            for fid, amp in ((A.MI_T, 0.5), (A.MI_U, 1.0), (A.MI_V, 1.0)):
-                f = case.field(fid)
-                rk.set_field(fid, f + prng.uniform(-amp, amp, size=f.shape))
-            doms.append(rk)
-        b = Batch(ctx, doms)
-        b.step(1)
+        ms = [W.ensemble_member(ctx, sp, prec) for sp in specs]
+        if autotune:
+            for d in ms:
+                d.autotune(1)
+        b = Batch(ctx, ms)
+        b.step(warmup)
         b.sync()
         t0 = time.perf_counter()
         b.step(steps)
         b.sync()
         dt = time.perf_counter() - t0
         out[tag] = {"value": pts * steps / dt, "unit": "grid-point-steps/s", "ms_per_step": dt * 1e3 / steps}
-        states[tag] = [{fid: d.field(fid) for fid in (A.MI_U, A.MI_V, A.MI_W, A.MI_T)} for d in doms]
+        log(f"ensemble {tag}: {dt * 1e3 / steps:.3f} ms/step, {pts * steps / dt:.3e} grid-point-steps/s")
         b.close()
-        for d in doms:
-            d.close()
+        doms[tag] = ms
     worst = {}
-    for name, fid in (("U", A.MI_U), ("V", A.MI_V), ("W", A.MI_W), ("T", A.MI_T)):
-        worst[name] = max(float(np.linalg.norm(a[fid] - b[fid]) / np.linalg.norm(b[fid]))
-                          for a, b in zip(states["fp32"], states["fp64"]))
+    for d32, d64 in zip(doms["fp32"], doms["fp64"]):
+        for name, v in W.error_report(d32, d64, False).items():
+            worst[name] = max(worst.get(name, 0.0), v["rel_l2"])
+    for ms in doms.values():
+        for d in ms:
+            d.close()
     out["speedup_fp32_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
     out["fp32_max_rel_l2_over_members"] = worst
+    out["after_steps"] = warmup + steps
     return out
 
 
-def serving_bench(ctx, seconds=20.0, rate=2.0, scale=0.5, seed=1):
-    """The serving trace (Poisson arrivals, domains 50-300 x 50-300 x 35/44/50 at 3-15 km, forecasts of 1-6 h,
+def domain_bench(ctx, name, steps, warmup, autotune, error_hours):
+    """one real-data-shaped domain (c2 or c3) in FP64 and FP32-bulk: timing, the FP32-bulk error after the timed
+    steps and per forecast hour to `error_hours`, and the FP32-bulk domain (for the roofline)"""
+    from cppwrf_amd import _abi as A
+    from cppwrf_amd import workloads as W
+    case, kw, desc, bdy, extra = W.hills_c2() if name == "c2" else W.hills_c3()
+    g = case.grid
+    pts = g.nx * g.ny * g.nz
+    doms, secs = {}, {}
+    for tag, prec in (("fp64", A.MI_FP64), ("fp32", A.MI_FP32_BULK)):
+        rk = W.make_domain(ctx, case, kw, bdy, extra, prec)
+        if autotune:
+            log(f"{name} {tag}: autotuned block shapes", rk.autotune(1))  # restores the state afterwards
+        rk.step(warmup)
+        secs[tag] = timed_steps(rk, steps)
+        log(f"{name} {tag}: {secs[tag] * 1e3 / steps:.3f} ms/step")
+        doms[tag] = rk
+    for tag, rk in doms.items():
+        if rk.check_finite() != -1:
+            raise SystemExit(f"{name} {tag}: non-finite values")
+    out = dict(desc)
+    out["fp32"] = {"value": pts * steps / secs["fp32"], "ms_per_step": secs["fp32"] * 1e3 / steps}
+    out["fp64"] = {"value": pts * steps / secs["fp64"], "ms_per_step": secs["fp64"] * 1e3 / steps}
+    out["speedup_fp32_vs_fp64"] = secs["fp64"] / secs["fp32"]
+    done = warmup + steps
+    out["fp32_error_vs_fp64"] = {"after_steps": done, "per_variable": W.error_report(doms["fp32"], doms["fp64"], True)}
+    hourly = []
+    for hour in range(1, error_hours + 1):
+        target = int(np.ceil(hour * 3600.0 / g.dt - 1e-9))
+        if target > done:
+            for rk in doms.values():
+                rk.step(target - done)
+            done = target
+        if target == done:
+            hourly.append({"hour": hour, "steps": done, "per_variable": W.error_report(doms["fp32"], doms["fp64"], True)})
+    if hourly:
+        out["fp32_error_vs_fp64"]["hourly"] = hourly
+        out["fp32_error_vs_fp64"]["max_rel_l2_last_hour"] = max(v["rel_l2"] for v in hourly[-1]["per_variable"].values())
+    doms["fp64"].close()
+    return out, doms["fp32"]
+
+
+def roofline(rk, steps, profile_name):
+    """the dominant kernel of the FP32-bulk step: event time per step of each candidate over two steps, then its
+    average launch over `steps` steps against the algorithmic bytes of a launch"""
+    per_kernel = {}
+    for name in CANDIDATES:
+        ms, n, _ = profile(rk, name, 2)
+        if n:
+            per_kernel[name] = ms / 2
+    dominant = max(per_kernel, key=per_kernel.get)
+    log("fp32 kernel ms/step:", json.dumps({k: round(v, 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1])}))
+    ev_ms, nl, b = profile(rk, dominant, steps)
+    avg_launch_s = ev_ms / 1e3 / nl
+    achieved = b / nl / avg_launch_s / 1e9
+    traffic = measured_traffic(profile_name, dominant, "fp32")
+    return {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
+            "traffic_source": traffic, "alg_bytes_per_launch": b / nl, "avg_launch_us": avg_launch_s * 1e6,
+            "launches": nl, "precision": "fp32-bulk", "workload": "c3 (425x300x50)",
+            "kernel_ms_per_step": {k: round(v, 4) for k, v in per_kernel.items()}}
+
+
+def serving_bench(ctx, seconds, rate=2.0, scale=1.0, seed=1):
+    """the serving trace (Poisson arrivals, domains 50-300 x 50-300 x 35/44/50 at 3-15 km, forecasts of 1-6 h,
     the warm-bubble recipe per request) replayed in real time through the continuous-batching server in
-    FP32-bulk, horizontal sizes and forecast lengths scaled by `scale`; 5 % of the requests (at least one) are
-    rerun alone in FP64 for the error of the served FP32-bulk results."""
+    FP32-bulk; 5 % of the requests (at least one) are rerun alone in FP64 for the error of the served results"""
     from cppwrf_amd import _abi as A
     from cppwrf_amd.dyn import RK3
     from cppwrf_amd.serving import poisson_trace, run_trace
@@ -202,14 +214,18 @@ def serving_bench(ctx, seconds=20.0, rate=2.0, scale=0.5, seed=1):
     return res
 
 
-def cpu_baseline(case, prm, bdy, budget_s=15.0):
-    """The oracle (C++/OpenMP, bit-identical to the reference CPU FP64 build with scalar pow) on this host's cores."""
+def cpu_baseline(budget_s=15.0):
+    """The oracle (C++/OpenMP, bit-identical to the reference CPU FP64 build with scalar pow) on this host's cores,
+    a bounded sample of the 119x89x44 workload."""
+    from cppwrf_amd import workloads as W
     from oracle_lib import OracleDomain, olib
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     olib().orc_set_threads(threads)
-    orc = OracleDomain(case, prm)
-    if bdy:
-        orc.load_boundary(bdy)
+    case, kw, _, bdy, extra = W.hills_c2()
+    orc = OracleDomain(case, case.params(**kw))
+    orc.load_boundary(bdy)
+    for fid, a in extra.items():
+        orc.set_field(fid, a)
     t = time.perf_counter()
     orc.step(1)
     one = time.perf_counter() - t
@@ -219,120 +235,71 @@ def cpu_baseline(case, prm, bdy, budget_s=15.0):
     dt = time.perf_counter() - t
     g = case.grid
     return {"value": g.nx * g.ny * g.nz * n / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (FP64), {n} steps of the same {g.nx}x{g.ny}x{g.nz} workload after 1 untimed step, "
+            "host_cpus": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "sample": f"oracle (FP64), {n} steps of the 119x89x44 real-data-shaped workload after 1 untimed step, "
                       f"{threads} OpenMP threads"}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["hills", "bubble64"], default="hills")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default block shapes")
-    ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-member batched ensemble")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 119x89x44 domain")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 425x300x50 domain (and the roofline)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving trace")
+    ap.add_argument("--serving-seconds", type=float, default=20.0, help="length of the serving trace")
     ap.add_argument("--error-hours", type=int, default=6,
-                    help="forecast hours over which the FP32-bulk error is reported hourly (0: only after the timed steps)")
+                    help="forecast hours over which the 119x89x44 FP32-bulk error is reported hourly")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each forecast domain runs on a "
               f"single GPU; independent domains shard across GPUs with no communication)", file=sys.stderr)
         sys.exit(2)
 
-    from cppwrf_amd import _abi as A
-    from cppwrf_amd.dyn import Context, RK3
-
-    case, kw, desc, bdy = workload(args.workload)
-    g = case.grid
+    from cppwrf_amd.dyn import Context
     ctx = Context(0)
-    doms = {}
-    for tag, prec in (("fp64", A.MI_FP64), ("fp32", A.MI_FP32_BULK)):
-        rk = RK3(ctx, g, case.params(precision=prec, **kw), case)
-        if bdy:
-            rk.load_boundary(bdy)
-        if not args.no_autotune:
-            log(f"{tag}: autotuned block shapes", rk.autotune(1))  # steps the domain, then restores its state
-        doms[tag] = rk
-    # the same steps on both: warm-up, then the timed region
-    secs = {}
-    for tag, rk in doms.items():
-        rk.step(args.warmup)
-        secs[tag] = timed_steps(rk, args.steps)
-        log(f"{tag}: {secs[tag] * 1e3 / args.steps:.3f} ms/step")
-    for tag, rk in doms.items():
-        bad = rk.check_finite()
-        if bad != -1:
-            raise SystemExit(f"{tag}: non-finite values in field {bad}")
-    err = error_report(doms["fp32"], doms["fp64"], kw.get("n_moist", 0))
-    # the FP32-bulk error per forecast hour up to 6 h: both domains stepped on to each whole hour
-    done = args.warmup + args.steps
-    hourly = []
-    for hour in range(1, args.error_hours + 1):
-        target = int(np.ceil(hour * 3600.0 / g.dt - 1e-9))
-        if target > done:
-            for rk in doms.values():
-                rk.step(target - done)
-            done = target
-        if target == done:
-            hourly.append({"hour": hour, "steps": done,
-                           "per_variable": error_report(doms["fp32"], doms["fp64"], kw.get("n_moist", 0))})
-    pts = g.nx * g.ny * g.nz
-    v64 = pts * args.steps / secs["fp64"]
-    v32 = pts * args.steps / secs["fp32"]
-
-    # the dominant kernel of the FP32-bulk step: event time per step of each candidate over two steps
-    rk = doms["fp32"]
-    per_kernel = {}
-    for name in CANDIDATES:
-        ms, n, _ = profile(rk, name, 2)
-        if n:
-            per_kernel[name] = ms / 2
-    dominant = max(per_kernel, key=per_kernel.get)
-    log("fp32 kernel ms/step:", json.dumps({k: round(v, 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1])}))
-    ev_ms, nl, b = profile(rk, dominant, args.steps)
-    avg_launch_s = ev_ms / 1e3 / nl
-    achieved = b / nl / avg_launch_s / 1e9
-    traffic = measured_traffic(args.workload, dominant, "fp32")
-
+    ens = ensemble_bench(ctx, args.steps, args.warmup, not args.no_autotune)
     out = {
         "metric": METRIC,
-        "value": v32,
+        "value": ens["fp32"]["value"],
         "unit": "grid-point-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": secs["fp32"] * 1e3 / args.steps,
+        "ms_per_step": ens["fp32"]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-bulk (f64 islands)",
         "data": "synthetic",
-        "config": dict(desc, precision="fp32-bulk"),
-        "fp64": {"value": v64, "ms_per_step": secs["fp64"] * 1e3 / args.steps, "dtype": "f64",
-                 "note": "plain FP64, bit-identical to the reference CPU FP64 build with CPPWRF_NO_LIBMVEC (scalar glibc pow in calc_p_rho_phi)"},
-        "speedup_fp32_vs_fp64": v32 / v64,
-        "fp32_error_vs_fp64": {"after_steps": args.warmup + args.steps,
-                               "simulated_s": (args.warmup + args.steps) * g.dt, "per_variable": err,
-                               "hourly": hourly,
-                               "max_rel_l2_6h": max(v["rel_l2"] for v in hourly[-1]["per_variable"].values())
-                               if hourly and hourly[-1]["hour"] == 6 else None},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
-                     "traffic_source": traffic, "alg_bytes_per_launch": b / nl,
-                     "avg_launch_us": avg_launch_s * 1e6, "launches": nl, "precision": "fp32-bulk"},
+        "config": {"workload": "batched ensemble: 32 members of uneven sizes (nx 60-220, ny 60-180, nz 35/44/50, "
+                               "dx 3-15 km), 3-D warm bubble per member seed with +-0.5 K / +-1 m/s perturbations, "
+                               "dt 6 s per km, 6 acoustic steps, damp_opt 3, one HIP graph per batch step",
+                   "members": ens["members"], "grid_points": ens["grid_points"], "precision": "fp32-bulk"},
+        "fp64": {"value": ens["fp64"]["value"], "ms_per_step": ens["fp64"]["ms_per_step"], "dtype": "f64",
+                 "note": "plain FP64, bit-identical to the reference CPU FP64 build with CPPWRF_NO_LIBMVEC (scalar "
+                         "glibc pow in calc_p_rho_phi)"},
+        "speedup_fp32_vs_fp64": ens["speedup_fp32_vs_fp64"],
+        "fp32_error_vs_fp64": {"after_steps": ens["after_steps"],
+                               "max_rel_l2_over_members": ens["fp32_max_rel_l2_over_members"]},
     }
-    for d in doms.values():
-        d.close()
-    if not args.no_ensemble:
-        out["ensemble"] = ensemble_bench(ctx)
-        log("ensemble:", json.dumps(out["ensemble"]))
+    if not args.no_c3:
+        c3, d32 = domain_bench(ctx, "c3", args.steps, args.warmup, not args.no_autotune, 0)
+        out["roofline"] = roofline(d32, args.steps, "hbm_traffic_hills425x300x50.md")
+        d32.close()
+        out["c3"] = c3
+    if not args.no_c2:
+        c2, d32 = domain_bench(ctx, "c2", max(args.steps, 20), args.warmup, not args.no_autotune, args.error_hours)
+        d32.close()
+        out["c2"] = c2
     if not args.no_serving:
-        out["serving"] = serving_bench(ctx)
+        out["serving"] = serving_bench(ctx, args.serving_seconds)
         log("serving:", json.dumps(out["serving"]))
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, case.params(**kw), bdy)
+        out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 
 

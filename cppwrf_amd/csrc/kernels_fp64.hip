@@ -95,29 +95,29 @@ void time_end(hipStream_t st, double bytes) {
   t.bytes += bytes;
 }
 
-// algorithmic bytes of one point of a kernel: distinct 3-D arrays it reads and writes, each of the
-// storage type (8 bytes in FP64, 4 in FP32-bulk)
+// algorithmic bytes of one point of a kernel: distinct 3-D arrays it reads and writes, the bulk ones in the
+// storage type (8 bytes in FP64, 4 in FP32-bulk) and the FP64 islands (php, the tridiagonal coefficients) in 8
 // (pointwise-recomputable inputs and 1-D / 2-D arrays are not counted)
 double point_bytes(const char* name) {
-  struct E { const char* n; int arrays; };
+  struct E { const char* n; int bulk, f64; };
   static const E table[] = {
-      {"large_prep", 11},  // u v w al ph -> ru rv rw alt php ww
-      {"tend_u", 11},      // u ru rv ww p ph alt al php rw -> ru_tend
-      {"tend_v", 11},      // v ru rv ww p ph alt al php rw -> rv_tend
-      {"tend_w", 8},       // w ru rv ww p u v -> rw_tend
-      {"tend_t", 5},       // t ru rv ww -> t_tend
-      {"tend_ph", 6},      // ph ww w u v -> ph_tend
-      {"ss_prep", 25},     // u v w t ph (_1 and _2) p alt ww -> u v w t ph, their 5 saves, c2a ww1
-      {"ss_column", 11},   // ph alt t t_save c2a -> al p pm1 a alpha gamma
-      {"uv", 11},          // u v ru_tend rv_tend p al alt ph php -> u v
-      // u v u_save v_save t_save t_tend t ww1 ww_m ru_m rv_m ph_tend w ph_save ph rw_tend c2a alt a alpha gamma
-      // w_save pm1 -> ww ww_m ru_m rv_m t w ph al p pm1
-      {"acoustic", 34},
-      {"finish", 20},      // u v w t ph ww and the six saves -> u v w t ph ww, al p
-      {"p_rho_phi", 4},    // ph t -> al p
+      {"large_prep", 10, 1},  // u v w al ph -> ru rv rw alt ww, php
+      {"tend_u", 10, 1},      // u ru rv ww p ph alt al rw php -> ru_tend
+      {"tend_v", 10, 1},      // v ru rv ww p ph alt al rw php -> rv_tend
+      {"tend_w", 8, 0},       // w ru rv ww p u v -> rw_tend
+      {"tend_t", 5, 0},       // t ru rv ww -> t_tend
+      {"tend_ph", 6, 0},      // ph ww w u v -> ph_tend
+      {"ss_prep", 25, 0},     // u v w t ph (_1 and _2) p alt ww -> u v w t ph, their 5 saves, c2a ww1
+      {"ss_column", 8, 3},    // ph alt t t_save c2a -> al p pm1, a alpha gamma
+      {"uv", 10, 1},          // u v ru_tend rv_tend p al alt ph php -> u v
+      // u v u_save v_save t_save t_tend t ww1 ww_m ru_m rv_m ph_tend w ph_save ph rw_tend c2a alt w_save pm1 cqw,
+      // a alpha gamma -> ww ww_m ru_m rv_m t w ph al p pm1
+      {"acoustic", 31, 3},
+      {"finish", 20, 0},      // u v w t ph ww and the six saves -> u v w t ph ww, al p
+      {"p_rho_phi", 4, 0},    // ph t -> al p
   };
   for (const E& e : table)
-    if (std::strcmp(e.n, name) == 0) return double(sizeof(real)) * e.arrays;  // bulk arrays in the storage type
+    if (std::strcmp(e.n, name) == 0) return double(sizeof(real)) * e.bulk + 8.0 * e.f64;
   return 0.0;
 }
 
@@ -2596,6 +2596,13 @@ void dispatch_cfg(int cfg, T&&... args) {
   }
 }
 
+// the points a column kernel advances: the substep kernel leaves the spec zone's columns to k_acoustic_ring
+double col_points(const char* name, const Args& a) {
+  double cols = double(a.g.nx) * double(a.g.ny);
+  if (a.g.bcx == MI_BC_SPECIFIED && std::strcmp(name, "acoustic") == 0) cols -= 2.0 * (a.g.nx + a.g.ny) - 4.0;
+  return cols * double(a.g.nz);
+}
+
 // one launch of a column kernel with block shape (CI, KT), timed when selected
 #define MI_COL_LAUNCHER(Name, kern)                                                                              \
   template <int CI, int KT>                                                                                    \
@@ -2608,7 +2615,7 @@ void dispatch_cfg(int cfg, T&&... args) {
       if (t) time_begin(st);                                                                                 \
       hipLaunchKernelGGL((kern<CI, KT>), dim3(unsigned((ncx + CI - 1) / CI), unsigned(ncy), 1), dim3(CI * KT, 1, 1), \
                          lds, st, a, args...);                                                               \
-      if (t) time_end(st, point_bytes(name) * double(a.g.nx) * double(a.g.ny) * double(a.g.nz));              \
+      if (t) time_end(st, point_bytes(name) * col_points(name, a));                                          \
     }                                                                                                          \
   };
 MI_COL_LAUNCHER(LargePrepL, k_large_prep)

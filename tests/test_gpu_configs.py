@@ -1,0 +1,123 @@
+"""The benchmark workloads at their full sizes on the GPU:
+
+ - the real-data-shaped 119x89x44 domain (six moist species with the positive-definite limiter, terrain, hybrid
+   coordinate, map factors, specified boundaries with drifting data): FP64 bit for bit against the oracle over
+   2 steps;
+ - the 425x300x50 domain at 3 km (the same recipe with sixth-order diffusion, specified boundaries, 6 acoustic
+   steps): FP32-bulk against FP64 every simulated hour to 6 h, relative L2 <= 1e-3 for U, V, W, T, PH, MU, QVAPOR;
+ - the 32-member ensemble of uneven sizes: stepped as one batch it gives each member's bits alone, and each
+   member's FP32-bulk state is within 1e-3 relative L2 of its FP64 state after 100 steps;
+ - a slice of the serving trace at full size (domains 50-300 x 50-300, forecasts cut to 20-40 steps): each result
+   is the same alone and in the mix, and FP32-bulk within 1e-3 of an FP64 rerun."""
+import numpy as np
+import pytest
+
+from conftest import first_diff
+from cppwrf_amd import _abi as A
+from cppwrf_amd import workloads as W
+from cppwrf_amd.dyn import Batch, Context, RK3
+from oracle_lib import OracleDomain, olib
+
+pytestmark = pytest.mark.gpu
+
+STATE = (A.MI_U, A.MI_V, A.MI_W, A.MI_T, A.MI_PH, A.MI_MU, A.MI_P, A.MI_AL)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return Context(0)
+
+
+def test_c2_full_size_bitwise(ctx):
+    print("[configs] 119x89x44 hills, 6 species, specified: FP64 vs oracle, 2 steps", flush=True)
+    case, kw, _, bdy, extra = W.hills_c2()
+    prm = case.params(**kw)
+    gpu = W.make_domain(ctx, case, kw, bdy, extra, A.MI_FP64)
+    olib().orc_set_threads(16)
+    orc = OracleDomain(case, prm)
+    orc.load_boundary(bdy)
+    for fid, a in extra.items():
+        orc.set_field(fid, a)
+    gpu.step(2)
+    orc.step(2)
+    for fid in STATE + tuple(A.MI_Q + n for n in range(prm.n_scalars)):
+        a, b = gpu.field(fid), orc.field(fid)
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (A.FIELD_NAMES.get(fid, fid), first_diff(a, b))
+
+
+def test_c3_fp32_error_to_six_hours(ctx):
+    print("[configs] 425x300x50 hills at 3 km: FP32-bulk vs FP64 hourly to 6 h (1200 steps)", flush=True)
+    case, kw, _, bdy, extra = W.hills_c3()
+    d64 = W.make_domain(ctx, case, kw, bdy, extra, A.MI_FP64)
+    d32 = W.make_domain(ctx, case, kw, bdy, extra, A.MI_FP32_BULK)
+    per_hour = int(round(3600.0 / case.grid.dt))
+    for hour in range(1, 7):
+        d64.step(per_hour)
+        d32.step(per_hour)
+        rep = W.error_report(d32, d64, True)
+        print(f"[configs] hour {hour}: " + " ".join(f"{k} {v['rel_l2']:.2e}" for k, v in rep.items()), flush=True)
+        for name, v in rep.items():
+            assert v["rel_l2"] <= 1e-3, (hour, name, v)
+    assert d32.check_finite() == -1 and d64.check_finite() == -1
+
+
+def test_c4_ensemble_batch_and_error(ctx):
+    print("[configs] 32-member ensemble: batch == alone (FP32-bulk, 3 steps); FP32 vs FP64 after 100 steps",
+          flush=True)
+    specs = W.ensemble_specs()
+    batch_m = [W.ensemble_member(ctx, s, A.MI_FP32_BULK) for s in specs]
+    b = Batch(ctx, batch_m)
+    b.step(3)
+    b.sync()
+    for q, s in enumerate(specs):
+        alone = W.ensemble_member(ctx, s, A.MI_FP32_BULK)
+        alone.step(3)
+        for fid in STATE:
+            x, y = alone.field(fid), batch_m[q].field(fid)
+            assert np.array_equal(x.view(np.uint64), y.view(np.uint64)), (q, A.FIELD_NAMES[fid])
+        alone.close()
+    b.step(97)
+    b.close()
+    m64 = [W.ensemble_member(ctx, s, A.MI_FP64) for s in specs]
+    b64 = Batch(ctx, m64)
+    b64.step(100)
+    b64.close()
+    worst = {}
+    for d32, d64 in zip(batch_m, m64):
+        for name, v in W.error_report(d32, d64, False).items():
+            worst[name] = max(worst.get(name, 0.0), v["rel_l2"])
+        d32.close()
+        d64.close()
+    print("[configs] ensemble worst member relative L2 after 100 steps:", worst, flush=True)
+    assert all(v <= 1e-3 for v in worst.values()), worst
+
+
+def test_c5_slice_alone_and_mixed(ctx):
+    print("[configs] serving slice at full size: alone == mixed, FP32 vs FP64 reruns", flush=True)
+    from cppwrf_amd.serving import ForecastServer, poisson_trace
+    reqs = poisson_trace(seconds=2.5, rate=2.0, seed=11, scale=1.0, precision=A.MI_FP32_BULK)[:5]
+    for q, r in enumerate(reqs):
+        r.steps = 20 + 5 * q
+    srv = ForecastServer(ctx, hbm_budget=64 << 30, clock=lambda: 0.0)
+    for q, r in enumerate(reqs):
+        srv.submit(r)
+        srv.tick(float(q))  # each joins at its own step boundary
+    t = len(reqs)
+    while any(r.taken < r.steps for r in reqs):
+        srv.tick(float(t))
+        t += 1
+    srv.tick(float(t))
+    for r in reqs:
+        case = r.case()
+        alone = RK3(ctx, case.grid, r.params(case), case)
+        alone.step(r.steps)
+        for fid in STATE:
+            x, y = alone.field(fid), r.domain.field(fid)
+            assert np.array_equal(x.view(np.uint64), y.view(np.uint64)), (r.rid, A.FIELD_NAMES[fid])
+        r.precision = A.MI_FP64
+        d64 = RK3(ctx, case.grid, r.params(case), case)
+        d64.step(r.steps)
+        for name, v in W.error_report(alone, d64, False).items():
+            assert v["rel_l2"] <= 1e-3, (r.rid, name, v)
+        alone.close()
+        d64.close()
