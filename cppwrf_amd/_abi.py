@@ -158,6 +158,12 @@ class MiFieldView(C.Structure):
                 ("stride_k", C.c_int64), ("stride_j", C.c_int64)]
 
 
+class MiFieldLayout(C.Structure):
+    """mi_field_layout"""
+    _fields_ = [("elem_bytes", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("stride_k", C.c_int64), ("stride_j", C.c_int64), ("offset", C.c_int64), ("alloc_elems", C.c_int64)]
+
+
 # mi_phys_fn (user, domain)
 PHYS_FN = C.CFUNCTYPE(None, _VP, _VP)
 
@@ -209,6 +215,12 @@ SIGNATURES = {
     "mi_domain_set_first_rk_prep_callback": (_I, [_VP, STAGE_FN, _VP]),
     "mi_domain_set_column_physics": (_I, [_VP, C.POINTER(MiColumnPhysics)]),
     "mi_domain_field_view": (_I, [_VP, _I, C.POINTER(MiFieldView)]),
+    "mi_domain_field_layout": (_I, [_VP, _I, C.POINTER(MiFieldLayout)]),
+    "mi_field_bind": (_I, [_VP, _I, _VP, _I64]),
+    "mi_device_alloc": (_I, [_VP, _I64, C.POINTER(_VP)]),
+    "mi_device_free": (_I, [_VP, _VP]),
+    "mi_device_upload": (_I, [_VP, _VP, _VP, _I64]),
+    "mi_device_download": (_I, [_VP, _VP, _VP, _I64]),
     "mi_check_finite": (_I, [_VP, C.POINTER(_I)]),
     "mi_domain_autotune": (_I, [_VP, _I]),
     "mi_domain_get_tuning": (_I, [_VP, C.POINTER(_I), _I]),

@@ -26,7 +26,7 @@ HIP_UNITS = [(src, f"{src}.{tag}.o", defs) for src in ("kernels_fp64.hip", "doma
 HOST_SOURCES = ["cases.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
+COMMON = ["-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
           "-Wno-unused-parameter"]
 HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                       "-munsafe-fp-atomics"]

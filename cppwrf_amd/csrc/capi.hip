@@ -201,6 +201,53 @@ int mi_domain_field_view(mi_domain* d, int id, mi_field_view* out) {
   });
 }
 
+int mi_domain_field_layout(const mi_domain* d, int id, mi_field_layout* out) {
+  return mi_detail::guard([&] {
+    if (!d || !out) throw Error(MI_EINVAL, "null argument");
+    d->field_layout(id, out);
+  });
+}
+
+int mi_field_bind(mi_domain* d, int id, void* dev, int64_t bytes) {
+  return mi_detail::guard([&] { checked(d)->field_bind(id, dev, bytes); });
+}
+
+int mi_device_alloc(mi_ctx* c, int64_t bytes, void** out) {
+  return mi_detail::guard([&] {
+    if (!c || !out || bytes <= 0) throw Error(MI_EINVAL, "bad argument");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    *out = nullptr;
+    hip_check(hipMalloc(out, size_t(bytes)), "hipMalloc");
+  });
+}
+
+int mi_device_free(mi_ctx* c, void* p) {
+  return mi_detail::guard([&] {
+    if (!c) throw Error(MI_EINVAL, "null argument");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (p) hip_check(hipFree(p), "hipFree");
+  });
+}
+
+int mi_device_upload(mi_ctx* c, void* dst, const void* src, int64_t n) {
+  return mi_detail::guard([&] {
+    if (!c || !dst || !src || n < 0) throw Error(MI_EINVAL, "bad argument");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hip_check(hipMemcpyAsync(dst, src, size_t(n), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
+int mi_device_download(mi_ctx* c, void* dst, const void* src, int64_t n) {
+  return mi_detail::guard([&] {
+    if (!c || !dst || !src || n < 0) throw Error(MI_EINVAL, "bad argument");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hip_check(hipMemcpyAsync(dst, src, size_t(n), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  });
+}
+
 int mi_domain_get_field(mi_domain* d, int id, double* host, int64_t n) {
   return mi_detail::guard([&] {
     if (!host) throw Error(MI_EINVAL, "null argument");

@@ -613,7 +613,7 @@ struct Domain final : mi_domain {
     for (int kern = 0; kern < kNumColKernels; ++kern) {
       double best_t = 1e300;
       int best = colcfg[kern];
-      for (int cfg = 0; cfg < kNumColCfgs; ++cfg) {
+      for (int cfg = 0; cfg <= kColMarch; ++cfg) {
         if (!col_cfg_fits(kern, cfg, grid.nz)) continue;
         colcfg[kern] = cfg;
         prof_name = kColKernelNames[kern];
@@ -696,6 +696,8 @@ struct Domain final : mi_domain {
   void set_field(int id, const double* host, int64_t n) override;
   void get_field(int id, double* host, int64_t n) override;
   void field_view(int id, mi_field_view* out) override;
+  void field_layout(int id, mi_field_layout* out) const override;
+  void field_bind(int id, void* dev, int64_t bytes) override;
   void phase(MiPhase which, int rk) override;
   void set_tuning(const int* shapes, int n) override;
   void get_tuning(int* shapes, int n) const override;
@@ -836,6 +838,10 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   add3(f.u_save, St::U, -1); add3(f.v_save, St::V, -1); add3(f.w_save, St::W, -1); add3(f.t_save, St::M, -1);
   add3(f.ph_save, St::W, -1); add3(f.ww1, St::W, -1); add3(f.c2a, St::M, -1); add3(f.pm1, St::M, -1);
   add3(f.ca, St::W, -1); add3(f.calpha, St::W, -1); add3(f.cgamma, St::W, -1);
+  if (sizeof(real) == 4) {  // the scratch of the FP32-bulk substep k-march
+    add3(f.ky, St::W, -1);
+    add3(f.kr, St::W, -1);
+  }
   add2(f.mu_save, St::M2, -1); add2(f.mudf, St::M2, -1); add2(f.muave, St::M2, -1);
   add2(f.mub, St::M2, MI_MUB);
   add2(f.msftx, St::M2, MI_MSFTX); add2(f.msfty, St::M2, MI_MSFTY);
@@ -1045,6 +1051,67 @@ void Domain::field_view(int id, mi_field_view* out) {
   out->nz = two ? 1 : nzs(fd->st);
   out->stride_k = two ? 0 : a.g.sk;
   out->stride_j = a.g.sj;
+}
+
+// the fields a caller may give storage of its own: the carried state, held tendencies, 2-D base fields and
+// map factors, and the 3-D base fields of a domain with terrain (not the scalars, whose buffers swap)
+static bool bindable(int id) {
+  return (id >= MI_U && id <= MI_AL) || (id >= MI_RU_TENDF && id <= MI_T_TENDF) || id == MI_MUB ||
+         (id >= MI_MSFTX && id <= MI_HT) || id == MI_PB || id == MI_ALB || id == MI_PHB || id == MI_T_INIT;
+}
+
+void Domain::field_layout(int id, mi_field_layout* out) const {
+  const FieldDesc* fd = nullptr;
+  for (const FieldDesc& f : fields)
+    if (f.id == id) fd = &f;
+  if (!fd || !bindable(id)) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " has no bindable storage");
+  const bool two = fd->st == St::M2 || fd->st == St::U2 || fd->st == St::V2;
+  out->elem_bytes = fd->esize;
+  out->nx = nxs(fd->st);
+  out->ny = nys(fd->st);
+  out->nz = two ? 1 : nzs(fd->st);
+  out->stride_k = two ? 0 : a.g.sk;
+  out->stride_j = a.g.sj;
+  out->offset = off;
+  out->alloc_elems = two ? n2 : n3;
+}
+
+void Domain::field_bind(int id, void* dev, int64_t nbytes) {
+  mi_field_layout lay{};
+  field_layout(id, &lay);
+  FieldDesc* fd = find(this, id);
+  const int64_t need = lay.alloc_elems * lay.elem_bytes;
+  hipStream_t st = ctx->stream;
+  char* old_base = static_cast<char*>(*fd->slot) - off * fd->esize;
+  char* new_base = static_cast<char*>(dev);
+  if (dev) {
+    if (nbytes < need) throw Error(MI_EINVAL, "bound buffer too small: " + std::to_string(need) + " bytes needed");
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, dev) != hipSuccess || (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged)) {
+      (void)hipGetLastError();
+      throw Error(MI_EINVAL, "bound buffer is not device memory");
+    }
+    if (at.device != ctx->device) throw Error(MI_EINVAL, "bound buffer is on another device");
+  } else {  // storage of the domain's own again, with the bound values
+    if (std::find(allocs.begin(), allocs.end(), static_cast<void*>(old_base)) != allocs.end()) return;  // not bound
+    new_base = static_cast<char*>(static_cast<void*>(alloc<char>(need)));
+    hip_check(hipMemcpyAsync(new_base, old_base, size_t(need), hipMemcpyDeviceToDevice, st), "hipMemcpyAsync");
+  }
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  auto own = std::find(allocs.begin(), allocs.end(), static_cast<void*>(old_base));
+  if (own != allocs.end() && static_cast<void*>(old_base) != static_cast<void*>(new_base)) {  // the domain's own
+    hip_check(hipFree(old_base), "hipFree");
+    allocs.erase(own);
+    bytes -= need;
+  }
+  *fd->slot = new_base + off * fd->esize;
+  // the base fields of a terrain domain are also read through their BField views
+  if (pb_buf && id == MI_PB) a.f.pb.p = pb_buf;
+  if (alb_buf && id == MI_ALB) a.f.alb.p = alb_buf;
+  if (phb_buf && id == MI_PHB) a.f.phb.p = phb_buf;
+  halo({{*fd->slot, -1, -1}});
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  drop_graphs();  // the captured steps name the old buffer
 }
 
 void Domain::get_field(int id, double* host, int64_t n) {

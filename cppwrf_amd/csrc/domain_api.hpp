@@ -33,6 +33,8 @@ struct mi_domain {
   virtual void set_field(int id, const double* host, int64_t n) = 0;
   virtual void get_field(int id, double* host, int64_t n) = 0;
   virtual void field_view(int id, mi_field_view* out) = 0;
+  virtual void field_layout(int id, mi_field_layout* out) const = 0;
+  virtual void field_bind(int id, void* dev, int64_t bytes) = 0;
   virtual void step(int nsteps) = 0;
   virtual void phase(MiPhase which, int rk) = 0;
   virtual void autotune(int trials) = 0;

@@ -242,6 +242,17 @@ class RK3:
         check(lib().mi_domain_field_view(self._h, fid, C.byref(v)))
         return v
 
+    def field_layout(self, fid: int) -> "A.MiFieldLayout":
+        """the layout a caller-owned buffer of the field must have (mi_domain_field_layout)"""
+        v = A.MiFieldLayout()
+        check(lib().mi_domain_field_layout(self._h, fid, C.byref(v)))
+        return v
+
+    def bind(self, fid: int, dev_ptr, nbytes: int) -> None:
+        """make a caller-owned device allocation the field's storage (mi_field_bind); None gives the field storage
+        of the domain's own again"""
+        check(lib().mi_field_bind(self._h, fid, dev_ptr, int(nbytes)))
+
     def set_column_physics(self, cp) -> None:
         """RK3::set_column_physics (reference src/dyn/column_physics.hpp:21-34): `cp` has the methods part1(rk3),
         part2(rk3), after_rk(rk3), after_all_rk_steps(rk3) (any may be missing) and the flags th_forcing,

@@ -200,6 +200,34 @@ typedef struct mi_field_view {
 } mi_field_view;
 int mi_domain_field_view(mi_domain* d, int field, mi_field_view* out);
 
+/* Caller-owned device buffers (the reference's State / Tendencies / BaseState own their Field3 buffers and RK3
+ * only steps them, src/dyn/rk3.hpp:34-78, src/core/field.hpp:76-143).  mi_domain_field_layout gives the layout
+ * a field's buffer must have: element size, logical points, strides (elements), the offset of point (0,0,0)
+ * from the start of the allocation and the allocation's size in elements (halos included).
+ * mi_field_bind makes a caller's device allocation of at least alloc_elems * elem_bytes bytes, laid out so,
+ * the storage of the field: the domain reads and writes it from then on and never frees it (the caller keeps
+ * it alive while the domain may step, and may read or write it between steps on the context's stream).  Its
+ * contents become the field's values (the halo points are refreshed from the interior at the bind).  Binding
+ * NULL gives the field storage of the domain's own again, with the bound buffer's values copied in.
+ * Bindable: the carried state (MI_U .. MI_AL), the held tendencies, the 2-D base fields and map factors, and
+ * the 3-D base fields of a domain with terrain.  Not the scalars (their two buffers swap at every stage) nor
+ * the base profiles of a flat domain (MI_EINVAL). */
+typedef struct mi_field_layout {
+  int32_t elem_bytes;
+  int32_t nx, ny, nz;
+  int64_t stride_k, stride_j;
+  int64_t offset;      /* elements from the start of the allocation to point (0,0,0) */
+  int64_t alloc_elems; /* elements of the allocation, halos included */
+} mi_field_layout;
+int mi_domain_field_layout(const mi_domain* d, int field, mi_field_layout* out);
+int mi_field_bind(mi_domain* d, int field, void* dev_alloc, int64_t bytes);
+/* device memory for such buffers, on the context's device (hipMalloc / hipFree) */
+int mi_device_alloc(mi_ctx* c, int64_t bytes, void** out);
+int mi_device_free(mi_ctx* c, void* p);
+/* host <-> device copies of n bytes on the context's stream, complete when they return */
+int mi_device_upload(mi_ctx* c, void* dst, const void* src, int64_t n);
+int mi_device_download(mi_ctx* c, void* dst, const void* src, int64_t n);
+
 /* reference RK3::step (src/dyn/rk3.cpp:86-170): advance nsteps large steps.  Asynchronous on the
  * context's stream; mi_domain_sync waits. */
 int mi_step(mi_domain* d, int nsteps);

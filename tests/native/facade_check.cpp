@@ -41,6 +41,18 @@ int main() {
   if (dyn::field_size(MI_RU_TENDF, g3) != 17 * 12 * 10 || dyn::field_size(MI_RW_TENDF, g3) != 16 * 12 * 11 ||
       dyn::field_size(MI_TH_FORCING, g3) != 16 * 12 * 10)
     return 14;
+  // the LateralBC object holds its data host side until an RK3 takes it (set_lateral_bc)
+  dyn::LateralBC bc(g3, 6.0);
+  if (bc.has(dyn::LateralBC::U) || bc.spec_zone() != 1 || bc.width() != 5) return 15;
+  bc.set(dyn::LateralBC::U, dyn::LateralBC::XS, std::vector<double>(12 * 10 * 5, 1.0), std::vector<double>(12 * 10 * 5, 0.0));
+  if (!bc.has(dyn::LateralBC::U)) return 16;
+  bc.advance_time(6.0);
+  if (bc.dtbc != 6.0) return 17;
+  try {
+    dyn::LateralBC bad(g3, 6.0, 6);
+    return 18;
+  } catch (const std::runtime_error&) {
+  }
   try {
     dyn::Context ctx(0);
     // a GPU is present: a domain with an unsupported advection order must be refused

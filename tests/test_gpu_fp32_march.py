@@ -1,0 +1,72 @@
+"""The FP32-bulk substep k-march (k_acoustic_march: one thread per column, FP32 bulk arithmetic, FP64 column sums,
+tridiagonal sweeps and p'', al'') against the FP64 path and against the FP32-bulk substep kernel that keeps the
+FP64 arithmetic: every variable within 1e-3 relative L2 of FP64, on periodic, open, specified (moist, terrain,
+diffusion) and x-z grids, with top_lid and damp_opt 3."""
+import numpy as np
+import pytest
+
+from cppwrf_amd import _abi as A
+from cppwrf_amd.boundary import drifting_boundary
+from cppwrf_amd.dyn import Case, Context, RK3
+
+pytestmark = pytest.mark.gpu
+
+K_MARCH = 12  # the substep kernel's k-march shape (kColMarch)
+VARS = {"U": A.MI_U, "V": A.MI_V, "W": A.MI_W, "T": A.MI_T, "PH": A.MI_PH, "MU": A.MI_MU}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return Context(0)
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def _cases():
+    return {
+        "bubble_periodic": (lambda: Case.bubble3d(70, 40, 40, dx=1000.0, ztop=20000.0, dt=6.0, seed=1),
+                            dict(time_step_sound=6, damp_opt=3, zdamp=5000.0), 60, None),
+        "bubble_open_lid": (lambda: Case.bubble3d(66, 30, 24, dx=1000.0, ztop=10000.0, dt=6.0, seed=2,
+                                                  bc=A.MI_BC_OPEN, bubble_zc=3000.0),
+                            dict(time_step_sound=6, top_lid=1), 40, None),
+        "xz_symmetric": (lambda: Case.bubble3d(130, 1, 30, dx=500.0, ztop=10000.0, dt=3.0, seed=3,
+                                               bc=A.MI_BC_SYMMETRIC, bubble_zc=3000.0),
+                         dict(time_step_sound=6, damp_opt=3, zdamp=3000.0), 60, None),
+        "hills_specified_moist": (lambda: Case.hills(70, 50, 30, dx=15000.0, dt=90.0, seed=4, bc=A.MI_BC_SPECIFIED),
+                                  dict(time_step_sound=4, damp_opt=3, zdamp=5000.0, n_moist=3, n_scalars=3,
+                                       diff_6th_opt=1), 40, True),
+    }
+
+
+@pytest.mark.parametrize("name", list(_cases()))
+def test_march_against_fp64(ctx, name):
+    make, kw, steps, spec = _cases()[name]
+    print(f"[march] {name}: {steps} steps", flush=True)
+    case = make()
+    bdy = drifting_boundary(case, seed=5) if spec else None
+    doms = {}
+    for tag, prec, shape in (("fp64", A.MI_FP64, None), ("fp32", A.MI_FP32_BULK, 4), ("march", A.MI_FP32_BULK, K_MARCH)):
+        d = RK3(ctx, case.grid, case.params(precision=prec, **kw), case)
+        if bdy:
+            d.load_boundary(bdy)
+        if shape is not None:
+            t = d.tuning
+            t[2] = shape
+            d.tuning = t
+            assert d.tuning[2] == shape
+        d.step(steps)
+        assert d.check_finite() == -1, tag
+        doms[tag] = d
+    ids = dict(VARS)
+    if kw.get("n_moist"):
+        ids["QVAPOR"] = A.MI_Q
+    for vname, fid in ids.items():
+        b = doms["fp64"].field(fid)
+        e_m = rel_l2(doms["march"].field(fid), b)
+        e_f = rel_l2(doms["fp32"].field(fid), b)
+        print(f"[march] {name} {vname}: k-march {e_m:.2e}, fp32 substep kernel {e_f:.2e}", flush=True)
+        assert e_m <= 1e-3, (vname, e_m)
+        assert e_m <= max(10.0 * e_f, 1e-6), (vname, e_m, e_f)
