@@ -35,6 +35,10 @@ import time
 
 import numpy as np
 
+# the CPU baseline's OpenMP threads pinned one per core, close together: set before any OpenMP runtime loads
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(HERE, "tests"))
@@ -214,6 +218,30 @@ def serving_bench(ctx, seconds, rate=2.0, scale=1.0, seed=1):
     return res
 
 
+def physical_cores():
+    """distinct (package, core) pairs of /proc/cpuinfo, or None"""
+    try:
+        cores, phys = set(), None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                phys = line.split(":")[1].strip()
+            elif line.startswith("core id"):
+                cores.add((phys, line.split(":")[1].strip()))
+        return len(cores) or None
+    except OSError:
+        return None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(budget_s=15.0):
     """The oracle (C++/OpenMP, bit-identical to the reference CPU FP64 build with scalar pow) on this host's cores,
     a bounded sample of the 119x89x44 workload."""
@@ -235,7 +263,9 @@ def cpu_baseline(budget_s=15.0):
     dt = time.perf_counter() - t
     g = case.grid
     return {"value": g.nx * g.ny * g.nz * n / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
-            "host_cpus": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "host_cpus": os.cpu_count(), "physical_cores": physical_cores(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
+            "cpu_model": cpu_model(), "oracle_flags": "-O3 -march=x86-64-v3 -ffp-contract=off -fopenmp",
             "sample": f"oracle (FP64), {n} steps of the 119x89x44 real-data-shaped workload after 1 untimed step, "
                       f"{threads} OpenMP threads"}
 

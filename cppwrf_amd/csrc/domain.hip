@@ -838,10 +838,8 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   add3(f.u_save, St::U, -1); add3(f.v_save, St::V, -1); add3(f.w_save, St::W, -1); add3(f.t_save, St::M, -1);
   add3(f.ph_save, St::W, -1); add3(f.ww1, St::W, -1); add3(f.c2a, St::M, -1); add3(f.pm1, St::M, -1);
   add3(f.ca, St::W, -1); add3(f.calpha, St::W, -1); add3(f.cgamma, St::W, -1);
-  if (sizeof(real) == 4) {  // the scratch of the FP32-bulk substep k-march
-    add3(f.ky, St::W, -1);
-    add3(f.kr, St::W, -1);
-  }
+  add3(f.ky, St::W, -1);  // the scratch of the substep k-march
+  add3(f.kr, St::W, -1);
   add2(f.mu_save, St::M2, -1); add2(f.mudf, St::M2, -1); add2(f.muave, St::M2, -1);
   add2(f.mub, St::M2, MI_MUB);
   add2(f.msftx, St::M2, MI_MSFTX); add2(f.msfty, St::M2, MI_MSFTY);

@@ -1168,6 +1168,58 @@ MI_COLUMN_KERNEL k_ss_column(Args a, double cof) {
 }
 
 // ----------------------------------------------------------------------------- advance_uv
+// FP32-bulk: the same update with the bulk arithmetic in FP32 (one multiply by a rounded reciprocal per division);
+// the differences of the full geopotential php and of the base pressure pb are taken in FP64 first
+template <bool OPEN>
+__device__ __forceinline__ void uv_fp32(const Args& a, int i, int j, int k, double dts_d) {
+  using F = float;
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nz = g.nz;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), sk = g.sk;
+  const F dts = F(dts_d);
+  auto dpn = [&](idx_t d, int kk) -> F {
+    auto ps = [&](int m) { const idx_t y = g.at(i, j, m); return F(f.p[y]) + F(f.p[y - d]); };
+    if (kk == 0) return F(0.5) * (F(c.cf1) * ps(0) + F(c.cf2) * ps(1) + F(c.cf3) * ps(2));
+    if (kk == nz) return a.q.top_lid ? F(0.5) * (F(c.cf1) * ps(nz - 1) + F(c.cf2) * ps(nz - 2) + F(c.cf3) * ps(nz - 3)) : F(0);
+    return F(0.5) * (F(c.fnm[kk]) * ps(kk) + F(c.fnp[kk]) * ps(kk - 1));
+  };
+  auto dp = [&](idx_t d, int di, int dj, double mface, double rd, double rm) -> F {
+    const F c1 = F(c.c1h[k]);
+    F dd = F(rm * 0.5 * rd) * (c1 * F(mface) + F(c.c2h[k])) *
+           (((F(f.ph2[x + sk]) - F(f.ph2[x + sk - d])) + (F(f.ph2[x]) - F(f.ph2[x - d]))) +
+            (F(f.alt[x]) + F(f.alt[x - d])) * (F(f.p[x]) - F(f.p[x - d])) +
+            (F(f.al[x]) + F(f.al[x - d])) * F(f.pb(i, j, k) - f.pb(i - di, j - dj, k)));
+    dd += F(rm * rd) * F(f.php[x] - f.php[x - d]) *
+          (F(c.rdnw[k]) * (dpn(d, k + 1) - dpn(d, k)) - F(0.5) * c1 * (F(f.mu2[x2 - d]) + F(f.mu2[x2])));
+    return dd;
+  };
+  const int nx = g.nx, ny = g.ny;
+  const bool sp = g.spec();
+  const bool uface = sp && (i <= 0 || i >= nx || j == 0 || j == ny - 1);
+  const bool vface = sp && (i == 0 || i == nx - 1 || j <= 0 || j >= ny);
+  const int ui0 = g.bcx != MI_BC_PERIODIC ? 1 : 0;
+  if ((uface || ((OPEN && g.ox()) && (i == 0 || i == nx))) && j < ny) {
+    f.u2[x] = real(F(f.u2[x]) + dts * F(f.ru_tend[x]));
+  } else if (!uface && i >= ui0 && i < nx && j < ny) {
+    const F mudf_x = F(-a.q.emdx * (f.mudf[x2] - f.mudf[x2 - 1]) / f.msfuy[x2]);
+    const F cq = a.q.moist ? F(f.cqu[x]) : F(1);
+    f.u2[x] = real(F(f.u2[x]) + dts * F(f.ru_tend[x]) - dts * cq * dp(1, 1, 0, f.muu[x2], c.rdx, f.msfux[x2] / f.msfuy[x2]) +
+                   F(c.c1h[k]) * mudf_x);
+  }
+  const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
+  if ((vface || ((OPEN && g.oy()) && (j == 0 || j == ny))) && i < nx) {
+    f.v2[x] = real(F(f.v2[x]) + dts * F(f.rv_tend[x]));
+  } else if (!vface && !g.two_d && j >= vj0 && j < ny && i < nx) {
+    const idx_t sj = g.sj;
+    const F mudf_y = F(-a.q.emdy * (f.mudf[x2] - f.mudf[x2 - sj]) * f.msfvx_inv[x2]);
+    const F cq = a.q.moist ? F(f.cqv[x]) : F(1);
+    f.v2[x] = real(F(f.v2[x]) + dts * F(f.rv_tend[x]) - dts * cq * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) +
+                   F(c.c1h[k]) * mudf_y);
+  }
+}
+
 // KUV (small_step.cpp:151-190)
 template <bool OPEN>
 MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
@@ -1177,6 +1229,10 @@ MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
   const Lv& c = a.c;
   const int nz = g.nz;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), sk = g.sk;
+  if constexpr (sizeof(real) == 4) {
+    uv_fp32<OPEN>(a, i, j, k, dts);
+    return;
+  }
   auto dpn = [&](idx_t d, int kk) {
     auto ps = [&](int m) { const idx_t y = g.at(i, j, m); return f.p[y] + f.p[y - d]; };
     if (kk == 0) return H * (c.cf1 * ps(0) + c.cf2 * ps(1) + c.cf3 * ps(2));
@@ -1608,22 +1664,20 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
   }
 }
 
-// ----------------------------------------------------------------------------- one acoustic substep, FP32-bulk k-march
-// The substep's column work (the same phases as k_acoustic) with one thread per column: a wavefront holds 64
-// consecutive columns of one row and marches up and down them, so every load and store of the wavefront is one
-// i-contiguous 256-byte run of a row and no lane idles on a level the column does not have.  No LDS and no
-// barriers: a column's recurrences are serial in its own thread, its vertical neighbours slide through
-// registers.  Arithmetic is FP32 for the bulk (the divergence, theta'', the time averages, the explicit phi'' and
-// w'', t_2ave) and FP64 for the islands: the column mass sums (dmdt, the omega'' integral), the tridiagonal sweeps
-// with the phi'' back-substitution, and p'', al''.  Three passes over the column:
-//   1 (up)    the mass divergence of every level and its FP64 column sum: mu'', mudf, muts, muave
-//   2 (up)    omega'' (the running FP64 integral), ww_m, ru_m, rv_m, theta'', t_2ave, the explicit phi'' R and w'',
-//             the forward sweep of the tridiagonal solve (FP64), whose values y and R go to two scratch arrays
+// ----------------------------------------------------------------------------- one acoustic substep, k-march
+// The substep's column work (the same phases and the same expressions as k_acoustic) with one thread per column: a
+// wavefront holds 64 consecutive columns of one row and marches up and down them, so every load and store of the
+// wavefront is one i-contiguous run of a row and no lane idles on a level its column does not have.  No LDS and no
+// barriers: a column's recurrences run in its own thread in the reference's order (k upward; the sweeps up, then
+// down), its vertical neighbours slide through registers.  Three passes over the column:
+//   1 (up)    the mass divergence of every level and its column sum: mu'', mudf, muts, muave
+//   2 (up)    omega'' (the running integral), ww_m, ru_m, rv_m, theta'', t_2ave, the explicit phi'' R and w'' and the
+//             forward sweep of the tridiagonal solve, whose values y and R go to two FP64 scratch arrays
 //   3 (down)  the back-substitution, damp_opt 3, phi'', then p'' and al'' with divergence damping
-// The divergence is recomputed in pass 2 from the same loads (the same bits as pass 1's), not stored.  The sums
-// associate as the serial reference order (k upward); the arithmetic differs from FP64 mode (FP32 bulk, one
-// multiply by a rounded reciprocal per division), within FP32-bulk's error bar.
-__global__ void __launch_bounds__(64) k_acoustic_march(Args a, double dts_d, int it, int ns) {
+// The divergence is recomputed in pass 2 from the same loads (the same bits as pass 1's), not stored.  Every
+// expression is k_acoustic's, operand for operand, in FP64 arithmetic, so the FP64 mode gives the reference's
+// bits; FP32-bulk rounds only its stores.
+__global__ void __launch_bounds__(64) k_acoustic_march(Args a, double dts, int it, int ns) {
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
@@ -1631,200 +1685,186 @@ __global__ void __launch_bounds__(64) k_acoustic_march(Args a, double dts_d, int
   const int i = b.x * 64 + int(threadIdx.x), j = b.y, nz = g.nz;
   if (i >= g.nx) return;
   if (g.bcx == MI_BC_SPECIFIED && (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1)) return;  // k_acoustic_ring
-  using F = float;
-  const bool first = it == 1, last = it == ns, two_d = g.two_d, moist = a.q.moist != 0;
+  const bool first = it == 1, last = it == ns, two_d = g.two_d;
+  const double rn = double(ns);
+  const double epssm = a.q.epssm;
   const idx_t sk = g.sk, sj = g.sj;
   const idx_t x2 = g.at2(i, j), x0 = g.at(i, j, 0);
-  const F dts = F(dts_d), rdx = F(c.rdx), rdy = F(c.rdy), rn = F(1) / F(ns);
-  const double epssm = a.q.epssm;
-  const F ope = F(1.0 + epssm), ome = F(1.0 - epssm), he = F(0.5 * (1.0 + epssm)), hme = F(0.5 * (1.0 - epssm));
-  const F G = F(kG), T0 = F(kT0);
   // the column's 2-D values
-  const double msfty_d = f.msfty[x2], mut_d = f.mut[x2];
-  const F msfty = F(msfty_d), msftx = F(f.msftx[x2]), rmsfty = F(1.0 / msfty_d), mut = F(mut_d);
-  const F muu0 = F(f.muu[x2]), muu1 = F(f.muu[x2 + 1]);
-  const F rmu0 = F(1.0 / f.msfuy[x2]), rmu1 = F(1.0 / f.msfuy[x2 + 1]);
-  F muv0 = 0, muv1 = 0, mvi0 = 1, mvi1 = 1;
+  const double msfty = f.msfty[x2], msftx = f.msftx[x2], mut = f.mut[x2];
+  const double muu0 = f.muu[x2], muu1 = f.muu[x2 + 1], msfuy0 = f.msfuy[x2], msfuy1 = f.msfuy[x2 + 1];
+  double muv0 = 0, muv1 = 0, mvi0 = 1, mvi1 = 1;
   if (!two_d) {
-    muv0 = F(f.muv[x2]);
-    muv1 = F(f.muv[x2 + sj]);
-    mvi0 = F(f.msfvx_inv[x2]);
-    mvi1 = F(f.msfvx_inv[x2 + sj]);
+    muv0 = f.muv[x2];
+    muv1 = f.muv[x2 + sj];
+    mvi0 = f.msfvx_inv[x2];
+    mvi1 = f.msfvx_inv[x2 + sj];
   }
-  const F mmt = msftx * msfty;
-  // the mass divergence of level k (KMuTDiv, small_step.cpp:194-207), FP32
-  auto divergence = [&](int k, idx_t x, F& u0, F& u1, F& us0, F& v0, F& v1, F& vs0) -> F {
-    const F c1 = F(c.c1h[k]), c2 = F(c.c2h[k]);
-    u0 = F(f.u2[x]);
-    u1 = F(f.u2[x + 1]);
-    us0 = F(f.u_save[x]);
-    const F us1 = F(f.u_save[x + 1]);
-    F d = rdx * ((u1 + (c1 * muu1 + c2) * us1 * rmu1) - (u0 + (c1 * muu0 + c2) * us0 * rmu0));
-    v0 = v1 = vs0 = 0;
+  // the mass divergence dvdxi of level k (KMuTDiv, small_step.cpp:194-207) and the level's u'', v'', saves
+  struct Lev {
+    real u0, u1, us0, v0, v1, vs0;
+  };
+  auto divergence = [&](int k, idx_t x, Lev& l) -> double {
+    l.u0 = f.u2[x];
+    l.u1 = f.u2[x + 1];
+    l.us0 = f.u_save[x];
+    const real us1 = f.u_save[x + 1];
+    const double fx = (l.u1 + (c.c1h[k] * muu1 + c.c2h[k]) * us1 / msfuy1) - (l.u0 + (c.c1h[k] * muu0 + c.c2h[k]) * l.us0 / msfuy0);
+    double d = c.rdx * fx;
+    l.v0 = l.v1 = l.vs0 = 0;
     if (!two_d) {
-      v0 = F(f.v2[x]);
-      v1 = F(f.v2[x + sj]);
-      vs0 = F(f.v_save[x]);
-      const F vs1 = F(f.v_save[x + sj]);
-      d += rdy * ((v1 + (c1 * muv1 + c2) * vs1 * mvi1) - (v0 + (c1 * muv0 + c2) * vs0 * mvi0));
+      l.v0 = f.v2[x];
+      l.v1 = f.v2[x + sj];
+      l.vs0 = f.v_save[x];
+      const real vs1 = f.v_save[x + sj];
+      d += c.rdy * ((l.v1 + (c.c1h[k] * muv1 + c.c2h[k]) * vs1 * mvi1) - (l.v0 + (c.c1h[k] * muv0 + c.c2h[k]) * l.vs0 * mvi0));
     }
-    return mmt * d;
+    return msftx * msfty * d;
   };
   // ---------------------------------------------------------------- pass 1: dmdt and the column masses
   double dmdt = 0;
-  F su[3][4] = {};  // u'' (i, i+1) and v'' (j, j+1) at levels 0..2, for the surface w'' over terrain
+  real su[3][4] = {};  // u'' (i, i+1) and v'' (j, j+1) of levels 0..2, for the surface w'' over terrain
   for (int k = 0; k < nz; ++k) {
-    F u0, u1, us0, v0, v1, vs0;
-    const F dv = divergence(k, x0 + k * sk, u0, u1, us0, v0, v1, vs0);
-    dmdt += c.dnw[k] * double(dv);
+    Lev l;
+    const double dv = divergence(k, x0 + k * sk, l);
+    dmdt += c.dnw[k] * dv;
     if (k < 3) {
-      su[k][0] = u0;
-      su[k][1] = u1;
-      su[k][2] = v0;
-      su[k][3] = v1;
+      su[k][0] = l.u0;
+      su[k][1] = l.u1;
+      su[k][2] = l.v0;
+      su[k][3] = l.v1;
     }
   }
-  const double mu_old = f.mu2[x2], mt = f.mu_tend[x2];
-  const double mu_new = mu_old + dts_d * (dmdt + mt);
-  const double muts_d = mut_d + mu_new;
-  const double muave_d = 0.5 * ((1.0 + epssm) * mu_new + (1.0 - epssm) * mu_old);
+  const double mu_old = f.mu2[x2], mtend = f.mu_tend[x2];
+  const double mu_new = mu_old + dts * (dmdt + mtend);
+  const double muts = mut + mu_new;
+  const double muave = H * ((1.0 + epssm) * mu_new + (1.0 - epssm) * mu_old);
   put<kHMass>(f.mu2, g, i, j, 0, mu_new);
-  put<kHMass>(f.mudf, g, i, j, 0, dmdt + mt);
-  put<kHMass>(f.muts, g, i, j, 0, muts_d);
-  f.muave[x2] = muave_d;
-  const F muts = F(muts_d), muave = F(muave_d);
+  put<kHMass>(f.mudf, g, i, j, 0, dmdt + mtend);
+  put<kHMass>(f.muts, g, i, j, 0, muts);
+  f.muave[x2] = muave;
   // ---------------------------------------------------------------- pass 2: omega'', theta'', explicit phi'' and w''
-  const double cdm = dmdt, cmt = mt, rmsfty_d = 1.0 / msfty_d;
+  const double ope = 1.0 + epssm, ome = 1.0 - epssm;
+  const double msft_inv = 1.0 / msfty;
   // the surface w'' (small_step.cpp:339-353): 0 on flat ground; with terrain from u'', v'' and the slope of ht
-  F w_expl0 = 0;
+  double w_expl0 = 0;
   if (a.q.terrain) {
     const double* ht = f.ht;
-    const F cf1 = F(c.cf1), cf2 = F(c.cf2), cf3 = F(c.cf3);
-    F wk = msftx * F(0.5) * rdx *
-           (F(ht[x2 + 1] - ht[x2]) * (cf1 * su[0][1] + cf2 * su[1][1] + cf3 * su[2][1]) +
-            F(ht[x2] - ht[x2 - 1]) * (cf1 * su[0][0] + cf2 * su[1][0] + cf3 * su[2][0]));
+    w_expl0 = msftx * H * c.rdx *
+              ((ht[x2 + 1] - ht[x2]) * (c.cf1 * su[0][1] + c.cf2 * su[1][1] + c.cf3 * su[2][1]) +
+               (ht[x2] - ht[x2 - 1]) * (c.cf1 * su[0][0] + c.cf2 * su[1][0] + c.cf3 * su[2][0]));
     if (!two_d)
-      wk = msfty * F(0.5) * rdy *
-               (F(ht[x2 + sj] - ht[x2]) * (cf1 * su[0][3] + cf2 * su[1][3] + cf3 * su[2][3]) +
-                F(ht[x2] - ht[x2 - sj]) * (cf1 * su[0][2] + cf2 * su[1][2] + cf3 * su[2][2])) +
-           wk;
-    w_expl0 = wk;
+      w_expl0 = msfty * H * c.rdy *
+                    ((ht[x2 + sj] - ht[x2]) * (c.cf1 * su[0][3] + c.cf2 * su[1][3] + c.cf3 * su[2][3]) +
+                     (ht[x2] - ht[x2 - sj]) * (c.cf1 * su[0][2] + c.cf2 * su[1][2] + c.cf3 * su[2][2])) +
+                w_expl0;
   }
-  // sliding windows (index n-1, n, n+1 of the level being worked on)
-  double S = 0;                          // the omega'' integral up to the current level
+  // sliding windows (levels n-2 .. n+1 of the level n being worked on)
+  double ww_full = 0;                     // the running omega'' integral (KMuTCol's ww_full)
   double ww_m1 = 0, ww_0 = 0, ww_p1 = 0;  // omega'' at w levels n-1, n, n+1
-  F ts_m1 = 0, ts_0 = F(f.t_save[x0]), ts_p1 = 0;
-  F phs_m1 = 0, phs_0 = F(f.ph_save[x0]), phs_p1 = 0;
+  real ts_m1 = 0, ts_0 = f.t_save[x0], ts_p1 = 0;
+  real phs_m1 = 0, phs_0 = f.ph_save[x0], phs_p1 = 0;
   double phb_m1 = 0, phb_0 = f.phb(i, j, 0), phb_p1 = 0;
-  F ph_m2 = 0, ph_m1 = 0, ph_0 = F(f.ph2[x0]);
-  F R_m2 = 0, R_m1 = 0;
-  F t2a_m2 = 0, t2a_m1 = 0, c2a_m2 = 0, c2a_m1 = 0, alt_m2 = 0, alt_m1 = 0;
+  real ph_m2 = 0, ph_m1 = 0, ph_0 = f.ph2[x0];
+  double R_m2 = 0, R_m1 = 0;
+  double t2a_m2 = 0, t2a_m1 = 0;
+  real c2a_m2 = 0, c2a_m1 = 0, alt_m2 = 0, alt_m1 = 0;
   double y_prev = 0;
+  // the explicit w'' at w level k (KWExpl, small_step.cpp:331-378)
+  auto w_explicit = [&](int k, double r_k, double r_km1, double r_kp1, real ph, real ph_m, real ph_p, double t2a_k,
+                        double t2a_km, real c2a, real c2a_m, real alt, real alt_m) -> double {
+    const idx_t xk = x0 + k * sk;
+    const double m = mut, mav = muave;
+    const real w2 = f.w2[xk], rwt = f.rw_tend[xk];
+    if (k < nz) {
+      return w2 + dts * rwt +
+             msft_inv * (a.q.moist ? 1.0 / (1.0 + f.cqw[xk]) : 1.0) *
+                 (H * dts * kG * c.rdn[k] *
+                  (c2a * c.rdnw[k] / (c.c1h[k] * m + c.c2h[k]) * (ope * (r_kp1 - r_k) + ome * (double(ph_p) - ph)) -
+                   c2a_m * c.rdnw[k - 1] / (c.c1h[k - 1] * m + c.c2h[k - 1]) * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)))) +
+             dts * kG * msft_inv * (c.rdn[k] * (double(c2a) * alt * t2a_k - double(c2a_m) * alt_m * t2a_km) - c.c1f[k] * mav);
+    }
+    const int km = nz - 1;
+    double wk = w2 + dts * rwt +
+                msft_inv * (-H * dts * kG / (c.c1h[km] * m + c.c2h[km]) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * c2a_m * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)) -
+                            dts * kG * (2.0 * c.rdnw[km] * c2a_m * alt_m * t2a_km + c.c1f[k] * mav));
+    if (a.q.top_lid) wk = 0;
+    return wk;
+  };
+  // the forward sweep of the tridiagonal solve (KWSolve, small_step.cpp:386), FP64
+  auto sweep = [&](int k, double wexp) {
+    const idx_t xk = x0 + k * sk;
+    const double y = k == 0 ? wexp : (wexp - f.ca[xk] * y_prev) * f.calpha[xk];
+    f.ky[xk] = y;
+    y_prev = y;
+  };
   for (int n = 0; n <= nz; ++n) {
     const idx_t x = x0 + n * sk;
     const bool mk = n < nz;
-    // omega'' at w level n+1 from the integral through level n (KMuTCol, small_step.cpp:231-247, FP64)
-    F u0 = 0, u1 = 0, us0 = 0, v0 = 0, v1 = 0, vs0 = 0;
+    Lev l{};
     if (mk) {
-      const F dv = divergence(n, x, u0, u1, us0, v0, v1, vs0);
+      // omega'' at w level n+1 from the running integral through level n (KMuTCol, small_step.cpp:231-247)
+      const double dv = divergence(n, x, l);
       if (n + 1 < nz) {
-        S += c.dnw[n] * (c.c1h[n] * cdm + double(dv) + c.c1h[n] * cmt) * rmsfty_d;
-        ww_p1 = -S - double(f.ww1[x + sk]);
+        ww_full = ww_full - c.dnw[n] * (c.c1h[n] * dmdt + dv + c.c1h[n] * mtend) / msfty;
+        ww_p1 = ww_full - double(f.ww1[x + sk]);
       } else {
         ww_p1 = 0;
       }
-      ts_p1 = n + 1 < nz ? F(f.t_save[x + sk]) : F(0);
-      phs_p1 = F(f.ph_save[x + sk]);
+      ts_p1 = n + 1 < nz ? f.t_save[x + sk] : real(0);
+      phs_p1 = f.ph_save[x + sk];
       phb_p1 = f.phb(i, j, n + 1);
     }
     // ww, ww_m at w level n
     {
-      const F w1 = F(f.ww1[x]);
-      f.ww[x] = real(ww_0);
-      const F sw = (first ? F(0) : F(f.ww_m[x])) + F(ww_0);
-      put<kHMass>(f.ww_m, g, i, j, n * sk, last ? sw * rn + w1 : sw);
+      const real ww1 = f.ww1[x];
+      f.ww[x] = ww_0;
+      const double sw = (first ? 0.0 : double(f.ww_m[x])) + ww_0;
+      put<kHMass>(f.ww_m, g, i, j, n * sk, last ? sw / rn + ww1 : sw);
     }
-    F t2a_0 = 0, c2a_0 = 0, alt_0 = 0;
+    double t2a_0 = 0;
+    real c2a_0 = 0, alt_0 = 0;
     if (mk) {
       // ru_m, rv_m (KMuTPoint, small_step.cpp:250-262)
-      const F c1 = F(c.c1h[n]), c2 = F(c.c2h[n]);
-      {
-        const F su0 = (first ? F(0) : F(f.ru_m[x])) + u0;
-        put<kHU>(f.ru_m, g, i, j, n * sk, last ? su0 * rn + (c1 * muu0 + c2) * us0 * rmu0 : su0);
-        if (!two_d) {
-          const F sv0 = (first ? F(0) : F(f.rv_m[x])) + v0;
-          put<kHV>(f.rv_m, g, i, j, n * sk, last ? sv0 * rn + (c1 * muv0 + c2) * vs0 * mvi0 : sv0);
-        }
+      const double su0 = (first ? 0.0 : double(f.ru_m[x])) + l.u0;
+      put<kHU>(f.ru_m, g, i, j, n * sk, last ? su0 / rn + (c.c1h[n] * muu0 + c.c2h[n]) * l.us0 / msfuy0 : su0);
+      if (!two_d) {
+        const double sv0 = (first ? 0.0 : double(f.rv_m[x])) + l.v0;
+        put<kHV>(f.rv_m, g, i, j, n * sk, last ? sv0 / rn + (c.c1h[n] * muv0 + c.c2h[n]) * l.vs0 * mvi0 : sv0);
       }
       // theta'' (small_step.cpp:263-276)
-      const F wdtn_k = n == 0 ? F(0) : F(ww_0) * (F(c.fnm[n]) * ts_0 + F(c.fnp[n]) * ts_m1);
-      const F wdtn_kp = n + 1 == nz ? F(0) : F(ww_p1) * (F(c.fnm[n + 1]) * ts_p1 + F(c.fnp[n + 1]) * ts_0);
-      F hflux = F(0.5) * rdx * (u1 * (F(f.t_save[x + 1]) + ts_0) - u0 * (ts_0 + F(f.t_save[x - 1])));
-      if (!two_d) hflux += F(0.5) * rdy * (v1 * (F(f.t_save[x + sj]) + ts_0) - v0 * (ts_0 + F(f.t_save[x - sj])));
-      const F t2 = F(f.t2[x]);
-      const F tn = t2 + msfty * dts * F(f.t_tend[x]);
-      const F t2new = tn - dts * msfty * (msftx * hflux + F(c.rdnw[n]) * (wdtn_kp - wdtn_k));
-      f.t2[x] = real(t2new);
-      const F tav = he * t2new + hme * t2;
-      t2a_0 = (tav + c1 * muave * T0) / ((c1 * muts + c2) * (T0 + ts_0));
-      c2a_0 = F(f.c2a[x]);
-      alt_0 = F(f.alt[x]);
+      const double ts = ts_0;
+      const double wdtn_k = n == 0 ? 0.0 : ww_0 * (c.fnm[n] * ts + c.fnp[n] * ts_m1);
+      const double wdtn_kp = n + 1 == nz ? 0.0 : ww_p1 * (c.fnm[n + 1] * ts_p1 + c.fnp[n + 1] * ts);
+      double hflux = H * c.rdx * (double(l.u1) * (double(f.t_save[x + 1]) + ts) - double(l.u0) * (double(ts) + f.t_save[x - 1]));
+      if (!two_d)
+        hflux += H * c.rdy * (double(l.v1) * (double(f.t_save[x + sj]) + ts) - double(l.v0) * (double(ts) + f.t_save[x - sj]));
+      const real t2 = f.t2[x];
+      const double tn = t2 + msfty * dts * f.t_tend[x];
+      const double t2new = tn - dts * msfty * (msftx * hflux + c.rdnw[n] * (wdtn_kp - wdtn_k));
+      f.t2[x] = t2new;
+      const double tav = H * ((1.0 + epssm) * t2new + (1.0 - epssm) * t2);
+      t2a_0 = (tav + c.c1h[n] * muave * kT0) / ((c.c1h[n] * muts + c.c2h[n]) * (kT0 + ts));
+      c2a_0 = f.c2a[x];
+      alt_0 = f.alt[x];
     }
     // the explicit phi'' R at w level n (KWRhs, small_step.cpp:312-328)
-    F R_0 = 0;
-    const F ph_p1 = mk ? F(f.ph2[x + sk]) : F(0);
+    double R_0 = 0;
+    const real ph_p1 = mk ? f.ph2[x + sk] : real(0);
     if (!(n == 0 || (a.q.top_lid && n == nz))) {
-      const F wdwn_k = F(0.5) * F(ww_0 + ww_m1) * F(c.rdnw[n - 1]) * ((phs_0 - phs_m1) + F(phb_0 - phb_m1));
-      F rr = dts * (F(f.ph_tend[x]) + F(0.5) * G * ome * F(f.w2[x]));
+      const double wdwn_k = H * (ww_0 + ww_m1) * c.rdnw[n - 1] * (double(phs_0) - phs_m1 + phb_0 - phb_m1);
+      double rr = dts * (f.ph_tend[x] + H * kG * (1.0 - epssm) * f.w2[x]);
       if (mk) {
-        const F wdwn_kp = F(0.5) * F(ww_p1 + ww_0) * F(c.rdnw[n]) * ((phs_p1 - phs_0) + F(phb_p1 - phb_0));
-        rr -= dts * (F(c.fnm[n]) * wdwn_kp + F(c.fnp[n]) * wdwn_k);
+        const double wdwn_kp = H * (ww_p1 + ww_0) * c.rdnw[n] * (double(phs_p1) - phs_0 + phb_p1 - phb_0);
+        rr -= dts * (c.fnm[n] * wdwn_kp + c.fnp[n] * wdwn_k);
       }
-      R_0 = ph_0 + msfty * rr / (F(c.c1f[n]) * mut + F(c.c2f[n]));
+      R_0 = ph_0 + msfty * rr / (c.c1f[n] * mut + c.c2f[n]);
     }
     f.kr[x] = R_0;
-    // the explicit w'' at w level k = n - 1 (KWExpl, small_step.cpp:331-378), then the forward sweep (FP64)
-    auto w_explicit = [&](int k, F Rk, F Rkm, F Rkp, F phk, F phkm, F phkp, F t2ak, F t2akm, F c2ak, F c2akm, F altk,
-                          F altkm) -> F {
-      const idx_t xk = x0 + k * sk;
-      const F cq1 = moist ? F(1) / (F(1) + F(f.cqw[xk])) : F(1);
-      const F mav = muave, m = mut;
-      const F w2k = F(f.w2[xk]), rwt = F(f.rw_tend[xk]);
-      if (k < nz) {
-        return w2k + dts * rwt +
-               rmsfty * cq1 *
-                   (F(0.5) * dts * G * F(c.rdn[k]) *
-                    (c2ak * F(c.rdnw[k]) / (F(c.c1h[k]) * m + F(c.c2h[k])) * (ope * (Rkp - Rk) + ome * (phkp - phk)) -
-                     c2akm * F(c.rdnw[k - 1]) / (F(c.c1h[k - 1]) * m + F(c.c2h[k - 1])) * (ope * (Rk - Rkm) + ome * (phk - phkm)))) +
-               dts * G * rmsfty * (F(c.rdn[k]) * (c2ak * altk * t2ak - c2akm * altkm * t2akm) - F(c.c1f[k]) * mav);
-      }
-      const int km = nz - 1;
-      const F rdw = F(c.rdnw[km]);
-      F wk = w2k + dts * rwt +
-             rmsfty * (-F(0.5) * dts * G / (F(c.c1h[km]) * m + F(c.c2h[km])) * (rdw * rdw) * F(2) * c2akm *
-                           (ope * (Rk - Rkm) + ome * (phk - phkm)) -
-                       dts * G * (F(2) * rdw * c2akm * altkm * t2akm + F(c.c1f[k]) * mav));
-      if (a.q.top_lid) wk = 0;
-      return wk;
-    };
-    auto sweep = [&](int k, F wexp) {
-      const idx_t xk = x0 + k * sk;
-      double y;
-      if (k == 0) {
-        y = double(wexp);
-      } else {
-        y = (double(wexp) - f.ca[xk] * y_prev) * f.calpha[xk];
-      }
-      f.ky[xk] = y;
-      y_prev = y;
-    };
+    // the explicit w'' and the forward sweep lag one level (w'' at n-1 needs R at n)
     if (n == 1) sweep(0, w_expl0);
     if (n >= 2) sweep(n - 1, w_explicit(n - 1, R_m1, R_m2, R_0, ph_m1, ph_m2, ph_0, t2a_m1, t2a_m2, c2a_m1, c2a_m2, alt_m1, alt_m2));
-    if (n == nz) {
-      if (nz == 1) sweep(0, w_expl0);
-      sweep(nz, w_explicit(nz, R_0, R_m1, 0, ph_0, ph_m1, 0, 0, t2a_m1, 0, c2a_m1, 0, alt_m1));
-    }
-    // slide the windows
+    if (n == nz) sweep(nz, w_explicit(nz, R_0, R_m1, 0.0, ph_0, ph_m1, real(0), 0.0, t2a_m1, real(0), c2a_m1, real(0), alt_m1));
     ww_m1 = ww_0;
     ww_0 = ww_p1;
     ts_m1 = ts_0;
@@ -1846,43 +1886,43 @@ __global__ void __launch_bounds__(64) k_acoustic_march(Args a, double dts_d, int
     alt_m1 = alt_0;
   }
   // ---------------------------------------------------------------- pass 3: back-substitution, damping, phi'', p'', al''
-  const double dampmag = a.q.damp_opt == 3 ? dts_d * a.q.dampcoef : 0.0;
-  const F hdepth = F(a.q.zdamp);
-  const F htop = (F(f.ph_save[g.at(i, j, nz)]) + F(f.phb(i, j, nz))) / G, hbot = htop - hdepth;
-  const F rG = F(1) / G;
+  const double dampmag = a.q.damp_opt == 3 ? dts * a.q.dampcoef : 0.0;
+  const double hdepth = a.q.zdamp;
+  const double pi = 3.141592653589793;  // 4 atan(1), folded
+  const double htop = (f.ph_save[g.at(i, j, nz)] + f.phb(i, j, nz)) / kG, hbot = htop - hdepth;
   double w_next = 0;
-  double ph_up = 0;  // phi'' at w level k + 1, FP64 for p'', al''
-  const double cphi = msfty_d * 0.5 * dts_d * kG * (1.0 + epssm);
+  double ph_up = 0;  // phi'' at w level k + 1
   for (int k = nz; k >= 0; --k) {
     const idx_t x = x0 + k * sk;
     const double y = f.ky[x];
     double wk = (k == nz || k == 0) ? y : y - f.cgamma[x] * w_next;
+    w_next = wk;  // the back-substitution runs on the undamped values (small_step.cpp:387 before 393-405)
     double phk;
     if (k >= 1) {
       if (dampmag != 0.0) {
-        const F hk = (F(f.ph_save[x]) + F(f.phb(i, j, k))) * rG;
+        const double hk = (f.ph_save[x] + f.phb(i, j, k)) / kG;
+        double dampwt = 0;
         if (hk >= hbot) {
-          const F sn = sinf(F(0.5 * 3.141592653589793) * (hk - hbot) / hdepth);
-          const double dampwt = dampmag * double(sn * sn);
-          wk = (wk - dampwt * (c.c1f[k] * mut_d + c.c2f[k]) * double(f.w_save[x])) / (1.0 + dampwt);
+          const double sn = mi_libm::sin(H * pi * (hk - hbot) / hdepth);
+          dampwt = dampmag * sn * sn;
         }
+        wk = (wk - dampwt * (c.c1f[k] * mut + c.c2f[k]) * f.w_save[x]) / (1.0 + dampwt);
       }
-      phk = double(f.kr[x]) + cphi * wk / (c.c1f[k] * muts_d + c.c2f[k]);
+      phk = f.kr[x] + msfty * H * dts * kG * (1.0 + epssm) * wk / (c.c1f[k] * muts + c.c2f[k]);
       put<kHMass>(f.ph2, g, i, j, k * sk, phk);
     } else {
-      phk = double(f.ph2[x]);
+      phk = f.ph2[x];
     }
-    f.w2[x] = real(wk);
-    w_next = wk;
-    if (k < nz) {  // p'', al'' of mass level k (KWPRho, small_step.cpp:409-414), FP64
-      const double m = c.c1h[k] * muts_d + c.c2h[k];
-      const double alt = f.alt[x], ts = f.t_save[x], c2a = f.c2a[x];
+    f.w2[x] = wk;
+    if (k < nz) {  // p'', al'' of mass level k (KWPRho, small_step.cpp:409-414)
+      const double m = c.c1h[k] * muts + c.c2h[k];
+      const real alt = f.alt[x], ts = f.t_save[x], c2a = f.c2a[x];
       const double al = -((1. / m) * (alt * (c.c1h[k] * mu_new) + c.rdnw[k] * (ph_up - phk)));
       put<kHMass>(f.al, g, i, j, k * sk, al);
       double pp = c2a * (alt * (double(f.t2[x]) - c.c1h[k] * mu_new * ts) / (m * (kT0 + ts)) - al);
       const double ptmp = pp;
-      pp = pp + a.q.smdiv * (pp - double(f.pm1[x]));
-      f.pm1[x] = real(ptmp);
+      pp = pp + a.q.smdiv * (pp - f.pm1[x]);
+      f.pm1[x] = ptmp;
       put<kHMass>(f.p, g, i, j, k * sk, pp);
     }
     ph_up = phk;
@@ -2908,7 +2948,7 @@ MI_COL_LAUNCHER(FinishL, k_finish)
 constexpr int kColArrays[kNumColKernels][2] = {{1, 0}, {3, 0}, {10, 5}, {1, 1}};
 
 bool col_cfg_fits(int kernel, int cfg, int nz) {
-  if (cfg == kColMarch) return kernel == kColAcoustic && sizeof(real) == 4;  // the FP32-bulk k-march
+  if (cfg == kColMarch) return kernel == kColAcoustic;  // the k-march
   if (cfg < 0 || cfg >= kNumColCfgs) return false;
   if (kernel == kColAcoustic && kColCfgs[cfg][1] < nz + 1) return false;  // one level per thread
   return col_lds_bytes(kColCfgs[cfg][0], kColArrays[kernel][0], nz, kColArrays[kernel][1]) <= kLdsMax;
@@ -2918,8 +2958,8 @@ int col_cfg_default(int kernel, int nx, int ny, int nz) {
   // the substep kernel: the narrowest one-level-per-thread shape; the others: one level per thread on
   // small domains (latency), 16 columns (full 128-byte rows) on larger ones
   if (kernel == kColAcoustic) {
-    // FP32-bulk on a large grid: the k-march (a wavefront per 64 columns has enough columns to fill the GPU)
-    if (sizeof(real) == 4 && int64_t(nx) * ny >= 40000) return kColMarch;
+    // a large grid: the k-march (a wavefront per 64 columns, enough of them to fill the GPU)
+    if (int64_t(nx) * ny >= 40000) return kColMarch;
     for (int c : {4, 5, 6})
       if (col_cfg_fits(kernel, c, nz)) return c;
     return 6;
@@ -2966,7 +3006,7 @@ void launch_uv(const Args& a, double dts, hipStream_t st) {
   launch3("uv", (g.ox() || g.oy()) ? k_uv<true> : k_uv<false>, Range{0, g.nx + ex, 0, g.ny + ey, 0, g.nz}, st, a, dts);
 }
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
-  if (cfg == kColMarch) {  // FP32-bulk: one thread per column, 64 columns per block
+  if (cfg == kColMarch) {  // one thread per column, 64 columns per block
     const bool t = timing("acoustic");
     if (t) time_begin(st);
     hipLaunchKernelGGL(k_acoustic_march, dim3(unsigned((a.g.nx + 63) / 64), unsigned(a.g.ny), 1), dim3(64, 1, 1), 0, st,

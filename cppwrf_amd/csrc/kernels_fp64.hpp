@@ -60,9 +60,8 @@ struct Fields {
   double *pd_dx, *pd_dxdt, *pd_dy, *pd_dydt, *pd_dz;
   // the advective forcings a column physics plugin asks for (WRF RTHFTEN, RQVFTEN); null when not wanted
   real *thf, *qvf;
-  // FP32-bulk substep k-march: the forward sweep's values (FP64) and the explicit phi'' of the substep
-  double* ky;
-  real* kr;
+  // the substep k-march's scratch: the forward sweep's values and the explicit phi'' of the substep (FP64)
+  double *ky, *kr;
 };
 
 // The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
@@ -153,7 +152,7 @@ constexpr int kNumColCfgs = 12;
 constexpr int kColCfgs[kNumColCfgs][2] = {{16, 16}, {16, 32}, {32, 16}, {8, 32}, {8, 64}, {4, 128}, {2, 256},
                                           {64, 4}, {64, 8}, {64, 16}, {32, 8}, {16, 64}};
 constexpr int kColCfgTallest = 6;  // 2 x 256: the most levels per column (the substep kernel's nz limit)
-// one more shape of the substep kernel in FP32-bulk: one thread per column marching k (k_acoustic_march)
+// one more shape of the substep kernel: one thread per column marching k (k_acoustic_march)
 constexpr int kColMarch = kNumColCfgs;
 constexpr const char* kColKernelNames[kNumColKernels] = {"large_prep", "ss_column", "acoustic", "finish"};
 bool col_cfg_fits(int kernel, int cfg, int nz);

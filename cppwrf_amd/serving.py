@@ -2,10 +2,14 @@
 
 Requests arrive over time, each one forecast: a domain (its own size, resolution and options), an initial
 state and a length in steps.  The server keeps every admitted forecast's state resident on the device and
-advances all of them together, one batch step per tick (mi_batch: one HIP graph holding every member's step
-as a branch, so the members' kernels overlap).  At each step boundary it retires the forecasts that are
-done, admits waiting ones while the HBM budget allows, and re-forms the batch.  It reports time-to-forecast
-(arrival to completion) percentiles and the aggregate throughput in grid-point-steps/s.
+advances all of them together, one step each per tick: each forecast replays its own captured step graph on
+one of a few streams, so the members' kernels overlap and a change of membership captures nothing.  The host
+does not wait for the device at a tick (events recorded after each tick bound how far it runs ahead and time
+each forecast's completion), and domains are built on background threads while the GPU steps the others.  At
+each step boundary it retires the forecasts that are done, admits waiting ones while the HBM budget allows (a
+domain's device bytes predicted from those already built), and re-forms the batch.  It reports time-to-forecast
+(arrival to completion) percentiles, the aggregate throughput in grid-point-steps/s, the offered load and the
+queue length.
 
     python -m cppwrf_amd.serving --seconds 30 --rate 2 [--precision fp32|fp64]
 
@@ -21,12 +25,13 @@ import json
 import math
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import _abi as A
-from .dyn import Batch, Case, Context, RK3
+from .dyn import Case, Context, RK3
 
 
 @dataclass
@@ -47,6 +52,10 @@ class Request:
     domain: RK3 | None = None
     taken: int = 0
     result: dict | None = None
+    n_scalars: int = 0
+    slot: int = 0            # the server's stream the forecast runs on
+    reserved: int = 0        # device bytes counted against the budget
+    last_event: object = None
 
     @property
     def points(self) -> int:
@@ -61,13 +70,34 @@ class Request:
         return case.params(time_step_sound=self.n_sound, damp_opt=3, zdamp=5000.0, dampcoef=0.2, precision=self.precision)
 
 
-def device_bytes_estimate(r: Request) -> int:
-    """Device memory of a domain before it exists: the padded allocation of the library's layout
-    (row pitch a multiple of 32 elements, a halo of 3) times the number of 3-D arrays of a dry domain."""
-    pitch = (8 + r.nx + 1 + 3 + 31) // 32 * 32
-    rows = r.ny + 1 + 6
-    per3 = pitch * rows * (r.nz + 1) * (4 if r.precision == A.MI_FP32_BULK else 8)
-    return int(per3 * 52 + pitch * rows * 8 * 30)
+def alloc_elems(nx: int, ny: int, nz: int) -> int:
+    """elements of one 3-D allocation of the library's layout: rows of a pitch that is a multiple of 32 elements
+    (8 of padding and 3 of halo west of i = 0, the U points and 3 east), nz + 1 levels, ny + 1 + 6 rows"""
+    pitch = (8 + nx + 1 + 3 + 31) // 32 * 32
+    rows = (ny + 1 + 6) if ny > 1 else 1
+    return pitch * (nz + 1) * rows
+
+
+class MemoryModel:
+    """Device bytes of a domain before it exists: bytes per allocation element of the domains already created with
+    the same precision and scalar count (a domain's memory is a fixed number of whole allocations), and before the
+    first one a bound from above (every array counted as FP64)."""
+
+    def __init__(self):
+        self.ratio: dict = {}
+
+    def estimate(self, r: "Request") -> int:
+        n = alloc_elems(r.nx, r.ny, r.nz)
+        key = (r.precision, r.n_scalars)
+        return int(n * self.ratio[key] * 1.01) if key in self.ratio else int(n * 8 * 80)
+
+    def learn(self, r: "Request", device_bytes: int) -> None:
+        self.ratio[(r.precision, r.n_scalars)] = device_bytes / alloc_elems(r.nx, r.ny, r.nz)
+
+
+def device_bytes_estimate(r: "Request") -> int:
+    """the bound from above of MemoryModel (no domain of the kind created yet)"""
+    return MemoryModel().estimate(r)
 
 
 @dataclass
@@ -79,77 +109,155 @@ class ServerStats:
     peak_bytes: int = 0
     peak_members: int = 0
     latencies: list = field(default_factory=list)
+    queue_samples: list = field(default_factory=list)   # (waiting + preparing) at each tick
+    active_samples: list = field(default_factory=list)  # active forecasts at each tick
 
 
 class ForecastServer:
-    def __init__(self, ctx: Context, hbm_budget: int, max_members: int = 64, clock=time.perf_counter):
-        self.ctx = ctx
+    """Continuous batching on one GPU.  Admitted forecasts are spread over `streams` contexts, each on a stream of
+    its own (default priority), so the members' steps overlap on the device; a tick enqueues one step of every
+    active forecast and returns without waiting for the device (at most `max_ahead` ticks are in flight: the host
+    then waits for the oldest).  Domains are built (case generation, allocation, upload) by `prepare_workers`
+    background threads while the GPU steps the others (0: built on the tick path).  A forecast's completion time is
+    when the device finishes its last step (an event recorded on its stream after that step)."""
+
+    def __init__(self, ctx: Context | None, hbm_budget: int, max_members: int = 64, clock=time.perf_counter,
+                 streams: int = 4, prepare_workers: int = 0, max_ahead: int = 2):
+        import torch
+        self._torch = torch
         self.clock = clock
         self.budget = hbm_budget
         self.max_members = max_members
+        self.max_ahead = max_ahead
+        dev = torch.cuda.current_device()
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(max(1, streams))]
+        self.ctxs = [Context(dev, s.cuda_stream) for s in self.streams]
+        self.members_on = [0] * len(self.ctxs)
+        self.pool = ThreadPoolExecutor(prepare_workers) if prepare_workers > 0 else None
+        self.mem = MemoryModel()
         self.waiting: list[Request] = []
+        self.preparing: list[tuple[Request, object]] = []
+        self.ready: list[Request] = []
         self.active: list[Request] = []
+        self.draining: list[Request] = []
         self.finished: list[Request] = []
-        self.batch: Batch | None = None
+        self.inflight: list[list] = []  # per tick: the events of the streams it used
         self.bytes_in_use = 0
         self.stats = ServerStats()
 
     def submit(self, r: Request) -> None:
         self.waiting.append(r)
 
-    def _admit(self, now: float) -> bool:
-        changed = False
+    def _prepare(self, r: Request) -> Request:
+        case = r.case()
+        r.domain = RK3(self.ctxs[r.slot], case.grid, r.params(case), case)
+        return r
+
+    def _start_preparing(self, now: float) -> None:
         keep = []
         for r in self.waiting:
-            need = device_bytes_estimate(r)
-            if len(self.active) < self.max_members and self.bytes_in_use + need <= self.budget:
-                case = r.case()
-                r.domain = RK3(self.ctx, case.grid, r.params(case), case)
-                r.admitted = now
-                self.bytes_in_use += r.domain.device_bytes
-                self.active.append(r)
-                self.stats.admitted += 1
-                changed = True
+            need = self.mem.estimate(r)
+            n_live = len(self.active) + len(self.preparing) + len(self.ready)
+            if n_live < self.max_members and self.bytes_in_use + need <= self.budget:
+                r.reserved = need
+                self.bytes_in_use += need
+                r.slot = min(range(len(self.ctxs)), key=lambda q: self.members_on[q])
+                self.members_on[r.slot] += 1
+                if self.pool is None:
+                    self.ready.append(self._prepare(r))
+                else:
+                    self.preparing.append((r, self.pool.submit(self._prepare, r)))
             else:
                 keep.append(r)
         self.waiting = keep
+
+    def _collect_prepared(self) -> None:
+        still = []
+        for r, fut in self.preparing:
+            if fut.done():
+                fut.result()  # raises what the preparation raised
+                self.ready.append(r)
+            else:
+                still.append((r, fut))
+        self.preparing = still
+
+    def _admit(self, now: float) -> bool:
+        changed = False
+        for r in self.ready:
+            real = r.domain.device_bytes
+            self.mem.learn(r, real)
+            self.bytes_in_use += real - r.reserved
+            r.reserved = real
+            r.admitted = now
+            self.active.append(r)
+            self.stats.admitted += 1
+            changed = True
+        self.ready = []
         self.stats.peak_bytes = max(self.stats.peak_bytes, self.bytes_in_use)
         self.stats.peak_members = max(self.stats.peak_members, len(self.active))
         return changed
 
     def _retire(self, now: float) -> bool:
-        done = [r for r in self.active if r.taken >= r.steps]
-        if not done:
-            return False
-        for r in done:
-            r.done = now
-            self.bytes_in_use -= r.domain.device_bytes
-            self.stats.latencies.append(r.done - r.arrival)
-            self.stats.completed += 1
-        self.active = [r for r in self.active if r.taken < r.steps]
-        self.finished.extend(done)
-        return True
+        """forecasts whose last step is enqueued leave the batch; they are done when their stream passes the
+        event recorded after that step"""
+        moved = [r for r in self.active if r.taken >= r.steps]
+        if moved:
+            self.active = [r for r in self.active if r.taken < r.steps]
+            self.draining.extend(moved)
+        still = []
+        for r in self.draining:
+            if r.last_event is None or r.last_event.query():
+                r.done = self.clock()
+                self.stats.latencies.append(r.done - r.arrival)
+                self.stats.completed += 1
+                self.bytes_in_use -= r.reserved
+                self.members_on[r.slot] -= 1
+                self.finished.append(r)
+            else:
+                still.append(r)
+        self.draining = still
+        return bool(moved)
 
     def tick(self, now: float) -> None:
-        """one step boundary: retire, admit, re-form the batch, advance every active forecast one step"""
-        changed = self._retire(now)
-        changed = self._admit(now) or changed
+        """one step boundary: collect built domains, retire, admit, enqueue one step of every active forecast"""
+        self._collect_prepared()
+        self._retire(now)
+        self._start_preparing(now)
+        self._admit(now)
+        self.stats.queue_samples.append(len(self.waiting) + len(self.preparing))
+        self.stats.active_samples.append(len(self.active))
         if not self.active:
-            self.batch = None
             return
-        if changed or self.batch is None:
-            if self.batch is not None:
-                self.batch.close()
-            self.batch = Batch(self.ctx, [r.domain for r in self.active])
-        self.batch.step(1)
-        self.batch.sync()  # host and device in step: the next boundary's decisions see the device's progress
+        used = set()
         for r in self.active:
+            r.domain.step(1)
             r.taken += 1
+            used.add(r.slot)
             self.stats.point_steps += r.points
+        events = {}
+        for q in used:
+            ev = self._torch.cuda.Event()
+            ev.record(self.streams[q])
+            events[q] = ev
+        for r in self.active:
+            if r.taken >= r.steps:
+                r.last_event = events[r.slot]
+        self.inflight.append(list(events.values()))
+        while len(self.inflight) > self.max_ahead:  # bound the run-ahead of the host
+            for ev in self.inflight.pop(0):
+                ev.synchronize()
         self.stats.ticks += 1
 
+    def drain(self) -> None:
+        """wait for every enqueued step and retire what finished"""
+        for evs in self.inflight:
+            for ev in evs:
+                ev.synchronize()
+        self.inflight = []
+        self._retire(self.clock())
+
     def release_finished(self, keep=frozenset()) -> None:
-        """free the device state of retired forecasts; the final u, v, w, t, mu of those in `keep` (request ids)
+        """free the device state of finished forecasts; the final u, v, w, t, mu of those in `keep` (request ids)
         are read back first into r.result"""
         for r in self.finished:
             if r.domain is not None:
@@ -157,6 +265,11 @@ class ForecastServer:
                     r.result = {fid: r.domain.field(fid) for fid in (A.MI_U, A.MI_V, A.MI_W, A.MI_T, A.MI_MU)}
                 r.domain.close()
                 r.domain = None
+
+    def close(self) -> None:
+        if self.pool is not None:
+            self.pool.shutdown(wait=True)
+        self.drain()
 
 
 def poisson_trace(seconds: float, rate: float, seed: int, scale: float = 1.0, precision: int = A.MI_FP64):
@@ -178,26 +291,43 @@ def poisson_trace(seconds: float, rate: float, seed: int, scale: float = 1.0, pr
     return out
 
 
-def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None, keep=frozenset()) -> dict:
+def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None, keep=frozenset(), streams: int = 4,
+              prepare_workers: int = 2, progress=None) -> dict:
     """replay the trace in real time (arrivals by the wall clock) until every request has finished; the final
     states of the requests in `keep` stay in their Request.result"""
-    ctx = ctx or Context(0)
     t0 = time.perf_counter()
-    srv = ForecastServer(ctx, hbm_budget, clock=lambda: time.perf_counter() - t0)
+    clock = lambda: time.perf_counter() - t0  # noqa: E731
+    srv = ForecastServer(ctx, hbm_budget, clock=clock, streams=streams, prepare_workers=prepare_workers)
     pending = sorted(reqs, key=lambda r: r.arrival)
-    while pending or srv.waiting or any(r.taken < r.steps for r in srv.active):
-        now = time.perf_counter() - t0
+    last_report = 0.0
+    busy_ticks = 0
+    while pending or srv.waiting or srv.preparing or srv.ready or srv.active or srv.draining:
+        now = clock()
         while pending and pending[0].arrival <= now:
             srv.submit(pending.pop(0))
-        if not srv.waiting and not srv.active:
+        if not (srv.waiting or srv.preparing or srv.ready or srv.active or srv.draining):
             time.sleep(max(0.0, min(0.05, pending[0].arrival - now)))
             continue
+        if not srv.active and not srv.ready:
+            srv.drain()
+            srv._collect_prepared()
+            if not srv.ready and srv.preparing:
+                time.sleep(0.002)
         srv.tick(now)
+        busy_ticks += 1 if srv.active else 0
         srv.release_finished(keep)
-    srv.tick(time.perf_counter() - t0)  # the last boundary retires what the last step finished
+        if progress and now - last_report > 30.0:
+            last_report = now
+            progress(f"serving: t={now:.0f}s completed {srv.stats.completed}/{len(reqs)} active {len(srv.active)} "
+                     f"waiting {len(srv.waiting) + len(srv.preparing)}")
+    srv.close()
     srv.release_finished(keep)
-    wall = time.perf_counter() - t0
+    wall = clock()
     lat = np.array(srv.stats.latencies)
+    q = np.array(srv.stats.queue_samples) if srv.stats.queue_samples else np.zeros(1)
+    act = np.array(srv.stats.active_samples) if srv.stats.active_samples else np.zeros(1)
+    offered = sum(r.points * r.steps for r in reqs)
+    trace_s = max((r.arrival for r in reqs), default=0.0)
     return {
         "requests": len(reqs),
         "completed": srv.stats.completed,
@@ -205,11 +335,21 @@ def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None, 
         "p99_time_to_forecast_s": float(np.percentile(lat, 99)) if lat.size else None,
         "mean_time_to_forecast_s": float(lat.mean()) if lat.size else None,
         "throughput_grid_point_steps_per_s": srv.stats.point_steps / wall,
+        "offered_grid_point_steps_per_s": offered / trace_s if trace_s > 0 else None,
         "wall_s": wall,
+        "trace_span_s": trace_s,
         "ticks": srv.stats.ticks,
+        "queue_mean": float(q.mean()),
+        "queue_max": int(q.max()),
+        "queue_last_quarter_mean": float(q[3 * len(q) // 4:].mean()),
+        "active_mean": float(act.mean()),
         "peak_members": srv.stats.peak_members,
         "peak_device_bytes": srv.stats.peak_bytes,
         "hbm_budget_bytes": hbm_budget,
+        "streams": streams,
+        "drain_s": wall - trace_s,
+        # stable: the backlog left when arrivals stop clears within the slowest forecast's own time to forecast
+        "queue_stable": bool(lat.size and wall - trace_s <= max(float(np.percentile(lat, 99)), 0.1 * trace_s)),
     }
 
 

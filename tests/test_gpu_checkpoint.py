@@ -1,7 +1,7 @@
 """Checkpoint and resume of a domain's carried state (no counterpart in the reference's src/dyn; its restart test,
 tests/robustness/run_test.sh:40-51, resumes from NetCDF restart files): a resumed forecast equals the
 uninterrupted one bit for bit, in a fresh domain and after rewinding the same one, in FP64 and FP32-bulk, with
-specified boundaries and scalars."""
+specified boundaries and scalars; in FP64 the resumed forecast is the oracle's uninterrupted one bit for bit."""
 import numpy as np
 import pytest
 
@@ -9,6 +9,7 @@ from conftest import bitwise_equal, first_diff
 from cppwrf_amd import _abi as A
 from cppwrf_amd.boundary import drifting_boundary
 from cppwrf_amd.dyn import Case, Context, RK3
+from oracle_lib import OracleDomain
 
 pytestmark = pytest.mark.gpu
 
@@ -57,6 +58,11 @@ def test_resume_is_bitwise(ctx, precision):
     d1.load_state(buf)  # rewind the same domain (its scalar buffers have swapped since the save)
     d1.step(4)
     _same(_fields(d1, fids), want, "rewound")
+    if precision == A.MI_FP64:  # the resumed forecast is the oracle's 7 uninterrupted steps
+        orc = OracleDomain(case, prm)
+        orc.load_boundary(bdy)
+        orc.step(7)
+        _same(_fields(orc, fids), want, "against the oracle")
 
 
 def test_rejects_foreign_state(ctx):
