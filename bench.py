@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default block shapes")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the batched ensemble (for profiling runs: the headline then falls back to c3)")
     ap.add_argument("--no-c2", action="store_true", help="skip the 119x89x44 domain")
     ap.add_argument("--no-c3", action="store_true", help="skip the 425x300x50 domain (and the roofline)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving trace")
@@ -291,6 +293,16 @@ def main():
 
     from cppwrf_amd.dyn import Context
     ctx = Context(0)
+    if args.no_ensemble:
+        c3, d32 = domain_bench(ctx, "c3", args.steps, args.warmup, not args.no_autotune, 0)
+        out = {"metric": METRIC, "value": c3["fp32"]["value"], "unit": "grid-point-steps/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": c3["fp32"]["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32-bulk (f64 islands)",
+               "data": "synthetic", "config": {"workload": "425x300x50 single domain (the ensemble was skipped)"},
+               "roofline": roofline(d32, args.steps, "hbm_traffic_hills425x300x50.md"), "c3": c3}
+        d32.close()
+        print(json.dumps(out), flush=True)
+        return
     ens = ensemble_bench(ctx, args.steps, args.warmup, not args.no_autotune)
     out = {
         "metric": METRIC,
