@@ -207,6 +207,7 @@ SIGNATURES = {
     "mi_batch_step": (_I, [_VP, _I]),
     "mi_batch_graph_count": (_I, [_VP]),
     "mi_cfl_max": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mi_cfl_max_noalloc": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mi_domain_set_dt": (_I, [_VP, C.c_double, _I]),
     "mi_domain_state_bytes": (_I64, [_VP]),
     "mi_domain_save_state": (_I, [_VP, _VP, _I64]),

@@ -385,6 +385,10 @@ int mi_cfl_max(mi_domain* d, double* vert, double* horiz) {
   return mi_detail::guard([&] { checked(d)->cfl_max(vert, horiz); });
 }
 
+int mi_cfl_max_noalloc(mi_domain* d, double* vert, double* horiz) {
+  return mi_detail::guard([&] { checked(d)->cfl_max_noalloc(vert, horiz); });
+}
+
 int mi_domain_set_dt(mi_domain* d, double dt, int n_sound) {
   return mi_detail::guard([&] { checked(d)->set_dt(dt, n_sound); });
 }

@@ -705,6 +705,7 @@ struct Domain final : mi_domain {
   void profile_read(double* total_ms, int* launches, double* alg_bytes) override;
   int check_finite(int* bad_field) override;
   void cfl_max(double* vert, double* horiz) override;
+  void cfl_max_noalloc(double* vert, double* horiz) override;
   void bdy_set(int var, int side, const double* values, const double* tends, int64_t n) override;
   void bdy_set_time(double dtbc) override {
     if (!spec) throw Error(MI_EINVAL, "the domain has no specified boundaries");
@@ -1195,6 +1196,23 @@ void Domain::cfl_max(double* vert, double* horiz) {
   if (grid.nz >= 2) launch_cfl(a, out, st);
   unsigned long long h[2];
   hip_check(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  double v[2];
+  std::memcpy(v, h, sizeof(v));
+  if (vert) *vert = v[0];
+  if (horiz) *horiz = v[1];
+}
+
+// the same maxima through the library's file-scope scratch array (no allocation of any kind on this path)
+void Domain::cfl_max_noalloc(double* vert, double* horiz) {
+  hipStream_t st = ctx->stream;
+  unsigned long long* sp = cfl_scratch(ctx->device);
+  if (!sp) throw Error(MI_EHIP, "hipGetSymbolAddress of the CFL scratch array failed");
+  unsigned long long h[kCflScratchSlots] = {0, 0};
+  if (grid.nz >= 2) {
+    launch_cfl_scratch(a, sp, st);
+    hip_check(hipMemcpyAsync(h, sp, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  }
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
   double v[2];
   std::memcpy(v, h, sizeof(v));

@@ -44,6 +44,7 @@ struct mi_domain {
   virtual void profile_read(double* total_ms, int* launches, double* alg_bytes) = 0;
   virtual int check_finite(int* bad_field) = 0;  // -1 or the first field id with a non-finite value
   virtual void cfl_max(double* vert, double* horiz) = 0;
+  virtual void cfl_max_noalloc(double* vert, double* horiz) = 0;
   virtual void set_dt(double dt, int n_sound) = 0;
   virtual int64_t state_bytes() = 0;
   virtual void save_state(void* host, int64_t bytes) = 0;

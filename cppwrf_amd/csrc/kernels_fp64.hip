@@ -11,6 +11,7 @@
 #include "kernels_fp64.hpp"
 
 #include <cstring>
+#include <mutex>
 
 #include "libm_port.hpp"
 
@@ -3040,6 +3041,36 @@ void launch_set_w_surface(const Args& a, hipStream_t st) {
 void launch_cfl(const Args& a, unsigned long long* out, hipStream_t st) {
   const Geo& g = a.g;
   launch3a("cfl", 4, k_cfl, Range{0, g.nx, 0, g.ny, 1, g.nz}, st, a, out);  // ww u v (mut 2-D) read
+}
+
+// The scratch of the allocation-free CFL maxima (mi_cfl_max_noalloc): the two running maxima live in this
+// file-scope device array, one copy per GPU, held until the process ends.  Host code never uses its name as an
+// address: hipGetSymbolAddress gives each device's copy once, with that device current.
+static __device__ __align__(256) unsigned long long scratch[kCflScratchSlots];
+namespace {
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int e = int(blockIdx.x) * 64 + int(threadIdx.x);
+  if (e < n) p[e] = 0ull;
+}
+constexpr int kMaxDevices = 64;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+
+unsigned long long* cfl_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratch_ptr[device] = static_cast<unsigned long long*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+
+void launch_cfl_scratch(const Args& a, unsigned long long* sp, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1, 1, 1), dim3(64, 1, 1), 0, st, sp, kCflScratchSlots);  // a kernel, not a memset
+  launch_cfl(a, sp, st);
 }
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st) {
   const Geo& g = a.g;

@@ -181,6 +181,12 @@ void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, 
 void launch_set_w_surface(const Args& a, hipStream_t st);
 // w_damp's Courant-number maxima into out[0] (vertical), out[1] (horizontal) as non-negative bit patterns
 void launch_cfl(const Args& a, unsigned long long* out, hipStream_t st);
+// the same into the file-scope scratch array of this file (no allocation): cfl_scratch gives the array's
+// address on a device (nullptr if hipGetSymbolAddress fails), launch_cfl_scratch zeroes its slots with a kernel
+// and runs the maxima into them
+constexpr int kCflScratchSlots = 2;
+unsigned long long* cfl_scratch(int device);
+void launch_cfl_scratch(const Args& a, unsigned long long* scratch_ptr, hipStream_t st);
 // calc_cq (KCq) of the stage state
 void launch_cq(const Args& a, const Scalars& s, hipStream_t st);
 // the transport of species sp[0..n): rk_update_scalar (pd = 0) or rk_update_scalar_pd (pd = 1)

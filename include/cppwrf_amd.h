@@ -282,6 +282,12 @@ int mi_bdy_get_time(mi_domain* d, double* dtbc);
  * max(|u rdx msfux dt|, |v rdy msfvy dt|) over levels 1..nz-1; a NaN never wins).  Called between
  * mi_rk_tendency and mi_small_steps it gives the reference's on_cfl values of that stage. */
 int mi_cfl_max(mi_domain* d, double* vert, double* horiz);
+/* The same maxima without allocating anything: no counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The two running
+ * maxima live in a file-scope device array of the library (2 x 8 bytes, one copy per GPU, held until the process
+ * ends), cleared by a kernel at every call; the results equal mi_cfl_max's.  Calls that overlap on one GPU share
+ * that array and race. */
+int mi_cfl_max_noalloc(mi_domain* d, double* vert, double* horiz);
 
 /* Adaptive time step.  mi_domain_set_dt is RK3::set_dt (rk3.cpp:79-84) together with what an adaptive-step
  * driver changes beside it (WRF adapt_timestep): the specified boundaries' relaxation weights
