@@ -103,15 +103,30 @@ def ensemble_bench(ctx, steps, warmup, autotune):
         if autotune:
             for d in ms:
                 d.autotune(1)
+        # both batch schedules (the same bits per member): a branch per member, and the fused substep launches
         b = Batch(ctx, ms)
-        b.step(warmup)
+        per = {}
+        for sched, sname in ((A.MI_BATCH_BRANCHES, "branches"), (A.MI_BATCH_FUSED, "fused")):
+            b.set_schedule(sched)
+            b.step(warmup)
+            b.sync()
+            t0 = time.perf_counter()
+            b.step(steps)
+            b.sync()
+            per[sname] = time.perf_counter() - t0
+            log(f"ensemble {tag} {sname}: {per[sname] * 1e3 / steps:.3f} ms/step, "
+                f"{pts * steps / per[sname]:.3e} grid-point-steps/s")
+        # the faster schedule times the headline's K steps
+        best = min(per, key=per.get)
+        b.set_schedule(A.MI_BATCH_BRANCHES if best == "branches" else A.MI_BATCH_FUSED)
         b.sync()
         t0 = time.perf_counter()
         b.step(steps)
         b.sync()
         dt = time.perf_counter() - t0
-        out[tag] = {"value": pts * steps / dt, "unit": "grid-point-steps/s", "ms_per_step": dt * 1e3 / steps}
-        log(f"ensemble {tag}: {dt * 1e3 / steps:.3f} ms/step, {pts * steps / dt:.3e} grid-point-steps/s")
+        log(f"ensemble {tag}: {best} schedule, {dt * 1e3 / steps:.3f} ms/step")
+        out[tag] = {"value": pts * steps / dt, "unit": "grid-point-steps/s", "ms_per_step": dt * 1e3 / steps,
+                    "schedule": best, "trial_by_schedule": {k: pts * steps / v for k, v in per.items()}}
         b.close()
         doms[tag] = ms
     worst = {}
@@ -123,7 +138,7 @@ def ensemble_bench(ctx, steps, warmup, autotune):
             d.close()
     out["speedup_fp32_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
     out["fp32_max_rel_l2_over_members"] = worst
-    out["after_steps"] = warmup + steps
+    out["after_steps"] = 2 * (warmup + steps) + steps
     return out
 
 
@@ -320,8 +335,10 @@ def main():
         "config": {"workload": "batched ensemble: 32 members of uneven sizes (nx 60-220, ny 60-180, nz 35/44/50, "
                                "dx 3-15 km), 3-D warm bubble per member seed with +-0.5 K / +-1 m/s perturbations, "
                                "dt 6 s per km, 6 acoustic steps, damp_opt 3, one HIP graph per batch step",
-                   "members": ens["members"], "grid_points": ens["grid_points"], "precision": "fp32-bulk"},
+                   "members": ens["members"], "grid_points": ens["grid_points"], "precision": "fp32-bulk",
+                   "batch_schedule": ens["fp32"]["schedule"], "trial_by_schedule": ens["fp32"]["trial_by_schedule"]},
         "fp64": {"value": ens["fp64"]["value"], "ms_per_step": ens["fp64"]["ms_per_step"], "dtype": "f64",
+                 "batch_schedule": ens["fp64"]["schedule"], "trial_by_schedule": ens["fp64"]["trial_by_schedule"],
                  "note": "plain FP64, bit-identical to the reference CPU FP64 build with CPPWRF_NO_LIBMVEC (scalar "
                          "glibc pow in calc_p_rho_phi)"},
         "speedup_fp32_vs_fp64": ens["speedup_fp32_vs_fp64"],

@@ -19,6 +19,7 @@ MI_ENONFINITE = -4
 MI_ENODEV = -5
 
 MI_BC_PERIODIC, MI_BC_SYMMETRIC, MI_BC_SPECIFIED, MI_BC_OPEN = 0, 1, 2, 3
+MI_BATCH_BRANCHES, MI_BATCH_FUSED = 0, 1
 MI_FP64, MI_FP32_BULK = 0, 1
 
 # field ids
@@ -30,11 +31,12 @@ MI_MSFTX, MI_MSFTY, MI_MSFUX, MI_MSFUY, MI_MSFVX, MI_MSFVY, MI_MSFVX_INV = range
 MI_CQU, MI_CQV, MI_CQW, MI_H_DIABATIC = range(20, 24)
 MI_RU_TENDF, MI_RV_TENDF, MI_RW_TENDF, MI_T_TENDF, MI_TH_FORCING, MI_QV_FORCING = range(24, 30)
 MI_F, MI_E, MI_SINA, MI_COSA, MI_HT = range(47, 52)
+MI_U_BASE, MI_V_BASE, MI_QV_BASE = range(52, 55)
 MI_Q, MI_Q_TENDF = 64, 96
 MI_MAX_MOIST, MI_MAX_SCALARS = 8, 16
-ABI_VERSION = 2
+ABI_VERSION = 3
 
-MI_CASE_REST_STATE, MI_CASE_WARM_BUBBLE, MI_CASE_IGW, MI_CASE_BUBBLE3D, MI_CASE_HILLS = range(5)
+MI_CASE_REST_STATE, MI_CASE_WARM_BUBBLE, MI_CASE_IGW, MI_CASE_BUBBLE3D, MI_CASE_HILLS, MI_CASE_STRAKA = range(6)
 
 # specified lateral boundaries
 MI_BDY_U, MI_BDY_V, MI_BDY_T, MI_BDY_PH, MI_BDY_MU, MI_BDY_Q = 0, 1, 2, 3, 4, 16
@@ -61,6 +63,7 @@ STAGGER = {
     MI_RU_TENDF: "U", MI_RV_TENDF: "V", MI_RW_TENDF: "W", MI_T_TENDF: "M", MI_TH_FORCING: "M", MI_QV_FORCING: "M",
     MI_MSFTX: "M2", MI_MSFTY: "M2", MI_MSFUX: "U2", MI_MSFUY: "U2", MI_MSFVX: "V2", MI_MSFVY: "V2",
     MI_MSFVX_INV: "V2", MI_F: "M2", MI_E: "M2", MI_SINA: "M2", MI_COSA: "M2", MI_HT: "M2",
+    MI_U_BASE: "Z", MI_V_BASE: "Z", MI_QV_BASE: "Z",
 }
 for _n in range(MI_MAX_SCALARS):
     STAGGER[MI_Q + _n] = "M"
@@ -82,6 +85,8 @@ for _n in range(MI_MAX_SCALARS):
 def field_shape(fid: int, nx: int, ny: int, nz: int) -> tuple[int, int, int]:
     """(nzs, nys, nxs) of a field's logical points, the dense array layout of the C ABI."""
     st = STAGGER[fid]
+    if st == "Z":  # a 1-D profile of the mass levels
+        return (nz, 1, 1)
     if st == "M2":
         return (1, ny, nx)
     if st == "U2":
@@ -107,7 +112,8 @@ class MiParams(C.Structure):
                 ("time_step_sound", C.c_int32), ("precision", C.c_int32), ("n_moist", C.c_int32),
                 ("n_scalars", C.c_int32), ("moist_adv_opt", C.c_int32), ("scalar_adv_opt", C.c_int32),
                 ("mapped", C.c_int32), ("coriolis", C.c_int32), ("diff_6th_opt", C.c_int32), ("terrain", C.c_int32),
-                ("diff_6th_factor", C.c_double)]
+                ("diff_6th_factor", C.c_double), ("diff_opt", C.c_int32), ("km_opt", C.c_int32),
+                ("khdif", C.c_double), ("kvdif", C.c_double), ("pr_inv", C.c_double)]
 
 
 _PD = C.POINTER(C.c_double)
@@ -133,7 +139,8 @@ def default_params(**kw) -> MiParams:
     p = MiParams(h_adv_order=5, v_adv_order=3, top_lid=0, w_damping=0, w_crit_cfl=1.0, epssm=0.1, smdiv=0.1,
                  emdiv=0.01, damp_opt=0, hypsometric_opt=1, zdamp=5000.0, dampcoef=0.2, time_step_sound=0,
                  precision=MI_FP64, n_moist=0, n_scalars=0, moist_adv_opt=1, scalar_adv_opt=1, mapped=0,
-                 coriolis=0, diff_6th_opt=0, terrain=0, diff_6th_factor=0.12)
+                 coriolis=0, diff_6th_opt=0, terrain=0, diff_6th_factor=0.12, diff_opt=0, km_opt=1, khdif=0.0,
+                 kvdif=0.0, pr_inv=3.0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -206,6 +213,7 @@ SIGNATURES = {
     "mi_batch_destroy": (_I, [_VP]),
     "mi_batch_step": (_I, [_VP, _I]),
     "mi_batch_graph_count": (_I, [_VP]),
+    "mi_batch_set_schedule": (_I, [_VP, _I]),
     "mi_cfl_max": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mi_cfl_max_noalloc": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mi_domain_set_dt": (_I, [_VP, C.c_double, _I]),

@@ -50,12 +50,23 @@ struct mi_batch {
     hipGraphExec_t x = nullptr;
   };
   std::vector<Graph> graphs;
+  int schedule = MI_BATCH_BRANCHES;
+  void* fused_dev = nullptr;  // the fused schedule's device tables (members' Args, block tables)
+  size_t fused_cap = 0;
+  FusedInfo fused_info;
 
   ~mi_batch() {
     for (Graph& gr : graphs) destroy(gr);
+    if (fused_dev) (void)hipFree(fused_dev);
     for (hipEvent_t e : join) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
     for (hipStream_t s : st) (void)hipStreamDestroy(s);
+  }
+  bool fusable() const {
+    if (m.empty() || m.size() > size_t(mi_fp64::kMaxFusedMembers)) return false;
+    for (mi_domain* d : m)
+      if (!d->fusable() || d->storage_bytes() != m[0]->storage_bytes() || d->n_sound() != m[0]->n_sound()) return false;
+    return true;
   }
   std::vector<int> key() const {
     std::vector<int> k;
@@ -82,14 +93,40 @@ struct mi_batch {
     }
     graphs.swap(keep);
     hipStream_t s0 = ctx->stream;
+    // MI_BATCH_FUSED: each acoustic substep's kernels as one launch over all members (every member can join:
+    // one arithmetic, one acoustic step count, no hooks); MI_BATCH_BRANCHES: each member's step a branch of its own
+    const bool fused = schedule == MI_BATCH_FUSED && fusable();
+    FusedInfo& info = fused_info;
+    if (fused && graphs.empty()) {  // the tables of the members' current schedules (any buffer parity)
+      const int n = int(m.size());
+      const size_t need = m[0]->storage_bytes() == 8 ? mi_fp64::fused_bytes(n) : mi_fp32::fused_bytes(n);
+      if (need > fused_cap) {
+        if (fused_dev) hip_check(hipFree(fused_dev), "hipFree");
+        fused_dev = nullptr;
+        fused_cap = 0;
+        hip_check(hipMalloc(&fused_dev, need), "hipMalloc");
+        fused_cap = need;
+      }
+      // the tables are filled before the capture (graphs read them at every replay; a new capture refills them)
+      hip_check(hipStreamSynchronize(s0), "hipStreamSynchronize");
+      if (m[0]->storage_bytes() == 8) mi_fp64::fused_prepare(m.data(), n, fused_dev, s0, info);
+      else mi_fp32::fused_prepare(m.data(), n, fused_dev, s0, info);
+    }
     hip_check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-      hip_check(hipEventRecord(fork, s0), "hipEventRecord");
-      for (size_t q = 0; q < m.size(); ++q) {
-        hip_check(hipStreamWaitEvent(st[q], fork, 0), "hipStreamWaitEvent");
-        m[q]->capture_step(st[q]);
-        hip_check(hipEventRecord(join[q], st[q]), "hipEventRecord");
-        hip_check(hipStreamWaitEvent(s0, join[q], 0), "hipStreamWaitEvent");
+      if (fused) {
+        if (m[0]->storage_bytes() == 8)
+          mi_fp64::fused_capture(m.data(), int(m.size()), s0, st.data(), fork, join.data(), fused_dev, info);
+        else
+          mi_fp32::fused_capture(m.data(), int(m.size()), s0, st.data(), fork, join.data(), fused_dev, info);
+      } else {
+        hip_check(hipEventRecord(fork, s0), "hipEventRecord");
+        for (size_t q = 0; q < m.size(); ++q) {
+          hip_check(hipStreamWaitEvent(st[q], fork, 0), "hipStreamWaitEvent");
+          m[q]->capture_step(st[q]);
+          hip_check(hipEventRecord(join[q], st[q]), "hipEventRecord");
+          hip_check(hipStreamWaitEvent(s0, join[q], 0), "hipStreamWaitEvent");
+        }
       }
     } catch (...) {
       hipGraph_t junk = nullptr;
@@ -376,6 +413,22 @@ int mi_batch_step(mi_batch* b, int nsteps) {
     for (mi_domain* d : b->m)
       if (d->context()->stream != b->ctx->stream)
         hip_check(hipStreamWaitEvent(d->context()->stream, b->fork, 0), "hipStreamWaitEvent");
+  });
+}
+
+int mi_batch_set_schedule(mi_batch* b, int schedule) {
+  return mi_detail::guard([&] {
+    if (!b) throw Error(MI_EINVAL, "null batch");
+    if (schedule != MI_BATCH_BRANCHES && schedule != MI_BATCH_FUSED)
+      throw Error(MI_EINVAL, "schedule: MI_BATCH_BRANCHES or MI_BATCH_FUSED");
+    if (schedule == MI_BATCH_FUSED && !b->fusable())
+      throw Error(MI_EINVAL, "fused schedule: every member needs the same arithmetic and acoustic step count, no "
+                             "callbacks, nz within the batched column kernel, and at most 64 members");
+    if (schedule == b->schedule) return;
+    hip_check(hipStreamSynchronize(b->ctx->stream), "hipStreamSynchronize");
+    for (mi_batch::Graph& gr : b->graphs) mi_batch::destroy(gr);
+    b->graphs.clear();
+    b->schedule = schedule;
   });
 }
 

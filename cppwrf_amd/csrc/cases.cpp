@@ -116,6 +116,7 @@ struct mi_case {
   double cf1 = 0, cf2 = 0, cf3 = 0, cfn = 0, cfn1 = 0, rdx = 0, rdy = 0, p_top = 0;
   double mub = 0;
   std::vector<double> pb, alb, phb, t_init;             // profiles
+  std::vector<double> u_base, v_base, qv_base;          // the diffusion's base profiles (model.cpp:149-177)
   std::vector<double> u, v, w, t, ph, mu, p, al;        // dense state
   // hills: the base state in full, terrain, map factors, Coriolis, moisture (dense arrays)
   bool full = false;
@@ -183,6 +184,7 @@ void build(mi_case& c, const mi_case_spec& sp) {
   const double p_surf = 1.0e5;
   const double xcen = 0.5 * nx * G.dx;
   std::vector<double> noise;
+  std::vector<double> exn(static_cast<size_t>(nz));  // the base state's Exner function (set below)
   switch (sp.kind) {
     case MI_CASE_REST_STATE: {
       const double th0 = 300.0, N = 0.01, u0 = sp.u0;
@@ -199,6 +201,19 @@ void build(mi_case& c, const mi_case_spec& sp) {
         if (!(L <= 1.0)) return 0.0;
         const double cs = std::cos(0.5 * kPi * L);
         return dth * (cs * cs);
+      };
+      break;
+    }
+    case MI_CASE_STRAKA: {
+      // neutral 300 K with a cold bubble: a -15 K temperature perturbation, cos-shaped within the ellipse of
+      // radii 4 km x 2 km centred at 3 km (reference straka, cases.cpp:103-113; Straka et al. 1993); the
+      // squares by the library pow, as the reference
+      const double xc = xcen, zc = 3000.0, xr = 4000.0, zr = 2000.0, dT = -15.0;
+      theta_env = [](double) { return 300.0; };
+      pert = [=, &exn](int, int, int k, double x, double, double z) {
+        volatile double two = 2.0;  // glibc pow, not a multiply folded by the compiler
+        const double L = std::sqrt(std::pow((x - xc) / xr, two) + std::pow((z - zc) / zr, two));
+        return L <= 1.0 ? dT * 0.5 * (std::cos(kPi * L) + 1.0) / exn[size_t(k)] : 0.0;
       };
       break;
     }
@@ -255,7 +270,7 @@ void build(mi_case& c, const mi_case_spec& sp) {
   // dry hydrostatic base state and perturbation state (cases.cpp:224-287)
   const double mub = p_surf - c.p_top;
   c.mub = mub;
-  std::vector<double> tb(static_cast<size_t>(nz)), zm(static_cast<size_t>(nz)), exn(static_cast<size_t>(nz));
+  std::vector<double> tb(static_cast<size_t>(nz)), zm(static_cast<size_t>(nz));
   c.pb.assign(size_t(nz), 0.0);
   c.alb.assign(size_t(nz), 0.0);
   c.phb.assign(size_t(nz) + 1, 0.0);
@@ -593,6 +608,17 @@ int mi_case_create(const mi_case_spec* spec, mi_case** out) {
     } else {
       build(*c, *spec);
     }
+    // the diffusion's base profiles: u, v and qv of column (0, 0) (module_initialize_ideal as the reference,
+    // model.cpp:149-177)
+    const int nx = c->g.nx, ny = c->g.ny, nz = c->nz, nxu = nx + 1;
+    c->u_base.assign(size_t(nz), 0.0);
+    c->v_base.assign(size_t(nz), 0.0);
+    c->qv_base.assign(size_t(nz), 0.0);
+    for (int k = 0; k < nz; ++k) {
+      c->u_base[size_t(k)] = c->u[size_t(k) * ny * nxu];
+      c->v_base[size_t(k)] = c->v[size_t(k) * (ny > 1 ? ny + 1 : 1) * nx];
+      if (!c->qv.empty()) c->qv_base[size_t(k)] = c->qv[size_t(k) * ny * nx];
+    }
     *out = c.release();
   });
 }
@@ -668,6 +694,9 @@ int mi_case_get_field(const mi_case* c, int id, double* h, int64_t n) {
       case MI_ALB: put_profile(c->alb); break;
       case MI_PHB: put_profile(c->phb); break;
       case MI_T_INIT: put_profile(c->t_init); break;
+      case MI_U_BASE: put(c->u_base); break;
+      case MI_V_BASE: put(c->v_base); break;
+      case MI_QV_BASE: put(c->qv_base); break;
       case MI_Q:
         if (c->qv.empty()) throw std::invalid_argument("case: dry, no qv");
         put(c->qv);
@@ -718,6 +747,9 @@ int mi_case_load(const mi_case* c, mi_domain* d) {
     chk(mi_domain_set_field(d, MI_MU, c->mu.data(), int64_t(c->mu.size())));
     chk(mi_domain_set_field(d, MI_P, c->p.data(), int64_t(c->p.size())));
     chk(mi_domain_set_field(d, MI_AL, c->al.data(), int64_t(c->al.size())));
+    set(MI_U_BASE, c->u_base);
+    set(MI_V_BASE, c->v_base);
+    set(MI_QV_BASE, c->qv_base);
   });
 }
 

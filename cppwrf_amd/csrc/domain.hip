@@ -260,13 +260,25 @@ struct Domain final : mi_domain {
   }
 
   // SmallStep::run_stage (small_step.cpp:490-607); with prhophi, calc_p_rho_phi (small_step.cpp:609-617)
-  // is fused into the finishing kernel
+  // is fused into the finishing kernel.  In four parts, which a batch interleaves with its members' (ss_begin, then
+  // per substep advance_uv, after_uv, the column kernel, after_acoustic, then ss_end).
+  int stage_ns(int rk) const { return rk == 1 ? 1 : (rk == 2 ? n_sound_ / 2 : n_sound_); }
+  double stage_dts(int rk) const { return rk == 1 ? grid.dt / 3.0 : grid.dt / n_sound_; }
   void small_steps(int rk, bool prhophi) {
+    const double dts = stage_dts(rk);
+    const int ns = stage_ns(rk);
+    ss_begin(rk);
+    for (int it = 1; it <= ns; ++it) {
+      launch_uv(a, dts, stream());
+      after_uv();
+      launch_acoustic(a, colcfg[kColAcoustic], dts, it, ns, stream());
+      after_acoustic(rk, it);
+    }
+    ss_end(rk, prhophi);
+  }
+  void ss_begin(int rk) {
     auto& f = a.f;
-    const double dt = grid.dt, dts_full = dt / n_sound_;
-    const double dt_rk = rk == 1 ? dt / 3.0 : (rk == 2 ? 0.5 * dt : dt);
-    const double dts = rk == 1 ? dt_rk : dts_full;
-    const int ns = rk == 1 ? 1 : (rk == 2 ? n_sound_ / 2 : n_sound_);
+    const double dts = stage_dts(rk);
     hipStream_t st = stream();
     if (rk == 1) launch_fill2d(a, f.mudf - off, n2, 0.0, st);
     launch_ss_prep(a, rk == 1, st);
@@ -278,23 +290,33 @@ struct Domain final : mi_domain {
     if (!spec)
       halo({{f.u2, -1, -1}, {f.v2, -1, -1}, {f.ph2, -1, -1}, {f.t_save, -1, -1}, {f.u_save, -1, -1},
             {f.v_save, -1, -1}, {f.mu2, -1, -1}, {f.muts, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
-    // With specified boundaries the substep refreshes below are left out: no kernel of the substeps reads
-    // those halo points (the spec zone's faces and columns are advanced from their own values, the interior's
-    // stencils reach the spec zone only), and the stage's last refreshes write them before any reader.
-    for (int it = 1; it <= ns; ++it) {
-      launch_uv(a, dts, st);
-      if (spec) {
-      } else if (!a.g.two_d) {
-        halo({{f.u2, -1, -1}, {f.v2, -1, -1}});  // small_step.cpp:540-541
-      } else {
-        halo({{f.u2, -1, -1}});
-      }
-      launch_acoustic(a, colcfg[kColAcoustic], dts, it, ns, st);
-      if (spec) launch_acoustic_ring(a, dts, it, ns, st);  // the spec zone's columns
-      // the refreshes of mu'', mudf, muts and of ph'', p, al (small_step.cpp:572, 593) after the fused kernel
-      if (!spec)
-        halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
+  }
+  // With specified boundaries the substep refreshes below are left out: no kernel of the substeps reads those halo
+  // points (the spec zone's faces and columns are advanced from their own values, the interior's stencils reach
+  // the spec zone only), and the stage's last refreshes write them before any reader.
+  bool needs_after_uv() const { return !spec; }
+  void after_uv() {
+    auto& f = a.f;
+    if (spec) {
+    } else if (!a.g.two_d) {
+      halo({{f.u2, -1, -1}, {f.v2, -1, -1}});  // small_step.cpp:540-541
+    } else {
+      halo({{f.u2, -1, -1}});
     }
+  }
+  // after the substep's column kernel: the spec zone's columns, and the refreshes of mu'', mudf, muts and of ph'', p,
+  // al (small_step.cpp:572, 593)
+  void after_acoustic(int rk, int it) {
+    auto& f = a.f;
+    if (spec) launch_acoustic_ring(a, stage_dts(rk), it, stage_ns(rk), stream());
+    if (!spec)
+      halo({{f.mu2, -1, -1}, {f.mudf, -1, -1}, {f.muts, -1, -1}, {f.ph2, -1, -1}, {f.p, -1, -1}, {f.al, -1, -1}});
+  }
+  void ss_end(int rk, bool prhophi) {
+    auto& f = a.f;
+    hipStream_t st = stream();
+    const double dts = stage_dts(rk);
+    const int ns = stage_ns(rk);
     if (spec) launch_bdy_zero_grad(a, st);  // zero_grad_bdy of the spec zone's w (lateral_bc.cpp:374-387)
     halo({{f.muts, -1, -1}, {f.ru_m, -1, -1}, {f.rv_m, -1, -1}, {f.ww_m, -1, -1}});  // small_step.cpp:597
     // the microphysics heating leaves theta_m at the last stage with moisture (small_step.cpp:602)
@@ -465,7 +487,7 @@ struct Domain final : mi_domain {
     prm.diff_6th_factor = factor;
     a.q.diff_6th_opt = opt;
     a.q.diff_6th_factor = factor;
-    held_base = opt != 0 || spec;
+    held_base = opt != 0 || prm.diff_opt != 0 || spec;
     a.q.use_tendf = (held_base || has_phys) ? 1 : 0;
     drop_graphs();
   }
@@ -613,7 +635,7 @@ struct Domain final : mi_domain {
     for (int kern = 0; kern < kNumColKernels; ++kern) {
       double best_t = 1e300;
       int best = colcfg[kern];
-      for (int cfg = 0; cfg <= kColMarch; ++cfg) {
+      for (int cfg = 0; cfg < kNumColShapes; ++cfg) {
         if (!col_cfg_fits(kern, cfg, grid.nz)) continue;
         colcfg[kern] = cfg;
         prof_name = kColKernelNames[kern];
@@ -693,6 +715,7 @@ struct Domain final : mi_domain {
   mi_ctx* context() const override { return ctx; }
   int n_sound() const override { return n_sound_; }
   int64_t device_bytes() const override { return bytes; }
+  double* base_profile(int id) const;
   void set_field(int id, const double* host, int64_t n) override;
   void get_field(int id, double* host, int64_t n) override;
   void field_view(int id, mi_field_view* out) override;
@@ -745,6 +768,8 @@ struct Domain final : mi_domain {
   }
   // the key of a batch's captured step: the scalar-buffer parity and the version of the launch schedule
   int buffer_parity() const override { return qpar + 2 * launch_version; }
+  int storage_bytes() const override { return int(sizeof(real)); }
+  bool fusable() const override { return !has_callbacks() && batch_fits(a); }
 };
 
 namespace {
@@ -781,6 +806,12 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     throw Error(MI_EINVAL, "mapped and coriolis must be 0 or 1");
   if (p.diff_6th_opt < 0 || p.diff_6th_opt > 2) throw Error(MI_EINVAL, "diff_6th_opt must be 0, 1 or 2");
   if (p.terrain != 0 && p.terrain != 1) throw Error(MI_EINVAL, "terrain must be 0 or 1");
+  if (p.diff_opt == 2) throw Error(MI_EINVAL, "diff_opt 2 (diffusion in physical space) is not built");
+  if (p.diff_opt < 0 || p.diff_opt > 1) throw Error(MI_EINVAL, "diff_opt must be 0 or 1");
+  if (p.diff_opt == 1 && p.km_opt != 1)
+    throw Error(MI_EINVAL, "km_opt must be 1 (constant K); the TKE and Smagorinsky options 2-4 are not built");
+  if (p.diff_opt == 1 && !(p.khdif >= 0 && p.kvdif >= 0 && p.pr_inv >= 0))
+    throw Error(MI_EINVAL, "khdif, kvdif and pr_inv must be >= 0");
   d.ctx = ctx;
   d.grid = g;
   d.prm = p;
@@ -852,7 +883,7 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   add2(f.fcor, St::M2, MI_F); add2(f.ecor, St::M2, MI_E); add2(f.sina, St::M2, MI_SINA); add2(f.cosa, St::M2, MI_COSA);
   launch_fill(f.cosa - d.off, d.n2, 1.0, ctx->stream);
   d.spec = g.bc_x == MI_BC_SPECIFIED;
-  const bool held = p.diff_6th_opt != 0 || d.spec;  // a diffusion driver or a LateralBC holds tendencies
+  const bool held = p.diff_6th_opt != 0 || p.diff_opt != 0 || d.spec;  // a diffusion driver or a LateralBC
   if (held) {
     add3(f.ru_tendf, St::U, MI_RU_TENDF); add3(f.rv_tendf, St::V, MI_RV_TENDF); add3(f.rw_tendf, St::W, MI_RW_TENDF);
     add3(f.t_tendf, St::M, MI_T_TENDF);
@@ -907,6 +938,10 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     f.alb = BField{d.alb_buf, 0, 0, 1};
     f.phb = BField{d.phb_buf, 0, 0, 1};
   }
+  f.t_init = p.terrain ? BField{d.t_init_buf, 1, G.sj, G.sk} : BField{d.t_init_buf, 0, 0, 1};
+  f.u_base = d.alloc(nz + 1);
+  f.v_base = d.alloc(nz + 1);
+  f.qv_base = d.alloc(nz + 1);
   {
     void* fl = nullptr;
     hip_check(hipMalloc(&fl, 64 * sizeof(int)), "hipMalloc");
@@ -953,6 +988,14 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   q.spec = d.spec ? 1 : 0;
   q.diff_6th_opt = p.diff_6th_opt;
   q.diff_6th_factor = p.diff_6th_factor;
+  q.diff_opt = p.diff_opt;
+  if (p.diff_opt == 1) {  // DiffusionDriver's constants (model.cpp:314, km_kh.hpp:35-45)
+    const double prandtl = 1.0 / (p.pr_inv > 0 ? p.pr_inv : 3.0);
+    q.xkmh = p.khdif;
+    q.xkhh = p.khdif / prandtl;
+    q.kvdif = p.kvdif;
+    q.kvdq = p.kvdif / prandtl;
+  }
   set_dt_params(q, g.dt, p.w_damping == 1, d.spec);
   q.terrain = p.terrain;
 }
@@ -991,9 +1034,19 @@ void copy_out(std::vector<double>& h, const void* dev, int esize, hipStream_t st
 
 }  // namespace
 
+double* Domain::base_profile(int id) const {
+  return id == MI_U_BASE ? a.f.u_base : id == MI_V_BASE ? a.f.v_base : id == MI_QV_BASE ? a.f.qv_base : nullptr;
+}
+
 void Domain::set_field(int id, const double* host, int64_t n) {
   hipStream_t st = ctx->stream;
   const int nz = grid.nz;
+  if (double* bp = base_profile(id)) {  // the diffusion's 1-D base profiles
+    if (n != nz) throw Error(MI_EINVAL, "base profiles take nz values");
+    hip_check(hipMemcpyAsync(bp, host, size_t(n) * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return;
+  }
   const bool base3 = id == MI_PB || id == MI_ALB || id == MI_PHB || id == MI_T_INIT;
   // a base 3-D profile on a domain with terrain: repeated over every column, halos included
   if (base3 && prm.terrain && n == nz + (id == MI_PHB ? 1 : 0)) {
@@ -1108,12 +1161,19 @@ void Domain::field_bind(int id, void* dev, int64_t nbytes) {
   if (pb_buf && id == MI_PB) a.f.pb.p = pb_buf;
   if (alb_buf && id == MI_ALB) a.f.alb.p = alb_buf;
   if (phb_buf && id == MI_PHB) a.f.phb.p = phb_buf;
+  if (t_init_buf && id == MI_T_INIT) a.f.t_init.p = t_init_buf;
   halo({{*fd->slot, -1, -1}});
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
   drop_graphs();  // the captured steps name the old buffer
 }
 
 void Domain::get_field(int id, double* host, int64_t n) {
+  if (double* bp = base_profile(id)) {
+    if (n != grid.nz) throw Error(MI_EINVAL, "base profiles hold nz values");
+    hip_check(hipMemcpyAsync(host, bp, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return;
+  }
   FieldDesc* fd = find(this, id);
   if (!fd) throw Error(MI_EINVAL, "field id " + std::to_string(id) + " cannot be read");
   const int fnx = nxs(fd->st), fny = nys(fd->st), fnz = nzs(fd->st);
@@ -1250,6 +1310,106 @@ mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const 
   build_domain(*d, ctx, g, p, lv);
   hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return d.release();
+}
+
+// ============================================================================= fused batch steps
+// A batch whose members share this arithmetic and the acoustic step count steps as one schedule: the phases every
+// member runs on a branch of its own (the stage's prep, tendencies, acoustic prologue and finish, the scalars, the
+// halo refreshes), and each acoustic substep's two kernels (advance_uv, the column kernel) as one launch over every
+// member's blocks.  The device buffer holds the members' Args, then six tables (advance_uv and the column kernel
+// of each stage; n + 1 entries each).
+size_t fused_bytes(int n) { return size_t(n) * sizeof(Args) + 6 * size_t(n + 1) * sizeof(BatchMember); }
+
+void fused_prepare(mi_domain* const* ms, int n, void* dev, hipStream_t st, FusedInfo& info) {
+  if (n > kMaxBatch) throw Error(MI_EINVAL, "internal: too many members for a fused batch");
+  std::vector<const Args*> as(static_cast<size_t>(n));
+  std::vector<Args> args(static_cast<size_t>(n));
+  info.nzmax = 0;
+  info.ns = static_cast<Domain*>(ms[0])->n_sound_;
+  for (int q = 0; q < n; ++q) {
+    Domain* d = static_cast<Domain*>(ms[q]);
+    args[size_t(q)] = d->a;
+    as[size_t(q)] = &d->a;
+    info.nzmax = std::max(info.nzmax, d->grid.nz);
+  }
+  std::vector<BatchMember> tabs(6 * size_t(n + 1));
+  for (int rk = 1; rk <= 3; ++rk) {
+    std::vector<double> dts(static_cast<size_t>(n));
+    for (int q = 0; q < n; ++q) dts[size_t(q)] = static_cast<Domain*>(ms[q])->stage_dts(rk);
+    info.blocks_uv[rk - 1] = batch_table_uv(as.data(), dts.data(), n, &tabs[size_t(rk - 1) * size_t(n + 1)]);
+    info.blocks_ac[rk - 1] = batch_table_acoustic(as.data(), dts.data(), n, &tabs[size_t(rk + 2) * size_t(n + 1)]);
+  }
+  char* p = static_cast<char*>(dev);
+  hip_check(hipMemcpyAsync(p, args.data(), args.size() * sizeof(Args), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(p + args.size() * sizeof(Args), tabs.data(), tabs.size() * sizeof(BatchMember),
+                           hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+void fused_capture(mi_domain* const* ms, int n, hipStream_t s0, const hipStream_t* st, hipEvent_t fork,
+                   const hipEvent_t* join, const void* dev, const FusedInfo& info) {
+  std::vector<Domain*> d(static_cast<size_t>(n));
+  std::vector<Scalars> keep_sc(static_cast<size_t>(n));
+  std::vector<int> keep_q(static_cast<size_t>(n));
+  for (int q = 0; q < n; ++q) {
+    d[size_t(q)] = static_cast<Domain*>(ms[q]);
+    keep_sc[size_t(q)] = d[size_t(q)]->sc;
+    keep_q[size_t(q)] = d[size_t(q)]->qpar;
+  }
+  const Args* dev_args = static_cast<const Args*>(dev);
+  const BatchMember* tabs = reinterpret_cast<const BatchMember*>(static_cast<const char*>(dev) + size_t(n) * sizeof(Args));
+  // every member's part of a phase on its own branch, forked from and joined back to s0
+  auto branch = [&](auto&& fn) {
+    hip_check(hipEventRecord(fork, s0), "hipEventRecord");
+    for (int q = 0; q < n; ++q) {
+      hip_check(hipStreamWaitEvent(st[q], fork, 0), "hipStreamWaitEvent");
+      d[size_t(q)]->launch_stream = st[q];
+      fn(*d[size_t(q)]);
+      d[size_t(q)]->launch_stream = nullptr;
+      hip_check(hipEventRecord(join[q], st[q]), "hipEventRecord");
+      hip_check(hipStreamWaitEvent(s0, join[q], 0), "hipStreamWaitEvent");
+    }
+  };
+  bool any_after_uv = false;
+  for (Domain* m : d) any_after_uv = any_after_uv || m->needs_after_uv();
+  try {
+    branch([](Domain& m) { m.step_begin(); });
+    for (int rk = 1; rk <= 3; ++rk) {
+      const int ns = rk == 1 ? 1 : (rk == 2 ? info.ns / 2 : info.ns);
+      branch([rk](Domain& m) {
+        m.rk_prep(rk);
+        m.rk_tendency(rk, false);
+        m.ss_begin(rk);
+      });
+      for (int it = 1; it <= ns; ++it) {
+        launch_uv_batch(dev_args, tabs + size_t(rk - 1) * size_t(n + 1), n, info.blocks_uv[rk - 1], s0);
+        if (any_after_uv) branch([](Domain& m) { m.after_uv(); });
+        launch_acoustic_batch(dev_args, tabs + size_t(rk + 2) * size_t(n + 1), n, info.blocks_ac[rk - 1], info.nzmax, it, ns,
+                              s0);
+        branch([rk, it](Domain& m) {
+          launch_open_sums(m.a, it, m.stage_ns(rk), m.stream());
+          m.after_acoustic(rk, it);
+        });
+      }
+      branch([rk](Domain& m) {
+        m.ss_end(rk, true);
+        m.scalar_transport(rk);
+      });
+    }
+    branch([](Domain& m) { m.step_end(false); });
+  } catch (...) {
+    for (int q = 0; q < n; ++q) {
+      d[size_t(q)]->launch_stream = nullptr;
+      d[size_t(q)]->sc = keep_sc[size_t(q)];
+      d[size_t(q)]->qpar = keep_q[size_t(q)];
+    }
+    throw;
+  }
+  // the host side of the captured step runs at each replay (advance_host): put the buffer names back
+  for (int q = 0; q < n; ++q) {
+    d[size_t(q)]->sc = keep_sc[size_t(q)];
+    d[size_t(q)]->qpar = keep_q[size_t(q)];
+  }
 }
 
 }  // namespace MI_NS

@@ -64,11 +64,30 @@ struct mi_domain {
   virtual void capture_step(hipStream_t s) = 0;
   virtual void advance_host() = 0;
   virtual int buffer_parity() const = 0;
+  // fused batch steps: the storage size of the bulk fields (4 or 8) and whether this domain can join one (no hooks,
+  // its levels fit the batched column kernel)
+  virtual int storage_bytes() const = 0;
+  virtual bool fusable() const = 0;
 };
 
+// what a fused batch step launches with: the acoustic step count, the largest nz, each stage's block counts
+struct FusedInfo {
+  int ns = 0, nzmax = 0;
+  int blocks_uv[3] = {0, 0, 0}, blocks_ac[3] = {0, 0, 0};
+};
+
+// one implementation per arithmetic: domains, and the fused batch steps of members of that arithmetic (fused_bytes
+// of device memory for n members; fused_prepare fills it, before a capture; fused_capture launches one step)
+#define MI_DOMAIN_NS_API                                                                                          \
+  mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv);              \
+  size_t fused_bytes(int n);                                                                                     \
+  void fused_prepare(mi_domain* const* ms, int n, void* dev, hipStream_t st, FusedInfo& info);                  \
+  void fused_capture(mi_domain* const* ms, int n, hipStream_t s0, const hipStream_t* st, hipEvent_t fork,        \
+                     const hipEvent_t* join, const void* dev, const FusedInfo& info);
 namespace mi_fp64 {
-mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv);
+MI_DOMAIN_NS_API
+constexpr int kMaxFusedMembers = 64;  // the batched kernels' member tables
 }
 namespace mi_fp32 {
-mi_domain* make_domain(mi_ctx* ctx, const mi_grid& g, const mi_params& p, const mi_levels& lv);
+MI_DOMAIN_NS_API
 }

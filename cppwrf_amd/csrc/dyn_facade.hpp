@@ -66,7 +66,8 @@ struct ScalarParams {
 };
 // What the reference's BaseState and drivers carry beside the parameter structs: the number of moist
 // species and tracers in State::q_2 (state.hpp:53-60), BaseState::mapped / coriolis (state.hpp:93-107),
-// a terrain-following base state, the sixth-order diffusion of DiffusionDriver (diffusion/sixth_order.hpp)
+// a terrain-following base state, the diffusion of DiffusionDriver (its DiffusionConfig's diff_opt 1 with
+// km_opt 1, khdif, kvdif and 1 / prandtl, diffusion_driver.hpp:23-51; and diff_6th_opt, diffusion/sixth_order.hpp)
 // and the storage precision of the bulk 3-D fields.
 struct DomainOptions {
   int n_moist = 0, n_scalars = 0;
@@ -74,6 +75,8 @@ struct DomainOptions {
   int diff_6th_opt = 0;
   double diff_6th_factor = 0.12;
   int precision = MI_FP64;
+  int diff_opt = 0, km_opt = 1;
+  double khdif = 0, kvdif = 0, pr_inv = 3.0;
 };
 
 // number of values of a field id in the dense layout of the C ABI
@@ -89,6 +92,7 @@ inline size_t field_size(int field, const mi_grid& g) {
     case MI_COSA: case MI_HT:
       return nx * ny;
     case MI_MSFUX: case MI_MSFUY: return (nx + 1) * ny;
+    case MI_U_BASE: case MI_V_BASE: case MI_QV_BASE: return nz;
     case MI_MSFVX: case MI_MSFVY: case MI_MSFVX_INV: return nx * (ny + vy);
     default: return nx * ny * nz;  // mass points: t, p, al, alt, php, tendencies, h_diabatic, scalars
   }
@@ -322,6 +326,11 @@ class RK3 {
     p.diff_6th_opt = opt.diff_6th_opt;
     p.terrain = opt.terrain ? 1 : 0;
     p.diff_6th_factor = opt.diff_6th_factor;
+    p.diff_opt = opt.diff_opt;
+    p.km_opt = opt.km_opt;
+    p.khdif = opt.khdif;
+    p.kvdif = opt.kvdif;
+    p.pr_inv = opt.pr_inv;
     check(mi_domain_create(ctx.get(), &g, &p, &levels, &d_));
   }
   ~RK3() { mi_domain_destroy(d_); }

@@ -278,7 +278,7 @@ __device__ __forceinline__ void put(T* p, const Geo& g, int i, int j, idx_t kofs
 // RK3::step's _2 -> _1 copies of whole allocations, halos included (reference rk3.cpp:94-98)
 __global__ void k_copy_step(Fields f, const double* base3, const double* base2, int64_t n3, int64_t n2,
                             int64_t off3_u, int64_t off3_v, int64_t off3_w, int64_t off3_t, int64_t off3_ph,
-                            int64_t off2_mu) {
+                            int64_t off2_mu, int64_t sj2, int64_t pitch2) {
   (void)base3;
   (void)base2;
   const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -295,9 +295,10 @@ __global__ void k_copy_step(Fields f, const double* base3, const double* base2, 
       default: dst = f.ph1; src = f.ph2; off = off3_ph; break;
     }
     (dst - off)[e] = (src - off)[e];
-  } else {
+  } else {  // mu, a 2-D field: the first pitch values of each row of sj (n2 = rows x pitch of them)
     if (e >= n2) return;
-    (f.mu1 - off2_mu)[e] = (f.mu2 - off2_mu)[e];
+    const int64_t q = sj2 > 0 ? (e / pitch2) * sj2 + e % pitch2 : e;
+    (f.mu1 - off2_mu)[q] = (f.mu2 - off2_mu)[q];
   }
 }
 
@@ -1223,8 +1224,7 @@ __device__ __forceinline__ void uv_fp32(const Args& a, int i, int j, int k, doub
 
 // KUV (small_step.cpp:151-190)
 template <bool OPEN>
-MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
-  MI_POINT_INDEX(r)
+__device__ __forceinline__ void uv_at(const Args& a, int i, int j, int k, double dts) {
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
@@ -1271,6 +1271,11 @@ MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
     const double mudf_y = -a.q.emdy * (f.mudf[x2] - f.mudf[x2 - sj]) * f.msfvx_inv[x2];
     put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x] - dts * (a.q.moist ? f.cqv[x] : 1.0) * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) + c.c1h[k] * mudf_y);
   }
+}
+template <bool OPEN>
+MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
+  MI_POINT_INDEX(r)
+  uv_at<OPEN>(a, i, j, k, dts);
 }
 
 // ----------------------------------------------------------------------------- one acoustic substep's column work
@@ -1324,12 +1329,11 @@ __device__ __forceinline__ double wave_affine_scan(double p, double q, int lane)
 //   9  KWPRho (small_step.cpp:409-414)
 // With specified boundaries the spec zone's columns (Edge::mass, small_step.cpp:39-45) are skipped here and
 // advanced by k_acoustic_ring.
-MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
-  extern __shared__ double lds[];
+template <int CI, int KT>
+__device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, double* lds, double dts, int it, int ns) {
   const Geo& g = a.g;
   const Fields& f = a.f;
   const Lv& c = a.c;
-  const ColPos cp = col_pos<CI>();
   const int i = cp.i, j = cp.j, nz = g.nz, ci = cp.ci, k = cp.kt;
   // with specified boundaries the spec zone's columns are left to k_acoustic_ring
   const bool ring = g.bcx == MI_BC_SPECIFIED && (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1);
@@ -1664,6 +1668,10 @@ MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
     put<kHMass>(f.p, g, i, j, k * sk, pp);
   }
 }
+MI_COLUMN_KERNEL k_acoustic(Args a, double dts, int it, int ns) {
+  extern __shared__ double lds[];
+  acoustic_cols<CI, KT>(a, col_pos<CI>(), lds, dts, it, ns);
+}
 
 // ----------------------------------------------------------------------------- one acoustic substep, k-march
 // The substep's column work (the same phases and the same expressions as k_acoustic) with one thread per column: a
@@ -1928,6 +1936,47 @@ __global__ void __launch_bounds__(64) k_acoustic_march(Args a, double dts, int i
     }
     ph_up = phk;
   }
+}
+
+// ----------------------------------------------------------------------------- batched substep launches
+// Every member of a batch in one launch (the acoustic substeps of an ensemble advance together): block b of the
+// launch's 1-D grid belongs to the member m with tab[m].start <= b < tab[m + 1].start and is that member's block
+// b - tab[m].start; the member's Args and the stage's dts come from device tables the batch keeps.  The blocks of
+// one member are contiguous, so the XCD remap keeps a member's neighbouring blocks on one XCD.
+__device__ __forceinline__ int batch_member(const BatchMember* tab, int n, int blk) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].start <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+// advance_uv of every member (point blocks of 64 x 4 over each member's uv range)
+MI_POINT_KERNEL k_uv_batch(const Args* as, const BatchMember* tab, int n) {
+  const Blk b = xcd_block();
+  const int m = batch_member(tab, n, b.x);
+  const BatchMember& t = tab[m];
+  const int l = b.x - t.start;
+  const int bx = l % t.gx, by = (l / t.gx) % t.gy, bz = l / (t.gx * t.gy);
+  const int i = t.i0 + bx * kBx + int(threadIdx.x), j = t.j0 + by * int(blockDim.y) + int(threadIdx.y), k = t.k0 + bz;
+  if (i >= t.i1 || j >= t.j1) return;
+  uv_at<true>(as[m], i, j, k, t.dts);  // the open-boundary forms are taken only where a member's grid is open
+}
+// the substep's column kernel of every member (blocks of CI columns x KT levels; KT >= nz + 1 of every member)
+template <int CI, int KT>
+__global__ void __launch_bounds__(CI * KT) k_acoustic_batch(const Args* as, const BatchMember* tab, int n, int it, int ns) {
+  extern __shared__ double lds[];
+  const Blk b = xcd_block();
+  const int m = batch_member(tab, n, b.x);
+  const BatchMember& t = tab[m];
+  const int l = b.x - t.start;
+  ColPos cp;
+  cp.ci = int(threadIdx.x) % CI;
+  cp.kt = int(threadIdx.x) / CI;
+  cp.i = (l % t.gx) * CI + cp.ci;
+  cp.j = l / t.gx;
+  acoustic_cols<CI, KT>(as[m], cp, lds, t.dts, it, ns);
 }
 
 // KSumEdge (small_step.cpp:279-290, 560-569) of an open direction, after each substep's column kernel: the east
@@ -2236,6 +2285,194 @@ __device__ __forceinline__ bool sixth_in(const Geo& g, int diff, char kind, int 
   return i >= is && i < ie && j >= js && j < je;
 }
 
+// ----------------------------------------------------------------------------- diff_opt 1 (constant K)
+// The second-order diffusion along eta of DiffusionDriver with diff_opt 1, km_opt 1 (reference
+// src/dyn/diffusion/big_step_diffusion.hpp; WRF module_big_step_utilities_em.F): every eddy viscosity is a constant
+// (xkmh = khdif, xkhh = khdif / prandtl), each face average of them is evaluated as the reference's sum of equal
+// terms so the rounding is the same.  In an x-z slice the y terms are left out: rdy = 0 makes them zeros that
+// leave the sum's bits unchanged (they are added to a tendency that is never -0).
+
+// the loop limits of horizontal_diffusion (big_step_diffusion.hpp:343-414): one row in from open and specified
+// sides, a periodic x resetting the x limits
+__device__ __forceinline__ bool hdiff_in(const Geo& g, char kind, int i, int j) {
+  const bool fx = g.ox() || g.spec(), fy = g.oy() || g.spec();
+  int is = fx ? 1 : 0, js = fy ? 1 : 0, ie, je;
+  if (kind == 'u') {
+    ie = fx ? g.nx : g.nx + 1;
+    je = fy ? g.ny - 1 : g.ny;
+  } else if (kind == 'v') {
+    ie = fx ? g.nx - 1 : g.nx;
+    je = fy ? g.ny : g.ny + 1;
+  } else {
+    ie = fx ? g.nx - 1 : g.nx;
+    je = fy ? g.ny - 1 : g.ny;
+  }
+  if (g.bcx == MI_BC_PERIODIC) {
+    is = 0;
+    ie = kind == 'u' ? g.nx + 1 : g.nx;
+  }
+  return i >= is && i < ie && j >= js && j < je;
+}
+
+// horizontal_diffusion (KHorizDiffU1 / V1 / W1 / M1, big_step_diffusion.hpp:26-186): tend + the flux differences
+// of field q at index x (2-D index x2) with the constant K; mu the column mass (mut, or muts for the scalars), c1 /
+// c2 the level's mass coefficients (c1f / c2f for 'w')
+template <char KIND>
+__device__ __forceinline__ double hdiff_at(double tend, const Args& a, const real* q, const double* mu, idx_t x,
+                                           idx_t x2, double c1, double c2, double K) {
+  const Fields& f = a.f;
+  const idx_t sj = a.g.sj;
+  const double rdx = a.c.rdx, rdy = a.c.rdy;
+  const bool do_y = !a.g.two_d;
+  auto m = [&](int di, int dj) { return c1 * mu[x2 + di + dj * sj] + c2; };
+  auto M = [&](const double* p, int di, int dj) { return p[x2 + di + dj * sj]; };
+  const double K4 = K + K + K + K, K2 = K + K;
+  double mkrdxm, mkrdxp, mrdx, mkrdym = 0.0, mkrdyp = 0.0, mrdy = 0.0;
+  if (KIND == 'u') {
+    mkrdxm = (M(f.msftx, -1, 0) / M(f.msfty, -1, 0)) * m(-1, 0) * K * rdx;
+    mkrdxp = (M(f.msftx, 0, 0) / M(f.msfty, 0, 0)) * m(0, 0) * K * rdx;
+    mrdx = M(f.msfux, 0, 0) * M(f.msfuy, 0, 0) * rdx;
+    if (do_y) {
+      mkrdym = ((M(f.msfuy, 0, 0) + M(f.msfuy, 0, -1)) / (M(f.msfux, 0, 0) + M(f.msfux, 0, -1))) * 0.25 *
+               (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * K4 * rdy;
+      mkrdyp = ((M(f.msfuy, 0, 0) + M(f.msfuy, 0, 1)) / (M(f.msfux, 0, 0) + M(f.msfux, 0, 1))) * 0.25 *
+               (m(0, 0) + m(0, 1) + m(-1, 1) + m(-1, 0)) * 0.25 * K4 * rdy;
+      mrdy = M(f.msfux, 0, 0) * M(f.msfuy, 0, 0) * rdy;
+    }
+  } else if (KIND == 'v') {  // no mass factor in y (WRF v4.8.0 lines 2854-2856)
+    mkrdxm = ((M(f.msfvx, 0, 0) + M(f.msfvx, -1, 0)) / (M(f.msfvy, 0, 0) + M(f.msfvy, -1, 0))) * 0.25 *
+             (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * K4 * rdx;
+    mkrdxp = ((M(f.msfvx, 0, 0) + M(f.msfvx, 1, 0)) / (M(f.msfvy, 0, 0) + M(f.msfvy, 1, 0))) * 0.25 *
+             (m(0, 0) + m(0, -1) + m(1, -1) + m(1, 0)) * 0.25 * K4 * rdx;
+    mrdx = M(f.msfvx, 0, 0) * M(f.msfvy, 0, 0) * rdx;
+    mkrdym = (M(f.msfty, 0, -1) / M(f.msftx, 0, -1)) * K * rdy;
+    mkrdyp = (M(f.msfty, 0, 0) / M(f.msftx, 0, 0)) * K * rdy;
+    mrdy = M(f.msfvx, 0, 0) * M(f.msfvy, 0, 0) * rdy;
+  } else if (KIND == 'w') {
+    mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.25 * (m(0, 0) + m(-1, 0) + m(0, 0) + m(-1, 0)) * 0.25 * K4 * rdx;
+    mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.25 * (m(1, 0) + m(0, 0) + m(1, 0) + m(0, 0)) * 0.25 * K4 * rdx;
+    mrdx = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdx;
+    if (do_y) {
+      mkrdym = (M(f.msfvy, 0, 0) * M(f.msfvx_inv, 0, 0)) * 0.25 * (m(0, 0) + m(0, -1) + m(0, 0) + m(0, -1)) * 0.25 *
+               K4 * rdy;
+      mkrdyp = (M(f.msfvy, 0, 1) * M(f.msfvx_inv, 0, 1)) * 0.25 * (m(0, 1) + m(0, 0) + m(0, 1) + m(0, 0)) * 0.25 *
+               K4 * rdy;
+      mrdy = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdy;
+    }
+  } else {
+    mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
+    mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.5 * K2 * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
+    mrdx = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdx;
+    if (do_y) {
+      mkrdym = (M(f.msfvy, 0, 0) * M(f.msfvx_inv, 0, 0)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
+      mkrdyp = (M(f.msfvy, 0, 1) * M(f.msfvx_inv, 0, 1)) * 0.5 * K2 * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
+      mrdy = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdy;
+    }
+  }
+  const double q0 = q[x];
+  const double tx = mrdx * (mkrdxp * (double(q[x + 1]) - q0) - mkrdxm * (q0 - double(q[x - 1])));
+  if (!do_y) return tend + tx;
+  return tend + (tx + mrdy * (mkrdyp * (double(q[x + sj]) - q0) - mkrdym * (q0 - double(q[x - sj]))));
+}
+
+// horizontal_diffusion_3dmp (KHorizDiff3DMP, big_step_diffusion.hpp:188-219): theta minus t_init, the differences
+// evaluated left to right as WRF
+__device__ __forceinline__ double hdiff_3dmp_at(double tend, const Args& a, int i, int j, int k, idx_t x, idx_t x2) {
+  const Fields& f = a.f;
+  const idx_t sj = a.g.sj;
+  const double rdx = a.c.rdx, rdy = a.c.rdy, K = a.q.xkhh, c1 = a.c.c1h[k], c2 = a.c.c2h[k];
+  auto m = [&](int di, int dj) { return c1 * f.mut[x2 + di + dj * sj] + c2; };
+  auto M = [&](const double* p, int di, int dj) { return p[x2 + di + dj * sj]; };
+  auto Q = [&](int di, int dj) { return double(f.t2[x + di + dj * sj]); };
+  auto B = [&](int di, int dj) { return f.t_init(i + di, j + dj, k); };
+  const double K2 = K + K;
+  const double mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
+  const double mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.5 * K2 * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
+  const double mrdx = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdx;
+  const double dxp = Q(1, 0) - Q(0, 0) - B(1, 0) + B(0, 0);
+  const double dxm = Q(0, 0) - Q(-1, 0) - B(0, 0) + B(-1, 0);
+  const double tx = mrdx * (mkrdxp * dxp - mkrdxm * dxm);
+  if (a.g.two_d) return tend + tx;
+  const double mkrdym = (M(f.msfvy, 0, 0) * M(f.msfvx_inv, 0, 0)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
+  const double mkrdyp = (M(f.msfvy, 0, 1) * M(f.msfvx_inv, 0, 1)) * 0.5 * K2 * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
+  const double mrdy = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdy;
+  const double dyp = Q(0, 1) - Q(0, 0) - B(0, 1) + B(0, 0);
+  const double dym = Q(0, 0) - Q(0, -1) - B(0, 0) + B(0, -1);
+  return tend + (tx + mrdy * (mkrdyp * dyp - mkrdym * dym));
+}
+
+// vertical_diffusion_u / _v (KVertDiffU1 / V1, big_step_diffusion.hpp:283-338): at a face of level k; vflux[kk] (kk
+// < nz - 1) the flux between levels kk and kk + 1 with alt averaged onto the face and the base profile subtracted;
+// no tendency at the top level
+template <char KIND>
+__device__ __forceinline__ double vdiff_face(double tend, const Args& a, idx_t x, idx_t x2, int k) {
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nz = a.g.nz;
+  const idx_t sk = a.g.sk, o = KIND == 'u' ? 1 : a.g.sj;  // o: the mass point across the face
+  const real* q = KIND == 'u' ? f.u2 : f.v2;
+  const double* bp = KIND == 'u' ? f.u_base : f.v_base;
+  const double* mu = KIND == 'u' ? f.muu : f.muv;
+  const double kv = a.q.kvdif;
+  auto AL = [&](idx_t y) { return double(f.alt[y]); };
+  auto vf = [&](int kk) {
+    const idx_t y = x + idx_t(kk - k) * sk;
+    const double av = 0.25 * (AL(y) + AL(y - o) + AL(y + sk) + AL(y - o + sk));
+    const double dp = double(q[y + sk]) - double(q[y]) - bp[kk + 1] + bp[kk];
+    return kv * c.rdn[kk + 1] / av * dp;
+  };
+  if (k >= nz - 1) return tend;
+  const double mm = c.c1h[k] * mu[x2] + c.c2h[k];
+  const double av = 0.5 * (AL(x - o) + AL(x));
+  if (k == 0) {
+    const double v0 = vf(0);
+    return tend + kG * kG * c.rdnw[0] / mm / av * (v0 - v0);
+  }
+  return tend + kG * kG * c.rdnw[k] / mm / av * (vf(k) - vf(k - 1));
+}
+
+// vertical_diffusion 'w' (KVertDiffW1, big_step_diffusion.hpp:225-245): w level k in 1..nz-1, the fluxes at the
+// mass levels (0 at the top one)
+__device__ __forceinline__ double vdiff_w(double tend, const Args& a, idx_t x, idx_t x2, int k) {
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nz = a.g.nz;
+  const idx_t sk = a.g.sk;
+  const double kv = a.q.kvdif;
+  auto vf = [&](int kk) {
+    if (kk >= nz - 1) return 0.0;
+    const idx_t y = x + idx_t(kk - k) * sk;
+    return (kv / double(f.alt[y])) * c.rdnw[kk] * (double(f.w2[y + sk]) - double(f.w2[y]));
+  };
+  const double mm = c.c1f[k] * f.mut[x2] + c.c2f[k];
+  const double av = 0.5 * (double(f.alt[x]) + double(f.alt[x - sk]));
+  return tend + c.rdn[k] * kG * kG / mm / av * (vf(k) - vf(k - 1));
+}
+
+// vertical_diffusion 'm' and vertical_diffusion_mp (KVertDiffM1 / MP1, big_step_diffusion.hpp:247-281): mass level
+// k of field q; base(kk) the base state subtracted from the differences (mp) when BASE
+template <bool BASE, class B>
+__device__ __forceinline__ double vdiff_m(double tend, const Args& a, const real* q, const double* mu, B base, double kv,
+                                          idx_t x, idx_t x2, int k) {
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nz = a.g.nz;
+  const idx_t sk = a.g.sk;
+  auto vf = [&](int kk) {
+    if (kk >= nz - 1) return 0.0;
+    const idx_t y = x + idx_t(kk - k) * sk;
+    const double dq = BASE ? double(q[y + sk]) - double(q[y]) - base(kk + 1) + base(kk) : double(q[y + sk]) - double(q[y]);
+    return kv * c.rdn[kk + 1] / (0.5 * (double(f.alt[y]) + double(f.alt[y + sk]))) * dq;
+  };
+  const double mm = c.c1h[k] * mu[x2] + c.c2h[k];
+  const double al = f.alt[x];
+  if (k == 0) {
+    const double v0 = vf(0);
+    return tend + kG * kG / mm / al * c.rdnw[0] * (v0 - v0);
+  }
+  return tend + kG * kG / mm / al * c.rdnw[k] * (vf(k) - vf(k - 1));
+}
+
 // zero_tendf + DiffusionDriver::dynamics (diffusion_driver.cpp:266-281): the held tendencies of u, v, w,
 // theta (and ph with specified boundaries) over the (nx+1) x nyv x (nz+1) points, 0 where
 // sixth_order_diffusion does not reach (w levels 0, nz; the specified edges; everywhere without it)
@@ -2245,25 +2482,47 @@ MI_POINT_KERNEL k_sixth_dyn(Range r, Args a, double coef, int mode) {
   const Fields& f = a.f;
   const Lv& c = a.c;
   const int nx = g.nx, ny = g.ny, nz = g.nz, diff = mode != kSixthZeroOnly && a.q.diff_6th_opt != 0;
+  const bool d2 = mode != kSixthZeroOnly && a.q.diff_opt == 1;
   const bool keep = mode == kSixthKeep;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
-  // tendency(i,j,k) = tendency + tend_x + tend_y with the tendency just zeroed, or holding 0 + the physics
-  // tendency of part2 (update_phy_ten adds into the zeroed field)
+  // tendency(i,j,k) = tendency + the diff_opt 1 terms (horizontal, then vertical) + tend_x + tend_y of the
+  // sixth-order diffusion, with the tendency just zeroed, or holding 0 + the physics tendency of part2
+  // (update_phy_ten adds into the zeroed field)
   if (j < ny && k < nz) {
-    const double t0 = keep ? 0.0 + double(f.ru_tendf[x]) : 0.0;
-    f.ru_tendf[x] = sixth_in(g, diff, 'u', i, j) ? sixth_at<'u'>(t0, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
+    double t = keep ? 0.0 + double(f.ru_tendf[x]) : 0.0;
+    if (d2) {
+      if (hdiff_in(g, 'u', i, j)) t = hdiff_at<'u'>(t, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], a.q.xkmh);
+      // vertical_diffusion_u: all j, i one in from open / specified sides (a periodic x: all faces)
+      const bool fx = (g.ox() || g.spec()) && g.bcx != MI_BC_PERIODIC;
+      if (i >= (fx ? 1 : 0) && i < (fx ? nx : nx + 1)) t = vdiff_face<'u'>(t, a, x, x2, k);
+    }
+    f.ru_tendf[x] = sixth_in(g, diff, 'u', i, j) ? sixth_at<'u'>(t, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t;
   }
   if (!g.two_d && i < nx && j < g.nyv && k < nz) {
-    const double t0 = keep ? 0.0 + double(f.rv_tendf[x]) : 0.0;
-    f.rv_tendf[x] = sixth_in(g, diff, 'v', i, j) ? sixth_at<'v'>(t0, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
+    double t = keep ? 0.0 + double(f.rv_tendf[x]) : 0.0;
+    if (d2) {
+      if (hdiff_in(g, 'v', i, j)) t = hdiff_at<'v'>(t, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], a.q.xkmh);
+      const bool fy = g.oy() || g.spec();  // vertical_diffusion_v: all i, j one in from open / specified sides
+      if (j >= (fy ? 1 : 0) && j < (fy ? ny : ny + 1)) t = vdiff_face<'v'>(t, a, x, x2, k);
+    }
+    f.rv_tendf[x] = sixth_in(g, diff, 'v', i, j) ? sixth_at<'v'>(t, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t;
   }
   if (i < nx && j < ny) {
-    const bool in = sixth_in(g, diff, 'm', i, j);
-    const double w0 = keep ? 0.0 + double(f.rw_tendf[x]) : 0.0;
-    f.rw_tendf[x] = (in && k >= 1 && k < nz) ? sixth_at<'m'>(w0, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], coef) : w0;
+    const bool in = sixth_in(g, diff, 'm', i, j), hin = d2 && hdiff_in(g, 'm', i, j);
+    double w = keep ? 0.0 + double(f.rw_tendf[x]) : 0.0;
+    if (d2 && k >= 1 && k < nz) {
+      if (hin) w = hdiff_at<'w'>(w, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], a.q.xkmh);
+      w = vdiff_w(w, a, x, x2, k);
+    }
+    f.rw_tendf[x] = (in && k >= 1 && k < nz) ? sixth_at<'m'>(w, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], coef) : w;
     if (k < nz) {
-      const double t0 = keep ? 0.0 + double(f.t_tendf[x]) : 0.0;
-      f.t_tendf[x] = in ? sixth_at<'m'>(t0, a, f.t2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
+      double t = keep ? 0.0 + double(f.t_tendf[x]) : 0.0;
+      if (d2) {  // horizontal_diffusion_3dmp, then vertical_diffusion_mp with kvdq = 3 kvdif (WRF rk_tendency)
+        if (hin) t = hdiff_3dmp_at(t, a, i, j, k, x, x2);
+        const BField tb = f.t_init;
+        t = vdiff_m<true>(t, a, f.t2, f.mut, [&](int kk) { return tb(i, j, kk); }, 3.0 * a.q.kvdif, x, x2, k);
+      }
+      f.t_tendf[x] = in ? sixth_at<'m'>(t, a, f.t2, f.mut, x, x2, c.c1h[k], c.c2h[k], coef) : t;
     }
     if (a.q.spec && !keep) f.ph_tendf[x] = 0.0;
   }
@@ -2278,10 +2537,19 @@ MI_POINT_KERNEL k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
   const Lv& c = a.c;
   const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
   const bool in = sixth_in(g, a.q.diff_6th_opt != 0, 'm', i, j);
+  const bool d2 = a.q.diff_opt == 1, hin = d2 && hdiff_in(g, 'm', i, j);
+  const double* qvb = a.f.qv_base;
 #pragma unroll 1
   for (int n = 0; n < n_sc; ++n) {
-    const double t0 = 0.0 + (a.q.qtf_on ? double(s.qtf[n][x]) : 0.0);
-    s.qtw[n][x] = in ? sixth_at<'m'>(t0, a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], coef) : t0;
+    double t = 0.0 + (a.q.qtf_on ? double(s.qtf[n][x]) : 0.0);
+    if (d2) {  // horizontal_diffusion 'm' with xkhh, then vertical_diffusion(_mp for qv) with kvdq; mut is muts
+      if (hin) t = hdiff_at<'m'>(t, a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], a.q.xkhh);
+      if (n == 0 && s.n_moist > 0)
+        t = vdiff_m<true>(t, a, s.q2[n], a.f.muts, [&](int kk) { return qvb[kk]; }, a.q.kvdq, x, x2, k);
+      else
+        t = vdiff_m<false>(t, a, s.q2[n], a.f.muts, [](int) { return 0.0; }, a.q.kvdq, x, x2, k);
+    }
+    s.qtw[n][x] = in ? sixth_at<'m'>(t, a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], coef) : t;
   }
 }
 
@@ -2877,8 +3145,10 @@ void launch_halo(const Args& a, const HaloSet& s, hipStream_t st) {
 void launch_copy_step(const Args& a, int64_t n3, int64_t n2, hipStream_t st) {
   // offsets of point (0,0,0) inside the allocations: identical for every field of the domain
   const int64_t off3 = int64_t(kLpad) + int64_t(a.g.hy) * a.g.sj;
+  // mu's rows only: rows x pitch values (an x-z slice: its one row of sk values)
+  const int64_t pitch = a.g.sk, sj = a.g.sj, rows = sj > 0 ? n2 / sj : 1, n2rows = rows * pitch;
   hipLaunchKernelGGL(k_copy_step, dim3(unsigned((n3 + 255) / 256), 6, 1), dim3(256, 1, 1), 0, st, a.f, nullptr, nullptr, n3,
-                     n2, off3, off3, off3, off3, off3, off3);
+                     n2rows, off3, off3, off3, off3, off3, off3, sj, pitch);
 }
 
 void launch_fill(double* p, int64_t n, double v, hipStream_t st) {
@@ -2959,8 +3229,6 @@ int col_cfg_default(int kernel, int nx, int ny, int nz) {
   // the substep kernel: the narrowest one-level-per-thread shape; the others: one level per thread on
   // small domains (latency), 16 columns (full 128-byte rows) on larger ones
   if (kernel == kColAcoustic) {
-    // a large grid: the k-march (a wavefront per 64 columns, enough of them to fill the GPU)
-    if (int64_t(nx) * ny >= 40000) return kColMarch;
     for (int c : {4, 5, 6})
       if (col_cfg_fits(kernel, c, nz)) return c;
     return 6;
@@ -3006,6 +3274,57 @@ void launch_uv(const Args& a, double dts, hipStream_t st) {
   const int ex = (g.spec() || g.ox()) ? 1 : 0, ey = (g.spec() || g.oy()) ? 1 : 0;
   launch3("uv", (g.ox() || g.oy()) ? k_uv<true> : k_uv<false>, Range{0, g.nx + ex, 0, g.ny + ey, 0, g.nz}, st, a, dts);
 }
+// the batched substep shapes: uv blocks of 64 x 4 points, acoustic blocks of 8 columns x 64 levels
+constexpr int kBatchCI = 8, kBatchKT = 64;
+int batch_table_uv(const Args* const* as, const double* dts, int n, BatchMember* out) {
+  int start = 0;
+  for (int m = 0; m < n; ++m) {
+    const Geo& g = as[m]->g;
+    const int ex = (g.spec() || g.ox()) ? 1 : 0, ey = (g.spec() || g.oy()) ? 1 : 0;
+    BatchMember& t = out[m];
+    t.i0 = 0; t.i1 = g.nx + ex; t.j0 = 0; t.j1 = g.ny + ey; t.k0 = 0; t.k1 = g.nz;
+    t.gx = (t.i1 - t.i0 + kBx - 1) / kBx;
+    t.gy = (t.j1 - t.j0 + 3) / 4;
+    t.start = start;
+    t.dts = dts[m];
+    start += t.gx * t.gy * (t.k1 - t.k0);
+  }
+  out[n].start = start;
+  return start;
+}
+int batch_table_acoustic(const Args* const* as, const double* dts, int n, BatchMember* out) {
+  int start = 0;
+  for (int m = 0; m < n; ++m) {
+    const Geo& g = as[m]->g;
+    BatchMember& t = out[m];
+    t.gx = (g.nx + kBatchCI - 1) / kBatchCI;
+    t.gy = g.ny;
+    t.i0 = t.j0 = t.k0 = 0;
+    t.i1 = g.nx; t.j1 = g.ny; t.k1 = g.nz;
+    t.start = start;
+    t.dts = dts[m];
+    start += t.gx * t.gy;
+  }
+  out[n].start = start;
+  return start;
+}
+bool batch_fits(const Args& a) { return a.g.nz + 1 <= kBatchKT; }
+void launch_uv_batch(const Args* dev_args, const BatchMember* dev_tab, int n, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_uv_batch, dim3(unsigned(blocks), 1, 1), dim3(kBx, 4, 1), 0, st, dev_args, dev_tab, n);
+}
+void launch_acoustic_batch(const Args* dev_args, const BatchMember* dev_tab, int n, int blocks, int nzmax, int it, int ns,
+                           hipStream_t st) {
+  const size_t lds = col_lds_bytes(kBatchCI, kColArrays[kColAcoustic][0], nzmax, kColArrays[kColAcoustic][1]);
+  hipLaunchKernelGGL((k_acoustic_batch<kBatchCI, kBatchKT>), dim3(unsigned(blocks), 1, 1), dim3(kBatchCI * kBatchKT, 1, 1),
+                     lds, st, dev_args, dev_tab, n, it, ns);
+}
+// KSumEdge of the open directions after a substep's column kernel
+void launch_open_sums(const Args& a, int it, int ns, hipStream_t st) {
+  const Geo& g = a.g;
+  if (g.ox()) launch3("open_sum", k_open_sum, Range{g.nx - 1, g.nx, 0, g.ny, 0, g.nz}, st, a, it, ns, 0);
+  if (g.oy()) launch3("open_sum", k_open_sum, Range{0, g.nx, g.ny - 1, g.ny, 0, g.nz}, st, a, it, ns, 1);
+}
+
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st) {
   if (cfg == kColMarch) {  // one thread per column, 64 columns per block
     const bool t = timing("acoustic");
@@ -3016,9 +3335,7 @@ void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStre
   } else {
     dispatch_cfg<AcousticL>(cfg, "acoustic", 10, 5, a.g.nx, a.g.ny, st, a, dts, it, ns);
   }
-  const Geo& g = a.g;
-  if (g.ox()) launch3("open_sum", k_open_sum, Range{g.nx - 1, g.nx, 0, g.ny, 0, g.nz}, st, a, it, ns, 0);
-  if (g.oy()) launch3("open_sum", k_open_sum, Range{0, g.nx, g.ny - 1, g.ny, 0, g.nz}, st, a, it, ns, 1);
+  launch_open_sums(a, it, ns, st);
 }
 void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st) {
   dispatch_cfg<FinishL>(cfg, "finish", 1, 1, a.g.nx + 1, a.g.nyv, st, a, prhophi ? 1 : 0, hd_dt);

@@ -62,6 +62,9 @@ struct Fields {
   real *thf, *qvf;
   // the substep k-march's scratch: the forward sweep's values and the explicit phi'' of the substep (FP64)
   double *ky, *kr;
+  // the diffusion's base state: theta's t_init (3-D), the 1-D profiles u_base, v_base, qv_base (nz values)
+  BField t_init;
+  double *u_base, *v_base, *qv_base;
 };
 
 // The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
@@ -106,6 +109,8 @@ struct Params {
   int qtf_on;          // a column physics plugin sets the scalars' q_tendf (else zero_tendf leaves it 0)
   int diff_6th_opt;    // 1 linear, 2 monotonic
   double diff_6th_factor;
+  int diff_opt;        // 1: second-order diffusion along eta with constant K (km_opt 1)
+  double xkmh, xkhh, kvdif, kvdq;  // its eddy viscosities: khdif, khdif / prandtl, kvdif, kvdif / prandtl
   double dt;           // the large step
   int terrain;         // the surface w follows the terrain (ht)
   int spec;            // specified lateral boundaries (both directions)
@@ -154,6 +159,7 @@ constexpr int kColCfgs[kNumColCfgs][2] = {{16, 16}, {16, 32}, {32, 16}, {8, 32},
 constexpr int kColCfgTallest = 6;  // 2 x 256: the most levels per column (the substep kernel's nz limit)
 // one more shape of the substep kernel: one thread per column marching k (k_acoustic_march)
 constexpr int kColMarch = kNumColCfgs;
+constexpr int kNumColShapes = kColMarch + 1;
 constexpr const char* kColKernelNames[kNumColKernels] = {"large_prep", "ss_column", "acoustic", "finish"};
 bool col_cfg_fits(int kernel, int cfg, int nz);
 int col_cfg_default(int kernel, int nx, int ny, int nz);
@@ -164,6 +170,22 @@ void launch_ss_prep(const Args& a, bool first, hipStream_t st);
 void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st);
 void launch_uv(const Args& a, double dts, hipStream_t st);
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st);
+void launch_open_sums(const Args& a, int it, int ns, hipStream_t st);
+// A batch's acoustic substeps in one launch per kernel over every member (the members' Args in a device array):
+// one table entry per member (its blocks' first index in the launch, its block grid, its stage dts) and one more
+// holding the launch's block count.  batch_table_* fill the tables on the host and return the block count.
+struct BatchMember {
+  int i0, i1, j0, j1, k0, k1;
+  int start, gx, gy;
+  double dts;
+};
+constexpr int kMaxBatch = 64;
+int batch_table_uv(const Args* const* as, const double* dts, int n, BatchMember* out);
+int batch_table_acoustic(const Args* const* as, const double* dts, int n, BatchMember* out);
+bool batch_fits(const Args& a);  // the batched column kernel's levels per block hold the member's
+void launch_uv_batch(const Args* dev_args, const BatchMember* dev_tab, int n, int blocks, hipStream_t st);
+void launch_acoustic_batch(const Args* dev_args, const BatchMember* dev_tab, int n, int blocks, int nzmax, int it, int ns,
+                           hipStream_t st);
 void launch_finish(const Args& a, int cfg, bool prhophi, double hd_dt, hipStream_t st);
 // the held tendencies of u, v, w, theta at the first stage: kSixthZero zeroes them and adds the sixth-order
 // diffusion; kSixthKeep adds it to what they hold (a physics plugin's part2); kSixthZeroOnly only zeroes

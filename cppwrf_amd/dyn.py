@@ -124,6 +124,12 @@ class Case:
         return cls(A.MI_CASE_WARM_BUBBLE, Grid(200, 1, 100, 100.0, 100.0, 1.0, A.MI_BC_SYMMETRIC), 10000.0)
 
     @classmethod
+    def straka(cls, nx=512, nz=64, dx=100.0, dt=1.0):
+        """the reference's straka case (configs/straka.ini: x-z density current, periodic, K = 75 m2/s with
+        pr_inv = 1; run it with params(diff_opt=1, khdif=75, kvdif=75, pr_inv=1))"""
+        return cls(A.MI_CASE_STRAKA, Grid(nx, 1, nz, dx, dx, dt, A.MI_BC_PERIODIC), 6400.0)
+
+    @classmethod
     def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0, bc=A.MI_BC_PERIODIC, bc_y=None, qv0=0.0, qv_hscale=0.0):
         return cls(A.MI_CASE_REST_STATE, Grid(nx, ny, nz, 2000.0, 2000.0, 10.0, bc, bc if bc_y is None else bc_y), 20000.0,
                    u0=u0, qv0=qv0, qv_hscale=qv_hscale)
@@ -162,15 +168,23 @@ class Context:
 
 
 class Batch:
-    """Domains of one context stepped together, one HIP graph per step with a branch per member (mi_batch)."""
+    """Domains of one context stepped together, one HIP graph per step (mi_batch): a branch per member, or each
+    acoustic substep's kernels as one launch over all members (the fused schedule)."""
 
-    def __init__(self, ctx: Context, members):
+    def __init__(self, ctx: Context, members, schedule: int = A.MI_BATCH_BRANCHES):
         self.ctx = ctx
         self.members = list(members)
         arr = (C.c_void_p * len(self.members))(*[m.handle.value for m in self.members])
         h = C.c_void_p()
         check(lib().mi_batch_create(ctx.handle, arr, len(self.members), C.byref(h)))
         self._h = h
+        if schedule != A.MI_BATCH_BRANCHES:
+            self.set_schedule(schedule)
+
+    def set_schedule(self, schedule: int) -> None:
+        """MI_BATCH_BRANCHES (a branch per member) or MI_BATCH_FUSED (each acoustic substep one launch over all
+        members)"""
+        check(lib().mi_batch_set_schedule(self._h, schedule))
 
     def step(self, n: int = 1) -> None:
         check(lib().mi_batch_step(self._h, n))

@@ -4,7 +4,7 @@
         per-kernel calls, total / average / min / max duration from a --kernel-trace run, FP64 and FP32-bulk apart
     python -m cppwrf_amd.profile_summary traffic <fetch_dir> <write_dir> --grid NXxNYxNZ
         HBM bytes per dispatch from one --pmc FETCH_SIZE run and one --pmc WRITE_SIZE run, each counter scaled by
-        what it reads for k_copy_step, a copy of known size (u, v, w, t, ph and mu of the grid's allocation), and a
+        what it reads for k_copy_step, a copy of known size (u, v, w, t, ph and the rows of mu), and a
         "traffic:" line per kernel and precision (the most-dispatched launch shape first) that bench.py reads
 
 The mode of a kernel is its namespace (mi_fp64 / mi_fp32); its shape the template arguments of the name.
@@ -72,11 +72,12 @@ def traffic(fetch_dir: str, write_dir: str, grid: str) -> str:
     nx, ny, nz = (int(v) for v in grid.lower().split("x"))
     n3 = alloc_elems(nx, ny, nz)
     f, w = _counter(fetch_dir, "FETCH_SIZE"), _counter(write_dir, "WRITE_SIZE")
-    out = [f"Calibration: `k_copy_step` copies u, v, w, t, ph ({n3:,} elements each, halos included) and mu ({n3:,} "
-           f"doubles: a 2-D field has the allocation size of a 3-D one)."]
+    n2 = n3 // (nz + 1)  # the rows of a 2-D field (level 0 of each j block)
+    out = [f"Calibration: `k_copy_step` copies u, v, w, t, ph ({n3:,} elements each, halos included) and the {n2:,} "
+           f"doubles of mu's rows."]
     scale = {}
     for mode, esize in (("fp32", 4), ("fp64", 8)):
-        known = n3 * (5 * esize + 8)
+        known = n3 * 5 * esize + n2 * 8
         kf, kw = f.get(("copy_step", mode, "-")), w.get(("copy_step", mode, "-"))
         if kf and kw:
             sf = known / (sum(kf) / len(kf) * 1024.0)

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 2
+#define MI_ABI_VERSION 3
 
 /* status codes */
 #define MI_OK 0
@@ -100,6 +100,17 @@ typedef struct mi_params {
    * and set_w_surface at the end of each step, src/dyn/small_step.cpp:339-353, rk3.cpp:173-200) */
   int32_t terrain;
   double diff_6th_factor;
+  /* second-order diffusion along eta (reference DiffusionDriver with diff_opt 1, km_opt 1: constant eddy
+   * viscosities, src/dyn/diffusion_driver.cpp:159-193, 297-327; src/dyn/diffusion/big_step_diffusion.hpp):
+   * diff_opt 0 off or 1 (2, physical space, is not built); km_opt 1 (constant K; the TKE and Smagorinsky
+   * options 2-4 are not built); khdif, kvdif the horizontal and vertical K (m2/s); pr_inv = 1 / prandtl for
+   * the scalars' K (0 takes the reference's default 3).  Applied at the first RK stage into the held
+   * tendencies of u, v, w, theta (vertical K of theta 3 kvdif, as WRF's rk_tendency) and every scalar,
+   * before the sixth-order diffusion.  The base profiles MI_U_BASE, MI_V_BASE, MI_QV_BASE enter the
+   * vertical diffusion of u, v and qv. */
+  int32_t diff_opt;
+  int32_t km_opt;
+  double khdif, kvdif, pr_inv;
 } mi_params;
 
 #define MI_MAX_MOIST 8    /* reference KCq::kMax (src/dyn/large_step.cpp:128) */
@@ -146,6 +157,10 @@ enum {
   /* Coriolis parameters 2 Omega sin(lat), 2 Omega cos(lat) and the grid rotation (2-D mass points) */
   MI_F = 47, MI_E = 48, MI_SINA = 49, MI_COSA = 50,
   MI_HT = 51, /* terrain height, m (2-D mass points; with terrain) */
+  /* the 1-D base profiles of the diffusion (reference BaseState::u_base, v_base, qv_base, src/dyn/state.hpp:100,
+   * 112; the ideal cases take u and v of column (0, 0), src/cases/model.cpp:149-177): nz values each, zero
+   * unless set */
+  MI_U_BASE = 52, MI_V_BASE = 53, MI_QV_BASE = 54,
   /* scalar n (0 <= n < n_scalars) at mass points: MI_Q + n is its state (q_2), MI_Q_TENDF + n its held
    * tendency (input, mass-coupled, held over the step as the reference's q_tendf) */
   MI_Q = 64, MI_Q_TENDF = 96
@@ -352,6 +367,14 @@ typedef struct mi_batch mi_batch;
 int mi_batch_create(mi_ctx* ctx, mi_domain* const* members, int n, mi_batch** out);
 int mi_batch_destroy(mi_batch* b);
 int mi_batch_step(mi_batch* b, int nsteps);
+/* How a batch step is laid out in its graph.  MI_BATCH_BRANCHES (the default): each member's step is a
+ * branch of its own.  MI_BATCH_FUSED: advance_uv and the column kernel of every acoustic substep run as one
+ * launch over all members (a table of the members' arguments and block ranges), the rest of each member's
+ * step in branches between them; every member needs the same arithmetic and acoustic step count, no
+ * callbacks, and at most 64 members (MI_EINVAL otherwise).  Both give each member the bits of its step alone;
+ * which is faster depends on the mix (measured by bench.py).  No counterpart in the reference. */
+enum { MI_BATCH_BRANCHES = 0, MI_BATCH_FUSED = 1 };
+int mi_batch_set_schedule(mi_batch* b, int schedule);
 /* The graphs a batch holds: at most one per buffer parity of the members' current launch schedules (a
  * schedule change such as mi_domain_set_dt on a member frees the graphs of the old one). */
 int mi_batch_graph_count(const mi_batch* b);
@@ -390,9 +413,10 @@ enum {
   MI_CASE_WARM_BUBBLE = 1, /* reference warm_bubble (x-z Bryan & Fritsch 2002) */
   MI_CASE_IGW = 2,         /* reference igw (Skamarock & Klemp 1994, f = 0) */
   MI_CASE_BUBBLE3D = 3,    /* 3-D dry bubble in an N = 0.01 atmosphere at rest, seeded noise */
-  MI_CASE_HILLS = 4        /* real-data-shaped: seeded hills, US standard atmosphere, jets, map factors,
+  MI_CASE_HILLS = 4,       /* real-data-shaped: seeded hills, US standard atmosphere, jets, map factors,
                               Coriolis, hybrid coordinate, optional qv; needs terrain, mapped, coriolis
                               (and n_moist >= 1 when moist) in mi_params */
+  MI_CASE_STRAKA = 5       /* reference straka (x-z density current, Straka et al. 1993) */
 };
 typedef struct mi_case_spec {
   int32_t kind; /* MI_CASE_* */

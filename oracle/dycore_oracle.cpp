@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -128,6 +129,12 @@ void pfor(int i0, int i1, int j0, int j1, int k0, int k1, K k) {  // backend.hpp
       for (int i = i0; i < i1; ++i) k(i, j, kk);
 }
 template <class K>
+void pfor2(int i0, int i1, int j0, int j1, K k) {  // column_for over a range (backend.hpp:520-528)
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int j = j0; j < j1; ++j)
+    for (int i = i0; i < i1; ++i) k(i, j);
+}
+template <class K>
 void cfor(int nx, int ny, K k) {  // backend.hpp:520-528
 #pragma omp parallel for collapse(2) schedule(static)
   for (int j = 0; j < ny; ++j)
@@ -234,6 +241,10 @@ struct Dom {
   F ru_tendf, rv_tendf, rw_tendf, t_tendf, ph_tendf;
   std::vector<F> qtw;  // q_tendf of the step: the caller's tendency plus the sixth-order diffusion
   bool mapped = false, cor = false, use_tendf = false, terrain = false;
+  // the diffusion's 1-D base profiles (BaseState::u_base, v_base, qv_base; state.hpp:100, 112) and its constant
+  // eddy viscosities (KIsotropicKm, km_kh.hpp:35-45: xkmh = khdif, xkmv = kvdif, xkhh = khdif / prandtl)
+  std::vector<double> u_base, v_base, qv_base;
+  double xkmh = 0, xkhh = 0, kvdif = 0, kvdq = 0;
   // a column physics plugin (ColumnPhysics, column_physics.hpp:21-34; RK3::set_column_physics): its hooks and
   // the advective forcings it asks for (RTHFTEN, RQVFTEN)
   orc_column_physics phys{};
@@ -301,7 +312,18 @@ struct Dom {
     oy = g.bcy == MI_BC_OPEN && !g.two_d();
     flx = spec || ox;
     fly = (spec || g.bcy == MI_BC_OPEN) && !g.two_d();
-    use_tendf = p.diff_6th_opt != 0 || spec;  // a DiffusionDriver or a LateralBC attached (large_step.cpp:828)
+    u_base.assign(size_t(nz), 0.0);
+    v_base.assign(size_t(nz), 0.0);
+    qv_base.assign(size_t(nz), 0.0);
+    if (p.diff_opt == 1) {  // DiffusionDriver's params (model.cpp:314: prandtl = 1 / pr_inv)
+      const double pr_inv = p.pr_inv > 0 ? p.pr_inv : 3.0;
+      const double prandtl = 1.0 / pr_inv;
+      xkmh = p.khdif;
+      xkhh = p.khdif / prandtl;
+      kvdif = p.kvdif;
+      kvdq = p.kvdif / prandtl;
+    }
+    use_tendf = p.diff_6th_opt != 0 || p.diff_opt != 0 || spec;  // a DiffusionDriver or a LateralBC (large_step.cpp:828)
     held_base = use_tendf;
     f3(thf, St::M);
     f3(qvf, St::M);
@@ -550,6 +572,159 @@ struct Dom {
       tend(i, j, k) = tend(i, j, k) + tx + ty;
     });
   }
+  // ------------------------------------------------------------------ diff_opt 1 (big_step_diffusion.hpp)
+  // horizontal_diffusion (big_step_diffusion.hpp:26-186, 343-389): the second-order fluxes along eta of one field
+  // with the constant K; name 'u', 'v', 'w' or 'm' (mass points); loop limits one row in from open and specified
+  // sides (a periodic x resets the x limits)
+  template <class Fld, class Mu>
+  void hdiff(char name, const Fld& q, F& tend, const Mu& mu, double K, const std::vector<double>& c1,
+             const std::vector<double>& c2) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool fx = ox || spec, fy = oy || spec, do_y = !g.two_d();
+    int is = fx ? 1 : 0, js = fy ? 1 : 0, ie, je, k0 = 0;
+    if (name == 'u') { ie = fx ? nx : nx + 1; je = fy ? ny - 1 : ny; }
+    else if (name == 'v') { ie = fx ? nx - 1 : nx; je = fy ? ny : ny + 1; }
+    else { ie = fx ? nx - 1 : nx; je = fy ? ny - 1 : ny; }
+    if (name == 'w') k0 = 1;
+    if (g.bcx == MI_BC_PERIODIC) { is = 0; ie = name == 'u' ? nx + 1 : nx; }
+    const double rdx = c.rdx, rdy = c.rdy;
+    pfor(is, ie, js, je, k0, nz, [&](int i, int j, int k) {
+      auto m = [&](int a, int b) { return c1[k] * mu(i + a, j + b) + c2[k]; };
+      const double K4 = K + K + K + K, K2 = K + K;
+      double mkrdxm, mkrdxp, mrdx, mkrdym = 0, mkrdyp = 0, mrdy = 0;
+      if (name == 'u') {  // KHorizDiffU1
+        mkrdxm = (msftx(i - 1, j) / msfty(i - 1, j)) * m(-1, 0) * K * rdx;
+        mkrdxp = (msftx(i, j) / msfty(i, j)) * m(0, 0) * K * rdx;
+        mrdx = msfux(i, j) * msfuy(i, j) * rdx;
+        if (do_y) {
+          mkrdym = ((msfuy(i, j) + msfuy(i, j - 1)) / (msfux(i, j) + msfux(i, j - 1))) * 0.25 *
+                   (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * K4 * rdy;
+          mkrdyp = ((msfuy(i, j) + msfuy(i, j + 1)) / (msfux(i, j) + msfux(i, j + 1))) * 0.25 *
+                   (m(0, 0) + m(0, 1) + m(-1, 1) + m(-1, 0)) * 0.25 * K4 * rdy;
+          mrdy = msfux(i, j) * msfuy(i, j) * rdy;
+        }
+      } else if (name == 'v') {  // KHorizDiffV1 (no mass factor in y, WRF v4.8.0 lines 2854-2856)
+        mkrdxm = ((msfvx(i, j) + msfvx(i - 1, j)) / (msfvy(i, j) + msfvy(i - 1, j))) * 0.25 *
+                 (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * K4 * rdx;
+        mkrdxp = ((msfvx(i, j) + msfvx(i + 1, j)) / (msfvy(i, j) + msfvy(i + 1, j))) * 0.25 *
+                 (m(0, 0) + m(0, -1) + m(1, -1) + m(1, 0)) * 0.25 * K4 * rdx;
+        mrdx = msfvx(i, j) * msfvy(i, j) * rdx;
+        mkrdym = (msfty(i, j - 1) / msftx(i, j - 1)) * K * rdy;
+        mkrdyp = (msfty(i, j) / msftx(i, j)) * K * rdy;
+        mrdy = msfvx(i, j) * msfvy(i, j) * rdy;
+      } else if (name == 'w') {  // KHorizDiffW1: mass factors of the level, K averaged over k and k-1
+        mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.25 * (m(0, 0) + m(-1, 0) + m(0, 0) + m(-1, 0)) * 0.25 * K4 * rdx;
+        mkrdxp = (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.25 * (m(1, 0) + m(0, 0) + m(1, 0) + m(0, 0)) * 0.25 * K4 * rdx;
+        mrdx = msftx(i, j) * msfty(i, j) * rdx;
+        if (do_y) {
+          mkrdym = (msfvy(i, j) * msfvx_inv(i, j)) * 0.25 * (m(0, 0) + m(0, -1) + m(0, 0) + m(0, -1)) * 0.25 * K4 * rdy;
+          mkrdyp = (msfvy(i, j + 1) * msfvx_inv(i, j + 1)) * 0.25 * (m(0, 1) + m(0, 0) + m(0, 1) + m(0, 0)) * 0.25 *
+                   K4 * rdy;
+          mrdy = msftx(i, j) * msfty(i, j) * rdy;
+        }
+      } else {  // KHorizDiffM1
+        mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
+        mkrdxp = (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.5 * K2 * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
+        mrdx = msftx(i, j) * msfty(i, j) * rdx;
+        if (do_y) {
+          mkrdym = (msfvy(i, j) * msfvx_inv(i, j)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
+          mkrdyp = (msfvy(i, j + 1) * msfvx_inv(i, j + 1)) * 0.5 * K2 * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
+          mrdy = msftx(i, j) * msfty(i, j) * rdy;
+        }
+      }
+      const double qx = mrdx * (mkrdxp * (q(i + 1, j, k) - q(i, j, k)) - mkrdxm * (q(i, j, k) - q(i - 1, j, k)));
+      // an x-z slice: rdy = 0 makes the y term a zero that leaves the sum's bits unchanged (the x term is added
+      // to a tendency that is never -0), so it is left out there (no y neighbours)
+      const double qy = do_y ? mrdy * (mkrdyp * (q(i, j + 1, k) - q(i, j, k)) - mkrdym * (q(i, j, k) - q(i, j - 1, k))) : 0.0;
+      tend(i, j, k) += do_y ? qx + qy : qx;
+    });
+  }
+  // horizontal_diffusion_3dmp (big_step_diffusion.hpp:188-219, 391-414): theta minus its base state t_init, the
+  // differences evaluated left to right as WRF
+  template <class Fld>
+  void hdiff_3dmp(const Fld& q, F& tend, double K) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool fx = ox || spec, fy = oy || spec, do_y = !g.two_d();
+    int is = fx ? 1 : 0, js = fy ? 1 : 0, ie = fx ? nx - 1 : nx, je = fy ? ny - 1 : ny;
+    if (g.bcx == MI_BC_PERIODIC) { is = 0; ie = nx; }
+    const double rdx = c.rdx, rdy = c.rdy;
+    pfor(is, ie, js, je, 0, nz, [&](int i, int j, int k) {
+      auto m = [&](int a, int b) { return c.c1h[k] * mut(i + a, j + b) + c.c2h[k]; };
+      const double K2 = K + K;
+      const double mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
+      const double mkrdxp = (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.5 * K2 * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
+      const double mrdx = msftx(i, j) * msfty(i, j) * rdx;
+      auto b = [&](int a, int bb) { return t_init(i + a, j + bb, k); };
+      const double dxp = q(i + 1, j, k) - q(i, j, k) - b(1, 0) + b(0, 0);
+      const double dxm = q(i, j, k) - q(i - 1, j, k) - b(0, 0) + b(-1, 0);
+      const double qx = mrdx * (mkrdxp * dxp - mkrdxm * dxm);
+      if (!do_y) {
+        tend(i, j, k) += qx;
+        return;
+      }
+      const double mkrdym = (msfvy(i, j) * msfvx_inv(i, j)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
+      const double mkrdyp = (msfvy(i, j + 1) * msfvx_inv(i, j + 1)) * 0.5 * K2 * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
+      const double mrdy = msftx(i, j) * msfty(i, j) * rdy;
+      const double dyp = q(i, j + 1, k) - q(i, j, k) - b(0, 1) + b(0, 0);
+      const double dym = q(i, j, k) - q(i, j - 1, k) - b(0, 0) + b(0, -1);
+      tend(i, j, k) += qx + mrdy * (mkrdyp * dyp - mkrdym * dym);
+    });
+  }
+  // vertical_diffusion, _mp, _u, _v (big_step_diffusion.hpp:225-338, 416-500): the column's fluxes at the w levels
+  // 1..nz-1 (vflux[k] between levels k and k+1, 0 at the top), then their differences.  'u'/'v' at the faces (muu /
+  // muv, alt averaged onto the face, the base profile subtracted, no tendency at the top level); 'w' at the w
+  // levels 1..nz-1; 'm' at the mass levels, with `base` (t_init, qv_base) subtracted when given
+  template <class Fld>
+  void vdiff(char name, const Fld& q, F& tend, double kv, const std::function<double(int, int, int)>* base) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool fx = ox || spec, fy = oy || spec;
+    int is = 0, ie = nx, js = 0, je = ny;
+    if (name == 'u') { is = fx ? 1 : 0; ie = fx ? nx : nx + 1; }
+    if (name == 'v') { js = fy ? 1 : 0; je = fy ? ny : ny + 1; }
+    if (name == 'u' && g.bcx == MI_BC_PERIODIC) { is = 0; ie = nx + 1; }
+    const double G = kG;
+    pfor2(is, ie, js, je, [&](int i, int j) {
+      std::vector<double> vf(size_t(nz), 0.0);
+      for (int k = 0; k < nz - 1; ++k) {
+        if (name == 'u' || name == 'v') {
+          const int di = name == 'u' ? -1 : 0, dj = name == 'v' ? -1 : 0;
+          const double a = 0.25 * (alt(i, j, k) + alt(i + di, j + dj, k) + alt(i, j, k + 1) + alt(i + di, j + dj, k + 1));
+          const std::vector<double>& bp = name == 'u' ? u_base : v_base;
+          const double dp = q(i, j, k + 1) - q(i, j, k) - bp[k + 1] + bp[k];
+          vf[k] = kv * c.rdn[k + 1] / a * dp;
+        } else if (name == 'w') {
+          vf[k] = (kv / alt(i, j, k)) * c.rdnw[k] * (q(i, j, k + 1) - q(i, j, k));
+        } else {
+          const double dq = base ? q(i, j, k + 1) - q(i, j, k) - (*base)(i, j, k + 1) + (*base)(i, j, k)
+                                 : q(i, j, k + 1) - q(i, j, k);
+          vf[k] = kv * c.rdn[k + 1] / (0.5 * (alt(i, j, k) + alt(i, j, k + 1))) * dq;
+        }
+      }
+      const double vf0 = vf[0];
+      if (name == 'u' || name == 'v') {
+        const int di = name == 'u' ? -1 : 0, dj = name == 'v' ? -1 : 0;
+        auto mm = [&](int k) { return c.c1h[k] * (name == 'u' ? muu(i, j) : muv(i, j)) + c.c2h[k]; };
+        tend(i, j, 0) += G * G * c.rdnw[0] / mm(0) / (0.5 * (alt(i + di, j + dj, 0) + alt(i, j, 0))) * (vf[0] - vf0);
+        for (int k = 1; k < nz - 1; ++k) {
+          const double a = 0.5 * (alt(i + di, j + dj, k) + alt(i, j, k));
+          tend(i, j, k) += G * G * c.rdnw[k] / mm(k) / a * (vf[k] - vf[k - 1]);
+        }
+      } else if (name == 'w') {
+        for (int k = 1; k < nz; ++k) {
+          const double mu = c.c1f[k] * mut(i, j) + c.c2f[k];
+          const double alt_avg = 0.5 * (alt(i, j, k) + alt(i, j, k - 1));
+          tend(i, j, k) += c.rdn[k] * G * G / mu / alt_avg * (vf[k] - vf[k - 1]);
+        }
+      } else {
+        const F& mus = name == 's' ? muts : mut;
+        tend(i, j, 0) += G * G / (c.c1h[0] * mus(i, j) + c.c2h[0]) / alt(i, j, 0) * c.rdnw[0] * (vf[0] - vf0);
+        for (int k = 1; k < nz; ++k) {
+          const double mu = c.c1h[k] * mus(i, j) + c.c2h[k];
+          tend(i, j, k) += G * G / mu / alt(i, j, k) * c.rdnw[k] * (vf[k] - vf[k - 1]);
+        }
+      }
+    });
+  }
   bool moist() const { return n_moist > 0; }
   // kern::MoistCoef (kernels_common.hpp:93-100): 1, 1, 0 when dry
   double mc_cqu(int i, int j, int k) const { return moist() ? cqu(i, j, k) : 1.0; }
@@ -662,6 +837,17 @@ struct Dom {
       auto fw = [&](int i, int j, int k) { return w2(i, j, k); };
       auto ft = [&](int i, int j, int k) { return t2(i, j, k); };
       auto mu_t = [&](int i, int j) { return mut(i, j); };
+      if (p.diff_opt == 1) {  // diffusion_driver.cpp:159-193: horizontal u, v, w, theta, then vertical
+        hdiff('u', fu, ru_tendf, mu_t, xkmh, c.c1h, c.c2h);
+        if (!g.two_d()) hdiff('v', fv, rv_tendf, mu_t, xkmh, c.c1h, c.c2h);
+        hdiff('w', fw, rw_tendf, mu_t, xkmh, c.c1f, c.c2f);
+        hdiff_3dmp(ft, t_tendf, xkhh);
+        vdiff('u', fu, ru_tendf, kvdif, nullptr);
+        if (!g.two_d()) vdiff('v', fv, rv_tendf, kvdif, nullptr);
+        vdiff('w', fw, rw_tendf, kvdif, nullptr);
+        const std::function<double(int, int, int)> tb = [&](int i, int j, int k) { return t_init(i, j, k); };
+        vdiff('m', ft, t_tendf, 3.0 * kvdif, &tb);  // kvdq = 3 kvdif for theta (WRF rk_tendency)
+      }
       if (p.diff_6th_opt != 0) {
         sixth('u', fu, ru_tendf, mu_t, g.dt, c.c1h, c.c2h);
         if (!g.two_d()) sixth('v', fv, rv_tendf, mu_t, g.dt, c.c1h, c.c2h);
@@ -1736,6 +1922,15 @@ struct Dom {
         const F& qq = q2[size_t(n)];
         auto fq = [&](int i, int j, int k) { return qq(i, j, k); };
         auto mu_s = [&](int i, int j) { return muts(i, j); };
+        if (p.diff_opt == 1) {  // DiffusionDriver::scalar (diffusion_driver.cpp:297-327), mut there is muts
+          hdiff('m', fq, w, mu_s, xkhh, c.c1h, c.c2h);
+          if (n == 0 && n_moist > 0) {  // qv: vertical_diffusion_mp with qv_base
+            const std::function<double(int, int, int)> qb = [&](int, int, int k) { return qv_base[size_t(k)]; };
+            vdiff('s', fq, w, kvdq, &qb);
+          } else {
+            vdiff('s', fq, w, kvdq, nullptr);
+          }
+        }
         if (p.diff_6th_opt != 0) sixth('m', fq, w, mu_s, dt_rk, c.c1h, c.c2h);
         // relax_bdy_scalar (lateral_bc.cpp:389-404): accumulated straight into the tendency
         if (spec && qbdy[size_t(n)].have)
@@ -1936,6 +2131,8 @@ int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv,
     if (p->n_moist < 0 || p->n_moist > MI_MAX_MOIST || p->n_scalars < p->n_moist || p->n_scalars > MI_MAX_SCALARS)
       return fail("bad n_moist / n_scalars");
     if (p->diff_6th_opt < 0 || p->diff_6th_opt > 2) return fail("diff_6th_opt must be 0, 1 or 2");
+    if (p->diff_opt < 0 || p->diff_opt > 1) return fail("oracle: diff_opt 0 or 1 only");
+    if (p->diff_opt == 1 && p->km_opt != 1) return fail("oracle: km_opt 1 only");
     auto* o = new orc_domain;
     o->d = std::make_unique<Dom>(*g, *p, *lv);
     if (o->d->n_sound < 2 || o->d->n_sound % 2) {
@@ -1963,8 +2160,18 @@ int orc_set_threads(int n) {
   return MI_OK;
 }
 
+// the diffusion's 1-D base profiles (nz values)
+static std::vector<double>* base_profile(Dom& d, int id) {
+  return id == MI_U_BASE ? &d.u_base : id == MI_V_BASE ? &d.v_base : id == MI_QV_BASE ? &d.qv_base : nullptr;
+}
+
 int orc_set_field(orc_domain* od, int id, const double* h, int64_t n) {
   Dom& d = *od->d;
+  if (std::vector<double>* bp = base_profile(d, id)) {
+    if (n != int64_t(bp->size())) return fail("base profiles take nz values");
+    bp->assign(h, h + n);
+    return MI_OK;
+  }
   F* f = d.field(id);
   if (!f) return fail("unknown field id " + std::to_string(id));
   const int64_t full = int64_t(f->nxs) * f->nys * f->nzs;
@@ -2018,7 +2225,7 @@ int orc_set_diffusion(orc_domain* od, int opt, double factor) {
   if (opt < 0 || opt > 2) return fail("diff_6th_opt must be 0, 1 or 2");
   d.p.diff_6th_opt = opt;
   d.p.diff_6th_factor = factor;
-  d.held_base = opt != 0 || d.spec;
+  d.held_base = opt != 0 || d.p.diff_opt != 0 || d.spec;
   d.use_tendf = d.held_base || d.has_phys;
   return MI_OK;
 }

@@ -1,6 +1,8 @@
 """Pins the CPU oracle (and the case generator feeding it) to the reference's own FP64 golden files:
-tools/regression/golden/{igw,warm_bubble,rest_state}_cpu_fp64/snap_000002.{bin,json}, two steps of
-each case from the reference's CPU FP64 build (copied to tests/golden/reference/).
+tools/regression/golden/{igw,warm_bubble,rest_state,straka}_cpu_fp64/snap_000002.{bin,json}, two steps of
+each case from the reference's CPU FP64 build (copied to tests/golden/reference/).  straka runs the
+second-order diffusion of diff_opt 1 with constant K (configs/straka.ini: khdif = kvdif = 75 m2/s,
+pr_inv = 1).
 
 The reference's default build evaluates calc_p_rho_phi's pow with glibc libmvec's 2-lane vector
 function; with that option on, the oracle must equal the goldens in every bit.  The scalar-pow oracle
@@ -35,7 +37,9 @@ def read_snap(name):
     return out
 
 
-CASES = {"igw": Case.igw, "warm_bubble": Case.warm_bubble, "rest_state": Case.rest_state}
+CASES = {"igw": Case.igw, "warm_bubble": Case.warm_bubble, "rest_state": Case.rest_state, "straka": Case.straka}
+# the run options of each golden beyond the reference's defaults (configs/*.ini)
+OPTIONS = {"straka": dict(diff_opt=1, km_opt=1, khdif=75.0, kvdif=75.0, pr_inv=1.0)}
 
 
 def snapshot(case, orc):
@@ -52,7 +56,7 @@ def snapshot(case, orc):
 def test_oracle_equals_reference_goldens_bitwise(name):
     gold = read_snap(name)
     case = CASES[name]()
-    orc = OracleDomain(case, case.params(), libmvec=True)
+    orc = OracleDomain(case, case.params(**OPTIONS.get(name, {})), libmvec=True)
     orc.step(2)
     mine = snapshot(case, orc)
     assert set(mine) == set(gold)
@@ -66,7 +70,7 @@ def test_scalar_pow_oracle_close_to_goldens(name):
     and glibc's pow; after two steps that stays at round-off level relative to each field's scale."""
     gold = read_snap(name)
     case = CASES[name]()
-    orc = OracleDomain(case, case.params(), libmvec=False)
+    orc = OracleDomain(case, case.params(**OPTIONS.get(name, {})), libmvec=False)
     orc.step(2)
     mine = snapshot(case, orc)
     for k in ("u", "w", "theta", "p", "ph", "mu"):
