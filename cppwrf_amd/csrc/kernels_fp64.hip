@@ -10,6 +10,7 @@
 // one thread per column and march in k, every load still i-contiguous across the wavefront.
 #include "kernels_fp64.hpp"
 
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -48,7 +49,31 @@ __device__ __forceinline__ Blk xcd_block() {
 
 struct Range {
   int i0, i1, j0, j1, k0, k1;
+  int kfast;  // the block order of a point kernel (below)
 };
+
+// The point kernels' blocks (64 i x 4 j at one level) in level-fastest order: the n-th block of an XCD's share is
+// level n % gz of tile n / gz (tiles i-fastest), so the levels of one tile run back to back and the k +- 1 and
+// k +- 2 rows of a vertical stencil are still in that XCD's L2 (a whole level-plane of the arrays a stencil
+// kernel reads outgrows the 4 MB L2 at 425x300).  kfast = 0 keeps the i, j, level order of xcd_block.
+__device__ __forceinline__ Blk point_block(int kfast) {
+  Blk b = xcd_block();
+  if (!kfast) return b;
+  const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const unsigned l = unsigned(b.x) + gx * (unsigned(b.y) + gy * unsigned(b.z));
+  b.z = int(l % gz);
+  b.x = int((l / gz) % gx);
+  b.y = int(l / (gz * gx));
+  return b;
+}
+// the order point kernels launch with: level-fastest unless MI_BLOCK_ORDER=ijk (for timing the two)
+int point_order() {
+  static const int o = [] {
+    const char* e = std::getenv("MI_BLOCK_ORDER");
+    return (e && std::strcmp(e, "ijk") == 0) ? 0 : 1;
+  }();
+  return o;
+}
 
 inline dim3 grid_of(const Range& r, int by) {
   return dim3(unsigned((r.i1 - r.i0 + kBx - 1) / kBx), unsigned((r.j1 - r.j0 + by - 1) / by), unsigned(r.k1 - r.k0));
@@ -60,7 +85,7 @@ inline dim3 grid_of(const Range& r, int by) {
 
 // the scalar kernels: block z = level + nz * (index in the species list), one species per thread
 #define MI_POINT_INDEX_SP(r)                                           \
-  const Blk blk_ = xcd_block();                                        \
+  const Blk blk_ = point_block((r).kfast);                             \
   const int nk_ = (r).k1 - (r).k0;                                     \
   const int i = (r).i0 + blk_.x * kBx + int(threadIdx.x);              \
   const int j = (r).j0 + blk_.y * int(blockDim.y) + int(threadIdx.y);  \
@@ -69,7 +94,7 @@ inline dim3 grid_of(const Range& r, int by) {
   if (i >= (r).i1 || j >= (r).j1) return;
 
 #define MI_POINT_INDEX(r)                          \
-  const Blk blk_ = xcd_block();                    \
+  const Blk blk_ = point_block((r).kfast);         \
   const int i = (r).i0 + blk_.x * kBx + int(threadIdx.x); \
   const int j = (r).j0 + blk_.y * int(blockDim.y) + int(threadIdx.y); \
   const int k = (r).k0 + blk_.z;                   \
@@ -123,9 +148,10 @@ double point_bytes(const char* name) {
 }
 
 template <class K, class... T>
-void launch3a(const char* name, double arrays, K kern, const Range& r, hipStream_t st, T... args) {
+void launch3a(const char* name, double arrays, K kern, Range r, hipStream_t st, T... args) {
   if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0) return;
   const int by = (r.j1 - r.j0) == 1 ? 1 : 4;
+  r.kfast = point_order();
   const bool t = timing(name);
   if (t) time_begin(st);
   hipLaunchKernelGGL(kern, grid_of(r, by), dim3(kBx, by, 1), 0, st, r, args...);
@@ -133,9 +159,10 @@ void launch3a(const char* name, double arrays, K kern, const Range& r, hipStream
 }
 // a scalar kernel over the points of r for each of nsp species (grid z = levels x species)
 template <class K, class... T>
-void launch3s(const char* name, double arrays, K kern, const Range& r, int nsp, hipStream_t st, T... args) {
+void launch3s(const char* name, double arrays, K kern, Range r, int nsp, hipStream_t st, T... args) {
   if (r.i1 <= r.i0 || r.j1 <= r.j0 || r.k1 <= r.k0 || nsp <= 0) return;
   const int by = (r.j1 - r.j0) == 1 ? 1 : 4;
+  r.kfast = point_order();
   dim3 gr = grid_of(r, by);
   gr.z *= unsigned(nsp);
   const bool t = timing(name);
