@@ -40,6 +40,8 @@ os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# the CPUs this process may run on, read before OpenMP binds the main thread to one of them
+AFFINITY_CPUS = len(os.sched_getaffinity(0))
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(HERE, "tests"))
 
@@ -278,7 +280,7 @@ def cpu_baseline(budget_s=15.0):
     dt = time.perf_counter() - t
     g = case.grid
     return {"value": g.nx * g.ny * g.nz * n / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
-            "host_cpus": os.cpu_count(), "physical_cores": physical_cores(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "host_cpus": os.cpu_count(), "physical_cores": physical_cores(), "affinity_cpus": AFFINITY_CPUS,
             "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
             "cpu_model": cpu_model(), "oracle_flags": "-O3 -march=x86-64-v3 -ffp-contract=off -fopenmp",
             "sample": f"oracle (FP64), {n} steps of the 119x89x44 real-data-shaped workload after 1 untimed step, "

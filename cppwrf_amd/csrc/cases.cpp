@@ -8,6 +8,7 @@
 // hydrostatic base and perturbation state (cases.cpp:224-287), with p_top the sounding's pressure at
 // ztop (src/cases/model.cpp:93-96).  Arithmetic is the host's double with no contraction, so the
 // fields equal the reference's bit for bit.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -27,6 +28,7 @@ constexpr double kCvpm = -kCv / kCp, kRcp = kRd / kCp, kP1000 = 100000.0, kT0 = 
 constexpr double kPi = 3.141592653589793;  // std::numbers::pi
 
 constexpr double kRvovrd = 461.6 / kRd;  // R_v / R_d (reference constants.hpp)
+constexpr double kEp2 = kRd / 461.6;      // R_d / R_v
 
 // Exner function of height for a given theta(z), hydrostatic, on a table of step dz (reference Sounding,
 // src/cases/cases.cpp:17-87).  With a vapour profile qv(z) the integration uses the density potential
@@ -185,6 +187,11 @@ void build(mi_case& c, const mi_case_spec& sp) {
   const double xcen = 0.5 * nx * G.dx;
   std::vector<double> noise;
   std::vector<double> exn(static_cast<size_t>(nz));  // the base state's Exner function (set below)
+  std::function<double(double)> qv_env;              // a vapour profile: the full moist state below
+  if (sp.qv0 > 0) {  // the optional unsaturated profile of any case (reference case.qv0, cases.cpp:184-188)
+    const double hq = sp.qv_hscale > 0 ? sp.qv_hscale : 2500.0, qv0 = sp.qv0;
+    qv_env = [=](double z) { return qv0 * std::exp(-z / hq); };
+  }
   switch (sp.kind) {
     case MI_CASE_REST_STATE: {
       const double th0 = 300.0, N = 0.01, u0 = sp.u0;
@@ -214,6 +221,44 @@ void build(mi_case& c, const mi_case_spec& sp) {
         volatile double two = 2.0;  // glibc pow, not a multiply folded by the compiler
         const double L = std::sqrt(std::pow((x - xc) / xr, two) + std::pow((z - zc) / zr, two));
         return L <= 1.0 ? dT * 0.5 * (std::cos(kPi * L) + 1.0) / exn[size_t(k)] : 0.0;
+      };
+      break;
+    }
+    case MI_CASE_SQUALL2D: {
+      // Weisman & Klemp (1982) analytic sounding with a 2 K warm bubble of radii 10 km x 1.4 km at 1.4 km
+      // (reference squall2d, cases.cpp:135-179): theta = theta0 + (theta_tr - theta0) (z/z_tr)^(5/4) below z_tr,
+      // isothermal T_tr above; RH = 1 - 0.75 (z/z_tr)^(5/4) (0.25 above); qv from RH, capped at 14 g/kg, by a
+      // fixed-point iteration with the moist sounding on a 50 m table (qv needs p(z), which needs qv); no wind
+      const double th0 = 300.0, thtr = 343.0, ztr = 12000.0, ttr = 213.0, qvmax = 0.014;
+      const double xc = xcen, zc = 1400.0, xr = 10000.0, zr = 1400.0, dth = 2.0;
+      theta_env = [=](double z) {
+        return z <= ztr ? th0 + (thtr - th0) * std::pow(z / ztr, 1.25) : thtr * std::exp(kG / (kCp * ttr) * (z - ztr));
+      };
+      auto rh = [=](double z) { return z <= ztr ? 1.0 - 0.75 * std::pow(z / ztr, 1.25) : 0.25; };
+      const double zmax = sp.ztop * 1.2 + 1000.0, dzt = 50.0;
+      const int nt = int(zmax / dzt) + 2;
+      auto qtab = std::make_shared<std::vector<double>>(size_t(nt), 0.0);
+      auto qv_of = [qtab, dzt, nt](double z) {
+        const double f = std::clamp(z / dzt, 0.0, double(nt - 1) - 1e-9);
+        const int n = int(f);
+        return (*qtab)[size_t(n)] + (f - n) * ((*qtab)[size_t(n) + 1] - (*qtab)[size_t(n)]);
+      };
+      for (int it = 0; it < 8; ++it) {
+        const Sounding snd(theta_env, p_surf, zmax, 0.5,
+                           it == 0 ? std::function<double(double)>{} : std::function<double(double)>(qv_of));
+        std::vector<double> q(static_cast<size_t>(nt));
+        for (int n = 0; n < nt; ++n) {
+          const double z = std::min(n * dzt, zmax), pz = snd.p(z), T = theta_env(z) * std::pow(pz / kP1000, kRcp);
+          const double es = 611.2 * std::exp(17.67 * (T - 273.15) / (T - 29.65));  // Pa
+          q[size_t(n)] = std::min(qvmax, rh(z) * kEp2 * es / (pz - es));
+        }
+        *qtab = q;
+      }
+      qv_env = qv_of;
+      pert = [=](int, int, int, double x, double, double z) {
+        volatile double two = 2.0;  // glibc pow, not a multiply folded by the compiler
+        const double L = std::sqrt(std::pow((x - xc) / xr, two) + std::pow((z - zc) / zr, two));
+        return L <= 1.0 ? dth * std::pow(std::cos(0.5 * kPi * L), two) : 0.0;
       };
       break;
     }
@@ -321,12 +366,11 @@ void build(mi_case& c, const mi_case_spec& sp) {
         if (G.bc_x == MI_BC_SYMMETRIC && (i == 0 || i == nx)) uu = 0.0;
         c.u[(size_t(k) * ny + j) * nxu + i] = uu;
       }
-  if (sp.qv0 > 0) {
+  if (qv_env) {
     // the full moist state (reference initialize_moist, cases.cpp:290-352; WRF module_initialize_ideal with
-    // use_theta_m = 1): qv(z) = qv0 exp(-z / qv_hscale) from a moist sounding, mu' from the vapour load, p' in
-    // hydrostatic balance with the moisture loading of pg_buoy_w from the top down, theta_m, al and ph
-    const double hq = sp.qv_hscale > 0 ? sp.qv_hscale : 2500.0, qv0 = sp.qv0;
-    const Sounding msnd(theta_env, p_surf, sp.ztop * 1.2 + 1000.0, 0.5, [=](double z) { return qv0 * std::exp(-z / hq); });
+    // use_theta_m = 1): qv(z) from a moist sounding, mu' from the vapour load, p' in hydrostatic balance with the
+    // moisture loading of pg_buoy_w from the top down, theta_m, al and ph
+    const Sounding msnd(theta_env, p_surf, sp.ztop * 1.2 + 1000.0, 0.5, qv_env);
     const double pd_surf = p_surf - msnd.vapour_load(0.0);
     const double mu1 = pd_surf - c.p_top - mub;
     std::vector<double> qv(static_cast<size_t>(nz)), thd(static_cast<size_t>(nz));

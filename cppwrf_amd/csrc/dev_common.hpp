@@ -24,6 +24,8 @@ constexpr double kCp = 7.0 * kRd / 2.0;
 constexpr double kCpovcv = kCp / (kCp - kRd);
 constexpr double kP1000 = 100000.0;
 constexpr double kT0 = 300.0;
+constexpr double kRcp = kRd / kCp;
+constexpr double kRvovrd = 461.6 / kRd;  // r_v / r_d
 constexpr double H = 0.5, Q = 0.25;
 constexpr double kReradius = 1. / 6370.0e03;
 constexpr int kHalo = 3;

@@ -389,8 +389,19 @@ struct Domain final : mi_domain {
   // the u, v halos first (set_w_surface, rk3.cpp:194; those refreshes change nothing without a
   // LateralBC).  The reference also reruns calc_p_rho_phi on the interior after spec_bdy_final: its
   // inputs there are unchanged and the scalar pow gives back the same bits, so it is left out.
+  // PhysicsCoupling::step after the RK loop (rk3.cpp:151-158): the microphysics, the halos of what it changed
+  // (physics_coupling.cpp:450-455), then calc_p_rho_phi, without the spec zone with specified boundaries
+  void microphysics() {
+    launch_microphysics(a, sc, stream());
+    std::vector<HaloReq> hs = {{a.f.t2, -1, -1}, {a.f.hdia, -1, -1}};
+    for (int n = 0; n < prm.n_moist; ++n) hs.push_back({sc.q2[n], -1, -1});
+    halo(hs.data(), int(hs.size()));
+    if (spec) calc_p_rho_phi_interior(1);
+    else calc_p_rho_phi();
+  }
   void step_end(bool hooks = true) {
     if (phys_on(hooks)) call_phys(phys.after_rk);  // phy_prep_part2 (rk3.cpp:150)
+    if (a.q.mp) microphysics();
     step_end_dyn();
     if (phys_on(hooks)) call_phys(phys.after_all_rk_steps);  // rk3.cpp:166
   }
@@ -524,12 +535,19 @@ struct Domain final : mi_domain {
   struct Carried {
     void* p;        // point (0,0,0)
     int64_t bytes;  // the whole allocation
+    int64_t esize;  // its element size
   };
+  // the state a step carries to the next: the prognostic fields, the scalars and, with the microphysics, the
+  // heating it leaves for the next step's dynamics and the accumulated precipitation
   std::vector<Carried> carried() {
-    const int64_t b3 = n3 * int64_t(sizeof(real)), b2 = n2 * int64_t(sizeof(double));
-    std::vector<Carried> c = {{a.f.u2, b3}, {a.f.v2, b3}, {a.f.w2, b3}, {a.f.t2, b3}, {a.f.ph2, b3},
-                              {a.f.mu2, b2}, {a.f.p, b3}, {a.f.al, b3}};
-    for (int n = 0; n < n_sc; ++n) c.push_back({sc.q2[n], b3});
+    const int64_t e3 = int64_t(sizeof(real)), b3 = n3 * e3, b2 = n2 * int64_t(sizeof(double));
+    std::vector<Carried> c = {{a.f.u2, b3, e3}, {a.f.v2, b3, e3}, {a.f.w2, b3, e3}, {a.f.t2, b3, e3},
+                              {a.f.ph2, b3, e3}, {a.f.mu2, b2, 8}, {a.f.p, b3, e3}, {a.f.al, b3, e3}};
+    for (int n = 0; n < n_sc; ++n) c.push_back({sc.q2[n], b3, e3});
+    if (a.q.mp) {
+      c.push_back({a.f.hdia, b3, e3});
+      c.push_back({a.f.rainnc, b2, 8});
+    }
     return c;
   }
   Scalars sc_saved{};
@@ -546,8 +564,7 @@ struct Domain final : mi_domain {
     if (backup.empty())
       for (const Carried& x : c) backup.push_back(alloc<char>(x.bytes));
     for (size_t q = 0; q < c.size(); ++q) {
-      const int64_t elt = c[q].p == static_cast<void*>(a.f.mu2) ? int64_t(sizeof(double)) : int64_t(sizeof(real));
-      char* live = static_cast<char*>(c[q].p) - off * elt;
+      char* live = static_cast<char*>(c[q].p) - off * c[q].esize;
       hip_check(hipMemcpyAsync(save ? backup[q] : live, save ? live : backup[q], size_t(c[q].bytes),
                                hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
     }
@@ -583,10 +600,7 @@ struct Domain final : mi_domain {
     h.payload = state_bytes();
     return h;
   }
-  char* carried_base(const Carried& c) {
-    const int64_t elt = c.p == static_cast<void*>(a.f.mu2) ? int64_t(sizeof(double)) : int64_t(sizeof(real));
-    return static_cast<char*>(c.p) - off * elt;
-  }
+  char* carried_base(const Carried& c) { return static_cast<char*>(c.p) - off * c.esize; }
   void save_state(void* host, int64_t bytes) override {
     if (!host || bytes < state_bytes()) throw Error(MI_EINVAL, "state buffer too small: mi_domain_state_bytes gives the size");
     const StateHeader h = state_header();
@@ -812,6 +826,10 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     throw Error(MI_EINVAL, "km_opt must be 1 (constant K); the TKE and Smagorinsky options 2-4 are not built");
   if (p.diff_opt == 1 && !(p.khdif >= 0 && p.kvdif >= 0 && p.pr_inv >= 0))
     throw Error(MI_EINVAL, "khdif, kvdif and pr_inv must be >= 0");
+  if (p.mp_physics != 0 && p.mp_physics != 1)
+    throw Error(MI_EINVAL, "mp_physics must be 0 or 1 (Kessler); the reference's ice schemes are not built");
+  if (p.mp_physics == 1 && (p.n_moist < 3 || g.nz > kKesslerMaxLevels || g.nz < 2))
+    throw Error(MI_EINVAL, "Kessler needs the moist species qv, qc, qr (n_moist >= 3) and nz <= 128");
   d.ctx = ctx;
   d.grid = g;
   d.prm = p;
@@ -939,6 +957,11 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     f.phb = BField{d.phb_buf, 0, 0, 1};
   }
   f.t_init = p.terrain ? BField{d.t_init_buf, 1, G.sj, G.sk} : BField{d.t_init_buf, 0, 0, 1};
+  if (p.mp_physics) {  // moist_physics_prep's fields and the precipitation
+    add3(f.th_phy, St::M, -1); add3(f.mrho, St::M, -1); add3(f.pii, St::M, -1); add3(f.zmid, St::M, -1);
+    add3(f.dz8w, St::M, -1); add3(f.qvd, St::M, -1); add3(f.qcd, St::M, -1);
+    add2(f.rainnc, St::M2, MI_RAINNC); add2(f.rainncv, St::M2, MI_RAINNCV);
+  }
   f.u_base = d.alloc(nz + 1);
   f.v_base = d.alloc(nz + 1);
   f.qv_base = d.alloc(nz + 1);
@@ -988,6 +1011,8 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   q.spec = d.spec ? 1 : 0;
   q.diff_6th_opt = p.diff_6th_opt;
   q.diff_6th_factor = p.diff_6th_factor;
+  q.mp = p.mp_physics;
+  q.mp_lim = p.mp_tend_lim > 0 ? p.mp_tend_lim : 10.0;
   q.diff_opt = p.diff_opt;
   if (p.diff_opt == 1) {  // DiffusionDriver's constants (model.cpp:314, km_kh.hpp:35-45)
     const double prandtl = 1.0 / (p.pr_inv > 0 ? p.pr_inv : 3.0);

@@ -77,6 +77,10 @@ struct DomainOptions {
   int precision = MI_FP64;
   int diff_opt = 0, km_opt = 1;
   double khdif = 0, kvdif = 0, pr_inv = 3.0;
+  // RK3::enable_physics(PhysicsParams) (rk3.hpp:34-78, physics_coupling.hpp:23-37): mp_physics 1 (Kessler) after the
+  // RK loop, with mp_tend_lim
+  int mp_physics = 0;
+  double mp_tend_lim = 10.0;
 };
 
 // number of values of a field id in the dense layout of the C ABI
@@ -89,7 +93,7 @@ inline size_t field_size(int field, const mi_grid& g) {
     case MI_RW_TENDF:
       return nx * ny * (nz + 1);
     case MI_MU: case MI_MUTS: case MI_MUB: case MI_MSFTX: case MI_MSFTY: case MI_F: case MI_E: case MI_SINA:
-    case MI_COSA: case MI_HT:
+    case MI_COSA: case MI_HT: case MI_RAINNC: case MI_RAINNCV:
       return nx * ny;
     case MI_MSFUX: case MI_MSFUY: return (nx + 1) * ny;
     case MI_U_BASE: case MI_V_BASE: case MI_QV_BASE: return nz;
@@ -331,6 +335,8 @@ class RK3 {
     p.khdif = opt.khdif;
     p.kvdif = opt.kvdif;
     p.pr_inv = opt.pr_inv;
+    p.mp_physics = opt.mp_physics;
+    p.mp_tend_lim = opt.mp_tend_lim;
     check(mi_domain_create(ctx.get(), &g, &p, &levels, &d_));
   }
   ~RK3() { mi_domain_destroy(d_); }

@@ -1311,6 +1311,17 @@ MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
 // sweeps become compositions of the affine maps y_k = p_k + q_k y_(k-1): the same solution in another rounding,
 // within FP32-bulk's error bar.  FP64 keeps the reference's serial order (bit-identical).
 constexpr bool kScanChains = sizeof(real) == 4;
+// a / b in the substep kernel: the IEEE division in FP64 (the reference's bits); in FP32-bulk a multiply by the
+// hardware reciprocal of b refined by one Newton step (a few FP64 ulps, far below the float storage's rounding),
+// which replaces the division's ten-instruction sequence
+__device__ __forceinline__ double bdiv(double a, double b) {
+  if constexpr (sizeof(real) == 8) {
+    return a / b;
+  } else {
+    const double r = __builtin_amdgcn_rcp(b);
+    return a * __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+  }
+}
 // inclusive prefix sum over the lanes of a wavefront
 __device__ __forceinline__ double wave_prefix_sum(double v, int lane) {
 #pragma unroll
@@ -1471,7 +1482,7 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
     LDS(AL, k, ci) = cal;
     LDS(GA, k, ci) = cga;
     if (mk) {
-      const double fx = (u1 + (c.c1h[k] * muu1 + c.c2h[k]) * us1 / msfuy1) - (u0 + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0);
+      const double fx = (u1 + bdiv((c.c1h[k] * muu1 + c.c2h[k]) * us1, msfuy1)) - (u0 + bdiv((c.c1h[k] * muu0 + c.c2h[k]) * us0, msfuy0));
       double d = c.rdx * fx;
       if (!g.two_d) d += c.rdy * ((v1 + (c.c1h[k] * muv1 + c.c2h[k]) * vs1 * mvi1) - (v0 + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0));
       dv = msftx * msfty * d;
@@ -1517,7 +1528,7 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
   // ---------------------------------------------------------------- 3: the omega'' terms
   if (on && k + 1 < nz) {
     const double mt = S2[4 * CI + ci];
-    LDS(TK, k, ci) = c.dnw[k] * (c.c1h[k] * dmdt + dv + c.c1h[k] * mt) / msfty;
+    LDS(TK, k, ci) = bdiv(c.dnw[k] * (c.c1h[k] * dmdt + dv + c.c1h[k] * mt), msfty);
   }
   __syncthreads();
   // ---------------------------------------------------------------- 4: the running integral
@@ -1545,17 +1556,17 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
     f.ww[x] = wwk;
     {
       const double sw = (first ? 0.0 : wwm) + wwk;
-      put<kHMass>(f.ww_m, g, i, j, k * sk, last ? sw / rn + ww1 : sw);
+      put<kHMass>(f.ww_m, g, i, j, k * sk, last ? bdiv(sw, rn) + ww1 : sw);
     }
     if (mk) {
       // wdtn(k) and wdtn(k+1): omega'' times theta at the w level (0 at the bottom and the top)
       const double wdtn_k = (k == 0) ? 0.0 : wwk * (c.fnm[k] * ts + c.fnp[k] * tskm);
       const double wdtn_kp = (k + 1 == nz) ? 0.0 : LDS(WW, k + 1, ci) * (c.fnm[k + 1] * tskp + c.fnp[k + 1] * ts);
       const double su = (first ? 0.0 : rum) + u0;
-      put<kHU>(f.ru_m, g, i, j, k * sk, last ? su / rn + (c.c1h[k] * muu0 + c.c2h[k]) * us0 / msfuy0 : su);
+      put<kHU>(f.ru_m, g, i, j, k * sk, last ? bdiv(su, rn) + bdiv((c.c1h[k] * muu0 + c.c2h[k]) * us0, msfuy0) : su);
       if (!g.two_d) {
         const double sv = (first ? 0.0 : rvm) + v0;
-        put<kHV>(f.rv_m, g, i, j, k * sk, last ? sv / rn + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
+        put<kHV>(f.rv_m, g, i, j, k * sk, last ? bdiv(sv, rn) + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
       }
       double hflux = H * c.rdx * (double(u1) * (double(tsxp) + ts) - double(u0) * (double(ts) + tsxm));
       if (!g.two_d) hflux += H * c.rdy * (double(v1) * (double(tsyp) + ts) - double(v0) * (double(ts) + tsym));
@@ -1563,7 +1574,7 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
       t2new = tn - dts * msfty * (msftx * hflux + c.rdnw[k] * (wdtn_kp - wdtn_k));
       f.t2[x] = t2new;
       const double tav = H * ((1.0 + epssm) * t2new + (1.0 - epssm) * t2);
-      LDS(T2A, k, ci) = (tav + c.c1h[k] * muave * kT0) / ((c.c1h[k] * muts + c.c2h[k]) * (kT0 + ts));
+      LDS(T2A, k, ci) = bdiv(tav + c.c1h[k] * muave * kT0, (c.c1h[k] * muts + c.c2h[k]) * (kT0 + ts));
     }
     double Rk;
     if (k == 0 || (a.q.top_lid && k == nz)) {
@@ -1576,7 +1587,7 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
         const double wdwn_kp = H * (LDS(WW, k + 1, ci) + wwk) * c.rdnw[k] * (double(phs_p) - phs + f.phb(i, j, k + 1) - f.phb(i, j, k));
         rr -= dts * (c.fnm[k] * wdwn_kp + c.fnp[k] * wdwn_k);
       }
-      Rk = ph + msfty * rr / (c.c1f[k] * mut + c.c2f[k]);
+      Rk = ph + bdiv(msfty * rr, c.c1f[k] * mut + c.c2f[k]);
     }
     LDS(R, k, ci) = Rk;
   }
@@ -1585,7 +1596,7 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
   if (on) {
     const double m = mut, mav = muave;
     const double ope = 1.0 + epssm, ome = 1.0 - epssm;
-    const double msft_inv = 1.0 / msfty;
+    const double msft_inv = bdiv(1.0, msfty);
     double wk;
     if (k == 0) {
       // the surface w'': 0 on flat ground; with terrain, from the small-step u'', v'' and the slope of ht
@@ -1609,15 +1620,15 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
       if (mk) {
         const double r_kp1 = LDS(R, k + 1, ci);
         wk = w2 + dts * rwt +
-             msft_inv * (a.q.moist ? 1.0 / (1.0 + f.cqw[x]) : 1.0) *
+             msft_inv * (a.q.moist ? bdiv(1.0, 1.0 + f.cqw[x]) : 1.0) *
                  (H * dts * kG * c.rdn[k] *
-                  (c2a * c.rdnw[k] / (c.c1h[k] * m + c.c2h[k]) * (ope * (r_kp1 - r_k) + ome * (double(ph_p) - ph)) -
-                   c2a_m * c.rdnw[k - 1] / (c.c1h[k - 1] * m + c.c2h[k - 1]) * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)))) +
+                  (bdiv(c2a * c.rdnw[k], c.c1h[k] * m + c.c2h[k]) * (ope * (r_kp1 - r_k) + ome * (double(ph_p) - ph)) -
+                   bdiv(c2a_m * c.rdnw[k - 1], c.c1h[k - 1] * m + c.c2h[k - 1]) * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)))) +
              dts * kG * msft_inv * (c.rdn[k] * (double(c2a) * alt * LDS(T2A, k, ci) - double(c2a_m) * alt_m * LDS(T2A, k - 1, ci)) - c.c1f[k] * mav);
       } else {
         const int km = nz - 1;
         wk = w2 + dts * rwt +
-             msft_inv * (-H * dts * kG / (c.c1h[km] * m + c.c2h[km]) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * c2a_m * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)) -
+             msft_inv * (bdiv(-H * dts * kG, c.c1h[km] * m + c.c2h[km]) * (c.rdnw[km] * c.rdnw[km]) * 2.0 * c2a_m * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)) -
                          dts * kG * (2.0 * c.rdnw[km] * c2a_m * alt_m * LDS(T2A, km, ci) + c.c1f[k] * mav));
         if (a.q.top_lid) wk = 0;
       }
@@ -1667,16 +1678,16 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
       if (dampmag != 0.0) {
         const double hdepth = a.q.zdamp;
         const double pi = 3.141592653589793;  // 4 atan(1), folded
-        const double htop = (ph_top + f.phb(i, j, nz)) / kG, hbot = htop - hdepth;
-        const double hk = (phs + f.phb(i, j, k)) / kG;
+        const double htop = bdiv(ph_top + f.phb(i, j, nz), kG), hbot = htop - hdepth;
+        const double hk = bdiv(phs + f.phb(i, j, k), kG);
         double dampwt = 0;
         if (hk >= hbot) {
-          const double sn = mi_libm::sin(H * pi * (hk - hbot) / hdepth);
+          const double sn = mi_libm::sin(bdiv(H * pi * (hk - hbot), hdepth));
           dampwt = dampmag * sn * sn;
         }
-        wk = (wk - dampwt * (c.c1f[k] * mut + c.c2f[k]) * wsv) / (1.0 + dampwt);
+        wk = bdiv(wk - dampwt * (c.c1f[k] * mut + c.c2f[k]) * wsv, 1.0 + dampwt);
       }
-      phk = LDS(R, k, ci) + msfty * H * dts * kG * (1.0 + epssm) * wk / (c.c1f[k] * muts + c.c2f[k]);
+      phk = LDS(R, k, ci) + bdiv(msfty * H * dts * kG * (1.0 + epssm) * wk, c.c1f[k] * muts + c.c2f[k]);
       put<kHMass>(f.ph2, g, i, j, k * sk, phk);
     }
     f.w2[x] = wk;
@@ -1686,9 +1697,9 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
   // ---------------------------------------------------------------- 9: p'', al'' with divergence damping
   if (on && mk) {
     const double m = c.c1h[k] * muts + c.c2h[k];
-    const double al = -((1. / m) * (alt * (c.c1h[k] * mu_new) + c.rdnw[k] * (LDS(PH, k + 1, ci) - LDS(PH, k, ci))));
+    const double al = -(bdiv(1., m) * (alt * (c.c1h[k] * mu_new) + c.rdnw[k] * (LDS(PH, k + 1, ci) - LDS(PH, k, ci))));
     put<kHMass>(f.al, g, i, j, k * sk, al);
-    double pp = c2a * (alt * (t2new - c.c1h[k] * mu_new * ts) / (m * (kT0 + ts)) - al);
+    double pp = c2a * (bdiv(alt * (t2new - c.c1h[k] * mu_new * ts), m * (kT0 + ts)) - al);
     const double ptmp = pp;
     pp = pp + a.q.smdiv * (pp - pm1);
     f.pm1[x] = ptmp;
@@ -2226,6 +2237,161 @@ MI_POINT_KERNEL k_p_rho_phi(Range r, Args a) {
   put<kHMass>(f.al, g, i, j, k * g.sk, al);
   const double base = kRd * (kT0 + f.t2[x]) / (kP1000 * (al + f.alb(i, j, k)));
   put<kHMass>(f.p, g, i, j, k * g.sk, kP1000 * mi_libm::pow(base, kCpovcv) - f.pb(i, j, k));
+}
+
+// ----------------------------------------------------------------------------- microphysics after the RK loop
+// PhysicsCoupling::step with Kessler warm rain (reference src/dyn/physics_coupling.cpp:262-455,
+// src/phys/mp_kessler/kessler.hpp:16-151; WRF moist_physics_prep_em, module_mp_kessler.F,
+// moist_physics_finish_em with use_theta_m = 1).  Species 0, 1, 2 are qv, qc, qr.  The libm calls are glibc's
+// scalar pow and exp (the reference built with CPPWRF_NO_LIBMVEC), in FP64 in both storage modes.
+
+// moist_physics_prep_em (KPrep, KSaveQc): every mass point
+MI_POINT_KERNEL k_mp_prep(Range r, Args a, Scalars s) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  Fields f = a.f;
+  const idx_t x = g.at(i, j, k);
+  const double zw0 = (double(f.ph2[x]) + f.phb(i, j, k)) / kG;
+  const double zw1 = (double(f.ph2[x + g.sk]) + f.phb(i, j, k + 1)) / kG;
+  f.dz8w[x] = real(zw1 - zw0);
+  f.zmid[x] = real(0.5 * (zw0 + zw1));
+  const double q = s.q2[0][x];
+  const double th = (double(f.t2[x]) + kT0) / (1.0 + kRvovrd * q);
+  f.th_phy[x] = real(th);
+  f.hdia[x] = real(th);
+  f.qvd[x] = real(q);
+  f.mrho[x] = real(1.0 / (double(f.al[x]) + f.alb(i, j, k)));
+  f.pii[x] = real(mi_libm::pow((double(f.p[x]) + f.pb(i, j, k)) / kP1000, kRcp));
+  f.qcd[x] = s.q2[1][x];
+}
+
+// Kessler (KesslerKernel, kessler.hpp:16-151): one thread per column outside the spec zone.  The per-level values
+// the reference keeps in column arrays and that do not change (density, the layer spacings, the density ratio of
+// the fall speed) are recomputed from the fields with the same expressions; the fall speeds and the falling rain
+// stay in thread arrays
+__global__ void __launch_bounds__(64) k_kessler(Args a, Scalars s, double dt, int i0, int i1, int j0, int j1) {
+  const Blk b = xcd_block();
+  const int i = i0 + b.x * 64 + int(threadIdx.x), j = j0 + b.y;
+  if (i >= i1 || j >= j1) return;
+  const Geo& g = a.g;
+  Fields f = a.f;
+  real* qv = s.q2[0];
+  real* qc = s.q2[1];
+  real* qr = s.q2[2];
+  const int nz = g.nz;
+  const idx_t x0 = g.at(i, j, 0), sk = g.sk;
+  double vt[kKesslerMaxLevels], prodk[kKesslerMaxLevels];
+  const double xlv = 2.5e6, cp = kCp, ep2 = kRd / 461.6, svp1 = 0.6112, svp2 = 17.67, svp3 = 29.65, svpt0 = 273.15,
+               rhowater = 1000.0;
+  const double f5 = svp2 * (svpt0 - svp3) * xlv / cp;
+  const double c1 = 0.001, c2 = 0.001, c3 = 2.2, c4 = 0.875, max_cr_sedimentation = 0.75;
+  auto rho = [&](int k) { return double(f.mrho[x0 + k * sk]); };
+  auto rdzk = [&](int k) {
+    return k < nz - 1 ? 1.0 / (double(f.zmid[x0 + (k + 1) * sk]) - double(f.zmid[x0 + k * sk]))
+                      : 1.0 / (double(f.zmid[x0 + (nz - 1) * sk]) - double(f.zmid[x0 + (nz - 2) * sk]));
+  };
+  auto rdzw = [&](int k) { return 1.0 / double(f.dz8w[x0 + k * sk]); };
+  const double rho0 = rho(0);
+  auto vtden = [&](int k) { return __builtin_sqrt(rho0 / rho(k)); };
+  auto mx = [](double p, double q) { return p < q ? q : p; };  // std::max(p, q): p on ties and NaN
+  auto mn = [](double p, double q) { return q < p ? q : p; };  // std::min(p, q): p on ties and NaN
+  double crmax = 0;
+  for (int k = 0; k < nz; ++k) {
+    prodk[k] = qr[x0 + k * sk];
+    const double qrr = mx(0.0, double(qr[x0 + k * sk]) * 0.001 * rho(k));
+    vt[k] = 36.34 * mi_libm::pow(qrr, 0.1364) * vtden(k);
+    crmax = mx(vt[k] * dt * rdzw(k), crmax);
+  }
+  int nfall = 1;
+  {
+    const int n = int(__builtin_round(0.5 + crmax / max_cr_sedimentation));
+    nfall = n > 1 ? n : 1;
+  }
+  double dtfall = dt / double(nfall);
+  double time_sediment = dt;
+  const idx_t x2 = g.at2(i, j);
+  while (nfall > 0) {
+    time_sediment = time_sediment - dtfall;
+    const double ppt = rho0 * prodk[0] * vt[0] * dtfall / rhowater;
+    f.rainncv[x2] = ppt * 1000.0;
+    f.rainnc[x2] = f.rainnc[x2] + ppt * 1000.0;
+    for (int k = 0; k < nz - 1; ++k) {
+      const double factor = dtfall * rdzk(k) / rho(k);
+      prodk[k] = prodk[k] - factor * (rho(k) * prodk[k] * vt[k] - rho(k + 1) * prodk[k + 1] * vt[k + 1]);
+    }
+    {
+      const double factor = dtfall * rdzk(nz - 1);
+      prodk[nz - 1] = prodk[nz - 1] - factor * prodk[nz - 1] * vt[nz - 1];
+    }
+    if (nfall > 1) {
+      nfall = nfall - 1;
+      crmax = 0;
+      for (int k = 0; k < nz; ++k) {
+        const double qrr = mx(0.0, prodk[k] * 0.001 * rho(k));
+        vt[k] = 36.34 * mi_libm::pow(qrr, 0.1364) * vtden(k);
+        crmax = mx(vt[k] * time_sediment * rdzw(k), crmax);
+      }
+      const int n = int(__builtin_round(0.5 + crmax / max_cr_sedimentation));
+      const int nfall_new = n > 1 ? n : 1;
+      if (nfall_new != nfall) {
+        nfall = nfall_new;
+        dtfall = time_sediment / double(nfall);
+      }
+    } else {
+      nfall = 0;  // prod = prodk
+    }
+  }
+  for (int k = 0; k < nz; ++k) {
+    const idx_t x = x0 + k * sk;
+    double qck = qc[x], qrk = qr[x], qvk = qv[x], th = f.th_phy[x];
+    const double pk = f.pii[x], rhk = f.mrho[x];
+    const double factorn = 1.0 / (1.0 + c3 * dt * mi_libm::pow(mx(0.0, qrk), c4));
+    const double qrprod = qck * (1.0 - factorn) + factorn * c1 * dt * mx(qck - c2, 0.0);
+    const double rcgs = 0.001 * rhk;
+    qck = mx(qck - qrprod, 0.0);
+    qrk = (qrk + prodk[k] - qrk);
+    qrk = mx(qrk + qrprod, 0.0);
+    const double temp = pk * th;
+    const double pressure = 1.000e+05 * mi_libm::pow(pk, 1004.0 / 287.0);
+    const double gam = 2.5e+06 / (1004.0 * pk);
+    const double es = 1000.0 * svp1 * mi_libm::exp(svp2 * (temp - svpt0) / (temp - svp3));
+    const double qvs = ep2 * es / (pressure - es);
+    const double prod = (qvk - qvs) / (1.0 + pressure / (pressure - es) * qvs * f5 / ((temp - svp3) * (temp - svp3)));
+    const double dim_qvs_qv = mx(0.0, qvs - qvk);
+    const double ern_num = (1.6 + 124.9 * mi_libm::pow(rcgs * qrk, 0.2046)) * mi_libm::pow(rcgs * qrk, 0.525);
+    const double ern_den = 2.55e8 / (pressure * qvs) + 5.4e5;
+    const double ern_term1 = dt * (ern_num / ern_den) * (dim_qvs_qv / (rcgs * qvs));
+    const double ern_term2 = mx(-prod - qck, 0.0);
+    const double ern = mn(ern_term1, mn(ern_term2, qrk));
+    const double product = mx(prod, -qck);
+    th = th + gam * (product - ern);
+    qvk = mx(qvk - product + ern, 0.0);
+    qck = qck + product;
+    qrk = qrk - ern;
+    f.th_phy[x] = real(th);
+    qv[x] = real(qvk);
+    qc[x] = real(qck);
+    qr[x] = real(qrk);
+  }
+}
+
+// moist_physics_finish_em (KFinish): every mass point; h_diabatic becomes the heating rate the next step's
+// dynamics feels
+MI_POINT_KERNEL k_mp_finish(Range r, Args a, Scalars s, double dt) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  Fields f = a.f;
+  const idx_t x = g.at(i, j, k);
+  const double lim = a.q.mp_lim;
+  const double th = f.th_phy[x], th_old = f.hdia[x], q = s.q2[0][x], q_old = f.qvd[x];
+  double mpten = th - th_old;
+  mpten = mpten < lim * dt ? mpten : lim * dt;
+  mpten = mpten > -lim * dt ? mpten : -lim * dt;
+  const double qvten = q - q_old;
+  f.t2[x] = real(th_old * (1.0 + kRvovrd * q_old) + mpten * (1.0 + kRvovrd * q) + kRvovrd * qvten * th - kT0);
+  f.hdia[x] = real((mpten * (1.0 + kRvovrd * q) + kRvovrd * qvten * th) / dt);
+  f.qvd[x] = real(qvten / dt);
+  f.qcd[x] = real((double(s.q2[1][x]) - double(f.qcd[x])) / dt);
 }
 
 // ----------------------------------------------------------------------------- sixth-order diffusion
@@ -3211,7 +3377,8 @@ void dispatch_cfg(int cfg, T&&... args) {
     case 8: Launch<64, 8>::go(args...); break;
     case 9: Launch<64, 16>::go(args...); break;
     case 10: Launch<32, 8>::go(args...); break;
-    default: Launch<16, 64>::go(args...); break;
+    case 11: Launch<16, 64>::go(args...); break;
+    default: Launch<4, 64>::go(args...); break;
   }
 }
 
@@ -3494,6 +3661,18 @@ void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hip
   if (n <= 0) return;
   const int64_t off = int64_t(kLpad) + int64_t(a.g.hy) * a.g.sj;
   hipLaunchKernelGGL(k_copy_scalars, dim3(unsigned((n3 + 255) / 256), unsigned(n), 1), dim3(256, 1, 1), 0, st, s, n3, off);
+}
+void launch_microphysics(const Args& a, const Scalars& s, hipStream_t st) {
+  const Geo& g = a.g;
+  const double dt = a.q.dt;
+  // moist_physics_prep over every mass point (q, t, p, al, ph -> the prep fields), Kessler over the columns
+  // outside the spec zone, moist_physics_finish over every mass point
+  launch3a("mp_prep", 12, k_mp_prep, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, s);
+  const int sz = g.spec() ? 1 : 0, i0 = sz, i1 = g.nx - sz, j0 = sz, j1 = g.ny - sz;
+  if (i1 > i0 && j1 > j0)
+    hipLaunchKernelGGL(k_kessler, dim3(unsigned((i1 - i0 + 63) / 64), unsigned(j1 - j0), 1), dim3(64, 1, 1), 0, st, a, s,
+                       dt, i0, i1, j0, j1);
+  launch3a("mp_finish", 9, k_mp_finish, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, s, dt);
 }
 void launch_p_rho_phi(const Args& a, hipStream_t st) {
   const Geo& g = a.g;

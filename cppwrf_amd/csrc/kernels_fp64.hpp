@@ -65,6 +65,10 @@ struct Fields {
   // the diffusion's base state: theta's t_init (3-D), the 1-D profiles u_base, v_base, qv_base (nz values)
   BField t_init;
   double *u_base, *v_base, *qv_base;
+  // the microphysics after the RK loop: moist_physics_prep's fields (dry theta, density, Exner function, height
+  // and thickness of the layer, the qv and qc before the scheme) and the grid-scale precipitation (2-D)
+  real *th_phy, *mrho, *pii, *zmid, *dz8w, *qvd, *qcd;
+  double *rainnc, *rainncv;
 };
 
 // The advected scalars of a domain (State::q_1 / q_2, Tendencies::q_tendf) and the work fields of their
@@ -111,6 +115,8 @@ struct Params {
   double diff_6th_factor;
   int diff_opt;        // 1: second-order diffusion along eta with constant K (km_opt 1)
   double xkmh, xkhh, kvdif, kvdq;  // its eddy viscosities: khdif, khdif / prandtl, kvdif, kvdif / prandtl
+  int mp;              // microphysics after the RK loop: 1 Kessler
+  double mp_lim;       // the heating-rate limit, K/s
   double dt;           // the large step
   int terrain;         // the surface w follows the terrain (ht)
   int spec;            // specified lateral boundaries (both directions)
@@ -152,10 +158,11 @@ void launch_fill(double* p, int64_t n, double v, hipStream_t st);
 void launch_fill2d(const Args& a, double* p, int64_t n_alloc, double v, hipStream_t st);
 // column kernels and their block shapes (columns x threads per column)
 enum ColKernel { kColLargePrep = 0, kColSsColumn = 1, kColAcoustic = 2, kColFinish = 3, kNumColKernels = 4 };
-constexpr int kNumColCfgs = 12;
-// 64 or 32 columns per block: a wavefront reads 64 (32) consecutive points of one level (two levels)
+constexpr int kNumColCfgs = 13;
+// 64 or 32 columns per block: a wavefront reads 64 (32) consecutive points of one level (two levels); 4x64: four
+// waves per block, so a barrier of the substep kernel holds fewer waves
 constexpr int kColCfgs[kNumColCfgs][2] = {{16, 16}, {16, 32}, {32, 16}, {8, 32}, {8, 64}, {4, 128}, {2, 256},
-                                          {64, 4}, {64, 8}, {64, 16}, {32, 8}, {16, 64}};
+                                          {64, 4}, {64, 8}, {64, 16}, {32, 8}, {16, 64}, {4, 64}};
 constexpr int kColCfgTallest = 6;  // 2 x 256: the most levels per column (the substep kernel's nz limit)
 // one more shape of the substep kernel: one thread per column marching k (k_acoustic_march)
 constexpr int kColMarch = kNumColCfgs;
@@ -216,6 +223,10 @@ void launch_scalars(const Args& a, const Scalars& s, const int* sp, int n, int s
 void launch_pd_geom(const Args& a, double dt, hipStream_t st);
 void launch_copy_scalars(const Args& a, const Scalars& s, int n, int64_t n3, hipStream_t st);
 void launch_p_rho_phi(const Args& a, hipStream_t st);
+// the microphysics after the RK loop (moist_physics_prep, Kessler, moist_physics_finish); the halos and the final
+// calc_p_rho_phi are the caller's
+constexpr int kKesslerMaxLevels = 128;
+void launch_microphysics(const Args& a, const Scalars& s, hipStream_t st);
 // the same on the columns i0 <= i < i1, j0 <= j < j1
 void launch_p_rho_phi_range(const Args& a, int i0, int i1, int j0, int j1, hipStream_t st);
 void launch_finite(const Args& a, int* flags, hipStream_t st);

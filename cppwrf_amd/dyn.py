@@ -94,7 +94,8 @@ class Case:
 
     @property
     def moist(self) -> bool:
-        return (self.kind == A.MI_CASE_HILLS and bool(self.spec.moist)) or self.spec.qv0 > 0
+        return ((self.kind == A.MI_CASE_HILLS and bool(self.spec.moist)) or self.spec.qv0 > 0 or
+                self.kind == A.MI_CASE_SQUALL2D)
 
     def params(self, **kw) -> A.MiParams:
         if self.kind == A.MI_CASE_HILLS:  # the flags a real-data-shaped domain needs
@@ -128,6 +129,12 @@ class Case:
         """the reference's straka case (configs/straka.ini: x-z density current, periodic, K = 75 m2/s with
         pr_inv = 1; run it with params(diff_opt=1, khdif=75, kvdif=75, pr_inv=1))"""
         return cls(A.MI_CASE_STRAKA, Grid(nx, 1, nz, dx, dx, dt, A.MI_BC_PERIODIC), 6400.0)
+
+    @classmethod
+    def squall2d(cls, nx=120, nz=40, dx=1000.0, dt=6.0):
+        """the reference's squall2d case (configs/squall2d.ini: x-z moist bubble in the Weisman-Klemp sounding,
+        periodic; run it with params(n_moist=3, n_scalars=3, mp_physics=1): qv, qc, qr and Kessler)"""
+        return cls(A.MI_CASE_SQUALL2D, Grid(nx, 1, nz, dx, dx, dt, A.MI_BC_PERIODIC), 20000.0)
 
     @classmethod
     def rest_state(cls, nx=24, ny=24, nz=40, u0=0.0, bc=A.MI_BC_PERIODIC, bc_y=None, qv0=0.0, qv_hscale=0.0):

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 3
+#define MI_ABI_VERSION 4
 
 /* status codes */
 #define MI_OK 0
@@ -111,6 +111,13 @@ typedef struct mi_params {
   int32_t diff_opt;
   int32_t km_opt;
   double khdif, kvdif, pr_inv;
+  /* microphysics after the RK loop (reference RK3::enable_physics with PhysicsCoupling, src/dyn/rk3.cpp:151-158,
+   * src/dyn/physics_coupling.cpp:262-455): mp_physics 0 off or 1 (Kessler warm rain, moist species 0..2 = qv,
+   * qc, qr; the reference's ice schemes are not built); mp_tend_lim the limit of the heating rate, K/s (0 takes
+   * the reference's default 10).  Each step ends with moist_physics_prep, Kessler, moist_physics_finish (the
+   * heating kept in MI_H_DIABATIC for the next step) and calc_p_rho_phi. */
+  int32_t mp_physics;
+  double mp_tend_lim;
 } mi_params;
 
 #define MI_MAX_MOIST 8    /* reference KCq::kMax (src/dyn/large_step.cpp:128) */
@@ -161,6 +168,8 @@ enum {
    * 112; the ideal cases take u and v of column (0, 0), src/cases/model.cpp:149-177): nz values each, zero
    * unless set */
   MI_U_BASE = 52, MI_V_BASE = 53, MI_QV_BASE = 54,
+  /* the microphysics' accumulated and last-step grid-scale precipitation, mm (2-D mass points; with mp_physics) */
+  MI_RAINNC = 55, MI_RAINNCV = 56,
   /* scalar n (0 <= n < n_scalars) at mass points: MI_Q + n is its state (q_2), MI_Q_TENDF + n its held
    * tendency (input, mass-coupled, held over the step as the reference's q_tendf) */
   MI_Q = 64, MI_Q_TENDF = 96
@@ -390,9 +399,10 @@ int mi_check_finite(mi_domain* d, int* bad_field);
  * (prep, acoustic prologue, acoustic substep, finish) into shapes[0..n). */
 int mi_domain_autotune(mi_domain* d, int trials);
 int mi_domain_get_tuning(const mi_domain* d, int* shapes, int n);
-/* sets the shapes explicitly (index 0..11 of the library's table of columns x threads per column: 16x16,
- * 16x32, 32x16, 8x32, 8x64, 4x128, 2x256, 64x4, 64x8, 64x16, 32x8, 16x64; the substep kernel needs at least
- * nz+1 threads per column; MI_EINVAL if a shape does not fit nz) */
+/* sets the shapes explicitly (index 0..12 of the library's table of columns x threads per column: 16x16,
+ * 16x32, 32x16, 8x32, 8x64, 4x128, 2x256, 64x4, 64x8, 64x16, 32x8, 16x64, 4x64; the substep kernel needs at
+ * least nz+1 threads per column, or index 13: one thread per column marching k; MI_EINVAL if a shape does not
+ * fit nz) */
 int mi_domain_set_tuning(mi_domain* d, const int* shapes, int n);
 
 /* Per-kernel timing, for measurement (no counterpart in the reference; its CPPWRF_KTIME timer,
@@ -416,7 +426,8 @@ enum {
   MI_CASE_HILLS = 4,       /* real-data-shaped: seeded hills, US standard atmosphere, jets, map factors,
                               Coriolis, hybrid coordinate, optional qv; needs terrain, mapped, coriolis
                               (and n_moist >= 1 when moist) in mi_params */
-  MI_CASE_STRAKA = 5       /* reference straka (x-z density current, Straka et al. 1993) */
+  MI_CASE_STRAKA = 5,      /* reference straka (x-z density current, Straka et al. 1993) */
+  MI_CASE_SQUALL2D = 6     /* reference squall2d (x-z moist bubble in the Weisman & Klemp 1982 sounding) */
 };
 typedef struct mi_case_spec {
   int32_t kind; /* MI_CASE_* */

@@ -23,6 +23,7 @@
 #include <vector>
 
 extern "C" __m128d _ZGVbN2vv_pow(__m128d, __m128d);
+extern "C" __m128d _ZGVbN2v_exp(__m128d);
 
 namespace {
 
@@ -34,6 +35,8 @@ constexpr double kCp = 7.0 * kRd / 2.0;       // constants.hpp:7
 constexpr double kCpovcv = kCp / (kCp - kRd); // constants.hpp:11
 constexpr double kP1000 = 100000.0;           // constants.hpp:12
 constexpr double kT0 = 300.0;                 // constants.hpp:13
+constexpr double kRcp = kRd / kCp;            // constants.hpp
+constexpr double kRvovrd = 461.6 / kRd;       // r_v / r_d, constants.hpp
 constexpr double H = 0.5, Q = 0.25;
 constexpr double kReradius = 1. / 6370.0e03;  // large_step.cpp:29
 
@@ -245,6 +248,11 @@ struct Dom {
   // eddy viscosities (KIsotropicKm, km_kh.hpp:35-45: xkmh = khdif, xkmv = kvdif, xkhh = khdif / prandtl)
   std::vector<double> u_base, v_base, qv_base;
   double xkmh = 0, xkhh = 0, kvdif = 0, kvdq = 0;
+  // microphysics after the RK loop (PhysicsCoupling, physics_coupling.hpp:52-94): moist_physics_prep's fields and
+  // the precipitation
+  int mp = 0;
+  double mp_lim = 10.0;
+  F th_phy, mrho, pii, zmid, dz8w, qvd, qcd, p_phy, rainnc, rainncv;
   // a column physics plugin (ColumnPhysics, column_physics.hpp:21-34; RK3::set_column_physics): its hooks and
   // the advective forcings it asks for (RTHFTEN, RQVFTEN)
   orc_column_physics phys{};
@@ -324,6 +332,13 @@ struct Dom {
       kvdq = p.kvdif / prandtl;
     }
     use_tendf = p.diff_6th_opt != 0 || p.diff_opt != 0 || spec;  // a DiffusionDriver or a LateralBC (large_step.cpp:828)
+    mp = p.mp_physics;
+    mp_lim = p.mp_tend_lim > 0 ? p.mp_tend_lim : 10.0;
+    if (mp) {
+      for (F* f : {&th_phy, &mrho, &pii, &zmid, &dz8w, &qvd, &qcd, &p_phy}) f3(*f, St::M);
+      f2(rainnc, St::M);
+      f2(rainncv, St::M);
+    }
     held_base = use_tendf;
     f3(thf, St::M);
     f3(qvf, St::M);
@@ -2035,8 +2050,146 @@ struct Dom {
     for (F& f : qtf) std::fill(f.a.begin(), f.a.end(), 0.0);
     if (phys.part1) phys.part1(phys.user);
   }
+  // ------------------------------------------------------------------ microphysics (physics_coupling.cpp:262-455)
+  // libm of WRF's vectorised loops (backend.hpp:144-175): libmvec's 2-lane function for the pairs of an n-trip loop,
+  // the scalar one for an odd last trip; scalar everywhere without the lanes
+  static bool lane(int i, int n) { return n > 1 && !((n & 1) && i == n - 1); }
+  double vpow(double a, double b, int i, int n) const {
+    if (libmvec && lane(i, n)) return _mm_cvtsd_f64(_ZGVbN2vv_pow(_mm_set1_pd(a), _mm_set1_pd(b)));
+    return std::pow(a, b);
+  }
+  double vexp(double a, int i, int n) const {
+    if (libmvec && lane(i, n)) return _mm_cvtsd_f64(_ZGVbN2v_exp(_mm_set1_pd(a)));
+    return std::exp(a);
+  }
+  // moist_physics_prep_em, Kessler (kessler.hpp:16-151), moist_physics_finish_em, then the halos
+  void microphysics() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const double dt = g.dt;
+    F& qv = q2[0];
+    F& qc = q2[1];
+    F& qr = q2[2];
+    // KPrep over every mass point; the lanes of WRF's i loop start one column in from specified sides
+    const int pz = spec ? 1 : 0, pi0 = g.bcx == MI_BC_PERIODIC ? 0 : pz, pni = nx - 2 * pi0;
+    pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+      const double zw0 = (ph2(i, j, k) + phb(i, j, k)) / kG;
+      const double zw1 = (ph2(i, j, k + 1) + phb(i, j, k + 1)) / kG;
+      dz8w(i, j, k) = zw1 - zw0;
+      zmid(i, j, k) = 0.5 * (zw0 + zw1);
+      const double q = qv(i, j, k);
+      const double th = (t2(i, j, k) + kT0) / (1.0 + kRvovrd * q);
+      th_phy(i, j, k) = th;
+      hdia(i, j, k) = th;
+      qvd(i, j, k) = q;
+      mrho(i, j, k) = 1.0 / (al(i, j, k) + alb(i, j, k));
+      p_phy(i, j, k) = pp(i, j, k) + pb(i, j, k);
+      pii(i, j, k) = vpow((pp(i, j, k) + pb(i, j, k)) / kP1000, kRcp, i - pi0, pni);
+      qcd(i, j, k) = qc(i, j, k);  // KSaveQc
+    });
+    // Kessler over the columns outside the spec zone; its lanes count from the first one
+    const int sx = spec ? 1 : 0, sy = spec ? 1 : 0, i0 = sx, ni = nx - 2 * sx;
+    const double xlv = 2.5e6, cp = kCp, ep2 = kRd / 461.6, svp1 = 0.6112, svp2 = 17.67, svp3 = 29.65,
+                 svpt0 = 273.15, rhowater = 1000.0;
+    cfor(nx, ny, [&](int i, int j) {
+      if (i < sx || i >= nx - sx || j < sy || j >= ny - sy) return;
+      const size_t nzs = static_cast<size_t>(nz);
+      std::vector<double> vt(nzs), prodk(nzs), vtden(nzs), rdzk(nzs), rhok(nzs), factor(nzs), rdzw(nzs), prod(nzs);
+      const double f5 = svp2 * (svpt0 - svp3) * xlv / cp;
+      const double c1 = 0.001, c2 = 0.001, c3 = 2.2, c4 = 0.875, max_cr_sedimentation = 0.75;
+      double crmax = 0;
+      for (int k = 0; k < nz; ++k) {  // the first fall-speed loop stays scalar in WRF's object (kessler.hpp:42-44)
+        prodk[k] = qr(i, j, k);
+        rhok[k] = mrho(i, j, k);
+        const double qrr = std::max(0.0, qr(i, j, k) * 0.001 * rhok[k]);
+        vtden[k] = std::sqrt(rhok[0] / rhok[k]);
+        vt[k] = 36.34 * std::pow(qrr, 0.1364) * vtden[k];
+        rdzw[k] = 1.0 / dz8w(i, j, k);
+        crmax = std::max(vt[k] * dt * rdzw[k], crmax);
+      }
+      for (int k = 0; k < nz - 1; ++k) rdzk[k] = 1.0 / (zmid(i, j, k + 1) - zmid(i, j, k));
+      rdzk[nz - 1] = 1.0 / (zmid(i, j, nz - 1) - zmid(i, j, nz - 2));
+      int nfall = std::max(1, int(std::round(0.5 + crmax / max_cr_sedimentation)));
+      double dtfall = dt / double(nfall);
+      double time_sediment = dt;
+      while (nfall > 0) {
+        time_sediment = time_sediment - dtfall;
+        for (int k = 0; k < nz - 1; ++k) factor[k] = dtfall * rdzk[k] / rhok[k];
+        factor[nz - 1] = dtfall * rdzk[nz - 1];
+        const double ppt = rhok[0] * prodk[0] * vt[0] * dtfall / rhowater;
+        rainncv(i, j) = ppt * 1000.0;
+        rainnc(i, j) = rainnc(i, j) + ppt * 1000.0;
+        for (int k = 0; k < nz - 1; ++k)
+          prodk[k] = prodk[k] - factor[k] * (rhok[k] * prodk[k] * vt[k] - rhok[k + 1] * prodk[k + 1] * vt[k + 1]);
+        prodk[nz - 1] = prodk[nz - 1] - factor[nz - 1] * prodk[nz - 1] * vt[nz - 1];
+        if (nfall > 1) {
+          nfall = nfall - 1;
+          crmax = 0;
+          for (int k = 0; k < nz; ++k) {  // vectorised over k in WRF's object
+            const double qrr = std::max(0.0, prodk[k] * 0.001 * rhok[k]);
+            vt[k] = 36.34 * vpow(qrr, 0.1364, k, nz) * vtden[k];
+            crmax = std::max(vt[k] * time_sediment * rdzw[k], crmax);
+          }
+          const int nfall_new = std::max(1, int(std::round(0.5 + crmax / max_cr_sedimentation)));
+          if (nfall_new != nfall) {
+            nfall = nfall_new;
+            dtfall = time_sediment / double(nfall);
+          }
+        } else {
+          for (int k = 0; k < nz; ++k) prod[k] = prodk[k];
+          nfall = 0;
+        }
+      }
+      const int li = i - i0;
+      for (int k = 0; k < nz; ++k) {
+        const double factorn = 1.0 / (1.0 + c3 * dt * vpow(std::max(0.0, qr(i, j, k)), c4, li, ni));
+        const double qrprod = qc(i, j, k) * (1.0 - factorn) + factorn * c1 * dt * std::max(qc(i, j, k) - c2, 0.0);
+        const double rcgs = 0.001 * mrho(i, j, k);
+        qc(i, j, k) = std::max(qc(i, j, k) - qrprod, 0.0);
+        qr(i, j, k) = (qr(i, j, k) + prod[k] - qr(i, j, k));
+        qr(i, j, k) = std::max(qr(i, j, k) + qrprod, 0.0);
+        const double temp = pii(i, j, k) * th_phy(i, j, k);
+        const double pressure = 1.000e+05 * vpow(pii(i, j, k), 1004.0 / 287.0, li, ni);
+        const double gam = 2.5e+06 / (1004.0 * pii(i, j, k));
+        const double es = 1000.0 * svp1 * vexp(svp2 * (temp - svpt0) / (temp - svp3), li, ni);
+        const double qvs = ep2 * es / (pressure - es);
+        prod[k] = (qv(i, j, k) - qvs) / (1.0 + pressure / (pressure - es) * qvs * f5 / ((temp - svp3) * (temp - svp3)));
+        const double dim_qvs_qv = std::max(0.0, qvs - qv(i, j, k));
+        const double ern_num = (1.6 + 124.9 * vpow(rcgs * qr(i, j, k), 0.2046, li, ni)) * vpow(rcgs * qr(i, j, k), 0.525, li, ni);
+        const double ern_den = 2.55e8 / (pressure * qvs) + 5.4e5;
+        const double ern_term1 = dt * (ern_num / ern_den) * (dim_qvs_qv / (rcgs * qvs));
+        const double ern_term2 = std::max(-prod[k] - qc(i, j, k), 0.0);
+        const double ern = std::min(ern_term1, std::min(ern_term2, qr(i, j, k)));
+        const double product = std::max(prod[k], -qc(i, j, k));
+        th_phy(i, j, k) = th_phy(i, j, k) + gam * (product - ern);
+        qv(i, j, k) = std::max(qv(i, j, k) - product + ern, 0.0);
+        qc(i, j, k) = qc(i, j, k) + product;
+        qr(i, j, k) = qr(i, j, k) - ern;
+      }
+    });
+    // KFinish over every mass point (use_theta_m = 1)
+    const double lim = mp_lim;
+    pfor(0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+      const double th = th_phy(i, j, k), th_old = hdia(i, j, k), q = qv(i, j, k), q_old = qvd(i, j, k);
+      double mpten = th - th_old;
+      mpten = mpten < lim * dt ? mpten : lim * dt;
+      mpten = mpten > -lim * dt ? mpten : -lim * dt;
+      const double qvten = q - q_old;
+      t2(i, j, k) = th_old * (1.0 + kRvovrd * q_old) + mpten * (1.0 + kRvovrd * q) + kRvovrd * qvten * th - kT0;
+      hdia(i, j, k) = (mpten * (1.0 + kRvovrd * q) + kRvovrd * qvten * th) / dt;
+      qvd(i, j, k) = qvten / dt;
+      qcd(i, j, k) = (qc(i, j, k) - qcd(i, j, k)) / dt;
+    });
+    fill(t2, g);
+    fill(hdia, g);
+    for (int n = 0; n < n_moist; ++n) fill(q2[size_t(n)], g);
+    // solve_em after the microphysics: calc_p_rho_phi, without the spec zone with specified boundaries
+    // (rk3.cpp:151-158)
+    if (spec) calc_p_rho_phi_range(kSpecZone, nx - kSpecZone, kSpecZone, ny - kSpecZone);
+    else calc_p_rho_phi();
+  }
   void step_end() {
     if (has_phys && phys.after_rk) phys.after_rk(phys.user);  // phy_prep_part2 (rk3.cpp:150)
+    if (mp) microphysics();
     step_end_dyn();
     if (has_phys && phys.after_all_rk_steps) phys.after_all_rk_steps(phys.user);  // rk3.cpp:166
   }
@@ -2079,6 +2232,8 @@ struct Dom {
       case MI_HT: return &ht;
       case MI_F: return &fcor; case MI_E: return &ecor; case MI_SINA: return &sina; case MI_COSA: return &cosa;
       case MI_CQU: return &cqu; case MI_CQV: return &cqv; case MI_CQW: return &cqw; case MI_H_DIABATIC: return &hdia;
+      case MI_RAINNC: return mp ? &rainnc : nullptr;
+      case MI_RAINNCV: return mp ? &rainncv : nullptr;
       case MI_RU_TENDF: return use_tendf ? &ru_tendf : nullptr;
       case MI_RV_TENDF: return use_tendf ? &rv_tendf : nullptr;
       case MI_RW_TENDF: return use_tendf ? &rw_tendf : nullptr;
@@ -2133,6 +2288,8 @@ int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv,
     if (p->diff_6th_opt < 0 || p->diff_6th_opt > 2) return fail("diff_6th_opt must be 0, 1 or 2");
     if (p->diff_opt < 0 || p->diff_opt > 1) return fail("oracle: diff_opt 0 or 1 only");
     if (p->diff_opt == 1 && p->km_opt != 1) return fail("oracle: km_opt 1 only");
+    if (p->mp_physics != 0 && p->mp_physics != 1) return fail("oracle: mp_physics 0 or 1 (Kessler) only");
+    if (p->mp_physics == 1 && (p->n_moist < 3 || g->nz > 128)) return fail("Kessler needs qv, qc, qr and nz <= 128");
     auto* o = new orc_domain;
     o->d = std::make_unique<Dom>(*g, *p, *lv);
     if (o->d->n_sound < 2 || o->d->n_sound % 2) {

@@ -11,7 +11,7 @@ from cppwrf_amd.dyn import Case, Context, RK3
 
 pytestmark = pytest.mark.gpu
 
-K_MARCH = 12  # the substep kernel's k-march shape (kColMarch)
+K_MARCH = 13  # the substep kernel's k-march shape (kColMarch)
 VARS = {"U": A.MI_U, "V": A.MI_V, "W": A.MI_W, "T": A.MI_T, "PH": A.MI_PH, "MU": A.MI_MU}
 
 

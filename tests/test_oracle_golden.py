@@ -1,8 +1,11 @@
 """Pins the CPU oracle (and the case generator feeding it) to the reference's own FP64 golden files:
-tools/regression/golden/{igw,warm_bubble,rest_state,straka}_cpu_fp64/snap_000002.{bin,json}, two steps of
-each case from the reference's CPU FP64 build (copied to tests/golden/reference/).  straka runs the
+tools/regression/golden/{igw,warm_bubble,rest_state,straka,squall2d}_cpu_fp64/snap_000002.{bin,json}, two steps
+of each case from the reference's CPU FP64 build (copied to tests/golden/reference/).  straka runs the
 second-order diffusion of diff_opt 1 with constant K (configs/straka.ini: khdif = kvdif = 75 m2/s,
-pr_inv = 1).
+pr_inv = 1); squall2d the moist dynamics with qv, qc, qr and Kessler microphysics after the RK loop
+(configs/squall2d.ini): in its first two steps the air stays subsaturated, so the golden pins the coupling (the
+moist initial state, moist_physics_prep / finish around the scheme, calc_p_rho_phi after it), not the scheme's
+condensation and rain (test_oracle_microphysics.py).
 
 The reference's default build evaluates calc_p_rho_phi's pow with glibc libmvec's 2-lane vector
 function; with that option on, the oracle must equal the goldens in every bit.  The scalar-pow oracle
@@ -37,19 +40,32 @@ def read_snap(name):
     return out
 
 
-CASES = {"igw": Case.igw, "warm_bubble": Case.warm_bubble, "rest_state": Case.rest_state, "straka": Case.straka}
+CASES = {"igw": Case.igw, "warm_bubble": Case.warm_bubble, "rest_state": Case.rest_state, "straka": Case.straka,
+         "squall2d": Case.squall2d}
 # the run options of each golden beyond the reference's defaults (configs/*.ini)
-OPTIONS = {"straka": dict(diff_opt=1, km_opt=1, khdif=75.0, kvdif=75.0, pr_inv=1.0)}
+OPTIONS = {"straka": dict(diff_opt=1, km_opt=1, khdif=75.0, kvdif=75.0, pr_inv=1.0),
+           "squall2d": dict(n_moist=3, n_scalars=3, mp_physics=1)}
+RVOVRD = 461.6 / 287.0
 
 
-def snapshot(case, orc):
-    """the reference's snapshot variables (src/dyn/diagnostics.cpp:110-130) from the oracle state"""
+def snapshot(case, orc, moist=False):
+    """the reference's snapshot variables (src/dyn/diagnostics.cpp:110-140) from the oracle state; moist: t holds
+    theta_m - t0, theta is the dry theta, then the species qv, qc, qr and the precipitation"""
     pb, phb, t_init, mub = (case.field(f) for f in (A.MI_PB, A.MI_PHB, A.MI_T_INIT, A.MI_MUB))
     t, p, ph, mu = (orc.field(f) for f in (A.MI_T, A.MI_P, A.MI_PH, A.MI_MU))
     theta = t + 300.0
-    return {"u": orc.field(A.MI_U), "v": orc.field(A.MI_V), "w": orc.field(A.MI_W), "th_pert": theta - 300.0 - t_init,
-            "theta": theta, "p": p + pb, "p_pert": p, "ph": ph + phb, "mu": mu + mub, "mu_pert": mu,
-            "z_m": 0.5 * (ph[:-1] + ph[1:] + phb[:-1] + phb[1:]) / 9.81}
+    out = {"u": orc.field(A.MI_U), "v": orc.field(A.MI_V), "w": orc.field(A.MI_W)}
+    if moist:
+        theta = (t + 300.0) / (1.0 + RVOVRD * orc.field(A.MI_Q))
+    out.update({"th_pert": theta - 300.0 - t_init, "theta": theta, "p": p + pb, "p_pert": p, "ph": ph + phb,
+                "mu": mu + mub, "mu_pert": mu, "z_m": 0.5 * (ph[:-1] + ph[1:] + phb[:-1] + phb[1:]) / 9.81})
+    if moist:
+        out["theta_m"] = t + 300.0
+        for n, name in enumerate(("qv", "qc", "qr")):
+            out[name] = orc.field(A.MI_Q + n)
+        out["rainnc"] = orc.field(A.MI_RAINNC)
+        out["rainncv"] = orc.field(A.MI_RAINNCV)
+    return out
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -58,7 +74,7 @@ def test_oracle_equals_reference_goldens_bitwise(name):
     case = CASES[name]()
     orc = OracleDomain(case, case.params(**OPTIONS.get(name, {})), libmvec=True)
     orc.step(2)
-    mine = snapshot(case, orc)
+    mine = snapshot(case, orc, name == "squall2d")
     assert set(mine) == set(gold)
     for k, a in mine.items():
         assert bitwise_equal(a, gold[k]), f"{name}.{k}: {first_diff(a, gold[k])}"
@@ -72,7 +88,7 @@ def test_scalar_pow_oracle_close_to_goldens(name):
     case = CASES[name]()
     orc = OracleDomain(case, case.params(**OPTIONS.get(name, {})), libmvec=False)
     orc.step(2)
-    mine = snapshot(case, orc)
+    mine = snapshot(case, orc, name == "squall2d")
     for k in ("u", "w", "theta", "p", "ph", "mu"):
         a, b = mine[k], gold[k]
         scale = max(np.abs(b).max(), 1.0)
