@@ -31,10 +31,10 @@ MI_MSFTX, MI_MSFTY, MI_MSFUX, MI_MSFUY, MI_MSFVX, MI_MSFVY, MI_MSFVX_INV = range
 MI_CQU, MI_CQV, MI_CQW, MI_H_DIABATIC = range(20, 24)
 MI_RU_TENDF, MI_RV_TENDF, MI_RW_TENDF, MI_T_TENDF, MI_TH_FORCING, MI_QV_FORCING = range(24, 30)
 MI_F, MI_E, MI_SINA, MI_COSA, MI_HT = range(47, 52)
-MI_U_BASE, MI_V_BASE, MI_QV_BASE, MI_RAINNC, MI_RAINNCV = range(52, 57)
+MI_U_BASE, MI_V_BASE, MI_QV_BASE, MI_RAINNC, MI_RAINNCV, MI_XKMH, MI_XKHH = range(52, 59)
 MI_Q, MI_Q_TENDF = 64, 96
 MI_MAX_MOIST, MI_MAX_SCALARS = 8, 16
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 MI_CASE_REST_STATE, MI_CASE_WARM_BUBBLE, MI_CASE_IGW, MI_CASE_BUBBLE3D, MI_CASE_HILLS, MI_CASE_STRAKA, MI_CASE_SQUALL2D = range(7)
 
@@ -63,7 +63,7 @@ STAGGER = {
     MI_RU_TENDF: "U", MI_RV_TENDF: "V", MI_RW_TENDF: "W", MI_T_TENDF: "M", MI_TH_FORCING: "M", MI_QV_FORCING: "M",
     MI_MSFTX: "M2", MI_MSFTY: "M2", MI_MSFUX: "U2", MI_MSFUY: "U2", MI_MSFVX: "V2", MI_MSFVY: "V2",
     MI_MSFVX_INV: "V2", MI_F: "M2", MI_E: "M2", MI_SINA: "M2", MI_COSA: "M2", MI_HT: "M2",
-    MI_U_BASE: "Z", MI_V_BASE: "Z", MI_QV_BASE: "Z", MI_RAINNC: "M2", MI_RAINNCV: "M2",
+    MI_U_BASE: "Z", MI_V_BASE: "Z", MI_QV_BASE: "Z", MI_RAINNC: "M2", MI_RAINNCV: "M2", MI_XKMH: "M", MI_XKHH: "M",
 }
 for _n in range(MI_MAX_SCALARS):
     STAGGER[MI_Q + _n] = "M"
@@ -76,7 +76,7 @@ FIELD_NAMES = {
     MI_T_TEND: "t_tend", MI_PH_TEND: "ph_tend", MI_CQU: "cqu", MI_CQV: "cqv", MI_CQW: "cqw",
     MI_H_DIABATIC: "h_diabatic", MI_RU_TENDF: "ru_tendf", MI_RV_TENDF: "rv_tendf", MI_RW_TENDF: "rw_tendf",
     MI_T_TENDF: "t_tendf", MI_TH_FORCING: "rthften", MI_QV_FORCING: "rqvften", MI_RAINNC: "rainnc",
-    MI_RAINNCV: "rainncv", MI_H_DIABATIC: "h_diabatic",
+    MI_RAINNCV: "rainncv", MI_H_DIABATIC: "h_diabatic", MI_XKMH: "xkmh", MI_XKHH: "xkhh",
 }
 for _n in range(MI_MAX_SCALARS):
     FIELD_NAMES[MI_Q + _n] = f"q{_n}"
@@ -115,7 +115,7 @@ class MiParams(C.Structure):
                 ("mapped", C.c_int32), ("coriolis", C.c_int32), ("diff_6th_opt", C.c_int32), ("terrain", C.c_int32),
                 ("diff_6th_factor", C.c_double), ("diff_opt", C.c_int32), ("km_opt", C.c_int32),
                 ("khdif", C.c_double), ("kvdif", C.c_double), ("pr_inv", C.c_double), ("mp_physics", C.c_int32),
-                ("mp_tend_lim", C.c_double)]
+                ("mp_tend_lim", C.c_double), ("c_s", C.c_double)]
 
 
 _PD = C.POINTER(C.c_double)
@@ -142,7 +142,7 @@ def default_params(**kw) -> MiParams:
                  emdiv=0.01, damp_opt=0, hypsometric_opt=1, zdamp=5000.0, dampcoef=0.2, time_step_sound=0,
                  precision=MI_FP64, n_moist=0, n_scalars=0, moist_adv_opt=1, scalar_adv_opt=1, mapped=0,
                  coriolis=0, diff_6th_opt=0, terrain=0, diff_6th_factor=0.12, diff_opt=0, km_opt=1, khdif=0.0,
-                 kvdif=0.0, pr_inv=3.0, mp_physics=0, mp_tend_lim=10.0)
+                 kvdif=0.0, pr_inv=3.0, mp_physics=0, mp_tend_lim=10.0, c_s=0.25)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

@@ -246,6 +246,10 @@ struct Domain final : mi_domain {
   // tendencies first (large_step.cpp:820-821) and the diffusion adds to them.
   void rk_tendency(int rk, bool hooks = true) {
     if (spec) launch_fill2d(a, a.f.mu_tend - off, n2, 0.0, stream());
+    if (rk == 1 && a.q.km4) {  // DiffusionDriver::prepare (rk3.cpp:113): the step's K, then its halos
+      launch_smag2d(a, stream());
+      halo({{a.f.xkmh, -1, -1}, {a.f.xkhh, -1, -1}});
+    }
     if (rk == 1 && a.q.use_tendf) {
       if (phys_on(hooks)) {
         call_phys(phys.part2);
@@ -822,10 +826,12 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   if (p.terrain != 0 && p.terrain != 1) throw Error(MI_EINVAL, "terrain must be 0 or 1");
   if (p.diff_opt == 2) throw Error(MI_EINVAL, "diff_opt 2 (diffusion in physical space) is not built");
   if (p.diff_opt < 0 || p.diff_opt > 1) throw Error(MI_EINVAL, "diff_opt must be 0 or 1");
-  if (p.diff_opt == 1 && p.km_opt != 1)
-    throw Error(MI_EINVAL, "km_opt must be 1 (constant K); the TKE and Smagorinsky options 2-4 are not built");
-  if (p.diff_opt == 1 && !(p.khdif >= 0 && p.kvdif >= 0 && p.pr_inv >= 0))
-    throw Error(MI_EINVAL, "khdif, kvdif and pr_inv must be >= 0");
+  if (p.diff_opt == 1 && p.km_opt != 1 && p.km_opt != 4)
+    throw Error(MI_EINVAL, "km_opt must be 1 (constant K) or 4 (2-D Smagorinsky); the TKE and 3-D Smagorinsky "
+                           "options 2 and 3 are not built");
+  if (p.diff_opt == 1 && !(p.khdif >= 0 && p.kvdif >= 0 && p.pr_inv >= 0 && p.c_s >= 0))
+    throw Error(MI_EINVAL, "khdif, kvdif, pr_inv and c_s must be >= 0");
+  if (p.diff_opt == 1 && p.km_opt == 4 && g.nz < 3) throw Error(MI_EINVAL, "km_opt 4 needs nz >= 3");
   if (p.mp_physics != 0 && p.mp_physics != 1)
     throw Error(MI_EINVAL, "mp_physics must be 0 or 1 (Kessler); the reference's ice schemes are not built");
   if (p.mp_physics == 1 && (p.n_moist < 3 || g.nz > kKesslerMaxLevels || g.nz < 2))
@@ -962,6 +968,10 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     add3(f.dz8w, St::M, -1); add3(f.qvd, St::M, -1); add3(f.qcd, St::M, -1);
     add2(f.rainnc, St::M2, MI_RAINNC); add2(f.rainncv, St::M2, MI_RAINNCV);
   }
+  if (p.diff_opt == 1 && p.km_opt == 4) {  // the step's eddy viscosities (0 where not set) and defor12
+    add3(f.xkmh, St::M, MI_XKMH); add3(f.xkhh, St::M, MI_XKHH);
+    add3(f.d12, St::U, -1);
+  }
   f.u_base = d.alloc(nz + 1);
   f.v_base = d.alloc(nz + 1);
   f.qv_base = d.alloc(nz + 1);
@@ -1014,12 +1024,17 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   q.mp = p.mp_physics;
   q.mp_lim = p.mp_tend_lim > 0 ? p.mp_tend_lim : 10.0;
   q.diff_opt = p.diff_opt;
-  if (p.diff_opt == 1) {  // DiffusionDriver's constants (model.cpp:314, km_kh.hpp:35-45)
+  if (p.diff_opt == 1) {  // DiffusionDriver's constants (model.cpp:309-314, km_kh.hpp:35-45)
     const double prandtl = 1.0 / (p.pr_inv > 0 ? p.pr_inv : 3.0);
     q.xkmh = p.khdif;
     q.xkhh = p.khdif / prandtl;
     q.kvdif = p.kvdif;
     q.kvdq = p.kvdif / prandtl;
+    q.km4 = p.km_opt == 4 ? 1 : 0;
+    q.prandtl = prandtl;
+    q.c_s = p.c_s > 0 ? p.c_s : 0.25;
+    q.dx = g.dx;
+    q.dy = g.dy;
   }
   set_dt_params(q, g.dt, p.w_damping == 1, d.spec);
   q.terrain = p.terrain;

@@ -67,7 +67,8 @@ struct ScalarParams {
 // What the reference's BaseState and drivers carry beside the parameter structs: the number of moist
 // species and tracers in State::q_2 (state.hpp:53-60), BaseState::mapped / coriolis (state.hpp:93-107),
 // a terrain-following base state, the diffusion of DiffusionDriver (its DiffusionConfig's diff_opt 1 with
-// km_opt 1, khdif, kvdif and 1 / prandtl, diffusion_driver.hpp:23-51; and diff_6th_opt, diffusion/sixth_order.hpp)
+// km_opt 1 or 4, khdif, kvdif, c_s and 1 / prandtl, diffusion_driver.hpp:23-51; and diff_6th_opt,
+// diffusion/sixth_order.hpp)
 // and the storage precision of the bulk 3-D fields.
 struct DomainOptions {
   int n_moist = 0, n_scalars = 0;
@@ -76,7 +77,7 @@ struct DomainOptions {
   double diff_6th_factor = 0.12;
   int precision = MI_FP64;
   int diff_opt = 0, km_opt = 1;
-  double khdif = 0, kvdif = 0, pr_inv = 3.0;
+  double khdif = 0, kvdif = 0, pr_inv = 3.0, c_s = 0.25;
   // RK3::enable_physics(PhysicsParams) (rk3.hpp:34-78, physics_coupling.hpp:23-37): mp_physics 1 (Kessler) after the
   // RK loop, with mp_tend_lim
   int mp_physics = 0;
@@ -95,6 +96,7 @@ inline size_t field_size(int field, const mi_grid& g) {
     case MI_MU: case MI_MUTS: case MI_MUB: case MI_MSFTX: case MI_MSFTY: case MI_F: case MI_E: case MI_SINA:
     case MI_COSA: case MI_HT: case MI_RAINNC: case MI_RAINNCV:
       return nx * ny;
+    case MI_XKMH: case MI_XKHH: return nx * ny * nz;
     case MI_MSFUX: case MI_MSFUY: return (nx + 1) * ny;
     case MI_U_BASE: case MI_V_BASE: case MI_QV_BASE: return nz;
     case MI_MSFVX: case MI_MSFVY: case MI_MSFVX_INV: return nx * (ny + vy);
@@ -337,6 +339,7 @@ class RK3 {
     p.pr_inv = opt.pr_inv;
     p.mp_physics = opt.mp_physics;
     p.mp_tend_lim = opt.mp_tend_lim;
+    p.c_s = opt.c_s;
     check(mi_domain_create(ctx.get(), &g, &p, &levels, &d_));
   }
   ~RK3() { mi_domain_destroy(d_); }

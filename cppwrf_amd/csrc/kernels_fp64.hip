@@ -2478,12 +2478,27 @@ __device__ __forceinline__ bool sixth_in(const Geo& g, int diff, char kind, int 
   return i >= is && i < ie && j >= js && j < je;
 }
 
-// ----------------------------------------------------------------------------- diff_opt 1 (constant K)
-// The second-order diffusion along eta of DiffusionDriver with diff_opt 1, km_opt 1 (reference
-// src/dyn/diffusion/big_step_diffusion.hpp; WRF module_big_step_utilities_em.F): every eddy viscosity is a constant
-// (xkmh = khdif, xkhh = khdif / prandtl), each face average of them is evaluated as the reference's sum of equal
-// terms so the rounding is the same.  In an x-z slice the y terms are left out: rdy = 0 makes them zeros that
-// leave the sum's bits unchanged (they are added to a tendency that is never -0).
+// ----------------------------------------------------------------------------- diff_opt 1
+// The second-order diffusion along eta of DiffusionDriver with diff_opt 1 (reference
+// src/dyn/diffusion/big_step_diffusion.hpp; WRF module_big_step_utilities_em.F): the horizontal eddy viscosities
+// xkmh (momenta) and xkhh (theta, scalars) at mass points are averaged onto each face as the reference, term by
+// term.  km_opt 1 makes them constants (xkmh = khdif, xkhh = khdif / prandtl) whose sums round as the reference's
+// sums of equal terms; km_opt 4 reads the fields the step's first stage sets (k_smag2d).  In an x-z slice the y
+// terms are left out: rdy = 0 makes them zeros that leave the sum's bits unchanged (they are added to a tendency
+// that is never -0).
+
+// the eddy viscosity at mass point offset (di, dj, dk) from index x: a field (km_opt 4) or the constant (km_opt 1)
+struct EddyK {
+  const real* p;
+  double c;
+  idx_t x, sj, sk;
+  __device__ double operator()(int di, int dj, int dk = 0) const {
+    return p ? double(p[x + di + dj * sj + dk * sk]) : c;
+  }
+};
+__device__ __forceinline__ EddyK eddy_k(const Args& a, const real* p, double c, idx_t x) {
+  return EddyK{a.q.km4 ? p : nullptr, c, x, a.g.sj, a.g.sk};
+}
 
 // the loop limits of horizontal_diffusion (big_step_diffusion.hpp:343-414): one row in from open and specified
 // sides, a periodic x resetting the x limits
@@ -2508,57 +2523,58 @@ __device__ __forceinline__ bool hdiff_in(const Geo& g, char kind, int i, int j) 
 }
 
 // horizontal_diffusion (KHorizDiffU1 / V1 / W1 / M1, big_step_diffusion.hpp:26-186): tend + the flux differences
-// of field q at index x (2-D index x2) with the constant K; mu the column mass (mut, or muts for the scalars), c1 /
-// c2 the level's mass coefficients (c1f / c2f for 'w')
+// of field q at index x (2-D index x2) with the eddy viscosity K; mu the column mass (mut, or muts for the
+// scalars), c1 / c2 the level's mass coefficients (c1f / c2f for 'w')
 template <char KIND>
 __device__ __forceinline__ double hdiff_at(double tend, const Args& a, const real* q, const double* mu, idx_t x,
-                                           idx_t x2, double c1, double c2, double K) {
+                                           idx_t x2, double c1, double c2, const EddyK K) {
   const Fields& f = a.f;
   const idx_t sj = a.g.sj;
   const double rdx = a.c.rdx, rdy = a.c.rdy;
   const bool do_y = !a.g.two_d;
   auto m = [&](int di, int dj) { return c1 * mu[x2 + di + dj * sj] + c2; };
   auto M = [&](const double* p, int di, int dj) { return p[x2 + di + dj * sj]; };
-  const double K4 = K + K + K + K, K2 = K + K;
   double mkrdxm, mkrdxp, mrdx, mkrdym = 0.0, mkrdyp = 0.0, mrdy = 0.0;
   if (KIND == 'u') {
-    mkrdxm = (M(f.msftx, -1, 0) / M(f.msfty, -1, 0)) * m(-1, 0) * K * rdx;
-    mkrdxp = (M(f.msftx, 0, 0) / M(f.msfty, 0, 0)) * m(0, 0) * K * rdx;
+    mkrdxm = (M(f.msftx, -1, 0) / M(f.msfty, -1, 0)) * m(-1, 0) * K(-1, 0) * rdx;
+    mkrdxp = (M(f.msftx, 0, 0) / M(f.msfty, 0, 0)) * m(0, 0) * K(0, 0) * rdx;
     mrdx = M(f.msfux, 0, 0) * M(f.msfuy, 0, 0) * rdx;
     if (do_y) {
       mkrdym = ((M(f.msfuy, 0, 0) + M(f.msfuy, 0, -1)) / (M(f.msfux, 0, 0) + M(f.msfux, 0, -1))) * 0.25 *
-               (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * K4 * rdy;
+               (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * (K(0, 0) + K(0, -1) + K(-1, -1) + K(-1, 0)) * rdy;
       mkrdyp = ((M(f.msfuy, 0, 0) + M(f.msfuy, 0, 1)) / (M(f.msfux, 0, 0) + M(f.msfux, 0, 1))) * 0.25 *
-               (m(0, 0) + m(0, 1) + m(-1, 1) + m(-1, 0)) * 0.25 * K4 * rdy;
+               (m(0, 0) + m(0, 1) + m(-1, 1) + m(-1, 0)) * 0.25 * (K(0, 0) + K(0, 1) + K(-1, 1) + K(-1, 0)) * rdy;
       mrdy = M(f.msfux, 0, 0) * M(f.msfuy, 0, 0) * rdy;
     }
   } else if (KIND == 'v') {  // no mass factor in y (WRF v4.8.0 lines 2854-2856)
     mkrdxm = ((M(f.msfvx, 0, 0) + M(f.msfvx, -1, 0)) / (M(f.msfvy, 0, 0) + M(f.msfvy, -1, 0))) * 0.25 *
-             (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * K4 * rdx;
+             (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * (K(0, 0) + K(0, -1) + K(-1, -1) + K(-1, 0)) * rdx;
     mkrdxp = ((M(f.msfvx, 0, 0) + M(f.msfvx, 1, 0)) / (M(f.msfvy, 0, 0) + M(f.msfvy, 1, 0))) * 0.25 *
-             (m(0, 0) + m(0, -1) + m(1, -1) + m(1, 0)) * 0.25 * K4 * rdx;
+             (m(0, 0) + m(0, -1) + m(1, -1) + m(1, 0)) * 0.25 * (K(0, 0) + K(0, -1) + K(1, -1) + K(1, 0)) * rdx;
     mrdx = M(f.msfvx, 0, 0) * M(f.msfvy, 0, 0) * rdx;
-    mkrdym = (M(f.msfty, 0, -1) / M(f.msftx, 0, -1)) * K * rdy;
-    mkrdyp = (M(f.msfty, 0, 0) / M(f.msftx, 0, 0)) * K * rdy;
+    mkrdym = (M(f.msfty, 0, -1) / M(f.msftx, 0, -1)) * K(0, -1) * rdy;
+    mkrdyp = (M(f.msfty, 0, 0) / M(f.msftx, 0, 0)) * K(0, 0) * rdy;
     mrdy = M(f.msfvx, 0, 0) * M(f.msfvy, 0, 0) * rdy;
-  } else if (KIND == 'w') {
-    mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.25 * (m(0, 0) + m(-1, 0) + m(0, 0) + m(-1, 0)) * 0.25 * K4 * rdx;
-    mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.25 * (m(1, 0) + m(0, 0) + m(1, 0) + m(0, 0)) * 0.25 * K4 * rdx;
+  } else if (KIND == 'w') {  // K averaged over the mass levels k and k - 1
+    mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.25 * (m(0, 0) + m(-1, 0) + m(0, 0) + m(-1, 0)) * 0.25 *
+             (K(0, 0) + K(-1, 0) + K(0, 0, -1) + K(-1, 0, -1)) * rdx;
+    mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.25 * (m(1, 0) + m(0, 0) + m(1, 0) + m(0, 0)) * 0.25 *
+             (K(1, 0) + K(0, 0) + K(1, 0, -1) + K(0, 0, -1)) * rdx;
     mrdx = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdx;
     if (do_y) {
       mkrdym = (M(f.msfvy, 0, 0) * M(f.msfvx_inv, 0, 0)) * 0.25 * (m(0, 0) + m(0, -1) + m(0, 0) + m(0, -1)) * 0.25 *
-               K4 * rdy;
+               (K(0, 0) + K(0, -1) + K(0, 0, -1) + K(0, -1, -1)) * rdy;
       mkrdyp = (M(f.msfvy, 0, 1) * M(f.msfvx_inv, 0, 1)) * 0.25 * (m(0, 1) + m(0, 0) + m(0, 1) + m(0, 0)) * 0.25 *
-               K4 * rdy;
+               (K(0, 1) + K(0, 0) + K(0, 1, -1) + K(0, 0, -1)) * rdy;
       mrdy = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdy;
     }
   } else {
-    mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
-    mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.5 * K2 * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
+    mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.5 * (K(0, 0) + K(-1, 0)) * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
+    mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.5 * (K(1, 0) + K(0, 0)) * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
     mrdx = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdx;
     if (do_y) {
-      mkrdym = (M(f.msfvy, 0, 0) * M(f.msfvx_inv, 0, 0)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
-      mkrdyp = (M(f.msfvy, 0, 1) * M(f.msfvx_inv, 0, 1)) * 0.5 * K2 * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
+      mkrdym = (M(f.msfvy, 0, 0) * M(f.msfvx_inv, 0, 0)) * 0.5 * (K(0, 0) + K(0, -1)) * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
+      mkrdyp = (M(f.msfvy, 0, 1) * M(f.msfvx_inv, 0, 1)) * 0.5 * (K(0, 1) + K(0, 0)) * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
       mrdy = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdy;
     }
   }
@@ -2568,26 +2584,28 @@ __device__ __forceinline__ double hdiff_at(double tend, const Args& a, const rea
   return tend + (tx + mrdy * (mkrdyp * (double(q[x + sj]) - q0) - mkrdym * (q0 - double(q[x - sj]))));
 }
 
-// horizontal_diffusion_3dmp (KHorizDiff3DMP, big_step_diffusion.hpp:188-219): theta minus t_init, the differences
-// evaluated left to right as WRF
+// horizontal_diffusion_3dmp (KHorizDiff3DMP, big_step_diffusion.hpp:188-219): theta minus t_init with xkhh, the
+// differences evaluated left to right as WRF
 __device__ __forceinline__ double hdiff_3dmp_at(double tend, const Args& a, int i, int j, int k, idx_t x, idx_t x2) {
   const Fields& f = a.f;
   const idx_t sj = a.g.sj;
-  const double rdx = a.c.rdx, rdy = a.c.rdy, K = a.q.xkhh, c1 = a.c.c1h[k], c2 = a.c.c2h[k];
+  const double rdx = a.c.rdx, rdy = a.c.rdy, c1 = a.c.c1h[k], c2 = a.c.c2h[k];
+  const EddyK K = eddy_k(a, f.xkhh, a.q.xkhh, x);
   auto m = [&](int di, int dj) { return c1 * f.mut[x2 + di + dj * sj] + c2; };
   auto M = [&](const double* p, int di, int dj) { return p[x2 + di + dj * sj]; };
   auto Q = [&](int di, int dj) { return double(f.t2[x + di + dj * sj]); };
   auto B = [&](int di, int dj) { return f.t_init(i + di, j + dj, k); };
-  const double K2 = K + K;
-  const double mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
-  const double mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.5 * K2 * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
+  const double mkrdxm = (M(f.msfux, 0, 0) / M(f.msfuy, 0, 0)) * 0.5 * (K(0, 0) + K(-1, 0)) * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
+  const double mkrdxp = (M(f.msfux, 1, 0) / M(f.msfuy, 1, 0)) * 0.5 * (K(1, 0) + K(0, 0)) * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
   const double mrdx = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdx;
   const double dxp = Q(1, 0) - Q(0, 0) - B(1, 0) + B(0, 0);
   const double dxm = Q(0, 0) - Q(-1, 0) - B(0, 0) + B(-1, 0);
   const double tx = mrdx * (mkrdxp * dxp - mkrdxm * dxm);
   if (a.g.two_d) return tend + tx;
-  const double mkrdym = (M(f.msfvy, 0, 0) * M(f.msfvx_inv, 0, 0)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
-  const double mkrdyp = (M(f.msfvy, 0, 1) * M(f.msfvx_inv, 0, 1)) * 0.5 * K2 * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
+  const double mkrdym =
+      (M(f.msfvy, 0, 0) * M(f.msfvx_inv, 0, 0)) * 0.5 * (K(0, 0) + K(0, -1)) * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
+  const double mkrdyp =
+      (M(f.msfvy, 0, 1) * M(f.msfvx_inv, 0, 1)) * 0.5 * (K(0, 1) + K(0, 0)) * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
   const double mrdy = M(f.msftx, 0, 0) * M(f.msfty, 0, 0) * rdy;
   const double dyp = Q(0, 1) - Q(0, 0) - B(0, 1) + B(0, 0);
   const double dym = Q(0, 0) - Q(0, -1) - B(0, 0) + B(0, -1);
@@ -2666,6 +2684,138 @@ __device__ __forceinline__ double vdiff_m(double tend, const Args& a, const real
   return tend + kG * kG / mm / al * c.rdnw[k] * (vf(k) - vf(k - 1));
 }
 
+// ----------------------------------------------------------------------------- km_opt 4 (2-D Smagorinsky)
+// DiffusionDriver::prepare with diff_opt 1, km_opt 4 at the first RK stage (reference diffusion_driver.cpp:92-138):
+// the coordinate slopes zx, zy and rdzw of compute_diff_metrics (diff_metrics.hpp:20-138, with the end faces of
+// set_physical_bc3d), the horizontal deformations of cal_deform_and_div (deform_div.hpp:20-170, 234-311) and the
+// eddy viscosities of KSmag2dKm (km_kh.hpp:47-102, 586-629).  The metrics are evaluated where they are read, from
+// ph and phb with the reference's expressions; defor12 is stored at the vorticity points by one launch, defor11
+// and defor22 are evaluated by the launch that sets xkmh and xkhh from it.  Every value is the reference's, operand
+// for operand, in FP64 arithmetic.
+
+// zx at u point (i, j), w level k in 0..nz: 0 on the boundary faces of a non-periodic x, except the end face of a
+// specified or open one (the last inner face copied); a periodic x leaves the top level 0 on its boundary faces
+__device__ __forceinline__ double diff_zx(const Args& a, int i, int j, int k) {
+  const Geo& g = a.g;
+  const int nx = g.nx;
+  const bool px = g.bcx == MI_BC_PERIODIC, cx = g.spec() || g.bcx == MI_BC_OPEN;
+  if (px && k == g.nz && (i == 0 || i == nx)) return 0.0;
+  if (!px && (i == 0 || (i == nx && !cx))) return 0.0;
+  const int i0 = (i == nx) ? (px ? 0 : nx - 1) : i, im1 = (i == 0) ? nx - 1 : (i == nx && !px ? nx - 2 : i - 1);
+  double v = a.c.rdx * (a.f.phb(i0, j, k) - a.f.phb(im1, j, k)) / kG;
+  v += a.c.rdx * (double(a.f.ph2[g.at(i0, j, k)]) - double(a.f.ph2[g.at(im1, j, k)])) / kG;
+  return v;
+}
+// zy at v point (i, j), w level k: the same rules along y (an x-z slice has no open y and no y differences)
+__device__ __forceinline__ double diff_zy(const Args& a, int i, int j, int k) {
+  const Geo& g = a.g;
+  const int ny = g.ny;
+  const bool py = g.bcy == MI_BC_PERIODIC, cy = g.spec() || g.oy();
+  if (py && k == g.nz && (j == 0 || j == ny)) return 0.0;
+  if (!py && (j == 0 || (j == ny && !cy))) return 0.0;
+  const int j0 = (j == ny) ? (py ? 0 : ny - 1) : j, jm1 = (j == 0) ? ny - 1 : (j == ny && !py ? ny - 2 : j - 1);
+  double v = a.c.rdy * (a.f.phb(i, j0, k) - a.f.phb(i, jm1, k)) / kG;
+  v += a.c.rdy * (double(a.f.ph2[g.at(i, j0, k)]) - double(a.f.ph2[g.at(i, jm1, k)])) / kG;
+  return v;
+}
+// rdzw at mass point (i, j, k): 1 / the layer's thickness from the full geopotential
+__device__ __forceinline__ double diff_rdzw(const Args& a, int i, int j, int k) {
+  const idx_t x = a.g.at(i, j, k);
+  const double z0 = (double(a.f.ph2[x]) + a.f.phb(i, j, k)) / kG;
+  const double z1 = (double(a.f.ph2[x + a.g.sk]) + a.f.phb(i, j, k + 1)) / kG;
+  return 1.0 / (z1 - z0);
+}
+// the average onto w level k (0..nz) of two columns' values h0(kk), h1(kk) at the mass levels (the hatavg of
+// KDeformDiagAndDiv and KDeform12): the surface and top levels extrapolated
+template <class H0, class H1>
+__device__ __forceinline__ double hat_avg(const Lv& c, int nz, int k, const H0& h0, const H1& h1) {
+  if (k == 0) return 0.5 * (c.cf1 * h0(0) + c.cf2 * h0(1) + c.cf3 * h0(2) + c.cf1 * h1(0) + c.cf2 * h1(1) + c.cf3 * h1(2));
+  if (k == nz) {
+    const double cft2 = -0.5 * c.dnw[nz - 1] / c.dn[nz - 1], cft1 = 1.0 - cft2;
+    return 0.5 * (cft1 * (h0(nz - 1) + h1(nz - 1)) + cft2 * (h0(nz - 2) + h1(nz - 2)));
+  }
+  return 0.5 * (c.fnm[k] * (h0(k) + h1(k)) + c.fnp[k] * (h0(k - 1) + h1(k - 1)));
+}
+
+// KDeform12 at vorticity point (i, j), level k (i in 0..nx, j in 0..ny; one row in an x-z slice).  A periodic
+// direction takes the start row's values at its end row (the reference computes and then overwrites them by that
+// copy); on open and specified sides the edge rows are the reference's copies, which KSmag2dKm never reads, and
+// are left as computed here.
+MI_POINT_KERNEL k_deform12(Range r, Args a) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const bool px = g.bcx == MI_BC_PERIODIC, py = g.bcy == MI_BC_PERIODIC;
+  const int ii = (px && i == nx) ? 0 : i, jj = (py && j == ny) ? 0 : j;
+  const int im1 = (ii == 0) ? (px ? nx - 1 : 0) : ii - 1, i0 = (ii == nx) ? nx - 1 : ii;
+  const int jm1 = (jj == 0) ? (py ? ny - 1 : 0) : jj - 1, j0 = (jj == ny) ? ny - 1 : jj;
+  const idx_t sk = g.sk;
+  const double mm = 0.25 * (f.msfux[g.at2(ii, jm1)] + f.msfux[g.at2(ii, j0)]) *
+                    (f.msfvy[g.at2(im1, jj)] + f.msfvy[g.at2(i0, jj)]);
+  const idx_t xum = g.at(ii, jm1, 0), xup = g.at(ii, j0, 0), xvm = g.at(im1, jj, 0), xvp = g.at(i0, jj, 0);
+  const double fum = f.msfux[g.at2(ii, jm1)], fup = f.msfux[g.at2(ii, j0)];
+  const double fvm = f.msfvy[g.at2(im1, jj)], fvp = f.msfvy[g.at2(i0, jj)];
+  auto hum = [&](int kk) { return double(f.u2[xum + kk * sk]) / fum; };
+  auto hup = [&](int kk) { return double(f.u2[xup + kk * sk]) / fup; };
+  auto hvm = [&](int kk) { return double(f.v2[xvm + kk * sk]) / fvm; };
+  auto hvp = [&](int kk) { return double(f.v2[xvp + kk * sk]) / fvp; };
+  const double tzy = 0.25 * (diff_zy(a, im1, jj, k) + diff_zy(a, i0, jj, k) + diff_zy(a, im1, jj, k + 1) +
+                             diff_zy(a, i0, jj, k + 1));
+  const double tu = (hat_avg(c, nz, k + 1, hum, hup) - hat_avg(c, nz, k, hum, hup)) * 0.25 * tzy *
+                    (diff_rdzw(a, i0, j0, k) + diff_rdzw(a, im1, j0, k) + diff_rdzw(a, im1, jm1, k) +
+                     diff_rdzw(a, i0, jm1, k));
+  const double term_u = mm * (c.rdy * (hup(k) - hum(k)) - tu);
+  const double tzx = 0.25 * (diff_zx(a, ii, jm1, k) + diff_zx(a, ii, j0, k) + diff_zx(a, ii, jm1, k + 1) +
+                             diff_zx(a, ii, j0, k + 1));
+  const double tv = (hat_avg(c, nz, k + 1, hvm, hvp) - hat_avg(c, nz, k, hvm, hvp)) * 0.25 * tzx *
+                    (diff_rdzw(a, i0, j0, k) + diff_rdzw(a, i0, jm1, k) + diff_rdzw(a, im1, jm1, k) +
+                     diff_rdzw(a, im1, j0, k));
+  const double term_v = mm * (c.rdx * (hvp(k) - hvm(k)) - tv);
+  f.d12[g.at(i, j, k)] = term_u + term_v;
+}
+
+// KDeformDiagAndDiv's defor11, defor22 at mass point (i, j, k), then KSmag2dKm with diff_opt 1: xkmh = the
+// Smagorinsky K (capped at 10 times the grid length), xkhh = xkmh / prandtl.  Launched one row in from open and
+// specified sides (a periodic x over every column), as calculate_km_kh; the rows outside keep 0.
+MI_POINT_KERNEL k_smag2d(Range r, Args a) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nz = g.nz;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j), sj = g.sj, sk = g.sk;
+  const double mm = f.msftx[x2] * f.msfty[x2];
+  const double fu0 = f.msfuy[x2], fu1 = f.msfuy[x2 + 1], fv0 = f.msfvx[x2], fv1 = f.msfvx[x2 + sj];
+  const idx_t x0 = x - idx_t(k) * sk;
+  auto hu0 = [&](int kk) { return double(f.u2[x0 + kk * sk]) / fu0; };
+  auto hu1 = [&](int kk) { return double(f.u2[x0 + 1 + kk * sk]) / fu1; };
+  auto hv0 = [&](int kk) { return double(f.v2[x0 + kk * sk]) / fv0; };
+  auto hv1 = [&](int kk) { return double(f.v2[x0 + sj + kk * sk]) / fv1; };
+  const double rz = diff_rdzw(a, i, j, k);
+  const double tzx = 0.25 * (diff_zx(a, i, j, k) + diff_zx(a, i + 1, j, k) + diff_zx(a, i, j, k + 1) +
+                             diff_zx(a, i + 1, j, k + 1));
+  const double dudx = mm * (c.rdx * (hu1(k) - hu0(k)) - (hat_avg(c, nz, k + 1, hu0, hu1) - hat_avg(c, nz, k, hu0, hu1)) * tzx * rz);
+  const double d11 = 2.0 * dudx;
+  const int jp = g.two_d ? j : j + 1;  // an x-z slice reads its one row (zero j stride)
+  const double tzy = 0.25 * (diff_zy(a, i, j, k) + diff_zy(a, i, jp, k) + diff_zy(a, i, j, k + 1) +
+                             diff_zy(a, i, jp, k + 1));
+  const double dvdy = mm * (c.rdy * (hv1(k) - hv0(k)) - (hat_avg(c, nz, k + 1, hv0, hv1) - hat_avg(c, nz, k, hv0, hv1)) * tzy * rz);
+  const double d22 = 2.0 * dvdy;
+  const real* D = f.d12;
+  const double dd = d11 - d22;
+  double def2 = 0.25 * (dd * dd);
+  const double d12a = 0.25 * (double(D[x]) + double(D[x + sj]) + double(D[x + 1]) + double(D[x + 1 + sj]));
+  def2 += d12a * d12a;
+  const double mlen = __builtin_sqrt(a.q.dx / f.msftx[x2] * a.q.dy / f.msfty[x2]);
+  const double tmp = __builtin_sqrt(def2);
+  double kmh = a.q.c_s * a.q.c_s * mlen * mlen * tmp;
+  kmh = kmh < 10.0 * mlen ? kmh : 10.0 * mlen;
+  f.xkmh[x] = kmh;
+  f.xkhh[x] = kmh / a.q.prandtl;
+}
+
 // zero_tendf + DiffusionDriver::dynamics (diffusion_driver.cpp:266-281): the held tendencies of u, v, w,
 // theta (and ph with specified boundaries) over the (nx+1) x nyv x (nz+1) points, 0 where
 // sixth_order_diffusion does not reach (w levels 0, nz; the specified edges; everywhere without it)
@@ -2684,7 +2834,7 @@ MI_POINT_KERNEL k_sixth_dyn(Range r, Args a, double coef, int mode) {
   if (j < ny && k < nz) {
     double t = keep ? 0.0 + double(f.ru_tendf[x]) : 0.0;
     if (d2) {
-      if (hdiff_in(g, 'u', i, j)) t = hdiff_at<'u'>(t, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], a.q.xkmh);
+      if (hdiff_in(g, 'u', i, j)) t = hdiff_at<'u'>(t, a, f.u2, f.mut, x, x2, c.c1h[k], c.c2h[k], eddy_k(a, f.xkmh, a.q.xkmh, x));
       // vertical_diffusion_u: all j, i one in from open / specified sides (a periodic x: all faces)
       const bool fx = (g.ox() || g.spec()) && g.bcx != MI_BC_PERIODIC;
       if (i >= (fx ? 1 : 0) && i < (fx ? nx : nx + 1)) t = vdiff_face<'u'>(t, a, x, x2, k);
@@ -2694,7 +2844,7 @@ MI_POINT_KERNEL k_sixth_dyn(Range r, Args a, double coef, int mode) {
   if (!g.two_d && i < nx && j < g.nyv && k < nz) {
     double t = keep ? 0.0 + double(f.rv_tendf[x]) : 0.0;
     if (d2) {
-      if (hdiff_in(g, 'v', i, j)) t = hdiff_at<'v'>(t, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], a.q.xkmh);
+      if (hdiff_in(g, 'v', i, j)) t = hdiff_at<'v'>(t, a, f.v2, f.mut, x, x2, c.c1h[k], c.c2h[k], eddy_k(a, f.xkmh, a.q.xkmh, x));
       const bool fy = g.oy() || g.spec();  // vertical_diffusion_v: all i, j one in from open / specified sides
       if (j >= (fy ? 1 : 0) && j < (fy ? ny : ny + 1)) t = vdiff_face<'v'>(t, a, x, x2, k);
     }
@@ -2704,7 +2854,7 @@ MI_POINT_KERNEL k_sixth_dyn(Range r, Args a, double coef, int mode) {
     const bool in = sixth_in(g, diff, 'm', i, j), hin = d2 && hdiff_in(g, 'm', i, j);
     double w = keep ? 0.0 + double(f.rw_tendf[x]) : 0.0;
     if (d2 && k >= 1 && k < nz) {
-      if (hin) w = hdiff_at<'w'>(w, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], a.q.xkmh);
+      if (hin) w = hdiff_at<'w'>(w, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], eddy_k(a, f.xkmh, a.q.xkmh, x));
       w = vdiff_w(w, a, x, x2, k);
     }
     f.rw_tendf[x] = (in && k >= 1 && k < nz) ? sixth_at<'m'>(w, a, f.w2, f.mut, x, x2, c.c1f[k], c.c2f[k], coef) : w;
@@ -2736,7 +2886,7 @@ MI_POINT_KERNEL k_sixth_sc(Range r, Args a, Scalars s, int n_sc, double coef) {
   for (int n = 0; n < n_sc; ++n) {
     double t = 0.0 + (a.q.qtf_on ? double(s.qtf[n][x]) : 0.0);
     if (d2) {  // horizontal_diffusion 'm' with xkhh, then vertical_diffusion(_mp for qv) with kvdq; mut is muts
-      if (hin) t = hdiff_at<'m'>(t, a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], a.q.xkhh);
+      if (hin) t = hdiff_at<'m'>(t, a, s.q2[n], a.f.muts, x, x2, c.c1h[k], c.c2h[k], eddy_k(a, a.f.xkhh, a.q.xkhh, x));
       if (n == 0 && s.n_moist > 0)
         t = vdiff_m<true>(t, a, s.q2[n], a.f.muts, [&](int kk) { return qvb[kk]; }, a.q.kvdq, x, x2, k);
       else
@@ -3538,6 +3688,18 @@ void launch_sixth_dyn(const Args& a, int mode, hipStream_t st) {
   const Geo& g = a.g;
   const double coef = a.q.diff_6th_factor * 0.015625 / (2.0 * a.q.dt);
   launch3a("sixth", 8, k_sixth_dyn, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, coef, mode);  // u v w t -> 4 tendf
+}
+void launch_smag2d(const Args& a, hipStream_t st) {
+  const Geo& g = a.g;
+  // defor12 over the vorticity points (u v ph -> defor12), then K (u v ph defor12 -> xkmh xkhh)
+  launch3a("deform12", 4, k_deform12, Range{0, g.nx + 1, 0, g.two_d ? 1 : g.ny + 1, 0, g.nz}, st, a);
+  const bool fx = g.ox() || g.spec(), fy = g.oy() || g.spec();
+  int is = fx ? 1 : 0, ie = fx ? g.nx - 1 : g.nx;
+  if (g.bcx == MI_BC_PERIODIC) {
+    is = 0;
+    ie = g.nx;
+  }
+  launch3a("smag2d", 6, k_smag2d, Range{is, ie, fy ? 1 : 0, fy ? g.ny - 1 : g.ny, 0, g.nz}, st, a);
 }
 void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, hipStream_t st) {
   if (n <= 0) return;

@@ -65,6 +65,10 @@ struct Fields {
   // the diffusion's base state: theta's t_init (3-D), the 1-D profiles u_base, v_base, qv_base (nz values)
   BField t_init;
   double *u_base, *v_base, *qv_base;
+  // km_opt 4: the horizontal eddy viscosities of the step at mass points (xkmh of the momenta, xkhh of theta and
+  // the scalars) and the deformation defor12 at the vorticity points ((nx+1) x (ny+1), u-staggered storage whose
+  // row ny lies in the y halo)
+  real *xkmh, *xkhh, *d12;
   // the microphysics after the RK loop: moist_physics_prep's fields (dry theta, density, Exner function, height
   // and thickness of the layer, the qv and qc before the scheme) and the grid-scale precipitation (2-D)
   real *th_phy, *mrho, *pii, *zmid, *dz8w, *qvd, *qcd;
@@ -113,8 +117,10 @@ struct Params {
   int qtf_on;          // a column physics plugin sets the scalars' q_tendf (else zero_tendf leaves it 0)
   int diff_6th_opt;    // 1 linear, 2 monotonic
   double diff_6th_factor;
-  int diff_opt;        // 1: second-order diffusion along eta with constant K (km_opt 1)
+  int diff_opt;        // 1: second-order diffusion along eta
+  int km4;             // km_opt 4: the horizontal K from the 2-D Smagorinsky closure (else the constants below)
   double xkmh, xkhh, kvdif, kvdq;  // its eddy viscosities: khdif, khdif / prandtl, kvdif, kvdif / prandtl
+  double c_s, prandtl, dx, dy;     // km_opt 4's constant, 1 / pr_inv and the grid spacing
   int mp;              // microphysics after the RK loop: 1 Kessler
   double mp_lim;       // the heating-rate limit, K/s
   double dt;           // the large step
@@ -226,6 +232,7 @@ void launch_p_rho_phi(const Args& a, hipStream_t st);
 // the microphysics after the RK loop (moist_physics_prep, Kessler, moist_physics_finish); the halos and the final
 // calc_p_rho_phi are the caller's
 constexpr int kKesslerMaxLevels = 128;
+void launch_smag2d(const Args& a, hipStream_t st);
 void launch_microphysics(const Args& a, const Scalars& s, hipStream_t st);
 // the same on the columns i0 <= i < i1, j0 <= j < j1
 void launch_p_rho_phi_range(const Args& a, int i0, int i1, int j0, int j1, hipStream_t st);

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 4
+#define MI_ABI_VERSION 5
 
 /* status codes */
 #define MI_OK 0
@@ -100,14 +100,15 @@ typedef struct mi_params {
    * and set_w_surface at the end of each step, src/dyn/small_step.cpp:339-353, rk3.cpp:173-200) */
   int32_t terrain;
   double diff_6th_factor;
-  /* second-order diffusion along eta (reference DiffusionDriver with diff_opt 1, km_opt 1: constant eddy
-   * viscosities, src/dyn/diffusion_driver.cpp:159-193, 297-327; src/dyn/diffusion/big_step_diffusion.hpp):
-   * diff_opt 0 off or 1 (2, physical space, is not built); km_opt 1 (constant K; the TKE and Smagorinsky
-   * options 2-4 are not built); khdif, kvdif the horizontal and vertical K (m2/s); pr_inv = 1 / prandtl for
-   * the scalars' K (0 takes the reference's default 3).  Applied at the first RK stage into the held
-   * tendencies of u, v, w, theta (vertical K of theta 3 kvdif, as WRF's rk_tendency) and every scalar,
-   * before the sixth-order diffusion.  The base profiles MI_U_BASE, MI_V_BASE, MI_QV_BASE enter the
-   * vertical diffusion of u, v and qv. */
+  /* second-order diffusion along eta (reference DiffusionDriver with diff_opt 1, src/dyn/diffusion_driver.cpp:92-193,
+   * 297-327; src/dyn/diffusion/big_step_diffusion.hpp): diff_opt 0 off or 1 (2, physical space, is not built);
+   * km_opt 1 (constant K: khdif, kvdif, m2/s) or 4 (the horizontal K from the 2-D Smagorinsky closure of the
+   * step's horizontal deformation, c_s its constant, 0 takes the reference's default 0.25; km_kh.hpp:47-102,
+   * deform_div.hpp:20-170; the vertical K stays kvdif); the TKE and 3-D Smagorinsky options 2 and 3 are not
+   * built.  pr_inv = 1 / prandtl for the scalars' K (0 takes the reference's default 3).  Applied at the first
+   * RK stage into the held tendencies of u, v, w, theta (vertical K of theta 3 kvdif, as WRF's rk_tendency)
+   * and every scalar, before the sixth-order diffusion.  The base profiles MI_U_BASE, MI_V_BASE, MI_QV_BASE
+   * enter the vertical diffusion of u, v and qv. */
   int32_t diff_opt;
   int32_t km_opt;
   double khdif, kvdif, pr_inv;
@@ -118,6 +119,7 @@ typedef struct mi_params {
    * heating kept in MI_H_DIABATIC for the next step) and calc_p_rho_phi. */
   int32_t mp_physics;
   double mp_tend_lim;
+  double c_s; /* the Smagorinsky constant of km_opt 4 (appended in ABI 5) */
 } mi_params;
 
 #define MI_MAX_MOIST 8    /* reference KCq::kMax (src/dyn/large_step.cpp:128) */
@@ -170,6 +172,10 @@ enum {
   MI_U_BASE = 52, MI_V_BASE = 53, MI_QV_BASE = 54,
   /* the microphysics' accumulated and last-step grid-scale precipitation, mm (2-D mass points; with mp_physics) */
   MI_RAINNC = 55, MI_RAINNCV = 56,
+  /* the horizontal eddy viscosities of the step (3-D mass points, read only; with diff_opt 1): xkmh of the
+   * momenta and xkhh = xkmh / prandtl of theta and the scalars, m2/s.  km_opt 4 leaves 0 on the outermost
+   * rows of open and specified sides, as the reference */
+  MI_XKMH = 57, MI_XKHH = 58,
   /* scalar n (0 <= n < n_scalars) at mass points: MI_Q + n is its state (q_2), MI_Q_TENDF + n its held
    * tendency (input, mass-coupled, held over the step as the reference's q_tendf) */
   MI_Q = 64, MI_Q_TENDF = 96

@@ -38,6 +38,7 @@ constexpr double kT0 = 300.0;                 // constants.hpp:13
 constexpr double kRcp = kRd / kCp;            // constants.hpp
 constexpr double kRvovrd = 461.6 / kRd;       // r_v / r_d, constants.hpp
 constexpr double H = 0.5, Q = 0.25;
+constexpr int kMaxLev = 256;                  // levels of a column's work arrays
 constexpr double kReradius = 1. / 6370.0e03;  // large_step.cpp:29
 
 enum class St { M, U, V, W };
@@ -49,11 +50,12 @@ struct F {
   long sj = 0, sk = 0, off = 0;
   bool face_x = false, face_y = false;
   std::vector<double> a;
-  void init(int nx, int ny, int nz, St s, bool rank3) {
+  // extra_k: levels beyond the staggering's own (the diffusion's zx, zy carry nz + 1 at u and v points)
+  void init(int nx, int ny, int nz, St s, bool rank3, int extra_k = 0) {
     const bool two_d = ny == 1;
     nxs = nx + (s == St::U ? 1 : 0);
     nys = ny + ((s == St::V && !two_d) ? 1 : 0);
-    nzs = rank3 ? nz + (s == St::W ? 1 : 0) : 1;
+    nzs = rank3 ? nz + (s == St::W ? 1 : 0) + extra_k : 1;
     hy = two_d ? 0 : 3;
     face_x = s == St::U;
     face_y = s == St::V && !two_d;
@@ -244,10 +246,13 @@ struct Dom {
   F ru_tendf, rv_tendf, rw_tendf, t_tendf, ph_tendf;
   std::vector<F> qtw;  // q_tendf of the step: the caller's tendency plus the sixth-order diffusion
   bool mapped = false, cor = false, use_tendf = false, terrain = false;
-  // the diffusion's 1-D base profiles (BaseState::u_base, v_base, qv_base; state.hpp:100, 112) and its constant
-  // eddy viscosities (KIsotropicKm, km_kh.hpp:35-45: xkmh = khdif, xkmv = kvdif, xkhh = khdif / prandtl)
+  // the diffusion's 1-D base profiles (BaseState::u_base, v_base, qv_base; state.hpp:100, 112) and its horizontal
+  // eddy viscosities at mass points: km_opt 1 the constants of KIsotropicKm (km_kh.hpp:35-45: xkmh = khdif, xkhh =
+  // khdif / prandtl), km_opt 4 those of KSmag2dKm, set each step from the metrics zx, zy, rdzw and the horizontal
+  // deformations defor11, defor22, defor12 (DiffusionDriver::prepare, diffusion_driver.cpp:92-138)
   std::vector<double> u_base, v_base, qv_base;
-  double xkmh = 0, xkhh = 0, kvdif = 0, kvdq = 0;
+  double kvdif = 0, kvdq = 0, prandtl = 1.0 / 3.0, c_s = 0.25;
+  F xkmh, xkhh, zx, zy, rdzw, d11, d22, d12;
   // microphysics after the RK loop (PhysicsCoupling, physics_coupling.hpp:52-94): moist_physics_prep's fields and
   // the precipitation
   int mp = 0;
@@ -325,11 +330,21 @@ struct Dom {
     qv_base.assign(size_t(nz), 0.0);
     if (p.diff_opt == 1) {  // DiffusionDriver's params (model.cpp:314: prandtl = 1 / pr_inv)
       const double pr_inv = p.pr_inv > 0 ? p.pr_inv : 3.0;
-      const double prandtl = 1.0 / pr_inv;
-      xkmh = p.khdif;
-      xkhh = p.khdif / prandtl;
+      prandtl = 1.0 / pr_inv;
       kvdif = p.kvdif;
       kvdq = p.kvdif / prandtl;
+      c_s = p.c_s > 0 ? p.c_s : 0.25;
+      f3(xkmh, St::M);
+      f3(xkhh, St::M);
+      if (p.km_opt == 1) {  // the constants everywhere, halos included
+        std::fill(xkmh.a.begin(), xkmh.a.end(), p.khdif);
+        std::fill(xkhh.a.begin(), xkhh.a.end(), p.khdif / prandtl);
+      } else {
+        zx.init(nx, ny, nz, St::U, true, 1);
+        zy.init(nx, ny, nz, St::V, true, 1);
+        for (F* f : {&rdzw, &d11, &d22}) f3(*f, St::M);
+        f3(d12, St::U);  // (nx+1) x (ny+1): row ny lives in the y halo (diffusion_driver.cpp:30)
+      }
     }
     use_tendf = p.diff_6th_opt != 0 || p.diff_opt != 0 || spec;  // a DiffusionDriver or a LateralBC (large_step.cpp:828)
     mp = p.mp_physics;
@@ -589,10 +604,10 @@ struct Dom {
   }
   // ------------------------------------------------------------------ diff_opt 1 (big_step_diffusion.hpp)
   // horizontal_diffusion (big_step_diffusion.hpp:26-186, 343-389): the second-order fluxes along eta of one field
-  // with the constant K; name 'u', 'v', 'w' or 'm' (mass points); loop limits one row in from open and specified
-  // sides (a periodic x resets the x limits)
+  // with the eddy viscosity K at mass points, averaged onto each face as the reference; name 'u', 'v', 'w' or 'm'
+  // (mass points); loop limits one row in from open and specified sides (a periodic x resets the x limits)
   template <class Fld, class Mu>
-  void hdiff(char name, const Fld& q, F& tend, const Mu& mu, double K, const std::vector<double>& c1,
+  void hdiff(char name, const Fld& q, F& tend, const Mu& mu, const F& K, const std::vector<double>& c1,
              const std::vector<double>& c2) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const bool fx = ox || spec, fy = oy || spec, do_y = !g.two_d();
@@ -605,45 +620,52 @@ struct Dom {
     const double rdx = c.rdx, rdy = c.rdy;
     pfor(is, ie, js, je, k0, nz, [&](int i, int j, int k) {
       auto m = [&](int a, int b) { return c1[k] * mu(i + a, j + b) + c2[k]; };
-      const double K4 = K + K + K + K, K2 = K + K;
+      auto k_ = [&](int a, int b, int dk = 0) { return K(i + a, j + b, k + dk); };
       double mkrdxm, mkrdxp, mrdx, mkrdym = 0, mkrdyp = 0, mrdy = 0;
       if (name == 'u') {  // KHorizDiffU1
-        mkrdxm = (msftx(i - 1, j) / msfty(i - 1, j)) * m(-1, 0) * K * rdx;
-        mkrdxp = (msftx(i, j) / msfty(i, j)) * m(0, 0) * K * rdx;
+        mkrdxm = (msftx(i - 1, j) / msfty(i - 1, j)) * m(-1, 0) * k_(-1, 0) * rdx;
+        mkrdxp = (msftx(i, j) / msfty(i, j)) * m(0, 0) * k_(0, 0) * rdx;
         mrdx = msfux(i, j) * msfuy(i, j) * rdx;
         if (do_y) {
           mkrdym = ((msfuy(i, j) + msfuy(i, j - 1)) / (msfux(i, j) + msfux(i, j - 1))) * 0.25 *
-                   (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * K4 * rdy;
+                   (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 *
+                   (k_(0, 0) + k_(0, -1) + k_(-1, -1) + k_(-1, 0)) * rdy;
           mkrdyp = ((msfuy(i, j) + msfuy(i, j + 1)) / (msfux(i, j) + msfux(i, j + 1))) * 0.25 *
-                   (m(0, 0) + m(0, 1) + m(-1, 1) + m(-1, 0)) * 0.25 * K4 * rdy;
+                   (m(0, 0) + m(0, 1) + m(-1, 1) + m(-1, 0)) * 0.25 * (k_(0, 0) + k_(0, 1) + k_(-1, 1) + k_(-1, 0)) *
+                   rdy;
           mrdy = msfux(i, j) * msfuy(i, j) * rdy;
         }
       } else if (name == 'v') {  // KHorizDiffV1 (no mass factor in y, WRF v4.8.0 lines 2854-2856)
         mkrdxm = ((msfvx(i, j) + msfvx(i - 1, j)) / (msfvy(i, j) + msfvy(i - 1, j))) * 0.25 *
-                 (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * K4 * rdx;
+                 (m(0, 0) + m(0, -1) + m(-1, -1) + m(-1, 0)) * 0.25 * (k_(0, 0) + k_(0, -1) + k_(-1, -1) + k_(-1, 0)) *
+                 rdx;
         mkrdxp = ((msfvx(i, j) + msfvx(i + 1, j)) / (msfvy(i, j) + msfvy(i + 1, j))) * 0.25 *
-                 (m(0, 0) + m(0, -1) + m(1, -1) + m(1, 0)) * 0.25 * K4 * rdx;
+                 (m(0, 0) + m(0, -1) + m(1, -1) + m(1, 0)) * 0.25 * (k_(0, 0) + k_(0, -1) + k_(1, -1) + k_(1, 0)) * rdx;
         mrdx = msfvx(i, j) * msfvy(i, j) * rdx;
-        mkrdym = (msfty(i, j - 1) / msftx(i, j - 1)) * K * rdy;
-        mkrdyp = (msfty(i, j) / msftx(i, j)) * K * rdy;
+        mkrdym = (msfty(i, j - 1) / msftx(i, j - 1)) * k_(0, -1) * rdy;
+        mkrdyp = (msfty(i, j) / msftx(i, j)) * k_(0, 0) * rdy;
         mrdy = msfvx(i, j) * msfvy(i, j) * rdy;
       } else if (name == 'w') {  // KHorizDiffW1: mass factors of the level, K averaged over k and k-1
-        mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.25 * (m(0, 0) + m(-1, 0) + m(0, 0) + m(-1, 0)) * 0.25 * K4 * rdx;
-        mkrdxp = (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.25 * (m(1, 0) + m(0, 0) + m(1, 0) + m(0, 0)) * 0.25 * K4 * rdx;
+        mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.25 * (m(0, 0) + m(-1, 0) + m(0, 0) + m(-1, 0)) * 0.25 *
+                 (k_(0, 0) + k_(-1, 0) + k_(0, 0, -1) + k_(-1, 0, -1)) * rdx;
+        mkrdxp = (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.25 * (m(1, 0) + m(0, 0) + m(1, 0) + m(0, 0)) * 0.25 *
+                 (k_(1, 0) + k_(0, 0) + k_(1, 0, -1) + k_(0, 0, -1)) * rdx;
         mrdx = msftx(i, j) * msfty(i, j) * rdx;
         if (do_y) {
-          mkrdym = (msfvy(i, j) * msfvx_inv(i, j)) * 0.25 * (m(0, 0) + m(0, -1) + m(0, 0) + m(0, -1)) * 0.25 * K4 * rdy;
+          mkrdym = (msfvy(i, j) * msfvx_inv(i, j)) * 0.25 * (m(0, 0) + m(0, -1) + m(0, 0) + m(0, -1)) * 0.25 *
+                   (k_(0, 0) + k_(0, -1) + k_(0, 0, -1) + k_(0, -1, -1)) * rdy;
           mkrdyp = (msfvy(i, j + 1) * msfvx_inv(i, j + 1)) * 0.25 * (m(0, 1) + m(0, 0) + m(0, 1) + m(0, 0)) * 0.25 *
-                   K4 * rdy;
+                   (k_(0, 1) + k_(0, 0) + k_(0, 1, -1) + k_(0, 0, -1)) * rdy;
           mrdy = msftx(i, j) * msfty(i, j) * rdy;
         }
       } else {  // KHorizDiffM1
-        mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
-        mkrdxp = (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.5 * K2 * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
+        mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.5 * (k_(0, 0) + k_(-1, 0)) * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
+        mkrdxp = (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.5 * (k_(1, 0) + k_(0, 0)) * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
         mrdx = msftx(i, j) * msfty(i, j) * rdx;
         if (do_y) {
-          mkrdym = (msfvy(i, j) * msfvx_inv(i, j)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
-          mkrdyp = (msfvy(i, j + 1) * msfvx_inv(i, j + 1)) * 0.5 * K2 * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
+          mkrdym = (msfvy(i, j) * msfvx_inv(i, j)) * 0.5 * (k_(0, 0) + k_(0, -1)) * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
+          mkrdyp = (msfvy(i, j + 1) * msfvx_inv(i, j + 1)) * 0.5 * (k_(0, 1) + k_(0, 0)) * 0.5 * (m(0, 1) + m(0, 0)) *
+                   rdy;
           mrdy = msftx(i, j) * msfty(i, j) * rdy;
         }
       }
@@ -657,7 +679,7 @@ struct Dom {
   // horizontal_diffusion_3dmp (big_step_diffusion.hpp:188-219, 391-414): theta minus its base state t_init, the
   // differences evaluated left to right as WRF
   template <class Fld>
-  void hdiff_3dmp(const Fld& q, F& tend, double K) {
+  void hdiff_3dmp(const Fld& q, F& tend, const F& K) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const bool fx = ox || spec, fy = oy || spec, do_y = !g.two_d();
     int is = fx ? 1 : 0, js = fy ? 1 : 0, ie = fx ? nx - 1 : nx, je = fy ? ny - 1 : ny;
@@ -665,9 +687,10 @@ struct Dom {
     const double rdx = c.rdx, rdy = c.rdy;
     pfor(is, ie, js, je, 0, nz, [&](int i, int j, int k) {
       auto m = [&](int a, int b) { return c.c1h[k] * mut(i + a, j + b) + c.c2h[k]; };
-      const double K2 = K + K;
-      const double mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
-      const double mkrdxp = (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.5 * K2 * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
+      auto k_ = [&](int a, int b) { return K(i + a, j + b, k); };
+      const double mkrdxm = (msfux(i, j) / msfuy(i, j)) * 0.5 * (k_(0, 0) + k_(-1, 0)) * 0.5 * (m(0, 0) + m(-1, 0)) * rdx;
+      const double mkrdxp =
+          (msfux(i + 1, j) / msfuy(i + 1, j)) * 0.5 * (k_(1, 0) + k_(0, 0)) * 0.5 * (m(1, 0) + m(0, 0)) * rdx;
       const double mrdx = msftx(i, j) * msfty(i, j) * rdx;
       auto b = [&](int a, int bb) { return t_init(i + a, j + bb, k); };
       const double dxp = q(i + 1, j, k) - q(i, j, k) - b(1, 0) + b(0, 0);
@@ -677,13 +700,156 @@ struct Dom {
         tend(i, j, k) += qx;
         return;
       }
-      const double mkrdym = (msfvy(i, j) * msfvx_inv(i, j)) * 0.5 * K2 * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
-      const double mkrdyp = (msfvy(i, j + 1) * msfvx_inv(i, j + 1)) * 0.5 * K2 * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
+      const double mkrdym =
+          (msfvy(i, j) * msfvx_inv(i, j)) * 0.5 * (k_(0, 0) + k_(0, -1)) * 0.5 * (m(0, 0) + m(0, -1)) * rdy;
+      const double mkrdyp =
+          (msfvy(i, j + 1) * msfvx_inv(i, j + 1)) * 0.5 * (k_(0, 1) + k_(0, 0)) * 0.5 * (m(0, 1) + m(0, 0)) * rdy;
       const double mrdy = msftx(i, j) * msfty(i, j) * rdy;
       const double dyp = q(i, j + 1, k) - q(i, j, k) - b(0, 1) + b(0, 0);
       const double dym = q(i, j, k) - q(i, j - 1, k) - b(0, 0) + b(0, -1);
       tend(i, j, k) += qx + mrdy * (mkrdyp * dyp - mkrdym * dym);
     });
+  }
+  // ------------------------------------------------------------------ km_opt 4: DiffusionDriver::prepare
+  // compute_diff_metrics with physical_bc (diff_metrics.hpp:20-138): rdzw at mass points, zx at u points and zy at
+  // v points over nz + 1 levels; a periodic side leaves the top level 0 on its boundary faces, a specified or open
+  // end side copies the last inner face, other non-periodic boundary faces are 0.  z and rdz are not read by
+  // diff_opt 1 and are left out.
+  void diff_metrics() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool px = g.bcx == MI_BC_PERIODIC, py = g.bcy == MI_BC_PERIODIC;
+    const bool cx = spec || g.bcx == MI_BC_OPEN, cy = spec || (g.bcy == MI_BC_OPEN && !g.two_d());
+    const double rdx = c.rdx, rdy = c.rdy;
+    pfor2(0, nx, 0, ny, [&](int i, int j) {
+      double zw[kMaxLev + 1];
+      for (int k = 0; k <= nz; ++k) zw[k] = (ph2(i, j, k) + phb(i, j, k)) / kG;
+      for (int k = 0; k < nz; ++k) rdzw(i, j, k) = 1.0 / (zw[k + 1] - zw[k]);
+    });
+    pfor(0, nx + 1, 0, ny, 0, nz + 1, [&](int i, int j, int k) {
+      if (px && k == nz && (i == 0 || i == nx)) {
+        zx(i, j, k) = 0;
+      } else if (!px && (i == 0 || (i == nx && !cx))) {
+        zx(i, j, k) = 0;
+      } else {
+        const int i0 = (i == nx) ? (px ? 0 : nx - 1) : i, im1 = (i == 0) ? nx - 1 : (i == nx && !px ? nx - 2 : i - 1);
+        double v = rdx * (phb(i0, j, k) - phb(im1, j, k)) / kG;
+        v += rdx * (ph2(i0, j, k) - ph2(im1, j, k)) / kG;
+        zx(i, j, k) = v;
+      }
+    });
+    pfor(0, nx, 0, ny + 1, 0, nz + 1, [&](int i, int j, int k) {
+      if (py && k == nz && (j == 0 || j == ny)) {
+        zy(i, j, k) = 0;
+      } else if (!py && (j == 0 || (j == ny && !cy))) {
+        zy(i, j, k) = 0;
+      } else {
+        const int j0 = (j == ny) ? (py ? 0 : ny - 1) : j, jm1 = (j == 0) ? ny - 1 : (j == ny && !py ? ny - 2 : j - 1);
+        double v = rdy * (phb(i, j0, k) - phb(i, jm1, k)) / kG;
+        v += rdy * (ph2(i, j0, k) - ph2(i, jm1, k)) / kG;
+        zy(i, j, k) = v;
+      }
+    });
+  }
+  // the vertical average of u / msf (or v / msf) onto the w levels of KDeformDiagAndDiv / KDeform12 (deform_div.hpp:
+  // 36-48): h0, h1 the two columns' values at the mass levels, out[0..nz]
+  void hat_avg(const double* h0, const double* h1, double* out) const {
+    const int nz = g.nz;
+    const double cft2 = -0.5 * c.dnw[size_t(nz - 1)] / c.dn[size_t(nz - 1)], cft1 = 1.0 - cft2;
+    for (int k = 1; k < nz; ++k) out[k] = 0.5 * (c.fnm[size_t(k)] * (h0[k] + h1[k]) + c.fnp[size_t(k)] * (h0[k - 1] + h1[k - 1]));
+    out[0] = 0.5 * (c.cf1 * h0[0] + c.cf2 * h0[1] + c.cf3 * h0[2] + c.cf1 * h1[0] + c.cf2 * h1[1] + c.cf3 * h1[2]);
+    out[nz] = 0.5 * (cft1 * (h0[nz - 1] + h1[nz - 1]) + cft2 * (h0[nz - 2] + h1[nz - 2]));
+  }
+  // cal_deform_and_div's defor11, defor22 (KDeformDiagAndDiv, deform_div.hpp:20-84) over the mass points and defor12
+  // (KDeform12, :86-155) over the vorticity points, with its edge copies in the reference's order (:257-311);
+  // defor33, defor13, defor23 and div are not read by diff_opt 1 with km_opt 4 and are left out
+  void deform() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool px = g.bcx == MI_BC_PERIODIC, py = g.bcy == MI_BC_PERIODIC;
+    const double rdx = c.rdx, rdy = c.rdy;
+    pfor2(0, nx, 0, ny, [&](int i, int j) {
+      const double mm = msftx(i, j) * msfty(i, j);
+      double h0[kMaxLev], h1[kMaxLev], ha[kMaxLev + 1], g0[kMaxLev], g1[kMaxLev], ga[kMaxLev + 1];
+      for (int k = 0; k < nz; ++k) {
+        h0[k] = u2(i, j, k) / msfuy(i, j);
+        h1[k] = u2(i + 1, j, k) / msfuy(i + 1, j);
+        g0[k] = v2(i, j, k) / msfvx(i, j);
+        g1[k] = v2(i, j + 1, k) / msfvx(i, j + 1);
+      }
+      hat_avg(h0, h1, ha);
+      hat_avg(g0, g1, ga);
+      for (int k = 0; k < nz; ++k) {
+        const double tzx = 0.25 * (zx(i, j, k) + zx(i + 1, j, k) + zx(i, j, k + 1) + zx(i + 1, j, k + 1));
+        const double dudx = mm * (rdx * (h1[k] - h0[k]) - (ha[k + 1] - ha[k]) * tzx * rdzw(i, j, k));
+        d11(i, j, k) = 2.0 * dudx;
+        const double tzy = 0.25 * (zy(i, j, k) + zy(i, j + 1, k) + zy(i, j, k + 1) + zy(i, j + 1, k + 1));
+        const double dvdy = mm * (rdy * (g1[k] - g0[k]) - (ga[k + 1] - ga[k]) * tzy * rdzw(i, j, k));
+        d22(i, j, k) = 2.0 * dvdy;
+      }
+    });
+    const bool bxs = !px && (spec || ox), bys = !py && (spec || oy);
+    pfor2(bxs ? 1 : 0, bxs ? nx : nx + 1, bys ? 1 : 0, bys ? ny : ny + 1, [&](int i, int j) {
+      const int im1 = (i == 0) ? (px ? nx - 1 : 0) : i - 1, i0 = (i == nx) ? (px ? 0 : nx - 1) : i;
+      const int jm1 = (j == 0) ? (py ? ny - 1 : 0) : j - 1, j0 = (j == ny) ? (py ? 0 : ny - 1) : j;
+      const double mm = 0.25 * (msfux(i, jm1) + msfux(i, j0)) * (msfvy(im1, j) + msfvy(i0, j));
+      double hm[kMaxLev], hp[kMaxLev], ha[kMaxLev + 1], gm[kMaxLev], gp[kMaxLev], ga[kMaxLev + 1];
+      for (int k = 0; k < nz; ++k) {
+        hm[k] = u2(i, jm1, k) / msfux(i, jm1);
+        hp[k] = u2(i, j0, k) / msfux(i, j0);
+        gm[k] = v2(im1, j, k) / msfvy(im1, j);
+        gp[k] = v2(i0, j, k) / msfvy(i0, j);
+      }
+      hat_avg(hm, hp, ha);
+      hat_avg(gm, gp, ga);
+      for (int k = 0; k < nz; ++k) {
+        const double tzy = 0.25 * (zy(im1, j, k) + zy(i0, j, k) + zy(im1, j, k + 1) + zy(i0, j, k + 1));
+        const double tu = (ha[k + 1] - ha[k]) * 0.25 * tzy *
+                          (rdzw(i0, j0, k) + rdzw(im1, j0, k) + rdzw(im1, jm1, k) + rdzw(i0, jm1, k));
+        const double term_u = mm * (rdy * (hp[k] - hm[k]) - tu);
+        const double tzx = 0.25 * (zx(i, jm1, k) + zx(i, j0, k) + zx(i, jm1, k + 1) + zx(i, j0, k + 1));
+        const double tv = (ga[k + 1] - ga[k]) * 0.25 * tzx *
+                          (rdzw(i0, j0, k) + rdzw(i0, jm1, k) + rdzw(im1, jm1, k) + rdzw(im1, j0, k));
+        const double term_v = mm * (rdx * (gp[k] - gm[k]) - tv);
+        d12(i, j, k) = term_u + term_v;
+      }
+    });
+    const bool bxe = bxs, bye = bys;
+    auto row = [&](int dst, int src) { cfor(1, ny + 1, [&](int, int j) { for (int k = 0; k < nz; ++k) d12(dst, j, k) = d12(src, j, k); }); };
+    auto col = [&](int dst, int src) { cfor(nx + 1, 1, [&](int i, int) { for (int k = 0; k < nz; ++k) d12(i, dst, k) = d12(i, src, k); }); };
+    if (px) row(nx, 0);
+    else if (bxs) row(0, 1);
+    if (py) col(ny, 0);
+    else if (bys) col(0, 1);
+    if (bxe) row(nx, nx - 1);
+    if (bye) col(ny, ny - 1);
+  }
+  // calculate_km_kh with km_opt 4 (km_kh.hpp:586-629): KSmag2dKm (:47-102) with diff_opt 1 over the mass points one
+  // row in from open and specified sides (a periodic x all of them); the rows outside keep 0; then fill_halos
+  void smag2d() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool fx = ox || spec, fy = oy || spec;
+    int is = fx ? 1 : 0, ie = fx ? nx - 1 : nx;
+    if (g.bcx == MI_BC_PERIODIC) { is = 0; ie = nx; }
+    const double dx = g.dx, dy = g.dy;
+    pfor(is, ie, fy ? 1 : 0, fy ? ny - 1 : ny, 0, nz, [&](int i, int j, int k) {
+      const double dd = d11(i, j, k) - d22(i, j, k);
+      double def2 = 0.25 * (dd * dd);
+      const double d12a = 0.25 * (d12(i, j, k) + d12(i, j + 1, k) + d12(i + 1, j, k) + d12(i + 1, j + 1, k));
+      def2 += d12a * d12a;
+      const double mlen = std::sqrt(dx / msftx(i, j) * dy / msfty(i, j));
+      const double tmp = std::sqrt(def2);
+      double kmh = c_s * c_s * mlen * mlen * tmp;
+      kmh = kmh < 10.0 * mlen ? kmh : 10.0 * mlen;
+      xkmh(i, j, k) = kmh;
+      xkhh(i, j, k) = kmh / prandtl;
+    });
+    fill(xkmh, g);
+    fill(xkhh, g);
+  }
+  void diffusion_prepare() {  // DiffusionDriver::prepare at the first RK stage (rk3.cpp:113)
+    if (p.diff_opt != 1 || p.km_opt != 4) return;
+    diff_metrics();
+    deform();
+    smag2d();
   }
   // vertical_diffusion, _mp, _u, _v (big_step_diffusion.hpp:225-338, 416-500): the column's fluxes at the w levels
   // 1..nz-1 (vflux[k] between levels k and k+1, 0 at the top), then their differences.  'u'/'v' at the faces (muu /
@@ -839,6 +1005,7 @@ struct Dom {
     // 888-893): a tendency the caller left there has no effect on the step
     if (rk == 1 && !has_phys)
       for (F& f : qtf) std::fill(f.a.begin(), f.a.end(), 0.0);
+    if (rk == 1) diffusion_prepare();
     if (rk == 1 && use_tendf) {  // zero_tendf, then DiffusionDriver::dynamics (diffusion_driver.cpp:266-281)
       if (!has_phys) {
         for (F* f : {&ru_tendf, &rv_tendf, &rw_tendf, &t_tendf, &ph_tendf}) std::fill(f->a.begin(), f->a.end(), 0.0);
@@ -2234,6 +2401,8 @@ struct Dom {
       case MI_CQU: return &cqu; case MI_CQV: return &cqv; case MI_CQW: return &cqw; case MI_H_DIABATIC: return &hdia;
       case MI_RAINNC: return mp ? &rainnc : nullptr;
       case MI_RAINNCV: return mp ? &rainncv : nullptr;
+      case MI_XKMH: return p.diff_opt == 1 ? &xkmh : nullptr;
+      case MI_XKHH: return p.diff_opt == 1 ? &xkhh : nullptr;
       case MI_RU_TENDF: return use_tendf ? &ru_tendf : nullptr;
       case MI_RV_TENDF: return use_tendf ? &rv_tendf : nullptr;
       case MI_RW_TENDF: return use_tendf ? &rw_tendf : nullptr;
@@ -2287,7 +2456,8 @@ int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv,
       return fail("bad n_moist / n_scalars");
     if (p->diff_6th_opt < 0 || p->diff_6th_opt > 2) return fail("diff_6th_opt must be 0, 1 or 2");
     if (p->diff_opt < 0 || p->diff_opt > 1) return fail("oracle: diff_opt 0 or 1 only");
-    if (p->diff_opt == 1 && p->km_opt != 1) return fail("oracle: km_opt 1 only");
+    if (p->diff_opt == 1 && p->km_opt != 1 && p->km_opt != 4) return fail("oracle: km_opt 1 or 4 only");
+    if (p->diff_opt == 1 && p->km_opt == 4 && g->nz + 1 >= kMaxLev) return fail("oracle: km_opt 4 needs nz < 255");
     if (p->mp_physics != 0 && p->mp_physics != 1) return fail("oracle: mp_physics 0 or 1 (Kessler) only");
     if (p->mp_physics == 1 && (p->n_moist < 3 || g->nz > 128)) return fail("Kessler needs qv, qc, qr and nz <= 128");
     auto* o = new orc_domain;

@@ -29,7 +29,7 @@ def test_exports_every_declared_function():
 
 
 def test_abi_version():
-    assert A.lib().mi_abi_version() == A.ABI_VERSION == 4
+    assert A.lib().mi_abi_version() == A.ABI_VERSION == 5
 
 
 @pytest.mark.parametrize("dt,dx,dy,want", [(6.0, 1000.0, 1000.0, 4), (1.0, 100.0, 100.0, 6), (10.0, 2000.0, 2000.0, 4),
