@@ -31,7 +31,7 @@ MI_MSFTX, MI_MSFTY, MI_MSFUX, MI_MSFUY, MI_MSFVX, MI_MSFVY, MI_MSFVX_INV = range
 MI_CQU, MI_CQV, MI_CQW, MI_H_DIABATIC = range(20, 24)
 MI_RU_TENDF, MI_RV_TENDF, MI_RW_TENDF, MI_T_TENDF, MI_TH_FORCING, MI_QV_FORCING = range(24, 30)
 MI_F, MI_E, MI_SINA, MI_COSA, MI_HT = range(47, 52)
-MI_U_BASE, MI_V_BASE, MI_QV_BASE, MI_RAINNC, MI_RAINNCV, MI_XKMH, MI_XKHH = range(52, 59)
+MI_U_BASE, MI_V_BASE, MI_QV_BASE, MI_RAINNC, MI_RAINNCV, MI_XKMH, MI_XKHH, MI_T_BASE, MI_Z_BASE = range(52, 61)
 MI_Q, MI_Q_TENDF = 64, 96
 MI_MAX_MOIST, MI_MAX_SCALARS = 8, 16
 ABI_VERSION = 5
@@ -63,7 +63,7 @@ STAGGER = {
     MI_RU_TENDF: "U", MI_RV_TENDF: "V", MI_RW_TENDF: "W", MI_T_TENDF: "M", MI_TH_FORCING: "M", MI_QV_FORCING: "M",
     MI_MSFTX: "M2", MI_MSFTY: "M2", MI_MSFUX: "U2", MI_MSFUY: "U2", MI_MSFVX: "V2", MI_MSFVY: "V2",
     MI_MSFVX_INV: "V2", MI_F: "M2", MI_E: "M2", MI_SINA: "M2", MI_COSA: "M2", MI_HT: "M2",
-    MI_U_BASE: "Z", MI_V_BASE: "Z", MI_QV_BASE: "Z", MI_RAINNC: "M2", MI_RAINNCV: "M2", MI_XKMH: "M", MI_XKHH: "M",
+    MI_U_BASE: "Z", MI_V_BASE: "Z", MI_QV_BASE: "Z", MI_RAINNC: "M2", MI_RAINNCV: "M2", MI_XKMH: "M", MI_XKHH: "M", MI_T_BASE: "Z", MI_Z_BASE: "Z",
 }
 for _n in range(MI_MAX_SCALARS):
     STAGGER[MI_Q + _n] = "M"
@@ -76,7 +76,7 @@ FIELD_NAMES = {
     MI_T_TEND: "t_tend", MI_PH_TEND: "ph_tend", MI_CQU: "cqu", MI_CQV: "cqv", MI_CQW: "cqw",
     MI_H_DIABATIC: "h_diabatic", MI_RU_TENDF: "ru_tendf", MI_RV_TENDF: "rv_tendf", MI_RW_TENDF: "rw_tendf",
     MI_T_TENDF: "t_tendf", MI_TH_FORCING: "rthften", MI_QV_FORCING: "rqvften", MI_RAINNC: "rainnc",
-    MI_RAINNCV: "rainncv", MI_H_DIABATIC: "h_diabatic", MI_XKMH: "xkmh", MI_XKHH: "xkhh",
+    MI_RAINNCV: "rainncv", MI_H_DIABATIC: "h_diabatic", MI_XKMH: "xkmh", MI_XKHH: "xkhh", MI_T_BASE: "t_base", MI_Z_BASE: "z_base",
 }
 for _n in range(MI_MAX_SCALARS):
     FIELD_NAMES[MI_Q + _n] = f"q{_n}"

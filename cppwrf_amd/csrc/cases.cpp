@@ -119,6 +119,7 @@ struct mi_case {
   double mub = 0;
   std::vector<double> pb, alb, phb, t_init;             // profiles
   std::vector<double> u_base, v_base, qv_base;          // the diffusion's base profiles (model.cpp:149-177)
+  std::vector<double> t_base, z_base;                   // and those of the Rayleigh damping (damp_opt 2)
   std::vector<double> u, v, w, t, ph, mu, p, al;        // dense state
   // hills: the base state in full, terrain, map factors, Coriolis, moisture (dense arrays)
   bool full = false;
@@ -652,16 +653,18 @@ int mi_case_create(const mi_case_spec* spec, mi_case** out) {
     } else {
       build(*c, *spec);
     }
-    // the diffusion's base profiles: u, v and qv of column (0, 0) (module_initialize_ideal as the reference,
-    // model.cpp:149-177)
+    // the 1-D base profiles of column (0, 0) (module_initialize_ideal as the reference, model.cpp:149-177): u, v,
+    // qv; theta of t_init; the height of the mass levels from phb
     const int nx = c->g.nx, ny = c->g.ny, nz = c->nz, nxu = nx + 1;
-    c->u_base.assign(size_t(nz), 0.0);
-    c->v_base.assign(size_t(nz), 0.0);
-    c->qv_base.assign(size_t(nz), 0.0);
+    for (auto* v : {&c->u_base, &c->v_base, &c->qv_base, &c->t_base, &c->z_base}) v->assign(size_t(nz), 0.0);
+    const size_t col = size_t(ny) * size_t(nx);  // the level stride of a dense mass or w field
+    auto phb0 = [&](int k) { return c->full ? c->phb3[size_t(k) * col] : c->phb[size_t(k)]; };
     for (int k = 0; k < nz; ++k) {
       c->u_base[size_t(k)] = c->u[size_t(k) * ny * nxu];
       c->v_base[size_t(k)] = c->v[size_t(k) * (ny > 1 ? ny + 1 : 1) * nx];
       if (!c->qv.empty()) c->qv_base[size_t(k)] = c->qv[size_t(k) * ny * nx];
+      c->t_base[size_t(k)] = c->full ? c->t_init3[size_t(k) * col] : c->t_init[size_t(k)];
+      c->z_base[size_t(k)] = 0.5 * (phb0(k) + phb0(k + 1)) / 9.81;
     }
     *out = c.release();
   });
@@ -741,6 +744,8 @@ int mi_case_get_field(const mi_case* c, int id, double* h, int64_t n) {
       case MI_U_BASE: put(c->u_base); break;
       case MI_V_BASE: put(c->v_base); break;
       case MI_QV_BASE: put(c->qv_base); break;
+      case MI_T_BASE: put(c->t_base); break;
+      case MI_Z_BASE: put(c->z_base); break;
       case MI_Q:
         if (c->qv.empty()) throw std::invalid_argument("case: dry, no qv");
         put(c->qv);
@@ -794,6 +799,8 @@ int mi_case_load(const mi_case* c, mi_domain* d) {
     set(MI_U_BASE, c->u_base);
     set(MI_V_BASE, c->v_base);
     set(MI_QV_BASE, c->qv_base);
+    set(MI_T_BASE, c->t_base);
+    set(MI_Z_BASE, c->z_base);
   });
 }
 

@@ -246,8 +246,9 @@ struct Domain final : mi_domain {
   // tendencies first (large_step.cpp:820-821) and the diffusion adds to them.
   void rk_tendency(int rk, bool hooks = true) {
     if (spec) launch_fill2d(a, a.f.mu_tend - off, n2, 0.0, stream());
-    if (rk == 1 && a.q.km4) {  // DiffusionDriver::prepare (rk3.cpp:113): the step's K, then its halos
-      launch_smag2d(a, stream());
+    if (rk == 1 && a.q.karr) {  // DiffusionDriver::prepare (rk3.cpp:113): the step's K, then its halos
+      if (a.q.km4) launch_smag2d(a, stream());
+      if (a.q.damp_opt == 1) launch_dampkm(a, stream());
       halo({{a.f.xkmh, -1, -1}, {a.f.xkhh, -1, -1}});
     }
     if (rk == 1 && a.q.use_tendf) {
@@ -257,6 +258,7 @@ struct Domain final : mi_domain {
       } else {
         launch_sixth_dyn(a, kSixthZero, stream());
       }
+      if (a.q.damp_opt == 2) launch_rayleigh(a, stream());  // the last term of DiffusionDriver::dynamics
     }
     if (rk == 1 && spec) launch_bdy(a, sc, bdy_dry(), kBdyRelax, stream());
     launch_tendencies(a, stream());
@@ -502,7 +504,7 @@ struct Domain final : mi_domain {
     prm.diff_6th_factor = factor;
     a.q.diff_6th_opt = opt;
     a.q.diff_6th_factor = factor;
-    held_base = opt != 0 || prm.diff_opt != 0 || spec;
+    held_base = opt != 0 || prm.diff_opt != 0 || prm.damp_opt == 2 || spec;
     a.q.use_tendf = (held_base || has_phys) ? 1 : 0;
     drop_graphs();
   }
@@ -812,10 +814,11 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   if (!(g.dx > 0) || !(g.dt > 0) || (g.ny > 1 && !(g.dy > 0))) throw Error(MI_EINVAL, "grid: dx, dy, dt must be > 0");
   if (p.h_adv_order != 5 && p.h_adv_order != 6) throw Error(MI_EINVAL, "h_adv_order must be 5 or 6");
   if (p.v_adv_order != 3 && p.v_adv_order != 4) throw Error(MI_EINVAL, "v_adv_order must be 3 or 4");
-  if (p.damp_opt != 0 && p.damp_opt != 3) throw Error(MI_EINVAL, "damp_opt must be 0 or 3");
+  if (p.damp_opt < 0 || p.damp_opt > 3) throw Error(MI_EINVAL, "damp_opt must be 0, 1, 2 or 3");
   if (p.hypsometric_opt != 1) throw Error(MI_EINVAL, "hypsometric_opt must be 1");
   if (p.precision != MI_FP64 && p.precision != MI_FP32_BULK) throw Error(MI_EINVAL, "precision: MI_FP64 or MI_FP32_BULK");
-  if (p.damp_opt == 3 && !(p.zdamp > 0)) throw Error(MI_EINVAL, "zdamp must be > 0");
+  if (p.damp_opt != 0 && !(p.zdamp > 0)) throw Error(MI_EINVAL, "zdamp must be > 0");
+  if (p.damp_opt == 2 && g.nz < 2) throw Error(MI_EINVAL, "damp_opt 2 needs nz >= 2");
   if (p.n_moist < 0 || p.n_moist > MI_MAX_MOIST) throw Error(MI_EINVAL, "n_moist must be in 0..8");
   if (p.n_scalars < p.n_moist || p.n_scalars > MI_MAX_SCALARS) throw Error(MI_EINVAL, "n_scalars must be in n_moist..16");
   if ((p.moist_adv_opt != 0 && p.moist_adv_opt != 1) || (p.scalar_adv_opt != 0 && p.scalar_adv_opt != 1))
@@ -907,7 +910,8 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   add2(f.fcor, St::M2, MI_F); add2(f.ecor, St::M2, MI_E); add2(f.sina, St::M2, MI_SINA); add2(f.cosa, St::M2, MI_COSA);
   launch_fill(f.cosa - d.off, d.n2, 1.0, ctx->stream);
   d.spec = g.bc_x == MI_BC_SPECIFIED;
-  const bool held = p.diff_6th_opt != 0 || p.diff_opt != 0 || d.spec;  // a diffusion driver or a LateralBC
+  // a diffusion driver (DiffusionConfig::on(), diffusion_driver.hpp:50) or a LateralBC
+  const bool held = p.diff_6th_opt != 0 || p.diff_opt != 0 || p.damp_opt == 2 || d.spec;
   if (held) {
     add3(f.ru_tendf, St::U, MI_RU_TENDF); add3(f.rv_tendf, St::V, MI_RV_TENDF); add3(f.rw_tendf, St::W, MI_RW_TENDF);
     add3(f.t_tendf, St::M, MI_T_TENDF);
@@ -968,10 +972,12 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     add3(f.dz8w, St::M, -1); add3(f.qvd, St::M, -1); add3(f.qcd, St::M, -1);
     add2(f.rainnc, St::M2, MI_RAINNC); add2(f.rainncv, St::M2, MI_RAINNCV);
   }
-  if (p.diff_opt == 1 && p.km_opt == 4) {  // the step's eddy viscosities (0 where not set) and defor12
+  if (p.diff_opt == 1 && (p.km_opt == 4 || p.damp_opt == 1)) {  // the step's eddy viscosities (0 where not set)
     add3(f.xkmh, St::M, MI_XKMH); add3(f.xkhh, St::M, MI_XKHH);
-    add3(f.d12, St::U, -1);
+    if (p.km_opt == 4) add3(f.d12, St::U, -1);
   }
+  f.t_base = d.alloc(nz + 1);
+  f.z_base = d.alloc(nz + 1);
   f.u_base = d.alloc(nz + 1);
   f.v_base = d.alloc(nz + 1);
   f.qv_base = d.alloc(nz + 1);
@@ -1031,6 +1037,7 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     q.kvdif = p.kvdif;
     q.kvdq = p.kvdif / prandtl;
     q.km4 = p.km_opt == 4 ? 1 : 0;
+    q.karr = (p.km_opt == 4 || p.damp_opt == 1) ? 1 : 0;
     q.prandtl = prandtl;
     q.c_s = p.c_s > 0 ? p.c_s : 0.25;
     q.dx = g.dx;
@@ -1075,7 +1082,8 @@ void copy_out(std::vector<double>& h, const void* dev, int esize, hipStream_t st
 }  // namespace
 
 double* Domain::base_profile(int id) const {
-  return id == MI_U_BASE ? a.f.u_base : id == MI_V_BASE ? a.f.v_base : id == MI_QV_BASE ? a.f.qv_base : nullptr;
+  return id == MI_U_BASE ? a.f.u_base : id == MI_V_BASE ? a.f.v_base : id == MI_QV_BASE ? a.f.qv_base
+         : id == MI_T_BASE ? a.f.t_base : id == MI_Z_BASE ? a.f.z_base : nullptr;
 }
 
 void Domain::set_field(int id, const double* host, int64_t n) {

@@ -98,7 +98,7 @@ inline size_t field_size(int field, const mi_grid& g) {
       return nx * ny;
     case MI_XKMH: case MI_XKHH: return nx * ny * nz;
     case MI_MSFUX: case MI_MSFUY: return (nx + 1) * ny;
-    case MI_U_BASE: case MI_V_BASE: case MI_QV_BASE: return nz;
+    case MI_U_BASE: case MI_V_BASE: case MI_QV_BASE: case MI_T_BASE: case MI_Z_BASE: return nz;
     case MI_MSFVX: case MI_MSFVY: case MI_MSFVX_INV: return nx * (ny + vy);
     default: return nx * ny * nz;  // mass points: t, p, al, alt, php, tendencies, h_diabatic, scalars
   }

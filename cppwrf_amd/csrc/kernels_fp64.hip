@@ -52,25 +52,52 @@ struct Range {
   int kfast;  // the block order of a point kernel (below)
 };
 
-// The point kernels' blocks (64 i x 4 j at one level) in level-fastest order: the n-th block of an XCD's share is
-// level n % gz of tile n / gz (tiles i-fastest), so the levels of one tile run back to back and the k +- 1 and
-// k +- 2 rows of a vertical stencil are still in that XCD's L2 (a whole level-plane of the arrays a stencil
-// kernel reads outgrows the 4 MB L2 at 425x300).  kfast = 0 keeps the i, j, level order of xcd_block.
+// The point kernels' blocks (64 i x 4 j at one level), in one of three orders within an XCD's share:
+//  kfast = 0: i, j, level (xcd_block's own);
+//  kfast = 1: level-fastest: the n-th block is level n % gz of tile n / gz (tiles i-fastest), so the levels of one
+//    tile run back to back and the k +- 1, k +- 2 rows of a vertical stencil are still in that XCD's L2 (a whole
+//    level-plane of the arrays a stencil kernel reads outgrows the 4 MB L2 at 425x300);
+//  kfast = G >= 2: groups of G tile rows (one tile column), the rows of a group fastest, then the levels, then the
+//    groups (i-fastest): the j +- 3 halo rows of a horizontal stencil are read while the neighbouring rows are hot,
+//    and a vertical stencil's levels are G blocks apart.
 __device__ __forceinline__ Blk point_block(int kfast) {
   Blk b = xcd_block();
   if (!kfast) return b;
   const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
   const unsigned l = unsigned(b.x) + gx * (unsigned(b.y) + gy * unsigned(b.z));
-  b.z = int(l % gz);
-  b.x = int((l / gz) % gx);
-  b.y = int(l / (gz * gx));
+  if (kfast == 1) {
+    b.z = int(l % gz);
+    b.x = int((l / gz) % gx);
+    b.y = int(l / (gz * gx));
+    return b;
+  }
+  const unsigned G = unsigned(kfast), ngy = (gy + G - 1) / G, full = gx * (ngy - 1) * G * gz;
+  unsigned grp, s, gs;
+  if (l < full) {  // the groups of G rows
+    gs = G;
+    grp = l / (G * gz);
+    s = l % (G * gz);
+  } else {  // the last row of groups holds the remaining gy - (ngy - 1) G rows
+    gs = gy - (ngy - 1) * G;
+    grp = gx * (ngy - 1) + (l - full) / (gs * gz);
+    s = (l - full) % (gs * gz);
+  }
+  b.x = int(grp % gx);
+  b.y = int((grp / gx) * G + s % gs);
+  b.z = int(s / gs);
   return b;
 }
-// the order point kernels launch with: level-fastest unless MI_BLOCK_ORDER=ijk (for timing the two)
+// the order point kernels launch with: level-fastest by default, MI_BLOCK_ORDER=ijk the i, j, level order,
+// MI_BLOCK_ORDER=group:G the groups of G tile rows (for timing them)
 int point_order() {
   static const int o = [] {
     const char* e = std::getenv("MI_BLOCK_ORDER");
-    return (e && std::strcmp(e, "ijk") == 0) ? 0 : 1;
+    if (e && std::strcmp(e, "ijk") == 0) return 0;
+    if (e && std::strncmp(e, "group:", 6) == 0) {
+      const int g = std::atoi(e + 6);
+      return g >= 2 && g <= 256 ? g : 1;
+    }
+    return 1;
   }();
   return o;
 }
@@ -2487,7 +2514,8 @@ __device__ __forceinline__ bool sixth_in(const Geo& g, int diff, char kind, int 
 // terms are left out: rdy = 0 makes them zeros that leave the sum's bits unchanged (they are added to a tendency
 // that is never -0).
 
-// the eddy viscosity at mass point offset (di, dj, dk) from index x: a field (km_opt 4) or the constant (km_opt 1)
+// the eddy viscosity at mass point offset (di, dj, dk) from index x: a field (km_opt 4, or damp_opt 1) or the
+// constant (km_opt 1)
 struct EddyK {
   const real* p;
   double c;
@@ -2497,7 +2525,7 @@ struct EddyK {
   }
 };
 __device__ __forceinline__ EddyK eddy_k(const Args& a, const real* p, double c, idx_t x) {
-  return EddyK{a.q.km4 ? p : nullptr, c, x, a.g.sj, a.g.sk};
+  return EddyK{a.q.karr ? p : nullptr, c, x, a.g.sj, a.g.sk};
 }
 
 // the loop limits of horizontal_diffusion (big_step_diffusion.hpp:343-414): one row in from open and specified
@@ -2814,6 +2842,134 @@ MI_POINT_KERNEL k_smag2d(Range r, Args a) {
   kmh = kmh < 10.0 * mlen ? kmh : 10.0 * mlen;
   f.xkmh[x] = kmh;
   f.xkhh[x] = kmh / a.q.prandtl;
+}
+
+// rdz at mass point (i, j, k): 2 / the distance between the w levels k - 1 and k + 1 (the first layer's thickness
+// at k = 0), from the full geopotential
+__device__ __forceinline__ double diff_rdz(const Args& a, int i, int j, int k) {
+  const idx_t x = a.g.at(i, j, k), sk = a.g.sk;
+  const int km = k == 0 ? 0 : k - 1;
+  const double z1 = (double(a.f.ph2[x + sk]) + a.f.phb(i, j, k + 1)) / kG;
+  const double z0 = (double(a.f.ph2[x + (km - k) * sk]) + a.f.phb(i, j, km)) / kG;
+  return 2.0 / (z1 - z0);
+}
+
+// ----------------------------------------------------------------------------- damp_opt 1
+// cal_dampkm (reference km_kh.hpp:485-568) after the eddy viscosities of the step: in each column (one row in from
+// the relaxation zone of specified sides) xkmh and xkhh are raised to kmmax cos^2(pi/2 min(1, depth / zdamp))
+// dampcoef, kmmax = min(dx, dy)^2 / dt of the column, the depth below the top summed down from half the top
+// layer in the reference's order.  With km_opt 1 the launch also writes the constants (KIsotropicKm) at every mass
+// point.  The vertical K it also raises is not read by diff_opt 1.
+MI_POINT_KERNEL k_dampkm(Range r, Args a, int is, int ie, int js, int je) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  double kh = a.q.km4 ? double(f.xkmh[x]) : a.q.xkmh;
+  double kt = a.q.km4 ? double(f.xkhh[x]) : a.q.xkhh;
+  if (i >= is && i < ie && j >= js && j < je) {
+    const int nz = g.nz;
+    double deltaz = 0.5 * (1.0 / diff_rdzw(a, i, j, nz - 1));
+    for (int kk = nz - 2; kk >= k; --kk) deltaz = deltaz + 1.0 / diff_rdz(a, i, j, kk);
+    const double dx_m = a.q.dx / f.msftx[x2], dy_m = a.q.dy / f.msfty[x2];
+    const double ds = dx_m < dy_m ? dx_m : dy_m;
+    const double kmmax = ds * ds / a.q.dt;
+    const double ratio = deltaz / a.q.zdamp;
+    const double tmp = ratio < 1.0 ? ratio : 1.0;
+    constexpr double piconst = 3.1415926535897932384626433, degrad = piconst / 180.0, degrad90 = degrad * 90.0;
+    const double c = mi_libm::cos(degrad90 * tmp);
+    const double dampk = c * c * kmmax * a.q.dampcoef;
+    if (dampk > kh) kh = dampk;
+    if (dampk > kt) kt = dampk;
+  }
+  f.xkmh[x] = kh;
+  f.xkhh[x] = kt;
+}
+
+// ----------------------------------------------------------------------------- damp_opt 2
+// rk_rayleigh_damp (reference rayleigh_damp.hpp:14-259) into the held tendencies after the diffusion: in the top
+// zdamp metres u, v, theta relax towards their base profiles interpolated to each point's height and w towards 0,
+// with the coefficient sin^2(pi/2 (1 - min(1, depth / zdamp))) dampcoef.  The damped levels of a u, v or theta
+// column run from the top down to the one above the first level below the layer (level 1 if none is below): a
+// thread scans its column from the top down to its own level, as the reference's loop does.
+// true when mass level k of a column is damped; zk its height zl(k) then
+template <class ZL>
+__device__ __forceinline__ bool rayleigh_damped(int k, int ktf, double ztop, double zd, const ZL& zl, double& zk) {
+  for (int kk = ktf; kk >= k; --kk) {
+    const double z = zl(kk);
+    if (!(z >= (ztop - zd))) return false;
+    zk = z;
+  }
+  return k >= 1;
+}
+// the base profile q_base interpolated (or extrapolated at the ends) to height z over z_base
+__device__ __forceinline__ double rayleigh_base(const double* qb, const double* zb, int ktf, double z) {
+  int k2 = ktf;
+  while (k2 > 0 && zb[k2] > z) k2 = k2 - 1;
+  if (k2 + 1 > ktf) return qb[k2] + (qb[k2] - qb[k2 - 1]) * (z - zb[k2]) / (zb[k2] - zb[k2 - 1]);
+  return qb[k2] + (qb[k2 + 1] - qb[k2]) * (z - zb[k2]) / (zb[k2 + 1] - zb[k2]);
+}
+MI_POINT_KERNEL k_rayleigh(Range r, Args a) {
+  MI_POINT_INDEX(r)
+  const Geo& g = a.g;
+  const Fields& f = a.f;
+  const Lv& c = a.c;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, ktf = nz - 1;
+  const bool px = g.bcx == MI_BC_PERIODIC, py = g.bcy == MI_BC_PERIODIC;
+  const double zd = a.q.zdamp, dc = a.q.dampcoef, pii = 3.141592653589793;  // 2 asin(1)
+  const idx_t x = g.at(i, j, k), x2 = g.at2(i, j);
+  auto PH = [&](int ii, int jj, int kk) { return double(f.ph2[g.at(ii, jj, kk)]); };
+  auto coef = [&](double ztop, double z) {
+    const double ratio = (ztop - z) / zd;
+    const double m = ratio < 1.0 ? ratio : 1.0;
+    const double s = mi_libm::sin(0.5 * pii * (1.0 - m));
+    return s * s;
+  };
+  if (j < ny && k < nz) {  // KRayleighDampU: u faces i = 0..nx
+    const int i0 = (i == nx) ? (px ? 0 : nx - 1) : i, im1 = (i == 0) ? (px ? nx - 1 : 0) : (i - 1);
+    const double ztop = 0.5 * (f.phb(i0, j, nz) + f.phb(im1, j, nz) + PH(i0, j, nz) + PH(im1, j, nz)) / kG;
+    auto zl = [&](int kk) {
+      return 0.25 * (f.phb(i0, j, kk) + f.phb(i0, j, kk + 1) + f.phb(im1, j, kk) + f.phb(im1, j, kk + 1) +
+                     PH(i0, j, kk) + PH(i0, j, kk + 1) + PH(im1, j, kk) + PH(im1, j, kk + 1)) / kG;
+    };
+    double z00 = 0;
+    if (rayleigh_damped(k, ktf, ztop, zd, zl, z00)) {
+      const double u00 = rayleigh_base(f.u_base, f.z_base, ktf, z00);
+      f.ru_tendf[x] = double(f.ru_tendf[x]) - (c.c1h[k] * f.muu[x2] + c.c2h[k]) * (coef(ztop, z00) * dc) *
+                                                  (double(f.u2[x]) - u00);
+    }
+  }
+  if (!g.two_d && i < nx && j <= ny && k < nz) {  // KRayleighDampV: v faces j = 0..ny (an x-z slice holds no v)
+    const int j0 = (j == ny) ? (py ? 0 : ny - 1) : j, jm1 = (j == 0) ? (py ? ny - 1 : 0) : (j - 1);
+    const double ztop = 0.5 * (f.phb(i, j0, nz) + f.phb(i, jm1, nz) + PH(i, j0, nz) + PH(i, jm1, nz)) / kG;
+    auto zl = [&](int kk) {
+      return 0.25 * (f.phb(i, j0, kk) + f.phb(i, j0, kk + 1) + f.phb(i, jm1, kk) + f.phb(i, jm1, kk + 1) +
+                     PH(i, j0, kk) + PH(i, j0, kk + 1) + PH(i, jm1, kk) + PH(i, jm1, kk + 1)) / kG;
+    };
+    double z00 = 0;
+    if (rayleigh_damped(k, ktf, ztop, zd, zl, z00)) {
+      const double v00 = rayleigh_base(f.v_base, f.z_base, ktf, z00);
+      f.rv_tendf[x] = double(f.rv_tendf[x]) - (c.c1h[k] * f.muv[x2] + c.c2h[k]) * (coef(ztop, z00) * dc) *
+                                                  (double(f.v2[x]) - v00);
+    }
+  }
+  if (i < nx && j < ny) {  // KRayleighDampWT
+    const double ztop = (f.phb(i, j, nz) + PH(i, j, nz)) / kG;
+    const double zw = (f.phb(i, j, k) + PH(i, j, k)) / kG;
+    if (zw >= (ztop - zd)) {  // w at every level of the layer, the scalar sine
+      const double s2 = coef(ztop, zw);
+      f.rw_tendf[x] = double(f.rw_tendf[x]) - (c.c1f[k] * f.mut[x2] + c.c2f[k]) * (s2 * dc) * double(f.w2[x]);
+    }
+    if (k < nz) {
+      auto zl = [&](int kk) { return 0.5 * (f.phb(i, j, kk) + f.phb(i, j, kk + 1) + PH(i, j, kk) + PH(i, j, kk + 1)) / kG; };
+      double z00 = 0;
+      if (rayleigh_damped(k, ktf, ztop, zd, zl, z00)) {
+        const double t00 = rayleigh_base(f.t_base, f.z_base, ktf, z00);
+        f.t_tendf[x] = double(f.t_tendf[x]) - (c.c1h[k] * f.mut[x2] + c.c2h[k]) * (coef(ztop, z00) * dc) *
+                                                 (double(f.t2[x]) - t00);
+      }
+    }
+  }
 }
 
 // zero_tendf + DiffusionDriver::dynamics (diffusion_driver.cpp:266-281): the held tendencies of u, v, w,
@@ -3700,6 +3856,22 @@ void launch_smag2d(const Args& a, hipStream_t st) {
     ie = g.nx;
   }
   launch3a("smag2d", 6, k_smag2d, Range{is, ie, fy ? 1 : 0, fy ? g.ny - 1 : g.ny, 0, g.nz}, st, a);
+}
+void launch_dampkm(const Args& a, hipStream_t st) {
+  const Geo& g = a.g;
+  int is = 0, ie = g.nx, js = 0, je = g.ny;
+  if (g.spec()) {  // one row in from the relaxation zone (spec_bdy_width 5)
+    is = std::max(is, MI_SPEC_BDY_WIDTH - 1);
+    ie = std::min(ie, g.nx - MI_SPEC_BDY_WIDTH + 1);
+    js = std::max(js, MI_SPEC_BDY_WIDTH - 1);
+    je = std::min(je, g.ny - MI_SPEC_BDY_WIDTH + 1);
+  }
+  launch3a("dampkm", 3, k_dampkm, Range{0, g.nx, 0, g.ny, 0, g.nz}, st, a, is, ie, js, je);  // ph -> xkmh xkhh
+}
+void launch_rayleigh(const Args& a, hipStream_t st) {
+  const Geo& g = a.g;
+  // u v w t ph -> their 4 held tendencies (read and written)
+  launch3a("rayleigh", 9, k_rayleigh, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a);
 }
 void launch_sixth_scalars(const Args& a, const Scalars& s, int n, double dt_rk, hipStream_t st) {
   if (n <= 0) return;

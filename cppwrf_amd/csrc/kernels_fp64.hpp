@@ -69,6 +69,8 @@ struct Fields {
   // the scalars) and the deformation defor12 at the vorticity points ((nx+1) x (ny+1), u-staggered storage whose
   // row ny lies in the y halo)
   real *xkmh, *xkhh, *d12;
+  // damp_opt 2: the base profiles of theta and of the mass levels' height (nz values)
+  double *t_base, *z_base;
   // the microphysics after the RK loop: moist_physics_prep's fields (dry theta, density, Exner function, height
   // and thickness of the layer, the qv and qc before the scheme) and the grid-scale precipitation (2-D)
   real *th_phy, *mrho, *pii, *zmid, *dz8w, *qvd, *qcd;
@@ -118,7 +120,8 @@ struct Params {
   int diff_6th_opt;    // 1 linear, 2 monotonic
   double diff_6th_factor;
   int diff_opt;        // 1: second-order diffusion along eta
-  int km4;             // km_opt 4: the horizontal K from the 2-D Smagorinsky closure (else the constants below)
+  int km4;             // km_opt 4: the horizontal K from the 2-D Smagorinsky closure
+  int karr;            // the horizontal K read from xkmh / xkhh (km_opt 4 or damp_opt 1), else the constants below
   double xkmh, xkhh, kvdif, kvdq;  // its eddy viscosities: khdif, khdif / prandtl, kvdif, kvdif / prandtl
   double c_s, prandtl, dx, dy;     // km_opt 4's constant, 1 / pr_inv and the grid spacing
   int mp;              // microphysics after the RK loop: 1 Kessler
@@ -233,6 +236,8 @@ void launch_p_rho_phi(const Args& a, hipStream_t st);
 // calc_p_rho_phi are the caller's
 constexpr int kKesslerMaxLevels = 128;
 void launch_smag2d(const Args& a, hipStream_t st);
+void launch_dampkm(const Args& a, hipStream_t st);
+void launch_rayleigh(const Args& a, hipStream_t st);
 void launch_microphysics(const Args& a, const Scalars& s, hipStream_t st);
 // the same on the columns i0 <= i < i1, j0 <= j < j1
 void launch_p_rho_phi_range(const Args& a, int i0, int i1, int j0, int j1, hipStream_t st);

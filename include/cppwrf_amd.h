@@ -74,7 +74,12 @@ typedef struct mi_params {
   int32_t w_damping;       /* 1: damp w where the vertical Courant number exceeds 1 */
   double w_crit_cfl;
   double epssm, smdiv, emdiv;
-  int32_t damp_opt;        /* 0 or 3 (implicit Rayleigh damping of w in the top zdamp metres) */
+  /* damping in the top zdamp metres (dampcoef its strength): 0 off; 1 the horizontal eddy viscosities raised to a
+   * damping profile (with diff_opt 1; reference cal_dampkm, src/dyn/diffusion/km_kh.hpp:485-568); 2 Rayleigh
+   * relaxation of u, v, w and theta towards MI_U_BASE, MI_V_BASE, 0 and MI_T_BASE interpolated to each point's
+   * height (MI_Z_BASE), into the held tendencies at the first RK stage (rk_rayleigh_damp,
+   * src/dyn/diffusion/rayleigh_damp.hpp); 3 implicit Rayleigh damping of w in the acoustic steps */
+  int32_t damp_opt;
   int32_t hypsometric_opt; /* 1 */
   double zdamp, dampcoef;
   int32_t time_step_sound; /* acoustic steps per large step (even); 0: the reference's automatic rule */
@@ -172,10 +177,13 @@ enum {
   MI_U_BASE = 52, MI_V_BASE = 53, MI_QV_BASE = 54,
   /* the microphysics' accumulated and last-step grid-scale precipitation, mm (2-D mass points; with mp_physics) */
   MI_RAINNC = 55, MI_RAINNCV = 56,
-  /* the horizontal eddy viscosities of the step (3-D mass points, read only; with diff_opt 1): xkmh of the
+  /* the horizontal eddy viscosities of the step (3-D mass points, read only; with km_opt 4 or damp_opt 1): xkmh of the
    * momenta and xkhh = xkmh / prandtl of theta and the scalars, m2/s.  km_opt 4 leaves 0 on the outermost
    * rows of open and specified sides, as the reference */
   MI_XKMH = 57, MI_XKHH = 58,
+  /* the 1-D base profiles of damp_opt 2 (reference BaseState::t_base, z_base; the ideal cases take theta of
+   * t_init and the height of the mass levels of column (0, 0), src/cases/model.cpp:149-177): nz values each */
+  MI_T_BASE = 59, MI_Z_BASE = 60,
   /* scalar n (0 <= n < n_scalars) at mass points: MI_Q + n is its state (q_2), MI_Q_TENDF + n its held
    * tendency (input, mass-coupled, held over the step as the reference's q_tendf) */
   MI_Q = 64, MI_Q_TENDF = 96

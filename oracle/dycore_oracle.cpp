@@ -5,10 +5,12 @@
 // must agree with this file bit for bit, and both must agree with the reference.  Compiled with
 // -ffp-contract=off (no fused multiply-add), as the reference (CMakeLists.txt:33-37).
 //
-// Scope: dynamics on flat grids with map factors and periodic or symmetric lateral boundaries,
-// h_adv_order 5/6, v_adv_order 3/4, damp_opt 0/3, w_damping 0/1, top_lid 0/1, hypsometric_opt 1; moist
-// species coupled through calc_cq and theta_m, and the transport of moist species and passive tracers
-// (advect_scalar, advect_scalar_pd) with the mass fluxes of the acoustic steps.
+// Scope: the dynamics step on flat or terrain-following grids (hybrid or sigma coordinate) with map factors,
+// Coriolis and periodic, symmetric, open or specified lateral boundaries; h_adv_order 5/6, v_adv_order 3/4,
+// damp_opt 0..3, w_damping 0/1, top_lid 0/1, hypsometric_opt 1; moist species coupled through calc_cq and
+// theta_m, the transport of moist species and passive tracers (advect_scalar, advect_scalar_pd) with the mass
+// fluxes of the acoustic steps; the sixth-order and the diff_opt 1 diffusion (km_opt 1 and 4); Kessler
+// microphysics after the RK loop; the column physics hooks.
 #include "oracle.h"
 
 #include <emmintrin.h>
@@ -24,6 +26,8 @@
 
 extern "C" __m128d _ZGVbN2vv_pow(__m128d, __m128d);
 extern "C" __m128d _ZGVbN2v_exp(__m128d);
+extern "C" __m128d _ZGVbN2v_sin(__m128d);
+extern "C" __m128d _ZGVbN2v_cos(__m128d);
 
 namespace {
 
@@ -250,9 +254,10 @@ struct Dom {
   // eddy viscosities at mass points: km_opt 1 the constants of KIsotropicKm (km_kh.hpp:35-45: xkmh = khdif, xkhh =
   // khdif / prandtl), km_opt 4 those of KSmag2dKm, set each step from the metrics zx, zy, rdzw and the horizontal
   // deformations defor11, defor22, defor12 (DiffusionDriver::prepare, diffusion_driver.cpp:92-138)
-  std::vector<double> u_base, v_base, qv_base;
+  std::vector<double> u_base, v_base, qv_base, t_base, z_base;
   double kvdif = 0, kvdq = 0, prandtl = 1.0 / 3.0, c_s = 0.25;
-  F xkmh, xkhh, zx, zy, rdzw, d11, d22, d12;
+  F xkmh, xkhh, zx, zy, rdz, rdzw, d11, d22, d12;
+  bool karr = false;  // K set each step (km_opt 4, or damp_opt 1 raising the constants)
   // microphysics after the RK loop (PhysicsCoupling, physics_coupling.hpp:52-94): moist_physics_prep's fields and
   // the precipitation
   int mp = 0;
@@ -336,17 +341,21 @@ struct Dom {
       c_s = p.c_s > 0 ? p.c_s : 0.25;
       f3(xkmh, St::M);
       f3(xkhh, St::M);
-      if (p.km_opt == 1) {  // the constants everywhere, halos included
+      karr = p.km_opt == 4 || p.damp_opt == 1;
+      if (!karr) {  // the constants everywhere, halos included
         std::fill(xkmh.a.begin(), xkmh.a.end(), p.khdif);
         std::fill(xkhh.a.begin(), xkhh.a.end(), p.khdif / prandtl);
       } else {
         zx.init(nx, ny, nz, St::U, true, 1);
         zy.init(nx, ny, nz, St::V, true, 1);
-        for (F* f : {&rdzw, &d11, &d22}) f3(*f, St::M);
+        for (F* f : {&rdz, &rdzw, &d11, &d22}) f3(*f, St::M);
         f3(d12, St::U);  // (nx+1) x (ny+1): row ny lives in the y halo (diffusion_driver.cpp:30)
       }
     }
-    use_tendf = p.diff_6th_opt != 0 || p.diff_opt != 0 || spec;  // a DiffusionDriver or a LateralBC (large_step.cpp:828)
+    t_base.assign(size_t(nz), 0.0);
+    z_base.assign(size_t(nz), 0.0);
+    // a DiffusionDriver (DiffusionConfig::on(), diffusion_driver.hpp:50) or a LateralBC (large_step.cpp:828)
+    use_tendf = p.diff_6th_opt != 0 || p.diff_opt != 0 || p.damp_opt == 2 || spec;
     mp = p.mp_physics;
     mp_lim = p.mp_tend_lim > 0 ? p.mp_tend_lim : 10.0;
     if (mp) {
@@ -721,9 +730,11 @@ struct Dom {
     const bool cx = spec || g.bcx == MI_BC_OPEN, cy = spec || (g.bcy == MI_BC_OPEN && !g.two_d());
     const double rdx = c.rdx, rdy = c.rdy;
     pfor2(0, nx, 0, ny, [&](int i, int j) {
-      double zw[kMaxLev + 1];
+      double zw[kMaxLev + 1] = {};
       for (int k = 0; k <= nz; ++k) zw[k] = (ph2(i, j, k) + phb(i, j, k)) / kG;
       for (int k = 0; k < nz; ++k) rdzw(i, j, k) = 1.0 / (zw[k + 1] - zw[k]);
+      rdz(i, j, 0) = 2.0 / (zw[1] - zw[0]);
+      for (int k = 1; k < nz; ++k) rdz(i, j, k) = 2.0 / (zw[k + 1] - zw[k - 1]);
     });
     pfor(0, nx + 1, 0, ny, 0, nz + 1, [&](int i, int j, int k) {
       if (px && k == nz && (i == 0 || i == nx)) {
@@ -845,11 +856,148 @@ struct Dom {
     fill(xkmh, g);
     fill(xkhh, g);
   }
+  // cal_dampkm (km_kh.hpp:485-568, damp_opt 1): in each column (one row in from the relaxation zone of specified
+  // sides) the eddy viscosities raised to kmmax cos^2(pi/2 min(1, depth below the top / zdamp)) dampcoef, kmmax =
+  // min(dx, dy)^2 / dt of the column, the depth summed down from half the top layer
+  void dampkm() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    int is = 0, ie = nx, js = 0, je = ny;
+    if (spec) {
+      is = std::max(is, kBw - 1);
+      ie = std::min(ie, nx - kBw + 1);
+      js = std::max(js, kBw - 1);
+      je = std::min(je, ny - kBw + 1);
+    }
+    const int ni = ie - is;
+    const double piconst = 3.1415926535897932384626433, degrad = piconst / 180.0, degrad90 = degrad * 90.0;
+    const double dx = g.dx, dy = g.dy, dt = g.dt, dc = p.dampcoef, zd = p.zdamp;
+    pfor2(is, ie, js, je, [&](int i, int j) {
+      double dampk[kMaxLev], dampkv[kMaxLev], deltaz = 0;
+      for (int k = nz - 1; k >= 0; --k) {
+        const double dx_m = dx / msftx(i, j), dy_m = dy / msfty(i, j);
+        const double ds = dx_m < dy_m ? dx_m : dy_m;
+        const double kmmax = ds * ds / dt;
+        double dz;
+        if (k == nz - 1) {
+          dz = 1.0 / rdzw(i, j, k);
+          deltaz = 0.5 * dz;
+        } else {
+          dz = 1.0 / rdz(i, j, k);
+          deltaz = deltaz + dz;
+          dz = 1.0 / rdzw(i, j, k);
+        }
+        const double kmmvmax = dz * dz / dt;
+        const double ratio = deltaz / zd;
+        const double tmp = ratio < 1.0 ? ratio : 1.0;
+        const double c = vcos(degrad90 * tmp, i - is, ni);
+        dampk[k] = c * c * kmmax * dc;
+        dampkv[k] = c * c * kmmvmax * dc;
+        if (dampk[k] < dampkv[k]) dampkv[k] = dampk[k];
+      }
+      for (int k = 0; k < nz; ++k) {
+        if (dampk[k] > xkmh(i, j, k)) xkmh(i, j, k) = dampk[k];
+        if (dampk[k] > xkhh(i, j, k)) xkhh(i, j, k) = dampk[k];
+      }
+    });
+  }
   void diffusion_prepare() {  // DiffusionDriver::prepare at the first RK stage (rk3.cpp:113)
-    if (p.diff_opt != 1 || p.km_opt != 4) return;
+    if (p.diff_opt != 1 || !karr) return;
     diff_metrics();
-    deform();
-    smag2d();
+    if (p.km_opt == 4) {
+      deform();
+      smag2d();
+    } else {  // KIsotropicKm over the mass points
+      pfor(0, g.nx, 0, g.ny, 0, g.nz, [&](int i, int j, int k) {
+        xkmh(i, j, k) = p.khdif;
+        xkhh(i, j, k) = p.khdif / prandtl;
+      });
+    }
+    if (p.damp_opt == 1) dampkm();
+    fill(xkmh, g);
+    fill(xkhh, g);
+  }
+  // rk_rayleigh_damp (rayleigh_damp.hpp:14-259, damp_opt 2): in the top zdamp metres u, v, theta relax towards their
+  // base profiles interpolated to each point's height (u_base, v_base, t_base over z_base) and w towards 0, with the
+  // coefficient sin^2(pi/2 (1 - min(1, depth below the top / zdamp))) dampcoef, into the held tendencies.  The
+  // damped levels of a column run from the top down to the one above the first point below the layer (level 1 if
+  // none is below); u, v and theta take libmvec's sine lanes over those levels, w the scalar sine.
+  void rayleigh() {
+    const int nx = g.nx, ny = g.ny, nz = g.nz, ktf = nz - 1;
+    const bool px = g.bcx == MI_BC_PERIODIC, py = g.bcy == MI_BC_PERIODIC;
+    const double pii = 2.0 * std::asin(1.0), dc = p.dampcoef, zd = p.zdamp;
+    // one column: zlev(k) the height of mass level k, base the profile, apply(k, coef, ref) the tendency update
+    auto column = [&](double ztop, auto zlev, const std::vector<double>& base, auto apply) {
+      double z00[kMaxLev], q00[kMaxLev];
+      int k1 = ktf;
+      double z = ztop;
+      while (k1 >= 0 && z >= (ztop - zd)) {
+        z = zlev(k1);
+        z00[k1] = z;
+        k1 = k1 - 1;
+      }
+      k1 = k1 + 2;
+      for (int k = k1; k <= ktf; ++k) {
+        int k2 = ktf;
+        while (k2 > 0 && z_base[size_t(k2)] > z00[k]) k2 = k2 - 1;
+        const auto b = [&](int q) { return base[size_t(q)]; };
+        const auto zb = [&](int q) { return z_base[size_t(q)]; };
+        if (k2 + 1 > ktf)
+          q00[k] = b(k2) + (b(k2) - b(k2 - 1)) * (z00[k] - zb(k2)) / (zb(k2) - zb(k2 - 1));
+        else
+          q00[k] = b(k2) + (b(k2 + 1) - b(k2)) * (z00[k] - zb(k2)) / (zb(k2 + 1) - zb(k2));
+      }
+      const int nk = ktf - k1 + 1;
+      for (int k = k1; k <= ktf; ++k) {
+        const double ratio = (ztop - z00[k]) / zd;
+        const double m = ratio < 1.0 ? ratio : 1.0;
+        const double s = vsin(0.5 * pii * (1.0 - m), k - k1, nk);
+        apply(k, s * s, q00[k]);
+      }
+    };
+    pfor2(0, nx + 1, 0, ny, [&](int i, int j) {  // KRayleighDampU
+      const int i0 = (i == nx) ? (px ? 0 : nx - 1) : i, im1 = (i == 0) ? (px ? nx - 1 : 0) : (i - 1);
+      const double ztop = 0.5 * (phb(i0, j, nz) + phb(im1, j, nz) + ph2(i0, j, nz) + ph2(im1, j, nz)) / kG;
+      auto zl = [&](int k) {
+        return 0.25 * (phb(i0, j, k) + phb(i0, j, k + 1) + phb(im1, j, k) + phb(im1, j, k + 1) + ph2(i0, j, k) +
+                       ph2(i0, j, k + 1) + ph2(im1, j, k) + ph2(im1, j, k + 1)) / kG;
+      };
+      column(ztop, zl, u_base, [&](int k, double dcoef, double u00) {
+        ru_tendf(i, j, k) = ru_tendf(i, j, k) - (c.c1h[size_t(k)] * muu(i, j) + c.c2h[size_t(k)]) * (dcoef * dc) *
+                                                    (u2(i, j, k) - u00);
+      });
+    });
+    if (!g.two_d()) {  // an x-z slice holds no v tendency
+      pfor2(0, nx, 0, ny + 1, [&](int i, int j) {  // KRayleighDampV
+        const int j0 = (j == ny) ? (py ? 0 : ny - 1) : j, jm1 = (j == 0) ? (py ? ny - 1 : 0) : (j - 1);
+        const double ztop = 0.5 * (phb(i, j0, nz) + phb(i, jm1, nz) + ph2(i, j0, nz) + ph2(i, jm1, nz)) / kG;
+        auto zl = [&](int k) {
+          return 0.25 * (phb(i, j0, k) + phb(i, j0, k + 1) + phb(i, jm1, k) + phb(i, jm1, k + 1) + ph2(i, j0, k) +
+                         ph2(i, j0, k + 1) + ph2(i, jm1, k) + ph2(i, jm1, k + 1)) / kG;
+        };
+        column(ztop, zl, v_base, [&](int k, double dcoef, double v00) {
+          rv_tendf(i, j, k) = rv_tendf(i, j, k) - (c.c1h[size_t(k)] * muv(i, j) + c.c2h[size_t(k)]) * (dcoef * dc) *
+                                                      (v2(i, j, k) - v00);
+        });
+      });
+    }
+    pfor2(0, nx, 0, ny, [&](int i, int j) {  // KRayleighDampWT
+      const double ztop = (phb(i, j, nz) + ph2(i, j, nz)) / kG;
+      for (int k = 0; k <= nz; ++k) {
+        const double z = (phb(i, j, k) + ph2(i, j, k)) / kG;
+        if (z >= (ztop - zd)) {
+          const double ratio = (ztop - z) / zd;
+          const double m = ratio < 1.0 ? ratio : 1.0;
+          const double s = std::sin(0.5 * pii * (1.0 - m));
+          rw_tendf(i, j, k) = rw_tendf(i, j, k) - (c.c1f[size_t(k)] * mut(i, j) + c.c2f[size_t(k)]) * (s * s * dc) *
+                                                      w2(i, j, k);
+        }
+      }
+      auto zl = [&](int k) { return 0.5 * (phb(i, j, k) + phb(i, j, k + 1) + ph2(i, j, k) + ph2(i, j, k + 1)) / kG; };
+      column(ztop, zl, t_base, [&](int k, double dcoef, double t00) {
+        t_tendf(i, j, k) = t_tendf(i, j, k) - (c.c1h[size_t(k)] * mut(i, j) + c.c2h[size_t(k)]) * (dcoef * dc) *
+                                                  (t2(i, j, k) - t00);
+      });
+    });
   }
   // vertical_diffusion, _mp, _u, _v (big_step_diffusion.hpp:225-338, 416-500): the column's fluxes at the w levels
   // 1..nz-1 (vflux[k] between levels k and k+1, 0 at the top), then their differences.  'u'/'v' at the faces (muu /
@@ -1036,6 +1184,7 @@ struct Dom {
         sixth('w', fw, rw_tendf, mu_t, g.dt, c.c1f, c.c2f);
         sixth('m', ft, t_tendf, mu_t, g.dt, c.c1h, c.c2h);
       }
+      if (p.damp_opt == 2) rayleigh();  // the last term of DiffusionDriver::dynamics (diffusion_driver.cpp:283-289)
       if (spec) relax_dry();  // relax_bdy_dry (large_step.cpp:825)
     }
     const bool do_y = !g.two_d();
@@ -2221,6 +2370,14 @@ struct Dom {
   // libm of WRF's vectorised loops (backend.hpp:144-175): libmvec's 2-lane function for the pairs of an n-trip loop,
   // the scalar one for an odd last trip; scalar everywhere without the lanes
   static bool lane(int i, int n) { return n > 1 && !((n & 1) && i == n - 1); }
+  double vsin(double a, int i, int n) const {
+    if (libmvec && lane(i, n)) return _mm_cvtsd_f64(_ZGVbN2v_sin(_mm_set1_pd(a)));
+    return std::sin(a);
+  }
+  double vcos(double a, int i, int n) const {
+    if (libmvec && lane(i, n)) return _mm_cvtsd_f64(_ZGVbN2v_cos(_mm_set1_pd(a)));
+    return std::cos(a);
+  }
   double vpow(double a, double b, int i, int n) const {
     if (libmvec && lane(i, n)) return _mm_cvtsd_f64(_ZGVbN2vv_pow(_mm_set1_pd(a), _mm_set1_pd(b)));
     return std::pow(a, b);
@@ -2451,7 +2608,8 @@ int orc_domain_create(const mi_grid* g, const mi_params* p, const mi_levels* lv,
     if (p->h_adv_order != 5 && p->h_adv_order != 6) return fail("h_adv_order must be 5 or 6");
     if (p->v_adv_order != 3 && p->v_adv_order != 4) return fail("v_adv_order must be 3 or 4");
     if (p->hypsometric_opt != 1) return fail("oracle: hypsometric_opt 1 only");
-    if (p->damp_opt != 0 && p->damp_opt != 3) return fail("oracle: damp_opt 0 or 3 only");
+    if (p->damp_opt < 0 || p->damp_opt > 3) return fail("oracle: damp_opt 0..3");
+    if ((p->damp_opt == 1 || p->damp_opt == 2) && g->nz >= kMaxLev) return fail("oracle: damp_opt 1, 2 need nz < 256");
     if (p->n_moist < 0 || p->n_moist > MI_MAX_MOIST || p->n_scalars < p->n_moist || p->n_scalars > MI_MAX_SCALARS)
       return fail("bad n_moist / n_scalars");
     if (p->diff_6th_opt < 0 || p->diff_6th_opt > 2) return fail("diff_6th_opt must be 0, 1 or 2");
@@ -2489,7 +2647,8 @@ int orc_set_threads(int n) {
 
 // the diffusion's 1-D base profiles (nz values)
 static std::vector<double>* base_profile(Dom& d, int id) {
-  return id == MI_U_BASE ? &d.u_base : id == MI_V_BASE ? &d.v_base : id == MI_QV_BASE ? &d.qv_base : nullptr;
+  return id == MI_U_BASE ? &d.u_base : id == MI_V_BASE ? &d.v_base : id == MI_QV_BASE ? &d.qv_base
+         : id == MI_T_BASE ? &d.t_base : id == MI_Z_BASE ? &d.z_base : nullptr;
 }
 
 int orc_set_field(orc_domain* od, int id, const double* h, int64_t n) {
@@ -2552,7 +2711,7 @@ int orc_set_diffusion(orc_domain* od, int opt, double factor) {
   if (opt < 0 || opt > 2) return fail("diff_6th_opt must be 0, 1 or 2");
   d.p.diff_6th_opt = opt;
   d.p.diff_6th_factor = factor;
-  d.held_base = opt != 0 || d.p.diff_opt != 0 || d.spec;
+  d.held_base = opt != 0 || d.p.diff_opt != 0 || d.p.damp_opt == 2 || d.spec;
   d.use_tendf = d.held_base || d.has_phys;
   return MI_OK;
 }

@@ -1,6 +1,7 @@
 // Compares the glibc pow/exp/sin ports of cppwrf_amd/csrc/libm_port.hpp (host build of the same source the
 // device compiles) with the host glibc on seeded samples over the ranges the dynamics path uses:
-// pow(x, cp/cv) for calc_p_rho_phi, pow(x, y) on a wide range, sin on [0, 2.4262] for the Rayleigh layer, and
+// pow(x, cp/cv) for calc_p_rho_phi, pow(x, y) on a wide range, sin on [0, 2.4262] for the Rayleigh layer, cos on
+// [0, pi/2] for the damping of the eddy viscosities, and
 // the Kessler microphysics' exp (saturation vapour pressure) and pow (fall speed, rates, pressure from Exner),
 // including pow(0, y) = 0.
 // Exit status 0 when every result is bit-identical.
@@ -25,7 +26,7 @@ static bool same(double a, double b) { return std::memcmp(&a, &b, sizeof a) == 0
 int main(int argc, char** argv) {
   const long n = argc > 1 ? std::atol(argv[1]) : 1000000;
   const double cpovcv = 1004.5 / (1004.5 - 287.0);
-  long bad[5] = {0, 0, 0, 0, 0};
+  long bad[6] = {0, 0, 0, 0, 0, 0};
   const double kexp[5] = {0.1364, 0.875, 1004.0 / 287.0, 0.2046, 0.525};
   for (long t = 0; t < n; ++t) {
     double x = 0.2 + 3.0 * u01();
@@ -40,6 +41,11 @@ int main(int argc, char** argv) {
     a = std::sin(z);
     b = mi_libm::sin(z);
     if (!same(a, b) && bad[2]++ < 3) std::printf("sin(%a): glibc %a port %a\n", z, a, b);
+    // cos over [0, pi/2] (the damping layer of the eddy viscosities), with tiny arguments now and then
+    const double zc = (t % 53 == 0) ? std::ldexp(u01(), -int(20 + splitmix64() % 20)) : 1.5707963267948966 * u01();
+    a = std::cos(zc);
+    b = mi_libm::cos(zc);
+    if (!same(a, b) && bad[5]++ < 3) std::printf("cos(%a): glibc %a port %a\n", zc, a, b);
     const double e = -60.0 + 70.0 * u01();
     a = std::exp(e);
     b = mi_libm::exp(e);
@@ -52,7 +58,7 @@ int main(int argc, char** argv) {
     b = mi_libm::pow(xk, ye);
     if (!same(a, b) && bad[4]++ < 3) std::printf("pow(%a, %a): glibc %a port %a\n", xk, ye, a, b);
   }
-  std::printf("samples %ld, mismatches: pow(x, cp/cv) %ld, pow(x, y) %ld, sin %ld, exp %ld, Kessler pow %ld\n", n,
-              bad[0], bad[1], bad[2], bad[3], bad[4]);
-  return bad[0] + bad[1] + bad[2] + bad[3] + bad[4] != 0;
+  std::printf("samples %ld, mismatches: pow(x, cp/cv) %ld, pow(x, y) %ld, sin %ld, exp %ld, Kessler pow %ld, cos %ld\n",
+              n, bad[0], bad[1], bad[2], bad[3], bad[4], bad[5]);
+  return bad[0] + bad[1] + bad[2] + bad[3] + bad[4] + bad[5] != 0;
 }

@@ -103,7 +103,7 @@ class OracleDomain:
             if case.moist:
                 a = case.field(A.MI_Q)
                 _ok(olib().orc_set_field(h, A.MI_Q, _ptr(a), a.size))
-        for fid in (A.MI_U_BASE, A.MI_V_BASE, A.MI_QV_BASE):  # the diffusion's base profiles
+        for fid in (A.MI_U_BASE, A.MI_V_BASE, A.MI_QV_BASE, A.MI_T_BASE, A.MI_Z_BASE):  # the 1-D base profiles
             prof = case.field(fid).reshape(-1)
             _ok(olib().orc_set_field(h, fid, _ptr(prof), prof.size))
         _ok(olib().orc_set_libmvec(h, 1 if libmvec else 0))
