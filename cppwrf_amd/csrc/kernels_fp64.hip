@@ -52,52 +52,27 @@ struct Range {
   int kfast;  // the block order of a point kernel (below)
 };
 
-// The point kernels' blocks (64 i x 4 j at one level), in one of three orders within an XCD's share:
-//  kfast = 0: i, j, level (xcd_block's own);
-//  kfast = 1: level-fastest: the n-th block is level n % gz of tile n / gz (tiles i-fastest), so the levels of one
-//    tile run back to back and the k +- 1, k +- 2 rows of a vertical stencil are still in that XCD's L2 (a whole
-//    level-plane of the arrays a stencil kernel reads outgrows the 4 MB L2 at 425x300);
-//  kfast = G >= 2: groups of G tile rows (one tile column), the rows of a group fastest, then the levels, then the
-//    groups (i-fastest): the j +- 3 halo rows of a horizontal stencil are read while the neighbouring rows are hot,
-//    and a vertical stencil's levels are G blocks apart.
+// The point kernels' blocks (64 i x 4 j at one level) in level-fastest order: the n-th block of an XCD's share is
+// level n % gz of tile n / gz (tiles i-fastest), so the levels of one tile run back to back and the k +- 1 and
+// k +- 2 rows of a vertical stencil are still in that XCD's L2 (a whole level-plane of the arrays a stencil
+// kernel reads outgrows the 4 MB L2 at 425x300).  kfast = 0 keeps the i, j, level order of xcd_block.  (Groups of
+// 4 to 32 tile rows walked level by level, which keep a horizontal stencil's halo rows hot, were measured 2-5 %
+// slower at 425x300x50.)
 __device__ __forceinline__ Blk point_block(int kfast) {
   Blk b = xcd_block();
   if (!kfast) return b;
   const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
   const unsigned l = unsigned(b.x) + gx * (unsigned(b.y) + gy * unsigned(b.z));
-  if (kfast == 1) {
-    b.z = int(l % gz);
-    b.x = int((l / gz) % gx);
-    b.y = int(l / (gz * gx));
-    return b;
-  }
-  const unsigned G = unsigned(kfast), ngy = (gy + G - 1) / G, full = gx * (ngy - 1) * G * gz;
-  unsigned grp, s, gs;
-  if (l < full) {  // the groups of G rows
-    gs = G;
-    grp = l / (G * gz);
-    s = l % (G * gz);
-  } else {  // the last row of groups holds the remaining gy - (ngy - 1) G rows
-    gs = gy - (ngy - 1) * G;
-    grp = gx * (ngy - 1) + (l - full) / (gs * gz);
-    s = (l - full) % (gs * gz);
-  }
-  b.x = int(grp % gx);
-  b.y = int((grp / gx) * G + s % gs);
-  b.z = int(s / gs);
+  b.z = int(l % gz);
+  b.x = int((l / gz) % gx);
+  b.y = int(l / (gz * gx));
   return b;
 }
-// the order point kernels launch with: level-fastest by default, MI_BLOCK_ORDER=ijk the i, j, level order,
-// MI_BLOCK_ORDER=group:G the groups of G tile rows (for timing them)
+// the order point kernels launch with: level-fastest unless MI_BLOCK_ORDER=ijk (for timing the two)
 int point_order() {
   static const int o = [] {
     const char* e = std::getenv("MI_BLOCK_ORDER");
-    if (e && std::strcmp(e, "ijk") == 0) return 0;
-    if (e && std::strncmp(e, "group:", 6) == 0) {
-      const int g = std::atoi(e + 6);
-      return g >= 2 && g <= 256 ? g : 1;
-    }
-    return 1;
+    return (e && std::strcmp(e, "ijk") == 0) ? 0 : 1;
   }();
   return o;
 }
