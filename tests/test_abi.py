@@ -79,3 +79,22 @@ def test_errors_are_reported():
     assert L.mi_case_create(C.byref(bad), C.byref(h)) == A.MI_EINVAL
     assert b"unknown kind" in L.mi_last_error()
     assert L.mi_case_create(None, C.byref(h)) == A.MI_EINVAL
+
+
+def test_header_constants_and_params_match_python():
+    """every `MI_NAME = value` of the header that the Python mirror names has the same value, and mi_params'
+    fields come in the header's order with its types (ctypes lays the struct out as the C compiler does)"""
+    src = open(os.path.join(ROOT, "include", "cppwrf_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    consts = dict((n, int(v)) for n, v in re.findall(r"\b(MI_[A-Z0-9_]+)\s*=\s*(-?\d+)", src))
+    shared = [n for n in consts if hasattr(A, n)]
+    assert len(shared) >= 40
+    for n in shared:
+        assert getattr(A, n) == consts[n], n
+    body = re.search(r"typedef struct mi_params \{(.*?)\} mi_params;", src, flags=re.S).group(1)
+    fields = []
+    for typ, names in re.findall(r"(int32_t|double)\s+([^;]+);", body):
+        for nm in names.split(","):
+            fields.append((nm.strip(), typ))
+    py = [(n, "int32_t" if t is C.c_int32 else "double") for n, t in A.MiParams._fields_]
+    assert fields == py
