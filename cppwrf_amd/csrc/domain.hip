@@ -289,7 +289,7 @@ struct Domain final : mi_domain {
     if (rk == 1) launch_fill2d(a, f.mudf - off, n2, 0.0, st);
     launch_ss_prep(a, rk == 1, st);
     const double cof_h = 0.5 * dts * kG * (1.0 + prm.epssm);
-    launch_ss_column(a, colcfg[kColSsColumn], cof_h * cof_h, st);
+    launch_ss_column(a, colcfg[kColSsColumn], cof_h * cof_h, dts, st);
     // the halos of u'', v'', ph'', t_save, u_save, v_save, mu'', muts, p, al (small_step.cpp:517-522), in
     // one batch after the two kernels: the column kernel reads its inputs in its own column only.  With
     // specified boundaries no substep kernel reads them (as the substep refreshes below)
@@ -976,6 +976,7 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     add3(f.xkmh, St::M, MI_XKMH); add3(f.xkhh, St::M, MI_XKHH);
     if (p.km_opt == 4) add3(f.d12, St::U, -1);
   }
+  if (p.damp_opt == 3) add3(f.dampwt, St::W, -1);  // the stage's damping weight of advance_w
   f.t_base = d.alloc(nz + 1);
   f.z_base = d.alloc(nz + 1);
   f.u_base = d.alloc(nz + 1);

@@ -1130,7 +1130,39 @@ __device__ __forceinline__ void p_rho_at(const Args& a, int i, int j, int k, dou
 
 // KPRho (small_step.cpp:116-121) and KCoefW calc_coef_w (small_step.cpp:124-148), dry, one column
 // group: p'', al'' and the b, c, a coefficients in parallel, then the alpha / gamma recurrence
-MI_COLUMN_KERNEL k_ss_column(Args a, double cof) {
+// a / b in the substep kernels: the IEEE division in FP64 (the reference's bits); in FP32-bulk a multiply by the
+// hardware reciprocal of b refined by one Newton step (a few FP64 ulps, far below the float storage's rounding),
+// which replaces the division's ten-instruction sequence
+__device__ __forceinline__ double bdiv(double a, double b) {
+  if constexpr (sizeof(real) == 8) {
+    return a / b;
+  } else {
+    const double r = __builtin_amdgcn_rcp(b);
+    return a * __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+  }
+}
+// The weight of advance_w's implicit Rayleigh damping (damp_opt 3, reference small_step.cpp:393-405) at w level
+// k >= 1 of column (i, j): dampmag sin^2(pi/2 (h_k - h_bot) / zdamp) above h_bot = h_top - zdamp, 0 below, the
+// heights from the stage's saved geopotential.  It does not change within a stage, so the stage's column kernel
+// evaluates it once and the substeps read it.
+__device__ __forceinline__ double damp_weight(const Args& a, int i, int j, int k, double dts) {
+  const Fields& f = a.f;
+  const Geo& g = a.g;
+  const double dampmag = dts * a.q.dampcoef;
+  const double hdepth = a.q.zdamp;
+  const double pi = 3.141592653589793;  // 4 atan(1), folded
+  const real ph_top = f.ph_save[g.at(i, j, g.nz)], phs = f.ph_save[g.at(i, j, k)];
+  const double htop = bdiv(ph_top + f.phb(i, j, g.nz), kG), hbot = htop - hdepth;
+  const double hk = bdiv(phs + f.phb(i, j, k), kG);
+  double dampwt = 0;
+  if (hk >= hbot) {
+    const double sn = mi_libm::sin(bdiv(H * pi * (hk - hbot), hdepth));
+    dampwt = dampmag * sn * sn;
+  }
+  return dampwt;
+}
+
+MI_COLUMN_KERNEL k_ss_column(Args a, double cof, double dts) {
   constexpr int kKT = KT;
   extern __shared__ double lds[];
   const Geo& g = a.g;
@@ -1161,6 +1193,7 @@ MI_COLUMN_KERNEL k_ss_column(Args a, double cof) {
       else ak = -cq1 * cof * c.rdn[k] * c.rdnw[k - 1] * c2a(k - 1) / (mh(k - 1) * mf(k - 1));
       f.ca[x] = ak;
       LDS(AA, k, cp.ci) = ak;
+      if (a.q.damp_opt == 3 && k >= 1) f.dampwt[x] = damp_weight(a, i, j, k, dts);
       if (k >= 1 && k < nz) {
         LDS(B, k, cp.ci) = 1.0 + cq1 * cof * c.rdn[k] * (c.rdnw[k] * c2a(k) / (mh(k) * mf(k)) + c.rdnw[k - 1] * c2a(k - 1) / (mh(k - 1) * mf(k)));
         LDS(CC, k, cp.ci) = -cq1 * cof * c.rdn[k] * c.rdnw[k] * c2a(k) / (mh(k) * mf(k + 1));
@@ -1313,17 +1346,6 @@ MI_POINT_KERNEL k_uv(Range r, Args a, double dts) {
 // sweeps become compositions of the affine maps y_k = p_k + q_k y_(k-1): the same solution in another rounding,
 // within FP32-bulk's error bar.  FP64 keeps the reference's serial order (bit-identical).
 constexpr bool kScanChains = sizeof(real) == 4;
-// a / b in the substep kernel: the IEEE division in FP64 (the reference's bits); in FP32-bulk a multiply by the
-// hardware reciprocal of b refined by one Newton step (a few FP64 ulps, far below the float storage's rounding),
-// which replaces the division's ten-instruction sequence
-__device__ __forceinline__ double bdiv(double a, double b) {
-  if constexpr (sizeof(real) == 8) {
-    return a / b;
-  } else {
-    const double r = __builtin_amdgcn_rcp(b);
-    return a * __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
-  }
-}
 // inclusive prefix sum over the lanes of a wavefront
 __device__ __forceinline__ double wave_prefix_sum(double v, int lane) {
 #pragma unroll
@@ -1408,7 +1430,7 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
   real c2a = 0, c2a_m = 0, alt = 0, alt_m = 0, wsv = 0, pm1 = 0;
   double msfty = 1, msftx = 1, mut = 0, muu0 = 0, muu1 = 0, muv0 = 0, muv1 = 0, msfuy0 = 1, msfuy1 = 1, mvi0 = 1, mvi1 = 1;
   double mu_old = 0, mtend = 0;
-  real ph_top = 0;
+  real dwt = 0;
   if (on) {
     msfty = f.msfty[x2];
     msftx = f.msftx[x2];
@@ -1436,14 +1458,16 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
     w2 = f.w2[x];
     phs = f.ph_save[x];
     ph = f.ph2[x];
-    ph_top = f.ph_save[g.at(i, j, nz)];
     if (ge1) {
       phs_m = f.ph_save[x - sk];
       ph_m = f.ph2[x - sk];
       rwt = f.rw_tend[x];
       c2a_m = f.c2a[x - sk];
       alt_m = f.alt[x - sk];
-      if (a.q.damp_opt == 3) wsv = f.w_save[x];
+      if (a.q.damp_opt == 3) {
+        wsv = f.w_save[x];
+        dwt = f.dampwt[x];
+      }
     }
     if (mk) {
       u0 = f.u2[x];
@@ -1677,16 +1701,8 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
     double phk = ph;  // the surface phi'' is not solved for
     if (k >= 1) {
       const double dampmag = a.q.damp_opt == 3 ? dts * a.q.dampcoef : 0.0;
-      if (dampmag != 0.0) {
-        const double hdepth = a.q.zdamp;
-        const double pi = 3.141592653589793;  // 4 atan(1), folded
-        const double htop = bdiv(ph_top + f.phb(i, j, nz), kG), hbot = htop - hdepth;
-        const double hk = bdiv(phs + f.phb(i, j, k), kG);
-        double dampwt = 0;
-        if (hk >= hbot) {
-          const double sn = mi_libm::sin(bdiv(H * pi * (hk - hbot), hdepth));
-          dampwt = dampmag * sn * sn;
-        }
+      if (dampmag != 0.0) {  // the stage's weight (damp_weight)
+        const double dampwt = dwt;
         wk = bdiv(wk - dampwt * (c.c1f[k] * mut + c.c2f[k]) * wsv, 1.0 + dampwt);
       }
       phk = LDS(R, k, ci) + bdiv(msfty * H * dts * kG * (1.0 + epssm) * wk, c.c1f[k] * muts + c.c2f[k]);
@@ -1936,9 +1952,6 @@ __global__ void __launch_bounds__(64) k_acoustic_march(Args a, double dts, int i
   }
   // ---------------------------------------------------------------- pass 3: back-substitution, damping, phi'', p'', al''
   const double dampmag = a.q.damp_opt == 3 ? dts * a.q.dampcoef : 0.0;
-  const double hdepth = a.q.zdamp;
-  const double pi = 3.141592653589793;  // 4 atan(1), folded
-  const double htop = (f.ph_save[g.at(i, j, nz)] + f.phb(i, j, nz)) / kG, hbot = htop - hdepth;
   double w_next = 0;
   double ph_up = 0;  // phi'' at w level k + 1
   for (int k = nz; k >= 0; --k) {
@@ -1948,13 +1961,8 @@ __global__ void __launch_bounds__(64) k_acoustic_march(Args a, double dts, int i
     w_next = wk;  // the back-substitution runs on the undamped values (small_step.cpp:387 before 393-405)
     double phk;
     if (k >= 1) {
-      if (dampmag != 0.0) {
-        const double hk = (f.ph_save[x] + f.phb(i, j, k)) / kG;
-        double dampwt = 0;
-        if (hk >= hbot) {
-          const double sn = mi_libm::sin(H * pi * (hk - hbot) / hdepth);
-          dampwt = dampmag * sn * sn;
-        }
+      if (dampmag != 0.0) {  // the stage's weight (damp_weight)
+        const double dampwt = f.dampwt[x];
         wk = (wk - dampwt * (c.c1f[k] * mut + c.c2f[k]) * f.w_save[x]) / (1.0 + dampwt);
       }
       phk = f.kr[x] + msfty * H * dts * kG * (1.0 + epssm) * wk / (c.c1f[k] * muts + c.c2f[k]);
@@ -3740,8 +3748,8 @@ void launch_ss_prep(const Args& a, bool first, hipStream_t st) {
   const Geo& g = a.g;
   launch3("ss_prep", k_ss_prep, Range{0, g.nx + 1, 0, g.nyv, 0, g.nz + 1}, st, a, first ? 1 : 0);
 }
-void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st) {
-  dispatch_cfg<SsColumnL>(cfg, "ss_column", 3, 0, a.g.nx, a.g.ny, st, a, cof);
+void launch_ss_column(const Args& a, int cfg, double cof, double dts, hipStream_t st) {
+  dispatch_cfg<SsColumnL>(cfg, "ss_column", 3, 0, a.g.nx, a.g.ny, st, a, cof, dts);
 }
 void launch_uv(const Args& a, double dts, hipStream_t st) {
   const Geo& g = a.g;

@@ -69,6 +69,8 @@ struct Fields {
   // the scalars) and the deformation defor12 at the vorticity points ((nx+1) x (ny+1), u-staggered storage whose
   // row ny lies in the y halo)
   real *xkmh, *xkhh, *d12;
+  // damp_opt 3: the implicit Rayleigh damping's weight at the w levels, set once per stage for its substeps
+  real* dampwt;
   // damp_opt 2: the base profiles of theta and of the mass levels' height (nz values)
   double *t_base, *z_base;
   // the microphysics after the RK loop: moist_physics_prep's fields (dry theta, density, Exner function, height
@@ -183,7 +185,7 @@ int col_cfg_default(int kernel, int nx, int ny, int nz);
 void launch_large_prep(const Args& a, int cfg, hipStream_t st);
 void launch_tendencies(const Args& a, hipStream_t st);
 void launch_ss_prep(const Args& a, bool first, hipStream_t st);
-void launch_ss_column(const Args& a, int cfg, double cof, hipStream_t st);
+void launch_ss_column(const Args& a, int cfg, double cof, double dts, hipStream_t st);
 void launch_uv(const Args& a, double dts, hipStream_t st);
 void launch_acoustic(const Args& a, int cfg, double dts, int it, int ns, hipStream_t st);
 void launch_open_sums(const Args& a, int it, int ns, hipStream_t st);
