@@ -976,7 +976,10 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
     add3(f.xkmh, St::M, MI_XKMH); add3(f.xkhh, St::M, MI_XKHH);
     if (p.km_opt == 4) add3(f.d12, St::U, -1);
   }
-  if (p.damp_opt == 3) add3(f.dampwt, St::W, -1);  // the stage's damping weight of advance_w
+  // what the substeps of a stage read unchanged: advance_w's damping weight, the coupled saves, its coefficients
+  if (p.damp_opt == 3) add3(f.dampwt, St::W, -1);
+  add3(f.ucs, St::U, -1); add3(f.vcs, St::V, -1); add3(f.cwc, St::M, -1);
+  if (p.n_moist > 0) add3(f.cq1, St::W, -1);
   f.t_base = d.alloc(nz + 1);
   f.z_base = d.alloc(nz + 1);
   f.u_base = d.alloc(nz + 1);

@@ -135,12 +135,12 @@ double point_bytes(const char* name) {
       {"tend_w", 8, 0},       // w ru rv ww p u v -> rw_tend
       {"tend_t", 5, 0},       // t ru rv ww -> t_tend
       {"tend_ph", 6, 0},      // ph ww w u v -> ph_tend
-      {"ss_prep", 25, 0},     // u v w t ph (_1 and _2) p alt ww -> u v w t ph, their 5 saves, c2a ww1
-      {"ss_column", 8, 3},    // ph alt t t_save c2a -> al p pm1, a alpha gamma
+      {"ss_prep", 27, 0},     // u v w t ph (_1 and _2) p alt ww -> u v w t ph, their 5 saves, c2a ww1, ucs vcs
+      {"ss_column", 11, 3},   // ph alt t t_save c2a -> al p pm1, the damping weight, cwc cq1, a alpha gamma
       {"uv", 10, 1},          // u v ru_tend rv_tend p al alt ph php -> u v
-      // u v u_save v_save t_save t_tend t ww1 ww_m ru_m rv_m ph_tend w ph_save ph rw_tend c2a alt w_save pm1 cqw,
-      // a alpha gamma -> ww ww_m ru_m rv_m t w ph al p pm1
-      {"acoustic", 31, 3},
+      // u v ucs vcs t_save t_tend t ww1 ww_m ru_m rv_m ph_tend w ph_save ph rw_tend c2a alt w_save pm1 cq1 cwc
+      // and the damping weight, a alpha gamma -> ww ww_m ru_m rv_m t w ph al p pm1
+      {"acoustic", 33, 3},
       {"finish", 20, 0},      // u v w t ph ww and the six saves -> u v w t ph ww, al p
       {"p_rho_phi", 4, 0},    // ph t -> al p
   };
@@ -1058,6 +1058,38 @@ MI_POINT_KERNEL k_tend_ph(Range r, Args a) {
   f.ph_tend[x] = t;
 }
 
+// a / b in the substep kernels: the IEEE division in FP64 (the reference's bits); in FP32-bulk a multiply by the
+// hardware reciprocal of b refined by one Newton step (a few FP64 ulps, far below the float storage's rounding),
+// which replaces the division's ten-instruction sequence
+__device__ __forceinline__ double bdiv(double a, double b) {
+  if constexpr (sizeof(real) == 8) {
+    return a / b;
+  } else {
+    const double r = __builtin_amdgcn_rcp(b);
+    return a * __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+  }
+}
+// The weight of advance_w's implicit Rayleigh damping (damp_opt 3, reference small_step.cpp:393-405) at w level
+// k >= 1 of column (i, j): dampmag sin^2(pi/2 (h_k - h_bot) / zdamp) above h_bot = h_top - zdamp, 0 below, the
+// heights from the stage's saved geopotential.  It does not change within a stage, so the stage's column kernel
+// evaluates it once and the substeps read it.
+__device__ __forceinline__ double damp_weight(const Args& a, int i, int j, int k, double dts) {
+  const Fields& f = a.f;
+  const Geo& g = a.g;
+  const double dampmag = dts * a.q.dampcoef;
+  const double hdepth = a.q.zdamp;
+  const double pi = 3.141592653589793;  // 4 atan(1), folded
+  const real ph_top = f.ph_save[g.at(i, j, g.nz)], phs = f.ph_save[g.at(i, j, k)];
+  const double htop = bdiv(ph_top + f.phb(i, j, g.nz), kG), hbot = htop - hdepth;
+  const double hk = bdiv(phs + f.phb(i, j, k), kG);
+  double dampwt = 0;
+  if (hk >= hbot) {
+    const double sn = mi_libm::sin(bdiv(H * pi * (hk - hbot), hdepth));
+    dampwt = dampmag * sn * sn;
+  }
+  return dampwt;
+}
+
 // ----------------------------------------------------------------------------- small_step_prep
 // KPrep (reference src/dyn/small_step.cpp:39-93)
 MI_POINT_KERNEL k_ss_prep(Range r, Args a, int first) {
@@ -1083,11 +1115,14 @@ MI_POINT_KERNEL k_ss_prep(Range r, Args a, int first) {
   if (j < ny && k < nz) {
     const double ui = f.u2[x];
     put<kHU>(f.u_save, g, i, j, k * g.sk, ui);
+    // the substeps' coupled save (mass x u_save / msfuy), constant within the stage
+    f.ucs[x] = bdiv((c.c1h[k] * f.muu[x2] + c.c2h[k]) * double(f.u_save[x]), f.msfuy[x2]);
     put<kHU>(f.u2, g, i, j, k * g.sk, ((c.c1h[k] * mus_u + c.c2h[k]) * f.u1[x] - (c.c1h[k] * f.muu[x2] + c.c2h[k]) * ui) / f.msfuy[x2]);
   }
   if (i < nx && j < g.nyv && k < nz) {
     const double vi = f.v2[x];
     put<kHV>(f.v_save, g, i, j, k * g.sk, vi);
+    f.vcs[x] = (c.c1h[k] * f.muv[x2] + c.c2h[k]) * double(f.v_save[x]) * f.msfvx_inv[x2];
     put<kHV>(f.v2, g, i, j, k * g.sk, ((c.c1h[k] * mus_v + c.c2h[k]) * f.v1[x] - (c.c1h[k] * f.muv[x2] + c.c2h[k]) * vi) * f.msfvx_inv[x2]);
   }
   if (i < nx && j < ny) {
@@ -1130,38 +1165,6 @@ __device__ __forceinline__ void p_rho_at(const Args& a, int i, int j, int k, dou
 
 // KPRho (small_step.cpp:116-121) and KCoefW calc_coef_w (small_step.cpp:124-148), dry, one column
 // group: p'', al'' and the b, c, a coefficients in parallel, then the alpha / gamma recurrence
-// a / b in the substep kernels: the IEEE division in FP64 (the reference's bits); in FP32-bulk a multiply by the
-// hardware reciprocal of b refined by one Newton step (a few FP64 ulps, far below the float storage's rounding),
-// which replaces the division's ten-instruction sequence
-__device__ __forceinline__ double bdiv(double a, double b) {
-  if constexpr (sizeof(real) == 8) {
-    return a / b;
-  } else {
-    const double r = __builtin_amdgcn_rcp(b);
-    return a * __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
-  }
-}
-// The weight of advance_w's implicit Rayleigh damping (damp_opt 3, reference small_step.cpp:393-405) at w level
-// k >= 1 of column (i, j): dampmag sin^2(pi/2 (h_k - h_bot) / zdamp) above h_bot = h_top - zdamp, 0 below, the
-// heights from the stage's saved geopotential.  It does not change within a stage, so the stage's column kernel
-// evaluates it once and the substeps read it.
-__device__ __forceinline__ double damp_weight(const Args& a, int i, int j, int k, double dts) {
-  const Fields& f = a.f;
-  const Geo& g = a.g;
-  const double dampmag = dts * a.q.dampcoef;
-  const double hdepth = a.q.zdamp;
-  const double pi = 3.141592653589793;  // 4 atan(1), folded
-  const real ph_top = f.ph_save[g.at(i, j, g.nz)], phs = f.ph_save[g.at(i, j, k)];
-  const double htop = bdiv(ph_top + f.phb(i, j, g.nz), kG), hbot = htop - hdepth;
-  const double hk = bdiv(phs + f.phb(i, j, k), kG);
-  double dampwt = 0;
-  if (hk >= hbot) {
-    const double sn = mi_libm::sin(bdiv(H * pi * (hk - hbot), hdepth));
-    dampwt = dampmag * sn * sn;
-  }
-  return dampwt;
-}
-
 MI_COLUMN_KERNEL k_ss_column(Args a, double cof, double dts) {
   constexpr int kKT = KT;
   extern __shared__ double lds[];
@@ -1187,6 +1190,9 @@ MI_COLUMN_KERNEL k_ss_column(Args a, double cof, double dts) {
       if (k < nz) p_rho_at(a, i, j, k, f.ph2[x], f.ph2[x + g.sk], mu2, muts, f.t2[x], false);
       // kern::MoistCoef::cq1 = 1 / (1 + cqw), 1 when dry
       const double cq1 = (a.q.moist && k >= 1 && k < nz) ? 1.0 / (1.0 + f.cqw[x]) : 1.0;
+      // for the substeps of the stage: cq1 and advance_w's c2a rdnw / (c1h mut + c2h) at the mass levels
+      if (a.q.moist && k >= 1 && k < nz) f.cq1[x] = cq1;
+      if (k < nz) f.cwc[x] = bdiv(double(f.c2a[x]) * c.rdnw[k], c.c1h[k] * m + c.c2h[k]);
       double ak;
       if (k <= 1) ak = 0;
       else if (k == nz) ak = -2.0 * cof * (c.rdnw[nz - 1] * c.rdnw[nz - 1]) * c2a(nz - 1) * lid / (mh(nz - 1) * mf(nz - 1));
@@ -1422,29 +1428,21 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
   // ---------------------------------------------------------------- the load burst
   // the values of the 3-D bulk fields stay in their storage type (half the registers in FP32-bulk); every
   // expression below converts them to double before it combines two of them
-  real u0 = 0, u1 = 0, us0 = 0, us1 = 0, v0 = 0, v1 = 0, vs0 = 0, vs1 = 0;
+  // ucs, vcs: the stage's coupled saves (mass x u_save / msfuy, mass x v_save / msfvx) of the two faces
+  real u0 = 0, u1 = 0, ucs0 = 0, ucs1 = 0, v0 = 0, v1 = 0, vcs0 = 0, vcs1 = 0;
   real ww1 = 0, wwm = 0, rum = 0, rvm = 0;
   double ca = 0, cal = 0, cga = 0;
   real t2 = 0, ttend = 0, ts = 0, tsxp = 0, tsxm = 0, tsyp = 0, tsym = 0, tskm = 0, tskp = 0;
   real phtend = 0, w2 = 0, phs = 0, phs_m = 0, phs_p = 0, ph = 0, ph_m = 0, ph_p = 0, rwt = 0;
   real c2a = 0, c2a_m = 0, alt = 0, alt_m = 0, wsv = 0, pm1 = 0;
-  double msfty = 1, msftx = 1, mut = 0, muu0 = 0, muu1 = 0, muv0 = 0, muv1 = 0, msfuy0 = 1, msfuy1 = 1, mvi0 = 1, mvi1 = 1;
+  real cw = 0, cw_m = 0, cq1 = 1;  // the stage's c2a rdnw / (c1h mut + c2h) at k and k - 1, 1 / (1 + cqw)
+  double msfty = 1, msftx = 1, mut = 0;
   double mu_old = 0, mtend = 0;
   real dwt = 0;
   if (on) {
     msfty = f.msfty[x2];
     msftx = f.msftx[x2];
     mut = f.mut[x2];
-    muu0 = f.muu[x2];
-    muu1 = f.muu[x2 + 1];
-    msfuy0 = f.msfuy[x2];
-    msfuy1 = f.msfuy[x2 + 1];
-    if (!g.two_d) {
-      muv0 = f.muv[x2];
-      muv1 = f.muv[x2 + sj];
-      mvi0 = f.msfvx_inv[x2];
-      mvi1 = f.msfvx_inv[x2 + sj];
-    }
     if (k == 0) {
       mu_old = f.mu2[x2];
       mtend = f.mu_tend[x2];
@@ -1464,6 +1462,8 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
       rwt = f.rw_tend[x];
       c2a_m = f.c2a[x - sk];
       alt_m = f.alt[x - sk];
+      cw_m = f.cwc[x - sk];
+      if (mk && a.q.moist) cq1 = f.cq1[x];
       if (a.q.damp_opt == 3) {
         wsv = f.w_save[x];
         dwt = f.dampwt[x];
@@ -1472,13 +1472,14 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
     if (mk) {
       u0 = f.u2[x];
       u1 = f.u2[x + 1];
-      us0 = f.u_save[x];
-      us1 = f.u_save[x + 1];
+      ucs0 = f.ucs[x];
+      ucs1 = f.ucs[x + 1];
+      cw = f.cwc[x];
       if (!g.two_d) {
         v0 = f.v2[x];
         v1 = f.v2[x + sj];
-        vs0 = f.v_save[x];
-        vs1 = f.v_save[x + sj];
+        vcs0 = f.vcs[x];
+        vcs1 = f.vcs[x + sj];
       }
       if (!first) {
         rum = f.ru_m[x];
@@ -1508,9 +1509,9 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
     LDS(AL, k, ci) = cal;
     LDS(GA, k, ci) = cga;
     if (mk) {
-      const double fx = (u1 + bdiv((c.c1h[k] * muu1 + c.c2h[k]) * us1, msfuy1)) - (u0 + bdiv((c.c1h[k] * muu0 + c.c2h[k]) * us0, msfuy0));
+      const double fx = (u1 + double(ucs1)) - (u0 + double(ucs0));
       double d = c.rdx * fx;
-      if (!g.two_d) d += c.rdy * ((v1 + (c.c1h[k] * muv1 + c.c2h[k]) * vs1 * mvi1) - (v0 + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0));
+      if (!g.two_d) d += c.rdy * ((v1 + double(vcs1)) - (v0 + double(vcs0)));
       dv = msftx * msfty * d;
       LDS(DV, k, ci) = c.dnw[k] * dv;
     }
@@ -1589,10 +1590,10 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
       const double wdtn_k = (k == 0) ? 0.0 : wwk * (c.fnm[k] * ts + c.fnp[k] * tskm);
       const double wdtn_kp = (k + 1 == nz) ? 0.0 : LDS(WW, k + 1, ci) * (c.fnm[k + 1] * tskp + c.fnp[k + 1] * ts);
       const double su = (first ? 0.0 : rum) + u0;
-      put<kHU>(f.ru_m, g, i, j, k * sk, last ? bdiv(su, rn) + bdiv((c.c1h[k] * muu0 + c.c2h[k]) * us0, msfuy0) : su);
+      put<kHU>(f.ru_m, g, i, j, k * sk, last ? bdiv(su, rn) + double(ucs0) : su);
       if (!g.two_d) {
         const double sv = (first ? 0.0 : rvm) + v0;
-        put<kHV>(f.rv_m, g, i, j, k * sk, last ? bdiv(sv, rn) + (c.c1h[k] * muv0 + c.c2h[k]) * vs0 * mvi0 : sv);
+        put<kHV>(f.rv_m, g, i, j, k * sk, last ? bdiv(sv, rn) + double(vcs0) : sv);
       }
       double hflux = H * c.rdx * (double(u1) * (double(tsxp) + ts) - double(u0) * (double(ts) + tsxm));
       if (!g.two_d) hflux += H * c.rdy * (double(v1) * (double(tsyp) + ts) - double(v0) * (double(ts) + tsym));
@@ -1646,10 +1647,10 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
       if (mk) {
         const double r_kp1 = LDS(R, k + 1, ci);
         wk = w2 + dts * rwt +
-             msft_inv * (a.q.moist ? bdiv(1.0, 1.0 + f.cqw[x]) : 1.0) *
+             msft_inv * (a.q.moist ? double(cq1) : 1.0) *
                  (H * dts * kG * c.rdn[k] *
-                  (bdiv(c2a * c.rdnw[k], c.c1h[k] * m + c.c2h[k]) * (ope * (r_kp1 - r_k) + ome * (double(ph_p) - ph)) -
-                   bdiv(c2a_m * c.rdnw[k - 1], c.c1h[k - 1] * m + c.c2h[k - 1]) * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)))) +
+                  (double(cw) * (ope * (r_kp1 - r_k) + ome * (double(ph_p) - ph)) -
+                   double(cw_m) * (ope * (r_k - r_km1) + ome * (double(ph) - ph_m)))) +
              dts * kG * msft_inv * (c.rdn[k] * (double(c2a) * alt * LDS(T2A, k, ci) - double(c2a_m) * alt_m * LDS(T2A, k - 1, ci)) - c.c1f[k] * mav);
       } else {
         const int km = nz - 1;

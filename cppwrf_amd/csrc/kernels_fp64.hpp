@@ -69,8 +69,10 @@ struct Fields {
   // the scalars) and the deformation defor12 at the vorticity points ((nx+1) x (ny+1), u-staggered storage whose
   // row ny lies in the y halo)
   real *xkmh, *xkhh, *d12;
-  // damp_opt 3: the implicit Rayleigh damping's weight at the w levels, set once per stage for its substeps
-  real* dampwt;
+  // what the acoustic substeps of a stage read unchanged, set by the stage's kernels: damp_opt 3's weight at the w
+  // levels; the coupled saves (c1h mu + c2h) u_save / msfuy and (c1h mu + c2h) v_save / msfvx; advance_w's
+  // c2a rdnw / (c1h mut + c2h) at the mass levels; 1 / (1 + cqw) with moisture
+  real *dampwt, *ucs, *vcs, *cwc, *cq1;
   // damp_opt 2: the base profiles of theta and of the mass levels' height (nz values)
   double *t_base, *z_base;
   // the microphysics after the RK loop: moist_physics_prep's fields (dry theta, density, Exner function, height
