@@ -186,7 +186,7 @@ def domain_bench(ctx, name, steps, warmup, autotune, error_hours):
     return out, doms["fp32"]
 
 
-def roofline(rk, steps, profile_name):
+def roofline(rk, steps, profile_name, workload="c3 (425x300x50)"):
     """the dominant kernel of the FP32-bulk step: event time per step of each candidate over two steps, then its
     average launch over `steps` steps against the algorithmic bytes of a launch"""
     per_kernel = {}
@@ -203,7 +203,7 @@ def roofline(rk, steps, profile_name):
     return {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
             "traffic_source": traffic, "alg_bytes_per_launch": b / nl, "avg_launch_us": avg_launch_s * 1e6,
-            "launches": nl, "precision": "fp32-bulk", "workload": "c3 (425x300x50)",
+            "launches": nl, "precision": "fp32-bulk", "workload": workload,
             "kernel_ms_per_step": {k: round(v, 4) for k, v in per_kernel.items()}}
 
 
@@ -297,6 +297,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched ensemble (for profiling runs: the headline then falls back to c3)")
     ap.add_argument("--no-c2", action="store_true", help="skip the 119x89x44 domain")
+    ap.add_argument("--domain", choices=("c3", "c2"), default="c3",
+                    help="with --no-ensemble: the single domain to time (c3 425x300x50, c2 119x89x44)")
     ap.add_argument("--no-c3", action="store_true", help="skip the 425x300x50 domain (and the roofline)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving trace")
     ap.add_argument("--serving-seconds", type=float, default=20.0, help="length of the serving trace")
@@ -311,12 +313,15 @@ def main():
     from cppwrf_amd.dyn import Context
     ctx = Context(0)
     if args.no_ensemble:
-        c3, d32 = domain_bench(ctx, "c3", args.steps, args.warmup, not args.no_autotune, 0)
-        out = {"metric": METRIC, "value": c3["fp32"]["value"], "unit": "grid-point-steps/s", "n_gpus": 1,
-               "steps": args.steps, "warmup": args.warmup, "ms_per_step": c3["fp32"]["ms_per_step"],
+        name = args.domain
+        grid = "425x300x50" if name == "c3" else "119x89x44"
+        one, d32 = domain_bench(ctx, name, args.steps, args.warmup, not args.no_autotune,
+                                0 if name == "c3" else args.error_hours)
+        out = {"metric": METRIC, "value": one["fp32"]["value"], "unit": "grid-point-steps/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": one["fp32"]["ms_per_step"],
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32-bulk (f64 islands)",
-               "data": "synthetic", "config": {"workload": "425x300x50 single domain (the ensemble was skipped)"},
-               "roofline": roofline(d32, args.steps, "hbm_traffic_hills425x300x50.md"), "c3": c3}
+               "data": "synthetic", "config": {"workload": f"{grid} single domain (the ensemble was skipped)"},
+               "roofline": roofline(d32, args.steps, f"hbm_traffic_hills{grid}.md", f"{name} ({grid})"), name: one}
         d32.close()
         print(json.dumps(out), flush=True)
         return

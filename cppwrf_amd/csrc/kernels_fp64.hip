@@ -1103,6 +1103,13 @@ MI_POINT_KERNEL k_ss_prep(Range r, Args a, int first) {
   const double mus_u = first ? f.muu[x2] : H * (mts + f.mub[x2 - 1] + f.mu1[x2 - 1]);
   const double mus_v = first ? f.muv[x2] : H * (mts + f.mub[x2 - g.sj] + f.mu1[x2 - g.sj]);
   if (k == 0) {
+    // the map-factor ratios advance_uv reads at every substep: msfux / msfuy, 1 / msfuy (u faces), msfvy / msfvx
+    // (v faces)
+    if (j < ny) {
+      f.rmu[x2] = f.msfux[x2] / f.msfuy[x2];
+      f.msfuy_inv[x2] = 1.0 / f.msfuy[x2];
+    }
+    if (i < nx && j < g.nyv) f.rmv[x2] = f.msfvy[x2] / f.msfvx[x2];
     if (i < nx && j < ny) {
       put<kHMass>(f.muts, g, i, j, 0, mts);
       const double m2 = f.mu2[x2];
@@ -1273,9 +1280,9 @@ __device__ __forceinline__ void uv_fp32(const Args& a, int i, int j, int k, doub
   if ((uface || ((OPEN && g.ox()) && (i == 0 || i == nx))) && j < ny) {
     f.u2[x] = real(F(f.u2[x]) + dts * F(f.ru_tend[x]));
   } else if (!uface && i >= ui0 && i < nx && j < ny) {
-    const F mudf_x = F(-a.q.emdx * (f.mudf[x2] - f.mudf[x2 - 1]) / f.msfuy[x2]);
+    const F mudf_x = F(-a.q.emdx * (f.mudf[x2] - f.mudf[x2 - 1]) * f.msfuy_inv[x2]);
     const F cq = a.q.moist ? F(f.cqu[x]) : F(1);
-    f.u2[x] = real(F(f.u2[x]) + dts * F(f.ru_tend[x]) - dts * cq * dp(1, 1, 0, f.muu[x2], c.rdx, f.msfux[x2] / f.msfuy[x2]) +
+    f.u2[x] = real(F(f.u2[x]) + dts * F(f.ru_tend[x]) - dts * cq * dp(1, 1, 0, f.muu[x2], c.rdx, f.rmu[x2]) +
                    F(c.c1h[k]) * mudf_x);
   }
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
@@ -1285,7 +1292,7 @@ __device__ __forceinline__ void uv_fp32(const Args& a, int i, int j, int k, doub
     const idx_t sj = g.sj;
     const F mudf_y = F(-a.q.emdy * (f.mudf[x2] - f.mudf[x2 - sj]) * f.msfvx_inv[x2]);
     const F cq = a.q.moist ? F(f.cqv[x]) : F(1);
-    f.v2[x] = real(F(f.v2[x]) + dts * F(f.rv_tend[x]) - dts * cq * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) +
+    f.v2[x] = real(F(f.v2[x]) + dts * F(f.rv_tend[x]) - dts * cq * dp(sj, 0, 1, f.muv[x2], c.rdy, f.rmv[x2]) +
                    F(c.c1h[k]) * mudf_y);
   }
 }
@@ -1329,7 +1336,7 @@ __device__ __forceinline__ void uv_at(const Args& a, int i, int j, int k, double
     put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x]);
   } else if (!uface && i >= ui0 && i < nx && j < ny) {
     const double mudf_x = -a.q.emdx * (f.mudf[x2] - f.mudf[x2 - 1]) / f.msfuy[x2];
-    put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x] - dts * (a.q.moist ? f.cqu[x] : 1.0) * dp(1, 1, 0, f.muu[x2], c.rdx, f.msfux[x2] / f.msfuy[x2]) + c.c1h[k] * mudf_x);
+    put<kHU>(f.u2, g, i, j, k * sk, f.u2[x] + dts * f.ru_tend[x] - dts * (a.q.moist ? f.cqu[x] : 1.0) * dp(1, 1, 0, f.muu[x2], c.rdx, f.rmu[x2]) + c.c1h[k] * mudf_x);
   }
   const int vj0 = (g.bcy != MI_BC_PERIODIC && !g.two_d) ? 1 : 0;
   if ((vface || ((OPEN && g.oy()) && (j == 0 || j == ny))) && i < nx) {
@@ -1337,7 +1344,7 @@ __device__ __forceinline__ void uv_at(const Args& a, int i, int j, int k, double
   } else if (!vface && !g.two_d && j >= vj0 && j < ny && i < nx) {
     const idx_t sj = g.sj;
     const double mudf_y = -a.q.emdy * (f.mudf[x2] - f.mudf[x2 - sj]) * f.msfvx_inv[x2];
-    put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x] - dts * (a.q.moist ? f.cqv[x] : 1.0) * dp(sj, 0, 1, f.muv[x2], c.rdy, f.msfvy[x2] / f.msfvx[x2]) + c.c1h[k] * mudf_y);
+    put<kHV>(f.v2, g, i, j, k * sk, f.v2[x] + dts * f.rv_tend[x] - dts * (a.q.moist ? f.cqv[x] : 1.0) * dp(sj, 0, 1, f.muv[x2], c.rdy, f.rmv[x2]) + c.c1h[k] * mudf_y);
   }
 }
 template <bool OPEN>

@@ -73,6 +73,8 @@ struct Fields {
   // levels; the coupled saves (c1h mu + c2h) u_save / msfuy and (c1h mu + c2h) v_save / msfvx; advance_w's
   // c2a rdnw / (c1h mut + c2h) at the mass levels; 1 / (1 + cqw) with moisture
   real *dampwt, *ucs, *vcs, *cwc, *cq1;
+  // and advance_uv's map-factor ratios (2-D): msfux / msfuy, 1 / msfuy at u faces, msfvy / msfvx at v faces
+  double *rmu, *msfuy_inv, *rmv;
   // damp_opt 2: the base profiles of theta and of the mass levels' height (nz values)
   double *t_base, *z_base;
   // the microphysics after the RK loop: moist_physics_prep's fields (dry theta, density, Exner function, height
