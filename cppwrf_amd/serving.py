@@ -361,10 +361,12 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32")
     ap.add_argument("--budget-gib", type=float, default=200.0, help="HBM budget for resident forecasts")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--progress", action="store_true", help="report the replay's progress on stderr every 30 s")
     a = ap.parse_args(argv)
     prec = A.MI_FP32_BULK if a.precision == "fp32" else A.MI_FP64
     reqs = poisson_trace(a.seconds, a.rate, a.seed, a.scale, prec)
-    res = run_trace(reqs, int(a.budget_gib * (1 << 30)))
+    report = (lambda m: print(m, file=sys.stderr, flush=True)) if a.progress else None
+    res = run_trace(reqs, int(a.budget_gib * (1 << 30)), progress=report)
     res.update({"trace_seconds": a.seconds, "rate_per_s": a.rate, "scale": a.scale, "precision": a.precision})
     print(json.dumps(res), flush=True)
 
