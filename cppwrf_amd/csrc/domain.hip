@@ -980,7 +980,7 @@ void build_domain(Domain& d, mi_ctx* ctx, const mi_grid& g, const mi_params& p, 
   if (p.damp_opt == 3) add3(f.dampwt, St::W, -1);
   add3(f.ucs, St::U, -1); add3(f.vcs, St::V, -1); add3(f.cwc, St::M, -1);
   if (p.n_moist > 0) add3(f.cq1, St::W, -1);
-  add2(f.rmu, St::U2, -1); add2(f.msfuy_inv, St::U2, -1); add2(f.rmv, St::V2, -1);
+  add2(f.rmu, St::U2, -1); add2(f.msfuy_inv, St::U2, -1); add2(f.rmv, St::V2, -1); add2(f.msfty_inv, St::M2, -1);
   f.t_base = d.alloc(nz + 1);
   f.z_base = d.alloc(nz + 1);
   f.u_base = d.alloc(nz + 1);

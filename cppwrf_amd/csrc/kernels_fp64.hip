@@ -1110,6 +1110,7 @@ MI_POINT_KERNEL k_ss_prep(Range r, Args a, int first) {
       f.msfuy_inv[x2] = 1.0 / f.msfuy[x2];
     }
     if (i < nx && j < g.nyv) f.rmv[x2] = f.msfvy[x2] / f.msfvx[x2];
+    if (i < nx && j < ny) f.msfty_inv[x2] = 1.0 / f.msfty[x2];  // and the substeps' 1 / msfty
     if (i < nx && j < ny) {
       put<kHMass>(f.muts, g, i, j, 0, mts);
       const double m2 = f.mu2[x2];
@@ -1443,12 +1444,13 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
   real phtend = 0, w2 = 0, phs = 0, phs_m = 0, phs_p = 0, ph = 0, ph_m = 0, ph_p = 0, rwt = 0;
   real c2a = 0, c2a_m = 0, alt = 0, alt_m = 0, wsv = 0, pm1 = 0;
   real cw = 0, cw_m = 0, cq1 = 1;  // the stage's c2a rdnw / (c1h mut + c2h) at k and k - 1, 1 / (1 + cqw)
-  double msfty = 1, msftx = 1, mut = 0;
+  double msfty = 1, msftx = 1, msft_inv = 1, mut = 0;
   double mu_old = 0, mtend = 0;
   real dwt = 0;
   if (on) {
     msfty = f.msfty[x2];
     msftx = f.msftx[x2];
+    msft_inv = f.msfty_inv[x2];
     mut = f.mut[x2];
     if (k == 0) {
       mu_old = f.mu2[x2];
@@ -1562,7 +1564,10 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
   // ---------------------------------------------------------------- 3: the omega'' terms
   if (on && k + 1 < nz) {
     const double mt = S2[4 * CI + ci];
-    LDS(TK, k, ci) = bdiv(c.dnw[k] * (c.c1h[k] * dmdt + dv + c.c1h[k] * mt), msfty);
+    const double tk = c.dnw[k] * (c.c1h[k] * dmdt + dv + c.c1h[k] * mt);
+    // FP64: the reference's division; FP32-bulk: the multiply by the stage's 1 / msfty
+    if constexpr (sizeof(real) == 8) LDS(TK, k, ci) = tk / msfty;
+    else LDS(TK, k, ci) = tk * msft_inv;
   }
   __syncthreads();
   // ---------------------------------------------------------------- 4: the running integral
@@ -1630,7 +1635,6 @@ __device__ __forceinline__ void acoustic_cols(const Args& a, const ColPos cp, do
   if (on) {
     const double m = mut, mav = muave;
     const double ope = 1.0 + epssm, ome = 1.0 - epssm;
-    const double msft_inv = bdiv(1.0, msfty);
     double wk;
     if (k == 0) {
       // the surface w'': 0 on flat ground; with terrain, from the small-step u'', v'' and the slope of ht

@@ -75,6 +75,7 @@ struct Fields {
   real *dampwt, *ucs, *vcs, *cwc, *cq1;
   // and advance_uv's map-factor ratios (2-D): msfux / msfuy, 1 / msfuy at u faces, msfvy / msfvx at v faces
   double *rmu, *msfuy_inv, *rmv;
+  double* msfty_inv;  // 1 / msfty (mass points), for the acoustic substeps
   // damp_opt 2: the base profiles of theta and of the mass levels' height (nz values)
   double *t_base, *z_base;
   // the microphysics after the RK loop: moist_physics_prep's fields (dry theta, density, Exner function, height

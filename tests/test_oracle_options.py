@@ -1,7 +1,8 @@
 """Options of the large step that no golden file of the reference exercises, checked on the oracle against what
 fixes them: top_lid holds w = 0 at the model top (the rigid lid of advance_w, small_step.cpp:375-378, 386); w_damp
 (large_step.cpp:475-480) acts only where the vertical Courant number exceeds 1 and there pulls w back; damp_opt 3
-(small_step.cpp:393-405) damps w in the top zdamp metres only in its first substep's column solve."""
+(small_step.cpp:393-405) damps w in the top zdamp metres only in its first substep's column solve, with the weight
+dts dampcoef sin^2(pi/2 (z - z_bot) / zdamp) (closed form below)."""
 import numpy as np
 
 from cppwrf_amd import _abi as A
@@ -67,3 +68,32 @@ def test_damp_opt3_damps_the_top_layer():
         out[d] = o.field(A.MI_W)
     top = slice(9, 12)  # w levels in the top 3 km of 8 km
     assert np.sqrt((out[3][top] ** 2).mean()) < np.sqrt((out[0][top] ** 2).mean())
+
+
+def test_damp_opt3_weight_closed_form():
+    """the first stage of a step from rest (w_save = 0, one acoustic substep of dt / 3): the damped column solve
+    divides w'' by 1 + dts dampcoef sin^2(pi/2 (z_k - z_bot) / zdamp) at the w levels k >= 1 inside the top zdamp
+    metres (z from the stage's geopotential, z_bot = z_top - zdamp), and small_step_finish recouples w linearly, so
+    the damped run's w over the undamped run's is that factor's inverse level by level.  Fails on a cos for the sin,
+    a dropped square, pi for pi/2, and the layer measured from the surface."""
+    case = _bubble()
+    zd, dc = 3000.0, 0.2
+    w = {}
+    for d in (0, 3):
+        o = OracleDomain(case, case.params(time_step_sound=4, damp_opt=d, zdamp=zd, dampcoef=dc))
+        assert np.all(o.field(A.MI_W) == 0.0)  # the stage starts at rest: w_save = 0
+        ph0 = o.field(A.MI_PH)
+        o.step_begin()
+        o.rk_prep(1)
+        o.rk_tendency(1)
+        o.small_steps(1)
+        w[d] = o.field(A.MI_W)
+    phb = case.profile(A.MI_PHB)
+    z = (ph0 + phb[:, None, None]) / 9.81
+    dts = case.grid.dt / 3.0
+    zbot = z[-1] - zd
+    s = np.sin(0.5 * np.pi * (z - zbot[None]) / zd)
+    wt = np.where((z >= zbot[None]) & (np.arange(case.grid.nz + 1)[:, None, None] >= 1), dts * dc * s * s, 0.0)
+    sel = np.abs(w[0]) > 1e-6 * np.abs(w[0]).max()
+    assert sel.sum() > 100 and (wt[sel] > 0).sum() > 20
+    np.testing.assert_allclose(w[3][sel], w[0][sel] / (1.0 + wt[sel]), rtol=1e-9)
