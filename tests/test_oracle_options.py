@@ -97,3 +97,41 @@ def test_damp_opt3_weight_closed_form():
     sel = np.abs(w[0]) > 1e-6 * np.abs(w[0]).max()
     assert sel.sum() > 100 and (wt[sel] > 0).sum() > 20
     np.testing.assert_allclose(w[3][sel], w[0][sel] / (1.0 + wt[sel]), rtol=1e-9)
+
+
+def test_w_damping_closed_form():
+    """w_damp (reference src/dyn/diffusion/w_damp.hpp:16-33, WRF w_alpha = 0.3, w_beta = 1): where the vertical
+    Courant number c = |omega / (c1f mut + c2f) rdnw dt| of a w level exceeds 1, the w tendency loses
+    sign(w) 0.3 (c - w_crit_cfl) (c1f mut + c2f); elsewhere it is unchanged.  Fails on w_alpha 0.25, the threshold at
+    w_crit_cfl = 0.8 instead of 1, the sign of w dropped, and rdn for rdnw in the Courant number."""
+    case = Case.bubble3d(16, 14, 12, dx=10000.0, ztop=8000.0, dt=30.0, seed=3, bubble_zc=3000.0)
+    g = case.grid
+    crit = 0.8
+    rw, inputs = {}, None
+    for wd in (0, 1):
+        o = OracleDomain(case, case.params(time_step_sound=8, w_damping=wd, w_crit_cfl=crit))
+        _converge(o, case, 250.0)
+        zz, yy, xx = np.indices(A.field_shape(A.MI_W, g.nx, g.ny, g.nz))
+        o.set_field(A.MI_W, 0.5 * np.sin(2 * np.pi * (xx + 0.5 * yy) / g.nx) * np.sin(np.pi * zz / g.nz))  # both signs
+        o.step_begin()
+        o.rk_prep(1)
+        if wd == 1:
+            inputs = (o.field(A.MI_WW), o.field(A.MI_W), o.field(A.MI_MU))
+        o.rk_tendency(1)
+        rw[wd] = o.field(A.MI_RW_TEND)
+    ww, w, mu = inputs
+    mut = case.profile(A.MI_MUB)[0] + mu[0]
+    lv = case.levels
+    c1f = np.array([lv.c1f[k] for k in range(g.nz + 1)])
+    c2f = np.array([lv.c2f[k] for k in range(g.nz + 1)])
+    rdnw = np.array([lv.rdnw[k] for k in range(g.nz)])
+    k = np.arange(1, g.nz)
+    muf = c1f[k, None, None] * mut[None] + c2f[k, None, None]
+    cfl = np.abs(ww[1:g.nz] / muf * rdnw[k, None, None] * g.dt)
+    want = np.where(cfl > 1.0, -np.where(w[1:g.nz] < 0, -1.0, 1.0) * 0.3 * (cfl - crit) * muf, 0.0)
+    got = rw[1][1:g.nz] - rw[0][1:g.nz]
+    assert (cfl > 1.0).sum() > 20 and (cfl < crit).sum() > 20
+    assert ((cfl > 1.0) & (w[1:g.nz] < 0)).sum() > 5 and ((cfl > 1.0) & (w[1:g.nz] > 0)).sum() > 5
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12 * np.abs(rw[0]).max())
+    assert scale > 0
