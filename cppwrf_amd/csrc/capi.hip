@@ -42,6 +42,9 @@ struct mi_batch {
   mi_ctx* ctx = nullptr;
   std::vector<mi_domain*> m;
   std::vector<hipStream_t> st;
+  // the stream the batch's graphs are captured on: the context's own stream may take other threads' work
+  // meanwhile (domains built on the context), which a capture on it would record into the graph
+  hipStream_t cap = nullptr;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
   struct Graph {
@@ -61,6 +64,7 @@ struct mi_batch {
     for (hipEvent_t e : join) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
     for (hipStream_t s : st) (void)hipStreamDestroy(s);
+    if (cap) (void)hipStreamDestroy(cap);
   }
   bool fusable() const {
     if (m.empty() || m.size() > size_t(mi_fp64::kMaxFusedMembers)) return false;
@@ -112,31 +116,31 @@ struct mi_batch {
       if (m[0]->storage_bytes() == 8) mi_fp64::fused_prepare(m.data(), n, fused_dev, s0, info);
       else mi_fp32::fused_prepare(m.data(), n, fused_dev, s0, info);
     }
-    hip_check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
       if (fused) {
         if (m[0]->storage_bytes() == 8)
-          mi_fp64::fused_capture(m.data(), int(m.size()), s0, st.data(), fork, join.data(), fused_dev, info);
+          mi_fp64::fused_capture(m.data(), int(m.size()), cap, st.data(), fork, join.data(), fused_dev, info);
         else
-          mi_fp32::fused_capture(m.data(), int(m.size()), s0, st.data(), fork, join.data(), fused_dev, info);
+          mi_fp32::fused_capture(m.data(), int(m.size()), cap, st.data(), fork, join.data(), fused_dev, info);
       } else {
-        hip_check(hipEventRecord(fork, s0), "hipEventRecord");
+        hip_check(hipEventRecord(fork, cap), "hipEventRecord");
         for (size_t q = 0; q < m.size(); ++q) {
           hip_check(hipStreamWaitEvent(st[q], fork, 0), "hipStreamWaitEvent");
           m[q]->capture_step(st[q]);
           hip_check(hipEventRecord(join[q], st[q]), "hipEventRecord");
-          hip_check(hipStreamWaitEvent(s0, join[q], 0), "hipStreamWaitEvent");
+          hip_check(hipStreamWaitEvent(cap, join[q], 0), "hipStreamWaitEvent");
         }
       }
     } catch (...) {
       hipGraph_t junk = nullptr;
-      (void)hipStreamEndCapture(s0, &junk);
+      (void)hipStreamEndCapture(cap, &junk);
       if (junk) (void)hipGraphDestroy(junk);
       throw;
     }
     Graph gr;
     gr.key = k;
-    hip_check(hipStreamEndCapture(s0, &gr.g), "hipStreamEndCapture");
+    hip_check(hipStreamEndCapture(cap, &gr.g), "hipStreamEndCapture");
     hip_check(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0), "hipGraphInstantiate");
     graphs.push_back(gr);
     return gr.x;
@@ -376,6 +380,7 @@ int mi_batch_create(mi_ctx* ctx, mi_domain* const* members, int n, mi_batch** ou
       b->join.push_back(e);
     }
     hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking), "hipStreamCreate");  // default priority
     ctx->refs.fetch_add(1);
     *out = b.release();
   });

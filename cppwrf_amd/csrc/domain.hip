@@ -74,10 +74,14 @@ struct Domain final : mi_domain {
   // one graph per scalar-buffer parity: the step's launches name the buffers (none without scalars)
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
+  // the domain's own stream for capturing its step: capture is a state of the stream, so work that another
+  // thread enqueues on the context's stream meanwhile (a domain built on the same context) would be recorded
+  // into this graph and replayed at every step, after that domain is gone too
+  hipStream_t cap_stream = nullptr;
   bool use_graph = true;
   std::string prof_name;
   KernelTimer timer{};
-  hipStream_t launch_stream = nullptr;  // while a batch captures this domain's step: its branch's stream
+  hipStream_t launch_stream = nullptr;  // while the step is captured: cap_stream, or a batch's branch stream
   hipStream_t stream() const { return launch_stream ? launch_stream : ctx->stream; }
   int colcfg[kNumColKernels] = {0, 0, 0, 0};  // block shape of each column kernel
   // specified boundaries: the data buffer of each variable (U V T PH MU, then the scalars) and which
@@ -109,6 +113,7 @@ struct Domain final : mi_domain {
   ~Domain() override {
     for (hipEvent_t e : timer.ev) (void)hipEventDestroy(e);
     drop_graphs();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (void* p : allocs) (void)hipFree(p);
   }
   int launch_version = 0;  // bumped whenever the step's launches change (shapes, boundary data present)
@@ -703,19 +708,24 @@ struct Domain final : mi_domain {
         // capturing runs the host side of step_launches (the buffer swaps) without running the step, so
         // the host state is put back before the launch below performs the step
         const Scalars keep = sc;
-        hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        if (!cap_stream)
+          hip_check(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking), "hipStreamCreate");  // default priority
+        hip_check(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        launch_stream = cap_stream;
         try {
           step_launches();
         } catch (...) {  // end the capture, so the stream takes launches again, and put the host state back
+          launch_stream = nullptr;
           hipGraph_t junk = nullptr;
-          (void)hipStreamEndCapture(ctx->stream, &junk);
+          (void)hipStreamEndCapture(cap_stream, &junk);
           if (junk) (void)hipGraphDestroy(junk);
           sc = keep;
           qpar = par;
           throw;
         }
+        launch_stream = nullptr;
         hipGraph_t gr = nullptr;
-        const hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
+        const hipError_t ec = hipStreamEndCapture(cap_stream, &gr);
         sc = keep;
         qpar = par;
         hip_check(ec, "hipStreamEndCapture");
