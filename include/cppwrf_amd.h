@@ -25,7 +25,10 @@
  *  - The caller owns every pointer it passes; the library copies what it
  *    needs before the call returns.
  *  - One domain must not be used by two host threads at once.  Different
- *    domains may be stepped from different threads.
+ *    domains may be stepped from different threads, also domains of one
+ *    context: a domain may be created on a context while another thread
+ *    steps the context's domains (graphs are captured on a stream of the
+ *    domain's or batch's own, never on the context's stream).
  */
 #ifndef CPPWRF_AMD_H
 #define CPPWRF_AMD_H

@@ -1,5 +1,7 @@
 """The continuous-batching server: forecasts admitted at different step boundaries, batched with others of
 other sizes and precisions, give the bits they give alone; latency and throughput are reported."""
+import time
+
 import numpy as np
 import pytest
 
@@ -39,6 +41,37 @@ def test_results_same_alone_and_in_the_mix(ctx):
         for fid in STATE:
             a, m = alone.field(fid), r.domain.field(fid)
             assert np.array_equal(a.view(np.uint64), m.view(np.uint64)), (r.rid, A.FIELD_NAMES[fid])
+
+
+def test_background_builds_beside_captures(ctx):
+    """domains built by background threads on the streams on which the tick captures other domains' first steps
+    give the bits they give alone (a capture on a shared stream would record the builders' fills and uploads)"""
+    print("[serving] background builds beside captures", flush=True)
+    reqs = [Request(q, 20 + 4 * (q % 4), 18 + 2 * (q % 3), 16, 3000.0 + 500.0 * q, 6 + q % 5, seed=10 + q,
+                    precision=A.MI_FP32_BULK if q % 3 == 2 else A.MI_FP64) for q in range(12)]
+    srv = ForecastServer(ctx, hbm_budget=1 << 34, clock=lambda: 0.0, streams=2, prepare_workers=2)
+    pending = list(reqs)
+    t0 = time.perf_counter()
+    t = 0
+    while pending or srv.waiting or srv.preparing or srv.ready or srv.active or srv.draining:
+        assert time.perf_counter() - t0 < 90.0
+        if pending and t % 2 == 0:
+            srv.submit(pending.pop(0))
+        if not srv.active and not srv.ready:
+            time.sleep(0.002)
+        srv.tick(float(t))
+        t += 1
+    srv.close()
+    assert all(r.taken == r.steps for r in reqs)
+    for r in reqs:
+        case = r.case()
+        alone = RK3(ctx, case.grid, r.params(case), case)
+        alone.step(r.steps)
+        for fid in STATE:
+            a, m = alone.field(fid), r.domain.field(fid)
+            assert np.array_equal(a.view(np.uint64), m.view(np.uint64)), (r.rid, A.FIELD_NAMES[fid])
+        alone.close()
+        r.domain.close()
 
 
 def test_short_poisson_trace(ctx):
