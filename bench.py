@@ -144,6 +144,37 @@ def ensemble_bench(ctx, steps, warmup, autotune):
     return out
 
 
+def ensemble_error_hours(ctx, hours):
+    """every member of the ensemble from its initial state in FP32-bulk and in FP64, side by side for `hours`
+    forecast hours (each member its own number of steps per hour, dt 6 s per km): the largest relative L2 over the
+    members per variable at each hour"""
+    from cppwrf_amd import _abi as A
+    from cppwrf_amd import workloads as W
+    worst = [dict() for _ in range(hours)]
+    steps = 0
+    for q, sp in enumerate(W.ensemble_specs()):
+        dt = 6.0 * sp[3] / 1000.0
+        d32, d64 = W.ensemble_member(ctx, sp, A.MI_FP32_BULK), W.ensemble_member(ctx, sp, A.MI_FP64)
+        done = 0
+        for h in range(hours):
+            target = int(round((h + 1) * 3600.0 / dt))
+            d32.step(target - done)
+            d64.step(target - done)
+            done = target
+            for name, v in W.error_report(d32, d64, False).items():
+                worst[h][name] = max(worst[h].get(name, 0.0), v["rel_l2"])
+        if d32.check_finite() != -1 or d64.check_finite() != -1:
+            raise RuntimeError(f"ensemble member {q} is not finite after {hours} h")
+        steps += done
+        d32.close()
+        d64.close()
+        log(f"ensemble member {q}: {done} steps to {hours} h, worst so far at {hours} h:",
+            {k: f"{v:.2e}" for k, v in worst[-1].items()})
+    return {"hours": hours, "member_steps": steps,
+            "hourly_max_rel_l2_over_members": [{"hour": h + 1, "per_variable": worst[h]} for h in range(hours)],
+            "max_rel_l2_last_hour": max(worst[-1].values())}
+
+
 def domain_bench(ctx, name, steps, warmup, autotune, error_hours):
     """one real-data-shaped domain (c2 or c3) in FP64 and FP32-bulk: timing, the FP32-bulk error after the timed
     steps and per forecast hour to `error_hours`, and the FP32-bulk domain (for the roofline)"""
@@ -304,6 +335,9 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=20.0, help="length of the serving trace")
     ap.add_argument("--error-hours", type=int, default=6,
                     help="forecast hours over which the 119x89x44 FP32-bulk error is reported hourly")
+    ap.add_argument("--ensemble-hours", type=int, default=6,
+                    help="forecast hours over which every ensemble member's FP32-bulk error is reported hourly "
+                         "(0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each forecast domain runs on a "
@@ -352,6 +386,8 @@ def main():
         "fp32_error_vs_fp64": {"after_steps": ens["after_steps"],
                                "max_rel_l2_over_members": ens["fp32_max_rel_l2_over_members"]},
     }
+    if args.ensemble_hours > 0:
+        out["fp32_error_vs_fp64"]["forecast"] = ensemble_error_hours(ctx, args.ensemble_hours)
     if not args.no_c3:
         c3, d32 = domain_bench(ctx, "c3", args.steps, args.warmup, not args.no_autotune, 0)
         out["roofline"] = roofline(d32, args.steps, "hbm_traffic_hills425x300x50.md")

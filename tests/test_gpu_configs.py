@@ -6,7 +6,8 @@
  - the 425x300x50 domain at 3 km (the same recipe with sixth-order diffusion, specified boundaries, 6 acoustic
    steps): FP32-bulk against FP64 every simulated hour to 6 h, relative L2 <= 1e-3 for U, V, W, T, PH, MU, QVAPOR;
  - the 32-member ensemble of uneven sizes: stepped as one batch it gives each member's bits alone, and each
-   member's FP32-bulk state is within 1e-3 relative L2 of its FP64 state after 100 steps;
+   member's FP32-bulk state is within 1e-3 relative L2 of its FP64 state after 100 steps; the three members that
+   bound the error over 6 forecast hours stay within 1e-3 every hour;
  - a slice of the serving trace at full size (domains 50-300 x 50-300, forecasts cut to 20-40 steps): each result
    is the same alone and in the mix, and FP32-bulk within 1e-3 of an FP64 rerun."""
 import numpy as np
@@ -90,6 +91,27 @@ def test_c4_ensemble_batch_and_error(ctx):
         d64.close()
     print("[configs] ensemble worst member relative L2 after 100 steps:", worst, flush=True)
     assert all(v <= 1e-3 for v in worst.values()), worst
+
+
+@pytest.mark.parametrize("member", [22, 9, 4])
+def test_c4_member_fp32_error_to_six_hours(ctx, member):
+    """the ensemble members with the largest FP32-bulk error after 6 h over all 32 (22, 9: W 3.1e-4) and the one
+    with the most steps (4: dx 3.06 km, 1178 steps): relative L2 <= 1e-3 every forecast hour, both runs finite"""
+    spec = W.ensemble_specs()[member]
+    dt = 6.0 * spec[3] / 1000.0
+    print(f"[configs] ensemble member {member} {spec[:4]}: FP32-bulk vs FP64 hourly to 6 h", flush=True)
+    d32, d64 = W.ensemble_member(ctx, spec, A.MI_FP32_BULK), W.ensemble_member(ctx, spec, A.MI_FP64)
+    done = 0
+    for hour in range(1, 7):
+        target = int(round(hour * 3600.0 / dt))
+        d32.step(target - done)
+        d64.step(target - done)
+        done = target
+        for name, v in W.error_report(d32, d64, False).items():
+            assert v["rel_l2"] <= 1e-3, (hour, name, v)
+    assert d32.check_finite() == -1 and d64.check_finite() == -1
+    d32.close()
+    d64.close()
 
 
 def test_c5_slice_alone_and_mixed(ctx):
