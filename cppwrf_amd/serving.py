@@ -56,6 +56,8 @@ class Request:
     slot: int = 0            # the server's stream the forecast runs on
     reserved: int = 0        # device bytes counted against the budget
     last_event: object = None
+    case_s: float = 0.0      # host time of building the case (initial state) and of creating and loading the domain
+    domain_s: float = 0.0
 
     @property
     def points(self) -> int:
@@ -149,8 +151,11 @@ class ForecastServer:
         self.waiting.append(r)
 
     def _prepare(self, r: Request) -> Request:
+        t0 = time.perf_counter()
         case = r.case()
+        t1 = time.perf_counter()
         r.domain = RK3(self.ctxs[r.slot], case.grid, r.params(case), case)
+        r.case_s, r.domain_s = t1 - t0, time.perf_counter() - t1
         return r
 
     def _start_preparing(self, now: float) -> None:
@@ -350,6 +355,10 @@ def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None, 
         "drain_s": wall - trace_s,
         # stable: the backlog left when arrivals stop clears within the slowest forecast's own time to forecast
         "queue_stable": bool(lat.size and wall - trace_s <= max(float(np.percentile(lat, 99)), 0.1 * trace_s)),
+        # host time per request of building its case and of creating and loading its domain (background threads)
+        "case_s_mean": float(np.mean([r.case_s for r in reqs])) if reqs else None,
+        "domain_s_mean": float(np.mean([r.domain_s for r in reqs])) if reqs else None,
+        "domain_s_max": float(max(r.domain_s for r in reqs)) if reqs else None,
     }
 
 
