@@ -297,14 +297,15 @@ def poisson_trace(seconds: float, rate: float, seed: int, scale: float = 1.0, pr
 
 
 def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None, keep=frozenset(), streams: int = 4,
-              prepare_workers: int = 2, progress=None, max_wall: float = 0.0) -> dict:
+              prepare_workers: int = 2, progress=None, max_wall: float = 0.0, max_ahead: int = 2) -> dict:
     """replay the trace in real time (arrivals by the wall clock) until every request has finished, or until
     `max_wall` seconds (> 0) have passed: then no further step is enqueued, and the requests not finished are
     counted as unfinished (the latencies are those of the finished ones); the final states of the requests in
     `keep` stay in their Request.result"""
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0  # noqa: E731
-    srv = ForecastServer(ctx, hbm_budget, clock=clock, streams=streams, prepare_workers=prepare_workers)
+    srv = ForecastServer(ctx, hbm_budget, clock=clock, streams=streams, prepare_workers=prepare_workers,
+                         max_ahead=max_ahead)
     pending = sorted(reqs, key=lambda r: r.arrival)
     last_report = 0.0
     busy_ticks = 0
@@ -380,13 +381,17 @@ def main(argv=None):
     ap.add_argument("--budget-gib", type=float, default=200.0, help="HBM budget for resident forecasts")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--progress", action="store_true", help="report the replay's progress on stderr every 30 s")
+    ap.add_argument("--streams", type=int, default=4, help="streams the forecasts are spread over")
+    ap.add_argument("--max-ahead", type=int, default=2, help="ticks the host may enqueue ahead of the device")
     ap.add_argument("--max-wall", type=float, default=0.0,
                     help="stop enqueuing steps after this many seconds (0: run until every request has finished)")
     a = ap.parse_args(argv)
     prec = A.MI_FP32_BULK if a.precision == "fp32" else A.MI_FP64
     reqs = poisson_trace(a.seconds, a.rate, a.seed, a.scale, prec)
     report = (lambda m: print(m, file=sys.stderr, flush=True)) if a.progress else None
-    res = run_trace(reqs, int(a.budget_gib * (1 << 30)), progress=report, max_wall=a.max_wall)
+    res = run_trace(reqs, int(a.budget_gib * (1 << 30)), progress=report, max_wall=a.max_wall, streams=a.streams,
+                    max_ahead=a.max_ahead)
+    res["max_ahead"] = a.max_ahead
     res.update({"trace_seconds": a.seconds, "rate_per_s": a.rate, "scale": a.scale, "precision": a.precision})
     print(json.dumps(res), flush=True)
 
