@@ -335,6 +335,11 @@ def run_trace(reqs: list[Request], hbm_budget: int, ctx: Context | None = None, 
     srv.close()
     srv.release_finished(keep)
     wall = clock()
+    if stopped:  # the device state of the forecasts left unfinished (their enqueued steps have completed)
+        for r in reqs:
+            if r.domain is not None:
+                r.domain.close()
+                r.domain = None
     lat = np.array(srv.stats.latencies)
     q = np.array(srv.stats.queue_samples) if srv.stats.queue_samples else np.zeros(1)
     act = np.array(srv.stats.active_samples) if srv.stats.active_samples else np.zeros(1)

@@ -81,3 +81,16 @@ def test_short_poisson_trace(ctx):
     print(res, flush=True)
     assert res["completed"] == len(reqs)
     assert res["p99_time_to_forecast_s"] >= res["p50_time_to_forecast_s"] > 0
+
+
+def test_trace_stopped_at_max_wall(ctx):
+    print("[serving] a trace stopped at its wall-clock limit", flush=True)
+    reqs = poisson_trace(seconds=4.0, rate=3.0, seed=7, scale=0.5, precision=A.MI_FP32_BULK)
+    for r in reqs:
+        r.steps = 100000  # none can finish within the limit
+    res = run_trace(reqs, hbm_budget=8 << 30, ctx=ctx, max_wall=2.0)
+    print({k: res[k] for k in ("requests", "completed", "unfinished", "stopped_at_max_wall", "wall_s")}, flush=True)
+    assert res["stopped_at_max_wall"] and not res["queue_stable"]
+    assert res["completed"] == 0 and res["unfinished"] == len(reqs) > 0
+    assert res["wall_s"] < 30.0
+    assert all(r.domain is None for r in reqs)  # the unfinished forecasts' device state is released
